@@ -1,0 +1,300 @@
+"""ctypes wrapper over oracle/liboracle.so (the C restatement of Puck's search
+path) plus numpy helpers: brute-force exact top-k and recall with ties.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.  The engine (puck_amd) never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(HERE, "liboracle.so")
+_REF_PATH = os.path.join(HERE, "_ref", "libpuckref.so")
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("cells_considered", ctypes.c_uint64), ("cells_scanned", ctypes.c_uint64),
+                ("points_offered", ctypes.c_uint64), ("heap_updates", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+def build(force=False):
+    """Compile liboracle.so (and oracle/_ref when the reference is present)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", HERE, "all"])
+    if os.path.isdir("/root/reference") and (force or not os.path.exists(_REF_PATH)):
+        subprocess.check_call(["make", "-s", "-C", HERE, "ref"])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    L = ctypes.CDLL(_LIB_PATH)
+    vp = ctypes.c_void_p
+    L.po_sentinel_value.restype = ctypes.c_float
+    L.po_dot.restype = ctypes.c_float
+    L.po_dot.argtypes = [_f32p, _f32p, ctypes.c_uint32]
+    L.po_heap_run.restype = ctypes.c_uint32
+    L.po_heap_run.argtypes = [ctypes.c_uint32, _f32p, _u32p, ctypes.c_uint32, ctypes.c_int, _f32p, _u32p]
+    L.po_lut.restype = ctypes.c_int
+    L.po_lut.argtypes = [_f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32, _f32p]
+    L.po_lookup.restype = ctypes.c_float
+    L.po_lookup.argtypes = [_u8p, _f32p, ctypes.c_uint32]
+    L.po_rerank_dist.restype = ctypes.c_float
+    L.po_rerank_dist.argtypes = [_f32p, _f32p, ctypes.c_uint32]
+    L.po_normalize.restype = ctypes.c_int
+    L.po_normalize.argtypes = [_f32p, ctypes.c_uint32, _f32p]
+    L.po_index_create.restype = vp
+    L.po_index_create.argtypes = [ctypes.c_uint32] * 5 + [ctypes.c_int, _f32p, _f32p, _f32p, _u32p, _u8p, _f32p]
+    L.po_index_destroy.argtypes = [vp]
+    L.po_index_export.argtypes = [vp, _f32p, _f32p, _u32p, _f32p, _u32p]
+    L.po_search_one.restype = ctypes.c_int
+    L.po_search_one.argtypes = [vp, _f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                ctypes.c_uint32, ctypes.c_double, _f32p, _u32p,
+                                ctypes.POINTER(ctypes.c_uint32), _f32p, _u32p,
+                                ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(Stats), _i32p, _i32p]
+    L.po_search_batch.restype = ctypes.c_int
+    L.po_search_batch.argtypes = [vp, ctypes.c_uint32, _f32p, ctypes.c_uint32, ctypes.c_uint32,
+                                  ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, _f32p, _u32p,
+                                  _u32p, ctypes.c_int, ctypes.POINTER(Stats)]
+    L.po_assign_cells.restype = ctypes.c_int
+    L.po_assign_cells.argtypes = [_f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                  _f32p, ctypes.c_uint32, _u32p, _f32p, ctypes.c_int]
+    L.po_nearest_center.restype = ctypes.c_int
+    L.po_nearest_center.argtypes = [ctypes.c_uint32, _f32p, ctypes.c_uint32, _f32p, ctypes.c_uint32,
+                                    _i32p, _f32p, ctypes.c_int]
+    L.po_encode_filter.restype = ctypes.c_int
+    L.po_encode_filter.argtypes = [_f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32, _f32p,
+                                   ctypes.c_uint32, _f32p, _u32p, ctypes.c_uint32, _u8p, ctypes.c_int]
+    _lib = L
+    return L
+
+
+def ref_lib():
+    """oracle/_ref/libpuckref.so (reference sources built as-is) or None."""
+    if not os.path.exists(_REF_PATH):
+        if os.path.isdir("/root/reference"):
+            build()
+        if not os.path.exists(_REF_PATH):
+            return None
+    L = ctypes.CDLL(_REF_PATH)
+    L.ref_dist_table.argtypes = [_f32p, _f32p, ctypes.c_uint, ctypes.c_uint, _f32p]
+    L.ref_rerank_dist.restype = ctypes.c_float
+    L.ref_rerank_dist.argtypes = [_f32p, _f32p, ctypes.c_uint]
+    return L
+
+
+def c32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+SENTINEL = np.frombuffer(np.uint32(0x7F7F7F7F).tobytes(), dtype=np.float32)[0]
+
+
+def dot(a, b):
+    a, b = c32(a), c32(b)
+    return float(lib().po_dot(a, b, a.size))
+
+
+def heap_run(cap, vals, tags=None, reorder=True):
+    vals = c32(vals)
+    tags = np.arange(vals.size, dtype=np.uint32) if tags is None else np.ascontiguousarray(tags, np.uint32)
+    ov = np.zeros(max(cap, 1), np.float32)
+    ot = np.zeros(max(cap, 1), np.uint32)
+    n = lib().po_heap_run(cap, vals, tags, vals.size, int(reorder), ov, ot)
+    return n, ov[:cap], ot[:cap]
+
+
+def lut(q, codebook, nsq, lsq):
+    out = np.zeros(nsq * 256, np.float32)
+    rc = lib().po_lut(c32(q), c32(codebook), nsq, lsq, out)
+    assert rc == 0
+    return out
+
+
+def lookup(codes, lut_, nsq):
+    return float(lib().po_lookup(np.ascontiguousarray(codes, np.uint8), c32(lut_), nsq))
+
+
+def rerank_dist(x, q):
+    x, q = c32(x), c32(q)
+    return float(lib().po_rerank_dist(x, q, x.size))
+
+
+def normalize(q):
+    q = c32(q)
+    out = np.zeros_like(q)
+    rc = lib().po_normalize(q, q.size, out)
+    return out if rc == 0 else None
+
+
+class OracleIndex:
+    """CPU restatement of a loaded PuckIndex (memory order, derived arrays)."""
+
+    def __init__(self, coarse, fine, codebook, cell_assign, records, raw, whether_norm):
+        self.coarse, self.fine, self.codebook = c32(coarse), c32(fine), c32(codebook)
+        self.C, self.dim = self.coarse.shape
+        self.F = self.fine.shape[0]
+        self.N = int(cell_assign.shape[0])
+        self.nsq = records.shape[1] - 4 if records.ndim == 2 else None
+        self.whether_norm = bool(whether_norm)
+        rec = np.ascontiguousarray(records, np.uint8).reshape(-1)
+        self._h = lib().po_index_create(self.dim, self.C, self.F, self.nsq, self.N,
+                                        int(whether_norm), self.coarse, self.fine, self.codebook,
+                                        np.ascontiguousarray(cell_assign, np.uint32), rec,
+                                        c32(raw).reshape(-1))
+        if not self._h:
+            raise ValueError("oracle index creation failed")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().po_index_destroy(self._h)
+            self._h = None
+
+    def export(self):
+        C, F, N = self.C, self.F, self.N
+        cn = np.zeros(C, np.float32)
+        st = np.zeros(C * F, np.float32)
+        cs = np.zeros(C * F + 1, np.uint32)
+        mdo = np.zeros(C, np.float32)
+        m2l = np.zeros(max(N, 1), np.uint32)
+        lib().po_index_export(self._h, cn, st, cs, mdo, m2l)
+        return dict(coarse_norms=cn, stationary=st, cell_start=cs, min_dist_offset=mdo,
+                    mem2local=m2l[:N])
+
+    def search_one(self, q, topk, scc, ftk, max_topk=None, radius_rate=1.0, trace=False):
+        max_topk = topk if max_topk is None else max_topk
+        D = np.zeros(max(topk, 1), np.float32)
+        I = np.zeros(max(topk, 1), np.uint32)
+        n = ctypes.c_uint32(0)
+        fv = np.zeros(max(ftk, 1), np.float32)
+        ft = np.zeros(max(ftk, 1), np.uint32)
+        fn = ctypes.c_uint32(0)
+        st = Stats()
+        co = np.zeros(max(scc, 1), np.int32)
+        fo = np.zeros(self.F, np.int32)
+        rc = lib().po_search_one(self._h, c32(q), topk, max_topk, scc, ftk, radius_rate, D, I,
+                                 ctypes.byref(n), fv, ft, ctypes.byref(fn), ctypes.byref(st), co, fo)
+        out = dict(rc=rc, dist=D[:topk], ids=I[:topk], n=n.value, filt_val=fv[:ftk],
+                   filt_mem=ft[:ftk], filt_n=fn.value, stats=st.as_dict(),
+                   coarse=co[:min(scc, self.C)], fine=fo)
+        return out
+
+    def search_batch(self, Q, topk, scc, ftk, max_topk=None, radius_rate=1.0, nthreads=0):
+        Q = c32(Q)
+        nq = Q.shape[0]
+        max_topk = topk if max_topk is None else max_topk
+        D = np.zeros((nq, topk), np.float32)
+        I = np.zeros((nq, topk), np.uint32)
+        nres = np.zeros(nq, np.uint32)
+        st = Stats()
+        lib().po_search_batch(self._h, nq, Q.reshape(-1), topk, max_topk, scc, ftk, radius_rate,
+                              D.reshape(-1), I.reshape(-1), nres, nthreads, ctypes.byref(st))
+        return D, I, nres, st.as_dict()
+
+
+def assign_cells(coarse, fine, X, nthreads=0):
+    coarse, fine, X = c32(coarse), c32(fine), c32(X)
+    C, d = coarse.shape
+    F = fine.shape[0]
+    n = X.shape[0]
+    cell = np.zeros(n, np.uint32)
+    dist = np.zeros(n, np.float32)
+    lib().po_assign_cells(coarse.reshape(-1), fine.reshape(-1), C, F, d, X.reshape(-1), n, cell, dist,
+                          nthreads)
+    return cell, dist
+
+
+def nearest_center(centroids, X, nthreads=0):
+    centroids, X = c32(centroids), c32(X)
+    k, d = centroids.shape
+    n = X.shape[0]
+    a = np.zeros(n, np.int32)
+    dis = np.zeros(n, np.float32)
+    lib().po_nearest_center(d, centroids.reshape(-1), k, X.reshape(-1), n, a, dis, nthreads)
+    return a, dis
+
+
+def encode_filter(coarse, fine, codebook, nsq, X, cell, nthreads=0):
+    coarse, fine, codebook, X = c32(coarse), c32(fine), c32(codebook), c32(X)
+    d = coarse.shape[1]
+    F = fine.shape[0]
+    n = X.shape[0]
+    out = np.zeros((n, 4 + nsq), np.uint8)
+    rc = lib().po_encode_filter(coarse.reshape(-1), fine.reshape(-1), F, d, codebook.reshape(-1), nsq,
+                                X.reshape(-1), np.ascontiguousarray(cell, np.uint32), n,
+                                out.reshape(-1), nthreads)
+    assert rc == 0
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Exact brute force and recall (ann-benchmarks measurement spec).
+
+def exact_topk(X, Q, k, recheck=8):
+    """Exact squared-L2 top-k: FP32 GEMM candidate pass, then an FP64 recheck
+    of the top (k + recheck) candidates so that ranks near the boundary are
+    decided in double precision.  Returns (ids int64 [nq,k], dists f64 [nq,k])."""
+    X = c32(X)
+    Q = c32(Q)
+    xn = np.einsum("ij,ij->i", X, X, dtype=np.float64).astype(np.float32)
+    kk = min(k + recheck, X.shape[0])
+    ids = np.zeros((Q.shape[0], k), np.int64)
+    dd = np.zeros((Q.shape[0], k), np.float64)
+    bs = 1024
+    for s in range(0, Q.shape[0], bs):
+        q = Q[s:s + bs]
+        d = xn[None, :] - 2.0 * (q @ X.T)
+        cand = np.argpartition(d, kk - 1, axis=1)[:, :kk]
+        xc = X[cand].astype(np.float64)
+        exact = ((xc - q.astype(np.float64)[:, None, :]) ** 2).sum(-1)
+        order = np.lexsort((cand, exact), axis=1) if False else np.argsort(exact, axis=1, kind="stable")
+        ids[s:s + bs] = np.take_along_axis(cand, order, 1)[:, :k]
+        dd[s:s + bs] = np.take_along_axis(exact, order, 1)[:, :k]
+    return ids, dd
+
+
+def recall_with_ties(true_ids, true_dists, run_ids, count):
+    """benchmark/plotting/metrics.py:15-40 semantics, returns (mean, per-query, n_ties)."""
+    nq = len(run_ids)
+    rec = np.zeros(nq)
+    ties = 0
+    for i in range(nq):
+        tid, tdist = true_ids[i], true_dists[i]
+        gt = len(tdist)
+        if gt == count:
+            r = len(set(np.asarray(tid[:count]).tolist()) & set(np.asarray(run_ids[i]).tolist()))
+        else:
+            anchor = tdist[count - 1]
+            end = gt
+            for j in range(count, gt):
+                if not abs(anchor - tdist[j]) < 1e-6:
+                    end = j
+                    break
+            if end > count:
+                ties += 1
+            r = len(set(np.asarray(tid[:end]).tolist()) & set(np.asarray(run_ids[i]).tolist()))
+        rec[i] = r
+    return float(np.mean(rec) / count), rec, ties
+
+
+def recall_without_ties(true_ids, run_ids, count):
+    rec = np.array([len(set(np.asarray(true_ids[i][:count]).tolist()) & set(np.asarray(run_ids[i]).tolist()))
+                    for i in range(len(run_ids))], dtype=np.float64)
+    return float(np.mean(rec) / count), rec

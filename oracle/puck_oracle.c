@@ -1,0 +1,676 @@
+/*
+ * puck_oracle.c -- CPU restatement of Puck's batched search path
+ * (PuckIndex::search with whether_pq=false), of its index-load derivations
+ * and of its build-time assignment / PQ encoding.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP engine is
+ * compared against.  It is compiled into oracle/liboracle.so and is used only
+ * by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg.
+ * The product library (libpuckgpu.so) never links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates.  Paths are
+ * relative to the reference checkout (heaoxiang/puck).
+ *
+ * Floating-point conventions (the parts the reference leaves to MKL):
+ *   - po_dot() is a sequential fused-multiply-add chain starting from +0.0f.
+ *     It stands in for cblas_sdot and for every entry of sgemm_.  MKL's order
+ *     is unspecified, so bit-parity with an MKL build is unpinned; the HIP
+ *     engine's exact-arithmetic mode reproduces po_dot() bit for bit.
+ *   - Everything else (LUT, LUT lookup, heap, cell walk, rerank) follows the
+ *     reference's own written order, which the reference fixes in source.
+ *
+ * Build: gcc -O2 -fopenmp -ffp-contract=off -fno-fast-math -shared -fPIC
+ * (contraction is OFF so that every fmaf() below is explicit and every other
+ * product is rounded on its own, as the reference's SSE intrinsics do).
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <float.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define PO_KS 256u
+
+/* 0x7f7f7f7f as float: the value MaxHeap's memset(0x7f) leaves in each slot
+ * (puck/hierarchical_cluster/max_heap.cpp:30-32). */
+static float po_sentinel(void) {
+    uint32_t b = 0x7f7f7f7fu;
+    float f;
+    memcpy(&f, &b, 4);
+    return f;
+}
+
+float po_sentinel_value(void) { return po_sentinel(); }
+
+/* ------------------------------------------------------------------------ */
+/* Dot product: stands in for cblas_sdot / sgemm_ entries
+ * (hierarchical_cluster_index.cpp:78-90, :279, :305, :1125; puck_index.cpp:398,
+ * :515; quantization.cpp:251; kmeans.cpp:230-245).  Sequential FMA chain. */
+float po_dot(const float* a, const float* b, uint32_t n) {
+    float acc = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) acc = fmaf(a[i], b[i], acc);
+    return acc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* MaxHeap (puck/hierarchical_cluster/max_heap.cpp:28-102, max_heap.h:28-70).
+ * 1-based arrays over caller memory, filled with the 0x7f7f7f7f sentinel,
+ * non-full inserts start sifting at slot _default_point_cnt--. */
+typedef struct {
+    float* val;      /* 1-based view: val[1] is caller[0] */
+    uint32_t* tag;
+    uint32_t size;
+    uint32_t default_cnt;
+} po_heap;
+
+static void po_heap_init(po_heap* h, uint32_t size, float* val, uint32_t* tag) {
+    /* max_heap.cpp:28-35 */
+    memset(val, 0x7f, sizeof(float) * size);
+    h->val = val - 1;
+    h->tag = tag - 1;
+    h->size = size;
+    h->default_cnt = size;
+}
+
+static void po_heap_insert(po_heap* h, uint32_t heap_size, uint32_t father, float v, uint32_t t) {
+    /* max_heap.cpp:67-100 */
+    for (;;) {
+        uint32_t l = father << 1;
+        if (l > heap_size) break;
+        uint32_t r = l + 1;
+        if (r > heap_size || h->val[l] > h->val[r]) {
+            if (h->val[l] < v) break;
+            h->val[father] = h->val[l];
+            h->tag[father] = h->tag[l];
+            father = l;
+        } else {
+            if (h->val[r] < v) break;
+            h->val[father] = h->val[r];
+            h->tag[father] = h->tag[r];
+            father = r;
+        }
+    }
+    h->val[father] = v;
+    h->tag[father] = t;
+}
+
+static void po_heap_update(po_heap* h, float v, uint32_t t) {
+    /* max_heap.cpp:52-65 */
+    uint32_t father = 1;
+    if (v >= h->val[father]) return;
+    if (h->default_cnt > 0) father = h->default_cnt--;
+    po_heap_insert(h, h->size, father, v, t);
+}
+
+static void po_heap_reorder(po_heap* h) {
+    /* max_heap.cpp:41-50 (heap sort to ascending) */
+    for (uint32_t i = 0; i < h->size; i++) {
+        float vb = h->val[1];
+        uint32_t tb = h->tag[1];
+        uint32_t hs = h->size - i;
+        po_heap_insert(h, hs - 1, 1, h->val[hs], h->tag[hs]);
+        h->val[hs] = vb;
+        h->tag[hs] = tb;
+    }
+}
+
+static uint32_t po_heap_count(const po_heap* h) { return h->size - h->default_cnt; }
+
+/* Exported heap driver for tests: feed n values, return the reordered arrays. */
+uint32_t po_heap_run(uint32_t cap, const float* vals, const uint32_t* tags, uint32_t n,
+                     int reorder, float* out_val, uint32_t* out_tag) {
+    po_heap h;
+    po_heap_init(&h, cap, out_val, out_tag);
+    for (uint32_t i = 0; i < n; ++i) po_heap_update(&h, vals[i], tags[i]);
+    if (reorder) po_heap_reorder(&h);
+    return po_heap_count(&h);
+}
+
+/* ------------------------------------------------------------------------ */
+/* LUT: Quantization::get_dist_table (quantization.cpp:260-270) ->
+ * faiss::fvec_L2sqr_ny_D{1,2,4} (distances_simd.cpp:72-133).
+ * Each product (x-y)^2 is rounded, then the lanes are combined by the
+ * hadd order of the matching SSE kernel.  Sub-vector lengths other than
+ * 1, 2 and 4 are rejected (the engine supports the same three; the D8/D12
+ * kernels are contracted differently by -Ofast and are not restated). */
+static float po_sq(float x) { return x * x; }
+
+static float po_lut_entry(const float* x, const float* y, uint32_t lsq) {
+    switch (lsq) {
+    case 1:  /* D1 :72-96 */
+        return po_sq(x[0] - y[0]);
+    case 2:  /* D2 :98-118 : hadd -> p0+p1 */
+        return po_sq(x[0] - y[0]) + po_sq(x[1] - y[1]);
+    case 4: { /* D4 :120-133 : (p0+p1)+(p2+p3) */
+        float p0 = po_sq(x[0] - y[0]), p1 = po_sq(x[1] - y[1]);
+        float p2 = po_sq(x[2] - y[2]), p3 = po_sq(x[3] - y[3]);
+        return (p0 + p1) + (p2 + p3);
+    }
+    default:
+        return NAN;
+    }
+}
+
+int po_lut(const float* q, const float* codebook, uint32_t nsq, uint32_t lsq, float* lut) {
+    if (!(lsq == 1 || lsq == 2 || lsq == 4)) return -1;
+    for (uint32_t m = 0; m < nsq; ++m) {
+        const float* cb = codebook + (size_t)m * PO_KS * lsq;
+        for (uint32_t k = 0; k < PO_KS; ++k)
+            lut[m * PO_KS + k] = po_lut_entry(q + m * lsq, cb + (size_t)k * lsq, lsq);
+    }
+    return 0;
+}
+
+/* LUT lookup sum: lookup_dist_table (puck_index.cpp:75-113).  Four SSE lanes,
+ * lane j accumulates subspaces m = j (mod 4) in order from +0, then two hadds
+ * give (s0+s1)+(s2+s3).  A partial last group adds zeros in the unused lanes. */
+float po_lookup(const uint8_t* codes, const float* lut, uint32_t nsq) {
+    float s[4] = {0.f, 0.f, 0.f, 0.f};
+    uint32_t m = 0;
+    for (; m + 4 <= nsq; m += 4)
+        for (int j = 0; j < 4; ++j) s[j] = s[j] + lut[(m + j) * PO_KS + codes[m + j]];
+    for (uint32_t j = 0; m + j < nsq; ++j) s[j] = s[j] + lut[(m + j) * PO_KS + codes[m + j]];
+    return (s[0] + s[1]) + (s[2] + s[3]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Exact squared L2 used by the rerank: similarity::L2Sqr16Ext and L2SqrExt
+ * (puck/tinker/method/hnsw_distfunc_opt_impl_inline.h:41-122), called in
+ * blocks of 16 by rank_topN_points (puck_index.cpp:428-453).
+ * The reference is built with -Ofast -march=native (CMakeLists.txt:16-22),
+ * which lets the compiler contract and reassociate.  The order below is the
+ * one GCC 11.4 emits for that loop on x86-64 with AVX (checked bit for bit
+ * against the reference sources compiled with those flags, oracle/_ref):
+ *   per 16-block: a_j = fma(d_j, d_j, d_{8+j}^2)          (j = 0..7)
+ *   t <- (((a0+a1)+(a2+a3)) + ((a4+a5)+a6)) + (a7 + t)
+ * The running total t enters the lane reduction: -Ofast reassociates the
+ * `temp_dist +=` of the caller into the inlined TmpRes sum. */
+static float po_l2sqr16_acc(const float* a, const float* b, float t) {
+    float l[8];
+    for (int j = 0; j < 8; ++j) {
+        float d0 = a[j] - b[j];
+        float d1 = a[8 + j] - b[8 + j];
+        l[j] = fmaf(d0, d0, d1 * d1);
+    }
+    return (((l[0] + l[1]) + (l[2] + l[3])) + ((l[4] + l[5]) + l[6])) + (l[7] + t);
+}
+
+static float po_l2sqr_tail(const float* a, const float* b, uint32_t n) {
+    /* L2SqrExt on n < 16 elements: no 16-blocks, 4-lane blocks, then scalars. */
+    float acc[4] = {0, 0, 0, 0};
+    uint32_t i = 0;
+    for (; i + 4 <= n; i += 4)
+        for (int j = 0; j < 4; ++j) {
+            float d = a[i + j] - b[i + j];
+            acc[j] = fmaf(d, d, acc[j]);
+        }
+    float s = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    for (; i < n; ++i) {
+        float d = a[i] - b[i];
+        s = fmaf(d, d, s);
+    }
+    return s;
+}
+
+float po_rerank_dist(const float* x, const float* q, uint32_t dim) {
+    float t = 0.0f;
+    uint32_t m = 0;
+    for (; m + 16 <= dim; m += 16) t = po_l2sqr16_acc(x + m, q + m, t);
+    if (dim - m > 0) t = t + po_l2sqr_tail(x + m, q + m, dim - m);
+    return t;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Index in memory order: HierarchicalClusterIndex::read_coodbooks
+ * (hierarchical_cluster_index.cpp:257-324), convert_local_to_memory_idx
+ * (puck_index.cpp:229-309), read_feature_index (puck_index.cpp:198-227,
+ * hierarchical_cluster_index.cpp:367-406), Quantization::load
+ * (quantization.cpp:152-226). */
+typedef struct {
+    uint32_t dim, C, F, nsq, lsq, N;
+    int whether_norm;
+    float* coarse;        /* C x dim */
+    float* fine;          /* F x dim (stored order) */
+    float* codebook;      /* nsq x 256 x lsq */
+    float* coarse_norms;  /* C : dot(c,c)/2 */
+    float* stationary;    /* C*F : dot(f,f)/2 + dot(f,c); empty cell -> sqrt(FLT_MAX) */
+    uint32_t* cell_start; /* C*F+1 */
+    float* min_dist_offset;/* C */
+    uint32_t* mem2local;  /* N */
+    uint8_t* records;     /* N x (4+nsq), memory order: [f32 offset][nsq codes] */
+    float* raw;           /* N x dim, memory order */
+} po_index;
+
+typedef struct { uint32_t cell; float off; uint32_t local; } po_order;
+
+static int po_order_cmp(const void* pa, const void* pb) {
+    /* std::pair<uint32_t, std::pair<float, uint32_t>> ordering, with the local
+     * id as last key: std::stable_sort over (cell, offset) with ties kept in
+     * local order (puck_index.cpp:248-272). */
+    const po_order* a = (const po_order*)pa;
+    const po_order* b = (const po_order*)pb;
+    if (a->cell != b->cell) return a->cell < b->cell ? -1 : 1;
+    if (a->off < b->off) return -1;
+    if (a->off > b->off) return 1;
+    if (a->local != b->local) return a->local < b->local ? -1 : 1;
+    return 0;
+}
+
+void* po_index_create(uint32_t dim, uint32_t C, uint32_t F, uint32_t nsq, uint32_t N,
+                      int whether_norm, const float* coarse, const float* fine,
+                      const float* codebook, const uint32_t* cell_assign,
+                      const uint8_t* records_local, const float* raw_local) {
+    if (nsq == 0 || dim % nsq != 0) return NULL;
+    if (!(dim / nsq == 1 || dim / nsq == 2 || dim / nsq == 4)) return NULL;
+    po_index* ix = (po_index*)calloc(1, sizeof(po_index));
+    ix->dim = dim; ix->C = C; ix->F = F; ix->nsq = nsq; ix->lsq = dim / nsq; ix->N = N;
+    ix->whether_norm = whether_norm;
+    size_t rec = 4 + nsq;
+    ix->coarse = (float*)malloc(sizeof(float) * C * dim);
+    ix->fine = (float*)malloc(sizeof(float) * F * dim);
+    ix->codebook = (float*)malloc(sizeof(float) * nsq * PO_KS * ix->lsq);
+    ix->coarse_norms = (float*)malloc(sizeof(float) * C);
+    ix->stationary = (float*)malloc(sizeof(float) * (size_t)C * F);
+    ix->cell_start = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)C * F + 1));
+    ix->min_dist_offset = (float*)malloc(sizeof(float) * C);
+    ix->mem2local = (uint32_t*)malloc(sizeof(uint32_t) * (N ? N : 1));
+    ix->records = (uint8_t*)malloc(rec * (N ? N : 1));
+    ix->raw = (float*)malloc(sizeof(float) * (size_t)(N ? N : 1) * dim);
+    memcpy(ix->coarse, coarse, sizeof(float) * C * dim);
+    memcpy(ix->fine, fine, sizeof(float) * F * dim);
+    memcpy(ix->codebook, codebook, sizeof(float) * nsq * PO_KS * ix->lsq);
+
+    /* read_coodbooks :278-321 */
+    for (uint32_t c = 0; c < C; ++c)
+        ix->coarse_norms[c] = po_dot(coarse + (size_t)c * dim, coarse + (size_t)c * dim, dim) / 2;
+    float* fnorm = (float*)malloc(sizeof(float) * F);
+    for (uint32_t f = 0; f < F; ++f)
+        fnorm[f] = po_dot(fine + (size_t)f * dim, fine + (size_t)f * dim, dim) / 2;
+    for (uint32_t c = 0; c < C; ++c)
+        for (uint32_t f = 0; f < F; ++f)
+            ix->stationary[(size_t)c * F + f] =
+                fnorm[f] + po_dot(fine + (size_t)f * dim, coarse + (size_t)c * dim, dim);
+    free(fnorm);
+
+    /* convert_local_to_memory_idx (puck_index.cpp:229-309); cell_start is
+     * pre-filled with N (hierarchical_cluster_index.cpp:352) */
+    uint32_t cells = C * F;
+    for (uint32_t i = 0; i <= cells; ++i) ix->cell_start[i] = N;
+    po_order* ord = (po_order*)malloc(sizeof(po_order) * (N ? N : 1));
+    for (uint32_t i = 0; i < N; ++i) {
+        if (cell_assign[i] >= cells) { free(ord); return NULL; }
+        ord[i].cell = cell_assign[i];
+        memcpy(&ord[i].off, records_local + (size_t)i * rec, 4);
+        ord[i].local = i;
+    }
+    qsort(ord, N, sizeof(po_order), po_order_cmp);
+    uint32_t* local2mem = (uint32_t*)malloc(sizeof(uint32_t) * (N ? N : 1));
+    uint32_t cc = 0;
+    ix->cell_start[0] = 0;
+    for (uint32_t i = 0; i < N; ++i) {
+        local2mem[ord[i].local] = i;
+        ix->mem2local[i] = ord[i].local;
+        while (cc + 1 <= ord[i].cell) { ++cc; ix->cell_start[cc] = i; }
+    }
+    free(ord);
+    for (uint32_t c = 0; c < C; ++c) ix->min_dist_offset[c] = FLT_MAX;
+    for (uint32_t i = 0; i < cells; ++i) {
+        uint32_t c = i / F;
+        if (ix->cell_start[i] == ix->cell_start[i + 1])
+            ix->stationary[i] = sqrtf(FLT_MAX);
+        if (ix->stationary[i] < ix->min_dist_offset[c]) ix->min_dist_offset[c] = ix->stationary[i];
+    }
+    /* records and raw features permuted to memory order */
+    for (uint32_t i = 0; i < N; ++i) {
+        uint32_t m = local2mem[i];
+        memcpy(ix->records + (size_t)m * rec, records_local + (size_t)i * rec, rec);
+        memcpy(ix->raw + (size_t)m * dim, raw_local + (size_t)i * dim, sizeof(float) * dim);
+    }
+    free(local2mem);
+    return ix;
+}
+
+void po_index_destroy(void* p) {
+    po_index* ix = (po_index*)p;
+    if (!ix) return;
+    free(ix->coarse); free(ix->fine); free(ix->codebook); free(ix->coarse_norms);
+    free(ix->stationary); free(ix->cell_start); free(ix->min_dist_offset);
+    free(ix->mem2local); free(ix->records); free(ix->raw);
+    free(ix);
+}
+
+/* Export the derived arrays so tests can compare them with the engine's. */
+void po_index_export(void* p, float* coarse_norms, float* stationary, uint32_t* cell_start,
+                     float* min_dist_offset, uint32_t* mem2local) {
+    po_index* ix = (po_index*)p;
+    size_t cells = (size_t)ix->C * ix->F;
+    if (coarse_norms) memcpy(coarse_norms, ix->coarse_norms, sizeof(float) * ix->C);
+    if (stationary) memcpy(stationary, ix->stationary, sizeof(float) * cells);
+    if (cell_start) memcpy(cell_start, ix->cell_start, sizeof(uint32_t) * (cells + 1));
+    if (min_dist_offset) memcpy(min_dist_offset, ix->min_dist_offset, sizeof(float) * ix->C);
+    if (mem2local) memcpy(mem2local, ix->mem2local, sizeof(uint32_t) * ix->N);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Query normalisation: HierarchicalClusterIndex::normalization
+ * (hierarchical_cluster_index.cpp:1513-1536), whether_norm branch:
+ * norm = snrm2(q); if norm < 1e-6 -> error; q *= (float)(1.0/norm). */
+int po_normalize(const float* q, uint32_t dim, float* out) {
+    float norm = sqrtf(po_dot(q, q, dim));
+    if (norm < 1e-6f) return -1;
+    float alpha = (float)(1.0 / (double)norm);
+    for (uint32_t i = 0; i < dim; ++i) out[i] = alpha * q[i];
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Search statistics returned for tests and for sizing GPU work. */
+typedef struct {
+    uint64_t cells_considered;  /* break-checked cells */
+    uint64_t cells_scanned;     /* cells passed to compute_quantized_distance */
+    uint64_t points_offered;    /* points with t_partial < top (LUT summed) */
+    uint64_t heap_updates;      /* max_heap_update calls that inserted */
+} po_stats;
+
+/* One query: PuckIndex::search (puck_index.cpp:559-600) with whether_pq=false.
+ * Returns 0, or -1 for topk > max_topk / null feature / zero-norm query.
+ * filt_val/filt_tag (ftk each) receive the filter heap as rank_topN_points
+ * reads it (heap order when full, ascending when not full). */
+int po_search_one(const void* p, const float* feature, uint32_t topk, uint32_t max_topk,
+                  uint32_t scc, uint32_t ftk, double radius_rate,
+                  float* out_dist, uint32_t* out_ids, uint32_t* out_n,
+                  float* filt_val, uint32_t* filt_tag, uint32_t* filt_n,
+                  po_stats* st, int* coarse_out, int* fine_out) {
+    const po_index* ix = (const po_index*)p;
+    uint32_t dim = ix->dim, C = ix->C, F = ix->F, nsq = ix->nsq;
+    /* puck_index.cpp:560-563 */
+    if (topk > max_topk || feature == NULL) return -1;
+    if (scc > C) scc = C;   /* index_conf.cpp:64 search_coarse_count = min(flag, C) */
+    float* q = (float*)malloc(sizeof(float) * dim);
+    if (ix->whether_norm) {
+        if (po_normalize(feature, dim, q) != 0) { free(q); return -1; }
+    } else {
+        memcpy(q, feature, sizeof(float) * dim);
+    }
+    /* search_nearest_coarse_cluster (hierarchical_cluster_index.cpp:700-727) */
+    float* coarse_d = (float*)malloc(sizeof(float) * (scc ? scc : 1));
+    uint32_t* coarse_t = (uint32_t*)malloc(sizeof(uint32_t) * (scc ? scc : 1));
+    {
+        po_heap h;
+        po_heap_init(&h, scc, coarse_d, coarse_t);
+        for (uint32_t c = 0; c < C; ++c) {
+            float ip = po_dot(ix->coarse + (size_t)c * dim, q, dim);
+            float t = ix->coarse_norms[c] - ip;
+            if (scc > 0 && t < coarse_d[0]) po_heap_update(&h, t, c);
+        }
+        po_heap_reorder(&h);
+    }
+    if (coarse_out) for (uint32_t l = 0; l < scc; ++l) coarse_out[l] = (int)coarse_t[l];
+    /* pre_filter_search -> get_dist_table (puck_index.cpp:459-483) */
+    float* lut = (float*)malloc(sizeof(float) * nsq * PO_KS);
+    po_lut(q, ix->codebook, nsq, ix->lsq, lut);
+    /* fine GEMV + full heap sort (puck_index.cpp:494-504) */
+    float* fine_d = (float*)malloc(sizeof(float) * F);
+    uint32_t* fine_t = (uint32_t*)malloc(sizeof(uint32_t) * F);
+    {
+        po_heap h;
+        po_heap_init(&h, F, fine_d, fine_t);
+        for (uint32_t k = 0; k < F; ++k)
+            po_heap_update(&h, -po_dot(ix->fine + (size_t)k * dim, q, dim), k);
+        po_heap_reorder(&h);
+    }
+    if (fine_out) for (uint32_t k = 0; k < F; ++k) fine_out[k] = (int)fine_t[k];
+    /* filter heap + cell walk (puck_index.cpp:506-557) */
+    float* fval = (float*)malloc(sizeof(float) * ftk);
+    uint32_t* ftag = (uint32_t*)malloc(sizeof(uint32_t) * ftk);
+    po_heap fh;
+    po_heap_init(&fh, ftk, fval, ftag);
+    float qn = po_dot(q, q, dim);
+    float pivot = (float)((double)(fval[0] - qn) / radius_rate / 2.0);
+    size_t rec = 4 + nsq;
+    po_stats s = {0, 0, 0, 0};
+    for (uint32_t l = 0; l < scc; ++l) {
+        uint32_t c = coarse_t[l];
+        const float* cst = ix->stationary + (size_t)c * F;
+        const uint32_t* cstart = ix->cell_start + (size_t)c * F;
+        float cd = coarse_d[l];
+        float min_dist = ix->min_dist_offset[c] + cd;
+        float max_st = pivot - cd - fine_d[0];
+        for (uint32_t idx = 0; idx < F; ++idx) {
+            s.cells_considered++;
+            if (fine_d[idx] + min_dist >= pivot) break;
+            uint32_t k = fine_t[idx];
+            if (cst[k] >= max_st) continue;
+            float cell_dist = cd + cst[k] + fine_d[idx];
+            s.cells_scanned++;
+            /* compute_quantized_distance (puck_index.cpp:346-390) */
+            uint32_t b = cstart[k], e = cstart[k + 1];
+            int updated = 0;
+            for (uint32_t i = b; i < e; ++i) {
+                const uint8_t* r = ix->records + (size_t)i * rec;
+                float off;
+                memcpy(&off, r, 4);
+                float t = (float)(2.0 * (double)cell_dist + (double)off);
+                if (t >= fval[0]) break;
+                s.points_offered++;
+                t = t + po_lookup(r + 4, lut, nsq);
+                if (t < fval[0]) {
+                    po_heap_update(&fh, t, i);
+                    ++updated;
+                    s.heap_updates++;
+                }
+            }
+            if (updated > 0) pivot = (float)((double)(fval[0] - qn) / radius_rate / 2.0);
+            float cand = pivot - cd - fine_d[idx];
+            if (cand < max_st) max_st = cand;
+        }
+    }
+    uint32_t nf = po_heap_count(&fh);
+    if (nf < ftk) po_heap_reorder(&fh);
+    if (filt_val) memcpy(filt_val, fval, sizeof(float) * ftk);
+    if (filt_tag) memcpy(filt_tag, ftag, sizeof(uint32_t) * ftk);
+    if (filt_n) *filt_n = nf;
+    /* rank_topN_points whether_pq=false (puck_index.cpp:392-457) */
+    {
+        po_heap rh;
+        po_heap_init(&rh, topk, out_dist, out_ids);
+        for (uint32_t i = 0; i < nf; ++i) {
+            const float* x = ix->raw + (size_t)ftag[i] * dim;
+            float t = po_rerank_dist(x, q, dim);
+            if (t < out_dist[0]) po_heap_update(&rh, t, ix->mem2local[ftag[i]]);
+        }
+        po_heap_reorder(&rh);
+        *out_n = po_heap_count(&rh);
+    }
+    if (st) *st = s;
+    free(q); free(coarse_d); free(coarse_t); free(lut); free(fine_d); free(fine_t);
+    free(fval); free(ftag);
+    return 0;
+}
+
+/* Batched entry: PySearcher::search + ParallelFor (pyapi_wrapper/py_api_wrapper.cpp:111-171):
+ * one query per thread, per-query return codes ignored (output rows of a failed
+ * query are left as the heap left them).  Row stride is topk. */
+int po_search_batch(const void* p, uint32_t nq, const float* queries, uint32_t topk,
+                    uint32_t max_topk, uint32_t scc, uint32_t ftk, double radius_rate,
+                    float* dist, uint32_t* ids, uint32_t* nres, int nthreads,
+                    po_stats* stats_out) {
+    const po_index* ix = (const po_index*)p;
+    uint64_t a = 0, b = 0, c = 0, d = 0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads) reduction(+ : a, b, c, d)
+#endif
+    for (int64_t i = 0; i < (int64_t)nq; ++i) {
+        po_stats s = {0, 0, 0, 0};
+        uint32_t n = 0;
+        int rc = po_search_one(ix, queries + (size_t)i * ix->dim, topk, max_topk, scc, ftk,
+                               radius_rate, dist + (size_t)i * topk, ids + (size_t)i * topk, &n,
+                               NULL, NULL, NULL, &s, NULL, NULL);
+        if (nres) nres[i] = rc == 0 ? n : 0;
+        a += s.cells_considered; b += s.cells_scanned; c += s.points_offered; d += s.heap_updates;
+    }
+    if (stats_out) {
+        stats_out->cells_considered = a; stats_out->cells_scanned = b;
+        stats_out->points_offered = c; stats_out->heap_updates = d;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Build-time cell assignment: HierarchicalClusterIndex::assign +
+ * nearest_cell_assign (hierarchical_cluster_index.cpp:1042-1138), with the
+ * fine-norm bound array of batch_assign (:1186-1195). */
+typedef struct { float term; int idx; } po_pair;
+static int po_pair_cmp(const void* pa, const void* pb) {
+    const po_pair* a = (const po_pair*)pa;
+    const po_pair* b = (const po_pair*)pb;
+    if (a->term < b->term) return -1;
+    if (a->term > b->term) return 1;
+    return a->idx < b->idx ? -1 : (a->idx > b->idx);
+}
+
+int po_assign_cells(const float* coarse, const float* fine, uint32_t C, uint32_t F, uint32_t dim,
+                    const float* x, uint32_t n, uint32_t* cell_out, float* dist_out, int nthreads) {
+    float* cn = (float*)malloc(sizeof(float) * C);
+    float* fnorm = (float*)malloc(sizeof(float) * F);
+    float* fbound = (float*)malloc(sizeof(float) * F);
+    float* st = (float*)malloc(sizeof(float) * (size_t)C * F);
+    for (uint32_t c = 0; c < C; ++c) cn[c] = po_dot(coarse + (size_t)c * dim, coarse + (size_t)c * dim, dim) / 2;
+    for (uint32_t f = 0; f < F; ++f) {
+        fnorm[f] = po_dot(fine + (size_t)f * dim, fine + (size_t)f * dim, dim) / 2;
+        fbound[f] = 0 - sqrtf(fnorm[f]);
+    }
+    for (uint32_t c = 0; c < C; ++c)
+        for (uint32_t f = 0; f < F; ++f)
+            st[(size_t)c * F + f] = fnorm[f] + po_dot(fine + (size_t)f * dim, coarse + (size_t)c * dim, dim);
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+        po_pair* pairs = (po_pair*)malloc(sizeof(po_pair) * C);
+        float* fdist = (float*)malloc(sizeof(float) * F);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 64)
+#endif
+        for (int64_t i = 0; i < (int64_t)n; ++i) {
+            const float* xi = x + (size_t)i * dim;
+            float xn = po_dot(xi, xi, dim) / 2;
+            for (uint32_t c = 0; c < C; ++c) {
+                pairs[c].idx = (int)c;
+                pairs[c].term = cn[c] - po_dot(coarse + (size_t)c * dim, xi, dim) + xn;
+            }
+            for (uint32_t f = 0; f < F; ++f) fdist[f] = po_dot(fine + (size_t)f * dim, xi, dim);
+            qsort(pairs, C, sizeof(po_pair), po_pair_cmp);
+            int best_cell = -1;
+            float best = FLT_MAX;
+            for (uint32_t j = 0; j < C; ++j) {
+                int c = pairs[j].idx;
+                float ct = pairs[j].term;
+                for (uint32_t f = 0; f < F; ++f) {
+                    if (sqrtf(best) - fbound[f] < sqrtf(ct)) break;
+                    float score = ct - fdist[f] + st[(size_t)c * F + f];
+                    if (score < best) { best = score; best_cell = c * (int)F + (int)f; }
+                }
+            }
+            cell_out[i] = (uint32_t)best_cell;
+            if (dist_out) dist_out[i] = best;
+        }
+        free(pairs); free(fdist);
+    }
+    free(cn); free(fnorm); free(fbound); free(st);
+    return 0;
+}
+
+/* nearest_center (puck/hierarchical_cluster/kmeans.cpp:221-258):
+ * dist = (float)((double)(pn + cn) - 2.0 * (double)sdot(x, c)), strict '<'
+ * argmin starting from FLT_MAX. */
+int po_nearest_center(uint32_t dim, const float* centroids, uint32_t k, const float* x,
+                      uint32_t n, int32_t* assign, float* dis, int nthreads) {
+    float* cnorm = (float*)malloc(sizeof(float) * k);
+    for (uint32_t c = 0; c < k; ++c) cnorm[c] = po_dot(centroids + (size_t)c * dim, centroids + (size_t)c * dim, dim);
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+#endif
+    for (int64_t j = 0; j < (int64_t)n; ++j) {
+        const float* xj = x + (size_t)j * dim;
+        float pn = po_dot(xj, xj, dim);
+        float best = FLT_MAX;
+        uint32_t arg = 0;
+        for (uint32_t c = 0; c < k; ++c) {
+            float d = (float)((double)(pn + cnorm[c]) - 2.0 * (double)po_dot(xj, centroids + (size_t)c * dim, dim));
+            if (d < best) { best = d; arg = c; }
+        }
+        assign[j] = (int32_t)arg;
+        if (dis) dis[j] = best;
+    }
+    free(cnorm);
+    return 0;
+}
+
+/* Filter-PQ encoding of one chunk: PuckIndex::puck_assign
+ * (puck_index.cpp:602-709) with quantizations = {filter} (whether_pq=false).
+ * residual r = (x - c) - f (two saxpy with alpha -1); per subspace the
+ * nearest codeword by nearest_center; then offset = 2*sdot(decode, c+f)
+ * (Quantization::set_static_value_of_formula, quantization.cpp:244-258).
+ * records_out: n x (4+nsq) in the filter_data.dat layout. */
+int po_encode_filter(const float* coarse, const float* fine, uint32_t F, uint32_t dim,
+                     const float* codebook, uint32_t nsq, const float* x, const uint32_t* cell,
+                     uint32_t n, uint8_t* records_out, int nthreads) {
+    if (nsq == 0 || dim % nsq != 0) return -1;
+    uint32_t lsq = dim / nsq;
+    size_t rec = 4 + nsq;
+    float* cbn = (float*)malloc(sizeof(float) * nsq * PO_KS);
+    for (uint32_t m = 0; m < nsq; ++m)
+        for (uint32_t k = 0; k < PO_KS; ++k) {
+            const float* cw = codebook + ((size_t)m * PO_KS + k) * lsq;
+            cbn[m * PO_KS + k] = po_dot(cw, cw, lsq);
+        }
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+        float* r = (float*)malloc(sizeof(float) * dim);
+        float* dec = (float*)malloc(sizeof(float) * dim);
+        float* v = (float*)malloc(sizeof(float) * dim);
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+        for (int64_t i = 0; i < (int64_t)n; ++i) {
+            uint32_t c = cell[i] / F, f = cell[i] % F;
+            const float* cc = coarse + (size_t)c * dim;
+            const float* ff = fine + (size_t)f * dim;
+            for (uint32_t j = 0; j < dim; ++j) {
+                float t = fmaf(-1.0f, cc[j], x[(size_t)i * dim + j]);
+                r[j] = fmaf(-1.0f, ff[j], t);
+            }
+            uint8_t* out = records_out + (size_t)i * rec;
+            for (uint32_t m = 0; m < nsq; ++m) {
+                const float* sub = r + m * lsq;
+                const float* cb = codebook + (size_t)m * PO_KS * lsq;
+                float pn = po_dot(sub, sub, lsq);
+                float best = FLT_MAX;
+                uint32_t arg = 0;
+                for (uint32_t k = 0; k < PO_KS; ++k) {
+                    float d = (float)((double)(pn + cbn[m * PO_KS + k]) -
+                                      2.0 * (double)po_dot(sub, cb + (size_t)k * lsq, lsq));
+                    if (d < best) { best = d; arg = k; }
+                }
+                out[4 + m] = (uint8_t)arg;
+                for (uint32_t j = 0; j < lsq; ++j) dec[m * lsq + j] = cb[(size_t)arg * lsq + j];
+            }
+            for (uint32_t j = 0; j < dim; ++j) v[j] = fmaf(1.0f, ff[j], cc[j]);
+            float off = 2 * po_dot(dec, v, dim);
+            memcpy(out, &off, 4);
+        }
+        free(r); free(dec); free(v);
+    }
+    free(cbn);
+    return 0;
+}
