@@ -1,0 +1,132 @@
+/*
+ * puck_gpu.h -- C ABI of libpuckgpu.so, the MI355X (gfx950) engine for Puck's
+ * batched search path (PuckIndex / HierarchicalClusterIndex, whether_pq=false).
+ *
+ * Plain C types only; no C++ exceptions cross this boundary.  Every call
+ * returns 0 on success or a negative code (the reference's convention: 0 = OK,
+ * < 0 = error); puckgpu_last_error() gives the message of the last failure on
+ * a context.  Pointers are host pointers unless `device_ptrs` is non-zero, in
+ * which case every array argument of that call is a device (HBM) pointer on
+ * the context's device and the call runs on the context's stream and returns
+ * after it has finished.  A context may be used from one host thread at a time.
+ *
+ * Reference interfaces replaced (paths relative to the heaoxiang/puck checkout):
+ *   puckgpu_index_load     HierarchicalClusterIndex::init / read_coodbooks /
+ *                          convert_local_to_memory_idx / read_feature_index
+ *                          (puck/hierarchical_cluster/hierarchical_cluster_index.cpp:220-255,
+ *                          :257-324; puck/puck/puck_index.cpp:173-309)
+ *   puckgpu_search         PySearcher::search + ParallelFor + PuckIndex::search
+ *                          (pyapi_wrapper/py_api_wrapper.cpp:111-171;
+ *                          puck/puck/puck_index.cpp:559-600; puck/index.h:30-81)
+ *   puckgpu_assign_cells   HierarchicalClusterIndex::assign / nearest_cell_assign
+ *                          (hierarchical_cluster_index.cpp:1042-1138)
+ *   puckgpu_encode_filter  PuckIndex::puck_assign, filter quantization only
+ *                          (puck/puck/puck_index.cpp:602-709)
+ *   puckgpu_kmeans_assign  nearest_center, the k-means assignment step
+ *                          (puck/hierarchical_cluster/kmeans.cpp:221-258)
+ */
+#ifndef PUCK_GPU_H
+#define PUCK_GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct puckgpu_ctx puckgpu_ctx;
+typedef struct puckgpu_index puckgpu_index;
+
+/* Error codes. */
+#define PUCKGPU_OK 0
+#define PUCKGPU_EINVAL (-1)   /* bad argument (the reference's -1 of search / init) */
+#define PUCKGPU_EHIP (-2)     /* HIP runtime failure */
+#define PUCKGPU_ENOMEM (-3)   /* device allocation failed */
+
+/* Index geometry and build flags (index.dat fields, hierarchical_cluster_index.cpp:522-567). */
+typedef struct {
+    uint32_t dim;                  /* feature_dim */
+    uint32_t coarse_cluster_count; /* C */
+    uint32_t fine_cluster_count;   /* F */
+    uint32_t filter_nsq;           /* filter PQ subspaces; dim / filter_nsq in {1, 2, 4} */
+    uint32_t total_point_count;    /* N */
+    int32_t whether_norm;          /* queries are L2-normalised before search */
+} puckgpu_index_params;
+
+/* Search knobs (IndexConf, puck/index_conf.cpp:28-95, :152-198). */
+typedef struct {
+    uint32_t topk_max;            /* FLAGS_topk: a request with topk > topk_max fails with -1 */
+    uint32_t search_coarse_count; /* clamped to C */
+    uint32_t filter_topk;         /* >= topk, <= puckgpu_filter_topk_max() */
+    double radius_rate;           /* >= 1.0 */
+} puckgpu_search_params;
+
+/* Context: one device, one stream, reusable scratch.  device = HIP ordinal. */
+int puckgpu_create(int device, puckgpu_ctx** out);
+void puckgpu_destroy(puckgpu_ctx* ctx);
+const char* puckgpu_last_error(const puckgpu_ctx* ctx);
+uint32_t puckgpu_filter_topk_max(void);
+
+/* Load an index from arrays in the reference file layouts, local (file) order:
+ *   coarse [C x dim] (coarse.dat rows), fine [F x dim] (fine.dat rows, stored
+ *   order), filter_codebook [nsq x 256 x dim/nsq] (filter_codebook.dat rows),
+ *   cell_assign [N] (cell_assign.dat), filter_records [N x (4 + nsq)]
+ *   (filter_data.dat: f32 offset then nsq code bytes), features [N x dim]
+ *   (all_data.feat.bin rows without the int32 length prefix).
+ * The engine derives |c|^2/2, the C x F stationary table, the memory order
+ * (cells ascending, then offset, then local id), empty-cell sentinels and the
+ * per-coarse minimum offsets, as PuckIndex::init does. */
+int puckgpu_index_load(puckgpu_ctx* ctx, const puckgpu_index_params* params, const float* coarse,
+                       const float* fine, const float* filter_codebook, const uint32_t* cell_assign,
+                       const uint8_t* filter_records, const float* features, int device_ptrs,
+                       puckgpu_index** out);
+void puckgpu_index_free(puckgpu_index* index);
+/* Copy derived arrays to host buffers (any may be NULL): coarse_norms [C],
+ * stationary [C*F], cell_start [C*F+1], min_dist_offset [C], mem2local [N]. */
+int puckgpu_index_export(puckgpu_ctx* ctx, const puckgpu_index* index, float* coarse_norms, float* stationary,
+                         uint32_t* cell_start, float* min_dist_offset, uint32_t* mem2local);
+
+/* Batched search of n queries [n x dim].  Outputs have row stride topk:
+ * distance [n x topk] ascending squared L2 (on normalised vectors when
+ * whether_norm), labels [n x topk] local ids, result_num [n] (may be NULL).
+ * Unfilled slots hold distance 0x7f7f7f7f and label 0xffffffff.  A query whose
+ * norm is below 1e-6 under whether_norm gets result_num 0.  Returns -1 without
+ * touching the outputs when topk > topk_max or topk == 0. */
+int puckgpu_search(puckgpu_ctx* ctx, const puckgpu_index* index, const puckgpu_search_params* params,
+                   uint32_t n, const float* queries, uint32_t topk, float* distance, uint32_t* labels,
+                   uint32_t* result_num, int device_ptrs);
+/* Stage output of the last search on this context (host buffers):
+ * filter-stage survivors per query (filter_topk entries per row, ascending
+ * filter distance, memory ids) and their count.  For tests and diagnostics. */
+int puckgpu_last_filter(puckgpu_ctx* ctx, uint32_t n, uint32_t filter_topk, float* filt_dist, uint32_t* filt_mem,
+                        uint32_t* filt_n);
+/* Per-stage device times (ms) of the last search, summed over its passes:
+ * [0] query prep + center GEMM, [1] center sort, [2] walk + filter scan,
+ * [3] rerank, [4] exact-tie replay.  counters5 (may be NULL) receives the
+ * walk's segments, uncertain cells, examined cells and scanned points summed over
+ * queries, then the number of queries replayed because their result depends
+ * on the order of exactly equal distances. */
+int puckgpu_last_stage_ms(puckgpu_ctx* ctx, float* ms5, uint64_t* counters5);
+
+/* Filter lookup table for n queries (test hook): out [n x nsq x 256]. */
+int puckgpu_lut(puckgpu_ctx* ctx, uint32_t n, uint32_t dim, const float* queries, const float* filter_codebook,
+                uint32_t nsq, float* out, int device_ptrs);
+
+/* Build-time nearest cell of n points [n x dim]: cell_out [n] = coarse*F + fine,
+ * dist_out [n] (may be NULL) = the winning score. */
+int puckgpu_assign_cells(puckgpu_ctx* ctx, const float* coarse, const float* fine, uint32_t C, uint32_t F,
+                         uint32_t dim, uint32_t n, const float* x, uint32_t* cell_out, float* dist_out,
+                         int device_ptrs);
+/* Filter-PQ records of n points: records_out [n x (4 + nsq)] in the
+ * filter_data.dat layout. */
+int puckgpu_encode_filter(puckgpu_ctx* ctx, const float* coarse, const float* fine, uint32_t C, uint32_t F,
+                          uint32_t dim, const float* filter_codebook, uint32_t nsq, uint32_t n, const float* x,
+                          const uint32_t* cell, uint8_t* records_out, int device_ptrs);
+/* k-means assignment: assign_out [n] nearest of k centroids [k x dim]
+ * (nearest_center's formula and strict '<'), dist_out [n] may be NULL. */
+int puckgpu_kmeans_assign(puckgpu_ctx* ctx, uint32_t dim, const float* centroids, uint32_t k, uint32_t n,
+                          const float* x, int32_t* assign_out, float* dist_out, int device_ptrs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

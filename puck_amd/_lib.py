@@ -1,0 +1,76 @@
+"""ctypes binding of libpuckgpu.so (include/puck_gpu.h).
+
+The engine has no CPU fallback: if the library is missing or cannot be loaded
+this module raises, and every search / build entry fails loudly.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpuckgpu.so")
+
+# Every symbol declared in include/puck_gpu.h.
+EXPORTS = [
+    "puckgpu_create", "puckgpu_destroy", "puckgpu_last_error", "puckgpu_filter_topk_max",
+    "puckgpu_index_load", "puckgpu_index_free", "puckgpu_index_export", "puckgpu_search",
+    "puckgpu_last_filter", "puckgpu_last_stage_ms", "puckgpu_lut", "puckgpu_assign_cells",
+    "puckgpu_encode_filter", "puckgpu_kmeans_assign",
+]
+
+
+class IndexParams(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_uint32), ("coarse_cluster_count", ctypes.c_uint32),
+                ("fine_cluster_count", ctypes.c_uint32), ("filter_nsq", ctypes.c_uint32),
+                ("total_point_count", ctypes.c_uint32), ("whether_norm", ctypes.c_int32)]
+
+
+class SearchParams(ctypes.Structure):
+    _fields_ = [("topk_max", ctypes.c_uint32), ("search_coarse_count", ctypes.c_uint32),
+                ("filter_topk", ctypes.c_uint32), ("radius_rate", ctypes.c_double)]
+
+
+_lib = None
+
+
+def load():
+    """Load libpuckgpu.so (built by puck_amd/build.py); raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python puck_amd/build.py` "
+                           "(the engine has no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u32, i32, c_int = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_int
+    L.puckgpu_create.argtypes = [c_int, ctypes.POINTER(vp)]
+    L.puckgpu_create.restype = c_int
+    L.puckgpu_destroy.argtypes = [vp]
+    L.puckgpu_destroy.restype = None
+    L.puckgpu_last_error.argtypes = [vp]
+    L.puckgpu_last_error.restype = ctypes.c_char_p
+    L.puckgpu_filter_topk_max.argtypes = []
+    L.puckgpu_filter_topk_max.restype = u32
+    L.puckgpu_index_load.argtypes = [vp, ctypes.POINTER(IndexParams), vp, vp, vp, vp, vp, vp, c_int,
+                                     ctypes.POINTER(vp)]
+    L.puckgpu_index_load.restype = c_int
+    L.puckgpu_index_free.argtypes = [vp]
+    L.puckgpu_index_free.restype = None
+    L.puckgpu_index_export.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.puckgpu_index_export.restype = c_int
+    L.puckgpu_search.argtypes = [vp, vp, ctypes.POINTER(SearchParams), u32, vp, u32, vp, vp, vp, c_int]
+    L.puckgpu_search.restype = c_int
+    L.puckgpu_last_filter.argtypes = [vp, u32, u32, vp, vp, vp]
+    L.puckgpu_last_filter.restype = c_int
+    L.puckgpu_last_stage_ms.argtypes = [vp, vp, vp]
+    L.puckgpu_last_stage_ms.restype = c_int
+    L.puckgpu_lut.argtypes = [vp, u32, u32, vp, vp, u32, vp, c_int]
+    L.puckgpu_lut.restype = c_int
+    L.puckgpu_assign_cells.argtypes = [vp, vp, vp, u32, u32, u32, u32, vp, vp, vp, c_int]
+    L.puckgpu_assign_cells.restype = c_int
+    L.puckgpu_encode_filter.argtypes = [vp, vp, vp, u32, u32, u32, vp, u32, u32, vp, vp, vp, c_int]
+    L.puckgpu_encode_filter.restype = c_int
+    L.puckgpu_kmeans_assign.argtypes = [vp, u32, vp, u32, u32, vp, vp, vp, c_int]
+    L.puckgpu_kmeans_assign.restype = c_int
+    del i32
+    _lib = L
+    return L

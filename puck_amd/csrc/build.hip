@@ -1,0 +1,385 @@
+// build.hip -- index construction and load-time derivations on gfx950.
+//
+//   pk_assign_cells     nearest_cell_assign (hierarchical_cluster_index.cpp:1042-1085)
+//                       on the inner products of pk_gemm_exact; one wave per point
+//   pk_encode_filter    filter-PQ codes and offsets (puck_index.cpp:602-709 with
+//                       nearest_center, kmeans.cpp:221-258, and
+//                       set_static_value_of_formula, quantization.cpp:244-258)
+//   pk_kmeans_argmin    nearest_center's argmin on a GEMM chunk (kmeans.cpp:240-254)
+//   pk_finalize_cells   empty-cell sentinel and per-coarse min_dist_offset
+//                       (puck_index.cpp:287-306)
+//   pk_gather_*         permutation of records / features into memory order
+#include "common.h"
+#include <float.h>
+
+// ---------------------------------------------------------------------------
+// nearest_cell_assign on one point per wave.
+//   ct_c  = (|c|^2/2 - <x,c>) + |x|^2/2,  coarse visited in (ct, c) order
+//   score = (ct - <x,f>) + stationary[c][f], fine in stored order, strict '<'
+//   break when sqrt(best) - fbound[f] < sqrt(ct)   (fbound = -sqrt(|f|^2/2))
+// Coarse clusters whose f = 0 check already breaks under the best found in
+// the nearest coarse cluster are never visited by the reference either; the
+// remaining ones are replayed exactly in (ct, c) order.
+__device__ __forceinline__ void pk_wave_argmin(float& v, uint32_t& idx) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        float ov = __shfl_xor(v, o, 64);
+        uint32_t oi = __shfl_xor(idx, o, 64);
+        if (ov < v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+    }
+}
+
+// Process one coarse cluster: returns updated (best, cell).  A wave covers
+// 4 * 64 consecutive fine centers per step: each lane scores 4 of them
+// (sequential running minimum), an exclusive wave scan of the lane minima
+// gives the best-so-far seen by every position, and the first position whose
+// triangle check fails ends the cluster.
+__device__ void pk_assign_one_coarse(uint32_t c, float ct, const float* __restrict__ ipf, const float* __restrict__ st,
+                                     const float* __restrict__ fbound, uint32_t F, float& best, int32_t& cell) {
+    const int lane = threadIdx.x & 63;
+    const float sct = __fsqrt_rn(ct);
+    const float* strow = st + (uint64_t)c * F;
+    for (uint32_t base = 0; base < F; base += 256) {
+        const uint32_t f0 = base + 4 * lane;
+        float sc[4];
+        bool val[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t f = f0 + e;
+            val[e] = f < F;
+            sc[e] = val[e] ? (ct - ipf[f]) + strow[f] : FLT_MAX;
+        }
+        // lane-local running minimum and the lane minimum
+        float lmin = sc[0];
+#pragma unroll
+        for (int e = 1; e < 4; ++e) lmin = sc[e] < lmin ? sc[e] : lmin;
+        // exclusive scan of lane minima
+        float incl = lmin;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            float y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl = y < incl ? y : incl;
+        }
+        float excl = __shfl_up(incl, 1, 64);
+        float run = lane == 0 ? best : (excl < best ? excl : best);
+        // first failing triangle check inside this lane (4 = none)
+        int bl = 4;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (bl == 4 && val[e] && (__fsqrt_rn(run) - fbound[f0 + e] < sct)) bl = e;
+            run = sc[e] < run ? sc[e] : run;
+        }
+        uint64_t bm = __ballot(bl < 4);
+        uint32_t blane = bm ? (uint32_t)__ffsll((unsigned long long)bm) - 1 : 64u;
+        uint32_t bpos = blane < 64 ? 4 * blane + (uint32_t)__shfl(bl, (int)blane, 64) : 256u;
+        // first occurrence of the minimum over positions before the break
+        float v = FLT_MAX;
+        uint32_t vi = 0xffffffffu;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            uint32_t pos = 4 * lane + e;
+            if (val[e] && pos < bpos && (vi == 0xffffffffu || sc[e] < v)) { v = sc[e]; vi = pos; }
+        }
+        pk_wave_argmin(v, vi);
+        if (vi != 0xffffffffu && v < best) {
+            best = v;
+            cell = (int32_t)(c * F + base + vi);
+        }
+        if (bpos < 256) return;
+    }
+}
+
+__global__ __launch_bounds__(256) void pk_assign_cells(const float* __restrict__ ipc, const float* __restrict__ ipf,
+                                                       const float* __restrict__ xn, const float* __restrict__ cn,
+                                                       const float* __restrict__ fbound,
+                                                       const float* __restrict__ st, uint32_t n, uint32_t C,
+                                                       uint32_t F, uint32_t* __restrict__ cell_out,
+                                                       float* __restrict__ dist_out) {
+    extern __shared__ __align__(16) unsigned char asmem[];
+    float* cand_ct = (float*)asmem;                       // 4 waves x C
+    uint32_t* cand_c = (uint32_t*)(asmem + sizeof(float) * 4 * C);
+    const uint32_t pt = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (pt >= n) return;
+    const float* pc = ipc + (uint64_t)pt * C;
+    const float* pf = ipf + (uint64_t)pt * F;
+    const float x2 = xn[pt];
+    // nearest coarse by (ct, c)
+    float bv = FLT_MAX;
+    uint32_t bi = 0xffffffffu;
+    for (uint32_t c = lane; c < C; c += 64) {
+        float ct = (cn[c] - pc[c]) + x2;
+        if (bi == 0xffffffffu || ct < bv) { bv = ct; bi = c; }
+    }
+    pk_wave_argmin(bv, bi);
+    float best = FLT_MAX;
+    int32_t cell = -1;
+    const uint32_t c0 = bi;
+    const float ct0 = bv;
+    pk_assign_one_coarse(c0, ct0, pf, st, fbound, F, best, cell);
+    // coarse clusters that survive the f = 0 check under the current best,
+    // compacted into this wave's LDS list, then visited in (ct, c) order
+    const float lim = __fsqrt_rn(best) - fbound[0];
+    float* cct = cand_ct + (threadIdx.x >> 6) * C;
+    uint32_t* cci = cand_c + (threadIdx.x >> 6) * C;
+    uint32_t K = 0;
+    for (uint32_t c0b = 0; c0b < C; c0b += 64) {
+        const uint32_t c = c0b + lane;
+        bool keep = false;
+        float ct = 0.f;
+        if (c < C && c != c0) {
+            ct = (cn[c] - pc[c]) + x2;
+            bool after = ct > ct0 || (ct == ct0 && c > c0);
+            keep = after && !(lim < __fsqrt_rn(ct));   // else it breaks at f = 0
+        }
+        uint64_t bm = __ballot(keep);
+        if (keep) {
+            uint32_t pos = K + (uint32_t)__popcll(bm & ((1ull << lane) - 1ull));
+            cct[pos] = ct;
+            cci[pos] = c;
+        }
+        K += (uint32_t)__popcll(bm);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t done_n = 0; done_n < K; ++done_n) {
+        float nv = FLT_MAX;
+        uint32_t ni = 0xffffffffu, np = 0xffffffffu;
+        for (uint32_t r = lane; r < K; r += 64) {
+            const uint32_t c = cci[r];
+            if (c == 0xffffffffu) continue;
+            const float ct = cct[r];
+            if (ni == 0xffffffffu || ct < nv || (ct == nv && c < ni)) { nv = ct; ni = c; np = r; }
+        }
+        // wave argmin carrying the list position
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            float ov = __shfl_xor(nv, o, 64);
+            uint32_t oi = __shfl_xor(ni, o, 64);
+            uint32_t op = __shfl_xor(np, o, 64);
+            if (ov < nv || (ov == nv && oi < ni)) { nv = ov; ni = oi; np = op; }
+        }
+        if (ni == 0xffffffffu) break;
+        if (lane == 0) cci[np] = 0xffffffffu;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        pk_assign_one_coarse(ni, nv, pf, st, fbound, F, best, cell);
+    }
+    if (lane == 0) {
+        cell_out[pt] = (uint32_t)cell;
+        if (dist_out) dist_out[pt] = best;
+    }
+}
+
+extern "C" hipError_t pk_launch_assign_cells(const float* ipc, const float* ipf, const float* xn, const float* cn,
+                                             const float* fbound, const float* st, uint32_t n, uint32_t C,
+                                             uint32_t F, uint32_t* cell_out, float* dist_out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    size_t smem = 8ull * 4 * C;
+    if (smem > 160 * 1024) return hipErrorInvalidValue;
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void*)pk_assign_cells, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    hipLaunchKernelGGL(pk_assign_cells, dim3((n + 3) / 4), dim3(256), smem, s, ipc, ipf, xn, cn, fbound, st, n, C, F,
+                       cell_out, dist_out);
+    return hipGetLastError();
+}
+
+// fbound[f] = 0 - sqrt(dot(f,f)/2)   (hierarchical_cluster_index.cpp:1186-1195)
+__global__ void pk_fine_bound(const float* __restrict__ fine, uint32_t F, uint32_t dim, float* __restrict__ out) {
+    uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    const float* v = fine + (uint64_t)f * dim;
+    float h = pk_dot(v, v, dim) / 2;
+    out[f] = 0.0f - __fsqrt_rn(h);
+}
+
+extern "C" hipError_t pk_launch_fine_bound(const float* fine, uint32_t F, uint32_t dim, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(pk_fine_bound, dim3((F + 255) / 256), dim3(256), 0, s, fine, F, dim, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Filter-PQ encode: one thread per point, one subspace's codebook in LDS at a
+// time.  residual r = (x - c) - f; code_m = argmin_k (float)((double)(pn + cn_k)
+// - 2.0 * (double)dot(r_m, cb_k)) with strict '<' from FLT_MAX; offset =
+// 2 * dot(decoded, c + f), accumulated in subspace order.
+__global__ __launch_bounds__(256) void pk_encode_filter(const float* __restrict__ X, const uint32_t* __restrict__ cell,
+                                                        uint32_t n, const float* __restrict__ coarse,
+                                                        const float* __restrict__ fine, uint32_t F, uint32_t dim,
+                                                        const float* __restrict__ codebook, uint32_t nsq,
+                                                        float* __restrict__ off_out, uint8_t* __restrict__ codes_out,
+                                                        uint64_t rec_stride, uint64_t codes_offset) {
+    __shared__ float cb[PK_KS * 4];
+    __shared__ float cbn[PK_KS];
+    const uint32_t lsq = dim / nsq;
+    const uint32_t pt = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = pt < n;
+    uint32_t cc = 0, ff = 0;
+    if (valid) {
+        cc = cell[pt] / F;
+        ff = cell[pt] % F;
+    }
+    const float* x = X + (uint64_t)(valid ? pt : 0) * dim;
+    const float* cv = coarse + (uint64_t)cc * dim;
+    const float* fv = fine + (uint64_t)ff * dim;
+    float offacc = 0.0f;
+    for (uint32_t m = 0; m < nsq; ++m) {
+        __syncthreads();
+        for (uint32_t e = threadIdx.x; e < PK_KS * lsq; e += blockDim.x)
+            cb[e] = codebook[(uint64_t)m * PK_KS * lsq + e];
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < PK_KS; k += blockDim.x) cbn[k] = pk_dot(cb + k * lsq, cb + k * lsq, lsq);
+        __syncthreads();
+        if (!valid) continue;
+        float r[4];
+        for (uint32_t j = 0; j < lsq; ++j) {
+            uint32_t i = m * lsq + j;
+            r[j] = (x[i] - cv[i]) - fv[i];
+        }
+        float pn = pk_dot(r, r, lsq);
+        float best = FLT_MAX;
+        uint32_t arg = 0;
+        for (uint32_t k = 0; k < PK_KS; ++k) {
+            float dt = pk_dot(r, cb + k * lsq, lsq);
+            float d = (float)((double)(pn + cbn[k]) - 2.0 * (double)dt);
+            if (d < best) { best = d; arg = k; }
+        }
+        codes_out[(uint64_t)pt * rec_stride + codes_offset + m] = (uint8_t)arg;
+        for (uint32_t j = 0; j < lsq; ++j) {
+            uint32_t i = m * lsq + j;
+            float v = cv[i] + fv[i];
+            offacc = fmaf(cb[arg * lsq + j], v, offacc);
+        }
+    }
+    if (valid) {
+        float off = 2 * offacc;
+        if (off_out) off_out[pt] = off;
+        else {
+            uint8_t* dst = codes_out + (uint64_t)pt * rec_stride;
+            uint32_t b = __float_as_uint(off);
+            dst[0] = b & 0xff; dst[1] = (b >> 8) & 0xff; dst[2] = (b >> 16) & 0xff; dst[3] = (b >> 24) & 0xff;
+        }
+    }
+}
+
+extern "C" hipError_t pk_launch_encode_filter(const float* X, const uint32_t* cell, uint32_t n, const float* coarse,
+                                              const float* fine, uint32_t F, uint32_t dim, const float* codebook,
+                                              uint32_t nsq, float* off_out, uint8_t* codes_out,
+                                              uint64_t rec_stride, uint64_t codes_offset, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (nsq == 0 || dim % nsq != 0 || dim / nsq > 4) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pk_encode_filter, dim3((n + 255) / 256), dim3(256), 0, s, X, cell, n, coarse, fine, F, dim,
+                       codebook, nsq, off_out, codes_out, rec_stride, codes_offset);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// nearest_center argmin over a chunk of GEMM inner products ip[n x k]:
+//   d = (float)((double)(pn + cn[c]) - 2.0 * (double)ip), strict '<' from FLT_MAX.
+__global__ __launch_bounds__(256) void pk_kmeans_argmin(const float* __restrict__ ip, const float* __restrict__ pn,
+                                                        const float* __restrict__ cn, uint32_t n, uint32_t k,
+                                                        int32_t* __restrict__ assign, float* __restrict__ dis) {
+    const uint32_t pt = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (pt >= n) return;
+    const float p = pn[pt];
+    const float* row = ip + (uint64_t)pt * k;
+    float bv = FLT_MAX;
+    uint32_t bi = 0xffffffffu;
+    for (uint32_t c = lane; c < k; c += 64) {
+        float d = (float)((double)(p + cn[c]) - 2.0 * (double)row[c]);
+        if (d < bv) { bv = d; bi = c; }
+    }
+    pk_wave_argmin(bv, bi);
+    if (lane == 0) {
+        assign[pt] = bi == 0xffffffffu ? 0 : (int32_t)bi;
+        if (dis) dis[pt] = bv;
+    }
+}
+
+extern "C" hipError_t pk_launch_kmeans_argmin(const float* ip, const float* pn, const float* cn, uint32_t n,
+                                              uint32_t k, int32_t* assign, float* dis, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(pk_kmeans_argmin, dim3((n + 3) / 4), dim3(256), 0, s, ip, pn, cn, n, k, assign, dis);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Empty cells get stationary = sqrt(FLT_MAX); min_dist_offset[c] = min over the
+// coarse cluster's cells (starting from FLT_MAX).  One block per coarse cluster.
+__global__ __launch_bounds__(256) void pk_finalize_cells(float* __restrict__ st, const uint32_t* __restrict__ cs,
+                                                         uint32_t F, float* __restrict__ mdo) {
+    __shared__ float red[256];
+    const uint32_t c = blockIdx.x;
+    float mn = FLT_MAX;
+    const float empty = __fsqrt_rn(FLT_MAX);
+    for (uint32_t f = threadIdx.x; f < F; f += blockDim.x) {
+        uint64_t i = (uint64_t)c * F + f;
+        float v = st[i];
+        if (cs[i] == cs[i + 1]) {
+            v = empty;
+            st[i] = v;
+        }
+        mn = v < mn ? v : mn;
+    }
+    red[threadIdx.x] = mn;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            float b = red[threadIdx.x + o];
+            if (b < red[threadIdx.x]) red[threadIdx.x] = b;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) mdo[c] = red[0];
+}
+
+extern "C" hipError_t pk_launch_finalize_cells(float* st, const uint32_t* cs, uint32_t C, uint32_t F, float* mdo,
+                                               hipStream_t s) {
+    hipLaunchKernelGGL(pk_finalize_cells, dim3(C), dim3(256), 0, s, st, cs, F, mdo);
+    return hipGetLastError();
+}
+
+// Gather rows into memory order: dst[m] = src[mem2local[m]] (row of `bytes`).
+__global__ void pk_gather_rows(const uint8_t* __restrict__ src, uint64_t src_stride, uint64_t src_off,
+                               uint8_t* __restrict__ dst, uint64_t dst_stride, uint32_t bytes,
+                               const uint32_t* __restrict__ mem2local, uint32_t n) {
+    uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t m = gid / bytes;
+    uint32_t b = (uint32_t)(gid % bytes);
+    if (m >= n) return;
+    dst[m * dst_stride + b] = src[(uint64_t)mem2local[m] * src_stride + src_off + b];
+}
+
+__global__ void pk_gather_f32(const float* __restrict__ src, uint32_t width, float* __restrict__ dst,
+                              const uint32_t* __restrict__ mem2local, uint32_t n) {
+    uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t m = gid / width;
+    uint32_t b = (uint32_t)(gid % width);
+    if (m >= n) return;
+    dst[m * width + b] = src[(uint64_t)mem2local[m] * width + b];
+}
+
+extern "C" hipError_t pk_launch_gather_rows(const uint8_t* src, uint64_t src_stride, uint64_t src_off, uint8_t* dst,
+                                            uint64_t dst_stride, uint32_t bytes, const uint32_t* mem2local,
+                                            uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    uint64_t total = (uint64_t)n * bytes;
+    hipLaunchKernelGGL(pk_gather_rows, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, src, src_stride,
+                       src_off, dst, dst_stride, bytes, mem2local, n);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t pk_launch_gather_f32(const float* src, uint32_t width, float* dst, const uint32_t* mem2local,
+                                           uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    uint64_t total = (uint64_t)n * width;
+    hipLaunchKernelGGL(pk_gather_f32, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, src, width, dst,
+                       mem2local, n);
+    return hipGetLastError();
+}

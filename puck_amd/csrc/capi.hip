@@ -1,0 +1,698 @@
+// capi.hip -- the C ABI of libpuckgpu.so (declared in include/puck_gpu.h).
+// Host-side orchestration of the gfx950 kernels: context, index load, batched
+// search, build-time assignment and encoding.
+#include "common.h"
+#include "../../include/puck_gpu.h"
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+// ---- kernel launchers (defined in gemm/sort/search/build.hip)
+struct PkWalkArgs;
+struct PkRerankArgs;
+extern "C" {
+hipError_t pk_launch_gemm_exact(const float*, const float*, float*, uint32_t, uint32_t, uint32_t, uint64_t, int,
+                                const float*, hipStream_t);
+hipError_t pk_launch_row_norms(const float*, uint32_t, uint32_t, int, float*, hipStream_t);
+hipError_t pk_launch_sort_rows(const float*, uint32_t, uint32_t, uint64_t, uint32_t, float*, uint32_t*, uint8_t*,
+                               hipStream_t);
+hipError_t pk_launch_exact_center_order(const float*, uint32_t, const float*, uint32_t, uint32_t, const uint32_t*,
+                                        uint32_t, float*, uint32_t*, float*, uint32_t*, hipStream_t);
+hipError_t pk_launch_prep_queries(const float*, uint32_t, uint32_t, int, float*, float*, int32_t*, hipStream_t);
+hipError_t pk_launch_lut(const float*, uint32_t, uint32_t, const float*, uint32_t, uint32_t, float*, hipStream_t);
+hipError_t pk_launch_walk(const PkWalkArgs*, uint32_t, hipStream_t);
+hipError_t pk_launch_rerank(const PkRerankArgs*, uint32_t, hipStream_t);
+uint32_t pk_walk_ftk_max(void);
+hipError_t pk_launch_assign_cells(const float*, const float*, const float*, const float*, const float*,
+                                  const float*, uint32_t, uint32_t, uint32_t, uint32_t*, float*, hipStream_t);
+hipError_t pk_launch_fine_bound(const float*, uint32_t, uint32_t, float*, hipStream_t);
+hipError_t pk_launch_encode_filter(const float*, const uint32_t*, uint32_t, const float*, const float*, uint32_t,
+                                   uint32_t, const float*, uint32_t, float*, uint8_t*, uint64_t, uint64_t,
+                                   hipStream_t);
+hipError_t pk_launch_kmeans_argmin(const float*, const float*, const float*, uint32_t, uint32_t, int32_t*, float*,
+                                   hipStream_t);
+hipError_t pk_launch_finalize_cells(float*, const uint32_t*, uint32_t, uint32_t, float*, hipStream_t);
+hipError_t pk_launch_gather_rows(const uint8_t*, uint64_t, uint64_t, uint8_t*, uint64_t, uint32_t,
+                                 const uint32_t*, uint32_t, hipStream_t);
+hipError_t pk_launch_gather_f32(const float*, uint32_t, float*, const uint32_t*, uint32_t, hipStream_t);
+}
+
+enum { PK_EPI_RAW = 0, PK_EPI_SUB_FROM_BIAS = 1, PK_EPI_NEG = 2, PK_EPI_ADD_BIAS = 3 };
+
+// Mirrors the struct of the same name in search.hip.
+struct PkWalkArgs {
+    const float* q;
+    const float* qn;
+    const int32_t* qstatus;
+    const float* csel_d;
+    const uint32_t* csel_id;
+    const float* fsort_d;
+    const uint32_t* fsort_id;
+    const float* codebook;
+    const float* stationary;
+    const uint32_t* cell_start;
+    const float* mdo;
+    const float* rec_off;
+    const uint8_t* rec_codes;
+    uint32_t dim, C, F, nsq, lsq, scc, ftk;
+    double rr;
+    float* out_t;
+    uint32_t* out_id;
+    uint32_t* out_n;
+    uint32_t* stats;
+    const uint32_t* qlist;
+    uint8_t* flags;
+    int exact;
+};
+struct PkRerankArgs {
+    const float* q;
+    const int32_t* qstatus;
+    const float* filt_t;
+    const uint32_t* filt_id;
+    const uint32_t* filt_n;
+    const float* raw;
+    const uint32_t* mem2local;
+    uint32_t dim, ftk, topk;
+    float* out_d;
+    uint32_t* out_id;
+    uint32_t* out_n;
+    const uint32_t* qlist;
+    uint8_t* flags;
+    int exact;
+};
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+}  // namespace
+
+struct puckgpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<DevBuf> bufs;          // named scratch slots
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    float stage_ms[5] = {0, 0, 0, 0, 0};
+    uint64_t counters[5] = {0, 0, 0, 0, 0};
+    uint32_t last_n = 0, last_ftk = 0;
+};
+
+struct puckgpu_index {
+    puckgpu_index_params p{};
+    uint32_t lsq = 0;
+    float* coarse = nullptr;
+    float* fine = nullptr;
+    float* codebook = nullptr;
+    float* coarse_norms = nullptr;
+    float* stationary = nullptr;
+    uint32_t* cell_start = nullptr;
+    float* mdo = nullptr;
+    uint32_t* mem2local = nullptr;
+    float* rec_off = nullptr;
+    uint8_t* rec_codes = nullptr;
+    float* raw = nullptr;
+};
+
+namespace {
+
+int set_err(puckgpu_ctx* ctx, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    return code;
+}
+
+#define PK_CHECK(expr)                                                                                   \
+    do {                                                                                                 \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess)                                                                            \
+            return set_err(ctx, PUCKGPU_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),   \
+                           __FILE__, __LINE__);                                                          \
+    } while (0)
+
+// scratch slot i, at least `bytes` long
+void* scratch(puckgpu_ctx* ctx, size_t slot, size_t bytes, int* rc) {
+    if (ctx->bufs.size() <= slot) ctx->bufs.resize(slot + 1);
+    DevBuf& b = ctx->bufs[slot];
+    if (b.cap < bytes) {
+        if (b.p) (void)hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+        size_t want = bytes < 256 ? 256 : bytes;
+        if (hipMalloc(&b.p, want) != hipSuccess) {
+            *rc = set_err(ctx, PUCKGPU_ENOMEM, "hipMalloc(%zu) failed for scratch slot %zu", want, slot);
+            return nullptr;
+        }
+        b.cap = want;
+    }
+    return b.p;
+}
+
+enum Slot {
+    S_Q = 0, S_QN, S_QST, S_CD, S_FD, S_CSD, S_CSI, S_FSD, S_FSI, S_FT, S_FI, S_FN, S_OD, S_OI, S_ON, S_STATS,
+    S_IN_Q, S_A0, S_A1, S_A2, S_A3, S_A4, S_A5, S_A6, S_A7, S_FLAGS, S_QLIST, S_COUNT
+};
+
+// Device view of an input array: the pointer itself, or a scratch copy of it.
+const void* dev_in(puckgpu_ctx* ctx, size_t slot, const void* p, size_t bytes, int device_ptrs, int* rc) {
+    if (device_ptrs || bytes == 0) return p;
+    void* d = scratch(ctx, slot, bytes, rc);
+    if (!d) return nullptr;
+    if (hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+        *rc = set_err(ctx, PUCKGPU_EHIP, "host-to-device copy of %zu bytes failed", bytes);
+        return nullptr;
+    }
+    return d;
+}
+
+template <typename T>
+int dalloc(puckgpu_ctx* ctx, T** p, size_t count) {
+    size_t bytes = sizeof(T) * (count ? count : 1);
+    if (hipMalloc((void**)p, bytes) != hipSuccess)
+        return set_err(ctx, PUCKGPU_ENOMEM, "hipMalloc(%zu) failed", bytes);
+    return 0;
+}
+
+void index_release(puckgpu_index* ix) {
+    if (!ix) return;
+    void* ptrs[] = {ix->coarse, ix->fine, ix->codebook, ix->coarse_norms, ix->stationary, ix->cell_start,
+                    ix->mdo, ix->mem2local, ix->rec_off, ix->rec_codes, ix->raw};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete ix;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t puckgpu_filter_topk_max(void) { return pk_walk_ftk_max(); }
+
+int puckgpu_create(int device, puckgpu_ctx** out) {
+    if (!out) return PUCKGPU_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return PUCKGPU_EHIP;
+    if (device < 0 || device >= n) return PUCKGPU_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return PUCKGPU_EHIP;
+    puckgpu_ctx* ctx = new puckgpu_ctx();
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return PUCKGPU_EHIP;
+    }
+    for (auto& e : ctx->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete ctx;
+            return PUCKGPU_EHIP;
+        }
+    *out = ctx;
+    return PUCKGPU_OK;
+}
+
+void puckgpu_destroy(puckgpu_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& b : ctx->bufs)
+        if (b.p) (void)hipFree(b.p);
+    for (auto& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* puckgpu_last_error(const puckgpu_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+// ---------------------------------------------------------------------------
+int puckgpu_index_load(puckgpu_ctx* ctx, const puckgpu_index_params* params, const float* coarse, const float* fine,
+                       const float* filter_codebook, const uint32_t* cell_assign, const uint8_t* filter_records,
+                       const float* features, int device_ptrs, puckgpu_index** out) {
+    if (!ctx || !params || !out) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    *out = nullptr;
+    const puckgpu_index_params p = *params;
+    if (p.dim == 0 || p.coarse_cluster_count == 0 || p.fine_cluster_count == 0 || p.filter_nsq == 0)
+        return set_err(ctx, PUCKGPU_EINVAL, "index geometry must be non-zero");
+    if (p.dim % p.filter_nsq != 0 || !(p.dim / p.filter_nsq == 1 || p.dim / p.filter_nsq == 2 ||
+                                       p.dim / p.filter_nsq == 4))
+        return set_err(ctx, PUCKGPU_EINVAL, "dim / filter_nsq must be 1, 2 or 4 (dim=%u nsq=%u)", p.dim,
+                       p.filter_nsq);
+    const uint64_t cells = (uint64_t)p.coarse_cluster_count * p.fine_cluster_count;
+    if (cells >= 0xffffffffull) return set_err(ctx, PUCKGPU_EINVAL, "C*F too large");
+    if (p.fine_cluster_count > 8192 || p.coarse_cluster_count > 8192)
+        return set_err(ctx, PUCKGPU_EINVAL, "C and F must be <= 8192");
+    if (!coarse || !fine || !filter_codebook || (p.total_point_count && (!cell_assign || !filter_records || !features)))
+        return set_err(ctx, PUCKGPU_EINVAL, "null array");
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    const uint32_t C = p.coarse_cluster_count, F = p.fine_cluster_count, d = p.dim, nsq = p.filter_nsq,
+                   N = p.total_point_count;
+    const uint32_t lsq = d / nsq;
+    const size_t rec = 4 + nsq;
+    hipStream_t s = ctx->stream;
+    int rc = 0;
+
+    puckgpu_index* ix = new puckgpu_index();
+    ix->p = p;
+    ix->lsq = lsq;
+#define PK_TRY(x)                  \
+    do {                           \
+        int r_ = (x);              \
+        if (r_) {                  \
+            index_release(ix);     \
+            return r_;             \
+        }                          \
+    } while (0)
+#define PK_TRYH(x)                                                                                  \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess) {                                                                     \
+            index_release(ix);                                                                      \
+            return set_err(ctx, PUCKGPU_EHIP, "%s failed: %s", #x, hipGetErrorString(e_));          \
+        }                                                                                           \
+    } while (0)
+    PK_TRY(dalloc(ctx, &ix->coarse, (size_t)C * d));
+    PK_TRY(dalloc(ctx, &ix->fine, (size_t)F * d));
+    PK_TRY(dalloc(ctx, &ix->codebook, (size_t)nsq * PK_KS * lsq));
+    PK_TRY(dalloc(ctx, &ix->coarse_norms, C));
+    PK_TRY(dalloc(ctx, &ix->stationary, cells));
+    PK_TRY(dalloc(ctx, &ix->cell_start, cells + 1));
+    PK_TRY(dalloc(ctx, &ix->mdo, C));
+    PK_TRY(dalloc(ctx, &ix->mem2local, N));
+    PK_TRY(dalloc(ctx, &ix->rec_off, N));
+    PK_TRY(dalloc(ctx, &ix->rec_codes, (size_t)N * nsq));
+    PK_TRY(dalloc(ctx, &ix->raw, (size_t)N * d));
+    const hipMemcpyKind kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    PK_TRYH(hipMemcpyAsync(ix->coarse, coarse, sizeof(float) * C * d, kind, s));
+    PK_TRYH(hipMemcpyAsync(ix->fine, fine, sizeof(float) * F * d, kind, s));
+    PK_TRYH(hipMemcpyAsync(ix->codebook, filter_codebook, sizeof(float) * nsq * PK_KS * lsq, kind, s));
+
+    // |c|^2/2, |f|^2/2 and stationary[c][f] = |f|^2/2 + <f,c> (read_coodbooks)
+    float* fnorm = (float*)scratch(ctx, S_A0, sizeof(float) * F, &rc);
+    if (!fnorm) { index_release(ix); return rc; }
+    PK_TRYH(pk_launch_row_norms(ix->coarse, C, d, 1, ix->coarse_norms, s));
+    PK_TRYH(pk_launch_row_norms(ix->fine, F, d, 1, fnorm, s));
+    PK_TRYH(pk_launch_gemm_exact(ix->coarse, ix->fine, ix->stationary, C, F, d, F, PK_EPI_ADD_BIAS, fnorm, s));
+
+    // memory order: cells ascending, then offset, then local id
+    // (convert_local_to_memory_idx, puck_index.cpp:248-282)
+    std::vector<uint32_t> h_cell(N);
+    std::vector<float> h_off(N);
+    const uint8_t* rec_dev = (const uint8_t*)dev_in(ctx, S_A1, filter_records, rec * N, device_ptrs, &rc);
+    if (N && !rec_dev) { index_release(ix); return rc; }
+    const float* feat_dev = (const float*)dev_in(ctx, S_A2, features, sizeof(float) * N * d, device_ptrs, &rc);
+    if (N && !feat_dev) { index_release(ix); return rc; }
+    if (N) {
+        if (device_ptrs) {
+            PK_TRYH(hipMemcpyAsync(h_cell.data(), cell_assign, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, s));
+            // offsets: first 4 bytes of each record
+            float* offs = (float*)scratch(ctx, S_A3, sizeof(float) * N, &rc);
+            if (!offs) { index_release(ix); return rc; }
+            std::vector<uint32_t> ident;  // gather with identity map via a device iota
+            uint32_t* iota = (uint32_t*)scratch(ctx, S_A4, sizeof(uint32_t) * N, &rc);
+            if (!iota) { index_release(ix); return rc; }
+            ident.resize(N);
+            for (uint32_t i = 0; i < N; ++i) ident[i] = i;
+            PK_TRYH(hipMemcpyAsync(iota, ident.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice, s));
+            PK_TRYH(pk_launch_gather_rows(rec_dev, rec, 0, (uint8_t*)offs, 4, 4, iota, N, s));
+            PK_TRYH(hipMemcpyAsync(h_off.data(), offs, sizeof(float) * N, hipMemcpyDeviceToHost, s));
+            PK_TRYH(hipStreamSynchronize(s));
+        } else {
+            std::memcpy(h_cell.data(), cell_assign, sizeof(uint32_t) * N);
+            for (uint32_t i = 0; i < N; ++i) std::memcpy(&h_off[i], filter_records + (size_t)i * rec, 4);
+        }
+    }
+    std::vector<uint32_t> h_cs(cells + 1, 0);
+    for (uint32_t i = 0; i < N; ++i) {
+        if (h_cell[i] >= cells) {
+            index_release(ix);
+            return set_err(ctx, PUCKGPU_EINVAL, "cell_assign[%u] = %u out of range", i, h_cell[i]);
+        }
+        h_cs[h_cell[i] + 1]++;
+    }
+    for (uint64_t c = 0; c < cells; ++c) h_cs[c + 1] += h_cs[c];
+    std::vector<uint32_t> h_m2l(N);
+    {
+        std::vector<uint32_t> fill(h_cs.begin(), h_cs.end() - 1);
+        for (uint32_t i = 0; i < N; ++i) h_m2l[fill[h_cell[i]]++] = i;  // stable by local id
+        for (uint64_t c = 0; c < cells; ++c) {
+            uint32_t b = h_cs[c], e = h_cs[c + 1];
+            if (e - b > 1)
+                std::stable_sort(h_m2l.begin() + b, h_m2l.begin() + e,
+                                 [&](uint32_t x, uint32_t y) { return h_off[x] < h_off[y]; });
+        }
+    }
+    PK_TRYH(hipMemcpyAsync(ix->cell_start, h_cs.data(), sizeof(uint32_t) * (cells + 1), hipMemcpyHostToDevice, s));
+    if (N) PK_TRYH(hipMemcpyAsync(ix->mem2local, h_m2l.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice, s));
+    PK_TRYH(pk_launch_finalize_cells(ix->stationary, ix->cell_start, C, F, ix->mdo, s));
+    if (N) {
+        PK_TRYH(pk_launch_gather_rows(rec_dev, rec, 0, (uint8_t*)ix->rec_off, 4, 4, ix->mem2local, N, s));
+        PK_TRYH(pk_launch_gather_rows(rec_dev, rec, 4, ix->rec_codes, nsq, nsq, ix->mem2local, N, s));
+        PK_TRYH(pk_launch_gather_f32(feat_dev, d, ix->raw, ix->mem2local, N, s));
+    }
+    PK_TRYH(hipStreamSynchronize(s));
+    // host staging copies of records / features are no longer needed
+    if (!device_ptrs)
+        for (size_t slot : {(size_t)S_A1, (size_t)S_A2})
+            if (slot < ctx->bufs.size() && ctx->bufs[slot].p) {
+                (void)hipFree(ctx->bufs[slot].p);
+                ctx->bufs[slot] = DevBuf();
+            }
+    *out = ix;
+    return PUCKGPU_OK;
+#undef PK_TRY
+#undef PK_TRYH
+}
+
+void puckgpu_index_free(puckgpu_index* index) { index_release(index); }
+
+int puckgpu_index_export(puckgpu_ctx* ctx, const puckgpu_index* ix, float* coarse_norms, float* stationary,
+                         uint32_t* cell_start, float* min_dist_offset, uint32_t* mem2local) {
+    if (!ctx || !ix) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    const uint64_t cells = (uint64_t)ix->p.coarse_cluster_count * ix->p.fine_cluster_count;
+    if (coarse_norms)
+        PK_CHECK(hipMemcpy(coarse_norms, ix->coarse_norms, 4 * ix->p.coarse_cluster_count, hipMemcpyDeviceToHost));
+    if (stationary) PK_CHECK(hipMemcpy(stationary, ix->stationary, 4 * cells, hipMemcpyDeviceToHost));
+    if (cell_start) PK_CHECK(hipMemcpy(cell_start, ix->cell_start, 4 * (cells + 1), hipMemcpyDeviceToHost));
+    if (min_dist_offset)
+        PK_CHECK(hipMemcpy(min_dist_offset, ix->mdo, 4 * ix->p.coarse_cluster_count, hipMemcpyDeviceToHost));
+    if (mem2local && ix->p.total_point_count)
+        PK_CHECK(hipMemcpy(mem2local, ix->mem2local, 4ull * ix->p.total_point_count, hipMemcpyDeviceToHost));
+    return PUCKGPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+int puckgpu_search(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_params* sp, uint32_t n,
+                   const float* queries, uint32_t topk, float* distance, uint32_t* labels, uint32_t* result_num,
+                   int device_ptrs) {
+    if (!ctx || !ix || !sp) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    if (topk == 0 || topk > sp->topk_max)
+        return set_err(ctx, PUCKGPU_EINVAL, "topk should <= topk, topk = %u, requested %u", sp->topk_max, topk);
+    if (n == 0) return PUCKGPU_OK;
+    if (!queries || !distance || !labels) return set_err(ctx, PUCKGPU_EINVAL, "null array");
+    const uint32_t ftk = sp->filter_topk;
+    if (ftk < topk || ftk == 0 || ftk > pk_walk_ftk_max())
+        return set_err(ctx, PUCKGPU_EINVAL, "filter_topk %u must be in [topk=%u, %u]", ftk, topk, pk_walk_ftk_max());
+    if (!(sp->radius_rate >= 1.0)) return set_err(ctx, PUCKGPU_EINVAL, "radius_rate should >= 1.0");
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    const uint32_t d = ix->p.dim, C = ix->p.coarse_cluster_count, F = ix->p.fine_cluster_count,
+                   nsq = ix->p.filter_nsq;
+    const uint32_t scc = sp->search_coarse_count < C ? sp->search_coarse_count : C;
+    hipStream_t s = ctx->stream;
+    int rc = 0;
+    const uint32_t QB = 16384;  // queries per device pass
+    const uint32_t nb = n < QB ? n : QB;
+    float* q = (float*)scratch(ctx, S_Q, sizeof(float) * nb * d, &rc);
+    float* qn = (float*)scratch(ctx, S_QN, sizeof(float) * nb, &rc);
+    int32_t* qst = (int32_t*)scratch(ctx, S_QST, sizeof(int32_t) * nb, &rc);
+    float* cdist = (float*)scratch(ctx, S_CD, sizeof(float) * nb * C, &rc);
+    float* fdist = (float*)scratch(ctx, S_FD, sizeof(float) * nb * F, &rc);
+    float* csd = (float*)scratch(ctx, S_CSD, sizeof(float) * nb * (scc ? scc : 1), &rc);
+    uint32_t* csi = (uint32_t*)scratch(ctx, S_CSI, sizeof(uint32_t) * nb * (scc ? scc : 1), &rc);
+    float* fsd = (float*)scratch(ctx, S_FSD, sizeof(float) * nb * F, &rc);
+    uint32_t* fsi = (uint32_t*)scratch(ctx, S_FSI, sizeof(uint32_t) * nb * F, &rc);
+    float* ft = (float*)scratch(ctx, S_FT, sizeof(float) * (size_t)n * ftk, &rc);
+    uint32_t* fi = (uint32_t*)scratch(ctx, S_FI, sizeof(uint32_t) * (size_t)n * ftk, &rc);
+    uint32_t* fn = (uint32_t*)scratch(ctx, S_FN, sizeof(uint32_t) * n, &rc);
+    uint32_t* stats = (uint32_t*)scratch(ctx, S_STATS, sizeof(uint32_t) * 4 * nb, &rc);
+    if (rc) return rc;
+    float* od;
+    uint32_t *oi, *on;
+    if (device_ptrs) {
+        od = distance;
+        oi = labels;
+        on = result_num ? result_num : (uint32_t*)scratch(ctx, S_ON, sizeof(uint32_t) * n, &rc);
+    } else {
+        od = (float*)scratch(ctx, S_OD, sizeof(float) * (size_t)n * topk, &rc);
+        oi = (uint32_t*)scratch(ctx, S_OI, sizeof(uint32_t) * (size_t)n * topk, &rc);
+        on = (uint32_t*)scratch(ctx, S_ON, sizeof(uint32_t) * n, &rc);
+    }
+    if (rc) return rc;
+    const float* qin = (const float*)dev_in(ctx, S_IN_Q, queries, sizeof(float) * (size_t)n * d, device_ptrs, &rc);
+    if (!qin) return rc;
+    uint8_t* flags = (uint8_t*)scratch(ctx, S_FLAGS, nb, &rc);
+    uint32_t* qlist = (uint32_t*)scratch(ctx, S_QLIST, sizeof(uint32_t) * nb, &rc);
+    if (rc) return rc;
+    for (int k = 0; k < 5; ++k) { ctx->stage_ms[k] = 0; ctx->counters[k] = 0; }
+    std::vector<uint32_t> hstats(4 * (size_t)nb);
+    std::vector<uint8_t> hflags(nb);
+    std::vector<uint32_t> hlist;
+    for (uint32_t q0 = 0; q0 < n; q0 += QB) {
+        const uint32_t m = (n - q0) < QB ? (n - q0) : QB;
+        PK_CHECK(hipMemsetAsync(flags, 0, m, s));
+        PK_CHECK(hipEventRecord(ctx->ev[0], s));
+        PK_CHECK(pk_launch_prep_queries(qin + (size_t)q0 * d, m, d, ix->p.whether_norm, q, qn, qst, s));
+        PK_CHECK(pk_launch_gemm_exact(q, ix->coarse, cdist, m, C, d, C, PK_EPI_SUB_FROM_BIAS, ix->coarse_norms, s));
+        PK_CHECK(pk_launch_gemm_exact(q, ix->fine, fdist, m, F, d, F, PK_EPI_NEG, nullptr, s));
+        PK_CHECK(hipEventRecord(ctx->ev[1], s));
+        if (scc) PK_CHECK(pk_launch_sort_rows(cdist, m, C, C, scc, csd, csi, flags, s));
+        PK_CHECK(pk_launch_sort_rows(fdist, m, F, F, F, fsd, fsi, nullptr, s));
+        PK_CHECK(hipEventRecord(ctx->ev[2], s));
+        PkWalkArgs wa;
+        wa.q = q; wa.qn = qn; wa.qstatus = qst; wa.csel_d = csd; wa.csel_id = csi; wa.fsort_d = fsd;
+        wa.fsort_id = fsi; wa.codebook = ix->codebook; wa.stationary = ix->stationary;
+        wa.cell_start = ix->cell_start; wa.mdo = ix->mdo; wa.rec_off = ix->rec_off; wa.rec_codes = ix->rec_codes;
+        wa.dim = d; wa.C = C; wa.F = F; wa.nsq = nsq; wa.lsq = ix->lsq; wa.scc = scc; wa.ftk = ftk;
+        wa.rr = sp->radius_rate; wa.out_t = ft + (size_t)q0 * ftk; wa.out_id = fi + (size_t)q0 * ftk;
+        wa.out_n = fn + q0; wa.stats = stats; wa.qlist = nullptr; wa.flags = flags; wa.exact = 0;
+        PK_CHECK(pk_launch_walk(&wa, m, s));
+        PK_CHECK(hipEventRecord(ctx->ev[3], s));
+        PkRerankArgs ra;
+        ra.q = q; ra.qstatus = qst; ra.filt_t = wa.out_t; ra.filt_id = wa.out_id; ra.filt_n = wa.out_n;
+        ra.raw = ix->raw; ra.mem2local = ix->mem2local; ra.dim = d; ra.ftk = ftk; ra.topk = topk;
+        ra.out_d = od + (size_t)q0 * topk; ra.out_id = oi + (size_t)q0 * topk; ra.out_n = on + q0;
+        ra.qlist = nullptr; ra.flags = flags; ra.exact = 0;
+        PK_CHECK(pk_launch_rerank(&ra, m, s));
+        PK_CHECK(hipEventRecord(ctx->ev[4], s));
+        PK_CHECK(hipMemcpyAsync(hstats.data(), stats, sizeof(uint32_t) * 4 * m, hipMemcpyDeviceToHost, s));
+        PK_CHECK(hipMemcpyAsync(hflags.data(), flags, m, hipMemcpyDeviceToHost, s));
+        PK_CHECK(hipStreamSynchronize(s));
+        // Queries whose result depends on the order among exactly equal
+        // distances are replayed with the reference heaps.
+        // flags: 1 coarse-distance tie in the kept range, 2 filter-boundary tie
+        // at the final top, 4 exact-distance tie in the top-k, 8 a decision on
+        // a fine-distance tie group.  1 and 8 need the reference's center order.
+        hlist.clear();
+        for (uint32_t i = 0; i < m; ++i)
+            if (hflags[i] & 9u) hlist.push_back(i);
+        const uint32_t ncenter = (uint32_t)hlist.size();
+        for (uint32_t i = 0; i < m; ++i)
+            if (hflags[i] && !(hflags[i] & 9u)) hlist.push_back(i);
+        if (!hlist.empty()) {
+            const uint32_t nl = (uint32_t)hlist.size();
+            PK_CHECK(hipMemcpyAsync(qlist, hlist.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s));
+            PK_CHECK(pk_launch_exact_center_order(cdist, C, fdist, F, scc, qlist, ncenter, csd, csi, fsd, fsi, s));
+            wa.qlist = qlist; wa.flags = nullptr; wa.exact = 1;
+            PK_CHECK(pk_launch_walk(&wa, nl, s));
+            ra.qlist = qlist; ra.flags = nullptr; ra.exact = 1;
+            PK_CHECK(pk_launch_rerank(&ra, nl, s));
+        }
+        PK_CHECK(hipEventRecord(ctx->ev[5], s));
+        PK_CHECK(hipStreamSynchronize(s));
+        for (int k = 0; k < 5; ++k) {
+            float ms = 0;
+            PK_CHECK(hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]));
+            ctx->stage_ms[k] += ms;
+        }
+        for (uint32_t i = 0; i < m; ++i)
+            for (int k = 0; k < 4; ++k) ctx->counters[k] += hstats[4 * (size_t)i + k];
+        ctx->counters[4] += hlist.size();
+    }
+    if (!device_ptrs) {
+        PK_CHECK(hipMemcpyAsync(distance, od, sizeof(float) * (size_t)n * topk, hipMemcpyDeviceToHost, s));
+        PK_CHECK(hipMemcpyAsync(labels, oi, sizeof(uint32_t) * (size_t)n * topk, hipMemcpyDeviceToHost, s));
+        if (result_num) PK_CHECK(hipMemcpyAsync(result_num, on, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+        PK_CHECK(hipStreamSynchronize(s));
+    }
+    ctx->last_n = n;
+    ctx->last_ftk = ftk;
+    return PUCKGPU_OK;
+}
+
+int puckgpu_last_filter(puckgpu_ctx* ctx, uint32_t n, uint32_t ftk, float* filt_dist, uint32_t* filt_mem,
+                        uint32_t* filt_n) {
+    if (!ctx) return PUCKGPU_EINVAL;
+    if (n != ctx->last_n || ftk != ctx->last_ftk) return set_err(ctx, PUCKGPU_EINVAL, "no matching last search");
+    if (ctx->bufs.size() <= S_FN) return set_err(ctx, PUCKGPU_EINVAL, "no last search");
+    if (filt_dist) PK_CHECK(hipMemcpy(filt_dist, ctx->bufs[S_FT].p, 4ull * n * ftk, hipMemcpyDeviceToHost));
+    if (filt_mem) PK_CHECK(hipMemcpy(filt_mem, ctx->bufs[S_FI].p, 4ull * n * ftk, hipMemcpyDeviceToHost));
+    if (filt_n) PK_CHECK(hipMemcpy(filt_n, ctx->bufs[S_FN].p, 4ull * n, hipMemcpyDeviceToHost));
+    return PUCKGPU_OK;
+}
+
+int puckgpu_last_stage_ms(puckgpu_ctx* ctx, float* ms5, uint64_t* counters5) {
+    if (!ctx) return PUCKGPU_EINVAL;
+    for (int k = 0; k < 5; ++k) {
+        if (ms5) ms5[k] = ctx->stage_ms[k];
+        if (counters5) counters5[k] = ctx->counters[k];
+    }
+    return PUCKGPU_OK;
+}
+
+int puckgpu_lut(puckgpu_ctx* ctx, uint32_t n, uint32_t dim, const float* queries, const float* codebook,
+                uint32_t nsq, float* out, int device_ptrs) {
+    if (!ctx || !queries || !codebook || !out) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    if (nsq == 0 || dim % nsq || !(dim / nsq == 1 || dim / nsq == 2 || dim / nsq == 4))
+        return set_err(ctx, PUCKGPU_EINVAL, "dim / nsq must be 1, 2 or 4");
+    if (n == 0) return PUCKGPU_OK;
+    int rc = 0;
+    const float* qd = (const float*)dev_in(ctx, S_A0, queries, 4ull * n * dim, device_ptrs, &rc);
+    const float* cb = (const float*)dev_in(ctx, S_A1, codebook, 4ull * nsq * PK_KS * (dim / nsq), device_ptrs, &rc);
+    float* o = device_ptrs ? out : (float*)scratch(ctx, S_A2, 4ull * n * nsq * PK_KS, &rc);
+    if (rc) return rc;
+    PK_CHECK(pk_launch_lut(qd, n, dim, cb, nsq, dim / nsq, o, ctx->stream));
+    if (!device_ptrs) PK_CHECK(hipMemcpyAsync(out, o, 4ull * n * nsq * PK_KS, hipMemcpyDeviceToHost, ctx->stream));
+    PK_CHECK(hipStreamSynchronize(ctx->stream));
+    return PUCKGPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+int puckgpu_assign_cells(puckgpu_ctx* ctx, const float* coarse, const float* fine, uint32_t C, uint32_t F,
+                         uint32_t dim, uint32_t n, const float* x, uint32_t* cell_out, float* dist_out,
+                         int device_ptrs) {
+    if (!ctx || !coarse || !fine || !cell_out || (n && !x)) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    if (C == 0 || F == 0 || dim == 0) return set_err(ctx, PUCKGPU_EINVAL, "empty geometry");
+    if ((uint64_t)C * F >= 0xffffffffull) return set_err(ctx, PUCKGPU_EINVAL, "C*F too large");
+    if (n == 0) return PUCKGPU_OK;
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    hipStream_t s = ctx->stream;
+    int rc = 0;
+    const float* cd = (const float*)dev_in(ctx, S_A0, coarse, 4ull * C * dim, device_ptrs, &rc);
+    const float* fd = (const float*)dev_in(ctx, S_A1, fine, 4ull * F * dim, device_ptrs, &rc);
+    float* cn = (float*)scratch(ctx, S_A2, 4ull * C, &rc);
+    float* fnorm = (float*)scratch(ctx, S_A3, 4ull * F, &rc);
+    float* fb = (float*)scratch(ctx, S_A4, 4ull * F, &rc);
+    float* st = (float*)scratch(ctx, S_A5, 4ull * C * F, &rc);
+    if (rc) return rc;
+    PK_CHECK(pk_launch_row_norms(cd, C, dim, 1, cn, s));
+    PK_CHECK(pk_launch_row_norms(fd, F, dim, 1, fnorm, s));
+    PK_CHECK(pk_launch_fine_bound(fd, F, dim, fb, s));
+    PK_CHECK(pk_launch_gemm_exact(cd, fd, st, C, F, dim, F, PK_EPI_ADD_BIAS, fnorm, s));
+    const uint32_t CHK = 32768;
+    const uint32_t cb = n < CHK ? n : CHK;
+    float* ipc = (float*)scratch(ctx, S_A6, 4ull * cb * C, &rc);
+    float* ipf = (float*)scratch(ctx, S_A7, 4ull * cb * F, &rc);
+    float* xn = (float*)scratch(ctx, S_QN, 4ull * cb, &rc);
+    float* xs = (float*)scratch(ctx, S_Q, 4ull * cb * dim, &rc);
+    uint32_t* co = (uint32_t*)scratch(ctx, S_OI, 4ull * cb, &rc);
+    float* cdo = (float*)scratch(ctx, S_OD, 4ull * cb, &rc);
+    if (rc) return rc;
+    for (uint32_t p0 = 0; p0 < n; p0 += CHK) {
+        const uint32_t m = (n - p0) < CHK ? (n - p0) : CHK;
+        const float* xp;
+        if (device_ptrs) xp = x + (size_t)p0 * dim;
+        else {
+            PK_CHECK(hipMemcpyAsync(xs, x + (size_t)p0 * dim, 4ull * m * dim, hipMemcpyHostToDevice, s));
+            xp = xs;
+        }
+        PK_CHECK(pk_launch_row_norms(xp, m, dim, 1, xn, s));
+        PK_CHECK(pk_launch_gemm_exact(xp, cd, ipc, m, C, dim, C, PK_EPI_RAW, nullptr, s));
+        PK_CHECK(pk_launch_gemm_exact(xp, fd, ipf, m, F, dim, F, PK_EPI_RAW, nullptr, s));
+        uint32_t* cout = device_ptrs ? cell_out + p0 : co;
+        float* dout = device_ptrs ? (dist_out ? dist_out + p0 : nullptr) : cdo;
+        PK_CHECK(pk_launch_assign_cells(ipc, ipf, xn, cn, fb, st, m, C, F, cout, dout, s));
+        if (!device_ptrs) {
+            PK_CHECK(hipMemcpyAsync(cell_out + p0, co, 4ull * m, hipMemcpyDeviceToHost, s));
+            if (dist_out) PK_CHECK(hipMemcpyAsync(dist_out + p0, cdo, 4ull * m, hipMemcpyDeviceToHost, s));
+            PK_CHECK(hipStreamSynchronize(s));
+        }
+    }
+    PK_CHECK(hipStreamSynchronize(s));
+    return PUCKGPU_OK;
+}
+
+int puckgpu_encode_filter(puckgpu_ctx* ctx, const float* coarse, const float* fine, uint32_t C, uint32_t F,
+                          uint32_t dim, const float* codebook, uint32_t nsq, uint32_t n, const float* x,
+                          const uint32_t* cell, uint8_t* records_out, int device_ptrs) {
+    if (!ctx || !coarse || !fine || !codebook || !records_out || (n && (!x || !cell)))
+        return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    if (nsq == 0 || dim % nsq || dim / nsq > 4) return set_err(ctx, PUCKGPU_EINVAL, "dim / nsq must be <= 4");
+    if (n == 0) return PUCKGPU_OK;
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    hipStream_t s = ctx->stream;
+    int rc = 0;
+    if (C == 0 || F == 0 || (uint64_t)C * F >= 0xffffffffull) return set_err(ctx, PUCKGPU_EINVAL, "bad C/F");
+    if (!device_ptrs)
+        for (uint32_t i = 0; i < n; ++i)
+            if (cell[i] >= C * F) return set_err(ctx, PUCKGPU_EINVAL, "cell[%u] = %u out of range", i, cell[i]);
+    const float* cd = (const float*)dev_in(ctx, S_A0, coarse, 4ull * C * dim, device_ptrs, &rc);
+    const float* fd = (const float*)dev_in(ctx, S_A1, fine, 4ull * F * dim, device_ptrs, &rc);
+    const float* cbd = (const float*)dev_in(ctx, S_A2, codebook, 4ull * nsq * PK_KS * (dim / nsq), device_ptrs, &rc);
+    if (rc) return rc;
+    const uint64_t rec = 4 + nsq;
+    const uint32_t CHK = 1u << 20;
+    const uint32_t cbn = n < CHK ? n : CHK;
+    float* xs = device_ptrs ? nullptr : (float*)scratch(ctx, S_A3, 4ull * cbn * dim, &rc);
+    uint32_t* cs = device_ptrs ? nullptr : (uint32_t*)scratch(ctx, S_A4, 4ull * cbn, &rc);
+    uint8_t* ro = device_ptrs ? nullptr : (uint8_t*)scratch(ctx, S_A5, rec * cbn, &rc);
+    if (rc) return rc;
+    for (uint32_t p0 = 0; p0 < n; p0 += CHK) {
+        const uint32_t m = (n - p0) < CHK ? (n - p0) : CHK;
+        if (device_ptrs) {
+            PK_CHECK(pk_launch_encode_filter(x + (size_t)p0 * dim, cell + p0, m, cd, fd, F, dim, cbd, nsq, nullptr,
+                                             records_out + (size_t)p0 * rec, rec, 4, s));
+        } else {
+            PK_CHECK(hipMemcpyAsync(xs, x + (size_t)p0 * dim, 4ull * m * dim, hipMemcpyHostToDevice, s));
+            PK_CHECK(hipMemcpyAsync(cs, cell + p0, 4ull * m, hipMemcpyHostToDevice, s));
+            PK_CHECK(pk_launch_encode_filter(xs, cs, m, cd, fd, F, dim, cbd, nsq, nullptr, ro, rec, 4, s));
+            PK_CHECK(hipMemcpyAsync(records_out + (size_t)p0 * rec, ro, rec * m, hipMemcpyDeviceToHost, s));
+            PK_CHECK(hipStreamSynchronize(s));
+        }
+    }
+    PK_CHECK(hipStreamSynchronize(s));
+    return PUCKGPU_OK;
+}
+
+int puckgpu_kmeans_assign(puckgpu_ctx* ctx, uint32_t dim, const float* centroids, uint32_t k, uint32_t n,
+                          const float* x, int32_t* assign_out, float* dist_out, int device_ptrs) {
+    if (!ctx || !centroids || !assign_out || (n && !x)) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    if (k == 0 || dim == 0) return set_err(ctx, PUCKGPU_EINVAL, "empty geometry");
+    if (n == 0) return PUCKGPU_OK;
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    hipStream_t s = ctx->stream;
+    int rc = 0;
+    const float* cd = (const float*)dev_in(ctx, S_A0, centroids, 4ull * k * dim, device_ptrs, &rc);
+    float* cn = (float*)scratch(ctx, S_A1, 4ull * k, &rc);
+    if (rc) return rc;
+    PK_CHECK(pk_launch_row_norms(cd, k, dim, 0, cn, s));
+    uint32_t CHK = (uint32_t)std::max<uint64_t>(1024, std::min<uint64_t>(1u << 20, (256ull << 20) / (4ull * k)));
+    const uint32_t cb = n < CHK ? n : CHK;
+    float* ip = (float*)scratch(ctx, S_A2, 4ull * cb * k, &rc);
+    float* pn = (float*)scratch(ctx, S_A3, 4ull * cb, &rc);
+    float* xs = device_ptrs ? nullptr : (float*)scratch(ctx, S_A4, 4ull * cb * dim, &rc);
+    int32_t* ao = device_ptrs ? nullptr : (int32_t*)scratch(ctx, S_A5, 4ull * cb, &rc);
+    float* dd = device_ptrs ? nullptr : (float*)scratch(ctx, S_A6, 4ull * cb, &rc);
+    if (rc) return rc;
+    for (uint32_t p0 = 0; p0 < n; p0 += CHK) {
+        const uint32_t m = (n - p0) < CHK ? (n - p0) : CHK;
+        const float* xp = x + (size_t)p0 * dim;
+        if (!device_ptrs) {
+            PK_CHECK(hipMemcpyAsync(xs, xp, 4ull * m * dim, hipMemcpyHostToDevice, s));
+            xp = xs;
+        }
+        PK_CHECK(pk_launch_row_norms(xp, m, dim, 0, pn, s));
+        PK_CHECK(pk_launch_gemm_exact(xp, cd, ip, m, k, dim, k, PK_EPI_RAW, nullptr, s));
+        int32_t* a = device_ptrs ? assign_out + p0 : ao;
+        float* dv = device_ptrs ? (dist_out ? dist_out + p0 : nullptr) : dd;
+        PK_CHECK(pk_launch_kmeans_argmin(ip, pn, cn, m, k, a, dv, s));
+        if (!device_ptrs) {
+            PK_CHECK(hipMemcpyAsync(assign_out + p0, ao, 4ull * m, hipMemcpyDeviceToHost, s));
+            if (dist_out) PK_CHECK(hipMemcpyAsync(dist_out + p0, dd, 4ull * m, hipMemcpyDeviceToHost, s));
+            PK_CHECK(hipStreamSynchronize(s));
+        }
+    }
+    PK_CHECK(hipStreamSynchronize(s));
+    return PUCKGPU_OK;
+}
+
+}  // extern "C"

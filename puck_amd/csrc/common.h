@@ -1,0 +1,126 @@
+// common.h -- device helpers shared by the gfx950 kernels of libpuckgpu.
+//
+// Arithmetic convention: every source is compiled with -ffp-contract=off, so
+// a*b+c is two rounded operations unless fmaf() is written.  Dot products are
+// sequential fmaf chains from +0.0f (the engine's exact-arithmetic order); the
+// lookup-table and rerank orders follow the reference's SIMD kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PK_KS 256
+
+// 0x7f7f7f7f: the value MaxHeap's memset(0x7f) leaves in an empty slot.
+__host__ __device__ inline float pk_sentinel() {
+    union { uint32_t u; float f; } v;
+    v.u = 0x7f7f7f7fu;
+    return v.f;
+}
+
+// Order-preserving float <-> uint32 map (total order of non-NaN floats).
+__device__ __forceinline__ uint32_t pk_f2ord(float f) {
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float pk_ord2f(uint32_t o) {
+    uint32_t b = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+    return __uint_as_float(b);
+}
+
+// Sequential FMA dot product (the engine's exact order).
+__device__ __forceinline__ float pk_dot(const float* a, const float* b, uint32_t n) {
+    float acc = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) acc = fmaf(a[i], b[i], acc);
+    return acc;
+}
+
+// Lookup-table entry ||x - y||^2 for a sub-vector of length 1, 2 or 4, with
+// the summation order of faiss fvec_L2sqr_ny_D1/D2/D4
+// (puck/puck/distances_simd.cpp:72-133): rounded squares, then pairwise sums.
+__device__ __forceinline__ float pk_lut_entry(const float* x, const float* y, uint32_t lsq) {
+    if (lsq == 1) {
+        float d0 = x[0] - y[0];
+        return d0 * d0;
+    }
+    if (lsq == 2) {
+        float d0 = x[0] - y[0], d1 = x[1] - y[1];
+        return d0 * d0 + d1 * d1;
+    }
+    float d0 = x[0] - y[0], d1 = x[1] - y[1], d2 = x[2] - y[2], d3 = x[3] - y[3];
+    return (d0 * d0 + d1 * d1) + (d2 * d2 + d3 * d3);
+}
+
+// Pivot (puck_index.cpp:517, :544): (top - |q|^2) / radius_rate / 2.0 with the
+// subtraction in float and the divisions in double, stored as float.
+__device__ __forceinline__ float pk_pivot(float top, float qn, double rr) {
+    return (float)((double)(top - qn) / rr / 2.0);
+}
+
+// Exact squared L2 in the order GCC -Ofast -march=native gives the reference's
+// rerank loop (L2Sqr16Ext/L2SqrExt, hnsw_distfunc_opt_impl_inline.h:41-122):
+//   per 16-block  l_j = fma(d_j, d_j, d_{8+j}^2);
+//                 t = (((l0+l1)+(l2+l3)) + ((l4+l5)+l6)) + (l7 + t)
+//   tail (<16)    4-lane fma blocks, (b0+b1)+(b2+b3), then a scalar fma chain.
+__device__ __forceinline__ float pk_rerank_dist(const float* __restrict__ x, const float* q, uint32_t dim) {
+    float t = 0.0f;
+    uint32_t m = 0;
+    for (; m + 16 <= dim; m += 16) {
+        float l[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            float d0 = x[m + j] - q[m + j];
+            float d1 = x[m + 8 + j] - q[m + 8 + j];
+            l[j] = fmaf(d0, d0, d1 * d1);
+        }
+        t = (((l[0] + l[1]) + (l[2] + l[3])) + ((l[4] + l[5]) + l[6])) + (l[7] + t);
+    }
+    uint32_t left = dim - m;
+    if (left > 0) {
+        float b[4] = {0.f, 0.f, 0.f, 0.f};
+        uint32_t i = 0;
+        for (; i + 4 <= left; i += 4)
+            for (int j = 0; j < 4; ++j) {
+                float d = x[m + i + j] - q[m + i + j];
+                b[j] = fmaf(d, d, b[j]);
+            }
+        float s = (b[0] + b[1]) + (b[2] + b[3]);
+        for (; i < left; ++i) {
+            float d = x[m + i] - q[m + i];
+            s = fmaf(d, d, s);
+        }
+        t = t + s;
+    }
+    return t;
+}
+
+// Block-wide exclusive scan of one uint32 per thread (blockDim.x <= 1024,
+// multiple of 64).  `tmp` needs blockDim.x/64 + 1 words of LDS.  Returns the
+// exclusive prefix; *total receives the block sum.  Must be called by every
+// thread of the block.
+__device__ __forceinline__ uint32_t pk_block_exscan(uint32_t v, uint32_t* tmp, uint32_t* total) {
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) tmp[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int w = 0; w < nw; ++w) {
+            uint32_t t = tmp[w];
+            tmp[w] = acc;
+            acc += t;
+        }
+        tmp[nw] = acc;
+    }
+    __syncthreads();
+    uint32_t r = tmp[wid] + x - v;
+    *total = tmp[nw];
+    __syncthreads();
+    return r;
+}

@@ -1,0 +1,867 @@
+// search.hip -- the per-query stages of PuckIndex::search on gfx950.
+//
+//   pk_prep_queries   normalisation (hierarchical_cluster_index.cpp:1513-1536)
+//                     and |q|^2 (puck_index.cpp:515)
+//   pk_lut            filter lookup table (quantization.cpp:260-270), test entry
+//   pk_walk           lookup table in LDS + pivot-pruned cell walk + filter-PQ
+//                     scan into the top-filter_topk set (puck_index.cpp:485-557,
+//                     :346-390), one workgroup per query
+//   pk_rerank         exact FP32 rerank of the filter survivors into top-k
+//                     (puck_index.cpp:392-457, whether_pq=false)
+//
+// pk_walk reproduces the reference's sequential walk exactly while doing the
+// per-point work in parallel.  The reference visits cells in (coarse rank l,
+// fine rank idx) order; a cell is skipped by `break` (fd + min_dist >= pivot)
+// or `continue` (stationary >= max_stationary), and the pivot and
+// max_stationary only decrease.  For a chunk of 256 cells the kernel
+//   1. computes every point value t = fl(2*cell_dist + offset) + LUT sum of the
+//      cells that can still be scanned, keeping those below the current top;
+//   2. bounds the pivot from below with P_low, the pivot the chunk would
+//      reach if all of those values were accepted;
+//   3. classifies each cell as sure-scanned (no break under P_low and
+//      stationary < min(M, P_low - cd - fd_prev)), sure-skipped (no break
+//      under P_low and stationary >= M) or uncertain;
+//   4. merges the values of the longest sure prefix into the sorted top set in
+//      one step, then replays the first uncertain cell exactly with the now
+//      exact state, and repeats from the next cell.
+// Because the final filter heap of the reference is the multiset of the
+// filter_topk smallest values offered (arrival order only breaks exact ties),
+// merging a sure segment at once gives the same set as the reference's
+// one-by-one heap updates.  Exact ties are broken by arrival order.
+#include "common.h"
+
+#define WALK_THREADS 256
+#define WALK_CH 256      // cells per chunk (one per thread)
+#define WALK_PMAX 512    // points buffered per chunk
+#define WALK_FTK_MAX 1024
+
+struct PkWalkArgs {
+    const float* q;          // nq x dim (normalised when whether_norm)
+    const float* qn;         // nq      |q|^2
+    const int32_t* qstatus;  // nq      0 = searchable
+    const float* csel_d;     // nq x scc
+    const uint32_t* csel_id; // nq x scc
+    const float* fsort_d;    // nq x F
+    const uint32_t* fsort_id;
+    const float* codebook;   // nsq x 256 x lsq
+    const float* stationary; // C x F (empty cell: sqrt(FLT_MAX))
+    const uint32_t* cell_start; // C*F + 1
+    const float* mdo;        // C
+    const float* rec_off;    // N
+    const uint8_t* rec_codes;// N x nsq
+    uint32_t dim, C, F, nsq, lsq, scc, ftk;
+    double rr;
+    float* out_t;            // nq x ftk
+    uint32_t* out_id;        // nq x ftk
+    uint32_t* out_n;         // nq
+    uint32_t* stats;         // nq x 4 or null
+    const uint32_t* qlist;   // block -> query index (null: identity)
+    uint8_t* flags;          // nq: bit 1 set when a filter-boundary tie was resolved
+    int exact;               // replay the reference MaxHeap (insertion order and ties)
+};
+
+// ---------------------------------------------------------------------------
+__global__ void pk_prep_queries(const float* __restrict__ qin, uint32_t nq, uint32_t dim, int whether_norm,
+                                float* __restrict__ qout, float* __restrict__ qn,
+                                int32_t* __restrict__ status) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const float* x = qin + (uint64_t)i * dim;
+    float* y = qout + (uint64_t)i * dim;
+    int st = 0;
+    if (whether_norm) {
+        float norm = __fsqrt_rn(pk_dot(x, x, dim));
+        if (norm < 1e-6f) {
+            st = -1;
+            for (uint32_t k = 0; k < dim; ++k) y[k] = x[k];
+        } else {
+            float alpha = (float)(1.0 / (double)norm);
+            for (uint32_t k = 0; k < dim; ++k) y[k] = alpha * x[k];
+        }
+    } else {
+        for (uint32_t k = 0; k < dim; ++k) y[k] = x[k];
+    }
+    qn[i] = pk_dot(y, y, dim);
+    status[i] = st;
+}
+
+extern "C" hipError_t pk_launch_prep_queries(const float* qin, uint32_t nq, uint32_t dim, int whether_norm,
+                                             float* qout, float* qn, int32_t* status, hipStream_t s) {
+    if (nq == 0) return hipSuccess;
+    hipLaunchKernelGGL(pk_prep_queries, dim3((nq + 127) / 128), dim3(128), 0, s, qin, nq, dim, whether_norm,
+                       qout, qn, status);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+__global__ void pk_lut(const float* __restrict__ q, uint32_t dim, const float* __restrict__ codebook,
+                       uint32_t nsq, uint32_t lsq, float* __restrict__ lut) {
+    const uint64_t qi = blockIdx.y;
+    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nsq * PK_KS) return;
+    uint32_t m = e / PK_KS;
+    lut[qi * nsq * PK_KS + e] = pk_lut_entry(q + qi * dim + m * lsq, codebook + (uint64_t)e * lsq, lsq);
+}
+
+extern "C" hipError_t pk_launch_lut(const float* q, uint32_t nq, uint32_t dim, const float* codebook,
+                                    uint32_t nsq, uint32_t lsq, float* lut, hipStream_t s) {
+    if (nq == 0) return hipSuccess;
+    hipLaunchKernelGGL(pk_lut, dim3((nsq * PK_KS + 255) / 256, nq), dim3(256), 0, s, q, dim, codebook, nsq, lsq,
+                       lut);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Walk kernel helpers.  All are called by every thread of the block with
+// block-uniform arguments.
+
+struct PkWalkSmem {
+    uint64_t* S_key;   // ftk      sorted top set keys (t bits << 32 | seq)
+    uint32_t* S_id;    // ftk      memory ids
+    uint64_t* W_key;   // PMAX     chunk candidates (sorted after build)
+    uint32_t* P_id;    // PMAX     memory id by local point index
+    uint16_t* P_cell;  // PMAX     chunk cell by local point index
+    uint16_t* X_pos;   // PMAX     selected W positions for a merge
+    uint8_t* W_done;   // PMAX
+    float* c_fd;       // CH
+    float* c_st;       // CH
+    uint32_t* c_b;     // CH
+    uint32_t* c_pre;   // CH + 1
+    uint8_t* c_cls;    // CH
+    float* lut;        // nsq*256
+    float* qv;         // dim
+    uint32_t* red;     // 16 words of reduction scratch
+    float* P_t;        // PMAX     value by local point index
+    uint8_t* P_sel;    // PMAX     selected for the current merge (exact mode)
+    float* H_val;      // ftk      reference heap (exact mode), slot i <-> index i+1
+    uint32_t* H_id;    // ftk
+};
+
+// The reference MaxHeap on LDS arrays (max_heap.cpp:52-100), run by one thread.
+__device__ void pk_heap_insert(float* hv, uint32_t* ht, uint32_t heap_size, uint32_t father, float v, uint32_t t) {
+    float* V = hv - 1;
+    uint32_t* Tg = ht - 1;
+    for (;;) {
+        uint32_t l = father << 1;
+        if (l > heap_size) break;
+        uint32_t r = l + 1;
+        if (r > heap_size || V[l] > V[r]) {
+            if (V[l] < v) break;
+            V[father] = V[l];
+            Tg[father] = Tg[l];
+            father = l;
+        } else {
+            if (V[r] < v) break;
+            V[father] = V[r];
+            Tg[father] = Tg[r];
+            father = r;
+        }
+    }
+    V[father] = v;
+    Tg[father] = t;
+}
+
+__device__ void pk_heap_update(float* hv, uint32_t* ht, uint32_t size, uint32_t* default_cnt, float v, uint32_t t) {
+    uint32_t father = 1;
+    if (v >= hv[0]) return;
+    if (*default_cnt > 0) father = (*default_cnt)--;
+    pk_heap_insert(hv, ht, size, father, v, t);
+}
+
+__device__ void pk_heap_reorder(float* hv, uint32_t* ht, uint32_t size) {
+    for (uint32_t i = 0; i < size; i++) {
+        float vb = hv[0];
+        uint32_t tb = ht[0];
+        uint32_t hs = size - i;
+        pk_heap_insert(hv, ht, hs - 1, 1, hv[hs - 1], ht[hs - 1]);
+        hv[hs - 1] = vb;
+        ht[hs - 1] = tb;
+    }
+}
+
+__device__ __forceinline__ uint32_t pk_bmin(uint32_t v, uint32_t* red) {
+    if (threadIdx.x == 0) red[0] = 0xffffffffu;
+    __syncthreads();
+    if (v != 0xffffffffu) atomicMin(&red[0], v);
+    __syncthreads();
+    uint32_t r = red[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ uint32_t pk_bmax(uint32_t v, uint32_t* red) {
+    if (threadIdx.x == 0) red[1] = 0u;
+    __syncthreads();
+    if (v != 0u) atomicMax(&red[1], v);
+    __syncthreads();
+    uint32_t r = red[1];
+    __syncthreads();
+    return r;
+}
+
+// Values of local points [p_lo, p_hi) of the chunk's active cells; keeps those
+// below `thr`.  Returns the number kept (W_key[0..n)).
+__device__ uint32_t pk_walk_values(const PkWalkArgs& a, PkWalkSmem& s, uint32_t cend, uint32_t p_lo,
+                                   uint32_t p_hi, float cd, float thr, uint32_t seqbase) {
+    if (threadIdx.x == 0) s.red[2] = 0;
+    __syncthreads();
+    const uint32_t nsq = a.nsq;
+    for (uint32_t p = p_lo + threadIdx.x; p < p_hi; p += blockDim.x) {
+        // cell containing p: last t with c_pre[t] <= p (c_pre[cend] = total)
+        uint32_t lo = 0, hi = cend;  // invariant c_pre[lo] <= p < c_pre[hi]
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (s.c_pre[mid] <= p) lo = mid; else hi = mid;
+        }
+        const uint32_t t = lo;
+        const uint32_t mid = s.c_b[t] + (p - s.c_pre[t]);
+        const float cell_dist = (cd + s.c_st[t]) + s.c_fd[t];
+        const float off = a.rec_off[mid];
+        float tv = 2.0f * cell_dist + off;   // 2x is exact: one rounding, as in double
+        if (!(tv < thr)) continue;
+        const uint8_t* codes = a.rec_codes + (uint64_t)mid * nsq;
+        float acc[4] = {0.f, 0.f, 0.f, 0.f};
+        if ((nsq & 15) == 0) {
+            for (uint32_t g = 0; g < nsq; g += 16) {
+                uint4 w = *(const uint4*)(codes + g);
+                uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        uint32_t code = (ws[h] >> (8 * j)) & 0xffu;
+                        acc[j] = acc[j] + s.lut[(g + 4 * h + j) * PK_KS + code];
+                    }
+                }
+            }
+        } else {
+            for (uint32_t m = 0; m < nsq; ++m) acc[m & 3] = acc[m & 3] + s.lut[m * PK_KS + codes[m]];
+        }
+        tv = tv + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
+        if (!(tv < thr)) continue;
+        uint32_t slot = atomicAdd(&s.red[2], 1u);
+        s.W_key[slot] = ((uint64_t)pk_f2ord(tv) << 32) | (uint64_t)(seqbase + p);
+        s.P_id[p - p_lo] = mid;
+        s.P_cell[p - p_lo] = (uint16_t)t;
+        s.P_t[p - p_lo] = tv;
+    }
+    __syncthreads();
+    uint32_t n = s.red[2];
+    __syncthreads();
+    // sort W_key[0..n) ascending
+    if (n > 1) {
+        uint32_t n2 = 2;
+        while (n2 < n) n2 <<= 1;
+        for (uint32_t i = n + threadIdx.x; i < n2; i += blockDim.x) s.W_key[i] = ~0ull;
+        __syncthreads();
+        for (uint32_t k = 2; k <= n2; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = threadIdx.x; i < (n2 >> 1); i += blockDim.x) {
+                    uint32_t lo = 2 * i - (i & (j - 1));
+                    uint32_t hi = lo + j;
+                    bool up = (lo & k) == 0;
+                    uint64_t x = s.W_key[lo], y = s.W_key[hi];
+                    if ((x > y) == up) {
+                        s.W_key[lo] = y;
+                        s.W_key[hi] = x;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s.W_done[i] = 0;
+    __syncthreads();
+    return n;
+}
+
+// Number of keys in sorted arr[0..n) strictly below key.
+__device__ __forceinline__ uint32_t pk_lower_bound(const uint64_t* arr, uint32_t n, uint64_t key) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (arr[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// Selects W positions r < nW with !W_done[r] and cell (by P_cell) in [clo, chi)
+// and c_cls[cell] == want (want == 0xff: any cell, used for big cells and T_low).
+// Compacts them into X_pos; returns the count.
+__device__ uint32_t pk_walk_select(PkWalkSmem& s, uint32_t nW, uint32_t clo, uint32_t chi, uint32_t want,
+                                   uint32_t seqbase, uint32_t p_lo) {
+    uint32_t total_sel = 0;
+    for (uint32_t base = 0; base < nW; base += blockDim.x) {
+        uint32_t r = base + threadIdx.x;
+        uint32_t sel = 0;
+        if (r < nW && !s.W_done[r]) {
+            uint32_t p = (uint32_t)(s.W_key[r] & 0xffffffffu) - seqbase - p_lo;
+            uint32_t cell = s.P_cell[p];
+            if (cell >= clo && cell < chi && (want == 0xffu || s.c_cls[cell] == want)) sel = 1;
+        }
+        uint32_t tot;
+        uint32_t pre = pk_block_exscan(sel, s.red + 4, &tot);
+        if (sel) s.X_pos[total_sel + pre] = (uint16_t)r;
+        total_sel += tot;
+        __syncthreads();
+    }
+    return total_sel;
+}
+
+// Merge the m selected W entries (X_pos) into the sorted top set S (ns
+// entries, capacity ftk).  Marks them consumed.  Returns the new size.
+// When dry != 0 nothing is written; instead the key of rank ftk-1 of the
+// union is stored in red[8..9] (or ~0 when the union has fewer than ftk).
+__device__ uint32_t pk_walk_merge(const PkWalkArgs& a, PkWalkSmem& s, uint32_t ns, uint32_t m,
+                                  uint32_t seqbase, uint32_t p_lo, int dry, uint32_t plen = 0) {
+    const uint32_t ftk = a.ftk;
+    if (dry && threadIdx.x == 0) {
+        s.red[8] = 0xffffffffu;
+        s.red[9] = 0xffffffffu;
+    }
+    if (m == 0) {
+        if (dry) {
+            __syncthreads();
+            if (threadIdx.x == 0 && ns >= ftk) {
+                uint64_t k = s.S_key[ftk - 1];
+                s.red[8] = (uint32_t)(k >> 32);
+                s.red[9] = (uint32_t)k;
+            }
+            __syncthreads();
+        }
+        return ns;
+    }
+    uint64_t sk[WALK_FTK_MAX / WALK_THREADS];
+    uint32_t sid[WALK_FTK_MAX / WALK_THREADS], spos[WALK_FTK_MAX / WALK_THREADS];
+    uint64_t xk[WALK_PMAX / WALK_THREADS];
+    uint32_t xid[WALK_PMAX / WALK_THREADS], xpos[WALK_PMAX / WALK_THREADS];
+    // ranks of S entries among X (X keys via W_key[X_pos[.]], sorted since W is)
+#pragma unroll
+    for (int e = 0; e < WALK_FTK_MAX / WALK_THREADS; ++e) {
+        uint32_t i = threadIdx.x + e * WALK_THREADS;
+        spos[e] = 0xffffffffu;
+        if (i < ns) {
+            uint64_t key = s.S_key[i];
+            uint32_t lo = 0, hi = m;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (s.W_key[s.X_pos[mid]] < key) lo = mid + 1; else hi = mid;
+            }
+            sk[e] = key;
+            sid[e] = s.S_id[i];
+            spos[e] = i + lo;
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < WALK_PMAX / WALK_THREADS; ++e) {
+        uint32_t r = threadIdx.x + e * WALK_THREADS;
+        xpos[e] = 0xffffffffu;
+        if (r < m) {
+            uint32_t wpos = s.X_pos[r];
+            uint64_t key = s.W_key[wpos];
+            uint32_t p = (uint32_t)(key & 0xffffffffu) - seqbase - p_lo;
+            xk[e] = key;
+            xid[e] = s.P_id[p];
+            xpos[e] = r + pk_lower_bound(s.S_key, ns, key);
+        }
+    }
+    __syncthreads();
+    const uint32_t target = ftk - 1;
+    if (dry) {
+#pragma unroll
+        for (int e = 0; e < WALK_FTK_MAX / WALK_THREADS; ++e)
+            if (spos[e] == target) { s.red[8] = (uint32_t)(sk[e] >> 32); s.red[9] = (uint32_t)sk[e]; }
+#pragma unroll
+        for (int e = 0; e < WALK_PMAX / WALK_THREADS; ++e)
+            if (xpos[e] == target) { s.red[8] = (uint32_t)(xk[e] >> 32); s.red[9] = (uint32_t)xk[e]; }
+        __syncthreads();
+        return ns;
+    }
+    // boundary tie: the first dropped key has the same distance as the last kept
+    if (threadIdx.x == 0) { s.red[10] = 0u; s.red[11] = 0u; }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < WALK_FTK_MAX / WALK_THREADS; ++e) {
+        if (spos[e] < ftk) { s.S_key[spos[e]] = sk[e]; s.S_id[spos[e]] = sid[e]; }
+        if (spos[e] == ftk) s.red[10] = (uint32_t)(sk[e] >> 32);
+        if (spos[e] == ftk - 1) s.red[11] = (uint32_t)(sk[e] >> 32);
+    }
+#pragma unroll
+    for (int e = 0; e < WALK_PMAX / WALK_THREADS; ++e) {
+        if (xpos[e] < ftk) { s.S_key[xpos[e]] = xk[e]; s.S_id[xpos[e]] = xid[e]; }
+        if (xpos[e] == ftk) s.red[10] = (uint32_t)(xk[e] >> 32);
+        if (xpos[e] == ftk - 1) s.red[11] = (uint32_t)(xk[e] >> 32);
+    }
+    for (uint32_t r = threadIdx.x; r < m; r += blockDim.x) {
+        uint32_t wpos = s.X_pos[r];
+        s.W_done[wpos] = 1;
+        if (a.exact) s.P_sel[(uint32_t)(s.W_key[wpos] & 0xffffffffu) - seqbase - p_lo] = 1;
+    }
+    __syncthreads();
+    // Smallest distance at which a boundary tie was resolved by arrival order;
+    // the reference may have kept another element of equal distance only if
+    // that distance is still the top at the end.
+    if (threadIdx.x == 0 && s.red[10] != 0u && s.red[10] == s.red[11] && s.red[10] < s.red[12])
+        s.red[12] = s.red[10];
+    if (a.exact) {
+        // replay the accepted values in arrival order through the reference heap
+        uint32_t cnt = 0;
+        for (uint32_t base = 0; base < plen; base += blockDim.x) {
+            uint32_t p = base + threadIdx.x;
+            uint32_t sel = (p < plen && s.P_sel[p]) ? 1u : 0u;
+            uint32_t tot;
+            uint32_t pre = pk_block_exscan(sel, s.red + 4, &tot);
+            if (sel) { s.X_pos[cnt + pre] = (uint16_t)p; s.P_sel[p] = 0; }
+            cnt += tot;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t dc = s.red[13];
+            for (uint32_t r = 0; r < cnt; ++r) {
+                uint32_t p = s.X_pos[r];
+                pk_heap_update(s.H_val, s.H_id, ftk, &dc, s.P_t[p], s.P_id[p]);
+            }
+            s.red[13] = dc;
+        }
+        __syncthreads();
+    }
+    uint32_t nn = ns + m;
+    return nn < ftk ? nn : ftk;
+}
+
+__device__ __forceinline__ float pk_walk_top(const PkWalkSmem& s, uint32_t ns, uint32_t ftk) {
+    return ns >= ftk ? pk_ord2f((uint32_t)(s.S_key[ftk - 1] >> 32)) : pk_sentinel();
+}
+
+__global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
+    const int tid = threadIdx.x;
+    const uint32_t ftk = a.ftk, F = a.F, nsq = a.nsq;
+    // carve LDS (8-byte arrays first)
+    PkWalkSmem s;
+    unsigned char* ptr = smem;
+    s.S_key = (uint64_t*)ptr; ptr += sizeof(uint64_t) * ((ftk + 1) & ~1u);
+    s.W_key = (uint64_t*)ptr; ptr += sizeof(uint64_t) * WALK_PMAX;
+    s.S_id = (uint32_t*)ptr; ptr += sizeof(uint32_t) * ((ftk + 3) & ~3u);
+    s.P_id = (uint32_t*)ptr; ptr += sizeof(uint32_t) * WALK_PMAX;
+    s.c_b = (uint32_t*)ptr; ptr += sizeof(uint32_t) * WALK_CH;
+    s.c_pre = (uint32_t*)ptr; ptr += sizeof(uint32_t) * (WALK_CH + 4);
+    s.red = (uint32_t*)ptr; ptr += sizeof(uint32_t) * 16;
+    s.c_fd = (float*)ptr; ptr += sizeof(float) * WALK_CH;
+    s.c_st = (float*)ptr; ptr += sizeof(float) * WALK_CH;
+    s.qv = (float*)ptr; ptr += sizeof(float) * ((a.dim + 3) & ~3u);
+    s.lut = (float*)ptr; ptr += sizeof(float) * nsq * PK_KS;
+    s.P_cell = (uint16_t*)ptr; ptr += sizeof(uint16_t) * WALK_PMAX;
+    s.X_pos = (uint16_t*)ptr; ptr += sizeof(uint16_t) * WALK_PMAX;
+    s.W_done = (uint8_t*)ptr; ptr += WALK_PMAX;
+    s.c_cls = (uint8_t*)ptr; ptr += WALK_CH;
+    s.P_t = (float*)ptr; ptr += sizeof(float) * WALK_PMAX;
+    s.P_sel = (uint8_t*)ptr; ptr += WALK_PMAX;
+    s.H_val = (float*)ptr; ptr += a.exact ? sizeof(float) * ((ftk + 3) & ~3u) : 0;
+    s.H_id = (uint32_t*)ptr; ptr += a.exact ? sizeof(uint32_t) * ((ftk + 3) & ~3u) : 0;
+
+    uint32_t st_seg = 0, st_unc = 0, st_cells = 0, st_pts = 0;
+    bool tie_dec = false;
+    for (uint32_t i = tid; i < WALK_PMAX; i += blockDim.x) s.P_sel[i] = 0;
+    if (a.exact)
+        for (uint32_t i = tid; i < ftk; i += blockDim.x) { s.H_val[i] = pk_sentinel(); s.H_id[i] = 0; }
+    if (tid == 0) { s.red[12] = 0xffffffffu; s.red[13] = ftk; }
+    __syncthreads();
+    if (a.qstatus[qi] != 0) {
+        if (tid == 0) a.out_n[qi] = 0;
+        return;
+    }
+    const float* qg = a.q + (uint64_t)qi * a.dim;
+    for (uint32_t i = tid; i < a.dim; i += blockDim.x) s.qv[i] = qg[i];
+    __syncthreads();
+    for (uint32_t e = tid; e < nsq * PK_KS; e += blockDim.x) {
+        uint32_t m = e / PK_KS;
+        s.lut[e] = pk_lut_entry(s.qv + m * a.lsq, a.codebook + (uint64_t)e * a.lsq, a.lsq);
+    }
+    __syncthreads();
+
+    const float qn = a.qn[qi];
+    const double rr = a.rr;
+    const float* fsd = a.fsort_d + (uint64_t)qi * F;
+    const uint32_t* fsi = a.fsort_id + (uint64_t)qi * F;
+    uint32_t ns = 0;
+    float T = pk_sentinel();
+    float P = pk_pivot(T, qn, rr);
+    uint32_t seqbase = 0;
+    const uint32_t scc = a.scc;
+
+    for (uint32_t l = 0; l < scc; ++l) {
+        const uint32_t c = a.csel_id[(uint64_t)qi * scc + l];
+        const float cd = a.csel_d[(uint64_t)qi * scc + l];
+        const float mind = a.mdo[c] + cd;
+        const float fd0 = fsd[0];
+        float M = (P - cd) - fd0;
+        const float* stc = a.stationary + (uint64_t)c * F;
+        const uint32_t* csc = a.cell_start + (uint64_t)c * F;
+        uint32_t j0 = 0;
+        bool done = false;
+        while (!done && j0 < F) {
+            const uint32_t j = j0 + tid;
+            const bool valid = j < F;
+            float fdj = 0.f, stj = 0.f;
+            uint32_t bj = 0, nj = 0;
+            if (valid) {
+                fdj = fsd[j];
+                uint32_t k = fsi[j];
+                stj = stc[k];
+                bj = csc[k];
+                nj = csc[k + 1] - bj;
+            }
+            const float Phi = P, Mhi = M, Thi = T;
+            const bool brk = !valid || (fdj + mind >= Phi);
+            const uint32_t Bhi = pk_bmin(brk ? (uint32_t)tid : 0xffffffffu, s.red);
+            const uint32_t bhi = Bhi == 0xffffffffu ? WALK_CH : Bhi;
+            const bool act = ((uint32_t)tid < bhi) && !(stj >= Mhi) && nj > 0;
+            const uint32_t cnt = act ? nj : 0u;
+            uint32_t tot;
+            const uint32_t pre = pk_block_exscan(cnt, s.red + 4, &tot);
+            uint32_t cut = 0xffffffffu;
+            if (act && pre + cnt > WALK_PMAX) cut = (pre == 0) ? (uint32_t)tid + 1 : (uint32_t)tid;
+            const uint32_t cutm = pk_bmin(cut, s.red);
+            const uint32_t cend = cutm < bhi ? cutm : bhi;
+            if ((uint32_t)tid < cend) {
+                s.c_fd[tid] = fdj;
+                s.c_st[tid] = stj;
+                s.c_b[tid] = bj;
+                s.c_pre[tid] = pre;
+            }
+            if ((uint32_t)tid == cend) s.c_pre[cend] = pre;
+            if (cend == WALK_CH && tid == 0) s.c_pre[WALK_CH] = tot;
+            const float fdprev0 = j0 > 0 ? fsd[j0 - 1] : fd0;
+            // equal fine distances across a chunk boundary: the reference's heap
+            // order of the tie group decides the visiting order; replay exactly
+            if (j0 > 0 && fdprev0 == fsd[j0]) tie_dec = true;
+            __syncthreads();
+            const uint32_t ptot = s.c_pre[cend];
+            const bool big = ptot > WALK_PMAX;
+            uint32_t nW = 0;
+            float Plow = Phi;
+            if (!big) {
+                nW = pk_walk_values(a, s, cend, 0, ptot, cd, Thi, seqbase);
+                uint32_t m = pk_walk_select(s, nW, 0, WALK_CH, 0xffu, seqbase, 0);
+                pk_walk_merge(a, s, ns, m, seqbase, 0, 1);
+                uint32_t hi = s.red[8], lo = s.red[9];
+                __syncthreads();
+                float Tlow = (hi == 0xffffffffu && lo == 0xffffffffu) ? pk_sentinel() : pk_ord2f(hi);
+                Plow = pk_pivot(Tlow, qn, rr);
+            }
+            uint32_t t0 = 0;
+            while (true) {
+                st_seg++;
+                const bool inr = (uint32_t)tid >= t0 && (uint32_t)tid < cend;
+                uint32_t cls = 0;  // 0 uncertain, 1 scan, 2 skip
+                if (inr) {
+                    const float f = s.c_fd[tid], sv = s.c_st[tid];
+                    // a cell whose fine distance ties with a neighbour may be
+                    // visited after it in the reference's order: bound its
+                    // max_stationary with its own fine distance
+                    const bool tied = (tid > 0 ? s.c_fd[tid - 1] == f : (j0 > 0 && fdprev0 == f)) ||
+                                      ((uint32_t)tid + 1 < cend && s.c_fd[tid + 1] == f);
+                    const float fprev = tied ? f : (tid > 0 ? s.c_fd[tid - 1] : fdprev0);
+                    const bool nobrk = f + mind < Plow;
+                    const bool forced = big && (uint32_t)tid == cend - 1;
+                    if (!forced && nobrk) {
+                        float bound = (Plow - cd) - fprev;
+                        bound = M < bound ? M : bound;
+                        if (sv >= M) cls = 2;
+                        else if (sv < bound) cls = 1;
+                    }
+                    s.c_cls[tid] = (uint8_t)cls;
+                }
+                const uint32_t u0 = pk_bmin((inr && cls == 0) ? (uint32_t)tid : 0xffffffffu, s.red);
+                const uint32_t u = u0 == 0xffffffffu ? cend : u0;
+                const uint32_t last1 = pk_bmax((inr && cls == 1 && (uint32_t)tid < u) ? (uint32_t)tid + 1 : 0u, s.red);
+                if (nW > 0) {
+                    uint32_t m = pk_walk_select(s, nW, t0, u, 1u, seqbase, 0);
+                    ns = pk_walk_merge(a, s, ns, m, seqbase, 0, 0, ptot);
+                }
+                T = pk_walk_top(s, ns, ftk);
+                P = pk_pivot(T, qn, rr);
+                if (last1 > 0) {
+                    const float cand = (P - cd) - s.c_fd[last1 - 1];
+                    M = cand < M ? cand : M;
+                }
+                if (u >= cend) break;
+                st_unc++;
+                const float fu = s.c_fd[u], su = s.c_st[u];
+                if ((u > 0 ? s.c_fd[u - 1] == fu : (j0 > 0 && fdprev0 == fu)) || (u + 1 < cend && s.c_fd[u + 1] == fu))
+                    tie_dec = true;
+                if (fu + mind >= P) { done = true; break; }
+                if (!(su >= M)) {
+                    if (big && u == cend - 1) {
+                        for (uint32_t p_lo = 0; p_lo < ptot; p_lo += WALK_PMAX) {
+                            uint32_t p_hi = p_lo + WALK_PMAX < ptot ? p_lo + WALK_PMAX : ptot;
+                            uint32_t nb = pk_walk_values(a, s, cend, p_lo, p_hi, cd, T, seqbase);
+                            uint32_t m = pk_walk_select(s, nb, 0, WALK_CH, 0xffu, seqbase, p_lo);
+                            ns = pk_walk_merge(a, s, ns, m, seqbase, p_lo, 0, p_hi - p_lo);
+                            T = pk_walk_top(s, ns, ftk);
+                        }
+                    } else if (nW > 0) {
+                        if (tid == 0) s.c_cls[u] = 3;
+                        __syncthreads();
+                        uint32_t m = pk_walk_select(s, nW, u, u + 1, 3u, seqbase, 0);
+                        ns = pk_walk_merge(a, s, ns, m, seqbase, 0, 0, ptot);
+                    }
+                    T = pk_walk_top(s, ns, ftk);
+                    P = pk_pivot(T, qn, rr);
+                    const float cand = (P - cd) - fu;
+                    M = cand < M ? cand : M;
+                }
+                t0 = u + 1;
+            }
+            st_pts += ptot;
+            st_cells += cend;
+            seqbase += ptot;
+            if (done) break;
+            if (cend == bhi && bhi < WALK_CH) break;  // sure break (or end of the fine list)
+            j0 += cend;
+        }
+    }
+    if (a.exact) {
+        // the filter heap as rank_topN_points reads it: heap order when full,
+        // heap-sorted ascending when not full (puck_index.cpp:552-554)
+        if (tid == 0 && ns < ftk) pk_heap_reorder(s.H_val, s.H_id, ftk);
+        __syncthreads();
+        for (uint32_t r = tid; r < ns; r += blockDim.x) {
+            a.out_t[(uint64_t)qi * ftk + r] = s.H_val[r];
+            a.out_id[(uint64_t)qi * ftk + r] = s.H_id[r];
+        }
+    } else {
+        for (uint32_t r = tid; r < ns; r += blockDim.x) {
+            a.out_t[(uint64_t)qi * ftk + r] = pk_ord2f((uint32_t)(s.S_key[r] >> 32));
+            a.out_id[(uint64_t)qi * ftk + r] = s.S_id[r];
+        }
+    }
+    if (tid == 0) {
+        a.out_n[qi] = ns;
+        if (a.flags && ns >= ftk && s.red[12] == (uint32_t)(s.S_key[ftk - 1] >> 32)) a.flags[qi] |= 2u;
+        if (a.flags && tie_dec) a.flags[qi] |= 8u;
+        if (a.stats) {
+            a.stats[qi * 4 + 0] = st_seg;
+            a.stats[qi * 4 + 1] = st_unc;
+            a.stats[qi * 4 + 2] = st_cells;
+            a.stats[qi * 4 + 3] = st_pts;
+        }
+    }
+}
+
+extern "C" size_t pk_walk_smem_bytes(uint32_t dim, uint32_t nsq, uint32_t ftk, int exact) {
+    size_t b = 0;
+    b += sizeof(float) * WALK_PMAX + WALK_PMAX;
+    if (exact) b += (sizeof(float) + sizeof(uint32_t)) * ((ftk + 3) & ~3u);
+    b += sizeof(uint64_t) * ((ftk + 1) & ~1u);
+    b += sizeof(uint64_t) * WALK_PMAX;
+    b += sizeof(uint32_t) * ((ftk + 3) & ~3u);
+    b += sizeof(uint32_t) * WALK_PMAX;
+    b += sizeof(uint32_t) * WALK_CH;
+    b += sizeof(uint32_t) * (WALK_CH + 4);
+    b += sizeof(uint32_t) * 16;
+    b += sizeof(float) * WALK_CH * 2;
+    b += sizeof(float) * ((dim + 3) & ~3u);
+    b += sizeof(float) * nsq * PK_KS;
+    b += sizeof(uint16_t) * WALK_PMAX * 2;
+    b += WALK_PMAX + WALK_CH;
+    return b;
+}
+
+extern "C" uint32_t pk_walk_ftk_max(void) { return WALK_FTK_MAX; }
+
+extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStream_t s) {
+    if (nq == 0) return hipSuccess;
+    if (args->ftk == 0 || args->ftk > WALK_FTK_MAX) return hipErrorInvalidValue;
+    size_t smem = pk_walk_smem_bytes(args->dim, args->nsq, args->ftk, args->exact);
+    if (smem > 160 * 1024) return hipErrorInvalidValue;
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void*)pk_walk, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(pk_walk, dim3(nq), dim3(WALK_THREADS), smem, s, *args);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+struct PkRerankArgs {
+    const float* q;          // nq x dim
+    const int32_t* qstatus;
+    const float* filt_t;     // nq x ftk
+    const uint32_t* filt_id; // memory ids
+    const uint32_t* filt_n;
+    const float* raw;        // N x dim, memory order
+    const uint32_t* mem2local;
+    uint32_t dim, ftk, topk;
+    float* out_d;            // nq x topk
+    uint32_t* out_id;
+    uint32_t* out_n;
+    const uint32_t* qlist;   // block -> query index (null: identity)
+    uint8_t* flags;          // bit 2 set when exact distances tie inside the top-k (+1)
+    int exact;               // replay the reference result heap over the filter heap order
+};
+
+#define RERANK_THREADS 256
+
+// Fast mode: distances sorted by (distance, filter rank).  Exact mode: the
+// result MaxHeap of rank_topN_points fed in filter-heap order, then reorder().
+__global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
+    const uint32_t n = a.qstatus[qi] == 0 ? a.filt_n[qi] : 0u;
+    const uint32_t ftk = a.ftk, topk = a.topk;
+    uint32_t n2 = 2;
+    while (n2 < ftk) n2 <<= 1;
+    uint64_t* keys = (uint64_t*)smem;
+    float* qv = (float*)(smem + sizeof(uint64_t) * n2);
+    float* dist = qv + ((a.dim + 3) & ~3u);
+    float* rv = dist + ((ftk + 3) & ~3u);
+    uint32_t* rt = (uint32_t*)(rv + ((topk + 3) & ~3u));
+    for (uint32_t i = threadIdx.x; i < a.dim; i += blockDim.x) qv[i] = a.q[(uint64_t)qi * a.dim + i];
+    __syncthreads();
+    for (uint32_t r = threadIdx.x; r < n2; r += blockDim.x) {
+        uint64_t key = ~0ull;
+        if (r < n) {
+            uint32_t mid = a.filt_id[(uint64_t)qi * ftk + r];
+            float d = pk_rerank_dist(a.raw + (uint64_t)mid * a.dim, qv, a.dim);
+            key = ((uint64_t)pk_f2ord(d) << 32) | r;
+            if (a.exact) dist[r] = d;
+        }
+        if (!a.exact) keys[r] = key;
+    }
+    __syncthreads();
+    if (a.exact) {
+        if (threadIdx.x == 0) {
+            for (uint32_t r = 0; r < topk; ++r) { rv[r] = pk_sentinel(); rt[r] = 0xffffffffu; }
+            uint32_t dc = topk;
+            for (uint32_t r = 0; r < n; ++r) {
+                float d = dist[r];
+                if (d < rv[0])
+                    pk_heap_update(rv, rt, topk, &dc, d, a.mem2local[a.filt_id[(uint64_t)qi * ftk + r]]);
+            }
+            pk_heap_reorder(rv, rt, topk);
+            a.out_n[qi] = topk - dc;
+        }
+        __syncthreads();
+        for (uint32_t r = threadIdx.x; r < topk; r += blockDim.x) {
+            a.out_d[(uint64_t)qi * topk + r] = rv[r];
+            a.out_id[(uint64_t)qi * topk + r] = rt[r];
+        }
+        return;
+    }
+    if (n > 1) {
+        uint32_t m2 = 2;
+        while (m2 < n) m2 <<= 1;
+        for (uint32_t k = 2; k <= m2; k <<= 1)
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = threadIdx.x; i < (m2 >> 1); i += blockDim.x) {
+                    uint32_t lo = 2 * i - (i & (j - 1));
+                    uint32_t hi = lo + j;
+                    bool up = (lo & k) == 0;
+                    uint64_t x = keys[lo], y = keys[hi];
+                    if ((x > y) == up) {
+                        keys[lo] = y;
+                        keys[hi] = x;
+                    }
+                }
+                __syncthreads();
+            }
+    }
+    const uint32_t kout = n < topk ? n : topk;
+    bool tie = false;
+    for (uint32_t r = threadIdx.x; r < topk; r += blockDim.x) {
+        float d = pk_sentinel();
+        uint32_t id = 0xffffffffu;
+        if (r < kout) {
+            uint64_t key = keys[r];
+            d = pk_ord2f((uint32_t)(key >> 32));
+            uint32_t rr = (uint32_t)(key & 0xffffffffu);
+            id = a.mem2local[a.filt_id[(uint64_t)qi * ftk + rr]];
+            if (r + 1 < n && pk_ord2f((uint32_t)(keys[r + 1] >> 32)) == d) tie = true;
+        }
+        a.out_d[(uint64_t)qi * topk + r] = d;
+        a.out_id[(uint64_t)qi * topk + r] = id;
+    }
+    if (tie && a.flags) a.flags[qi] = 4u | a.flags[qi];
+    if (threadIdx.x == 0) a.out_n[qi] = kout;
+}
+
+extern "C" hipError_t pk_launch_rerank(const PkRerankArgs* args, uint32_t nq, hipStream_t s) {
+    if (nq == 0) return hipSuccess;
+    uint32_t n2 = 2;
+    while (n2 < args->ftk) n2 <<= 1;
+    size_t smem = sizeof(uint64_t) * n2 + sizeof(float) * ((args->dim + 3) & ~3u) +
+                  sizeof(float) * ((args->ftk + 3) & ~3u) + 8 * ((args->topk + 3) & ~3u);
+    if (smem > 64 * 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pk_rerank, dim3(nq), dim3(RERANK_THREADS), smem, s, *args);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Exact center order for flagged queries: the coarse top-scc MaxHeap
+// (hierarchical_cluster_index.cpp:709-726) and the fine full heap sort
+// (puck_index.cpp:497-504), replayed by one thread on LDS copies of the rows.
+__global__ __launch_bounds__(256) void pk_exact_center_order(const float* __restrict__ cdist, uint32_t C,
+                                                             const float* __restrict__ fdist, uint32_t F,
+                                                             uint32_t scc, const uint32_t* __restrict__ qlist,
+                                                             float* __restrict__ csel_d, uint32_t* __restrict__ csel_id,
+                                                             float* __restrict__ fsort_d, uint32_t* __restrict__ fsort_id) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t qi = qlist[blockIdx.x];
+    float* row = (float*)smem;                         // max(C, F)
+    const uint32_t L = C > F ? C : F;
+    float* hv = row + L;                               // max(scc, F)
+    uint32_t* ht = (uint32_t*)(hv + L);
+    // coarse
+    for (uint32_t i = threadIdx.x; i < C; i += blockDim.x) row[i] = cdist[(uint64_t)qi * C + i];
+    __syncthreads();
+    if (threadIdx.x == 0 && scc > 0) {
+        for (uint32_t i = 0; i < scc; ++i) { hv[i] = pk_sentinel(); ht[i] = 0; }
+        uint32_t dc = scc;
+        for (uint32_t c = 0; c < C; ++c)
+            if (row[c] < hv[0]) pk_heap_update(hv, ht, scc, &dc, row[c], c);
+        pk_heap_reorder(hv, ht, scc);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < scc; i += blockDim.x) {
+        csel_d[(uint64_t)qi * scc + i] = hv[i];
+        csel_id[(uint64_t)qi * scc + i] = ht[i];
+    }
+    __syncthreads();
+    // fine
+    for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) row[i] = fdist[(uint64_t)qi * F + i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t i = 0; i < F; ++i) { hv[i] = pk_sentinel(); ht[i] = 0; }
+        uint32_t dc = F;
+        for (uint32_t k = 0; k < F; ++k) pk_heap_update(hv, ht, F, &dc, row[k], k);
+        pk_heap_reorder(hv, ht, F);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) {
+        fsort_d[(uint64_t)qi * F + i] = hv[i];
+        fsort_id[(uint64_t)qi * F + i] = ht[i];
+    }
+}
+
+extern "C" hipError_t pk_launch_exact_center_order(const float* cdist, uint32_t C, const float* fdist, uint32_t F,
+                                                   uint32_t scc, const uint32_t* qlist, uint32_t nlist,
+                                                   float* csel_d, uint32_t* csel_id, float* fsort_d,
+                                                   uint32_t* fsort_id, hipStream_t s) {
+    if (nlist == 0) return hipSuccess;
+    const uint32_t L = C > F ? C : F;
+    size_t smem = sizeof(float) * L * 3;
+    if (smem > 160 * 1024) return hipErrorInvalidValue;
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void*)pk_exact_center_order, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        attr = true;
+    }
+    hipLaunchKernelGGL(pk_exact_center_order, dim3(nlist), dim3(256), smem, s, cdist, C, fdist, F, scc, qlist,
+                       csel_d, csel_id, fsort_d, fsort_id);
+    return hipGetLastError();
+}

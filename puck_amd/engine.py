@@ -1,0 +1,210 @@
+"""Python handles over the C ABI: a device context and a loaded index.
+
+Arrays may be numpy arrays (host; copied in and out by the library) or torch
+CUDA tensors (device; used in place).  Every call goes through libpuckgpu.so.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+class PuckGpuError(RuntimeError):
+    def __init__(self, rc, msg):
+        super().__init__(f"libpuckgpu error {rc}: {msg}")
+        self.rc = rc
+
+
+def _is_dev(a):
+    return a is not None and not isinstance(a, np.ndarray) and getattr(a, "is_cuda", False)
+
+
+def _ptr(a, dtype=None):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        if dtype is not None and a.dtype != dtype:
+            raise TypeError(f"expected {dtype}, got {a.dtype}")
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("array must be C-contiguous")
+        return a.ctypes.data_as(ctypes.c_void_p)
+    if not a.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return ctypes.c_void_p(a.data_ptr())
+
+
+def _mode(*arrays):
+    devs = [_is_dev(a) for a in arrays if a is not None]
+    if any(devs) and not all(devs):
+        raise ValueError("mix of host and device arrays in one call")
+    return 1 if devs and devs[0] else 0
+
+
+class Context:
+    def __init__(self, device=0):
+        self.L = _lib.load()
+        h = ctypes.c_void_p()
+        rc = self.L.puckgpu_create(int(device), ctypes.byref(h))
+        if rc != 0:
+            raise PuckGpuError(rc, f"puckgpu_create(device={device}) failed")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.puckgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        if rc != 0:
+            msg = self.L.puckgpu_last_error(self.h)
+            raise PuckGpuError(rc, msg.decode() if msg else "")
+        return rc
+
+    def sync_torch(self, *arrays):
+        if any(_is_dev(a) for a in arrays if a is not None):
+            import torch
+            torch.cuda.synchronize()
+
+    # ---- test hooks / build entry points -------------------------------
+    def lut(self, queries, codebook, nsq):
+        n, dim = queries.shape
+        out = np.zeros((n, nsq * 256), np.float32) if not _is_dev(queries) else None
+        if out is None:
+            import torch
+            out = torch.empty((n, nsq * 256), dtype=torch.float32, device=queries.device)
+        m = _mode(queries, codebook, out)
+        self.sync_torch(queries)
+        self.check(self.L.puckgpu_lut(self.h, n, dim, _ptr(queries), _ptr(codebook), nsq, _ptr(out), m))
+        return out
+
+    def assign_cells(self, coarse, fine, x, with_dist=False):
+        C, dim = coarse.shape
+        F = fine.shape[0]
+        n = x.shape[0]
+        if _is_dev(x):
+            import torch
+            cell = torch.empty(n, dtype=torch.int32, device=x.device)
+            dist = torch.empty(n, dtype=torch.float32, device=x.device) if with_dist else None
+        else:
+            cell = np.zeros(n, np.uint32)
+            dist = np.zeros(n, np.float32) if with_dist else None
+        m = _mode(coarse, fine, x, cell)
+        self.sync_torch(x, coarse)
+        self.check(self.L.puckgpu_assign_cells(self.h, _ptr(coarse), _ptr(fine), C, F, dim, n, _ptr(x),
+                                               _ptr(cell), _ptr(dist), m))
+        return (cell, dist) if with_dist else cell
+
+    def encode_filter(self, coarse, fine, codebook, nsq, x, cell):
+        C, dim = coarse.shape
+        F = fine.shape[0]
+        n = x.shape[0]
+        if _is_dev(x):
+            import torch
+            rec = torch.empty((n, 4 + nsq), dtype=torch.uint8, device=x.device)
+        else:
+            rec = np.zeros((n, 4 + nsq), np.uint8)
+        m = _mode(coarse, fine, codebook, x, cell, rec)
+        self.sync_torch(x, cell)
+        self.check(self.L.puckgpu_encode_filter(self.h, _ptr(coarse), _ptr(fine), C, F, dim, _ptr(codebook),
+                                                nsq, n, _ptr(x), _ptr(cell), _ptr(rec), m))
+        return rec
+
+    def kmeans_assign(self, centroids, x, with_dist=False):
+        k, dim = centroids.shape
+        n = x.shape[0]
+        if _is_dev(x):
+            import torch
+            a = torch.empty(n, dtype=torch.int32, device=x.device)
+            dd = torch.empty(n, dtype=torch.float32, device=x.device) if with_dist else None
+        else:
+            a = np.zeros(n, np.int32)
+            dd = np.zeros(n, np.float32) if with_dist else None
+        m = _mode(centroids, x, a)
+        self.sync_torch(x, centroids)
+        self.check(self.L.puckgpu_kmeans_assign(self.h, dim, _ptr(centroids), k, n, _ptr(x), _ptr(a), _ptr(dd), m))
+        return (a, dd) if with_dist else a
+
+
+class GpuIndex:
+    """A Puck index resident in HBM (memory order, derived arrays on device)."""
+
+    def __init__(self, ctx, coarse, fine, codebook, cell_assign, records, features, whether_norm):
+        self.ctx = ctx
+        C, dim = coarse.shape
+        F = fine.shape[0]
+        N = cell_assign.shape[0]
+        nsq = records.shape[1] - 4
+        self.dim, self.C, self.F, self.nsq, self.N = dim, C, F, nsq, N
+        self.whether_norm = bool(whether_norm)
+        p = _lib.IndexParams(dim, C, F, nsq, N, int(bool(whether_norm)))
+        h = ctypes.c_void_p()
+        m = _mode(coarse, fine, codebook, cell_assign, records, features)
+        ctx.sync_torch(coarse, features)
+        ctx.check(ctx.L.puckgpu_index_load(ctx.h, ctypes.byref(p), _ptr(coarse), _ptr(fine), _ptr(codebook),
+                                           _ptr(cell_assign), _ptr(records), _ptr(features), m, ctypes.byref(h)))
+        self.h = h
+
+    def free(self):
+        if getattr(self, "h", None):
+            self.ctx.L.puckgpu_index_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def export(self):
+        C, F, N = self.C, self.F, self.N
+        out = dict(coarse_norms=np.zeros(C, np.float32), stationary=np.zeros(C * F, np.float32),
+                   cell_start=np.zeros(C * F + 1, np.uint32), min_dist_offset=np.zeros(C, np.float32),
+                   mem2local=np.zeros(max(N, 1), np.uint32))
+        self.ctx.check(self.ctx.L.puckgpu_index_export(self.ctx.h, self.h, _ptr(out["coarse_norms"]),
+                                                       _ptr(out["stationary"]), _ptr(out["cell_start"]),
+                                                       _ptr(out["min_dist_offset"]), _ptr(out["mem2local"])))
+        out["mem2local"] = out["mem2local"][:N]
+        return out
+
+    def search(self, queries, topk, search_coarse_count, filter_topk, topk_max=None, radius_rate=1.0,
+               out=None):
+        """Returns (distance [n, topk], labels [n, topk], result_num [n])."""
+        n = queries.shape[0]
+        topk_max = topk if topk_max is None else topk_max
+        sp = _lib.SearchParams(topk_max, search_coarse_count, filter_topk, radius_rate)
+        if out is None:
+            if _is_dev(queries):
+                import torch
+                dv = queries.device
+                out = (torch.empty((n, topk), dtype=torch.float32, device=dv),
+                       torch.empty((n, topk), dtype=torch.int32, device=dv),
+                       torch.empty(n, dtype=torch.int32, device=dv))
+            else:
+                out = (np.zeros((n, topk), np.float32), np.zeros((n, topk), np.uint32), np.zeros(n, np.uint32))
+        D, I, R = out
+        m = _mode(queries, D, I, R)
+        self.ctx.sync_torch(queries)
+        self.ctx.check(self.ctx.L.puckgpu_search(self.ctx.h, self.h, ctypes.byref(sp), n, _ptr(queries), topk,
+                                                 _ptr(D), _ptr(I), _ptr(R), m))
+        return D, I, R
+
+    def last_filter(self, n, filter_topk):
+        t = np.zeros((n, filter_topk), np.float32)
+        i = np.zeros((n, filter_topk), np.uint32)
+        c = np.zeros(n, np.uint32)
+        self.ctx.check(self.ctx.L.puckgpu_last_filter(self.ctx.h, n, filter_topk, _ptr(t), _ptr(i), _ptr(c)))
+        return t, i, c
+
+    def last_stage_ms(self):
+        ms = np.zeros(5, np.float32)
+        cnt = np.zeros(5, np.uint64)
+        self.ctx.check(self.ctx.L.puckgpu_last_stage_ms(self.ctx.h, _ptr(ms), _ptr(cnt)))
+        return ms, cnt

@@ -1,0 +1,79 @@
+"""GPU index construction: train coarse / fine centers and the filter-PQ
+codebook, assign every point to its cell and encode its filter record.
+
+Follows HierarchicalClusterIndex::train (hierarchical_cluster_index.cpp:882-985:
+coarse k-means, fine k-means on the coarse residuals, fine centers stored by
+descending norm) and PuckIndex::train (puck_index.cpp:764-874: per-subspace
+256-center k-means on the cell residuals).  The Lloyd assignment step of every
+k-means run is the engine's GEMM + argmin kernel (puckgpu_kmeans_assign, the
+nearest_center formula of kmeans.cpp:221-258); the centroid means are torch
+reductions.  The reference seeds its sampling and initialisation with time(0),
+so trained codebooks are not reproducible against it; these are seeded.
+"""
+import torch
+
+
+def kmeans(ctx, x, k, niter=10, seed=0):
+    """Lloyd k-means on a CUDA float32 tensor x [n, d]; returns [k, d]."""
+    n, d = x.shape
+    if k >= n:
+        raise ValueError("better to have fewer clusters than points")
+    g = torch.Generator(device=x.device)
+    g.manual_seed(seed)
+    c = x[torch.randperm(n, generator=g, device=x.device)[:k]].clone().contiguous()
+    for _ in range(niter):
+        a = ctx.kmeans_assign(c, x).long()
+        cnt = torch.bincount(a, minlength=k)
+        sums = torch.zeros_like(c).index_add_(0, a, x)
+        nz = cnt > 0
+        c[nz] = sums[nz] / cnt[nz, None].to(c.dtype)
+        empty = torch.nonzero(~nz).flatten()
+        if empty.numel():
+            # split the most populated clusters (kmeans_reassign_empty, kmeans.cpp:112-147)
+            donors = torch.argsort(cnt, descending=True)[: empty.numel()]
+            eps = torch.randn((empty.numel(), d), generator=g, device=x.device) * 1e-7
+            eps = eps * c[donors].norm(dim=1, keepdim=True)
+            c[empty] = c[donors] - eps
+            c[donors] = c[donors] + eps
+        c = c.contiguous()
+    return c
+
+
+def train_centers(ctx, xs, C, F, niter=10, seed=0):
+    """Coarse centers, fine (residual) centers stored by descending norm."""
+    coarse = kmeans(ctx, xs, C, niter, seed)
+    a = ctx.kmeans_assign(coarse, xs).long()
+    res = (xs - coarse[a]).contiguous()
+    fine = kmeans(ctx, res, F, niter, seed + 1)
+    half = (fine * fine).sum(1) / 2
+    order = torch.argsort(-torch.sqrt(half), stable=True)
+    return coarse, fine[order].contiguous()
+
+
+def train_codebook(ctx, coarse, fine, xs, nsq, niter=10, seed=7):
+    """Filter-PQ codebook [nsq, 256, lsq] on the cell residuals of xs."""
+    d = xs.shape[1]
+    lsq = d // nsq
+    F = fine.shape[0]
+    cell = ctx.assign_cells(coarse, fine, xs).long()
+    res = xs - coarse[cell // F] - fine[cell % F]
+    cb = torch.empty((nsq, 256, lsq), dtype=torch.float32, device=xs.device)
+    for m in range(nsq):
+        cb[m] = kmeans(ctx, res[:, m * lsq:(m + 1) * lsq].contiguous(), 256, niter, seed + 31 * m)
+    return cb
+
+
+def build(ctx, x, C, F, nsq, whether_norm, train_n=262144, niter=10, seed=0):
+    """Train, assign and encode x (CUDA [N, d]); returns the index arrays
+    (device tensors in the reference file layouts) ready for GpuIndex."""
+    N = x.shape[0]
+    g = torch.Generator(device=x.device)
+    g.manual_seed(seed + 99)
+    sel = torch.randperm(N, generator=g, device=x.device)[: min(train_n, N)]
+    xs = x[sel].contiguous()
+    coarse, fine = train_centers(ctx, xs, C, F, niter, seed)
+    codebook = train_codebook(ctx, coarse, fine, xs, nsq, niter, seed + 7)
+    cell = ctx.assign_cells(coarse, fine, x)
+    rec = ctx.encode_filter(coarse, fine, codebook.reshape(-1).contiguous(), nsq, x, cell)
+    return dict(coarse=coarse, fine=fine, codebook=codebook.reshape(-1).contiguous(), cell_assign=cell,
+                records=rec, raw=x, whether_norm=whether_norm, nsq=nsq)

@@ -1,0 +1,151 @@
+"""GPU parity: libpuckgpu.so (through the C ABI) against the CPU oracle on the
+same seeded inputs.  Integer/byte/index outputs and the reference-order float
+arithmetic are compared bit for bit."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from puck_amd import datasets as D
+import cpu_index as CI
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from puck_amd.engine import Context
+    return Context(0)
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+@pytest.fixture(scope="module")
+def idx_l2():
+    x = D.mixture_uint8_grid(12000, 128, 200, seed=10, means_seed=11)
+    q = D.mixture_uint8_grid(64, 128, 200, seed=12, means_seed=11)
+    ix = CI.build_index(x, 48, 48, 32, whether_norm=False, train_n=6000)
+    return ix, q
+
+
+@pytest.fixture(scope="module")
+def idx_cos():
+    x = D.unit_norm_mixture(15000, 96, 300, seed=20, means_seed=21)
+    q = D.unit_norm_mixture(64, 96, 300, seed=22, means_seed=21)
+    ix = CI.build_index(x, 40, 40, 48, whether_norm=True, train_n=6000)
+    return ix, q
+
+
+def _gpu_index(ctx, ix):
+    from puck_amd.engine import GpuIndex
+    return GpuIndex(ctx, ix["coarse"], ix["fine"], ix["codebook"].reshape(-1), ix["cell_assign"],
+                    ix["records"], ix["raw"], ix["whether_norm"])
+
+
+@pytest.mark.parametrize("lsq", [1, 2, 4])
+def test_lut_bitexact(ctx, lsq):
+    rng = np.random.default_rng(lsq)
+    nsq = 16
+    q = (rng.standard_normal((5, nsq * lsq)) * 30).astype(np.float32)
+    cb = (rng.standard_normal(nsq * 256 * lsq) * 30).astype(np.float32)
+    g = ctx.lut(q, cb, nsq)
+    for i in range(5):
+        assert np.array_equal(_bits(g[i]), _bits(O.lut(q[i], cb, nsq, lsq)))
+
+
+@pytest.mark.parametrize("which", ["l2", "cos"])
+def test_index_derivation_bitexact(ctx, which, idx_l2, idx_cos):
+    ix, _ = idx_l2 if which == "l2" else idx_cos
+    g = _gpu_index(ctx, ix).export()
+    o = CI.oracle_index(ix).export()
+    assert np.array_equal(_bits(g["coarse_norms"]), _bits(o["coarse_norms"]))
+    assert np.array_equal(_bits(g["stationary"]), _bits(o["stationary"]))
+    assert np.array_equal(g["cell_start"], o["cell_start"])
+    assert np.array_equal(_bits(g["min_dist_offset"]), _bits(o["min_dist_offset"]))
+    assert np.array_equal(g["mem2local"], o["mem2local"])
+
+
+def _sorted_pairs(v, i):
+    order = np.lexsort((i, v))
+    return v[order], i[order]
+
+
+@pytest.mark.parametrize("which,scc,ftk,topk", [("l2", 4, 100, 10), ("l2", 16, 500, 10), ("l2", 48, 300, 100),
+                                                ("cos", 5, 200, 10), ("cos", 20, 1000, 10), ("cos", 40, 64, 1)])
+def test_search_bitexact(ctx, which, scc, ftk, topk, idx_l2, idx_cos):
+    ix, q = idx_l2 if which == "l2" else idx_cos
+    gi = _gpu_index(ctx, ix)
+    oi = CI.oracle_index(ix)
+    Dg, Ig, Rg = gi.search(q, topk, scc, ftk)
+    ft, fm, fn = gi.last_filter(q.shape[0], ftk)
+    for k in range(q.shape[0]):
+        o = oi.search_one(q[k], topk, scc, ftk)
+        assert o["rc"] == 0
+        # filter-stage set: same (distance, memory id) multiset
+        assert fn[k] == o["filt_n"], f"query {k}: filter count {fn[k]} vs {o['filt_n']}"
+        gv, gid = _sorted_pairs(ft[k, :fn[k]], fm[k, :fn[k]])
+        ov, oid = _sorted_pairs(o["filt_val"][:o["filt_n"]], o["filt_mem"][:o["filt_n"]])
+        assert np.array_equal(_bits(gv), _bits(ov)), f"query {k}: filter distances differ"
+        assert np.array_equal(gid, oid), f"query {k}: filter ids differ"
+        # final result
+        assert Rg[k] == o["n"]
+        assert np.array_equal(_bits(Dg[k, :o["n"]]), _bits(o["dist"][:o["n"]])), f"query {k}"
+        assert np.array_equal(Ig[k, :o["n"]], o["ids"][:o["n"]]), f"query {k}"
+
+
+def test_search_not_full_and_clamps(ctx, idx_l2):
+    ix, q = idx_l2
+    gi = _gpu_index(ctx, ix)
+    oi = CI.oracle_index(ix)
+    # scc > C is clamped; a tiny scc leaves the filter heap not full
+    for scc, ftk, topk in [(1000, 200, 10), (1, 1024, 50)]:
+        Dg, Ig, Rg = gi.search(q[:8], topk, scc, ftk, topk_max=100)
+        for k in range(8):
+            o = oi.search_one(q[k], topk, scc, ftk, max_topk=100)
+            assert Rg[k] == o["n"]
+            assert np.array_equal(Ig[k, :o["n"]], o["ids"][:o["n"]])
+            assert np.array_equal(_bits(Dg[k, :o["n"]]), _bits(o["dist"][:o["n"]]))
+
+
+def test_search_errors(ctx, idx_cos):
+    from puck_amd.engine import PuckGpuError
+    ix, q = idx_cos
+    gi = _gpu_index(ctx, ix)
+    with pytest.raises(PuckGpuError):
+        gi.search(q[:2], 20, 5, 100, topk_max=10)   # topk > FLAGS_topk
+    with pytest.raises(PuckGpuError):
+        gi.search(q[:2], 10, 5, 5)                 # filter_topk < topk
+    qz = q[:3].copy()
+    qz[1] = 0.0                                    # zero-norm query under whether_norm
+    Dg, Ig, Rg = gi.search(qz, 10, 5, 100)
+    assert Rg[1] == 0 and Rg[0] == 10 and Rg[2] == 10
+
+
+def test_assign_cells_bitexact(ctx, idx_l2, idx_cos):
+    for ix, _ in (idx_l2, idx_cos):
+        x = ix["raw"][:3000]
+        cg, dg = ctx.assign_cells(ix["coarse"], ix["fine"], x, with_dist=True)
+        co, do = O.assign_cells(ix["coarse"], ix["fine"], x)
+        assert np.array_equal(cg, co)
+        assert np.array_equal(_bits(dg), _bits(do))
+
+
+def test_encode_filter_bitexact(ctx, idx_l2, idx_cos):
+    for ix, _ in (idx_l2, idx_cos):
+        x = ix["raw"][:3000]
+        cell = ix["cell_assign"][:3000]
+        rg = ctx.encode_filter(ix["coarse"], ix["fine"], ix["codebook"].reshape(-1), ix["nsq"], x, cell)
+        ro = O.encode_filter(ix["coarse"], ix["fine"], ix["codebook"], ix["nsq"], x, cell)
+        assert np.array_equal(rg, ro)
+
+
+def test_kmeans_assign_bitexact(ctx):
+    rng = np.random.default_rng(3)
+    for d, k in [(128, 300), (96, 64), (2, 256), (4, 256)]:
+        x = (rng.standard_normal((4000, d)) * 10).astype(np.float32)
+        c = (rng.standard_normal((k, d)) * 10).astype(np.float32)
+        ag, dg = ctx.kmeans_assign(c, x, with_dist=True)
+        ao, do = O.nearest_center(c, x)
+        assert np.array_equal(ag, ao)
+        assert np.array_equal(_bits(dg), _bits(do))
