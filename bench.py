@@ -146,6 +146,12 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes_per_launch": int(walk_bytes),
                      "avg_launch_ms": round(avg_walk_s * 1000.0, 3)},
+        "walk_counters_per_query": {k: round(float(v) / nq, 3) for k, v in
+                                    zip(["segments", "uncertain_cells", "examined_cells", "scanned_points"], cnt[:4])},
+        "tie_replays": {k: int(v) for k, v in zip(["replayed_queries", "coarse_tie", "filter_boundary_tie",
+                                                   "exact_distance_tie", "fine_tie_decision", "fine_tie_back",
+                                                   "fine_tie_chunk_edge", "fine_tie_group_ge3",
+                                                   "fine_tie_orders_disagree"], cnt[4:13])},
         "build_s": round(build_s, 1),
     }
     if not args.no_cpu:

@@ -58,6 +58,7 @@ typedef struct {
     uint32_t search_coarse_count; /* clamped to C */
     uint32_t filter_topk;         /* >= topk, <= puckgpu_filter_topk_max() */
     double radius_rate;           /* >= 1.0 */
+    uint32_t replay_all;          /* non-zero: replay every query with the reference heaps (tests) */
 } puckgpu_search_params;
 
 /* Context: one device, one stream, reusable scratch.  device = HIP ordinal. */
@@ -101,11 +102,15 @@ int puckgpu_last_filter(puckgpu_ctx* ctx, uint32_t n, uint32_t filter_topk, floa
                         uint32_t* filt_n);
 /* Per-stage device times (ms) of the last search, summed over its passes:
  * [0] query prep + center GEMM, [1] center sort, [2] walk + filter scan,
- * [3] rerank, [4] exact-tie replay.  counters5 (may be NULL) receives the
- * walk's segments, uncertain cells, examined cells and scanned points summed over
- * queries, then the number of queries replayed because their result depends
- * on the order of exactly equal distances. */
-int puckgpu_last_stage_ms(puckgpu_ctx* ctx, float* ms5, uint64_t* counters5);
+ * [3] rerank, [4] exact-tie replay.  counters13 (may be NULL) receives: the
+ * walk's segments, uncertain cells, examined cells and scanned points summed
+ * over queries; the number of queries replayed with the reference heaps
+ * because their result depends on the order of exactly equal distances; and
+ * how many queries raised each cause (coarse-distance tie, filter-boundary
+ * tie, exact-distance tie in the top-k, decision on a fine-distance tie) and,
+ * for the last cause, its detail (tie with an earlier cell, tie across a
+ * chunk boundary, group of three or more, visiting orders disagree). */
+int puckgpu_last_stage_ms(puckgpu_ctx* ctx, float* ms5, uint64_t* counters13);
 
 /* Filter lookup table for n queries (test hook): out [n x nsq x 256]. */
 int puckgpu_lut(puckgpu_ctx* ctx, uint32_t n, uint32_t dim, const float* queries, const float* filter_codebook,

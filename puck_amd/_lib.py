@@ -26,7 +26,8 @@ class IndexParams(ctypes.Structure):
 
 class SearchParams(ctypes.Structure):
     _fields_ = [("topk_max", ctypes.c_uint32), ("search_coarse_count", ctypes.c_uint32),
-                ("filter_topk", ctypes.c_uint32), ("radius_rate", ctypes.c_double)]
+                ("filter_topk", ctypes.c_uint32), ("radius_rate", ctypes.c_double),
+                ("replay_all", ctypes.c_uint32)]
 
 
 _lib = None

@@ -30,12 +30,14 @@ __device__ __forceinline__ void pk_wave_argmin(float& v, uint32_t& idx) {
 }
 
 // Process one coarse cluster: returns updated (best, cell).  A wave covers
-// 4 * 64 consecutive fine centers per step: each lane scores 4 of them
-// (sequential running minimum), an exclusive wave scan of the lane minima
-// gives the best-so-far seen by every position, and the first position whose
-// triangle check fails ends the cluster.
+// 4 * 64 consecutive fine centers per step: each lane scores 4 of them, an
+// exclusive wave scan of the lane minima gives the best-so-far seen by every
+// position, and the first position whose triangle check fails ends the
+// cluster.  The check is skipped for a whole step when even the smallest
+// best-so-far of the step cannot fail it at the step's last position.
 __device__ void pk_assign_one_coarse(uint32_t c, float ct, const float* __restrict__ ipf, const float* __restrict__ st,
-                                     const float* __restrict__ fbound, uint32_t F, float& best, int32_t& cell) {
+                                     const float* __restrict__ fbound, uint32_t F, int fb_sorted, float& best,
+                                     int32_t& cell) {
     const int lane = threadIdx.x & 63;
     const float sct = __fsqrt_rn(ct);
     const float* strow = st + (uint64_t)c * F;
@@ -49,123 +51,110 @@ __device__ void pk_assign_one_coarse(uint32_t c, float ct, const float* __restri
             val[e] = f < F;
             sc[e] = val[e] ? (ct - ipf[f]) + strow[f] : FLT_MAX;
         }
-        // lane-local running minimum and the lane minimum
         float lmin = sc[0];
 #pragma unroll
         for (int e = 1; e < 4; ++e) lmin = sc[e] < lmin ? sc[e] : lmin;
-        // exclusive scan of lane minima
         float incl = lmin;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
             float y = __shfl_up(incl, o, 64);
             if (lane >= o) incl = y < incl ? y : incl;
         }
-        float excl = __shfl_up(incl, 1, 64);
-        float run = lane == 0 ? best : (excl < best ? excl : best);
-        // first failing triangle check inside this lane (4 = none)
-        int bl = 4;
+        const float stepmin = __shfl(incl, 63, 64);
+        const float lo = stepmin < best ? stepmin : best;
+        const uint32_t flast = (base + 256 <= F ? base + 256 : F) - 1;
+        // sqrt(run) - fbound[f] >= sqrt(lo) - fbound[flast] for every position
+        // of the step (run >= lo, fbound non-decreasing along the stored order)
+        // (needs fbound non-decreasing: fine centers stored by descending norm)
+        const bool nobreak = fb_sorted && !(__fsqrt_rn(lo) - fbound[flast] < sct);
+        uint32_t bpos = 256u;
+        if (!nobreak) {
+            float excl = __shfl_up(incl, 1, 64);
+            float run = lane == 0 ? best : (excl < best ? excl : best);
+            int bl = 4;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            if (bl == 4 && val[e] && (__fsqrt_rn(run) - fbound[f0 + e] < sct)) bl = e;
-            run = sc[e] < run ? sc[e] : run;
+            for (int e = 0; e < 4; ++e) {
+                if (bl == 4 && val[e] && (__fsqrt_rn(run) - fbound[f0 + e] < sct)) bl = e;
+                run = sc[e] < run ? sc[e] : run;
+            }
+            uint64_t bm = __ballot(bl < 4);
+            uint32_t blane = bm ? (uint32_t)__ffsll((unsigned long long)bm) - 1 : 64u;
+            bpos = blane < 64 ? 4 * blane + (uint32_t)__shfl(bl, (int)blane, 64) : 256u;
         }
-        uint64_t bm = __ballot(bl < 4);
-        uint32_t blane = bm ? (uint32_t)__ffsll((unsigned long long)bm) - 1 : 64u;
-        uint32_t bpos = blane < 64 ? 4 * blane + (uint32_t)__shfl(bl, (int)blane, 64) : 256u;
-        // first occurrence of the minimum over positions before the break
-        float v = FLT_MAX;
-        uint32_t vi = 0xffffffffu;
+        if (stepmin < best || bpos < 256) {
+            float v = FLT_MAX;
+            uint32_t vi = 0xffffffffu;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            uint32_t pos = 4 * lane + e;
-            if (val[e] && pos < bpos && (vi == 0xffffffffu || sc[e] < v)) { v = sc[e]; vi = pos; }
-        }
-        pk_wave_argmin(v, vi);
-        if (vi != 0xffffffffu && v < best) {
-            best = v;
-            cell = (int32_t)(c * F + base + vi);
+            for (int e = 0; e < 4; ++e) {
+                uint32_t pos = 4 * lane + e;
+                if (val[e] && pos < bpos && (vi == 0xffffffffu || sc[e] < v)) { v = sc[e]; vi = pos; }
+            }
+            pk_wave_argmin(v, vi);
+            if (vi != 0xffffffffu && v < best) {
+                best = v;
+                cell = (int32_t)(c * F + base + vi);
+            }
         }
         if (bpos < 256) return;
     }
 }
 
+__device__ __forceinline__ void pk_wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One point per wave.  The coarse terms are sorted once per point (wave-level
+// bitonic sort of (ct, c) keys in LDS); coarse clusters are visited in that
+// order until one fails the triangle check at f = 0 -- every later one fails
+// it too, because sqrt(ct) does not decrease and best does not increase.
 __global__ __launch_bounds__(256) void pk_assign_cells(const float* __restrict__ ipc, const float* __restrict__ ipf,
                                                        const float* __restrict__ xn, const float* __restrict__ cn,
                                                        const float* __restrict__ fbound,
                                                        const float* __restrict__ st, uint32_t n, uint32_t C,
-                                                       uint32_t F, uint32_t* __restrict__ cell_out,
+                                                       uint32_t F, uint32_t C2, int fb_sorted,
+                                                       const uint32_t* __restrict__ order,
+                                                       uint32_t* __restrict__ cell_out,
                                                        float* __restrict__ dist_out) {
     extern __shared__ __align__(16) unsigned char asmem[];
-    float* cand_ct = (float*)asmem;                       // 4 waves x C
-    uint32_t* cand_c = (uint32_t*)(asmem + sizeof(float) * 4 * C);
-    const uint32_t pt = blockIdx.x * 4 + (threadIdx.x >> 6);
+    uint64_t* keys = (uint64_t*)asmem + (threadIdx.x >> 6) * C2;
+    const uint32_t wpt = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (pt >= n) return;
+    if (wpt >= n) return;
+    const uint32_t pt = order ? order[wpt] : wpt;
     const float* pc = ipc + (uint64_t)pt * C;
     const float* pf = ipf + (uint64_t)pt * F;
     const float x2 = xn[pt];
-    // nearest coarse by (ct, c)
-    float bv = FLT_MAX;
-    uint32_t bi = 0xffffffffu;
-    for (uint32_t c = lane; c < C; c += 64) {
-        float ct = (cn[c] - pc[c]) + x2;
-        if (bi == 0xffffffffu || ct < bv) { bv = ct; bi = c; }
+    for (uint32_t c = lane; c < C2; c += 64) {
+        uint64_t k = ~0ull;
+        if (c < C) k = ((uint64_t)pk_f2ord((cn[c] - pc[c]) + x2) << 32) | c;
+        keys[c] = k;
     }
-    pk_wave_argmin(bv, bi);
+    pk_wave_lds_sync();
+    for (uint32_t k = 2; k <= C2; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = lane; i < (C2 >> 1); i += 64) {
+                uint32_t lo = 2 * i - (i & (j - 1));
+                uint32_t hi = lo + j;
+                bool up = (lo & k) == 0;
+                uint64_t x = keys[lo], y = keys[hi];
+                if ((x > y) == up) {
+                    keys[lo] = y;
+                    keys[hi] = x;
+                }
+            }
+            pk_wave_lds_sync();
+        }
+    }
     float best = FLT_MAX;
     int32_t cell = -1;
-    const uint32_t c0 = bi;
-    const float ct0 = bv;
-    pk_assign_one_coarse(c0, ct0, pf, st, fbound, F, best, cell);
-    // coarse clusters that survive the f = 0 check under the current best,
-    // compacted into this wave's LDS list, then visited in (ct, c) order
-    const float lim = __fsqrt_rn(best) - fbound[0];
-    float* cct = cand_ct + (threadIdx.x >> 6) * C;
-    uint32_t* cci = cand_c + (threadIdx.x >> 6) * C;
-    uint32_t K = 0;
-    for (uint32_t c0b = 0; c0b < C; c0b += 64) {
-        const uint32_t c = c0b + lane;
-        bool keep = false;
-        float ct = 0.f;
-        if (c < C && c != c0) {
-            ct = (cn[c] - pc[c]) + x2;
-            bool after = ct > ct0 || (ct == ct0 && c > c0);
-            keep = after && !(lim < __fsqrt_rn(ct));   // else it breaks at f = 0
-        }
-        uint64_t bm = __ballot(keep);
-        if (keep) {
-            uint32_t pos = K + (uint32_t)__popcll(bm & ((1ull << lane) - 1ull));
-            cct[pos] = ct;
-            cci[pos] = c;
-        }
-        K += (uint32_t)__popcll(bm);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (uint32_t done_n = 0; done_n < K; ++done_n) {
-        float nv = FLT_MAX;
-        uint32_t ni = 0xffffffffu, np = 0xffffffffu;
-        for (uint32_t r = lane; r < K; r += 64) {
-            const uint32_t c = cci[r];
-            if (c == 0xffffffffu) continue;
-            const float ct = cct[r];
-            if (ni == 0xffffffffu || ct < nv || (ct == nv && c < ni)) { nv = ct; ni = c; np = r; }
-        }
-        // wave argmin carrying the list position
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            float ov = __shfl_xor(nv, o, 64);
-            uint32_t oi = __shfl_xor(ni, o, 64);
-            uint32_t op = __shfl_xor(np, o, 64);
-            if (ov < nv || (ov == nv && oi < ni)) { nv = ov; ni = oi; np = op; }
-        }
-        if (ni == 0xffffffffu) break;
-        if (lane == 0) cci[np] = 0xffffffffu;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        pk_assign_one_coarse(ni, nv, pf, st, fbound, F, best, cell);
+    const float fb0 = fbound[0];
+    for (uint32_t r = 0; r < C; ++r) {
+        const uint64_t key = keys[r];
+        const float ct = pk_ord2f((uint32_t)(key >> 32));
+        if (__fsqrt_rn(best) - fb0 < __fsqrt_rn(ct)) break;
+        pk_assign_one_coarse((uint32_t)(key & 0xffffffffu), ct, pf, st, fbound, F, fb_sorted, best, cell);
     }
     if (lane == 0) {
         cell_out[pt] = (uint32_t)cell;
@@ -173,11 +162,40 @@ __global__ __launch_bounds__(256) void pk_assign_cells(const float* __restrict__
     }
 }
 
+// Nearest coarse term per point (used to order points so that waves running
+// together share stationary-table rows in L2).
+__global__ void pk_nearest_coarse(const float* __restrict__ ipc, const float* __restrict__ xn,
+                                  const float* __restrict__ cn, uint32_t n, uint32_t C, uint32_t* __restrict__ out) {
+    const uint32_t pt = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (pt >= n) return;
+    const float* pc = ipc + (uint64_t)pt * C;
+    const float x2 = xn[pt];
+    float bv = FLT_MAX;
+    uint32_t bi = 0xffffffffu;
+    for (uint32_t c = lane; c < C; c += 64) {
+        float ct = (cn[c] - pc[c]) + x2;
+        if (bi == 0xffffffffu || ct < bv) { bv = ct; bi = c; }
+    }
+    pk_wave_argmin(bv, bi);
+    if (lane == 0) out[pt] = bi;
+}
+
+extern "C" hipError_t pk_launch_nearest_coarse(const float* ipc, const float* xn, const float* cn, uint32_t n,
+                                               uint32_t C, uint32_t* out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(pk_nearest_coarse, dim3((n + 3) / 4), dim3(256), 0, s, ipc, xn, cn, n, C, out);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t pk_launch_assign_cells(const float* ipc, const float* ipf, const float* xn, const float* cn,
                                              const float* fbound, const float* st, uint32_t n, uint32_t C,
-                                             uint32_t F, uint32_t* cell_out, float* dist_out, hipStream_t s) {
+                                             uint32_t F, int fb_sorted, const uint32_t* order, uint32_t* cell_out,
+                                             float* dist_out, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    size_t smem = 8ull * 4 * C;
+    uint32_t C2 = 64;
+    while (C2 < C) C2 <<= 1;
+    size_t smem = sizeof(uint64_t) * 4 * C2;
     if (smem > 160 * 1024) return hipErrorInvalidValue;
     static bool attr = false;
     if (!attr) {
@@ -185,7 +203,7 @@ extern "C" hipError_t pk_launch_assign_cells(const float* ipc, const float* ipf,
         attr = true;
     }
     hipLaunchKernelGGL(pk_assign_cells, dim3((n + 3) / 4), dim3(256), smem, s, ipc, ipf, xn, cn, fbound, st, n, C, F,
-                       cell_out, dist_out);
+                       C2, fb_sorted, order, cell_out, dist_out);
     return hipGetLastError();
 }
 

@@ -28,7 +28,10 @@ hipError_t pk_launch_walk(const PkWalkArgs*, uint32_t, hipStream_t);
 hipError_t pk_launch_rerank(const PkRerankArgs*, uint32_t, hipStream_t);
 uint32_t pk_walk_ftk_max(void);
 hipError_t pk_launch_assign_cells(const float*, const float*, const float*, const float*, const float*,
-                                  const float*, uint32_t, uint32_t, uint32_t, uint32_t*, float*, hipStream_t);
+                                  const float*, uint32_t, uint32_t, uint32_t, int, const uint32_t*, uint32_t*,
+                                  float*, hipStream_t);
+hipError_t pk_launch_nearest_coarse(const float*, const float*, const float*, uint32_t, uint32_t, uint32_t*,
+                                    hipStream_t);
 hipError_t pk_launch_fine_bound(const float*, uint32_t, uint32_t, float*, hipStream_t);
 hipError_t pk_launch_encode_filter(const float*, const uint32_t*, uint32_t, const float*, const float*, uint32_t,
                                    uint32_t, const float*, uint32_t, float*, uint8_t*, uint64_t, uint64_t,
@@ -67,6 +70,8 @@ struct PkWalkArgs {
     const uint32_t* qlist;
     uint8_t* flags;
     int exact;
+    uint32_t* tie_ids;
+    uint8_t* tie_n;
 };
 struct PkRerankArgs {
     const float* q;
@@ -83,6 +88,8 @@ struct PkRerankArgs {
     const uint32_t* qlist;
     uint8_t* flags;
     int exact;
+    const uint32_t* tie_ids;
+    const uint8_t* tie_n;
 };
 
 namespace {
@@ -101,7 +108,7 @@ struct puckgpu_ctx {
     std::vector<DevBuf> bufs;          // named scratch slots
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     float stage_ms[5] = {0, 0, 0, 0, 0};
-    uint64_t counters[5] = {0, 0, 0, 0, 0};
+    uint64_t counters[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t last_n = 0, last_ftk = 0;
 };
 
@@ -161,7 +168,7 @@ void* scratch(puckgpu_ctx* ctx, size_t slot, size_t bytes, int* rc) {
 
 enum Slot {
     S_Q = 0, S_QN, S_QST, S_CD, S_FD, S_CSD, S_CSI, S_FSD, S_FSI, S_FT, S_FI, S_FN, S_OD, S_OI, S_ON, S_STATS,
-    S_IN_Q, S_A0, S_A1, S_A2, S_A3, S_A4, S_A5, S_A6, S_A7, S_FLAGS, S_QLIST, S_COUNT
+    S_IN_Q, S_A0, S_A1, S_A2, S_A3, S_A4, S_A5, S_A6, S_A7, S_FLAGS, S_QLIST, S_TIEID, S_TIEN, S_COUNT
 };
 
 // Device view of an input array: the pointer itself, or a scratch copy of it.
@@ -442,8 +449,11 @@ int puckgpu_search(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_sear
     if (!qin) return rc;
     uint8_t* flags = (uint8_t*)scratch(ctx, S_FLAGS, nb, &rc);
     uint32_t* qlist = (uint32_t*)scratch(ctx, S_QLIST, sizeof(uint32_t) * nb, &rc);
+    uint32_t* tie_ids = (uint32_t*)scratch(ctx, S_TIEID, sizeof(uint32_t) * 16 * nb, &rc);
+    uint8_t* tie_n = (uint8_t*)scratch(ctx, S_TIEN, nb, &rc);
     if (rc) return rc;
-    for (int k = 0; k < 5; ++k) { ctx->stage_ms[k] = 0; ctx->counters[k] = 0; }
+    for (int k = 0; k < 5; ++k) ctx->stage_ms[k] = 0;
+    for (int k = 0; k < 13; ++k) ctx->counters[k] = 0;
     std::vector<uint32_t> hstats(4 * (size_t)nb);
     std::vector<uint8_t> hflags(nb);
     std::vector<uint32_t> hlist;
@@ -465,13 +475,14 @@ int puckgpu_search(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_sear
         wa.dim = d; wa.C = C; wa.F = F; wa.nsq = nsq; wa.lsq = ix->lsq; wa.scc = scc; wa.ftk = ftk;
         wa.rr = sp->radius_rate; wa.out_t = ft + (size_t)q0 * ftk; wa.out_id = fi + (size_t)q0 * ftk;
         wa.out_n = fn + q0; wa.stats = stats; wa.qlist = nullptr; wa.flags = flags; wa.exact = 0;
+        wa.tie_ids = tie_ids; wa.tie_n = tie_n;
         PK_CHECK(pk_launch_walk(&wa, m, s));
         PK_CHECK(hipEventRecord(ctx->ev[3], s));
         PkRerankArgs ra;
         ra.q = q; ra.qstatus = qst; ra.filt_t = wa.out_t; ra.filt_id = wa.out_id; ra.filt_n = wa.out_n;
         ra.raw = ix->raw; ra.mem2local = ix->mem2local; ra.dim = d; ra.ftk = ftk; ra.topk = topk;
         ra.out_d = od + (size_t)q0 * topk; ra.out_id = oi + (size_t)q0 * topk; ra.out_n = on + q0;
-        ra.qlist = nullptr; ra.flags = flags; ra.exact = 0;
+        ra.qlist = nullptr; ra.flags = flags; ra.exact = 0; ra.tie_ids = tie_ids; ra.tie_n = tie_n;
         PK_CHECK(pk_launch_rerank(&ra, m, s));
         PK_CHECK(hipEventRecord(ctx->ev[4], s));
         PK_CHECK(hipMemcpyAsync(hstats.data(), stats, sizeof(uint32_t) * 4 * m, hipMemcpyDeviceToHost, s));
@@ -483,18 +494,20 @@ int puckgpu_search(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_sear
         // at the final top, 4 exact-distance tie in the top-k, 8 a decision on
         // a fine-distance tie group.  1 and 8 need the reference's center order.
         hlist.clear();
+        if (sp->replay_all)
+            for (uint32_t i = 0; i < m; ++i) hflags[i] |= 8u;
         for (uint32_t i = 0; i < m; ++i)
             if (hflags[i] & 9u) hlist.push_back(i);
         const uint32_t ncenter = (uint32_t)hlist.size();
         for (uint32_t i = 0; i < m; ++i)
-            if (hflags[i] && !(hflags[i] & 9u)) hlist.push_back(i);
+            if ((hflags[i] & 15u) && !(hflags[i] & 9u)) hlist.push_back(i);
         if (!hlist.empty()) {
             const uint32_t nl = (uint32_t)hlist.size();
             PK_CHECK(hipMemcpyAsync(qlist, hlist.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s));
             PK_CHECK(pk_launch_exact_center_order(cdist, C, fdist, F, scc, qlist, ncenter, csd, csi, fsd, fsi, s));
-            wa.qlist = qlist; wa.flags = nullptr; wa.exact = 1;
+            wa.qlist = qlist; wa.flags = nullptr; wa.exact = 1; wa.tie_ids = nullptr; wa.tie_n = nullptr;
             PK_CHECK(pk_launch_walk(&wa, nl, s));
-            ra.qlist = qlist; ra.flags = nullptr; ra.exact = 1;
+            ra.qlist = qlist; ra.flags = nullptr; ra.exact = 1; ra.tie_ids = nullptr; ra.tie_n = nullptr;
             PK_CHECK(pk_launch_rerank(&ra, nl, s));
         }
         PK_CHECK(hipEventRecord(ctx->ev[5], s));
@@ -507,6 +520,9 @@ int puckgpu_search(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_sear
         for (uint32_t i = 0; i < m; ++i)
             for (int k = 0; k < 4; ++k) ctx->counters[k] += hstats[4 * (size_t)i + k];
         ctx->counters[4] += hlist.size();
+        for (uint32_t i = 0; i < m; ++i)
+            for (int b = 0; b < 8; ++b)
+                if (hflags[i] & (1u << b)) ctx->counters[5 + b]++;
     }
     if (!device_ptrs) {
         PK_CHECK(hipMemcpyAsync(distance, od, sizeof(float) * (size_t)n * topk, hipMemcpyDeviceToHost, s));
@@ -530,12 +546,12 @@ int puckgpu_last_filter(puckgpu_ctx* ctx, uint32_t n, uint32_t ftk, float* filt_
     return PUCKGPU_OK;
 }
 
-int puckgpu_last_stage_ms(puckgpu_ctx* ctx, float* ms5, uint64_t* counters5) {
+int puckgpu_last_stage_ms(puckgpu_ctx* ctx, float* ms5, uint64_t* counters13) {
     if (!ctx) return PUCKGPU_EINVAL;
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < 5; ++k)
         if (ms5) ms5[k] = ctx->stage_ms[k];
-        if (counters5) counters5[k] = ctx->counters[k];
-    }
+    for (int k = 0; k < 13; ++k)
+        if (counters13) counters13[k] = ctx->counters[k];
     return PUCKGPU_OK;
 }
 
@@ -586,7 +602,16 @@ int puckgpu_assign_cells(puckgpu_ctx* ctx, const float* coarse, const float* fin
     float* xs = (float*)scratch(ctx, S_Q, 4ull * cb * dim, &rc);
     uint32_t* co = (uint32_t*)scratch(ctx, S_OI, 4ull * cb, &rc);
     float* cdo = (float*)scratch(ctx, S_OD, 4ull * cb, &rc);
+    uint32_t* ord_d = (uint32_t*)scratch(ctx, S_QLIST, 4ull * cb, &rc);
     if (rc) return rc;
+    std::vector<uint32_t> ord_h(cb), perm_h(cb);
+    // is fbound non-decreasing (fine centers stored by descending norm)?
+    std::vector<float> fb_h(F);
+    PK_CHECK(hipMemcpyAsync(fb_h.data(), fb, 4ull * F, hipMemcpyDeviceToHost, s));
+    PK_CHECK(hipStreamSynchronize(s));
+    int fb_sorted = 1;
+    for (uint32_t f = 1; f < F; ++f)
+        if (!(fb_h[f - 1] <= fb_h[f])) fb_sorted = 0;
     for (uint32_t p0 = 0; p0 < n; p0 += CHK) {
         const uint32_t m = (n - p0) < CHK ? (n - p0) : CHK;
         const float* xp;
@@ -600,7 +625,19 @@ int puckgpu_assign_cells(puckgpu_ctx* ctx, const float* coarse, const float* fin
         PK_CHECK(pk_launch_gemm_exact(xp, fd, ipf, m, F, dim, F, PK_EPI_RAW, nullptr, s));
         uint32_t* cout = device_ptrs ? cell_out + p0 : co;
         float* dout = device_ptrs ? (dist_out ? dist_out + p0 : nullptr) : cdo;
-        PK_CHECK(pk_launch_assign_cells(ipc, ipf, xn, cn, fb, st, m, C, F, cout, dout, s));
+        // visit points grouped by nearest coarse cluster (L2 reuse of the
+        // stationary rows); results are written by point index
+        PK_CHECK(pk_launch_nearest_coarse(ipc, xn, cn, m, C, ord_d, s));
+        PK_CHECK(hipMemcpyAsync(ord_h.data(), ord_d, 4ull * m, hipMemcpyDeviceToHost, s));
+        PK_CHECK(hipStreamSynchronize(s));
+        {
+            std::vector<uint32_t> cnt(C + 1, 0);
+            for (uint32_t i = 0; i < m; ++i) cnt[ord_h[i] + 1]++;
+            for (uint32_t c = 0; c < C; ++c) cnt[c + 1] += cnt[c];
+            for (uint32_t i = 0; i < m; ++i) perm_h[cnt[ord_h[i]]++] = i;
+        }
+        PK_CHECK(hipMemcpyAsync(ord_d, perm_h.data(), 4ull * m, hipMemcpyHostToDevice, s));
+        PK_CHECK(pk_launch_assign_cells(ipc, ipf, xn, cn, fb, st, m, C, F, fb_sorted, ord_d, cout, dout, s));
         if (!device_ptrs) {
             PK_CHECK(hipMemcpyAsync(cell_out + p0, co, 4ull * m, hipMemcpyDeviceToHost, s));
             if (dist_out) PK_CHECK(hipMemcpyAsync(dist_out + p0, cdo, 4ull * m, hipMemcpyDeviceToHost, s));

@@ -56,9 +56,14 @@ struct PkWalkArgs {
     uint32_t* out_n;         // nq
     uint32_t* stats;         // nq x 4 or null
     const uint32_t* qlist;   // block -> query index (null: identity)
-    uint8_t* flags;          // nq: bit 1 set when a filter-boundary tie was resolved
+    uint8_t* flags;          // nq: bit 3 set when a decision touched a fine-distance tie group
     int exact;               // replay the reference MaxHeap (insertion order and ties)
+    uint32_t* tie_ids;       // nq x PK_TIE_MAX: memory ids dropped at a tie with the final top
+    uint8_t* tie_n;          // nq: count, PK_TIE_OVERFLOW when the list overflowed
 };
+
+#define PK_TIE_MAX 16
+#define PK_TIE_OVERFLOW 255
 
 // ---------------------------------------------------------------------------
 __global__ void pk_prep_queries(const float* __restrict__ qin, uint32_t nq, uint32_t dim, int whether_norm,
@@ -133,50 +138,64 @@ struct PkWalkSmem {
     uint32_t* red;     // 16 words of reduction scratch
     float* P_t;        // PMAX     value by local point index
     uint8_t* P_sel;    // PMAX     selected for the current merge (exact mode)
-    float* H_val;      // ftk      reference heap (exact mode), slot i <-> index i+1
-    uint32_t* H_id;    // ftk
+    uint2* H;          // ftk + 1  reference heap (exact mode), slot i = heap index i
+    uint32_t* td_val;  // PK_TIE_MAX  dropped-at-tie entries: distance bits, memory id
+    uint32_t* td_id;
 };
 
-// The reference MaxHeap on LDS arrays (max_heap.cpp:52-100), run by one thread.
-__device__ void pk_heap_insert(float* hv, uint32_t* ht, uint32_t heap_size, uint32_t father, float v, uint32_t t) {
-    float* V = hv - 1;
-    uint32_t* Tg = ht - 1;
+// The reference MaxHeap (max_heap.cpp:52-100) on an LDS array run by one
+// thread.  Entries are (value bits, tag) pairs; slot i holds heap index i
+// (slot 0 unused) so that the two children of index i, 2i and 2i+1, are one
+// aligned 16-byte read.
+__device__ __forceinline__ float pk_hv(uint2 e) { return __uint_as_float(e.x); }
+
+__device__ void pk_heap_insert(uint2* H, uint32_t heap_size, uint32_t father, float v, uint32_t t) {
     for (;;) {
         uint32_t l = father << 1;
         if (l > heap_size) break;
         uint32_t r = l + 1;
-        if (r > heap_size || V[l] > V[r]) {
-            if (V[l] < v) break;
-            V[father] = V[l];
-            Tg[father] = Tg[l];
+        uint2 el, er;
+        if (r <= heap_size) {
+            uint4 ch = *(const uint4*)(H + l);
+            el = make_uint2(ch.x, ch.y);
+            er = make_uint2(ch.z, ch.w);
+        } else {
+            el = H[l];
+            er = el;
+        }
+        if (r > heap_size || pk_hv(el) > pk_hv(er)) {
+            if (pk_hv(el) < v) break;
+            H[father] = el;
             father = l;
         } else {
-            if (V[r] < v) break;
-            V[father] = V[r];
-            Tg[father] = Tg[r];
+            if (pk_hv(er) < v) break;
+            H[father] = er;
             father = r;
         }
     }
-    V[father] = v;
-    Tg[father] = t;
+    H[father] = make_uint2(__float_as_uint(v), t);
 }
 
-__device__ void pk_heap_update(float* hv, uint32_t* ht, uint32_t size, uint32_t* default_cnt, float v, uint32_t t) {
+__device__ void pk_heap_update(uint2* H, uint32_t size, uint32_t* default_cnt, float v, uint32_t t) {
     uint32_t father = 1;
-    if (v >= hv[0]) return;
+    if (v >= pk_hv(H[1])) return;
     if (*default_cnt > 0) father = (*default_cnt)--;
-    pk_heap_insert(hv, ht, size, father, v, t);
+    pk_heap_insert(H, size, father, v, t);
 }
 
-__device__ void pk_heap_reorder(float* hv, uint32_t* ht, uint32_t size) {
+__device__ void pk_heap_reorder(uint2* H, uint32_t size) {
     for (uint32_t i = 0; i < size; i++) {
-        float vb = hv[0];
-        uint32_t tb = ht[0];
+        uint2 top = H[1];
         uint32_t hs = size - i;
-        pk_heap_insert(hv, ht, hs - 1, 1, hv[hs - 1], ht[hs - 1]);
-        hv[hs - 1] = vb;
-        ht[hs - 1] = tb;
+        uint2 last = H[hs];
+        pk_heap_insert(H, hs - 1, 1, pk_hv(last), last.y);
+        H[hs] = top;
     }
+}
+
+__device__ void pk_heap_init(uint2* H, uint32_t size) {
+    const uint2 sent = make_uint2(0x7f7f7f7fu, 0u);
+    for (uint32_t i = threadIdx.x; i <= size; i += blockDim.x) H[i] = sent;
 }
 
 __device__ __forceinline__ uint32_t pk_bmin(uint32_t v, uint32_t* red) {
@@ -398,11 +417,36 @@ __device__ uint32_t pk_walk_merge(const PkWalkArgs& a, PkWalkSmem& s, uint32_t n
         if (a.exact) s.P_sel[(uint32_t)(s.W_key[wpos] & 0xffffffffu) - seqbase - p_lo] = 1;
     }
     __syncthreads();
-    // Smallest distance at which a boundary tie was resolved by arrival order;
-    // the reference may have kept another element of equal distance only if
-    // that distance is still the top at the end.
-    if (threadIdx.x == 0 && s.red[10] != 0u && s.red[10] == s.red[11] && s.red[10] < s.red[12])
-        s.red[12] = s.red[10];
+    // Entries dropped with the same distance as the new top: the reference may
+    // have kept any of them instead.  Remember them (the list only keeps
+    // entries at the current top; older ones can no longer be in either set).
+    if (!a.exact && s.red[10] != 0u && s.red[10] == s.red[11]) {
+        const uint32_t K = s.red[11];
+        if (threadIdx.x == 0) {
+            uint32_t n0 = s.red[14], w = 0;
+            if (n0 != PK_TIE_OVERFLOW) {
+                for (uint32_t i = 0; i < n0; ++i)
+                    if (s.td_val[i] == K) { s.td_val[w] = K; s.td_id[w] = s.td_id[i]; ++w; }
+                s.red[14] = w;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < WALK_FTK_MAX / WALK_THREADS; ++e)
+            if (spos[e] != 0xffffffffu && spos[e] >= ftk && (uint32_t)(sk[e] >> 32) == K) {
+                uint32_t slot = atomicAdd(&s.red[14], 1u);
+                if (slot < PK_TIE_MAX) { s.td_val[slot] = K; s.td_id[slot] = sid[e]; }
+            }
+#pragma unroll
+        for (int e = 0; e < WALK_PMAX / WALK_THREADS; ++e)
+            if (xpos[e] != 0xffffffffu && xpos[e] >= ftk && (uint32_t)(xk[e] >> 32) == K) {
+                uint32_t slot = atomicAdd(&s.red[14], 1u);
+                if (slot < PK_TIE_MAX) { s.td_val[slot] = K; s.td_id[slot] = xid[e]; }
+            }
+        __syncthreads();
+        if (threadIdx.x == 0 && s.red[14] > PK_TIE_MAX) s.red[14] = PK_TIE_OVERFLOW;
+        __syncthreads();
+    }
     if (a.exact) {
         // replay the accepted values in arrival order through the reference heap
         uint32_t cnt = 0;
@@ -419,7 +463,7 @@ __device__ uint32_t pk_walk_merge(const PkWalkArgs& a, PkWalkSmem& s, uint32_t n
             uint32_t dc = s.red[13];
             for (uint32_t r = 0; r < cnt; ++r) {
                 uint32_t p = s.X_pos[r];
-                pk_heap_update(s.H_val, s.H_id, ftk, &dc, s.P_t[p], s.P_id[p]);
+                pk_heap_update(s.H, ftk, &dc, s.P_t[p], s.P_id[p]);
             }
             s.red[13] = dc;
         }
@@ -458,15 +502,17 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
     s.c_cls = (uint8_t*)ptr; ptr += WALK_CH;
     s.P_t = (float*)ptr; ptr += sizeof(float) * WALK_PMAX;
     s.P_sel = (uint8_t*)ptr; ptr += WALK_PMAX;
-    s.H_val = (float*)ptr; ptr += a.exact ? sizeof(float) * ((ftk + 3) & ~3u) : 0;
-    s.H_id = (uint32_t*)ptr; ptr += a.exact ? sizeof(uint32_t) * ((ftk + 3) & ~3u) : 0;
+    ptr = (unsigned char*)(((uintptr_t)ptr + 15) & ~(uintptr_t)15);
+    s.H = (uint2*)ptr; ptr += a.exact ? sizeof(uint2) * ((ftk + 2) & ~1u) : 0;
+    s.td_val = (uint32_t*)ptr; ptr += sizeof(uint32_t) * PK_TIE_MAX;
+    s.td_id = (uint32_t*)ptr; ptr += sizeof(uint32_t) * PK_TIE_MAX;
 
     uint32_t st_seg = 0, st_unc = 0, st_cells = 0, st_pts = 0;
     bool tie_dec = false;
+    uint32_t tie_sub = 0;
     for (uint32_t i = tid; i < WALK_PMAX; i += blockDim.x) s.P_sel[i] = 0;
-    if (a.exact)
-        for (uint32_t i = tid; i < ftk; i += blockDim.x) { s.H_val[i] = pk_sentinel(); s.H_id[i] = 0; }
-    if (tid == 0) { s.red[12] = 0xffffffffu; s.red[13] = ftk; }
+    if (a.exact) pk_heap_init(s.H, ftk);
+    if (tid == 0) { s.red[13] = ftk; s.red[14] = 0u; }
     __syncthreads();
     if (a.qstatus[qi] != 0) {
         if (tid == 0) a.out_n[qi] = 0;
@@ -524,7 +570,14 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
             uint32_t cut = 0xffffffffu;
             if (act && pre + cnt > WALK_PMAX) cut = (pre == 0) ? (uint32_t)tid + 1 : (uint32_t)tid;
             const uint32_t cutm = pk_bmin(cut, s.red);
-            const uint32_t cend = cutm < bhi ? cutm : bhi;
+            uint32_t cend = cutm < bhi ? cutm : bhi;
+            // do not split a fine-distance tie group across chunks: end the
+            // chunk at the group's first cell when the group starts inside it
+            if (j0 + cend < F && cend > 1 && fsd[j0 + cend] == fsd[j0 + cend - 1]) {
+                uint32_t g = cend - 1;
+                while (g > 0 && fsd[j0 + g - 1] == fsd[j0 + g]) --g;
+                if (g > 0) cend = g;
+            }
             if ((uint32_t)tid < cend) {
                 s.c_fd[tid] = fdj;
                 s.c_st[tid] = stj;
@@ -551,6 +604,15 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
                 float Tlow = (hi == 0xffffffffu && lo == 0xffffffffu) ? pk_sentinel() : pk_ord2f(hi);
                 Plow = pk_pivot(Tlow, qn, rr);
             }
+            // While the top set is not full the pivot is exactly the sentinel
+            // pivot: cells up to the one whose points complete the fill are
+            // classified with it; later cells with P_low.
+            uint32_t tfill = 0;
+            if (ns < ftk) {
+                uint32_t tf = pk_bmin(((uint32_t)tid < cend && ns + s.c_pre[tid + 1] >= ftk) ? (uint32_t)tid
+                                                                                             : 0xffffffffu, s.red);
+                tfill = tf == 0xffffffffu ? cend : tf + 1;
+            }
             uint32_t t0 = 0;
             while (true) {
                 st_seg++;
@@ -564,16 +626,28 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
                     const bool tied = (tid > 0 ? s.c_fd[tid - 1] == f : (j0 > 0 && fdprev0 == f)) ||
                                       ((uint32_t)tid + 1 < cend && s.c_fd[tid + 1] == f);
                     const float fprev = tied ? f : (tid > 0 ? s.c_fd[tid - 1] : fdprev0);
-                    const bool nobrk = f + mind < Plow;
+                    const float Pc = (uint32_t)tid < tfill ? Phi : Plow;
+                    const bool nobrk = f + mind < Pc;
                     const bool forced = big && (uint32_t)tid == cend - 1;
                     if (!forced && nobrk) {
-                        float bound = (Plow - cd) - fprev;
+                        float bound = (Pc - cd) - fprev;
                         bound = M < bound ? M : bound;
                         if (sv >= M) cls = 2;
                         else if (sv < bound) cls = 1;
                     }
                     s.c_cls[tid] = (uint8_t)cls;
                 }
+                // a member of a fine-distance tie group is sure only when its
+                // neighbours in the group are: the group is then decided as a unit
+                __syncthreads();
+                if (inr && cls != 0) {
+                    const float f = s.c_fd[tid];
+                    const bool tp = (uint32_t)tid > t0 && s.c_fd[tid - 1] == f && s.c_cls[tid - 1] == 0;
+                    const bool tn = (uint32_t)tid + 1 < cend && s.c_fd[tid + 1] == f && s.c_cls[tid + 1] == 0;
+                    if (tp || tn) cls = 0;
+                }
+                __syncthreads();
+                if (inr) s.c_cls[tid] = (uint8_t)cls;
                 const uint32_t u0 = pk_bmin((inr && cls == 0) ? (uint32_t)tid : 0xffffffffu, s.red);
                 const uint32_t u = u0 == 0xffffffffu ? cend : u0;
                 const uint32_t last1 = pk_bmax((inr && cls == 1 && (uint32_t)tid < u) ? (uint32_t)tid + 1 : 0u, s.red);
@@ -590,8 +664,71 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
                 if (u >= cend) break;
                 st_unc++;
                 const float fu = s.c_fd[u], su = s.c_st[u];
-                if ((u > 0 ? s.c_fd[u - 1] == fu : (j0 > 0 && fdprev0 == fu)) || (u + 1 < cend && s.c_fd[u + 1] == fu))
-                    tie_dec = true;
+                {
+                    // fine-distance tie group starting at u: the reference may
+                    // visit its members in another order (its heap sort of the
+                    // fine list breaks ties by heap position)
+                    const bool back_tie = u > 0 ? s.c_fd[u - 1] == fu : (j0 > 0 && fdprev0 == fu);
+                    uint32_t gend = u;
+                    while (gend + 1 < cend && s.c_fd[gend + 1] == fu) ++gend;
+                    const bool fwd_tie = gend + 1 == cend && j0 + cend < F && fsd[j0 + cend] == fu;
+                    const uint32_t gs = gend - u + 1;
+                    if (back_tie) tie_sub |= 0x10u;
+                    if (fwd_tie) tie_sub |= 0x20u;
+                    if (gs > 2) tie_sub |= 0x40u;
+                    if (gs == 2 && !back_tie && !fwd_tie && !(big && gend == cend - 1)) {
+                        // decide the pair under both visiting orders (dry merges)
+                        if (tid == 0) { s.c_cls[u] = 3; s.c_cls[u + 1] = 4; }
+                        __syncthreads();
+                        uint32_t outc[2];
+                        for (int ord = 0; ord < 2; ++ord) {
+                            float Pw = P, Mw = M;
+                            uint32_t res = 0;  // bit0 first scanned, bit1 second scanned, bit2 list ended
+                            for (int k = 0; k < 2; ++k) {
+                                const uint32_t cell = (ord == 0) ? u + k : u + 1 - k;
+                                if (fu + mind >= Pw) { res |= 4u; break; }
+                                if (s.c_st[cell] >= Mw) continue;
+                                res |= (cell == u) ? 1u : 2u;
+                                if (k == 0) {
+                                    uint32_t mm = pk_walk_select(s, nW, cell, cell + 1, cell == u ? 3u : 4u, seqbase, 0);
+                                    pk_walk_merge(a, s, ns, mm, seqbase, 0, 1);
+                                    const uint32_t hi = s.red[8], lo = s.red[9];
+                                    __syncthreads();
+                                    const float T1 = (hi == 0xffffffffu && lo == 0xffffffffu) ? T : pk_ord2f(hi);
+                                    Pw = pk_pivot(T1, qn, rr);
+                                    const float cand = (Pw - cd) - fu;
+                                    Mw = cand < Mw ? cand : Mw;
+                                }
+                            }
+                            outc[ord] = res;
+                        }
+                        if (outc[0] == outc[1]) {
+                            const uint32_t res = outc[0];
+                            if (res & 3u) {
+                                if (tid == 0) {
+                                    s.c_cls[u] = (res & 1u) ? 5 : 6;
+                                    s.c_cls[u + 1] = (res & 2u) ? 5 : 6;
+                                }
+                                __syncthreads();
+                                if (nW > 0) {
+                                    uint32_t mm = pk_walk_select(s, nW, u, u + 2, 5u, seqbase, 0);
+                                    ns = pk_walk_merge(a, s, ns, mm, seqbase, 0, 0, ptot);
+                                }
+                                T = pk_walk_top(s, ns, ftk);
+                                P = pk_pivot(T, qn, rr);
+                                const float cand = (P - cd) - fu;
+                                M = cand < M ? cand : M;
+                            }
+                            if (res & 4u) { done = true; break; }
+                            t0 = u + 2;
+                            continue;
+                        }
+                        tie_dec = true;
+                        tie_sub |= 0x80u;
+                    } else if (gs > 1 || back_tie || fwd_tie) {
+                        tie_dec = true;
+                    }
+                }
                 if (fu + mind >= P) { done = true; break; }
                 if (!(su >= M)) {
                     if (big && u == cend - 1) {
@@ -626,11 +763,11 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
     if (a.exact) {
         // the filter heap as rank_topN_points reads it: heap order when full,
         // heap-sorted ascending when not full (puck_index.cpp:552-554)
-        if (tid == 0 && ns < ftk) pk_heap_reorder(s.H_val, s.H_id, ftk);
+        if (tid == 0 && ns < ftk) pk_heap_reorder(s.H, ftk);
         __syncthreads();
         for (uint32_t r = tid; r < ns; r += blockDim.x) {
-            a.out_t[(uint64_t)qi * ftk + r] = s.H_val[r];
-            a.out_id[(uint64_t)qi * ftk + r] = s.H_id[r];
+            a.out_t[(uint64_t)qi * ftk + r] = pk_hv(s.H[r + 1]);
+            a.out_id[(uint64_t)qi * ftk + r] = s.H[r + 1].y;
         }
     } else {
         for (uint32_t r = tid; r < ns; r += blockDim.x) {
@@ -640,8 +777,18 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
     }
     if (tid == 0) {
         a.out_n[qi] = ns;
-        if (a.flags && ns >= ftk && s.red[12] == (uint32_t)(s.S_key[ftk - 1] >> 32)) a.flags[qi] |= 2u;
-        if (a.flags && tie_dec) a.flags[qi] |= 8u;
+        if (a.flags && tie_dec) a.flags[qi] |= (uint8_t)(8u | tie_sub);
+        if (a.tie_n) {
+            uint32_t tn = 0;
+            if (ns >= ftk) {
+                const uint32_t K = (uint32_t)(s.S_key[ftk - 1] >> 32);
+                if (s.red[14] == PK_TIE_OVERFLOW) tn = PK_TIE_OVERFLOW;
+                else
+                    for (uint32_t i = 0; i < s.red[14]; ++i)
+                        if (s.td_val[i] == K) a.tie_ids[(uint64_t)qi * PK_TIE_MAX + tn++] = s.td_id[i];
+            }
+            a.tie_n[qi] = (uint8_t)tn;
+        }
         if (a.stats) {
             a.stats[qi * 4 + 0] = st_seg;
             a.stats[qi * 4 + 1] = st_unc;
@@ -653,8 +800,8 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
 
 extern "C" size_t pk_walk_smem_bytes(uint32_t dim, uint32_t nsq, uint32_t ftk, int exact) {
     size_t b = 0;
-    b += sizeof(float) * WALK_PMAX + WALK_PMAX;
-    if (exact) b += (sizeof(float) + sizeof(uint32_t)) * ((ftk + 3) & ~3u);
+    b += sizeof(float) * WALK_PMAX + WALK_PMAX + 2 * sizeof(uint32_t) * PK_TIE_MAX + 16;
+    if (exact) b += sizeof(uint2) * ((ftk + 2) & ~1u);
     b += sizeof(uint64_t) * ((ftk + 1) & ~1u);
     b += sizeof(uint64_t) * WALK_PMAX;
     b += sizeof(uint32_t) * ((ftk + 3) & ~3u);
@@ -700,8 +847,11 @@ struct PkRerankArgs {
     uint32_t* out_id;
     uint32_t* out_n;
     const uint32_t* qlist;   // block -> query index (null: identity)
-    uint8_t* flags;          // bit 2 set when exact distances tie inside the top-k (+1)
+    uint8_t* flags;          // bit 2: exact distances tie inside the top-k (+1); bit 1: a
+                             // filter-boundary tie group reaches the top-k
     int exact;               // replay the reference result heap over the filter heap order
+    const uint32_t* tie_ids; // from the walk (may be null)
+    const uint8_t* tie_n;
 };
 
 #define RERANK_THREADS 256
@@ -718,10 +868,15 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
     uint64_t* keys = (uint64_t*)smem;
     float* qv = (float*)(smem + sizeof(uint64_t) * n2);
     float* dist = qv + ((a.dim + 3) & ~3u);
-    float* rv = dist + ((ftk + 3) & ~3u);
-    uint32_t* rt = (uint32_t*)(rv + ((topk + 3) & ~3u));
+    uint2* RH = (uint2*)(((uintptr_t)(dist + ((ftk + 3) & ~3u)) + 15) & ~(uintptr_t)15);
     for (uint32_t i = threadIdx.x; i < a.dim; i += blockDim.x) qv[i] = a.q[(uint64_t)qi * a.dim + i];
     __syncthreads();
+    // filter-boundary tie group (entries whose filter distance equals the final
+    // top, kept or dropped): smallest exact distance among its members
+    const uint32_t tn = (!a.exact && a.tie_n && n >= ftk) ? a.tie_n[qi] : 0u;
+    const float tfin = n >= ftk ? a.filt_t[(uint64_t)qi * ftk + ftk - 1] : 0.f;
+    __shared__ uint32_t tie_min;
+    if (threadIdx.x == 0) tie_min = tn == PK_TIE_OVERFLOW ? 0u : 0xffffffffu;
     for (uint32_t r = threadIdx.x; r < n2; r += blockDim.x) {
         uint64_t key = ~0ull;
         if (r < n) {
@@ -733,22 +888,36 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
         if (!a.exact) keys[r] = key;
     }
     __syncthreads();
+    if (tn > 0 && tn != PK_TIE_OVERFLOW) {
+        for (uint32_t r = threadIdx.x; r < n + tn; r += blockDim.x) {
+            float d;
+            if (r < n) {
+                if (!(a.filt_t[(uint64_t)qi * ftk + r] == tfin)) continue;
+                d = pk_ord2f((uint32_t)(keys[r] >> 32));
+            } else {
+                uint32_t mid = a.tie_ids[(uint64_t)qi * PK_TIE_MAX + (r - n)];
+                d = pk_rerank_dist(a.raw + (uint64_t)mid * a.dim, qv, a.dim);
+            }
+            atomicMin(&tie_min, pk_f2ord(d));
+        }
+    }
+    __syncthreads();
     if (a.exact) {
         if (threadIdx.x == 0) {
-            for (uint32_t r = 0; r < topk; ++r) { rv[r] = pk_sentinel(); rt[r] = 0xffffffffu; }
+            for (uint32_t r = 0; r <= topk; ++r) RH[r] = make_uint2(0x7f7f7f7fu, 0xffffffffu);
             uint32_t dc = topk;
             for (uint32_t r = 0; r < n; ++r) {
                 float d = dist[r];
-                if (d < rv[0])
-                    pk_heap_update(rv, rt, topk, &dc, d, a.mem2local[a.filt_id[(uint64_t)qi * ftk + r]]);
+                if (d < pk_hv(RH[1]))
+                    pk_heap_update(RH, topk, &dc, d, a.mem2local[a.filt_id[(uint64_t)qi * ftk + r]]);
             }
-            pk_heap_reorder(rv, rt, topk);
+            pk_heap_reorder(RH, topk);
             a.out_n[qi] = topk - dc;
         }
         __syncthreads();
         for (uint32_t r = threadIdx.x; r < topk; r += blockDim.x) {
-            a.out_d[(uint64_t)qi * topk + r] = rv[r];
-            a.out_id[(uint64_t)qi * topk + r] = rt[r];
+            a.out_d[(uint64_t)qi * topk + r] = pk_hv(RH[r + 1]);
+            a.out_id[(uint64_t)qi * topk + r] = RH[r + 1].y;
         }
         return;
     }
@@ -786,6 +955,12 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
         a.out_id[(uint64_t)qi * topk + r] = id;
     }
     if (tie && a.flags) a.flags[qi] = 4u | a.flags[qi];
+    // a tie-group member at or inside the k-th distance: which members the
+    // reference kept decides the output
+    if (threadIdx.x == 0 && tn > 0 && a.flags) {
+        const uint32_t kth = kout == topk && kout > 0 ? (uint32_t)(keys[kout - 1] >> 32) : 0xffffffffu;
+        if (tie_min <= kth) a.flags[qi] = 2u | a.flags[qi];
+    }
     if (threadIdx.x == 0) a.out_n[qi] = kout;
 }
 
@@ -794,7 +969,7 @@ extern "C" hipError_t pk_launch_rerank(const PkRerankArgs* args, uint32_t nq, hi
     uint32_t n2 = 2;
     while (n2 < args->ftk) n2 <<= 1;
     size_t smem = sizeof(uint64_t) * n2 + sizeof(float) * ((args->dim + 3) & ~3u) +
-                  sizeof(float) * ((args->ftk + 3) & ~3u) + 8 * ((args->topk + 3) & ~3u);
+                  sizeof(float) * ((args->ftk + 3) & ~3u) + 16 + 8 * ((args->topk + 2) & ~1u);
     if (smem > 64 * 1024) return hipErrorInvalidValue;
     hipLaunchKernelGGL(pk_rerank, dim3(nq), dim3(RERANK_THREADS), smem, s, *args);
     return hipGetLastError();
@@ -811,39 +986,38 @@ __global__ __launch_bounds__(256) void pk_exact_center_order(const float* __rest
                                                              float* __restrict__ fsort_d, uint32_t* __restrict__ fsort_id) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t qi = qlist[blockIdx.x];
-    float* row = (float*)smem;                         // max(C, F)
     const uint32_t L = C > F ? C : F;
-    float* hv = row + L;                               // max(scc, F)
-    uint32_t* ht = (uint32_t*)(hv + L);
-    // coarse
+    uint2* H = (uint2*)smem;                           // L + 2 entries
+    float* row = (float*)(H + ((L + 2) & ~1u));        // max(C, F)
+    // coarse top-scc (hierarchical_cluster_index.cpp:709-726)
     for (uint32_t i = threadIdx.x; i < C; i += blockDim.x) row[i] = cdist[(uint64_t)qi * C + i];
+    if (scc > 0) pk_heap_init(H, scc);
     __syncthreads();
     if (threadIdx.x == 0 && scc > 0) {
-        for (uint32_t i = 0; i < scc; ++i) { hv[i] = pk_sentinel(); ht[i] = 0; }
         uint32_t dc = scc;
         for (uint32_t c = 0; c < C; ++c)
-            if (row[c] < hv[0]) pk_heap_update(hv, ht, scc, &dc, row[c], c);
-        pk_heap_reorder(hv, ht, scc);
+            if (row[c] < pk_hv(H[1])) pk_heap_update(H, scc, &dc, row[c], c);
+        pk_heap_reorder(H, scc);
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < scc; i += blockDim.x) {
-        csel_d[(uint64_t)qi * scc + i] = hv[i];
-        csel_id[(uint64_t)qi * scc + i] = ht[i];
+        csel_d[(uint64_t)qi * scc + i] = pk_hv(H[i + 1]);
+        csel_id[(uint64_t)qi * scc + i] = H[i + 1].y;
     }
     __syncthreads();
-    // fine
+    // fine full heap sort (puck_index.cpp:497-504)
     for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) row[i] = fdist[(uint64_t)qi * F + i];
+    pk_heap_init(H, F);
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (uint32_t i = 0; i < F; ++i) { hv[i] = pk_sentinel(); ht[i] = 0; }
         uint32_t dc = F;
-        for (uint32_t k = 0; k < F; ++k) pk_heap_update(hv, ht, F, &dc, row[k], k);
-        pk_heap_reorder(hv, ht, F);
+        for (uint32_t k = 0; k < F; ++k) pk_heap_update(H, F, &dc, row[k], k);
+        pk_heap_reorder(H, F);
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) {
-        fsort_d[(uint64_t)qi * F + i] = hv[i];
-        fsort_id[(uint64_t)qi * F + i] = ht[i];
+        fsort_d[(uint64_t)qi * F + i] = pk_hv(H[i + 1]);
+        fsort_id[(uint64_t)qi * F + i] = H[i + 1].y;
     }
 }
 
@@ -853,7 +1027,7 @@ extern "C" hipError_t pk_launch_exact_center_order(const float* cdist, uint32_t 
                                                    uint32_t* fsort_id, hipStream_t s) {
     if (nlist == 0) return hipSuccess;
     const uint32_t L = C > F ? C : F;
-    size_t smem = sizeof(float) * L * 3;
+    size_t smem = sizeof(uint2) * ((L + 2) & ~1u) + sizeof(float) * L;
     if (smem > 160 * 1024) return hipErrorInvalidValue;
     static bool attr = false;
     if (!attr) {

@@ -175,11 +175,11 @@ class GpuIndex:
         return out
 
     def search(self, queries, topk, search_coarse_count, filter_topk, topk_max=None, radius_rate=1.0,
-               out=None):
+               out=None, replay_all=False):
         """Returns (distance [n, topk], labels [n, topk], result_num [n])."""
         n = queries.shape[0]
         topk_max = topk if topk_max is None else topk_max
-        sp = _lib.SearchParams(topk_max, search_coarse_count, filter_topk, radius_rate)
+        sp = _lib.SearchParams(topk_max, search_coarse_count, filter_topk, radius_rate, int(bool(replay_all)))
         if out is None:
             if _is_dev(queries):
                 import torch
@@ -205,6 +205,6 @@ class GpuIndex:
 
     def last_stage_ms(self):
         ms = np.zeros(5, np.float32)
-        cnt = np.zeros(5, np.uint64)
+        cnt = np.zeros(13, np.uint64)
         self.ctx.check(self.ctx.L.puckgpu_last_stage_ms(self.ctx.h, _ptr(ms), _ptr(cnt)))
         return ms, cnt

@@ -94,6 +94,27 @@ def test_search_bitexact(ctx, which, scc, ftk, topk, idx_l2, idx_cos):
         assert np.array_equal(Ig[k, :o["n"]], o["ids"][:o["n"]]), f"query {k}"
 
 
+@pytest.mark.parametrize("which", ["l2", "cos"])
+def test_search_replay_path_bitexact(ctx, which, idx_l2, idx_cos):
+    # every query through the reference-heap replay (the path taken when the
+    # result depends on the order of exactly equal distances)
+    ix, q = idx_l2 if which == "l2" else idx_cos
+    gi = _gpu_index(ctx, ix)
+    oi = CI.oracle_index(ix)
+    for scc, ftk, topk in [(6, 300, 10), (2, 1000, 50)]:
+        Dg, Ig, Rg = gi.search(q, topk, scc, ftk, replay_all=True)
+        ft, fm, fn = gi.last_filter(q.shape[0], ftk)
+        for k in range(q.shape[0]):
+            o = oi.search_one(q[k], topk, scc, ftk)
+            # replay returns the filter heap in the reference's array order
+            assert fn[k] == o["filt_n"]
+            assert np.array_equal(_bits(ft[k, :fn[k]]), _bits(o["filt_val"][:fn[k]]))
+            assert np.array_equal(fm[k, :fn[k]], o["filt_mem"][:fn[k]])
+            assert Rg[k] == o["n"]
+            assert np.array_equal(Ig[k, :o["n"]], o["ids"][:o["n"]])
+            assert np.array_equal(_bits(Dg[k, :o["n"]]), _bits(o["dist"][:o["n"]]))
+
+
 def test_search_not_full_and_clamps(ctx, idx_l2):
     ix, q = idx_l2
     gi = _gpu_index(ctx, ix)
@@ -123,12 +144,15 @@ def test_search_errors(ctx, idx_cos):
 
 
 def test_assign_cells_bitexact(ctx, idx_l2, idx_cos):
+    rng = np.random.default_rng(9)
     for ix, _ in (idx_l2, idx_cos):
         x = ix["raw"][:3000]
-        cg, dg = ctx.assign_cells(ix["coarse"], ix["fine"], x, with_dist=True)
-        co, do = O.assign_cells(ix["coarse"], ix["fine"], x)
-        assert np.array_equal(cg, co)
-        assert np.array_equal(_bits(dg), _bits(do))
+        # fine centers in stored (descending-norm) order, then in a shuffled order
+        for fine in (ix["fine"], np.ascontiguousarray(ix["fine"][rng.permutation(ix["fine"].shape[0])])):
+            cg, dg = ctx.assign_cells(ix["coarse"], fine, x, with_dist=True)
+            co, do = O.assign_cells(ix["coarse"], fine, x)
+            assert np.array_equal(cg, co)
+            assert np.array_equal(_bits(dg), _bits(do))
 
 
 def test_encode_filter_bitexact(ctx, idx_l2, idx_cos):
