@@ -39,6 +39,8 @@ def main():
     ap.add_argument("--target-recall", type=float, default=0.95)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="one 2 GiB device-to-device copy before the search (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design "
@@ -73,6 +75,14 @@ def main():
     gt_i, gt_d = gt_i.cpu().numpy(), gt_d.cpu().numpy()
     log(f"ground truth {time.time() - t0:.1f}s")
 
+    if args.calib_copy:
+        src = torch.empty(1 << 29, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("calibration copy: 2 GiB device-to-device")
     out = (torch.empty((nq, topk), dtype=torch.float32, device=dev),
            torch.empty((nq, topk), dtype=torch.int32, device=dev),
            torch.empty(nq, dtype=torch.int32, device=dev))
@@ -122,6 +132,12 @@ def main():
     walk_bytes = cells * 8 + points * (4 + nsq)
     avg_walk_s = np.mean(walk_ms) / 1000.0
     achieved = walk_bytes / avg_walk_s / 1e9
+    # HBM traffic per launch of the same kernel and workload, from the
+    # committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (copy-calibrated)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "pk_walk_10Mx96_traffic.json")
+    if N == 10_000_000 and nq == 10_000 and os.path.exists(tpath):
+        traffic = int(json.load(open(tpath))["traffic_bytes_per_launch"])
     result = {
         "metric": "QPS at recall@10=0.95 (10M x 96, plain GPU Puck search)",
         "value": round(qps, 1),
@@ -143,7 +159,7 @@ def main():
         "stage_ms_per_step": {k: round(float(v) / args.steps, 3) for k, v in
                               zip(["prep+gemm", "center_sort", "walk_scan", "rerank", "tie_replay"], stage_tot)},
         "roofline": {"kernel": "pk_walk", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": int(walk_bytes),
                      "avg_launch_ms": round(avg_walk_s * 1000.0, 3)},
         "walk_counters_per_query": {k: round(float(v) / nq, 3) for k, v in
