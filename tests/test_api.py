@@ -1,0 +1,94 @@
+"""Host-side API: flags, reference on-disk formats, PySearcher error paths
+(no GPU needed)."""
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from puck_amd import flags as FL
+from puck_amd import index_io
+from puck_amd import py_puck_api as api
+
+
+@pytest.fixture(autouse=True)
+def _reset_flags():
+    FL.reset()
+    yield
+    FL.reset()
+
+
+def test_flag_defaults_and_update():
+    assert FL.get("whether_norm") is True          # puck_gflags.cpp:42
+    assert FL.get("filter_topk") == 1100            # FLAGS_topk * 11 at static init
+    assert FL.get("filter_nsq") == 64               # default feature_dim / 4
+    assert FL.update_gflag("topk", "10") and FL.get("topk") == 10
+    assert FL.update_gflag("whether_norm", "false") and FL.get("whether_norm") is False
+    assert FL.update_gflag("radius_rate", "1.5") and FL.get("radius_rate") == 1.5
+    assert not FL.update_gflag("no_such_flag", "1")
+    assert not FL.update_gflag("topk", "ten")
+    assert FL.is_default("filter_topk") and not FL.is_default("topk")
+
+
+def test_index_conf_layout(tmp_path):
+    p = tmp_path / "index.dat"
+    index_io.write_index_conf(str(p), 96, True, 1234, 48, 2000, 2000)
+    raw = p.read_bytes()
+    assert len(raw) == 8 + 43
+    assert struct.unpack_from("<Q", raw)[0] == 43
+    # packed fields (hierarchical_cluster_index.cpp:522-543)
+    assert struct.unpack_from("<iI", raw, 8) == (1, 96)
+    assert raw[16] == 1
+    c = index_io.read_index_conf(str(p))
+    assert (c["feature_dim"], c["whether_norm"], c["total_point_count"], c["filter_nsq"],
+            c["coarse_cluster_count"], c["fine_cluster_count"], c["whether_pq"]) == (96, True, 1234, 48, 2000, 2000, False)
+
+
+def test_fvecs_roundtrip_and_checks(tmp_path):
+    x = np.random.default_rng(0).standard_normal((7, 5)).astype(np.float32)
+    p = str(tmp_path / "a.dat")
+    index_io.write_fvecs(p, x)
+    assert os.path.getsize(p) == 7 * (4 + 20)
+    assert np.array_equal(index_io.read_fvecs(p, 5, 7), x)
+    with pytest.raises(ValueError):
+        index_io.read_fvecs(p, 5, 8)
+    with pytest.raises(ValueError):
+        index_io.read_fvecs(p, 4)
+
+
+def test_index_dir_roundtrip(tmp_path):
+    rng = np.random.default_rng(1)
+    C, F, d, nsq, N = 3, 4, 8, 4, 50
+    arrs = dict(coarse=rng.standard_normal((C, d)).astype(np.float32),
+                fine=rng.standard_normal((F, d)).astype(np.float32),
+                codebook=rng.standard_normal(nsq * 256 * 2).astype(np.float32),
+                cell_assign=rng.integers(0, C * F, N).astype(np.uint32),
+                records=rng.integers(0, 256, (N, 4 + nsq)).astype(np.uint8),
+                features=rng.standard_normal((N, d)).astype(np.float32))
+    index_io.write_index(str(tmp_path), FL, arrs["coarse"], arrs["fine"], arrs["codebook"], arrs["cell_assign"],
+                         arrs["records"], arrs["features"], False)
+    conf, back = index_io.read_index(str(tmp_path), FL)
+    assert conf["index_type"] == index_io.INDEX_PUCK and conf["filter_nsq"] == nsq
+    for k in arrs:
+        assert np.array_equal(back[k], arrs[k]), k
+    # a truncated file fails the length check
+    p = index_io.paths(str(tmp_path), FL)["filter_data"]
+    open(p, "r+b").truncate(10)
+    with pytest.raises(ValueError):
+        index_io.read_index(str(tmp_path), FL)
+
+
+def test_searcher_init_errors(tmp_path):
+    s = api.PySearcher()
+    api.update_gflag("index_path", str(tmp_path / "missing"))
+    assert s.init() == -1
+    d = tmp_path / "tinker"
+    d.mkdir()
+    index_io.write_index_conf(str(d / "index.dat"), 8, False, 0, 4, 2, 2, index_type=index_io.INDEX_TINKER)
+    api.update_gflag("index_path", str(d))
+    assert s.init() == -1
+    e = tmp_path / "pq"
+    e.mkdir()
+    index_io.write_index_conf(str(e / "index.dat"), 8, False, 0, 4, 2, 2, whether_pq=True)
+    api.update_gflag("index_path", str(e))
+    assert s.init() == -1
