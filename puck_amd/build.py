@@ -25,6 +25,19 @@ def sources():
     return sorted(glob.glob(os.path.join(CSRC, "*.hip")))
 
 
+def build_host_adapter(verbose=False):
+    """C++ puck::Index adapter over the C ABI and its test driver (g++)."""
+    root = os.path.dirname(HERE)
+    out = os.path.join(root, "tests", "cpp", "host_adapter_test")
+    cmd = ["g++", "-O2", "-std=c++17", "-o", out, os.path.join(root, "tests", "cpp", "host_adapter_main.cpp"),
+           os.path.join(HERE, "host", "gpu_puck_index.cpp"), "-L" + HERE, "-lpuckgpu",
+           "-Wl,-rpath,$ORIGIN/../../puck_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    return out
+
+
 def build(verbose=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = LIB + ".tmp"
@@ -33,6 +46,7 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    build_host_adapter(verbose)
     return LIB
 
 

@@ -1,0 +1,87 @@
+// gpu_puck_index.h -- the reference's puck::Index API served by libpuckgpu.so.
+//
+// Request / Response / Index have the shape of puck/index.h:30-81 so that a
+// caller written against the reference compiles unchanged; GpuPuckIndex
+// stands in for PuckIndex (puck/puck/puck_index.h) with whether_pq = false.
+// The host code reaches the GPU only through include/puck_gpu.h.
+#pragma once
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "../../include/puck_gpu.h"
+
+namespace puck {
+
+enum class IndexType { HIERARCHICAL_CLUSTER = 0, PUCK = 1, TINKER = 2 };
+
+struct Request {
+    uint32_t topk = 100;
+    const float* feature = nullptr;
+};
+
+struct Response {
+    float* distance = nullptr;
+    uint32_t* local_idx = nullptr;
+    uint32_t result_num = 0;
+};
+
+class Index {
+public:
+    virtual ~Index() {}
+    virtual int init() = 0;
+    virtual int search(const Request* request, Response* response) = 0;
+    virtual int train() = 0;
+    virtual int build() = 0;
+};
+
+// The gflags this path reads (puck/gflags/puck_gflags.cpp:26-81), as plain
+// fields with the reference's defaults.
+struct GpuIndexFlags {
+    std::string index_path = "puck_index";
+    std::string index_file_name = "index.dat";
+    std::string feature_file_name = "all_data.feat.bin";
+    std::string coarse_codebook_file_name = "coarse.dat";
+    std::string fine_codebook_file_name = "fine.dat";
+    std::string cell_assign_file_name = "cell_assign.dat";
+    std::string filter_codebook_file_name = "filter_codebook.dat";
+    std::string filter_data_file_name = "filter_data.dat";
+    uint32_t topk = 100;
+    uint32_t search_coarse_count = 200;
+    int32_t filter_topk = -1;  // unset: 11 * topk (index_conf.cpp:174-177)
+    double radius_rate = 1.0;
+    int device = 0;
+};
+
+class GpuPuckIndex : public Index {
+public:
+    explicit GpuPuckIndex(const GpuIndexFlags& flags = GpuIndexFlags());
+    ~GpuPuckIndex() override;
+    // Reads index.dat and the index files, loads them into HBM (0 / -1).
+    int init() override;
+    // One query (puck_index.cpp:559-600): -1 for topk > FLAGS_topk or a null
+    // feature, leaving the response untouched.
+    int search(const Request* request, Response* response) override;
+    // n queries, row stride topk (PySearcher::search semantics).
+    int search_batch(uint32_t n, const float* queries, uint32_t topk, float* distance, uint32_t* labels,
+                     uint32_t* result_num = nullptr);
+    // Assigns all_data.feat.bin to cells and writes cell_assign.dat,
+    // filter_data.dat and index.dat with the codebooks already on disk.
+    int build() override;
+    // Training runs in puck_amd.train (Python); returns -1 here.
+    int train() override;
+    const char* last_error() const { return _err.c_str(); }
+    uint32_t dim() const { return _dim; }
+
+private:
+    GpuIndexFlags _flags;
+    puckgpu_ctx* _ctx = nullptr;
+    puckgpu_index* _index = nullptr;
+    puckgpu_search_params _sp{};
+    uint32_t _dim = 0;
+    std::string _err;
+    std::mutex _mu;  // one context: searches from several threads are serialised
+};
+
+}  // namespace puck

@@ -1,0 +1,70 @@
+"""The C++ puck::Index adapter (puck_amd/host) on the GPU: Request/Response
+searches, -1 on topk > FLAGS_topk, and build() of cell_assign / filter_data,
+checked against the oracle on the same index files."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from puck_amd import datasets as D
+from puck_amd import flags as FL
+from puck_amd import index_io
+import cpu_index as CI
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "host_adapter_test")
+
+
+def _run(index_path, q, topk, flags_topk, scc, ftk, tmp, build=False):
+    qf = str(tmp / "q.f32")
+    np.ascontiguousarray(q, np.float32).tofile(qf)
+    out = str(tmp / "out.bin")
+    args = [BIN, index_path, qf, str(q.shape[0]), str(q.shape[1]), str(topk), str(flags_topk), str(scc), str(ftk), out]
+    if build:
+        args.append("build")
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(out, dtype=np.uint32)
+    n = q.shape[0]
+    Dm = raw[:n * topk].view(np.float32).reshape(n, topk)
+    Im = raw[n * topk:2 * n * topk].reshape(n, topk)
+    R = raw[2 * n * topk:]
+    return Dm, Im, R
+
+
+def test_cpp_adapter_search(tmp_path):
+    x = D.mixture_uint8_grid(5000, 32, 60, seed=50, means_seed=51)
+    q = D.mixture_uint8_grid(24, 32, 60, seed=52, means_seed=51)
+    ix = CI.build_index(x, 16, 16, 8, whether_norm=False, train_n=4000)
+    d = tmp_path / "idx"
+    index_io.write_index(str(d), FL, ix["coarse"], ix["fine"], ix["codebook"], ix["cell_assign"], ix["records"],
+                         ix["raw"], False)
+    Dm, Im, R = _run(str(d), q, 10, 50, 5, 200, tmp_path)
+    Do, Io, Ro, _ = CI.oracle_index(ix).search_batch(q, 10, 5, 200, max_topk=50)
+    assert np.array_equal(R, Ro)
+    assert np.array_equal(Im, Io)
+    assert np.array_equal(Dm.view(np.uint32), Do.view(np.uint32))
+
+
+def test_cpp_adapter_build(tmp_path):
+    x = D.unit_norm_mixture(6000, 32, 40, seed=60, means_seed=61)
+    q = D.unit_norm_mixture(16, 32, 40, seed=62, means_seed=61)
+    ix = CI.build_index(x, 12, 12, 16, whether_norm=True, train_n=4000)
+    d = tmp_path / "idx"
+    p = index_io.paths(str(d), FL)
+    os.makedirs(str(d))
+    index_io.write_index_conf(p["index"], 32, True, x.shape[0], 16, 12, 12)
+    index_io.write_fvecs(p["coarse"], ix["coarse"])
+    index_io.write_fvecs(p["fine"], ix["fine"])
+    index_io.write_fvecs(p["filter_codebook"], ix["codebook"].reshape(16 * 256, 2))
+    index_io.write_fvecs(p["feature"], x)
+    Dm, Im, R = _run(str(d), q, 10, 10, 4, 100, tmp_path, build=True)
+    conf, arr = index_io.read_index(str(d), FL)
+    assert np.array_equal(arr["cell_assign"], ix["cell_assign"])       # GPU assignment == oracle assignment
+    assert np.array_equal(arr["records"], ix["records"])               # GPU encoding == oracle encoding
+    Do, Io, _, _ = CI.oracle_index(ix).search_batch(q, 10, 4, 100)
+    assert np.array_equal(Im, Io)
+    assert np.array_equal(Dm.view(np.uint32), Do.view(np.uint32))
