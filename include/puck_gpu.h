@@ -18,6 +18,8 @@
  *   puckgpu_search         PySearcher::search + ParallelFor + PuckIndex::search
  *                          (pyapi_wrapper/py_api_wrapper.cpp:111-171;
  *                          puck/puck/puck_index.cpp:559-600; puck/index.h:30-81)
+ *   puckgpu_search_batch   the same, one Request (own topk) per query, with
+ *                          per-query search_coarse_count and fine budget
  *   puckgpu_assign_cells   HierarchicalClusterIndex::assign / nearest_cell_assign
  *                          (hierarchical_cluster_index.cpp:1042-1138)
  *   puckgpu_encode_filter  PuckIndex::puck_assign, filter quantization only
@@ -95,6 +97,90 @@ int puckgpu_index_export(puckgpu_ctx* ctx, const puckgpu_index* index, float* co
 int puckgpu_search(puckgpu_ctx* ctx, const puckgpu_index* index, const puckgpu_search_params* params,
                    uint32_t n, const float* queries, uint32_t topk, float* distance, uint32_t* labels,
                    uint32_t* result_num, int device_ptrs);
+/* Per-query parameters of puckgpu_search_batch.  The reference's Request
+ * carries only topk (puck/index.h:60-69); search_coarse_count is a process
+ * flag there (index_conf.cpp:64) and the fine budget has no counterpart, so
+ * they live in this side struct and Request stays as it is.  Each pointer is
+ * NULL (the batch-wide value applies) or an array of n entries, host or device
+ * memory as the call's device_ptrs says. */
+typedef struct {
+    const uint32_t* topk;                /* 1..topk_max; another value gives that query result_num 0 */
+    const uint32_t* search_coarse_count; /* coarse lists walked, clamped to params->search_coarse_count
+                                            (the batch maximum, itself clamped to C) */
+    const uint32_t* fine_budget;         /* fine clusters considered per coarse list, a prefix of the
+                                            query's ascending fine-distance order; 0 = all F (the
+                                            reference's walk) */
+} puckgpu_query_params;
+
+/* Batched search with per-query parameters (PySearcher::search's batch, one
+ * Request per query with its own topk; py_api_wrapper.cpp:154-171).  Output
+ * rows have stride params->topk_max; entries past a query's topk hold the
+ * unfilled-slot values.  filter_topk must be >= topk_max. */
+int puckgpu_search_batch(puckgpu_ctx* ctx, const puckgpu_index* index, const puckgpu_search_params* params,
+                         const puckgpu_query_params* query_params, uint32_t n, const float* queries,
+                         float* distance, uint32_t* labels, uint32_t* result_num, int device_ptrs);
+/* ---- Learned probe ------------------------------------------------------
+ * No counterpart in the reference.  A small MLP predicts, per query, the log
+ * of the fine budget (puckgpu_query_params.fine_budget) the cell walk needs
+ * to reach its top-k; the fixed-budget walk (all F fine clusters, the
+ * reference's) is its fallback.  Network: in = dim + coarse_features +
+ * fine_features inputs (the normalised query, its smallest coarse distances
+ * |c|^2/2 - <c,q> and its smallest fine distances -<f,q>, ascending), each
+ * standardised as (x - mu) / sigma; two hidden GELU layers of `hidden` units;
+ * one output.  Weights are used in BF16 with FP32 accumulation.
+ * Packed FP32 parameters (puckgpu_probe_param_count floats):
+ *   W1 [hidden x in], b1 [hidden], W2 [hidden x hidden], b2 [hidden],
+ *   w3 [hidden], b3 [1], mu [in], 1/sigma [in]. */
+typedef struct puckgpu_probe puckgpu_probe;
+typedef struct {
+    uint32_t coarse_features; /* smallest coarse distances used (<= C) */
+    uint32_t fine_features;   /* smallest fine distances used (<= F) */
+    uint32_t hidden;          /* units per hidden layer */
+} puckgpu_probe_shape;
+typedef struct {
+    float lr, beta1, beta2, eps;
+} puckgpu_adam_params;
+/* Budget from the prediction y: ceil(exp(y + margin)) clamped to
+ * [min_budget, F]; all F (the reference's walk) when y is not finite or
+ * exp(y + margin) >= fallback_frac * F. */
+typedef struct {
+    float margin;
+    uint32_t min_budget;
+    float fallback_frac;
+} puckgpu_budget_params;
+
+int puckgpu_probe_create(puckgpu_ctx* ctx, uint32_t dim, const puckgpu_probe_shape* shape, puckgpu_probe** out);
+void puckgpu_probe_free(puckgpu_probe* probe);
+uint32_t puckgpu_probe_param_count(const puckgpu_probe* probe);
+/* Load all parameters (resets the Adam state) / read them back. */
+int puckgpu_probe_set_params(puckgpu_ctx* ctx, puckgpu_probe* probe, const float* params, int device_ptrs);
+int puckgpu_probe_get_params(puckgpu_ctx* ctx, const puckgpu_probe* probe, float* params, int device_ptrs);
+/* Raw (unstandardised) features of n queries [n x in], computed by the search
+ * pipeline's own center GEMM and sorts; fine_order [n x F] (may be NULL)
+ * receives each query's ascending fine-distance order (fine ids). */
+int puckgpu_probe_features(puckgpu_ctx* ctx, const puckgpu_index* index, const puckgpu_probe* probe, uint32_t n,
+                           const float* queries, float* features, uint32_t* fine_order, int device_ptrs);
+/* One training step on a batch: forward, mean-squared-error loss against
+ * target [n], backward.  With adam != NULL the parameters take one Adam step;
+ * loss (host, may be NULL) receives the batch's mean squared error, grads
+ * (may be NULL) the gradient of that loss for the trained block (the first
+ * param_count - 2*in floats of the packed layout). */
+int puckgpu_probe_train_step(puckgpu_ctx* ctx, puckgpu_probe* probe, uint32_t n, const float* features,
+                             const float* target, const puckgpu_adam_params* adam, float* loss, float* grads,
+                             int device_ptrs);
+/* Network output y [n] for raw features [n x in]. */
+int puckgpu_probe_predict(puckgpu_ctx* ctx, const puckgpu_probe* probe, uint32_t n, const float* features,
+                          float* y, int device_ptrs);
+/* puckgpu_search_batch with the fine budget of every query predicted by the
+ * probe inside the same pipeline (features, network and budget run on the
+ * search's stream between the center sorts and the walk).  query_params'
+ * fine_budget is ignored; budget_out [n] (may be NULL) receives the budgets. */
+int puckgpu_search_learned(puckgpu_ctx* ctx, const puckgpu_index* index, const puckgpu_probe* probe,
+                           const puckgpu_search_params* params, const puckgpu_budget_params* budget,
+                           const puckgpu_query_params* query_params, uint32_t n, const float* queries,
+                           float* distance, uint32_t* labels, uint32_t* result_num, uint32_t* budget_out,
+                           int device_ptrs);
+
 /* Stage output of the last search on this context (host buffers):
  * filter-stage survivors per query (filter_topk entries per row, ascending
  * filter distance, memory ids) and their count.  For tests and diagnostics. */

@@ -66,13 +66,16 @@ def lib():
     L.po_index_export.argtypes = [vp, _f32p, _f32p, _u32p, _f32p, _u32p]
     L.po_search_one.restype = ctypes.c_int
     L.po_search_one.argtypes = [vp, _f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
-                                ctypes.c_uint32, ctypes.c_double, _f32p, _u32p,
+                                ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32, _f32p, _u32p,
                                 ctypes.POINTER(ctypes.c_uint32), _f32p, _u32p,
                                 ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(Stats), _i32p, _i32p]
     L.po_search_batch.restype = ctypes.c_int
     L.po_search_batch.argtypes = [vp, ctypes.c_uint32, _f32p, ctypes.c_uint32, ctypes.c_uint32,
                                   ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, _f32p, _u32p,
                                   _u32p, ctypes.c_int, ctypes.POINTER(Stats)]
+    L.po_search_batch_pq.restype = ctypes.c_int
+    L.po_search_batch_pq.argtypes = [vp, ctypes.c_uint32, _f32p, vp, ctypes.c_uint32, vp, ctypes.c_uint32, vp,
+                                     ctypes.c_uint32, ctypes.c_double, _f32p, _u32p, _u32p, ctypes.c_int]
     L.po_assign_cells.restype = ctypes.c_int
     L.po_assign_cells.argtypes = [_f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                   _f32p, ctypes.c_uint32, _u32p, _f32p, ctypes.c_int]
@@ -178,7 +181,7 @@ class OracleIndex:
         return dict(coarse_norms=cn, stationary=st, cell_start=cs, min_dist_offset=mdo,
                     mem2local=m2l[:N])
 
-    def search_one(self, q, topk, scc, ftk, max_topk=None, radius_rate=1.0, trace=False):
+    def search_one(self, q, topk, scc, ftk, max_topk=None, radius_rate=1.0, trace=False, fine_budget=0):
         max_topk = topk if max_topk is None else max_topk
         D = np.zeros(max(topk, 1), np.float32)
         I = np.zeros(max(topk, 1), np.uint32)
@@ -189,7 +192,7 @@ class OracleIndex:
         st = Stats()
         co = np.zeros(max(scc, 1), np.int32)
         fo = np.zeros(self.F, np.int32)
-        rc = lib().po_search_one(self._h, c32(q), topk, max_topk, scc, ftk, radius_rate, D, I,
+        rc = lib().po_search_one(self._h, c32(q), topk, max_topk, scc, ftk, radius_rate, fine_budget, D, I,
                                  ctypes.byref(n), fv, ft, ctypes.byref(fn), ctypes.byref(st), co, fo)
         out = dict(rc=rc, dist=D[:topk], ids=I[:topk], n=n.value, filt_val=fv[:ftk],
                    filt_mem=ft[:ftk], filt_n=fn.value, stats=st.as_dict(),
@@ -207,6 +210,28 @@ class OracleIndex:
         lib().po_search_batch(self._h, nq, Q.reshape(-1), topk, max_topk, scc, ftk, radius_rate,
                               D.reshape(-1), I.reshape(-1), nres, nthreads, ctypes.byref(st))
         return D, I, nres, st.as_dict()
+
+    def search_batch_pq(self, Q, max_topk, scc, ftk, topk=None, scc_arr=None, fine_budget=None,
+                        radius_rate=1.0, nthreads=0):
+        """Per-query topk / search_coarse_count / fine budget (None: scalar);
+        rows of stride max_topk."""
+        Q = c32(Q)
+        nq = Q.shape[0]
+        keep = []
+
+        def arr(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, np.uint32)
+            keep.append(a)
+            return a.ctypes.data_as(ctypes.c_void_p)
+
+        D = np.zeros((nq, max_topk), np.float32)
+        I = np.zeros((nq, max_topk), np.uint32)
+        nres = np.zeros(nq, np.uint32)
+        lib().po_search_batch_pq(self._h, nq, Q.reshape(-1), arr(topk), max_topk, arr(scc_arr), scc,
+                                 arr(fine_budget), ftk, radius_rate, D.reshape(-1), I.reshape(-1), nres, nthreads)
+        return D, I, nres
 
 
 def assign_cells(coarse, fine, X, nthreads=0):

@@ -378,9 +378,12 @@ typedef struct {
 /* One query: PuckIndex::search (puck_index.cpp:559-600) with whether_pq=false.
  * Returns 0, or -1 for topk > max_topk / null feature / zero-norm query.
  * filt_val/filt_tag (ftk each) receive the filter heap as rank_topN_points
- * reads it (heap order when full, ascending when not full). */
+ * reads it (heap order when full, ascending when not full).
+ * fine_budget (0 = F, the reference's walk) limits each coarse list's walk to
+ * the first fine_budget entries of the heap-sorted fine list: an extension
+ * with no counterpart in the reference, for the per-query budget. */
 int po_search_one(const void* p, const float* feature, uint32_t topk, uint32_t max_topk,
-                  uint32_t scc, uint32_t ftk, double radius_rate,
+                  uint32_t scc, uint32_t ftk, double radius_rate, uint32_t fine_budget,
                   float* out_dist, uint32_t* out_ids, uint32_t* out_n,
                   float* filt_val, uint32_t* filt_tag, uint32_t* filt_n,
                   po_stats* st, int* coarse_out, int* fine_out) {
@@ -439,7 +442,8 @@ int po_search_one(const void* p, const float* feature, uint32_t topk, uint32_t m
         float cd = coarse_d[l];
         float min_dist = ix->min_dist_offset[c] + cd;
         float max_st = pivot - cd - fine_d[0];
-        for (uint32_t idx = 0; idx < F; ++idx) {
+        uint32_t Fq = (fine_budget == 0 || fine_budget > F) ? F : fine_budget;
+        for (uint32_t idx = 0; idx < Fq; ++idx) {
             s.cells_considered++;
             if (fine_d[idx] + min_dist >= pivot) break;
             uint32_t k = fine_t[idx];
@@ -508,7 +512,7 @@ int po_search_batch(const void* p, uint32_t nq, const float* queries, uint32_t t
         po_stats s = {0, 0, 0, 0};
         uint32_t n = 0;
         int rc = po_search_one(ix, queries + (size_t)i * ix->dim, topk, max_topk, scc, ftk,
-                               radius_rate, dist + (size_t)i * topk, ids + (size_t)i * topk, &n,
+                               radius_rate, 0, dist + (size_t)i * topk, ids + (size_t)i * topk, &n,
                                NULL, NULL, NULL, &s, NULL, NULL);
         if (nres) nres[i] = rc == 0 ? n : 0;
         a += s.cells_considered; b += s.cells_scanned; c += s.points_offered; d += s.heap_updates;
@@ -516,6 +520,44 @@ int po_search_batch(const void* p, uint32_t nq, const float* queries, uint32_t t
     if (stats_out) {
         stats_out->cells_considered = a; stats_out->cells_scanned = b;
         stats_out->points_offered = c; stats_out->heap_updates = d;
+    }
+    return 0;
+}
+
+/* Batch with per-query topk / search_coarse_count / fine budget (arrays may
+ * be NULL: the scalar applies).  Rows have stride max_topk; slots past a
+ * query's topk, and every slot of a query whose search fails (topk 0 or
+ * > max_topk, zero-norm query), hold distance 0x7f7f7f7f and id 0xffffffff,
+ * with nres 0 for a failed query. */
+int po_search_batch_pq(const void* p, uint32_t nq, const float* queries, const uint32_t* topk_arr,
+                       uint32_t max_topk, const uint32_t* scc_arr, uint32_t scc, const uint32_t* nf_arr,
+                       uint32_t ftk, double radius_rate, float* dist, uint32_t* ids, uint32_t* nres,
+                       int nthreads) {
+    const po_index* ix = (const po_index*)p;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+#endif
+    for (int64_t i = 0; i < (int64_t)nq; ++i) {
+        uint32_t k = topk_arr ? topk_arr[i] : max_topk;
+        uint32_t sc = scc_arr ? (scc_arr[i] < scc ? scc_arr[i] : scc) : scc;
+        uint32_t nf = nf_arr ? nf_arr[i] : 0;
+        float* D = dist + (size_t)i * max_topk;
+        uint32_t* I = ids + (size_t)i * max_topk;
+        uint32_t n = 0;
+        int rc = -1;
+        if (k >= 1 && k <= max_topk)
+            rc = po_search_one(ix, queries + (size_t)i * ix->dim, k, max_topk, sc, ftk, radius_rate, nf, D, I,
+                               &n, NULL, NULL, NULL, NULL, NULL, NULL);
+        uint32_t first = rc == 0 ? k : 0;
+        for (uint32_t r = first; r < max_topk; ++r) {
+            uint32_t sent = 0x7f7f7f7fu;
+            memcpy(D + r, &sent, 4);
+            I[r] = 0xffffffffu;
+        }
+        if (rc == 0)
+            for (uint32_t r = n; r < k; ++r) I[r] = 0xffffffffu;
+        if (nres) nres[i] = rc == 0 ? n : 0;
     }
     return 0;
 }

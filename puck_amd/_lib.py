@@ -14,7 +14,10 @@ EXPORTS = [
     "puckgpu_create", "puckgpu_destroy", "puckgpu_last_error", "puckgpu_filter_topk_max",
     "puckgpu_index_load", "puckgpu_index_free", "puckgpu_index_export", "puckgpu_search",
     "puckgpu_last_filter", "puckgpu_last_stage_ms", "puckgpu_lut", "puckgpu_assign_cells",
-    "puckgpu_encode_filter", "puckgpu_kmeans_assign",
+    "puckgpu_encode_filter", "puckgpu_kmeans_assign", "puckgpu_search_batch",
+    "puckgpu_probe_create", "puckgpu_probe_free", "puckgpu_probe_param_count", "puckgpu_probe_set_params",
+    "puckgpu_probe_get_params", "puckgpu_probe_features", "puckgpu_probe_train_step", "puckgpu_probe_predict",
+    "puckgpu_search_learned",
 ]
 
 
@@ -30,6 +33,25 @@ class SearchParams(ctypes.Structure):
                 ("replay_all", ctypes.c_uint32)]
 
 
+class QueryParams(ctypes.Structure):
+    _fields_ = [("topk", ctypes.c_void_p), ("search_coarse_count", ctypes.c_void_p),
+                ("fine_budget", ctypes.c_void_p)]
+
+
+class ProbeShape(ctypes.Structure):
+    _fields_ = [("coarse_features", ctypes.c_uint32), ("fine_features", ctypes.c_uint32),
+                ("hidden", ctypes.c_uint32)]
+
+
+class AdamParams(ctypes.Structure):
+    _fields_ = [("lr", ctypes.c_float), ("beta1", ctypes.c_float), ("beta2", ctypes.c_float),
+                ("eps", ctypes.c_float)]
+
+
+class BudgetParams(ctypes.Structure):
+    _fields_ = [("margin", ctypes.c_float), ("min_budget", ctypes.c_uint32), ("fallback_frac", ctypes.c_float)]
+
+
 _lib = None
 
 
@@ -41,6 +63,13 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `python puck_amd/build.py` "
                            "(the engine has no CPU fallback)")
+    # torch ships its own HIP/HSA runtime under the same sonames.  Loading it
+    # first makes libpuckgpu.so bind to it; loading /opt/rocm's first would put
+    # two HSA runtimes in the process, and torch's would then find no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, u32, i32, c_int = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_int
     L.puckgpu_create.argtypes = [c_int, ctypes.POINTER(vp)]
@@ -60,6 +89,9 @@ def load():
     L.puckgpu_index_export.restype = c_int
     L.puckgpu_search.argtypes = [vp, vp, ctypes.POINTER(SearchParams), u32, vp, u32, vp, vp, vp, c_int]
     L.puckgpu_search.restype = c_int
+    L.puckgpu_search_batch.argtypes = [vp, vp, ctypes.POINTER(SearchParams), ctypes.POINTER(QueryParams), u32,
+                                       vp, vp, vp, vp, c_int]
+    L.puckgpu_search_batch.restype = c_int
     L.puckgpu_last_filter.argtypes = [vp, u32, u32, vp, vp, vp]
     L.puckgpu_last_filter.restype = c_int
     L.puckgpu_last_stage_ms.argtypes = [vp, vp, vp]
@@ -72,6 +104,26 @@ def load():
     L.puckgpu_encode_filter.restype = c_int
     L.puckgpu_kmeans_assign.argtypes = [vp, u32, vp, u32, u32, vp, vp, vp, c_int]
     L.puckgpu_kmeans_assign.restype = c_int
+    L.puckgpu_probe_create.argtypes = [vp, u32, ctypes.POINTER(ProbeShape), ctypes.POINTER(vp)]
+    L.puckgpu_probe_create.restype = c_int
+    L.puckgpu_probe_free.argtypes = [vp]
+    L.puckgpu_probe_free.restype = None
+    L.puckgpu_probe_param_count.argtypes = [vp]
+    L.puckgpu_probe_param_count.restype = u32
+    L.puckgpu_probe_set_params.argtypes = [vp, vp, vp, c_int]
+    L.puckgpu_probe_set_params.restype = c_int
+    L.puckgpu_probe_get_params.argtypes = [vp, vp, vp, c_int]
+    L.puckgpu_probe_get_params.restype = c_int
+    L.puckgpu_probe_features.argtypes = [vp, vp, vp, u32, vp, vp, vp, c_int]
+    L.puckgpu_probe_features.restype = c_int
+    L.puckgpu_probe_train_step.argtypes = [vp, vp, u32, vp, vp, ctypes.POINTER(AdamParams),
+                                           ctypes.POINTER(ctypes.c_float), vp, c_int]
+    L.puckgpu_probe_train_step.restype = c_int
+    L.puckgpu_probe_predict.argtypes = [vp, vp, u32, vp, vp, c_int]
+    L.puckgpu_probe_predict.restype = c_int
+    L.puckgpu_search_learned.argtypes = [vp, vp, vp, ctypes.POINTER(SearchParams), ctypes.POINTER(BudgetParams),
+                                         ctypes.POINTER(QueryParams), u32, vp, vp, vp, vp, vp, c_int]
+    L.puckgpu_search_learned.restype = c_int
     del i32
     _lib = L
     return L

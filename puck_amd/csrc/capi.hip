@@ -20,13 +20,26 @@ hipError_t pk_launch_gemm_exact(const float*, const float*, float*, uint32_t, ui
 hipError_t pk_launch_row_norms(const float*, uint32_t, uint32_t, int, float*, hipStream_t);
 hipError_t pk_launch_sort_rows(const float*, uint32_t, uint32_t, uint64_t, uint32_t, float*, uint32_t*, uint8_t*,
                                hipStream_t);
-hipError_t pk_launch_exact_center_order(const float*, uint32_t, const float*, uint32_t, uint32_t, const uint32_t*,
-                                        uint32_t, float*, uint32_t*, float*, uint32_t*, hipStream_t);
+hipError_t pk_launch_exact_center_order(const float*, uint32_t, const float*, uint32_t, uint32_t, uint32_t,
+                                        const uint32_t*, const uint32_t*, uint32_t, float*, uint32_t*, float*,
+                                        uint32_t*, hipStream_t);
 hipError_t pk_launch_prep_queries(const float*, uint32_t, uint32_t, int, float*, float*, int32_t*, hipStream_t);
 hipError_t pk_launch_lut(const float*, uint32_t, uint32_t, const float*, uint32_t, uint32_t, float*, hipStream_t);
 hipError_t pk_launch_walk(const PkWalkArgs*, uint32_t, hipStream_t);
 hipError_t pk_launch_rerank(const PkRerankArgs*, uint32_t, hipStream_t);
 uint32_t pk_walk_ftk_max(void);
+uint32_t pk_probe_trained_count(uint32_t, uint32_t);
+uint32_t pk_probe_param_count(uint32_t, uint32_t);
+size_t pk_probe_work_bytes(uint32_t, uint32_t, uint32_t);
+hipError_t pk_launch_probe_raw_features(const float*, uint32_t, const float*, uint32_t, uint32_t, const float*,
+                                        uint32_t, uint32_t, uint32_t, float*, hipStream_t);
+hipError_t pk_probe_pass(const float*, const uint16_t*, uint32_t, uint32_t, uint32_t, const float*, const float*,
+                         void*, float*, float*, float*, hipStream_t);
+hipError_t pk_probe_adam_step(float*, float*, float*, const float*, uint16_t*, uint32_t, uint32_t, float, float,
+                              float, float, uint32_t, hipStream_t);
+hipError_t pk_launch_probe_budget(const float*, uint32_t, float, uint32_t, float, uint32_t, uint32_t*,
+                                  hipStream_t);
+hipError_t pk_launch_to_bf16(const float*, uint32_t, uint16_t*, hipStream_t);
 hipError_t pk_launch_assign_cells(const float*, const float*, const float*, const float*, const float*,
                                   const float*, uint32_t, uint32_t, uint32_t, int, const uint32_t*, uint32_t*,
                                   float*, hipStream_t);
@@ -72,6 +85,9 @@ struct PkWalkArgs {
     int exact;
     uint32_t* tie_ids;
     uint8_t* tie_n;
+    const uint32_t* q_scc;
+    const uint32_t* q_nf;
+    uint32_t cstride;
 };
 struct PkRerankArgs {
     const float* q;
@@ -90,6 +106,8 @@ struct PkRerankArgs {
     int exact;
     const uint32_t* tie_ids;
     const uint8_t* tie_n;
+    const uint32_t* q_topk;
+    uint32_t out_stride;
 };
 
 namespace {
@@ -126,6 +144,18 @@ struct puckgpu_index {
     float* rec_off = nullptr;
     uint8_t* rec_codes = nullptr;
     float* raw = nullptr;
+};
+
+struct puckgpu_probe {
+    int device = 0;
+    uint32_t dim = 0, nc = 0, nfine = 0, D = 0, H = 0;
+    uint32_t count = 0, trained = 0, step = 0;
+    float* params = nullptr;  // packed FP32 (trained block, then mu, 1/sigma)
+    float* m = nullptr;       // Adam moments (trained block)
+    float* v = nullptr;
+    float* grads = nullptr;
+    float* loss = nullptr;
+    uint16_t* wbf = nullptr;  // BF16 copy of the trained block
 };
 
 namespace {
@@ -168,7 +198,8 @@ void* scratch(puckgpu_ctx* ctx, size_t slot, size_t bytes, int* rc) {
 
 enum Slot {
     S_Q = 0, S_QN, S_QST, S_CD, S_FD, S_CSD, S_CSI, S_FSD, S_FSI, S_FT, S_FI, S_FN, S_OD, S_OI, S_ON, S_STATS,
-    S_IN_Q, S_A0, S_A1, S_A2, S_A3, S_A4, S_A5, S_A6, S_A7, S_FLAGS, S_QLIST, S_TIEID, S_TIEN, S_COUNT
+    S_IN_Q, S_A0, S_A1, S_A2, S_A3, S_A4, S_A5, S_A6, S_A7, S_FLAGS, S_QLIST, S_TIEID, S_TIEN, S_QTOPK, S_QSCC,
+    S_QNF, S_PFEAT, S_PWORK, S_PNF, S_PY, S_PT, S_COUNT
 };
 
 // Device view of an input array: the pointer itself, or a scratch copy of it.
@@ -399,12 +430,21 @@ int puckgpu_index_export(puckgpu_ctx* ctx, const puckgpu_index* ix, float* coars
 }
 
 // ---------------------------------------------------------------------------
-int puckgpu_search(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_params* sp, uint32_t n,
-                   const float* queries, uint32_t topk, float* distance, uint32_t* labels, uint32_t* result_num,
-                   int device_ptrs) {
-    if (!ctx || !ix || !sp) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
-    if (topk == 0 || topk > sp->topk_max)
-        return set_err(ctx, PUCKGPU_EINVAL, "topk should <= topk, topk = %u, requested %u", sp->topk_max, topk);
+namespace {
+
+// Batched search.  `topk` is the largest per-query top-k (the rerank
+// capacity) and `ostride` the output row stride; q_topk / q_scc / q_nf are
+// optional per-query arrays (host or device as device_ptrs says).
+struct ProbeUse {
+    const puckgpu_probe* probe = nullptr;
+    puckgpu_budget_params bp{};
+    uint32_t* budget_out = nullptr;
+};
+
+int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_params* sp,
+                const uint32_t* q_topk_in, const uint32_t* q_scc_in, const uint32_t* q_nf_in, uint32_t n,
+                const float* queries, uint32_t topk, uint32_t ostride, float* distance, uint32_t* labels,
+                uint32_t* result_num, int device_ptrs, const ProbeUse* pu = nullptr) {
     if (n == 0) return PUCKGPU_OK;
     if (!queries || !distance || !labels) return set_err(ctx, PUCKGPU_EINVAL, "null array");
     const uint32_t ftk = sp->filter_topk;
@@ -415,6 +455,12 @@ int puckgpu_search(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_sear
     const uint32_t d = ix->p.dim, C = ix->p.coarse_cluster_count, F = ix->p.fine_cluster_count,
                    nsq = ix->p.filter_nsq;
     const uint32_t scc = sp->search_coarse_count < C ? sp->search_coarse_count : C;
+    const puckgpu_probe* pr = pu ? pu->probe : nullptr;
+    if (pr && (pr->dim != d || pr->nc > C || pr->nfine > F))
+        return set_err(ctx, PUCKGPU_EINVAL, "probe shape (dim %u, %u coarse, %u fine features) does not fit the index",
+                       pr->dim, pr->nc, pr->nfine);
+    // sorted coarse entries kept per query: the walk's scc, or the probe's features
+    const uint32_t keep = pr && pr->nc > scc ? pr->nc : scc;
     hipStream_t s = ctx->stream;
     int rc = 0;
     const uint32_t QB = 16384;  // queries per device pass
@@ -424,8 +470,16 @@ int puckgpu_search(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_sear
     int32_t* qst = (int32_t*)scratch(ctx, S_QST, sizeof(int32_t) * nb, &rc);
     float* cdist = (float*)scratch(ctx, S_CD, sizeof(float) * nb * C, &rc);
     float* fdist = (float*)scratch(ctx, S_FD, sizeof(float) * nb * F, &rc);
-    float* csd = (float*)scratch(ctx, S_CSD, sizeof(float) * nb * (scc ? scc : 1), &rc);
-    uint32_t* csi = (uint32_t*)scratch(ctx, S_CSI, sizeof(uint32_t) * nb * (scc ? scc : 1), &rc);
+    float* csd = (float*)scratch(ctx, S_CSD, sizeof(float) * nb * (keep ? keep : 1), &rc);
+    uint32_t* csi = (uint32_t*)scratch(ctx, S_CSI, sizeof(uint32_t) * nb * (keep ? keep : 1), &rc);
+    float* pfeat = nullptr;
+    void* pwork = nullptr;
+    uint32_t* pnf = nullptr;
+    if (pr) {
+        pfeat = (float*)scratch(ctx, S_PFEAT, sizeof(float) * nb * pr->D, &rc);
+        pwork = scratch(ctx, S_PWORK, pk_probe_work_bytes(nb, pr->D, pr->H), &rc);
+        pnf = (uint32_t*)scratch(ctx, S_PNF, sizeof(uint32_t) * nb, &rc);
+    }
     float* fsd = (float*)scratch(ctx, S_FSD, sizeof(float) * nb * F, &rc);
     uint32_t* fsi = (uint32_t*)scratch(ctx, S_FSI, sizeof(uint32_t) * nb * F, &rc);
     float* ft = (float*)scratch(ctx, S_FT, sizeof(float) * (size_t)n * ftk, &rc);
@@ -440,13 +494,17 @@ int puckgpu_search(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_sear
         oi = labels;
         on = result_num ? result_num : (uint32_t*)scratch(ctx, S_ON, sizeof(uint32_t) * n, &rc);
     } else {
-        od = (float*)scratch(ctx, S_OD, sizeof(float) * (size_t)n * topk, &rc);
-        oi = (uint32_t*)scratch(ctx, S_OI, sizeof(uint32_t) * (size_t)n * topk, &rc);
+        od = (float*)scratch(ctx, S_OD, sizeof(float) * (size_t)n * ostride, &rc);
+        oi = (uint32_t*)scratch(ctx, S_OI, sizeof(uint32_t) * (size_t)n * ostride, &rc);
         on = (uint32_t*)scratch(ctx, S_ON, sizeof(uint32_t) * n, &rc);
     }
     if (rc) return rc;
     const float* qin = (const float*)dev_in(ctx, S_IN_Q, queries, sizeof(float) * (size_t)n * d, device_ptrs, &rc);
     if (!qin) return rc;
+    const uint32_t* q_topk = (const uint32_t*)dev_in(ctx, S_QTOPK, q_topk_in, q_topk_in ? 4ull * n : 0, device_ptrs, &rc);
+    const uint32_t* q_scc = (const uint32_t*)dev_in(ctx, S_QSCC, q_scc_in, q_scc_in ? 4ull * n : 0, device_ptrs, &rc);
+    const uint32_t* q_nf = (const uint32_t*)dev_in(ctx, S_QNF, q_nf_in, q_nf_in ? 4ull * n : 0, device_ptrs, &rc);
+    if (rc) return rc;
     uint8_t* flags = (uint8_t*)scratch(ctx, S_FLAGS, nb, &rc);
     uint32_t* qlist = (uint32_t*)scratch(ctx, S_QLIST, sizeof(uint32_t) * nb, &rc);
     uint32_t* tie_ids = (uint32_t*)scratch(ctx, S_TIEID, sizeof(uint32_t) * 16 * nb, &rc);
@@ -465,8 +523,23 @@ int puckgpu_search(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_sear
         PK_CHECK(pk_launch_gemm_exact(q, ix->coarse, cdist, m, C, d, C, PK_EPI_SUB_FROM_BIAS, ix->coarse_norms, s));
         PK_CHECK(pk_launch_gemm_exact(q, ix->fine, fdist, m, F, d, F, PK_EPI_NEG, nullptr, s));
         PK_CHECK(hipEventRecord(ctx->ev[1], s));
-        if (scc) PK_CHECK(pk_launch_sort_rows(cdist, m, C, C, scc, csd, csi, flags, s));
+        if (keep) PK_CHECK(pk_launch_sort_rows(cdist, m, C, C, keep, csd, csi, flags, s));
         PK_CHECK(pk_launch_sort_rows(fdist, m, F, F, F, fsd, fsi, nullptr, s));
+        const uint32_t* nf_chunk = q_nf ? q_nf + q0 : nullptr;
+        if (pr) {
+            // learned fine budget: features -> MLP -> budget, on this stream
+            PK_CHECK(pk_launch_probe_raw_features(q, d, csd, keep, pr->nc, fsd, F, pr->nfine, m, pfeat, s));
+            float* yb = (float*)scratch(ctx, S_PY, sizeof(float) * nb, &rc);
+            if (!yb) return rc;
+            PK_CHECK(pk_probe_pass(pr->params, pr->wbf, pr->D, pr->H, m, pfeat, nullptr, pwork, nullptr, nullptr, yb,
+                                   s));
+            PK_CHECK(pk_launch_probe_budget(yb, m, pu->bp.margin, pu->bp.min_budget, pu->bp.fallback_frac, F, pnf,
+                                            s));
+            if (pu->budget_out)
+                PK_CHECK(hipMemcpyAsync(pu->budget_out + q0, pnf, sizeof(uint32_t) * m,
+                                        device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+            nf_chunk = pnf;
+        }
         PK_CHECK(hipEventRecord(ctx->ev[2], s));
         PkWalkArgs wa;
         wa.q = q; wa.qn = qn; wa.qstatus = qst; wa.csel_d = csd; wa.csel_id = csi; wa.fsort_d = fsd;
@@ -476,13 +549,15 @@ int puckgpu_search(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_sear
         wa.rr = sp->radius_rate; wa.out_t = ft + (size_t)q0 * ftk; wa.out_id = fi + (size_t)q0 * ftk;
         wa.out_n = fn + q0; wa.stats = stats; wa.qlist = nullptr; wa.flags = flags; wa.exact = 0;
         wa.tie_ids = tie_ids; wa.tie_n = tie_n;
+        wa.q_scc = q_scc ? q_scc + q0 : nullptr; wa.q_nf = nf_chunk; wa.cstride = keep ? keep : 1;
         PK_CHECK(pk_launch_walk(&wa, m, s));
         PK_CHECK(hipEventRecord(ctx->ev[3], s));
         PkRerankArgs ra;
         ra.q = q; ra.qstatus = qst; ra.filt_t = wa.out_t; ra.filt_id = wa.out_id; ra.filt_n = wa.out_n;
         ra.raw = ix->raw; ra.mem2local = ix->mem2local; ra.dim = d; ra.ftk = ftk; ra.topk = topk;
-        ra.out_d = od + (size_t)q0 * topk; ra.out_id = oi + (size_t)q0 * topk; ra.out_n = on + q0;
+        ra.out_d = od + (size_t)q0 * ostride; ra.out_id = oi + (size_t)q0 * ostride; ra.out_n = on + q0;
         ra.qlist = nullptr; ra.flags = flags; ra.exact = 0; ra.tie_ids = tie_ids; ra.tie_n = tie_n;
+        ra.q_topk = q_topk ? q_topk + q0 : nullptr; ra.out_stride = ostride;
         PK_CHECK(pk_launch_rerank(&ra, m, s));
         PK_CHECK(hipEventRecord(ctx->ev[4], s));
         PK_CHECK(hipMemcpyAsync(hstats.data(), stats, sizeof(uint32_t) * 4 * m, hipMemcpyDeviceToHost, s));
@@ -504,7 +579,8 @@ int puckgpu_search(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_sear
         if (!hlist.empty()) {
             const uint32_t nl = (uint32_t)hlist.size();
             PK_CHECK(hipMemcpyAsync(qlist, hlist.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s));
-            PK_CHECK(pk_launch_exact_center_order(cdist, C, fdist, F, scc, qlist, ncenter, csd, csi, fsd, fsi, s));
+            PK_CHECK(pk_launch_exact_center_order(cdist, C, fdist, F, wa.cstride, scc, wa.q_scc, qlist, ncenter,
+                                                  csd, csi, fsd, fsi, s));
             wa.qlist = qlist; wa.flags = nullptr; wa.exact = 1; wa.tie_ids = nullptr; wa.tie_n = nullptr;
             PK_CHECK(pk_launch_walk(&wa, nl, s));
             ra.qlist = qlist; ra.flags = nullptr; ra.exact = 1; ra.tie_ids = nullptr; ra.tie_n = nullptr;
@@ -525,13 +601,217 @@ int puckgpu_search(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_sear
                 if (hflags[i] & (1u << b)) ctx->counters[5 + b]++;
     }
     if (!device_ptrs) {
-        PK_CHECK(hipMemcpyAsync(distance, od, sizeof(float) * (size_t)n * topk, hipMemcpyDeviceToHost, s));
-        PK_CHECK(hipMemcpyAsync(labels, oi, sizeof(uint32_t) * (size_t)n * topk, hipMemcpyDeviceToHost, s));
+        PK_CHECK(hipMemcpyAsync(distance, od, sizeof(float) * (size_t)n * ostride, hipMemcpyDeviceToHost, s));
+        PK_CHECK(hipMemcpyAsync(labels, oi, sizeof(uint32_t) * (size_t)n * ostride, hipMemcpyDeviceToHost, s));
         if (result_num) PK_CHECK(hipMemcpyAsync(result_num, on, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
         PK_CHECK(hipStreamSynchronize(s));
     }
     ctx->last_n = n;
     ctx->last_ftk = ftk;
+    return PUCKGPU_OK;
+}
+
+}  // namespace
+
+int puckgpu_search(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_params* sp, uint32_t n,
+                   const float* queries, uint32_t topk, float* distance, uint32_t* labels, uint32_t* result_num,
+                   int device_ptrs) {
+    if (!ctx || !ix || !sp) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    if (topk == 0 || topk > sp->topk_max)
+        return set_err(ctx, PUCKGPU_EINVAL, "topk should <= topk, topk = %u, requested %u", sp->topk_max, topk);
+    return search_impl(ctx, ix, sp, nullptr, nullptr, nullptr, n, queries, topk, topk, distance, labels,
+                       result_num, device_ptrs);
+}
+
+int puckgpu_search_batch(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_params* sp,
+                         const puckgpu_query_params* qp, uint32_t n, const float* queries, float* distance,
+                         uint32_t* labels, uint32_t* result_num, int device_ptrs) {
+    if (!ctx || !ix || !sp) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    if (sp->topk_max == 0) return set_err(ctx, PUCKGPU_EINVAL, "topk_max must be positive");
+    const uint32_t* qt = qp ? qp->topk : nullptr;
+    const uint32_t* qs = qp ? qp->search_coarse_count : nullptr;
+    const uint32_t* qf = qp ? qp->fine_budget : nullptr;
+    return search_impl(ctx, ix, sp, qt, qs, qf, n, queries, sp->topk_max, sp->topk_max, distance, labels,
+                       result_num, device_ptrs);
+}
+
+int puckgpu_search_learned(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_probe* probe,
+                           const puckgpu_search_params* sp, const puckgpu_budget_params* bp,
+                           const puckgpu_query_params* qp, uint32_t n, const float* queries, float* distance,
+                           uint32_t* labels, uint32_t* result_num, uint32_t* budget_out, int device_ptrs) {
+    if (!ctx || !ix || !sp || !probe || !bp) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    if (sp->topk_max == 0) return set_err(ctx, PUCKGPU_EINVAL, "topk_max must be positive");
+    if (!(bp->fallback_frac > 0.f)) return set_err(ctx, PUCKGPU_EINVAL, "fallback_frac must be positive");
+    ProbeUse pu;
+    pu.probe = probe;
+    pu.bp = *bp;
+    pu.budget_out = budget_out;
+    const uint32_t* qt = qp ? qp->topk : nullptr;
+    const uint32_t* qs = qp ? qp->search_coarse_count : nullptr;
+    return search_impl(ctx, ix, sp, qt, qs, nullptr, n, queries, sp->topk_max, sp->topk_max, distance, labels,
+                       result_num, device_ptrs, &pu);
+}
+
+// ---------------------------------------------------------------------------
+// Learned probe
+int puckgpu_probe_create(puckgpu_ctx* ctx, uint32_t dim, const puckgpu_probe_shape* shape, puckgpu_probe** out) {
+    if (!ctx || !shape || !out) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    *out = nullptr;
+    if (dim == 0 || shape->hidden == 0 || shape->hidden > 4096)
+        return set_err(ctx, PUCKGPU_EINVAL, "probe: dim and hidden must be in 1..4096");
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    puckgpu_probe* p = new puckgpu_probe();
+    p->device = ctx->device;
+    p->dim = dim;
+    p->nc = shape->coarse_features;
+    p->nfine = shape->fine_features;
+    p->D = dim + p->nc + p->nfine;
+    p->H = shape->hidden;
+    p->count = pk_probe_param_count(p->D, p->H);
+    p->trained = pk_probe_trained_count(p->D, p->H);
+    int rc = 0;
+    if ((rc = dalloc(ctx, &p->params, p->count)) || (rc = dalloc(ctx, &p->m, p->trained)) ||
+        (rc = dalloc(ctx, &p->v, p->trained)) || (rc = dalloc(ctx, &p->grads, p->trained)) ||
+        (rc = dalloc(ctx, &p->loss, 4)) || (rc = dalloc(ctx, &p->wbf, p->trained))) {
+        puckgpu_probe_free(p);
+        return rc;
+    }
+    PK_CHECK(hipMemset(p->params, 0, sizeof(float) * p->count));
+    PK_CHECK(hipMemset(p->m, 0, sizeof(float) * p->trained));
+    PK_CHECK(hipMemset(p->v, 0, sizeof(float) * p->trained));
+    PK_CHECK(hipMemset(p->wbf, 0, sizeof(uint16_t) * p->trained));
+    *out = p;
+    return PUCKGPU_OK;
+}
+
+void puckgpu_probe_free(puckgpu_probe* p) {
+    if (!p) return;
+    (void)hipSetDevice(p->device);
+    void* ptrs[] = {p->params, p->m, p->v, p->grads, p->loss, p->wbf};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    delete p;
+}
+
+uint32_t puckgpu_probe_param_count(const puckgpu_probe* p) { return p ? p->count : 0u; }
+
+int puckgpu_probe_set_params(puckgpu_ctx* ctx, puckgpu_probe* p, const float* params, int device_ptrs) {
+    if (!ctx || !p || !params) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    hipStream_t s = ctx->stream;
+    PK_CHECK(hipMemcpyAsync(p->params, params, sizeof(float) * p->count,
+                            device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    PK_CHECK(hipMemsetAsync(p->m, 0, sizeof(float) * p->trained, s));
+    PK_CHECK(hipMemsetAsync(p->v, 0, sizeof(float) * p->trained, s));
+    PK_CHECK(pk_launch_to_bf16(p->params, p->trained, p->wbf, s));
+    PK_CHECK(hipStreamSynchronize(s));
+    p->step = 0;
+    return PUCKGPU_OK;
+}
+
+int puckgpu_probe_get_params(puckgpu_ctx* ctx, const puckgpu_probe* p, float* params, int device_ptrs) {
+    if (!ctx || !p || !params) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    PK_CHECK(hipMemcpyAsync(params, p->params, sizeof(float) * p->count,
+                            device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx->stream));
+    PK_CHECK(hipStreamSynchronize(ctx->stream));
+    return PUCKGPU_OK;
+}
+
+int puckgpu_probe_features(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_probe* p, uint32_t n,
+                           const float* queries, float* features, uint32_t* fine_order, int device_ptrs) {
+    if (!ctx || !ix || !p || !features || (n && !queries)) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    const uint32_t d = ix->p.dim, C = ix->p.coarse_cluster_count, F = ix->p.fine_cluster_count;
+    if (p->dim != d || p->nc > C || p->nfine > F)
+        return set_err(ctx, PUCKGPU_EINVAL, "probe shape does not fit the index");
+    if (n == 0) return PUCKGPU_OK;
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    hipStream_t s = ctx->stream;
+    int rc = 0;
+    const uint32_t QB = 16384;
+    const uint32_t nb = n < QB ? n : QB;
+    const uint32_t keep = p->nc ? p->nc : 1;
+    float* q = (float*)scratch(ctx, S_Q, sizeof(float) * nb * d, &rc);
+    float* qn = (float*)scratch(ctx, S_QN, sizeof(float) * nb, &rc);
+    int32_t* qst = (int32_t*)scratch(ctx, S_QST, sizeof(int32_t) * nb, &rc);
+    float* cdist = (float*)scratch(ctx, S_CD, sizeof(float) * nb * C, &rc);
+    float* fdist = (float*)scratch(ctx, S_FD, sizeof(float) * nb * F, &rc);
+    float* csd = (float*)scratch(ctx, S_CSD, sizeof(float) * nb * keep, &rc);
+    uint32_t* csi = (uint32_t*)scratch(ctx, S_CSI, sizeof(uint32_t) * nb * keep, &rc);
+    float* fsd = (float*)scratch(ctx, S_FSD, sizeof(float) * nb * F, &rc);
+    uint32_t* fsi = (uint32_t*)scratch(ctx, S_FSI, sizeof(uint32_t) * nb * F, &rc);
+    float* feat = device_ptrs ? features : (float*)scratch(ctx, S_PFEAT, sizeof(float) * nb * p->D, &rc);
+    if (rc) return rc;
+    const float* qin = (const float*)dev_in(ctx, S_IN_Q, queries, sizeof(float) * (size_t)n * d, device_ptrs, &rc);
+    if (!qin) return rc;
+    for (uint32_t q0 = 0; q0 < n; q0 += QB) {
+        const uint32_t m = (n - q0) < QB ? (n - q0) : QB;
+        PK_CHECK(pk_launch_prep_queries(qin + (size_t)q0 * d, m, d, ix->p.whether_norm, q, qn, qst, s));
+        PK_CHECK(pk_launch_gemm_exact(q, ix->coarse, cdist, m, C, d, C, PK_EPI_SUB_FROM_BIAS, ix->coarse_norms, s));
+        PK_CHECK(pk_launch_gemm_exact(q, ix->fine, fdist, m, F, d, F, PK_EPI_NEG, nullptr, s));
+        if (p->nc) PK_CHECK(pk_launch_sort_rows(cdist, m, C, C, p->nc, csd, csi, nullptr, s));
+        PK_CHECK(pk_launch_sort_rows(fdist, m, F, F, F, fsd, fsi, nullptr, s));
+        float* fo = device_ptrs ? feat + (size_t)q0 * p->D : feat;
+        PK_CHECK(pk_launch_probe_raw_features(q, d, csd, keep, p->nc, fsd, F, p->nfine, m, fo, s));
+        if (!device_ptrs)
+            PK_CHECK(hipMemcpyAsync(features + (size_t)q0 * p->D, fo, sizeof(float) * m * p->D,
+                                    hipMemcpyDeviceToHost, s));
+        if (fine_order)
+            PK_CHECK(hipMemcpyAsync(fine_order + (size_t)q0 * F, fsi, sizeof(uint32_t) * m * F,
+                                    device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+        PK_CHECK(hipStreamSynchronize(s));
+    }
+    return PUCKGPU_OK;
+}
+
+int puckgpu_probe_train_step(puckgpu_ctx* ctx, puckgpu_probe* p, uint32_t n, const float* features,
+                             const float* target, const puckgpu_adam_params* adam, float* loss, float* grads,
+                             int device_ptrs) {
+    if (!ctx || !p || (n && (!features || !target))) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    if (n == 0) return set_err(ctx, PUCKGPU_EINVAL, "empty batch");
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    hipStream_t s = ctx->stream;
+    int rc = 0;
+    const float* f = (const float*)dev_in(ctx, S_A0, features, sizeof(float) * (size_t)n * p->D, device_ptrs, &rc);
+    const float* t = (const float*)dev_in(ctx, S_PT, target, sizeof(float) * n, device_ptrs, &rc);
+    void* work = scratch(ctx, S_PWORK, pk_probe_work_bytes(n, p->D, p->H), &rc);
+    if (rc) return rc;
+    PK_CHECK(pk_probe_pass(p->params, p->wbf, p->D, p->H, n, f, t, work, p->grads, p->loss, nullptr, s));
+    if (grads)
+        PK_CHECK(hipMemcpyAsync(grads, p->grads, sizeof(float) * p->trained,
+                                device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    if (adam) {
+        p->step++;
+        PK_CHECK(pk_probe_adam_step(p->params, p->m, p->v, p->grads, p->wbf, p->D, p->H, adam->lr, adam->beta1,
+                                    adam->beta2, adam->eps, p->step, s));
+    }
+    float ls = 0.f;
+    if (loss) PK_CHECK(hipMemcpyAsync(&ls, p->loss, sizeof(float), hipMemcpyDeviceToHost, s));
+    PK_CHECK(hipStreamSynchronize(s));
+    if (loss) *loss = ls / (float)n;
+    return PUCKGPU_OK;
+}
+
+int puckgpu_probe_predict(puckgpu_ctx* ctx, const puckgpu_probe* p, uint32_t n, const float* features, float* y,
+                          int device_ptrs) {
+    if (!ctx || !p || (n && (!features || !y))) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    if (n == 0) return PUCKGPU_OK;
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    hipStream_t s = ctx->stream;
+    int rc = 0;
+    const uint32_t QB = 65536;
+    const uint32_t nb = n < QB ? n : QB;
+    const float* f = (const float*)dev_in(ctx, S_A0, features, sizeof(float) * (size_t)n * p->D, device_ptrs, &rc);
+    void* work = scratch(ctx, S_PWORK, pk_probe_work_bytes(nb, p->D, p->H), &rc);
+    float* yb = device_ptrs ? y : (float*)scratch(ctx, S_PY, sizeof(float) * n, &rc);
+    if (rc) return rc;
+    for (uint32_t q0 = 0; q0 < n; q0 += QB) {
+        const uint32_t m = (n - q0) < QB ? (n - q0) : QB;
+        PK_CHECK(pk_probe_pass(p->params, p->wbf, p->D, p->H, m, f + (size_t)q0 * p->D, nullptr, work, nullptr,
+                               nullptr, yb + q0, s));
+    }
+    if (!device_ptrs) PK_CHECK(hipMemcpyAsync(y, yb, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    PK_CHECK(hipStreamSynchronize(s));
     return PUCKGPU_OK;
 }
 

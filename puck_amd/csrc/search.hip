@@ -33,7 +33,7 @@
 #define WALK_THREADS 256
 #define WALK_CH 256      // cells per chunk (one per thread)
 #define WALK_PMAX 512    // points buffered per chunk
-#define WALK_FTK_MAX 1024
+#define WALK_FTK_MAX 2048   // filter_topk limit (instances: <= 1024, <= 2048)
 
 struct PkWalkArgs {
     const float* q;          // nq x dim (normalised when whether_norm)
@@ -60,6 +60,9 @@ struct PkWalkArgs {
     int exact;               // replay the reference MaxHeap (insertion order and ties)
     uint32_t* tie_ids;       // nq x PK_TIE_MAX: memory ids dropped at a tie with the final top
     uint8_t* tie_n;          // nq: count, PK_TIE_OVERFLOW when the list overflowed
+    const uint32_t* q_scc;   // nq or null: per-query coarse lists walked (clamped to scc)
+    const uint32_t* q_nf;    // nq or null: per-query fine budget (0 = all F)
+    uint32_t cstride;        // csel row stride (>= scc)
 };
 
 #define PK_TIE_MAX 16
@@ -331,6 +334,7 @@ __device__ uint32_t pk_walk_select(PkWalkSmem& s, uint32_t nW, uint32_t clo, uin
 // entries, capacity ftk).  Marks them consumed.  Returns the new size.
 // When dry != 0 nothing is written; instead the key of rank ftk-1 of the
 // union is stored in red[8..9] (or ~0 when the union has fewer than ftk).
+template <int FTKM>
 __device__ uint32_t pk_walk_merge(const PkWalkArgs& a, PkWalkSmem& s, uint32_t ns, uint32_t m,
                                   uint32_t seqbase, uint32_t p_lo, int dry, uint32_t plen = 0) {
     const uint32_t ftk = a.ftk;
@@ -350,13 +354,13 @@ __device__ uint32_t pk_walk_merge(const PkWalkArgs& a, PkWalkSmem& s, uint32_t n
         }
         return ns;
     }
-    uint64_t sk[WALK_FTK_MAX / WALK_THREADS];
-    uint32_t sid[WALK_FTK_MAX / WALK_THREADS], spos[WALK_FTK_MAX / WALK_THREADS];
+    uint64_t sk[FTKM / WALK_THREADS];
+    uint32_t sid[FTKM / WALK_THREADS], spos[FTKM / WALK_THREADS];
     uint64_t xk[WALK_PMAX / WALK_THREADS];
     uint32_t xid[WALK_PMAX / WALK_THREADS], xpos[WALK_PMAX / WALK_THREADS];
     // ranks of S entries among X (X keys via W_key[X_pos[.]], sorted since W is)
 #pragma unroll
-    for (int e = 0; e < WALK_FTK_MAX / WALK_THREADS; ++e) {
+    for (int e = 0; e < FTKM / WALK_THREADS; ++e) {
         uint32_t i = threadIdx.x + e * WALK_THREADS;
         spos[e] = 0xffffffffu;
         if (i < ns) {
@@ -388,7 +392,7 @@ __device__ uint32_t pk_walk_merge(const PkWalkArgs& a, PkWalkSmem& s, uint32_t n
     const uint32_t target = ftk - 1;
     if (dry) {
 #pragma unroll
-        for (int e = 0; e < WALK_FTK_MAX / WALK_THREADS; ++e)
+        for (int e = 0; e < FTKM / WALK_THREADS; ++e)
             if (spos[e] == target) { s.red[8] = (uint32_t)(sk[e] >> 32); s.red[9] = (uint32_t)sk[e]; }
 #pragma unroll
         for (int e = 0; e < WALK_PMAX / WALK_THREADS; ++e)
@@ -400,7 +404,7 @@ __device__ uint32_t pk_walk_merge(const PkWalkArgs& a, PkWalkSmem& s, uint32_t n
     if (threadIdx.x == 0) { s.red[10] = 0u; s.red[11] = 0u; }
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < WALK_FTK_MAX / WALK_THREADS; ++e) {
+    for (int e = 0; e < FTKM / WALK_THREADS; ++e) {
         if (spos[e] < ftk) { s.S_key[spos[e]] = sk[e]; s.S_id[spos[e]] = sid[e]; }
         if (spos[e] == ftk) s.red[10] = (uint32_t)(sk[e] >> 32);
         if (spos[e] == ftk - 1) s.red[11] = (uint32_t)(sk[e] >> 32);
@@ -432,7 +436,7 @@ __device__ uint32_t pk_walk_merge(const PkWalkArgs& a, PkWalkSmem& s, uint32_t n
         }
         __syncthreads();
 #pragma unroll
-        for (int e = 0; e < WALK_FTK_MAX / WALK_THREADS; ++e)
+        for (int e = 0; e < FTKM / WALK_THREADS; ++e)
             if (spos[e] != 0xffffffffu && spos[e] >= ftk && (uint32_t)(sk[e] >> 32) == K) {
                 uint32_t slot = atomicAdd(&s.red[14], 1u);
                 if (slot < PK_TIE_MAX) { s.td_val[slot] = K; s.td_id[slot] = sid[e]; }
@@ -477,6 +481,7 @@ __device__ __forceinline__ float pk_walk_top(const PkWalkSmem& s, uint32_t ns, u
     return ns >= ftk ? pk_ord2f((uint32_t)(s.S_key[ftk - 1] >> 32)) : pk_sentinel();
 }
 
+template <int FTKM>
 __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
@@ -535,11 +540,18 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
     float T = pk_sentinel();
     float P = pk_pivot(T, qn, rr);
     uint32_t seqbase = 0;
-    const uint32_t scc = a.scc;
+    const uint32_t scs = a.cstride;
+    uint32_t scc = a.scc;
+    if (a.q_scc) scc = a.q_scc[qi] < scc ? a.q_scc[qi] : scc;
+    // fine budget: only the first Fq entries of the sorted fine list are
+    // considered in each coarse list (Fq = F is the reference's walk)
+    uint32_t Fq = F;
+    if (a.q_nf && a.q_nf[qi] != 0u && a.q_nf[qi] < F) Fq = a.q_nf[qi];
+    if (Fq < F && fsd[Fq - 1] == fsd[Fq]) tie_dec = true;  // the cut splits a tie group
 
     for (uint32_t l = 0; l < scc; ++l) {
-        const uint32_t c = a.csel_id[(uint64_t)qi * scc + l];
-        const float cd = a.csel_d[(uint64_t)qi * scc + l];
+        const uint32_t c = a.csel_id[(uint64_t)qi * scs + l];
+        const float cd = a.csel_d[(uint64_t)qi * scs + l];
         const float mind = a.mdo[c] + cd;
         const float fd0 = fsd[0];
         float M = (P - cd) - fd0;
@@ -547,9 +559,9 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
         const uint32_t* csc = a.cell_start + (uint64_t)c * F;
         uint32_t j0 = 0;
         bool done = false;
-        while (!done && j0 < F) {
+        while (!done && j0 < Fq) {
             const uint32_t j = j0 + tid;
-            const bool valid = j < F;
+            const bool valid = j < Fq;
             float fdj = 0.f, stj = 0.f;
             uint32_t bj = 0, nj = 0;
             if (valid) {
@@ -573,7 +585,7 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
             uint32_t cend = cutm < bhi ? cutm : bhi;
             // do not split a fine-distance tie group across chunks: end the
             // chunk at the group's first cell when the group starts inside it
-            if (j0 + cend < F && cend > 1 && fsd[j0 + cend] == fsd[j0 + cend - 1]) {
+            if (j0 + cend < Fq && cend > 1 && fsd[j0 + cend] == fsd[j0 + cend - 1]) {
                 uint32_t g = cend - 1;
                 while (g > 0 && fsd[j0 + g - 1] == fsd[j0 + g]) --g;
                 if (g > 0) cend = g;
@@ -598,7 +610,7 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
             if (!big) {
                 nW = pk_walk_values(a, s, cend, 0, ptot, cd, Thi, seqbase);
                 uint32_t m = pk_walk_select(s, nW, 0, WALK_CH, 0xffu, seqbase, 0);
-                pk_walk_merge(a, s, ns, m, seqbase, 0, 1);
+                pk_walk_merge<FTKM>(a, s, ns, m, seqbase, 0, 1);
                 uint32_t hi = s.red[8], lo = s.red[9];
                 __syncthreads();
                 float Tlow = (hi == 0xffffffffu && lo == 0xffffffffu) ? pk_sentinel() : pk_ord2f(hi);
@@ -653,7 +665,7 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
                 const uint32_t last1 = pk_bmax((inr && cls == 1 && (uint32_t)tid < u) ? (uint32_t)tid + 1 : 0u, s.red);
                 if (nW > 0) {
                     uint32_t m = pk_walk_select(s, nW, t0, u, 1u, seqbase, 0);
-                    ns = pk_walk_merge(a, s, ns, m, seqbase, 0, 0, ptot);
+                    ns = pk_walk_merge<FTKM>(a, s, ns, m, seqbase, 0, 0, ptot);
                 }
                 T = pk_walk_top(s, ns, ftk);
                 P = pk_pivot(T, qn, rr);
@@ -671,7 +683,7 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
                     const bool back_tie = u > 0 ? s.c_fd[u - 1] == fu : (j0 > 0 && fdprev0 == fu);
                     uint32_t gend = u;
                     while (gend + 1 < cend && s.c_fd[gend + 1] == fu) ++gend;
-                    const bool fwd_tie = gend + 1 == cend && j0 + cend < F && fsd[j0 + cend] == fu;
+                    const bool fwd_tie = gend + 1 == cend && j0 + cend < Fq && fsd[j0 + cend] == fu;
                     const uint32_t gs = gend - u + 1;
                     if (back_tie) tie_sub |= 0x10u;
                     if (fwd_tie) tie_sub |= 0x20u;
@@ -691,7 +703,7 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
                                 res |= (cell == u) ? 1u : 2u;
                                 if (k == 0) {
                                     uint32_t mm = pk_walk_select(s, nW, cell, cell + 1, cell == u ? 3u : 4u, seqbase, 0);
-                                    pk_walk_merge(a, s, ns, mm, seqbase, 0, 1);
+                                    pk_walk_merge<FTKM>(a, s, ns, mm, seqbase, 0, 1);
                                     const uint32_t hi = s.red[8], lo = s.red[9];
                                     __syncthreads();
                                     const float T1 = (hi == 0xffffffffu && lo == 0xffffffffu) ? T : pk_ord2f(hi);
@@ -712,7 +724,7 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
                                 __syncthreads();
                                 if (nW > 0) {
                                     uint32_t mm = pk_walk_select(s, nW, u, u + 2, 5u, seqbase, 0);
-                                    ns = pk_walk_merge(a, s, ns, mm, seqbase, 0, 0, ptot);
+                                    ns = pk_walk_merge<FTKM>(a, s, ns, mm, seqbase, 0, 0, ptot);
                                 }
                                 T = pk_walk_top(s, ns, ftk);
                                 P = pk_pivot(T, qn, rr);
@@ -736,14 +748,14 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
                             uint32_t p_hi = p_lo + WALK_PMAX < ptot ? p_lo + WALK_PMAX : ptot;
                             uint32_t nb = pk_walk_values(a, s, cend, p_lo, p_hi, cd, T, seqbase);
                             uint32_t m = pk_walk_select(s, nb, 0, WALK_CH, 0xffu, seqbase, p_lo);
-                            ns = pk_walk_merge(a, s, ns, m, seqbase, p_lo, 0, p_hi - p_lo);
+                            ns = pk_walk_merge<FTKM>(a, s, ns, m, seqbase, p_lo, 0, p_hi - p_lo);
                             T = pk_walk_top(s, ns, ftk);
                         }
                     } else if (nW > 0) {
                         if (tid == 0) s.c_cls[u] = 3;
                         __syncthreads();
                         uint32_t m = pk_walk_select(s, nW, u, u + 1, 3u, seqbase, 0);
-                        ns = pk_walk_merge(a, s, ns, m, seqbase, 0, 0, ptot);
+                        ns = pk_walk_merge<FTKM>(a, s, ns, m, seqbase, 0, 0, ptot);
                     }
                     T = pk_walk_top(s, ns, ftk);
                     P = pk_pivot(T, qn, rr);
@@ -821,15 +833,21 @@ extern "C" uint32_t pk_walk_ftk_max(void) { return WALK_FTK_MAX; }
 
 extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStream_t s) {
     if (nq == 0) return hipSuccess;
-    if (args->ftk == 0 || args->ftk > WALK_FTK_MAX) return hipErrorInvalidValue;
+    if (args->ftk == 0 || args->ftk > WALK_FTK_MAX || args->cstride < args->scc) return hipErrorInvalidValue;
     size_t smem = pk_walk_smem_bytes(args->dim, args->nsq, args->ftk, args->exact);
     if (smem > 160 * 1024) return hipErrorInvalidValue;
     static bool attr_set = false;
     if (!attr_set) {
-        (void)hipFuncSetAttribute((const void*)pk_walk, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void*)pk_walk<1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        (void)hipFuncSetAttribute((const void*)pk_walk<2048>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
         attr_set = true;
     }
-    hipLaunchKernelGGL(pk_walk, dim3(nq), dim3(WALK_THREADS), smem, s, *args);
+    if (args->ftk <= 1024)
+        hipLaunchKernelGGL(pk_walk<1024>, dim3(nq), dim3(WALK_THREADS), smem, s, *args);
+    else
+        hipLaunchKernelGGL(pk_walk<2048>, dim3(nq), dim3(WALK_THREADS), smem, s, *args);
     return hipGetLastError();
 }
 
@@ -852,6 +870,8 @@ struct PkRerankArgs {
     int exact;               // replay the reference result heap over the filter heap order
     const uint32_t* tie_ids; // from the walk (may be null)
     const uint8_t* tie_n;
+    const uint32_t* q_topk;  // nq or null: per-query top-k (1..topk; other values give no result)
+    uint32_t out_stride;     // row stride of out_d / out_id (>= topk)
 };
 
 #define RERANK_THREADS 256
@@ -861,8 +881,10 @@ struct PkRerankArgs {
 __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
-    const uint32_t n = a.qstatus[qi] == 0 ? a.filt_n[qi] : 0u;
-    const uint32_t ftk = a.ftk, topk = a.topk;
+    uint32_t topk = a.topk;
+    if (a.q_topk) topk = (a.q_topk[qi] >= 1u && a.q_topk[qi] <= a.topk) ? a.q_topk[qi] : 0u;
+    const uint32_t n = (a.qstatus[qi] == 0 && topk > 0) ? a.filt_n[qi] : 0u;
+    const uint32_t ftk = a.ftk, ostr = a.out_stride;
     uint32_t n2 = 2;
     while (n2 < ftk) n2 <<= 1;
     uint64_t* keys = (uint64_t*)smem;
@@ -915,9 +937,10 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
             a.out_n[qi] = topk - dc;
         }
         __syncthreads();
-        for (uint32_t r = threadIdx.x; r < topk; r += blockDim.x) {
-            a.out_d[(uint64_t)qi * topk + r] = pk_hv(RH[r + 1]);
-            a.out_id[(uint64_t)qi * topk + r] = RH[r + 1].y;
+        for (uint32_t r = threadIdx.x; r < ostr; r += blockDim.x) {
+            const bool in = r < topk;
+            a.out_d[(uint64_t)qi * ostr + r] = in ? pk_hv(RH[r + 1]) : pk_sentinel();
+            a.out_id[(uint64_t)qi * ostr + r] = in ? RH[r + 1].y : 0xffffffffu;
         }
         return;
     }
@@ -941,7 +964,7 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
     }
     const uint32_t kout = n < topk ? n : topk;
     bool tie = false;
-    for (uint32_t r = threadIdx.x; r < topk; r += blockDim.x) {
+    for (uint32_t r = threadIdx.x; r < ostr; r += blockDim.x) {
         float d = pk_sentinel();
         uint32_t id = 0xffffffffu;
         if (r < kout) {
@@ -951,13 +974,13 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
             id = a.mem2local[a.filt_id[(uint64_t)qi * ftk + rr]];
             if (r + 1 < n && pk_ord2f((uint32_t)(keys[r + 1] >> 32)) == d) tie = true;
         }
-        a.out_d[(uint64_t)qi * topk + r] = d;
-        a.out_id[(uint64_t)qi * topk + r] = id;
+        a.out_d[(uint64_t)qi * ostr + r] = d;
+        a.out_id[(uint64_t)qi * ostr + r] = id;
     }
     if (tie && a.flags) a.flags[qi] = 4u | a.flags[qi];
     // a tie-group member at or inside the k-th distance: which members the
     // reference kept decides the output
-    if (threadIdx.x == 0 && tn > 0 && a.flags) {
+    if (threadIdx.x == 0 && tn > 0 && a.flags && topk > 0) {
         const uint32_t kth = kout == topk && kout > 0 ? (uint32_t)(keys[kout - 1] >> 32) : 0xffffffffu;
         if (tie_min <= kth) a.flags[qi] = 2u | a.flags[qi];
     }
@@ -966,6 +989,7 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
 
 extern "C" hipError_t pk_launch_rerank(const PkRerankArgs* args, uint32_t nq, hipStream_t s) {
     if (nq == 0) return hipSuccess;
+    if (args->out_stride < args->topk) return hipErrorInvalidValue;
     uint32_t n2 = 2;
     while (n2 < args->ftk) n2 <<= 1;
     size_t smem = sizeof(uint64_t) * n2 + sizeof(float) * ((args->dim + 3) & ~3u) +
@@ -981,11 +1005,16 @@ extern "C" hipError_t pk_launch_rerank(const PkRerankArgs* args, uint32_t nq, hi
 // (puck_index.cpp:497-504), replayed by one thread on LDS copies of the rows.
 __global__ __launch_bounds__(256) void pk_exact_center_order(const float* __restrict__ cdist, uint32_t C,
                                                              const float* __restrict__ fdist, uint32_t F,
-                                                             uint32_t scc, const uint32_t* __restrict__ qlist,
+                                                             uint32_t scs, uint32_t scap,
+                                                             const uint32_t* __restrict__ q_scc,
+                                                             const uint32_t* __restrict__ qlist,
                                                              float* __restrict__ csel_d, uint32_t* __restrict__ csel_id,
                                                              float* __restrict__ fsort_d, uint32_t* __restrict__ fsort_id) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t qi = qlist[blockIdx.x];
+    // heap capacity = this query's search_coarse_count (its tie order depends
+    // on it), at most scap; rows keep the stride scs
+    const uint32_t scc = q_scc ? (q_scc[qi] < scap ? q_scc[qi] : scap) : scap;
     const uint32_t L = C > F ? C : F;
     uint2* H = (uint2*)smem;                           // L + 2 entries
     float* row = (float*)(H + ((L + 2) & ~1u));        // max(C, F)
@@ -1001,8 +1030,8 @@ __global__ __launch_bounds__(256) void pk_exact_center_order(const float* __rest
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < scc; i += blockDim.x) {
-        csel_d[(uint64_t)qi * scc + i] = pk_hv(H[i + 1]);
-        csel_id[(uint64_t)qi * scc + i] = H[i + 1].y;
+        csel_d[(uint64_t)qi * scs + i] = pk_hv(H[i + 1]);
+        csel_id[(uint64_t)qi * scs + i] = H[i + 1].y;
     }
     __syncthreads();
     // fine full heap sort (puck_index.cpp:497-504)
@@ -1022,9 +1051,11 @@ __global__ __launch_bounds__(256) void pk_exact_center_order(const float* __rest
 }
 
 extern "C" hipError_t pk_launch_exact_center_order(const float* cdist, uint32_t C, const float* fdist, uint32_t F,
-                                                   uint32_t scc, const uint32_t* qlist, uint32_t nlist,
-                                                   float* csel_d, uint32_t* csel_id, float* fsort_d,
-                                                   uint32_t* fsort_id, hipStream_t s) {
+                                                   uint32_t scs, uint32_t scap, const uint32_t* q_scc,
+                                                   const uint32_t* qlist, uint32_t nlist, float* csel_d,
+                                                   uint32_t* csel_id, float* fsort_d, uint32_t* fsort_id,
+                                                   hipStream_t s) {
+    if (scap > scs) return hipErrorInvalidValue;
     if (nlist == 0) return hipSuccess;
     const uint32_t L = C > F ? C : F;
     size_t smem = sizeof(uint2) * ((L + 2) & ~1u) + sizeof(float) * L;
@@ -1035,7 +1066,7 @@ extern "C" hipError_t pk_launch_exact_center_order(const float* cdist, uint32_t 
                                   160 * 1024);
         attr = true;
     }
-    hipLaunchKernelGGL(pk_exact_center_order, dim3(nlist), dim3(256), smem, s, cdist, C, fdist, F, scc, qlist,
-                       csel_d, csel_id, fsort_d, fsort_id);
+    hipLaunchKernelGGL(pk_exact_center_order, dim3(nlist), dim3(256), smem, s, cdist, C, fdist, F, scs, scap,
+                       q_scc, qlist, csel_d, csel_id, fsort_d, fsort_id);
     return hipGetLastError();
 }

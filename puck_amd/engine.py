@@ -196,6 +196,44 @@ class GpuIndex:
                                                  _ptr(D), _ptr(I), _ptr(R), m))
         return D, I, R
 
+    def search_batch(self, queries, topk_max, search_coarse_count, filter_topk, topk=None, scc=None,
+                     fine_budget=None, radius_rate=1.0, out=None, replay_all=False):
+        """Per-query topk / search_coarse_count / fine_budget arrays (uint32,
+        n entries, or None for the batch-wide value).  search_coarse_count is
+        the batch maximum.  Returns (distance [n, topk_max], labels, result_num)."""
+        n = queries.shape[0]
+        sp = _lib.SearchParams(topk_max, search_coarse_count, filter_topk, radius_rate, int(bool(replay_all)))
+        dev = _is_dev(queries)
+
+        def arr(a):
+            if a is None:
+                return None
+            if dev:
+                import torch
+                if not _is_dev(a):
+                    a = torch.as_tensor(np.asarray(a, np.uint32).view(np.int32), device=queries.device)
+                return a.to(torch.int32).contiguous()
+            return np.ascontiguousarray(a, np.uint32)
+
+        qt, qs, qf = arr(topk), arr(scc), arr(fine_budget)
+        if out is None:
+            if dev:
+                import torch
+                dv = queries.device
+                out = (torch.empty((n, topk_max), dtype=torch.float32, device=dv),
+                       torch.empty((n, topk_max), dtype=torch.int32, device=dv),
+                       torch.empty(n, dtype=torch.int32, device=dv))
+            else:
+                out = (np.zeros((n, topk_max), np.float32), np.zeros((n, topk_max), np.uint32),
+                       np.zeros(n, np.uint32))
+        D, I, R = out
+        m = _mode(queries, D, I, R, qt, qs, qf)
+        qp = _lib.QueryParams(_ptr(qt), _ptr(qs), _ptr(qf))
+        self.ctx.sync_torch(queries, qt, qs, qf)
+        self.ctx.check(self.ctx.L.puckgpu_search_batch(self.ctx.h, self.h, ctypes.byref(sp), ctypes.byref(qp), n,
+                                                       _ptr(queries), _ptr(D), _ptr(I), _ptr(R), m))
+        return D, I, R
+
     def last_filter(self, n, filter_topk):
         t = np.zeros((n, filter_topk), np.float32)
         i = np.zeros((n, filter_topk), np.uint32)
@@ -208,3 +246,130 @@ class GpuIndex:
         cnt = np.zeros(13, np.uint64)
         self.ctx.check(self.ctx.L.puckgpu_last_stage_ms(self.ctx.h, _ptr(ms), _ptr(cnt)))
         return ms, cnt
+
+
+class Probe:
+    """The learned fine-budget predictor resident on the device (BF16 MLP,
+    FP32 master weights and Adam state); see include/puck_gpu.h."""
+
+    def __init__(self, ctx, dim, coarse_features=64, fine_features=64, hidden=256):
+        self.ctx = ctx
+        self.dim, self.nc, self.nf, self.hidden = dim, coarse_features, fine_features, hidden
+        self.D = dim + coarse_features + fine_features
+        h = ctypes.c_void_p()
+        shape = _lib.ProbeShape(coarse_features, fine_features, hidden)
+        ctx.check(ctx.L.puckgpu_probe_create(ctx.h, dim, ctypes.byref(shape), ctypes.byref(h)))
+        self.h = h
+        self.count = ctx.L.puckgpu_probe_param_count(h)
+
+    def free(self):
+        if getattr(self, "h", None):
+            self.ctx.L.puckgpu_probe_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        if not _is_dev(p):
+            p = np.ascontiguousarray(p, np.float32)
+        if p.shape[0] != self.count:
+            raise ValueError(f"expected {self.count} parameters, got {p.shape[0]}")
+        self.ctx.sync_torch(p)
+        self.ctx.check(self.ctx.L.puckgpu_probe_set_params(self.ctx.h, self.h, _ptr(p), _mode(p)))
+
+    def get_params(self):
+        p = np.zeros(self.count, np.float32)
+        self.ctx.check(self.ctx.L.puckgpu_probe_get_params(self.ctx.h, self.h, _ptr(p), 0))
+        return p
+
+    def features(self, index, queries, fine_order=False):
+        """Raw features [n, D] (and fine order [n, F]) like the queries' arrays
+        (numpy in, numpy out; CUDA tensor in, CUDA tensors out)."""
+        n = queries.shape[0]
+        if _is_dev(queries):
+            import torch
+            f = torch.empty((n, self.D), dtype=torch.float32, device=queries.device)
+            fo = torch.empty((n, index.F), dtype=torch.int32, device=queries.device) if fine_order else None
+        else:
+            f = np.zeros((n, self.D), np.float32)
+            fo = np.zeros((n, index.F), np.uint32) if fine_order else None
+        self.ctx.sync_torch(queries)
+        self.ctx.check(self.ctx.L.puckgpu_probe_features(self.ctx.h, index.h, self.h, n, _ptr(queries), _ptr(f),
+                                                         _ptr(fo), _mode(queries, f, fo)))
+        return (f, fo) if fine_order else f
+
+    def train_step(self, features, target, lr=None, betas=(0.9, 0.999), eps=1e-8, want_loss=True,
+                   want_grads=False):
+        n = features.shape[0]
+        adam = _lib.AdamParams(lr, betas[0], betas[1], eps) if lr is not None else None
+        g = None
+        if want_grads:
+            k = self.count - 2 * self.D
+            if _is_dev(features):
+                import torch
+                g = torch.empty(k, dtype=torch.float32, device=features.device)
+            else:
+                g = np.zeros(k, np.float32)
+        loss = ctypes.c_float(0.0)
+        self.ctx.sync_torch(features, target)
+        self.ctx.check(self.ctx.L.puckgpu_probe_train_step(
+            self.ctx.h, self.h, n, _ptr(features), _ptr(target), ctypes.byref(adam) if adam else None,
+            ctypes.byref(loss) if want_loss else None, _ptr(g), _mode(features, target, g)))
+        return (loss.value, g) if want_grads else loss.value
+
+    def predict(self, features):
+        n = features.shape[0]
+        if _is_dev(features):
+            import torch
+            y = torch.empty(n, dtype=torch.float32, device=features.device)
+        else:
+            y = np.zeros(n, np.float32)
+        self.ctx.sync_torch(features)
+        self.ctx.check(self.ctx.L.puckgpu_probe_predict(self.ctx.h, self.h, n, _ptr(features), _ptr(y),
+                                                        _mode(features, y)))
+        return y
+
+
+def search_learned(index, probe, queries, topk_max, search_coarse_count, filter_topk, margin=0.0, min_budget=1,
+                   fallback_frac=1.0, topk=None, scc=None, radius_rate=1.0, replay_all=False, out=None):
+    """Batched search whose per-query fine budgets come from the probe, in the
+    same pipeline.  Returns (distance, labels, result_num, budgets)."""
+    ctx = index.ctx
+    n = queries.shape[0]
+    sp = _lib.SearchParams(topk_max, search_coarse_count, filter_topk, radius_rate, int(bool(replay_all)))
+    bp = _lib.BudgetParams(margin, min_budget, fallback_frac)
+    dev = _is_dev(queries)
+
+    def arr(a):
+        if a is None:
+            return None
+        if dev:
+            import torch
+            if not _is_dev(a):
+                a = torch.as_tensor(np.asarray(a, np.uint32).view(np.int32), device=queries.device)
+            return a.to(torch.int32).contiguous()
+        return np.ascontiguousarray(a, np.uint32)
+
+    qt, qs = arr(topk), arr(scc)
+    if out is None:
+        if dev:
+            import torch
+            dv = queries.device
+            out = (torch.empty((n, topk_max), dtype=torch.float32, device=dv),
+                   torch.empty((n, topk_max), dtype=torch.int32, device=dv),
+                   torch.empty(n, dtype=torch.int32, device=dv), torch.empty(n, dtype=torch.int32, device=dv))
+        else:
+            out = (np.zeros((n, topk_max), np.float32), np.zeros((n, topk_max), np.uint32), np.zeros(n, np.uint32),
+                   np.zeros(n, np.uint32))
+    D, I, R, B = out
+    m = _mode(queries, D, I, R, B, qt, qs)
+    qp = _lib.QueryParams(_ptr(qt), _ptr(qs), None)
+    ctx.sync_torch(queries, qt, qs)
+    ctx.check(ctx.L.puckgpu_search_learned(ctx.h, index.h, probe.h, ctypes.byref(sp), ctypes.byref(bp),
+                                           ctypes.byref(qp), n, _ptr(queries), _ptr(D), _ptr(I), _ptr(R), _ptr(B),
+                                           m))
+    return D, I, R, B

@@ -129,6 +129,46 @@ def test_search_not_full_and_clamps(ctx, idx_l2):
             assert np.array_equal(_bits(Dg[k, :o["n"]]), _bits(o["dist"][:o["n"]]))
 
 
+@pytest.mark.parametrize("which,replay", [("l2", False), ("cos", False), ("cos", True)])
+def test_search_batch_per_query_bitexact(ctx, which, replay, idx_l2, idx_cos):
+    # per-query topk / search_coarse_count / fine budget, and an invalid topk
+    ix, q = idx_l2 if which == "l2" else idx_cos
+    gi = _gpu_index(ctx, ix)
+    oi = CI.oracle_index(ix)
+    n, F = q.shape[0], ix["fine"].shape[0]
+    rng = np.random.default_rng(31)
+    topk = rng.choice([1, 10, 100], n).astype(np.uint32)
+    topk[7] = 0
+    topk[9] = 101
+    scc = rng.integers(1, 24, n).astype(np.uint32)
+    nf = rng.integers(0, F + 4, n).astype(np.uint32)
+    nf[:8] = [1, 2, 3, 5, F - 1, F, 0, 4]
+    Dg, Ig, Rg = gi.search_batch(q, 100, 20, 1100, topk=topk, scc=scc, fine_budget=nf, replay_all=replay)
+    Do, Io, Ro = oi.search_batch_pq(q, 100, 20, 1100, topk=topk, scc_arr=scc, fine_budget=nf)
+    assert np.array_equal(Rg, Ro)
+    for k in range(n):
+        m = int(Ro[k])
+        assert np.array_equal(Ig[k, :m], Io[k, :m]), f"query {k}"
+        assert np.array_equal(_bits(Dg[k, :m]), _bits(Do[k, :m])), f"query {k}"
+        assert np.all(Ig[k, m:] == 0xffffffff) and np.all(_bits(Dg[k, m:]) == 0x7f7f7f7f), f"query {k}"
+
+
+def test_search_batch_device_arrays(ctx, idx_cos):
+    import torch
+    ix, q = idx_cos
+    gi = _gpu_index(ctx, ix)
+    n = q.shape[0]
+    rng = np.random.default_rng(5)
+    topk = rng.choice([1, 10], n).astype(np.uint32)
+    nf = rng.integers(1, 40, n).astype(np.uint32)
+    Dh, Ih, Rh = gi.search_batch(q, 10, 8, 300, topk=topk, fine_budget=nf)
+    qd = torch.from_numpy(q).cuda()
+    Dd, Id, Rd = gi.search_batch(qd, 10, 8, 300, topk=topk, fine_budget=nf)
+    assert np.array_equal(Rd.cpu().numpy().view(np.uint32), Rh)
+    assert np.array_equal(Id.cpu().numpy().view(np.uint32), Ih)
+    assert np.array_equal(_bits(Dd.cpu().numpy()), _bits(Dh))
+
+
 def test_search_errors(ctx, idx_cos):
     from puck_amd.engine import PuckGpuError
     ix, q = idx_cos

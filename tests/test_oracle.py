@@ -304,3 +304,53 @@ def test_recall_known_answers(kat):
     assert r_ties_o == pytest.approx(kat["expected_with_ties"], abs=1e-12)
     r_plain, _ = O.recall_without_ties(ti, ri, c)
     assert r_plain == pytest.approx(kat["expected_no_ties"], abs=1e-12)
+
+
+def test_fine_budget_full_scan_is_restricted_brute_force(small_l2):
+    # scc = C and filter_topk >= N with fine budget b: nothing prunes, so the
+    # candidates are exactly the points whose fine cluster is among the query's
+    # first b fine clusters (ascending -<f, q>), and the result is their exact
+    # top-k.  A budget of F (or 0) is the reference's walk.
+    ix, q = small_l2
+    oi = CI.oracle_index(ix)
+    raw, fine = ix["raw"], ix["fine"]
+    N, F = raw.shape[0], fine.shape[0]
+    fine_of = ix["cell_assign"] % F
+    for k in range(8):
+        b = [1, 3, 7, 20][k % 4]
+        o = oi.search_one(q[k], 10, 1000, N, fine_budget=b)
+        fo = np.argsort(-(fine.astype(np.float64) @ q[k].astype(np.float64)), kind="stable")[:b]
+        cand = np.nonzero(np.isin(fine_of, fo))[0]
+        assert o["filt_n"] == cand.size
+        got = np.sort(o["ids"][:o["n"]])
+        d = ((raw[cand].astype(np.float64) - q[k]) ** 2).sum(1)
+        want = np.sort(cand[np.argsort(d, kind="stable")[:min(10, cand.size)]])
+        assert np.array_equal(got, want)
+        full = oi.search_one(q[k], 10, 4, 200)
+        for bb in (0, F, F + 5):
+            o2 = oi.search_one(q[k], 10, 4, 200, fine_budget=bb)
+            assert np.array_equal(o2["ids"], full["ids"]) and np.array_equal(o2["dist"], full["dist"])
+
+
+def test_per_query_batch_matches_single_calls(small_l2):
+    ix, q = small_l2
+    oi = CI.oracle_index(ix)
+    n = q.shape[0]
+    rng = np.random.default_rng(7)
+    topk = rng.choice([1, 5, 10], n).astype(np.uint32)
+    topk[3] = 0
+    topk[5] = 11  # > max_topk: that query fails
+    scc = rng.integers(1, 6, n).astype(np.uint32)
+    nf = rng.integers(0, ix["fine"].shape[0] + 3, n).astype(np.uint32)
+    D, I, R = oi.search_batch_pq(q, 10, 4, 64, topk=topk, scc_arr=scc, fine_budget=nf)
+    sent = np.frombuffer(np.uint32(0x7f7f7f7f).tobytes(), np.float32)[0]
+    for i in range(n):
+        if topk[i] == 0 or topk[i] > 10:
+            assert R[i] == 0 and np.all(I[i] == 0xffffffff) and np.all(D[i] == sent)
+            continue
+        o = oi.search_one(q[i], int(topk[i]), int(min(scc[i], 4)), 64, max_topk=10, fine_budget=int(nf[i]))
+        k = int(topk[i])
+        assert R[i] == o["n"]
+        assert np.array_equal(D[i, :o["n"]].view(np.uint32), o["dist"][:o["n"]].view(np.uint32))
+        assert np.array_equal(I[i, :o["n"]], o["ids"][:o["n"]])
+        assert np.all(I[i, o["n"]:] == 0xffffffff) and np.all(D[i, k:] == sent)
