@@ -1,15 +1,24 @@
 """Benchmark: QPS at recall@10 >= 0.95 on 10M x 96 unit-norm vectors (cosine as
 L2), C = F = 2000, filter_nsq = 48, filter_topk = 1000, top-k = 10, one batch
-of 10,000 queries, on one MI355X.
+of 10,000 queries, on one MI355X: the learned-probe search against the plain
+GPU search and the CPU restatement of the reference.
 
 Flow: synthetic data generated on the GPU (seeded) -> GPU training / cell
 assignment / filter-PQ encoding -> index load -> exact ground truth (FP32
-GEMM + FP64 recheck) -> sweep of search_coarse_count -> the smallest value
-reaching recall@10 >= 0.95 is timed for --steps batches after --warmup.
+GEMM + FP64 recheck).
+  plain:   sweep search_coarse_count; the smallest value reaching the target
+           recall is the operating point, timed for --steps batches.
+  learned: extra training queries (seed 5) -> exact ground truth -> features
+           and fine-budget labels from the engine at the plain operating point
+           -> probe trained by the device kernels (Adam) -> sweep of the budget
+           margin; the fastest margin reaching the target recall is timed.
+  uniform: the same fine budget for every query (no network), swept, for the
+           gain of the network over the method without it.
 The CPU baseline is the oracle's restatement of PuckIndex::search on the host
-cores, on a bounded query sample at the same operating point.
+cores, on a bounded query sample at the plain operating point.
 
-Prints one JSON line (see the "metric" key)."""
+Prints one JSON line (see the "metric" key); `value` is the learned-probe
+form's QPS."""
 import argparse
 import json
 import os
@@ -28,6 +37,24 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def timed(run, steps, warmup, gi):
+    """Runs `run()` warmup + steps times; returns wall seconds of the timed
+    steps, per-stage device ms summed over them, walk ms per step, counters."""
+    import torch
+    for _ in range(warmup):
+        run()
+    torch.cuda.synchronize()
+    walk_ms, stage_tot, cnt = [], np.zeros(5), None
+    t1 = time.time()
+    for _ in range(steps):
+        run()
+        ms, cnt = gi.last_stage_ms()
+        walk_ms.append(float(ms[2]))
+        stage_tot += ms
+    torch.cuda.synchronize()
+    return time.time() - t1, stage_tot, walk_ms, cnt
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -36,7 +63,18 @@ def main():
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--nq", type=int, default=10_000)
     ap.add_argument("--sweep", type=str, default="1,2,3,4,6,8,12,16,24,32,48,64,96,128")
+    ap.add_argument("--scc", type=int, default=0, help="plain operating point (skips the sweep)")
     ap.add_argument("--target-recall", type=float, default=0.95)
+    ap.add_argument("--probe-train", type=int, default=250_000, help="training queries of the learned probe")
+    ap.add_argument("--probe-epochs", type=int, default=8)
+    ap.add_argument("--margins", type=str, default="-1.5,-1.25,-1,-0.75,-0.5,-0.25,0,0.25,0.5,0.75,1,1.5")
+    ap.add_argument("--margin", type=float, default=None, help="learned operating margin (skips the sweep)")
+    ap.add_argument("--fallback-frac", type=float, default=0.5)
+    ap.add_argument("--budgets", type=str, default="2,3,4,6,8,12,16,24,32,48,64,96,128,192,256,512,2000")
+    ap.add_argument("--no-probe", action="store_true")
+    ap.add_argument("--no-uniform", action="store_true")
+    ap.add_argument("--profile-form", choices=["plain", "learned"], default=None,
+                    help="time only this form (for profiler runs); needs --scc (and --margin for learned)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
@@ -50,7 +88,8 @@ def main():
 
     import torch
     from puck_amd import datasets, train
-    from puck_amd.engine import Context, GpuIndex
+    from puck_amd import probe as PR
+    from puck_amd.engine import Context, GpuIndex, search_learned
     from puck_amd.metrics import exact_topk, recall_with_ties
 
     N, d, C, F, nsq, ftk, topk, nq = args.n, 96, 2000, 2000, 48, 1000, 10, args.nq
@@ -83,48 +122,105 @@ def main():
         torch.cuda.synchronize()
         del src, dst
         log("calibration copy: 2 GiB device-to-device")
+
+    def rec(ids):
+        return recall_with_ties(gt_i, gt_d, ids.cpu().numpy().astype(np.int64), topk)[0]
+
+    # ---- plain search: sweep search_coarse_count
     out = (torch.empty((nq, topk), dtype=torch.float32, device=dev),
            torch.empty((nq, topk), dtype=torch.int32, device=dev),
            torch.empty(nq, dtype=torch.int32, device=dev))
     curve = []
-    chosen = None
-    for scc in [int(s) for s in args.sweep.split(",")]:
-        t1 = time.time()
-        gi.search(q, topk, scc, ftk, out=out)
-        torch.cuda.synchronize()
-        el = time.time() - t1
-        ids = out[1].cpu().numpy().astype(np.int64)
-        r, _ = recall_with_ties(gt_i, gt_d, ids, topk)
-        ms, cnt = gi.last_stage_ms()
-        curve.append({"search_coarse_count": scc, "recall": round(r, 5), "qps_single_run": round(nq / el, 1),
-                      "walk_ms": round(float(ms[2]), 3)})
-        log(f"scc={scc} recall@10={r:.4f} {nq / el:.0f} qps stage_ms={np.round(ms, 2).tolist()} "
-            f"counters={cnt.tolist()}")
-        if r >= args.target_recall:
-            chosen = scc
-            break
-        if el > 60:
-            log("single search above 60 s; stopping the sweep")
-            break
-    if chosen is None:
-        chosen = int(args.sweep.split(",")[-1])
+    chosen = args.scc or None
+    if not args.scc:
+        for scc in [int(s) for s in args.sweep.split(",")]:
+            t1 = time.time()
+            gi.search(q, topk, scc, ftk, out=out)
+            torch.cuda.synchronize()
+            el = time.time() - t1
+            r = rec(out[1])
+            ms, cnt = gi.last_stage_ms()
+            curve.append({"search_coarse_count": scc, "recall": round(r, 5), "qps_single_run": round(nq / el, 1),
+                          "walk_ms": round(float(ms[2]), 3)})
+            log(f"plain scc={scc} recall@10={r:.4f} {nq / el:.0f} qps stage_ms={np.round(ms, 2).tolist()}")
+            if r >= args.target_recall:
+                chosen = scc
+                break
+            if el > 60:
+                log("single search above 60 s; stopping the sweep")
+                break
+        if chosen is None:
+            chosen = int(args.sweep.split(",")[-1])
 
-    for _ in range(args.warmup):
-        gi.search(q, topk, chosen, ftk, out=out)
-    torch.cuda.synchronize()
-    walk_ms, stage_tot = [], np.zeros(5)
-    t1 = time.time()
-    for _ in range(args.steps):
-        gi.search(q, topk, chosen, ftk, out=out)
-        ms, cnt = gi.last_stage_ms()
-        walk_ms.append(float(ms[2]))
-        stage_tot += ms
-    torch.cuda.synchronize()
-    el = time.time() - t1
+    plain = None
+    if args.profile_form in (None, "plain"):
+        el, stage_tot, walk_ms, cnt = timed(lambda: gi.search(q, topk, chosen, ftk, out=out), args.steps,
+                                            args.warmup, gi)
+        plain = dict(el=el, stage=stage_tot, walk_ms=walk_ms, cnt=cnt, recall=rec(out[1]))
+        log(f"plain timed: scc={chosen} {nq * args.steps / el:.0f} qps recall {plain['recall']:.4f}")
+
+    # ---- uniform fine budget (no network)
+    uniform_curve, uniform_best = [], None
+    if not args.no_uniform and args.profile_form is None:
+        for b in [int(v) for v in args.budgets.split(",")]:
+            nfb = torch.full((nq,), min(b, F), dtype=torch.int32, device=dev)
+            Db, Ib, Rb = gi.search_batch(q, topk, chosen, ftk, fine_budget=nfb)
+            el, _, _, _ = timed(lambda: gi.search_batch(q, topk, chosen, ftk, fine_budget=nfb, out=(Db, Ib, Rb)),
+                                2, 1, gi)
+            r = rec(Ib)
+            uniform_curve.append({"fine_budget": b, "recall": round(r, 5), "qps": round(2 * nq / el, 1)})
+            log(f"uniform budget {b}: recall {r:.4f} {2 * nq / el:.0f} qps")
+            if r >= args.target_recall and (uniform_best is None or uniform_curve[-1]["qps"] > uniform_best["qps"]):
+                uniform_best = uniform_curve[-1]
+
+    # ---- learned probe
+    learned, probe_info, margin_curve = None, None, []
+    if not args.no_probe and args.profile_form in (None, "learned"):
+        t0 = time.time()
+        qt = datasets.unit_norm_mixture(args.probe_train, d, 4096, seed=5, means_seed=2000, backend="torch",
+                                        device=dev)
+        gtt, _ = exact_topk(x, qt, topk)
+        log(f"probe training queries {args.probe_train} + ground truth {time.time() - t0:.1f}s")
+        t0 = time.time()
+        cell_dev = ix["cell_assign"] if torch.is_tensor(ix["cell_assign"]) else torch.from_numpy(ix["cell_assign"])
+        probe, probe_info = PR.fit(ctx, gi, qt, gtt, cell_dev.to(dev).view(torch.int32), chosen, ftk, topk,
+                                   epochs=args.probe_epochs, log=log)
+        torch.cuda.synchronize()
+        probe_info["train_s"] = round(time.time() - t0, 1)
+        log(f"probe fit {probe_info['train_s']}s: {probe_info}")
+        del qt, gtt
+        lout = out + (torch.empty(nq, dtype=torch.int32, device=dev),)
+        margins = [args.margin] if args.margin is not None else [float(v) for v in args.margins.split(",")]
+        best = None
+        for mg in margins:
+            run = lambda: search_learned(gi, probe, q, topk, chosen, ftk, margin=mg,
+                                         fallback_frac=args.fallback_frac, out=lout)
+            run()
+            r = rec(lout[1])
+            if args.margin is None:
+                el, _, _, _ = timed(run, 2, 1, gi)
+                qps = 2 * nq / el
+            else:
+                qps = float("nan")
+            bud = lout[3].double()
+            margin_curve.append({"margin": mg, "recall": round(r, 5), "qps": round(qps, 1),
+                                 "mean_budget": round(float(bud.mean()), 1),
+                                 "fallback_frac_of_queries": round(float((bud >= F).double().mean()), 4)})
+            log(f"learned margin {mg}: recall {r:.4f} {qps:.0f} qps mean budget {float(bud.mean()):.1f}")
+            if r >= args.target_recall and (best is None or qps > best[1]):
+                best = (mg, qps)
+        mg = best[0] if best else margins[-1]
+        run = lambda: search_learned(gi, probe, q, topk, chosen, ftk, margin=mg, fallback_frac=args.fallback_frac,
+                                     out=lout)
+        el, stage_tot, walk_ms, cnt = timed(run, args.steps, args.warmup, gi)
+        learned = dict(el=el, stage=stage_tot, walk_ms=walk_ms, cnt=cnt, recall=rec(lout[1]), margin=mg,
+                       mean_budget=float(lout[3].double().mean()))
+        log(f"learned timed: margin {mg} {nq * args.steps / el:.0f} qps recall {learned['recall']:.4f}")
+
+    head = learned if learned is not None else plain
+    el, stage_tot, walk_ms, cnt = head["el"], head["stage"], head["walk_ms"], head["cnt"]
     ms_per_step = 1000.0 * el / args.steps
     qps = nq * args.steps / el
-    ids = out[1].cpu().numpy().astype(np.int64)
-    recall, _ = recall_with_ties(gt_i, gt_d, ids, topk)
     # algorithmic bytes of the walk + filter scan launch: per examined cell its
     # stationary distance and start offset (8 B); per scanned point its offset
     # and nsq code bytes.
@@ -132,14 +228,33 @@ def main():
     walk_bytes = cells * 8 + points * (4 + nsq)
     avg_walk_s = np.mean(walk_ms) / 1000.0
     achieved = walk_bytes / avg_walk_s / 1e9
+    form = "learned" if learned is not None else "plain"
     # HBM traffic per launch of the same kernel and workload, from the
     # committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (copy-calibrated)
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "pk_walk_10Mx96_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", f"pk_walk_10Mx96_{form}_traffic.json")
     if N == 10_000_000 and nq == 10_000 and os.path.exists(tpath):
         traffic = int(json.load(open(tpath))["traffic_bytes_per_launch"])
+
+    def form_summary(f):
+        if f is None:
+            return None
+        return {"qps": round(nq * args.steps / f["el"], 1), "ms_per_step": round(1000.0 * f["el"] / args.steps, 3),
+                "recall_at_10": round(f["recall"], 5),
+                "stage_ms_per_step": {k: round(float(v) / args.steps, 3) for k, v in
+                                      zip(["prep+gemm", "center_sort+probe", "walk_scan", "rerank", "tie_replay"],
+                                          f["stage"])},
+                "walk_counters_per_query": {k: round(float(v) / nq, 3) for k, v in
+                                            zip(["segments", "uncertain_cells", "examined_cells", "scanned_points"],
+                                                f["cnt"][:4])},
+                "replayed_queries": int(f["cnt"][4]),
+                "replay_causes": {k: int(v) for k, v in zip(["coarse_tie", "filter_boundary_tie", "exact_distance_tie",
+                                                             "fine_tie_decision", "fine_tie_back",
+                                                             "fine_tie_chunk_edge", "fine_tie_group_ge3",
+                                                             "fine_tie_orders_disagree"], f["cnt"][5:13])}}
+
     result = {
-        "metric": "QPS at recall@10=0.95 (10M x 96, plain GPU Puck search)",
+        "metric": "QPS at recall@10=0.95 (10M x 96): learned-probe vs plain GPU vs CPU",
         "value": round(qps, 1),
         "unit": "queries/s",
         "n_gpus": 1,
@@ -153,17 +268,22 @@ def main():
         "data": "synthetic (seeded 4096-mean unit-norm mixture, generated on the GPU)",
         "config": {"workload": "10M x 96 unit-norm, cosine as L2, C=F=2000, filter_nsq=48, filter_topk=1000, "
                                "top-k=10, batch of 10000 queries",
-                   "N": N, "dim": d, "coarse": C, "fine": F, "filter_nsq": nsq, "filter_topk": ftk, "topk": topk,
-                   "nq": nq, "search_coarse_count": chosen, "recall_at_10": round(recall, 5)},
+                   "form": form, "N": N, "dim": d, "coarse": C, "fine": F, "filter_nsq": nsq, "filter_topk": ftk,
+                   "topk": topk, "nq": nq, "search_coarse_count": chosen,
+                   "recall_at_10": round(head["recall"], 5)},
+        "plain_gpu": form_summary(plain),
+        "learned_probe": None if learned is None else dict(form_summary(learned), margin=learned["margin"],
+                                                           mean_fine_budget=round(learned["mean_budget"], 1),
+                                                           fallback_frac=args.fallback_frac, training=probe_info,
+                                                           margin_curve=margin_curve),
+        "learned_vs_plain": (round((nq * args.steps / learned["el"]) / (nq * args.steps / plain["el"]), 3)
+                             if learned is not None and plain is not None else None),
+        "uniform_budget": {"curve": uniform_curve, "best_at_target": uniform_best} if uniform_curve else None,
         "recall_curve": curve,
-        "stage_ms_per_step": {k: round(float(v) / args.steps, 3) for k, v in
-                              zip(["prep+gemm", "center_sort", "walk_scan", "rerank", "tie_replay"], stage_tot)},
-        "roofline": {"kernel": "pk_walk", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes_per_launch": int(walk_bytes),
+        "roofline": {"kernel": "pk_walk", "form": form, "bound": "hbm", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "algorithmic_bytes_per_launch": int(walk_bytes),
                      "avg_launch_ms": round(avg_walk_s * 1000.0, 3)},
-        "walk_counters_per_query": {k: round(float(v) / nq, 3) for k, v in
-                                    zip(["segments", "uncertain_cells", "examined_cells", "scanned_points"], cnt[:4])},
         "tie_replays": {k: int(v) for k, v in zip(["replayed_queries", "coarse_tie", "filter_boundary_tie",
                                                    "exact_distance_tie", "fine_tie_decision", "fine_tie_back",
                                                    "fine_tie_chunk_edge", "fine_tie_group_ge3",
@@ -171,6 +291,7 @@ def main():
         "build_s": round(build_s, 1),
     }
     if not args.no_cpu:
+        gi.search(q, topk, chosen, ftk, out=out)
         result["cpu_baseline"] = cpu_baseline(ix, q.cpu().numpy(), topk, chosen, ftk, args.cpu_seconds, out,
                                               gt_i, gt_d)
     print(json.dumps(result), flush=True)

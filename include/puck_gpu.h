@@ -107,9 +107,14 @@ typedef struct {
     const uint32_t* topk;                /* 1..topk_max; another value gives that query result_num 0 */
     const uint32_t* search_coarse_count; /* coarse lists walked, clamped to params->search_coarse_count
                                             (the batch maximum, itself clamped to C) */
-    const uint32_t* fine_budget;         /* fine clusters considered per coarse list, a prefix of the
-                                            query's ascending fine-distance order; 0 = all F (the
-                                            reference's walk) */
+    const uint32_t* fine_budget;         /* b: 0 or >= F = the reference's cell walk.  0 < b < F = the
+                                            budget scan: the coarse lists are the nearest by (distance,
+                                            id); in each, every point of the cells whose cell distance
+                                            (|q-c-f|^2 - |q|^2)/2 is at most the b-th smallest of the
+                                            list (ties included) gets the reference's filter distance;
+                                            the filter set is the filter_topk smallest (distance, memory
+                                            id) and the result the topk smallest (exact distance, local
+                                            id) of it */
 } puckgpu_query_params;
 
 /* Batched search with per-query parameters (PySearcher::search's batch, one
@@ -122,10 +127,11 @@ int puckgpu_search_batch(puckgpu_ctx* ctx, const puckgpu_index* index, const puc
 /* ---- Learned probe ------------------------------------------------------
  * No counterpart in the reference.  A small MLP predicts, per query, the log
  * of the fine budget (puckgpu_query_params.fine_budget) the cell walk needs
- * to reach its top-k; the fixed-budget walk (all F fine clusters, the
- * reference's) is its fallback.  Network: in = dim + coarse_features +
+ * to reach its top-k; the walk with every cell (the reference's) is its
+ * fallback.  Network: in = dim + coarse_features +
  * fine_features inputs (the normalised query, its smallest coarse distances
- * |c|^2/2 - <c,q> and its smallest fine distances -<f,q>, ascending), each
+ * |c|^2/2 - <c,q>, and the smallest cell distances (|q-c-f|^2 - |q|^2)/2 of
+ * the fine clusters of its nearest coarse list, both ascending), each
  * standardised as (x - mu) / sigma; two hidden GELU layers of `hidden` units;
  * one output.  Weights are used in BF16 with FP32 accumulation.
  * Packed FP32 parameters (puckgpu_probe_param_count floats):
@@ -134,7 +140,7 @@ int puckgpu_search_batch(puckgpu_ctx* ctx, const puckgpu_index* index, const puc
 typedef struct puckgpu_probe puckgpu_probe;
 typedef struct {
     uint32_t coarse_features; /* smallest coarse distances used (<= C) */
-    uint32_t fine_features;   /* smallest fine distances used (<= F) */
+    uint32_t fine_features;   /* smallest cell distances of the nearest list used (<= F) */
     uint32_t hidden;          /* units per hidden layer */
 } puckgpu_probe_shape;
 typedef struct {
@@ -157,7 +163,7 @@ int puckgpu_probe_set_params(puckgpu_ctx* ctx, puckgpu_probe* probe, const float
 int puckgpu_probe_get_params(puckgpu_ctx* ctx, const puckgpu_probe* probe, float* params, int device_ptrs);
 /* Raw (unstandardised) features of n queries [n x in], computed by the search
  * pipeline's own center GEMM and sorts; fine_order [n x F] (may be NULL)
- * receives each query's ascending fine-distance order (fine ids). */
+ * receives each query's ascending fine-distance (-<f,q>) order (fine ids). */
 int puckgpu_probe_features(puckgpu_ctx* ctx, const puckgpu_index* index, const puckgpu_probe* probe, uint32_t n,
                            const float* queries, float* features, uint32_t* fine_order, int device_ptrs);
 /* One training step on a batch: forward, mean-squared-error loss against
@@ -181,9 +187,19 @@ int puckgpu_search_learned(puckgpu_ctx* ctx, const puckgpu_index* index, const p
                            float* distance, uint32_t* labels, uint32_t* result_num, uint32_t* budget_out,
                            int device_ptrs);
 
+/* Fine-budget ranks (labels of the learned probe): for query i and entry e of
+ * cells [n x k] (cell id coarse * F + fine, or 0xffffffff), the smallest
+ * fine_budget under which the walk of that coarse list scores the cell -- 1 +
+ * the number of cells of the list with a smaller cell distance -- or
+ * 0xffffffff when the entry is empty or its coarse center is not among the
+ * query's search_coarse_count nearest.  ranks [n x k]. */
+int puckgpu_budget_ranks(puckgpu_ctx* ctx, const puckgpu_index* index, uint32_t search_coarse_count, uint32_t n,
+                         const float* queries, uint32_t k, const uint32_t* cells, uint32_t* ranks,
+                         int device_ptrs);
 /* Stage output of the last search on this context (host buffers):
  * filter-stage survivors per query (filter_topk entries per row, ascending
- * filter distance, memory ids) and their count.  For tests and diagnostics. */
+ * filter distance -- in no particular order for a query searched with a fine
+ * budget -- memory ids) and their count.  For tests and diagnostics. */
 int puckgpu_last_filter(puckgpu_ctx* ctx, uint32_t n, uint32_t filter_topk, float* filt_dist, uint32_t* filt_mem,
                         uint32_t* filt_n);
 /* Per-stage device times (ms) of the last search, summed over its passes:

@@ -375,13 +375,123 @@ typedef struct {
     uint64_t heap_updates;      /* max_heap_update calls that inserted */
 } po_stats;
 
+/* Order-preserving map of float bits to uint32 (ascending floats give
+ * ascending keys; -0 sorts just below +0). */
+static uint32_t po_f2ord(float v) {
+    uint32_t u;
+    memcpy(&u, &v, 4);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+static int po_u32_cmp(const void* a, const void* b) {
+    uint32_t x = *(const uint32_t*)a, y = *(const uint32_t*)b;
+    return x < y ? -1 : (x > y);
+}
+
+static int po_u64_cmp(const void* a, const void* b) {
+    uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+    return x < y ? -1 : (x > y);
+}
+
+/* Budget scan (no counterpart in the reference; the learned probe's fine
+ * budget).  q is the prepared query.
+ *   coarse lists: the scc smallest |c|^2/2 - <c,q> by (distance, id);
+ *   cells of a list: the b smallest cell distances (cd + stationary) + fine
+ *     distance, compared as order-preserving keys, ties at the b-th included;
+ *   filter set: the ftk smallest (filter distance, memory id) over every
+ *     point of those cells, the filter distance computed as the reference's
+ *     compute_quantized_distance does (puck_index.cpp:346-390) without its
+ *     early exits (which drop only points that cannot be among the smallest);
+ *   result: the topk smallest (exact distance, local id) of the filter set.
+ * filt_val / filt_tag receive the filter set ascending by (distance, id). */
+static int po_search_budget(const po_index* ix, const float* q, uint32_t topk, uint32_t scc, uint32_t ftk,
+                            uint32_t b, float* out_dist, uint32_t* out_ids, uint32_t* out_n, float* filt_val,
+                            uint32_t* filt_tag, uint32_t* filt_n, po_stats* st) {
+    uint32_t dim = ix->dim, C = ix->C, F = ix->F, nsq = ix->nsq;
+    size_t rec = 4 + nsq;
+    uint64_t* ck = (uint64_t*)malloc(sizeof(uint64_t) * C);
+    float* cdv = (float*)malloc(sizeof(float) * C);
+    for (uint32_t c = 0; c < C; ++c) {
+        cdv[c] = ix->coarse_norms[c] - po_dot(ix->coarse + (size_t)c * dim, q, dim);
+        ck[c] = ((uint64_t)po_f2ord(cdv[c]) << 32) | c;
+    }
+    qsort(ck, C, sizeof(uint64_t), po_u64_cmp);
+    float* fd = (float*)malloc(sizeof(float) * F);
+    for (uint32_t k = 0; k < F; ++k) fd[k] = -po_dot(ix->fine + (size_t)k * dim, q, dim);
+    float* lut = (float*)malloc(sizeof(float) * nsq * PO_KS);
+    po_lut(q, ix->codebook, nsq, ix->lsq, lut);
+    uint32_t* keys = (uint32_t*)malloc(sizeof(uint32_t) * F);
+    uint32_t* sorted = (uint32_t*)malloc(sizeof(uint32_t) * F);
+    size_t cap = 1024, n = 0;
+    uint64_t* cand = (uint64_t*)malloc(sizeof(uint64_t) * cap);
+    po_stats s = {0, 0, 0, 0};
+    for (uint32_t l = 0; l < scc; ++l) {
+        uint32_t c = (uint32_t)(ck[l] & 0xffffffffu);
+        float cd = cdv[c];
+        const float* cst = ix->stationary + (size_t)c * F;
+        for (uint32_t k = 0; k < F; ++k) keys[k] = po_f2ord(cd + cst[k] + fd[k]);
+        memcpy(sorted, keys, sizeof(uint32_t) * F);
+        qsort(sorted, F, sizeof(uint32_t), po_u32_cmp);
+        uint32_t kcut = sorted[b - 1];
+        for (uint32_t k = 0; k < F; ++k) {
+            s.cells_considered++;
+            if (keys[k] > kcut) continue;
+            s.cells_scanned++;
+            float cell_dist = cd + cst[k] + fd[k];
+            uint32_t b0 = ix->cell_start[(size_t)c * F + k], e0 = ix->cell_start[(size_t)c * F + k + 1];
+            for (uint32_t i = b0; i < e0; ++i) {
+                const uint8_t* r = ix->records + (size_t)i * rec;
+                float off;
+                memcpy(&off, r, 4);
+                float t = (float)(2.0 * (double)cell_dist + (double)off);
+                t = t + po_lookup(r + 4, lut, nsq);
+                if (n == cap) {
+                    cap *= 2;
+                    cand = (uint64_t*)realloc(cand, sizeof(uint64_t) * cap);
+                }
+                cand[n++] = ((uint64_t)po_f2ord(t) << 32) | i;
+                s.points_offered++;
+            }
+        }
+    }
+    qsort(cand, n, sizeof(uint64_t), po_u64_cmp);
+    uint32_t nf = n < ftk ? (uint32_t)n : ftk;
+    uint64_t* rk = (uint64_t*)malloc(sizeof(uint64_t) * (nf ? nf : 1));
+    for (uint32_t i = 0; i < nf; ++i) {
+        uint32_t mid = (uint32_t)(cand[i] & 0xffffffffu);
+        uint32_t tb = (uint32_t)(cand[i] >> 32);
+        tb = (tb & 0x80000000u) ? (tb & 0x7fffffffu) : ~tb;
+        if (filt_val) memcpy(filt_val + i, &tb, 4);
+        if (filt_tag) filt_tag[i] = mid;
+        float d = po_rerank_dist(ix->raw + (size_t)mid * dim, q, dim);
+        rk[i] = ((uint64_t)po_f2ord(d) << 32) | ix->mem2local[mid];
+    }
+    if (filt_n) *filt_n = nf;
+    qsort(rk, nf, sizeof(uint64_t), po_u64_cmp);
+    uint32_t kout = nf < topk ? nf : topk;
+    for (uint32_t i = 0; i < topk; ++i) {
+        uint32_t db = 0x7f7f7f7fu, id = 0xffffffffu;
+        if (i < kout) {
+            db = (uint32_t)(rk[i] >> 32);
+            db = (db & 0x80000000u) ? (db & 0x7fffffffu) : ~db;
+            id = (uint32_t)(rk[i] & 0xffffffffu);
+        }
+        memcpy(out_dist + i, &db, 4);
+        out_ids[i] = id;
+    }
+    *out_n = kout;
+    s.heap_updates = nf;
+    if (st) *st = s;
+    free(ck); free(cdv); free(fd); free(lut); free(keys); free(sorted); free(cand); free(rk);
+    return 0;
+}
+
 /* One query: PuckIndex::search (puck_index.cpp:559-600) with whether_pq=false.
  * Returns 0, or -1 for topk > max_topk / null feature / zero-norm query.
  * filt_val/filt_tag (ftk each) receive the filter heap as rank_topN_points
  * reads it (heap order when full, ascending when not full).
- * fine_budget (0 = F, the reference's walk) limits each coarse list's walk to
- * the first fine_budget entries of the heap-sorted fine list: an extension
- * with no counterpart in the reference, for the per-query budget. */
+ * fine_budget b: 0 or >= F is the reference's walk; 0 < b < F is the budget
+ * scan of po_search_budget (an extension with no counterpart in the
+ * reference). */
 int po_search_one(const void* p, const float* feature, uint32_t topk, uint32_t max_topk,
                   uint32_t scc, uint32_t ftk, double radius_rate, uint32_t fine_budget,
                   float* out_dist, uint32_t* out_ids, uint32_t* out_n,
@@ -397,6 +507,12 @@ int po_search_one(const void* p, const float* feature, uint32_t topk, uint32_t m
         if (po_normalize(feature, dim, q) != 0) { free(q); return -1; }
     } else {
         memcpy(q, feature, sizeof(float) * dim);
+    }
+    if (fine_budget != 0 && fine_budget < F) {
+        int rc = po_search_budget(ix, q, topk, scc, ftk, fine_budget, out_dist, out_ids, out_n, filt_val,
+                                  filt_tag, filt_n, st);
+        free(q);
+        return rc;
     }
     /* search_nearest_coarse_cluster (hierarchical_cluster_index.cpp:700-727) */
     float* coarse_d = (float*)malloc(sizeof(float) * (scc ? scc : 1));
@@ -435,6 +551,7 @@ int po_search_one(const void* p, const float* feature, uint32_t topk, uint32_t m
     float pivot = (float)((double)(fval[0] - qn) / radius_rate / 2.0);
     size_t rec = 4 + nsq;
     po_stats s = {0, 0, 0, 0};
+
     for (uint32_t l = 0; l < scc; ++l) {
         uint32_t c = coarse_t[l];
         const float* cst = ix->stationary + (size_t)c * F;
@@ -442,8 +559,7 @@ int po_search_one(const void* p, const float* feature, uint32_t topk, uint32_t m
         float cd = coarse_d[l];
         float min_dist = ix->min_dist_offset[c] + cd;
         float max_st = pivot - cd - fine_d[0];
-        uint32_t Fq = (fine_budget == 0 || fine_budget > F) ? F : fine_budget;
-        for (uint32_t idx = 0; idx < Fq; ++idx) {
+        for (uint32_t idx = 0; idx < F; ++idx) {
             s.cells_considered++;
             if (fine_d[idx] + min_dist >= pivot) break;
             uint32_t k = fine_t[idx];

@@ -17,7 +17,7 @@ EXPORTS = [
     "puckgpu_encode_filter", "puckgpu_kmeans_assign", "puckgpu_search_batch",
     "puckgpu_probe_create", "puckgpu_probe_free", "puckgpu_probe_param_count", "puckgpu_probe_set_params",
     "puckgpu_probe_get_params", "puckgpu_probe_features", "puckgpu_probe_train_step", "puckgpu_probe_predict",
-    "puckgpu_search_learned",
+    "puckgpu_search_learned", "puckgpu_budget_ranks",
 ]
 
 
@@ -124,6 +124,8 @@ def load():
     L.puckgpu_search_learned.argtypes = [vp, vp, vp, ctypes.POINTER(SearchParams), ctypes.POINTER(BudgetParams),
                                          ctypes.POINTER(QueryParams), u32, vp, vp, vp, vp, vp, c_int]
     L.puckgpu_search_learned.restype = c_int
+    L.puckgpu_budget_ranks.argtypes = [vp, vp, u32, u32, vp, u32, vp, vp, c_int]
+    L.puckgpu_budget_ranks.restype = c_int
     del i32
     _lib = L
     return L

@@ -19,7 +19,7 @@ hipError_t pk_launch_gemm_exact(const float*, const float*, float*, uint32_t, ui
                                 const float*, hipStream_t);
 hipError_t pk_launch_row_norms(const float*, uint32_t, uint32_t, int, float*, hipStream_t);
 hipError_t pk_launch_sort_rows(const float*, uint32_t, uint32_t, uint64_t, uint32_t, float*, uint32_t*, uint8_t*,
-                               hipStream_t);
+                               uint32_t, hipStream_t);
 hipError_t pk_launch_exact_center_order(const float*, uint32_t, const float*, uint32_t, uint32_t, uint32_t,
                                         const uint32_t*, const uint32_t*, uint32_t, float*, uint32_t*, float*,
                                         uint32_t*, hipStream_t);
@@ -32,7 +32,9 @@ uint32_t pk_probe_trained_count(uint32_t, uint32_t);
 uint32_t pk_probe_param_count(uint32_t, uint32_t);
 size_t pk_probe_work_bytes(uint32_t, uint32_t, uint32_t);
 hipError_t pk_launch_probe_raw_features(const float*, uint32_t, const float*, uint32_t, uint32_t, const float*,
-                                        uint32_t, uint32_t, uint32_t, float*, hipStream_t);
+                                        uint32_t, uint32_t, float*, hipStream_t);
+hipError_t pk_launch_probe_cell_row(const float*, const uint32_t*, uint32_t, const float*, const float*, uint32_t,
+                                    uint32_t, float*, hipStream_t);
 hipError_t pk_probe_pass(const float*, const uint16_t*, uint32_t, uint32_t, uint32_t, const float*, const float*,
                          void*, float*, float*, float*, hipStream_t);
 hipError_t pk_probe_adam_step(float*, float*, float*, const float*, uint16_t*, uint32_t, uint32_t, float, float,
@@ -40,6 +42,8 @@ hipError_t pk_probe_adam_step(float*, float*, float*, const float*, uint16_t*, u
 hipError_t pk_launch_probe_budget(const float*, uint32_t, float, uint32_t, float, uint32_t, uint32_t*,
                                   hipStream_t);
 hipError_t pk_launch_to_bf16(const float*, uint32_t, uint16_t*, hipStream_t);
+hipError_t pk_launch_budget_rank(const float*, const uint32_t*, uint32_t, uint32_t, const float*, const float*,
+                                 uint32_t, uint32_t, const uint32_t*, uint32_t, uint32_t, uint32_t*, hipStream_t);
 hipError_t pk_launch_assign_cells(const float*, const float*, const float*, const float*, const float*,
                                   const float*, uint32_t, uint32_t, uint32_t, int, const uint32_t*, uint32_t*,
                                   float*, hipStream_t);
@@ -88,6 +92,7 @@ struct PkWalkArgs {
     const uint32_t* q_scc;
     const uint32_t* q_nf;
     uint32_t cstride;
+    const float* fdist;
 };
 struct PkRerankArgs {
     const float* q;
@@ -108,6 +113,8 @@ struct PkRerankArgs {
     const uint8_t* tie_n;
     const uint32_t* q_topk;
     uint32_t out_stride;
+    const uint32_t* q_nf;
+    uint32_t F;
 };
 
 namespace {
@@ -199,7 +206,7 @@ void* scratch(puckgpu_ctx* ctx, size_t slot, size_t bytes, int* rc) {
 enum Slot {
     S_Q = 0, S_QN, S_QST, S_CD, S_FD, S_CSD, S_CSI, S_FSD, S_FSI, S_FT, S_FI, S_FN, S_OD, S_OI, S_ON, S_STATS,
     S_IN_Q, S_A0, S_A1, S_A2, S_A3, S_A4, S_A5, S_A6, S_A7, S_FLAGS, S_QLIST, S_TIEID, S_TIEN, S_QTOPK, S_QSCC,
-    S_QNF, S_PFEAT, S_PWORK, S_PNF, S_PY, S_PT, S_COUNT
+    S_QNF, S_PFEAT, S_PWORK, S_PNF, S_PY, S_PT, S_WL, S_PROW, S_PCV, S_PCI, S_COUNT
 };
 
 // Device view of an input array: the pointer itself, or a scratch copy of it.
@@ -435,6 +442,25 @@ namespace {
 // Batched search.  `topk` is the largest per-query top-k (the rerank
 // capacity) and `ostride` the output row stride; q_topk / q_scc / q_nf are
 // optional per-query arrays (host or device as device_ptrs says).
+// Probe features of m prepared queries: cell-distance row of the nearest
+// coarse list, its nfine smallest values, then the raw feature rows.
+int probe_features_chunk(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_probe* p, uint32_t m,
+                         const float* q, const float* csd, const uint32_t* csi, uint32_t cstride, const float* fdist,
+                         float* feat, hipStream_t s) {
+    const uint32_t F = ix->p.fine_cluster_count;
+    int rc = 0;
+    float* row = (float*)scratch(ctx, S_PROW, sizeof(float) * (size_t)m * F, &rc);
+    float* cv = (float*)scratch(ctx, S_PCV, sizeof(float) * (size_t)m * (p->nfine ? p->nfine : 1), &rc);
+    uint32_t* ci = (uint32_t*)scratch(ctx, S_PCI, sizeof(uint32_t) * (size_t)m * (p->nfine ? p->nfine : 1), &rc);
+    if (rc) return rc;
+    if (p->nfine) {
+        PK_CHECK(pk_launch_probe_cell_row(csd, csi, cstride, fdist, ix->stationary, F, m, row, s));
+        PK_CHECK(pk_launch_sort_rows(row, m, F, F, p->nfine, cv, ci, nullptr, 0, s));
+    }
+    PK_CHECK(pk_launch_probe_raw_features(q, ix->p.dim, csd, cstride, p->nc, cv, p->nfine, m, feat, s));
+    return PUCKGPU_OK;
+}
+
 struct ProbeUse {
     const puckgpu_probe* probe = nullptr;
     puckgpu_budget_params bp{};
@@ -460,7 +486,8 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
         return set_err(ctx, PUCKGPU_EINVAL, "probe shape (dim %u, %u coarse, %u fine features) does not fit the index",
                        pr->dim, pr->nc, pr->nfine);
     // sorted coarse entries kept per query: the walk's scc, or the probe's features
-    const uint32_t keep = pr && pr->nc > scc ? pr->nc : scc;
+    uint32_t keep = pr && pr->nc > scc ? pr->nc : scc;
+    if (pr && keep == 0) keep = 1;  // the probe's cell features need the nearest coarse list
     hipStream_t s = ctx->stream;
     int rc = 0;
     const uint32_t QB = 16384;  // queries per device pass
@@ -523,12 +550,12 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
         PK_CHECK(pk_launch_gemm_exact(q, ix->coarse, cdist, m, C, d, C, PK_EPI_SUB_FROM_BIAS, ix->coarse_norms, s));
         PK_CHECK(pk_launch_gemm_exact(q, ix->fine, fdist, m, F, d, F, PK_EPI_NEG, nullptr, s));
         PK_CHECK(hipEventRecord(ctx->ev[1], s));
-        if (keep) PK_CHECK(pk_launch_sort_rows(cdist, m, C, C, keep, csd, csi, flags, s));
-        PK_CHECK(pk_launch_sort_rows(fdist, m, F, F, F, fsd, fsi, nullptr, s));
+        if (keep) PK_CHECK(pk_launch_sort_rows(cdist, m, C, C, keep, csd, csi, flags, scc, s));
+        PK_CHECK(pk_launch_sort_rows(fdist, m, F, F, F, fsd, fsi, nullptr, 0, s));
         const uint32_t* nf_chunk = q_nf ? q_nf + q0 : nullptr;
         if (pr) {
             // learned fine budget: features -> MLP -> budget, on this stream
-            PK_CHECK(pk_launch_probe_raw_features(q, d, csd, keep, pr->nc, fsd, F, pr->nfine, m, pfeat, s));
+            if ((rc = probe_features_chunk(ctx, ix, pr, m, q, csd, csi, keep, fdist, pfeat, s))) return rc;
             float* yb = (float*)scratch(ctx, S_PY, sizeof(float) * nb, &rc);
             if (!yb) return rc;
             PK_CHECK(pk_probe_pass(pr->params, pr->wbf, pr->D, pr->H, m, pfeat, nullptr, pwork, nullptr, nullptr, yb,
@@ -550,6 +577,7 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
         wa.out_n = fn + q0; wa.stats = stats; wa.qlist = nullptr; wa.flags = flags; wa.exact = 0;
         wa.tie_ids = tie_ids; wa.tie_n = tie_n;
         wa.q_scc = q_scc ? q_scc + q0 : nullptr; wa.q_nf = nf_chunk; wa.cstride = keep ? keep : 1;
+        wa.fdist = fdist;
         PK_CHECK(pk_launch_walk(&wa, m, s));
         PK_CHECK(hipEventRecord(ctx->ev[3], s));
         PkRerankArgs ra;
@@ -558,6 +586,7 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
         ra.out_d = od + (size_t)q0 * ostride; ra.out_id = oi + (size_t)q0 * ostride; ra.out_n = on + q0;
         ra.qlist = nullptr; ra.flags = flags; ra.exact = 0; ra.tie_ids = tie_ids; ra.tie_n = tie_n;
         ra.q_topk = q_topk ? q_topk + q0 : nullptr; ra.out_stride = ostride;
+        ra.q_nf = nf_chunk; ra.F = F;
         PK_CHECK(pk_launch_rerank(&ra, m, s));
         PK_CHECK(hipEventRecord(ctx->ev[4], s));
         PK_CHECK(hipMemcpyAsync(hstats.data(), stats, sizeof(uint32_t) * 4 * m, hipMemcpyDeviceToHost, s));
@@ -652,6 +681,44 @@ int puckgpu_search_learned(puckgpu_ctx* ctx, const puckgpu_index* ix, const puck
                        result_num, device_ptrs, &pu);
 }
 
+int puckgpu_budget_ranks(puckgpu_ctx* ctx, const puckgpu_index* ix, uint32_t search_coarse_count, uint32_t n,
+                         const float* queries, uint32_t k, const uint32_t* cells, uint32_t* ranks, int device_ptrs) {
+    if (!ctx || !ix || (n && (!queries || !cells || !ranks))) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    if (n == 0 || k == 0) return PUCKGPU_OK;
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    const uint32_t d = ix->p.dim, C = ix->p.coarse_cluster_count, F = ix->p.fine_cluster_count;
+    const uint32_t scc = search_coarse_count < C ? search_coarse_count : C;
+    hipStream_t s = ctx->stream;
+    int rc = 0;
+    const uint32_t QB = 16384;
+    const uint32_t nb = n < QB ? n : QB;
+    const uint32_t keep = scc ? scc : 1;
+    float* q = (float*)scratch(ctx, S_Q, sizeof(float) * nb * d, &rc);
+    float* qn = (float*)scratch(ctx, S_QN, sizeof(float) * nb, &rc);
+    int32_t* qst = (int32_t*)scratch(ctx, S_QST, sizeof(int32_t) * nb, &rc);
+    float* cdist = (float*)scratch(ctx, S_CD, sizeof(float) * nb * C, &rc);
+    float* fdist = (float*)scratch(ctx, S_FD, sizeof(float) * nb * F, &rc);
+    float* csd = (float*)scratch(ctx, S_CSD, sizeof(float) * nb * keep, &rc);
+    uint32_t* csi = (uint32_t*)scratch(ctx, S_CSI, sizeof(uint32_t) * nb * keep, &rc);
+    if (rc) return rc;
+    const float* qin = (const float*)dev_in(ctx, S_IN_Q, queries, sizeof(float) * (size_t)n * d, device_ptrs, &rc);
+    const uint32_t* cin = (const uint32_t*)dev_in(ctx, S_A0, cells, sizeof(uint32_t) * (size_t)n * k, device_ptrs, &rc);
+    uint32_t* rk = device_ptrs ? ranks : (uint32_t*)scratch(ctx, S_A1, sizeof(uint32_t) * (size_t)n * k, &rc);
+    if (rc) return rc;
+    for (uint32_t q0 = 0; q0 < n; q0 += QB) {
+        const uint32_t m = (n - q0) < QB ? (n - q0) : QB;
+        PK_CHECK(pk_launch_prep_queries(qin + (size_t)q0 * d, m, d, ix->p.whether_norm, q, qn, qst, s));
+        PK_CHECK(pk_launch_gemm_exact(q, ix->coarse, cdist, m, C, d, C, PK_EPI_SUB_FROM_BIAS, ix->coarse_norms, s));
+        PK_CHECK(pk_launch_gemm_exact(q, ix->fine, fdist, m, F, d, F, PK_EPI_NEG, nullptr, s));
+        if (scc) PK_CHECK(pk_launch_sort_rows(cdist, m, C, C, scc, csd, csi, nullptr, 0, s));
+        PK_CHECK(pk_launch_budget_rank(csd, csi, keep, scc, fdist, ix->stationary, C, F, cin + (size_t)q0 * k, k, m,
+                                       rk + (size_t)q0 * k, s));
+    }
+    if (!device_ptrs) PK_CHECK(hipMemcpyAsync(ranks, rk, sizeof(uint32_t) * (size_t)n * k, hipMemcpyDeviceToHost, s));
+    PK_CHECK(hipStreamSynchronize(s));
+    return PUCKGPU_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Learned probe
 int puckgpu_probe_create(puckgpu_ctx* ctx, uint32_t dim, const puckgpu_probe_shape* shape, puckgpu_probe** out) {
@@ -730,7 +797,7 @@ int puckgpu_probe_features(puckgpu_ctx* ctx, const puckgpu_index* ix, const puck
     int rc = 0;
     const uint32_t QB = 16384;
     const uint32_t nb = n < QB ? n : QB;
-    const uint32_t keep = p->nc ? p->nc : 1;
+    const uint32_t keep = p->nc ? p->nc : 1;  // the nearest coarse list is always needed
     float* q = (float*)scratch(ctx, S_Q, sizeof(float) * nb * d, &rc);
     float* qn = (float*)scratch(ctx, S_QN, sizeof(float) * nb, &rc);
     int32_t* qst = (int32_t*)scratch(ctx, S_QST, sizeof(int32_t) * nb, &rc);
@@ -749,10 +816,10 @@ int puckgpu_probe_features(puckgpu_ctx* ctx, const puckgpu_index* ix, const puck
         PK_CHECK(pk_launch_prep_queries(qin + (size_t)q0 * d, m, d, ix->p.whether_norm, q, qn, qst, s));
         PK_CHECK(pk_launch_gemm_exact(q, ix->coarse, cdist, m, C, d, C, PK_EPI_SUB_FROM_BIAS, ix->coarse_norms, s));
         PK_CHECK(pk_launch_gemm_exact(q, ix->fine, fdist, m, F, d, F, PK_EPI_NEG, nullptr, s));
-        if (p->nc) PK_CHECK(pk_launch_sort_rows(cdist, m, C, C, p->nc, csd, csi, nullptr, s));
-        PK_CHECK(pk_launch_sort_rows(fdist, m, F, F, F, fsd, fsi, nullptr, s));
+        PK_CHECK(pk_launch_sort_rows(cdist, m, C, C, keep, csd, csi, nullptr, 0, s));
+        PK_CHECK(pk_launch_sort_rows(fdist, m, F, F, F, fsd, fsi, nullptr, 0, s));
         float* fo = device_ptrs ? feat + (size_t)q0 * p->D : feat;
-        PK_CHECK(pk_launch_probe_raw_features(q, d, csd, keep, p->nc, fsd, F, p->nfine, m, fo, s));
+        if ((rc = probe_features_chunk(ctx, ix, p, m, q, csd, csi, keep, fdist, fo, s))) return rc;
         if (!device_ptrs)
             PK_CHECK(hipMemcpyAsync(features + (size_t)q0 * p->D, fo, sizeof(float) * m * p->D,
                                     hipMemcpyDeviceToHost, s));

@@ -1,7 +1,7 @@
 // probe.hip -- the learned probe: a small MLP that predicts, per query, how
-// many fine clusters (a prefix of the query's ascending fine-distance order)
-// the cell walk needs to reach its top-k.  No counterpart in the reference;
-// it feeds the fine budget of the batched search.
+// many fine clusters (cells of a coarse list, nearest first by cell
+// distance) the cell walk needs to reach its top-k.  No counterpart in the
+// reference; it feeds the fine budget of the batched search.
 //
 // Network: x[D] -> Z1 = W1 x + b1 -> H1 = GELU(Z1) -> Z2 = W2 H1 + b2 ->
 // H2 = GELU(Z2) -> y = w3 . H2 + b3, GELU(z) = z/2 (1 + erf(z/sqrt 2)).
@@ -10,7 +10,9 @@
 // accumulated in FP32 on the matrix cores (v_mfma_f32_32x32x8_bf16).
 //
 // Features (D = dim + nc + nf): the normalised query, its nc smallest coarse
-// distances |c|^2/2 - <c,q> and its nf smallest fine distances -<f,q>, each
+// distances |c|^2/2 - <c,q> and the nf smallest cell distances
+// (|q-c-f|^2 - |q|^2)/2 = (cd + stationary) + fine distance of its nearest
+// coarse list (the fine clusters of that list the walk scores first), each
 // standardised with the probe's (mu, 1/sigma).
 #include "common.h"
 
@@ -128,9 +130,23 @@ static hipError_t pk_mm(bool at, bool bt, const PkMmArgs& a, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// Raw features of n queries from the search pipeline's sorted center rows.
+// Cell distances of every fine cluster of the query's nearest coarse list:
+// row[k] = (cd0 + stationary[c0][k]) + fine distance of k (fdist unsorted).
+__global__ void pk_probe_cell_row(const float* __restrict__ csd, const uint32_t* __restrict__ csi, uint32_t cstride,
+                                  const float* __restrict__ fdist, const float* __restrict__ stationary, uint32_t F,
+                                  uint32_t n, float* __restrict__ row) {
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (uint64_t)n * F) return;
+    const uint32_t i = (uint32_t)(e / F), k = (uint32_t)(e % F);
+    const uint32_t c0 = csi[(uint64_t)i * cstride];
+    row[e] = (csd[(uint64_t)i * cstride] + stationary[(uint64_t)c0 * F + k]) + fdist[e];
+}
+
+// Raw features of n queries from the search pipeline's sorted rows: the
+// query, the nc smallest coarse distances (csd, stride cstride) and the nfine
+// smallest cell distances of the nearest list (cell, stride nfine).
 __global__ void pk_probe_raw_features(const float* __restrict__ q, uint32_t dim, const float* __restrict__ csd,
-                                      uint32_t cstride, uint32_t nc, const float* __restrict__ fsd, uint32_t F,
+                                      uint32_t cstride, uint32_t nc, const float* __restrict__ cell,
                                       uint32_t nfine, uint32_t n, float* __restrict__ out) {
     const uint32_t D = dim + nc + nfine;
     const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -139,7 +155,7 @@ __global__ void pk_probe_raw_features(const float* __restrict__ q, uint32_t dim,
     float v;
     if (j < dim) v = q[(uint64_t)i * dim + j];
     else if (j < dim + nc) v = csd[(uint64_t)i * cstride + (j - dim)];
-    else v = fsd[(uint64_t)i * F + (j - dim - nc)];
+    else v = cell[(uint64_t)i * nfine + (j - dim - nc)];
     out[e] = v;
 }
 
@@ -325,13 +341,21 @@ static PkProbeWork pk_probe_carve(void* base, uint32_t n, uint32_t D, uint32_t H
     return w;
 }
 
+hipError_t pk_launch_probe_cell_row(const float* csd, const uint32_t* csi, uint32_t cstride, const float* fdist,
+                                    const float* stationary, uint32_t F, uint32_t n, float* row, hipStream_t s) {
+    const uint64_t tot = (uint64_t)n * F;
+    if (tot == 0) return hipSuccess;
+    hipLaunchKernelGGL(pk_probe_cell_row, dim3((uint32_t)((tot + 255) / 256)), dim3(256), 0, s, csd, csi, cstride,
+                       fdist, stationary, F, n, row);
+    return hipGetLastError();
+}
+
 hipError_t pk_launch_probe_raw_features(const float* q, uint32_t dim, const float* csd, uint32_t cstride, uint32_t nc,
-                                        const float* fsd, uint32_t F, uint32_t nfine, uint32_t n, float* out,
-                                        hipStream_t s) {
+                                        const float* cell, uint32_t nfine, uint32_t n, float* out, hipStream_t s) {
     const uint64_t tot = (uint64_t)n * (dim + nc + nfine);
     if (tot == 0) return hipSuccess;
     hipLaunchKernelGGL(pk_probe_raw_features, dim3((uint32_t)((tot + 255) / 256)), dim3(256), 0, s, q, dim, csd,
-                       cstride, nc, fsd, F, nfine, n, out);
+                       cstride, nc, cell, nfine, n, out);
     return hipGetLastError();
 }
 

@@ -61,8 +61,9 @@ struct PkWalkArgs {
     uint32_t* tie_ids;       // nq x PK_TIE_MAX: memory ids dropped at a tie with the final top
     uint8_t* tie_n;          // nq: count, PK_TIE_OVERFLOW when the list overflowed
     const uint32_t* q_scc;   // nq or null: per-query coarse lists walked (clamped to scc)
-    const uint32_t* q_nf;    // nq or null: per-query fine budget (0 = all F)
+    const uint32_t* q_nf;    // nq or null: per-query fine budget (0 or >= F: all cells)
     uint32_t cstride;        // csel row stride (>= scc)
+    const float* fdist;      // nq x F unsorted fine distances -<f,q> (budget scan)
 };
 
 #define PK_TIE_MAX 16
@@ -219,6 +220,36 @@ __device__ __forceinline__ uint32_t pk_bmax(uint32_t v, uint32_t* red) {
     uint32_t r = red[1];
     __syncthreads();
     return r;
+}
+
+// Key of rank `want` (1-based) among key(0..n-1): MSB-first radix select
+// with 8-bit digits, keys recomputed on every pass.  blockDim.x == 256 (one
+// histogram bin per thread); hist has 258 words.  Every thread returns it.
+template <typename KeyF>
+__device__ uint32_t pk_select_rank(KeyF key, uint32_t n, uint32_t want, uint32_t* hist, uint32_t* tmp) {
+    uint32_t prefix = 0, mask = 0;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        hist[threadIdx.x] = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+            const uint32_t k = key(i);
+            if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        const uint32_t c = hist[threadIdx.x];
+        uint32_t tot;
+        const uint32_t before = pk_block_exscan(c, tmp, &tot);
+        if (c > 0 && before < want && want <= before + c) {
+            hist[256] = threadIdx.x;
+            hist[257] = want - before;
+        }
+        __syncthreads();
+        prefix |= hist[256] << shift;
+        mask |= 255u << shift;
+        want = hist[257];
+        __syncthreads();
+    }
+    return prefix;
 }
 
 // Values of local points [p_lo, p_hi) of the chunk's active cells; keeps those
@@ -477,6 +508,169 @@ __device__ uint32_t pk_walk_merge(const PkWalkArgs& a, PkWalkSmem& s, uint32_t n
     return nn < ftk ? nn : ftk;
 }
 
+// k-th smallest (1-based) of the 64-bit keys v[0..n) in LDS: MSB-first radix
+// select, 8-bit digits.  blockDim.x == 256; hist has 258 words.
+__device__ uint64_t pk_select_rank64(const uint64_t* v, uint32_t n, uint32_t want, uint32_t* hist, uint32_t* tmp) {
+    uint64_t prefix = 0, mask = 0;
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        hist[threadIdx.x] = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+            const uint64_t k = v[i];
+            if ((k & mask) == prefix) atomicAdd(&hist[(uint32_t)(k >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        const uint32_t c = hist[threadIdx.x];
+        uint32_t tot;
+        const uint32_t before = pk_block_exscan(c, tmp, &tot);
+        if (c > 0 && before < want && want <= before + c) {
+            hist[256] = threadIdx.x;
+            hist[257] = want - before;
+        }
+        __syncthreads();
+        prefix |= (uint64_t)hist[256] << shift;
+        mask |= 255ull << shift;
+        want = hist[257];
+        __syncthreads();
+    }
+    return prefix;
+}
+
+// Keeps the keys <= K of v[0..n) (in place, order kept); returns the count.
+__device__ uint32_t pk_keep_le(uint64_t* v, uint32_t n, uint64_t K, uint32_t* tmp) {
+    uint32_t out = 0;
+    for (uint32_t r0 = 0; r0 < n; r0 += blockDim.x) {
+        const uint32_t i = r0 + threadIdx.x;
+        const uint64_t x = i < n ? v[i] : ~0ull;
+        const uint32_t keep = (i < n && x <= K) ? 1u : 0u;
+        uint32_t tot;
+        const uint32_t pre = pk_block_exscan(keep, tmp, &tot);  // all reads of this round precede the writes
+        if (keep) v[out + pre] = x;
+        out += tot;
+        __syncthreads();
+    }
+    return out;
+}
+
+// Budget scan of one query (fine budget b < F; no counterpart in the
+// reference).  In each walked coarse list the cells whose cell distance
+// (cd + stationary) + fine distance is at most the b-th smallest of the list
+// are scored in full: every point's filter distance is computed as the
+// reference's compute_quantized_distance computes it, and the filter set is
+// the ftk smallest (distance, memory id) keys, kept in an unsorted LDS buffer
+// (the walk's S_key + W_key region) that is cut back to ftk by a radix select
+// whenever it could overflow.  The result does not depend on any visiting
+// order, so it needs no replay.
+template <int FTKM>
+__device__ void pk_scan_budget(const PkWalkArgs& a, PkWalkSmem& s, uint32_t qi, uint32_t b) {
+    const uint32_t F = a.F, ftk = a.ftk, nsq = a.nsq;
+    const int tid = threadIdx.x;
+    uint64_t* B = s.S_key;  // S_key and W_key are contiguous
+    const uint32_t cap = ((ftk + 1) & ~1u) + WALK_PMAX;
+    uint32_t* hist = s.P_id;  // 512 words, unused by this path
+    uint32_t n = 0;
+    uint64_t K = ~0ull;  // keys above K cannot enter the filter set
+    uint32_t scc = a.scc;
+    if (a.q_scc) scc = a.q_scc[qi] < scc ? a.q_scc[qi] : scc;
+    const float* fdq = a.fdist + (uint64_t)qi * F;
+    uint32_t st_cells = 0, st_pts = 0;
+    for (uint32_t l = 0; l < scc; ++l) {
+        const uint32_t c = a.csel_id[(uint64_t)qi * a.cstride + l];
+        const float cd = a.csel_d[(uint64_t)qi * a.cstride + l];
+        const float* stc = a.stationary + (uint64_t)c * F;
+        const uint32_t* csc = a.cell_start + (uint64_t)c * F;
+        auto key = [&](uint32_t k) { return pk_f2ord((cd + stc[k]) + fdq[k]); };
+        const uint32_t kcut = pk_select_rank(key, F, b, hist, s.red + 4);
+        for (uint32_t w0 = 0; w0 < F; w0 += blockDim.x) {
+            const uint32_t k = w0 + tid;
+            uint32_t in = 0, sz = 0, cb = 0;
+            float cdist = 0.f;
+            if (k < F) {
+                const float cv = (cd + stc[k]) + fdq[k];
+                if (pk_f2ord(cv) <= kcut) {
+                    cb = csc[k];
+                    sz = csc[k + 1] - cb;
+                    in = sz > 0 ? 1u : 0u;
+                    cdist = cv;
+                }
+            }
+            uint32_t ne, tot;
+            const uint32_t pos = pk_block_exscan(in, s.red + 4, &ne);
+            const uint32_t ppre = pk_block_exscan(in ? sz : 0u, s.red + 4, &tot);
+            if (in) {
+                s.c_b[pos] = cb;
+                s.c_fd[pos] = cdist;
+                s.c_pre[pos] = ppre;
+            }
+            if (tid == 0) s.c_pre[ne] = tot;
+            __syncthreads();
+            st_cells += ne;
+            st_pts += tot;
+            for (uint32_t p0 = 0; p0 < tot; p0 += blockDim.x) {
+                if (n + blockDim.x > cap) {
+                    K = pk_select_rank64(B, n, ftk, hist, s.red + 4);
+                    n = pk_keep_le(B, n, K, s.red + 4);
+                }
+                if (tid == 0) s.red[2] = n;
+                __syncthreads();
+                const uint32_t p = p0 + tid;
+                if (p < tot) {
+                    uint32_t lo = 0, hi = ne;  // c_pre[lo] <= p < c_pre[hi]
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (s.c_pre[mid] <= p) lo = mid; else hi = mid;
+                    }
+                    const uint32_t mid = s.c_b[lo] + (p - s.c_pre[lo]);
+                    float tv = 2.0f * s.c_fd[lo] + a.rec_off[mid];  // 2x is exact: one rounding, as in double
+                    const uint8_t* codes = a.rec_codes + (uint64_t)mid * nsq;
+                    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+                    if ((nsq & 15) == 0) {
+                        for (uint32_t g = 0; g < nsq; g += 16) {
+                            uint4 w = *(const uint4*)(codes + g);
+                            uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                            for (int h = 0; h < 4; ++h) {
+#pragma unroll
+                                for (int j = 0; j < 4; ++j) {
+                                    uint32_t code = (ws[h] >> (8 * j)) & 0xffu;
+                                    acc[j] = acc[j] + s.lut[(g + 4 * h + j) * PK_KS + code];
+                                }
+                            }
+                        }
+                    } else {
+                        for (uint32_t m = 0; m < nsq; ++m) acc[m & 3] = acc[m & 3] + s.lut[m * PK_KS + codes[m]];
+                    }
+                    tv = tv + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
+                    const uint64_t kk = ((uint64_t)pk_f2ord(tv) << 32) | mid;
+                    if (kk < K) B[atomicAdd(&s.red[2], 1u)] = kk;
+                }
+                __syncthreads();
+                n = s.red[2];
+                __syncthreads();
+            }
+        }
+    }
+    if (n > ftk) {
+        K = pk_select_rank64(B, n, ftk, hist, s.red + 4);
+        n = pk_keep_le(B, n, K, s.red + 4);
+    }
+    for (uint32_t r = tid; r < n; r += blockDim.x) {
+        a.out_t[(uint64_t)qi * ftk + r] = pk_ord2f((uint32_t)(B[r] >> 32));
+        a.out_id[(uint64_t)qi * ftk + r] = (uint32_t)B[r];
+    }
+    if (tid == 0) {
+        a.out_n[qi] = n;
+        if (a.flags) a.flags[qi] = 0;  // deterministic: nothing to replay
+        if (a.tie_n) a.tie_n[qi] = 0;
+        if (a.stats) {
+            a.stats[qi * 4 + 0] = 0;
+            a.stats[qi * 4 + 1] = 0;
+            a.stats[qi * 4 + 2] = st_cells;
+            a.stats[qi * 4 + 3] = st_pts;
+        }
+    }
+}
+
 __device__ __forceinline__ float pk_walk_top(const PkWalkSmem& s, uint32_t ns, uint32_t ftk) {
     return ns >= ftk ? pk_ord2f((uint32_t)(s.S_key[ftk - 1] >> 32)) : pk_sentinel();
 }
@@ -512,6 +706,7 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
     s.td_val = (uint32_t*)ptr; ptr += sizeof(uint32_t) * PK_TIE_MAX;
     s.td_id = (uint32_t*)ptr; ptr += sizeof(uint32_t) * PK_TIE_MAX;
 
+    if (a.q_nf && a.q_nf[qi] != 0u && a.q_nf[qi] < F) return;  // pk_scan_budget_kernel's query
     uint32_t st_seg = 0, st_unc = 0, st_cells = 0, st_pts = 0;
     bool tie_dec = false;
     uint32_t tie_sub = 0;
@@ -543,11 +738,6 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
     const uint32_t scs = a.cstride;
     uint32_t scc = a.scc;
     if (a.q_scc) scc = a.q_scc[qi] < scc ? a.q_scc[qi] : scc;
-    // fine budget: only the first Fq entries of the sorted fine list are
-    // considered in each coarse list (Fq = F is the reference's walk)
-    uint32_t Fq = F;
-    if (a.q_nf && a.q_nf[qi] != 0u && a.q_nf[qi] < F) Fq = a.q_nf[qi];
-    if (Fq < F && fsd[Fq - 1] == fsd[Fq]) tie_dec = true;  // the cut splits a tie group
 
     for (uint32_t l = 0; l < scc; ++l) {
         const uint32_t c = a.csel_id[(uint64_t)qi * scs + l];
@@ -559,9 +749,9 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
         const uint32_t* csc = a.cell_start + (uint64_t)c * F;
         uint32_t j0 = 0;
         bool done = false;
-        while (!done && j0 < Fq) {
+        while (!done && j0 < F) {
             const uint32_t j = j0 + tid;
-            const bool valid = j < Fq;
+            const bool valid = j < F;
             float fdj = 0.f, stj = 0.f;
             uint32_t bj = 0, nj = 0;
             if (valid) {
@@ -585,7 +775,7 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
             uint32_t cend = cutm < bhi ? cutm : bhi;
             // do not split a fine-distance tie group across chunks: end the
             // chunk at the group's first cell when the group starts inside it
-            if (j0 + cend < Fq && cend > 1 && fsd[j0 + cend] == fsd[j0 + cend - 1]) {
+            if (j0 + cend < F && cend > 1 && fsd[j0 + cend] == fsd[j0 + cend - 1]) {
                 uint32_t g = cend - 1;
                 while (g > 0 && fsd[j0 + g - 1] == fsd[j0 + g]) --g;
                 if (g > 0) cend = g;
@@ -683,7 +873,7 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
                     const bool back_tie = u > 0 ? s.c_fd[u - 1] == fu : (j0 > 0 && fdprev0 == fu);
                     uint32_t gend = u;
                     while (gend + 1 < cend && s.c_fd[gend + 1] == fu) ++gend;
-                    const bool fwd_tie = gend + 1 == cend && j0 + cend < Fq && fsd[j0 + cend] == fu;
+                    const bool fwd_tie = gend + 1 == cend && j0 + cend < F && fsd[j0 + cend] == fu;
                     const uint32_t gs = gend - u + 1;
                     if (back_tie) tie_sub |= 0x10u;
                     if (fwd_tie) tie_sub |= 0x20u;
@@ -810,6 +1000,47 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
     }
 }
 
+// The budget scan as a kernel of its own (its register needs differ from the
+// walk's): one block per query, returning at once for queries without a
+// budget below F.  Same LDS layout as pk_walk.
+template <int FTKM>
+__global__ __launch_bounds__(WALK_THREADS) void pk_scan_budget_kernel(PkWalkArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
+    const uint32_t ftk = a.ftk, F = a.F, nsq = a.nsq;
+    if (!(a.q_nf && a.q_nf[qi] != 0u && a.q_nf[qi] < F)) return;
+    PkWalkSmem s;
+    unsigned char* ptr = smem;
+    s.S_key = (uint64_t*)ptr; ptr += sizeof(uint64_t) * ((ftk + 1) & ~1u);
+    s.W_key = (uint64_t*)ptr; ptr += sizeof(uint64_t) * WALK_PMAX;
+    s.S_id = (uint32_t*)ptr; ptr += sizeof(uint32_t) * ((ftk + 3) & ~3u);
+    s.P_id = (uint32_t*)ptr; ptr += sizeof(uint32_t) * WALK_PMAX;
+    s.c_b = (uint32_t*)ptr; ptr += sizeof(uint32_t) * WALK_CH;
+    s.c_pre = (uint32_t*)ptr; ptr += sizeof(uint32_t) * (WALK_CH + 4);
+    s.red = (uint32_t*)ptr; ptr += sizeof(uint32_t) * 16;
+    s.c_fd = (float*)ptr; ptr += sizeof(float) * WALK_CH;
+    s.c_st = (float*)ptr; ptr += sizeof(float) * WALK_CH;
+    s.qv = (float*)ptr; ptr += sizeof(float) * ((a.dim + 3) & ~3u);
+    s.lut = (float*)ptr;
+    if (a.qstatus[qi] != 0) {
+        if (threadIdx.x == 0) {
+            a.out_n[qi] = 0;
+            if (a.flags) a.flags[qi] = 0;
+            if (a.tie_n) a.tie_n[qi] = 0;
+        }
+        return;
+    }
+    const float* qg = a.q + (uint64_t)qi * a.dim;
+    for (uint32_t i = threadIdx.x; i < a.dim; i += blockDim.x) s.qv[i] = qg[i];
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < nsq * PK_KS; e += blockDim.x) {
+        uint32_t m = e / PK_KS;
+        s.lut[e] = pk_lut_entry(s.qv + m * a.lsq, a.codebook + (uint64_t)e * a.lsq, a.lsq);
+    }
+    __syncthreads();
+    pk_scan_budget<FTKM>(a, s, qi, a.q_nf[qi]);
+}
+
 extern "C" size_t pk_walk_smem_bytes(uint32_t dim, uint32_t nsq, uint32_t ftk, int exact) {
     size_t b = 0;
     b += sizeof(float) * WALK_PMAX + WALK_PMAX + 2 * sizeof(uint32_t) * PK_TIE_MAX + 16;
@@ -834,20 +1065,23 @@ extern "C" uint32_t pk_walk_ftk_max(void) { return WALK_FTK_MAX; }
 extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStream_t s) {
     if (nq == 0) return hipSuccess;
     if (args->ftk == 0 || args->ftk > WALK_FTK_MAX || args->cstride < args->scc) return hipErrorInvalidValue;
+    if (args->q_nf && !args->fdist) return hipErrorInvalidValue;
     size_t smem = pk_walk_smem_bytes(args->dim, args->nsq, args->ftk, args->exact);
     if (smem > 160 * 1024) return hipErrorInvalidValue;
     static bool attr_set = false;
     if (!attr_set) {
-        (void)hipFuncSetAttribute((const void*)pk_walk<1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024);
-        (void)hipFuncSetAttribute((const void*)pk_walk<2048>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024);
+        const void* fns[] = {(const void*)pk_walk<1024>, (const void*)pk_walk<2048>,
+                             (const void*)pk_scan_budget_kernel<1024>, (const void*)pk_scan_budget_kernel<2048>};
+        for (const void* f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr_set = true;
     }
-    if (args->ftk <= 1024)
+    if (args->ftk <= 1024) {
         hipLaunchKernelGGL(pk_walk<1024>, dim3(nq), dim3(WALK_THREADS), smem, s, *args);
-    else
+        if (args->q_nf) hipLaunchKernelGGL(pk_scan_budget_kernel<1024>, dim3(nq), dim3(WALK_THREADS), smem, s, *args);
+    } else {
         hipLaunchKernelGGL(pk_walk<2048>, dim3(nq), dim3(WALK_THREADS), smem, s, *args);
+        if (args->q_nf) hipLaunchKernelGGL(pk_scan_budget_kernel<2048>, dim3(nq), dim3(WALK_THREADS), smem, s, *args);
+    }
     return hipGetLastError();
 }
 
@@ -872,6 +1106,8 @@ struct PkRerankArgs {
     const uint8_t* tie_n;
     const uint32_t* q_topk;  // nq or null: per-query top-k (1..topk; other values give no result)
     uint32_t out_stride;     // row stride of out_d / out_id (>= topk)
+    const uint32_t* q_nf;    // nq or null: fine budgets; a query with 0 < budget < F came from the
+    uint32_t F;              // budget scan and ranks by (distance, local id), without replay
 };
 
 #define RERANK_THREADS 256
@@ -885,6 +1121,8 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
     if (a.q_topk) topk = (a.q_topk[qi] >= 1u && a.q_topk[qi] <= a.topk) ? a.q_topk[qi] : 0u;
     const uint32_t n = (a.qstatus[qi] == 0 && topk > 0) ? a.filt_n[qi] : 0u;
     const uint32_t ftk = a.ftk, ostr = a.out_stride;
+    const bool bud = a.q_nf && a.q_nf[qi] != 0u && a.q_nf[qi] < a.F;
+    const bool exact = a.exact && !bud;
     uint32_t n2 = 2;
     while (n2 < ftk) n2 <<= 1;
     uint64_t* keys = (uint64_t*)smem;
@@ -895,7 +1133,7 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
     __syncthreads();
     // filter-boundary tie group (entries whose filter distance equals the final
     // top, kept or dropped): smallest exact distance among its members
-    const uint32_t tn = (!a.exact && a.tie_n && n >= ftk) ? a.tie_n[qi] : 0u;
+    const uint32_t tn = (!exact && !bud && a.tie_n && n >= ftk) ? a.tie_n[qi] : 0u;
     const float tfin = n >= ftk ? a.filt_t[(uint64_t)qi * ftk + ftk - 1] : 0.f;
     __shared__ uint32_t tie_min;
     if (threadIdx.x == 0) tie_min = tn == PK_TIE_OVERFLOW ? 0u : 0xffffffffu;
@@ -904,10 +1142,10 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
         if (r < n) {
             uint32_t mid = a.filt_id[(uint64_t)qi * ftk + r];
             float d = pk_rerank_dist(a.raw + (uint64_t)mid * a.dim, qv, a.dim);
-            key = ((uint64_t)pk_f2ord(d) << 32) | r;
-            if (a.exact) dist[r] = d;
+            key = ((uint64_t)pk_f2ord(d) << 32) | (bud ? a.mem2local[mid] : r);
+            if (exact) dist[r] = d;
         }
-        if (!a.exact) keys[r] = key;
+        if (!exact) keys[r] = key;
     }
     __syncthreads();
     if (tn > 0 && tn != PK_TIE_OVERFLOW) {
@@ -924,7 +1162,7 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
         }
     }
     __syncthreads();
-    if (a.exact) {
+    if (exact) {
         if (threadIdx.x == 0) {
             for (uint32_t r = 0; r <= topk; ++r) RH[r] = make_uint2(0x7f7f7f7fu, 0xffffffffu);
             uint32_t dc = topk;
@@ -971,8 +1209,8 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
             uint64_t key = keys[r];
             d = pk_ord2f((uint32_t)(key >> 32));
             uint32_t rr = (uint32_t)(key & 0xffffffffu);
-            id = a.mem2local[a.filt_id[(uint64_t)qi * ftk + rr]];
-            if (r + 1 < n && pk_ord2f((uint32_t)(keys[r + 1] >> 32)) == d) tie = true;
+            id = bud ? rr : a.mem2local[a.filt_id[(uint64_t)qi * ftk + rr]];
+            if (!bud && r + 1 < n && pk_ord2f((uint32_t)(keys[r + 1] >> 32)) == d) tie = true;
         }
         a.out_d[(uint64_t)qi * ostr + r] = d;
         a.out_id[(uint64_t)qi * ostr + r] = id;
@@ -1068,5 +1306,57 @@ extern "C" hipError_t pk_launch_exact_center_order(const float* cdist, uint32_t 
     }
     hipLaunchKernelGGL(pk_exact_center_order, dim3(nlist), dim3(256), smem, s, cdist, C, fdist, F, scs, scap,
                        q_scc, qlist, csel_d, csel_id, fsort_d, fsort_id);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Fine-budget rank of given cells (labels of the learned probe).  For query
+// qi and entry e, cell = cells[qi * k + e] (coarse * F + fine, or ~0): when
+// the coarse center is among the query's first scc sorted coarse centers, the
+// smallest fine budget under which the walk of that list scores the cell,
+// 1 + #cells of the list with a smaller cell-distance key; otherwise ~0.
+__global__ __launch_bounds__(256) void pk_budget_rank(const float* __restrict__ csd,
+                                                      const uint32_t* __restrict__ csi, uint32_t cstride,
+                                                      uint32_t scc, const float* __restrict__ fdist,
+                                                      const float* __restrict__ stationary, uint32_t C,
+                                                      uint32_t F, const uint32_t* __restrict__ cells, uint32_t k,
+                                                      uint32_t* __restrict__ ranks) {
+    __shared__ uint32_t cnt;
+    const uint32_t qi = blockIdx.x;
+    const float* fd = fdist + (uint64_t)qi * F;
+    for (uint32_t e = 0; e < k; ++e) {
+        const uint32_t cell = cells[(uint64_t)qi * k + e];
+        uint32_t r = 0xffffffffu;
+        if (cell != 0xffffffffu && cell / F < C) {
+            const uint32_t c = cell / F, f = cell % F;
+            uint32_t l = 0xffffffffu;
+            for (uint32_t i = 0; i < scc; ++i)
+                if (csi[(uint64_t)qi * cstride + i] == c) { l = i; break; }
+            if (l != 0xffffffffu) {
+                const float cd = csd[(uint64_t)qi * cstride + l];
+                const float* st = stationary + (uint64_t)c * F;
+                const uint32_t key = pk_f2ord((cd + st[f]) + fd[f]);
+                if (threadIdx.x == 0) cnt = 0;
+                __syncthreads();
+                uint32_t m = 0;
+                for (uint32_t j = threadIdx.x; j < F; j += blockDim.x)
+                    if (pk_f2ord((cd + st[j]) + fd[j]) < key) ++m;
+                atomicAdd(&cnt, m);
+                __syncthreads();
+                r = cnt + 1;
+                __syncthreads();
+            }
+        }
+        if (threadIdx.x == 0) ranks[(uint64_t)qi * k + e] = r;
+    }
+}
+
+extern "C" hipError_t pk_launch_budget_rank(const float* csd, const uint32_t* csi, uint32_t cstride, uint32_t scc,
+                                            const float* fdist, const float* stationary, uint32_t C, uint32_t F,
+                                            const uint32_t* cells, uint32_t k, uint32_t nq, uint32_t* ranks,
+                                            hipStream_t s) {
+    if (nq == 0 || k == 0) return hipSuccess;
+    hipLaunchKernelGGL(pk_budget_rank, dim3(nq), dim3(256), 0, s, csd, csi, cstride, scc, fdist, stationary, C, F,
+                       cells, k, ranks);
     return hipGetLastError();
 }

@@ -33,7 +33,7 @@ __global__ __launch_bounds__(SORT_THREADS) void pk_sort_rows(const float* __rest
                                                              uint64_t ld_in, uint32_t keep,
                                                              float* __restrict__ out_val,
                                                              uint32_t* __restrict__ out_idx, uint32_t n2,
-                                                             uint8_t* __restrict__ flags) {
+                                                             uint8_t* __restrict__ flags, uint32_t flag_lim) {
     extern __shared__ uint64_t sk[];
     const uint64_t row = blockIdx.x;
     const float* v = vals + row * ld_in;
@@ -47,16 +47,16 @@ __global__ __launch_bounds__(SORT_THREADS) void pk_sort_rows(const float* __rest
         float v = pk_ord2f((uint32_t)(key >> 32));
         out_val[row * keep + i] = v;
         out_idx[row * keep + i] = (uint32_t)(key & 0xffffffffu);
-        if (i + 1 < L && pk_ord2f((uint32_t)(sk[i + 1] >> 32)) == v) tie = true;
+        if (i < flag_lim && i + 1 < L && pk_ord2f((uint32_t)(sk[i + 1] >> 32)) == v) tie = true;
     }
-    // equal distances inside the kept range (or at its boundary): the
-    // reference's heap order decides, flag the row for the exact replay
+    // equal distances among the first flag_lim entries (or at that boundary):
+    // the reference's heap order decides, flag the row for the exact replay
     if (tie && flags) flags[row] = 1u | flags[row];
 }
 
 extern "C" hipError_t pk_launch_sort_rows(const float* vals, uint32_t rows, uint32_t L, uint64_t ld_in,
                                           uint32_t keep, float* out_val, uint32_t* out_idx, uint8_t* flags,
-                                          hipStream_t s) {
+                                          uint32_t flag_lim, hipStream_t s) {
     if (rows == 0 || keep == 0) return hipSuccess;
     if (L > 8192 || keep > L) return hipErrorInvalidValue;
     static bool attr = false;
@@ -67,6 +67,6 @@ extern "C" hipError_t pk_launch_sort_rows(const float* vals, uint32_t rows, uint
     uint32_t n2 = 2;
     while (n2 < L) n2 <<= 1;
     hipLaunchKernelGGL(pk_sort_rows, dim3(rows), dim3(SORT_THREADS), n2 * sizeof(uint64_t), s, vals, L, ld_in,
-                       keep, out_val, out_idx, n2, flags);
+                       keep, out_val, out_idx, n2, flags, flag_lim);
     return hipGetLastError();
 }

@@ -234,6 +234,20 @@ class GpuIndex:
                                                        _ptr(queries), _ptr(D), _ptr(I), _ptr(R), m))
         return D, I, R
 
+    def budget_ranks(self, queries, cells, search_coarse_count):
+        """Smallest fine budget scoring each given cell (include/puck_gpu.h)."""
+        n, k = cells.shape
+        if _is_dev(queries):
+            import torch
+            r = torch.empty((n, k), dtype=torch.int32, device=queries.device)
+        else:
+            r = np.zeros((n, k), np.uint32)
+            cells = np.ascontiguousarray(cells, np.uint32)
+        self.ctx.sync_torch(queries, cells)
+        self.ctx.check(self.ctx.L.puckgpu_budget_ranks(self.ctx.h, self.h, search_coarse_count, n, _ptr(queries), k,
+                                                       _ptr(cells), _ptr(r), _mode(queries, cells, r)))
+        return r
+
     def last_filter(self, n, filter_topk):
         t = np.zeros((n, filter_topk), np.float32)
         i = np.zeros((n, filter_topk), np.uint32)

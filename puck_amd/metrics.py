@@ -20,7 +20,7 @@ def exact_topk(x, q, k, extra=8, qchunk=4096, bchunk=1 << 21):
         cand_i, cand_d = [], []
         for bs in range(0, N, bchunk):
             xb = x[bs:bs + bchunk]
-            d = xn[bs:bs + bchunk][None, :] - 2.0 * (qq @ xb.T)
+            d = torch.addmm(xn[bs:bs + bchunk][None, :], qq, xb.T, beta=1.0, alpha=-2.0)
             v, i = torch.topk(d, min(kk, xb.shape[0]), dim=1, largest=False)
             cand_i.append(i + bs)
             cand_d.append(v)

@@ -1,15 +1,18 @@
 """Training pipeline of the learned probe (labels, initialisation, Adam
 loop) driving the device kernels through the C ABI.
 
-Label of a query: the fine budget its fixed-budget search needs.  Run the
-plain search (every fine cluster, the reference's walk) at the operating
-search_coarse_count; of the exact top-k neighbours it returns, take the
-largest rank of a neighbour's fine cluster in the query's ascending
-fine-distance order; the label is that rank + 1 (1 when it returns none).
-A budget >= the label walks the same cells up to that point, so the
-neighbours the plain search finds are found again (the filter heap of a
-truncated walk is a prefix state of the full walk's).  The network
-regresses log(label).
+Fine budget b of a query: in each coarse list the walk scores only the b
+cells of smallest cell distance (include/puck_gpu.h, puckgpu_query_params).
+Label of a query: run the plain search (every cell, the reference's walk) at
+the operating search_coarse_count; of the exact top-k neighbours it returns,
+take the largest fine-budget rank of a neighbour's cell (puckgpu_budget_ranks);
+the label is that rank (1 when it returns none).  Under a budget >= the label
+those neighbours' cells are all in the budget; fewer points are offered, so
+the heap top and the pivot never fall below the plain walk's and every cell
+and point the plain walk scored in the budget is scored again, while the
+points with a filter distance below a neighbour's are a subset of the plain
+walk's (its pruning bounds are exact), so the neighbour stays in the filter
+set and in the top-k.  The network regresses log(label).
 """
 import numpy as np
 import torch
@@ -28,26 +31,26 @@ def init_params(D, H, mu, sigma, seed=0):
 
 
 @torch.no_grad()
-def budget_labels(index, queries, fine_order, gt_ids, cell_assign, scc, filter_topk, topk=10, chunk=16384):
+def budget_labels(index, queries, gt_ids, cell_assign, scc, filter_topk, topk=10, chunk=16384):
     """Labels (int64 tensor [n]) as described in the module docstring.
-    queries, fine_order [n, F], gt_ids [n, >=topk] and cell_assign [N] are
-    CUDA tensors; cell_assign holds local-order cell ids (coarse * F + fine)."""
-    n, F = queries.shape[0], index.F
+    queries, gt_ids [n, >=topk] and cell_assign [N] (local-order cell ids,
+    coarse * F + fine) are CUDA tensors."""
+    n = queries.shape[0]
     lab = torch.empty(n, dtype=torch.int64, device=queries.device)
-    ar = torch.arange(F, dtype=torch.int64, device=queries.device)
+    ar = torch.arange(topk, device=queries.device)
     for s in range(0, n, chunk):
         e = min(n, s + chunk)
-        _, I, R = index.search(queries[s:e].contiguous(), topk, scc, filter_topk)
+        qc = queries[s:e].contiguous()
+        _, I, R = index.search(qc, topk, scc, filter_topk)
         I = I.long()
-        valid = ar[None, :topk] < R.long()[:, None]
-        gt = gt_ids[s:e, :topk].long()
-        hit = (I[:, :, None] == gt[:, None, :]).any(-1) & valid
-        fo = fine_order[s:e].long()
-        rank = torch.empty_like(fo)
-        rank.scatter_(1, fo, ar[None, :].expand(e - s, F))
-        fine_of = (cell_assign.long()[I.clamp(min=0, max=cell_assign.shape[0] - 1)] % F)
-        r = torch.gather(rank, 1, fine_of)
-        r = torch.where(hit, r + 1, torch.zeros_like(r))
+        valid = ar[None, :] < R.long()[:, None]
+        hit = (I[:, :, None] == gt_ids[s:e, None, :topk].long()).any(-1) & valid
+        cells = cell_assign.long()[I.clamp(min=0, max=cell_assign.shape[0] - 1)]
+        cells = torch.where(hit, cells, torch.full_like(cells, 0xffffffff)).to(torch.int64)
+        cells32 = (cells & 0xffffffff).to(torch.int64)
+        cells32 = torch.where(cells32 >= 2 ** 31, cells32 - 2 ** 32, cells32).to(torch.int32).contiguous()
+        r = index.budget_ranks(qc, cells32, scc).long() & 0xffffffff
+        r = torch.where(r == 0xffffffff, torch.zeros_like(r), r)
         lab[s:e] = r.max(1).values.clamp(min=1)
     return lab
 
@@ -87,9 +90,8 @@ def fit(ctx, index, train_queries, gt_ids, cell_assign, scc, filter_topk, topk=1
     """Build a Probe for `index` from training queries (CUDA tensors): features
     and labels from the engine, then Adam.  Returns (probe, info)."""
     probe = Probe(ctx, index.dim, nc, nf, hidden)
-    feats, forder = probe.features(index, train_queries, fine_order=True)
-    lab = budget_labels(index, train_queries, forder, gt_ids, cell_assign, scc, filter_topk, topk)
-    del forder
+    feats = probe.features(index, train_queries)
+    lab = budget_labels(index, train_queries, gt_ids, cell_assign, scc, filter_topk, topk)
     target = torch.log(lab.double()).float().contiguous()
     mu = feats.double().mean(0).cpu().numpy()
     sd = feats.double().std(0).cpu().numpy()

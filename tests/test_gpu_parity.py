@@ -144,8 +144,17 @@ def test_search_batch_per_query_bitexact(ctx, which, replay, idx_l2, idx_cos):
     nf = rng.integers(0, F + 4, n).astype(np.uint32)
     nf[:8] = [1, 2, 3, 5, F - 1, F, 0, 4]
     Dg, Ig, Rg = gi.search_batch(q, 100, 20, 1100, topk=topk, scc=scc, fine_budget=nf, replay_all=replay)
+    ft, fm, fn = gi.last_filter(n, 1100)
     Do, Io, Ro = oi.search_batch_pq(q, 100, 20, 1100, topk=topk, scc_arr=scc, fine_budget=nf)
     assert np.array_equal(Rg, Ro)
+    for k in range(n):
+        if not (0 < nf[k] < F) or topk[k] == 0 or topk[k] > 100:
+            continue
+        # budget scan: the filter set (unordered rows) is the oracle's
+        o = oi.search_one(q[k], int(topk[k]), int(min(scc[k], 20)), 1100, max_topk=100, fine_budget=int(nf[k]))
+        assert fn[k] == o["filt_n"], k
+        gv, gid = _sorted_pairs(ft[k, :fn[k]], fm[k, :fn[k]])
+        assert np.array_equal(_bits(gv), _bits(o["filt_val"][:fn[k]])) and np.array_equal(gid, o["filt_mem"][:fn[k]])
     for k in range(n):
         m = int(Ro[k])
         assert np.array_equal(Ig[k, :m], Io[k, :m]), f"query {k}"

@@ -133,7 +133,9 @@ def test_probe_features_match_oracle(ctx, idx_cos):
         cd = np.array([o["coarse_norms"][c] - O.dot(ix["coarse"][c], qn) for c in range(40)], np.float32)
         fd = np.array([-O.dot(ix["fine"][j], qn) for j in range(40)], np.float32)
         assert np.array_equal(f[k, 96:96 + nc].view(np.uint32), np.sort(cd)[:nc].view(np.uint32))
-        assert np.array_equal(f[k, 96 + nc:].view(np.uint32), np.sort(fd)[:nf].view(np.uint32))
+        c0 = int(np.argmin(cd))
+        cell = (cd[c0] + o["stationary"].reshape(40, 40)[c0]) + fd      # float32, reference order
+        assert np.array_equal(f[k, 96 + nc:].view(np.uint32), np.sort(cell)[:nf].view(np.uint32))
         assert np.array_equal(fd[fo[k]], np.sort(fd))
         assert np.array_equal(np.sort(fo[k]), np.arange(40))
 
@@ -168,17 +170,46 @@ def test_search_learned_equals_explicit_budgets(ctx, idx_cos, margin, frac):
                                                                         Do[k, :m].view(np.uint32))
 
 
+def test_budget_ranks_match_oracle_definition(ctx, idx_cos):
+    ix, q = idx_cos
+    gi = _gpu_index(ctx, ix)
+    o = CI.oracle_index(ix)
+    ex = o.export()
+    C, F = 40, 40
+    st = ex["stationary"].reshape(C, F)
+    rng = np.random.default_rng(9)
+    cells = rng.integers(0, C * F, (q.shape[0], 5)).astype(np.uint32)
+    cells[:, 4] = 0xffffffff
+    scc = 6
+    r = gi.budget_ranks(q, cells, scc)
+    for k in range(q.shape[0]):
+        qn = O.normalize(q[k])
+        cd = np.array([ex["coarse_norms"][c] - O.dot(ix["coarse"][c], qn) for c in range(C)], np.float32)
+        fd = np.array([-O.dot(ix["fine"][f], qn) for f in range(F)], np.float32)
+        near = o.search_one(q[k], 10, scc, 100)["coarse"]
+        for e in range(5):
+            cell = int(cells[k, e])
+            if cell == 0xffffffff or cell // F not in set(near.tolist()):
+                assert r[k, e] == 0xffffffff
+                continue
+            c, f = divmod(cell, F)
+            keys = _f2ord((cd[c] + st[c]) + fd)
+            assert r[k, e] == 1 + int((keys < keys[f]).sum()), (k, e)
+
+
+def _f2ord(v):
+    u = np.ascontiguousarray(v, np.float32).view(np.uint32)
+    return np.where(u & 0x80000000, ~u, u | 0x80000000).astype(np.uint32)
+
+
 def test_budget_labels_refind_plain_hits(ctx, idx_cos):
     import torch
     from puck_amd.probe import budget_labels
-    from puck_amd.engine import Probe
     ix, q = idx_cos
     gi = _gpu_index(ctx, ix)
     gt, _ = O.exact_topk(ix["raw"], q, 10)
-    pr = Probe(ctx, 96, 8, 8, 32)
     qd = torch.from_numpy(q).cuda()
-    _, fo = pr.features(gi, qd, fine_order=True)
-    lab = budget_labels(gi, qd, fo, torch.from_numpy(gt.astype(np.int64)).cuda(),
+    lab = budget_labels(gi, qd, torch.from_numpy(gt.astype(np.int64)).cuda(),
                         torch.from_numpy(ix["cell_assign"].view(np.int32)).cuda(), 4, 200).cpu().numpy()
     _, Ip, Rp = gi.search(q, 10, 4, 200)
     _, Il, Rl = gi.search_batch(q, 10, 4, 200, fine_budget=lab.astype(np.uint32))

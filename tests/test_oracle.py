@@ -306,21 +306,32 @@ def test_recall_known_answers(kat):
     assert r_plain == pytest.approx(kat["expected_no_ties"], abs=1e-12)
 
 
+def _f2ord(v):
+    u = np.ascontiguousarray(v, np.float32).view(np.uint32)
+    return np.where(u & 0x80000000, ~u, u | 0x80000000).astype(np.uint32)
+
+
 def test_fine_budget_full_scan_is_restricted_brute_force(small_l2):
     # scc = C and filter_topk >= N with fine budget b: nothing prunes, so the
-    # candidates are exactly the points whose fine cluster is among the query's
-    # first b fine clusters (ascending -<f, q>), and the result is their exact
-    # top-k.  A budget of F (or 0) is the reference's walk.
+    # candidates are exactly the points of the cells whose cell distance
+    # (cd + stationary) + fine distance is among the b smallest of their coarse
+    # list, and the result is their exact top-k.  A budget of F (or 0) is the
+    # reference's walk.
     ix, q = small_l2
     oi = CI.oracle_index(ix)
-    raw, fine = ix["raw"], ix["fine"]
-    N, F = raw.shape[0], fine.shape[0]
-    fine_of = ix["cell_assign"] % F
+    ex = oi.export()
+    raw, coarse, fine = ix["raw"], ix["coarse"], ix["fine"]
+    N, C, F = raw.shape[0], coarse.shape[0], fine.shape[0]
+    st = ex["stationary"].reshape(C, F)
     for k in range(8):
-        b = [1, 3, 7, 20][k % 4]
+        b = [1, 2, 3, 5][k % 4]
         o = oi.search_one(q[k], 10, 1000, N, fine_budget=b)
-        fo = np.argsort(-(fine.astype(np.float64) @ q[k].astype(np.float64)), kind="stable")[:b]
-        cand = np.nonzero(np.isin(fine_of, fo))[0]
+        cd = np.array([ex["coarse_norms"][c] - O.dot(coarse[c], q[k]) for c in range(C)], np.float32)
+        fd = np.array([-O.dot(fine[f], q[k]) for f in range(F)], np.float32)
+        keys = _f2ord((cd[:, None] + st) + fd[None, :])          # float32 adds, reference order
+        kcut = np.sort(keys, axis=1)[:, b - 1]
+        ok = keys <= kcut[:, None]
+        cand = np.nonzero(ok.reshape(-1)[ix["cell_assign"]])[0]
         assert o["filt_n"] == cand.size
         got = np.sort(o["ids"][:o["n"]])
         d = ((raw[cand].astype(np.float64) - q[k]) ** 2).sum(1)
