@@ -19,7 +19,11 @@ hipError_t pk_launch_gemm_exact(const float*, const float*, float*, uint32_t, ui
                                 const float*, hipStream_t);
 hipError_t pk_launch_row_norms(const float*, uint32_t, uint32_t, int, float*, hipStream_t);
 hipError_t pk_launch_sort_rows(const float*, uint32_t, uint32_t, uint64_t, uint32_t, float*, uint32_t*, uint8_t*,
+                               uint32_t, const uint32_t*, hipStream_t);
+hipError_t pk_launch_topk_rows(const float*, uint32_t, uint32_t, uint64_t, uint32_t, float*, uint32_t*, uint8_t*,
                                uint32_t, hipStream_t);
+hipError_t pk_launch_cell_topk(const float*, const uint32_t*, uint32_t, const float*, const float*, uint32_t, uint32_t,
+                               uint32_t, float*, hipStream_t);
 hipError_t pk_launch_exact_center_order(const float*, uint32_t, const float*, uint32_t, uint32_t, uint32_t,
                                         const uint32_t*, const uint32_t*, uint32_t, float*, uint32_t*, float*,
                                         uint32_t*, hipStream_t);
@@ -33,8 +37,6 @@ uint32_t pk_probe_param_count(uint32_t, uint32_t);
 size_t pk_probe_work_bytes(uint32_t, uint32_t, uint32_t);
 hipError_t pk_launch_probe_raw_features(const float*, uint32_t, const float*, uint32_t, uint32_t, const float*,
                                         uint32_t, uint32_t, float*, hipStream_t);
-hipError_t pk_launch_probe_cell_row(const float*, const uint32_t*, uint32_t, const float*, const float*, uint32_t,
-                                    uint32_t, float*, hipStream_t);
 hipError_t pk_probe_pass(const float*, const uint16_t*, uint32_t, uint32_t, uint32_t, const float*, const float*,
                          void*, float*, float*, float*, hipStream_t);
 hipError_t pk_probe_adam_step(float*, float*, float*, const float*, uint16_t*, uint32_t, uint32_t, float, float,
@@ -206,7 +208,7 @@ void* scratch(puckgpu_ctx* ctx, size_t slot, size_t bytes, int* rc) {
 enum Slot {
     S_Q = 0, S_QN, S_QST, S_CD, S_FD, S_CSD, S_CSI, S_FSD, S_FSI, S_FT, S_FI, S_FN, S_OD, S_OI, S_ON, S_STATS,
     S_IN_Q, S_A0, S_A1, S_A2, S_A3, S_A4, S_A5, S_A6, S_A7, S_FLAGS, S_QLIST, S_TIEID, S_TIEN, S_QTOPK, S_QSCC,
-    S_QNF, S_PFEAT, S_PWORK, S_PNF, S_PY, S_PT, S_WL, S_PROW, S_PCV, S_PCI, S_COUNT
+    S_QNF, S_PFEAT, S_PWORK, S_PNF, S_PY, S_PT, S_PCV, S_COUNT
 };
 
 // Device view of an input array: the pointer itself, or a scratch copy of it.
@@ -442,6 +444,15 @@ namespace {
 // Batched search.  `topk` is the largest per-query top-k (the rerank
 // capacity) and `ostride` the output row stride; q_topk / q_scc / q_nf are
 // optional per-query arrays (host or device as device_ptrs says).
+// The `keep` smallest coarse distances of m rows, ascending by (distance, id):
+// a select-based top-k for short prefixes, a full row sort otherwise.
+hipError_t coarse_select(const float* cdist, uint32_t m, uint32_t C, uint32_t keep, float* csd, uint32_t* csi,
+                         uint8_t* flags, uint32_t flag_lim, hipStream_t s) {
+    if (keep == 0) return hipSuccess;
+    if (keep <= 256) return pk_launch_topk_rows(cdist, m, C, C, keep, csd, csi, flags, flag_lim, s);
+    return pk_launch_sort_rows(cdist, m, C, C, keep, csd, csi, flags, flag_lim, nullptr, s);
+}
+
 // Probe features of m prepared queries: cell-distance row of the nearest
 // coarse list, its nfine smallest values, then the raw feature rows.
 int probe_features_chunk(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_probe* p, uint32_t m,
@@ -449,14 +460,9 @@ int probe_features_chunk(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgp
                          float* feat, hipStream_t s) {
     const uint32_t F = ix->p.fine_cluster_count;
     int rc = 0;
-    float* row = (float*)scratch(ctx, S_PROW, sizeof(float) * (size_t)m * F, &rc);
     float* cv = (float*)scratch(ctx, S_PCV, sizeof(float) * (size_t)m * (p->nfine ? p->nfine : 1), &rc);
-    uint32_t* ci = (uint32_t*)scratch(ctx, S_PCI, sizeof(uint32_t) * (size_t)m * (p->nfine ? p->nfine : 1), &rc);
     if (rc) return rc;
-    if (p->nfine) {
-        PK_CHECK(pk_launch_probe_cell_row(csd, csi, cstride, fdist, ix->stationary, F, m, row, s));
-        PK_CHECK(pk_launch_sort_rows(row, m, F, F, p->nfine, cv, ci, nullptr, 0, s));
-    }
+    if (p->nfine) PK_CHECK(pk_launch_cell_topk(csd, csi, cstride, fdist, ix->stationary, F, p->nfine, m, cv, s));
     PK_CHECK(pk_launch_probe_raw_features(q, ix->p.dim, csd, cstride, p->nc, cv, p->nfine, m, feat, s));
     return PUCKGPU_OK;
 }
@@ -550,8 +556,7 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
         PK_CHECK(pk_launch_gemm_exact(q, ix->coarse, cdist, m, C, d, C, PK_EPI_SUB_FROM_BIAS, ix->coarse_norms, s));
         PK_CHECK(pk_launch_gemm_exact(q, ix->fine, fdist, m, F, d, F, PK_EPI_NEG, nullptr, s));
         PK_CHECK(hipEventRecord(ctx->ev[1], s));
-        if (keep) PK_CHECK(pk_launch_sort_rows(cdist, m, C, C, keep, csd, csi, flags, scc, s));
-        PK_CHECK(pk_launch_sort_rows(fdist, m, F, F, F, fsd, fsi, nullptr, 0, s));
+        PK_CHECK(coarse_select(cdist, m, C, keep, csd, csi, flags, scc, s));
         const uint32_t* nf_chunk = q_nf ? q_nf + q0 : nullptr;
         if (pr) {
             // learned fine budget: features -> MLP -> budget, on this stream
@@ -567,6 +572,8 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
                                         device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
             nf_chunk = pnf;
         }
+        // fine order for the walk (rows of budget-scan queries are skipped)
+        PK_CHECK(pk_launch_sort_rows(fdist, m, F, F, F, fsd, fsi, nullptr, 0, nf_chunk, s));
         PK_CHECK(hipEventRecord(ctx->ev[2], s));
         PkWalkArgs wa;
         wa.q = q; wa.qn = qn; wa.qstatus = qst; wa.csel_d = csd; wa.csel_id = csi; wa.fsort_d = fsd;
@@ -710,7 +717,7 @@ int puckgpu_budget_ranks(puckgpu_ctx* ctx, const puckgpu_index* ix, uint32_t sea
         PK_CHECK(pk_launch_prep_queries(qin + (size_t)q0 * d, m, d, ix->p.whether_norm, q, qn, qst, s));
         PK_CHECK(pk_launch_gemm_exact(q, ix->coarse, cdist, m, C, d, C, PK_EPI_SUB_FROM_BIAS, ix->coarse_norms, s));
         PK_CHECK(pk_launch_gemm_exact(q, ix->fine, fdist, m, F, d, F, PK_EPI_NEG, nullptr, s));
-        if (scc) PK_CHECK(pk_launch_sort_rows(cdist, m, C, C, scc, csd, csi, nullptr, 0, s));
+        PK_CHECK(coarse_select(cdist, m, C, scc, csd, csi, nullptr, 0, s));
         PK_CHECK(pk_launch_budget_rank(csd, csi, keep, scc, fdist, ix->stationary, C, F, cin + (size_t)q0 * k, k, m,
                                        rk + (size_t)q0 * k, s));
     }
@@ -816,8 +823,8 @@ int puckgpu_probe_features(puckgpu_ctx* ctx, const puckgpu_index* ix, const puck
         PK_CHECK(pk_launch_prep_queries(qin + (size_t)q0 * d, m, d, ix->p.whether_norm, q, qn, qst, s));
         PK_CHECK(pk_launch_gemm_exact(q, ix->coarse, cdist, m, C, d, C, PK_EPI_SUB_FROM_BIAS, ix->coarse_norms, s));
         PK_CHECK(pk_launch_gemm_exact(q, ix->fine, fdist, m, F, d, F, PK_EPI_NEG, nullptr, s));
-        PK_CHECK(pk_launch_sort_rows(cdist, m, C, C, keep, csd, csi, nullptr, 0, s));
-        PK_CHECK(pk_launch_sort_rows(fdist, m, F, F, F, fsd, fsi, nullptr, 0, s));
+        PK_CHECK(coarse_select(cdist, m, C, keep, csd, csi, nullptr, 0, s));
+        if (fine_order) PK_CHECK(pk_launch_sort_rows(fdist, m, F, F, F, fsd, fsi, nullptr, 0, nullptr, s));
         float* fo = device_ptrs ? feat + (size_t)q0 * p->D : feat;
         if ((rc = probe_features_chunk(ctx, ix, p, m, q, csd, csi, keep, fdist, fo, s))) return rc;
         if (!device_ptrs)

@@ -124,3 +124,34 @@ __device__ __forceinline__ uint32_t pk_block_exscan(uint32_t v, uint32_t* tmp, u
     __syncthreads();
     return r;
 }
+
+// Key of rank `want` (1-based) among key(0..n-1): MSB-first radix select
+// with 8-bit digits, keys recomputed on every pass.  blockDim.x == 256 (one
+// histogram bin per thread); hist has 258 words.  Every thread returns it.
+template <typename KeyF>
+__device__ uint32_t pk_select_rank(KeyF key, uint32_t n, uint32_t want, uint32_t* hist, uint32_t* tmp) {
+    uint32_t prefix = 0, mask = 0;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        hist[threadIdx.x] = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+            const uint32_t k = key(i);
+            if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        const uint32_t c = hist[threadIdx.x];
+        uint32_t tot;
+        const uint32_t before = pk_block_exscan(c, tmp, &tot);
+        if (c > 0 && before < want && want <= before + c) {
+            hist[256] = threadIdx.x;
+            hist[257] = want - before;
+        }
+        __syncthreads();
+        prefix |= hist[256] << shift;
+        mask |= 255u << shift;
+        want = hist[257];
+        __syncthreads();
+    }
+    return prefix;
+}
+

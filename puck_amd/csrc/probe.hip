@@ -130,18 +130,6 @@ static hipError_t pk_mm(bool at, bool bt, const PkMmArgs& a, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// Cell distances of every fine cluster of the query's nearest coarse list:
-// row[k] = (cd0 + stationary[c0][k]) + fine distance of k (fdist unsorted).
-__global__ void pk_probe_cell_row(const float* __restrict__ csd, const uint32_t* __restrict__ csi, uint32_t cstride,
-                                  const float* __restrict__ fdist, const float* __restrict__ stationary, uint32_t F,
-                                  uint32_t n, float* __restrict__ row) {
-    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (uint64_t)n * F) return;
-    const uint32_t i = (uint32_t)(e / F), k = (uint32_t)(e % F);
-    const uint32_t c0 = csi[(uint64_t)i * cstride];
-    row[e] = (csd[(uint64_t)i * cstride] + stationary[(uint64_t)c0 * F + k]) + fdist[e];
-}
-
 // Raw features of n queries from the search pipeline's sorted rows: the
 // query, the nc smallest coarse distances (csd, stride cstride) and the nfine
 // smallest cell distances of the nearest list (cell, stride nfine).
@@ -339,15 +327,6 @@ static PkProbeWork pk_probe_carve(void* base, uint32_t n, uint32_t D, uint32_t H
     w.dy = (float*)p; p += al(4ull * n);
     w.sq = (float*)p; p += al(4ull * n);
     return w;
-}
-
-hipError_t pk_launch_probe_cell_row(const float* csd, const uint32_t* csi, uint32_t cstride, const float* fdist,
-                                    const float* stationary, uint32_t F, uint32_t n, float* row, hipStream_t s) {
-    const uint64_t tot = (uint64_t)n * F;
-    if (tot == 0) return hipSuccess;
-    hipLaunchKernelGGL(pk_probe_cell_row, dim3((uint32_t)((tot + 255) / 256)), dim3(256), 0, s, csd, csi, cstride,
-                       fdist, stationary, F, n, row);
-    return hipGetLastError();
 }
 
 hipError_t pk_launch_probe_raw_features(const float* q, uint32_t dim, const float* csd, uint32_t cstride, uint32_t nc,

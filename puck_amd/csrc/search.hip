@@ -222,36 +222,6 @@ __device__ __forceinline__ uint32_t pk_bmax(uint32_t v, uint32_t* red) {
     return r;
 }
 
-// Key of rank `want` (1-based) among key(0..n-1): MSB-first radix select
-// with 8-bit digits, keys recomputed on every pass.  blockDim.x == 256 (one
-// histogram bin per thread); hist has 258 words.  Every thread returns it.
-template <typename KeyF>
-__device__ uint32_t pk_select_rank(KeyF key, uint32_t n, uint32_t want, uint32_t* hist, uint32_t* tmp) {
-    uint32_t prefix = 0, mask = 0;
-    for (int shift = 24; shift >= 0; shift -= 8) {
-        hist[threadIdx.x] = 0;
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-            const uint32_t k = key(i);
-            if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
-        }
-        __syncthreads();
-        const uint32_t c = hist[threadIdx.x];
-        uint32_t tot;
-        const uint32_t before = pk_block_exscan(c, tmp, &tot);
-        if (c > 0 && before < want && want <= before + c) {
-            hist[256] = threadIdx.x;
-            hist[257] = want - before;
-        }
-        __syncthreads();
-        prefix |= hist[256] << shift;
-        mask |= 255u << shift;
-        want = hist[257];
-        __syncthreads();
-    }
-    return prefix;
-}
-
 // Values of local points [p_lo, p_hi) of the chunk's active cells; keeps those
 // below `thr`.  Returns the number kept (W_key[0..n)).
 __device__ uint32_t pk_walk_values(const PkWalkArgs& a, PkWalkSmem& s, uint32_t cend, uint32_t p_lo,
@@ -558,13 +528,16 @@ __device__ uint32_t pk_keep_le(uint64_t* v, uint32_t n, uint64_t K, uint32_t* tm
 // are scored in full: every point's filter distance is computed as the
 // reference's compute_quantized_distance computes it, and the filter set is
 // the ftk smallest (distance, memory id) keys, kept in an unsorted LDS buffer
-// (the walk's S_key + W_key region) that is cut back to ftk by a radix select
-// whenever it could overflow.  The result does not depend on any visiting
-// order, so it needs no replay.
+// (the walk's S_key + W_key region).  When the buffer could overflow it is
+// cut back to the keys whose distance is at most the ftk-th smallest distance
+// (a 4-pass select on the distance bits); the exact (distance, id) cut to ftk
+// entries is made once, at the end.  The result does not depend on any
+// visiting order, so it needs no replay.
 template <int FTKM>
 __device__ void pk_scan_budget(const PkWalkArgs& a, PkWalkSmem& s, uint32_t qi, uint32_t b) {
     const uint32_t F = a.F, ftk = a.ftk, nsq = a.nsq;
     const int tid = threadIdx.x;
+    constexpr uint32_t R = 2 * WALK_THREADS;  // points per round
     uint64_t* B = s.S_key;  // S_key and W_key are contiguous
     const uint32_t cap = ((ftk + 1) & ~1u) + WALK_PMAX;
     uint32_t* hist = s.P_id;  // 512 words, unused by this path
@@ -574,6 +547,7 @@ __device__ void pk_scan_budget(const PkWalkArgs& a, PkWalkSmem& s, uint32_t qi, 
     if (a.q_scc) scc = a.q_scc[qi] < scc ? a.q_scc[qi] : scc;
     const float* fdq = a.fdist + (uint64_t)qi * F;
     uint32_t st_cells = 0, st_pts = 0;
+    auto hi32 = [&](uint32_t i) { return (uint32_t)(B[i] >> 32); };
     for (uint32_t l = 0; l < scc; ++l) {
         const uint32_t c = a.csel_id[(uint64_t)qi * a.cstride + l];
         const float cd = a.csel_d[(uint64_t)qi * a.cstride + l];
@@ -594,27 +568,35 @@ __device__ void pk_scan_budget(const PkWalkArgs& a, PkWalkSmem& s, uint32_t qi, 
                     cdist = cv;
                 }
             }
-            uint32_t ne, tot;
-            const uint32_t pos = pk_block_exscan(in, s.red + 4, &ne);
-            const uint32_t ppre = pk_block_exscan(in ? sz : 0u, s.red + 4, &tot);
+            // one scan of (points << 9 | cell): at most 256 cells per window
+            uint32_t tot2;
+            const uint32_t pre2 = pk_block_exscan(in ? ((sz << 9) | 1u) : 0u, s.red + 4, &tot2);
+            const uint32_t ne = tot2 & 511u, tot = tot2 >> 9;
             if (in) {
-                s.c_b[pos] = cb;
-                s.c_fd[pos] = cdist;
-                s.c_pre[pos] = ppre;
+                s.c_b[pre2 & 511u] = cb;
+                s.c_fd[pre2 & 511u] = cdist;
+                s.c_pre[pre2 & 511u] = pre2 >> 9;
             }
             if (tid == 0) s.c_pre[ne] = tot;
             __syncthreads();
             st_cells += ne;
             st_pts += tot;
-            for (uint32_t p0 = 0; p0 < tot; p0 += blockDim.x) {
-                if (n + blockDim.x > cap) {
-                    K = pk_select_rank64(B, n, ftk, hist, s.red + 4);
+            for (uint32_t p0 = 0; p0 < tot; p0 += R) {
+                if (n + R > cap) {
+                    const uint32_t kc = pk_select_rank(hi32, n, ftk, hist, s.red + 4);
+                    K = ((uint64_t)kc << 32) | 0xffffffffull;
                     n = pk_keep_le(B, n, K, s.red + 4);
+                    if (n + R > cap) {  // many equal distances at the cut: cut exactly
+                        K = pk_select_rank64(B, n, ftk, hist, s.red + 4);
+                        n = pk_keep_le(B, n, K, s.red + 4);
+                    }
                 }
                 if (tid == 0) s.red[2] = n;
                 __syncthreads();
-                const uint32_t p = p0 + tid;
-                if (p < tot) {
+#pragma unroll
+                for (uint32_t h = 0; h < R / WALK_THREADS; ++h) {
+                    const uint32_t p = p0 + h * WALK_THREADS + tid;
+                    if (p >= tot) break;
                     uint32_t lo = 0, hi = ne;  // c_pre[lo] <= p < c_pre[hi]
                     while (hi - lo > 1) {
                         const uint32_t mid = (lo + hi) >> 1;
@@ -629,11 +611,11 @@ __device__ void pk_scan_budget(const PkWalkArgs& a, PkWalkSmem& s, uint32_t qi, 
                             uint4 w = *(const uint4*)(codes + g);
                             uint32_t ws[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-                            for (int h = 0; h < 4; ++h) {
+                            for (int hh = 0; hh < 4; ++hh) {
 #pragma unroll
                                 for (int j = 0; j < 4; ++j) {
-                                    uint32_t code = (ws[h] >> (8 * j)) & 0xffu;
-                                    acc[j] = acc[j] + s.lut[(g + 4 * h + j) * PK_KS + code];
+                                    uint32_t code = (ws[hh] >> (8 * j)) & 0xffu;
+                                    acc[j] = acc[j] + s.lut[(g + 4 * hh + j) * PK_KS + code];
                                 }
                             }
                         }
@@ -642,7 +624,7 @@ __device__ void pk_scan_budget(const PkWalkArgs& a, PkWalkSmem& s, uint32_t qi, 
                     }
                     tv = tv + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
                     const uint64_t kk = ((uint64_t)pk_f2ord(tv) << 32) | mid;
-                    if (kk < K) B[atomicAdd(&s.red[2], 1u)] = kk;
+                    if (kk <= K) B[atomicAdd(&s.red[2], 1u)] = kk;
                 }
                 __syncthreads();
                 n = s.red[2];
@@ -651,7 +633,20 @@ __device__ void pk_scan_budget(const PkWalkArgs& a, PkWalkSmem& s, uint32_t qi, 
         }
     }
     if (n > ftk) {
-        K = pk_select_rank64(B, n, ftk, hist, s.red + 4);
+        // exact cut: the ftk smallest (distance, id) keys
+        const uint32_t kc = pk_select_rank(hi32, n, ftk, hist, s.red + 4);
+        uint32_t nle = 0;
+        {
+            if (tid == 0) s.red[3] = 0;
+            __syncthreads();
+            uint32_t c = 0;
+            for (uint32_t i = tid; i < n; i += blockDim.x) c += hi32(i) <= kc ? 1u : 0u;
+            atomicAdd(&s.red[3], c);
+            __syncthreads();
+            nle = s.red[3];
+            __syncthreads();
+        }
+        K = nle == ftk ? (((uint64_t)kc << 32) | 0xffffffffull) : pk_select_rank64(B, n, ftk, hist, s.red + 4);
         n = pk_keep_le(B, n, K, s.red + 4);
     }
     for (uint32_t r = tid; r < n; r += blockDim.x) {

@@ -33,9 +33,12 @@ __global__ __launch_bounds__(SORT_THREADS) void pk_sort_rows(const float* __rest
                                                              uint64_t ld_in, uint32_t keep,
                                                              float* __restrict__ out_val,
                                                              uint32_t* __restrict__ out_idx, uint32_t n2,
-                                                             uint8_t* __restrict__ flags, uint32_t flag_lim) {
+                                                             uint8_t* __restrict__ flags, uint32_t flag_lim,
+                                                             const uint32_t* __restrict__ skip_nf) {
     extern __shared__ uint64_t sk[];
     const uint64_t row = blockIdx.x;
+    // rows of queries with a fine budget below L are not walked in fine order
+    if (skip_nf && skip_nf[row] != 0u && skip_nf[row] < L) return;
     const float* v = vals + row * ld_in;
     for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x)
         sk[i] = i < L ? (((uint64_t)pk_f2ord(v[i]) << 32) | i) : ~0ull;
@@ -56,7 +59,7 @@ __global__ __launch_bounds__(SORT_THREADS) void pk_sort_rows(const float* __rest
 
 extern "C" hipError_t pk_launch_sort_rows(const float* vals, uint32_t rows, uint32_t L, uint64_t ld_in,
                                           uint32_t keep, float* out_val, uint32_t* out_idx, uint8_t* flags,
-                                          uint32_t flag_lim, hipStream_t s) {
+                                          uint32_t flag_lim, const uint32_t* skip_nf, hipStream_t s) {
     if (rows == 0 || keep == 0) return hipSuccess;
     if (L > 8192 || keep > L) return hipErrorInvalidValue;
     static bool attr = false;
@@ -67,6 +70,108 @@ extern "C" hipError_t pk_launch_sort_rows(const float* vals, uint32_t rows, uint
     uint32_t n2 = 2;
     while (n2 < L) n2 <<= 1;
     hipLaunchKernelGGL(pk_sort_rows, dim3(rows), dim3(SORT_THREADS), n2 * sizeof(uint64_t), s, vals, L, ld_in,
-                       keep, out_val, out_idx, n2, flags, flag_lim);
+                       keep, out_val, out_idx, n2, flags, flag_lim, skip_nf);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// k smallest entries of a row by (value, column), ascending, without sorting
+// the row: a radix select finds the k-th smallest value kc; the entries below
+// it and then the first ties at it (column order) are compacted into LDS and
+// only those k are sorted.
+#define TOPK_CAP 1024
+
+template <typename ValF>
+__device__ void pk_topk_block(ValF val, uint32_t L, uint32_t k, float* out_val, uint32_t* out_idx, bool* tie,
+                              uint32_t flag_lim) {
+    __shared__ uint32_t hist[258];
+    __shared__ uint32_t tmp[8];
+    __shared__ uint64_t buf[TOPK_CAP];
+    auto key = [&](uint32_t i) { return pk_f2ord(val(i)); };
+    const uint32_t kc = pk_select_rank(key, L, k, hist, tmp);
+    uint32_t nlt = 0, neq = 0;
+    for (uint32_t r0 = 0; r0 < L; r0 += blockDim.x) {  // entries below kc (fewer than k)
+        const uint32_t i = r0 + threadIdx.x;
+        const uint32_t kk = i < L ? key(i) : 0xffffffffu;
+        const uint32_t f = (i < L && kk < kc) ? 1u : 0u;
+        uint32_t tot;
+        const uint32_t pre = pk_block_exscan(f, tmp, &tot);
+        if (f) buf[nlt + pre] = ((uint64_t)kk << 32) | i;
+        nlt += tot;
+    }
+    for (uint32_t r0 = 0; r0 < L; r0 += blockDim.x) {  // ties at kc, lowest columns first
+        const uint32_t i = r0 + threadIdx.x;
+        const uint32_t f = (i < L && key(i) == kc) ? 1u : 0u;
+        uint32_t tot;
+        const uint32_t pre = pk_block_exscan(f, tmp, &tot);
+        if (f && nlt + neq + pre < k) buf[nlt + neq + pre] = ((uint64_t)kc << 32) | i;
+        neq += tot;
+    }
+    uint32_t n2 = 2;
+    while (n2 < k) n2 <<= 1;
+    for (uint32_t i = k + threadIdx.x; i < n2; i += blockDim.x) buf[i] = ~0ull;
+    __syncthreads();
+    pk_bitonic_u64(buf, n2);
+    bool t = false;
+    for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) {
+        if (out_val) out_val[i] = pk_ord2f((uint32_t)(buf[i] >> 32));
+        if (out_idx) out_idx[i] = (uint32_t)(buf[i] & 0xffffffffu);
+        if (i < flag_lim) {
+            // equal to the next entry of the full order (inside the k, or a tie cut at the k-th)
+            if (i + 1 < k ? (uint32_t)(buf[i + 1] >> 32) == (uint32_t)(buf[i] >> 32) : nlt + neq > k) t = true;
+        }
+    }
+    *tie = t;
+}
+
+__global__ __launch_bounds__(SORT_THREADS) void pk_topk_rows(const float* __restrict__ vals, uint32_t L,
+                                                             uint64_t ld_in, uint32_t k, float* __restrict__ out_val,
+                                                             uint32_t* __restrict__ out_idx, uint8_t* __restrict__ flags,
+                                                             uint32_t flag_lim) {
+    const uint64_t row = blockIdx.x;
+    const float* v = vals + row * ld_in;
+    bool tie = false;
+    pk_topk_block([&](uint32_t i) { return v[i]; }, L, k, out_val + row * k, out_idx + row * k, &tie, flag_lim);
+    __shared__ uint32_t any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    if (tie) any = 1;
+    __syncthreads();
+    if (threadIdx.x == 0 && any && flags) flags[row] = 1u | flags[row];
+}
+
+// Cell distances of the fine clusters of each query's nearest coarse list,
+// row[f] = (cd0 + stationary[c0][f]) + fdist[f]; the k smallest, ascending
+// (values only).
+__global__ __launch_bounds__(SORT_THREADS) void pk_cell_topk(const float* __restrict__ csd,
+                                                             const uint32_t* __restrict__ csi, uint32_t cstride,
+                                                             const float* __restrict__ fdist,
+                                                             const float* __restrict__ stationary, uint32_t F,
+                                                             uint32_t k, float* __restrict__ out_val) {
+    const uint64_t row = blockIdx.x;
+    const float cd = csd[row * cstride];
+    const float* st = stationary + (uint64_t)csi[row * cstride] * F;
+    const float* fd = fdist + row * F;
+    bool tie = false;
+    pk_topk_block([&](uint32_t f) { return (cd + st[f]) + fd[f]; }, F, k, out_val + row * k, nullptr, &tie, 0);
+}
+
+extern "C" hipError_t pk_launch_topk_rows(const float* vals, uint32_t rows, uint32_t L, uint64_t ld_in, uint32_t k,
+                                          float* out_val, uint32_t* out_idx, uint8_t* flags, uint32_t flag_lim,
+                                          hipStream_t s) {
+    if (rows == 0 || k == 0) return hipSuccess;
+    if (k > TOPK_CAP || k > L) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pk_topk_rows, dim3(rows), dim3(SORT_THREADS), 0, s, vals, L, ld_in, k, out_val, out_idx, flags,
+                       flag_lim);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t pk_launch_cell_topk(const float* csd, const uint32_t* csi, uint32_t cstride, const float* fdist,
+                                          const float* stationary, uint32_t F, uint32_t k, uint32_t rows,
+                                          float* out_val, hipStream_t s) {
+    if (rows == 0 || k == 0) return hipSuccess;
+    if (k > TOPK_CAP || k > F) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pk_cell_topk, dim3(rows), dim3(SORT_THREADS), 0, s, csd, csi, cstride, fdist, stationary, F, k,
+                       out_val);
     return hipGetLastError();
 }
