@@ -224,15 +224,23 @@ def main():
     # algorithmic bytes of the walk + filter scan launch: per examined cell its
     # stationary distance and start offset (8 B); per scanned point its offset
     # and nsq code bytes.
+    # Algorithmic bytes of the dominant launch.
+    #   plain (pk_walk): per examined cell its stationary distance and start
+    #     offset (8 B); per scanned point its offset and nsq code bytes.
+    #   learned (pk_scan_budget_kernel): per query and walked list the
+    #     stationary row and the fine-distance row (8 B per fine cluster) for
+    #     the budget selection; per scored cell its start offsets (8 B); per
+    #     point its offset and nsq code bytes.
+    form = "learned" if learned is not None else "plain"
+    kernel = "pk_scan_budget_kernel" if form == "learned" else "pk_walk"
     cells, points = float(cnt[2]), float(cnt[3])
-    walk_bytes = cells * 8 + points * (4 + nsq)
+    walk_bytes = cells * 8 + points * (4 + nsq) + (nq * chosen * F * 8 if form == "learned" else 0)
     avg_walk_s = np.mean(walk_ms) / 1000.0
     achieved = walk_bytes / avg_walk_s / 1e9
-    form = "learned" if learned is not None else "plain"
     # HBM traffic per launch of the same kernel and workload, from the
     # committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (copy-calibrated)
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"pk_walk_10Mx96_{form}_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", f"{kernel}_10Mx96_traffic.json")
     if N == 10_000_000 and nq == 10_000 and os.path.exists(tpath):
         traffic = int(json.load(open(tpath))["traffic_bytes_per_launch"])
 
@@ -280,7 +288,7 @@ def main():
                              if learned is not None and plain is not None else None),
         "uniform_budget": {"curve": uniform_curve, "best_at_target": uniform_best} if uniform_curve else None,
         "recall_curve": curve,
-        "roofline": {"kernel": "pk_walk", "form": form, "bound": "hbm", "achieved": round(achieved, 1),
+        "roofline": {"kernel": kernel, "form": form, "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes_per_launch": int(walk_bytes),
                      "avg_launch_ms": round(avg_walk_s * 1000.0, 3)},

@@ -187,6 +187,35 @@ int puckgpu_search_learned(puckgpu_ctx* ctx, const puckgpu_index* index, const p
                            float* distance, uint32_t* labels, uint32_t* result_num, uint32_t* budget_out,
                            int device_ptrs);
 
+/* ---- Batched serving ------------------------------------------------------
+ * No counterpart in the reference (PySearcher::search runs one Request per
+ * thread, py_api_wrapper.cpp:111-171).  A server owns, for every power-of-two
+ * batch size up to max_batch, device buffers and a HIP graph of the whole
+ * device pipeline (captured on first use); a batch is padded to its bucket and
+ * served by one graph launch.  Queries whose result depends on the order of
+ * exactly equal distances are replayed outside the graph.  With a probe the
+ * fine budgets are predicted in the graph (puckgpu_search_learned).  The
+ * server uses the context's stream; one call at a time per server. */
+typedef struct puckgpu_server puckgpu_server;
+int puckgpu_server_create(puckgpu_ctx* ctx, const puckgpu_index* index, const puckgpu_probe* probe,
+                          const puckgpu_search_params* params, const puckgpu_budget_params* budget,
+                          uint32_t max_batch, puckgpu_server** out);
+void puckgpu_server_free(puckgpu_server* server);
+uint32_t puckgpu_server_buckets(const puckgpu_server* server);
+/* One batch of n <= max_batch queries; topk / search_coarse_count [n] may be
+ * NULL (params->topk_max / params->search_coarse_count).  Outputs as
+ * puckgpu_search_batch (row stride topk_max). */
+int puckgpu_server_search(puckgpu_server* server, uint32_t n, const float* queries, const uint32_t* topk,
+                          const uint32_t* search_coarse_count, float* distance, uint32_t* labels,
+                          uint32_t* result_num, int device_ptrs);
+/* Autotuner: for every bucket, times each GEMM tile shape on the first cap
+ * rows of sample [max_batch x dim] (host) over reps passes and keeps the
+ * fastest (every shape gives the same bits); graphs are re-captured.
+ * shapes_out [buckets] and times_out [buckets x shapes] (ms per pass) may be
+ * NULL. */
+int puckgpu_server_tune(puckgpu_server* server, const float* sample, uint32_t reps, int32_t* shapes_out,
+                        float* times_out);
+
 /* Fine-budget ranks (labels of the learned probe): for query i and entry e of
  * cells [n x k] (cell id coarse * F + fine, or 0xffffffff), the smallest
  * fine_budget under which the walk of that coarse list scores the cell -- 1 +

@@ -17,7 +17,8 @@ EXPORTS = [
     "puckgpu_encode_filter", "puckgpu_kmeans_assign", "puckgpu_search_batch",
     "puckgpu_probe_create", "puckgpu_probe_free", "puckgpu_probe_param_count", "puckgpu_probe_set_params",
     "puckgpu_probe_get_params", "puckgpu_probe_features", "puckgpu_probe_train_step", "puckgpu_probe_predict",
-    "puckgpu_search_learned", "puckgpu_budget_ranks",
+    "puckgpu_search_learned", "puckgpu_budget_ranks", "puckgpu_server_create", "puckgpu_server_free",
+    "puckgpu_server_buckets", "puckgpu_server_search", "puckgpu_server_tune",
 ]
 
 
@@ -126,6 +127,17 @@ def load():
     L.puckgpu_search_learned.restype = c_int
     L.puckgpu_budget_ranks.argtypes = [vp, vp, u32, u32, vp, u32, vp, vp, c_int]
     L.puckgpu_budget_ranks.restype = c_int
+    L.puckgpu_server_create.argtypes = [vp, vp, vp, ctypes.POINTER(SearchParams), ctypes.POINTER(BudgetParams), u32,
+                                        ctypes.POINTER(vp)]
+    L.puckgpu_server_create.restype = c_int
+    L.puckgpu_server_free.argtypes = [vp]
+    L.puckgpu_server_free.restype = None
+    L.puckgpu_server_buckets.argtypes = [vp]
+    L.puckgpu_server_buckets.restype = u32
+    L.puckgpu_server_search.argtypes = [vp, u32, vp, vp, vp, vp, vp, vp, c_int]
+    L.puckgpu_server_search.restype = c_int
+    L.puckgpu_server_tune.argtypes = [vp, vp, u32, vp, vp]
+    L.puckgpu_server_tune.restype = c_int
     del i32
     _lib = L
     return L

@@ -28,6 +28,11 @@ hipError_t pk_launch_exact_center_order(const float*, uint32_t, const float*, ui
                                         const uint32_t*, const uint32_t*, uint32_t, float*, uint32_t*, float*,
                                         uint32_t*, hipStream_t);
 hipError_t pk_launch_prep_queries(const float*, uint32_t, uint32_t, int, float*, float*, int32_t*, hipStream_t);
+hipError_t pk_launch_prep_queries_ex(const float*, uint32_t, uint32_t, int, float*, float*, int32_t*, const uint32_t*,
+                                     uint8_t*, hipStream_t);
+hipError_t pk_launch_gemm_exact_shape(const float*, const float*, float*, uint32_t, uint32_t, uint32_t, uint64_t, int,
+                                      const float*, int, hipStream_t);
+uint32_t pk_gemm_shapes(void);
 hipError_t pk_launch_lut(const float*, uint32_t, uint32_t, const float*, uint32_t, uint32_t, float*, hipStream_t);
 hipError_t pk_launch_walk(const PkWalkArgs*, uint32_t, hipStream_t);
 hipError_t pk_launch_rerank(const PkRerankArgs*, uint32_t, hipStream_t);
@@ -165,6 +170,18 @@ struct puckgpu_probe {
     float* grads = nullptr;
     float* loss = nullptr;
     uint16_t* wbf = nullptr;  // BF16 copy of the trained block
+};
+
+struct PkBucket;
+
+struct puckgpu_server {
+    puckgpu_ctx* ctx = nullptr;
+    const puckgpu_index* ix = nullptr;
+    puckgpu_search_params sp{};
+    const puckgpu_probe* probe = nullptr;
+    puckgpu_budget_params bp{};
+    uint32_t max_batch = 0;
+    std::vector<PkBucket*> buckets;
 };
 
 namespace {
@@ -455,13 +472,11 @@ hipError_t coarse_select(const float* cdist, uint32_t m, uint32_t C, uint32_t ke
 
 // Probe features of m prepared queries: cell-distance row of the nearest
 // coarse list, its nfine smallest values, then the raw feature rows.
+// cv: m x nfine scratch.
 int probe_features_chunk(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_probe* p, uint32_t m,
                          const float* q, const float* csd, const uint32_t* csi, uint32_t cstride, const float* fdist,
-                         float* feat, hipStream_t s) {
+                         float* cv, float* feat, hipStream_t s) {
     const uint32_t F = ix->p.fine_cluster_count;
-    int rc = 0;
-    float* cv = (float*)scratch(ctx, S_PCV, sizeof(float) * (size_t)m * (p->nfine ? p->nfine : 1), &rc);
-    if (rc) return rc;
     if (p->nfine) PK_CHECK(pk_launch_cell_topk(csd, csi, cstride, fdist, ix->stationary, F, p->nfine, m, cv, s));
     PK_CHECK(pk_launch_probe_raw_features(q, ix->p.dim, csd, cstride, p->nc, cv, p->nfine, m, feat, s));
     return PUCKGPU_OK;
@@ -473,52 +488,203 @@ struct ProbeUse {
     uint32_t* budget_out = nullptr;
 };
 
+// Device buffers of one pipeline pass over up to `cap` queries.
+struct PassBufs {
+    float* q = nullptr;
+    float* qn = nullptr;
+    int32_t* qst = nullptr;
+    float* cdist = nullptr;
+    float* fdist = nullptr;
+    float* csd = nullptr;
+    uint32_t* csi = nullptr;
+    float* fsd = nullptr;
+    uint32_t* fsi = nullptr;
+    uint8_t* flags = nullptr;
+    uint32_t* tie_ids = nullptr;
+    uint8_t* tie_n = nullptr;
+    uint32_t* stats = nullptr;
+    uint32_t* qlist = nullptr;
+    float* pfeat = nullptr;
+    void* pwork = nullptr;
+    uint32_t* pnf = nullptr;
+    float* py = nullptr;
+    float* pcv = nullptr;  // probe cell features, cap x nfine
+};
+
+// Per-pass configuration.
+struct PassCfg {
+    uint32_t scc = 0, keep = 0, ftk = 0, topk = 0, ostride = 0;
+    double rr = 1.0;
+    int gemm_shape = 0;
+    const ProbeUse* pu = nullptr;
+};
+
+// Enqueues the device stages of one pass over m queries: no host
+// synchronisation and no memset, so the sequence can be captured in a graph.
+// ev (5 events, may be null) brackets the stages.  nvalid (device, may be
+// null): rows at or past it are padding.  wa / ra receive the launch arguments
+// (for a replay).
+int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, const PassBufs& b, uint32_t m,
+                 const float* qin, const uint32_t* nvalid, const uint32_t* q_topk, const uint32_t* q_scc,
+                 const uint32_t* q_nf, float* ft, uint32_t* fi, uint32_t* fn, float* od, uint32_t* oi, uint32_t* on,
+                 uint32_t* budget_out, int budget_out_host, hipEvent_t* ev, PkWalkArgs* wa_out, PkRerankArgs* ra_out,
+                 hipStream_t s) {
+    const uint32_t d = ix->p.dim, C = ix->p.coarse_cluster_count, F = ix->p.fine_cluster_count;
+    const puckgpu_probe* pr = cfg.pu ? cfg.pu->probe : nullptr;
+    int rc = 0;
+    if (ev) PK_CHECK(hipEventRecord(ev[0], s));
+    PK_CHECK(pk_launch_prep_queries_ex(qin, m, d, ix->p.whether_norm, b.q, b.qn, b.qst, nvalid, b.flags, s));
+    PK_CHECK(pk_launch_gemm_exact_shape(b.q, ix->coarse, b.cdist, m, C, d, C, PK_EPI_SUB_FROM_BIAS, ix->coarse_norms,
+                                        cfg.gemm_shape, s));
+    PK_CHECK(pk_launch_gemm_exact_shape(b.q, ix->fine, b.fdist, m, F, d, F, PK_EPI_NEG, nullptr, cfg.gemm_shape, s));
+    if (ev) PK_CHECK(hipEventRecord(ev[1], s));
+    PK_CHECK(coarse_select(b.cdist, m, C, cfg.keep, b.csd, b.csi, b.flags, cfg.scc, s));
+    const uint32_t* nf = q_nf;
+    if (pr) {
+        // learned fine budget: features -> MLP -> budget, on this stream
+        if ((rc = probe_features_chunk(ctx, ix, pr, m, b.q, b.csd, b.csi, cfg.keep, b.fdist, b.pcv, b.pfeat, s)))
+            return rc;
+        PK_CHECK(pk_probe_pass(pr->params, pr->wbf, pr->D, pr->H, m, b.pfeat, nullptr, b.pwork, nullptr, nullptr,
+                               b.py, s));
+        PK_CHECK(pk_launch_probe_budget(b.py, m, cfg.pu->bp.margin, cfg.pu->bp.min_budget, cfg.pu->bp.fallback_frac,
+                                        F, b.pnf, s));
+        if (budget_out)
+            PK_CHECK(hipMemcpyAsync(budget_out, b.pnf, sizeof(uint32_t) * m,
+                                    budget_out_host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s));
+        nf = b.pnf;
+    }
+    // fine order for the walk (rows of budget-scan queries are skipped)
+    PK_CHECK(pk_launch_sort_rows(b.fdist, m, F, F, F, b.fsd, b.fsi, nullptr, 0, nf, s));
+    if (ev) PK_CHECK(hipEventRecord(ev[2], s));
+    PkWalkArgs wa;
+    wa.q = b.q; wa.qn = b.qn; wa.qstatus = b.qst; wa.csel_d = b.csd; wa.csel_id = b.csi; wa.fsort_d = b.fsd;
+    wa.fsort_id = b.fsi; wa.codebook = ix->codebook; wa.stationary = ix->stationary;
+    wa.cell_start = ix->cell_start; wa.mdo = ix->mdo; wa.rec_off = ix->rec_off; wa.rec_codes = ix->rec_codes;
+    wa.dim = d; wa.C = C; wa.F = F; wa.nsq = ix->p.filter_nsq; wa.lsq = ix->lsq; wa.scc = cfg.scc; wa.ftk = cfg.ftk;
+    wa.rr = cfg.rr; wa.out_t = ft; wa.out_id = fi; wa.out_n = fn; wa.stats = b.stats; wa.qlist = nullptr;
+    wa.flags = b.flags; wa.exact = 0; wa.tie_ids = b.tie_ids; wa.tie_n = b.tie_n;
+    wa.q_scc = q_scc; wa.q_nf = nf; wa.cstride = cfg.keep ? cfg.keep : 1; wa.fdist = b.fdist;
+    PK_CHECK(pk_launch_walk(&wa, m, s));
+    if (ev) PK_CHECK(hipEventRecord(ev[3], s));
+    PkRerankArgs ra;
+    ra.q = b.q; ra.qstatus = b.qst; ra.filt_t = ft; ra.filt_id = fi; ra.filt_n = fn;
+    ra.raw = ix->raw; ra.mem2local = ix->mem2local; ra.dim = d; ra.ftk = cfg.ftk; ra.topk = cfg.topk;
+    ra.out_d = od; ra.out_id = oi; ra.out_n = on;
+    ra.qlist = nullptr; ra.flags = b.flags; ra.exact = 0; ra.tie_ids = b.tie_ids; ra.tie_n = b.tie_n;
+    ra.q_topk = q_topk; ra.out_stride = cfg.ostride; ra.q_nf = nf; ra.F = F;
+    PK_CHECK(pk_launch_rerank(&ra, m, s));
+    if (ev) PK_CHECK(hipEventRecord(ev[4], s));
+    if (wa_out) *wa_out = wa;
+    if (ra_out) *ra_out = ra;
+    return PUCKGPU_OK;
+}
+
+// After the pass has completed (host flags copied in): queries whose result
+// depends on the order among exactly equal distances are replayed with the
+// reference heaps.  flags: 1 coarse-distance tie in the kept range, 2
+// filter-boundary tie at the final top, 4 exact-distance tie in the top-k, 8 a
+// decision on a fine-distance tie group; 1 and 8 need the reference's center
+// order.  Returns the number of queries replayed (enqueued on s), or < 0.
+int replay_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, const PassBufs& b, uint32_t m,
+                uint8_t* hflags, int replay_all, PkWalkArgs wa, PkRerankArgs ra, std::vector<uint32_t>& hlist,
+                hipStream_t s) {
+    const uint32_t C = ix->p.coarse_cluster_count, F = ix->p.fine_cluster_count;
+    hlist.clear();
+    if (replay_all)
+        for (uint32_t i = 0; i < m; ++i) hflags[i] |= 8u;
+    for (uint32_t i = 0; i < m; ++i)
+        if (hflags[i] & 9u) hlist.push_back(i);
+    const uint32_t ncenter = (uint32_t)hlist.size();
+    for (uint32_t i = 0; i < m; ++i)
+        if ((hflags[i] & 15u) && !(hflags[i] & 9u)) hlist.push_back(i);
+    if (hlist.empty()) return 0;
+    const uint32_t nl = (uint32_t)hlist.size();
+    PK_CHECK(hipMemcpyAsync(b.qlist, hlist.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s));
+    PK_CHECK(pk_launch_exact_center_order(b.cdist, C, b.fdist, F, wa.cstride, cfg.scc, wa.q_scc, b.qlist, ncenter,
+                                          b.csd, b.csi, b.fsd, b.fsi, s));
+    wa.qlist = b.qlist; wa.flags = nullptr; wa.exact = 1; wa.tie_ids = nullptr; wa.tie_n = nullptr;
+    PK_CHECK(pk_launch_walk(&wa, nl, s));
+    ra.qlist = b.qlist; ra.flags = nullptr; ra.exact = 1; ra.tie_ids = nullptr; ra.tie_n = nullptr;
+    PK_CHECK(pk_launch_rerank(&ra, nl, s));
+    return (int)nl;
+}
+
+// Allocates pass buffers for `cap` queries from context scratch slots.
+int scratch_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, uint32_t cap, PassBufs& b) {
+    const uint32_t d = ix->p.dim, C = ix->p.coarse_cluster_count, F = ix->p.fine_cluster_count;
+    const puckgpu_probe* pr = cfg.pu ? cfg.pu->probe : nullptr;
+    const uint32_t keep = cfg.keep ? cfg.keep : 1;
+    int rc = 0;
+    b.q = (float*)scratch(ctx, S_Q, sizeof(float) * cap * d, &rc);
+    b.qn = (float*)scratch(ctx, S_QN, sizeof(float) * cap, &rc);
+    b.qst = (int32_t*)scratch(ctx, S_QST, sizeof(int32_t) * cap, &rc);
+    b.cdist = (float*)scratch(ctx, S_CD, sizeof(float) * cap * C, &rc);
+    b.fdist = (float*)scratch(ctx, S_FD, sizeof(float) * cap * F, &rc);
+    b.csd = (float*)scratch(ctx, S_CSD, sizeof(float) * cap * keep, &rc);
+    b.csi = (uint32_t*)scratch(ctx, S_CSI, sizeof(uint32_t) * cap * keep, &rc);
+    b.fsd = (float*)scratch(ctx, S_FSD, sizeof(float) * cap * F, &rc);
+    b.fsi = (uint32_t*)scratch(ctx, S_FSI, sizeof(uint32_t) * cap * F, &rc);
+    b.flags = (uint8_t*)scratch(ctx, S_FLAGS, cap, &rc);
+    b.qlist = (uint32_t*)scratch(ctx, S_QLIST, sizeof(uint32_t) * cap, &rc);
+    b.tie_ids = (uint32_t*)scratch(ctx, S_TIEID, sizeof(uint32_t) * 16 * cap, &rc);
+    b.tie_n = (uint8_t*)scratch(ctx, S_TIEN, cap, &rc);
+    b.stats = (uint32_t*)scratch(ctx, S_STATS, sizeof(uint32_t) * 4 * cap, &rc);
+    if (pr) {
+        b.pfeat = (float*)scratch(ctx, S_PFEAT, sizeof(float) * cap * pr->D, &rc);
+        b.pwork = scratch(ctx, S_PWORK, pk_probe_work_bytes(cap, pr->D, pr->H), &rc);
+        b.pnf = (uint32_t*)scratch(ctx, S_PNF, sizeof(uint32_t) * cap, &rc);
+        b.py = (float*)scratch(ctx, S_PY, sizeof(float) * cap, &rc);
+        b.pcv = (float*)scratch(ctx, S_PCV, sizeof(float) * cap * (pr->nfine ? pr->nfine : 1), &rc);
+    }
+    return rc;
+}
+
+// Validates the search knobs and derives the pass configuration.
+int make_cfg(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_params* sp, uint32_t topk,
+             uint32_t ostride, const ProbeUse* pu, PassCfg& cfg) {
+    const uint32_t d = ix->p.dim, C = ix->p.coarse_cluster_count, F = ix->p.fine_cluster_count;
+    const uint32_t ftk = sp->filter_topk;
+    if (ftk < topk || ftk == 0 || ftk > pk_walk_ftk_max())
+        return set_err(ctx, PUCKGPU_EINVAL, "filter_topk %u must be in [topk=%u, %u]", ftk, topk, pk_walk_ftk_max());
+    if (!(sp->radius_rate >= 1.0)) return set_err(ctx, PUCKGPU_EINVAL, "radius_rate should >= 1.0");
+    const puckgpu_probe* pr = pu ? pu->probe : nullptr;
+    if (pr && (pr->dim != d || pr->nc > C || pr->nfine > F))
+        return set_err(ctx, PUCKGPU_EINVAL, "probe shape (dim %u, %u coarse, %u fine features) does not fit the index",
+                       pr->dim, pr->nc, pr->nfine);
+    cfg.scc = sp->search_coarse_count < C ? sp->search_coarse_count : C;
+    // sorted coarse entries kept per query: the walk's scc, or the probe's features
+    cfg.keep = pr && pr->nc > cfg.scc ? pr->nc : cfg.scc;
+    if (pr && cfg.keep == 0) cfg.keep = 1;  // the probe's cell features need the nearest coarse list
+    cfg.ftk = ftk;
+    cfg.topk = topk;
+    cfg.ostride = ostride;
+    cfg.rr = sp->radius_rate;
+    cfg.pu = pu;
+    return PUCKGPU_OK;
+}
+
+// Batched search.  `topk` is the largest per-query top-k (the rerank
+// capacity) and `ostride` the output row stride; q_topk / q_scc / q_nf are
+// optional per-query arrays (host or device as device_ptrs says).
 int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_params* sp,
                 const uint32_t* q_topk_in, const uint32_t* q_scc_in, const uint32_t* q_nf_in, uint32_t n,
                 const float* queries, uint32_t topk, uint32_t ostride, float* distance, uint32_t* labels,
                 uint32_t* result_num, int device_ptrs, const ProbeUse* pu = nullptr) {
     if (n == 0) return PUCKGPU_OK;
     if (!queries || !distance || !labels) return set_err(ctx, PUCKGPU_EINVAL, "null array");
-    const uint32_t ftk = sp->filter_topk;
-    if (ftk < topk || ftk == 0 || ftk > pk_walk_ftk_max())
-        return set_err(ctx, PUCKGPU_EINVAL, "filter_topk %u must be in [topk=%u, %u]", ftk, topk, pk_walk_ftk_max());
-    if (!(sp->radius_rate >= 1.0)) return set_err(ctx, PUCKGPU_EINVAL, "radius_rate should >= 1.0");
+    PassCfg cfg;
+    int rc = make_cfg(ctx, ix, sp, topk, ostride, pu, cfg);
+    if (rc) return rc;
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
-    const uint32_t d = ix->p.dim, C = ix->p.coarse_cluster_count, F = ix->p.fine_cluster_count,
-                   nsq = ix->p.filter_nsq;
-    const uint32_t scc = sp->search_coarse_count < C ? sp->search_coarse_count : C;
-    const puckgpu_probe* pr = pu ? pu->probe : nullptr;
-    if (pr && (pr->dim != d || pr->nc > C || pr->nfine > F))
-        return set_err(ctx, PUCKGPU_EINVAL, "probe shape (dim %u, %u coarse, %u fine features) does not fit the index",
-                       pr->dim, pr->nc, pr->nfine);
-    // sorted coarse entries kept per query: the walk's scc, or the probe's features
-    uint32_t keep = pr && pr->nc > scc ? pr->nc : scc;
-    if (pr && keep == 0) keep = 1;  // the probe's cell features need the nearest coarse list
+    const uint32_t d = ix->p.dim, ftk = cfg.ftk;
     hipStream_t s = ctx->stream;
-    int rc = 0;
     const uint32_t QB = 16384;  // queries per device pass
     const uint32_t nb = n < QB ? n : QB;
-    float* q = (float*)scratch(ctx, S_Q, sizeof(float) * nb * d, &rc);
-    float* qn = (float*)scratch(ctx, S_QN, sizeof(float) * nb, &rc);
-    int32_t* qst = (int32_t*)scratch(ctx, S_QST, sizeof(int32_t) * nb, &rc);
-    float* cdist = (float*)scratch(ctx, S_CD, sizeof(float) * nb * C, &rc);
-    float* fdist = (float*)scratch(ctx, S_FD, sizeof(float) * nb * F, &rc);
-    float* csd = (float*)scratch(ctx, S_CSD, sizeof(float) * nb * (keep ? keep : 1), &rc);
-    uint32_t* csi = (uint32_t*)scratch(ctx, S_CSI, sizeof(uint32_t) * nb * (keep ? keep : 1), &rc);
-    float* pfeat = nullptr;
-    void* pwork = nullptr;
-    uint32_t* pnf = nullptr;
-    if (pr) {
-        pfeat = (float*)scratch(ctx, S_PFEAT, sizeof(float) * nb * pr->D, &rc);
-        pwork = scratch(ctx, S_PWORK, pk_probe_work_bytes(nb, pr->D, pr->H), &rc);
-        pnf = (uint32_t*)scratch(ctx, S_PNF, sizeof(uint32_t) * nb, &rc);
-    }
-    float* fsd = (float*)scratch(ctx, S_FSD, sizeof(float) * nb * F, &rc);
-    uint32_t* fsi = (uint32_t*)scratch(ctx, S_FSI, sizeof(uint32_t) * nb * F, &rc);
+    PassBufs b;
+    if ((rc = scratch_pass(ctx, ix, cfg, nb, b))) return rc;
     float* ft = (float*)scratch(ctx, S_FT, sizeof(float) * (size_t)n * ftk, &rc);
     uint32_t* fi = (uint32_t*)scratch(ctx, S_FI, sizeof(uint32_t) * (size_t)n * ftk, &rc);
     uint32_t* fn = (uint32_t*)scratch(ctx, S_FN, sizeof(uint32_t) * n, &rc);
-    uint32_t* stats = (uint32_t*)scratch(ctx, S_STATS, sizeof(uint32_t) * 4 * nb, &rc);
     if (rc) return rc;
     float* od;
     uint32_t *oi, *on;
@@ -538,11 +704,6 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
     const uint32_t* q_scc = (const uint32_t*)dev_in(ctx, S_QSCC, q_scc_in, q_scc_in ? 4ull * n : 0, device_ptrs, &rc);
     const uint32_t* q_nf = (const uint32_t*)dev_in(ctx, S_QNF, q_nf_in, q_nf_in ? 4ull * n : 0, device_ptrs, &rc);
     if (rc) return rc;
-    uint8_t* flags = (uint8_t*)scratch(ctx, S_FLAGS, nb, &rc);
-    uint32_t* qlist = (uint32_t*)scratch(ctx, S_QLIST, sizeof(uint32_t) * nb, &rc);
-    uint32_t* tie_ids = (uint32_t*)scratch(ctx, S_TIEID, sizeof(uint32_t) * 16 * nb, &rc);
-    uint8_t* tie_n = (uint8_t*)scratch(ctx, S_TIEN, nb, &rc);
-    if (rc) return rc;
     for (int k = 0; k < 5; ++k) ctx->stage_ms[k] = 0;
     for (int k = 0; k < 13; ++k) ctx->counters[k] = 0;
     std::vector<uint32_t> hstats(4 * (size_t)nb);
@@ -550,78 +711,19 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
     std::vector<uint32_t> hlist;
     for (uint32_t q0 = 0; q0 < n; q0 += QB) {
         const uint32_t m = (n - q0) < QB ? (n - q0) : QB;
-        PK_CHECK(hipMemsetAsync(flags, 0, m, s));
-        PK_CHECK(hipEventRecord(ctx->ev[0], s));
-        PK_CHECK(pk_launch_prep_queries(qin + (size_t)q0 * d, m, d, ix->p.whether_norm, q, qn, qst, s));
-        PK_CHECK(pk_launch_gemm_exact(q, ix->coarse, cdist, m, C, d, C, PK_EPI_SUB_FROM_BIAS, ix->coarse_norms, s));
-        PK_CHECK(pk_launch_gemm_exact(q, ix->fine, fdist, m, F, d, F, PK_EPI_NEG, nullptr, s));
-        PK_CHECK(hipEventRecord(ctx->ev[1], s));
-        PK_CHECK(coarse_select(cdist, m, C, keep, csd, csi, flags, scc, s));
-        const uint32_t* nf_chunk = q_nf ? q_nf + q0 : nullptr;
-        if (pr) {
-            // learned fine budget: features -> MLP -> budget, on this stream
-            if ((rc = probe_features_chunk(ctx, ix, pr, m, q, csd, csi, keep, fdist, pfeat, s))) return rc;
-            float* yb = (float*)scratch(ctx, S_PY, sizeof(float) * nb, &rc);
-            if (!yb) return rc;
-            PK_CHECK(pk_probe_pass(pr->params, pr->wbf, pr->D, pr->H, m, pfeat, nullptr, pwork, nullptr, nullptr, yb,
-                                   s));
-            PK_CHECK(pk_launch_probe_budget(yb, m, pu->bp.margin, pu->bp.min_budget, pu->bp.fallback_frac, F, pnf,
-                                            s));
-            if (pu->budget_out)
-                PK_CHECK(hipMemcpyAsync(pu->budget_out + q0, pnf, sizeof(uint32_t) * m,
-                                        device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
-            nf_chunk = pnf;
-        }
-        // fine order for the walk (rows of budget-scan queries are skipped)
-        PK_CHECK(pk_launch_sort_rows(fdist, m, F, F, F, fsd, fsi, nullptr, 0, nf_chunk, s));
-        PK_CHECK(hipEventRecord(ctx->ev[2], s));
         PkWalkArgs wa;
-        wa.q = q; wa.qn = qn; wa.qstatus = qst; wa.csel_d = csd; wa.csel_id = csi; wa.fsort_d = fsd;
-        wa.fsort_id = fsi; wa.codebook = ix->codebook; wa.stationary = ix->stationary;
-        wa.cell_start = ix->cell_start; wa.mdo = ix->mdo; wa.rec_off = ix->rec_off; wa.rec_codes = ix->rec_codes;
-        wa.dim = d; wa.C = C; wa.F = F; wa.nsq = nsq; wa.lsq = ix->lsq; wa.scc = scc; wa.ftk = ftk;
-        wa.rr = sp->radius_rate; wa.out_t = ft + (size_t)q0 * ftk; wa.out_id = fi + (size_t)q0 * ftk;
-        wa.out_n = fn + q0; wa.stats = stats; wa.qlist = nullptr; wa.flags = flags; wa.exact = 0;
-        wa.tie_ids = tie_ids; wa.tie_n = tie_n;
-        wa.q_scc = q_scc ? q_scc + q0 : nullptr; wa.q_nf = nf_chunk; wa.cstride = keep ? keep : 1;
-        wa.fdist = fdist;
-        PK_CHECK(pk_launch_walk(&wa, m, s));
-        PK_CHECK(hipEventRecord(ctx->ev[3], s));
         PkRerankArgs ra;
-        ra.q = q; ra.qstatus = qst; ra.filt_t = wa.out_t; ra.filt_id = wa.out_id; ra.filt_n = wa.out_n;
-        ra.raw = ix->raw; ra.mem2local = ix->mem2local; ra.dim = d; ra.ftk = ftk; ra.topk = topk;
-        ra.out_d = od + (size_t)q0 * ostride; ra.out_id = oi + (size_t)q0 * ostride; ra.out_n = on + q0;
-        ra.qlist = nullptr; ra.flags = flags; ra.exact = 0; ra.tie_ids = tie_ids; ra.tie_n = tie_n;
-        ra.q_topk = q_topk ? q_topk + q0 : nullptr; ra.out_stride = ostride;
-        ra.q_nf = nf_chunk; ra.F = F;
-        PK_CHECK(pk_launch_rerank(&ra, m, s));
-        PK_CHECK(hipEventRecord(ctx->ev[4], s));
-        PK_CHECK(hipMemcpyAsync(hstats.data(), stats, sizeof(uint32_t) * 4 * m, hipMemcpyDeviceToHost, s));
-        PK_CHECK(hipMemcpyAsync(hflags.data(), flags, m, hipMemcpyDeviceToHost, s));
+        rc = enqueue_pass(ctx, ix, cfg, b, m, qin + (size_t)q0 * d, nullptr, q_topk ? q_topk + q0 : nullptr,
+                          q_scc ? q_scc + q0 : nullptr, q_nf ? q_nf + q0 : nullptr, ft + (size_t)q0 * ftk,
+                          fi + (size_t)q0 * ftk, fn + q0, od + (size_t)q0 * ostride, oi + (size_t)q0 * ostride,
+                          on + q0, pu && pu->budget_out ? pu->budget_out + q0 : nullptr, !device_ptrs, ctx->ev, &wa,
+                          &ra, s);
+        if (rc) return rc;
+        PK_CHECK(hipMemcpyAsync(hstats.data(), b.stats, sizeof(uint32_t) * 4 * m, hipMemcpyDeviceToHost, s));
+        PK_CHECK(hipMemcpyAsync(hflags.data(), b.flags, m, hipMemcpyDeviceToHost, s));
         PK_CHECK(hipStreamSynchronize(s));
-        // Queries whose result depends on the order among exactly equal
-        // distances are replayed with the reference heaps.
-        // flags: 1 coarse-distance tie in the kept range, 2 filter-boundary tie
-        // at the final top, 4 exact-distance tie in the top-k, 8 a decision on
-        // a fine-distance tie group.  1 and 8 need the reference's center order.
-        hlist.clear();
-        if (sp->replay_all)
-            for (uint32_t i = 0; i < m; ++i) hflags[i] |= 8u;
-        for (uint32_t i = 0; i < m; ++i)
-            if (hflags[i] & 9u) hlist.push_back(i);
-        const uint32_t ncenter = (uint32_t)hlist.size();
-        for (uint32_t i = 0; i < m; ++i)
-            if ((hflags[i] & 15u) && !(hflags[i] & 9u)) hlist.push_back(i);
-        if (!hlist.empty()) {
-            const uint32_t nl = (uint32_t)hlist.size();
-            PK_CHECK(hipMemcpyAsync(qlist, hlist.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s));
-            PK_CHECK(pk_launch_exact_center_order(cdist, C, fdist, F, wa.cstride, scc, wa.q_scc, qlist, ncenter,
-                                                  csd, csi, fsd, fsi, s));
-            wa.qlist = qlist; wa.flags = nullptr; wa.exact = 1; wa.tie_ids = nullptr; wa.tie_n = nullptr;
-            PK_CHECK(pk_launch_walk(&wa, nl, s));
-            ra.qlist = qlist; ra.flags = nullptr; ra.exact = 1; ra.tie_ids = nullptr; ra.tie_n = nullptr;
-            PK_CHECK(pk_launch_rerank(&ra, nl, s));
-        }
+        const int nrep = replay_pass(ctx, ix, cfg, b, m, hflags.data(), sp->replay_all, wa, ra, hlist, s);
+        if (nrep < 0) return nrep;
         PK_CHECK(hipEventRecord(ctx->ev[5], s));
         PK_CHECK(hipStreamSynchronize(s));
         for (int k = 0; k < 5; ++k) {
@@ -631,10 +733,10 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
         }
         for (uint32_t i = 0; i < m; ++i)
             for (int k = 0; k < 4; ++k) ctx->counters[k] += hstats[4 * (size_t)i + k];
-        ctx->counters[4] += hlist.size();
+        ctx->counters[4] += (uint64_t)nrep;
         for (uint32_t i = 0; i < m; ++i)
-            for (int b = 0; b < 8; ++b)
-                if (hflags[i] & (1u << b)) ctx->counters[5 + b]++;
+            for (int bb = 0; bb < 8; ++bb)
+                if (hflags[i] & (1u << bb)) ctx->counters[5 + bb]++;
     }
     if (!device_ptrs) {
         PK_CHECK(hipMemcpyAsync(distance, od, sizeof(float) * (size_t)n * ostride, hipMemcpyDeviceToHost, s));
@@ -826,7 +928,9 @@ int puckgpu_probe_features(puckgpu_ctx* ctx, const puckgpu_index* ix, const puck
         PK_CHECK(coarse_select(cdist, m, C, keep, csd, csi, nullptr, 0, s));
         if (fine_order) PK_CHECK(pk_launch_sort_rows(fdist, m, F, F, F, fsd, fsi, nullptr, 0, nullptr, s));
         float* fo = device_ptrs ? feat + (size_t)q0 * p->D : feat;
-        if ((rc = probe_features_chunk(ctx, ix, p, m, q, csd, csi, keep, fdist, fo, s))) return rc;
+        float* cv = (float*)scratch(ctx, S_PCV, sizeof(float) * (size_t)m * (p->nfine ? p->nfine : 1), &rc);
+        if (rc) return rc;
+        if ((rc = probe_features_chunk(ctx, ix, p, m, q, csd, csi, keep, fdist, cv, fo, s))) return rc;
         if (!device_ptrs)
             PK_CHECK(hipMemcpyAsync(features + (size_t)q0 * p->D, fo, sizeof(float) * m * p->D,
                                     hipMemcpyDeviceToHost, s));
@@ -886,6 +990,274 @@ int puckgpu_probe_predict(puckgpu_ctx* ctx, const puckgpu_probe* p, uint32_t n, 
     }
     if (!device_ptrs) PK_CHECK(hipMemcpyAsync(y, yb, sizeof(float) * n, hipMemcpyDeviceToHost, s));
     PK_CHECK(hipStreamSynchronize(s));
+    return PUCKGPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched server
+}  // extern "C"
+
+struct PkBucket {
+    uint32_t cap = 0;
+    int shape = 0;
+    PassBufs b;
+    float* qin = nullptr;
+    uint32_t* qtopk = nullptr;
+    uint32_t* qscc = nullptr;
+    uint32_t* nvalid = nullptr;
+    float* ft = nullptr;
+    uint32_t* fi = nullptr;
+    uint32_t* fn = nullptr;
+    float* od = nullptr;
+    uint32_t* oi = nullptr;
+    uint32_t* on = nullptr;
+    std::vector<void*> allocs;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    PkWalkArgs wa{};
+    PkRerankArgs ra{};
+};
+
+namespace {
+
+template <typename T>
+int balloc(puckgpu_ctx* ctx, PkBucket* bk, T** p, size_t count) {
+    int rc = dalloc(ctx, p, count);
+    if (!rc) bk->allocs.push_back((void*)*p);
+    return rc;
+}
+
+void bucket_release(PkBucket* bk) {
+    if (!bk) return;
+    if (bk->exec) (void)hipGraphExecDestroy(bk->exec);
+    if (bk->graph) (void)hipGraphDestroy(bk->graph);
+    for (void* p : bk->allocs) (void)hipFree(p);
+    delete bk;
+}
+
+int bucket_create(puckgpu_server* sv, uint32_t cap, const PassCfg& cfg, PkBucket** out) {
+    puckgpu_ctx* ctx = sv->ctx;
+    const puckgpu_index* ix = sv->ix;
+    const uint32_t d = ix->p.dim, C = ix->p.coarse_cluster_count, F = ix->p.fine_cluster_count;
+    const uint32_t keep = cfg.keep ? cfg.keep : 1, ftk = cfg.ftk, os = cfg.ostride;
+    PkBucket* bk = new PkBucket();
+    bk->cap = cap;
+    PassBufs& b = bk->b;
+    int rc = 0;
+    if ((rc = balloc(ctx, bk, &b.q, (size_t)cap * d)) || (rc = balloc(ctx, bk, &b.qn, cap)) ||
+        (rc = balloc(ctx, bk, &b.qst, cap)) || (rc = balloc(ctx, bk, &b.cdist, (size_t)cap * C)) ||
+        (rc = balloc(ctx, bk, &b.fdist, (size_t)cap * F)) || (rc = balloc(ctx, bk, &b.csd, (size_t)cap * keep)) ||
+        (rc = balloc(ctx, bk, &b.csi, (size_t)cap * keep)) || (rc = balloc(ctx, bk, &b.fsd, (size_t)cap * F)) ||
+        (rc = balloc(ctx, bk, &b.fsi, (size_t)cap * F)) || (rc = balloc(ctx, bk, &b.flags, cap)) ||
+        (rc = balloc(ctx, bk, &b.tie_ids, (size_t)cap * 16)) || (rc = balloc(ctx, bk, &b.tie_n, cap)) ||
+        (rc = balloc(ctx, bk, &b.stats, (size_t)cap * 4)) || (rc = balloc(ctx, bk, &b.qlist, cap)) ||
+        (rc = balloc(ctx, bk, &bk->qin, (size_t)cap * d)) || (rc = balloc(ctx, bk, &bk->qtopk, cap)) ||
+        (rc = balloc(ctx, bk, &bk->qscc, cap)) || (rc = balloc(ctx, bk, &bk->nvalid, 1)) ||
+        (rc = balloc(ctx, bk, &bk->ft, (size_t)cap * ftk)) || (rc = balloc(ctx, bk, &bk->fi, (size_t)cap * ftk)) ||
+        (rc = balloc(ctx, bk, &bk->fn, cap)) || (rc = balloc(ctx, bk, &bk->od, (size_t)cap * os)) ||
+        (rc = balloc(ctx, bk, &bk->oi, (size_t)cap * os)) || (rc = balloc(ctx, bk, &bk->on, cap))) {
+        bucket_release(bk);
+        return rc;
+    }
+    if (sv->probe) {
+        const puckgpu_probe* pr = sv->probe;
+        void* w = nullptr;
+        if ((rc = balloc(ctx, bk, &b.pfeat, (size_t)cap * pr->D)) || (rc = balloc(ctx, bk, &b.pnf, cap)) ||
+            (rc = balloc(ctx, bk, &b.py, cap)) || (rc = balloc(ctx, bk, &b.pcv, (size_t)cap * (pr->nfine ? pr->nfine : 1))) ||
+            (rc = balloc(ctx, bk, (char**)&w, pk_probe_work_bytes(cap, pr->D, pr->H)))) {
+            bucket_release(bk);
+            return rc;
+        }
+        b.pwork = w;
+    }
+    *out = bk;
+    return PUCKGPU_OK;
+}
+
+// The bucket's pass on its own buffers (eager, or under capture).
+int bucket_enqueue(puckgpu_server* sv, PkBucket* bk, const PassCfg& cfg, hipStream_t s) {
+    return enqueue_pass(sv->ctx, sv->ix, cfg, bk->b, bk->cap, bk->qin, bk->nvalid, bk->qtopk, bk->qscc, nullptr,
+                        bk->ft, bk->fi, bk->fn, bk->od, bk->oi, bk->on, nullptr, 0, nullptr, &bk->wa, &bk->ra, s);
+}
+
+int bucket_capture(puckgpu_server* sv, PkBucket* bk, const PassCfg& cfg) {
+    puckgpu_ctx* ctx = sv->ctx;
+    hipStream_t s = ctx->stream;
+    if (bk->exec) (void)hipGraphExecDestroy(bk->exec);
+    if (bk->graph) (void)hipGraphDestroy(bk->graph);
+    bk->exec = nullptr;
+    bk->graph = nullptr;
+    PK_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = bucket_enqueue(sv, bk, cfg, s);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+    bk->graph = g;
+    PK_CHECK(hipGraphInstantiate(&bk->exec, g, nullptr, nullptr, 0));
+    return PUCKGPU_OK;
+}
+
+int server_cfg(puckgpu_server* sv, PassCfg& cfg, ProbeUse& pu) {
+    pu.probe = sv->probe;
+    pu.bp = sv->bp;
+    pu.budget_out = nullptr;
+    return make_cfg(sv->ctx, sv->ix, &sv->sp, sv->sp.topk_max, sv->sp.topk_max, sv->probe ? &pu : nullptr, cfg);
+}
+
+}  // namespace
+
+extern "C" {
+
+int puckgpu_server_create(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_probe* probe,
+                          const puckgpu_search_params* sp, const puckgpu_budget_params* bp, uint32_t max_batch,
+                          puckgpu_server** out) {
+    if (!ctx || !ix || !sp || !out) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    *out = nullptr;
+    if (max_batch == 0 || max_batch > (1u << 16)) return set_err(ctx, PUCKGPU_EINVAL, "max_batch must be in 1..65536");
+    if (sp->topk_max == 0) return set_err(ctx, PUCKGPU_EINVAL, "topk_max must be positive");
+    if (probe && !bp) return set_err(ctx, PUCKGPU_EINVAL, "a probe needs budget parameters");
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    puckgpu_server* sv = new puckgpu_server();
+    sv->ctx = ctx;
+    sv->ix = ix;
+    sv->sp = *sp;
+    sv->sp.replay_all = 0;
+    sv->probe = probe;
+    if (bp) sv->bp = *bp;
+    sv->max_batch = max_batch;
+    PassCfg cfg;
+    ProbeUse pu;
+    int rc = server_cfg(sv, cfg, pu);
+    for (uint32_t cap = 1; !rc; cap <<= 1) {
+        PkBucket* bk = nullptr;
+        rc = bucket_create(sv, cap < max_batch ? cap : max_batch, cfg, &bk);
+        if (!rc) sv->buckets.push_back(bk);
+        if (cap >= max_batch) break;
+    }
+    if (rc) {
+        puckgpu_server_free(sv);
+        return rc;
+    }
+    *out = sv;
+    return PUCKGPU_OK;
+}
+
+void puckgpu_server_free(puckgpu_server* sv) {
+    if (!sv) return;
+    (void)hipSetDevice(sv->ctx->device);
+    (void)hipStreamSynchronize(sv->ctx->stream);
+    for (PkBucket* bk : sv->buckets) bucket_release(bk);
+    delete sv;
+}
+
+uint32_t puckgpu_server_buckets(const puckgpu_server* sv) { return sv ? (uint32_t)sv->buckets.size() : 0u; }
+
+int puckgpu_server_search(puckgpu_server* sv, uint32_t n, const float* queries, const uint32_t* topk,
+                          const uint32_t* scc, float* distance, uint32_t* labels, uint32_t* result_num,
+                          int device_ptrs) {
+    if (!sv) return PUCKGPU_EINVAL;
+    puckgpu_ctx* ctx = sv->ctx;
+    if (n == 0) return PUCKGPU_OK;
+    if (!queries || !distance || !labels) return set_err(ctx, PUCKGPU_EINVAL, "null array");
+    if (n > sv->max_batch) return set_err(ctx, PUCKGPU_EINVAL, "batch of %u above max_batch %u", n, sv->max_batch);
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    PassCfg cfg;
+    ProbeUse pu;
+    int rc = server_cfg(sv, cfg, pu);
+    if (rc) return rc;
+    PkBucket* bk = nullptr;
+    for (PkBucket* c : sv->buckets)
+        if (c->cap >= n) { bk = c; break; }
+    cfg.gemm_shape = bk->shape;
+    hipStream_t s = ctx->stream;
+    const uint32_t d = sv->ix->p.dim, os = cfg.ostride;
+    const hipMemcpyKind in = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind out = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    PK_CHECK(hipMemcpyAsync(bk->qin, queries, sizeof(float) * (size_t)n * d, in, s));
+    std::vector<uint32_t> fill;
+    if (topk) {
+        PK_CHECK(hipMemcpyAsync(bk->qtopk, topk, sizeof(uint32_t) * n, in, s));
+    } else {
+        fill.assign(n, sv->sp.topk_max);
+        PK_CHECK(hipMemcpyAsync(bk->qtopk, fill.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+    }
+    std::vector<uint32_t> fill2;
+    if (scc) {
+        PK_CHECK(hipMemcpyAsync(bk->qscc, scc, sizeof(uint32_t) * n, in, s));
+    } else {
+        fill2.assign(n, cfg.scc);
+        PK_CHECK(hipMemcpyAsync(bk->qscc, fill2.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+    }
+    uint32_t nv = n;
+    PK_CHECK(hipMemcpyAsync(bk->nvalid, &nv, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    if (!bk->exec) {
+        // one eager pass first (sets kernel attributes outside any capture)
+        if ((rc = bucket_enqueue(sv, bk, cfg, s))) return rc;
+        PK_CHECK(hipStreamSynchronize(s));
+        if ((rc = bucket_capture(sv, bk, cfg))) return rc;
+    }
+    PK_CHECK(hipGraphLaunch(bk->exec, s));
+    std::vector<uint8_t> hflags(n);
+    PK_CHECK(hipMemcpyAsync(hflags.data(), bk->b.flags, n, hipMemcpyDeviceToHost, s));
+    PK_CHECK(hipStreamSynchronize(s));
+    std::vector<uint32_t> hlist;
+    const int nrep = replay_pass(ctx, sv->ix, cfg, bk->b, n, hflags.data(), 0, bk->wa, bk->ra, hlist, s);
+    if (nrep < 0) return nrep;
+    PK_CHECK(hipMemcpyAsync(distance, bk->od, sizeof(float) * (size_t)n * os, out, s));
+    PK_CHECK(hipMemcpyAsync(labels, bk->oi, sizeof(uint32_t) * (size_t)n * os, out, s));
+    if (result_num) PK_CHECK(hipMemcpyAsync(result_num, bk->on, sizeof(uint32_t) * n, out, s));
+    PK_CHECK(hipStreamSynchronize(s));
+    ctx->counters[4] = (uint64_t)nrep;
+    return PUCKGPU_OK;
+}
+
+int puckgpu_server_tune(puckgpu_server* sv, const float* sample, uint32_t reps, int32_t* shapes_out,
+                        float* times_out) {
+    if (!sv || !sample) return sv ? set_err(sv->ctx, PUCKGPU_EINVAL, "null argument") : PUCKGPU_EINVAL;
+    puckgpu_ctx* ctx = sv->ctx;
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    PassCfg cfg;
+    ProbeUse pu;
+    int rc = server_cfg(sv, cfg, pu);
+    if (rc) return rc;
+    hipStream_t s = ctx->stream;
+    const uint32_t d = sv->ix->p.dim, ns = pk_gemm_shapes();
+    if (reps == 0) reps = 1;
+    for (size_t j = 0; j < sv->buckets.size(); ++j) {
+        PkBucket* bk = sv->buckets[j];
+        PK_CHECK(hipMemcpyAsync(bk->qin, sample, sizeof(float) * (size_t)bk->cap * d, hipMemcpyHostToDevice, s));
+        std::vector<uint32_t> t(bk->cap, sv->sp.topk_max), c(bk->cap, cfg.scc);
+        PK_CHECK(hipMemcpyAsync(bk->qtopk, t.data(), sizeof(uint32_t) * bk->cap, hipMemcpyHostToDevice, s));
+        PK_CHECK(hipMemcpyAsync(bk->qscc, c.data(), sizeof(uint32_t) * bk->cap, hipMemcpyHostToDevice, s));
+        PK_CHECK(hipMemcpyAsync(bk->nvalid, &bk->cap, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        float best = 0.f;
+        int bshape = 0;
+        for (uint32_t sh = 0; sh < ns; ++sh) {
+            cfg.gemm_shape = (int)sh;
+            if ((rc = bucket_enqueue(sv, bk, cfg, s))) return rc;  // warm-up
+            PK_CHECK(hipEventRecord(ctx->ev[0], s));
+            for (uint32_t r = 0; r < reps; ++r)
+                if ((rc = bucket_enqueue(sv, bk, cfg, s))) return rc;
+            PK_CHECK(hipEventRecord(ctx->ev[1], s));
+            PK_CHECK(hipEventSynchronize(ctx->ev[1]));
+            float ms = 0.f;
+            PK_CHECK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+            ms /= (float)reps;
+            if (times_out) times_out[j * ns + sh] = ms;
+            if (sh == 0 || ms < best) {
+                best = ms;
+                bshape = (int)sh;
+            }
+        }
+        bk->shape = bshape;
+        if (shapes_out) shapes_out[j] = bshape;
+        cfg.gemm_shape = bshape;
+        if ((rc = bucket_capture(sv, bk, cfg))) return rc;
+    }
     return PUCKGPU_OK;
 }
 

@@ -1,8 +1,7 @@
 // gemm.hip -- center-distance contraction C[m][n] = epi( sum_k A[m][k]*B[n][k] ).
 //
 // pk_gemm_exact: every output is a sequential fmaf chain over k = 0..K-1
-// (the engine's exact order), tiled 64x64x16 through LDS, 4x4 outputs per
-// thread.  Used for query x center distances in the exact search mode, for
+// (the engine's exact order), tiled TM x TN x 16 through LDS.  Used for query x center distances in the exact search mode, for
 // the C x F stationary table and for build-time assignment, where results
 // must match the CPU restatement bit for bit.
 //
@@ -13,57 +12,61 @@
 
 enum { PK_EPI_RAW = 0, PK_EPI_SUB_FROM_BIAS = 1, PK_EPI_NEG = 2, PK_EPI_ADD_BIAS = 3 };
 
-#define GB_M 64
-#define GB_N 64
 #define GB_K 16
 
+// Tile TM x TN per workgroup of 16 x 16 threads, (TM/16) x (TN/16) outputs per
+// thread.  The tile shape changes only which thread owns an output, never the
+// order of its fmaf chain, so every shape gives the same bits.
+template <int TM, int TN>
 __global__ __launch_bounds__(256) void pk_gemm_exact(const float* __restrict__ A, const float* __restrict__ B,
                                                      float* __restrict__ Cout, uint32_t M, uint32_t N,
                                                      uint32_t K, uint64_t ldc, int epi,
                                                      const float* __restrict__ bias) {
-    __shared__ float As[GB_K][GB_M + 4];
-    __shared__ float Bs[GB_K][GB_N + 4];
+    constexpr int RM = TM / 16, RN = TN / 16;
+    __shared__ float As[GB_K][TM + 4];
+    __shared__ float Bs[GB_K][TN + 4];
     const int tid = threadIdx.x;
     const int tx = tid & 15, ty = tid >> 4;
-    const uint32_t m0 = blockIdx.y * GB_M, n0 = blockIdx.x * GB_N;
-    float acc[4][4];
+    const uint32_t m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    float acc[RM][RN];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < RM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+        for (int j = 0; j < RN; ++j) acc[i][j] = 0.0f;
 
     for (uint32_t k0 = 0; k0 < K; k0 += GB_K) {
-        // 64 rows x 16 k of A and B: 1024 elements each, 4 per thread.
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            int idx = tid + e * 256;
-            int r = idx >> 4, kk = idx & 15;
-            uint32_t gm = m0 + r, gn = n0 + r, gk = k0 + kk;
+        for (int idx = tid; idx < TM * GB_K; idx += 256) {
+            const int r = idx >> 4, kk = idx & 15;
+            const uint32_t gm = m0 + r, gk = k0 + kk;
             As[kk][r] = (gm < M && gk < K) ? A[(uint64_t)gm * K + gk] : 0.0f;
+        }
+        for (int idx = tid; idx < TN * GB_K; idx += 256) {
+            const int r = idx >> 4, kk = idx & 15;
+            const uint32_t gn = n0 + r, gk = k0 + kk;
             Bs[kk][r] = (gn < N && gk < K) ? B[(uint64_t)gn * K + gk] : 0.0f;
         }
         __syncthreads();
         const uint32_t kend = (K - k0) < GB_K ? (K - k0) : GB_K;
         for (uint32_t kk = 0; kk < kend; ++kk) {
-            float a[4], b[4];
+            float a[RM], b[RN];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = As[kk][ty * 4 + i];
+            for (int i = 0; i < RM; ++i) a[i] = As[kk][ty * RM + i];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx * 4 + j];
+            for (int j = 0; j < RN; ++j) b[j] = Bs[kk][tx * RN + j];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < RM; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
+                for (int j = 0; j < RN; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        uint32_t gm = m0 + ty * 4 + i;
+    for (int i = 0; i < RM; ++i) {
+        uint32_t gm = m0 + ty * RM + i;
         if (gm >= M) continue;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            uint32_t gn = n0 + tx * 4 + j;
+        for (int j = 0; j < RN; ++j) {
+            uint32_t gn = n0 + tx * RN + j;
             if (gn >= N) continue;
             float v = acc[i][j];
             if (epi == PK_EPI_SUB_FROM_BIAS) v = bias[gn] - v;
@@ -74,13 +77,41 @@ __global__ __launch_bounds__(256) void pk_gemm_exact(const float* __restrict__ A
     }
 }
 
+// Tile shapes: 0 = 64 x 64 (default), 1 = 16 x 128, 2 = 32 x 128, 3 = 128 x 64.
+extern "C" uint32_t pk_gemm_shapes(void) { return 4; }
+
+extern "C" hipError_t pk_launch_gemm_exact_shape(const float* A, const float* B, float* C, uint32_t M, uint32_t N,
+                                                 uint32_t K, uint64_t ldc, int epi, const float* bias, int shape,
+                                                 hipStream_t s) {
+    if (M == 0 || N == 0) return hipSuccess;
+    switch (shape) {
+        case 1: {
+            dim3 g((N + 127) / 128, (M + 15) / 16);
+            hipLaunchKernelGGL((pk_gemm_exact<16, 128>), g, dim3(256), 0, s, A, B, C, M, N, K, ldc, epi, bias);
+            break;
+        }
+        case 2: {
+            dim3 g((N + 127) / 128, (M + 31) / 32);
+            hipLaunchKernelGGL((pk_gemm_exact<32, 128>), g, dim3(256), 0, s, A, B, C, M, N, K, ldc, epi, bias);
+            break;
+        }
+        case 3: {
+            dim3 g((N + 63) / 64, (M + 127) / 128);
+            hipLaunchKernelGGL((pk_gemm_exact<128, 64>), g, dim3(256), 0, s, A, B, C, M, N, K, ldc, epi, bias);
+            break;
+        }
+        default: {
+            dim3 g((N + 63) / 64, (M + 63) / 64);
+            hipLaunchKernelGGL((pk_gemm_exact<64, 64>), g, dim3(256), 0, s, A, B, C, M, N, K, ldc, epi, bias);
+        }
+    }
+    return hipGetLastError();
+}
+
 extern "C" hipError_t pk_launch_gemm_exact(const float* A, const float* B, float* C, uint32_t M, uint32_t N,
                                            uint32_t K, uint64_t ldc, int epi, const float* bias,
                                            hipStream_t s) {
-    if (M == 0 || N == 0) return hipSuccess;
-    dim3 grid((N + GB_N - 1) / GB_N, (M + GB_M - 1) / GB_M);
-    hipLaunchKernelGGL(pk_gemm_exact, grid, dim3(256), 0, s, A, B, C, M, N, K, ldc, epi, bias);
-    return hipGetLastError();
+    return pk_launch_gemm_exact_shape(A, B, C, M, N, K, ldc, epi, bias, 0, s);
 }
 
 // Half squared norms v[i] = dot(x_i, x_i) / 2 (or the plain dot when half == 0).

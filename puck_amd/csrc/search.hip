@@ -70,13 +70,24 @@ struct PkWalkArgs {
 #define PK_TIE_OVERFLOW 255
 
 // ---------------------------------------------------------------------------
+// Rows at or past *nvalid (when given) are padding of a fixed-size batch:
+// status 2, skipped by every later stage.  flags (when given) are cleared,
+// with a kernel rather than a memset so that the pass can be graph-captured.
 __global__ void pk_prep_queries(const float* __restrict__ qin, uint32_t nq, uint32_t dim, int whether_norm,
                                 float* __restrict__ qout, float* __restrict__ qn,
-                                int32_t* __restrict__ status) {
+                                int32_t* __restrict__ status, const uint32_t* __restrict__ nvalid,
+                                uint8_t* __restrict__ flags) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nq) return;
+    if (flags) flags[i] = 0;
     const float* x = qin + (uint64_t)i * dim;
     float* y = qout + (uint64_t)i * dim;
+    if (nvalid && i >= *nvalid) {
+        for (uint32_t k = 0; k < dim; ++k) y[k] = 0.f;
+        qn[i] = 0.f;
+        status[i] = 2;
+        return;
+    }
     int st = 0;
     if (whether_norm) {
         float norm = __fsqrt_rn(pk_dot(x, x, dim));
@@ -94,12 +105,18 @@ __global__ void pk_prep_queries(const float* __restrict__ qin, uint32_t nq, uint
     status[i] = st;
 }
 
-extern "C" hipError_t pk_launch_prep_queries(const float* qin, uint32_t nq, uint32_t dim, int whether_norm,
-                                             float* qout, float* qn, int32_t* status, hipStream_t s) {
+extern "C" hipError_t pk_launch_prep_queries_ex(const float* qin, uint32_t nq, uint32_t dim, int whether_norm,
+                                                float* qout, float* qn, int32_t* status, const uint32_t* nvalid,
+                                                uint8_t* flags, hipStream_t s) {
     if (nq == 0) return hipSuccess;
     hipLaunchKernelGGL(pk_prep_queries, dim3((nq + 127) / 128), dim3(128), 0, s, qin, nq, dim, whether_norm,
-                       qout, qn, status);
+                       qout, qn, status, nvalid, flags);
     return hipGetLastError();
+}
+
+extern "C" hipError_t pk_launch_prep_queries(const float* qin, uint32_t nq, uint32_t dim, int whether_norm,
+                                             float* qout, float* qn, int32_t* status, hipStream_t s) {
+    return pk_launch_prep_queries_ex(qin, nq, dim, whether_norm, qout, qn, status, nullptr, nullptr, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -387,3 +387,77 @@ def search_learned(index, probe, queries, topk_max, search_coarse_count, filter_
                                            ctypes.byref(qp), n, _ptr(queries), _ptr(D), _ptr(I), _ptr(R), _ptr(B),
                                            m))
     return D, I, R, B
+
+
+class Server:
+    """Batched serving (include/puck_gpu.h, puckgpu_server_*): per-bucket HIP
+    graphs of the whole pipeline, optional learned budgets, autotuned GEMM
+    tiles."""
+
+    def __init__(self, index, topk_max, search_coarse_count, filter_topk, max_batch=4096, radius_rate=1.0,
+                 probe=None, margin=0.0, min_budget=1, fallback_frac=1.0):
+        self.index, self.ctx = index, index.ctx
+        self.topk_max = topk_max
+        sp = _lib.SearchParams(topk_max, search_coarse_count, filter_topk, radius_rate, 0)
+        bp = _lib.BudgetParams(margin, min_budget, fallback_frac)
+        h = ctypes.c_void_p()
+        self.ctx.check(self.ctx.L.puckgpu_server_create(self.ctx.h, index.h, probe.h if probe else None,
+                                                        ctypes.byref(sp), ctypes.byref(bp), max_batch,
+                                                        ctypes.byref(h)))
+        self.h = h
+        self.probe = probe
+        self.max_batch = max_batch
+        self.buckets = self.ctx.L.puckgpu_server_buckets(h)
+
+    def free(self):
+        if getattr(self, "h", None):
+            self.ctx.L.puckgpu_server_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def search(self, queries, topk=None, scc=None, out=None):
+        n = queries.shape[0]
+        dev = _is_dev(queries)
+
+        def arr(a):
+            if a is None:
+                return None
+            if dev:
+                import torch
+                if not _is_dev(a):
+                    a = torch.as_tensor(np.asarray(a, np.uint32).view(np.int32), device=queries.device)
+                return a.to(torch.int32).contiguous()
+            return np.ascontiguousarray(a, np.uint32)
+
+        qt, qs = arr(topk), arr(scc)
+        if out is None:
+            if dev:
+                import torch
+                dv = queries.device
+                out = (torch.empty((n, self.topk_max), dtype=torch.float32, device=dv),
+                       torch.empty((n, self.topk_max), dtype=torch.int32, device=dv),
+                       torch.empty(n, dtype=torch.int32, device=dv))
+            else:
+                out = (np.zeros((n, self.topk_max), np.float32), np.zeros((n, self.topk_max), np.uint32),
+                       np.zeros(n, np.uint32))
+        D, I, R = out
+        m = _mode(queries, D, I, R, qt, qs)
+        self.ctx.sync_torch(queries, qt, qs)
+        self.ctx.check(self.ctx.L.puckgpu_server_search(self.h, n, _ptr(queries), _ptr(qt), _ptr(qs), _ptr(D),
+                                                        _ptr(I), _ptr(R), m))
+        return D, I, R
+
+    def tune(self, sample, reps=3):
+        """sample: host float32 [max_batch, dim].  Returns (shapes [buckets],
+        times [buckets, shapes] ms)."""
+        sample = np.ascontiguousarray(sample, np.float32)
+        nsh = 4
+        shapes = np.zeros(self.buckets, np.int32)
+        times = np.zeros((self.buckets, nsh), np.float32)
+        self.ctx.check(self.ctx.L.puckgpu_server_tune(self.h, _ptr(sample), reps, _ptr(shapes), _ptr(times)))
+        return shapes, times

@@ -222,3 +222,32 @@ def test_kmeans_assign_bitexact(ctx):
         ao, do = O.nearest_center(c, x)
         assert np.array_equal(ag, ao)
         assert np.array_equal(_bits(dg), _bits(do))
+
+
+@pytest.mark.parametrize("which", ["l2", "cos"])
+def test_server_matches_search_batch(ctx, which, idx_l2, idx_cos):
+    # graph-captured buckets (padded batches) give the same bits as the plain
+    # batch call, before and after autotuning; uneven batch sizes
+    from puck_amd.engine import Server
+    ix, q = idx_l2 if which == "l2" else idx_cos
+    gi = _gpu_index(ctx, ix)
+    n = q.shape[0]
+    rng = np.random.default_rng(12)
+    topk = rng.choice([1, 10, 100], n).astype(np.uint32)
+    scc = rng.integers(1, 24, n).astype(np.uint32)
+    Dw, Iw, Rw = gi.search_batch(q, 100, 20, 1100, topk=topk, scc=scc)
+    sv = Server(gi, 100, 20, 1100, max_batch=64)
+    assert sv.buckets == 7
+    for tuned in (False, True):
+        if tuned:
+            shapes, times = sv.tune(np.repeat(q, 1, axis=0)[:64], reps=1)
+            assert shapes.shape == (7,) and np.all((shapes >= 0) & (shapes < 4)) and np.all(times > 0)
+        s0 = 0
+        for bs in (1, 3, 17, 2, 41):
+            e = min(n, s0 + bs)
+            D, I, R = sv.search(q[s0:e], topk=topk[s0:e], scc=scc[s0:e])
+            assert np.array_equal(R, Rw[s0:e])
+            assert np.array_equal(I, Iw[s0:e]) and np.array_equal(_bits(D), _bits(Dw[s0:e]))
+            s0 = e
+            if s0 >= n:
+                break
