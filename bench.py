@@ -39,7 +39,8 @@ def log(msg):
 
 def timed(run, steps, warmup, gi):
     """Runs `run()` warmup + steps times; returns wall seconds of the timed
-    steps, per-stage device ms summed over them, walk ms per step, counters."""
+    steps, per-stage device ms summed over them, the dominant launch's ms per
+    step (the budget scan when there is one, else the walk), counters."""
     import torch
     for _ in range(warmup):
         run()
@@ -49,7 +50,8 @@ def timed(run, steps, warmup, gi):
     for _ in range(steps):
         run()
         ms, cnt = gi.last_stage_ms()
-        walk_ms.append(float(ms[2]))
+        scan = gi.last_scan_ms()
+        walk_ms.append(scan if scan >= 0 else float(ms[2]))  # the dominant launch alone
         stage_tot += ms
     torch.cuda.synchronize()
     return time.time() - t1, stage_tot, walk_ms, cnt
@@ -75,6 +77,8 @@ def main():
     ap.add_argument("--no-uniform", action="store_true")
     ap.add_argument("--profile-form", choices=["plain", "learned"], default=None,
                     help="time only this form (for profiler runs); needs --scc (and --margin for learned)")
+    ap.add_argument("--no-serving", action="store_true", help="skip the batched serving workload")
+    ap.add_argument("--serving-queries", type=int, default=100_000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
@@ -298,11 +302,72 @@ def main():
                                                    "fine_tie_orders_disagree"], cnt[4:13])},
         "build_s": round(build_s, 1),
     }
+    if not args.no_serving and args.profile_form is None:
+        result["batched_serving"] = serving(gi, dev, d, args.serving_queries)
     if not args.no_cpu:
         gi.search(q, topk, chosen, ftk, out=out)
         result["cpu_baseline"] = cpu_baseline(ix, q.cpu().numpy(), topk, chosen, ftk, args.cpu_seconds, out,
                                               gt_i, gt_d)
     print(json.dumps(result), flush=True)
+
+
+def serving(gi, dev, d, nq_total, max_batch=4096):
+    """Batched serving on the same index: nq_total queries (seed 4) in batches
+    of size floor(exp(U[0, ln 4096])), per-query top-k in {1, 10, 100} and
+    search_coarse_count uniform in 20..400, topk_max 100, filter_topk 1100
+    (the reference's default 11 x FLAGS_topk).  Served by the graph server
+    (autotuned GEMM tiles) and, for comparison, by one eager batch call per
+    batch; queries and outputs stay in HBM."""
+    import torch
+    from puck_amd import datasets
+    from puck_amd.engine import Server
+    qs = datasets.unit_norm_mixture(nq_total, d, 4096, seed=4, means_seed=2000, backend="torch", device=dev)
+    rng = np.random.default_rng(4)
+    sizes, tot = [], 0
+    while tot < nq_total:
+        b = min(int(np.floor(np.exp(rng.uniform(0.0, np.log(max_batch))))), nq_total - tot)
+        sizes.append(b)
+        tot += b
+    topk = torch.as_tensor(rng.choice([1, 10, 100], nq_total).astype(np.int32), device=dev)
+    scc = torch.as_tensor(rng.integers(20, 401, nq_total).astype(np.int32), device=dev)
+    t0 = time.time()
+    sv = Server(gi, 100, 400, 1100, max_batch=max_batch)
+    shapes, times = sv.tune(qs[:max_batch].cpu().numpy(), reps=3)
+    D = torch.empty((max_batch, 100), dtype=torch.float32, device=dev)
+    I = torch.empty((max_batch, 100), dtype=torch.int32, device=dev)
+    R = torch.empty(max_batch, dtype=torch.int32, device=dev)
+    for j in range(sv.buckets):  # capture every bucket's graph before timing
+        c = min(1 << j, max_batch)
+        sv.search(qs[:c], topk[:c], scc[:c], out=(D[:c], I[:c], R[:c]))
+    torch.cuda.synchronize()
+    setup_s = time.time() - t0
+
+    def run(fn):
+        lat, s0 = [], 0
+        torch.cuda.synchronize()
+        t1 = time.time()
+        for b in sizes:
+            t2 = time.time()
+            fn(s0, b)
+            lat.append(time.time() - t2)
+            s0 += b
+        torch.cuda.synchronize()
+        el = time.time() - t1
+        lat = np.array(lat) * 1000.0
+        return {"qps": round(nq_total / el, 1), "p50_batch_ms": round(float(np.percentile(lat, 50)), 3),
+                "p99_batch_ms": round(float(np.percentile(lat, 99)), 3), "wall_s": round(el, 3)}
+
+    graph = run(lambda s0, b: sv.search(qs[s0:s0 + b], topk[s0:s0 + b], scc[s0:s0 + b],
+                                        out=(D[:b], I[:b], R[:b])))
+    eager = run(lambda s0, b: gi.search_batch(qs[s0:s0 + b], 100, 400, 1100, topk=topk[s0:s0 + b],
+                                              scc=scc[s0:s0 + b], out=(D[:b], I[:b], R[:b])))
+    log(f"serving: graph {graph} eager {eager} shapes {shapes.tolist()}")
+    sv.free()
+    return {"workload": f"{nq_total} queries in {len(sizes)} batches of floor(exp(U[0, ln {max_batch}])) "
+                        "queries, topk in {1,10,100}, search_coarse_count in 20..400, filter_topk 1100",
+            "graph_server": graph, "eager_batches": eager, "setup_s": round(setup_s, 1),
+            "autotuned_gemm_shape_per_bucket": shapes.tolist(),
+            "gemm_ms_per_bucket_and_shape": np.round(times, 4).tolist()}
 
 
 def cpu_baseline(ix, qh, topk, scc, ftk, budget_s, gpu_out, gt_i, gt_d):

@@ -61,6 +61,14 @@ typedef struct {
     uint32_t filter_topk;         /* >= topk, <= puckgpu_filter_topk_max() */
     double radius_rate;           /* >= 1.0 */
     uint32_t replay_all;          /* non-zero: replay every query with the reference heaps (tests) */
+    uint32_t precision;           /* center distances: 0 = FP32 in the engine's exact order (the default);
+                                     1 = BF16, 2 = per-row scaled FP8 e4m3, on the matrix cores with FP32
+                                     accumulation (dim <= 256).  With 1 or 2 the coarse candidates near the
+                                     search_coarse_count boundary are rescored in the exact FP32 order and
+                                     the rerank stays exact FP32, so returned distances are exact; the
+                                     fine distances, cell distances and filter distances carry the reduced
+                                     precision and results are not bit-identical to the reference.  No
+                                     tie replay is made in these forms. */
 } puckgpu_search_params;
 
 /* Context: one device, one stream, reusable scratch.  device = HIP ordinal. */
@@ -208,11 +216,11 @@ uint32_t puckgpu_server_buckets(const puckgpu_server* server);
 int puckgpu_server_search(puckgpu_server* server, uint32_t n, const float* queries, const uint32_t* topk,
                           const uint32_t* search_coarse_count, float* distance, uint32_t* labels,
                           uint32_t* result_num, int device_ptrs);
-/* Autotuner: for every bucket, times each GEMM tile shape on the first cap
- * rows of sample [max_batch x dim] (host) over reps passes and keeps the
- * fastest (every shape gives the same bits); graphs are re-captured.
- * shapes_out [buckets] and times_out [buckets x shapes] (ms per pass) may be
- * NULL. */
+/* Autotuner: for every bucket, times the two center GEMMs in each tile shape
+ * on the first cap rows of sample [max_batch x dim] (host) over reps
+ * repetitions and keeps the fastest shape (every shape gives the same bits);
+ * graphs are re-captured.  shapes_out [buckets] and times_out [buckets x
+ * shapes] (ms per repetition) may be NULL. */
 int puckgpu_server_tune(puckgpu_server* server, const float* sample, uint32_t reps, int32_t* shapes_out,
                         float* times_out);
 
@@ -242,6 +250,16 @@ int puckgpu_last_filter(puckgpu_ctx* ctx, uint32_t n, uint32_t filter_topk, floa
  * for the last cause, its detail (tie with an earlier cell, tie across a
  * chunk boundary, group of three or more, visiting orders disagree). */
 int puckgpu_last_stage_ms(puckgpu_ctx* ctx, float* ms5, uint64_t* counters13);
+
+/* Query x center distances of n queries as the search computes them in the
+ * given precision (0 exact, 1 BF16, 2 FP8; see puckgpu_search_params):
+ * coarse_out [n x C] = |c|^2/2 - <c,q>, fine_out [n x F] = -<f,q>, on the
+ * normalised query under whether_norm.  Test and measurement hook. */
+int puckgpu_center_distances(puckgpu_ctx* ctx, const puckgpu_index* index, uint32_t precision, uint32_t n,
+                             const float* queries, float* coarse_out, float* fine_out, int device_ptrs);
+/* Device time (ms) of the budget-scan launch of the last search (part of
+ * stage [2]); -1 when the search launched no budget scan. */
+int puckgpu_last_scan_ms(puckgpu_ctx* ctx, float* scan_ms);
 
 /* Filter lookup table for n queries (test hook): out [n x nsq x 256]. */
 int puckgpu_lut(puckgpu_ctx* ctx, uint32_t n, uint32_t dim, const float* queries, const float* filter_codebook,

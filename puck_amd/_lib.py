@@ -18,7 +18,8 @@ EXPORTS = [
     "puckgpu_probe_create", "puckgpu_probe_free", "puckgpu_probe_param_count", "puckgpu_probe_set_params",
     "puckgpu_probe_get_params", "puckgpu_probe_features", "puckgpu_probe_train_step", "puckgpu_probe_predict",
     "puckgpu_search_learned", "puckgpu_budget_ranks", "puckgpu_server_create", "puckgpu_server_free",
-    "puckgpu_server_buckets", "puckgpu_server_search", "puckgpu_server_tune",
+    "puckgpu_server_buckets", "puckgpu_server_search", "puckgpu_server_tune", "puckgpu_last_scan_ms",
+    "puckgpu_center_distances",
 ]
 
 
@@ -31,7 +32,7 @@ class IndexParams(ctypes.Structure):
 class SearchParams(ctypes.Structure):
     _fields_ = [("topk_max", ctypes.c_uint32), ("search_coarse_count", ctypes.c_uint32),
                 ("filter_topk", ctypes.c_uint32), ("radius_rate", ctypes.c_double),
-                ("replay_all", ctypes.c_uint32)]
+                ("replay_all", ctypes.c_uint32), ("precision", ctypes.c_uint32)]
 
 
 class QueryParams(ctypes.Structure):
@@ -138,6 +139,10 @@ def load():
     L.puckgpu_server_search.restype = c_int
     L.puckgpu_server_tune.argtypes = [vp, vp, u32, vp, vp]
     L.puckgpu_server_tune.restype = c_int
+    L.puckgpu_last_scan_ms.argtypes = [vp, vp]
+    L.puckgpu_last_scan_ms.restype = c_int
+    L.puckgpu_center_distances.argtypes = [vp, vp, u32, u32, vp, vp, vp, c_int]
+    L.puckgpu_center_distances.restype = c_int
     del i32
     _lib = L
     return L

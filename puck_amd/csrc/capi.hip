@@ -33,8 +33,14 @@ hipError_t pk_launch_prep_queries_ex(const float*, uint32_t, uint32_t, int, floa
 hipError_t pk_launch_gemm_exact_shape(const float*, const float*, float*, uint32_t, uint32_t, uint32_t, uint64_t, int,
                                       const float*, int, hipStream_t);
 uint32_t pk_gemm_shapes(void);
+uint32_t pkc_ld(uint32_t);
+hipError_t pkc_launch_convert(const float*, uint32_t, uint32_t, int, uint16_t*, uint8_t*, float*, hipStream_t);
+hipError_t pkc_launch_center(const void*, const float*, const void*, const float*, uint32_t, uint32_t, uint32_t, int,
+                             int, const float*, float*, uint64_t, hipStream_t);
+hipError_t pkc_launch_rescore_coarse(const float*, uint32_t, const float*, const float*, const uint32_t*, uint32_t,
+                                     uint32_t, uint32_t, float*, uint32_t*, uint8_t*, uint32_t, hipStream_t);
 hipError_t pk_launch_lut(const float*, uint32_t, uint32_t, const float*, uint32_t, uint32_t, float*, hipStream_t);
-hipError_t pk_launch_walk(const PkWalkArgs*, uint32_t, hipStream_t);
+hipError_t pk_launch_walk(const PkWalkArgs*, uint32_t, hipStream_t, hipEvent_t);
 hipError_t pk_launch_rerank(const PkRerankArgs*, uint32_t, hipStream_t);
 uint32_t pk_walk_ftk_max(void);
 uint32_t pk_probe_trained_count(uint32_t, uint32_t);
@@ -138,8 +144,9 @@ struct puckgpu_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     std::vector<DevBuf> bufs;          // named scratch slots
-    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     float stage_ms[5] = {0, 0, 0, 0, 0};
+    float scan_ms = 0;  // budget-scan launch alone (inside stage 2)
     uint64_t counters[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t last_n = 0, last_ftk = 0;
 };
@@ -158,6 +165,13 @@ struct puckgpu_index {
     float* rec_off = nullptr;
     uint8_t* rec_codes = nullptr;
     float* raw = nullptr;
+    // reduced-precision copies of the centers (built on first use)
+    mutable uint16_t* coarse16 = nullptr;
+    mutable uint16_t* fine16 = nullptr;
+    mutable uint8_t* coarse8 = nullptr;
+    mutable uint8_t* fine8 = nullptr;
+    mutable float* coarse8s = nullptr;
+    mutable float* fine8s = nullptr;
 };
 
 struct puckgpu_probe {
@@ -225,7 +239,7 @@ void* scratch(puckgpu_ctx* ctx, size_t slot, size_t bytes, int* rc) {
 enum Slot {
     S_Q = 0, S_QN, S_QST, S_CD, S_FD, S_CSD, S_CSI, S_FSD, S_FSI, S_FT, S_FI, S_FN, S_OD, S_OI, S_ON, S_STATS,
     S_IN_Q, S_A0, S_A1, S_A2, S_A3, S_A4, S_A5, S_A6, S_A7, S_FLAGS, S_QLIST, S_TIEID, S_TIEN, S_QTOPK, S_QSCC,
-    S_QNF, S_PFEAT, S_PWORK, S_PNF, S_PY, S_PT, S_PCV, S_COUNT
+    S_QNF, S_PFEAT, S_PWORK, S_PNF, S_PY, S_PT, S_PCV, S_LP16, S_LP8, S_LPS, S_CAND_D, S_CAND_I, S_COUNT
 };
 
 // Device view of an input array: the pointer itself, or a scratch copy of it.
@@ -251,7 +265,8 @@ int dalloc(puckgpu_ctx* ctx, T** p, size_t count) {
 void index_release(puckgpu_index* ix) {
     if (!ix) return;
     void* ptrs[] = {ix->coarse, ix->fine, ix->codebook, ix->coarse_norms, ix->stationary, ix->cell_start,
-                    ix->mdo, ix->mem2local, ix->rec_off, ix->rec_codes, ix->raw};
+                    ix->mdo, ix->mem2local, ix->rec_off, ix->rec_codes, ix->raw, ix->coarse16, ix->fine16,
+                    ix->coarse8, ix->fine8, ix->coarse8s, ix->fine8s};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete ix;
@@ -509,11 +524,18 @@ struct PassBufs {
     uint32_t* pnf = nullptr;
     float* py = nullptr;
     float* pcv = nullptr;  // probe cell features, cap x nfine
+    uint16_t* q16 = nullptr;   // reduced-precision queries (precision 1 / 2)
+    uint8_t* q8 = nullptr;
+    float* qs = nullptr;
+    float* cand_d = nullptr;   // approximate coarse candidates before rescoring
+    uint32_t* cand_i = nullptr;
 };
 
 // Per-pass configuration.
 struct PassCfg {
     uint32_t scc = 0, keep = 0, ftk = 0, topk = 0, ostride = 0;
+    int precision = 0;
+    uint32_t kc = 0;  // coarse candidates rescored in FP32 (precision 1 / 2)
     double rr = 1.0;
     int gemm_shape = 0;
     const ProbeUse* pu = nullptr;
@@ -534,11 +556,29 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     int rc = 0;
     if (ev) PK_CHECK(hipEventRecord(ev[0], s));
     PK_CHECK(pk_launch_prep_queries_ex(qin, m, d, ix->p.whether_norm, b.q, b.qn, b.qst, nvalid, b.flags, s));
-    PK_CHECK(pk_launch_gemm_exact_shape(b.q, ix->coarse, b.cdist, m, C, d, C, PK_EPI_SUB_FROM_BIAS, ix->coarse_norms,
-                                        cfg.gemm_shape, s));
-    PK_CHECK(pk_launch_gemm_exact_shape(b.q, ix->fine, b.fdist, m, F, d, F, PK_EPI_NEG, nullptr, cfg.gemm_shape, s));
+    if (cfg.precision == 0) {
+        PK_CHECK(pk_launch_gemm_exact_shape(b.q, ix->coarse, b.cdist, m, C, d, C, PK_EPI_SUB_FROM_BIAS,
+                                            ix->coarse_norms, cfg.gemm_shape, s));
+        PK_CHECK(pk_launch_gemm_exact_shape(b.q, ix->fine, b.fdist, m, F, d, F, PK_EPI_NEG, nullptr, cfg.gemm_shape,
+                                            s));
+    } else {
+        const bool bf = cfg.precision == 1;
+        PK_CHECK(pkc_launch_convert(b.q, m, d, cfg.precision, b.q16, b.q8, b.qs, s));
+        const void* qa = bf ? (const void*)b.q16 : (const void*)b.q8;
+        PK_CHECK(pkc_launch_center(qa, b.qs, bf ? (const void*)ix->coarse16 : (const void*)ix->coarse8, ix->coarse8s,
+                                   m, C, d, cfg.precision, PK_EPI_SUB_FROM_BIAS, ix->coarse_norms, b.cdist, C, s));
+        PK_CHECK(pkc_launch_center(qa, b.qs, bf ? (const void*)ix->fine16 : (const void*)ix->fine8, ix->fine8s, m, F,
+                                   d, cfg.precision, PK_EPI_NEG, nullptr, b.fdist, F, s));
+    }
     if (ev) PK_CHECK(hipEventRecord(ev[1], s));
-    PK_CHECK(coarse_select(b.cdist, m, C, cfg.keep, b.csd, b.csi, b.flags, cfg.scc, s));
+    if (cfg.precision == 0) {
+        PK_CHECK(coarse_select(b.cdist, m, C, cfg.keep, b.csd, b.csi, b.flags, cfg.scc, s));
+    } else if (cfg.keep) {
+        // approximate candidates, then the exact FP32 order decides the kept lists
+        PK_CHECK(coarse_select(b.cdist, m, C, cfg.kc, b.cand_d, b.cand_i, nullptr, 0, s));
+        PK_CHECK(pkc_launch_rescore_coarse(b.q, d, ix->coarse, ix->coarse_norms, b.cand_i, cfg.kc, cfg.keep, m, b.csd,
+                                           b.csi, b.flags, cfg.scc, s));
+    }
     const uint32_t* nf = q_nf;
     if (pr) {
         // learned fine budget: features -> MLP -> budget, on this stream
@@ -564,7 +604,7 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     wa.rr = cfg.rr; wa.out_t = ft; wa.out_id = fi; wa.out_n = fn; wa.stats = b.stats; wa.qlist = nullptr;
     wa.flags = b.flags; wa.exact = 0; wa.tie_ids = b.tie_ids; wa.tie_n = b.tie_n;
     wa.q_scc = q_scc; wa.q_nf = nf; wa.cstride = cfg.keep ? cfg.keep : 1; wa.fdist = b.fdist;
-    PK_CHECK(pk_launch_walk(&wa, m, s));
+    PK_CHECK(pk_launch_walk(&wa, m, s, ev ? ev[6] : nullptr));
     if (ev) PK_CHECK(hipEventRecord(ev[3], s));
     PkRerankArgs ra;
     ra.q = b.q; ra.qstatus = b.qst; ra.filt_t = ft; ra.filt_id = fi; ra.filt_n = fn;
@@ -603,7 +643,7 @@ int replay_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, c
     PK_CHECK(pk_launch_exact_center_order(b.cdist, C, b.fdist, F, wa.cstride, cfg.scc, wa.q_scc, b.qlist, ncenter,
                                           b.csd, b.csi, b.fsd, b.fsi, s));
     wa.qlist = b.qlist; wa.flags = nullptr; wa.exact = 1; wa.tie_ids = nullptr; wa.tie_n = nullptr;
-    PK_CHECK(pk_launch_walk(&wa, nl, s));
+    PK_CHECK(pk_launch_walk(&wa, nl, s, nullptr));
     ra.qlist = b.qlist; ra.flags = nullptr; ra.exact = 1; ra.tie_ids = nullptr; ra.tie_n = nullptr;
     PK_CHECK(pk_launch_rerank(&ra, nl, s));
     return (int)nl;
@@ -636,7 +676,38 @@ int scratch_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
         b.py = (float*)scratch(ctx, S_PY, sizeof(float) * cap, &rc);
         b.pcv = (float*)scratch(ctx, S_PCV, sizeof(float) * cap * (pr->nfine ? pr->nfine : 1), &rc);
     }
+    if (cfg.precision) {
+        const uint32_t ld = pkc_ld(d), kc = cfg.kc ? cfg.kc : 1;
+        b.q16 = (uint16_t*)scratch(ctx, S_LP16, sizeof(uint16_t) * cap * ld, &rc);
+        b.q8 = (uint8_t*)scratch(ctx, S_LP8, (size_t)cap * ld, &rc);
+        b.qs = (float*)scratch(ctx, S_LPS, sizeof(float) * cap, &rc);
+        b.cand_d = (float*)scratch(ctx, S_CAND_D, sizeof(float) * cap * kc, &rc);
+        b.cand_i = (uint32_t*)scratch(ctx, S_CAND_I, sizeof(uint32_t) * cap * kc, &rc);
+    }
     return rc;
+}
+
+// Reduced-precision copies of the index centers (once per precision).
+int ensure_lowp(puckgpu_ctx* ctx, const puckgpu_index* ix, int prec) {
+    const uint32_t d = ix->p.dim, C = ix->p.coarse_cluster_count, F = ix->p.fine_cluster_count, ld = pkc_ld(d);
+    hipStream_t s = ctx->stream;
+    int rc = 0;
+    if (prec == 1 && !ix->coarse16) {
+        if ((rc = dalloc(ctx, &ix->coarse16, (size_t)C * ld)) || (rc = dalloc(ctx, &ix->fine16, (size_t)F * ld)))
+            return rc;
+        PK_CHECK(pkc_launch_convert(ix->coarse, C, d, 1, ix->coarse16, nullptr, nullptr, s));
+        PK_CHECK(pkc_launch_convert(ix->fine, F, d, 1, ix->fine16, nullptr, nullptr, s));
+        PK_CHECK(hipStreamSynchronize(s));
+    }
+    if (prec == 2 && !ix->coarse8) {
+        if ((rc = dalloc(ctx, &ix->coarse8, (size_t)C * ld)) || (rc = dalloc(ctx, &ix->fine8, (size_t)F * ld)) ||
+            (rc = dalloc(ctx, &ix->coarse8s, C)) || (rc = dalloc(ctx, &ix->fine8s, F)))
+            return rc;
+        PK_CHECK(pkc_launch_convert(ix->coarse, C, d, 2, nullptr, ix->coarse8, ix->coarse8s, s));
+        PK_CHECK(pkc_launch_convert(ix->fine, F, d, 2, nullptr, ix->fine8, ix->fine8s, s));
+        PK_CHECK(hipStreamSynchronize(s));
+    }
+    return PUCKGPU_OK;
 }
 
 // Validates the search knobs and derives the pass configuration.
@@ -660,6 +731,19 @@ int make_cfg(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_par
     cfg.ostride = ostride;
     cfg.rr = sp->radius_rate;
     cfg.pu = pu;
+    if (sp->precision > 2) return set_err(ctx, PUCKGPU_EINVAL, "precision must be 0, 1 or 2");
+    cfg.precision = (int)sp->precision;
+    if (cfg.precision) {
+        if (pkc_ld(d) > 256) return set_err(ctx, PUCKGPU_EINVAL, "reduced precision needs dim <= 256");
+        // margin of 32 coarse candidates (at most 512 and C) rescored in FP32
+        uint32_t kc = cfg.keep + 32;
+        if (kc > C) kc = C;
+        if (kc > 512) kc = 512;
+        if (cfg.keep > kc) return set_err(ctx, PUCKGPU_EINVAL, "reduced precision supports search_coarse_count <= 480");
+        cfg.kc = kc;
+        int rc = ensure_lowp(ctx, ix, cfg.precision);
+        if (rc) return rc;
+    }
     return PUCKGPU_OK;
 }
 
@@ -705,6 +789,7 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
     const uint32_t* q_nf = (const uint32_t*)dev_in(ctx, S_QNF, q_nf_in, q_nf_in ? 4ull * n : 0, device_ptrs, &rc);
     if (rc) return rc;
     for (int k = 0; k < 5; ++k) ctx->stage_ms[k] = 0;
+    ctx->scan_ms = 0;
     for (int k = 0; k < 13; ++k) ctx->counters[k] = 0;
     std::vector<uint32_t> hstats(4 * (size_t)nb);
     std::vector<uint8_t> hflags(nb);
@@ -722,7 +807,9 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
         PK_CHECK(hipMemcpyAsync(hstats.data(), b.stats, sizeof(uint32_t) * 4 * m, hipMemcpyDeviceToHost, s));
         PK_CHECK(hipMemcpyAsync(hflags.data(), b.flags, m, hipMemcpyDeviceToHost, s));
         PK_CHECK(hipStreamSynchronize(s));
-        const int nrep = replay_pass(ctx, ix, cfg, b, m, hflags.data(), sp->replay_all, wa, ra, hlist, s);
+        if (cfg.precision) std::fill(hflags.begin(), hflags.begin() + m, (uint8_t)0);  // no replay in these forms
+        const int nrep = replay_pass(ctx, ix, cfg, b, m, hflags.data(), cfg.precision ? 0 : sp->replay_all, wa, ra,
+                                     hlist, s);
         if (nrep < 0) return nrep;
         PK_CHECK(hipEventRecord(ctx->ev[5], s));
         PK_CHECK(hipStreamSynchronize(s));
@@ -731,6 +818,11 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
             PK_CHECK(hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]));
             ctx->stage_ms[k] += ms;
         }
+        {
+            float ms = 0;
+            PK_CHECK(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[6]));
+            ctx->scan_ms += ms;
+        }
         for (uint32_t i = 0; i < m; ++i)
             for (int k = 0; k < 4; ++k) ctx->counters[k] += hstats[4 * (size_t)i + k];
         ctx->counters[4] += (uint64_t)nrep;
@@ -738,6 +830,7 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
             for (int bb = 0; bb < 8; ++bb)
                 if (hflags[i] & (1u << bb)) ctx->counters[5 + bb]++;
     }
+    if (!q_nf && !(pu && pu->probe)) ctx->scan_ms = -1.f;  // no budget scan was launched
     if (!device_ptrs) {
         PK_CHECK(hipMemcpyAsync(distance, od, sizeof(float) * (size_t)n * ostride, hipMemcpyDeviceToHost, s));
         PK_CHECK(hipMemcpyAsync(labels, oi, sizeof(uint32_t) * (size_t)n * ostride, hipMemcpyDeviceToHost, s));
@@ -1059,6 +1152,15 @@ int bucket_create(puckgpu_server* sv, uint32_t cap, const PassCfg& cfg, PkBucket
         bucket_release(bk);
         return rc;
     }
+    if (cfg.precision) {
+        const uint32_t ld = pkc_ld(d);
+        if ((rc = balloc(ctx, bk, &b.q16, (size_t)cap * ld)) || (rc = balloc(ctx, bk, &b.q8, (size_t)cap * ld)) ||
+            (rc = balloc(ctx, bk, &b.qs, cap)) || (rc = balloc(ctx, bk, &b.cand_d, (size_t)cap * cfg.kc)) ||
+            (rc = balloc(ctx, bk, &b.cand_i, (size_t)cap * cfg.kc))) {
+            bucket_release(bk);
+            return rc;
+        }
+    }
     if (sv->probe) {
         const puckgpu_probe* pr = sv->probe;
         void* w = nullptr;
@@ -1205,6 +1307,7 @@ int puckgpu_server_search(puckgpu_server* sv, uint32_t n, const float* queries, 
     PK_CHECK(hipMemcpyAsync(hflags.data(), bk->b.flags, n, hipMemcpyDeviceToHost, s));
     PK_CHECK(hipStreamSynchronize(s));
     std::vector<uint32_t> hlist;
+    if (cfg.precision) std::fill(hflags.begin(), hflags.end(), (uint8_t)0);
     const int nrep = replay_pass(ctx, sv->ix, cfg, bk->b, n, hflags.data(), 0, bk->wa, bk->ra, hlist, s);
     if (nrep < 0) return nrep;
     PK_CHECK(hipMemcpyAsync(distance, bk->od, sizeof(float) * (size_t)n * os, out, s));
@@ -1236,12 +1339,21 @@ int puckgpu_server_tune(puckgpu_server* sv, const float* sample, uint32_t reps, 
         PK_CHECK(hipMemcpyAsync(bk->nvalid, &bk->cap, sizeof(uint32_t), hipMemcpyHostToDevice, s));
         float best = 0.f;
         int bshape = 0;
+        const uint32_t C = sv->ix->p.coarse_cluster_count, F = sv->ix->p.fine_cluster_count;
+        PK_CHECK(pk_launch_prep_queries_ex(bk->qin, bk->cap, d, sv->ix->p.whether_norm, bk->b.q, bk->b.qn,
+                                           bk->b.qst, nullptr, nullptr, s));
         for (uint32_t sh = 0; sh < ns; ++sh) {
-            cfg.gemm_shape = (int)sh;
-            if ((rc = bucket_enqueue(sv, bk, cfg, s))) return rc;  // warm-up
+            // the tuned stage: both center GEMMs of the bucket
+            auto gemms = [&]() -> hipError_t {
+                hipError_t e = pk_launch_gemm_exact_shape(bk->b.q, sv->ix->coarse, bk->b.cdist, bk->cap, C, d, C,
+                                                          PK_EPI_SUB_FROM_BIAS, sv->ix->coarse_norms, (int)sh, s);
+                if (e != hipSuccess) return e;
+                return pk_launch_gemm_exact_shape(bk->b.q, sv->ix->fine, bk->b.fdist, bk->cap, F, d, F, PK_EPI_NEG,
+                                                  nullptr, (int)sh, s);
+            };
+            PK_CHECK(gemms());  // warm-up
             PK_CHECK(hipEventRecord(ctx->ev[0], s));
-            for (uint32_t r = 0; r < reps; ++r)
-                if ((rc = bucket_enqueue(sv, bk, cfg, s))) return rc;
+            for (uint32_t r = 0; r < reps; ++r) PK_CHECK(gemms());
             PK_CHECK(hipEventRecord(ctx->ev[1], s));
             PK_CHECK(hipEventSynchronize(ctx->ev[1]));
             float ms = 0.f;
@@ -1278,6 +1390,56 @@ int puckgpu_last_stage_ms(puckgpu_ctx* ctx, float* ms5, uint64_t* counters13) {
         if (ms5) ms5[k] = ctx->stage_ms[k];
     for (int k = 0; k < 13; ++k)
         if (counters13) counters13[k] = ctx->counters[k];
+    return PUCKGPU_OK;
+}
+
+int puckgpu_center_distances(puckgpu_ctx* ctx, const puckgpu_index* ix, uint32_t precision, uint32_t n,
+                             const float* queries, float* coarse_out, float* fine_out, int device_ptrs) {
+    if (!ctx || !ix || (n && (!queries || !coarse_out || !fine_out)))
+        return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    if (precision > 2) return set_err(ctx, PUCKGPU_EINVAL, "precision must be 0, 1 or 2");
+    if (n == 0) return PUCKGPU_OK;
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    const uint32_t d = ix->p.dim, C = ix->p.coarse_cluster_count, F = ix->p.fine_cluster_count, ld = pkc_ld(d);
+    if (precision && ld > 256) return set_err(ctx, PUCKGPU_EINVAL, "reduced precision needs dim <= 256");
+    hipStream_t s = ctx->stream;
+    int rc = 0;
+    if (precision && (rc = ensure_lowp(ctx, ix, (int)precision))) return rc;
+    float* q = (float*)scratch(ctx, S_Q, sizeof(float) * (size_t)n * d, &rc);
+    float* qn = (float*)scratch(ctx, S_QN, sizeof(float) * n, &rc);
+    int32_t* qst = (int32_t*)scratch(ctx, S_QST, sizeof(int32_t) * n, &rc);
+    uint16_t* q16 = (uint16_t*)scratch(ctx, S_LP16, sizeof(uint16_t) * (size_t)n * ld, &rc);
+    uint8_t* q8 = (uint8_t*)scratch(ctx, S_LP8, (size_t)n * ld, &rc);
+    float* qs = (float*)scratch(ctx, S_LPS, sizeof(float) * n, &rc);
+    float* co = device_ptrs ? coarse_out : (float*)scratch(ctx, S_CD, sizeof(float) * (size_t)n * C, &rc);
+    float* fo = device_ptrs ? fine_out : (float*)scratch(ctx, S_FD, sizeof(float) * (size_t)n * F, &rc);
+    if (rc) return rc;
+    const float* qin = (const float*)dev_in(ctx, S_IN_Q, queries, sizeof(float) * (size_t)n * d, device_ptrs, &rc);
+    if (!qin) return rc;
+    PK_CHECK(pk_launch_prep_queries(qin, n, d, ix->p.whether_norm, q, qn, qst, s));
+    if (precision == 0) {
+        PK_CHECK(pk_launch_gemm_exact(q, ix->coarse, co, n, C, d, C, PK_EPI_SUB_FROM_BIAS, ix->coarse_norms, s));
+        PK_CHECK(pk_launch_gemm_exact(q, ix->fine, fo, n, F, d, F, PK_EPI_NEG, nullptr, s));
+    } else {
+        const bool bf = precision == 1;
+        PK_CHECK(pkc_launch_convert(q, n, d, (int)precision, q16, q8, qs, s));
+        const void* qa = bf ? (const void*)q16 : (const void*)q8;
+        PK_CHECK(pkc_launch_center(qa, qs, bf ? (const void*)ix->coarse16 : (const void*)ix->coarse8, ix->coarse8s, n,
+                                   C, d, (int)precision, PK_EPI_SUB_FROM_BIAS, ix->coarse_norms, co, C, s));
+        PK_CHECK(pkc_launch_center(qa, qs, bf ? (const void*)ix->fine16 : (const void*)ix->fine8, ix->fine8s, n, F, d,
+                                   (int)precision, PK_EPI_NEG, nullptr, fo, F, s));
+    }
+    if (!device_ptrs) {
+        PK_CHECK(hipMemcpyAsync(coarse_out, co, sizeof(float) * (size_t)n * C, hipMemcpyDeviceToHost, s));
+        PK_CHECK(hipMemcpyAsync(fine_out, fo, sizeof(float) * (size_t)n * F, hipMemcpyDeviceToHost, s));
+    }
+    PK_CHECK(hipStreamSynchronize(s));
+    return PUCKGPU_OK;
+}
+
+int puckgpu_last_scan_ms(puckgpu_ctx* ctx, float* scan_ms) {
+    if (!ctx || !scan_ms) return PUCKGPU_EINVAL;
+    *scan_ms = ctx->scan_ms;
     return PUCKGPU_OK;
 }
 

@@ -175,11 +175,12 @@ class GpuIndex:
         return out
 
     def search(self, queries, topk, search_coarse_count, filter_topk, topk_max=None, radius_rate=1.0,
-               out=None, replay_all=False):
+               out=None, replay_all=False, precision=0):
         """Returns (distance [n, topk], labels [n, topk], result_num [n])."""
         n = queries.shape[0]
         topk_max = topk if topk_max is None else topk_max
-        sp = _lib.SearchParams(topk_max, search_coarse_count, filter_topk, radius_rate, int(bool(replay_all)))
+        sp = _lib.SearchParams(topk_max, search_coarse_count, filter_topk, radius_rate, int(bool(replay_all)),
+                               int(precision))
         if out is None:
             if _is_dev(queries):
                 import torch
@@ -197,12 +198,13 @@ class GpuIndex:
         return D, I, R
 
     def search_batch(self, queries, topk_max, search_coarse_count, filter_topk, topk=None, scc=None,
-                     fine_budget=None, radius_rate=1.0, out=None, replay_all=False):
+                     fine_budget=None, radius_rate=1.0, out=None, replay_all=False, precision=0):
         """Per-query topk / search_coarse_count / fine_budget arrays (uint32,
         n entries, or None for the batch-wide value).  search_coarse_count is
         the batch maximum.  Returns (distance [n, topk_max], labels, result_num)."""
         n = queries.shape[0]
-        sp = _lib.SearchParams(topk_max, search_coarse_count, filter_topk, radius_rate, int(bool(replay_all)))
+        sp = _lib.SearchParams(topk_max, search_coarse_count, filter_topk, radius_rate, int(bool(replay_all)),
+                               int(precision))
         dev = _is_dev(queries)
 
         def arr(a):
@@ -234,6 +236,21 @@ class GpuIndex:
                                                        _ptr(queries), _ptr(D), _ptr(I), _ptr(R), m))
         return D, I, R
 
+    def center_distances(self, queries, precision=0):
+        """(coarse [n, C], fine [n, F]) as the search computes them."""
+        n = queries.shape[0]
+        if _is_dev(queries):
+            import torch
+            co = torch.empty((n, self.C), dtype=torch.float32, device=queries.device)
+            fo = torch.empty((n, self.F), dtype=torch.float32, device=queries.device)
+        else:
+            co = np.zeros((n, self.C), np.float32)
+            fo = np.zeros((n, self.F), np.float32)
+        self.ctx.sync_torch(queries)
+        self.ctx.check(self.ctx.L.puckgpu_center_distances(self.ctx.h, self.h, int(precision), n, _ptr(queries),
+                                                           _ptr(co), _ptr(fo), _mode(queries, co, fo)))
+        return co, fo
+
     def budget_ranks(self, queries, cells, search_coarse_count):
         """Smallest fine budget scoring each given cell (include/puck_gpu.h)."""
         n, k = cells.shape
@@ -254,6 +271,11 @@ class GpuIndex:
         c = np.zeros(n, np.uint32)
         self.ctx.check(self.ctx.L.puckgpu_last_filter(self.ctx.h, n, filter_topk, _ptr(t), _ptr(i), _ptr(c)))
         return t, i, c
+
+    def last_scan_ms(self):
+        v = ctypes.c_float(0.0)
+        self.ctx.check(self.ctx.L.puckgpu_last_scan_ms(self.ctx.h, ctypes.byref(v)))
+        return v.value
 
     def last_stage_ms(self):
         ms = np.zeros(5, np.float32)
@@ -349,12 +371,13 @@ class Probe:
 
 
 def search_learned(index, probe, queries, topk_max, search_coarse_count, filter_topk, margin=0.0, min_budget=1,
-                   fallback_frac=1.0, topk=None, scc=None, radius_rate=1.0, replay_all=False, out=None):
+                   fallback_frac=1.0, topk=None, scc=None, radius_rate=1.0, replay_all=False, out=None, precision=0):
     """Batched search whose per-query fine budgets come from the probe, in the
     same pipeline.  Returns (distance, labels, result_num, budgets)."""
     ctx = index.ctx
     n = queries.shape[0]
-    sp = _lib.SearchParams(topk_max, search_coarse_count, filter_topk, radius_rate, int(bool(replay_all)))
+    sp = _lib.SearchParams(topk_max, search_coarse_count, filter_topk, radius_rate, int(bool(replay_all)),
+                           int(precision))
     bp = _lib.BudgetParams(margin, min_budget, fallback_frac)
     dev = _is_dev(queries)
 
@@ -395,10 +418,10 @@ class Server:
     tiles."""
 
     def __init__(self, index, topk_max, search_coarse_count, filter_topk, max_batch=4096, radius_rate=1.0,
-                 probe=None, margin=0.0, min_budget=1, fallback_frac=1.0):
+                 probe=None, margin=0.0, min_budget=1, fallback_frac=1.0, precision=0):
         self.index, self.ctx = index, index.ctx
         self.topk_max = topk_max
-        sp = _lib.SearchParams(topk_max, search_coarse_count, filter_topk, radius_rate, 0)
+        sp = _lib.SearchParams(topk_max, search_coarse_count, filter_topk, radius_rate, 0, int(precision))
         bp = _lib.BudgetParams(margin, min_budget, fallback_frac)
         h = ctypes.c_void_p()
         self.ctx.check(self.ctx.L.puckgpu_server_create(self.ctx.h, index.h, probe.h if probe else None,
