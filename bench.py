@@ -78,6 +78,7 @@ def main():
     ap.add_argument("--profile-form", choices=["plain", "learned"], default=None,
                     help="time only this form (for profiler runs); needs --scc (and --margin for learned)")
     ap.add_argument("--no-serving", action="store_true", help="skip the batched serving workload")
+    ap.add_argument("--no-precision", action="store_true", help="skip the reduced-precision forms")
     ap.add_argument("--serving-queries", type=int, default=100_000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -302,6 +303,14 @@ def main():
                                                    "fine_tie_orders_disagree"], cnt[4:13])},
         "build_s": round(build_s, 1),
     }
+    if not args.no_precision and args.profile_form is None:
+        lrun = None
+        if learned is not None:
+            bud = torch.empty(nq, dtype=torch.int32, device=dev)
+            lrun = lambda prec: search_learned(gi, probe, q, topk, chosen, ftk, margin=learned["margin"],
+                                               fallback_frac=args.fallback_frac, precision=prec, out=out + (bud,))
+        result["precision_forms"] = precision_forms(gi, ix, q, gt_i, gt_d, topk, chosen, ftk, out, args,
+                                                    learned_run=lrun)
     if not args.no_serving and args.profile_form is None:
         result["batched_serving"] = serving(gi, dev, d, args.serving_queries)
     if not args.no_cpu:
@@ -309,6 +318,63 @@ def main():
         result["cpu_baseline"] = cpu_baseline(ix, q.cpu().numpy(), topk, chosen, ftk, args.cpu_seconds, out,
                                               gt_i, gt_d)
     print(json.dumps(result), flush=True)
+
+
+def precision_forms(gi, ix, q, gt_i, gt_d, topk, scc, ftk, out, args, learned_run=None, nsample=1000):
+    """BF16 / FP8 center distances on the matrix cores (coarse boundary
+    rescored in FP32, exact rerank) against the exact FP32 form: QPS, recall,
+    slot agreement with the exact form, and the error of the center distances
+    against FP64 on a query sample (|err| / sum_k |q_k c_k|, and the fraction
+    of queries whose nearest coarse center differs from FP64's)."""
+    import torch
+    from puck_amd.metrics import recall_with_ties
+    nq = q.shape[0]
+    gi.search(q, topk, scc, ftk, out=out)
+    exact_ids = out[1].cpu().numpy().copy()
+    qs = q[:nsample].double()
+    qs = qs / qs.norm(dim=1, keepdim=True)
+    C64 = torch.as_tensor(ix["coarse"], device=q.device).double()
+    F64 = torch.as_tensor(ix["fine"], device=q.device).double()
+    want_c = 0.5 * (C64 ** 2).sum(1)[None, :] - qs @ C64.T
+    want_f = -(qs @ F64.T)
+    mag_c = qs.abs() @ C64.abs().T
+    mag_f = qs.abs() @ F64.abs().T
+    res = {}
+    for prec, name in ((0, "fp32_exact"), (1, "bf16_mfma"), (2, "fp8_e4m3_mfma")):
+        co, fo = gi.center_distances(q[:nsample], precision=prec)
+        ec = ((co.double() - want_c).abs() / mag_c.clamp_min(1e-12))
+        ef = ((fo.double() - want_f).abs() / mag_f.clamp_min(1e-12))
+        top1 = float((co.argmin(1) != want_c.argmin(1)).double().mean())
+        run = lambda: gi.search(q, topk, scc, ftk, out=out, precision=prec)
+        run()
+        torch.cuda.synchronize()
+        t1 = time.time()
+        for _ in range(3):
+            run()
+        torch.cuda.synchronize()
+        qps = 3 * nq / (time.time() - t1)
+        ids = out[1].cpu().numpy()
+        rec = recall_with_ties(gt_i, gt_d, ids.astype(np.int64), topk)[0]
+        ms, _ = gi.last_stage_ms()
+        entry = {"qps": round(qps, 1), "recall_at_10": round(rec, 5),
+                 "slot_agreement_with_fp32_exact": round(float((ids == exact_ids).mean()), 5),
+                 "center_gemm_ms": round(float(ms[0]), 4),
+                 "coarse_rel_err_max": float(ec.max()), "coarse_rel_err_mean": float(ec.mean()),
+                 "fine_rel_err_max": float(ef.max()), "fine_rel_err_mean": float(ef.mean()),
+                 "nearest_coarse_differs_from_fp64": round(top1, 5),
+                 "returned_distances": "exact FP32 rerank (no error from the center precision)"}
+        if learned_run is not None:
+            learned_run(prec)
+            t1 = time.time()
+            for _ in range(3):
+                learned_run(prec)
+            torch.cuda.synchronize()
+            entry["learned_qps"] = round(3 * nq / (time.time() - t1), 1)
+            entry["learned_recall_at_10"] = round(recall_with_ties(gt_i, gt_d, out[1].cpu().numpy().astype(np.int64),
+                                                                   topk)[0], 5)
+        res[name] = entry
+        log(f"precision {name}: {entry}")
+    return res
 
 
 def serving(gi, dev, d, nq_total, max_batch=4096):

@@ -25,8 +25,8 @@ hipError_t pk_launch_topk_rows(const float*, uint32_t, uint32_t, uint64_t, uint3
 hipError_t pk_launch_cell_topk(const float*, const uint32_t*, uint32_t, const float*, const float*, uint32_t, uint32_t,
                                uint32_t, float*, hipStream_t);
 hipError_t pk_launch_exact_center_order(const float*, uint32_t, const float*, uint32_t, uint32_t, uint32_t,
-                                        const uint32_t*, const uint32_t*, uint32_t, float*, uint32_t*, float*,
-                                        uint32_t*, hipStream_t);
+                                        const uint32_t*, const uint32_t*, uint32_t, const uint8_t*, int, float*,
+                                        uint32_t*, float*, uint32_t*, hipStream_t);
 hipError_t pk_launch_prep_queries(const float*, uint32_t, uint32_t, int, float*, float*, int32_t*, hipStream_t);
 hipError_t pk_launch_prep_queries_ex(const float*, uint32_t, uint32_t, int, float*, float*, int32_t*, const uint32_t*,
                                      uint8_t*, hipStream_t);
@@ -623,30 +623,50 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
 // depends on the order among exactly equal distances are replayed with the
 // reference heaps.  flags: 1 coarse-distance tie in the kept range, 2
 // filter-boundary tie at the final top, 4 exact-distance tie in the top-k, 8 a
-// decision on a fine-distance tie group; 1 and 8 need the reference's center
-// order.  Returns the number of queries replayed (enqueued on s), or < 0.
+// decision on a fine-distance tie group.
+//   stage 1: queries flagged only 1 get the reference's coarse order and run
+//            the fast walk and rerank again (which flag any remaining tie);
+//   stage 2: queries still flagged run the walk and rerank with the reference
+//            heaps (the reference's fine order first when flagged 8).
+// Synchronises the stream.  Returns the number of queries replayed, or < 0.
 int replay_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, const PassBufs& b, uint32_t m,
                 uint8_t* hflags, int replay_all, PkWalkArgs wa, PkRerankArgs ra, std::vector<uint32_t>& hlist,
                 hipStream_t s) {
     const uint32_t C = ix->p.coarse_cluster_count, F = ix->p.fine_cluster_count;
+    std::vector<uint32_t> stage1;
     hlist.clear();
-    if (replay_all)
-        for (uint32_t i = 0; i < m; ++i) hflags[i] |= 8u;
-    for (uint32_t i = 0; i < m; ++i)
-        if (hflags[i] & 9u) hlist.push_back(i);
-    const uint32_t ncenter = (uint32_t)hlist.size();
-    for (uint32_t i = 0; i < m; ++i)
-        if ((hflags[i] & 15u) && !(hflags[i] & 9u)) hlist.push_back(i);
-    if (hlist.empty()) return 0;
+    for (uint32_t i = 0; i < m; ++i) {
+        if (replay_all) hlist.push_back(i);
+        else if (hflags[i] == 1u) stage1.push_back(i);
+        else if (hflags[i] & 15u) hlist.push_back(i);
+    }
+    const uint32_t n1 = (uint32_t)stage1.size();
+    if (n1) {
+        PK_CHECK(hipMemcpyAsync(b.qlist, stage1.data(), sizeof(uint32_t) * n1, hipMemcpyHostToDevice, s));
+        PK_CHECK(pk_launch_exact_center_order(b.cdist, C, b.fdist, F, wa.cstride, cfg.scc, wa.q_scc, b.qlist, n1,
+                                              b.flags, 0, b.csd, b.csi, b.fsd, b.fsi, s));
+        PkWalkArgs w1 = wa;
+        w1.qlist = b.qlist;
+        PK_CHECK(pk_launch_walk(&w1, n1, s, nullptr));
+        PkRerankArgs r1 = ra;
+        r1.qlist = b.qlist;
+        PK_CHECK(pk_launch_rerank(&r1, n1, s));
+        PK_CHECK(hipMemcpyAsync(hflags, b.flags, m, hipMemcpyDeviceToHost, s));
+        PK_CHECK(hipStreamSynchronize(s));
+        for (uint32_t i : stage1)
+            if (hflags[i] & 14u) hlist.push_back(i);
+    }
     const uint32_t nl = (uint32_t)hlist.size();
-    PK_CHECK(hipMemcpyAsync(b.qlist, hlist.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s));
-    PK_CHECK(pk_launch_exact_center_order(b.cdist, C, b.fdist, F, wa.cstride, cfg.scc, wa.q_scc, b.qlist, ncenter,
-                                          b.csd, b.csi, b.fsd, b.fsi, s));
-    wa.qlist = b.qlist; wa.flags = nullptr; wa.exact = 1; wa.tie_ids = nullptr; wa.tie_n = nullptr;
-    PK_CHECK(pk_launch_walk(&wa, nl, s, nullptr));
-    ra.qlist = b.qlist; ra.flags = nullptr; ra.exact = 1; ra.tie_ids = nullptr; ra.tie_n = nullptr;
-    PK_CHECK(pk_launch_rerank(&ra, nl, s));
-    return (int)nl;
+    if (nl) {
+        PK_CHECK(hipMemcpyAsync(b.qlist, hlist.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s));
+        PK_CHECK(pk_launch_exact_center_order(b.cdist, C, b.fdist, F, wa.cstride, cfg.scc, wa.q_scc, b.qlist, nl,
+                                              b.flags, replay_all, b.csd, b.csi, b.fsd, b.fsi, s));
+        wa.qlist = b.qlist; wa.flags = nullptr; wa.exact = 1; wa.tie_ids = nullptr; wa.tie_n = nullptr;
+        PK_CHECK(pk_launch_walk(&wa, nl, s, nullptr));
+        ra.qlist = b.qlist; ra.flags = nullptr; ra.exact = 1; ra.tie_ids = nullptr; ra.tie_n = nullptr;
+        PK_CHECK(pk_launch_rerank(&ra, nl, s));
+    }
+    return (int)(n1 + nl);
 }
 
 // Allocates pass buffers for `cap` queries from context scratch slots.

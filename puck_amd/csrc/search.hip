@@ -756,6 +756,9 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
         const float cd = a.csel_d[(uint64_t)qi * scs + l];
         const float mind = a.mdo[c] + cd;
         const float fd0 = fsd[0];
+        // the reference breaks at the list's first fine cluster: skip the list
+        // without staging a chunk (puck_index.cpp:528-530)
+        if (fd0 + mind >= P) continue;
         float M = (P - cd) - fd0;
         const float* stc = a.stationary + (uint64_t)c * F;
         const uint32_t* csc = a.cell_start + (uint64_t)c * F;
@@ -1262,10 +1265,15 @@ __global__ __launch_bounds__(256) void pk_exact_center_order(const float* __rest
                                                              uint32_t scs, uint32_t scap,
                                                              const uint32_t* __restrict__ q_scc,
                                                              const uint32_t* __restrict__ qlist,
+                                                             const uint8_t* __restrict__ flags, int force,
                                                              float* __restrict__ csel_d, uint32_t* __restrict__ csel_id,
                                                              float* __restrict__ fsort_d, uint32_t* __restrict__ fsort_id) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t qi = qlist[blockIdx.x];
+    // coarse heap order for a coarse-distance tie (flag 1), fine heap order for
+    // a decision on a fine-distance tie (flag 8); both when forced
+    const bool do_coarse = force || (flags[qi] & 1u);
+    const bool do_fine = force || (flags[qi] & 8u);
     // heap capacity = this query's search_coarse_count (its tie order depends
     // on it), at most scap; rows keep the stride scs
     const uint32_t scc = q_scc ? (q_scc[qi] < scap ? q_scc[qi] : scap) : scap;
@@ -1273,21 +1281,24 @@ __global__ __launch_bounds__(256) void pk_exact_center_order(const float* __rest
     uint2* H = (uint2*)smem;                           // L + 2 entries
     float* row = (float*)(H + ((L + 2) & ~1u));        // max(C, F)
     // coarse top-scc (hierarchical_cluster_index.cpp:709-726)
-    for (uint32_t i = threadIdx.x; i < C; i += blockDim.x) row[i] = cdist[(uint64_t)qi * C + i];
-    if (scc > 0) pk_heap_init(H, scc);
-    __syncthreads();
-    if (threadIdx.x == 0 && scc > 0) {
-        uint32_t dc = scc;
-        for (uint32_t c = 0; c < C; ++c)
-            if (row[c] < pk_hv(H[1])) pk_heap_update(H, scc, &dc, row[c], c);
-        pk_heap_reorder(H, scc);
+    if (do_coarse) {
+        for (uint32_t i = threadIdx.x; i < C; i += blockDim.x) row[i] = cdist[(uint64_t)qi * C + i];
+        if (scc > 0) pk_heap_init(H, scc);
+        __syncthreads();
+        if (threadIdx.x == 0 && scc > 0) {
+            uint32_t dc = scc;
+            for (uint32_t c = 0; c < C; ++c)
+                if (row[c] < pk_hv(H[1])) pk_heap_update(H, scc, &dc, row[c], c);
+            pk_heap_reorder(H, scc);
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < scc; i += blockDim.x) {
+            csel_d[(uint64_t)qi * scs + i] = pk_hv(H[i + 1]);
+            csel_id[(uint64_t)qi * scs + i] = H[i + 1].y;
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < scc; i += blockDim.x) {
-        csel_d[(uint64_t)qi * scs + i] = pk_hv(H[i + 1]);
-        csel_id[(uint64_t)qi * scs + i] = H[i + 1].y;
-    }
-    __syncthreads();
+    if (!do_fine) return;
     // fine full heap sort (puck_index.cpp:497-504)
     for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) row[i] = fdist[(uint64_t)qi * F + i];
     pk_heap_init(H, F);
@@ -1306,9 +1317,9 @@ __global__ __launch_bounds__(256) void pk_exact_center_order(const float* __rest
 
 extern "C" hipError_t pk_launch_exact_center_order(const float* cdist, uint32_t C, const float* fdist, uint32_t F,
                                                    uint32_t scs, uint32_t scap, const uint32_t* q_scc,
-                                                   const uint32_t* qlist, uint32_t nlist, float* csel_d,
-                                                   uint32_t* csel_id, float* fsort_d, uint32_t* fsort_id,
-                                                   hipStream_t s) {
+                                                   const uint32_t* qlist, uint32_t nlist, const uint8_t* flags,
+                                                   int force, float* csel_d, uint32_t* csel_id, float* fsort_d,
+                                                   uint32_t* fsort_id, hipStream_t s) {
     if (scap > scs) return hipErrorInvalidValue;
     if (nlist == 0) return hipSuccess;
     const uint32_t L = C > F ? C : F;
@@ -1321,7 +1332,7 @@ extern "C" hipError_t pk_launch_exact_center_order(const float* cdist, uint32_t 
         attr = true;
     }
     hipLaunchKernelGGL(pk_exact_center_order, dim3(nlist), dim3(256), smem, s, cdist, C, fdist, F, scs, scap,
-                       q_scc, qlist, csel_d, csel_id, fsort_d, fsort_id);
+                       q_scc, qlist, flags, force, csel_d, csel_id, fsort_d, fsort_id);
     return hipGetLastError();
 }
 
