@@ -26,6 +26,11 @@
  *                          (puck/puck/puck_index.cpp:602-709)
  *   puckgpu_kmeans_assign  nearest_center, the k-means assignment step
  *                          (puck/hierarchical_cluster/kmeans.cpp:221-258)
+ * Entry points without a counterpart in the reference (each says so below):
+ * the learned probe (puckgpu_probe_*, puckgpu_search_learned,
+ * puckgpu_budget_ranks), the batched server (puckgpu_server_*), the
+ * reduced-precision center distances (puckgpu_search_params.precision,
+ * puckgpu_center_distances) and the measurement hooks.
  */
 #ifndef PUCK_GPU_H
 #define PUCK_GPU_H
