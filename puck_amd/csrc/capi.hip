@@ -106,6 +106,8 @@ struct PkWalkArgs {
     const uint32_t* q_nf;
     uint32_t cstride;
     const float* fdist;
+    uint32_t scan_cap;
+    uint32_t scan_round;
 };
 struct PkRerankArgs {
     const float* q;
@@ -965,10 +967,13 @@ int puckgpu_probe_create(puckgpu_ctx* ctx, uint32_t dim, const puckgpu_probe_sha
         puckgpu_probe_free(p);
         return rc;
     }
-    PK_CHECK(hipMemset(p->params, 0, sizeof(float) * p->count));
-    PK_CHECK(hipMemset(p->m, 0, sizeof(float) * p->trained));
-    PK_CHECK(hipMemset(p->v, 0, sizeof(float) * p->trained));
-    PK_CHECK(hipMemset(p->wbf, 0, sizeof(uint16_t) * p->trained));
+    if (hipMemset(p->params, 0, sizeof(float) * p->count) != hipSuccess ||
+        hipMemset(p->m, 0, sizeof(float) * p->trained) != hipSuccess ||
+        hipMemset(p->v, 0, sizeof(float) * p->trained) != hipSuccess ||
+        hipMemset(p->wbf, 0, sizeof(uint16_t) * p->trained) != hipSuccess) {
+        puckgpu_probe_free(p);
+        return set_err(ctx, PUCKGPU_EHIP, "probe: clearing parameters failed");
+    }
     *out = p;
     return PUCKGPU_OK;
 }

@@ -64,6 +64,8 @@ struct PkWalkArgs {
     const uint32_t* q_nf;    // nq or null: per-query fine budget (0 or >= F: all cells)
     uint32_t cstride;        // csel row stride (>= scc)
     const float* fdist;      // nq x F unsorted fine distances -<f,q> (budget scan)
+    uint32_t scan_cap;       // budget scan: candidate buffer entries (set by the launcher)
+    uint32_t scan_round;     // budget scan: points scored per round (multiple of WALK_THREADS)
 };
 
 #define PK_TIE_MAX 16
@@ -545,19 +547,16 @@ __device__ uint32_t pk_keep_le(uint64_t* v, uint32_t n, uint64_t K, uint32_t* tm
 // are scored in full: every point's filter distance is computed as the
 // reference's compute_quantized_distance computes it, and the filter set is
 // the ftk smallest (distance, memory id) keys, kept in an unsorted LDS buffer
-// (the walk's S_key + W_key region).  When the buffer could overflow it is
+// (cap entries, R points scored per round).  When the buffer could overflow it is
 // cut back to the keys whose distance is at most the ftk-th smallest distance
 // (a 4-pass select on the distance bits); the exact (distance, id) cut to ftk
 // entries is made once, at the end.  The result does not depend on any
 // visiting order, so it needs no replay.
 template <int FTKM>
-__device__ void pk_scan_budget(const PkWalkArgs& a, PkWalkSmem& s, uint32_t qi, uint32_t b) {
+__device__ void pk_scan_budget(const PkWalkArgs& a, PkWalkSmem& s, uint32_t qi, uint32_t b, uint64_t* B,
+                               uint32_t cap, uint32_t R, uint32_t* hist) {
     const uint32_t F = a.F, ftk = a.ftk, nsq = a.nsq;
     const int tid = threadIdx.x;
-    constexpr uint32_t R = 2 * WALK_THREADS;  // points per round
-    uint64_t* B = s.S_key;  // S_key and W_key are contiguous
-    const uint32_t cap = ((ftk + 1) & ~1u) + WALK_PMAX;
-    uint32_t* hist = s.P_id;  // 512 words, unused by this path
     uint32_t n = 0;
     uint64_t K = ~0ull;  // keys above K cannot enter the filter set
     uint32_t scc = a.scc;
@@ -610,7 +609,6 @@ __device__ void pk_scan_budget(const PkWalkArgs& a, PkWalkSmem& s, uint32_t qi, 
                 }
                 if (tid == 0) s.red[2] = n;
                 __syncthreads();
-#pragma unroll
                 for (uint32_t h = 0; h < R / WALK_THREADS; ++h) {
                     const uint32_t p = p0 + h * WALK_THREADS + tid;
                     if (p >= tot) break;
@@ -1024,19 +1022,19 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_scan_budget_kernel(PkWalkArgs
     const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
     const uint32_t ftk = a.ftk, F = a.F, nsq = a.nsq;
     if (!(a.q_nf && a.q_nf[qi] != 0u && a.q_nf[qi] < F)) return;
+    // own LDS plan: LUT, small arrays, then the candidate buffer (a.scan_cap keys)
     PkWalkSmem s;
     unsigned char* ptr = smem;
-    s.S_key = (uint64_t*)ptr; ptr += sizeof(uint64_t) * ((ftk + 1) & ~1u);
-    s.W_key = (uint64_t*)ptr; ptr += sizeof(uint64_t) * WALK_PMAX;
-    s.S_id = (uint32_t*)ptr; ptr += sizeof(uint32_t) * ((ftk + 3) & ~3u);
-    s.P_id = (uint32_t*)ptr; ptr += sizeof(uint32_t) * WALK_PMAX;
+    s.lut = (float*)ptr; ptr += sizeof(float) * nsq * PK_KS;
+    s.qv = (float*)ptr; ptr += sizeof(float) * ((a.dim + 3) & ~3u);
+    s.red = (uint32_t*)ptr; ptr += sizeof(uint32_t) * 16;
     s.c_b = (uint32_t*)ptr; ptr += sizeof(uint32_t) * WALK_CH;
     s.c_pre = (uint32_t*)ptr; ptr += sizeof(uint32_t) * (WALK_CH + 4);
-    s.red = (uint32_t*)ptr; ptr += sizeof(uint32_t) * 16;
     s.c_fd = (float*)ptr; ptr += sizeof(float) * WALK_CH;
-    s.c_st = (float*)ptr; ptr += sizeof(float) * WALK_CH;
-    s.qv = (float*)ptr; ptr += sizeof(float) * ((a.dim + 3) & ~3u);
-    s.lut = (float*)ptr;
+    uint32_t* hist = (uint32_t*)ptr; ptr += sizeof(uint32_t) * 260;
+    ptr = (unsigned char*)(((uintptr_t)ptr + 15) & ~(uintptr_t)15);
+    uint64_t* B = (uint64_t*)ptr;
+    (void)ftk;
     if (a.qstatus[qi] != 0) {
         if (threadIdx.x == 0) {
             a.out_n[qi] = 0;
@@ -1053,7 +1051,14 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_scan_budget_kernel(PkWalkArgs
         s.lut[e] = pk_lut_entry(s.qv + m * a.lsq, a.codebook + (uint64_t)e * a.lsq, a.lsq);
     }
     __syncthreads();
-    pk_scan_budget<FTKM>(a, s, qi, a.q_nf[qi]);
+    pk_scan_budget<FTKM>(a, s, qi, a.q_nf[qi], B, a.scan_cap, a.scan_round, hist);
+}
+
+extern "C" size_t pk_scan_smem_fixed(uint32_t dim, uint32_t nsq) {
+    size_t b = sizeof(float) * nsq * PK_KS + sizeof(float) * ((dim + 3) & ~3u) + sizeof(uint32_t) * 16 +
+               sizeof(uint32_t) * WALK_CH + sizeof(uint32_t) * (WALK_CH + 4) + sizeof(float) * WALK_CH +
+               sizeof(uint32_t) * 260;
+    return (b + 15) & ~(size_t)15;
 }
 
 extern "C" size_t pk_walk_smem_bytes(uint32_t dim, uint32_t nsq, uint32_t ftk, int exact) {
@@ -1082,6 +1087,19 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
     if (args->ftk == 0 || args->ftk > WALK_FTK_MAX || args->cstride < args->scc) return hipErrorInvalidValue;
     if (args->q_nf && !args->fdist) return hipErrorInvalidValue;
     size_t smem = pk_walk_smem_bytes(args->dim, args->nsq, args->ftk, args->exact);
+    // budget scan: fill the same footprint as the walk (at least ftk + 256
+    // candidates), up to 1024 points per round
+    PkWalkArgs sa = *args;
+    const size_t fixed = pk_scan_smem_fixed(args->dim, args->nsq);
+    size_t want = smem > 80 * 1024 ? smem : 80 * 1024;
+    size_t cap = want > fixed ? (want - fixed) / 8 : 0;
+    if (cap < (size_t)args->ftk + WALK_THREADS) cap = (size_t)args->ftk + WALK_THREADS;
+    size_t rnd = (cap - args->ftk) & ~(size_t)(WALK_THREADS - 1);
+    if (rnd > 4 * WALK_THREADS) rnd = 4 * WALK_THREADS;
+    sa.scan_cap = (uint32_t)cap;
+    sa.scan_round = (uint32_t)rnd;
+    const size_t smem_scan = fixed + 8 * cap;
+    if (args->q_nf && smem_scan > 160 * 1024) return hipErrorInvalidValue;
     if (smem > 160 * 1024) return hipErrorInvalidValue;
     static bool attr_set = false;
     if (!attr_set) {
@@ -1093,11 +1111,11 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
     // budget scan first (its own launch, timed between ev_mid and the end),
     // then the walk for the queries without a budget
     if (args->ftk <= 1024) {
-        if (args->q_nf) hipLaunchKernelGGL(pk_scan_budget_kernel<1024>, dim3(nq), dim3(WALK_THREADS), smem, s, *args);
+        if (args->q_nf) hipLaunchKernelGGL(pk_scan_budget_kernel<1024>, dim3(nq), dim3(WALK_THREADS), smem_scan, s, sa);
         if (ev_mid) (void)hipEventRecord(ev_mid, s);
         hipLaunchKernelGGL(pk_walk<1024>, dim3(nq), dim3(WALK_THREADS), smem, s, *args);
     } else {
-        if (args->q_nf) hipLaunchKernelGGL(pk_scan_budget_kernel<2048>, dim3(nq), dim3(WALK_THREADS), smem, s, *args);
+        if (args->q_nf) hipLaunchKernelGGL(pk_scan_budget_kernel<2048>, dim3(nq), dim3(WALK_THREADS), smem_scan, s, sa);
         if (ev_mid) (void)hipEventRecord(ev_mid, s);
         hipLaunchKernelGGL(pk_walk<2048>, dim3(nq), dim3(WALK_THREADS), smem, s, *args);
     }
