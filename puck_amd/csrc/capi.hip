@@ -40,7 +40,7 @@ hipError_t pkc_launch_center(const void*, const float*, const void*, const float
 hipError_t pkc_launch_rescore_coarse(const float*, uint32_t, const float*, const float*, const uint32_t*, uint32_t,
                                      uint32_t, uint32_t, float*, uint32_t*, uint8_t*, uint32_t, hipStream_t);
 hipError_t pk_launch_lut(const float*, uint32_t, uint32_t, const float*, uint32_t, uint32_t, float*, hipStream_t);
-hipError_t pk_launch_walk(const PkWalkArgs*, uint32_t, hipStream_t, hipEvent_t);
+hipError_t pk_launch_walk(const PkWalkArgs*, uint32_t, hipStream_t, hipEvent_t, hipStream_t, hipEvent_t, hipEvent_t);
 hipError_t pk_launch_rerank(const PkRerankArgs*, uint32_t, hipStream_t);
 uint32_t pk_walk_ftk_max(void);
 uint32_t pk_probe_trained_count(uint32_t, uint32_t);
@@ -55,6 +55,7 @@ hipError_t pk_probe_adam_step(float*, float*, float*, const float*, uint16_t*, u
 hipError_t pk_launch_probe_budget(const float*, uint32_t, float, uint32_t, float, uint32_t, uint32_t*,
                                   hipStream_t);
 hipError_t pk_launch_to_bf16(const float*, uint32_t, uint16_t*, hipStream_t);
+hipError_t pk_launch_build_walk_list(const uint32_t*, uint32_t, uint32_t, uint32_t*, uint32_t*, hipStream_t);
 hipError_t pk_launch_budget_rank(const float*, const uint32_t*, uint32_t, uint32_t, const float*, const float*,
                                  uint32_t, uint32_t, const uint32_t*, uint32_t, uint32_t, uint32_t*, hipStream_t);
 hipError_t pk_launch_assign_cells(const float*, const float*, const float*, const float*, const float*,
@@ -108,6 +109,8 @@ struct PkWalkArgs {
     const float* fdist;
     uint32_t scan_cap;
     uint32_t scan_round;
+    const uint32_t* wlist;
+    const uint32_t* wcount;
 };
 struct PkRerankArgs {
     const float* q;
@@ -144,6 +147,8 @@ struct DevBuf {
 struct puckgpu_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;        // the fallback walk beside the budget scan
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     std::string err;
     std::vector<DevBuf> bufs;          // named scratch slots
     hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -241,7 +246,7 @@ void* scratch(puckgpu_ctx* ctx, size_t slot, size_t bytes, int* rc) {
 enum Slot {
     S_Q = 0, S_QN, S_QST, S_CD, S_FD, S_CSD, S_CSI, S_FSD, S_FSI, S_FT, S_FI, S_FN, S_OD, S_OI, S_ON, S_STATS,
     S_IN_Q, S_A0, S_A1, S_A2, S_A3, S_A4, S_A5, S_A6, S_A7, S_FLAGS, S_QLIST, S_TIEID, S_TIEN, S_QTOPK, S_QSCC,
-    S_QNF, S_PFEAT, S_PWORK, S_PNF, S_PY, S_PT, S_PCV, S_LP16, S_LP8, S_LPS, S_CAND_D, S_CAND_I, S_COUNT
+    S_QNF, S_PFEAT, S_PWORK, S_PNF, S_PY, S_PT, S_PCV, S_LP16, S_LP8, S_LPS, S_CAND_D, S_CAND_I, S_WLIST, S_COUNT
 };
 
 // Device view of an input array: the pointer itself, or a scratch copy of it.
@@ -298,6 +303,12 @@ int puckgpu_create(int device, puckgpu_ctx** out) {
             delete ctx;
             return PUCKGPU_EHIP;
         }
+    if (hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming) != hipSuccess) {
+        delete ctx;
+        return PUCKGPU_EHIP;
+    }
     *out = ctx;
     return PUCKGPU_OK;
 }
@@ -310,6 +321,12 @@ void puckgpu_destroy(puckgpu_ctx* ctx) {
         if (b.p) (void)hipFree(b.p);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+    if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
+    if (ctx->stream2) {
+        (void)hipStreamSynchronize(ctx->stream2);
+        (void)hipStreamDestroy(ctx->stream2);
+    }
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -531,6 +548,8 @@ struct PassBufs {
     float* qs = nullptr;
     float* cand_d = nullptr;   // approximate coarse candidates before rescoring
     uint32_t* cand_i = nullptr;
+    uint32_t* wlist = nullptr;   // queries taking the walk when budgets are given
+    uint32_t* wcount = nullptr;
 };
 
 // Per-pass configuration.
@@ -606,7 +625,16 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     wa.rr = cfg.rr; wa.out_t = ft; wa.out_id = fi; wa.out_n = fn; wa.stats = b.stats; wa.qlist = nullptr;
     wa.flags = b.flags; wa.exact = 0; wa.tie_ids = b.tie_ids; wa.tie_n = b.tie_n;
     wa.q_scc = q_scc; wa.q_nf = nf; wa.cstride = cfg.keep ? cfg.keep : 1; wa.fdist = b.fdist;
-    PK_CHECK(pk_launch_walk(&wa, m, s, ev ? ev[6] : nullptr));
+    wa.wlist = nullptr; wa.wcount = nullptr;
+    if (nf) {
+        // the walk runs only over the queries without a budget below F
+        PK_CHECK(pk_launch_build_walk_list(nf, m, F, b.wlist, b.wcount, s));
+        wa.wlist = b.wlist;
+        wa.wcount = b.wcount;
+    }
+    PK_CHECK(pk_launch_walk(&wa, m, s, ev ? ev[6] : nullptr, ctx->stream2, ctx->fork_ev, ctx->join_ev));
+    wa.wlist = nullptr;  // replays address queries through qlist
+    wa.wcount = nullptr;
     if (ev) PK_CHECK(hipEventRecord(ev[3], s));
     PkRerankArgs ra;
     ra.q = b.q; ra.qstatus = b.qst; ra.filt_t = ft; ra.filt_id = fi; ra.filt_n = fn;
@@ -649,7 +677,7 @@ int replay_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, c
                                               b.flags, 0, b.csd, b.csi, b.fsd, b.fsi, s));
         PkWalkArgs w1 = wa;
         w1.qlist = b.qlist;
-        PK_CHECK(pk_launch_walk(&w1, n1, s, nullptr));
+        PK_CHECK(pk_launch_walk(&w1, n1, s, nullptr, nullptr, nullptr, nullptr));
         PkRerankArgs r1 = ra;
         r1.qlist = b.qlist;
         PK_CHECK(pk_launch_rerank(&r1, n1, s));
@@ -664,7 +692,7 @@ int replay_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, c
         PK_CHECK(pk_launch_exact_center_order(b.cdist, C, b.fdist, F, wa.cstride, cfg.scc, wa.q_scc, b.qlist, nl,
                                               b.flags, replay_all, b.csd, b.csi, b.fsd, b.fsi, s));
         wa.qlist = b.qlist; wa.flags = nullptr; wa.exact = 1; wa.tie_ids = nullptr; wa.tie_n = nullptr;
-        PK_CHECK(pk_launch_walk(&wa, nl, s, nullptr));
+        PK_CHECK(pk_launch_walk(&wa, nl, s, nullptr, nullptr, nullptr, nullptr));
         ra.qlist = b.qlist; ra.flags = nullptr; ra.exact = 1; ra.tie_ids = nullptr; ra.tie_n = nullptr;
         PK_CHECK(pk_launch_rerank(&ra, nl, s));
     }
@@ -691,6 +719,8 @@ int scratch_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     b.tie_ids = (uint32_t*)scratch(ctx, S_TIEID, sizeof(uint32_t) * 16 * cap, &rc);
     b.tie_n = (uint8_t*)scratch(ctx, S_TIEN, cap, &rc);
     b.stats = (uint32_t*)scratch(ctx, S_STATS, sizeof(uint32_t) * 4 * cap, &rc);
+    b.wlist = (uint32_t*)scratch(ctx, S_WLIST, sizeof(uint32_t) * (cap + 1), &rc);
+    b.wcount = b.wlist ? b.wlist + cap : nullptr;
     if (pr) {
         b.pfeat = (float*)scratch(ctx, S_PFEAT, sizeof(float) * cap * pr->D, &rc);
         b.pwork = scratch(ctx, S_PWORK, pk_probe_work_bytes(cap, pr->D, pr->H), &rc);
@@ -1169,6 +1199,7 @@ int bucket_create(puckgpu_server* sv, uint32_t cap, const PassCfg& cfg, PkBucket
         (rc = balloc(ctx, bk, &b.fsi, (size_t)cap * F)) || (rc = balloc(ctx, bk, &b.flags, cap)) ||
         (rc = balloc(ctx, bk, &b.tie_ids, (size_t)cap * 16)) || (rc = balloc(ctx, bk, &b.tie_n, cap)) ||
         (rc = balloc(ctx, bk, &b.stats, (size_t)cap * 4)) || (rc = balloc(ctx, bk, &b.qlist, cap)) ||
+        (rc = balloc(ctx, bk, &b.wlist, (size_t)cap + 1)) ||
         (rc = balloc(ctx, bk, &bk->qin, (size_t)cap * d)) || (rc = balloc(ctx, bk, &bk->qtopk, cap)) ||
         (rc = balloc(ctx, bk, &bk->qscc, cap)) || (rc = balloc(ctx, bk, &bk->nvalid, 1)) ||
         (rc = balloc(ctx, bk, &bk->ft, (size_t)cap * ftk)) || (rc = balloc(ctx, bk, &bk->fi, (size_t)cap * ftk)) ||
@@ -1186,6 +1217,7 @@ int bucket_create(puckgpu_server* sv, uint32_t cap, const PassCfg& cfg, PkBucket
             return rc;
         }
     }
+    b.wcount = b.wlist + cap;
     if (sv->probe) {
         const puckgpu_probe* pr = sv->probe;
         void* w = nullptr;

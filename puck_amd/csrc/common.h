@@ -61,6 +61,23 @@ __device__ __forceinline__ float pk_pivot(float top, float qn, double rr) {
 //   per 16-block  l_j = fma(d_j, d_j, d_{8+j}^2);
 //                 t = (((l0+l1)+(l2+l3)) + ((l4+l5)+l6)) + (l7 + t)
 //   tail (<16)    4-lane fma blocks, (b0+b1)+(b2+b3), then a scalar fma chain.
+// Sum over the last dim % 16 elements (left > 0) in the compiled tail order.
+__device__ __forceinline__ float pk_rerank_tail(const float* __restrict__ x, const float* q, uint32_t left) {
+    float b[4] = {0.f, 0.f, 0.f, 0.f};
+    uint32_t i = 0;
+    for (; i + 4 <= left; i += 4)
+        for (int j = 0; j < 4; ++j) {
+            float d = x[i + j] - q[i + j];
+            b[j] = fmaf(d, d, b[j]);
+        }
+    float s = (b[0] + b[1]) + (b[2] + b[3]);
+    for (; i < left; ++i) {
+        float d = x[i] - q[i];
+        s = fmaf(d, d, s);
+    }
+    return s;
+}
+
 __device__ __forceinline__ float pk_rerank_dist(const float* __restrict__ x, const float* q, uint32_t dim) {
     float t = 0.0f;
     uint32_t m = 0;
@@ -74,22 +91,7 @@ __device__ __forceinline__ float pk_rerank_dist(const float* __restrict__ x, con
         }
         t = (((l[0] + l[1]) + (l[2] + l[3])) + ((l[4] + l[5]) + l[6])) + (l[7] + t);
     }
-    uint32_t left = dim - m;
-    if (left > 0) {
-        float b[4] = {0.f, 0.f, 0.f, 0.f};
-        uint32_t i = 0;
-        for (; i + 4 <= left; i += 4)
-            for (int j = 0; j < 4; ++j) {
-                float d = x[m + i + j] - q[m + i + j];
-                b[j] = fmaf(d, d, b[j]);
-            }
-        float s = (b[0] + b[1]) + (b[2] + b[3]);
-        for (; i < left; ++i) {
-            float d = x[m + i] - q[m + i];
-            s = fmaf(d, d, s);
-        }
-        t = t + s;
-    }
+    if (m < dim) t = t + pk_rerank_tail(x + m, q + m, dim - m);
     return t;
 }
 

@@ -66,6 +66,8 @@ struct PkWalkArgs {
     const float* fdist;      // nq x F unsorted fine distances -<f,q> (budget scan)
     uint32_t scan_cap;       // budget scan: candidate buffer entries (set by the launcher)
     uint32_t scan_round;     // budget scan: points scored per round (multiple of WALK_THREADS)
+    const uint32_t* wlist;   // with wcount: the queries that take the walk (pk_build_walk_list)
+    const uint32_t* wcount;
 };
 
 #define PK_TIE_MAX 16
@@ -686,9 +688,8 @@ __device__ __forceinline__ float pk_walk_top(const PkWalkSmem& s, uint32_t ns, u
 }
 
 template <int FTKM>
-__global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
+__device__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
     const int tid = threadIdx.x;
     const uint32_t ftk = a.ftk, F = a.F, nsq = a.nsq;
     // carve LDS (8-byte arrays first)
@@ -1013,6 +1014,46 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
     }
 }
 
+// One query per workgroup (qlist or identity), or, with wcount, grid-stride
+// over the device list wlist[0..*wcount) of the queries that take the walk
+// (the others of the batch are budget-scan queries); every workgroup leaves
+// the loop when the list is exhausted.
+template <int FTKM>
+__global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
+    if (a.wcount) {
+        const uint32_t n = *a.wcount;
+        for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+            pk_walk_query<FTKM>(a, a.wlist[i]);
+            __syncthreads();
+        }
+        return;
+    }
+    pk_walk_query<FTKM>(a, a.qlist ? a.qlist[blockIdx.x] : blockIdx.x);
+}
+
+// Ordered list of the queries of a batch that take the walk (no fine budget
+// below F): one workgroup, block scans in index order.
+__global__ __launch_bounds__(1024) void pk_build_walk_list(const uint32_t* __restrict__ q_nf, uint32_t n, uint32_t F,
+                                                          uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
+    __shared__ uint32_t tmp[20];
+    uint32_t base = 0;
+    for (uint32_t r0 = 0; r0 < n; r0 += blockDim.x) {
+        const uint32_t i = r0 + threadIdx.x;
+        const uint32_t w = (i < n && !(q_nf[i] != 0u && q_nf[i] < F)) ? 1u : 0u;
+        uint32_t tot;
+        const uint32_t pre = pk_block_exscan(w, tmp, &tot);
+        if (w) list[base + pre] = i;
+        base += tot;
+    }
+    if (threadIdx.x == 0) *count = base;
+}
+
+extern "C" hipError_t pk_launch_build_walk_list(const uint32_t* q_nf, uint32_t n, uint32_t F, uint32_t* list,
+                                                uint32_t* count, hipStream_t s) {
+    hipLaunchKernelGGL(pk_build_walk_list, dim3(1), dim3(1024), 0, s, q_nf, n, F, list, count);
+    return hipGetLastError();
+}
+
 // The budget scan as a kernel of its own (its register needs differ from the
 // walk's): one block per query, returning at once for queries without a
 // budget below F.  Same LDS layout as pk_walk.
@@ -1082,7 +1123,12 @@ extern "C" size_t pk_walk_smem_bytes(uint32_t dim, uint32_t nsq, uint32_t ftk, i
 
 extern "C" uint32_t pk_walk_ftk_max(void) { return WALK_FTK_MAX; }
 
-extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStream_t s, hipEvent_t ev_mid) {
+// With fine budgets (q_nf) and a second stream s2, the walk over the queries
+// without a budget runs on s2 concurrently with the budget scan on s (event
+// fork / join, capturable in a graph); ev_mid is recorded on s right after
+// the scan.
+extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStream_t s, hipEvent_t ev_mid,
+                                     hipStream_t s2, hipEvent_t ev_fork, hipEvent_t ev_join) {
     if (nq == 0) return hipSuccess;
     if (args->ftk == 0 || args->ftk > WALK_FTK_MAX || args->cstride < args->scc) return hipErrorInvalidValue;
     if (args->q_nf && !args->fdist) return hipErrorInvalidValue;
@@ -1099,6 +1145,7 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
     sa.scan_cap = (uint32_t)cap;
     sa.scan_round = (uint32_t)rnd;
     const size_t smem_scan = fixed + 8 * cap;
+    const uint32_t wgrid = args->wcount ? (nq < 1024 ? nq : 1024) : nq;
     if (args->q_nf && smem_scan > 160 * 1024) return hipErrorInvalidValue;
     if (smem > 160 * 1024) return hipErrorInvalidValue;
     static bool attr_set = false;
@@ -1110,15 +1157,32 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
     }
     // budget scan first (its own launch, timed between ev_mid and the end),
     // then the walk for the queries without a budget
-    if (args->ftk <= 1024) {
-        if (args->q_nf) hipLaunchKernelGGL(pk_scan_budget_kernel<1024>, dim3(nq), dim3(WALK_THREADS), smem_scan, s, sa);
-        if (ev_mid) (void)hipEventRecord(ev_mid, s);
-        hipLaunchKernelGGL(pk_walk<1024>, dim3(nq), dim3(WALK_THREADS), smem, s, *args);
-    } else {
-        if (args->q_nf) hipLaunchKernelGGL(pk_scan_budget_kernel<2048>, dim3(nq), dim3(WALK_THREADS), smem_scan, s, sa);
-        if (ev_mid) (void)hipEventRecord(ev_mid, s);
-        hipLaunchKernelGGL(pk_walk<2048>, dim3(nq), dim3(WALK_THREADS), smem, s, *args);
+    const bool fork = args->q_nf && s2 && ev_fork && ev_join;
+    hipStream_t sw = fork ? s2 : s;
+    hipError_t e;
+    if (fork) {
+        if ((e = hipEventRecord(ev_fork, s)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(s2, ev_fork, 0)) != hipSuccess) return e;
+        if (args->ftk <= 1024)
+            hipLaunchKernelGGL(pk_walk<1024>, dim3(wgrid), dim3(WALK_THREADS), smem, sw, *args);
+        else
+            hipLaunchKernelGGL(pk_walk<2048>, dim3(wgrid), dim3(WALK_THREADS), smem, sw, *args);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = hipEventRecord(ev_join, s2)) != hipSuccess) return e;
     }
+    if (args->q_nf) {
+        if (args->ftk <= 1024)
+            hipLaunchKernelGGL(pk_scan_budget_kernel<1024>, dim3(nq), dim3(WALK_THREADS), smem_scan, s, sa);
+        else
+            hipLaunchKernelGGL(pk_scan_budget_kernel<2048>, dim3(nq), dim3(WALK_THREADS), smem_scan, s, sa);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (ev_mid) (void)hipEventRecord(ev_mid, s);
+    if (fork) return hipStreamWaitEvent(s, ev_join, 0);
+    if (args->ftk <= 1024)
+        hipLaunchKernelGGL(pk_walk<1024>, dim3(wgrid), dim3(WALK_THREADS), smem, s, *args);
+    else
+        hipLaunchKernelGGL(pk_walk<2048>, dim3(wgrid), dim3(WALK_THREADS), smem, s, *args);
     return hipGetLastError();
 }
 
@@ -1174,15 +1238,38 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
     const float tfin = n >= ftk ? a.filt_t[(uint64_t)qi * ftk + ftk - 1] : 0.f;
     __shared__ uint32_t tie_min;
     if (threadIdx.x == 0) tie_min = tn == PK_TIE_OVERFLOW ? 0u : 0xffffffffu;
-    for (uint32_t r = threadIdx.x; r < n2; r += blockDim.x) {
-        uint64_t key = ~0ull;
-        if (r < n) {
-            uint32_t mid = a.filt_id[(uint64_t)qi * ftk + r];
-            float d = pk_rerank_dist(a.raw + (uint64_t)mid * a.dim, qv, a.dim);
-            key = ((uint64_t)pk_f2ord(d) << 32) | (bud ? a.mem2local[mid] : r);
-            if (exact) dist[r] = d;
+    // 8 lanes per row: lane j holds element j of each 8-lane half block (the
+    // compiled order of pk_rerank_dist); shuffles combine the lanes in that
+    // order, so every lane ends with the same bits as pk_rerank_dist
+    {
+        const uint32_t g = threadIdx.x >> 3, j = threadIdx.x & 7, ng = blockDim.x >> 3;
+        const int src = (int)(threadIdx.x & 63u & ~7u);
+        for (uint32_t r0 = 0; r0 < n2; r0 += ng) {
+            const uint32_t r = r0 + g;
+            const bool valid = r < n;
+            const uint32_t mid = valid ? a.filt_id[(uint64_t)qi * ftk + r] : 0u;
+            const float* x = a.raw + (uint64_t)mid * a.dim;
+            float t = 0.0f;
+            uint32_t m = 0;
+            for (; m + 16 <= a.dim; m += 16) {
+                const float d0 = x[m + j] - qv[m + j];
+                const float d1 = x[m + 8 + j] - qv[m + 8 + j];
+                const float l = fmaf(d0, d0, d1 * d1);
+                float lj[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) lj[k] = __shfl(l, src + k, 64);
+                t = (((lj[0] + lj[1]) + (lj[2] + lj[3])) + ((lj[4] + lj[5]) + lj[6])) + (lj[7] + t);
+            }
+            if (m < a.dim) t = t + pk_rerank_tail(x + m, qv + m, a.dim - m);
+            if (j == 0 && r < n2) {
+                uint64_t key = ~0ull;
+                if (valid) {
+                    key = ((uint64_t)pk_f2ord(t) << 32) | (bud ? a.mem2local[mid] : r);
+                    if (exact) dist[r] = t;
+                }
+                if (!exact) keys[r] = key;
+            }
         }
-        if (!exact) keys[r] = key;
     }
     __syncthreads();
     if (tn > 0 && tn != PK_TIE_OVERFLOW) {

@@ -217,3 +217,26 @@ def test_budget_labels_refind_plain_hits(ctx, idx_cos):
         hits = set(Ip[k, :Rp[k]].tolist()) & set(gt[k, :10].tolist())
         assert hits <= set(Il[k, :Rl[k]].tolist()), k
         assert 1 <= lab[k] <= 40
+
+
+def test_server_with_probe_matches_search_learned(ctx, idx_cos):
+    # graph-captured learned pipeline (budget scan + forked fallback walk) vs
+    # the eager fused call
+    from puck_amd.engine import Server, search_learned
+    from puck_amd.probe import init_params
+    ix, q = idx_cos
+    gi = _gpu_index(ctx, ix)
+    nc, nf, H = 16, 16, 64
+    D = 96 + nc + nf
+    pr = _probe(ctx, np.zeros(PM.param_count(D, H), np.float32), nc=nc, nf=nf, hidden=H)
+    feats = pr.features(gi, q)
+    p = init_params(D, H, feats.mean(0), feats.std(0), 5)
+    p[PM.offsets(D, H)["b3"][0]] = 2.5
+    pr.set_params(p)
+    De, Ie, Re, B = search_learned(gi, pr, q, 10, 6, 200, margin=0.0, fallback_frac=0.5)
+    assert np.any(B == 40) and np.any(B < 40)   # both the scan and the fallback walk are used
+    sv = Server(gi, 10, 6, 200, max_batch=64, probe=pr, margin=0.0, fallback_frac=0.5)
+    for s0, bs in ((0, 64), (0, 5), (7, 33)):
+        D, I, R = sv.search(q[s0:s0 + bs])
+        assert np.array_equal(R, Re[s0:s0 + bs]) and np.array_equal(I, Ie[s0:s0 + bs])
+        assert np.array_equal(D.view(np.uint32), De[s0:s0 + bs].view(np.uint32))
