@@ -611,37 +611,73 @@ __device__ void pk_scan_budget(const PkWalkArgs& a, PkWalkSmem& s, uint32_t qi, 
                 }
                 if (tid == 0) s.red[2] = n;
                 __syncthreads();
-                for (uint32_t h = 0; h < R / WALK_THREADS; ++h) {
-                    const uint32_t p = p0 + h * WALK_THREADS + tid;
-                    if (p >= tot) break;
-                    uint32_t lo = 0, hi = ne;  // c_pre[lo] <= p < c_pre[hi]
-                    while (hi - lo > 1) {
-                        const uint32_t mid = (lo + hi) >> 1;
-                        if (s.c_pre[mid] <= p) lo = mid; else hi = mid;
+                if ((nsq & 15) == 0 && nsq <= 64 && R <= 4 * WALK_THREADS) {
+                    // all records of the round first (up to 4 points x 4 x 16 B of
+                    // codes per thread), then their filter distances
+                    constexpr int PP = 4;
+                    uint4 cw[PP][4];
+                    float off[PP], cdl[PP];
+                    uint32_t mids[PP];
+                    const uint32_t ng = nsq >> 4;
+#pragma unroll
+                    for (int h = 0; h < PP; ++h) {
+                        const uint32_t p = p0 + h * WALK_THREADS + tid;
+                        mids[h] = 0xffffffffu;
+                        if (h * WALK_THREADS >= R || p >= tot) continue;
+                        uint32_t lo = 0, hi = ne;  // c_pre[lo] <= p < c_pre[hi]
+                        while (hi - lo > 1) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (s.c_pre[mid] <= p) lo = mid; else hi = mid;
+                        }
+                        const uint32_t mid = s.c_b[lo] + (p - s.c_pre[lo]);
+                        mids[h] = mid;
+                        cdl[h] = s.c_fd[lo];
+                        off[h] = a.rec_off[mid];
+                        const uint4* cp = (const uint4*)(a.rec_codes + (uint64_t)mid * nsq);
+#pragma unroll
+                        for (int g = 0; g < 4; ++g)
+                            if ((uint32_t)g < ng) cw[h][g] = cp[g];
                     }
-                    const uint32_t mid = s.c_b[lo] + (p - s.c_pre[lo]);
-                    float tv = 2.0f * s.c_fd[lo] + a.rec_off[mid];  // 2x is exact: one rounding, as in double
-                    const uint8_t* codes = a.rec_codes + (uint64_t)mid * nsq;
-                    float acc[4] = {0.f, 0.f, 0.f, 0.f};
-                    if ((nsq & 15) == 0) {
-                        for (uint32_t g = 0; g < nsq; g += 16) {
-                            uint4 w = *(const uint4*)(codes + g);
-                            uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                    for (int h = 0; h < PP; ++h) {
+                        if (mids[h] == 0xffffffffu) continue;
+                        float tv = 2.0f * cdl[h] + off[h];  // 2x is exact: one rounding, as in double
+                        float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                        for (int g = 0; g < 4; ++g) {
+                            if ((uint32_t)g >= ng) break;
+                            const uint32_t ws[4] = {cw[h][g].x, cw[h][g].y, cw[h][g].z, cw[h][g].w};
 #pragma unroll
                             for (int hh = 0; hh < 4; ++hh) {
 #pragma unroll
                                 for (int j = 0; j < 4; ++j) {
-                                    uint32_t code = (ws[hh] >> (8 * j)) & 0xffu;
-                                    acc[j] = acc[j] + s.lut[(g + 4 * hh + j) * PK_KS + code];
+                                    const uint32_t code = (ws[hh] >> (8 * j)) & 0xffu;
+                                    acc[j] = acc[j] + s.lut[(16 * g + 4 * hh + j) * PK_KS + code];
                                 }
                             }
                         }
-                    } else {
-                        for (uint32_t m = 0; m < nsq; ++m) acc[m & 3] = acc[m & 3] + s.lut[m * PK_KS + codes[m]];
+                        tv = tv + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
+                        const uint64_t kk = ((uint64_t)pk_f2ord(tv) << 32) | mids[h];
+                        if (kk <= K) B[atomicAdd(&s.red[2], 1u)] = kk;
                     }
-                    tv = tv + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
-                    const uint64_t kk = ((uint64_t)pk_f2ord(tv) << 32) | mid;
-                    if (kk <= K) B[atomicAdd(&s.red[2], 1u)] = kk;
+                } else {
+                    for (uint32_t h = 0; h < R / WALK_THREADS; ++h) {
+                        const uint32_t p = p0 + h * WALK_THREADS + tid;
+                        if (p >= tot) break;
+                        uint32_t lo = 0, hi = ne;  // c_pre[lo] <= p < c_pre[hi]
+                        while (hi - lo > 1) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (s.c_pre[mid] <= p) lo = mid; else hi = mid;
+                        }
+                        const uint32_t mid = s.c_b[lo] + (p - s.c_pre[lo]);
+                        float tv = 2.0f * s.c_fd[lo] + a.rec_off[mid];  // 2x is exact: one rounding, as in double
+                        const uint8_t* codes = a.rec_codes + (uint64_t)mid * nsq;
+                        float acc[4] = {0.f, 0.f, 0.f, 0.f};
+                        for (uint32_t m = 0; m < nsq; ++m) acc[m & 3] = acc[m & 3] + s.lut[m * PK_KS + codes[m]];
+                        tv = tv + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
+                        const uint64_t kk = ((uint64_t)pk_f2ord(tv) << 32) | mid;
+                        if (kk <= K) B[atomicAdd(&s.red[2], 1u)] = kk;
+                    }
                 }
                 __syncthreads();
                 n = s.red[2];
@@ -1014,21 +1050,23 @@ __device__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
     }
 }
 
-// One query per workgroup (qlist or identity), or, with wcount, grid-stride
+// pk_walk: one query per workgroup (qlist or identity).  pk_walk_list:
+// grid-stride
 // over the device list wlist[0..*wcount) of the queries that take the walk
 // (the others of the batch are budget-scan queries); every workgroup leaves
 // the loop when the list is exhausted.
 template <int FTKM>
 __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
-    if (a.wcount) {
-        const uint32_t n = *a.wcount;
-        for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
-            pk_walk_query<FTKM>(a, a.wlist[i]);
-            __syncthreads();
-        }
-        return;
-    }
     pk_walk_query<FTKM>(a, a.qlist ? a.qlist[blockIdx.x] : blockIdx.x);
+}
+
+template <int FTKM>
+__global__ __launch_bounds__(WALK_THREADS) void pk_walk_list(PkWalkArgs a) {
+    const uint32_t n = *a.wcount;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        pk_walk_query<FTKM>(a, a.wlist[i]);
+        __syncthreads();
+    }
 }
 
 // Ordered list of the queries of a batch that take the walk (no fine budget
@@ -1151,22 +1189,30 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
     static bool attr_set = false;
     if (!attr_set) {
         const void* fns[] = {(const void*)pk_walk<1024>, (const void*)pk_walk<2048>,
+                             (const void*)pk_walk_list<1024>, (const void*)pk_walk_list<2048>,
                              (const void*)pk_scan_budget_kernel<1024>, (const void*)pk_scan_budget_kernel<2048>};
         for (const void* f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr_set = true;
     }
     // budget scan first (its own launch, timed between ev_mid and the end),
     // then the walk for the queries without a budget
+    auto launch_walk = [&](hipStream_t st) {
+        const bool lst = args->wcount != nullptr;
+        if (args->ftk <= 1024) {
+            if (lst) hipLaunchKernelGGL(pk_walk_list<1024>, dim3(wgrid), dim3(WALK_THREADS), smem, st, *args);
+            else hipLaunchKernelGGL(pk_walk<1024>, dim3(wgrid), dim3(WALK_THREADS), smem, st, *args);
+        } else {
+            if (lst) hipLaunchKernelGGL(pk_walk_list<2048>, dim3(wgrid), dim3(WALK_THREADS), smem, st, *args);
+            else hipLaunchKernelGGL(pk_walk<2048>, dim3(wgrid), dim3(WALK_THREADS), smem, st, *args);
+        }
+    };
     const bool fork = args->q_nf && s2 && ev_fork && ev_join;
     hipStream_t sw = fork ? s2 : s;
     hipError_t e;
     if (fork) {
         if ((e = hipEventRecord(ev_fork, s)) != hipSuccess) return e;
         if ((e = hipStreamWaitEvent(s2, ev_fork, 0)) != hipSuccess) return e;
-        if (args->ftk <= 1024)
-            hipLaunchKernelGGL(pk_walk<1024>, dim3(wgrid), dim3(WALK_THREADS), smem, sw, *args);
-        else
-            hipLaunchKernelGGL(pk_walk<2048>, dim3(wgrid), dim3(WALK_THREADS), smem, sw, *args);
+        launch_walk(sw);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = hipEventRecord(ev_join, s2)) != hipSuccess) return e;
     }
@@ -1179,10 +1225,7 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
     }
     if (ev_mid) (void)hipEventRecord(ev_mid, s);
     if (fork) return hipStreamWaitEvent(s, ev_join, 0);
-    if (args->ftk <= 1024)
-        hipLaunchKernelGGL(pk_walk<1024>, dim3(wgrid), dim3(WALK_THREADS), smem, s, *args);
-    else
-        hipLaunchKernelGGL(pk_walk<2048>, dim3(wgrid), dim3(WALK_THREADS), smem, s, *args);
+    launch_walk(s);
     return hipGetLastError();
 }
 
