@@ -1258,6 +1258,82 @@ struct PkRerankArgs {
 
 // Fast mode: distances sorted by (distance, filter rank).  Exact mode: the
 // result MaxHeap of rank_topN_points fed in filter-heap order, then reorder().
+// keys[0 .. K1-1] = the K1 smallest of the distinct keys[0 .. n-1], ascending
+// (the rest of keys is left as it was).  A radix select of the K1-th smallest
+// high word, starting at the highest bit in which the high words differ, then
+// the entries at or below it placed by rank.  Returns false (keys untouched)
+// when more than PK_PREFIX_CAP entries share or undercut that high word.
+#define PK_PREFIX_CAP 256
+__device__ bool pk_sorted_prefix(uint64_t* keys, uint32_t n, uint32_t K1) {
+    __shared__ uint32_t hist[258];
+    __shared__ uint32_t tmp[8];
+    __shared__ uint32_t red[9];
+    __shared__ uint64_t sb[PK_PREFIX_CAP];
+    const uint32_t t = threadIdx.x, nb = blockDim.x;
+    uint32_t ka = 0xffffffffu, ko = 0u;
+    for (uint32_t i = t; i < n; i += nb) {
+        const uint32_t h = (uint32_t)(keys[i] >> 32);
+        ka &= h;
+        ko |= h;
+    }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        ka &= __shfl_xor(ka, o, 64);
+        ko |= __shfl_xor(ko, o, 64);
+    }
+    if ((t & 63) == 0) { red[2 * (t >> 6)] = ka; red[2 * (t >> 6) + 1] = ko; }
+    if (t == 0) red[8] = 0;
+    __syncthreads();
+    ka = 0xffffffffu; ko = 0u;
+    for (uint32_t w = 0; w < (nb >> 6); ++w) { ka &= red[2 * w]; ko |= red[2 * w + 1]; }
+    const uint32_t diff = ka ^ ko;
+    const int top = diff ? 32 - __clz(diff) : 0;
+    uint32_t mask = top >= 32 ? 0u : ~((1u << top) - 1u);
+    uint32_t prefix = ka & mask, want = K1;
+    for (int hi = top; hi > 0; hi -= 8) {
+        const int shift = hi > 8 ? hi - 8 : 0;
+        const uint32_t dm = (1u << (hi - shift)) - 1u;
+        hist[t] = 0;
+        __syncthreads();
+        for (uint32_t i = t; i < n; i += nb) {
+            const uint32_t h = (uint32_t)(keys[i] >> 32);
+            if ((h & mask) == prefix) atomicAdd(&hist[(h >> shift) & dm], 1u);
+        }
+        __syncthreads();
+        const uint32_t c = hist[t];
+        uint32_t tot;
+        const uint32_t before = pk_block_exscan(c, tmp, &tot);
+        if (c > 0 && before < want && want <= before + c) {
+            hist[256] = t;
+            hist[257] = want - before;
+        }
+        __syncthreads();
+        prefix |= hist[256] << shift;
+        mask |= dm << shift;
+        want = hist[257];
+        __syncthreads();
+    }
+    const uint32_t kc = prefix;
+    for (uint32_t i = t; i < n; i += nb) {
+        const uint64_t e = keys[i];
+        if ((uint32_t)(e >> 32) <= kc) {
+            const uint32_t p = atomicAdd(&red[8], 1u);
+            if (p < PK_PREFIX_CAP) sb[p] = e;
+        }
+    }
+    __syncthreads();
+    const uint32_t c = red[8];
+    if (c > PK_PREFIX_CAP) return false;  // uniform across the block
+    for (uint32_t i = t; i < c; i += nb) {
+        const uint64_t e = sb[i];
+        uint32_t r = 0;
+        for (uint32_t j = 0; j < c; ++j) r += sb[j] < e ? 1u : 0u;
+        if (r < K1) keys[r] = e;
+    }
+    __syncthreads();
+    return true;
+}
+
 __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
@@ -1284,33 +1360,64 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
     // 8 lanes per row: lane j holds element j of each 8-lane half block (the
     // compiled order of pk_rerank_dist); shuffles combine the lanes in that
     // order, so every lane ends with the same bits as pk_rerank_dist
+    // Two rows per group and iteration, every load of a 128-element block of
+    // both rows issued before the first is used (the gathers are latency
+    // bound: one row is a few cache lines).
     {
+        constexpr int RP = 2, SB = 8;  // rows per group, 16-element steps per block
         const uint32_t g = threadIdx.x >> 3, j = threadIdx.x & 7, ng = blockDim.x >> 3;
         const int src = (int)(threadIdx.x & 63u & ~7u);
-        for (uint32_t r0 = 0; r0 < n2; r0 += ng) {
-            const uint32_t r = r0 + g;
-            const bool valid = r < n;
-            const uint32_t mid = valid ? a.filt_id[(uint64_t)qi * ftk + r] : 0u;
-            const float* x = a.raw + (uint64_t)mid * a.dim;
-            float t = 0.0f;
-            uint32_t m = 0;
-            for (; m + 16 <= a.dim; m += 16) {
-                const float d0 = x[m + j] - qv[m + j];
-                const float d1 = x[m + 8 + j] - qv[m + 8 + j];
-                const float l = fmaf(d0, d0, d1 * d1);
-                float lj[8];
+        const uint32_t dim = a.dim, nfull = dim >> 4;
+        for (uint32_t r0 = 0; r0 < n2; r0 += RP * ng) {
+            uint32_t rr[RP], mid[RP];
+            const float* x[RP];
+            float t[RP];
 #pragma unroll
-                for (int k = 0; k < 8; ++k) lj[k] = __shfl(l, src + k, 64);
-                t = (((lj[0] + lj[1]) + (lj[2] + lj[3])) + ((lj[4] + lj[5]) + lj[6])) + (lj[7] + t);
+            for (int h = 0; h < RP; ++h) {
+                rr[h] = r0 + h * ng + g;
+                mid[h] = rr[h] < n ? a.filt_id[(uint64_t)qi * ftk + rr[h]] : 0u;
+                x[h] = a.raw + (uint64_t)mid[h] * dim;
+                t[h] = 0.0f;
             }
-            if (m < a.dim) t = t + pk_rerank_tail(x + m, qv + m, a.dim - m);
-            if (j == 0 && r < n2) {
-                uint64_t key = ~0ull;
-                if (valid) {
-                    key = ((uint64_t)pk_f2ord(t) << 32) | (bud ? a.mem2local[mid] : r);
-                    if (exact) dist[r] = t;
+            for (uint32_t s0 = 0; s0 < nfull; s0 += SB) {
+                float v0[RP][SB], v1[RP][SB];
+#pragma unroll
+                for (int h = 0; h < RP; ++h)
+#pragma unroll
+                    for (int st = 0; st < SB; ++st)
+                        if (s0 + st < nfull) {
+                            v0[h][st] = x[h][16 * (s0 + st) + j];
+                            v1[h][st] = x[h][16 * (s0 + st) + 8 + j];
+                        }
+#pragma unroll
+                for (int st = 0; st < SB; ++st) {
+                    if (s0 + st >= nfull) break;
+                    const uint32_t m = 16 * (s0 + st);
+                    const float q0 = qv[m + j], q1 = qv[m + 8 + j];
+#pragma unroll
+                    for (int h = 0; h < RP; ++h) {
+                        const float d0 = v0[h][st] - q0;
+                        const float d1 = v1[h][st] - q1;
+                        const float l = fmaf(d0, d0, d1 * d1);
+                        float lj[8];
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) lj[k] = __shfl(l, src + k, 64);
+                        t[h] = (((lj[0] + lj[1]) + (lj[2] + lj[3])) + ((lj[4] + lj[5]) + lj[6])) + (lj[7] + t[h]);
+                    }
                 }
-                if (!exact) keys[r] = key;
+            }
+#pragma unroll
+            for (int h = 0; h < RP; ++h) {
+                if (16 * nfull < dim) t[h] = t[h] + pk_rerank_tail(x[h] + 16 * nfull, qv + 16 * nfull, dim - 16 * nfull);
+                const uint32_t r = rr[h];
+                if (j == 0 && r < n2) {
+                    uint64_t key = ~0ull;
+                    if (r < n) {
+                        key = ((uint64_t)pk_f2ord(t[h]) << 32) | (bud ? a.mem2local[mid[h]] : r);
+                        if (exact) dist[r] = t[h];
+                    }
+                    if (!exact) keys[r] = key;
+                }
             }
         }
     }
@@ -1349,7 +1456,8 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
         }
         return;
     }
-    if (n > 1) {
+    // only the first min(n, topk + 1) entries of the sorted order are read
+    if (n > 1 && !pk_sorted_prefix(keys, n, n < topk + 1 ? n : topk + 1)) {
         uint32_t m2 = 2;
         while (m2 < n) m2 <<= 1;
         for (uint32_t k = 2; k <= m2; k <<= 1)

@@ -124,6 +124,108 @@ __device__ void pk_topk_block(ValF val, uint32_t L, uint32_t k, float* out_val, 
     *tie = t;
 }
 
+// The same selection for rows of at most SORT_THREADS * PER entries, with
+// every key read once into registers: thread t holds columns t*PER ..
+// t*PER+PER-1 (so a scan over threads is column order).  The radix select
+// starts at the highest bit in which the row's keys differ (distances of one
+// row usually share their sign and top exponent bits, which would send every
+// key of the first pass to one histogram bin); the entries below kc and the
+// first ties at it are compacted with one packed scan and placed by rank.
+template <int PER, typename KeyF>
+__device__ void pk_topk_block_reg(KeyF key, uint32_t L, uint32_t k, float* out_val, uint32_t* out_idx, bool* tie,
+                                  uint32_t flag_lim) {
+    __shared__ uint32_t hist[258];
+    __shared__ uint32_t tmp[8];
+    __shared__ uint32_t red[8];
+    __shared__ uint64_t buf[TOPK_CAP];
+    const uint32_t t = threadIdx.x, base = t * PER;
+    const uint32_t nt = base < L ? min((uint32_t)PER, L - base) : 0u;
+    uint32_t kv[PER];
+    uint32_t ka = 0xffffffffu, ko = 0u;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        kv[j] = (uint32_t)j < nt ? key(base + j) : 0u;
+        if ((uint32_t)j < nt) { ka &= kv[j]; ko |= kv[j]; }
+    }
+    // bits shared by every key of the row
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        ka &= __shfl_xor(ka, o, 64);
+        ko |= __shfl_xor(ko, o, 64);
+    }
+    if ((t & 63) == 0) { red[2 * (t >> 6)] = ka; red[2 * (t >> 6) + 1] = ko; }
+    __syncthreads();
+    ka = 0xffffffffu; ko = 0u;
+    for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) { ka &= red[2 * w]; ko |= red[2 * w + 1]; }
+    const uint32_t diff = ka ^ ko;
+    const int top = diff ? 32 - __clz(diff) : 0;  // bits [top-1, 0] vary
+    uint32_t mask = top >= 32 ? 0u : ~((1u << top) - 1u);
+    uint32_t prefix = ka & mask, want = k;
+    for (int hi = top; hi > 0; hi -= 8) {
+        const int shift = hi > 8 ? hi - 8 : 0;
+        const uint32_t dm = (1u << (hi - shift)) - 1u;
+        hist[t] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PER; ++j)
+            if ((uint32_t)j < nt && (kv[j] & mask) == prefix) atomicAdd(&hist[(kv[j] >> shift) & dm], 1u);
+        __syncthreads();
+        const uint32_t c = hist[t];
+        uint32_t tot;
+        const uint32_t before = pk_block_exscan(c, tmp, &tot);
+        if (c > 0 && before < want && want <= before + c) {
+            hist[256] = t;
+            hist[257] = want - before;
+        }
+        __syncthreads();
+        prefix |= hist[256] << shift;
+        mask |= dm << shift;
+        want = hist[257];
+        __syncthreads();
+    }
+    const uint32_t kc = prefix;
+    uint32_t cl = 0, ce = 0;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        if ((uint32_t)j < nt) {
+            cl += kv[j] < kc ? 1u : 0u;
+            ce += kv[j] == kc ? 1u : 0u;
+        }
+    }
+    uint32_t tot;
+    const uint32_t pre = pk_block_exscan(cl | (ce << 16), tmp, &tot);
+    const uint32_t nlt = tot & 0xffffu, neq = tot >> 16;
+    uint32_t pl = pre & 0xffffu, pe = nlt + (pre >> 16);
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        if ((uint32_t)j < nt) {
+            const uint64_t e = ((uint64_t)kv[j] << 32) | (base + j);
+            if (kv[j] < kc) buf[pl++] = e;
+            else if (kv[j] == kc) { if (pe < k) buf[pe] = e; ++pe; }
+        }
+    }
+    __syncthreads();
+    bool tf = false;
+    for (uint32_t i = t; i < k; i += blockDim.x) {
+        const uint64_t e = buf[i];
+        const uint32_t ek = (uint32_t)(e >> 32);
+        uint32_t r = 0, eqa = 0;  // rank; equal keys after it
+        for (uint32_t j = 0; j < k; ++j) {
+            const uint64_t f = buf[j];
+            r += f < e ? 1u : 0u;
+            eqa += ((uint32_t)(f >> 32) == ek && f > e) ? 1u : 0u;
+        }
+        if (out_val) out_val[r] = pk_ord2f(ek);
+        if (out_idx) out_idx[r] = (uint32_t)(e & 0xffffffffu);
+        // equal to the next entry of the full order (inside the k, or a tie cut at the k-th)
+        if (r < flag_lim && (r + 1 < k ? eqa > 0 : nlt + neq > k)) tf = true;
+    }
+    *tie = tf;
+}
+
+#define TOPK_REG_PER 8
+#define TOPK_REG_RANK 256  // largest k placed by rank (larger k: the LDS sort)
+
 __global__ __launch_bounds__(SORT_THREADS) void pk_topk_rows(const float* __restrict__ vals, uint32_t L,
                                                              uint64_t ld_in, uint32_t k, float* __restrict__ out_val,
                                                              uint32_t* __restrict__ out_idx, uint8_t* __restrict__ flags,
@@ -156,13 +258,49 @@ __global__ __launch_bounds__(SORT_THREADS) void pk_cell_topk(const float* __rest
     pk_topk_block([&](uint32_t f) { return (cd + st[f]) + fd[f]; }, F, k, out_val + row * k, nullptr, &tie, 0);
 }
 
+__global__ __launch_bounds__(SORT_THREADS) void pk_topk_rows_reg(const float* __restrict__ vals, uint32_t L,
+                                                                 uint64_t ld_in, uint32_t k,
+                                                                 float* __restrict__ out_val,
+                                                                 uint32_t* __restrict__ out_idx,
+                                                                 uint8_t* __restrict__ flags, uint32_t flag_lim) {
+    const uint64_t row = blockIdx.x;
+    const float* v = vals + row * ld_in;
+    bool tie = false;
+    pk_topk_block_reg<TOPK_REG_PER>([&](uint32_t i) { return pk_f2ord(v[i]); }, L, k, out_val + row * k,
+                                    out_idx + row * k, &tie, flag_lim);
+    __shared__ uint32_t any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    if (tie) any = 1;
+    __syncthreads();
+    if (threadIdx.x == 0 && any && flags) flags[row] = 1u | flags[row];
+}
+
+__global__ __launch_bounds__(SORT_THREADS) void pk_cell_topk_reg(const float* __restrict__ csd,
+                                                                 const uint32_t* __restrict__ csi, uint32_t cstride,
+                                                                 const float* __restrict__ fdist,
+                                                                 const float* __restrict__ stationary, uint32_t F,
+                                                                 uint32_t k, float* __restrict__ out_val) {
+    const uint64_t row = blockIdx.x;
+    const float cd = csd[row * cstride];
+    const float* st = stationary + (uint64_t)csi[row * cstride] * F;
+    const float* fd = fdist + row * F;
+    bool tie = false;
+    pk_topk_block_reg<TOPK_REG_PER>([&](uint32_t f) { return pk_f2ord((cd + st[f]) + fd[f]); }, F, k,
+                                    out_val + row * k, nullptr, &tie, 0);
+}
+
 extern "C" hipError_t pk_launch_topk_rows(const float* vals, uint32_t rows, uint32_t L, uint64_t ld_in, uint32_t k,
                                           float* out_val, uint32_t* out_idx, uint8_t* flags, uint32_t flag_lim,
                                           hipStream_t s) {
     if (rows == 0 || k == 0) return hipSuccess;
     if (k > TOPK_CAP || k > L) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(pk_topk_rows, dim3(rows), dim3(SORT_THREADS), 0, s, vals, L, ld_in, k, out_val, out_idx, flags,
-                       flag_lim);
+    if (L <= SORT_THREADS * TOPK_REG_PER && k <= TOPK_REG_RANK)
+        hipLaunchKernelGGL(pk_topk_rows_reg, dim3(rows), dim3(SORT_THREADS), 0, s, vals, L, ld_in, k, out_val,
+                           out_idx, flags, flag_lim);
+    else
+        hipLaunchKernelGGL(pk_topk_rows, dim3(rows), dim3(SORT_THREADS), 0, s, vals, L, ld_in, k, out_val, out_idx,
+                           flags, flag_lim);
     return hipGetLastError();
 }
 
@@ -171,7 +309,11 @@ extern "C" hipError_t pk_launch_cell_topk(const float* csd, const uint32_t* csi,
                                           float* out_val, hipStream_t s) {
     if (rows == 0 || k == 0) return hipSuccess;
     if (k > TOPK_CAP || k > F) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(pk_cell_topk, dim3(rows), dim3(SORT_THREADS), 0, s, csd, csi, cstride, fdist, stationary, F, k,
-                       out_val);
+    if (F <= SORT_THREADS * TOPK_REG_PER && k <= TOPK_REG_RANK)
+        hipLaunchKernelGGL(pk_cell_topk_reg, dim3(rows), dim3(SORT_THREADS), 0, s, csd, csi, cstride, fdist,
+                           stationary, F, k, out_val);
+    else
+        hipLaunchKernelGGL(pk_cell_topk, dim3(rows), dim3(SORT_THREADS), 0, s, csd, csi, cstride, fdist, stationary,
+                           F, k, out_val);
     return hipGetLastError();
 }
