@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--serving-queries", type=int, default=100_000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--index-cache", type=str, default=None,
+                    help="directory: load the trained index arrays from it when present, else save them there")
     ap.add_argument("--calib-copy", action="store_true",
                     help="one 2 GiB device-to-device copy before the search (HBM counter calibration)")
     args = ap.parse_args()
@@ -106,7 +108,17 @@ def main():
     log(f"data {N}x{d} + {nq} queries in {time.time() - t0:.1f}s")
     ctx = Context(0)
     t0 = time.time()
-    ix = train.build(ctx, x, C, F, nsq, whether_norm=True)
+    keys = ("coarse", "fine", "codebook", "cell_assign", "records")
+    cache = os.path.join(args.index_cache, f"index_{N}x{d}_{C}_{F}_{nsq}.pt") if args.index_cache else None
+    if cache and os.path.exists(cache):
+        saved = torch.load(cache, map_location=dev, weights_only=True)
+        ix = dict({k: saved[k] for k in keys}, raw=x, whether_norm=True, nsq=nsq)
+        log(f"index arrays loaded from {cache}")
+    else:
+        ix = train.build(ctx, x, C, F, nsq, whether_norm=True)
+        if cache:
+            os.makedirs(args.index_cache, exist_ok=True)
+            torch.save({k: ix[k] for k in keys}, cache)
     torch.cuda.synchronize()
     build_s = time.time() - t0
     log(f"train+assign+encode {build_s:.1f}s")

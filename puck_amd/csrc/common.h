@@ -127,6 +127,74 @@ __device__ __forceinline__ uint32_t pk_block_exscan(uint32_t v, uint32_t* tmp, u
     return r;
 }
 
+// AND and OR of ka / ko over the block (blockDim.x a multiple of 64, at most
+// 1024); red needs 2 * blockDim.x / 64 words.  Every thread gets both.
+__device__ __forceinline__ void pk_block_andor(uint32_t& ka, uint32_t& ko, uint32_t* red) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        ka &= __shfl_xor(ka, o, 64);
+        ko |= __shfl_xor(ko, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[2 * (threadIdx.x >> 6)] = ka;
+        red[2 * (threadIdx.x >> 6) + 1] = ko;
+    }
+    __syncthreads();
+    ka = 0xffffffffu;
+    ko = 0u;
+    for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) {
+        ka &= red[2 * w];
+        ko |= red[2 * w + 1];
+    }
+    __syncthreads();
+}
+
+// Key of rank `want` (1-based, at most the number of keys) among the keys the
+// threads hand over: each(sink) calls sink(key) for each of the calling
+// thread's keys, and is called once per pass.  MSB-first radix select with
+// 8-bit digits that starts at the highest bit in which the keys differ (the
+// distances of one row usually share their sign and top exponent bits,
+// which would send every key of a first pass to one histogram bin).
+// blockDim.x >= 256, a multiple of 64; hist has 258 words, tmp
+// blockDim.x / 64 + 1, red 2 * blockDim.x / 64.  Every thread returns it.
+template <typename EachF>
+__device__ uint32_t pk_select_rank_each(EachF each, uint32_t want, uint32_t* hist, uint32_t* tmp, uint32_t* red) {
+    const uint32_t t = threadIdx.x;
+    uint32_t ka = 0xffffffffu, ko = 0u;
+    each([&](uint32_t k) {
+        ka &= k;
+        ko |= k;
+    });
+    pk_block_andor(ka, ko, red);
+    const uint32_t diff = ka ^ ko;
+    const int top = diff ? 32 - __clz(diff) : 0;  // bits [top-1, 0] vary
+    uint32_t mask = top >= 32 ? 0u : ~((1u << top) - 1u);
+    uint32_t prefix = ka & mask;
+    for (int hi = top; hi > 0; hi -= 8) {
+        const int shift = hi > 8 ? hi - 8 : 0;
+        const uint32_t dm = (1u << (hi - shift)) - 1u;
+        if (t < 256) hist[t] = 0;
+        __syncthreads();
+        each([&](uint32_t k) {
+            if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & dm], 1u);
+        });
+        __syncthreads();
+        const uint32_t c = t < 256 ? hist[t] : 0u;
+        uint32_t tot;
+        const uint32_t before = pk_block_exscan(c, tmp, &tot);
+        if (c > 0 && before < want && want <= before + c) {
+            hist[256] = t;
+            hist[257] = want - before;
+        }
+        __syncthreads();
+        prefix |= hist[256] << shift;
+        mask |= dm << shift;
+        want = hist[257];
+        __syncthreads();
+    }
+    return prefix;
+}
+
 // Key of rank `want` (1-based) among key(0..n-1): MSB-first radix select
 // with 8-bit digits, keys recomputed on every pass.  blockDim.x == 256 (one
 // histogram bin per thread); hist has 258 words.  Every thread returns it.

@@ -500,18 +500,18 @@ __device__ uint32_t pk_walk_merge(const PkWalkArgs& a, PkWalkSmem& s, uint32_t n
 }
 
 // k-th smallest (1-based) of the 64-bit keys v[0..n) in LDS: MSB-first radix
-// select, 8-bit digits.  blockDim.x == 256; hist has 258 words.
+// select, 8-bit digits.  blockDim.x >= 256; hist has 258 words.
 __device__ uint64_t pk_select_rank64(const uint64_t* v, uint32_t n, uint32_t want, uint32_t* hist, uint32_t* tmp) {
     uint64_t prefix = 0, mask = 0;
     for (int shift = 56; shift >= 0; shift -= 8) {
-        hist[threadIdx.x] = 0;
+        if (threadIdx.x < 256) hist[threadIdx.x] = 0;
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
             const uint64_t k = v[i];
             if ((k & mask) == prefix) atomicAdd(&hist[(uint32_t)(k >> shift) & 255u], 1u);
         }
         __syncthreads();
-        const uint32_t c = hist[threadIdx.x];
+        const uint32_t c = threadIdx.x < 256 ? hist[threadIdx.x] : 0u;
         uint32_t tot;
         const uint32_t before = pk_block_exscan(c, tmp, &tot);
         if (c > 0 && before < want && want <= before + c) {
@@ -551,156 +551,212 @@ __device__ uint32_t pk_keep_le(uint64_t* v, uint32_t n, uint64_t K, uint32_t* tm
 // the ftk smallest (distance, memory id) keys, kept in an unsorted LDS buffer
 // (cap entries, R points scored per round).  When the buffer could overflow it is
 // cut back to the keys whose distance is at most the ftk-th smallest distance
-// (a 4-pass select on the distance bits); the exact (distance, id) cut to ftk
+// (a select on the distance bits); the exact (distance, id) cut to ftk
 // entries is made once, at the end.  The result does not depend on any
 // visiting order, so it needs no replay.
+//
+// Cells are taken in segments of SCAN_THREADS * SCAN_CPT (thread t holds
+// cells seg + j * SCAN_THREADS + t in registers; one segment when F fits),
+// and the selected cells of a segment are scored in batches of up to SCAN_CB
+// cells, so a list costs one chain of record loads per R points rather than
+// one per window of cells.
+#ifndef SCAN_THREADS
+#define SCAN_THREADS 256
+#endif
+#ifndef SCAN_CPT
+#define SCAN_CPT 8
+#endif
+#ifndef SCAN_CB
+#define SCAN_CB 512
+#endif
+#ifndef SCAN_PP
+#define SCAN_PP 4  // points per thread and round
+#endif
+#define SCAN_RED 32  // reduction words: 0-3 counters, 4-13 scan, 16-31 and/or (up to 512 threads)
+
 template <int FTKM>
 __device__ void pk_scan_budget(const PkWalkArgs& a, PkWalkSmem& s, uint32_t qi, uint32_t b, uint64_t* B,
                                uint32_t cap, uint32_t R, uint32_t* hist) {
     const uint32_t F = a.F, ftk = a.ftk, nsq = a.nsq;
-    const int tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, NT = SCAN_THREADS, SEG = SCAN_THREADS * SCAN_CPT;
+    uint32_t* tmp = s.red + 4;
+    uint32_t* rao = s.red + 16;
     uint32_t n = 0;
     uint64_t K = ~0ull;  // keys above K cannot enter the filter set
     uint32_t scc = a.scc;
     if (a.q_scc) scc = a.q_scc[qi] < scc ? a.q_scc[qi] : scc;
     const float* fdq = a.fdist + (uint64_t)qi * F;
     uint32_t st_cells = 0, st_pts = 0;
-    auto hi32 = [&](uint32_t i) { return (uint32_t)(B[i] >> 32); };
+    auto hi32_each = [&](auto sink) {
+        for (uint32_t i = tid; i < n; i += NT) sink((uint32_t)(B[i] >> 32));
+    };
     for (uint32_t l = 0; l < scc; ++l) {
         const uint32_t c = a.csel_id[(uint64_t)qi * a.cstride + l];
         const float cd = a.csel_d[(uint64_t)qi * a.cstride + l];
         const float* stc = a.stationary + (uint64_t)c * F;
         const uint32_t* csc = a.cell_start + (uint64_t)c * F;
-        auto key = [&](uint32_t k) { return pk_f2ord((cd + stc[k]) + fdq[k]); };
-        const uint32_t kcut = pk_select_rank(key, F, b, hist, s.red + 4);
-        for (uint32_t w0 = 0; w0 < F; w0 += blockDim.x) {
-            const uint32_t k = w0 + tid;
-            uint32_t in = 0, sz = 0, cb = 0;
-            float cdist = 0.f;
-            if (k < F) {
-                const float cv = (cd + stc[k]) + fdq[k];
-                if (pk_f2ord(cv) <= kcut) {
-                    cb = csc[k];
-                    sz = csc[k + 1] - cb;
-                    in = sz > 0 ? 1u : 0u;
-                    cdist = cv;
-                }
-            }
-            // one scan of (points << 9 | cell): at most 256 cells per window
-            uint32_t tot2;
-            const uint32_t pre2 = pk_block_exscan(in ? ((sz << 9) | 1u) : 0u, s.red + 4, &tot2);
-            const uint32_t ne = tot2 & 511u, tot = tot2 >> 9;
-            if (in) {
-                s.c_b[pre2 & 511u] = cb;
-                s.c_fd[pre2 & 511u] = cdist;
-                s.c_pre[pre2 & 511u] = pre2 >> 9;
-            }
-            if (tid == 0) s.c_pre[ne] = tot;
-            __syncthreads();
-            st_cells += ne;
-            st_pts += tot;
-            for (uint32_t p0 = 0; p0 < tot; p0 += R) {
-                if (n + R > cap) {
-                    const uint32_t kc = pk_select_rank(hi32, n, ftk, hist, s.red + 4);
-                    K = ((uint64_t)kc << 32) | 0xffffffffull;
-                    n = pk_keep_le(B, n, K, s.red + 4);
-                    if (n + R > cap) {  // many equal distances at the cut: cut exactly
-                        K = pk_select_rank64(B, n, ftk, hist, s.red + 4);
-                        n = pk_keep_le(B, n, K, s.red + 4);
+        const bool one = F <= SEG;
+        float cv[SCAN_CPT];
+#pragma unroll
+        for (int j = 0; j < SCAN_CPT; ++j) {
+            const uint32_t k = j * NT + tid;
+            cv[j] = one && k < F ? (cd + stc[k]) + fdq[k] : 0.f;
+        }
+        // the list's cut: b-th smallest cell key
+        uint32_t kcut;
+        if (one)
+            kcut = pk_select_rank_each(
+                [&](auto sink) {
+#pragma unroll
+                    for (int j = 0; j < SCAN_CPT; ++j)
+                        if (j * NT + tid < F) sink(pk_f2ord(cv[j]));
+                },
+                b, hist, tmp, rao);
+        else
+            kcut = pk_select_rank_each(
+                [&](auto sink) {
+                    for (uint32_t k = tid; k < F; k += NT) sink(pk_f2ord((cd + stc[k]) + fdq[k]));
+                },
+                b, hist, tmp, rao);
+        for (uint32_t seg = 0; seg < F; seg += SEG) {
+            // selected non-empty cells of the segment held by this thread
+            uint32_t cb[SCAN_CPT], sz[SCAN_CPT];
+            uint32_t ns = 0, np = 0;
+#pragma unroll
+            for (int j = 0; j < SCAN_CPT; ++j) {
+                const uint32_t k = seg + j * NT + tid;
+                sz[j] = 0;
+                cb[j] = 0;
+                if (k < F) {
+                    if (!one) cv[j] = (cd + stc[k]) + fdq[k];
+                    if (pk_f2ord(cv[j]) <= kcut) {
+                        cb[j] = csc[k];
+                        sz[j] = csc[k + 1] - cb[j];
                     }
                 }
-                if (tid == 0) s.red[2] = n;
+                ns += sz[j] > 0 ? 1u : 0u;
+                np += sz[j];
+            }
+            uint32_t nsel, npts;
+            const uint32_t pos0 = pk_block_exscan(ns, tmp, &nsel);
+            const uint32_t pp0 = pk_block_exscan(np, tmp, &npts);
+            st_cells += nsel;
+            st_pts += npts;
+            for (uint32_t b0 = 0; b0 < nsel; b0 += SCAN_CB) {
+                const uint32_t ne = nsel - b0 < SCAN_CB ? nsel - b0 : SCAN_CB;
+                // the batch's cells with their first point (segment order)
+                uint32_t pos = pos0, pp = pp0;
+#pragma unroll
+                for (int j = 0; j < SCAN_CPT; ++j) {
+                    if (sz[j] > 0) {
+                        if (pos >= b0 && pos < b0 + ne) {
+                            s.c_b[pos - b0] = cb[j];
+                            s.c_fd[pos - b0] = cv[j];
+                            s.c_pre[pos - b0] = pp;
+                        }
+                        if (pos == b0 + ne) s.red[1] = pp;  // first point of the next batch
+                        ++pos;
+                        pp += sz[j];
+                    }
+                }
                 __syncthreads();
-                if ((nsq & 15) == 0 && nsq <= 64 && R <= 4 * WALK_THREADS) {
-                    // all records of the round first (up to 4 points x 4 x 16 B of
-                    // codes per thread), then their filter distances
-                    constexpr int PP = 4;
+                const uint32_t pbase = s.c_pre[0];
+                const uint32_t pend = b0 + ne < nsel ? s.red[1] : npts;
+                if (tid == 0) s.c_pre[ne] = pend;
+                const uint32_t tot = pend - pbase;
+                __syncthreads();
+                for (uint32_t p0 = 0; p0 < tot; p0 += R) {
+                    if (n + R > cap) {
+                        const uint32_t kc = pk_select_rank_each(hi32_each, ftk, hist, tmp, rao);
+                        K = ((uint64_t)kc << 32) | 0xffffffffull;
+                        n = pk_keep_le(B, n, K, tmp);
+                        if (n + R > cap) {  // many equal distances at the cut: cut exactly
+                            K = pk_select_rank64(B, n, ftk, hist, tmp);
+                            n = pk_keep_le(B, n, K, tmp);
+                        }
+                    }
+                    if (tid == 0) s.red[2] = n;
+                    __syncthreads();
+                    // all records of the round first, then their filter distances
+                    constexpr int PP = SCAN_PP;
+                    const bool vec = (nsq & 15) == 0 && nsq <= 64;
+                    const uint32_t ng = nsq >> 4;
                     uint4 cw[PP][4];
                     float off[PP], cdl[PP];
                     uint32_t mids[PP];
-                    const uint32_t ng = nsq >> 4;
 #pragma unroll
                     for (int h = 0; h < PP; ++h) {
-                        const uint32_t p = p0 + h * WALK_THREADS + tid;
+                        const uint32_t p = p0 + h * NT + tid;
                         mids[h] = 0xffffffffu;
-                        if (h * WALK_THREADS >= R || p >= tot) continue;
-                        uint32_t lo = 0, hi = ne;  // c_pre[lo] <= p < c_pre[hi]
+                        if (h * NT >= R || p >= tot) continue;
+                        const uint32_t P = pbase + p;
+                        uint32_t lo = 0, hi = ne;  // c_pre[lo] <= P < c_pre[hi]
                         while (hi - lo > 1) {
                             const uint32_t mid = (lo + hi) >> 1;
-                            if (s.c_pre[mid] <= p) lo = mid; else hi = mid;
+                            if (s.c_pre[mid] <= P) lo = mid; else hi = mid;
                         }
-                        const uint32_t mid = s.c_b[lo] + (p - s.c_pre[lo]);
+                        const uint32_t mid = s.c_b[lo] + (P - s.c_pre[lo]);
                         mids[h] = mid;
                         cdl[h] = s.c_fd[lo];
                         off[h] = a.rec_off[mid];
-                        const uint4* cp = (const uint4*)(a.rec_codes + (uint64_t)mid * nsq);
+                        if (vec) {
+                            const uint4* cp = (const uint4*)(a.rec_codes + (uint64_t)mid * nsq);
 #pragma unroll
-                        for (int g = 0; g < 4; ++g)
-                            if ((uint32_t)g < ng) cw[h][g] = cp[g];
+                            for (int g = 0; g < 4; ++g)
+                                if ((uint32_t)g < ng) cw[h][g] = cp[g];
+                        }
                     }
 #pragma unroll
                     for (int h = 0; h < PP; ++h) {
                         if (mids[h] == 0xffffffffu) continue;
                         float tv = 2.0f * cdl[h] + off[h];  // 2x is exact: one rounding, as in double
                         float acc[4] = {0.f, 0.f, 0.f, 0.f};
+                        if (vec) {
 #pragma unroll
-                        for (int g = 0; g < 4; ++g) {
-                            if ((uint32_t)g >= ng) break;
-                            const uint32_t ws[4] = {cw[h][g].x, cw[h][g].y, cw[h][g].z, cw[h][g].w};
+                            for (int g = 0; g < 4; ++g) {
+                                if ((uint32_t)g >= ng) break;
+                                const uint32_t ws[4] = {cw[h][g].x, cw[h][g].y, cw[h][g].z, cw[h][g].w};
 #pragma unroll
-                            for (int hh = 0; hh < 4; ++hh) {
+                                for (int hh = 0; hh < 4; ++hh) {
 #pragma unroll
-                                for (int j = 0; j < 4; ++j) {
-                                    const uint32_t code = (ws[hh] >> (8 * j)) & 0xffu;
-                                    acc[j] = acc[j] + s.lut[(16 * g + 4 * hh + j) * PK_KS + code];
+                                    for (int j = 0; j < 4; ++j) {
+                                        const uint32_t code = (ws[hh] >> (8 * j)) & 0xffu;
+                                        acc[j] = acc[j] + s.lut[(16 * g + 4 * hh + j) * PK_KS + code];
+                                    }
                                 }
                             }
+                        } else {
+                            const uint8_t* codes = a.rec_codes + (uint64_t)mids[h] * nsq;
+                            for (uint32_t m = 0; m < nsq; ++m)
+                                acc[m & 3] = acc[m & 3] + s.lut[m * PK_KS + codes[m]];
                         }
                         tv = tv + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
                         const uint64_t kk = ((uint64_t)pk_f2ord(tv) << 32) | mids[h];
                         if (kk <= K) B[atomicAdd(&s.red[2], 1u)] = kk;
                     }
-                } else {
-                    for (uint32_t h = 0; h < R / WALK_THREADS; ++h) {
-                        const uint32_t p = p0 + h * WALK_THREADS + tid;
-                        if (p >= tot) break;
-                        uint32_t lo = 0, hi = ne;  // c_pre[lo] <= p < c_pre[hi]
-                        while (hi - lo > 1) {
-                            const uint32_t mid = (lo + hi) >> 1;
-                            if (s.c_pre[mid] <= p) lo = mid; else hi = mid;
-                        }
-                        const uint32_t mid = s.c_b[lo] + (p - s.c_pre[lo]);
-                        float tv = 2.0f * s.c_fd[lo] + a.rec_off[mid];  // 2x is exact: one rounding, as in double
-                        const uint8_t* codes = a.rec_codes + (uint64_t)mid * nsq;
-                        float acc[4] = {0.f, 0.f, 0.f, 0.f};
-                        for (uint32_t m = 0; m < nsq; ++m) acc[m & 3] = acc[m & 3] + s.lut[m * PK_KS + codes[m]];
-                        tv = tv + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
-                        const uint64_t kk = ((uint64_t)pk_f2ord(tv) << 32) | mid;
-                        if (kk <= K) B[atomicAdd(&s.red[2], 1u)] = kk;
-                    }
+                    __syncthreads();
+                    n = s.red[2];
+                    __syncthreads();
                 }
-                __syncthreads();
-                n = s.red[2];
-                __syncthreads();
             }
         }
     }
     if (n > ftk) {
         // exact cut: the ftk smallest (distance, id) keys
-        const uint32_t kc = pk_select_rank(hi32, n, ftk, hist, s.red + 4);
+        const uint32_t kc = pk_select_rank_each(hi32_each, ftk, hist, tmp, rao);
         uint32_t nle = 0;
         {
             if (tid == 0) s.red[3] = 0;
             __syncthreads();
             uint32_t c = 0;
-            for (uint32_t i = tid; i < n; i += blockDim.x) c += hi32(i) <= kc ? 1u : 0u;
+            for (uint32_t i = tid; i < n; i += NT) c += (uint32_t)(B[i] >> 32) <= kc ? 1u : 0u;
             atomicAdd(&s.red[3], c);
             __syncthreads();
             nle = s.red[3];
             __syncthreads();
         }
-        K = nle == ftk ? (((uint64_t)kc << 32) | 0xffffffffull) : pk_select_rank64(B, n, ftk, hist, s.red + 4);
-        n = pk_keep_le(B, n, K, s.red + 4);
+        K = nle == ftk ? (((uint64_t)kc << 32) | 0xffffffffull) : pk_select_rank64(B, n, ftk, hist, tmp);
+        n = pk_keep_le(B, n, K, tmp);
     }
     for (uint32_t r = tid; r < n; r += blockDim.x) {
         a.out_t[(uint64_t)qi * ftk + r] = pk_ord2f((uint32_t)(B[r] >> 32));
@@ -1096,7 +1152,7 @@ extern "C" hipError_t pk_launch_build_walk_list(const uint32_t* q_nf, uint32_t n
 // walk's): one block per query, returning at once for queries without a
 // budget below F.  Same LDS layout as pk_walk.
 template <int FTKM>
-__global__ __launch_bounds__(WALK_THREADS) void pk_scan_budget_kernel(PkWalkArgs a) {
+__global__ __launch_bounds__(SCAN_THREADS) void pk_scan_budget_kernel(PkWalkArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
     const uint32_t ftk = a.ftk, F = a.F, nsq = a.nsq;
@@ -1106,10 +1162,10 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_scan_budget_kernel(PkWalkArgs
     unsigned char* ptr = smem;
     s.lut = (float*)ptr; ptr += sizeof(float) * nsq * PK_KS;
     s.qv = (float*)ptr; ptr += sizeof(float) * ((a.dim + 3) & ~3u);
-    s.red = (uint32_t*)ptr; ptr += sizeof(uint32_t) * 16;
-    s.c_b = (uint32_t*)ptr; ptr += sizeof(uint32_t) * WALK_CH;
-    s.c_pre = (uint32_t*)ptr; ptr += sizeof(uint32_t) * (WALK_CH + 4);
-    s.c_fd = (float*)ptr; ptr += sizeof(float) * WALK_CH;
+    s.red = (uint32_t*)ptr; ptr += sizeof(uint32_t) * SCAN_RED;
+    s.c_b = (uint32_t*)ptr; ptr += sizeof(uint32_t) * SCAN_CB;
+    s.c_pre = (uint32_t*)ptr; ptr += sizeof(uint32_t) * (SCAN_CB + 4);
+    s.c_fd = (float*)ptr; ptr += sizeof(float) * SCAN_CB;
     uint32_t* hist = (uint32_t*)ptr; ptr += sizeof(uint32_t) * 260;
     ptr = (unsigned char*)(((uintptr_t)ptr + 15) & ~(uintptr_t)15);
     uint64_t* B = (uint64_t*)ptr;
@@ -1134,8 +1190,8 @@ __global__ __launch_bounds__(WALK_THREADS) void pk_scan_budget_kernel(PkWalkArgs
 }
 
 extern "C" size_t pk_scan_smem_fixed(uint32_t dim, uint32_t nsq) {
-    size_t b = sizeof(float) * nsq * PK_KS + sizeof(float) * ((dim + 3) & ~3u) + sizeof(uint32_t) * 16 +
-               sizeof(uint32_t) * WALK_CH + sizeof(uint32_t) * (WALK_CH + 4) + sizeof(float) * WALK_CH +
+    size_t b = sizeof(float) * nsq * PK_KS + sizeof(float) * ((dim + 3) & ~3u) + sizeof(uint32_t) * SCAN_RED +
+               sizeof(uint32_t) * SCAN_CB + sizeof(uint32_t) * (SCAN_CB + 4) + sizeof(float) * SCAN_CB +
                sizeof(uint32_t) * 260;
     return (b + 15) & ~(size_t)15;
 }
@@ -1171,15 +1227,16 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
     if (args->ftk == 0 || args->ftk > WALK_FTK_MAX || args->cstride < args->scc) return hipErrorInvalidValue;
     if (args->q_nf && !args->fdist) return hipErrorInvalidValue;
     size_t smem = pk_walk_smem_bytes(args->dim, args->nsq, args->ftk, args->exact);
-    // budget scan: fill the same footprint as the walk (at least ftk + 256
-    // candidates), up to 1024 points per round
+    // budget scan: fill the same footprint as the walk, at least 80 KB (two
+    // blocks per CU) and at least ftk + SCAN_THREADS candidates, up to
+    // SCAN_PP points per thread and round
     PkWalkArgs sa = *args;
     const size_t fixed = pk_scan_smem_fixed(args->dim, args->nsq);
     size_t want = smem > 80 * 1024 ? smem : 80 * 1024;
     size_t cap = want > fixed ? (want - fixed) / 8 : 0;
-    if (cap < (size_t)args->ftk + WALK_THREADS) cap = (size_t)args->ftk + WALK_THREADS;
-    size_t rnd = (cap - args->ftk) & ~(size_t)(WALK_THREADS - 1);
-    if (rnd > 4 * WALK_THREADS) rnd = 4 * WALK_THREADS;
+    if (cap < (size_t)args->ftk + SCAN_THREADS) cap = (size_t)args->ftk + SCAN_THREADS;
+    size_t rnd = (cap - args->ftk) & ~(size_t)(SCAN_THREADS - 1);
+    if (rnd > SCAN_PP * SCAN_THREADS) rnd = SCAN_PP * SCAN_THREADS;
     sa.scan_cap = (uint32_t)cap;
     sa.scan_round = (uint32_t)rnd;
     const size_t smem_scan = fixed + 8 * cap;
@@ -1218,9 +1275,9 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
     }
     if (args->q_nf) {
         if (args->ftk <= 1024)
-            hipLaunchKernelGGL(pk_scan_budget_kernel<1024>, dim3(nq), dim3(WALK_THREADS), smem_scan, s, sa);
+            hipLaunchKernelGGL(pk_scan_budget_kernel<1024>, dim3(nq), dim3(SCAN_THREADS), smem_scan, s, sa);
         else
-            hipLaunchKernelGGL(pk_scan_budget_kernel<2048>, dim3(nq), dim3(WALK_THREADS), smem_scan, s, sa);
+            hipLaunchKernelGGL(pk_scan_budget_kernel<2048>, dim3(nq), dim3(SCAN_THREADS), smem_scan, s, sa);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (ev_mid) (void)hipEventRecord(ev_mid, s);
@@ -1270,50 +1327,12 @@ __device__ bool pk_sorted_prefix(uint64_t* keys, uint32_t n, uint32_t K1) {
     __shared__ uint32_t red[9];
     __shared__ uint64_t sb[PK_PREFIX_CAP];
     const uint32_t t = threadIdx.x, nb = blockDim.x;
-    uint32_t ka = 0xffffffffu, ko = 0u;
-    for (uint32_t i = t; i < n; i += nb) {
-        const uint32_t h = (uint32_t)(keys[i] >> 32);
-        ka &= h;
-        ko |= h;
-    }
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        ka &= __shfl_xor(ka, o, 64);
-        ko |= __shfl_xor(ko, o, 64);
-    }
-    if ((t & 63) == 0) { red[2 * (t >> 6)] = ka; red[2 * (t >> 6) + 1] = ko; }
     if (t == 0) red[8] = 0;
-    __syncthreads();
-    ka = 0xffffffffu; ko = 0u;
-    for (uint32_t w = 0; w < (nb >> 6); ++w) { ka &= red[2 * w]; ko |= red[2 * w + 1]; }
-    const uint32_t diff = ka ^ ko;
-    const int top = diff ? 32 - __clz(diff) : 0;
-    uint32_t mask = top >= 32 ? 0u : ~((1u << top) - 1u);
-    uint32_t prefix = ka & mask, want = K1;
-    for (int hi = top; hi > 0; hi -= 8) {
-        const int shift = hi > 8 ? hi - 8 : 0;
-        const uint32_t dm = (1u << (hi - shift)) - 1u;
-        hist[t] = 0;
-        __syncthreads();
-        for (uint32_t i = t; i < n; i += nb) {
-            const uint32_t h = (uint32_t)(keys[i] >> 32);
-            if ((h & mask) == prefix) atomicAdd(&hist[(h >> shift) & dm], 1u);
-        }
-        __syncthreads();
-        const uint32_t c = hist[t];
-        uint32_t tot;
-        const uint32_t before = pk_block_exscan(c, tmp, &tot);
-        if (c > 0 && before < want && want <= before + c) {
-            hist[256] = t;
-            hist[257] = want - before;
-        }
-        __syncthreads();
-        prefix |= hist[256] << shift;
-        mask |= dm << shift;
-        want = hist[257];
-        __syncthreads();
-    }
-    const uint32_t kc = prefix;
+    const uint32_t kc = pk_select_rank_each(
+        [&](auto sink) {
+            for (uint32_t i = t; i < n; i += nb) sink((uint32_t)(keys[i] >> 32));
+        },
+        K1, hist, tmp, red);
     for (uint32_t i = t; i < n; i += nb) {
         const uint64_t e = keys[i];
         if ((uint32_t)(e >> 32) <= kc) {

@@ -141,49 +141,15 @@ __device__ void pk_topk_block_reg(KeyF key, uint32_t L, uint32_t k, float* out_v
     const uint32_t t = threadIdx.x, base = t * PER;
     const uint32_t nt = base < L ? min((uint32_t)PER, L - base) : 0u;
     uint32_t kv[PER];
-    uint32_t ka = 0xffffffffu, ko = 0u;
 #pragma unroll
-    for (int j = 0; j < PER; ++j) {
-        kv[j] = (uint32_t)j < nt ? key(base + j) : 0u;
-        if ((uint32_t)j < nt) { ka &= kv[j]; ko |= kv[j]; }
-    }
-    // bits shared by every key of the row
+    for (int j = 0; j < PER; ++j) kv[j] = (uint32_t)j < nt ? key(base + j) : 0u;
+    const uint32_t kc = pk_select_rank_each(
+        [&](auto sink) {
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        ka &= __shfl_xor(ka, o, 64);
-        ko |= __shfl_xor(ko, o, 64);
-    }
-    if ((t & 63) == 0) { red[2 * (t >> 6)] = ka; red[2 * (t >> 6) + 1] = ko; }
-    __syncthreads();
-    ka = 0xffffffffu; ko = 0u;
-    for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) { ka &= red[2 * w]; ko |= red[2 * w + 1]; }
-    const uint32_t diff = ka ^ ko;
-    const int top = diff ? 32 - __clz(diff) : 0;  // bits [top-1, 0] vary
-    uint32_t mask = top >= 32 ? 0u : ~((1u << top) - 1u);
-    uint32_t prefix = ka & mask, want = k;
-    for (int hi = top; hi > 0; hi -= 8) {
-        const int shift = hi > 8 ? hi - 8 : 0;
-        const uint32_t dm = (1u << (hi - shift)) - 1u;
-        hist[t] = 0;
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < PER; ++j)
-            if ((uint32_t)j < nt && (kv[j] & mask) == prefix) atomicAdd(&hist[(kv[j] >> shift) & dm], 1u);
-        __syncthreads();
-        const uint32_t c = hist[t];
-        uint32_t tot;
-        const uint32_t before = pk_block_exscan(c, tmp, &tot);
-        if (c > 0 && before < want && want <= before + c) {
-            hist[256] = t;
-            hist[257] = want - before;
-        }
-        __syncthreads();
-        prefix |= hist[256] << shift;
-        mask |= dm << shift;
-        want = hist[257];
-        __syncthreads();
-    }
-    const uint32_t kc = prefix;
+            for (int j = 0; j < PER; ++j)
+                if ((uint32_t)j < nt) sink(kv[j]);
+        },
+        k, hist, tmp, red);
     uint32_t cl = 0, ce = 0;
 #pragma unroll
     for (int j = 0; j < PER; ++j) {

@@ -162,6 +162,44 @@ def test_search_batch_per_query_bitexact(ctx, which, replay, idx_l2, idx_cos):
         assert np.all(Ig[k, m:] == 0xffffffff) and np.all(_bits(Dg[k, m:]) == 0x7f7f7f7f), f"query {k}"
 
 
+@pytest.fixture(scope="module")
+def idx_wide():
+    # F above 2048 (cells taken in two segments), nsq not a multiple of 16,
+    # lists of ~12k points: budgets of hundreds of cells score thousands of points
+    x = D.unit_norm_mixture(24000, 32, 400, seed=40, means_seed=41)
+    q = D.unit_norm_mixture(16, 32, 400, seed=42, means_seed=41)
+    ix = CI.build_index(x, 2, 2100, 8, whether_norm=True, train_n=24000)
+    return ix, q
+
+
+def test_budget_scan_wide_bitexact(ctx, idx_wide):
+    # budget scan over several batches of selected cells and candidate-buffer
+    # cuts, against the oracle's filter sets and results
+    ix, q = idx_wide
+    gi = _gpu_index(ctx, ix)
+    oi = CI.oracle_index(ix)
+    n, F = q.shape[0], ix["fine"].shape[0]
+    assert F > 2048
+    nf = np.array([1, 7, 300, 513, 600, 1100, 1500, F - 1, 0, F, 2000, 900, 64, 2048, 2049, 1024], np.uint32)[:n]
+    scc = np.array([1, 2] * (n // 2), np.uint32)
+    topk = np.full(n, 10, np.uint32)
+    for ftk in (300, 1000):
+        Dg, Ig, Rg = gi.search_batch(q, 10, 2, ftk, topk=topk, scc=scc, fine_budget=nf)
+        ft, fm, fn = gi.last_filter(n, ftk)
+        Do, Io, Ro = oi.search_batch_pq(q, 10, 2, ftk, topk=topk, scc_arr=scc, fine_budget=nf)
+        assert np.array_equal(Rg, Ro)
+        for k in range(n):
+            if 0 < nf[k] < F:
+                o = oi.search_one(q[k], 10, int(scc[k]), ftk, max_topk=10, fine_budget=int(nf[k]))
+                assert fn[k] == o["filt_n"], (ftk, k)
+                gv, gid = _sorted_pairs(ft[k, :fn[k]], fm[k, :fn[k]])
+                assert np.array_equal(_bits(gv), _bits(o["filt_val"][:fn[k]])), (ftk, k)
+                assert np.array_equal(gid, o["filt_mem"][:fn[k]]), (ftk, k)
+            m = int(Ro[k])
+            assert np.array_equal(Ig[k, :m], Io[k, :m]), (ftk, k)
+            assert np.array_equal(_bits(Dg[k, :m]), _bits(Do[k, :m])), (ftk, k)
+
+
 def test_search_batch_device_arrays(ctx, idx_cos):
     import torch
     ix, q = idx_cos
