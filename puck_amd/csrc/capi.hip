@@ -2,6 +2,7 @@
 // Host-side orchestration of the gfx950 kernels: context, index load, batched
 // search, build-time assignment and encoding.
 #include "common.h"
+#include "launch_args.h"
 #include "../../include/puck_gpu.h"
 
 #include <algorithm>
@@ -12,8 +13,6 @@
 #include <vector>
 
 // ---- kernel launchers (defined in gemm/sort/search/build.hip)
-struct PkWalkArgs;
-struct PkRerankArgs;
 extern "C" {
 hipError_t pk_launch_gemm_exact(const float*, const float*, float*, uint32_t, uint32_t, uint32_t, uint64_t, int,
                                 const float*, hipStream_t);
@@ -77,63 +76,6 @@ hipError_t pk_launch_gather_f32(const float*, uint32_t, float*, const uint32_t*,
 
 enum { PK_EPI_RAW = 0, PK_EPI_SUB_FROM_BIAS = 1, PK_EPI_NEG = 2, PK_EPI_ADD_BIAS = 3 };
 
-// Mirrors the struct of the same name in search.hip.
-struct PkWalkArgs {
-    const float* q;
-    const float* qn;
-    const int32_t* qstatus;
-    const float* csel_d;
-    const uint32_t* csel_id;
-    const float* fsort_d;
-    const uint32_t* fsort_id;
-    const float* codebook;
-    const float* stationary;
-    const uint32_t* cell_start;
-    const float* mdo;
-    const float* rec_off;
-    const uint8_t* rec_codes;
-    uint32_t dim, C, F, nsq, lsq, scc, ftk;
-    double rr;
-    float* out_t;
-    uint32_t* out_id;
-    uint32_t* out_n;
-    uint32_t* stats;
-    const uint32_t* qlist;
-    uint8_t* flags;
-    int exact;
-    uint32_t* tie_ids;
-    uint8_t* tie_n;
-    const uint32_t* q_scc;
-    const uint32_t* q_nf;
-    uint32_t cstride;
-    const float* fdist;
-    uint32_t scan_cap;
-    uint32_t scan_round;
-    const uint32_t* wlist;
-    const uint32_t* wcount;
-};
-struct PkRerankArgs {
-    const float* q;
-    const int32_t* qstatus;
-    const float* filt_t;
-    const uint32_t* filt_id;
-    const uint32_t* filt_n;
-    const float* raw;
-    const uint32_t* mem2local;
-    uint32_t dim, ftk, topk;
-    float* out_d;
-    uint32_t* out_id;
-    uint32_t* out_n;
-    const uint32_t* qlist;
-    uint8_t* flags;
-    int exact;
-    const uint32_t* tie_ids;
-    const uint8_t* tie_n;
-    const uint32_t* q_topk;
-    uint32_t out_stride;
-    const uint32_t* q_nf;
-    uint32_t F;
-};
 
 namespace {
 
@@ -617,7 +559,7 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     // fine order for the walk (rows of budget-scan queries are skipped)
     PK_CHECK(pk_launch_sort_rows(b.fdist, m, F, F, F, b.fsd, b.fsi, nullptr, 0, nf, s));
     if (ev) PK_CHECK(hipEventRecord(ev[2], s));
-    PkWalkArgs wa;
+    PkWalkArgs wa{};
     wa.q = b.q; wa.qn = b.qn; wa.qstatus = b.qst; wa.csel_d = b.csd; wa.csel_id = b.csi; wa.fsort_d = b.fsd;
     wa.fsort_id = b.fsi; wa.codebook = ix->codebook; wa.stationary = ix->stationary;
     wa.cell_start = ix->cell_start; wa.mdo = ix->mdo; wa.rec_off = ix->rec_off; wa.rec_codes = ix->rec_codes;
@@ -654,10 +596,14 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
 // reference heaps.  flags: 1 coarse-distance tie in the kept range, 2
 // filter-boundary tie at the final top, 4 exact-distance tie in the top-k, 8 a
 // decision on a fine-distance tie group.
-//   stage 1: queries flagged only 1 get the reference's coarse order and run
-//            the fast walk and rerank again (which flag any remaining tie);
+//   stage 1: queries flagged only 1 and/or 8 get the reference's coarse order
+//            (1) and fine order (8) and run the fast walk and rerank again;
+//            the walk visits the cells of a fine row given in the reference's
+//            order in that order, so fine ties need no flag there, and any
+//            other remaining tie is flagged anew;
 //   stage 2: queries still flagged run the walk and rerank with the reference
-//            heaps (the reference's fine order first when flagged 8).
+//            heaps (the reference's fine order first when flagged 8; orders
+//            computed in stage 1 are already in place).
 // Synchronises the stream.  Returns the number of queries replayed, or < 0.
 int replay_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, const PassBufs& b, uint32_t m,
                 uint8_t* hflags, int replay_all, PkWalkArgs wa, PkRerankArgs ra, std::vector<uint32_t>& hlist,
@@ -667,7 +613,7 @@ int replay_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, c
     hlist.clear();
     for (uint32_t i = 0; i < m; ++i) {
         if (replay_all) hlist.push_back(i);
-        else if (hflags[i] == 1u) stage1.push_back(i);
+        else if ((hflags[i] & 15u) != 0u && (hflags[i] & 6u) == 0u) stage1.push_back(i);
         else if (hflags[i] & 15u) hlist.push_back(i);
     }
     const uint32_t n1 = (uint32_t)stage1.size();
@@ -677,6 +623,7 @@ int replay_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, c
                                               b.flags, 0, b.csd, b.csi, b.fsd, b.fsi, s));
         PkWalkArgs w1 = wa;
         w1.qlist = b.qlist;
+        w1.fine_given = 1;
         PK_CHECK(pk_launch_walk(&w1, n1, s, nullptr, nullptr, nullptr, nullptr));
         PkRerankArgs r1 = ra;
         r1.qlist = b.qlist;
@@ -860,6 +807,9 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
         PK_CHECK(hipMemcpyAsync(hflags.data(), b.flags, m, hipMemcpyDeviceToHost, s));
         PK_CHECK(hipStreamSynchronize(s));
         if (cfg.precision) std::fill(hflags.begin(), hflags.begin() + m, (uint8_t)0);  // no replay in these forms
+        for (uint32_t i = 0; i < m; ++i)  // first-pass replay causes
+            for (int bb = 0; bb < 8; ++bb)
+                if (hflags[i] & (1u << bb)) ctx->counters[5 + bb]++;
         const int nrep = replay_pass(ctx, ix, cfg, b, m, hflags.data(), cfg.precision ? 0 : sp->replay_all, wa, ra,
                                      hlist, s);
         if (nrep < 0) return nrep;
@@ -878,9 +828,6 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
         for (uint32_t i = 0; i < m; ++i)
             for (int k = 0; k < 4; ++k) ctx->counters[k] += hstats[4 * (size_t)i + k];
         ctx->counters[4] += (uint64_t)nrep;
-        for (uint32_t i = 0; i < m; ++i)
-            for (int bb = 0; bb < 8; ++bb)
-                if (hflags[i] & (1u << bb)) ctx->counters[5 + bb]++;
     }
     if (!q_nf && !(pu && pu->probe)) ctx->scan_ms = -1.f;  // no budget scan was launched
     if (!device_ptrs) {

@@ -29,46 +29,13 @@
 // merging a sure segment at once gives the same set as the reference's
 // one-by-one heap updates.  Exact ties are broken by arrival order.
 #include "common.h"
+#include "launch_args.h"
 
 #define WALK_THREADS 256
 #define WALK_CH 256      // cells per chunk (one per thread)
 #define WALK_PMAX 512    // points buffered per chunk
 #define WALK_FTK_MAX 2048   // filter_topk limit (instances: <= 1024, <= 2048)
 
-struct PkWalkArgs {
-    const float* q;          // nq x dim (normalised when whether_norm)
-    const float* qn;         // nq      |q|^2
-    const int32_t* qstatus;  // nq      0 = searchable
-    const float* csel_d;     // nq x scc
-    const uint32_t* csel_id; // nq x scc
-    const float* fsort_d;    // nq x F
-    const uint32_t* fsort_id;
-    const float* codebook;   // nsq x 256 x lsq
-    const float* stationary; // C x F (empty cell: sqrt(FLT_MAX))
-    const uint32_t* cell_start; // C*F + 1
-    const float* mdo;        // C
-    const float* rec_off;    // N
-    const uint8_t* rec_codes;// N x nsq
-    uint32_t dim, C, F, nsq, lsq, scc, ftk;
-    double rr;
-    float* out_t;            // nq x ftk
-    uint32_t* out_id;        // nq x ftk
-    uint32_t* out_n;         // nq
-    uint32_t* stats;         // nq x 4 or null
-    const uint32_t* qlist;   // block -> query index (null: identity)
-    uint8_t* flags;          // nq: bit 3 set when a decision touched a fine-distance tie group
-    int exact;               // replay the reference MaxHeap (insertion order and ties)
-    uint32_t* tie_ids;       // nq x PK_TIE_MAX: memory ids dropped at a tie with the final top
-    uint8_t* tie_n;          // nq: count, PK_TIE_OVERFLOW when the list overflowed
-    const uint32_t* q_scc;   // nq or null: per-query coarse lists walked (clamped to scc)
-    const uint32_t* q_nf;    // nq or null: per-query fine budget (0 or >= F: all cells)
-    uint32_t cstride;        // csel row stride (>= scc)
-    const float* fdist;      // nq x F unsorted fine distances -<f,q> (budget scan)
-    uint32_t scan_cap;       // budget scan: candidate buffer entries (set by the launcher)
-    uint32_t scan_round;     // budget scan: points scored per round (multiple of WALK_THREADS)
-    const uint32_t* wlist;   // with wcount: the queries that take the walk (pk_build_walk_list)
-    const uint32_t* wcount;
-};
 
 #define PK_TIE_MAX 16
 #define PK_TIE_OVERFLOW 255
@@ -168,53 +135,109 @@ struct PkWalkSmem {
     uint32_t* td_id;
 };
 
-// The reference MaxHeap (max_heap.cpp:52-100) on an LDS array run by one
-// thread.  Entries are (value bits, tag) pairs; slot i holds heap index i
-// (slot 0 unused) so that the two children of index i, 2i and 2i+1, are one
-// aligned 16-byte read.
+// The reference MaxHeap (max_heap.cpp:52-100) on an LDS array.  Entries are
+// (value bits, tag) pairs; slot i holds heap index i (slot 0 unused).
 __device__ __forceinline__ float pk_hv(uint2 e) { return __uint_as_float(e.x); }
 
-__device__ void pk_heap_insert(uint2* H, uint32_t heap_size, uint32_t father, float v, uint32_t t) {
+// The heap is run by one full wave (64 active lanes, uniform arguments).  A
+// sift-down (MaxHeap::insert) reads the 30 nodes of the four levels
+// below the current node in one LDS round trip (one node per lane) and then
+// follows the reference's comparisons through them with readlane and scalar
+// branches; lane 0 writes the moved entries.
+__device__ void pk_heap_insert_wave(uint2* H, uint32_t heap_size, uint32_t father, float v, uint32_t t) {
+    const uint32_t lane = threadIdx.x & 63u;
+    // lane -> level 1..4 below the subtree root (lanes 0-1, 2-5, 6-13, 14-29)
+    const uint32_t lv = lane < 2u ? 1u : lane < 6u ? 2u : lane < 14u ? 3u : lane < 30u ? 4u : 0u;
+    const uint32_t off = lane - ((1u << lv) - 2u);
+    heap_size = __builtin_amdgcn_readfirstlane(heap_size);
+    father = __builtin_amdgcn_readfirstlane(father);
+    const uint32_t vb = __builtin_amdgcn_readfirstlane(__float_as_uint(v));
+    t = __builtin_amdgcn_readfirstlane(t);
+    v = __uint_as_float(vb);
     for (;;) {
-        uint32_t l = father << 1;
-        if (l > heap_size) break;
-        uint32_t r = l + 1;
-        uint2 el, er;
-        if (r <= heap_size) {
-            uint4 ch = *(const uint4*)(H + l);
-            el = make_uint2(ch.x, ch.y);
-            er = make_uint2(ch.z, ch.w);
-        } else {
-            el = H[l];
-            er = el;
+        father = __builtin_amdgcn_readfirstlane(father);
+        uint32_t nv = 0u, nt = 0u;
+        if (lv) {
+            const uint32_t idx = (father << lv) + off;
+            if (idx <= heap_size) {
+                const uint2 e = H[idx];
+                nv = e.x;
+                nt = e.y;
+            }
         }
-        if (r > heap_size || pk_hv(el) > pk_hv(er)) {
-            if (pk_hv(el) < v) break;
-            H[father] = el;
-            father = l;
-        } else {
-            if (pk_hv(er) < v) break;
-            H[father] = er;
-            father = r;
+        uint32_t cur = father;
+        bool stop = false;
+#pragma unroll
+        for (uint32_t d = 1; d <= 4; ++d) {
+            const uint32_t l = cur << 1;
+            if (l > heap_size) { stop = true; break; }
+            const uint32_t ll = ((1u << d) - 2u) + (l - (father << d));  // lane of node l
+            uint32_t cv = __builtin_amdgcn_readlane(nv, ll), ct = __builtin_amdgcn_readlane(nt, ll), c = l;
+            if (l + 1 <= heap_size) {
+                const uint32_t rv = __builtin_amdgcn_readlane(nv, ll + 1);
+                if (!(__uint_as_float(cv) > __uint_as_float(rv))) {
+                    cv = rv;
+                    ct = __builtin_amdgcn_readlane(nt, ll + 1);
+                    c = l + 1;
+                }
+            }
+            if (__uint_as_float(cv) < v) { stop = true; break; }
+            if (lane == 0) H[cur] = make_uint2(cv, ct);
+            cur = __builtin_amdgcn_readfirstlane(c);
         }
+        father = cur;
+        if (stop) break;
     }
-    H[father] = make_uint2(__float_as_uint(v), t);
+    if (lane == 0) H[father] = make_uint2(vb, t);
 }
 
-__device__ void pk_heap_update(uint2* H, uint32_t size, uint32_t* default_cnt, float v, uint32_t t) {
+// An LDS word read by every lane of the wave, as a wave-uniform value (LDS
+// loads are otherwise taken as per-lane, which turns the heap's scalar
+// control flow into exec-mask loops).
+__device__ __forceinline__ uint32_t pk_uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float pk_uni_top(const uint2* H) { return __uint_as_float(pk_uni(H[1].x)); }
+
+__device__ void pk_heap_update_wave(uint2* H, uint32_t size, uint32_t* default_cnt, float v, uint32_t t) {
+    if (v >= pk_uni_top(H)) return;
     uint32_t father = 1;
-    if (v >= pk_hv(H[1])) return;
     if (*default_cnt > 0) father = (*default_cnt)--;
-    pk_heap_insert(H, size, father, v, t);
+    pk_heap_insert_wave(H, size, father, v, t);
 }
 
-__device__ void pk_heap_reorder(uint2* H, uint32_t size) {
+__device__ void pk_heap_reorder_wave(uint2* H, uint32_t size) {
     for (uint32_t i = 0; i < size; i++) {
-        uint2 top = H[1];
-        uint32_t hs = size - i;
-        uint2 last = H[hs];
-        pk_heap_insert(H, hs - 1, 1, pk_hv(last), last.y);
-        H[hs] = top;
+        const uint2 top = H[1];
+        const uint32_t hs = size - i;
+        const uint2 last = H[hs];
+        pk_heap_insert_wave(H, hs - 1, 1, __uint_as_float(pk_uni(last.x)), pk_uni(last.y));
+        if ((threadIdx.x & 63u) == 0) H[hs] = top;
+    }
+}
+
+// Offers get(0..n-1) (value, tag) to the heap in order, as a loop of
+// max_heap_update calls would; run by one full wave.  Each group of 64 is
+// first tested against the heap top at the start of the group (the top only
+// decreases, so a value that fails then fails at its turn too): LT offers a
+// value only when it is below the top (callers that test `<` first),
+// otherwise unless it is at or above it (max_heap_update's own test).
+template <bool LT, typename GetF>
+__device__ void pk_heap_offer_wave(uint2* H, uint32_t size, uint32_t* default_cnt, uint32_t n, GetF get) {
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t b = 0; b < n; b += 64) {
+        const uint32_t i = b + lane;
+        float v = 0.f;
+        uint32_t tg = 0;
+        if (i < n) get(i, v, tg);
+        const float top = pk_hv(H[1]);
+        uint64_t m = __ballot(i < n && (LT ? v < top : !(v >= top)));
+        while (m) {
+            const uint32_t j = (uint32_t)__builtin_ctzll(m);
+            m &= m - 1;
+            const float vj = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), j));
+            const uint32_t tj = __builtin_amdgcn_readlane(tg, j);
+            if (LT && !(vj < pk_uni_top(H))) continue;
+            pk_heap_update_wave(H, size, default_cnt, vj, tj);
+        }
     }
 }
 
@@ -485,13 +508,14 @@ __device__ uint32_t pk_walk_merge(const PkWalkArgs& a, PkWalkSmem& s, uint32_t n
             cnt += tot;
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t dc = s.red[13];
-            for (uint32_t r = 0; r < cnt; ++r) {
-                uint32_t p = s.X_pos[r];
-                pk_heap_update(s.H, ftk, &dc, s.P_t[p], s.P_id[p]);
-            }
-            s.red[13] = dc;
+        if (threadIdx.x < 64) {
+            uint32_t dc = pk_uni(s.red[13]);
+            pk_heap_offer_wave<false>(s.H, ftk, &dc, cnt, [&](uint32_t r, float& v, uint32_t& tg) {
+                const uint32_t p = s.X_pos[r];
+                v = s.P_t[p];
+                tg = s.P_id[p];
+            });
+            if (threadIdx.x == 0) s.red[13] = dc;
         }
         __syncthreads();
     }
@@ -810,6 +834,7 @@ __device__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
     s.td_id = (uint32_t*)ptr; ptr += sizeof(uint32_t) * PK_TIE_MAX;
 
     if (a.q_nf && a.q_nf[qi] != 0u && a.q_nf[qi] < F) return;  // pk_scan_budget_kernel's query
+    const bool fine_exact = a.fine_given && a.flags && (a.flags[qi] & 8u);
     uint32_t st_seg = 0, st_unc = 0, st_cells = 0, st_pts = 0;
     bool tie_dec = false;
     uint32_t tie_sub = 0;
@@ -1071,7 +1096,7 @@ __device__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
     if (a.exact) {
         // the filter heap as rank_topN_points reads it: heap order when full,
         // heap-sorted ascending when not full (puck_index.cpp:552-554)
-        if (tid == 0 && ns < ftk) pk_heap_reorder(s.H, ftk);
+        if (tid < 64 && ns < ftk) pk_heap_reorder_wave(s.H, ftk);
         __syncthreads();
         for (uint32_t r = tid; r < ns; r += blockDim.x) {
             a.out_t[(uint64_t)qi * ftk + r] = pk_hv(s.H[r + 1]);
@@ -1085,7 +1110,10 @@ __device__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
     }
     if (tid == 0) {
         a.out_n[qi] = ns;
-        if (a.flags && tie_dec) a.flags[qi] |= (uint8_t)(8u | tie_sub);
+        if (a.flags) {
+            if (a.fine_given) a.flags[qi] = (tie_dec && !fine_exact) ? (uint8_t)(8u | tie_sub) : (uint8_t)0;
+            else if (tie_dec) a.flags[qi] |= (uint8_t)(8u | tie_sub);
+        }
         if (a.tie_n) {
             uint32_t tn = 0;
             if (ns >= ftk) {
@@ -1287,29 +1315,6 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
 }
 
 // ---------------------------------------------------------------------------
-struct PkRerankArgs {
-    const float* q;          // nq x dim
-    const int32_t* qstatus;
-    const float* filt_t;     // nq x ftk
-    const uint32_t* filt_id; // memory ids
-    const uint32_t* filt_n;
-    const float* raw;        // N x dim, memory order
-    const uint32_t* mem2local;
-    uint32_t dim, ftk, topk;
-    float* out_d;            // nq x topk
-    uint32_t* out_id;
-    uint32_t* out_n;
-    const uint32_t* qlist;   // block -> query index (null: identity)
-    uint8_t* flags;          // bit 2: exact distances tie inside the top-k (+1); bit 1: a
-                             // filter-boundary tie group reaches the top-k
-    int exact;               // replay the reference result heap over the filter heap order
-    const uint32_t* tie_ids; // from the walk (may be null)
-    const uint8_t* tie_n;
-    const uint32_t* q_topk;  // nq or null: per-query top-k (1..topk; other values give no result)
-    uint32_t out_stride;     // row stride of out_d / out_id (>= topk)
-    const uint32_t* q_nf;    // nq or null: fine budgets; a query with 0 < budget < F came from the
-    uint32_t F;              // budget scan and ranks by (distance, local id), without replay
-};
 
 #define RERANK_THREADS 256
 
@@ -1456,16 +1461,15 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
     }
     __syncthreads();
     if (exact) {
-        if (threadIdx.x == 0) {
-            for (uint32_t r = 0; r <= topk; ++r) RH[r] = make_uint2(0x7f7f7f7fu, 0xffffffffu);
+        if (threadIdx.x < 64) {
+            for (uint32_t r = threadIdx.x; r <= topk; r += 64) RH[r] = make_uint2(0x7f7f7f7fu, 0xffffffffu);
             uint32_t dc = topk;
-            for (uint32_t r = 0; r < n; ++r) {
-                float d = dist[r];
-                if (d < pk_hv(RH[1]))
-                    pk_heap_update(RH, topk, &dc, d, a.mem2local[a.filt_id[(uint64_t)qi * ftk + r]]);
-            }
-            pk_heap_reorder(RH, topk);
-            a.out_n[qi] = topk - dc;
+            pk_heap_offer_wave<true>(RH, topk, &dc, n, [&](uint32_t r, float& v, uint32_t& tg) {
+                v = dist[r];
+                tg = a.mem2local[a.filt_id[(uint64_t)qi * ftk + r]];
+            });
+            pk_heap_reorder_wave(RH, topk);
+            if (threadIdx.x == 0) a.out_n[qi] = topk - dc;
         }
         __syncthreads();
         for (uint32_t r = threadIdx.x; r < ostr; r += blockDim.x) {
@@ -1534,7 +1538,7 @@ extern "C" hipError_t pk_launch_rerank(const PkRerankArgs* args, uint32_t nq, hi
 // ---------------------------------------------------------------------------
 // Exact center order for flagged queries: the coarse top-scc MaxHeap
 // (hierarchical_cluster_index.cpp:709-726) and the fine full heap sort
-// (puck_index.cpp:497-504), replayed by one thread on LDS copies of the rows.
+// (puck_index.cpp:497-504), replayed by one wave on LDS copies of the rows.
 __global__ __launch_bounds__(256) void pk_exact_center_order(const float* __restrict__ cdist, uint32_t C,
                                                              const float* __restrict__ fdist, uint32_t F,
                                                              uint32_t scs, uint32_t scap,
@@ -1560,11 +1564,13 @@ __global__ __launch_bounds__(256) void pk_exact_center_order(const float* __rest
         for (uint32_t i = threadIdx.x; i < C; i += blockDim.x) row[i] = cdist[(uint64_t)qi * C + i];
         if (scc > 0) pk_heap_init(H, scc);
         __syncthreads();
-        if (threadIdx.x == 0 && scc > 0) {
+        if (threadIdx.x < 64 && scc > 0) {
             uint32_t dc = scc;
-            for (uint32_t c = 0; c < C; ++c)
-                if (row[c] < pk_hv(H[1])) pk_heap_update(H, scc, &dc, row[c], c);
-            pk_heap_reorder(H, scc);
+            pk_heap_offer_wave<true>(H, scc, &dc, C, [&](uint32_t c, float& v, uint32_t& tg) {
+                v = row[c];
+                tg = c;
+            });
+            pk_heap_reorder_wave(H, scc);
         }
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < scc; i += blockDim.x) {
@@ -1578,10 +1584,13 @@ __global__ __launch_bounds__(256) void pk_exact_center_order(const float* __rest
     for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) row[i] = fdist[(uint64_t)qi * F + i];
     pk_heap_init(H, F);
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {
         uint32_t dc = F;
-        for (uint32_t k = 0; k < F; ++k) pk_heap_update(H, F, &dc, row[k], k);
-        pk_heap_reorder(H, F);
+        pk_heap_offer_wave<false>(H, F, &dc, F, [&](uint32_t k, float& v, uint32_t& tg) {
+            v = row[k];
+            tg = k;
+        });
+        pk_heap_reorder_wave(H, F);
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) {

@@ -70,6 +70,20 @@ def build_index(x, C, F, nsq, whether_norm=False, seed=0, train_n=None):
                 raw=x, whether_norm=whether_norm, nsq=nsq)
 
 
+def build_index_with_centers(x, coarse, fine, nsq, whether_norm=False, seed=0):
+    """Index over given coarse / fine centers (codebook trained on the residuals)."""
+    x = np.asarray(x, np.float32)
+    coarse = np.ascontiguousarray(coarse, np.float32)
+    fine = np.ascontiguousarray(fine, np.float32)
+    F = fine.shape[0]
+    cell, _ = O.assign_cells(coarse, fine, x)
+    res = x - coarse[cell // F] - fine[cell % F]
+    codebook = train_codebook(res, nsq, seed=seed + 3)
+    records = O.encode_filter(coarse, fine, codebook, nsq, x, cell)
+    return dict(coarse=coarse, fine=fine, codebook=codebook, cell_assign=cell, records=records,
+                raw=x, whether_norm=whether_norm, nsq=nsq)
+
+
 def oracle_index(ix):
     return O.OracleIndex(ix["coarse"], ix["fine"], ix["codebook"], ix["cell_assign"],
                          ix["records"], ix["raw"], ix["whether_norm"])

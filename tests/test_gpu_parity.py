@@ -200,6 +200,45 @@ def test_budget_scan_wide_bitexact(ctx, idx_wide):
             assert np.array_equal(_bits(Dg[k, :m]), _bits(Do[k, :m])), (ftk, k)
 
 
+@pytest.fixture(scope="module")
+def idx_ties():
+    # Integer data and centers, all distances exact integers.  Fine centers come
+    # in pairs that differ by swapping coordinates 0 and 1 (coarse centers
+    # too), and every query has q[0] == q[1]: both members of a pair are at
+    # exactly the same distance from every query, with both cells populated.
+    rng = np.random.default_rng(60)
+    d = 16
+    g = rng.integers(-6, 7, (2, d))
+    g[:, 1] = g[:, 0] + rng.integers(3, 9, 2)
+    coarse = np.concatenate([g, g[:, [1, 0] + list(range(2, d))]]) * 4 + 40
+    h = rng.integers(-3, 4, (8, d))
+    h[:, 1] = h[:, 0] + rng.integers(1, 4, 8)
+    fine = np.concatenate([h, h[:, [1, 0] + list(range(2, d))]])
+    n = 6000
+    x = coarse[rng.integers(0, 4, n)] + fine[rng.integers(0, 16, n)] + rng.integers(-1, 2, (n, d))
+    q = coarse[rng.integers(0, 4, 48)] + rng.integers(-4, 5, (48, d))
+    q[:, 1] = q[:, 0]
+    ix = CI.build_index_with_centers(x.astype(np.float32), coarse, fine, 8)
+    return ix, q.astype(np.float32)
+
+
+@pytest.mark.parametrize("scc,ftk,topk", [(1, 50, 10), (2, 200, 10), (4, 100, 20)])
+def test_search_tied_centers_bitexact(ctx, idx_ties, scc, ftk, topk):
+    # coarse and fine distance ties: replays (reference coarse / fine heap
+    # orders, then the walk) must give the reference's result
+    ix, q = idx_ties
+    gi = _gpu_index(ctx, ix)
+    oi = CI.oracle_index(ix)
+    Dg, Ig, Rg = gi.search(q, topk, scc, ftk)
+    _, cnt = gi.last_stage_ms()
+    assert cnt[4] > 0 and cnt[8] > 0  # replays, some for a fine-tie decision
+    for k in range(q.shape[0]):
+        o = oi.search_one(q[k], topk, scc, ftk)
+        assert Rg[k] == o["n"], k
+        assert np.array_equal(Ig[k, :o["n"]], o["ids"][:o["n"]]), k
+        assert np.array_equal(_bits(Dg[k, :o["n"]]), _bits(o["dist"][:o["n"]])), k
+
+
 def test_search_batch_device_arrays(ctx, idx_cos):
     import torch
     ix, q = idx_cos
