@@ -196,12 +196,12 @@ extern "C" hipError_t pk_launch_assign_cells(const float* ipc, const float* ipf,
     uint32_t C2 = 64;
     while (C2 < C) C2 <<= 1;
     size_t smem = sizeof(uint64_t) * 4 * C2;
-    if (smem > 160 * 1024) return hipErrorInvalidValue;
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute((const void*)pk_assign_cells, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr = true;
+    static size_t allowed = 0;
+    if (!allowed) {
+        const hipError_t e = pk_allow_lds((const void*)pk_assign_cells, 160 * 1024, &allowed);
+        if (e != hipSuccess) return e;
     }
+    if (smem > allowed) return hipErrorInvalidValue;
     hipLaunchKernelGGL(pk_assign_cells, dim3((n + 3) / 4), dim3(256), smem, s, ipc, ipf, xn, cn, fbound, st, n, C, F,
                        C2, fb_sorted, order, cell_out, dist_out);
     return hipGetLastError();

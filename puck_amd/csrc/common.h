@@ -8,6 +8,19 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Lets kernel f take up to `bytes` of dynamic LDS, less its own static LDS
+// (a CU holds 160 KB); *allowed receives the dynamic bytes allowed.
+static inline hipError_t pk_allow_lds(const void* f, size_t bytes, size_t* allowed) {
+    hipFuncAttributes at;
+    hipError_t e = hipFuncGetAttributes(&at, f);
+    if (e != hipSuccess) return e;
+    const size_t cap = 160 * 1024 - at.sharedSizeBytes;
+    if (bytes > cap) bytes = cap;
+    e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess && allowed) *allowed = bytes;
+    return e;
+}
+
 #define PK_KS 256
 
 // 0x7f7f7f7f: the value MaxHeap's memset(0x7f) leaves in an empty slot.

@@ -140,51 +140,72 @@ struct PkWalkSmem {
 __device__ __forceinline__ float pk_hv(uint2 e) { return __uint_as_float(e.x); }
 
 // The heap is run by one full wave (64 active lanes, uniform arguments).  A
-// sift-down (MaxHeap::insert) reads the 30 nodes of the four levels
-// below the current node in one LDS round trip (one node per lane) and then
-// follows the reference's comparisons through them with readlane and scalar
-// branches; lane 0 writes the moved entries.
+// sift-down (MaxHeap::insert) takes the five levels below the current node
+// in one step: lane i holds node i of that 62-node subtree with its two
+// children, three ballots give, per node, whether it has a left child,
+// whether the reference's rule takes its right child (right present and not
+// left > right) and whether it is below the value sifted down; the path and
+// the stop node are then found with scalar bit operations, and the entries
+// on the path move up one level in one masked store.
+__device__ __forceinline__ bool pk_bit(uint64_t m, uint32_t i) { return (m >> i) & 1ull; }
+
 __device__ void pk_heap_insert_wave(uint2* H, uint32_t heap_size, uint32_t father, float v, uint32_t t) {
     const uint32_t lane = threadIdx.x & 63u;
-    // lane -> level 1..4 below the subtree root (lanes 0-1, 2-5, 6-13, 14-29)
-    const uint32_t lv = lane < 2u ? 1u : lane < 6u ? 2u : lane < 14u ? 3u : lane < 30u ? 4u : 0u;
-    const uint32_t off = lane - ((1u << lv) - 2u);
+    // lane -> level 1..5 below the subtree root: lanes [2^d - 2, 2^(d+1) - 2)
+    const uint32_t lv = 31u - __clz(lane + 2u);
+    const uint32_t off = lane + 2u - (1u << lv);
+    const bool inl = lane < 62u;
     heap_size = __builtin_amdgcn_readfirstlane(heap_size);
-    father = __builtin_amdgcn_readfirstlane(father);
     const uint32_t vb = __builtin_amdgcn_readfirstlane(__float_as_uint(v));
     t = __builtin_amdgcn_readfirstlane(t);
     v = __uint_as_float(vb);
     for (;;) {
         father = __builtin_amdgcn_readfirstlane(father);
+        const uint32_t idx = inl ? (father << lv) + off : 0xffffffffu;
+        const bool has = inl && idx <= heap_size;
         uint32_t nv = 0u, nt = 0u;
-        if (lv) {
-            const uint32_t idx = (father << lv) + off;
-            if (idx <= heap_size) {
-                const uint2 e = H[idx];
-                nv = e.x;
-                nt = e.y;
-            }
-        }
-        uint32_t cur = father;
-        bool stop = false;
-#pragma unroll
-        for (uint32_t d = 1; d <= 4; ++d) {
-            const uint32_t l = cur << 1;
-            if (l > heap_size) { stop = true; break; }
-            const uint32_t ll = ((1u << d) - 2u) + (l - (father << d));  // lane of node l
-            uint32_t cv = __builtin_amdgcn_readlane(nv, ll), ct = __builtin_amdgcn_readlane(nt, ll), c = l;
-            if (l + 1 <= heap_size) {
-                const uint32_t rv = __builtin_amdgcn_readlane(nv, ll + 1);
-                if (!(__uint_as_float(cv) > __uint_as_float(rv))) {
-                    cv = rv;
-                    ct = __builtin_amdgcn_readlane(nt, ll + 1);
-                    c = l + 1;
+        float fl = 0.f, fr = 0.f;
+        bool hl = false, hr = false;
+        if (has) {
+            const uint2 e = H[idx];
+            nv = e.x;
+            nt = e.y;
+            if (lv < 5u && 2u * idx <= heap_size) {
+                hl = true;
+                hr = 2u * idx + 1u <= heap_size;
+                if (hr) {
+                    const uint4 ch = *(const uint4*)(H + 2u * idx);
+                    fl = __uint_as_float(ch.x);
+                    fr = __uint_as_float(ch.z);
                 }
             }
-            if (__uint_as_float(cv) < v) { stop = true; break; }
-            if (lane == 0) H[cur] = make_uint2(cv, ct);
-            cur = __builtin_amdgcn_readfirstlane(c);
         }
+        const uint64_t mL = __ballot(hl);
+        const uint64_t mR = __ballot(hr && !(fl > fr));
+        const uint64_t mLt = __ballot(has && __uint_as_float(nv) < v);
+        const uint64_t mHas = __ballot(has);
+        // the subtree root's own choice: children in lanes 0 and 1
+        const float c0 = __uint_as_float(__builtin_amdgcn_readlane(nv, 0));
+        const float c1 = __uint_as_float(__builtin_amdgcn_readlane(nv, 1));
+        const bool root_l = pk_bit(mHas, 0);
+        const bool root_r = pk_bit(mHas, 1) && !(c0 > c1);
+        uint32_t cur = father;
+        uint32_t cl = 0;  // lane of cur (from level 1 on)
+        uint64_t moved = 0;
+        bool stop = false;
+#pragma unroll
+        for (uint32_t d = 1; d <= 5; ++d) {
+            const bool left = d == 1 ? root_l : pk_bit(mL, cl);
+            if (!left) { stop = true; break; }
+            const bool right = d == 1 ? root_r : pk_bit(mR, cl);
+            const uint32_t o = d == 1 ? 0u : cl + 2u - (1u << (d - 1));  // offset of cur in its level
+            const uint32_t ch = (1u << d) - 2u + 2u * o + (right ? 1u : 0u);
+            if (pk_bit(mLt, ch)) { stop = true; break; }  // chosen child < v: v stays at cur
+            moved |= 1ull << ch;
+            cur = 2u * cur + (right ? 1u : 0u);
+            cl = ch;
+        }
+        if (pk_bit(moved, lane)) H[idx >> 1] = make_uint2(nv, nt);
         father = cur;
         if (stop) break;
     }
@@ -1269,16 +1290,22 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
     sa.scan_round = (uint32_t)rnd;
     const size_t smem_scan = fixed + 8 * cap;
     const uint32_t wgrid = args->wcount ? (nq < 1024 ? nq : 1024) : nq;
-    if (args->q_nf && smem_scan > 160 * 1024) return hipErrorInvalidValue;
-    if (smem > 160 * 1024) return hipErrorInvalidValue;
-    static bool attr_set = false;
-    if (!attr_set) {
+    static size_t allowed = 0;
+    if (!allowed) {
         const void* fns[] = {(const void*)pk_walk<1024>, (const void*)pk_walk<2048>,
                              (const void*)pk_walk_list<1024>, (const void*)pk_walk_list<2048>,
                              (const void*)pk_scan_budget_kernel<1024>, (const void*)pk_scan_budget_kernel<2048>};
-        for (const void* f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr_set = true;
+        size_t lo = 160 * 1024;
+        for (const void* f : fns) {
+            size_t a = 0;
+            const hipError_t e = pk_allow_lds(f, 160 * 1024, &a);
+            if (e != hipSuccess) return e;
+            lo = a < lo ? a : lo;
+        }
+        allowed = lo;
     }
+    if (args->q_nf && smem_scan > allowed) return hipErrorInvalidValue;
+    if (smem > allowed) return hipErrorInvalidValue;
     // budget scan first (its own launch, timed between ev_mid and the end),
     // then the walk for the queries without a budget
     auto launch_walk = [&](hipStream_t st) {
@@ -1581,10 +1608,47 @@ __global__ __launch_bounds__(256) void pk_exact_center_order(const float* __rest
     }
     if (!do_fine) return;
     // fine full heap sort (puck_index.cpp:497-504)
-    for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) row[i] = fdist[(uint64_t)qi * F + i];
+    int below = 1;
+    for (uint32_t i = threadIdx.x; i < F; i += blockDim.x) {
+        row[i] = fdist[(uint64_t)qi * F + i];
+        if (row[i] >= __uint_as_float(0x7f7f7f7fu)) below = 0;
+    }
     pk_heap_init(H, F);
-    __syncthreads();
-    if (threadIdx.x < 64) {
+    // Update k places row[k] at position F - k and sifts it down (Floyd's
+    // construction over positions F .. 1), since the top stays the empty
+    // sentinel until the last update and no value reaches it.  Positions of
+    // one depth sift down disjoint subtrees, so each depth runs in parallel,
+    // deepest first; leaves need no sift.
+    if (__syncthreads_and(below)) {
+        for (uint32_t p = 1 + threadIdx.x; p <= F; p += blockDim.x) H[p] = make_uint2(__float_as_uint(row[F - p]), F - p);
+        __syncthreads();
+        int D = 0;
+        while ((2u << D) <= F / 2u) ++D;  // depth of the last internal node F / 2
+        for (; D >= 0; --D) {
+            const uint32_t p_end = min((2u << D) - 1u, F / 2u);
+            for (uint32_t p = (1u << D) + threadIdx.x; p <= p_end; p += blockDim.x) {
+                const uint2 e = H[p];
+                const float v = pk_hv(e);
+                uint32_t f = p;
+                for (;;) {
+                    const uint32_t l = 2u * f;
+                    if (l > F) break;
+                    uint2 c = H[l];
+                    uint32_t ci = l;
+                    if (l + 1u <= F && !(pk_hv(c) > pk_hv(H[l + 1u]))) {
+                        c = H[l + 1u];
+                        ci = l + 1u;
+                    }
+                    if (pk_hv(c) < v) break;
+                    H[f] = c;
+                    f = ci;
+                }
+                H[f] = e;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x < 64) pk_heap_reorder_wave(H, F);
+    } else if (threadIdx.x < 64) {
         uint32_t dc = F;
         pk_heap_offer_wave<false>(H, F, &dc, F, [&](uint32_t k, float& v, uint32_t& tg) {
             v = row[k];
@@ -1608,13 +1672,12 @@ extern "C" hipError_t pk_launch_exact_center_order(const float* cdist, uint32_t 
     if (nlist == 0) return hipSuccess;
     const uint32_t L = C > F ? C : F;
     size_t smem = sizeof(uint2) * ((L + 2) & ~1u) + sizeof(float) * L;
-    if (smem > 160 * 1024) return hipErrorInvalidValue;
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute((const void*)pk_exact_center_order, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024);
-        attr = true;
+    static size_t allowed = 0;
+    if (!allowed) {
+        const hipError_t e = pk_allow_lds((const void*)pk_exact_center_order, 160 * 1024, &allowed);
+        if (e != hipSuccess) return e;
     }
+    if (smem > allowed) return hipErrorInvalidValue;
     hipLaunchKernelGGL(pk_exact_center_order, dim3(nlist), dim3(256), smem, s, cdist, C, fdist, F, scs, scap,
                        q_scc, qlist, flags, force, csel_d, csel_id, fsort_d, fsort_id);
     return hipGetLastError();

@@ -62,13 +62,14 @@ extern "C" hipError_t pk_launch_sort_rows(const float* vals, uint32_t rows, uint
                                           uint32_t flag_lim, const uint32_t* skip_nf, hipStream_t s) {
     if (rows == 0 || keep == 0) return hipSuccess;
     if (L > 8192 || keep > L) return hipErrorInvalidValue;
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute((const void*)pk_sort_rows, hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024);
-        attr = true;
+    static size_t allowed = 0;
+    if (!allowed) {
+        const hipError_t e = pk_allow_lds((const void*)pk_sort_rows, 64 * 1024, &allowed);
+        if (e != hipSuccess) return e;
     }
     uint32_t n2 = 2;
     while (n2 < L) n2 <<= 1;
+    if (n2 * sizeof(uint64_t) > allowed) return hipErrorInvalidValue;
     hipLaunchKernelGGL(pk_sort_rows, dim3(rows), dim3(SORT_THREADS), n2 * sizeof(uint64_t), s, vals, L, ld_in,
                        keep, out_val, out_idx, n2, flags, flag_lim, skip_nf);
     return hipGetLastError();
