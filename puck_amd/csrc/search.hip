@@ -400,7 +400,7 @@ __device__ uint32_t pk_walk_select(PkWalkSmem& s, uint32_t nW, uint32_t clo, uin
 // entries, capacity ftk).  Marks them consumed.  Returns the new size.
 // When dry != 0 nothing is written; instead the key of rank ftk-1 of the
 // union is stored in red[8..9] (or ~0 when the union has fewer than ftk).
-template <int FTKM>
+template <int FTKM, bool EX>
 __device__ uint32_t pk_walk_merge(const PkWalkArgs& a, PkWalkSmem& s, uint32_t ns, uint32_t m,
                                   uint32_t seqbase, uint32_t p_lo, int dry, uint32_t plen = 0) {
     const uint32_t ftk = a.ftk;
@@ -484,13 +484,13 @@ __device__ uint32_t pk_walk_merge(const PkWalkArgs& a, PkWalkSmem& s, uint32_t n
     for (uint32_t r = threadIdx.x; r < m; r += blockDim.x) {
         uint32_t wpos = s.X_pos[r];
         s.W_done[wpos] = 1;
-        if (a.exact) s.P_sel[(uint32_t)(s.W_key[wpos] & 0xffffffffu) - seqbase - p_lo] = 1;
+        if (EX) s.P_sel[(uint32_t)(s.W_key[wpos] & 0xffffffffu) - seqbase - p_lo] = 1;
     }
     __syncthreads();
     // Entries dropped with the same distance as the new top: the reference may
     // have kept any of them instead.  Remember them (the list only keeps
     // entries at the current top; older ones can no longer be in either set).
-    if (!a.exact && s.red[10] != 0u && s.red[10] == s.red[11]) {
+    if (!EX && s.red[10] != 0u && s.red[10] == s.red[11]) {
         const uint32_t K = s.red[11];
         if (threadIdx.x == 0) {
             uint32_t n0 = s.red[14], w = 0;
@@ -517,7 +517,7 @@ __device__ uint32_t pk_walk_merge(const PkWalkArgs& a, PkWalkSmem& s, uint32_t n
         if (threadIdx.x == 0 && s.red[14] > PK_TIE_MAX) s.red[14] = PK_TIE_OVERFLOW;
         __syncthreads();
     }
-    if (a.exact) {
+    if (EX) {
         // replay the accepted values in arrival order through the reference heap
         uint32_t cnt = 0;
         for (uint32_t base = 0; base < plen; base += blockDim.x) {
@@ -824,7 +824,7 @@ __device__ __forceinline__ float pk_walk_top(const PkWalkSmem& s, uint32_t ns, u
     return ns >= ftk ? pk_ord2f((uint32_t)(s.S_key[ftk - 1] >> 32)) : pk_sentinel();
 }
 
-template <int FTKM>
+template <int FTKM, bool EX>
 __device__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x;
@@ -850,7 +850,7 @@ __device__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
     s.P_t = (float*)ptr; ptr += sizeof(float) * WALK_PMAX;
     s.P_sel = (uint8_t*)ptr; ptr += WALK_PMAX;
     ptr = (unsigned char*)(((uintptr_t)ptr + 15) & ~(uintptr_t)15);
-    s.H = (uint2*)ptr; ptr += a.exact ? sizeof(uint2) * ((ftk + 2) & ~1u) : 0;
+    s.H = (uint2*)ptr; ptr += EX ? sizeof(uint2) * ((ftk + 2) & ~1u) : 0;
     s.td_val = (uint32_t*)ptr; ptr += sizeof(uint32_t) * PK_TIE_MAX;
     s.td_id = (uint32_t*)ptr; ptr += sizeof(uint32_t) * PK_TIE_MAX;
 
@@ -860,7 +860,7 @@ __device__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
     bool tie_dec = false;
     uint32_t tie_sub = 0;
     for (uint32_t i = tid; i < WALK_PMAX; i += blockDim.x) s.P_sel[i] = 0;
-    if (a.exact) pk_heap_init(s.H, ftk);
+    if (EX) pk_heap_init(s.H, ftk);
     if (tid == 0) { s.red[13] = ftk; s.red[14] = 0u; }
     __syncthreads();
     if (a.qstatus[qi] != 0) {
@@ -952,7 +952,7 @@ __device__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
             if (!big) {
                 nW = pk_walk_values(a, s, cend, 0, ptot, cd, Thi, seqbase);
                 uint32_t m = pk_walk_select(s, nW, 0, WALK_CH, 0xffu, seqbase, 0);
-                pk_walk_merge<FTKM>(a, s, ns, m, seqbase, 0, 1);
+                pk_walk_merge<FTKM, EX>(a, s, ns, m, seqbase, 0, 1);
                 uint32_t hi = s.red[8], lo = s.red[9];
                 __syncthreads();
                 float Tlow = (hi == 0xffffffffu && lo == 0xffffffffu) ? pk_sentinel() : pk_ord2f(hi);
@@ -1007,7 +1007,7 @@ __device__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
                 const uint32_t last1 = pk_bmax((inr && cls == 1 && (uint32_t)tid < u) ? (uint32_t)tid + 1 : 0u, s.red);
                 if (nW > 0) {
                     uint32_t m = pk_walk_select(s, nW, t0, u, 1u, seqbase, 0);
-                    ns = pk_walk_merge<FTKM>(a, s, ns, m, seqbase, 0, 0, ptot);
+                    ns = pk_walk_merge<FTKM, EX>(a, s, ns, m, seqbase, 0, 0, ptot);
                 }
                 T = pk_walk_top(s, ns, ftk);
                 P = pk_pivot(T, qn, rr);
@@ -1045,7 +1045,7 @@ __device__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
                                 res |= (cell == u) ? 1u : 2u;
                                 if (k == 0) {
                                     uint32_t mm = pk_walk_select(s, nW, cell, cell + 1, cell == u ? 3u : 4u, seqbase, 0);
-                                    pk_walk_merge<FTKM>(a, s, ns, mm, seqbase, 0, 1);
+                                    pk_walk_merge<FTKM, EX>(a, s, ns, mm, seqbase, 0, 1);
                                     const uint32_t hi = s.red[8], lo = s.red[9];
                                     __syncthreads();
                                     const float T1 = (hi == 0xffffffffu && lo == 0xffffffffu) ? T : pk_ord2f(hi);
@@ -1066,7 +1066,7 @@ __device__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
                                 __syncthreads();
                                 if (nW > 0) {
                                     uint32_t mm = pk_walk_select(s, nW, u, u + 2, 5u, seqbase, 0);
-                                    ns = pk_walk_merge<FTKM>(a, s, ns, mm, seqbase, 0, 0, ptot);
+                                    ns = pk_walk_merge<FTKM, EX>(a, s, ns, mm, seqbase, 0, 0, ptot);
                                 }
                                 T = pk_walk_top(s, ns, ftk);
                                 P = pk_pivot(T, qn, rr);
@@ -1090,14 +1090,14 @@ __device__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
                             uint32_t p_hi = p_lo + WALK_PMAX < ptot ? p_lo + WALK_PMAX : ptot;
                             uint32_t nb = pk_walk_values(a, s, cend, p_lo, p_hi, cd, T, seqbase);
                             uint32_t m = pk_walk_select(s, nb, 0, WALK_CH, 0xffu, seqbase, p_lo);
-                            ns = pk_walk_merge<FTKM>(a, s, ns, m, seqbase, p_lo, 0, p_hi - p_lo);
+                            ns = pk_walk_merge<FTKM, EX>(a, s, ns, m, seqbase, p_lo, 0, p_hi - p_lo);
                             T = pk_walk_top(s, ns, ftk);
                         }
                     } else if (nW > 0) {
                         if (tid == 0) s.c_cls[u] = 3;
                         __syncthreads();
                         uint32_t m = pk_walk_select(s, nW, u, u + 1, 3u, seqbase, 0);
-                        ns = pk_walk_merge<FTKM>(a, s, ns, m, seqbase, 0, 0, ptot);
+                        ns = pk_walk_merge<FTKM, EX>(a, s, ns, m, seqbase, 0, 0, ptot);
                     }
                     T = pk_walk_top(s, ns, ftk);
                     P = pk_pivot(T, qn, rr);
@@ -1114,7 +1114,7 @@ __device__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
             j0 += cend;
         }
     }
-    if (a.exact) {
+    if (EX) {
         // the filter heap as rank_topN_points reads it: heap order when full,
         // heap-sorted ascending when not full (puck_index.cpp:552-554)
         if (tid < 64 && ns < ftk) pk_heap_reorder_wave(s.H, ftk);
@@ -1160,16 +1160,18 @@ __device__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
 // over the device list wlist[0..*wcount) of the queries that take the walk
 // (the others of the batch are budget-scan queries); every workgroup leaves
 // the loop when the list is exhausted.
-template <int FTKM>
+// EX: the reference-heap form (replays), compiled apart so that the fast
+// form does not carry its registers
+template <int FTKM, bool EX>
 __global__ __launch_bounds__(WALK_THREADS) void pk_walk(PkWalkArgs a) {
-    pk_walk_query<FTKM>(a, a.qlist ? a.qlist[blockIdx.x] : blockIdx.x);
+    pk_walk_query<FTKM, EX>(a, a.qlist ? a.qlist[blockIdx.x] : blockIdx.x);
 }
 
 template <int FTKM>
 __global__ __launch_bounds__(WALK_THREADS) void pk_walk_list(PkWalkArgs a) {
     const uint32_t n = *a.wcount;
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
-        pk_walk_query<FTKM>(a, a.wlist[i]);
+        pk_walk_query<FTKM, false>(a, a.wlist[i]);
         __syncthreads();
     }
 }
@@ -1275,6 +1277,7 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
     if (nq == 0) return hipSuccess;
     if (args->ftk == 0 || args->ftk > WALK_FTK_MAX || args->cstride < args->scc) return hipErrorInvalidValue;
     if (args->q_nf && !args->fdist) return hipErrorInvalidValue;
+    if (args->wcount && args->exact) return hipErrorInvalidValue;  // the list form is the fast walk only
     size_t smem = pk_walk_smem_bytes(args->dim, args->nsq, args->ftk, args->exact);
     // budget scan: fill the same footprint as the walk, at least 80 KB (two
     // blocks per CU) and at least ftk + SCAN_THREADS candidates, up to
@@ -1292,7 +1295,8 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
     const uint32_t wgrid = args->wcount ? (nq < 1024 ? nq : 1024) : nq;
     static size_t allowed = 0;
     if (!allowed) {
-        const void* fns[] = {(const void*)pk_walk<1024>, (const void*)pk_walk<2048>,
+        const void* fns[] = {(const void*)pk_walk<1024, false>, (const void*)pk_walk<2048, false>,
+                             (const void*)pk_walk<1024, true>, (const void*)pk_walk<2048, true>,
                              (const void*)pk_walk_list<1024>, (const void*)pk_walk_list<2048>,
                              (const void*)pk_scan_budget_kernel<1024>, (const void*)pk_scan_budget_kernel<2048>};
         size_t lo = 160 * 1024;
@@ -1310,12 +1314,15 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
     // then the walk for the queries without a budget
     auto launch_walk = [&](hipStream_t st) {
         const bool lst = args->wcount != nullptr;
+        const bool ex = args->exact != 0;
         if (args->ftk <= 1024) {
             if (lst) hipLaunchKernelGGL(pk_walk_list<1024>, dim3(wgrid), dim3(WALK_THREADS), smem, st, *args);
-            else hipLaunchKernelGGL(pk_walk<1024>, dim3(wgrid), dim3(WALK_THREADS), smem, st, *args);
+            else if (ex) hipLaunchKernelGGL((pk_walk<1024, true>), dim3(wgrid), dim3(WALK_THREADS), smem, st, *args);
+            else hipLaunchKernelGGL((pk_walk<1024, false>), dim3(wgrid), dim3(WALK_THREADS), smem, st, *args);
         } else {
             if (lst) hipLaunchKernelGGL(pk_walk_list<2048>, dim3(wgrid), dim3(WALK_THREADS), smem, st, *args);
-            else hipLaunchKernelGGL(pk_walk<2048>, dim3(wgrid), dim3(WALK_THREADS), smem, st, *args);
+            else if (ex) hipLaunchKernelGGL((pk_walk<2048, true>), dim3(wgrid), dim3(WALK_THREADS), smem, st, *args);
+            else hipLaunchKernelGGL((pk_walk<2048, false>), dim3(wgrid), dim3(WALK_THREADS), smem, st, *args);
         }
     };
     const bool fork = args->q_nf && s2 && ev_fork && ev_join;
