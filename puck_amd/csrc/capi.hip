@@ -18,7 +18,9 @@ hipError_t pk_launch_gemm_exact(const float*, const float*, float*, uint32_t, ui
                                 const float*, hipStream_t);
 hipError_t pk_launch_row_norms(const float*, uint32_t, uint32_t, int, float*, hipStream_t);
 hipError_t pk_launch_sort_rows(const float*, uint32_t, uint32_t, uint64_t, uint32_t, float*, uint32_t*, uint8_t*,
-                               uint32_t, const uint32_t*, hipStream_t);
+                               uint32_t, hipStream_t);
+hipError_t pk_launch_sort_rows_list(const float*, uint32_t, uint32_t, uint64_t, uint32_t, float*, uint32_t*,
+                                    const uint32_t*, const uint32_t*, hipStream_t);
 hipError_t pk_launch_topk_rows(const float*, uint32_t, uint32_t, uint64_t, uint32_t, float*, uint32_t*, uint8_t*,
                                uint32_t, hipStream_t);
 hipError_t pk_launch_cell_topk(const float*, const uint32_t*, uint32_t, const float*, const float*, uint32_t, uint32_t,
@@ -443,7 +445,7 @@ hipError_t coarse_select(const float* cdist, uint32_t m, uint32_t C, uint32_t ke
                          uint8_t* flags, uint32_t flag_lim, hipStream_t s) {
     if (keep == 0) return hipSuccess;
     if (keep <= 256) return pk_launch_topk_rows(cdist, m, C, C, keep, csd, csi, flags, flag_lim, s);
-    return pk_launch_sort_rows(cdist, m, C, C, keep, csd, csi, flags, flag_lim, nullptr, s);
+    return pk_launch_sort_rows(cdist, m, C, C, keep, csd, csi, flags, flag_lim, s);
 }
 
 // Probe features of m prepared queries: cell-distance row of the nearest
@@ -556,8 +558,14 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
                                     budget_out_host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s));
         nf = b.pnf;
     }
-    // fine order for the walk (rows of budget-scan queries are skipped)
-    PK_CHECK(pk_launch_sort_rows(b.fdist, m, F, F, F, b.fsd, b.fsi, nullptr, 0, nf, s));
+    // fine order for the walk: every row, or with budgets only the rows of the
+    // queries that take the walk
+    if (nf) {
+        PK_CHECK(pk_launch_build_walk_list(nf, m, F, b.wlist, b.wcount, s));
+        PK_CHECK(pk_launch_sort_rows_list(b.fdist, m, F, F, F, b.fsd, b.fsi, b.wlist, b.wcount, s));
+    } else {
+        PK_CHECK(pk_launch_sort_rows(b.fdist, m, F, F, F, b.fsd, b.fsi, nullptr, 0, s));
+    }
     if (ev) PK_CHECK(hipEventRecord(ev[2], s));
     PkWalkArgs wa{};
     wa.q = b.q; wa.qn = b.qn; wa.qstatus = b.qst; wa.csel_d = b.csd; wa.csel_id = b.csi; wa.fsort_d = b.fsd;
@@ -569,8 +577,7 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     wa.q_scc = q_scc; wa.q_nf = nf; wa.cstride = cfg.keep ? cfg.keep : 1; wa.fdist = b.fdist;
     wa.wlist = nullptr; wa.wcount = nullptr;
     if (nf) {
-        // the walk runs only over the queries without a budget below F
-        PK_CHECK(pk_launch_build_walk_list(nf, m, F, b.wlist, b.wcount, s));
+        // the walk runs only over the queries without a budget below F (the list built above)
         wa.wlist = b.wlist;
         wa.wcount = b.wcount;
     }
@@ -1021,7 +1028,7 @@ int puckgpu_probe_features(puckgpu_ctx* ctx, const puckgpu_index* ix, const puck
         PK_CHECK(pk_launch_gemm_exact(q, ix->coarse, cdist, m, C, d, C, PK_EPI_SUB_FROM_BIAS, ix->coarse_norms, s));
         PK_CHECK(pk_launch_gemm_exact(q, ix->fine, fdist, m, F, d, F, PK_EPI_NEG, nullptr, s));
         PK_CHECK(coarse_select(cdist, m, C, keep, csd, csi, nullptr, 0, s));
-        if (fine_order) PK_CHECK(pk_launch_sort_rows(fdist, m, F, F, F, fsd, fsi, nullptr, 0, nullptr, s));
+        if (fine_order) PK_CHECK(pk_launch_sort_rows(fdist, m, F, F, F, fsd, fsi, nullptr, 0, s));
         float* fo = device_ptrs ? feat + (size_t)q0 * p->D : feat;
         float* cv = (float*)scratch(ctx, S_PCV, sizeof(float) * (size_t)m * (p->nfine ? p->nfine : 1), &rc);
         if (rc) return rc;

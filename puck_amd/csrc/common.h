@@ -63,6 +63,55 @@ __device__ __forceinline__ float pk_lut_entry(const float* x, const float* y, ui
     return (d0 * d0 + d1 * d1) + (d2 * d2 + d3 * d3);
 }
 
+// The filter lookup table of one query (quantization.cpp:260-270): entry e
+// = pk_lut_entry(q sub-vector e / 256, codebook row e), for the nsq * 256
+// entries, written to LDS by the whole block.  Each thread issues the
+// codebook loads of PK_LUT_UNROLL entries before computing any of them (one
+// load latency per group instead of one per entry; 8 keeps the budget scan
+// within two waves per SIMD).
+#ifndef PK_LUT_UNROLL
+#define PK_LUT_UNROLL 8
+#endif
+template <int LSQ>
+__device__ void pk_build_lut_t(float* __restrict__ lut, const float* __restrict__ qv,
+                               const float* __restrict__ codebook, uint32_t n) {
+    constexpr int U = PK_LUT_UNROLL;
+    for (uint32_t e0 = threadIdx.x; e0 < n; e0 += U * blockDim.x) {
+        float c[U][LSQ];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t e = e0 + u * blockDim.x;
+            if (e < n) {
+                if constexpr (LSQ == 4) {
+                    const float4 w = *(const float4*)(codebook + (uint64_t)e * 4);
+                    c[u][0] = w.x;
+                    c[u][1] = w.y;
+                    c[u][2] = w.z;
+                    c[u][3] = w.w;
+                } else if constexpr (LSQ == 2) {
+                    const float2 w = *(const float2*)(codebook + (uint64_t)e * 2);
+                    c[u][0] = w.x;
+                    c[u][1] = w.y;
+                } else {
+                    c[u][0] = codebook[e];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t e = e0 + u * blockDim.x;
+            if (e < n) lut[e] = pk_lut_entry(qv + (e / 256u) * LSQ, c[u], LSQ);
+        }
+    }
+}
+
+__device__ __forceinline__ void pk_build_lut(float* lut, const float* qv, const float* codebook, uint32_t nsq,
+                                             uint32_t lsq) {
+    if (lsq == 1) pk_build_lut_t<1>(lut, qv, codebook, nsq * 256u);
+    else if (lsq == 2) pk_build_lut_t<2>(lut, qv, codebook, nsq * 256u);
+    else pk_build_lut_t<4>(lut, qv, codebook, nsq * 256u);
+}
+
 // Pivot (puck_index.cpp:517, :544): (top - |q|^2) / radius_rate / 2.0 with the
 // subtraction in float and the divisions in double, stored as float.
 __device__ __forceinline__ float pk_pivot(float top, float qn, double rr) {

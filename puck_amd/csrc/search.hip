@@ -44,21 +44,10 @@
 // Rows at or past *nvalid (when given) are padding of a fixed-size batch:
 // status 2, skipped by every later stage.  flags (when given) are cleared,
 // with a kernel rather than a memset so that the pass can be graph-captured.
-__global__ void pk_prep_queries(const float* __restrict__ qin, uint32_t nq, uint32_t dim, int whether_norm,
-                                float* __restrict__ qout, float* __restrict__ qn,
-                                int32_t* __restrict__ status, const uint32_t* __restrict__ nvalid,
-                                uint8_t* __restrict__ flags) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
-    if (flags) flags[i] = 0;
-    const float* x = qin + (uint64_t)i * dim;
-    float* y = qout + (uint64_t)i * dim;
-    if (nvalid && i >= *nvalid) {
-        for (uint32_t k = 0; k < dim; ++k) y[k] = 0.f;
-        qn[i] = 0.f;
-        status[i] = 2;
-        return;
-    }
+// One query: y = x normalised (whether_norm) or copied, |y|^2, status (0, or
+// -1 for a zero-norm query under whether_norm); y may alias x.
+__device__ __forceinline__ void pk_prep_one(const float* x, float* y, uint32_t dim, int whether_norm, float* qn,
+                                            int32_t* status) {
     int st = 0;
     if (whether_norm) {
         float norm = __fsqrt_rn(pk_dot(x, x, dim));
@@ -72,16 +61,72 @@ __global__ void pk_prep_queries(const float* __restrict__ qin, uint32_t nq, uint
     } else {
         for (uint32_t k = 0; k < dim; ++k) y[k] = x[k];
     }
-    qn[i] = pk_dot(y, y, dim);
-    status[i] = st;
+    *qn = pk_dot(y, y, dim);
+    *status = st;
+}
+
+__global__ void pk_prep_queries(const float* __restrict__ qin, uint32_t nq, uint32_t dim, int whether_norm,
+                                float* __restrict__ qout, float* __restrict__ qn,
+                                int32_t* __restrict__ status, const uint32_t* __restrict__ nvalid,
+                                uint8_t* __restrict__ flags) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    if (flags) flags[i] = 0;
+    float* y = qout + (uint64_t)i * dim;
+    if (nvalid && i >= *nvalid) {
+        for (uint32_t k = 0; k < dim; ++k) y[k] = 0.f;
+        qn[i] = 0.f;
+        status[i] = 2;
+        return;
+    }
+    pk_prep_one(qin + (uint64_t)i * dim, y, dim, whether_norm, qn + i, status + i);
+}
+
+// The same for dim <= PREP_LDS_DIM: 64 queries per block staged through LDS
+// (rows of dim + 1 words, conflict-free), so the loads and stores coalesce.
+#define PREP_LDS_DIM 255
+__global__ __launch_bounds__(64) void pk_prep_queries_lds(const float* __restrict__ qin, uint32_t nq, uint32_t dim,
+                                                          int whether_norm, float* __restrict__ qout,
+                                                          float* __restrict__ qn, int32_t* __restrict__ status,
+                                                          const uint32_t* __restrict__ nvalid,
+                                                          uint8_t* __restrict__ flags) {
+    extern __shared__ float tile[];
+    const uint32_t q0 = blockIdx.x * 64u, nrow = nq - q0 < 64u ? nq - q0 : 64u, ld = dim + 1u;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t e = t; e < nrow * dim; e += 64u) {
+        const uint32_t r = e / dim;
+        tile[r * ld + (e - r * dim)] = qin[(uint64_t)q0 * dim + e];
+    }
+    __syncthreads();
+    if (t < nrow) {
+        const uint32_t i = q0 + t;
+        float* row = tile + t * ld;
+        if (flags) flags[i] = 0;
+        if (nvalid && i >= *nvalid) {
+            for (uint32_t k = 0; k < dim; ++k) row[k] = 0.f;
+            qn[i] = 0.f;
+            status[i] = 2;
+        } else {
+            pk_prep_one(row, row, dim, whether_norm, qn + i, status + i);
+        }
+    }
+    __syncthreads();
+    for (uint32_t e = t; e < nrow * dim; e += 64u) {
+        const uint32_t r = e / dim;
+        qout[(uint64_t)q0 * dim + e] = tile[r * ld + (e - r * dim)];
+    }
 }
 
 extern "C" hipError_t pk_launch_prep_queries_ex(const float* qin, uint32_t nq, uint32_t dim, int whether_norm,
                                                 float* qout, float* qn, int32_t* status, const uint32_t* nvalid,
                                                 uint8_t* flags, hipStream_t s) {
     if (nq == 0) return hipSuccess;
-    hipLaunchKernelGGL(pk_prep_queries, dim3((nq + 127) / 128), dim3(128), 0, s, qin, nq, dim, whether_norm,
-                       qout, qn, status, nvalid, flags);
+    if (dim <= PREP_LDS_DIM)
+        hipLaunchKernelGGL(pk_prep_queries_lds, dim3((nq + 63) / 64), dim3(64), sizeof(float) * 64 * (dim + 1), s,
+                           qin, nq, dim, whether_norm, qout, qn, status, nvalid, flags);
+    else
+        hipLaunchKernelGGL(pk_prep_queries, dim3((nq + 127) / 128), dim3(128), 0, s, qin, nq, dim, whether_norm,
+                           qout, qn, status, nvalid, flags);
     return hipGetLastError();
 }
 
@@ -294,44 +339,70 @@ __device__ uint32_t pk_walk_values(const PkWalkArgs& a, PkWalkSmem& s, uint32_t 
     if (threadIdx.x == 0) s.red[2] = 0;
     __syncthreads();
     const uint32_t nsq = a.nsq;
-    for (uint32_t p = p_lo + threadIdx.x; p < p_hi; p += blockDim.x) {
-        // cell containing p: last t with c_pre[t] <= p (c_pre[cend] = total)
-        uint32_t lo = 0, hi = cend;  // invariant c_pre[lo] <= p < c_pre[hi]
-        while (hi - lo > 1) {
-            uint32_t mid = (lo + hi) >> 1;
-            if (s.c_pre[mid] <= p) lo = mid; else hi = mid;
+    const bool vec = (nsq & 15) == 0 && nsq <= 64;
+    const uint32_t ng = nsq >> 4;
+    // two points per thread and pass, every record load issued before the
+    // first is used (a lane past p_hi loads the last point again: loads under
+    // a lane condition are waited for one at a time)
+    for (uint32_t pb = p_lo; pb < p_hi; pb += 2u * blockDim.x) {
+        uint32_t mids[2], ts[2];
+        float offs[2];
+        uint4 cw[2][4];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t p = pb + h * blockDim.x + threadIdx.x;
+            const uint32_t pc = p < p_hi ? p : p_hi - 1u;
+            // cell containing pc: last t with c_pre[t] <= pc (c_pre[cend] = total)
+            uint32_t lo = 0, hi = cend;  // invariant c_pre[lo] <= pc < c_pre[hi]
+            while (hi - lo > 1) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (s.c_pre[mid] <= pc) lo = mid; else hi = mid;
+            }
+            ts[h] = lo;
+            mids[h] = s.c_b[lo] + (pc - s.c_pre[lo]);
+            offs[h] = a.rec_off[mids[h]];
+            if (vec) {
+                const uint4* cp = (const uint4*)(a.rec_codes + (uint64_t)mids[h] * nsq);
+#pragma unroll
+                for (int g = 0; g < 4; ++g)
+                    if ((uint32_t)g < ng) cw[h][g] = cp[g];
+            }
         }
-        const uint32_t t = lo;
-        const uint32_t mid = s.c_b[t] + (p - s.c_pre[t]);
-        const float cell_dist = (cd + s.c_st[t]) + s.c_fd[t];
-        const float off = a.rec_off[mid];
-        float tv = 2.0f * cell_dist + off;   // 2x is exact: one rounding, as in double
-        if (!(tv < thr)) continue;
-        const uint8_t* codes = a.rec_codes + (uint64_t)mid * nsq;
-        float acc[4] = {0.f, 0.f, 0.f, 0.f};
-        if ((nsq & 15) == 0) {
-            for (uint32_t g = 0; g < nsq; g += 16) {
-                uint4 w = *(const uint4*)(codes + g);
-                uint32_t ws[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-                for (int h = 0; h < 4; ++h) {
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t p = pb + h * blockDim.x + threadIdx.x;
+            if (p >= p_hi) continue;
+            const uint32_t t = ts[h], mid = mids[h];
+            const float cell_dist = (cd + s.c_st[t]) + s.c_fd[t];
+            float tv = 2.0f * cell_dist + offs[h];   // 2x is exact: one rounding, as in double
+            if (!(tv < thr)) continue;
+            float acc[4] = {0.f, 0.f, 0.f, 0.f};
+            if (vec) {
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        uint32_t code = (ws[h] >> (8 * j)) & 0xffu;
-                        acc[j] = acc[j] + s.lut[(g + 4 * h + j) * PK_KS + code];
+                for (int g = 0; g < 4; ++g) {
+                    if ((uint32_t)g >= ng) break;
+                    const uint32_t ws[4] = {cw[h][g].x, cw[h][g].y, cw[h][g].z, cw[h][g].w};
+#pragma unroll
+                    for (int hh = 0; hh < 4; ++hh) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const uint32_t code = (ws[hh] >> (8 * j)) & 0xffu;
+                            acc[j] = acc[j] + s.lut[(16 * g + 4 * hh + j) * PK_KS + code];
+                        }
                     }
                 }
+            } else {
+                const uint8_t* codes = a.rec_codes + (uint64_t)mid * nsq;
+                for (uint32_t m = 0; m < nsq; ++m) acc[m & 3] = acc[m & 3] + s.lut[m * PK_KS + codes[m]];
             }
-        } else {
-            for (uint32_t m = 0; m < nsq; ++m) acc[m & 3] = acc[m & 3] + s.lut[m * PK_KS + codes[m]];
+            tv = tv + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
+            if (!(tv < thr)) continue;
+            uint32_t slot = atomicAdd(&s.red[2], 1u);
+            s.W_key[slot] = ((uint64_t)pk_f2ord(tv) << 32) | (uint64_t)(seqbase + p);
+            s.P_id[p - p_lo] = mid;
+            s.P_cell[p - p_lo] = (uint16_t)t;
+            s.P_t[p - p_lo] = tv;
         }
-        tv = tv + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
-        if (!(tv < thr)) continue;
-        uint32_t slot = atomicAdd(&s.red[2], 1u);
-        s.W_key[slot] = ((uint64_t)pk_f2ord(tv) << 32) | (uint64_t)(seqbase + p);
-        s.P_id[p - p_lo] = mid;
-        s.P_cell[p - p_lo] = (uint16_t)t;
-        s.P_t[p - p_lo] = tv;
     }
     __syncthreads();
     uint32_t n = s.red[2];
@@ -588,6 +659,43 @@ __device__ uint32_t pk_keep_le(uint64_t* v, uint32_t n, uint64_t K, uint32_t* tm
     return out;
 }
 
+// The same with one scan: thread t takes the contiguous entries
+// [t*c, t*c + c) (c = ceil(n / blockDim.x) <= KEEP_REG) into registers, so
+// the kept entries keep their order.  out == v: in place; otherwise out gets
+// the kept keys split into (distance, id) rows out_t / out_id (out null).
+#ifndef KEEP_REG
+#define KEEP_REG 16
+#endif
+__device__ uint32_t pk_keep_le_reg(uint64_t* v, uint32_t n, uint64_t K, uint32_t* tmp, float* out_t,
+                                   uint32_t* out_id) {
+    const uint32_t c = (n + blockDim.x - 1) / blockDim.x, b0 = threadIdx.x * c;
+    uint64_t x[KEEP_REG];
+    uint32_t keep = 0;
+#pragma unroll
+    for (int j = 0; j < KEEP_REG; ++j) {
+        const uint32_t i = b0 + j;
+        x[j] = ((uint32_t)j < c && i < n) ? v[i] : ~0ull;
+        keep += ((uint32_t)j < c && i < n && x[j] <= K) ? 1u : 0u;
+    }
+    uint32_t tot;
+    uint32_t p = pk_block_exscan(keep, tmp, &tot);  // every read precedes every write
+#pragma unroll
+    for (int j = 0; j < KEEP_REG; ++j) {
+        const uint32_t i = b0 + j;
+        if ((uint32_t)j < c && i < n && x[j] <= K) {
+            if (out_t) {
+                out_t[p] = pk_ord2f((uint32_t)(x[j] >> 32));
+                out_id[p] = (uint32_t)x[j];
+            } else {
+                v[p] = x[j];
+            }
+            ++p;
+        }
+    }
+    __syncthreads();
+    return tot;
+}
+
 // Budget scan of one query (fine budget b < F; no counterpart in the
 // reference).  In each walked coarse list the cells whose cell distance
 // (cd + stationary) + fine distance is at most the b-th smallest of the list
@@ -642,10 +750,18 @@ __device__ void pk_scan_budget(const PkWalkArgs& a, PkWalkSmem& s, uint32_t qi, 
         const uint32_t* csc = a.cell_start + (uint64_t)c * F;
         const bool one = F <= SEG;
         float cv[SCAN_CPT];
+        if (one) {
+            // unconditional loads at in-bounds cells (conditional ones are issued one at a time)
 #pragma unroll
-        for (int j = 0; j < SCAN_CPT; ++j) {
-            const uint32_t k = j * NT + tid;
-            cv[j] = one && k < F ? (cd + stc[k]) + fdq[k] : 0.f;
+            for (int j = 0; j < SCAN_CPT; ++j) {
+                const uint32_t k = min(j * NT + tid, F - 1u);
+                cv[j] = (cd + stc[k]) + fdq[k];
+            }
+        }
+        if (l == 0) {
+            // the lookup table, built while the first list's cell keys load
+            pk_build_lut(s.lut, s.qv, a.codebook, nsq, a.lsq);
+            __syncthreads();
         }
         // the list's cut: b-th smallest cell key
         uint32_t kcut;
@@ -667,18 +783,23 @@ __device__ void pk_scan_budget(const PkWalkArgs& a, PkWalkSmem& s, uint32_t qi, 
             // selected non-empty cells of the segment held by this thread
             uint32_t cb[SCAN_CPT], sz[SCAN_CPT];
             uint32_t ns = 0, np = 0;
+            if (!one) {
+#pragma unroll
+                for (int j = 0; j < SCAN_CPT; ++j) {
+                    const uint32_t k = min(seg + j * NT + tid, F - 1u);
+                    cv[j] = (cd + stc[k]) + fdq[k];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < SCAN_CPT; ++j) {  // cell starts of every held cell, in bounds
+                const uint32_t k = min(seg + j * NT + tid, F - 1u);
+                cb[j] = csc[k];
+                sz[j] = csc[k + 1];
+            }
 #pragma unroll
             for (int j = 0; j < SCAN_CPT; ++j) {
                 const uint32_t k = seg + j * NT + tid;
-                sz[j] = 0;
-                cb[j] = 0;
-                if (k < F) {
-                    if (!one) cv[j] = (cd + stc[k]) + fdq[k];
-                    if (pk_f2ord(cv[j]) <= kcut) {
-                        cb[j] = csc[k];
-                        sz[j] = csc[k + 1] - cb[j];
-                    }
-                }
+                sz[j] = (k < F && pk_f2ord(cv[j]) <= kcut) ? sz[j] - cb[j] : 0u;
                 ns += sz[j] > 0 ? 1u : 0u;
                 np += sz[j];
             }
@@ -714,10 +835,11 @@ __device__ void pk_scan_budget(const PkWalkArgs& a, PkWalkSmem& s, uint32_t qi, 
                     if (n + R > cap) {
                         const uint32_t kc = pk_select_rank_each(hi32_each, ftk, hist, tmp, rao);
                         K = ((uint64_t)kc << 32) | 0xffffffffull;
-                        n = pk_keep_le(B, n, K, tmp);
+                        n = n <= KEEP_REG * NT ? pk_keep_le_reg(B, n, K, tmp, nullptr, nullptr) : pk_keep_le(B, n, K, tmp);
                         if (n + R > cap) {  // many equal distances at the cut: cut exactly
                             K = pk_select_rank64(B, n, ftk, hist, tmp);
-                            n = pk_keep_le(B, n, K, tmp);
+                            n = n <= KEEP_REG * NT ? pk_keep_le_reg(B, n, K, tmp, nullptr, nullptr)
+                                                   : pk_keep_le(B, n, K, tmp);
                         }
                     }
                     if (tid == 0) s.red[2] = n;
@@ -731,17 +853,19 @@ __device__ void pk_scan_budget(const PkWalkArgs& a, PkWalkSmem& s, uint32_t qi, 
                     uint32_t mids[PP];
 #pragma unroll
                     for (int h = 0; h < PP; ++h) {
-                        const uint32_t p = p0 + h * NT + tid;
                         mids[h] = 0xffffffffu;
-                        if (h * NT >= R || p >= tot) continue;
-                        const uint32_t P = pbase + p;
+                        if (h * NT >= R) continue;  // uniform
+                        // a lane past the end loads the last point again: loads under a
+                        // lane condition would be issued one at a time
+                        const uint32_t p = p0 + h * NT + tid;
+                        const uint32_t P = pbase + (p < tot ? p : tot - 1u);
                         uint32_t lo = 0, hi = ne;  // c_pre[lo] <= P < c_pre[hi]
                         while (hi - lo > 1) {
                             const uint32_t mid = (lo + hi) >> 1;
                             if (s.c_pre[mid] <= P) lo = mid; else hi = mid;
                         }
                         const uint32_t mid = s.c_b[lo] + (P - s.c_pre[lo]);
-                        mids[h] = mid;
+                        mids[h] = p < tot ? mid : 0xffffffffu;
                         cdl[h] = s.c_fd[lo];
                         off[h] = a.rec_off[mid];
                         if (vec) {
@@ -753,31 +877,43 @@ __device__ void pk_scan_budget(const PkWalkArgs& a, PkWalkSmem& s, uint32_t qi, 
                     }
 #pragma unroll
                     for (int h = 0; h < PP; ++h) {
-                        if (mids[h] == 0xffffffffu) continue;
-                        float tv = 2.0f * cdl[h] + off[h];  // 2x is exact: one rounding, as in double
-                        float acc[4] = {0.f, 0.f, 0.f, 0.f};
-                        if (vec) {
+                        const bool valid = mids[h] != 0xffffffffu;
+                        uint64_t kk = ~0ull;
+                        if (valid) {
+                            float tv = 2.0f * cdl[h] + off[h];  // 2x is exact: one rounding, as in double
+                            float acc[4] = {0.f, 0.f, 0.f, 0.f};
+                            if (vec) {
 #pragma unroll
-                            for (int g = 0; g < 4; ++g) {
-                                if ((uint32_t)g >= ng) break;
-                                const uint32_t ws[4] = {cw[h][g].x, cw[h][g].y, cw[h][g].z, cw[h][g].w};
+                                for (int g = 0; g < 4; ++g) {
+                                    if ((uint32_t)g >= ng) break;
+                                    const uint32_t ws[4] = {cw[h][g].x, cw[h][g].y, cw[h][g].z, cw[h][g].w};
 #pragma unroll
-                                for (int hh = 0; hh < 4; ++hh) {
+                                    for (int hh = 0; hh < 4; ++hh) {
 #pragma unroll
-                                    for (int j = 0; j < 4; ++j) {
-                                        const uint32_t code = (ws[hh] >> (8 * j)) & 0xffu;
-                                        acc[j] = acc[j] + s.lut[(16 * g + 4 * hh + j) * PK_KS + code];
+                                        for (int j = 0; j < 4; ++j) {
+                                            const uint32_t code = (ws[hh] >> (8 * j)) & 0xffu;
+                                            acc[j] = acc[j] + s.lut[(16 * g + 4 * hh + j) * PK_KS + code];
+                                        }
                                     }
                                 }
+                            } else {
+                                const uint8_t* codes = a.rec_codes + (uint64_t)mids[h] * nsq;
+                                for (uint32_t m = 0; m < nsq; ++m)
+                                    acc[m & 3] = acc[m & 3] + s.lut[m * PK_KS + codes[m]];
                             }
-                        } else {
-                            const uint8_t* codes = a.rec_codes + (uint64_t)mids[h] * nsq;
-                            for (uint32_t m = 0; m < nsq; ++m)
-                                acc[m & 3] = acc[m & 3] + s.lut[m * PK_KS + codes[m]];
+                            tv = tv + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
+                            kk = ((uint64_t)pk_f2ord(tv) << 32) | mids[h];
                         }
-                        tv = tv + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
-                        const uint64_t kk = ((uint64_t)pk_f2ord(tv) << 32) | mids[h];
-                        if (kk <= K) B[atomicAdd(&s.red[2], 1u)] = kk;
+                        // one LDS atomic per wave for its kept keys
+                        const bool keep = valid && kk <= K;
+                        const uint64_t bal = __ballot(keep);
+                        if (bal) {
+                            const uint32_t lane = tid & 63u;
+                            uint32_t base = 0;
+                            if (lane == 0) base = atomicAdd(&s.red[2], (uint32_t)__popcll(bal));
+                            base = __shfl(base, 0, 64);
+                            if (keep) B[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = kk;
+                        }
                     }
                     __syncthreads();
                     n = s.red[2];
@@ -801,11 +937,16 @@ __device__ void pk_scan_budget(const PkWalkArgs& a, PkWalkSmem& s, uint32_t qi, 
             __syncthreads();
         }
         K = nle == ftk ? (((uint64_t)kc << 32) | 0xffffffffull) : pk_select_rank64(B, n, ftk, hist, tmp);
-        n = pk_keep_le(B, n, K, tmp);
     }
-    for (uint32_t r = tid; r < n; r += blockDim.x) {
-        a.out_t[(uint64_t)qi * ftk + r] = pk_ord2f((uint32_t)(B[r] >> 32));
-        a.out_id[(uint64_t)qi * ftk + r] = (uint32_t)B[r];
+    // the kept entries (all when n <= ftk) straight to the output rows
+    if (n <= KEEP_REG * NT) {
+        n = pk_keep_le_reg(B, n, K, tmp, a.out_t + (uint64_t)qi * ftk, a.out_id + (uint64_t)qi * ftk);
+    } else {
+        n = pk_keep_le(B, n, K, tmp);
+        for (uint32_t r = tid; r < n; r += blockDim.x) {
+            a.out_t[(uint64_t)qi * ftk + r] = pk_ord2f((uint32_t)(B[r] >> 32));
+            a.out_id[(uint64_t)qi * ftk + r] = (uint32_t)B[r];
+        }
     }
     if (tid == 0) {
         a.out_n[qi] = n;
@@ -870,10 +1011,7 @@ __device__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
     const float* qg = a.q + (uint64_t)qi * a.dim;
     for (uint32_t i = tid; i < a.dim; i += blockDim.x) s.qv[i] = qg[i];
     __syncthreads();
-    for (uint32_t e = tid; e < nsq * PK_KS; e += blockDim.x) {
-        uint32_t m = e / PK_KS;
-        s.lut[e] = pk_lut_entry(s.qv + m * a.lsq, a.codebook + (uint64_t)e * a.lsq, a.lsq);
-    }
+    pk_build_lut(s.lut, s.qv, a.codebook, nsq, a.lsq);
     __syncthreads();
 
     const float qn = a.qn[qi];
@@ -1232,11 +1370,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void pk_scan_budget_kernel(PkWalkArgs
     const float* qg = a.q + (uint64_t)qi * a.dim;
     for (uint32_t i = threadIdx.x; i < a.dim; i += blockDim.x) s.qv[i] = qg[i];
     __syncthreads();
-    for (uint32_t e = threadIdx.x; e < nsq * PK_KS; e += blockDim.x) {
-        uint32_t m = e / PK_KS;
-        s.lut[e] = pk_lut_entry(s.qv + m * a.lsq, a.codebook + (uint64_t)e * a.lsq, a.lsq);
-    }
-    __syncthreads();
+    (void)nsq;
     pk_scan_budget<FTKM>(a, s, qi, a.q_nf[qi], B, a.scan_cap, a.scan_round, hist);
 }
 
@@ -1351,6 +1485,9 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
 // ---------------------------------------------------------------------------
 
 #define RERANK_THREADS 256
+#ifndef RERANK_RP
+#define RERANK_RP 2  // rows per 8-lane group and iteration of the distance loop
+#endif
 
 // Fast mode: distances sorted by (distance, filter rank).  Exact mode: the
 // result MaxHeap of rank_topN_points fed in filter-heap order, then reorder().
@@ -1422,7 +1559,7 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
     // both rows issued before the first is used (the gathers are latency
     // bound: one row is a few cache lines).
     {
-        constexpr int RP = 2, SB = 8;  // rows per group, 16-element steps per block
+        constexpr int RP = RERANK_RP, SB = 8;  // rows per group, 16-element steps per block
         const uint32_t g = threadIdx.x >> 3, j = threadIdx.x & 7, ng = blockDim.x >> 3;
         const int src = (int)(threadIdx.x & 63u & ~7u);
         const uint32_t dim = a.dim, nfull = dim >> 4;
@@ -1433,7 +1570,9 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
 #pragma unroll
             for (int h = 0; h < RP; ++h) {
                 rr[h] = r0 + h * ng + g;
-                mid[h] = rr[h] < n ? a.filt_id[(uint64_t)qi * ftk + rr[h]] : 0u;
+                // in-row load for every lane (a lane-conditional load is waited for at once)
+                const uint32_t fid = a.filt_id[(uint64_t)qi * ftk + (rr[h] < ftk ? rr[h] : ftk - 1u)];
+                mid[h] = rr[h] < n ? fid : 0u;
                 x[h] = a.raw + (uint64_t)mid[h] * dim;
                 t[h] = 0.0f;
             }

@@ -29,16 +29,9 @@ __device__ __forceinline__ void pk_bitonic_u64(uint64_t* keys, uint32_t n2) {
     }
 }
 
-__global__ __launch_bounds__(SORT_THREADS) void pk_sort_rows(const float* __restrict__ vals, uint32_t L,
-                                                             uint64_t ld_in, uint32_t keep,
-                                                             float* __restrict__ out_val,
-                                                             uint32_t* __restrict__ out_idx, uint32_t n2,
-                                                             uint8_t* __restrict__ flags, uint32_t flag_lim,
-                                                             const uint32_t* __restrict__ skip_nf) {
-    extern __shared__ uint64_t sk[];
-    const uint64_t row = blockIdx.x;
-    // rows of queries with a fine budget below L are not walked in fine order
-    if (skip_nf && skip_nf[row] != 0u && skip_nf[row] < L) return;
+__device__ void pk_sort_row(const float* __restrict__ vals, uint64_t row, uint32_t L, uint64_t ld_in, uint32_t keep,
+                            float* __restrict__ out_val, uint32_t* __restrict__ out_idx, uint32_t n2,
+                            uint8_t* __restrict__ flags, uint32_t flag_lim, uint64_t* sk) {
     const float* v = vals + row * ld_in;
     for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x)
         sk[i] = i < L ? (((uint64_t)pk_f2ord(v[i]) << 32) | i) : ~0ull;
@@ -55,11 +48,54 @@ __global__ __launch_bounds__(SORT_THREADS) void pk_sort_rows(const float* __rest
     // equal distances among the first flag_lim entries (or at that boundary):
     // the reference's heap order decides, flag the row for the exact replay
     if (tie && flags) flags[row] = 1u | flags[row];
+    __syncthreads();  // sk is reused by the next row of a list
+}
+
+__global__ __launch_bounds__(SORT_THREADS) void pk_sort_rows(const float* __restrict__ vals, uint32_t L,
+                                                             uint64_t ld_in, uint32_t keep,
+                                                             float* __restrict__ out_val,
+                                                             uint32_t* __restrict__ out_idx, uint32_t n2,
+                                                             uint8_t* __restrict__ flags, uint32_t flag_lim) {
+    extern __shared__ uint64_t sk[];
+    const uint64_t row = blockIdx.x;
+    pk_sort_row(vals, row, L, ld_in, keep, out_val, out_idx, n2, flags, flag_lim, sk);
+}
+
+// The rows listed in list[0 .. *count), grid-stride (the rows a walk will use).
+__global__ __launch_bounds__(SORT_THREADS) void pk_sort_rows_list(const float* __restrict__ vals, uint32_t L,
+                                                                  uint64_t ld_in, uint32_t keep,
+                                                                  float* __restrict__ out_val,
+                                                                  uint32_t* __restrict__ out_idx, uint32_t n2,
+                                                                  const uint32_t* __restrict__ list,
+                                                                  const uint32_t* __restrict__ count) {
+    extern __shared__ uint64_t sk[];
+    const uint32_t n = *count;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x)
+        pk_sort_row(vals, list[i], L, ld_in, keep, out_val, out_idx, n2, nullptr, 0, sk);
+}
+
+extern "C" hipError_t pk_launch_sort_rows_list(const float* vals, uint32_t max_rows, uint32_t L, uint64_t ld_in,
+                                               uint32_t keep, float* out_val, uint32_t* out_idx,
+                                               const uint32_t* list, const uint32_t* count, hipStream_t s) {
+    if (max_rows == 0 || keep == 0) return hipSuccess;
+    if (L > 8192 || keep > L) return hipErrorInvalidValue;
+    static size_t allowed = 0;
+    if (!allowed) {
+        const hipError_t e = pk_allow_lds((const void*)pk_sort_rows_list, 64 * 1024, &allowed);
+        if (e != hipSuccess) return e;
+    }
+    uint32_t n2 = 2;
+    while (n2 < L) n2 <<= 1;
+    if (n2 * sizeof(uint64_t) > allowed) return hipErrorInvalidValue;
+    const uint32_t grid = max_rows < 1024 ? max_rows : 1024;
+    hipLaunchKernelGGL(pk_sort_rows_list, dim3(grid), dim3(SORT_THREADS), n2 * sizeof(uint64_t), s, vals, L, ld_in,
+                       keep, out_val, out_idx, n2, list, count);
+    return hipGetLastError();
 }
 
 extern "C" hipError_t pk_launch_sort_rows(const float* vals, uint32_t rows, uint32_t L, uint64_t ld_in,
                                           uint32_t keep, float* out_val, uint32_t* out_idx, uint8_t* flags,
-                                          uint32_t flag_lim, const uint32_t* skip_nf, hipStream_t s) {
+                                          uint32_t flag_lim, hipStream_t s) {
     if (rows == 0 || keep == 0) return hipSuccess;
     if (L > 8192 || keep > L) return hipErrorInvalidValue;
     static size_t allowed = 0;
@@ -71,7 +107,7 @@ extern "C" hipError_t pk_launch_sort_rows(const float* vals, uint32_t rows, uint
     while (n2 < L) n2 <<= 1;
     if (n2 * sizeof(uint64_t) > allowed) return hipErrorInvalidValue;
     hipLaunchKernelGGL(pk_sort_rows, dim3(rows), dim3(SORT_THREADS), n2 * sizeof(uint64_t), s, vals, L, ld_in,
-                       keep, out_val, out_idx, n2, flags, flag_lim, skip_nf);
+                       keep, out_val, out_idx, n2, flags, flag_lim);
     return hipGetLastError();
 }
 
@@ -125,55 +161,114 @@ __device__ void pk_topk_block(ValF val, uint32_t L, uint32_t k, float* out_val, 
     *tie = t;
 }
 
-// The same selection for rows of at most SORT_THREADS * PER entries, with
-// every key read once into registers: thread t holds columns t*PER ..
-// t*PER+PER-1 (so a scan over threads is column order).  The radix select
-// starts at the highest bit in which the row's keys differ (distances of one
-// row usually share their sign and top exponent bits, which would send every
-// key of the first pass to one histogram bin); the entries below kc and the
-// first ties at it are compacted with one packed scan and placed by rank.
-template <int PER, typename KeyF>
-__device__ void pk_topk_block_reg(KeyF key, uint32_t L, uint32_t k, float* out_val, uint32_t* out_idx, bool* tie,
-                                  uint32_t flag_lim) {
-    __shared__ uint32_t hist[258];
-    __shared__ uint32_t tmp[8];
-    __shared__ uint32_t red[8];
-    __shared__ uint64_t buf[TOPK_CAP];
-    const uint32_t t = threadIdx.x, base = t * PER;
-    const uint32_t nt = base < L ? min((uint32_t)PER, L - base) : 0u;
-    uint32_t kv[PER];
+// The same selection by one wave per row (rows of at most 64 * TOPK_WAVE_PER
+// entries, k <= TOPK_WAVE_K): lane l holds columns j * 64 + l in registers.
+// A radix select of the k-th smallest key kc over a per-wave LDS histogram,
+// starting at the highest bit in which the row's keys differ (distances of
+// one row usually share their sign and top exponent bits, which would send
+// every key of a first pass to one bin); the entries below kc and then the
+// first ties at it in column order are compacted with ballots, and the k
+// kept entries are placed by rank.  No block barrier: the wave's LDS
+// operations complete in order.
+#define TOPK_WAVE_PER 32
+#define TOPK_WAVE_K 256
+__device__ __forceinline__ void pk_wave_lds_order() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <typename KeyF>
+__device__ void pk_topk_wave(KeyF key, uint32_t L, uint32_t k, float* out_val, uint32_t* out_idx, bool* tie,
+                             uint32_t flag_lim, uint32_t* hist, uint64_t* buf) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t kv[TOPK_WAVE_PER];
+    uint32_t ka = 0xffffffffu, ko = 0u;
+    // every load at an in-bounds column (a load under a lane condition is
+    // waited for before the next one is issued)
 #pragma unroll
-    for (int j = 0; j < PER; ++j) kv[j] = (uint32_t)j < nt ? key(base + j) : 0u;
-    const uint32_t kc = pk_select_rank_each(
-        [&](auto sink) {
+    for (int j = 0; j < TOPK_WAVE_PER; ++j) {
+        const uint32_t c = j * 64u + lane;
+        kv[j] = key(c < L ? c : L - 1u);
+    }
 #pragma unroll
-            for (int j = 0; j < PER; ++j)
-                if ((uint32_t)j < nt) sink(kv[j]);
-        },
-        k, hist, tmp, red);
-    uint32_t cl = 0, ce = 0;
-#pragma unroll
-    for (int j = 0; j < PER; ++j) {
-        if ((uint32_t)j < nt) {
-            cl += kv[j] < kc ? 1u : 0u;
-            ce += kv[j] == kc ? 1u : 0u;
+    for (int j = 0; j < TOPK_WAVE_PER; ++j) {
+        if (j * 64u + lane < L) {
+            ka &= kv[j];
+            ko |= kv[j];
         }
     }
-    uint32_t tot;
-    const uint32_t pre = pk_block_exscan(cl | (ce << 16), tmp, &tot);
-    const uint32_t nlt = tot & 0xffffu, neq = tot >> 16;
-    uint32_t pl = pre & 0xffffu, pe = nlt + (pre >> 16);
 #pragma unroll
-    for (int j = 0; j < PER; ++j) {
-        if ((uint32_t)j < nt) {
-            const uint64_t e = ((uint64_t)kv[j] << 32) | (base + j);
-            if (kv[j] < kc) buf[pl++] = e;
-            else if (kv[j] == kc) { if (pe < k) buf[pe] = e; ++pe; }
-        }
+    for (int o = 1; o < 64; o <<= 1) {
+        ka &= __shfl_xor(ka, o, 64);
+        ko |= __shfl_xor(ko, o, 64);
     }
-    __syncthreads();
+    const uint32_t diff = ka ^ ko;
+    const int top = diff ? 32 - __clz(diff) : 0;  // bits [top-1, 0] vary
+    uint32_t mask = top >= 32 ? 0u : ~((1u << top) - 1u);
+    uint32_t prefix = ka & mask, want = k;
+    for (int hi = top; hi > 0; hi -= 8) {
+        const int shift = hi > 8 ? hi - 8 : 0;
+        const uint32_t dm = (1u << (hi - shift)) - 1u;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) hist[4u * lane + q] = 0u;
+        pk_wave_lds_order();
+#pragma unroll
+        for (int j = 0; j < TOPK_WAVE_PER; ++j)
+            if (j * 64u + lane < L && (kv[j] & mask) == prefix) atomicAdd(&hist[(kv[j] >> shift) & dm], 1u);
+        pk_wave_lds_order();
+        // lane l owns bins 4l .. 4l+3: exclusive prefix over lanes, then the bin of rank `want`
+        uint32_t h[4], sum = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            h[q] = hist[4u * lane + q];
+            sum += h[q];
+        }
+        uint32_t inc = sum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(inc, o, 64);
+            if (lane >= (uint32_t)o) inc += y;
+        }
+        uint32_t before = inc - sum, bin = 0, rest = 0;
+        const bool mine = sum > 0 && before < want && want <= inc;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (mine && rest == 0 && want <= before + h[q]) {
+                bin = 4u * lane + q;
+                rest = want - before;
+            }
+            before += h[q];
+        }
+        const uint32_t src = (uint32_t)__builtin_ctzll(__ballot(mine));
+        bin = __shfl(bin, src, 64);
+        want = __shfl(rest, src, 64);
+        prefix |= bin << shift;
+        mask |= dm << shift;
+        pk_wave_lds_order();  // every lane has read the histogram before the next pass clears it
+    }
+    const uint32_t kc = prefix;
+    // entries below kc (fewer than k), then ties at kc by column, into buf
+    uint32_t nlt = 0;
+#pragma unroll
+    for (int j = 0; j < TOPK_WAVE_PER; ++j) {
+        const bool lt = j * 64u + lane < L && kv[j] < kc;
+        const uint64_t m = __ballot(lt);
+        if (lt) buf[nlt + (uint32_t)__popcll(m & below)] = ((uint64_t)kv[j] << 32) | (j * 64u + lane);
+        nlt += (uint32_t)__popcll(m);
+    }
+    uint32_t neq = 0;
+#pragma unroll
+    for (int j = 0; j < TOPK_WAVE_PER; ++j) {
+        const bool eq = j * 64u + lane < L && kv[j] == kc;
+        const uint64_t m = __ballot(eq);
+        const uint32_t r = nlt + neq + (uint32_t)__popcll(m & below);
+        if (eq && r < k) buf[r] = ((uint64_t)kc << 32) | (j * 64u + lane);
+        neq += (uint32_t)__popcll(m);
+    }
+    pk_wave_lds_order();
     bool tf = false;
-    for (uint32_t i = t; i < k; i += blockDim.x) {
+    for (uint32_t i = lane; i < k; i += 64u) {
         const uint64_t e = buf[i];
         const uint32_t ek = (uint32_t)(e >> 32);
         uint32_t r = 0, eqa = 0;  // rank; equal keys after it
@@ -187,11 +282,11 @@ __device__ void pk_topk_block_reg(KeyF key, uint32_t L, uint32_t k, float* out_v
         // equal to the next entry of the full order (inside the k, or a tie cut at the k-th)
         if (r < flag_lim && (r + 1 < k ? eqa > 0 : nlt + neq > k)) tf = true;
     }
-    *tie = tf;
+    *tie = __ballot(tf) != 0ull;
 }
 
-#define TOPK_REG_PER 8
-#define TOPK_REG_RANK 256  // largest k placed by rank (larger k: the LDS sort)
+// Per-wave LDS of the wave kernels: histogram (256 words) and k entries.
+#define TOPK_WAVE_LDS (256 * 4 + TOPK_WAVE_K * 8)
 
 __global__ __launch_bounds__(SORT_THREADS) void pk_topk_rows(const float* __restrict__ vals, uint32_t L,
                                                              uint64_t ld_in, uint32_t k, float* __restrict__ out_val,
@@ -225,36 +320,42 @@ __global__ __launch_bounds__(SORT_THREADS) void pk_cell_topk(const float* __rest
     pk_topk_block([&](uint32_t f) { return (cd + st[f]) + fd[f]; }, F, k, out_val + row * k, nullptr, &tie, 0);
 }
 
-__global__ __launch_bounds__(SORT_THREADS) void pk_topk_rows_reg(const float* __restrict__ vals, uint32_t L,
-                                                                 uint64_t ld_in, uint32_t k,
-                                                                 float* __restrict__ out_val,
-                                                                 uint32_t* __restrict__ out_idx,
-                                                                 uint8_t* __restrict__ flags, uint32_t flag_lim) {
-    const uint64_t row = blockIdx.x;
+__global__ __launch_bounds__(SORT_THREADS) void pk_topk_rows_wave(const float* __restrict__ vals, uint32_t rows,
+                                                                  uint32_t L, uint64_t ld_in, uint32_t k,
+                                                                  float* __restrict__ out_val,
+                                                                  uint32_t* __restrict__ out_idx,
+                                                                  uint8_t* __restrict__ flags, uint32_t flag_lim) {
+    __shared__ __align__(16) unsigned char lds[(SORT_THREADS / 64) * TOPK_WAVE_LDS];
+    const uint32_t w = threadIdx.x >> 6;
+    const uint64_t row = (uint64_t)blockIdx.x * (SORT_THREADS / 64) + w;
+    if (row >= rows) return;
+    uint64_t* buf = (uint64_t*)(lds + w * TOPK_WAVE_LDS);
+    uint32_t* hist = (uint32_t*)(lds + w * TOPK_WAVE_LDS + TOPK_WAVE_K * 8);
     const float* v = vals + row * ld_in;
     bool tie = false;
-    pk_topk_block_reg<TOPK_REG_PER>([&](uint32_t i) { return pk_f2ord(v[i]); }, L, k, out_val + row * k,
-                                    out_idx + row * k, &tie, flag_lim);
-    __shared__ uint32_t any;
-    if (threadIdx.x == 0) any = 0;
-    __syncthreads();
-    if (tie) any = 1;
-    __syncthreads();
-    if (threadIdx.x == 0 && any && flags) flags[row] = 1u | flags[row];
+    pk_topk_wave([&](uint32_t i) { return pk_f2ord(v[i]); }, L, k, out_val + row * k, out_idx + row * k, &tie,
+                 flag_lim, hist, buf);
+    if ((threadIdx.x & 63u) == 0 && tie && flags) flags[row] = 1u | flags[row];
 }
 
-__global__ __launch_bounds__(SORT_THREADS) void pk_cell_topk_reg(const float* __restrict__ csd,
-                                                                 const uint32_t* __restrict__ csi, uint32_t cstride,
-                                                                 const float* __restrict__ fdist,
-                                                                 const float* __restrict__ stationary, uint32_t F,
-                                                                 uint32_t k, float* __restrict__ out_val) {
-    const uint64_t row = blockIdx.x;
+__global__ __launch_bounds__(SORT_THREADS) void pk_cell_topk_wave(const float* __restrict__ csd,
+                                                                  const uint32_t* __restrict__ csi, uint32_t cstride,
+                                                                  const float* __restrict__ fdist,
+                                                                  const float* __restrict__ stationary, uint32_t F,
+                                                                  uint32_t k, uint32_t rows,
+                                                                  float* __restrict__ out_val) {
+    __shared__ __align__(16) unsigned char lds[(SORT_THREADS / 64) * TOPK_WAVE_LDS];
+    const uint32_t w = threadIdx.x >> 6;
+    const uint64_t row = (uint64_t)blockIdx.x * (SORT_THREADS / 64) + w;
+    if (row >= rows) return;
+    uint64_t* buf = (uint64_t*)(lds + w * TOPK_WAVE_LDS);
+    uint32_t* hist = (uint32_t*)(lds + w * TOPK_WAVE_LDS + TOPK_WAVE_K * 8);
     const float cd = csd[row * cstride];
     const float* st = stationary + (uint64_t)csi[row * cstride] * F;
     const float* fd = fdist + row * F;
     bool tie = false;
-    pk_topk_block_reg<TOPK_REG_PER>([&](uint32_t f) { return pk_f2ord((cd + st[f]) + fd[f]); }, F, k,
-                                    out_val + row * k, nullptr, &tie, 0);
+    pk_topk_wave([&](uint32_t f) { return pk_f2ord((cd + st[f]) + fd[f]); }, F, k, out_val + row * k, nullptr, &tie,
+                 0, hist, buf);
 }
 
 extern "C" hipError_t pk_launch_topk_rows(const float* vals, uint32_t rows, uint32_t L, uint64_t ld_in, uint32_t k,
@@ -262,9 +363,9 @@ extern "C" hipError_t pk_launch_topk_rows(const float* vals, uint32_t rows, uint
                                           hipStream_t s) {
     if (rows == 0 || k == 0) return hipSuccess;
     if (k > TOPK_CAP || k > L) return hipErrorInvalidValue;
-    if (L <= SORT_THREADS * TOPK_REG_PER && k <= TOPK_REG_RANK)
-        hipLaunchKernelGGL(pk_topk_rows_reg, dim3(rows), dim3(SORT_THREADS), 0, s, vals, L, ld_in, k, out_val,
-                           out_idx, flags, flag_lim);
+    if (L <= 64 * TOPK_WAVE_PER && k <= TOPK_WAVE_K)
+        hipLaunchKernelGGL(pk_topk_rows_wave, dim3((rows + 3) / 4), dim3(SORT_THREADS), 0, s, vals, rows, L, ld_in, k,
+                           out_val, out_idx, flags, flag_lim);
     else
         hipLaunchKernelGGL(pk_topk_rows, dim3(rows), dim3(SORT_THREADS), 0, s, vals, L, ld_in, k, out_val, out_idx,
                            flags, flag_lim);
@@ -276,9 +377,9 @@ extern "C" hipError_t pk_launch_cell_topk(const float* csd, const uint32_t* csi,
                                           float* out_val, hipStream_t s) {
     if (rows == 0 || k == 0) return hipSuccess;
     if (k > TOPK_CAP || k > F) return hipErrorInvalidValue;
-    if (F <= SORT_THREADS * TOPK_REG_PER && k <= TOPK_REG_RANK)
-        hipLaunchKernelGGL(pk_cell_topk_reg, dim3(rows), dim3(SORT_THREADS), 0, s, csd, csi, cstride, fdist,
-                           stationary, F, k, out_val);
+    if (F <= 64 * TOPK_WAVE_PER && k <= TOPK_WAVE_K)
+        hipLaunchKernelGGL(pk_cell_topk_wave, dim3((rows + 3) / 4), dim3(SORT_THREADS), 0, s, csd, csi, cstride, fdist,
+                           stationary, F, k, rows, out_val);
     else
         hipLaunchKernelGGL(pk_cell_topk, dim3(rows), dim3(SORT_THREADS), 0, s, csd, csi, cstride, fdist, stationary,
                            F, k, out_val);
