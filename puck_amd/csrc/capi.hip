@@ -562,7 +562,11 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     // queries that take the walk
     if (nf) {
         PK_CHECK(pk_launch_build_walk_list(nf, m, F, b.wlist, b.wcount, s));
-        PK_CHECK(pk_launch_sort_rows_list(b.fdist, m, F, F, F, b.fsd, b.fsi, b.wlist, b.wcount, s));
+        // on the second stream, ahead of the walk that reads them (the walk's
+        // fork below orders it after this sort and joins it back)
+        PK_CHECK(hipEventRecord(ctx->fork_ev, s));
+        PK_CHECK(hipStreamWaitEvent(ctx->stream2, ctx->fork_ev, 0));
+        PK_CHECK(pk_launch_sort_rows_list(b.fdist, m, F, F, F, b.fsd, b.fsi, b.wlist, b.wcount, ctx->stream2));
     } else {
         PK_CHECK(pk_launch_sort_rows(b.fdist, m, F, F, F, b.fsd, b.fsi, nullptr, 0, s));
     }

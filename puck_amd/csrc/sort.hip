@@ -163,7 +163,8 @@ __device__ void pk_topk_block(ValF val, uint32_t L, uint32_t k, float* out_val, 
 
 // The same selection by one wave per row (rows of at most 64 * TOPK_WAVE_PER
 // entries, k <= TOPK_WAVE_K): lane l holds columns j * 64 + l in registers.
-// A radix select of the k-th smallest key kc over a per-wave LDS histogram,
+// A radix select of the k-th smallest key kc over a per-wave LDS histogram
+// (of the keys that can be among the k smallest, see below),
 // starting at the highest bit in which the row's keys differ (distances of
 // one row usually share their sign and top exponent bits, which would send
 // every key of a first pass to one bin); the entries below kc and then the
@@ -191,9 +192,23 @@ __device__ void pk_topk_wave(KeyF key, uint32_t L, uint32_t k, float* out_val, u
         const uint32_t c = j * 64u + lane;
         kv[j] = key(c < L ? c : L - 1u);
     }
+    // Candidates: with k <= 64 and a key in every lane, the largest lane minimum
+    // T bounds the k-th smallest key from above (each lane has a key <= T), so
+    // only keys <= T enter the histograms (on random columns ~1/8 of the row:
+    // LDS atomics are the select's cost).
+    uint32_t T = 0xffffffffu;
+    if (k <= 64u && L >= 64u) {
+        uint32_t mn = 0xffffffffu;
+#pragma unroll
+        for (int j = 0; j < TOPK_WAVE_PER; ++j)
+            if (j * 64u + lane < L) mn = min(mn, kv[j]);
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) mn = max(mn, (uint32_t)__shfl_xor(mn, o, 64));
+        T = mn;
+    }
 #pragma unroll
     for (int j = 0; j < TOPK_WAVE_PER; ++j) {
-        if (j * 64u + lane < L) {
+        if (j * 64u + lane < L && kv[j] <= T) {
             ka &= kv[j];
             ko |= kv[j];
         }
@@ -215,7 +230,8 @@ __device__ void pk_topk_wave(KeyF key, uint32_t L, uint32_t k, float* out_val, u
         pk_wave_lds_order();
 #pragma unroll
         for (int j = 0; j < TOPK_WAVE_PER; ++j)
-            if (j * 64u + lane < L && (kv[j] & mask) == prefix) atomicAdd(&hist[(kv[j] >> shift) & dm], 1u);
+            if (j * 64u + lane < L && kv[j] <= T && (kv[j] & mask) == prefix)
+                atomicAdd(&hist[(kv[j] >> shift) & dm], 1u);
         pk_wave_lds_order();
         // lane l owns bins 4l .. 4l+3: exclusive prefix over lanes, then the bin of rank `want`
         uint32_t h[4], sum = 0;
