@@ -93,6 +93,8 @@ struct puckgpu_ctx {
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;        // the fallback walk beside the budget scan
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    uint8_t* hpin = nullptr;  // pinned host staging of per-query flags and walk counters
+    size_t hpin_cap = 0;
     std::string err;
     std::vector<DevBuf> bufs;          // named scratch slots
     hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -265,6 +267,7 @@ void puckgpu_destroy(puckgpu_ctx* ctx) {
         if (b.p) (void)hipFree(b.p);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->hpin) (void)hipHostFree(ctx->hpin);
     if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
     if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
     if (ctx->stream2) {
@@ -435,6 +438,19 @@ int puckgpu_index_export(puckgpu_ctx* ctx, const puckgpu_index* ix, float* coars
 
 // ---------------------------------------------------------------------------
 namespace {
+
+// Pinned host memory of at least `bytes` (grown on demand, kept by the context).
+uint8_t* host_pinned(puckgpu_ctx* ctx, size_t bytes) {
+    if (bytes <= ctx->hpin_cap) return ctx->hpin;
+    if (ctx->hpin) (void)hipHostFree(ctx->hpin);
+    ctx->hpin = nullptr;
+    ctx->hpin_cap = 0;
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+    ctx->hpin = (uint8_t*)p;
+    ctx->hpin_cap = bytes;
+    return ctx->hpin;
+}
 
 // Batched search.  `topk` is the largest per-query top-k (the rerank
 // capacity) and `ostride` the output row stride; q_topk / q_scc / q_nf are
@@ -801,8 +817,10 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
     for (int k = 0; k < 5; ++k) ctx->stage_ms[k] = 0;
     ctx->scan_ms = 0;
     for (int k = 0; k < 13; ++k) ctx->counters[k] = 0;
-    std::vector<uint32_t> hstats(4 * (size_t)nb);
-    std::vector<uint8_t> hflags(nb);
+    uint8_t* hpin = host_pinned(ctx, 17 * (size_t)nb);
+    if (!hpin) return set_err(ctx, PUCKGPU_ENOMEM, "pinned host allocation failed");
+    uint32_t* hstats = (uint32_t*)hpin;
+    uint8_t* hflags = hpin + 16 * (size_t)nb;
     std::vector<uint32_t> hlist;
     for (uint32_t q0 = 0; q0 < n; q0 += QB) {
         const uint32_t m = (n - q0) < QB ? (n - q0) : QB;
@@ -814,22 +832,29 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
                           on + q0, pu && pu->budget_out ? pu->budget_out + q0 : nullptr, !device_ptrs, ctx->ev, &wa,
                           &ra, s);
         if (rc) return rc;
-        PK_CHECK(hipMemcpyAsync(hstats.data(), b.stats, sizeof(uint32_t) * 4 * m, hipMemcpyDeviceToHost, s));
-        PK_CHECK(hipMemcpyAsync(hflags.data(), b.flags, m, hipMemcpyDeviceToHost, s));
+        PK_CHECK(hipMemcpyAsync(hstats, b.stats, sizeof(uint32_t) * 4 * m, hipMemcpyDeviceToHost, s));
+        PK_CHECK(hipMemcpyAsync(hflags, b.flags, m, hipMemcpyDeviceToHost, s));
         PK_CHECK(hipStreamSynchronize(s));
-        if (cfg.precision) std::fill(hflags.begin(), hflags.begin() + m, (uint8_t)0);  // no replay in these forms
+        if (cfg.precision) std::fill(hflags, hflags + m, (uint8_t)0);  // no replay in these forms
         for (uint32_t i = 0; i < m; ++i)  // first-pass replay causes
             for (int bb = 0; bb < 8; ++bb)
                 if (hflags[i] & (1u << bb)) ctx->counters[5 + bb]++;
-        const int nrep = replay_pass(ctx, ix, cfg, b, m, hflags.data(), cfg.precision ? 0 : sp->replay_all, wa, ra,
+        const int nrep = replay_pass(ctx, ix, cfg, b, m, hflags, cfg.precision ? 0 : sp->replay_all, wa, ra,
                                      hlist, s);
         if (nrep < 0) return nrep;
-        PK_CHECK(hipEventRecord(ctx->ev[5], s));
-        PK_CHECK(hipStreamSynchronize(s));
-        for (int k = 0; k < 5; ++k) {
+        // stages 0-3 are complete (the flags copy was waited for); the replay
+        // stage is timed only when something was replayed
+        for (int k = 0; k < 4; ++k) {
             float ms = 0;
             PK_CHECK(hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]));
             ctx->stage_ms[k] += ms;
+        }
+        if (nrep > 0) {
+            PK_CHECK(hipEventRecord(ctx->ev[5], s));
+            PK_CHECK(hipEventSynchronize(ctx->ev[5]));
+            float ms = 0;
+            PK_CHECK(hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
+            ctx->stage_ms[4] += ms;
         }
         {
             float ms = 0;
@@ -1318,12 +1343,13 @@ int puckgpu_server_search(puckgpu_server* sv, uint32_t n, const float* queries, 
         if ((rc = bucket_capture(sv, bk, cfg))) return rc;
     }
     PK_CHECK(hipGraphLaunch(bk->exec, s));
-    std::vector<uint8_t> hflags(n);
-    PK_CHECK(hipMemcpyAsync(hflags.data(), bk->b.flags, n, hipMemcpyDeviceToHost, s));
+    uint8_t* hflags = host_pinned(ctx, n);
+    if (!hflags) return set_err(ctx, PUCKGPU_ENOMEM, "pinned host allocation failed");
+    PK_CHECK(hipMemcpyAsync(hflags, bk->b.flags, n, hipMemcpyDeviceToHost, s));
     PK_CHECK(hipStreamSynchronize(s));
     std::vector<uint32_t> hlist;
-    if (cfg.precision) std::fill(hflags.begin(), hflags.end(), (uint8_t)0);
-    const int nrep = replay_pass(ctx, sv->ix, cfg, bk->b, n, hflags.data(), 0, bk->wa, bk->ra, hlist, s);
+    if (cfg.precision) std::fill(hflags, hflags + n, (uint8_t)0);
+    const int nrep = replay_pass(ctx, sv->ix, cfg, bk->b, n, hflags, 0, bk->wa, bk->ra, hlist, s);
     if (nrep < 0) return nrep;
     PK_CHECK(hipMemcpyAsync(distance, bk->od, sizeof(float) * (size_t)n * os, out, s));
     PK_CHECK(hipMemcpyAsync(labels, bk->oi, sizeof(uint32_t) * (size_t)n * os, out, s));

@@ -82,20 +82,64 @@ __global__ void pk_prep_queries(const float* __restrict__ qin, uint32_t nq, uint
     pk_prep_one(qin + (uint64_t)i * dim, y, dim, whether_norm, qn + i, status + i);
 }
 
-// The same for dim <= PREP_LDS_DIM: 64 queries per block staged through LDS
-// (rows of dim + 1 words, conflict-free), so the loads and stores coalesce.
+// The same for dim <= PREP_LDS_DIM: 64 queries per block of 256 threads,
+// staged through LDS (rows of dim + 1 words, conflict-free): every thread
+// issues its share of the loads (16-byte when vec4: dim % 4 == 0 and the
+// input 16-byte aligned) before storing
+// any, threads 0..63 run the per-query arithmetic, and the rows go back
+// coalesced.
 #define PREP_LDS_DIM 255
-__global__ __launch_bounds__(64) void pk_prep_queries_lds(const float* __restrict__ qin, uint32_t nq, uint32_t dim,
-                                                          int whether_norm, float* __restrict__ qout,
-                                                          float* __restrict__ qn, int32_t* __restrict__ status,
-                                                          const uint32_t* __restrict__ nvalid,
-                                                          uint8_t* __restrict__ flags) {
+#define PREP_ROWS 64
+__global__ __launch_bounds__(256) void pk_prep_queries_lds(const float* __restrict__ qin, uint32_t nq, uint32_t dim,
+                                                           int whether_norm, float* __restrict__ qout,
+                                                           float* __restrict__ qn, int32_t* __restrict__ status,
+                                                           const uint32_t* __restrict__ nvalid,
+                                                           uint8_t* __restrict__ flags, int vec4) {
     extern __shared__ float tile[];
-    const uint32_t q0 = blockIdx.x * 64u, nrow = nq - q0 < 64u ? nq - q0 : 64u, ld = dim + 1u;
-    const uint32_t t = threadIdx.x;
-    for (uint32_t e = t; e < nrow * dim; e += 64u) {
-        const uint32_t r = e / dim;
-        tile[r * ld + (e - r * dim)] = qin[(uint64_t)q0 * dim + e];
+    const uint32_t q0 = blockIdx.x * PREP_ROWS, nrow = nq - q0 < PREP_ROWS ? nq - q0 : PREP_ROWS;
+    const uint32_t ld = dim + 1u, t = threadIdx.x, nt = blockDim.x;
+    const uint32_t total = nrow * dim;
+    const float* src = qin + (uint64_t)q0 * dim;
+    float* dst = qout + (uint64_t)q0 * dim;
+    if (vec4) {
+        const uint32_t t4 = total >> 2;  // float4s of the block's rows
+        for (uint32_t e0 = t; e0 < t4; e0 += 8u * nt) {
+            float4 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const uint32_t e = e0 + u * nt;
+                v[u] = ((const float4*)src)[e < t4 ? e : t4 - 1u];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const uint32_t e = (e0 + u * nt) * 4u;
+                if (e < total) {
+                    const uint32_t r = e / dim, k = e - r * dim;  // 4 | dim: the float4 stays in one row
+                    float* row = tile + r * ld + k;
+                    row[0] = v[u].x;
+                    row[1] = v[u].y;
+                    row[2] = v[u].z;
+                    row[3] = v[u].w;
+                }
+            }
+        }
+    } else {
+        for (uint32_t e0 = t; e0 < total; e0 += 8u * nt) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const uint32_t e = e0 + u * nt;
+                v[u] = src[e < total ? e : total - 1u];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const uint32_t e = e0 + u * nt;
+                if (e < total) {
+                    const uint32_t r = e / dim;
+                    tile[r * ld + (e - r * dim)] = v[u];
+                }
+            }
+        }
     }
     __syncthreads();
     if (t < nrow) {
@@ -111,9 +155,9 @@ __global__ __launch_bounds__(64) void pk_prep_queries_lds(const float* __restric
         }
     }
     __syncthreads();
-    for (uint32_t e = t; e < nrow * dim; e += 64u) {
+    for (uint32_t e = t; e < total; e += nt) {
         const uint32_t r = e / dim;
-        qout[(uint64_t)q0 * dim + e] = tile[r * ld + (e - r * dim)];
+        dst[e] = tile[r * ld + (e - r * dim)];
     }
 }
 
@@ -122,8 +166,9 @@ extern "C" hipError_t pk_launch_prep_queries_ex(const float* qin, uint32_t nq, u
                                                 uint8_t* flags, hipStream_t s) {
     if (nq == 0) return hipSuccess;
     if (dim <= PREP_LDS_DIM)
-        hipLaunchKernelGGL(pk_prep_queries_lds, dim3((nq + 63) / 64), dim3(64), sizeof(float) * 64 * (dim + 1), s,
-                           qin, nq, dim, whether_norm, qout, qn, status, nvalid, flags);
+        hipLaunchKernelGGL(pk_prep_queries_lds, dim3((nq + PREP_ROWS - 1) / PREP_ROWS), dim3(256),
+                           sizeof(float) * PREP_ROWS * (dim + 1), s, qin, nq, dim, whether_norm, qout, qn, status,
+                           nvalid, flags, (dim % 4 == 0 && ((uintptr_t)qin & 15) == 0) ? 1 : 0);
     else
         hipLaunchKernelGGL(pk_prep_queries, dim3((nq + 127) / 128), dim3(128), 0, s, qin, nq, dim, whether_norm,
                            qout, qn, status, nvalid, flags);
