@@ -140,6 +140,46 @@ def test_probe_features_match_oracle(ctx, idx_cos):
         assert np.array_equal(np.sort(fo[k]), np.arange(40))
 
 
+@pytest.fixture(scope="module")
+def idx_mid():
+    # integer data (exact distance ties) with 300 coarse and 120 fine clusters:
+    # rows of 64 keys or more go through the candidate bound of the row top-k
+    x = DS.mixture_uint8_grid(20000, 32, 300, seed=70, means_seed=71)
+    q = DS.mixture_uint8_grid(48, 32, 300, seed=72, means_seed=71)
+    ix = CI.build_index(x, 300, 120, 16, whether_norm=False, train_n=8000)
+    return ix, q
+
+
+def test_probe_features_rows_over_64(ctx, idx_mid):
+    ix, q = idx_mid
+    gi = _gpu_index(ctx, ix)
+    C, F, d = 300, 120, 32
+    for nc, nf in [(64, 64), (1, 1), (200, 100)]:
+        pr = _probe(ctx, np.zeros(PM.param_count(d + nc + nf, 16), np.float32), nc=nc, nf=nf, dim=d, hidden=16)
+        f = pr.features(gi, q)
+        o = CI.oracle_index(ix).export()
+        for k in range(q.shape[0]):
+            cd = np.array([o["coarse_norms"][c] - O.dot(ix["coarse"][c], q[k]) for c in range(C)], np.float32)
+            fd = np.array([-O.dot(ix["fine"][j], q[k]) for j in range(F)], np.float32)
+            assert np.array_equal(f[k, d:d + nc].view(np.uint32), np.sort(cd)[:nc].view(np.uint32)), (nc, k)
+            c0 = int(np.argmin(cd))
+            cell = (cd[c0] + o["stationary"].reshape(C, F)[c0]) + fd
+            assert np.array_equal(f[k, d + nc:].view(np.uint32), np.sort(cell)[:nf].view(np.uint32)), (nf, k)
+
+
+@pytest.mark.parametrize("scc,ftk,topk", [(1, 100, 10), (16, 300, 10), (64, 500, 20)])
+def test_search_rows_over_64_bitexact(ctx, idx_mid, scc, ftk, topk):
+    ix, q = idx_mid
+    gi = _gpu_index(ctx, ix)
+    oi = CI.oracle_index(ix)
+    Dg, Ig, Rg = gi.search(q, topk, scc, ftk)
+    for k in range(q.shape[0]):
+        o = oi.search_one(q[k], topk, scc, ftk)
+        assert Rg[k] == o["n"], k
+        assert np.array_equal(Ig[k, :o["n"]], o["ids"][:o["n"]]), k
+        assert np.array_equal(Dg[k, :o["n"]].view(np.uint32), o["dist"][:o["n"]].view(np.uint32)), k
+
+
 @pytest.mark.parametrize("margin,frac", [(0.0, 1.0), (0.7, 0.5), (-1.0, 1.0)])
 def test_search_learned_equals_explicit_budgets(ctx, idx_cos, margin, frac):
     from puck_amd.engine import search_learned
