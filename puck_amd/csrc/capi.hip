@@ -56,6 +56,7 @@ hipError_t pk_probe_adam_step(float*, float*, float*, const float*, uint16_t*, u
 hipError_t pk_launch_probe_budget(const float*, uint32_t, float, uint32_t, float, uint32_t, uint32_t*,
                                   hipStream_t);
 hipError_t pk_launch_to_bf16(const float*, uint32_t, uint16_t*, hipStream_t);
+hipError_t pk_launch_build_query_order(const uint32_t*, uint32_t, uint32_t, uint32_t, uint32_t*, hipStream_t);
 hipError_t pk_launch_build_walk_list(const uint32_t*, uint32_t, uint32_t, uint32_t*, uint32_t*, hipStream_t);
 hipError_t pk_launch_budget_rank(const float*, const uint32_t*, uint32_t, uint32_t, const float*, const float*,
                                  uint32_t, uint32_t, const uint32_t*, uint32_t, uint32_t, uint32_t*, hipStream_t);
@@ -92,7 +93,7 @@ struct puckgpu_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;        // the fallback walk beside the budget scan
-    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr, order_ev = nullptr;
     uint8_t* hpin = nullptr;  // pinned host staging of per-query flags and walk counters
     size_t hpin_cap = 0;
     std::string err;
@@ -251,7 +252,8 @@ int puckgpu_create(int device, puckgpu_ctx** out) {
         }
     if (hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->order_ev, hipEventDisableTiming) != hipSuccess) {
         delete ctx;
         return PUCKGPU_EHIP;
     }
@@ -270,6 +272,7 @@ void puckgpu_destroy(puckgpu_ctx* ctx) {
     if (ctx->hpin) (void)hipHostFree(ctx->hpin);
     if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
     if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
+    if (ctx->order_ev) (void)hipEventDestroy(ctx->order_ev);
     if (ctx->stream2) {
         (void)hipStreamSynchronize(ctx->stream2);
         (void)hipStreamDestroy(ctx->stream2);
@@ -509,6 +512,7 @@ struct PassBufs {
     float* cand_d = nullptr;   // approximate coarse candidates before rescoring
     uint32_t* cand_i = nullptr;
     uint32_t* wlist = nullptr;   // queries taking the walk when budgets are given
+    uint32_t* qorder = nullptr;  // workgroup -> query order of the per-query kernels
     uint32_t* wcount = nullptr;
 };
 
@@ -560,6 +564,17 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
         PK_CHECK(pkc_launch_rescore_coarse(b.q, d, ix->coarse, ix->coarse_norms, b.cand_i, cfg.kc, cfg.keep, m, b.csd,
                                            b.csi, b.flags, cfg.scc, s));
     }
+    // large batches: the per-query kernels take the queries grouped by nearest
+    // coarse list, dealt to the XCDs in slices (shared records stay in one L2)
+    // (on the second stream, beside the probe or the fine sort; joined before the walk)
+    const uint32_t* order = nullptr;
+    if (m >= 2048 && cfg.keep) {
+        PK_CHECK(hipEventRecord(ctx->fork_ev, s));
+        PK_CHECK(hipStreamWaitEvent(ctx->stream2, ctx->fork_ev, 0));
+        PK_CHECK(pk_launch_build_query_order(b.csi, cfg.keep, m, C, b.qorder, ctx->stream2));
+        PK_CHECK(hipEventRecord(ctx->order_ev, ctx->stream2));
+        order = b.qorder;
+    }
     const uint32_t* nf = q_nf;
     if (pr) {
         // learned fine budget: features -> MLP -> budget, on this stream
@@ -586,13 +601,14 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     } else {
         PK_CHECK(pk_launch_sort_rows(b.fdist, m, F, F, F, b.fsd, b.fsi, nullptr, 0, s));
     }
+    if (order) PK_CHECK(hipStreamWaitEvent(s, ctx->order_ev, 0));
     if (ev) PK_CHECK(hipEventRecord(ev[2], s));
     PkWalkArgs wa{};
     wa.q = b.q; wa.qn = b.qn; wa.qstatus = b.qst; wa.csel_d = b.csd; wa.csel_id = b.csi; wa.fsort_d = b.fsd;
     wa.fsort_id = b.fsi; wa.codebook = ix->codebook; wa.stationary = ix->stationary;
     wa.cell_start = ix->cell_start; wa.mdo = ix->mdo; wa.rec_off = ix->rec_off; wa.rec_codes = ix->rec_codes;
     wa.dim = d; wa.C = C; wa.F = F; wa.nsq = ix->p.filter_nsq; wa.lsq = ix->lsq; wa.scc = cfg.scc; wa.ftk = cfg.ftk;
-    wa.rr = cfg.rr; wa.out_t = ft; wa.out_id = fi; wa.out_n = fn; wa.stats = b.stats; wa.qlist = nullptr;
+    wa.rr = cfg.rr; wa.out_t = ft; wa.out_id = fi; wa.out_n = fn; wa.stats = b.stats; wa.qlist = order;
     wa.flags = b.flags; wa.exact = 0; wa.tie_ids = b.tie_ids; wa.tie_n = b.tie_n;
     wa.q_scc = q_scc; wa.q_nf = nf; wa.cstride = cfg.keep ? cfg.keep : 1; wa.fdist = b.fdist;
     wa.wlist = nullptr; wa.wcount = nullptr;
@@ -609,7 +625,7 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     ra.q = b.q; ra.qstatus = b.qst; ra.filt_t = ft; ra.filt_id = fi; ra.filt_n = fn;
     ra.raw = ix->raw; ra.mem2local = ix->mem2local; ra.dim = d; ra.ftk = cfg.ftk; ra.topk = cfg.topk;
     ra.out_d = od; ra.out_id = oi; ra.out_n = on;
-    ra.qlist = nullptr; ra.flags = b.flags; ra.exact = 0; ra.tie_ids = b.tie_ids; ra.tie_n = b.tie_n;
+    ra.qlist = order; ra.flags = b.flags; ra.exact = 0; ra.tie_ids = b.tie_ids; ra.tie_n = b.tie_n;
     ra.q_topk = q_topk; ra.out_stride = cfg.ostride; ra.q_nf = nf; ra.F = F;
     PK_CHECK(pk_launch_rerank(&ra, m, s));
     if (ev) PK_CHECK(hipEventRecord(ev[4], s));
@@ -693,8 +709,9 @@ int scratch_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     b.tie_ids = (uint32_t*)scratch(ctx, S_TIEID, sizeof(uint32_t) * 16 * cap, &rc);
     b.tie_n = (uint8_t*)scratch(ctx, S_TIEN, cap, &rc);
     b.stats = (uint32_t*)scratch(ctx, S_STATS, sizeof(uint32_t) * 4 * cap, &rc);
-    b.wlist = (uint32_t*)scratch(ctx, S_WLIST, sizeof(uint32_t) * (cap + 1), &rc);
-    b.wcount = b.wlist ? b.wlist + cap : nullptr;
+    b.wlist = (uint32_t*)scratch(ctx, S_WLIST, sizeof(uint32_t) * (2 * (size_t)cap + 1), &rc);
+    b.qorder = b.wlist ? b.wlist + cap : nullptr;
+    b.wcount = b.wlist ? b.wlist + 2 * (size_t)cap : nullptr;
     if (pr) {
         b.pfeat = (float*)scratch(ctx, S_PFEAT, sizeof(float) * cap * pr->D, &rc);
         b.pwork = scratch(ctx, S_PWORK, pk_probe_work_bytes(cap, pr->D, pr->H), &rc);
@@ -1182,7 +1199,7 @@ int bucket_create(puckgpu_server* sv, uint32_t cap, const PassCfg& cfg, PkBucket
         (rc = balloc(ctx, bk, &b.fsi, (size_t)cap * F)) || (rc = balloc(ctx, bk, &b.flags, cap)) ||
         (rc = balloc(ctx, bk, &b.tie_ids, (size_t)cap * 16)) || (rc = balloc(ctx, bk, &b.tie_n, cap)) ||
         (rc = balloc(ctx, bk, &b.stats, (size_t)cap * 4)) || (rc = balloc(ctx, bk, &b.qlist, cap)) ||
-        (rc = balloc(ctx, bk, &b.wlist, (size_t)cap + 1)) ||
+        (rc = balloc(ctx, bk, &b.wlist, 2 * (size_t)cap + 1)) ||
         (rc = balloc(ctx, bk, &bk->qin, (size_t)cap * d)) || (rc = balloc(ctx, bk, &bk->qtopk, cap)) ||
         (rc = balloc(ctx, bk, &bk->qscc, cap)) || (rc = balloc(ctx, bk, &bk->nvalid, 1)) ||
         (rc = balloc(ctx, bk, &bk->ft, (size_t)cap * ftk)) || (rc = balloc(ctx, bk, &bk->fi, (size_t)cap * ftk)) ||
@@ -1200,7 +1217,8 @@ int bucket_create(puckgpu_server* sv, uint32_t cap, const PassCfg& cfg, PkBucket
             return rc;
         }
     }
-    b.wcount = b.wlist + cap;
+    b.qorder = b.wlist + cap;
+    b.wcount = b.wlist + 2 * (size_t)cap;
     if (sv->probe) {
         const puckgpu_probe* pr = sv->probe;
         void* w = nullptr;

@@ -1376,6 +1376,61 @@ __global__ __launch_bounds__(1024) void pk_build_walk_list(const uint32_t* __res
     if (threadIdx.x == 0) *count = base;
 }
 
+// Workgroup -> query order for the per-query kernels (walk, budget scan,
+// rerank): the queries grouped by their nearest coarse list (counting sort
+// over csi[q * cstride]) and the sorted order dealt to the 8 XCDs in
+// contiguous slices (workgroup b is dispatched to XCD b % 8), so that the
+// queries reading one list's records and rows run on one L2, close in time.
+// Each query is still processed once, by the same code: results do not
+// depend on the order.  One workgroup of 1024 threads; C words of LDS.
+#define PK_XCDS 8u
+__global__ __launch_bounds__(1024) void pk_build_query_order(const uint32_t* __restrict__ csi, uint32_t cstride,
+                                                            uint32_t m, uint32_t C, uint32_t* __restrict__ order) {
+    extern __shared__ uint32_t bin[];  // C counters, then bin starts
+    __shared__ uint32_t tmp[20];
+    const uint32_t t = threadIdx.x, nt = blockDim.x;
+    for (uint32_t i = t; i < C; i += nt) bin[i] = 0;
+    __syncthreads();
+    for (uint32_t q = t; q < m; q += nt) {
+        const uint32_t c = csi[(uint64_t)q * cstride];
+        atomicAdd(&bin[c < C ? c : C - 1u], 1u);
+    }
+    __syncthreads();
+    // exclusive prefix over the bins: thread t owns bins [t*per, t*per + per)
+    const uint32_t per = (C + nt - 1u) / nt, b0 = t * per;
+    uint32_t sum = 0;
+    for (uint32_t i = b0; i < b0 + per && i < C; ++i) sum += bin[i];
+    uint32_t tot;
+    uint32_t run = pk_block_exscan(sum, tmp, &tot);
+    for (uint32_t i = b0; i < b0 + per && i < C; ++i) {
+        const uint32_t c = bin[i];
+        bin[i] = run;
+        run += c;
+    }
+    __syncthreads();
+    // sorted position -> workgroup: XCD x takes positions [start_x, start_x + n_x),
+    // n_x = ceil((m - x) / 8) workgroups x, x + 8, x + 16, ...
+    for (uint32_t q = t; q < m; q += nt) {
+        const uint32_t c = csi[(uint64_t)q * cstride];
+        const uint32_t pos = atomicAdd(&bin[c < C ? c : C - 1u], 1u);
+        uint32_t x = 0, start = 0;
+        for (; x + 1 < PK_XCDS; ++x) {
+            const uint32_t nx = m > x ? (m - x + PK_XCDS - 1u) / PK_XCDS : 0u;
+            if (pos < start + nx) break;
+            start += nx;
+        }
+        order[x + PK_XCDS * (pos - start)] = q;
+    }
+}
+
+extern "C" hipError_t pk_launch_build_query_order(const uint32_t* csi, uint32_t cstride, uint32_t m, uint32_t C,
+                                                  uint32_t* order, hipStream_t s) {
+    if (m == 0) return hipSuccess;
+    if (C == 0 || C > 8192) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pk_build_query_order, dim3(1), dim3(1024), sizeof(uint32_t) * C, s, csi, cstride, m, C, order);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t pk_launch_build_walk_list(const uint32_t* q_nf, uint32_t n, uint32_t F, uint32_t* list,
                                                 uint32_t* count, hipStream_t s) {
     hipLaunchKernelGGL(pk_build_walk_list, dim3(1), dim3(1024), 0, s, q_nf, n, F, list, count);

@@ -239,6 +239,31 @@ def test_search_tied_centers_bitexact(ctx, idx_ties, scc, ftk, topk):
         assert np.array_equal(_bits(Dg[k, :o["n"]]), _bits(o["dist"][:o["n"]])), k
 
 
+def test_large_batch_query_order(ctx, idx_cos):
+    # batches of 2048 queries or more run the per-query kernels in an order
+    # grouped by nearest coarse list: results equal the same queries in small
+    # batches (natural order) and the oracle
+    ix, _ = idx_cos
+    gi = _gpu_index(ctx, ix)
+    q = D.unit_norm_mixture(2500, 96, 300, seed=23, means_seed=21)
+    rng = np.random.default_rng(41)
+    nf = rng.integers(0, 44, q.shape[0]).astype(np.uint32)
+    for budgets in (None, nf):
+        Dg, Ig, Rg = gi.search_batch(q, 10, 4, 300, fine_budget=budgets)
+        for s0 in range(0, q.shape[0], 500):
+            Ds, Is, Rs = gi.search_batch(q[s0:s0 + 500], 10, 4, 300,
+                                         fine_budget=None if budgets is None else budgets[s0:s0 + 500])
+            assert np.array_equal(Rg[s0:s0 + 500], Rs)
+            assert np.array_equal(Ig[s0:s0 + 500], Is)
+            assert np.array_equal(_bits(Dg[s0:s0 + 500]), _bits(Ds))
+    oi = CI.oracle_index(ix)
+    Dg, Ig, Rg = gi.search(q, 10, 4, 300)
+    for k in range(0, q.shape[0], 97):
+        o = oi.search_one(q[k], 10, 4, 300)
+        assert Rg[k] == o["n"] and np.array_equal(Ig[k, :o["n"]], o["ids"][:o["n"]]), k
+        assert np.array_equal(_bits(Dg[k, :o["n"]]), _bits(o["dist"][:o["n"]])), k
+
+
 def test_search_batch_device_arrays(ctx, idx_cos):
     import torch
     ix, q = idx_cos
