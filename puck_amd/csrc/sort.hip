@@ -178,20 +178,18 @@ __device__ __forceinline__ void pk_wave_lds_order() {
     __builtin_amdgcn_wave_barrier();
 }
 
-template <typename KeyF>
-__device__ void pk_topk_wave(KeyF key, uint32_t L, uint32_t k, float* out_val, uint32_t* out_idx, bool* tie,
+// load(kv) fills kv[j] with the key of column min(j * 64 + lane, L - 1) for
+// every j, issuing all its loads before combining any (a load under a lane
+// condition, or one whose value is used at once, is waited for before the
+// next is issued).
+template <typename LoadF>
+__device__ void pk_topk_wave(LoadF load, uint32_t L, uint32_t k, float* out_val, uint32_t* out_idx, bool* tie,
                              uint32_t flag_lim, uint32_t* hist, uint64_t* buf) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t below = (1ull << lane) - 1ull;
     uint32_t kv[TOPK_WAVE_PER];
     uint32_t ka = 0xffffffffu, ko = 0u;
-    // every load at an in-bounds column (a load under a lane condition is
-    // waited for before the next one is issued)
-#pragma unroll
-    for (int j = 0; j < TOPK_WAVE_PER; ++j) {
-        const uint32_t c = j * 64u + lane;
-        kv[j] = key(c < L ? c : L - 1u);
-    }
+    load(kv);
     // Candidates: with k <= 64 and a key in every lane, the largest lane minimum
     // T bounds the k-th smallest key from above (each lane has a key <= T), so
     // only keys <= T enter the histograms (on random columns ~1/8 of the row:
@@ -349,8 +347,16 @@ __global__ __launch_bounds__(SORT_THREADS) void pk_topk_rows_wave(const float* _
     uint32_t* hist = (uint32_t*)(lds + w * TOPK_WAVE_LDS + TOPK_WAVE_K * 8);
     const float* v = vals + row * ld_in;
     bool tie = false;
-    pk_topk_wave([&](uint32_t i) { return pk_f2ord(v[i]); }, L, k, out_val + row * k, out_idx + row * k, &tie,
-                 flag_lim, hist, buf);
+    const uint32_t lane = threadIdx.x & 63u;
+    pk_topk_wave(
+        [&](uint32_t(&kv)[TOPK_WAVE_PER]) {
+            float x[TOPK_WAVE_PER];
+#pragma unroll
+            for (int j = 0; j < TOPK_WAVE_PER; ++j) x[j] = v[min(j * 64u + lane, L - 1u)];
+#pragma unroll
+            for (int j = 0; j < TOPK_WAVE_PER; ++j) kv[j] = pk_f2ord(x[j]);
+        },
+        L, k, out_val + row * k, out_idx + row * k, &tie, flag_lim, hist, buf);
     if ((threadIdx.x & 63u) == 0 && tie && flags) flags[row] = 1u | flags[row];
 }
 
@@ -370,8 +376,18 @@ __global__ __launch_bounds__(SORT_THREADS) void pk_cell_topk_wave(const float* _
     const float* st = stationary + (uint64_t)csi[row * cstride] * F;
     const float* fd = fdist + row * F;
     bool tie = false;
-    pk_topk_wave([&](uint32_t f) { return pk_f2ord((cd + st[f]) + fd[f]); }, F, k, out_val + row * k, nullptr, &tie,
-                 0, hist, buf);
+    const uint32_t lane = threadIdx.x & 63u;
+    pk_topk_wave(
+        [&](uint32_t(&kv)[TOPK_WAVE_PER]) {
+            float x[TOPK_WAVE_PER], y[TOPK_WAVE_PER];
+#pragma unroll
+            for (int j = 0; j < TOPK_WAVE_PER; ++j) x[j] = st[min(j * 64u + lane, F - 1u)];
+#pragma unroll
+            for (int j = 0; j < TOPK_WAVE_PER; ++j) y[j] = fd[min(j * 64u + lane, F - 1u)];
+#pragma unroll
+            for (int j = 0; j < TOPK_WAVE_PER; ++j) kv[j] = pk_f2ord((cd + x[j]) + y[j]);
+        },
+        F, k, out_val + row * k, nullptr, &tie, 0, hist, buf);
 }
 
 extern "C" hipError_t pk_launch_topk_rows(const float* vals, uint32_t rows, uint32_t L, uint64_t ld_in, uint32_t k,
