@@ -70,22 +70,40 @@ __global__ __launch_bounds__(256) void pk_mm_bf16(PkMmArgs a) {
     pk_v16f acc;
     for (int r = 0; r < 16; ++r) acc[r] = 0.f;
     for (uint32_t k0 = 0; k0 < a.K; k0 += MM_KT) {
-        // stage A and B tiles as [row][k]
-        for (int e = tid; e < MM_T * MM_KT; e += 256) {
+        // stage A and B tiles as [row][k]; every load at an in-bounds index
+        // (clamped), then zeroed outside the matrix: loads under a lane
+        // condition are waited for one at a time
+        constexpr int PT = MM_T * MM_KT / 256;  // elements per thread and tile
+        uint16_t va[PT], vb[PT];
+#pragma unroll
+        for (int u = 0; u < PT; ++u) {
+            const int e = tid + u * 256;
             int r, k;
             if (AT) { r = e % MM_T; k = e / MM_T; } else { r = e / MM_KT; k = e % MM_KT; }
-            const uint32_t gm = m0 + r, gk = k0 + k;
-            uint16_t v = 0;
-            if (gm < a.M && gk < a.K) v = AT ? a.A[(uint64_t)gk * a.lda + gm] : a.A[(uint64_t)gm * a.lda + gk];
-            As[r * MM_LDS_STRIDE + k] = v;
+            const uint32_t gm = min(m0 + r, a.M - 1u), gk = min(k0 + k, a.K - 1u);
+            va[u] = AT ? a.A[(uint64_t)gk * a.lda + gm] : a.A[(uint64_t)gm * a.lda + gk];
         }
-        for (int e = tid; e < MM_T * MM_KT; e += 256) {
+#pragma unroll
+        for (int u = 0; u < PT; ++u) {
+            const int e = tid + u * 256;
             int r, k;
             if (BT) { r = e % MM_T; k = e / MM_T; } else { r = e / MM_KT; k = e % MM_KT; }
-            const uint32_t gn = n0 + r, gk = k0 + k;
-            uint16_t v = 0;
-            if (gn < a.N && gk < a.K) v = BT ? a.B[(uint64_t)gk * a.ldb + gn] : a.B[(uint64_t)gn * a.ldb + gk];
-            Bs[r * MM_LDS_STRIDE + k] = v;
+            const uint32_t gn = min(n0 + r, a.N - 1u), gk = min(k0 + k, a.K - 1u);
+            vb[u] = BT ? a.B[(uint64_t)gk * a.ldb + gn] : a.B[(uint64_t)gn * a.ldb + gk];
+        }
+#pragma unroll
+        for (int u = 0; u < PT; ++u) {
+            const int e = tid + u * 256;
+            int r, k;
+            if (AT) { r = e % MM_T; k = e / MM_T; } else { r = e / MM_KT; k = e % MM_KT; }
+            As[r * MM_LDS_STRIDE + k] = (m0 + r < a.M && k0 + k < a.K) ? va[u] : (uint16_t)0;
+        }
+#pragma unroll
+        for (int u = 0; u < PT; ++u) {
+            const int e = tid + u * 256;
+            int r, k;
+            if (BT) { r = e % MM_T; k = e / MM_T; } else { r = e / MM_KT; k = e % MM_KT; }
+            Bs[r * MM_LDS_STRIDE + k] = (n0 + r < a.N && k0 + k < a.K) ? vb[u] : (uint16_t)0;
         }
         __syncthreads();
 #pragma unroll
