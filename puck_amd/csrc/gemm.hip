@@ -34,16 +34,30 @@ __global__ __launch_bounds__(256) void pk_gemm_exact(const float* __restrict__ A
 #pragma unroll
         for (int j = 0; j < RN; ++j) acc[i][j] = 0.0f;
 
+    constexpr int PA = TM * GB_K / 256, PB = TN * GB_K / 256;  // tile elements per thread
     for (uint32_t k0 = 0; k0 < K; k0 += GB_K) {
-        for (int idx = tid; idx < TM * GB_K; idx += 256) {
-            const int r = idx >> 4, kk = idx & 15;
-            const uint32_t gm = m0 + r, gk = k0 + kk;
-            As[kk][r] = (gm < M && gk < K) ? A[(uint64_t)gm * K + gk] : 0.0f;
+        // every load at a clamped in-bounds index, issued before any is used
+        // (loads under a lane condition are waited for one at a time)
+        float va[PA], vb[PB];
+#pragma unroll
+        for (int u = 0; u < PA; ++u) {
+            const int idx = tid + u * 256, r = idx >> 4, kk = idx & 15;
+            va[u] = A[(uint64_t)min(m0 + r, M - 1u) * K + min(k0 + kk, K - 1u)];
         }
-        for (int idx = tid; idx < TN * GB_K; idx += 256) {
-            const int r = idx >> 4, kk = idx & 15;
-            const uint32_t gn = n0 + r, gk = k0 + kk;
-            Bs[kk][r] = (gn < N && gk < K) ? B[(uint64_t)gn * K + gk] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < PB; ++u) {
+            const int idx = tid + u * 256, r = idx >> 4, kk = idx & 15;
+            vb[u] = B[(uint64_t)min(n0 + r, N - 1u) * K + min(k0 + kk, K - 1u)];
+        }
+#pragma unroll
+        for (int u = 0; u < PA; ++u) {
+            const int idx = tid + u * 256, r = idx >> 4, kk = idx & 15;
+            As[kk][r] = (m0 + r < M && k0 + kk < K) ? va[u] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < PB; ++u) {
+            const int idx = tid + u * 256, r = idx >> 4, kk = idx & 15;
+            Bs[kk][r] = (n0 + r < N && k0 + kk < K) ? vb[u] : 0.0f;
         }
         __syncthreads();
         const uint32_t kend = (K - k0) < GB_K ? (K - k0) : GB_K;
