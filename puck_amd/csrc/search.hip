@@ -1664,7 +1664,7 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
         const int src = (int)(threadIdx.x & 63u & ~7u);
         const uint32_t dim = a.dim, nfull = dim >> 4;
         for (uint32_t r0 = 0; r0 < n2; r0 += RP * ng) {
-            uint32_t rr[RP], mid[RP];
+            uint32_t rr[RP], mid[RP], lid[RP];
             const float* x[RP];
             float t[RP];
 #pragma unroll
@@ -1676,6 +1676,8 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
                 x[h] = a.raw + (uint64_t)mid[h] * dim;
                 t[h] = 0.0f;
             }
+#pragma unroll
+            for (int h = 0; h < RP; ++h) lid[h] = bud ? a.mem2local[mid[h]] : 0u;  // with the row loads
             for (uint32_t s0 = 0; s0 < nfull; s0 += SB) {
                 float v0[RP][SB], v1[RP][SB];
 #pragma unroll
@@ -1710,7 +1712,7 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
                 if (j == 0 && r < n2) {
                     uint64_t key = ~0ull;
                     if (r < n) {
-                        key = ((uint64_t)pk_f2ord(t[h]) << 32) | (bud ? a.mem2local[mid[h]] : r);
+                        key = ((uint64_t)pk_f2ord(t[h]) << 32) | (bud ? lid[h] : r);
                         if (exact) dist[r] = t[h];
                     }
                     if (!exact) keys[r] = key;
