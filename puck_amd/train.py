@@ -6,11 +6,27 @@ coarse k-means, fine k-means on the coarse residuals, fine centers stored by
 descending norm) and PuckIndex::train (puck_index.cpp:764-874: per-subspace
 256-center k-means on the cell residuals).  The Lloyd assignment step of every
 k-means run is the engine's GEMM + argmin kernel (puckgpu_kmeans_assign, the
-nearest_center formula of kmeans.cpp:221-258); the centroid means are torch
-reductions.  The reference seeds its sampling and initialisation with time(0),
+nearest_center formula of kmeans.cpp:221-258); the centroid means are
+fixed-order torch reductions, so a build is reproducible run to run.  The reference seeds its sampling and initialisation with time(0),
 so trained codebooks are not reproducible against it; these are seeded.
 """
 import torch
+
+
+def cluster_sums(x, a, k):
+    """Per-cluster sums of the rows of x [n, d] (assignments a [n], long) and
+    counts, in a fixed order: rows stably sorted by cluster, a float64 prefix
+    sum, differences at the segment ends.  (index_add_ on a CUDA tensor adds
+    with float atomics in no fixed order, so trained centers, and with them
+    the index and its exact ties, would differ from run to run.)"""
+    n, d = x.shape
+    cnt = torch.bincount(a, minlength=k)
+    order = torch.argsort(a, stable=True)
+    cs = torch.cumsum(x[order].double(), 0)
+    pad = torch.cat([torch.zeros((1, d), dtype=torch.float64, device=x.device), cs])
+    ends = torch.cumsum(cnt, 0)
+    sums = pad[ends] - pad[ends - cnt]
+    return sums, cnt
 
 
 def kmeans(ctx, x, k, niter=10, seed=0):
@@ -23,10 +39,9 @@ def kmeans(ctx, x, k, niter=10, seed=0):
     c = x[torch.randperm(n, generator=g, device=x.device)[:k]].clone().contiguous()
     for _ in range(niter):
         a = ctx.kmeans_assign(c, x).long()
-        cnt = torch.bincount(a, minlength=k)
-        sums = torch.zeros_like(c).index_add_(0, a, x)
+        sums, cnt = cluster_sums(x, a, k)
         nz = cnt > 0
-        c[nz] = sums[nz] / cnt[nz, None].to(c.dtype)
+        c[nz] = (sums[nz] / cnt[nz, None].double()).to(c.dtype)
         empty = torch.nonzero(~nz).flatten()
         if empty.numel():
             # split the most populated clusters (kmeans_reassign_empty, kmeans.cpp:112-147)
