@@ -101,3 +101,21 @@ def fit(ctx, index, train_queries, gt_ids, cell_assign, scc, filter_topk, topk=1
                 label_p50=float(lab.double().median()), label_p99=float(torch.quantile(lab.double(), 0.99)),
                 mse_per_epoch=[round(h, 4) for h in hist], target_var=float(target.double().var()))
     return probe, info
+
+
+def save(probe, path):
+    """Writes the probe's shape and packed float32 parameters (W1, b1, W2, b2,
+    w3, b3, feature mean, 1/std; the layout of puckgpu_probe_set_params) to
+    an .npz file."""
+    np.savez(path, shape=np.array([probe.dim, probe.nc, probe.nf, probe.hidden], np.int64),
+             params=probe.get_params())
+
+
+def load(ctx, path):
+    """A Probe on ctx's device from a file written by save()."""
+    with np.load(path, allow_pickle=False) as z:
+        dim, nc, nf, hidden = (int(v) for v in z["shape"])
+        params = np.ascontiguousarray(z["params"], np.float32)
+    probe = Probe(ctx, dim, nc, nf, hidden)
+    probe.set_params(params)
+    return probe

@@ -280,3 +280,16 @@ def test_server_with_probe_matches_search_learned(ctx, idx_cos):
         D, I, R = sv.search(q[s0:s0 + bs])
         assert np.array_equal(R, Re[s0:s0 + bs]) and np.array_equal(I, Ie[s0:s0 + bs])
         assert np.array_equal(D.view(np.uint32), De[s0:s0 + bs].view(np.uint32))
+
+
+def test_probe_save_load_round_trip(ctx, tmp_path):
+    from puck_amd import probe as PR
+    p = _rand_params(D_, H_, 11)
+    pr = _probe(ctx, p)
+    path = str(tmp_path / "probe.npz")
+    PR.save(pr, path)
+    pr2 = PR.load(ctx, path)
+    assert (pr2.dim, pr2.nc, pr2.nf, pr2.hidden) == (pr.dim, pr.nc, pr.nf, pr.hidden)
+    assert np.array_equal(pr2.get_params().view(np.uint32), pr.get_params().view(np.uint32))
+    x = np.random.default_rng(2).standard_normal((37, pr.D)).astype(np.float32)
+    assert np.array_equal(pr2.predict(x).view(np.uint32), pr.predict(x).view(np.uint32))
