@@ -634,6 +634,23 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     return PUCKGPU_OK;
 }
 
+// Calls f(i) for each i < m with flags[i] != 0, testing eight flags per
+// 64-bit word (flags are almost all zero: the scan after every pass is on
+// the host, between the GPU's batches).
+extern "C++" template <typename Fn>
+void for_each_flagged(const uint8_t* flags, uint32_t m, Fn f) {
+    uint32_t i = 0;
+    for (; i + 8 <= m; i += 8) {
+        uint64_t w;
+        std::memcpy(&w, flags + i, 8);
+        if (!w) continue;
+        for (uint32_t j = 0; j < 8; ++j)
+            if (flags[i + j]) f(i + j);
+    }
+    for (; i < m; ++i)
+        if (flags[i]) f(i);
+}
+
 // After the pass has completed (host flags copied in): queries whose result
 // depends on the order among exactly equal distances are replayed with the
 // reference heaps.  flags: 1 coarse-distance tie in the kept range, 2
@@ -654,10 +671,13 @@ int replay_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, c
     const uint32_t C = ix->p.coarse_cluster_count, F = ix->p.fine_cluster_count;
     std::vector<uint32_t> stage1;
     hlist.clear();
-    for (uint32_t i = 0; i < m; ++i) {
-        if (replay_all) hlist.push_back(i);
-        else if ((hflags[i] & 15u) != 0u && (hflags[i] & 6u) == 0u) stage1.push_back(i);
-        else if (hflags[i] & 15u) hlist.push_back(i);
+    if (replay_all) {
+        for (uint32_t i = 0; i < m; ++i) hlist.push_back(i);
+    } else {
+        for_each_flagged(hflags, m, [&](uint32_t i) {
+            if ((hflags[i] & 15u) != 0u && (hflags[i] & 6u) == 0u) stage1.push_back(i);
+            else if (hflags[i] & 15u) hlist.push_back(i);
+        });
     }
     const uint32_t n1 = (uint32_t)stage1.size();
     if (n1) {
@@ -853,9 +873,10 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
         PK_CHECK(hipMemcpyAsync(hflags, b.flags, m, hipMemcpyDeviceToHost, s));
         PK_CHECK(hipStreamSynchronize(s));
         if (cfg.precision) std::fill(hflags, hflags + m, (uint8_t)0);  // no replay in these forms
-        for (uint32_t i = 0; i < m; ++i)  // first-pass replay causes
+        for_each_flagged(hflags, m, [&](uint32_t i) {  // first-pass replay causes
             for (int bb = 0; bb < 8; ++bb)
                 if (hflags[i] & (1u << bb)) ctx->counters[5 + bb]++;
+        });
         const int nrep = replay_pass(ctx, ix, cfg, b, m, hflags, cfg.precision ? 0 : sp->replay_all, wa, ra,
                                      hlist, s);
         if (nrep < 0) return nrep;
@@ -878,8 +899,10 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
             PK_CHECK(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[6]));
             ctx->scan_ms += ms;
         }
+        uint64_t acc[4] = {0, 0, 0, 0};  // local sums: a tight loop over the pinned copy
         for (uint32_t i = 0; i < m; ++i)
-            for (int k = 0; k < 4; ++k) ctx->counters[k] += hstats[4 * (size_t)i + k];
+            for (int k = 0; k < 4; ++k) acc[k] += hstats[4 * (size_t)i + k];
+        for (int k = 0; k < 4; ++k) ctx->counters[k] += acc[k];
         ctx->counters[4] += (uint64_t)nrep;
     }
     if (!q_nf && !(pu && pu->probe)) ctx->scan_ms = -1.f;  // no budget scan was launched
