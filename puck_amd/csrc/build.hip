@@ -196,11 +196,14 @@ extern "C" hipError_t pk_launch_assign_cells(const float* ipc, const float* ipf,
     uint32_t C2 = 64;
     while (C2 < C) C2 <<= 1;
     size_t smem = sizeof(uint64_t) * 4 * C2;
-    static size_t allowed = 0;
-    if (!allowed) {
-        const hipError_t e = pk_allow_lds((const void*)pk_assign_cells, 160 * 1024, &allowed);
+    static PkLdsCache lds_cache;
+    size_t* slot = pk_lds_slot(lds_cache);
+    if (!slot) return hipErrorInvalidDevice;
+    if (!*slot) {
+        const hipError_t e = pk_allow_lds((const void*)pk_assign_cells, 160 * 1024, slot);
         if (e != hipSuccess) return e;
     }
+    const size_t allowed = *slot;
     if (smem > allowed) return hipErrorInvalidValue;
     hipLaunchKernelGGL(pk_assign_cells, dim3((n + 3) / 4), dim3(256), smem, s, ipc, ipf, xn, cn, fbound, st, n, C, F,
                        C2, fb_sorted, order, cell_out, dist_out);

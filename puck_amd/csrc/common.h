@@ -8,6 +8,19 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// A launcher's dynamic-LDS allowance per device (kernel attributes are set
+// per device): a zero-initialised static of the launcher, slot = the current
+// device.
+#define PK_MAX_DEVICES 64
+struct PkLdsCache {
+    size_t v[PK_MAX_DEVICES];
+};
+static inline size_t* pk_lds_slot(PkLdsCache& c) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= PK_MAX_DEVICES) return nullptr;
+    return &c.v[dev];
+}
+
 // Lets kernel f take up to `bytes` of dynamic LDS, less its own static LDS
 // (a CU holds 160 KB); *allowed receives the dynamic bytes allowed.
 static inline hipError_t pk_allow_lds(const void* f, size_t bytes, size_t* allowed) {

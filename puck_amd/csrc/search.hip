@@ -1527,8 +1527,10 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
     sa.scan_round = (uint32_t)rnd;
     const size_t smem_scan = fixed + 8 * cap;
     const uint32_t wgrid = args->wcount ? (nq < 1024 ? nq : 1024) : nq;
-    static size_t allowed = 0;
-    if (!allowed) {
+    static PkLdsCache lds_cache;
+    size_t* slot = pk_lds_slot(lds_cache);
+    if (!slot) return hipErrorInvalidDevice;
+    if (!*slot) {
         const void* fns[] = {(const void*)pk_walk<1024, false>, (const void*)pk_walk<2048, false>,
                              (const void*)pk_walk<1024, true>, (const void*)pk_walk<2048, true>,
                              (const void*)pk_walk_list<1024>, (const void*)pk_walk_list<2048>,
@@ -1540,8 +1542,9 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
             if (e != hipSuccess) return e;
             lo = a < lo ? a : lo;
         }
-        allowed = lo;
+        *slot = lo;
     }
+    const size_t allowed = *slot;
     if (args->q_nf && smem_scan > allowed) return hipErrorInvalidValue;
     if (smem > allowed) return hipErrorInvalidValue;
     // budget scan first (its own launch, timed between ev_mid and the end),
@@ -1920,11 +1923,14 @@ extern "C" hipError_t pk_launch_exact_center_order(const float* cdist, uint32_t 
     if (nlist == 0) return hipSuccess;
     const uint32_t L = C > F ? C : F;
     size_t smem = sizeof(uint2) * ((L + 2) & ~1u) + sizeof(float) * L;
-    static size_t allowed = 0;
-    if (!allowed) {
-        const hipError_t e = pk_allow_lds((const void*)pk_exact_center_order, 160 * 1024, &allowed);
+    static PkLdsCache lds_cache;
+    size_t* slot = pk_lds_slot(lds_cache);
+    if (!slot) return hipErrorInvalidDevice;
+    if (!*slot) {
+        const hipError_t e = pk_allow_lds((const void*)pk_exact_center_order, 160 * 1024, slot);
         if (e != hipSuccess) return e;
     }
+    const size_t allowed = *slot;
     if (smem > allowed) return hipErrorInvalidValue;
     hipLaunchKernelGGL(pk_exact_center_order, dim3(nlist), dim3(256), smem, s, cdist, C, fdist, F, scs, scap,
                        q_scc, qlist, flags, force, csel_d, csel_id, fsort_d, fsort_id);

@@ -79,11 +79,14 @@ extern "C" hipError_t pk_launch_sort_rows_list(const float* vals, uint32_t max_r
                                                const uint32_t* list, const uint32_t* count, hipStream_t s) {
     if (max_rows == 0 || keep == 0) return hipSuccess;
     if (L > 8192 || keep > L) return hipErrorInvalidValue;
-    static size_t allowed = 0;
-    if (!allowed) {
-        const hipError_t e = pk_allow_lds((const void*)pk_sort_rows_list, 64 * 1024, &allowed);
+    static PkLdsCache lds_cache;
+    size_t* slot = pk_lds_slot(lds_cache);
+    if (!slot) return hipErrorInvalidDevice;
+    if (!*slot) {
+        const hipError_t e = pk_allow_lds((const void*)pk_sort_rows_list, 64 * 1024, slot);
         if (e != hipSuccess) return e;
     }
+    const size_t allowed = *slot;
     uint32_t n2 = 2;
     while (n2 < L) n2 <<= 1;
     if (n2 * sizeof(uint64_t) > allowed) return hipErrorInvalidValue;
@@ -98,11 +101,14 @@ extern "C" hipError_t pk_launch_sort_rows(const float* vals, uint32_t rows, uint
                                           uint32_t flag_lim, hipStream_t s) {
     if (rows == 0 || keep == 0) return hipSuccess;
     if (L > 8192 || keep > L) return hipErrorInvalidValue;
-    static size_t allowed = 0;
-    if (!allowed) {
-        const hipError_t e = pk_allow_lds((const void*)pk_sort_rows, 64 * 1024, &allowed);
+    static PkLdsCache lds_cache;
+    size_t* slot = pk_lds_slot(lds_cache);
+    if (!slot) return hipErrorInvalidDevice;
+    if (!*slot) {
+        const hipError_t e = pk_allow_lds((const void*)pk_sort_rows, 64 * 1024, slot);
         if (e != hipSuccess) return e;
     }
+    const size_t allowed = *slot;
     uint32_t n2 = 2;
     while (n2 < L) n2 <<= 1;
     if (n2 * sizeof(uint64_t) > allowed) return hipErrorInvalidValue;
