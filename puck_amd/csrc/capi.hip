@@ -241,20 +241,14 @@ int puckgpu_create(int device, puckgpu_ctx** out) {
     if (hipSetDevice(device) != hipSuccess) return PUCKGPU_EHIP;
     puckgpu_ctx* ctx = new puckgpu_ctx();
     ctx->device = device;
-    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete ctx;
-        return PUCKGPU_EHIP;
-    }
-    for (auto& e : ctx->ev)
-        if (hipEventCreate(&e) != hipSuccess) {
-            delete ctx;
-            return PUCKGPU_EHIP;
-        }
-    if (hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->order_ev, hipEventDisableTiming) != hipSuccess) {
-        delete ctx;
+    bool ok = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) == hipSuccess;
+    for (auto& e : ctx->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+    ok = ok && hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&ctx->order_ev, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        puckgpu_destroy(ctx);  // releases whatever was created
         return PUCKGPU_EHIP;
     }
     *out = ctx;
@@ -264,7 +258,7 @@ int puckgpu_create(int device, puckgpu_ctx** out) {
 void puckgpu_destroy(puckgpu_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto& b : ctx->bufs)
         if (b.p) (void)hipFree(b.p);
     for (auto& e : ctx->ev)
@@ -277,7 +271,7 @@ void puckgpu_destroy(puckgpu_ctx* ctx) {
         (void)hipStreamSynchronize(ctx->stream2);
         (void)hipStreamDestroy(ctx->stream2);
     }
-    (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 

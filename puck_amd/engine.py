@@ -479,6 +479,9 @@ class Server:
         """sample: host float32 [max_batch, dim].  Returns (shapes [buckets],
         times [buckets, shapes] ms)."""
         sample = np.ascontiguousarray(sample, np.float32)
+        if sample.shape != (self.max_batch, self.index.dim):
+            # the largest bucket reads max_batch rows of the sample
+            raise ValueError(f"sample must be [{self.max_batch}, {self.index.dim}], got {list(sample.shape)}")
         nsh = 4
         shapes = np.zeros(self.buckets, np.int32)
         times = np.zeros((self.buckets, nsh), np.float32)

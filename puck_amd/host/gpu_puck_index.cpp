@@ -111,6 +111,7 @@ int GpuPuckIndex::init() {
     uint32_t ftk = _flags.filter_topk < 0 ? 11 * _flags.topk : (uint32_t)_flags.filter_topk;
     if (_flags.radius_rate < 1.0) { _err = "radius_rate should >= 1.0"; return -1; }
     if (ftk < _flags.topk) { _err = "filter_topk should >= topk"; return -1; }
+    if (ftk > puckgpu_filter_topk_max()) { _err = "filter_topk above the engine's maximum"; return -1; }
     if (!_ctx && puckgpu_create(_flags.device, &_ctx) != 0) { _err = "no GPU context"; return -1; }
     if (_index) { puckgpu_index_free(_index); _index = nullptr; }
     puckgpu_index_params ip{c.dim, c.C, c.F, c.filter_nsq, c.N, c.whether_norm ? 1 : 0};
@@ -130,6 +131,7 @@ int GpuPuckIndex::init() {
 
 int GpuPuckIndex::search(const Request* request, Response* response) {
     if (!request || !response || request->feature == nullptr || request->topk > _flags.topk) {
+        std::lock_guard<std::mutex> g(_mu);
         _err = "topk should <= topk, or feature is nullptr";
         return -1;
     }

@@ -49,7 +49,7 @@ struct GpuIndexFlags {
     std::string filter_data_file_name = "filter_data.dat";
     uint32_t topk = 100;
     uint32_t search_coarse_count = 200;
-    int32_t filter_topk = -1;  // unset: 11 * topk (index_conf.cpp:174-177)
+    int32_t filter_topk = -1;  // unset: 11 * topk (index_conf.cpp:178-180)
     double radius_rate = 1.0;
     int device = 0;
 };
