@@ -294,6 +294,68 @@ def test_search_errors(ctx, idx_cos):
     assert Rg[1] == 0 and Rg[0] == 10 and Rg[2] == 10
 
 
+def test_search_max_filter_topk(ctx, idx_wide):
+    # filter_topk at the engine's maximum on lists of ~12k points: the filter
+    # set fills on the fast walk, on the reference-heap replay and in the
+    # budget scan; one above the maximum is refused
+    from puck_amd.engine import PuckGpuError
+    ix, q = idx_wide
+    gi = _gpu_index(ctx, ix)
+    oi = CI.oracle_index(ix)
+    ftk = int(ctx.L.puckgpu_filter_topk_max())
+    assert ftk == 2048
+    n = q.shape[0]
+    for replay in (False, True):
+        Dg, Ig, Rg = gi.search(q, 100, 2, ftk, replay_all=replay)
+        ft, fm, fn = gi.last_filter(n, ftk)
+        for k in range(n):
+            o = oi.search_one(q[k], 100, 2, ftk)
+            assert o["filt_n"] == ftk and fn[k] == ftk, (replay, k)
+            if replay:  # the reference heap's array order
+                assert np.array_equal(_bits(ft[k]), _bits(o["filt_val"][:ftk])), (replay, k)
+                assert np.array_equal(fm[k], o["filt_mem"][:ftk]), (replay, k)
+            else:
+                gv, gid = _sorted_pairs(ft[k], fm[k])
+                ov, oid = _sorted_pairs(o["filt_val"][:ftk], o["filt_mem"][:ftk])
+                assert np.array_equal(_bits(gv), _bits(ov)) and np.array_equal(gid, oid), (replay, k)
+            assert Rg[k] == o["n"] == 100
+            assert np.array_equal(Ig[k], o["ids"][:100]), (replay, k)
+            assert np.array_equal(_bits(Dg[k]), _bits(o["dist"][:100])), (replay, k)
+    nf = np.array([1, 300, 2000, 0] * (n // 4), np.uint32)
+    Dg, Ig, Rg = gi.search_batch(q, 100, 2, ftk, fine_budget=nf)
+    ft, fm, fn = gi.last_filter(n, ftk)
+    for k in range(n):
+        o = oi.search_one(q[k], 100, 2, ftk, fine_budget=int(nf[k]))
+        assert fn[k] == o["filt_n"], k
+        gv, gid = _sorted_pairs(ft[k, :fn[k]], fm[k, :fn[k]])
+        ov, oid = _sorted_pairs(o["filt_val"][:fn[k]], o["filt_mem"][:fn[k]])
+        assert np.array_equal(_bits(gv), _bits(ov)) and np.array_equal(gid, oid), k
+        assert Rg[k] == o["n"]
+        assert np.array_equal(Ig[k, :o["n"]], o["ids"][:o["n"]]), k
+        assert np.array_equal(_bits(Dg[k, :o["n"]]), _bits(o["dist"][:o["n"]])), k
+    with pytest.raises(PuckGpuError):
+        gi.search(q[:2], 10, 2, ftk + 1)
+
+
+def test_empty_batches(ctx, idx_cos):
+    # a batch of no queries returns at once from every search entry point
+    from puck_amd.engine import Server
+    ix, q = idx_cos
+    gi = _gpu_index(ctx, ix)
+    q0 = np.ascontiguousarray(q[:0])
+    D0, I0, R0 = gi.search(q0, 10, 5, 100)
+    assert D0.shape == (0, 10) and I0.shape == (0, 10) and R0.shape == (0,)
+    D0, I0, R0 = gi.search_batch(q0, 10, 5, 100, topk=np.zeros(0, np.uint32), fine_budget=np.zeros(0, np.uint32))
+    assert D0.shape == (0, 10) and R0.shape == (0,)
+    sv = Server(gi, 10, 5, 100, max_batch=8)
+    D0, I0, R0 = sv.search(q0)
+    assert D0.shape == (0, 10) and R0.shape == (0,)
+    Dg, Ig, Rg = sv.search(q[:3])  # the server still serves after an empty batch
+    Dw, Iw, Rw = gi.search(q[:3], 10, 5, 100)
+    assert np.array_equal(Ig, Iw) and np.array_equal(_bits(Dg), _bits(Dw)) and np.array_equal(Rg, Rw)
+    sv.free()
+
+
 def test_assign_cells_bitexact(ctx, idx_l2, idx_cos):
     rng = np.random.default_rng(9)
     for ix, _ in (idx_l2, idx_cos):
