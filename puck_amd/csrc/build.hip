@@ -367,40 +367,51 @@ extern "C" hipError_t pk_launch_finalize_cells(float* st, const uint32_t* cs, ui
 }
 
 // Gather rows into memory order: dst[m] = src[mem2local[m]] (row of `bytes`).
+// One element per work-item in a grid-stride loop: an index's arrays hold
+// more elements (100M x 128 features) than one dispatch has work-items (2^32).
+#define GATHER_BLOCKS 4096u
+
 __global__ void pk_gather_rows(const uint8_t* __restrict__ src, uint64_t src_stride, uint64_t src_off,
                                uint8_t* __restrict__ dst, uint64_t dst_stride, uint32_t bytes,
-                               const uint32_t* __restrict__ mem2local, uint32_t n) {
-    uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t m = gid / bytes;
-    uint32_t b = (uint32_t)(gid % bytes);
-    if (m >= n) return;
-    dst[m * dst_stride + b] = src[(uint64_t)mem2local[m] * src_stride + src_off + b];
+                               const uint32_t* __restrict__ mem2local, uint64_t total) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total; gid += step) {
+        const uint64_t m = gid / bytes;
+        const uint32_t b = (uint32_t)(gid % bytes);
+        dst[m * dst_stride + b] = src[(uint64_t)mem2local[m] * src_stride + src_off + b];
+    }
 }
 
 __global__ void pk_gather_f32(const float* __restrict__ src, uint32_t width, float* __restrict__ dst,
-                              const uint32_t* __restrict__ mem2local, uint32_t n) {
-    uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t m = gid / width;
-    uint32_t b = (uint32_t)(gid % width);
-    if (m >= n) return;
-    dst[m * width + b] = src[(uint64_t)mem2local[m] * width + b];
+                              const uint32_t* __restrict__ mem2local, uint64_t total) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total; gid += step) {
+        const uint64_t m = gid / width;
+        const uint32_t b = (uint32_t)(gid % width);
+        dst[gid] = src[(uint64_t)mem2local[m] * width + b];
+    }
+}
+
+static unsigned gather_blocks(uint64_t total) {
+    const uint64_t b = (total + 255) / 256;
+    return (unsigned)(b < GATHER_BLOCKS ? b : GATHER_BLOCKS);
 }
 
 extern "C" hipError_t pk_launch_gather_rows(const uint8_t* src, uint64_t src_stride, uint64_t src_off, uint8_t* dst,
                                             uint64_t dst_stride, uint32_t bytes, const uint32_t* mem2local,
                                             uint32_t n, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    uint64_t total = (uint64_t)n * bytes;
-    hipLaunchKernelGGL(pk_gather_rows, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, src, src_stride,
-                       src_off, dst, dst_stride, bytes, mem2local, n);
+    if (n == 0 || bytes == 0) return hipSuccess;
+    const uint64_t total = (uint64_t)n * bytes;
+    hipLaunchKernelGGL(pk_gather_rows, dim3(gather_blocks(total)), dim3(256), 0, s, src, src_stride, src_off, dst,
+                       dst_stride, bytes, mem2local, total);
     return hipGetLastError();
 }
 
 extern "C" hipError_t pk_launch_gather_f32(const float* src, uint32_t width, float* dst, const uint32_t* mem2local,
                                            uint32_t n, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    uint64_t total = (uint64_t)n * width;
-    hipLaunchKernelGGL(pk_gather_f32, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, src, width, dst,
-                       mem2local, n);
+    if (n == 0 || width == 0) return hipSuccess;
+    const uint64_t total = (uint64_t)n * width;
+    hipLaunchKernelGGL(pk_gather_f32, dim3(gather_blocks(total)), dim3(256), 0, s, src, width, dst, mem2local,
+                       total);
     return hipGetLastError();
 }
