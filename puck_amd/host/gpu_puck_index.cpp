@@ -93,6 +93,7 @@ int GpuPuckIndex::init() {
     if (!read_conf(join(P, _flags.index_file_name), &c)) { _err = "model file read error"; return -1; }
     if (c.index_type != (int32_t)IndexType::PUCK) { _err = "index_type is not PUCK"; return -1; }
     if (c.whether_pq) { _err = "whether_pq = true is not served by the GPU engine"; return -1; }
+    if (c.ip2cos) { _err = "ip2cos indexes are not served by the GPU engine"; return -1; }
     if (c.filter_nsq == 0 || c.dim % c.filter_nsq) { _err = "filter_nsq does not divide dim"; return -1; }
     const uint32_t lsq = c.dim / c.filter_nsq;
     std::vector<float> coarse, fine, cb, feat;
@@ -125,7 +126,7 @@ int GpuPuckIndex::init() {
     _sp.filter_topk = ftk;
     _sp.radius_rate = _flags.radius_rate;
     _sp.replay_all = 0;
-    _dim = c.dim - (c.ip2cos ? 1 : 0);
+    _dim = c.dim;
     return 0;
 }
 

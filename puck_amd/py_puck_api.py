@@ -5,7 +5,7 @@ Mirrors pyapi_wrapper/py_puck_api.i + py_api_wrapper.{h,cpp}:
     update_gflag(key, value)                 py_api_wrapper.cpp:33-35
     PySearcher().init() -> 0 / -1            py_api_wrapper.cpp:78-108
     PySearcher().build(total_cnt) -> 0 / -1  py_api_wrapper.cpp:43-76
-    PySearcher().search(n, query, topk, distance, labels) -> None
+    PySearcher().search(n, query, topk, distance, labels) -> 0
                                              py_api_wrapper.cpp:154-171
 
 `search` writes into caller-owned arrays with row stride `topk` (numpy
@@ -13,7 +13,7 @@ float32 / uint32, or CUDA tensors); a request the reference would reject
 (topk > FLAGS_topk) leaves them untouched, as PuckIndex::search returns -1
 before touching its response (puck_index.cpp:560-563) and PySearcher ignores
 per-query return codes.  Only PUCK indexes with whether_pq = false (filter PQ
-+ exact rerank) are served; other index types make init() return -1.
++ exact rerank) and ip2cos = 0 are served; other indexes make init() return -1.
 """
 import sys
 
@@ -51,6 +51,9 @@ class PySearcher:
         if filter_topk < topk:
             _err("filter_topk should >= topk, should update the search params")
             return None
+        if filter_topk > self._ctx.L.puckgpu_filter_topk_max():
+            _err(f"filter_topk {filter_topk} above the engine's maximum {self._ctx.L.puckgpu_filter_topk_max()}")
+            return None
         scc = min(FL.get("search_coarse_count"), self._conf["coarse_cluster_count"])
         return dict(topk=topk, filter_topk=filter_topk, radius_rate=rr, search_coarse_count=scc)
 
@@ -67,6 +70,9 @@ class PySearcher:
             return -1
         if conf["whether_pq"]:
             _err("whether_pq = true (second-stage PQ rerank) is not served by the GPU engine")
+            return -1
+        if conf["ip2cos"]:
+            _err("ip2cos indexes (inner product stored as an extra dimension) are not served by the GPU engine")
             return -1
         self._conf = conf
         if self._loaded_path != path or self._index is None:
@@ -87,7 +93,7 @@ class PySearcher:
         self._params = self._search_params()
         if self._params is None:
             return -1
-        self._dim = conf["feature_dim"] - (1 if conf["ip2cos"] else 0)
+        self._dim = conf["feature_dim"]
         return 0
 
     def build(self, total_cnt):
@@ -123,14 +129,17 @@ class PySearcher:
         return 0
 
     def search(self, n, query_fea, topk, distance, labels):
+        """0 as PySearcher::search returns (py_api_wrapper.cpp:169), also when
+        every query is refused; -1 only for a search before init(), where the
+        reference has no index to call."""
         if self._index is None or self._params is None:
             _err("search before a successful init()")
-            return
+            return -1
         if topk > self._params["topk"] or topk == 0:
             _err(f"topk should <= topk, topk = {self._params['topk']}")
-            return
+            return 0
         if n == 0:
-            return
+            return 0
         q = query_fea
         if isinstance(q, np.ndarray):
             q = np.ascontiguousarray(q, np.float32).reshape(n, self._conf["feature_dim"])
@@ -147,6 +156,7 @@ class PySearcher:
             self._index.search(q, topk, self._params["search_coarse_count"], self._params["filter_topk"],
                                topk_max=self._params["topk"], radius_rate=self._params["radius_rate"],
                                out=(distance.view(n, topk), labels.view(n, topk), None))
+        return 0
 
     def show(self):
         print("1")

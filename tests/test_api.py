@@ -92,3 +92,11 @@ def test_searcher_init_errors(tmp_path):
     index_io.write_index_conf(str(e / "index.dat"), 8, False, 0, 4, 2, 2, whether_pq=True)
     api.update_gflag("index_path", str(e))
     assert s.init() == -1
+    f = tmp_path / "ip2cos"
+    f.mkdir()
+    index_io.write_index_conf(str(f / "index.dat"), 9, False, 0, 3, 2, 2, ip2cos=1)
+    api.update_gflag("index_path", str(f))
+    assert s.init() == -1
+    # a search before a successful init() is refused; the reference's wrapper
+    # returns 0 from every search it runs
+    assert s.search(1, np.zeros(9, np.float32), 1, np.zeros(1, np.float32), np.zeros(1, np.uint32)) == -1

@@ -1,6 +1,6 @@
 """The C++ puck::Index adapter (puck_amd/host) refuses bad search parameters
-in init() before it creates a device context, so these checks need no GPU:
-filter_topk below topk (index_conf.cpp:192-195) and, beyond the reference,
+and index types in init() before it creates a device context, so these checks
+need no GPU: an ip2cos index, filter_topk below topk (index_conf.cpp:192-195) and, beyond the reference,
 a filter_topk above the engine's maximum (the unset default is 11 * topk,
 index_conf.cpp:178-180, which passes 2048 from topk 187 on)."""
 import os
@@ -53,3 +53,13 @@ def test_init_accepts_default_filter_topk(index_dir, tmp_path):
     r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     if r.returncode != 0:
         assert r.returncode == 4 and "no GPU context" in r.stderr, (r.returncode, r.stderr)
+
+
+def test_init_rejects_ip2cos_index(tmp_path):
+    d = tmp_path / "ip2cos"
+    d.mkdir()
+    index_io.write_index_conf(str(d / "index.dat"), 17, False, 0, 17, 4, 4, ip2cos=1)
+    out = str(tmp_path / "out.bin")
+    r = subprocess.run([BIN, str(d), str(d / "q.f32"), "1", "16", "1", "10", "2", "-1", out], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 4 and "ip2cos" in r.stderr, (r.returncode, r.stderr)
