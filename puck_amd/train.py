@@ -7,8 +7,10 @@ descending norm) and PuckIndex::train (puck_index.cpp:764-874: per-subspace
 256-center k-means on the cell residuals).  The Lloyd assignment step of every
 k-means run is the engine's GEMM + argmin kernel (puckgpu_kmeans_assign, the
 nearest_center formula of kmeans.cpp:221-258); the centroid means are
-fixed-order torch reductions, so a build is reproducible run to run.  The reference seeds its sampling and initialisation with time(0),
-so trained codebooks are not reproducible against it; these are seeded.
+fixed-order torch reductions, so a build is reproducible run to run.  The
+reference seeds its sampling and initialisation with time(0), so trained
+codebooks are not reproducible against it; these are seeded.  Empty clusters
+take half of the most populated ones (kmeans.cpp:112-147).
 """
 import torch
 
