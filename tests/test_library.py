@@ -55,3 +55,15 @@ def test_cluster_sums_fixed_order():
     assert torch.equal(c1, torch.bincount(a, minlength=40)) and c1[13] == 0
     assert torch.allclose(s1, ref, rtol=0, atol=1e-9)
     assert torch.equal(s1, s2) and torch.equal(c1, c2)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # the bench measures one GPU; --gpus N > 1 exits with a message before touching a device
+    import json
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120, cwd=ROOT)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert msg["n_gpus"] == 2 and "one GPU" in msg["error"]
