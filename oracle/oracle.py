@@ -50,6 +50,8 @@ def lib():
     L.po_sentinel_value.restype = ctypes.c_float
     L.po_dot.restype = ctypes.c_float
     L.po_dot.argtypes = [_f32p, _f32p, ctypes.c_uint32]
+    L.po_sdot_short.restype = ctypes.c_float
+    L.po_sdot_short.argtypes = [_f32p, _f32p, ctypes.c_uint32]
     L.po_heap_run.restype = ctypes.c_uint32
     L.po_heap_run.argtypes = [ctypes.c_uint32, _f32p, _u32p, ctypes.c_uint32, ctypes.c_int, _f32p, _u32p]
     L.po_lut.restype = ctypes.c_int
@@ -113,6 +115,12 @@ SENTINEL = np.frombuffer(np.uint32(0x7F7F7F7F).tobytes(), dtype=np.float32)[0]
 def dot(a, b):
     a, b = c32(a), c32(b)
     return float(lib().po_dot(a, b, a.size))
+
+
+def sdot_short(a, b):
+    """cblas_sdot of 1, 2 or 4 floats in MKL's order (float32 result)."""
+    a, b = c32(a), c32(b)
+    return np.float32(lib().po_sdot_short(a, b, a.size))
 
 
 def heap_run(cap, vals, tags=None, reorder=True):

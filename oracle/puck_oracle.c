@@ -16,6 +16,9 @@
  *     It stands in for cblas_sdot and for every entry of sgemm_.  MKL's order
  *     is unspecified, so bit-parity with an MKL build is unpinned; the HIP
  *     engine's exact-arithmetic mode reproduces po_dot() bit for bit.
+ *   - The one exception: the dot products of 1, 2 or 4 floats that pick the
+ *     filter-PQ codes follow MKL's own order for those lengths
+ *     (po_sdot_short, checked against MKL where it is installed).
  *   - Everything else (LUT, LUT lookup, heap, cell walk, rerank) follows the
  *     reference's own written order, which the reference fixes in source.
  *
@@ -53,6 +56,28 @@ float po_dot(const float* a, const float* b, uint32_t n) {
     float acc = 0.0f;
     for (uint32_t i = 0; i < n; ++i) acc = fmaf(a[i], b[i], acc);
     return acc;
+}
+
+/* cblas_sdot of 1, 2 or 4 floats, in the order Intel MKL computes it: every
+ * product rounded, then p0 + p1 (n = 2) or (p0 + p1) + (p2 + p3) (n = 4), no
+ * fused multiply-add.  Determined by calling MKL 2021.4's cblas_sdot on an
+ * x86-64 AVX-512 host and comparing result bits with candidate orders (every
+ * sample agreed with this one); tests/test_oracle.py repeats that comparison
+ * wherever libmkl_rt.so is installed.  nearest_center (kmeans.cpp:221-258)
+ * takes these dot products with n = lsq when it picks the filter-PQ codes
+ * (puck_index.cpp:667-668).  Other n: po_dot. */
+float po_sdot_short(const float* a, const float* b, uint32_t n) {
+    if (n == 1) return a[0] * b[0];
+    if (n == 2) {
+        float p0 = a[0] * b[0], p1 = a[1] * b[1];
+        return p0 + p1;
+    }
+    if (n == 4) {
+        float p0 = a[0] * b[0], p1 = a[1] * b[1], p2 = a[2] * b[2], p3 = a[3] * b[3];
+        float s01 = p0 + p1, s23 = p2 + p3;
+        return s01 + s23;
+    }
+    return po_dot(a, b, n);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -787,7 +812,7 @@ int po_encode_filter(const float* coarse, const float* fine, uint32_t F, uint32_
     for (uint32_t m = 0; m < nsq; ++m)
         for (uint32_t k = 0; k < PO_KS; ++k) {
             const float* cw = codebook + ((size_t)m * PO_KS + k) * lsq;
-            cbn[m * PO_KS + k] = po_dot(cw, cw, lsq);
+            cbn[m * PO_KS + k] = po_sdot_short(cw, cw, lsq);
         }
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
@@ -812,12 +837,12 @@ int po_encode_filter(const float* coarse, const float* fine, uint32_t F, uint32_
             for (uint32_t m = 0; m < nsq; ++m) {
                 const float* sub = r + m * lsq;
                 const float* cb = codebook + (size_t)m * PO_KS * lsq;
-                float pn = po_dot(sub, sub, lsq);
+                float pn = po_sdot_short(sub, sub, lsq);
                 float best = FLT_MAX;
                 uint32_t arg = 0;
                 for (uint32_t k = 0; k < PO_KS; ++k) {
                     float d = (float)((double)(pn + cbn[m * PO_KS + k]) -
-                                      2.0 * (double)po_dot(sub, cb + (size_t)k * lsq, lsq));
+                                      2.0 * (double)po_sdot_short(sub, cb + (size_t)k * lsq, lsq));
                     if (d < best) { best = d; arg = k; }
                 }
                 out[4 + m] = (uint8_t)arg;

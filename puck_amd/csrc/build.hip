@@ -227,8 +227,9 @@ extern "C" hipError_t pk_launch_fine_bound(const float* fine, uint32_t F, uint32
 // ---------------------------------------------------------------------------
 // Filter-PQ encode: one thread per point, one subspace's codebook in LDS at a
 // time.  residual r = (x - c) - f; code_m = argmin_k (float)((double)(pn + cn_k)
-// - 2.0 * (double)dot(r_m, cb_k)) with strict '<' from FLT_MAX; offset =
-// 2 * dot(decoded, c + f), accumulated in subspace order.
+// - 2.0 * (double)dot(r_m, cb_k)) with strict '<' from FLT_MAX, the three dot
+// products of lsq floats in MKL's order for such lengths (pk_sdot_short);
+// offset = 2 * dot(decoded, c + f), accumulated in subspace order.
 __global__ __launch_bounds__(256) void pk_encode_filter(const float* __restrict__ X, const uint32_t* __restrict__ cell,
                                                         uint32_t n, const float* __restrict__ coarse,
                                                         const float* __restrict__ fine, uint32_t F, uint32_t dim,
@@ -254,7 +255,7 @@ __global__ __launch_bounds__(256) void pk_encode_filter(const float* __restrict_
         for (uint32_t e = threadIdx.x; e < PK_KS * lsq; e += blockDim.x)
             cb[e] = codebook[(uint64_t)m * PK_KS * lsq + e];
         __syncthreads();
-        for (uint32_t k = threadIdx.x; k < PK_KS; k += blockDim.x) cbn[k] = pk_dot(cb + k * lsq, cb + k * lsq, lsq);
+        for (uint32_t k = threadIdx.x; k < PK_KS; k += blockDim.x) cbn[k] = pk_sdot_short(cb + k * lsq, cb + k * lsq, lsq);
         __syncthreads();
         if (!valid) continue;
         float r[4];
@@ -262,11 +263,11 @@ __global__ __launch_bounds__(256) void pk_encode_filter(const float* __restrict_
             uint32_t i = m * lsq + j;
             r[j] = (x[i] - cv[i]) - fv[i];
         }
-        float pn = pk_dot(r, r, lsq);
+        float pn = pk_sdot_short(r, r, lsq);
         float best = FLT_MAX;
         uint32_t arg = 0;
         for (uint32_t k = 0; k < PK_KS; ++k) {
-            float dt = pk_dot(r, cb + k * lsq, lsq);
+            float dt = pk_sdot_short(r, cb + k * lsq, lsq);
             float d = (float)((double)(pn + cbn[k]) - 2.0 * (double)dt);
             if (d < best) { best = d; arg = k; }
         }

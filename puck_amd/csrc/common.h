@@ -60,6 +60,20 @@ __device__ __forceinline__ float pk_dot(const float* a, const float* b, uint32_t
     return acc;
 }
 
+// cblas_sdot of 1, 2 or 4 floats in the order Intel MKL takes for those
+// lengths (found by calling it and comparing result bits): rounded products,
+// then p0 + p1 or (p0 + p1) + (p2 + p3), no fused multiply-add.  These are the
+// dot products with which nearest_center (kmeans.cpp:221-258) picks the
+// filter-PQ codes (puck_index.cpp:667-668).  Other n: pk_dot.
+__device__ __forceinline__ float pk_sdot_short(const float* a, const float* b, uint32_t n) {
+    if (n == 1) return __fmul_rn(a[0], b[0]);
+    if (n == 2) return __fadd_rn(__fmul_rn(a[0], b[0]), __fmul_rn(a[1], b[1]));
+    if (n == 4)
+        return __fadd_rn(__fadd_rn(__fmul_rn(a[0], b[0]), __fmul_rn(a[1], b[1])),
+                         __fadd_rn(__fmul_rn(a[2], b[2]), __fmul_rn(a[3], b[3])));
+    return pk_dot(a, b, n);
+}
+
 // Lookup-table entry ||x - y||^2 for a sub-vector of length 1, 2 or 4, with
 // the summation order of faiss fvec_L2sqr_ny_D1/D2/D4
 // (puck/puck/distances_simd.cpp:72-133): rounded squares, then pairwise sums.

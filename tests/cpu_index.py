@@ -87,3 +87,19 @@ def build_index_with_centers(x, coarse, fine, nsq, whether_norm=False, seed=0):
 def oracle_index(ix):
     return O.OracleIndex(ix["coarse"], ix["fine"], ix["codebook"], ix["cell_assign"],
                          ix["records"], ix["raw"], ix["whether_norm"])
+
+
+def near_tie_encode_input(seed, nsq, lsq, n):
+    """Filter-PQ encode input on which the order of the short dot products
+    decides the codes: codewords and points a few ulps around one center, with
+    zero coarse and fine centers (residual = point), so pn + cn - 2 dot cancels
+    to rounding noise.  Returns (coarse, fine, codebook [nsq*256, lsq], x, cell)."""
+    rng = np.random.default_rng(seed)
+    d = nsq * lsq
+    center = rng.uniform(1.0, 100.0, d).astype(np.float32)
+    sp = np.spacing(center)
+    cb = center.reshape(nsq, 1, lsq) + rng.integers(-8, 9, (nsq, 256, lsq)).astype(np.float32) * sp.reshape(nsq, 1, lsq)
+    x = center + rng.integers(-64, 65, (n, d)).astype(np.float32) * sp
+    z = np.zeros((1, d), np.float32)
+    return (z, z.copy(), np.ascontiguousarray(cb.astype(np.float32).reshape(-1, lsq)), x.astype(np.float32),
+            np.zeros(n, np.uint32))

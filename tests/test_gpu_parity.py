@@ -377,6 +377,16 @@ def test_encode_filter_bitexact(ctx, idx_l2, idx_cos):
         assert np.array_equal(rg, ro)
 
 
+@pytest.mark.parametrize("lsq,nsq", [(1, 32), (2, 16), (4, 8)])
+def test_encode_filter_near_ties_bitexact(ctx, lsq, nsq):
+    # codes decided by the order of the short dot products (MKL's order for
+    # 1, 2 or 4 floats): distances that cancel to rounding noise
+    coarse, fine, cb, x, cell = CI.near_tie_encode_input(31, nsq, lsq, 3000)
+    rg = ctx.encode_filter(coarse, fine, cb.reshape(-1), nsq, x, cell)
+    ro = O.encode_filter(coarse, fine, cb, nsq, x, cell)
+    assert np.array_equal(rg, ro)
+
+
 def test_kmeans_assign_bitexact(ctx):
     rng = np.random.default_rng(3)
     for d, k in [(128, 300), (96, 64), (2, 256), (4, 256)]:

@@ -365,3 +365,82 @@ def test_per_query_batch_matches_single_calls(small_l2):
         assert np.array_equal(D[i, :o["n"]].view(np.uint32), o["dist"][:o["n"]].view(np.uint32))
         assert np.array_equal(I[i, :o["n"]], o["ids"][:o["n"]])
         assert np.all(I[i, o["n"]:] == 0xffffffff) and np.all(D[i, k:] == sent)
+
+
+# ------------------------------------------------- MKL's order for short sdot
+MKL_PATH = "/opt/conda/lib/libmkl_rt.so"
+needs_mkl = pytest.mark.skipif(not os.path.exists(MKL_PATH), reason="MKL runtime not installed")
+
+
+def _mkl_sdot():
+    import ctypes
+    mkl = ctypes.CDLL(MKL_PATH)
+    mkl.cblas_sdot.restype = ctypes.c_float
+    mkl.cblas_sdot.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+
+    def sdot(a, b):
+        a, b = np.ascontiguousarray(a, F32), np.ascontiguousarray(b, F32)
+        return F32(mkl.cblas_sdot(a.size, a.ctypes.data, 1, b.ctypes.data, 1))
+    return sdot
+
+
+def test_sdot_short_closed_form():
+    # rounded products, then p0 + p1 / (p0 + p1) + (p2 + p3); an FMA chain differs on these inputs
+    rng = np.random.default_rng(11)
+    diff = 0
+    for n in (1, 2, 4):
+        for _ in range(500):
+            a = (rng.standard_normal(n) * 10 ** rng.uniform(-2, 2, n)).astype(F32)
+            b = (rng.standard_normal(n) * 10 ** rng.uniform(-2, 2, n)).astype(F32)
+            p = a * b
+            want = p[0] if n == 1 else (p[0] + p[1] if n == 2 else (p[0] + p[1]) + (p[2] + p[3]))
+            got = O.sdot_short(a, b)
+            assert got.view(np.uint32) == F32(want).view(np.uint32), (a, b)
+            diff += got != F32(O.dot(a, b))
+    assert diff > 0
+
+
+@needs_mkl
+def test_sdot_short_matches_mkl():
+    # the order was read off MKL's results: every sample must agree bit for bit
+    sdot = _mkl_sdot()
+    rng = np.random.default_rng(12)
+    for n in (1, 2, 4):
+        for _ in range(3000):
+            a = (rng.standard_normal(n) * 10 ** rng.uniform(-3, 3, n)).astype(F32)
+            b = (rng.standard_normal(n) * 10 ** rng.uniform(-3, 3, n)).astype(F32)
+            assert O.sdot_short(a, b).view(np.uint32) == sdot(a, b).view(np.uint32), (n, a, b)
+
+
+@needs_mkl
+@pytest.mark.parametrize("lsq,nsq", [(1, 32), (2, 16), (4, 8)])
+def test_encode_codes_match_nearest_center_with_mkl(lsq, nsq):
+    # the filter-PQ codes the oracle writes are the ones nearest_center
+    # (kmeans.cpp:221-258) picks with MKL's own cblas_sdot: float pn + cn,
+    # minus 2.0 * sdot in double, strict '<' from FLT_MAX.  The input makes
+    # the distances cancel to rounding noise, so the order of the short dot
+    # products decides the codes (an FMA chain picks other codes for lsq 2, 4).
+    sdot = _mkl_sdot()
+    coarse, fine, cb, x, cell = CI.near_tie_encode_input(31, nsq, lsq, 12)
+    rec = O.encode_filter(coarse, fine, cb, nsq, x, cell)
+    cbm = cb.reshape(nsq, 256, lsq)
+
+    def nearest(dotf, sub, m):
+        pn = F32(dotf(sub, sub))
+        best, arg = F32(np.finfo(F32).max), 0
+        for k in range(256):
+            cn = F32(dotf(cbm[m, k], cbm[m, k]))
+            dist = F32(float(F32(pn + cn)) - 2.0 * float(F32(dotf(sub, cbm[m, k]))))
+            if dist < best:
+                best, arg = dist, k
+        return arg
+
+    fma_differs = 0
+    for i in range(x.shape[0]):
+        for m in range(nsq):
+            sub = x[i, m * lsq:(m + 1) * lsq]
+            want = nearest(sdot, sub, m)
+            assert rec[i, 4 + m] == want, (i, m)
+            fma_differs += int(nearest(O.dot, sub, m) != want)
+    assert rec[:, :4].view(F32).ravel().tolist() == [0.0] * x.shape[0]  # offsets against zero centers
+    assert fma_differs > 0 if lsq > 1 else fma_differs == 0
