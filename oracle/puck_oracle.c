@@ -16,9 +16,10 @@
  *     It stands in for cblas_sdot and for every entry of sgemm_.  MKL's order
  *     is unspecified, so bit-parity with an MKL build is unpinned; the HIP
  *     engine's exact-arithmetic mode reproduces po_dot() bit for bit.
- *   - The one exception: the dot products of 1, 2 or 4 floats that pick the
- *     filter-PQ codes follow MKL's own order for those lengths
- *     (po_sdot_short, checked against MKL where it is installed).
+ *   - The one exception: nearest_center's dot products of 1, 2 or 4 floats
+ *     (the filter-PQ codes and the codebook k-means on sub-vectors) follow
+ *     MKL's own order for those lengths (po_sdot_short, checked against MKL
+ *     where it is installed).
  *   - Everything else (LUT, LUT lookup, heap, cell walk, rerank) follows the
  *     reference's own written order, which the reference fixes in source.
  *
@@ -771,22 +772,25 @@ int po_assign_cells(const float* coarse, const float* fine, uint32_t C, uint32_t
 
 /* nearest_center (puck/hierarchical_cluster/kmeans.cpp:221-258):
  * dist = (float)((double)(pn + cn) - 2.0 * (double)sdot(x, c)), strict '<'
- * argmin starting from FLT_MAX. */
+ * argmin starting from FLT_MAX; sdot is po_sdot_short (MKL's order for 1, 2
+ * or 4 floats, po_dot for longer vectors). */
 int po_nearest_center(uint32_t dim, const float* centroids, uint32_t k, const float* x,
                       uint32_t n, int32_t* assign, float* dis, int nthreads) {
     float* cnorm = (float*)malloc(sizeof(float) * k);
-    for (uint32_t c = 0; c < k; ++c) cnorm[c] = po_dot(centroids + (size_t)c * dim, centroids + (size_t)c * dim, dim);
+    for (uint32_t c = 0; c < k; ++c)
+        cnorm[c] = po_sdot_short(centroids + (size_t)c * dim, centroids + (size_t)c * dim, dim);
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(static) num_threads(nthreads)
 #endif
     for (int64_t j = 0; j < (int64_t)n; ++j) {
         const float* xj = x + (size_t)j * dim;
-        float pn = po_dot(xj, xj, dim);
+        float pn = po_sdot_short(xj, xj, dim);
         float best = FLT_MAX;
         uint32_t arg = 0;
         for (uint32_t c = 0; c < k; ++c) {
-            float d = (float)((double)(pn + cnorm[c]) - 2.0 * (double)po_dot(xj, centroids + (size_t)c * dim, dim));
+            float d = (float)((double)(pn + cnorm[c]) -
+                              2.0 * (double)po_sdot_short(xj, centroids + (size_t)c * dim, dim));
             if (d < best) { best = d; arg = c; }
         }
         assign[j] = (int32_t)arg;

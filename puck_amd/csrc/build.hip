@@ -6,6 +6,8 @@
 //                       nearest_center, kmeans.cpp:221-258, and
 //                       set_static_value_of_formula, quantization.cpp:244-258)
 //   pk_kmeans_argmin    nearest_center's argmin on a GEMM chunk (kmeans.cpp:240-254)
+//   pk_kmeans_short     nearest_center for points of 1, 2 or 4 floats with MKL's
+//                       order for those dot products (the filter-PQ codebook k-means)
 //   pk_finalize_cells   empty-cell sentinel and per-coarse min_dist_offset
 //                       (puck_index.cpp:287-306)
 //   pk_gather_*         permutation of records / features into memory order
@@ -328,6 +330,53 @@ extern "C" hipError_t pk_launch_kmeans_argmin(const float* ip, const float* pn, 
                                               uint32_t k, int32_t* assign, float* dis, hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(pk_kmeans_argmin, dim3((n + 3) / 4), dim3(256), 0, s, ip, pn, cn, n, k, assign, dis);
+    return hipGetLastError();
+}
+
+// nearest_center for points and centroids of 1, 2 or 4 floats (the
+// filter-PQ codebook k-means on sub-vectors): the reference takes one
+// cblas_sdot per (point, centroid) pair and per norm, so every dot product
+// here is pk_sdot_short; d = (float)((double)(pn + cn) - 2.0 * (double)dot),
+// strict '<' from FLT_MAX in centroid order.  One thread per point, the
+// centroids and their norms staged through LDS in tiles of KM_TILE.
+#define KM_TILE 1024u
+
+__global__ __launch_bounds__(256) void pk_kmeans_short(const float* __restrict__ x, uint32_t n, uint32_t dim,
+                                                       const float* __restrict__ cent, uint32_t k,
+                                                       int32_t* __restrict__ assign, float* __restrict__ dis) {
+    __shared__ float cs[KM_TILE * 4];
+    __shared__ float csn[KM_TILE];
+    const uint32_t pt = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = pt < n;
+    float xv[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    const float* xp = x + (uint64_t)(valid ? pt : 0) * dim;
+    for (uint32_t j = 0; j < dim; ++j) xv[j] = xp[j];
+    const float pn = pk_sdot_short(xv, xv, dim);
+    float best = FLT_MAX;
+    uint32_t arg = 0;
+    for (uint32_t c0 = 0; c0 < k; c0 += KM_TILE) {
+        const uint32_t ct = (k - c0) < KM_TILE ? (k - c0) : KM_TILE;
+        __syncthreads();
+        for (uint32_t e = threadIdx.x; e < ct * dim; e += blockDim.x) cs[e] = cent[(uint64_t)c0 * dim + e];
+        __syncthreads();
+        for (uint32_t c = threadIdx.x; c < ct; c += blockDim.x) csn[c] = pk_sdot_short(cs + c * dim, cs + c * dim, dim);
+        __syncthreads();
+        for (uint32_t c = 0; c < ct; ++c) {
+            const float d = (float)((double)(pn + csn[c]) - 2.0 * (double)pk_sdot_short(xv, cs + c * dim, dim));
+            if (d < best) { best = d; arg = c0 + c; }
+        }
+    }
+    if (valid) {
+        assign[pt] = (int32_t)arg;
+        if (dis) dis[pt] = best;
+    }
+}
+
+extern "C" hipError_t pk_launch_kmeans_short(const float* x, uint32_t n, uint32_t dim, const float* cent, uint32_t k,
+                                             int32_t* assign, float* dis, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (!(dim == 1 || dim == 2 || dim == 4) || k == 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pk_kmeans_short, dim3((n + 255) / 256), dim3(256), 0, s, x, n, dim, cent, k, assign, dis);
     return hipGetLastError();
 }
 

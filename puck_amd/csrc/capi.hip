@@ -71,6 +71,8 @@ hipError_t pk_launch_encode_filter(const float*, const uint32_t*, uint32_t, cons
                                    hipStream_t);
 hipError_t pk_launch_kmeans_argmin(const float*, const float*, const float*, uint32_t, uint32_t, int32_t*, float*,
                                    hipStream_t);
+hipError_t pk_launch_kmeans_short(const float*, uint32_t, uint32_t, const float*, uint32_t, int32_t*, float*,
+                                  hipStream_t);
 hipError_t pk_launch_finalize_cells(float*, const uint32_t*, uint32_t, uint32_t, float*, hipStream_t);
 hipError_t pk_launch_gather_rows(const uint8_t*, uint64_t, uint64_t, uint8_t*, uint64_t, uint32_t,
                                  const uint32_t*, uint32_t, hipStream_t);
@@ -1663,6 +1665,34 @@ int puckgpu_kmeans_assign(puckgpu_ctx* ctx, uint32_t dim, const float* centroids
     hipStream_t s = ctx->stream;
     int rc = 0;
     const float* cd = (const float*)dev_in(ctx, S_A0, centroids, 4ull * k * dim, device_ptrs, &rc);
+    if (rc) return rc;
+    if (dim == 1 || dim == 2 || dim == 4) {
+        // sub-vector k-means (filter-PQ codebooks): one sdot per pair, in
+        // MKL's order for such lengths, instead of a GEMM with K <= 4
+        const uint32_t CHS = 1u << 20;
+        float* xs = device_ptrs ? nullptr : (float*)scratch(ctx, S_A4, 4ull * (n < CHS ? n : CHS) * dim, &rc);
+        int32_t* ao = device_ptrs ? nullptr : (int32_t*)scratch(ctx, S_A5, 4ull * (n < CHS ? n : CHS), &rc);
+        float* dd = device_ptrs ? nullptr : (float*)scratch(ctx, S_A6, 4ull * (n < CHS ? n : CHS), &rc);
+        if (rc) return rc;
+        for (uint32_t p0 = 0; p0 < n; p0 += CHS) {
+            const uint32_t m = (n - p0) < CHS ? (n - p0) : CHS;
+            const float* xp = x + (size_t)p0 * dim;
+            if (!device_ptrs) {
+                PK_CHECK(hipMemcpyAsync(xs, xp, 4ull * m * dim, hipMemcpyHostToDevice, s));
+                xp = xs;
+            }
+            int32_t* a = device_ptrs ? assign_out + p0 : ao;
+            float* dv = device_ptrs ? (dist_out ? dist_out + p0 : nullptr) : dd;
+            PK_CHECK(pk_launch_kmeans_short(xp, m, dim, cd, k, a, dv, s));
+            if (!device_ptrs) {
+                PK_CHECK(hipMemcpyAsync(assign_out + p0, ao, 4ull * m, hipMemcpyDeviceToHost, s));
+                if (dist_out) PK_CHECK(hipMemcpyAsync(dist_out + p0, dd, 4ull * m, hipMemcpyDeviceToHost, s));
+                PK_CHECK(hipStreamSynchronize(s));
+            }
+        }
+        PK_CHECK(hipStreamSynchronize(s));
+        return PUCKGPU_OK;
+    }
     float* cn = (float*)scratch(ctx, S_A1, 4ull * k, &rc);
     if (rc) return rc;
     PK_CHECK(pk_launch_row_norms(cd, k, dim, 0, cn, s));

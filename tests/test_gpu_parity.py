@@ -396,6 +396,14 @@ def test_kmeans_assign_bitexact(ctx):
         ao, do = O.nearest_center(c, x)
         assert np.array_equal(ag, ao)
         assert np.array_equal(_bits(dg), _bits(do))
+    # sub-vector k-means on distances that cancel to rounding noise: the order
+    # of the short dot products (MKL's for 1, 2 or 4 floats) decides the argmin
+    for dim in (1, 2, 4):
+        _, _, cb, xs, _ = CI.near_tie_encode_input(41, 1, dim, 5000)
+        ag, dg = ctx.kmeans_assign(cb, xs, with_dist=True)
+        ao, do = O.nearest_center(cb, xs)
+        assert np.array_equal(ag, ao), dim
+        assert np.array_equal(_bits(dg), _bits(do)), dim
 
 
 @pytest.mark.parametrize("which", ["l2", "cos"])

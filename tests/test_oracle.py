@@ -444,3 +444,30 @@ def test_encode_codes_match_nearest_center_with_mkl(lsq, nsq):
             fma_differs += int(nearest(O.dot, sub, m) != want)
     assert rec[:, :4].view(F32).ravel().tolist() == [0.0] * x.shape[0]  # offsets against zero centers
     assert fma_differs > 0 if lsq > 1 else fma_differs == 0
+
+
+@needs_mkl
+@pytest.mark.parametrize("dim", [1, 2, 4])
+def test_nearest_center_short_dims_match_mkl(dim):
+    # the codebook k-means' assignment (nearest_center on sub-vectors) equals
+    # nearest_center evaluated with MKL's cblas_sdot, on points and centroids
+    # whose distances cancel to rounding noise
+    sdot = _mkl_sdot()
+    _, _, cb, x, _ = CI.near_tie_encode_input(41, 1, dim, 12)
+    a, dis = O.nearest_center(cb, x)
+    cn = [sdot(c, c) for c in cb]
+    fma_differs = 0
+    for i in range(x.shape[0]):
+        for dotf, is_mkl in ((sdot, True), (O.dot, False)):
+            pn = F32(dotf(x[i], x[i]))
+            best, arg = F32(np.finfo(F32).max), 0
+            for k in range(cb.shape[0]):
+                c_n = cn[k] if is_mkl else F32(O.dot(cb[k], cb[k]))
+                dist = F32(float(F32(pn + c_n)) - 2.0 * float(F32(dotf(x[i], cb[k]))))
+                if dist < best:
+                    best, arg = dist, k
+            if is_mkl:
+                assert a[i] == arg and dis[i].view(np.uint32) == best.view(np.uint32), i
+            else:
+                fma_differs += int(arg != a[i])
+    assert fma_differs > 0 if dim > 1 else fma_differs == 0
