@@ -1,18 +1,19 @@
 """Training pipeline of the learned probe (labels, initialisation, Adam
 loop) driving the device kernels through the C ABI.
 
-Fine budget b of a query: in each coarse list the walk scores only the b
-cells of smallest cell distance (include/puck_gpu.h, puckgpu_query_params).
-Label of a query: run the plain search (every cell, the reference's walk) at
-the operating search_coarse_count; of the exact top-k neighbours it returns,
-take the largest fine-budget rank of a neighbour's cell (puckgpu_budget_ranks);
-the label is that rank (1 when it returns none).  Under a budget >= the label
-those neighbours' cells are all in the budget; fewer points are offered, so
-the heap top and the pivot never fall below the plain walk's and every cell
-and point the plain walk scored in the budget is scored again, while the
-points with a filter distance below a neighbour's are a subset of the plain
-walk's (its pruning bounds are exact), so the neighbour stays in the filter
-set and in the top-k.  The network regresses log(label).
+Fine budget b of a query: in each coarse list the budget scan scores every
+point of the b cells of smallest cell distance (include/puck_gpu.h,
+puckgpu_query_params).  Label of a query: run the plain search (the
+reference's walk) at the operating search_coarse_count; of the exact top-k
+neighbours it returns, take the largest fine-budget rank of a neighbour's cell
+(puckgpu_budget_ranks); the label is that rank (1 when it returns none).  A
+budget >= the label scores those neighbours in full, and a neighbour among
+the 10 nearest points stays in the top-10 once it is in the filter set.  It
+leaves the filter set only if filter_topk scored points have smaller filter
+distances: the plain walk prunes whole cells by the distance of their centers
+(a heuristic for the points inside), so the scan can score points the walk
+skipped.  On the test indexes every neighbour is found again; the labels rely
+on that, it is not a guarantee.  The network regresses log(label).
 """
 import numpy as np
 import torch
