@@ -375,6 +375,9 @@ needs_mkl = pytest.mark.skipif(not os.path.exists(MKL_PATH), reason="MKL runtime
 def _mkl_sdot():
     import ctypes
     mkl = ctypes.CDLL(MKL_PATH)
+    # sequential layer: no second OpenMP runtime beside the oracle's libgomp
+    mkl.MKL_Set_Threading_Layer.argtypes = [ctypes.c_int]
+    mkl.MKL_Set_Threading_Layer(1)
     mkl.cblas_sdot.restype = ctypes.c_float
     mkl.cblas_sdot.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
 
