@@ -63,3 +63,33 @@ def test_init_rejects_ip2cos_index(tmp_path):
     r = subprocess.run([BIN, str(d), str(d / "q.f32"), "1", "16", "1", "10", "2", "-1", out], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode == 4 and "ip2cos" in r.stderr, (r.returncode, r.stderr)
+
+
+def test_adapter_host_paths_clean_under_sanitizers(index_dir, tmp_path):
+    # the adapter's file reading and refusals (host code only, built with g++)
+    # under AddressSanitizer / UndefinedBehaviorSanitizer / LeakSanitizer:
+    # valid files up to the missing device, refusals, truncated files
+    exe = str(tmp_path / "adapter_asan")
+    cc = subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined", "-fno-omit-frame-pointer",
+                         "-o", exe, os.path.join(ROOT, "tests", "cpp", "host_adapter_main.cpp"),
+                         os.path.join(ROOT, "puck_amd", "host", "gpu_puck_index.cpp"),
+                         "-L" + os.path.join(ROOT, "puck_amd"), "-lpuckgpu",
+                         "-Wl,-rpath," + os.path.join(ROOT, "puck_amd")], capture_output=True, text=True, timeout=300)
+    if cc.returncode != 0:
+        pytest.skip("g++ sanitizer runtime unavailable: " + cc.stderr[-200:])
+    d, qf = index_dir
+    cut = {}
+    for name, victim, keep in (("data", FL.get("filter_data_file_name"), -7), ("conf", FL.get("index_file_name"), 20)):
+        cut[name] = tmp_path / ("trunc_" + name)
+        cut[name].mkdir()
+        for f in os.listdir(d):
+            b = open(os.path.join(d, f), "rb").read()
+            open(os.path.join(cut[name], f), "wb").write(b[:keep] if f == victim else b)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1", UBSAN_OPTIONS="print_stacktrace=1")
+    out = str(tmp_path / "out.bin")
+    for path, flags_topk, ftk in ((d, "200", "-1"), (d, "10", "5"), (d, "100", "-1"), (cut["data"], "100", "-1"),
+                                  (cut["conf"], "100", "-1")):
+        r = subprocess.run([exe, str(path), str(qf), "2", "16", "10", flags_topk, "2", ftk, out], capture_output=True,
+                           text=True, timeout=120, env=env)
+        assert "Sanitizer" not in r.stderr and "runtime error" not in r.stderr, r.stderr[-2000:]
+        assert r.returncode in (0, 4), (path, r.returncode, r.stderr[-500:])
