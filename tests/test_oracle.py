@@ -474,3 +474,21 @@ def test_nearest_center_short_dims_match_mkl(dim):
             else:
                 fma_differs += int(arg != a[i])
     assert fma_differs > 0 if dim > 1 else fma_differs == 0
+
+
+def test_oracle_clean_under_sanitizers(tmp_path):
+    # the checker itself: walk, budget scan, batch, heap, assignment, encoding
+    # and nearest_center under AddressSanitizer / UBSan / LeakSanitizer
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "oracle_sanitize")
+    cc = subprocess.run(["gcc", "-O1", "-g", "-fopenmp", "-ffp-contract=off", "-fsanitize=address,undefined",
+                         "-fno-omit-frame-pointer", "-o", exe, os.path.join(root, "tests", "cpp", "oracle_sanitize_main.c"),
+                         os.path.join(root, "oracle", "puck_oracle.c"), "-lm"], capture_output=True, text=True,
+                        timeout=300)
+    if cc.returncode != 0:
+        pytest.skip("gcc sanitizer runtime unavailable: " + cc.stderr[-200:])
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300,
+                       env=dict(os.environ, ASAN_OPTIONS="detect_leaks=1", OMP_NUM_THREADS="2"))
+    assert r.returncode == 0 and "Sanitizer" not in r.stderr and "runtime error" not in r.stderr, r.stderr[-2000:]
+    assert "oracle driver done" in r.stdout
