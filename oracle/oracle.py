@@ -71,6 +71,12 @@ def lib():
                                 ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32, _f32p, _u32p,
                                 ctypes.POINTER(ctypes.c_uint32), _f32p, _u32p,
                                 ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(Stats), _i32p, _i32p]
+    L.po_search_one_given.restype = ctypes.c_int
+    L.po_search_one_given.argtypes = [vp, _f32p, _f32p, _f32p, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32,
+                                      ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, _f32p, _u32p,
+                                      ctypes.POINTER(ctypes.c_uint32)]
+    L.po_index_set_derived.restype = None
+    L.po_index_set_derived.argtypes = [vp, _f32p, _f32p]
     L.po_search_batch.restype = ctypes.c_int
     L.po_search_batch.argtypes = [vp, ctypes.c_uint32, _f32p, ctypes.c_uint32, ctypes.c_uint32,
                                   ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, _f32p, _u32p,
@@ -206,6 +212,24 @@ class OracleIndex:
                    filt_mem=ft[:ftk], filt_n=fn.value, stats=st.as_dict(),
                    coarse=co[:min(scc, self.C)], fine=fo)
         return out
+
+    def set_derived(self, coarse_norms, stationary):
+        """Replace the coarse half-norms and stationary values (e.g. with a
+        BLAS's results); empty cells and min_dist_offset are re-derived."""
+        lib().po_index_set_derived(self._h, c32(coarse_norms).reshape(-1), c32(stationary).reshape(-1))
+
+    def search_one_given(self, q, coarse_ip, fine_ip, qn, topk, scc, ftk, max_topk=None, radius_rate=1.0):
+        """The walk and rerank for a prepared query with its center inner
+        products and squared norm supplied.  Returns (dist, ids, n)."""
+        max_topk = topk if max_topk is None else max_topk
+        D = np.zeros(max(topk, 1), np.float32)
+        I = np.zeros(max(topk, 1), np.uint32)
+        n = ctypes.c_uint32(0)
+        rc = lib().po_search_one_given(self._h, c32(q), c32(coarse_ip), c32(fine_ip), float(qn), topk, max_topk, scc,
+                                       ftk, radius_rate, D, I, ctypes.byref(n))
+        if rc != 0:
+            raise ValueError("po_search_one_given refused its arguments")
+        return D[:topk], I[:topk], n.value
 
     def search_batch(self, Q, topk, scc, ftk, max_topk=None, radius_rate=1.0, nthreads=0):
         Q = c32(Q)

@@ -380,6 +380,23 @@ void po_index_export(void* p, float* coarse_norms, float* stationary, uint32_t* 
     if (mem2local) memcpy(mem2local, ix->mem2local, sizeof(uint32_t) * ix->N);
 }
 
+/* Test hook: replace the index's coarse half-norms and stationary values with
+ * ones computed elsewhere (e.g. by a BLAS in its own summation order), then
+ * re-apply the empty-cell sentinel and min_dist_offset as po_index_create
+ * does (puck_index.cpp:287-306). */
+void po_index_set_derived(void* p, const float* coarse_norms, const float* stationary) {
+    po_index* ix = (po_index*)p;
+    uint32_t cells = ix->C * ix->F;
+    memcpy(ix->coarse_norms, coarse_norms, sizeof(float) * ix->C);
+    memcpy(ix->stationary, stationary, sizeof(float) * cells);
+    for (uint32_t c = 0; c < ix->C; ++c) ix->min_dist_offset[c] = FLT_MAX;
+    for (uint32_t i = 0; i < cells; ++i) {
+        uint32_t c = i / ix->F;
+        if (ix->cell_start[i] == ix->cell_start[i + 1]) ix->stationary[i] = sqrtf(FLT_MAX);
+        if (ix->stationary[i] < ix->min_dist_offset[c]) ix->min_dist_offset[c] = ix->stationary[i];
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Query normalisation: HierarchicalClusterIndex::normalization
  * (hierarchical_cluster_index.cpp:1513-1536), whether_norm branch:
@@ -511,35 +528,16 @@ static int po_search_budget(const po_index* ix, const float* q, uint32_t topk, u
     return 0;
 }
 
-/* One query: PuckIndex::search (puck_index.cpp:559-600) with whether_pq=false.
- * Returns 0, or -1 for topk > max_topk / null feature / zero-norm query.
- * filt_val/filt_tag (ftk each) receive the filter heap as rank_topN_points
- * reads it (heap order when full, ascending when not full).
- * fine_budget b: 0 or >= F is the reference's walk; 0 < b < F is the budget
- * scan of po_search_budget (an extension with no counterpart in the
- * reference). */
-int po_search_one(const void* p, const float* feature, uint32_t topk, uint32_t max_topk,
-                  uint32_t scc, uint32_t ftk, double radius_rate, uint32_t fine_budget,
-                  float* out_dist, uint32_t* out_ids, uint32_t* out_n,
-                  float* filt_val, uint32_t* filt_tag, uint32_t* filt_n,
-                  po_stats* st, int* coarse_out, int* fine_out) {
-    const po_index* ix = (const po_index*)p;
+/* The reference walk and rerank for one prepared query (puck_index.cpp:485-600,
+ * hierarchical_cluster_index.cpp:700-727).  coarse_ip / fine_ip / qn_in: the
+ * query's inner products with every coarse and fine center and its squared
+ * norm, or NULL to take them with po_dot. */
+static void po_walk_query(const po_index* ix, const float* q, uint32_t topk, uint32_t scc, uint32_t ftk,
+                          double radius_rate, const float* coarse_ip, const float* fine_ip, const float* qn_in,
+                          float* out_dist, uint32_t* out_ids, uint32_t* out_n,
+                          float* filt_val, uint32_t* filt_tag, uint32_t* filt_n,
+                          po_stats* st, int* coarse_out, int* fine_out) {
     uint32_t dim = ix->dim, C = ix->C, F = ix->F, nsq = ix->nsq;
-    /* puck_index.cpp:560-563 */
-    if (topk > max_topk || feature == NULL) return -1;
-    if (scc > C) scc = C;   /* index_conf.cpp:64 search_coarse_count = min(flag, C) */
-    float* q = (float*)malloc(sizeof(float) * dim);
-    if (ix->whether_norm) {
-        if (po_normalize(feature, dim, q) != 0) { free(q); return -1; }
-    } else {
-        memcpy(q, feature, sizeof(float) * dim);
-    }
-    if (fine_budget != 0 && fine_budget < F) {
-        int rc = po_search_budget(ix, q, topk, scc, ftk, fine_budget, out_dist, out_ids, out_n, filt_val,
-                                  filt_tag, filt_n, st);
-        free(q);
-        return rc;
-    }
     /* search_nearest_coarse_cluster (hierarchical_cluster_index.cpp:700-727) */
     float* coarse_d = (float*)malloc(sizeof(float) * (scc ? scc : 1));
     uint32_t* coarse_t = (uint32_t*)malloc(sizeof(uint32_t) * (scc ? scc : 1));
@@ -547,7 +545,7 @@ int po_search_one(const void* p, const float* feature, uint32_t topk, uint32_t m
         po_heap h;
         po_heap_init(&h, scc, coarse_d, coarse_t);
         for (uint32_t c = 0; c < C; ++c) {
-            float ip = po_dot(ix->coarse + (size_t)c * dim, q, dim);
+            float ip = coarse_ip ? coarse_ip[c] : po_dot(ix->coarse + (size_t)c * dim, q, dim);
             float t = ix->coarse_norms[c] - ip;
             if (scc > 0 && t < coarse_d[0]) po_heap_update(&h, t, c);
         }
@@ -564,7 +562,7 @@ int po_search_one(const void* p, const float* feature, uint32_t topk, uint32_t m
         po_heap h;
         po_heap_init(&h, F, fine_d, fine_t);
         for (uint32_t k = 0; k < F; ++k)
-            po_heap_update(&h, -po_dot(ix->fine + (size_t)k * dim, q, dim), k);
+            po_heap_update(&h, -(fine_ip ? fine_ip[k] : po_dot(ix->fine + (size_t)k * dim, q, dim)), k);
         po_heap_reorder(&h);
     }
     if (fine_out) for (uint32_t k = 0; k < F; ++k) fine_out[k] = (int)fine_t[k];
@@ -573,7 +571,7 @@ int po_search_one(const void* p, const float* feature, uint32_t topk, uint32_t m
     uint32_t* ftag = (uint32_t*)malloc(sizeof(uint32_t) * ftk);
     po_heap fh;
     po_heap_init(&fh, ftk, fval, ftag);
-    float qn = po_dot(q, q, dim);
+    float qn = qn_in ? *qn_in : po_dot(q, q, dim);
     float pivot = (float)((double)(fval[0] - qn) / radius_rate / 2.0);
     size_t rec = 4 + nsq;
     po_stats s = {0, 0, 0, 0};
@@ -632,8 +630,56 @@ int po_search_one(const void* p, const float* feature, uint32_t topk, uint32_t m
         *out_n = po_heap_count(&rh);
     }
     if (st) *st = s;
-    free(q); free(coarse_d); free(coarse_t); free(lut); free(fine_d); free(fine_t);
+    free(coarse_d); free(coarse_t); free(lut); free(fine_d); free(fine_t);
     free(fval); free(ftag);
+}
+
+/* One query: PuckIndex::search (puck_index.cpp:559-600) with whether_pq=false.
+ * Returns 0, or -1 for topk > max_topk / null feature / zero-norm query.
+ * filt_val/filt_tag (ftk each) receive the filter heap as rank_topN_points
+ * reads it (heap order when full, ascending when not full).
+ * fine_budget b: 0 or >= F is the reference's walk; 0 < b < F is the budget
+ * scan of po_search_budget (an extension with no counterpart in the
+ * reference). */
+int po_search_one(const void* p, const float* feature, uint32_t topk, uint32_t max_topk,
+                  uint32_t scc, uint32_t ftk, double radius_rate, uint32_t fine_budget,
+                  float* out_dist, uint32_t* out_ids, uint32_t* out_n,
+                  float* filt_val, uint32_t* filt_tag, uint32_t* filt_n,
+                  po_stats* st, int* coarse_out, int* fine_out) {
+    const po_index* ix = (const po_index*)p;
+    uint32_t dim = ix->dim, C = ix->C, F = ix->F;
+    /* puck_index.cpp:560-563 */
+    if (topk > max_topk || feature == NULL) return -1;
+    if (scc > C) scc = C;   /* index_conf.cpp:64 search_coarse_count = min(flag, C) */
+    float* q = (float*)malloc(sizeof(float) * dim);
+    if (ix->whether_norm) {
+        if (po_normalize(feature, dim, q) != 0) { free(q); return -1; }
+    } else {
+        memcpy(q, feature, sizeof(float) * dim);
+    }
+    if (fine_budget != 0 && fine_budget < F) {
+        int rc = po_search_budget(ix, q, topk, scc, ftk, fine_budget, out_dist, out_ids, out_n, filt_val,
+                                  filt_tag, filt_n, st);
+        free(q);
+        return rc;
+    }
+    po_walk_query(ix, q, topk, scc, ftk, radius_rate, NULL, NULL, NULL, out_dist, out_ids, out_n, filt_val,
+                  filt_tag, filt_n, st, coarse_out, fine_out);
+    free(q);
+    return 0;
+}
+
+/* Test hook: po_search_one's walk and rerank for a prepared (already
+ * normalised) query whose center inner products and squared norm are given,
+ * e.g. computed by a BLAS in its own summation order. */
+int po_search_one_given(const void* p, const float* q, const float* coarse_ip, const float* fine_ip, float qn,
+                        uint32_t topk, uint32_t max_topk, uint32_t scc, uint32_t ftk, double radius_rate,
+                        float* out_dist, uint32_t* out_ids, uint32_t* out_n) {
+    const po_index* ix = (const po_index*)p;
+    if (topk > max_topk || q == NULL || coarse_ip == NULL || fine_ip == NULL) return -1;
+    if (scc > ix->C) scc = ix->C;
+    po_walk_query(ix, q, topk, scc, ftk, radius_rate, coarse_ip, fine_ip, &qn, out_dist, out_ids, out_n, NULL,
+                  NULL, NULL, NULL, NULL, NULL);
     return 0;
 }
 
