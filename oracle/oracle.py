@@ -77,6 +77,9 @@ def lib():
                                       ctypes.POINTER(ctypes.c_uint32)]
     L.po_index_set_derived.restype = None
     L.po_index_set_derived.argtypes = [vp, _f32p, _f32p]
+    L.po_assign_cells_given.restype = ctypes.c_int
+    L.po_assign_cells_given.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p, _f32p, ctypes.c_uint32,
+                                        _f32p, _f32p, _f32p, _u32p, _f32p, ctypes.c_int]
     L.po_search_batch.restype = ctypes.c_int
     L.po_search_batch.argtypes = [vp, ctypes.c_uint32, _f32p, ctypes.c_uint32, ctypes.c_uint32,
                                   ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, _f32p, _u32p,
@@ -275,6 +278,20 @@ def assign_cells(coarse, fine, X, nthreads=0):
     dist = np.zeros(n, np.float32)
     lib().po_assign_cells(coarse.reshape(-1), fine.reshape(-1), C, F, d, X.reshape(-1), n, cell, dist,
                           nthreads)
+    return cell, dist
+
+
+def assign_cells_given(cn, st, fbound, cip, fip, xn, nthreads=0):
+    """po_assign_cells on supplied arithmetic: coarse half-norms cn [C],
+    stationary st [C*F], fbound [F], inner products cip [n, C], fip [n, F] and
+    half squared norms xn [n].  Returns (cell, dist)."""
+    cip, fip = c32(cip), c32(fip)
+    n, C = cip.shape
+    F = fip.shape[1]
+    cell = np.zeros(n, np.uint32)
+    dist = np.zeros(n, np.float32)
+    lib().po_assign_cells_given(C, F, c32(cn), c32(st).reshape(-1), c32(fbound), n, cip.reshape(-1),
+                                fip.reshape(-1), c32(xn), cell, dist, nthreads)
     return cell, dist
 
 

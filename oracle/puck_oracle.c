@@ -763,6 +763,33 @@ static int po_pair_cmp(const void* pa, const void* pb) {
     return a->idx < b->idx ? -1 : (a->idx > b->idx);
 }
 
+/* nearest_cell_assign (hierarchical_cluster_index.cpp:1042-1085) for one point
+ * from its inner products with every coarse (cip) and fine (fip) center and
+ * its half squared norm xn; cn are the coarse half-norms, st stationary,
+ * fbound = -sqrt(fine half-norm). */
+static void po_assign_one(uint32_t C, uint32_t F, const float* cn, const float* st, const float* fbound,
+                          const float* cip, const float* fip, float xn, po_pair* pairs, uint32_t* cell_out,
+                          float* dist_out) {
+    for (uint32_t c = 0; c < C; ++c) {
+        pairs[c].idx = (int)c;
+        pairs[c].term = cn[c] - cip[c] + xn;
+    }
+    qsort(pairs, C, sizeof(po_pair), po_pair_cmp);
+    int best_cell = -1;
+    float best = FLT_MAX;
+    for (uint32_t j = 0; j < C; ++j) {
+        int c = pairs[j].idx;
+        float ct = pairs[j].term;
+        for (uint32_t f = 0; f < F; ++f) {
+            if (sqrtf(best) - fbound[f] < sqrtf(ct)) break;
+            float score = ct - fip[f] + st[(size_t)c * F + f];
+            if (score < best) { best = score; best_cell = c * (int)F + (int)f; }
+        }
+    }
+    *cell_out = (uint32_t)best_cell;
+    if (dist_out) *dist_out = best;
+}
+
 int po_assign_cells(const float* coarse, const float* fine, uint32_t C, uint32_t F, uint32_t dim,
                     const float* x, uint32_t n, uint32_t* cell_out, float* dist_out, int nthreads) {
     float* cn = (float*)malloc(sizeof(float) * C);
@@ -783,6 +810,7 @@ int po_assign_cells(const float* coarse, const float* fine, uint32_t C, uint32_t
 #endif
     {
         po_pair* pairs = (po_pair*)malloc(sizeof(po_pair) * C);
+        float* cip = (float*)malloc(sizeof(float) * C);
         float* fdist = (float*)malloc(sizeof(float) * F);
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 64)
@@ -790,29 +818,37 @@ int po_assign_cells(const float* coarse, const float* fine, uint32_t C, uint32_t
         for (int64_t i = 0; i < (int64_t)n; ++i) {
             const float* xi = x + (size_t)i * dim;
             float xn = po_dot(xi, xi, dim) / 2;
-            for (uint32_t c = 0; c < C; ++c) {
-                pairs[c].idx = (int)c;
-                pairs[c].term = cn[c] - po_dot(coarse + (size_t)c * dim, xi, dim) + xn;
-            }
+            for (uint32_t c = 0; c < C; ++c) cip[c] = po_dot(coarse + (size_t)c * dim, xi, dim);
             for (uint32_t f = 0; f < F; ++f) fdist[f] = po_dot(fine + (size_t)f * dim, xi, dim);
-            qsort(pairs, C, sizeof(po_pair), po_pair_cmp);
-            int best_cell = -1;
-            float best = FLT_MAX;
-            for (uint32_t j = 0; j < C; ++j) {
-                int c = pairs[j].idx;
-                float ct = pairs[j].term;
-                for (uint32_t f = 0; f < F; ++f) {
-                    if (sqrtf(best) - fbound[f] < sqrtf(ct)) break;
-                    float score = ct - fdist[f] + st[(size_t)c * F + f];
-                    if (score < best) { best = score; best_cell = c * (int)F + (int)f; }
-                }
-            }
-            cell_out[i] = (uint32_t)best_cell;
-            if (dist_out) dist_out[i] = best;
+            po_assign_one(C, F, cn, st, fbound, cip, fdist, xn, pairs, cell_out + i, dist_out ? dist_out + i : NULL);
         }
-        free(pairs); free(fdist);
+        free(pairs); free(cip); free(fdist);
     }
     free(cn); free(fnorm); free(fbound); free(st);
+    return 0;
+}
+
+/* Test hook: po_assign_cells with every arithmetic input supplied, e.g.
+ * computed by a BLAS in its own summation order: coarse half-norms cn [C],
+ * stationary st [C*F], fbound [F], per point the inner products cip [n*C] and
+ * fip [n*F] (row per point) and the half squared norm xn [n]. */
+int po_assign_cells_given(uint32_t C, uint32_t F, const float* cn, const float* st, const float* fbound, uint32_t n,
+                          const float* cip, const float* fip, const float* xn, uint32_t* cell_out, float* dist_out,
+                          int nthreads) {
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+        po_pair* pairs = (po_pair*)malloc(sizeof(po_pair) * C);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 64)
+#endif
+        for (int64_t i = 0; i < (int64_t)n; ++i)
+            po_assign_one(C, F, cn, st, fbound, cip + (size_t)i * C, fip + (size_t)i * F, xn[i], pairs, cell_out + i,
+                          dist_out ? dist_out + i : NULL);
+        free(pairs);
+    }
     return 0;
 }
 

@@ -146,3 +146,59 @@ def test_results_match_emulated_mkl_build(mkl, l2):
         same += int(np.sum(Im[:nm] == o["ids"][:nm]))
         total += nm
     assert same / total >= 0.999
+
+
+@pytest.mark.parametrize("l2", [False, True])
+def test_cell_assignment_matches_emulated_mkl_build(mkl, l2):
+    # nearest_cell_assign (hierarchical_cluster_index.cpp:1042-1138) on MKL's
+    # arithmetic -- sgemm_ inner products per chunk (the sequential FMA chain,
+    # see below), cblas_sdot half-norms (another order), stationary and the
+    # fine bound from read_coodbooks / :1186-1194 -- gives the oracle's cells
+    # (which the engine reproduces bit for bit); a point may differ only where
+    # its two cells are equally far in FP64
+    gen, d = (D.mixture_uint8_grid, 64) if l2 else (D.unit_norm_mixture, 96)
+    x = gen(20000, d, 200, seed=97, means_seed=98)
+    ix = CI.build_index(x, 48, 48, 16, whether_norm=not l2, train_n=20000)
+    coarse, fine = ix["coarse"], ix["fine"]
+    C, F = coarse.shape[0], fine.shape[0]
+    cell_own, _ = O.assign_cells(coarse, fine, x)
+    cn = np.array([_sdot(mkl, c, c) / F32(2) for c in coarse], F32)
+    fn = np.array([_sdot(mkl, g, g) / F32(2) for g in fine], F32)
+    fbound = (F32(0) - np.sqrt(fn)).astype(F32)
+    st = (fn[None, :] + _sgemm_tn(mkl, fine, coarse, F, C, d).reshape(C, F)).astype(F32)
+    xs = np.ascontiguousarray(x, F32)
+    cip = _sgemm_tn(mkl, coarse, xs, C, xs.shape[0], d).reshape(-1, C)
+    fip = _sgemm_tn(mkl, fine, xs, F, xs.shape[0], d).reshape(-1, F)
+    xn = np.array([_sdot(mkl, v, v) / F32(2) for v in xs], F32)
+    own_n = np.array([F32(O.dot(v, v)) / F32(2) for v in np.concatenate([coarse, fine])], F32)
+    assert np.mean(np.concatenate([cn, fn]).view(np.uint32) != own_n.view(np.uint32)) > 0.3   # MKL's sdot differs
+    cell_mkl, _ = O.assign_cells_given(cn, st, fbound, cip, fip, xn)
+    diff = np.nonzero(cell_mkl != cell_own)[0]
+    assert diff.size <= 0.001 * x.shape[0]
+    for i in diff:
+        a, b = int(cell_mkl[i]), int(cell_own[i])
+        da = np.sum((xs[i].astype(np.float64) - coarse[a // F] - fine[a % F]) ** 2)
+        db = np.sum((xs[i].astype(np.float64) - coarse[b // F] - fine[b % F]) ** 2)
+        assert abs(da - db) <= 1e-6 * max(da, db), i
+
+
+def test_multi_column_sgemm_is_the_fma_chain(mkl):
+    # MKL's sgemm_ "TN" with six or more columns accumulates every entry as a
+    # sequential fused multiply-add chain from +0 -- the engine's and the
+    # oracle's order -- on this host; with one to five columns (the search's
+    # single-query products) it takes another order.  The reference's
+    # multi-column calls are stationary's product (C columns) and the build
+    # assignment's chunks of points.
+    rng = np.random.default_rng(99)
+    for n, k in ((6, 96), (8, 128), (64, 96), (1000, 64), (256, 128)):
+        A = rng.standard_normal((300, k)).astype(F32)
+        B = rng.standard_normal((n, k)).astype(F32)
+        R = _sgemm_tn(mkl, A, B, 300, n, k).reshape(n, 300)
+        for j in rng.choice(n, min(n, 8), replace=False):
+            own = np.array([O.dot(A[i], B[j]) for i in range(300)], F32)
+            assert np.array_equal(R[j].view(np.uint32), own.view(np.uint32)), (n, k, j)
+    A = rng.standard_normal((300, 96)).astype(F32)
+    b = rng.standard_normal((1, 96)).astype(F32)
+    one = _sgemm_tn(mkl, A, b, 300, 1, 96)
+    own = np.array([O.dot(A[i], b[0]) for i in range(300)], F32)
+    assert np.mean(one.view(np.uint32) != own.view(np.uint32)) > 0.3
