@@ -13,9 +13,11 @@
  *
  * Floating-point conventions (the parts the reference leaves to MKL):
  *   - po_dot() is a sequential fused-multiply-add chain starting from +0.0f.
- *     It stands in for cblas_sdot and for every entry of sgemm_.  MKL's order
- *     is unspecified, so bit-parity with an MKL build is unpinned; the HIP
- *     engine's exact-arithmetic mode reproduces po_dot() bit for bit.
+ *     It stands in for cblas_sdot and for every entry of sgemm_; the HIP
+ *     engine reproduces it bit for bit.  MKL's sgemm_ with six or more
+ *     columns computes exactly this chain (checked by calling it); its
+ *     single-column sgemm_ and cblas_sdot of long vectors take other,
+ *     unspecified orders, so those values are unpinned against an MKL build.
  *   - The one exception: nearest_center's dot products of 1, 2 or 4 floats
  *     (the filter-PQ codes and the codebook k-means on sub-vectors) follow
  *     MKL's own order for those lengths (po_sdot_short, checked against MKL
