@@ -289,7 +289,8 @@ def test_encode_filter_reconstruction(small_l2):
 
 
 # ---------------------------------------------------------------- recall KATs
-KATS = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "recall_kats.json")))
+with open(os.path.join(os.path.dirname(__file__), "golden", "recall_kats.json")) as _f:
+    KATS = json.load(_f)
 
 
 @pytest.mark.parametrize("kat", KATS, ids=[k["name"] for k in KATS])
