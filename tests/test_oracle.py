@@ -493,3 +493,13 @@ def test_oracle_clean_under_sanitizers(tmp_path):
                        env=dict(os.environ, ASAN_OPTIONS="detect_leaks=1", OMP_NUM_THREADS="2"))
     assert r.returncode == 0 and "Sanitizer" not in r.stderr and "runtime error" not in r.stderr, r.stderr[-2000:]
     assert "oracle driver done" in r.stdout
+
+
+def test_reference_build_keeps_subnormals():
+    # oracle/_ref is compiled with the reference's -Ofast but linked without it:
+    # an -Ofast link adds crtfastmath.o, whose constructor would switch this
+    # process (and every later oracle and numpy computation) to flush-to-zero
+    if O.ref_lib() is None:
+        pytest.skip("oracle/_ref not built (reference checkout absent)")
+    tiny = np.float32(1e-39)
+    assert tiny * np.float32(1.0) == tiny and tiny != 0
