@@ -67,3 +67,23 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     msg = json.loads(r.stdout.strip().splitlines()[-1])
     assert msg["n_gpus"] == 2 and "one GPU" in msg["error"]
+
+
+def test_binding_argument_checks_before_any_device_call():
+    # the Python binding refuses bad arrays before it reaches the library
+    import types
+    import numpy as np
+    from puck_amd import engine as E
+    with pytest.raises(TypeError):
+        E._ptr(np.zeros(3, np.float64), np.float32)
+    with pytest.raises(ValueError):
+        E._ptr(np.zeros((4, 4), np.float32)[:, ::2])
+    assert E._mode(np.zeros(2, np.float32), None) == 0
+    dev = types.SimpleNamespace(is_cuda=True)            # stands in for a device tensor
+    assert E._mode(dev, None, dev) == 1
+    with pytest.raises(ValueError):
+        E._mode(np.zeros(2, np.float32), dev)           # host and device arrays mixed
+    sv = object.__new__(E.Server)                       # no device: only the shape check runs
+    sv.max_batch, sv.index = 8, types.SimpleNamespace(dim=4)
+    with pytest.raises(ValueError):
+        sv.tune(np.zeros((4, 4), np.float32))           # the largest bucket reads max_batch rows
