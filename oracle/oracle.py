@@ -36,6 +36,15 @@ def build(force=False):
         subprocess.check_call(["make", "-s", "-C", HERE, "ref"])
 
 
+class WalkStep(ctypes.Structure):
+    """po_walk_step: one considered cell of the walk (decision 0 break,
+    1 continue, 2 scanned)."""
+    _fields_ = [("l", ctypes.c_uint32), ("idx", ctypes.c_uint32), ("k", ctypes.c_uint32),
+                ("coarse", ctypes.c_uint32), ("decision", ctypes.c_int32), ("pivot_before", ctypes.c_float),
+                ("top_before", ctypes.c_float), ("max_st_before", ctypes.c_float), ("pivot_after", ctypes.c_float),
+                ("max_st_after", ctypes.c_float), ("offered", ctypes.c_uint32), ("updated", ctypes.c_uint32)]
+
+
 _lib = None
 
 
@@ -45,7 +54,14 @@ def lib():
         return _lib
     if not os.path.exists(_LIB_PATH):
         build()
-    L = ctypes.CDLL(_LIB_PATH)
+    _lib = load_variant(_LIB_PATH)
+    return _lib
+
+
+def load_variant(path):
+    """Load a build of puck_oracle.c at `path` (liboracle.so, or a variant
+    compiled by a test) with its ctypes signatures."""
+    L = ctypes.CDLL(path)
     vp = ctypes.c_void_p
     L.po_sentinel_value.restype = ctypes.c_float
     L.po_dot.restype = ctypes.c_float
@@ -96,7 +112,11 @@ def lib():
     L.po_encode_filter.restype = ctypes.c_int
     L.po_encode_filter.argtypes = [_f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32, _f32p,
                                    ctypes.c_uint32, _f32p, _u32p, ctypes.c_uint32, _u8p, ctypes.c_int]
-    _lib = L
+    L.po_walk_trace.restype = ctypes.c_int
+    L.po_walk_trace.argtypes = [vp, _f32p, _f32p, _f32p, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32,
+                                ctypes.c_double, ctypes.POINTER(WalkStep), ctypes.c_uint32,
+                                ctypes.POINTER(ctypes.c_uint32), _f32p, _u32p, ctypes.POINTER(ctypes.c_uint32),
+                                ctypes.POINTER(Stats)]
     return L
 
 
@@ -167,7 +187,8 @@ def normalize(q):
 class OracleIndex:
     """CPU restatement of a loaded PuckIndex (memory order, derived arrays)."""
 
-    def __init__(self, coarse, fine, codebook, cell_assign, records, raw, whether_norm):
+    def __init__(self, coarse, fine, codebook, cell_assign, records, raw, whether_norm, L=None):
+        self._L = lib() if L is None else L
         self.coarse, self.fine, self.codebook = c32(coarse), c32(fine), c32(codebook)
         self.C, self.dim = self.coarse.shape
         self.F = self.fine.shape[0]
@@ -175,7 +196,7 @@ class OracleIndex:
         self.nsq = records.shape[1] - 4 if records.ndim == 2 else None
         self.whether_norm = bool(whether_norm)
         rec = np.ascontiguousarray(records, np.uint8).reshape(-1)
-        self._h = lib().po_index_create(self.dim, self.C, self.F, self.nsq, self.N,
+        self._h = self._L.po_index_create(self.dim, self.C, self.F, self.nsq, self.N,
                                         int(whether_norm), self.coarse, self.fine, self.codebook,
                                         np.ascontiguousarray(cell_assign, np.uint32), rec,
                                         c32(raw).reshape(-1))
@@ -184,7 +205,7 @@ class OracleIndex:
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().po_index_destroy(self._h)
+            self._L.po_index_destroy(self._h)
             self._h = None
 
     def export(self):
@@ -194,7 +215,7 @@ class OracleIndex:
         cs = np.zeros(C * F + 1, np.uint32)
         mdo = np.zeros(C, np.float32)
         m2l = np.zeros(max(N, 1), np.uint32)
-        lib().po_index_export(self._h, cn, st, cs, mdo, m2l)
+        self._L.po_index_export(self._h, cn, st, cs, mdo, m2l)
         return dict(coarse_norms=cn, stationary=st, cell_start=cs, min_dist_offset=mdo,
                     mem2local=m2l[:N])
 
@@ -209,7 +230,7 @@ class OracleIndex:
         st = Stats()
         co = np.zeros(max(scc, 1), np.int32)
         fo = np.zeros(self.F, np.int32)
-        rc = lib().po_search_one(self._h, c32(q), topk, max_topk, scc, ftk, radius_rate, fine_budget, D, I,
+        rc = self._L.po_search_one(self._h, c32(q), topk, max_topk, scc, ftk, radius_rate, fine_budget, D, I,
                                  ctypes.byref(n), fv, ft, ctypes.byref(fn), ctypes.byref(st), co, fo)
         out = dict(rc=rc, dist=D[:topk], ids=I[:topk], n=n.value, filt_val=fv[:ftk],
                    filt_mem=ft[:ftk], filt_n=fn.value, stats=st.as_dict(),
@@ -219,7 +240,24 @@ class OracleIndex:
     def set_derived(self, coarse_norms, stationary):
         """Replace the coarse half-norms and stationary values (e.g. with a
         BLAS's results); empty cells and min_dist_offset are re-derived."""
-        lib().po_index_set_derived(self._h, c32(coarse_norms).reshape(-1), c32(stationary).reshape(-1))
+        self._L.po_index_set_derived(self._h, c32(coarse_norms).reshape(-1), c32(stationary).reshape(-1))
+
+    def walk_trace(self, q, coarse_ip, fine_ip, qn, scc, ftk, radius_rate=1.0, cap=100000):
+        """The walk of a prepared query with supplied center inner products and
+        squared norm, with its trace.  Returns (steps as a list of dicts, filter
+        values, filter memory ids, filter count, stats)."""
+        tr = (WalkStep * cap)()
+        tn = ctypes.c_uint32(0)
+        fv = np.zeros(max(ftk, 1), np.float32)
+        ft = np.zeros(max(ftk, 1), np.uint32)
+        fn = ctypes.c_uint32(0)
+        st = Stats()
+        rc = self._L.po_walk_trace(self._h, c32(q), c32(coarse_ip), c32(fine_ip), float(qn), scc, ftk, radius_rate,
+                                   tr, cap, ctypes.byref(tn), fv, ft, ctypes.byref(fn), ctypes.byref(st))
+        if rc != 0:
+            raise ValueError("po_walk_trace refused its arguments")
+        steps = [{k: getattr(tr[i], k) for k, _ in WalkStep._fields_} for i in range(min(tn.value, cap))]
+        return steps, fv[:ftk], ft[:ftk], fn.value, st.as_dict()
 
     def search_one_given(self, q, coarse_ip, fine_ip, qn, topk, scc, ftk, max_topk=None, radius_rate=1.0):
         """The walk and rerank for a prepared query with its center inner
@@ -228,7 +266,7 @@ class OracleIndex:
         D = np.zeros(max(topk, 1), np.float32)
         I = np.zeros(max(topk, 1), np.uint32)
         n = ctypes.c_uint32(0)
-        rc = lib().po_search_one_given(self._h, c32(q), c32(coarse_ip), c32(fine_ip), float(qn), topk, max_topk, scc,
+        rc = self._L.po_search_one_given(self._h, c32(q), c32(coarse_ip), c32(fine_ip), float(qn), topk, max_topk, scc,
                                        ftk, radius_rate, D, I, ctypes.byref(n))
         if rc != 0:
             raise ValueError("po_search_one_given refused its arguments")
@@ -242,7 +280,7 @@ class OracleIndex:
         I = np.zeros((nq, topk), np.uint32)
         nres = np.zeros(nq, np.uint32)
         st = Stats()
-        lib().po_search_batch(self._h, nq, Q.reshape(-1), topk, max_topk, scc, ftk, radius_rate,
+        self._L.po_search_batch(self._h, nq, Q.reshape(-1), topk, max_topk, scc, ftk, radius_rate,
                               D.reshape(-1), I.reshape(-1), nres, nthreads, ctypes.byref(st))
         return D, I, nres, st.as_dict()
 
@@ -264,7 +302,7 @@ class OracleIndex:
         D = np.zeros((nq, max_topk), np.float32)
         I = np.zeros((nq, max_topk), np.uint32)
         nres = np.zeros(nq, np.uint32)
-        lib().po_search_batch_pq(self._h, nq, Q.reshape(-1), arr(topk), max_topk, arr(scc_arr), scc,
+        self._L.po_search_batch_pq(self._h, nq, Q.reshape(-1), arr(topk), max_topk, arr(scc_arr), scc,
                                  arr(fine_budget), ftk, radius_rate, D.reshape(-1), I.reshape(-1), nres, nthreads)
         return D, I, nres
 

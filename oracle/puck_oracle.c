@@ -530,6 +530,66 @@ static int po_search_budget(const po_index* ix, const float* q, uint32_t topk, u
     return 0;
 }
 
+/* One considered cell of the walk, for the trace mode of po_walk_query:
+ * position (coarse rank l, fine rank idx, fine id k, coarse id), the state
+ * before the cell's break / continue tests, the decision (0 break, 1
+ * continue, 2 scanned), the state after it, and for a scanned cell the
+ * points whose LUT sum was taken and the heap updates. */
+typedef struct {
+    uint32_t l, idx, k, coarse;
+    int32_t decision;
+    float pivot_before, top_before, max_st_before;
+    float pivot_after, max_st_after;
+    uint32_t offered, updated;
+} po_walk_step;
+
+/* The walk's decision rules (puck_index.cpp:514-547).  Each is a macro so
+ * that tests/test_walk_pinning.py can compile deliberately broken variants
+ * of this file (-DPO_MUT_...) and check that the hand-traced known answers
+ * reject every one of them; liboracle.so defines none of them.
+ *   pivot      (top - |q|^2) / radius_rate / 2.0: float subtraction, then
+ *              double divisions, rounded to float (:517, :544)
+ *   min_dist   min_dist_offset[c] + coarse_distance[l] (:522)
+ *   max_st     pivot - coarse_distance[l] - fine_distance[0] at the list's
+ *              start (:523-524), lowered after every scanned cell (:547)
+ *   break      fine_distance[idx] + min_dist >= pivot (:527)
+ *   continue   stationary >= max_stationary_dist (:534)
+ *   point      2.0 * cell_dist + offset >= heap top ends the cell (:362-364) */
+#if defined(PO_MUT_PIVOT_NO_HALF)
+#define PO_PIVOT(top, qn, rr) ((float)((double)((top) - (qn)) / (rr)))
+#elif defined(PO_MUT_PIVOT_NO_RR)
+#define PO_PIVOT(top, qn, rr) ((float)((double)((top) - (qn)) / 2.0))
+#elif defined(PO_MUT_PIVOT_NO_QN)
+#define PO_PIVOT(top, qn, rr) ((float)((double)(top) / (rr) / 2.0))
+#else
+#define PO_PIVOT(top, qn, rr) ((float)((double)((top) - (qn)) / (rr) / 2.0))
+#endif
+#ifdef PO_MUT_MIN_DIST_NO_MDO
+#define PO_MIN_DIST(mdo, cd) (cd)
+#else
+#define PO_MIN_DIST(mdo, cd) ((mdo) + (cd))
+#endif
+#ifdef PO_MUT_MAXST_INIT_NO_FD0
+#define PO_MAX_ST_INIT(p, cd, fd0) ((p) - (cd))
+#else
+#define PO_MAX_ST_INIT(p, cd, fd0) ((p) - (cd) - (fd0))
+#endif
+#ifdef PO_MUT_BREAK_GT
+#define PO_BREAK_TEST(x, p) ((x) > (p))
+#else
+#define PO_BREAK_TEST(x, p) ((x) >= (p))
+#endif
+#ifdef PO_MUT_CONT_GT
+#define PO_CONT_TEST(st, m) ((st) > (m))
+#else
+#define PO_CONT_TEST(st, m) ((st) >= (m))
+#endif
+#ifdef PO_MUT_POINT_GT
+#define PO_POINT_BREAK_TEST(t, top) ((t) > (top))
+#else
+#define PO_POINT_BREAK_TEST(t, top) ((t) >= (top))
+#endif
+
 /* The reference walk and rerank for one prepared query (puck_index.cpp:485-600,
  * hierarchical_cluster_index.cpp:700-727).  coarse_ip / fine_ip / qn_in: the
  * query's inner products with every coarse and fine center and its squared
@@ -538,7 +598,8 @@ static void po_walk_query(const po_index* ix, const float* q, uint32_t topk, uin
                           double radius_rate, const float* coarse_ip, const float* fine_ip, const float* qn_in,
                           float* out_dist, uint32_t* out_ids, uint32_t* out_n,
                           float* filt_val, uint32_t* filt_tag, uint32_t* filt_n,
-                          po_stats* st, int* coarse_out, int* fine_out) {
+                          po_stats* st, int* coarse_out, int* fine_out,
+                          po_walk_step* trace, uint32_t trace_cap, uint32_t* trace_n) {
     uint32_t dim = ix->dim, C = ix->C, F = ix->F, nsq = ix->nsq;
     /* search_nearest_coarse_cluster (hierarchical_cluster_index.cpp:700-727) */
     float* coarse_d = (float*)malloc(sizeof(float) * (scc ? scc : 1));
@@ -574,34 +635,53 @@ static void po_walk_query(const po_index* ix, const float* q, uint32_t topk, uin
     po_heap fh;
     po_heap_init(&fh, ftk, fval, ftag);
     float qn = qn_in ? *qn_in : po_dot(q, q, dim);
-    float pivot = (float)((double)(fval[0] - qn) / radius_rate / 2.0);
+    float pivot = PO_PIVOT(fval[0], qn, radius_rate);
     size_t rec = 4 + nsq;
     po_stats s = {0, 0, 0, 0};
+    uint32_t nt = 0;
 
     for (uint32_t l = 0; l < scc; ++l) {
         uint32_t c = coarse_t[l];
         const float* cst = ix->stationary + (size_t)c * F;
         const uint32_t* cstart = ix->cell_start + (size_t)c * F;
         float cd = coarse_d[l];
-        float min_dist = ix->min_dist_offset[c] + cd;
-        float max_st = pivot - cd - fine_d[0];
+        float min_dist = PO_MIN_DIST(ix->min_dist_offset[c], cd);
+        float max_st = PO_MAX_ST_INIT(pivot, cd, fine_d[0]);
         for (uint32_t idx = 0; idx < F; ++idx) {
             s.cells_considered++;
-            if (fine_d[idx] + min_dist >= pivot) break;
+            po_walk_step* ts = (trace && nt < trace_cap) ? trace + nt : NULL;
+            ++nt;
+            if (ts) {
+                ts->l = l; ts->idx = idx; ts->k = fine_t[idx]; ts->coarse = c;
+                ts->pivot_before = pivot; ts->top_before = fval[0]; ts->max_st_before = max_st;
+                ts->offered = 0; ts->updated = 0;
+            }
+            if (PO_BREAK_TEST(fine_d[idx] + min_dist, pivot)) {
+                if (ts) { ts->decision = 0; ts->pivot_after = pivot; ts->max_st_after = max_st; }
+                break;
+            }
             uint32_t k = fine_t[idx];
-            if (cst[k] >= max_st) continue;
+            if (PO_CONT_TEST(cst[k], max_st)) {
+#ifdef PO_MUT_MAXST_ON_CONTINUE
+                { float cand = pivot - cd - fine_d[idx]; if (cand < max_st) max_st = cand; }
+#endif
+                if (ts) { ts->decision = 1; ts->pivot_after = pivot; ts->max_st_after = max_st; }
+                continue;
+            }
             float cell_dist = cd + cst[k] + fine_d[idx];
             s.cells_scanned++;
             /* compute_quantized_distance (puck_index.cpp:346-390) */
             uint32_t b = cstart[k], e = cstart[k + 1];
             int updated = 0;
+            uint32_t offered = 0;
             for (uint32_t i = b; i < e; ++i) {
                 const uint8_t* r = ix->records + (size_t)i * rec;
                 float off;
                 memcpy(&off, r, 4);
                 float t = (float)(2.0 * (double)cell_dist + (double)off);
-                if (t >= fval[0]) break;
+                if (PO_POINT_BREAK_TEST(t, fval[0])) break;
                 s.points_offered++;
+                ++offered;
                 t = t + po_lookup(r + 4, lut, nsq);
                 if (t < fval[0]) {
                     po_heap_update(&fh, t, i);
@@ -609,11 +689,18 @@ static void po_walk_query(const po_index* ix, const float* q, uint32_t topk, uin
                     s.heap_updates++;
                 }
             }
-            if (updated > 0) pivot = (float)((double)(fval[0] - qn) / radius_rate / 2.0);
+            if (updated > 0) pivot = PO_PIVOT(fval[0], qn, radius_rate);
+#ifndef PO_MUT_NO_MAXST_UPDATE
             float cand = pivot - cd - fine_d[idx];
             if (cand < max_st) max_st = cand;
+#endif
+            if (ts) {
+                ts->decision = 2; ts->pivot_after = pivot; ts->max_st_after = max_st;
+                ts->offered = offered; ts->updated = (uint32_t)updated;
+            }
         }
     }
+    if (trace_n) *trace_n = nt;
     uint32_t nf = po_heap_count(&fh);
     if (nf < ftk) po_heap_reorder(&fh);
     if (filt_val) memcpy(filt_val, fval, sizeof(float) * ftk);
@@ -666,7 +753,7 @@ int po_search_one(const void* p, const float* feature, uint32_t topk, uint32_t m
         return rc;
     }
     po_walk_query(ix, q, topk, scc, ftk, radius_rate, NULL, NULL, NULL, out_dist, out_ids, out_n, filt_val,
-                  filt_tag, filt_n, st, coarse_out, fine_out);
+                  filt_tag, filt_n, st, coarse_out, fine_out, NULL, 0, NULL);
     free(q);
     return 0;
 }
@@ -681,7 +768,24 @@ int po_search_one_given(const void* p, const float* q, const float* coarse_ip, c
     if (topk > max_topk || q == NULL || coarse_ip == NULL || fine_ip == NULL) return -1;
     if (scc > ix->C) scc = ix->C;
     po_walk_query(ix, q, topk, scc, ftk, radius_rate, coarse_ip, fine_ip, &qn, out_dist, out_ids, out_n, NULL,
-                  NULL, NULL, NULL, NULL, NULL);
+                  NULL, NULL, NULL, NULL, NULL, NULL, 0, NULL);
+    return 0;
+}
+
+/* Test hook: the walk of po_search_one_given with its trace.  trace
+ * receives up to trace_cap considered cells in visiting order (*trace_n =
+ * the number considered); filt_val / filt_tag / filt_n the filter heap as
+ * po_search_one gives it; st the walk's counters. */
+int po_walk_trace(const void* p, const float* q, const float* coarse_ip, const float* fine_ip, float qn,
+                  uint32_t scc, uint32_t ftk, double radius_rate, po_walk_step* trace, uint32_t trace_cap,
+                  uint32_t* trace_n, float* filt_val, uint32_t* filt_tag, uint32_t* filt_n, po_stats* st) {
+    const po_index* ix = (const po_index*)p;
+    if (q == NULL || coarse_ip == NULL || fine_ip == NULL || ftk == 0) return -1;
+    if (scc > ix->C) scc = ix->C;
+    float od[1];
+    uint32_t oi[1], on = 0;
+    po_walk_query(ix, q, 1, scc, ftk, radius_rate, coarse_ip, fine_ip, &qn, od, oi, &on, filt_val, filt_tag,
+                  filt_n, st, NULL, NULL, trace, trace_cap, trace_n);
     return 0;
 }
 

@@ -997,8 +997,18 @@ int puckgpu_budget_ranks(puckgpu_ctx* ctx, const puckgpu_index* ix, uint32_t sea
 int puckgpu_probe_create(puckgpu_ctx* ctx, uint32_t dim, const puckgpu_probe_shape* shape, puckgpu_probe** out) {
     if (!ctx || !shape || !out) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
     *out = nullptr;
-    if (dim == 0 || shape->hidden == 0 || shape->hidden > 4096)
+    if (dim == 0 || dim > 4096 || shape->hidden == 0 || shape->hidden > 4096)
         return set_err(ctx, PUCKGPU_EINVAL, "probe: dim and hidden must be in 1..4096");
+    if (shape->coarse_features > 8192 || shape->fine_features > 8192)
+        return set_err(ctx, PUCKGPU_EINVAL, "probe: coarse_features and fine_features must be at most 8192");
+    {
+        // the parameter count in 64 bits (the kernels index parameters in 32;
+        // with the bounds above it stays near 1e8)
+        const uint64_t D = (uint64_t)dim + shape->coarse_features + shape->fine_features, H = shape->hidden;
+        const uint64_t cnt = H * D + H + H * H + H + H + 1 + 2 * D;
+        if (cnt > UINT32_MAX)
+            return set_err(ctx, PUCKGPU_EINVAL, "probe: parameter count exceeds 32-bit indexing");
+    }
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
     puckgpu_probe* p = new puckgpu_probe();
     p->device = ctx->device;

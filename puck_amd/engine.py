@@ -34,6 +34,44 @@ def _ptr(a, dtype=None):
     return ctypes.c_void_p(a.data_ptr())
 
 
+_F32 = ("float32",)
+_U32 = ("uint32", "int32")
+_U8 = ("uint8",)
+
+
+def _dtype_name(a):
+    return str(a.dtype).rsplit(".", 1)[-1]
+
+
+def _check(a, name, kinds, shape=None, min_shape=None, device=None):
+    """Refuses an argument before any library call: its dtype is one of
+    `kinds`, it is contiguous, its shape equals `shape` (None entries are
+    free) or is at least `min_shape` in every dimension, and a device tensor
+    lives on `device` (the context's ordinal).  A None argument passes."""
+    if a is None:
+        return
+    dn = _dtype_name(a)
+    if dn not in kinds:
+        raise TypeError(f"{name}: expected dtype {' or '.join(kinds)}, got {dn}")
+    sh = tuple(int(v) for v in a.shape)
+    if shape is not None:
+        if len(sh) != len(shape) or any(w is not None and v != w for v, w in zip(sh, shape)):
+            raise ValueError(f"{name}: expected shape {list(shape)}, got {list(sh)}")
+    if min_shape is not None:
+        if len(sh) != len(min_shape) or any(v < w for v, w in zip(sh, min_shape)):
+            raise ValueError(f"{name}: expected at least shape {list(min_shape)}, got {list(sh)}")
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError(f"{name}: array must be C-contiguous")
+    else:
+        if not a.is_contiguous():
+            raise ValueError(f"{name}: tensor must be contiguous")
+        if _is_dev(a) and device is not None:
+            idx = getattr(a.device, "index", None)
+            if idx is not None and int(idx) != int(device):
+                raise ValueError(f"{name}: tensor on device {idx}, context on device {device}")
+
+
 def _mode(*arrays):
     devs = [_is_dev(a) for a in arrays if a is not None]
     if any(devs) and not all(devs):
@@ -76,6 +114,8 @@ class Context:
     # ---- test hooks / build entry points -------------------------------
     def lut(self, queries, codebook, nsq):
         n, dim = queries.shape
+        _check(queries, "queries", _F32, device=self.device)
+        _check(codebook, "codebook", _F32, shape=(nsq * 256 * (dim // max(nsq, 1)),), device=self.device)
         out = np.zeros((n, nsq * 256), np.float32) if not _is_dev(queries) else None
         if out is None:
             import torch
@@ -89,6 +129,9 @@ class Context:
         C, dim = coarse.shape
         F = fine.shape[0]
         n = x.shape[0]
+        _check(coarse, "coarse", _F32, device=self.device)
+        _check(fine, "fine", _F32, shape=(F, dim), device=self.device)
+        _check(x, "x", _F32, shape=(n, dim), device=self.device)
         if _is_dev(x):
             import torch
             cell = torch.empty(n, dtype=torch.int32, device=x.device)
@@ -106,6 +149,15 @@ class Context:
         C, dim = coarse.shape
         F = fine.shape[0]
         n = x.shape[0]
+        _check(coarse, "coarse", _F32, device=self.device)
+        _check(fine, "fine", _F32, shape=(F, dim), device=self.device)
+        if nsq == 0 or dim % nsq:
+            raise ValueError(f"filter_nsq {nsq} must divide dim {dim}")
+        _check(codebook, "codebook", _F32, device=self.device)
+        if int(np.prod(codebook.shape)) != nsq * 256 * (dim // nsq):
+            raise ValueError(f"codebook: expected {nsq * 256 * (dim // nsq)} floats")
+        _check(x, "x", _F32, shape=(n, dim), device=self.device)
+        _check(cell, "cell", _U32, shape=(n,), device=self.device)
         if _is_dev(x):
             import torch
             rec = torch.empty((n, 4 + nsq), dtype=torch.uint8, device=x.device)
@@ -120,6 +172,8 @@ class Context:
     def kmeans_assign(self, centroids, x, with_dist=False):
         k, dim = centroids.shape
         n = x.shape[0]
+        _check(centroids, "centroids", _F32, device=self.device)
+        _check(x, "x", _F32, shape=(n, dim), device=self.device)
         if _is_dev(x):
             import torch
             a = torch.empty(n, dtype=torch.int32, device=x.device)
@@ -144,6 +198,15 @@ class GpuIndex:
         nsq = records.shape[1] - 4
         self.dim, self.C, self.F, self.nsq, self.N = dim, C, F, nsq, N
         self.whether_norm = bool(whether_norm)
+        dv = ctx.device
+        _check(coarse, "coarse", _F32, device=dv)
+        _check(fine, "fine", _F32, shape=(F, dim), device=dv)
+        _check(codebook, "codebook", _F32, device=dv)
+        if nsq <= 0 or dim % nsq or int(np.prod(codebook.shape)) != nsq * 256 * (dim // nsq):
+            raise ValueError(f"codebook: expected {nsq} x 256 x {dim // max(nsq, 1)} floats")
+        _check(cell_assign, "cell_assign", _U32, shape=(N,), device=dv)
+        _check(records, "records", _U8, shape=(N, 4 + nsq), device=dv)
+        _check(features, "features", _F32, shape=(N, dim), device=dv)
         p = _lib.IndexParams(dim, C, F, nsq, N, int(bool(whether_norm)))
         h = ctypes.c_void_p()
         m = _mode(coarse, fine, codebook, cell_assign, records, features)
@@ -174,6 +237,21 @@ class GpuIndex:
         out["mem2local"] = out["mem2local"][:N]
         return out
 
+    def _check_io(self, queries, n, D, I, R, width, per_query=()):
+        """Queries [n, dim] float32, distances [>= n, >= width] float32,
+        labels [>= n, >= width] and counts [>= n] 32-bit integers, per-query
+        arrays [n] 32-bit integers, all on the context's device."""
+        dv = self.ctx.device
+        _check(queries, "queries", _F32, shape=(n, self.dim), device=dv)
+        _check(D, "distance", _F32, min_shape=(n, width), device=dv)
+        _check(I, "labels", _U32, min_shape=(n, width), device=dv)
+        _check(R, "result_num", _U32, min_shape=(n,), device=dv)
+        for nm, a in per_query:
+            _check(a, nm, _U32, shape=(n,), device=dv)
+        for a in (D, I):
+            if a is not None and int(a.shape[1]) != width:
+                raise ValueError(f"output rows must have exactly {width} columns (the library's row stride)")
+
     def search(self, queries, topk, search_coarse_count, filter_topk, topk_max=None, radius_rate=1.0,
                out=None, replay_all=False, precision=0):
         """Returns (distance [n, topk], labels [n, topk], result_num [n])."""
@@ -191,6 +269,7 @@ class GpuIndex:
             else:
                 out = (np.zeros((n, topk), np.float32), np.zeros((n, topk), np.uint32), np.zeros(n, np.uint32))
         D, I, R = out
+        self._check_io(queries, n, D, I, R, topk)
         m = _mode(queries, D, I, R)
         self.ctx.sync_torch(queries)
         self.ctx.check(self.ctx.L.puckgpu_search(self.ctx.h, self.h, ctypes.byref(sp), n, _ptr(queries), topk,
@@ -229,6 +308,8 @@ class GpuIndex:
                 out = (np.zeros((n, topk_max), np.float32), np.zeros((n, topk_max), np.uint32),
                        np.zeros(n, np.uint32))
         D, I, R = out
+        self._check_io(queries, n, D, I, R, topk_max, (("topk", qt), ("search_coarse_count", qs),
+                                                       ("fine_budget", qf)))
         m = _mode(queries, D, I, R, qt, qs, qf)
         qp = _lib.QueryParams(_ptr(qt), _ptr(qs), _ptr(qf))
         self.ctx.sync_torch(queries, qt, qs, qf)
@@ -246,6 +327,7 @@ class GpuIndex:
         else:
             co = np.zeros((n, self.C), np.float32)
             fo = np.zeros((n, self.F), np.float32)
+        _check(queries, "queries", _F32, shape=(n, self.dim), device=self.ctx.device)
         self.ctx.sync_torch(queries)
         self.ctx.check(self.ctx.L.puckgpu_center_distances(self.ctx.h, self.h, int(precision), n, _ptr(queries),
                                                            _ptr(co), _ptr(fo), _mode(queries, co, fo)))
@@ -260,6 +342,8 @@ class GpuIndex:
         else:
             r = np.zeros((n, k), np.uint32)
             cells = np.ascontiguousarray(cells, np.uint32)
+        _check(queries, "queries", _F32, shape=(n, self.dim), device=self.ctx.device)
+        _check(cells, "cells", _U32, shape=(n, k), device=self.ctx.device)
         self.ctx.sync_torch(queries, cells)
         self.ctx.check(self.ctx.L.puckgpu_budget_ranks(self.ctx.h, self.h, search_coarse_count, n, _ptr(queries), k,
                                                        _ptr(cells), _ptr(r), _mode(queries, cells, r)))
@@ -333,6 +417,7 @@ class Probe:
         else:
             f = np.zeros((n, self.D), np.float32)
             fo = np.zeros((n, index.F), np.uint32) if fine_order else None
+        _check(queries, "queries", _F32, shape=(n, self.dim), device=self.ctx.device)
         self.ctx.sync_torch(queries)
         self.ctx.check(self.ctx.L.puckgpu_probe_features(self.ctx.h, index.h, self.h, n, _ptr(queries), _ptr(f),
                                                          _ptr(fo), _mode(queries, f, fo)))
@@ -341,6 +426,8 @@ class Probe:
     def train_step(self, features, target, lr=None, betas=(0.9, 0.999), eps=1e-8, want_loss=True,
                    want_grads=False):
         n = features.shape[0]
+        _check(features, "features", _F32, shape=(n, self.D), device=self.ctx.device)
+        _check(target, "target", _F32, shape=(n,), device=self.ctx.device)
         adam = _lib.AdamParams(lr, betas[0], betas[1], eps) if lr is not None else None
         g = None
         if want_grads:
@@ -359,6 +446,7 @@ class Probe:
 
     def predict(self, features):
         n = features.shape[0]
+        _check(features, "features", _F32, shape=(n, self.D), device=self.ctx.device)
         if _is_dev(features):
             import torch
             y = torch.empty(n, dtype=torch.float32, device=features.device)
@@ -403,6 +491,10 @@ def search_learned(index, probe, queries, topk_max, search_coarse_count, filter_
             out = (np.zeros((n, topk_max), np.float32), np.zeros((n, topk_max), np.uint32), np.zeros(n, np.uint32),
                    np.zeros(n, np.uint32))
     D, I, R, B = out
+    index._check_io(queries, n, D, I, R, topk_max, (("topk", qt), ("search_coarse_count", qs)))
+    _check(B, "budget_out", _U32, min_shape=(n,), device=ctx.device)
+    if probe.dim != index.dim:
+        raise ValueError(f"probe built for dim {probe.dim}, index has dim {index.dim}")
     m = _mode(queries, D, I, R, B, qt, qs)
     qp = _lib.QueryParams(_ptr(qt), _ptr(qs), None)
     ctx.sync_torch(queries, qt, qs)
@@ -469,6 +561,9 @@ class Server:
                 out = (np.zeros((n, self.topk_max), np.float32), np.zeros((n, self.topk_max), np.uint32),
                        np.zeros(n, np.uint32))
         D, I, R = out
+        if n > self.max_batch:
+            raise ValueError(f"batch of {n} queries exceeds the server's max_batch {self.max_batch}")
+        self.index._check_io(queries, n, D, I, R, self.topk_max, (("topk", qt), ("search_coarse_count", qs)))
         m = _mode(queries, D, I, R, qt, qs)
         self.ctx.sync_torch(queries, qt, qs)
         self.ctx.check(self.ctx.L.puckgpu_server_search(self.h, n, _ptr(queries), _ptr(qt), _ptr(qs), _ptr(D),

@@ -152,7 +152,10 @@ class PySearcher:
             distance.reshape(-1)[: n * topk] = D.reshape(-1)
             labels.reshape(-1)[: n * topk] = I.reshape(-1)
         else:
-            q = q.reshape(n, self._conf["feature_dim"]).contiguous()
+            # as the numpy path: the queries are read as float32; the outputs
+            # must already be float32 / 32-bit integer tensors (checked by the index)
+            import torch
+            q = q.reshape(n, self._conf["feature_dim"]).to(torch.float32).contiguous()
             self._index.search(q, topk, self._params["search_coarse_count"], self._params["filter_topk"],
                                topk_max=self._params["topk"], radius_rate=self._params["radius_rate"],
                                out=(distance.view(n, topk), labels.view(n, topk), None))

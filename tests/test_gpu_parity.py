@@ -94,6 +94,28 @@ def test_search_bitexact(ctx, which, scc, ftk, topk, idx_l2, idx_cos):
         assert np.array_equal(Ig[k, :o["n"]], o["ids"][:o["n"]]), f"query {k}"
 
 
+@pytest.mark.parametrize("which,rr,replay", [("l2", 1.5, False), ("cos", 2.0, False), ("cos", 1.25, True)])
+def test_search_radius_rate_bitexact(ctx, which, rr, replay, idx_l2, idx_cos):
+    # radius_rate > 1 shrinks the pivot (puck_index.cpp:517, :544) and prunes
+    # more cells; a small filter_topk makes the pivot move early
+    ix, q = idx_l2 if which == "l2" else idx_cos
+    gi = _gpu_index(ctx, ix)
+    oi = CI.oracle_index(ix)
+    for scc, ftk, topk in [(12, 40, 10), (30, 200, 10)]:
+        Dg, Ig, Rg = gi.search(q, topk, scc, ftk, radius_rate=rr, replay_all=replay)
+        ft, fm, fn = gi.last_filter(q.shape[0], ftk)
+        for k in range(q.shape[0]):
+            o = oi.search_one(q[k], topk, scc, ftk, radius_rate=rr)
+            assert fn[k] == o["filt_n"], f"query {k}"
+            gv, gid = _sorted_pairs(ft[k, :fn[k]], fm[k, :fn[k]])
+            ov, oid = _sorted_pairs(o["filt_val"][:o["filt_n"]], o["filt_mem"][:o["filt_n"]])
+            assert np.array_equal(_bits(gv), _bits(ov)), f"query {k}: filter distances differ"
+            assert np.array_equal(gid, oid), f"query {k}: filter ids differ"
+            assert Rg[k] == o["n"]
+            assert np.array_equal(Ig[k, :o["n"]], o["ids"][:o["n"]]), f"query {k}"
+            assert np.array_equal(_bits(Dg[k, :o["n"]]), _bits(o["dist"][:o["n"]])), f"query {k}"
+
+
 @pytest.mark.parametrize("which", ["l2", "cos"])
 def test_search_replay_path_bitexact(ctx, which, idx_l2, idx_cos):
     # every query through the reference-heap replay (the path taken when the

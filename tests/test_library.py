@@ -87,3 +87,43 @@ def test_binding_argument_checks_before_any_device_call():
     sv.max_batch, sv.index = 8, types.SimpleNamespace(dim=4)
     with pytest.raises(ValueError):
         sv.tune(np.zeros((4, 4), np.float32))           # the largest bucket reads max_batch rows
+
+
+def test_binding_refuses_bad_search_arrays_before_any_device_call():
+    # dtype, shape, output size and device are checked before the library is
+    # reached; stand-in objects play the index and device tensors (no GPU here)
+    import types
+    import numpy as np
+    from puck_amd import engine as E
+    ix = object.__new__(E.GpuIndex)
+    ix.ctx = types.SimpleNamespace(device=0)
+    ix.dim, ix.C, ix.F = 8, 4, 4
+    q = np.zeros((3, 8), np.float32)
+    D, I, R = np.zeros((3, 5), np.float32), np.zeros((3, 5), np.uint32), np.zeros(3, np.uint32)
+    ix._check_io(q, 3, D, I, R, 5)                                   # accepted
+    with pytest.raises(TypeError):
+        ix._check_io(q.astype(np.float64), 3, D, I, R, 5)            # float64 queries
+    with pytest.raises(ValueError):
+        ix._check_io(np.zeros((3, 7), np.float32), 3, D, I, R, 5)    # wrong dim
+    with pytest.raises(ValueError):
+        ix._check_io(q, 3, np.zeros((2, 5), np.float32), I, R, 5)    # too few output rows
+    with pytest.raises(ValueError):
+        ix._check_io(q, 3, D, np.zeros((3, 4), np.uint32), R, 5)     # rows narrower than the stride
+    with pytest.raises(TypeError):
+        ix._check_io(q, 3, D, I.astype(np.int64), R, 5)              # 64-bit labels
+    with pytest.raises(ValueError):
+        ix._check_io(q, 3, D, I, R, 5, (("topk", np.zeros(2, np.uint32)),))   # per-query array too short
+
+    def dev_tensor(shape, dtype, index):
+        return types.SimpleNamespace(is_cuda=True, dtype=dtype, shape=shape, device=types.SimpleNamespace(index=index),
+                                     is_contiguous=lambda: True)
+    ok = dev_tensor((3, 8), "torch.float32", 0)
+    E._check(ok, "queries", E._F32, shape=(3, 8), device=0)
+    with pytest.raises(ValueError):
+        E._check(dev_tensor((3, 8), "torch.float32", 1), "queries", E._F32, shape=(3, 8), device=0)
+    with pytest.raises(TypeError):
+        E._check(dev_tensor((3, 8), "torch.float64", 0), "queries", E._F32, shape=(3, 8), device=0)
+    sv = object.__new__(E.Server)
+    sv.max_batch, sv.index, sv.topk_max, sv.ctx = 2, ix, 5, ix.ctx
+    with pytest.raises(ValueError):
+        sv.search(q, out=(D, I, R))                                  # more queries than max_batch
