@@ -26,6 +26,11 @@
  *                          (puck/puck/puck_index.cpp:602-709)
  *   puckgpu_kmeans_assign  nearest_center, the k-means assignment step
  *                          (puck/hierarchical_cluster/kmeans.cpp:221-258)
+ *   puckgpu_index_attach_pq  PuckIndex::read_coodbooks / read_feature_index for
+ *                          whether_pq = true (the PQ codebook and pq_data.dat,
+ *                          puck/puck/puck_index.cpp:173-227)
+ *   puckgpu_encode_pq      PuckIndex::puck_assign with quantizations = {filter,
+ *                          pq} (puck/puck/puck_index.cpp:602-709)
  * Entry points without a counterpart in the reference (each says so below):
  * the learned probe (puckgpu_probe_*, puckgpu_search_learned,
  * puckgpu_budget_ranks), the batched server (puckgpu_server_*), the
@@ -87,7 +92,10 @@ uint32_t puckgpu_filter_topk_max(void);
  *   order), filter_codebook [nsq x 256 x dim/nsq] (filter_codebook.dat rows),
  *   cell_assign [N] (cell_assign.dat), filter_records [N x (4 + nsq)]
  *   (filter_data.dat: f32 offset then nsq code bytes), features [N x dim]
- *   (all_data.feat.bin rows without the int32 length prefix).
+ *   (all_data.feat.bin rows without the int32 length prefix) or NULL for an
+ *   index that ranks by a second-stage PQ (whether_pq = true, which does not
+ *   load the raw vectors: puck_index.cpp:198-227; attach the PQ with
+ *   puckgpu_index_attach_pq before searching).
  * The engine derives |c|^2/2, the C x F stationary table, the memory order
  * (cells ascending, then offset, then local id), empty-cell sentinels and the
  * per-coarse minimum offsets, as PuckIndex::init does. */
@@ -96,6 +104,16 @@ int puckgpu_index_load(puckgpu_ctx* ctx, const puckgpu_index_params* params, con
                        const uint8_t* filter_records, const float* features, int device_ptrs,
                        puckgpu_index** out);
 void puckgpu_index_free(puckgpu_index* index);
+/* Second-stage PQ of a whether_pq = true index: pq_nsq subspaces of
+ * dim / pq_nsq floats (1, 2 or 4; pq_nsq <= 128), pq_codebook [pq_nsq x 256 x
+ * dim / pq_nsq] (pq_codebook.dat rows), pq_records [N x (4 + pq_nsq)] in local
+ * order (pq_data.dat: f32 offset then pq_nsq code bytes).  The rerank then
+ * ranks the filter survivors by rank_topN_points' PQ distance
+ * (puck_index.cpp:402-427): (filter distance - |q|^2) + offset + the sum of
+ * the query's PQ lookup table over the codes, instead of exact L2 over the
+ * raw vectors. */
+int puckgpu_index_attach_pq(puckgpu_ctx* ctx, puckgpu_index* index, uint32_t pq_nsq, const float* pq_codebook,
+                            const uint8_t* pq_records, int device_ptrs);
 /* Copy derived arrays to host buffers (any may be NULL): coarse_norms [C],
  * stationary [C*F], cell_start [C*F+1], min_dist_offset [C], mem2local [N]. */
 int puckgpu_index_export(puckgpu_ctx* ctx, const puckgpu_index* index, float* coarse_norms, float* stationary,
@@ -280,6 +298,15 @@ int puckgpu_assign_cells(puckgpu_ctx* ctx, const float* coarse, const float* fin
 int puckgpu_encode_filter(puckgpu_ctx* ctx, const float* coarse, const float* fine, uint32_t C, uint32_t F,
                           uint32_t dim, const float* filter_codebook, uint32_t nsq, uint32_t n, const float* x,
                           const uint32_t* cell, uint8_t* records_out, int device_ptrs);
+/* Second-stage PQ records of n points: records_out [n x (4 + pq_nsq)] in the
+ * pq_data.dat layout.  filter_records [n x (4 + filter_nsq)] are the points'
+ * filter records (puckgpu_encode_filter): their codewords are subtracted
+ * from the residual x - c - f before the PQ codes are chosen, and the offset
+ * is 2 <PQ decode, c + f + filter decode>. */
+int puckgpu_encode_pq(puckgpu_ctx* ctx, const float* coarse, const float* fine, uint32_t C, uint32_t F, uint32_t dim,
+                      const float* filter_codebook, uint32_t filter_nsq, const uint8_t* filter_records,
+                      const float* pq_codebook, uint32_t pq_nsq, uint32_t n, const float* x, const uint32_t* cell,
+                      uint8_t* records_out, int device_ptrs);
 /* k-means assignment: assign_out [n] nearest of k centroids [k x dim]
  * (nearest_center's formula and strict '<'), dist_out [n] may be NULL. */
 int puckgpu_kmeans_assign(puckgpu_ctx* ctx, uint32_t dim, const float* centroids, uint32_t k, uint32_t n,

@@ -58,6 +58,21 @@ def lib():
     return _lib
 
 
+_fast = None
+
+
+def lib_fast():
+    """liboracle_fast.so: the same source built -O3 -march=x86-64-v3 (the
+    bench's CPU baseline); results are bit-identical to lib()'s."""
+    global _fast
+    if _fast is None:
+        path = os.path.join(HERE, "liboracle_fast.so")
+        if not os.path.exists(path):
+            subprocess.check_call(["make", "-s", "-C", HERE, "all"])
+        _fast = load_variant(path)
+    return _fast
+
+
 def load_variant(path):
     """Load a build of puck_oracle.c at `path` (liboracle.so, or a variant
     compiled by a test) with its ctypes signatures."""
@@ -112,6 +127,11 @@ def load_variant(path):
     L.po_encode_filter.restype = ctypes.c_int
     L.po_encode_filter.argtypes = [_f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32, _f32p,
                                    ctypes.c_uint32, _f32p, _u32p, ctypes.c_uint32, _u8p, ctypes.c_int]
+    L.po_index_set_pq.restype = ctypes.c_int
+    L.po_index_set_pq.argtypes = [vp, ctypes.c_uint32, _f32p, _u8p]
+    L.po_encode_puck.restype = ctypes.c_int
+    L.po_encode_puck.argtypes = [_f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32, _f32p, ctypes.c_uint32, _f32p,
+                                 ctypes.c_uint32, _f32p, _u32p, ctypes.c_uint32, _u8p, _u8p, ctypes.c_int]
     L.po_walk_trace.restype = ctypes.c_int
     L.po_walk_trace.argtypes = [vp, _f32p, _f32p, _f32p, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32,
                                 ctypes.c_double, ctypes.POINTER(WalkStep), ctypes.c_uint32,
@@ -207,6 +227,16 @@ class OracleIndex:
         if getattr(self, "_h", None):
             self._L.po_index_destroy(self._h)
             self._h = None
+
+    def set_pq(self, pq_codebook, pq_records):
+        """Second-stage PQ (whether_pq = true): codebook [pnsq, 256, dim/pnsq]
+        and records [N, 4 + pnsq] in local order; the rerank then ranks by
+        the PQ distance instead of the raw vectors."""
+        pq_records = np.ascontiguousarray(pq_records, np.uint8)
+        rc = self._L.po_index_set_pq(self._h, pq_records.shape[1] - 4, c32(pq_codebook).reshape(-1),
+                                     pq_records.reshape(-1))
+        if rc != 0:
+            raise ValueError("po_index_set_pq refused its arguments")
 
     def export(self):
         C, F, N = self.C, self.F, self.N
@@ -341,6 +371,22 @@ def nearest_center(centroids, X, nthreads=0):
     dis = np.zeros(n, np.float32)
     lib().po_nearest_center(d, centroids.reshape(-1), k, X.reshape(-1), n, a, dis, nthreads)
     return a, dis
+
+
+def encode_puck(coarse, fine, filter_codebook, fnsq, pq_codebook, pnsq, X, cell, nthreads=0):
+    """Filter and second-stage PQ records of X (puck_assign with both
+    quantizers): returns (filter records [n, 4 + fnsq], pq records [n, 4 + pnsq])."""
+    coarse, fine, X = c32(coarse), c32(fine), c32(X)
+    d = coarse.shape[1]
+    F = fine.shape[0]
+    n = X.shape[0]
+    fo = np.zeros((n, 4 + fnsq), np.uint8)
+    po = np.zeros((n, 4 + pnsq), np.uint8)
+    rc = lib().po_encode_puck(coarse.reshape(-1), fine.reshape(-1), F, d, c32(filter_codebook).reshape(-1), fnsq,
+                              c32(pq_codebook).reshape(-1), pnsq, X.reshape(-1), np.ascontiguousarray(cell, np.uint32),
+                              n, fo.reshape(-1), po.reshape(-1), nthreads)
+    assert rc == 0
+    return fo, po
 
 
 def encode_filter(coarse, fine, codebook, nsq, X, cell, nthreads=0):

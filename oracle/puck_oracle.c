@@ -270,6 +270,10 @@ typedef struct {
     uint32_t* mem2local;  /* N */
     uint8_t* records;     /* N x (4+nsq), memory order: [f32 offset][nsq codes] */
     float* raw;           /* N x dim, memory order */
+    /* second-stage PQ (whether_pq = true), or pnsq = 0 */
+    uint32_t pnsq, plsq;
+    float* pq_codebook;   /* pnsq x 256 x plsq */
+    uint8_t* pq_records;  /* N x (4+pnsq), memory order: [f32 offset][pnsq codes] */
 } po_index;
 
 typedef struct { uint32_t cell; float off; uint32_t local; } po_order;
@@ -367,6 +371,7 @@ void po_index_destroy(void* p) {
     free(ix->coarse); free(ix->fine); free(ix->codebook); free(ix->coarse_norms);
     free(ix->stationary); free(ix->cell_start); free(ix->min_dist_offset);
     free(ix->mem2local); free(ix->records); free(ix->raw);
+    free(ix->pq_codebook); free(ix->pq_records);
     free(ix);
 }
 
@@ -380,6 +385,40 @@ void po_index_export(void* p, float* coarse_norms, float* stationary, uint32_t* 
     if (cell_start) memcpy(cell_start, ix->cell_start, sizeof(uint32_t) * (cells + 1));
     if (min_dist_offset) memcpy(min_dist_offset, ix->min_dist_offset, sizeof(float) * ix->C);
     if (mem2local) memcpy(mem2local, ix->mem2local, sizeof(uint32_t) * ix->N);
+}
+
+/* Second-stage PQ of a whether_pq = true index (PuckIndex::read_coodbooks /
+ * read_feature_index, puck_index.cpp:173-227; Quantization::load,
+ * quantization.cpp:152-226): pnsq subspaces of dim / pnsq floats (1, 2 or
+ * 4), codebook [pnsq x 256 x plsq], records [N x (4 + pnsq)] in local order
+ * (pq_data.dat), permuted to memory order here.  The rerank then uses them
+ * in place of the raw vectors. */
+int po_index_set_pq(void* p, uint32_t pnsq, const float* pq_codebook, const uint8_t* pq_records_local) {
+    po_index* ix = (po_index*)p;
+    if (pnsq == 0 || ix->dim % pnsq != 0) return -1;
+    uint32_t plsq = ix->dim / pnsq;
+    if (!(plsq == 1 || plsq == 2 || plsq == 4)) return -1;
+    size_t rec = 4 + pnsq;
+    free(ix->pq_codebook); free(ix->pq_records);
+    ix->pnsq = pnsq; ix->plsq = plsq;
+    ix->pq_codebook = (float*)malloc(sizeof(float) * pnsq * PO_KS * plsq);
+    memcpy(ix->pq_codebook, pq_codebook, sizeof(float) * pnsq * PO_KS * plsq);
+    ix->pq_records = (uint8_t*)malloc(rec * (ix->N ? ix->N : 1));
+    for (uint32_t i = 0; i < ix->N; ++i)
+        memcpy(ix->pq_records + (size_t)i * rec, pq_records_local + (size_t)ix->mem2local[i] * rec, rec);
+    return 0;
+}
+
+/* Second-stage distance of filter survivor `mid` (memory id) with filter
+ * distance fd: rank_topN_points' whether_pq branch (puck_index.cpp:402-427),
+ * (fd - |q|^2) + offset, then + the LUT sum of its PQ codes in the 4-lane
+ * order of lookup_dist_table (:75-113). */
+static float po_pq_dist(const po_index* ix, uint32_t mid, float fd, float qn, const float* plut) {
+    const uint8_t* r = ix->pq_records + (size_t)mid * (4 + ix->pnsq);
+    float off;
+    memcpy(&off, r, 4);
+    float t = fd - qn + off;
+    return t + po_lookup(r + 4, plut, ix->pnsq);
 }
 
 /* Test hook: replace the index's coarse half-norms and stationary values with
@@ -501,15 +540,25 @@ static int po_search_budget(const po_index* ix, const float* q, uint32_t topk, u
     qsort(cand, n, sizeof(uint64_t), po_u64_cmp);
     uint32_t nf = n < ftk ? (uint32_t)n : ftk;
     uint64_t* rk = (uint64_t*)malloc(sizeof(uint64_t) * (nf ? nf : 1));
+    float* plut = NULL;
+    float qn = 0.f;
+    if (ix->pnsq) {
+        plut = (float*)malloc(sizeof(float) * ix->pnsq * PO_KS);
+        po_lut(q, ix->pq_codebook, ix->pnsq, ix->plsq, plut);
+        qn = po_dot(q, q, dim);
+    }
     for (uint32_t i = 0; i < nf; ++i) {
         uint32_t mid = (uint32_t)(cand[i] & 0xffffffffu);
         uint32_t tb = (uint32_t)(cand[i] >> 32);
         tb = (tb & 0x80000000u) ? (tb & 0x7fffffffu) : ~tb;
-        if (filt_val) memcpy(filt_val + i, &tb, 4);
+        float fv;
+        memcpy(&fv, &tb, 4);
+        if (filt_val) filt_val[i] = fv;
         if (filt_tag) filt_tag[i] = mid;
-        float d = po_rerank_dist(ix->raw + (size_t)mid * dim, q, dim);
+        float d = plut ? po_pq_dist(ix, mid, fv, qn, plut) : po_rerank_dist(ix->raw + (size_t)mid * dim, q, dim);
         rk[i] = ((uint64_t)po_f2ord(d) << 32) | ix->mem2local[mid];
     }
+    free(plut);
     if (filt_n) *filt_n = nf;
     qsort(rk, nf, sizeof(uint64_t), po_u64_cmp);
     uint32_t kout = nf < topk ? nf : topk;
@@ -706,15 +755,23 @@ static void po_walk_query(const po_index* ix, const float* q, uint32_t topk, uin
     if (filt_val) memcpy(filt_val, fval, sizeof(float) * ftk);
     if (filt_tag) memcpy(filt_tag, ftag, sizeof(uint32_t) * ftk);
     if (filt_n) *filt_n = nf;
-    /* rank_topN_points whether_pq=false (puck_index.cpp:392-457) */
+    /* rank_topN_points (puck_index.cpp:392-457): exact L2 over the raw
+     * vectors (whether_pq = false, :428-453) or the second-stage PQ distance
+     * (whether_pq = true, :402-427), both over the filter heap in array order */
     {
         po_heap rh;
         po_heap_init(&rh, topk, out_dist, out_ids);
+        float* plut = NULL;
+        if (ix->pnsq) {
+            plut = (float*)malloc(sizeof(float) * ix->pnsq * PO_KS);
+            po_lut(q, ix->pq_codebook, ix->pnsq, ix->plsq, plut);
+        }
         for (uint32_t i = 0; i < nf; ++i) {
-            const float* x = ix->raw + (size_t)ftag[i] * dim;
-            float t = po_rerank_dist(x, q, dim);
+            float t = plut ? po_pq_dist(ix, ftag[i], fval[i], qn, plut)
+                           : po_rerank_dist(ix->raw + (size_t)ftag[i] * dim, q, dim);
             if (t < out_dist[0]) po_heap_update(&rh, t, ix->mem2local[ftag[i]]);
         }
+        free(plut);
         po_heap_reorder(&rh);
         *out_n = po_heap_count(&rh);
     }
@@ -985,6 +1042,83 @@ int po_nearest_center(uint32_t dim, const float* centroids, uint32_t k, const fl
         if (dis) dis[j] = best;
     }
     free(cnorm);
+    return 0;
+}
+
+/* Filter-PQ and second-stage PQ encoding (whether_pq = true):
+ * PuckIndex::puck_assign (puck_index.cpp:602-709) with quantizations =
+ * {filter, pq}.  r = (x - c) - f; per filter subspace the nearest codeword
+ * (nearest_center, kmeans.cpp:221-258) is chosen and subtracted from r
+ * (saxpy with alpha -1, :672-675); the PQ codes are then chosen on that
+ * residual and subtracted the same way.  Offsets
+ * (Quantization::set_static_value_of_formula, quantization.cpp:244-258, in
+ * the quantizers' order): v = c + f; filter offset = 2 sdot(dec_f, v);
+ * v += dec_f; PQ offset = 2 sdot(dec_pq, v).  Both sub-vector lengths must be
+ * 1, 2 or 4.  frec_out [n x (4 + fnsq)], prec_out [n x (4 + pnsq)]. */
+int po_encode_puck(const float* coarse, const float* fine, uint32_t F, uint32_t dim, const float* fcb,
+                   uint32_t fnsq, const float* pcb, uint32_t pnsq, const float* x, const uint32_t* cell, uint32_t n,
+                   uint8_t* frec_out, uint8_t* prec_out, int nthreads) {
+    if (fnsq == 0 || pnsq == 0 || dim % fnsq != 0 || dim % pnsq != 0) return -1;
+    const uint32_t nq2[2] = {fnsq, pnsq}, lq2[2] = {dim / fnsq, dim / pnsq};
+    const float* cb2[2] = {fcb, pcb};
+    uint8_t* out2[2] = {frec_out, prec_out};
+    float* cbn[2];
+    for (int z = 0; z < 2; ++z) {
+        if (!(lq2[z] == 1 || lq2[z] == 2 || lq2[z] == 4)) return -1;
+        cbn[z] = (float*)malloc(sizeof(float) * nq2[z] * PO_KS);
+        for (uint32_t e = 0; e < nq2[z] * PO_KS; ++e)
+            cbn[z][e] = po_sdot_short(cb2[z] + (size_t)e * lq2[z], cb2[z] + (size_t)e * lq2[z], lq2[z]);
+    }
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+        float* r = (float*)malloc(sizeof(float) * dim);
+        float* dec = (float*)malloc(sizeof(float) * dim * 2);
+        float* v = (float*)malloc(sizeof(float) * dim);
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+        for (int64_t i = 0; i < (int64_t)n; ++i) {
+            uint32_t c = cell[i] / F, f = cell[i] % F;
+            const float* cc = coarse + (size_t)c * dim;
+            const float* ff = fine + (size_t)f * dim;
+            for (uint32_t j = 0; j < dim; ++j) {
+                float t = fmaf(-1.0f, cc[j], x[(size_t)i * dim + j]);
+                r[j] = fmaf(-1.0f, ff[j], t);
+            }
+            for (int z = 0; z < 2; ++z) {
+                const uint32_t nsq = nq2[z], lsq = lq2[z];
+                uint8_t* out = out2[z] + (size_t)i * (4 + nsq);
+                for (uint32_t m = 0; m < nsq; ++m) {
+                    float* sub = r + m * lsq;
+                    const float* cb = cb2[z] + (size_t)m * PO_KS * lsq;
+                    float pn = po_sdot_short(sub, sub, lsq);
+                    float best = FLT_MAX;
+                    uint32_t arg = 0;
+                    for (uint32_t k = 0; k < PO_KS; ++k) {
+                        float d = (float)((double)(pn + cbn[z][m * PO_KS + k]) -
+                                          2.0 * (double)po_sdot_short(sub, cb + (size_t)k * lsq, lsq));
+                        if (d < best) { best = d; arg = k; }
+                    }
+                    out[4 + m] = (uint8_t)arg;
+                    for (uint32_t j = 0; j < lsq; ++j) {
+                        dec[z * dim + m * lsq + j] = cb[(size_t)arg * lsq + j];
+                        sub[j] = fmaf(-1.0f, cb[(size_t)arg * lsq + j], sub[j]);
+                    }
+                }
+            }
+            for (uint32_t j = 0; j < dim; ++j) v[j] = fmaf(1.0f, ff[j], cc[j]);
+            float off = 2 * po_dot(dec, v, dim);
+            memcpy(out2[0] + (size_t)i * (4 + fnsq), &off, 4);
+            for (uint32_t j = 0; j < dim; ++j) v[j] = v[j] + dec[j];
+            off = 2 * po_dot(dec + dim, v, dim);
+            memcpy(out2[1] + (size_t)i * (4 + pnsq), &off, 4);
+        }
+        free(r); free(dec); free(v);
+    }
+    free(cbn[0]); free(cbn[1]);
     return 0;
 }
 

@@ -302,6 +302,88 @@ extern "C" hipError_t pk_launch_encode_filter(const float* X, const uint32_t* ce
     return hipGetLastError();
 }
 
+// Second-stage PQ encode (whether_pq = true; puck_index.cpp:602-709 with
+// quantizations = {filter, pq}): one thread per point, one PQ subspace's
+// codebook in LDS at a time.  The residual the PQ encodes is
+// r1 = ((x - c) - f) - dec_f, dec_f the point's filter codewords (the filter
+// quantizer's saxpy, :672-675); code_m = argmin_k (float)((double)(pn + cn_k)
+// - 2.0 * (double)dot(r1_m, cb_k)) with strict '<' from FLT_MAX and the short
+// dot products in MKL's order; offset = 2 * dot(dec_pq, (c + f) + dec_f)
+// accumulated in subspace order (set_static_value_of_formula after the
+// filter's, quantization.cpp:244-258).  fcodes: the point's filter codes
+// (frec_stride bytes apart); records_out rows of 4 + pnsq bytes.
+__global__ __launch_bounds__(256) void pk_encode_pq(const float* __restrict__ X, const uint32_t* __restrict__ cell,
+                                                    uint32_t n, const float* __restrict__ coarse,
+                                                    const float* __restrict__ fine, uint32_t F, uint32_t dim,
+                                                    const float* __restrict__ fcb, uint32_t fnsq,
+                                                    const uint8_t* __restrict__ fcodes, uint64_t frec_stride,
+                                                    const float* __restrict__ pcb, uint32_t pnsq,
+                                                    uint8_t* __restrict__ records_out) {
+    __shared__ float cb[PK_KS * 4];
+    __shared__ float cbn[PK_KS];
+    const uint32_t flsq = dim / fnsq, plsq = dim / pnsq;
+    const uint32_t pt = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = pt < n;
+    uint32_t cc = 0, ff = 0;
+    if (valid) {
+        cc = cell[pt] / F;
+        ff = cell[pt] % F;
+    }
+    const float* x = X + (uint64_t)(valid ? pt : 0) * dim;
+    const float* cv = coarse + (uint64_t)cc * dim;
+    const float* fv = fine + (uint64_t)ff * dim;
+    const uint8_t* fc = fcodes + (uint64_t)(valid ? pt : 0) * frec_stride;
+    const uint64_t rec = 4 + pnsq;
+    float offacc = 0.0f;
+    for (uint32_t m = 0; m < pnsq; ++m) {
+        __syncthreads();
+        for (uint32_t e = threadIdx.x; e < PK_KS * plsq; e += blockDim.x)
+            cb[e] = pcb[(uint64_t)m * PK_KS * plsq + e];
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < PK_KS; k += blockDim.x)
+            cbn[k] = pk_sdot_short(cb + k * plsq, cb + k * plsq, plsq);
+        __syncthreads();
+        if (!valid) continue;
+        float r[4], df[4];
+        for (uint32_t j = 0; j < plsq; ++j) {
+            const uint32_t i = m * plsq + j, mf = i / flsq;
+            df[j] = fcb[((uint64_t)mf * PK_KS + fc[mf]) * flsq + (i - mf * flsq)];
+            r[j] = ((x[i] - cv[i]) - fv[i]) - df[j];
+        }
+        float pn = pk_sdot_short(r, r, plsq);
+        float best = FLT_MAX;
+        uint32_t arg = 0;
+        for (uint32_t k = 0; k < PK_KS; ++k) {
+            float dt = pk_sdot_short(r, cb + k * plsq, plsq);
+            float d = (float)((double)(pn + cbn[k]) - 2.0 * (double)dt);
+            if (d < best) { best = d; arg = k; }
+        }
+        records_out[(uint64_t)pt * rec + 4 + m] = (uint8_t)arg;
+        for (uint32_t j = 0; j < plsq; ++j) {
+            const uint32_t i = m * plsq + j;
+            const float v = (cv[i] + fv[i]) + df[j];
+            offacc = fmaf(cb[arg * plsq + j], v, offacc);
+        }
+    }
+    if (valid) {
+        const uint32_t b = __float_as_uint(2 * offacc);
+        uint8_t* dst = records_out + (uint64_t)pt * rec;
+        dst[0] = b & 0xff; dst[1] = (b >> 8) & 0xff; dst[2] = (b >> 16) & 0xff; dst[3] = (b >> 24) & 0xff;
+    }
+}
+
+extern "C" hipError_t pk_launch_encode_pq(const float* X, const uint32_t* cell, uint32_t n, const float* coarse,
+                                          const float* fine, uint32_t F, uint32_t dim, const float* fcb, uint32_t fnsq,
+                                          const uint8_t* fcodes, uint64_t frec_stride, const float* pcb,
+                                          uint32_t pnsq, uint8_t* records_out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (fnsq == 0 || pnsq == 0 || dim % fnsq || dim % pnsq || dim / fnsq > 4 || dim / pnsq > 4)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pk_encode_pq, dim3((n + 255) / 256), dim3(256), 0, s, X, cell, n, coarse, fine, F, dim, fcb,
+                       fnsq, fcodes, frec_stride, pcb, pnsq, records_out);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // nearest_center argmin over a chunk of GEMM inner products ip[n x k]:
 //   d = (float)((double)(pn + cn[c]) - 2.0 * (double)ip), strict '<' from FLT_MAX.

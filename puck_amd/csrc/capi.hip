@@ -66,6 +66,9 @@ hipError_t pk_launch_assign_cells(const float*, const float*, const float*, cons
 hipError_t pk_launch_nearest_coarse(const float*, const float*, const float*, uint32_t, uint32_t, uint32_t*,
                                     hipStream_t);
 hipError_t pk_launch_fine_bound(const float*, uint32_t, uint32_t, float*, hipStream_t);
+hipError_t pk_launch_encode_pq(const float*, const uint32_t*, uint32_t, const float*, const float*, uint32_t, uint32_t,
+                               const float*, uint32_t, const uint8_t*, uint64_t, const float*, uint32_t, uint8_t*,
+                               hipStream_t);
 hipError_t pk_launch_encode_filter(const float*, const uint32_t*, uint32_t, const float*, const float*, uint32_t,
                                    uint32_t, const float*, uint32_t, float*, uint8_t*, uint64_t, uint64_t,
                                    hipStream_t);
@@ -120,7 +123,12 @@ struct puckgpu_index {
     uint32_t* mem2local = nullptr;
     float* rec_off = nullptr;
     uint8_t* rec_codes = nullptr;
-    float* raw = nullptr;
+    float* raw = nullptr;  // null when the index holds no raw vectors (whether_pq = true)
+    // second-stage PQ (whether_pq = true), attached by puckgpu_index_attach_pq
+    uint32_t pnsq = 0, plsq = 0;
+    float* pq_codebook = nullptr;
+    float* pq_off = nullptr;
+    uint8_t* pq_codes = nullptr;
     // reduced-precision copies of the centers (built on first use)
     mutable uint16_t* coarse16 = nullptr;
     mutable uint16_t* fine16 = nullptr;
@@ -195,7 +203,8 @@ void* scratch(puckgpu_ctx* ctx, size_t slot, size_t bytes, int* rc) {
 enum Slot {
     S_Q = 0, S_QN, S_QST, S_CD, S_FD, S_CSD, S_CSI, S_FSD, S_FSI, S_FT, S_FI, S_FN, S_OD, S_OI, S_ON, S_STATS,
     S_IN_Q, S_A0, S_A1, S_A2, S_A3, S_A4, S_A5, S_A6, S_A7, S_FLAGS, S_QLIST, S_TIEID, S_TIEN, S_QTOPK, S_QSCC,
-    S_QNF, S_PFEAT, S_PWORK, S_PNF, S_PY, S_PT, S_PCV, S_LP16, S_LP8, S_LPS, S_CAND_D, S_CAND_I, S_WLIST, S_COUNT
+    S_QNF, S_PFEAT, S_PWORK, S_PNF, S_PY, S_PT, S_PCV, S_LP16, S_LP8, S_LPS, S_CAND_D, S_CAND_I, S_WLIST, S_SCI,
+    S_SCK, S_SCM, S_COUNT
 };
 
 // Device view of an input array: the pointer itself, or a scratch copy of it.
@@ -222,7 +231,7 @@ void index_release(puckgpu_index* ix) {
     if (!ix) return;
     void* ptrs[] = {ix->coarse, ix->fine, ix->codebook, ix->coarse_norms, ix->stationary, ix->cell_start,
                     ix->mdo, ix->mem2local, ix->rec_off, ix->rec_codes, ix->raw, ix->coarse16, ix->fine16,
-                    ix->coarse8, ix->fine8, ix->coarse8s, ix->fine8s};
+                    ix->coarse8, ix->fine8, ix->coarse8s, ix->fine8s, ix->pq_codebook, ix->pq_off, ix->pq_codes};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete ix;
@@ -296,7 +305,7 @@ int puckgpu_index_load(puckgpu_ctx* ctx, const puckgpu_index_params* params, con
     if (cells >= 0xffffffffull) return set_err(ctx, PUCKGPU_EINVAL, "C*F too large");
     if (p.fine_cluster_count > 8192 || p.coarse_cluster_count > 8192)
         return set_err(ctx, PUCKGPU_EINVAL, "C and F must be <= 8192");
-    if (!coarse || !fine || !filter_codebook || (p.total_point_count && (!cell_assign || !filter_records || !features)))
+    if (!coarse || !fine || !filter_codebook || (p.total_point_count && (!cell_assign || !filter_records)))
         return set_err(ctx, PUCKGPU_EINVAL, "null array");
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
     const uint32_t C = p.coarse_cluster_count, F = p.fine_cluster_count, d = p.dim, nsq = p.filter_nsq,
@@ -335,7 +344,7 @@ int puckgpu_index_load(puckgpu_ctx* ctx, const puckgpu_index_params* params, con
     PK_TRY(dalloc(ctx, &ix->mem2local, N));
     PK_TRY(dalloc(ctx, &ix->rec_off, N));
     PK_TRY(dalloc(ctx, &ix->rec_codes, (size_t)N * nsq));
-    PK_TRY(dalloc(ctx, &ix->raw, (size_t)N * d));
+    if (features) PK_TRY(dalloc(ctx, &ix->raw, (size_t)N * d));
     const hipMemcpyKind kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     PK_TRYH(hipMemcpyAsync(ix->coarse, coarse, sizeof(float) * C * d, kind, s));
     PK_TRYH(hipMemcpyAsync(ix->fine, fine, sizeof(float) * F * d, kind, s));
@@ -354,8 +363,9 @@ int puckgpu_index_load(puckgpu_ctx* ctx, const puckgpu_index_params* params, con
     std::vector<float> h_off(N);
     const uint8_t* rec_dev = (const uint8_t*)dev_in(ctx, S_A1, filter_records, rec * N, device_ptrs, &rc);
     if (N && !rec_dev) { index_release(ix); return rc; }
-    const float* feat_dev = (const float*)dev_in(ctx, S_A2, features, sizeof(float) * N * d, device_ptrs, &rc);
-    if (N && !feat_dev) { index_release(ix); return rc; }
+    const float* feat_dev =
+        features ? (const float*)dev_in(ctx, S_A2, features, sizeof(float) * N * d, device_ptrs, &rc) : nullptr;
+    if (N && features && !feat_dev) { index_release(ix); return rc; }
     if (N) {
         if (device_ptrs) {
             PK_TRYH(hipMemcpyAsync(h_cell.data(), cell_assign, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, s));
@@ -402,7 +412,7 @@ int puckgpu_index_load(puckgpu_ctx* ctx, const puckgpu_index_params* params, con
     if (N) {
         PK_TRYH(pk_launch_gather_rows(rec_dev, rec, 0, (uint8_t*)ix->rec_off, 4, 4, ix->mem2local, N, s));
         PK_TRYH(pk_launch_gather_rows(rec_dev, rec, 4, ix->rec_codes, nsq, nsq, ix->mem2local, N, s));
-        PK_TRYH(pk_launch_gather_f32(feat_dev, d, ix->raw, ix->mem2local, N, s));
+        if (feat_dev) PK_TRYH(pk_launch_gather_f32(feat_dev, d, ix->raw, ix->mem2local, N, s));
     }
     PK_TRYH(hipStreamSynchronize(s));
     // host staging copies of records / features are no longer needed
@@ -419,6 +429,56 @@ int puckgpu_index_load(puckgpu_ctx* ctx, const puckgpu_index_params* params, con
 }
 
 void puckgpu_index_free(puckgpu_index* index) { index_release(index); }
+
+int puckgpu_index_attach_pq(puckgpu_ctx* ctx, puckgpu_index* ix, uint32_t pq_nsq, const float* pq_codebook,
+                            const uint8_t* pq_records, int device_ptrs) {
+    if (!ctx || !ix || !pq_codebook || (ix->p.total_point_count && !pq_records))
+        return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    const uint32_t d = ix->p.dim, N = ix->p.total_point_count;
+    if (pq_nsq == 0 || d % pq_nsq || !(d / pq_nsq == 1 || d / pq_nsq == 2 || d / pq_nsq == 4))
+        return set_err(ctx, PUCKGPU_EINVAL, "pq_nsq %u must divide dim %u into sub-vectors of 1, 2 or 4", pq_nsq, d);
+    if (pq_nsq > 128) return set_err(ctx, PUCKGPU_EINVAL, "pq_nsq must be <= 128 (its lookup table lives in LDS)");
+    if (ix->pq_codes) return set_err(ctx, PUCKGPU_EINVAL, "the index already has a second-stage PQ");
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    hipStream_t s = ctx->stream;
+    const uint32_t plsq = d / pq_nsq;
+    const size_t rec = 4 + pq_nsq;
+    int rc = 0;
+    float* cb = nullptr;
+    float* off = nullptr;
+    uint8_t* codes = nullptr;
+    if ((rc = dalloc(ctx, &cb, (size_t)pq_nsq * PK_KS * plsq)) || (rc = dalloc(ctx, &off, N)) ||
+        (rc = dalloc(ctx, &codes, (size_t)N * pq_nsq))) {
+        for (void* p : {(void*)cb, (void*)off, (void*)codes})
+            if (p) (void)hipFree(p);
+        return rc;
+    }
+    const uint8_t* rec_dev = (const uint8_t*)dev_in(ctx, S_A1, pq_records, rec * N, device_ptrs, &rc);
+    hipError_t e = hipSuccess;
+    if (N && !rec_dev) e = hipErrorOutOfMemory;
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(cb, pq_codebook, sizeof(float) * pq_nsq * PK_KS * plsq,
+                           device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s);
+    // records into memory order with the filter records' permutation
+    if (e == hipSuccess && N) e = pk_launch_gather_rows(rec_dev, rec, 0, (uint8_t*)off, 4, 4, ix->mem2local, N, s);
+    if (e == hipSuccess && N) e = pk_launch_gather_rows(rec_dev, rec, 4, codes, pq_nsq, pq_nsq, ix->mem2local, N, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (!device_ptrs && S_A1 < ctx->bufs.size() && ctx->bufs[S_A1].p) {
+        (void)hipFree(ctx->bufs[S_A1].p);
+        ctx->bufs[S_A1] = DevBuf();
+    }
+    if (e != hipSuccess) {
+        for (void* p : {(void*)cb, (void*)off, (void*)codes})
+            if (p) (void)hipFree(p);
+        return rc ? rc : set_err(ctx, PUCKGPU_EHIP, "attaching the second-stage PQ failed: %s", hipGetErrorString(e));
+    }
+    ix->pnsq = pq_nsq;
+    ix->plsq = plsq;
+    ix->pq_codebook = cb;
+    ix->pq_off = off;
+    ix->pq_codes = codes;
+    return PUCKGPU_OK;
+}
 
 int puckgpu_index_export(puckgpu_ctx* ctx, const puckgpu_index* ix, float* coarse_norms, float* stationary,
                          uint32_t* cell_start, float* min_dist_offset, uint32_t* mem2local) {
@@ -510,7 +570,18 @@ struct PassBufs {
     uint32_t* wlist = nullptr;   // queries taking the walk when budgets are given
     uint32_t* qorder = nullptr;  // workgroup -> query order of the per-query kernels
     uint32_t* wcount = nullptr;
+    // split budget scan: work items, candidate keys, per-query counts, the
+    // list of queries left to the per-query kernel and its length
+    uint4* sc_items = nullptr;
+    uint64_t* sc_keys = nullptr;
+    uint32_t* sc_meta = nullptr;
+    uint32_t* sc_olist = nullptr;
+    uint32_t* sc_ocount = nullptr;
 };
+
+// Candidate points (keys) and work items per query of the split budget scan;
+// a query with more is scanned by the per-query kernel.
+constexpr uint32_t PK_SC_CAPP = 8192, PK_SC_CAPI = 4096;
 
 // Per-pass configuration.
 struct PassCfg {
@@ -520,6 +591,7 @@ struct PassCfg {
     double rr = 1.0;
     int gemm_shape = 0;
     const ProbeUse* pu = nullptr;
+    bool budget = false;  // fine budgets may be given (explicit or predicted): the split scan's buffers
 };
 
 // Enqueues the device stages of one pass over m queries: no host
@@ -608,6 +680,8 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     wa.flags = b.flags; wa.exact = 0; wa.tie_ids = b.tie_ids; wa.tie_n = b.tie_n;
     wa.q_scc = q_scc; wa.q_nf = nf; wa.cstride = cfg.keep ? cfg.keep : 1; wa.fdist = b.fdist;
     wa.wlist = nullptr; wa.wcount = nullptr;
+    wa.sc_items = b.sc_items; wa.sc_keys = b.sc_keys; wa.sc_meta = b.sc_meta; wa.sc_olist = b.sc_olist;
+    wa.sc_ocount = b.sc_ocount; wa.sc_capp = PK_SC_CAPP; wa.sc_capi = PK_SC_CAPI;
     if (nf) {
         // the walk runs only over the queries without a budget below F (the list built above)
         wa.wlist = b.wlist;
@@ -623,6 +697,8 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     ra.out_d = od; ra.out_id = oi; ra.out_n = on;
     ra.qlist = order; ra.flags = b.flags; ra.exact = 0; ra.tie_ids = b.tie_ids; ra.tie_n = b.tie_n;
     ra.q_topk = q_topk; ra.out_stride = cfg.ostride; ra.q_nf = nf; ra.F = F;
+    ra.qn = b.qn; ra.pq_off = ix->pq_off; ra.pq_codes = ix->pq_codes; ra.pq_codebook = ix->pq_codebook;
+    ra.pnsq = ix->pnsq; ra.plsq = ix->plsq;
     PK_CHECK(pk_launch_rerank(&ra, m, s));
     if (ev) PK_CHECK(hipEventRecord(ev[4], s));
     if (wa_out) *wa_out = wa;
@@ -728,6 +804,13 @@ int scratch_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     b.wlist = (uint32_t*)scratch(ctx, S_WLIST, sizeof(uint32_t) * (2 * (size_t)cap + 1), &rc);
     b.qorder = b.wlist ? b.wlist + cap : nullptr;
     b.wcount = b.wlist ? b.wlist + 2 * (size_t)cap : nullptr;
+    if (cfg.budget) {
+        b.sc_items = (uint4*)scratch(ctx, S_SCI, sizeof(uint4) * (size_t)cap * PK_SC_CAPI, &rc);
+        b.sc_keys = (uint64_t*)scratch(ctx, S_SCK, sizeof(uint64_t) * (size_t)cap * PK_SC_CAPP, &rc);
+        b.sc_meta = (uint32_t*)scratch(ctx, S_SCM, sizeof(uint32_t) * (3 * (size_t)cap + 1), &rc);
+        b.sc_olist = b.sc_meta ? b.sc_meta + 2 * (size_t)cap : nullptr;
+        b.sc_ocount = b.sc_meta ? b.sc_meta + 3 * (size_t)cap : nullptr;
+    }
     if (pr) {
         b.pfeat = (float*)scratch(ctx, S_PFEAT, sizeof(float) * cap * pr->D, &rc);
         b.pwork = scratch(ctx, S_PWORK, pk_probe_work_bytes(cap, pr->D, pr->H), &rc);
@@ -790,6 +873,8 @@ int make_cfg(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_par
     cfg.ostride = ostride;
     cfg.rr = sp->radius_rate;
     cfg.pu = pu;
+    if (!ix->raw && !ix->pq_codes && ix->p.total_point_count)
+        return set_err(ctx, PUCKGPU_EINVAL, "the index holds neither raw vectors nor a second-stage PQ to rank by");
     if (sp->precision > 2) return set_err(ctx, PUCKGPU_EINVAL, "precision must be 0, 1 or 2");
     cfg.precision = (int)sp->precision;
     if (cfg.precision) {
@@ -818,6 +903,7 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
     PassCfg cfg;
     int rc = make_cfg(ctx, ix, sp, topk, ostride, pu, cfg);
     if (rc) return rc;
+    cfg.budget = q_nf_in != nullptr || (pu && pu->probe);
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
     const uint32_t d = ix->p.dim, ftk = cfg.ftk;
     hipStream_t s = ctx->stream;
@@ -1259,6 +1345,16 @@ int bucket_create(puckgpu_server* sv, uint32_t cap, const PassCfg& cfg, PkBucket
         }
         b.pwork = w;
     }
+    if (cfg.budget) {
+        if ((rc = balloc(ctx, bk, &b.sc_items, (size_t)cap * PK_SC_CAPI)) ||
+            (rc = balloc(ctx, bk, &b.sc_keys, (size_t)cap * PK_SC_CAPP)) ||
+            (rc = balloc(ctx, bk, &b.sc_meta, 3 * (size_t)cap + 1))) {
+            bucket_release(bk);
+            return rc;
+        }
+        b.sc_olist = b.sc_meta + 2 * (size_t)cap;
+        b.sc_ocount = b.sc_meta + 3 * (size_t)cap;
+    }
     *out = bk;
     return PUCKGPU_OK;
 }
@@ -1294,7 +1390,10 @@ int server_cfg(puckgpu_server* sv, PassCfg& cfg, ProbeUse& pu) {
     pu.probe = sv->probe;
     pu.bp = sv->bp;
     pu.budget_out = nullptr;
-    return make_cfg(sv->ctx, sv->ix, &sv->sp, sv->sp.topk_max, sv->sp.topk_max, sv->probe ? &pu : nullptr, cfg);
+    const int rc = make_cfg(sv->ctx, sv->ix, &sv->sp, sv->sp.topk_max, sv->sp.topk_max, sv->probe ? &pu : nullptr,
+                            cfg);
+    cfg.budget = sv->probe != nullptr;
+    return rc;
 }
 
 }  // namespace
@@ -1617,6 +1716,57 @@ int puckgpu_assign_cells(puckgpu_ctx* ctx, const float* coarse, const float* fin
         if (!device_ptrs) {
             PK_CHECK(hipMemcpyAsync(cell_out + p0, co, 4ull * m, hipMemcpyDeviceToHost, s));
             if (dist_out) PK_CHECK(hipMemcpyAsync(dist_out + p0, cdo, 4ull * m, hipMemcpyDeviceToHost, s));
+            PK_CHECK(hipStreamSynchronize(s));
+        }
+    }
+    PK_CHECK(hipStreamSynchronize(s));
+    return PUCKGPU_OK;
+}
+
+int puckgpu_encode_pq(puckgpu_ctx* ctx, const float* coarse, const float* fine, uint32_t C, uint32_t F, uint32_t dim,
+                      const float* filter_codebook, uint32_t filter_nsq, const uint8_t* filter_records,
+                      const float* pq_codebook, uint32_t pq_nsq, uint32_t n, const float* x, const uint32_t* cell,
+                      uint8_t* records_out, int device_ptrs) {
+    if (!ctx || !coarse || !fine || !filter_codebook || !pq_codebook || !records_out ||
+        (n && (!x || !cell || !filter_records)))
+        return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    if (filter_nsq == 0 || pq_nsq == 0 || dim % filter_nsq || dim % pq_nsq || dim / filter_nsq > 4 || dim / pq_nsq > 4)
+        return set_err(ctx, PUCKGPU_EINVAL, "dim / filter_nsq and dim / pq_nsq must be <= 4");
+    if (n == 0) return PUCKGPU_OK;
+    if (C == 0 || F == 0 || (uint64_t)C * F >= 0xffffffffull) return set_err(ctx, PUCKGPU_EINVAL, "bad C/F");
+    if (!device_ptrs)
+        for (uint32_t i = 0; i < n; ++i)
+            if (cell[i] >= C * F) return set_err(ctx, PUCKGPU_EINVAL, "cell[%u] = %u out of range", i, cell[i]);
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    hipStream_t s = ctx->stream;
+    int rc = 0;
+    const float* cd = (const float*)dev_in(ctx, S_A0, coarse, 4ull * C * dim, device_ptrs, &rc);
+    const float* fd = (const float*)dev_in(ctx, S_A1, fine, 4ull * F * dim, device_ptrs, &rc);
+    const float* fcb = (const float*)dev_in(ctx, S_A2, filter_codebook, 4ull * filter_nsq * PK_KS * (dim / filter_nsq),
+                                            device_ptrs, &rc);
+    const float* pcb = (const float*)dev_in(ctx, S_A6, pq_codebook, 4ull * pq_nsq * PK_KS * (dim / pq_nsq),
+                                            device_ptrs, &rc);
+    if (rc) return rc;
+    const uint64_t frec = 4 + filter_nsq, prec = 4 + pq_nsq;
+    const uint32_t CHK = 1u << 20;
+    const uint32_t cbn = n < CHK ? n : CHK;
+    float* xs = device_ptrs ? nullptr : (float*)scratch(ctx, S_A3, 4ull * cbn * dim, &rc);
+    uint32_t* cs = device_ptrs ? nullptr : (uint32_t*)scratch(ctx, S_A4, 4ull * cbn, &rc);
+    uint8_t* fr = device_ptrs ? nullptr : (uint8_t*)scratch(ctx, S_A7, frec * cbn, &rc);
+    uint8_t* ro = device_ptrs ? nullptr : (uint8_t*)scratch(ctx, S_A5, prec * cbn, &rc);
+    if (rc) return rc;
+    for (uint32_t p0 = 0; p0 < n; p0 += CHK) {
+        const uint32_t m = (n - p0) < CHK ? (n - p0) : CHK;
+        if (device_ptrs) {
+            PK_CHECK(pk_launch_encode_pq(x + (size_t)p0 * dim, cell + p0, m, cd, fd, F, dim, fcb, filter_nsq,
+                                         filter_records + (size_t)p0 * frec + 4, frec, pcb, pq_nsq,
+                                         records_out + (size_t)p0 * prec, s));
+        } else {
+            PK_CHECK(hipMemcpyAsync(xs, x + (size_t)p0 * dim, 4ull * m * dim, hipMemcpyHostToDevice, s));
+            PK_CHECK(hipMemcpyAsync(cs, cell + p0, 4ull * m, hipMemcpyHostToDevice, s));
+            PK_CHECK(hipMemcpyAsync(fr, filter_records + (size_t)p0 * frec, frec * m, hipMemcpyHostToDevice, s));
+            PK_CHECK(pk_launch_encode_pq(xs, cs, m, cd, fd, F, dim, fcb, filter_nsq, fr + 4, frec, pcb, pq_nsq, ro, s));
+            PK_CHECK(hipMemcpyAsync(records_out + (size_t)p0 * prec, ro, prec * m, hipMemcpyDeviceToHost, s));
             PK_CHECK(hipStreamSynchronize(s));
         }
     }

@@ -107,21 +107,21 @@ __device__ void pk_build_lut_t(float* __restrict__ lut, const float* __restrict_
         float c[U][LSQ];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const uint32_t e = e0 + u * blockDim.x;
-            if (e < n) {
-                if constexpr (LSQ == 4) {
-                    const float4 w = *(const float4*)(codebook + (uint64_t)e * 4);
-                    c[u][0] = w.x;
-                    c[u][1] = w.y;
-                    c[u][2] = w.z;
-                    c[u][3] = w.w;
-                } else if constexpr (LSQ == 2) {
-                    const float2 w = *(const float2*)(codebook + (uint64_t)e * 2);
-                    c[u][0] = w.x;
-                    c[u][1] = w.y;
-                } else {
-                    c[u][0] = codebook[e];
-                }
+            // at an in-bounds entry for every lane: a load under a lane
+            // condition is waited for before the next one is issued
+            const uint32_t e = min(e0 + u * blockDim.x, n - 1u);
+            if constexpr (LSQ == 4) {
+                const float4 w = *(const float4*)(codebook + (uint64_t)e * 4);
+                c[u][0] = w.x;
+                c[u][1] = w.y;
+                c[u][2] = w.z;
+                c[u][3] = w.w;
+            } else if constexpr (LSQ == 2) {
+                const float2 w = *(const float2*)(codebook + (uint64_t)e * 2);
+                c[u][0] = w.x;
+                c[u][1] = w.y;
+            } else {
+                c[u][0] = codebook[e];
             }
         }
 #pragma unroll

@@ -39,6 +39,13 @@ struct PkWalkArgs {
     int fine_given;          // a query whose flags hold bit 3 on entry has its fine row in the
                              // reference's heap order: its fine-distance ties need no flag, and
                              // its flags are rewritten (not or-ed) with what this walk finds
+    // split budget scan (scan.hip); sc_items null: every budget query takes the per-query kernel
+    uint4* sc_items;         // nq x sc_capi work items {first memory id, points, key slot, cell distance bits}
+    uint64_t* sc_keys;       // nq x sc_capp (filter distance ord << 32 | memory id)
+    uint32_t* sc_meta;       // nq x 2: items (0xffffffff: left to the per-query kernel), points
+    uint32_t* sc_olist;      // nq: the queries left to the per-query kernel
+    uint32_t* sc_ocount;     // 1
+    uint32_t sc_capp, sc_capi;
 };
 
 struct PkRerankArgs {
@@ -63,4 +70,10 @@ struct PkRerankArgs {
     uint32_t out_stride;     // row stride of out_d / out_id (>= topk)
     const uint32_t* q_nf;    // nq or null: fine budgets; a query with 0 < budget < F came from the
     uint32_t F;              // budget scan and ranks by (distance, local id), without replay
+    // second-stage PQ (whether_pq = true; pq_codes null: exact L2 over raw)
+    const float* qn;         // nq      |q|^2
+    const float* pq_off;     // N       PQ offsets, memory order
+    const uint8_t* pq_codes; // N x pnsq
+    const float* pq_codebook;// pnsq x 256 x plsq
+    uint32_t pnsq, plsq;
 };

@@ -1,0 +1,365 @@
+// scan.hip -- the budget scan (a query's fine budget b < F; no counterpart in
+// the reference) as three data-parallel kernels:
+//
+//   pk_scan_cells   one workgroup per query: in each walked coarse list, the cut
+//                   at the b-th smallest cell distance (cd + stationary) + fine
+//                   distance (the reference's cell distance expression,
+//                   puck_index.cpp:535), and the selected non-empty cells cut
+//                   into work items of at most 4 consecutive points
+//                   {first memory id, count, key slot, cell distance}
+//   pk_scan_score   one workgroup per query: the query's filter lookup table
+//                   in LDS (quantization.cpp:260-270), then every point of every
+//                   item gets the reference's filter distance
+//                   (compute_quantized_distance, puck_index.cpp:346-390: t =
+//                   fl(2 * cell_dist + offset) + lookup sum in the 4-lane order of
+//                   puck_index.cpp:75-113), written as a (distance, memory id)
+//                   key; no block barrier after the table is built
+//   pk_scan_cut     one workgroup per query: the filter_topk smallest keys
+//                   (radix select on the distance bits, then on the ids of the
+//                   keys tied at the cut) into the filter rows
+//
+// The result is the one of the per-query scan (pk_scan_budget in search.hip):
+// the filter set is the filter_topk smallest (distance, memory id) keys over
+// every point of the selected cells, which does not depend on the order in
+// which points are scored.  A query whose candidates exceed the item or key
+// buffers, or whose index has F > PK_SC_CPT * PK_SC_CELL_THREADS, is marked
+// and left to the per-query kernel (pk_scan_overflow_list builds its list).
+#include "common.h"
+#include "launch_args.h"
+
+#define PK_SC_OVERFLOW 0xffffffffu
+#define PK_SC_ITEM 4          // points per work item
+
+__device__ __forceinline__ bool pk_sc_budget_query(const PkWalkArgs& a, uint32_t qi) {
+    return a.q_nf && a.q_nf[qi] != 0u && a.q_nf[qi] < a.F;
+}
+
+// ---------------------------------------------------------------------------
+#define PK_SC_CELL_THREADS 256
+#define PK_SC_CPT 8  // columns (fine clusters) per thread: F <= 2048
+__global__ __launch_bounds__(PK_SC_CELL_THREADS) void pk_scan_cells(PkWalkArgs a) {
+    __shared__ uint32_t hist[260];
+    __shared__ uint32_t tmp[PK_SC_CELL_THREADS / 64 + 1];
+    __shared__ uint32_t red[2 * PK_SC_CELL_THREADS / 64];
+    __shared__ uint32_t cnt_sh;
+    const uint32_t tid = threadIdx.x, NT = PK_SC_CELL_THREADS;
+    const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
+    if (!pk_sc_budget_query(a, qi)) return;
+    uint32_t* meta = a.sc_meta + 2ull * qi;
+    if (a.qstatus[qi] != 0) {
+        if (tid == 0) {
+            meta[0] = 0;
+            meta[1] = 0;
+            if (a.stats) {
+                for (int k = 0; k < 4; ++k) a.stats[qi * 4 + k] = 0;
+            }
+        }
+        return;
+    }
+    const uint32_t F = a.F, b = a.q_nf[qi];
+    if (F > PK_SC_CPT * NT) {
+        if (tid == 0) meta[0] = PK_SC_OVERFLOW;
+        return;
+    }
+    uint32_t scc = a.scc;
+    if (a.q_scc) scc = a.q_scc[qi] < scc ? a.q_scc[qi] : scc;
+    const float* fdq = a.fdist + (uint64_t)qi * F;
+    uint4* items = a.sc_items + (uint64_t)qi * a.sc_capi;
+    uint32_t nit = 0, npt = 0, ncell = 0;
+    bool ovf = false;
+    for (uint32_t l = 0; l < scc; ++l) {
+        const uint32_t c = a.csel_id[(uint64_t)qi * a.cstride + l];
+        const float cd = a.csel_d[(uint64_t)qi * a.cstride + l];
+        const float* stc = a.stationary + (uint64_t)c * F;
+        const uint32_t* csc = a.cell_start + (uint64_t)c * F;
+        uint32_t kv[PK_SC_CPT], c0[PK_SC_CPT], c1[PK_SC_CPT];
+        // every load of the list at once, at in-bounds columns (a load under a
+        // lane condition is waited for before the next one is issued)
+        float sv[PK_SC_CPT], fv[PK_SC_CPT];
+#pragma unroll
+        for (int j = 0; j < PK_SC_CPT; ++j) {
+            const uint32_t k = min(j * NT + tid, F - 1u);
+            sv[j] = stc[k];
+            fv[j] = fdq[k];
+            c0[j] = csc[k];
+            c1[j] = csc[k + 1];
+        }
+#pragma unroll
+        for (int j = 0; j < PK_SC_CPT; ++j) kv[j] = pk_f2ord((cd + sv[j]) + fv[j]);
+        // the list's cut: b-th smallest cell key
+        const uint32_t kcut = pk_select_rank_each(
+            [&](auto sink) {
+#pragma unroll
+                for (int j = 0; j < PK_SC_CPT; ++j)
+                    if (j * NT + tid < F) sink(kv[j]);
+            },
+            b, hist, tmp, red);
+        uint32_t li = 0, lp = 0, lc = 0;
+#pragma unroll
+        for (int j = 0; j < PK_SC_CPT; ++j) {
+            const bool sel = j * NT + tid < F && kv[j] <= kcut;
+            c1[j] = sel ? c1[j] - c0[j] : 0u;  // the cell's point count when selected
+            li += (c1[j] + PK_SC_ITEM - 1) / PK_SC_ITEM;
+            lp += c1[j];
+            lc += c1[j] > 0 ? 1u : 0u;
+        }
+        uint32_t ti, tp;
+        uint32_t ib = pk_block_exscan(li, tmp, &ti);
+        uint32_t pb = pk_block_exscan(lp, tmp, &tp);
+        if (tid == 0) cnt_sh = 0;
+        __syncthreads();
+        atomicAdd(&cnt_sh, lc);
+        __syncthreads();
+        const uint32_t tc = cnt_sh;
+        __syncthreads();
+        if (nit + ti > a.sc_capi || npt + tp > a.sc_capp) {  // block-uniform
+            ovf = true;
+            break;
+        }
+        ib += nit;
+        pb += npt;
+#pragma unroll
+        for (int j = 0; j < PK_SC_CPT; ++j) {
+            const uint32_t cnt = c1[j];
+            const uint32_t cdb = __float_as_uint(pk_ord2f(kv[j]));
+            for (uint32_t p = 0; p < cnt; p += PK_SC_ITEM) {
+                const uint32_t m = cnt - p < PK_SC_ITEM ? cnt - p : PK_SC_ITEM;
+                items[ib++] = make_uint4(c0[j] + p, m, pb, cdb);
+                pb += m;
+            }
+        }
+        nit += ti;
+        npt += tp;
+        ncell += tc;
+    }
+    if (tid == 0) {
+        meta[0] = ovf ? PK_SC_OVERFLOW : nit;
+        meta[1] = npt;
+        if (a.stats && !ovf) {
+            a.stats[qi * 4 + 0] = 0;
+            a.stats[qi * 4 + 1] = 0;
+            a.stats[qi * 4 + 2] = ncell;
+            a.stats[qi * 4 + 3] = npt;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Filter distance of up to PK_SC_ITEM points of one item; NV: nsq / 16 code
+// vectors per record (0: byte loads for any nsq).
+template <int NV>
+__device__ __forceinline__ void pk_sc_score_item(const PkWalkArgs& a, const float* __restrict__ lut, uint4 it,
+                                                 bool valid, uint64_t* __restrict__ keys) {
+    const uint32_t nsq = a.nsq;
+    const uint32_t m = valid ? it.y : 0u;
+    const float cdl = __uint_as_float(it.w);
+    float off[PK_SC_ITEM];
+    uint4 cw[PK_SC_ITEM][NV > 0 ? NV : 1];
+#pragma unroll
+    for (int h = 0; h < PK_SC_ITEM; ++h) {
+        // every load at an in-bounds index (the item's last point again past its end)
+        const uint32_t P = it.x + (h < (int)it.y ? (uint32_t)h : (it.y ? it.y - 1u : 0u));
+        off[h] = a.rec_off[P];
+        if constexpr (NV > 0) {
+            const uint4* cp = (const uint4*)(a.rec_codes + (uint64_t)P * nsq);
+#pragma unroll
+            for (int g = 0; g < NV; ++g) cw[h][g] = cp[g];
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < PK_SC_ITEM; ++h) {
+        const uint32_t P = it.x + h;
+        float tv = 2.0f * cdl + off[h];  // 2x is exact: one rounding, as in double
+        float acc[4] = {0.f, 0.f, 0.f, 0.f};
+        if constexpr (NV > 0) {
+#pragma unroll
+            for (int g = 0; g < NV; ++g) {
+                const uint32_t ws[4] = {cw[h][g].x, cw[h][g].y, cw[h][g].z, cw[h][g].w};
+#pragma unroll
+                for (int hh = 0; hh < 4; ++hh) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const uint32_t code = (ws[hh] >> (8 * j)) & 0xffu;
+                        acc[j] = acc[j] + lut[(16 * g + 4 * hh + j) * PK_KS + code];
+                    }
+                }
+            }
+        } else {
+            if ((uint32_t)h < m) {
+                const uint8_t* codes = a.rec_codes + (uint64_t)P * nsq;
+                for (uint32_t q = 0; q < nsq; ++q) acc[q & 3] = acc[q & 3] + lut[q * PK_KS + codes[q]];
+            }
+        }
+        tv = tv + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
+        if ((uint32_t)h < m) keys[it.z + h] = ((uint64_t)pk_f2ord(tv) << 32) | P;
+    }
+}
+
+template <int NT, int NV>
+__global__ __launch_bounds__(NT, 6) void pk_scan_score(PkWalkArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
+    if (!pk_sc_budget_query(a, qi)) return;
+    const uint32_t nit = a.sc_meta[2ull * qi];
+    if (nit == PK_SC_OVERFLOW || nit == 0) return;  // block-uniform
+    float* lut = (float*)smem;
+    float* qv = lut + a.nsq * PK_KS;
+    const float* qg = a.q + (uint64_t)qi * a.dim;
+    for (uint32_t i = threadIdx.x; i < a.dim; i += NT) qv[i] = qg[i];
+    __syncthreads();
+    pk_build_lut(lut, qv, a.codebook, a.nsq, a.lsq);
+    __syncthreads();
+    const uint4* items = a.sc_items + (uint64_t)qi * a.sc_capi;
+    uint64_t* keys = a.sc_keys + (uint64_t)qi * a.sc_capp;
+    for (uint32_t i0 = 0; i0 < nit; i0 += NT) {
+        const uint32_t i = i0 + threadIdx.x;
+        const uint4 it = items[i < nit ? i : nit - 1u];
+        pk_sc_score_item<NV>(a, lut, it, i < nit, keys);
+    }
+}
+
+// ---------------------------------------------------------------------------
+#define PK_SC_CUT_THREADS 256
+#define PK_SC_CUT_KPT 32  // keys per thread: sc_capp <= 8192
+
+__global__ __launch_bounds__(PK_SC_CUT_THREADS) void pk_scan_cut(PkWalkArgs a) {
+    __shared__ uint32_t hist[260];
+    __shared__ uint32_t tmp[PK_SC_CUT_THREADS / 64 + 1];
+    __shared__ uint32_t red[2 * PK_SC_CUT_THREADS / 64];
+    __shared__ uint32_t cnt[2];
+    const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
+    const uint32_t tid = threadIdx.x;
+    if (!pk_sc_budget_query(a, qi)) return;
+    const uint32_t nit = a.sc_meta[2ull * qi];
+    if (nit == PK_SC_OVERFLOW) return;  // the per-query kernel's query
+    const uint32_t ftk = a.ftk;
+    if (tid == 0) {
+        if (a.flags) a.flags[qi] = 0;  // deterministic: nothing to replay
+        if (a.tie_n) a.tie_n[qi] = 0;
+    }
+    if (a.qstatus[qi] != 0 || nit == 0) {
+        if (tid == 0) a.out_n[qi] = 0;
+        return;
+    }
+    const uint32_t n = a.sc_meta[2ull * qi + 1];
+    const uint64_t* keys = a.sc_keys + (uint64_t)qi * a.sc_capp;
+    // thread t holds keys t, t + NT, ... (coalesced loads, in bounds)
+    uint64_t x[PK_SC_CUT_KPT];
+#pragma unroll
+    for (int j = 0; j < PK_SC_CUT_KPT; ++j) {
+        // at an in-bounds index for every lane (conditional loads are waited
+        // for one at a time); keys past n are masked below
+        const uint32_t i = j * PK_SC_CUT_THREADS + tid;
+        x[j] = keys[i < n ? i : n - 1u];
+    }
+#pragma unroll
+    for (int j = 0; j < PK_SC_CUT_KPT; ++j)
+        if (j * PK_SC_CUT_THREADS + tid >= n) x[j] = ~0ull;
+    uint64_t K = ~0ull;
+    if (n > ftk) {
+        const uint32_t kc = pk_select_rank_each(
+            [&](auto sink) {
+#pragma unroll
+                for (int j = 0; j < PK_SC_CUT_KPT; ++j)
+                    if (j * PK_SC_CUT_THREADS + tid < n) sink((uint32_t)(x[j] >> 32));
+            },
+            ftk, hist, tmp, red);
+        // keys strictly below the cut distance, and at it
+        uint32_t lt = 0, eq = 0;
+#pragma unroll
+        for (int j = 0; j < PK_SC_CUT_KPT; ++j) {
+            const uint32_t hv = (uint32_t)(x[j] >> 32);
+            const bool v = j * PK_SC_CUT_THREADS + tid < n;
+            lt += (v && hv < kc) ? 1u : 0u;
+            eq += (v && hv == kc) ? 1u : 0u;
+        }
+        if (tid == 0) cnt[0] = cnt[1] = 0;
+        __syncthreads();
+        atomicAdd(&cnt[0], lt);
+        atomicAdd(&cnt[1], eq);
+        __syncthreads();
+        const uint32_t nlt = cnt[0], neq = cnt[1];
+        if (nlt + neq == ftk) {
+            K = ((uint64_t)kc << 32) | 0xffffffffull;
+        } else {
+            // the cut falls inside the keys tied at kc: rank ftk - nlt of their ids
+            const uint32_t idc = pk_select_rank_each(
+                [&](auto sink) {
+#pragma unroll
+                    for (int j = 0; j < PK_SC_CUT_KPT; ++j)
+                        if (j * PK_SC_CUT_THREADS + tid < n && (uint32_t)(x[j] >> 32) == kc) sink((uint32_t)x[j]);
+                },
+                ftk - nlt, hist, tmp, red);
+            K = ((uint64_t)kc << 32) | idc;
+        }
+    }
+    uint32_t keep = 0;
+#pragma unroll
+    for (int j = 0; j < PK_SC_CUT_KPT; ++j) keep += (j * PK_SC_CUT_THREADS + tid < n && x[j] <= K) ? 1u : 0u;
+    uint32_t tot;
+    uint32_t p = pk_block_exscan(keep, tmp, &tot);
+    float* ot = a.out_t + (uint64_t)qi * ftk;
+    uint32_t* oid = a.out_id + (uint64_t)qi * ftk;
+#pragma unroll
+    for (int j = 0; j < PK_SC_CUT_KPT; ++j) {
+        if (j * PK_SC_CUT_THREADS + tid < n && x[j] <= K) {
+            ot[p] = pk_ord2f((uint32_t)(x[j] >> 32));
+            oid[p] = (uint32_t)x[j];
+            ++p;
+        }
+    }
+    if (tid == 0) a.out_n[qi] = tot;
+}
+
+// Ordered list of the batch's budget queries marked for the per-query kernel
+// (one workgroup of 1024 threads, block scans in index order).
+__global__ __launch_bounds__(1024) void pk_scan_overflow_list(PkWalkArgs a, uint32_t n) {
+    __shared__ uint32_t tmp[1024 / 64 + 1];
+    uint32_t base = 0;
+    for (uint32_t i0 = 0; i0 < n; i0 += 1024) {
+        const uint32_t i = i0 + threadIdx.x;
+        const bool o = i < n && pk_sc_budget_query(a, i) && a.qstatus[i] == 0 &&
+                       a.sc_meta[2ull * i] == PK_SC_OVERFLOW;
+        uint32_t tot;
+        const uint32_t p = pk_block_exscan(o ? 1u : 0u, tmp, &tot);
+        if (o) a.sc_olist[base + p] = i;
+        base += tot;
+    }
+    if (threadIdx.x == 0) *a.sc_ocount = base;
+}
+
+// LDS of pk_scan_score: the lookup table and the query.
+extern "C" size_t pk_scan_score_smem(uint32_t dim, uint32_t nsq) {
+    return sizeof(float) * ((size_t)nsq * PK_KS + ((dim + 3) & ~3u));
+}
+
+// Whether the three kernels serve an index of these shapes (else every budget
+// query takes the per-query kernel): a lookup table within 64 KB of LDS.
+extern "C" int pk_scan_split_ok(uint32_t dim, uint32_t nsq, uint32_t F) {
+    (void)F;
+    return pk_scan_score_smem(dim, nsq) <= 64 * 1024;
+}
+
+// Enqueues the three kernels and the overflow list over nq queries.
+extern "C" hipError_t pk_launch_scan_split(const PkWalkArgs* args, uint32_t nq, hipStream_t s) {
+    if (nq == 0) return hipSuccess;
+    const PkWalkArgs& a = *args;
+    if (!a.q_nf || !a.fdist || !a.sc_items || !a.sc_keys || !a.sc_meta || !a.sc_olist || !a.sc_ocount)
+        return hipErrorInvalidValue;
+    if (a.sc_capp > PK_SC_CUT_KPT * PK_SC_CUT_THREADS || a.sc_capp < a.ftk) return hipErrorInvalidValue;
+    const size_t smem = pk_scan_score_smem(a.dim, a.nsq);
+    if (smem > 64 * 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pk_scan_cells, dim3(nq), dim3(PK_SC_CELL_THREADS), 0, s, a);
+    const uint32_t nv = (a.nsq & 15u) == 0 && a.nsq <= 64 ? a.nsq / 16u : 0u;
+    constexpr int NT = 512;
+    switch (nv) {
+    case 1: hipLaunchKernelGGL((pk_scan_score<NT, 1>), dim3(nq), dim3(NT), smem, s, a); break;
+    case 2: hipLaunchKernelGGL((pk_scan_score<NT, 2>), dim3(nq), dim3(NT), smem, s, a); break;
+    case 3: hipLaunchKernelGGL((pk_scan_score<NT, 3>), dim3(nq), dim3(NT), smem, s, a); break;
+    case 4: hipLaunchKernelGGL((pk_scan_score<NT, 4>), dim3(nq), dim3(NT), smem, s, a); break;
+    default: hipLaunchKernelGGL((pk_scan_score<NT, 0>), dim3(nq), dim3(NT), smem, s, a); break;
+    }
+    hipLaunchKernelGGL(pk_scan_cut, dim3(nq), dim3(PK_SC_CUT_THREADS), 0, s, a);
+    hipLaunchKernelGGL(pk_scan_overflow_list, dim3(1), dim3(1024), 0, s, a, nq);
+    return hipGetLastError();
+}

@@ -1441,9 +1441,7 @@ extern "C" hipError_t pk_launch_build_walk_list(const uint32_t* q_nf, uint32_t n
 // walk's): one block per query, returning at once for queries without a
 // budget below F.  Same LDS layout as pk_walk.
 template <int FTKM>
-__global__ __launch_bounds__(SCAN_THREADS) void pk_scan_budget_kernel(PkWalkArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
+__device__ void pk_scan_budget_one(const PkWalkArgs& a, uint32_t qi, unsigned char* smem) {
     const uint32_t ftk = a.ftk, F = a.F, nsq = a.nsq;
     if (!(a.q_nf && a.q_nf[qi] != 0u && a.q_nf[qi] < F)) return;
     // own LDS plan: LUT, small arrays, then the candidate buffer (a.scan_cap keys)
@@ -1472,6 +1470,24 @@ __global__ __launch_bounds__(SCAN_THREADS) void pk_scan_budget_kernel(PkWalkArgs
     __syncthreads();
     (void)nsq;
     pk_scan_budget<FTKM>(a, s, qi, a.q_nf[qi], B, a.scan_cap, a.scan_round, hist);
+}
+
+template <int FTKM>
+__global__ __launch_bounds__(SCAN_THREADS) void pk_scan_budget_kernel(PkWalkArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    pk_scan_budget_one<FTKM>(a, a.qlist ? a.qlist[blockIdx.x] : blockIdx.x, smem);
+}
+
+// The same over the device list sc_olist[0..*sc_ocount) (the budget queries
+// the split scan of scan.hip leaves to this kernel), grid-stride.
+template <int FTKM>
+__global__ __launch_bounds__(SCAN_THREADS) void pk_scan_budget_list(PkWalkArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t cnt = *a.sc_ocount;
+    for (uint32_t i = blockIdx.x; i < cnt; i += gridDim.x) {
+        pk_scan_budget_one<FTKM>(a, a.sc_olist[i], smem);
+        __syncthreads();  // every thread is done with the LDS before the next query
+    }
 }
 
 extern "C" size_t pk_scan_smem_fixed(uint32_t dim, uint32_t nsq) {
@@ -1506,6 +1522,9 @@ extern "C" uint32_t pk_walk_ftk_max(void) { return WALK_FTK_MAX; }
 // without a budget runs on s2 concurrently with the budget scan on s (event
 // fork / join, capturable in a graph); ev_mid is recorded on s right after
 // the scan.
+extern "C" int pk_scan_split_ok(uint32_t dim, uint32_t nsq, uint32_t F);
+extern "C" hipError_t pk_launch_scan_split(const PkWalkArgs* args, uint32_t nq, hipStream_t s);
+
 extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStream_t s, hipEvent_t ev_mid,
                                      hipStream_t s2, hipEvent_t ev_fork, hipEvent_t ev_join) {
     if (nq == 0) return hipSuccess;
@@ -1534,7 +1553,8 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
         const void* fns[] = {(const void*)pk_walk<1024, false>, (const void*)pk_walk<2048, false>,
                              (const void*)pk_walk<1024, true>, (const void*)pk_walk<2048, true>,
                              (const void*)pk_walk_list<1024>, (const void*)pk_walk_list<2048>,
-                             (const void*)pk_scan_budget_kernel<1024>, (const void*)pk_scan_budget_kernel<2048>};
+                             (const void*)pk_scan_budget_kernel<1024>, (const void*)pk_scan_budget_kernel<2048>,
+                             (const void*)pk_scan_budget_list<1024>, (const void*)pk_scan_budget_list<2048>};
         size_t lo = 160 * 1024;
         for (const void* f : fns) {
             size_t a = 0;
@@ -1573,10 +1593,19 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
         if ((e = hipEventRecord(ev_join, s2)) != hipSuccess) return e;
     }
     if (args->q_nf) {
-        if (args->ftk <= 1024)
+        if (args->sc_items && pk_scan_split_ok(args->dim, args->nsq, args->F)) {
+            // the split scan, then the per-query kernel over the queries it leaves
+            if ((e = pk_launch_scan_split(&sa, nq, s)) != hipSuccess) return e;
+            const uint32_t lg = nq < 1024 ? nq : 1024;
+            if (args->ftk <= 1024)
+                hipLaunchKernelGGL(pk_scan_budget_list<1024>, dim3(lg), dim3(SCAN_THREADS), smem_scan, s, sa);
+            else
+                hipLaunchKernelGGL(pk_scan_budget_list<2048>, dim3(lg), dim3(SCAN_THREADS), smem_scan, s, sa);
+        } else if (args->ftk <= 1024) {
             hipLaunchKernelGGL(pk_scan_budget_kernel<1024>, dim3(nq), dim3(SCAN_THREADS), smem_scan, s, sa);
-        else
+        } else {
             hipLaunchKernelGGL(pk_scan_budget_kernel<2048>, dim3(nq), dim3(SCAN_THREADS), smem_scan, s, sa);
+        }
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (ev_mid) (void)hipEventRecord(ev_mid, s);
@@ -1632,6 +1661,33 @@ __device__ bool pk_sorted_prefix(uint64_t* keys, uint32_t n, uint32_t K1) {
     return true;
 }
 
+// Second-stage PQ distance of a filter survivor (rank_topN_points,
+// puck_index.cpp:402-427): (filter distance - |q|^2) + PQ offset, then + the
+// LUT sum of its PQ codes in the 4-lane order of lookup_dist_table (:75-113).
+__device__ __forceinline__ float pk_pq_dist(const PkRerankArgs& a, const float* __restrict__ lut, uint32_t mid,
+                                            float fd, float qn) {
+    const uint32_t nsq = a.pnsq;
+    float t = (fd - qn) + a.pq_off[mid];
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    const uint8_t* codes = a.pq_codes + (uint64_t)mid * nsq;
+    uint32_t m = 0;
+    if ((nsq & 15u) == 0) {
+        // 16 codes per load, issued before the lookups of a group
+        for (; m < nsq; m += 16) {
+            const uint4 w = *(const uint4*)(codes + m);
+            const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+            for (int hh = 0; hh < 4; ++hh)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[j] = acc[j] + lut[(m + 4 * hh + j) * PK_KS + ((ws[hh] >> (8 * j)) & 0xffu)];
+        }
+    } else {
+        for (; m < nsq; ++m) acc[m & 3] = acc[m & 3] + lut[m * PK_KS + codes[m]];
+    }
+    return t + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
+}
+
 __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
@@ -1647,8 +1703,15 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
     float* qv = (float*)(smem + sizeof(uint64_t) * n2);
     float* dist = qv + ((a.dim + 3) & ~3u);
     uint2* RH = (uint2*)(((uintptr_t)(dist + ((ftk + 3) & ~3u)) + 15) & ~(uintptr_t)15);
+    float* plut = (float*)(RH + ((a.topk + 2) & ~1u));  // PQ lookup table (whether_pq)
+    const bool pq = a.pq_codes != nullptr;
     for (uint32_t i = threadIdx.x; i < a.dim; i += blockDim.x) qv[i] = a.q[(uint64_t)qi * a.dim + i];
     __syncthreads();
+    const float qn = pq ? a.qn[qi] : 0.f;
+    if (pq && n > 0) {
+        pk_build_lut(plut, qv, a.pq_codebook, a.pnsq, a.plsq);
+        __syncthreads();
+    }
     // filter-boundary tie group (entries whose filter distance equals the final
     // top, kept or dropped): smallest exact distance among its members
     const uint32_t tn = (!exact && !bud && a.tie_n && n >= ftk) ? a.tie_n[qi] : 0u;
@@ -1661,7 +1724,22 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
     // Two rows per group and iteration, every load of a 128-element block of
     // both rows issued before the first is used (the gathers are latency
     // bound: one row is a few cache lines).
-    {
+    if (pq) {
+        // one survivor per thread
+        for (uint32_t r = threadIdx.x; r < n2; r += blockDim.x) {
+            const uint32_t rc = r < ftk ? r : ftk - 1u;
+            const uint32_t fid = a.filt_id[(uint64_t)qi * ftk + rc];
+            const float fd = a.filt_t[(uint64_t)qi * ftk + rc];
+            const uint32_t mid = r < n ? fid : 0u;
+            const float t = pk_pq_dist(a, plut, mid, fd, qn);
+            uint64_t key = ~0ull;
+            if (r < n) {
+                key = ((uint64_t)pk_f2ord(t) << 32) | (bud ? a.mem2local[mid] : r);
+                if (exact) dist[r] = t;
+            }
+            if (!exact) keys[r] = key;
+        }
+    } else {
         constexpr int RP = RERANK_RP, SB = 8;  // rows per group, 16-element steps per block
         const uint32_t g = threadIdx.x >> 3, j = threadIdx.x & 7, ng = blockDim.x >> 3;
         const int src = (int)(threadIdx.x & 63u & ~7u);
@@ -1732,7 +1810,7 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
                 d = pk_ord2f((uint32_t)(keys[r] >> 32));
             } else {
                 uint32_t mid = a.tie_ids[(uint64_t)qi * PK_TIE_MAX + (r - n)];
-                d = pk_rerank_dist(a.raw + (uint64_t)mid * a.dim, qv, a.dim);
+                d = pq ? pk_pq_dist(a, plut, mid, tfin, qn) : pk_rerank_dist(a.raw + (uint64_t)mid * a.dim, qv, a.dim);
             }
             atomicMin(&tie_min, pk_f2ord(d));
         }
@@ -1808,7 +1886,20 @@ extern "C" hipError_t pk_launch_rerank(const PkRerankArgs* args, uint32_t nq, hi
     while (n2 < args->ftk) n2 <<= 1;
     size_t smem = sizeof(uint64_t) * n2 + sizeof(float) * ((args->dim + 3) & ~3u) +
                   sizeof(float) * ((args->ftk + 3) & ~3u) + 16 + 8 * ((args->topk + 2) & ~1u);
-    if (smem > 64 * 1024) return hipErrorInvalidValue;
+    if (args->pq_codes) {
+        if (!args->qn || !args->pq_off || !args->pq_codebook || args->pnsq == 0) return hipErrorInvalidValue;
+        smem += sizeof(float) * (size_t)args->pnsq * PK_KS;
+    }
+    static PkLdsCache lds_cache;
+    size_t* slot = pk_lds_slot(lds_cache);
+    if (!slot) return hipErrorInvalidDevice;
+    if (!*slot) {
+        size_t al = 0;
+        const hipError_t e = pk_allow_lds((const void*)pk_rerank, 160 * 1024, &al);
+        if (e != hipSuccess) return e;
+        *slot = al;
+    }
+    if (smem > *slot) return hipErrorInvalidValue;
     hipLaunchKernelGGL(pk_rerank, dim3(nq), dim3(RERANK_THREADS), smem, s, *args);
     return hipGetLastError();
 }

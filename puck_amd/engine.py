@@ -169,6 +169,35 @@ class Context:
                                                 nsq, n, _ptr(x), _ptr(cell), _ptr(rec), m))
         return rec
 
+    def encode_pq(self, coarse, fine, codebook, nsq, filter_records, pq_codebook, pq_nsq, x, cell):
+        """Second-stage PQ records [n, 4 + pq_nsq] (whether_pq = true) of x,
+        given its cells and filter records (include/puck_gpu.h)."""
+        C, dim = coarse.shape
+        F = fine.shape[0]
+        n = x.shape[0]
+        _check(coarse, "coarse", _F32, device=self.device)
+        _check(fine, "fine", _F32, shape=(F, dim), device=self.device)
+        if nsq == 0 or dim % nsq or pq_nsq == 0 or dim % pq_nsq:
+            raise ValueError(f"filter_nsq {nsq} and pq_nsq {pq_nsq} must divide dim {dim}")
+        _check(codebook, "codebook", _F32, device=self.device)
+        _check(pq_codebook, "pq_codebook", _F32, device=self.device)
+        if int(np.prod(pq_codebook.shape)) != pq_nsq * 256 * (dim // pq_nsq):
+            raise ValueError(f"pq_codebook: expected {pq_nsq * 256 * (dim // pq_nsq)} floats")
+        _check(filter_records, "filter_records", _U8, shape=(n, 4 + nsq), device=self.device)
+        _check(x, "x", _F32, shape=(n, dim), device=self.device)
+        _check(cell, "cell", _U32, shape=(n,), device=self.device)
+        if _is_dev(x):
+            import torch
+            rec = torch.empty((n, 4 + pq_nsq), dtype=torch.uint8, device=x.device)
+        else:
+            rec = np.zeros((n, 4 + pq_nsq), np.uint8)
+        m = _mode(coarse, fine, codebook, filter_records, pq_codebook, x, cell, rec)
+        self.sync_torch(x, cell, filter_records)
+        self.check(self.L.puckgpu_encode_pq(self.h, _ptr(coarse), _ptr(fine), C, F, dim, _ptr(codebook), nsq,
+                                            _ptr(filter_records), _ptr(pq_codebook), pq_nsq, n, _ptr(x), _ptr(cell),
+                                            _ptr(rec), m))
+        return rec
+
     def kmeans_assign(self, centroids, x, with_dist=False):
         k, dim = centroids.shape
         n = x.shape[0]
@@ -190,7 +219,11 @@ class Context:
 class GpuIndex:
     """A Puck index resident in HBM (memory order, derived arrays on device)."""
 
-    def __init__(self, ctx, coarse, fine, codebook, cell_assign, records, features, whether_norm):
+    def __init__(self, ctx, coarse, fine, codebook, cell_assign, records, features, whether_norm, pq_codebook=None,
+                 pq_records=None):
+        """features may be None for an index ranked by a second-stage PQ
+        (pq_codebook [pq_nsq, 256, dim / pq_nsq] and pq_records [N, 4 + pq_nsq],
+        whether_pq = true)."""
         self.ctx = ctx
         C, dim = coarse.shape
         F = fine.shape[0]
@@ -207,13 +240,25 @@ class GpuIndex:
         _check(cell_assign, "cell_assign", _U32, shape=(N,), device=dv)
         _check(records, "records", _U8, shape=(N, 4 + nsq), device=dv)
         _check(features, "features", _F32, shape=(N, dim), device=dv)
+        if features is None and pq_records is None:
+            raise ValueError("an index needs its raw features or a second-stage PQ to rank by")
+        self.pq_nsq = 0
+        if pq_records is not None:
+            pq_nsq = int(pq_records.shape[1]) - 4
+            _check(pq_records, "pq_records", _U8, shape=(N, 4 + pq_nsq), device=dv)
+            _check(pq_codebook, "pq_codebook", _F32, device=dv)
+            if pq_nsq <= 0 or dim % pq_nsq or int(np.prod(pq_codebook.shape)) != pq_nsq * 256 * (dim // pq_nsq):
+                raise ValueError(f"pq_codebook: expected {pq_nsq} x 256 x {dim // max(pq_nsq, 1)} floats")
         p = _lib.IndexParams(dim, C, F, nsq, N, int(bool(whether_norm)))
         h = ctypes.c_void_p()
-        m = _mode(coarse, fine, codebook, cell_assign, records, features)
-        ctx.sync_torch(coarse, features)
+        m = _mode(coarse, fine, codebook, cell_assign, records, features, pq_codebook, pq_records)
+        ctx.sync_torch(coarse, features, pq_records)
         ctx.check(ctx.L.puckgpu_index_load(ctx.h, ctypes.byref(p), _ptr(coarse), _ptr(fine), _ptr(codebook),
                                            _ptr(cell_assign), _ptr(records), _ptr(features), m, ctypes.byref(h)))
         self.h = h
+        if pq_records is not None:
+            ctx.check(ctx.L.puckgpu_index_attach_pq(ctx.h, self.h, pq_nsq, _ptr(pq_codebook), _ptr(pq_records), m))
+            self.pq_nsq = pq_nsq
 
     def free(self):
         if getattr(self, "h", None):

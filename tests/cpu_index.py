@@ -70,6 +70,27 @@ def build_index(x, C, F, nsq, whether_norm=False, seed=0, train_n=None):
                 raw=x, whether_norm=whether_norm, nsq=nsq)
 
 
+def build_index_pq(x, C, F, nsq, pq_nsq, whether_norm=False, seed=0, train_n=None):
+    """A whether_pq = true index: build_index's filter PQ, then a second
+    codebook of pq_nsq subspaces trained on the residuals left after the
+    filter codewords (puck_index.cpp:836-871), and both quantizers' records
+    (po_encode_puck).  Adds pq_codebook and pq_records."""
+    ix = build_index(x, C, F, nsq, whether_norm, seed, train_n)
+    x = ix["raw"]
+    tn = x.shape[0] if train_n is None else min(train_n, x.shape[0])
+    cell, coarse, fine, cb = ix["cell_assign"], ix["coarse"], ix["fine"], ix["codebook"]
+    lsq = x.shape[1] // nsq
+    res = x - coarse[cell // F] - fine[cell % F]
+    codes = ix["records"][:, 4:].astype(np.int64)
+    dec = np.concatenate([cb[m][codes[:, m]] for m in range(nsq)], axis=1).reshape(x.shape[0], nsq * lsq)
+    res1 = (res - dec).astype(np.float32)
+    pcb = train_codebook(res1[:tn], pq_nsq, seed=seed + 7)
+    frec, prec = O.encode_puck(coarse, fine, cb, nsq, pcb, pq_nsq, x, cell)
+    assert np.array_equal(frec, ix["records"])
+    ix.update(pq_codebook=pcb, pq_records=prec, pq_nsq=pq_nsq)
+    return ix
+
+
 def build_index_with_centers(x, coarse, fine, nsq, whether_norm=False, seed=0):
     """Index over given coarse / fine centers (codebook trained on the residuals)."""
     x = np.asarray(x, np.float32)
@@ -85,8 +106,11 @@ def build_index_with_centers(x, coarse, fine, nsq, whether_norm=False, seed=0):
 
 
 def oracle_index(ix):
-    return O.OracleIndex(ix["coarse"], ix["fine"], ix["codebook"], ix["cell_assign"],
-                         ix["records"], ix["raw"], ix["whether_norm"])
+    oi = O.OracleIndex(ix["coarse"], ix["fine"], ix["codebook"], ix["cell_assign"],
+                       ix["records"], ix["raw"], ix["whether_norm"])
+    if "pq_records" in ix:
+        oi.set_pq(ix["pq_codebook"], ix["pq_records"])
+    return oi
 
 
 def near_tie_encode_input(seed, nsq, lsq, n):

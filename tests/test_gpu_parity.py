@@ -223,6 +223,50 @@ def test_budget_scan_wide_bitexact(ctx, idx_wide):
 
 
 @pytest.fixture(scope="module")
+def idx_dense():
+    # few large cells (C = 4, F = 8, 40k points): a budget of most cells over
+    # every list gives more candidates than the split scan's key buffer (8192),
+    # a budget of 1 cell far fewer, so one batch mixes the split scan and the
+    # per-query kernel it leaves queries to
+    x = D.unit_norm_mixture(40000, 32, 64, seed=70, means_seed=71)
+    q = D.unit_norm_mixture(48, 32, 64, seed=72, means_seed=71)
+    ix = CI.build_index(x, 4, 8, 16, whether_norm=True, train_n=20000)
+    return ix, q
+
+
+@pytest.mark.parametrize("ftk", [64, 1000, 2048])
+def test_budget_scan_split_and_overflow_bitexact(ctx, idx_dense, ftk):
+    ix, q = idx_dense
+    gi = _gpu_index(ctx, ix)
+    oi = CI.oracle_index(ix)
+    n, F = q.shape[0], ix["fine"].shape[0]
+    nf = np.array([1, 2, 7, 6, 3, 5, 4, 7] * (n // 8), np.uint32)
+    scc = np.array([4, 1, 4, 2, 3, 4, 1, 4] * (n // 8), np.uint32)
+    topk = np.full(n, 10, np.uint32)
+    Dg, Ig, Rg = gi.search_batch(q, 10, 4, ftk, topk=topk, scc=scc, fine_budget=nf)
+    ft, fm, fn = gi.last_filter(n, ftk)
+    ms, cnt = gi.last_stage_ms()
+    Do, Io, Ro = oi.search_batch_pq(q, 10, 4, ftk, topk=topk, scc_arr=scc, fine_budget=nf)
+    assert np.array_equal(Rg, Ro)
+    big = 0
+    for k in range(n):
+        o = oi.search_one(q[k], 10, int(scc[k]), ftk, max_topk=10, fine_budget=int(nf[k]))
+        big += o["stats"]["points_offered"] > 8192
+        assert fn[k] == o["filt_n"], k
+        gv, gid = _sorted_pairs(ft[k, :fn[k]], fm[k, :fn[k]])
+        assert np.array_equal(_bits(gv), _bits(o["filt_val"][:fn[k]])), k
+        assert np.array_equal(gid, o["filt_mem"][:fn[k]]), k
+        m = int(Ro[k])
+        assert np.array_equal(Ig[k, :m], Io[k, :m]), k
+        assert np.array_equal(_bits(Dg[k, :m]), _bits(Do[k, :m])), k
+    # both paths ran: some queries above the key buffer, most below
+    assert 0 < big < n
+    # the scan's counters (cells scored, points scored) match the oracle's
+    assert int(cnt[3]) == sum(oi.search_one(q[k], 10, int(scc[k]), ftk, max_topk=10,
+                                            fine_budget=int(nf[k]))["stats"]["points_offered"] for k in range(n))
+
+
+@pytest.fixture(scope="module")
 def idx_ties():
     # Integer data and centers, all distances exact integers.  Fine centers come
     # in pairs that differ by swapping coordinates 0 and 1 (coarse centers

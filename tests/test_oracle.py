@@ -503,3 +503,19 @@ def test_reference_build_keeps_subnormals():
         pytest.skip("oracle/_ref not built (reference checkout absent)")
     tiny = np.float32(1e-39)
     assert tiny * np.float32(1.0) == tiny and tiny != 0
+
+
+def test_fast_build_is_bit_identical(small_l2):
+    # the CPU baseline's -O3 -march=x86-64-v3 build gives the checker's bits
+    ix, q = small_l2
+    ref = CI.oracle_index(ix)
+    fast = O.OracleIndex(ix["coarse"], ix["fine"], ix["codebook"], ix["cell_assign"], ix["records"], ix["raw"],
+                         ix["whether_norm"], L=O.lib_fast())
+    for scc, ftk in [(3, 100), (12, 500)]:
+        a = ref.search_batch(q, 10, scc, ftk)
+        b = fast.search_batch(q, 10, scc, ftk)
+        assert np.array_equal(a[1], b[1]) and np.array_equal(bits(a[0]), bits(b[0])) and np.array_equal(a[2], b[2])
+        nf = (np.arange(q.shape[0]) % 5 + 1).astype(np.uint32)
+        a = ref.search_batch_pq(q, 10, scc, ftk, fine_budget=nf)
+        b = fast.search_batch_pq(q, 10, scc, ftk, fine_budget=nf)
+        assert np.array_equal(a[1], b[1]) and np.array_equal(bits(a[0]), bits(b[0]))
