@@ -572,16 +572,16 @@ struct PassBufs {
     uint32_t* wcount = nullptr;
     // split budget scan: work items, candidate keys, per-query counts, the
     // list of queries left to the per-query kernel and its length
-    uint4* sc_items = nullptr;
+    uint2* sc_items = nullptr;
     uint64_t* sc_keys = nullptr;
     uint32_t* sc_meta = nullptr;
     uint32_t* sc_olist = nullptr;
     uint32_t* sc_ocount = nullptr;
 };
 
-// Candidate points (keys) and work items per query of the split budget scan;
+// Candidate points (work items and keys) per query of the split budget scan;
 // a query with more is scanned by the per-query kernel.
-constexpr uint32_t PK_SC_CAPP = 8192, PK_SC_CAPI = 4096;
+constexpr uint32_t PK_SC_CAPP = 8192;
 
 // Per-pass configuration.
 struct PassCfg {
@@ -681,7 +681,7 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     wa.q_scc = q_scc; wa.q_nf = nf; wa.cstride = cfg.keep ? cfg.keep : 1; wa.fdist = b.fdist;
     wa.wlist = nullptr; wa.wcount = nullptr;
     wa.sc_items = b.sc_items; wa.sc_keys = b.sc_keys; wa.sc_meta = b.sc_meta; wa.sc_olist = b.sc_olist;
-    wa.sc_ocount = b.sc_ocount; wa.sc_capp = PK_SC_CAPP; wa.sc_capi = PK_SC_CAPI;
+    wa.sc_ocount = b.sc_ocount; wa.sc_capp = PK_SC_CAPP;
     if (nf) {
         // the walk runs only over the queries without a budget below F (the list built above)
         wa.wlist = b.wlist;
@@ -805,7 +805,7 @@ int scratch_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     b.qorder = b.wlist ? b.wlist + cap : nullptr;
     b.wcount = b.wlist ? b.wlist + 2 * (size_t)cap : nullptr;
     if (cfg.budget) {
-        b.sc_items = (uint4*)scratch(ctx, S_SCI, sizeof(uint4) * (size_t)cap * PK_SC_CAPI, &rc);
+        b.sc_items = (uint2*)scratch(ctx, S_SCI, sizeof(uint2) * (size_t)cap * PK_SC_CAPP, &rc);
         b.sc_keys = (uint64_t*)scratch(ctx, S_SCK, sizeof(uint64_t) * (size_t)cap * PK_SC_CAPP, &rc);
         b.sc_meta = (uint32_t*)scratch(ctx, S_SCM, sizeof(uint32_t) * (3 * (size_t)cap + 1), &rc);
         b.sc_olist = b.sc_meta ? b.sc_meta + 2 * (size_t)cap : nullptr;
@@ -1346,7 +1346,7 @@ int bucket_create(puckgpu_server* sv, uint32_t cap, const PassCfg& cfg, PkBucket
         b.pwork = w;
     }
     if (cfg.budget) {
-        if ((rc = balloc(ctx, bk, &b.sc_items, (size_t)cap * PK_SC_CAPI)) ||
+        if ((rc = balloc(ctx, bk, &b.sc_items, (size_t)cap * PK_SC_CAPP)) ||
             (rc = balloc(ctx, bk, &b.sc_keys, (size_t)cap * PK_SC_CAPP)) ||
             (rc = balloc(ctx, bk, &b.sc_meta, 3 * (size_t)cap + 1))) {
             bucket_release(bk);

@@ -40,12 +40,12 @@ struct PkWalkArgs {
                              // reference's heap order: its fine-distance ties need no flag, and
                              // its flags are rewritten (not or-ed) with what this walk finds
     // split budget scan (scan.hip); sc_items null: every budget query takes the per-query kernel
-    uint4* sc_items;         // nq x sc_capi work items {first memory id, points, key slot, cell distance bits}
-    uint64_t* sc_keys;       // nq x sc_capp (filter distance ord << 32 | memory id)
-    uint32_t* sc_meta;       // nq x 2: items (0xffffffff: left to the per-query kernel), points
+    uint2* sc_items;         // nq x sc_capp: per scored point {memory id, cell distance bits}
+    uint64_t* sc_keys;       // nq x sc_capp (filter distance ord << 32 | memory id), same order
+    uint32_t* sc_meta;       // nq x 2: points (0xffffffff: left to the per-query kernel), points
     uint32_t* sc_olist;      // nq: the queries left to the per-query kernel
     uint32_t* sc_ocount;     // 1
-    uint32_t sc_capp, sc_capi;
+    uint32_t sc_capp;
 };
 
 struct PkRerankArgs {

@@ -4,16 +4,17 @@
 //   pk_scan_cells   one workgroup per query: in each walked coarse list, the cut
 //                   at the b-th smallest cell distance (cd + stationary) + fine
 //                   distance (the reference's cell distance expression,
-//                   puck_index.cpp:535), and the selected non-empty cells cut
-//                   into work items of at most 4 consecutive points
-//                   {first memory id, count, key slot, cell distance}
+//                   puck_index.cpp:535), and one work item per point of the
+//                   selected cells {memory id, cell distance}
 //   pk_scan_score   one workgroup per query: the query's filter lookup table
 //                   in LDS (quantization.cpp:260-270), then every point of every
 //                   item gets the reference's filter distance
 //                   (compute_quantized_distance, puck_index.cpp:346-390: t =
 //                   fl(2 * cell_dist + offset) + lookup sum in the 4-lane order of
 //                   puck_index.cpp:75-113), written as a (distance, memory id)
-//                   key; no block barrier after the table is built
+//                   key in the item's slot (consecutive threads take
+//                   consecutive points: the record loads coalesce); no block
+//                   barrier after the table is built
 //   pk_scan_cut     one workgroup per query: the filter_topk smallest keys
 //                   (radix select on the distance bits, then on the ids of the
 //                   keys tied at the cut) into the filter rows
@@ -28,20 +29,23 @@
 #include "launch_args.h"
 
 #define PK_SC_OVERFLOW 0xffffffffu
-#define PK_SC_ITEM 4          // points per work item
 
 __device__ __forceinline__ bool pk_sc_budget_query(const PkWalkArgs& a, uint32_t qi) {
     return a.q_nf && a.q_nf[qi] != 0u && a.q_nf[qi] < a.F;
 }
 
 // ---------------------------------------------------------------------------
+#ifndef PK_SC_CELL_THREADS
 #define PK_SC_CELL_THREADS 256
-#define PK_SC_CPT 8  // columns (fine clusters) per thread: F <= 2048
+#endif
+#define PK_SC_CPT (2048 / PK_SC_CELL_THREADS)  // columns (fine clusters) per thread: F <= 2048
 __global__ __launch_bounds__(PK_SC_CELL_THREADS) void pk_scan_cells(PkWalkArgs a) {
     __shared__ uint32_t hist[260];
     __shared__ uint32_t tmp[PK_SC_CELL_THREADS / 64 + 1];
     __shared__ uint32_t red[2 * PK_SC_CELL_THREADS / 64];
-    __shared__ uint32_t cnt_sh;
+    // the list's selected non-empty cells: first memory id, first item, cell distance bits
+    __shared__ uint32_t sel_c0[PK_SC_CPT * PK_SC_CELL_THREADS + 1], sel_pre[PK_SC_CPT * PK_SC_CELL_THREADS + 1];
+    __shared__ uint32_t sel_cd[PK_SC_CPT * PK_SC_CELL_THREADS];
     const uint32_t tid = threadIdx.x, NT = PK_SC_CELL_THREADS;
     const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
     if (!pk_sc_budget_query(a, qi)) return;
@@ -64,8 +68,8 @@ __global__ __launch_bounds__(PK_SC_CELL_THREADS) void pk_scan_cells(PkWalkArgs a
     uint32_t scc = a.scc;
     if (a.q_scc) scc = a.q_scc[qi] < scc ? a.q_scc[qi] : scc;
     const float* fdq = a.fdist + (uint64_t)qi * F;
-    uint4* items = a.sc_items + (uint64_t)qi * a.sc_capi;
-    uint32_t nit = 0, npt = 0, ncell = 0;
+    uint2* items = a.sc_items + (uint64_t)qi * a.sc_capp;
+    uint32_t npt = 0, ncell = 0;
     bool ovf = false;
     for (uint32_t l = 0; l < scc; ++l) {
         const uint32_t c = a.csel_id[(uint64_t)qi * a.cstride + l];
@@ -94,46 +98,49 @@ __global__ __launch_bounds__(PK_SC_CELL_THREADS) void pk_scan_cells(PkWalkArgs a
                     if (j * NT + tid < F) sink(kv[j]);
             },
             b, hist, tmp, red);
-        uint32_t li = 0, lp = 0, lc = 0;
+        uint32_t lp = 0, lc = 0;
 #pragma unroll
         for (int j = 0; j < PK_SC_CPT; ++j) {
             const bool sel = j * NT + tid < F && kv[j] <= kcut;
             c1[j] = sel ? c1[j] - c0[j] : 0u;  // the cell's point count when selected
-            li += (c1[j] + PK_SC_ITEM - 1) / PK_SC_ITEM;
             lp += c1[j];
             lc += c1[j] > 0 ? 1u : 0u;
         }
-        uint32_t ti, tp;
-        uint32_t ib = pk_block_exscan(li, tmp, &ti);
+        uint32_t tp, tc;
         uint32_t pb = pk_block_exscan(lp, tmp, &tp);
-        if (tid == 0) cnt_sh = 0;
-        __syncthreads();
-        atomicAdd(&cnt_sh, lc);
-        __syncthreads();
-        const uint32_t tc = cnt_sh;
-        __syncthreads();
-        if (nit + ti > a.sc_capi || npt + tp > a.sc_capp) {  // block-uniform
+        uint32_t cb = pk_block_exscan(lc, tmp, &tc);
+        if (npt + tp > a.sc_capp) {  // block-uniform
             ovf = true;
             break;
         }
-        ib += nit;
-        pb += npt;
+        // the selected cells into LDS in column order, then every thread writes
+        // consecutive items (the item's cell by binary search over the prefixes)
 #pragma unroll
         for (int j = 0; j < PK_SC_CPT; ++j) {
-            const uint32_t cnt = c1[j];
-            const uint32_t cdb = __float_as_uint(pk_ord2f(kv[j]));
-            for (uint32_t p = 0; p < cnt; p += PK_SC_ITEM) {
-                const uint32_t m = cnt - p < PK_SC_ITEM ? cnt - p : PK_SC_ITEM;
-                items[ib++] = make_uint4(c0[j] + p, m, pb, cdb);
-                pb += m;
+            if (c1[j] > 0) {
+                sel_c0[cb] = c0[j];
+                sel_pre[cb] = pb;
+                sel_cd[cb] = __float_as_uint(pk_ord2f(kv[j]));
+                ++cb;
+                pb += c1[j];
             }
         }
-        nit += ti;
+        if (tid == 0) sel_pre[tc] = tp;
+        __syncthreads();
+        for (uint32_t p = tid; p < tp; p += NT) {
+            uint32_t lo = 0, hi = tc;  // sel_pre[lo] <= p < sel_pre[hi]
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (sel_pre[mid] <= p) lo = mid; else hi = mid;
+            }
+            items[npt + p] = make_uint2(sel_c0[lo] + (p - sel_pre[lo]), sel_cd[lo]);
+        }
+        __syncthreads();  // the LDS list is rewritten by the next list
         npt += tp;
         ncell += tc;
     }
     if (tid == 0) {
-        meta[0] = ovf ? PK_SC_OVERFLOW : nit;
+        meta[0] = ovf ? PK_SC_OVERFLOW : npt;
         meta[1] = npt;
         if (a.stats && !ovf) {
             a.stats[qi * 4 + 0] = 0;
@@ -145,31 +152,39 @@ __global__ __launch_bounds__(PK_SC_CELL_THREADS) void pk_scan_cells(PkWalkArgs a
 }
 
 // ---------------------------------------------------------------------------
-// Filter distance of up to PK_SC_ITEM points of one item; NV: nsq / 16 code
-// vectors per record (0: byte loads for any nsq).
-template <int NV>
-__device__ __forceinline__ void pk_sc_score_item(const PkWalkArgs& a, const float* __restrict__ lut, uint4 it,
-                                                 bool valid, uint64_t* __restrict__ keys) {
+// Filter distances of PP points per thread: item i = j * NT + tid of the
+// round (consecutive threads, consecutive records); NV: nsq / 16 code vectors
+// per record (0: byte loads for any nsq).
+#define PK_SC_PP 2
+template <int NT, int NV>
+__device__ __forceinline__ void pk_sc_score_round(const PkWalkArgs& a, const float* __restrict__ lut,
+                                                  const uint2* __restrict__ items, uint32_t i0, uint32_t n,
+                                                  uint64_t* __restrict__ keys) {
     const uint32_t nsq = a.nsq;
-    const uint32_t m = valid ? it.y : 0u;
-    const float cdl = __uint_as_float(it.w);
-    float off[PK_SC_ITEM];
-    uint4 cw[PK_SC_ITEM][NV > 0 ? NV : 1];
+    uint2 it[PK_SC_PP];
+    float off[PK_SC_PP];
+    uint4 cw[PK_SC_PP][NV > 0 ? NV : 1];
+    // every load at an in-bounds index (a load under a lane condition is
+    // waited for before the next one is issued)
 #pragma unroll
-    for (int h = 0; h < PK_SC_ITEM; ++h) {
-        // every load at an in-bounds index (the item's last point again past its end)
-        const uint32_t P = it.x + (h < (int)it.y ? (uint32_t)h : (it.y ? it.y - 1u : 0u));
-        off[h] = a.rec_off[P];
+    for (int h = 0; h < PK_SC_PP; ++h) {
+        const uint32_t i = i0 + h * NT + threadIdx.x;
+        it[h] = items[i < n ? i : n - 1u];
+    }
+#pragma unroll
+    for (int h = 0; h < PK_SC_PP; ++h) {
+        off[h] = a.rec_off[it[h].x];
         if constexpr (NV > 0) {
-            const uint4* cp = (const uint4*)(a.rec_codes + (uint64_t)P * nsq);
+            const uint4* cp = (const uint4*)(a.rec_codes + (uint64_t)it[h].x * nsq);
 #pragma unroll
             for (int g = 0; g < NV; ++g) cw[h][g] = cp[g];
         }
     }
 #pragma unroll
-    for (int h = 0; h < PK_SC_ITEM; ++h) {
-        const uint32_t P = it.x + h;
-        float tv = 2.0f * cdl + off[h];  // 2x is exact: one rounding, as in double
+    for (int h = 0; h < PK_SC_PP; ++h) {
+        const uint32_t i = i0 + h * NT + threadIdx.x;
+        const uint32_t P = it[h].x;
+        float tv = 2.0f * __uint_as_float(it[h].y) + off[h];  // 2x is exact: one rounding, as in double
         float acc[4] = {0.f, 0.f, 0.f, 0.f};
         if constexpr (NV > 0) {
 #pragma unroll
@@ -185,13 +200,11 @@ __device__ __forceinline__ void pk_sc_score_item(const PkWalkArgs& a, const floa
                 }
             }
         } else {
-            if ((uint32_t)h < m) {
-                const uint8_t* codes = a.rec_codes + (uint64_t)P * nsq;
-                for (uint32_t q = 0; q < nsq; ++q) acc[q & 3] = acc[q & 3] + lut[q * PK_KS + codes[q]];
-            }
+            const uint8_t* codes = a.rec_codes + (uint64_t)P * nsq;
+            for (uint32_t q = 0; q < nsq; ++q) acc[q & 3] = acc[q & 3] + lut[q * PK_KS + codes[q]];
         }
         tv = tv + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
-        if ((uint32_t)h < m) keys[it.z + h] = ((uint64_t)pk_f2ord(tv) << 32) | P;
+        if (i < n) keys[i] = ((uint64_t)pk_f2ord(tv) << 32) | P;
     }
 }
 
@@ -200,8 +213,8 @@ __global__ __launch_bounds__(NT, 6) void pk_scan_score(PkWalkArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
     if (!pk_sc_budget_query(a, qi)) return;
-    const uint32_t nit = a.sc_meta[2ull * qi];
-    if (nit == PK_SC_OVERFLOW || nit == 0) return;  // block-uniform
+    const uint32_t n = a.sc_meta[2ull * qi];
+    if (n == PK_SC_OVERFLOW || n == 0) return;  // block-uniform
     float* lut = (float*)smem;
     float* qv = lut + a.nsq * PK_KS;
     const float* qg = a.q + (uint64_t)qi * a.dim;
@@ -209,18 +222,16 @@ __global__ __launch_bounds__(NT, 6) void pk_scan_score(PkWalkArgs a) {
     __syncthreads();
     pk_build_lut(lut, qv, a.codebook, a.nsq, a.lsq);
     __syncthreads();
-    const uint4* items = a.sc_items + (uint64_t)qi * a.sc_capi;
+    const uint2* items = a.sc_items + (uint64_t)qi * a.sc_capp;
     uint64_t* keys = a.sc_keys + (uint64_t)qi * a.sc_capp;
-    for (uint32_t i0 = 0; i0 < nit; i0 += NT) {
-        const uint32_t i = i0 + threadIdx.x;
-        const uint4 it = items[i < nit ? i : nit - 1u];
-        pk_sc_score_item<NV>(a, lut, it, i < nit, keys);
-    }
+    for (uint32_t i0 = 0; i0 < n; i0 += PK_SC_PP * NT) pk_sc_score_round<NT, NV>(a, lut, items, i0, n, keys);
 }
 
 // ---------------------------------------------------------------------------
+#ifndef PK_SC_CUT_THREADS
 #define PK_SC_CUT_THREADS 256
-#define PK_SC_CUT_KPT 32  // keys per thread: sc_capp <= 8192
+#endif
+#define PK_SC_CUT_KPT (8192 / PK_SC_CUT_THREADS)  // keys per thread: sc_capp <= 8192
 
 __global__ __launch_bounds__(PK_SC_CUT_THREADS) void pk_scan_cut(PkWalkArgs a) {
     __shared__ uint32_t hist[260];
@@ -230,7 +241,7 @@ __global__ __launch_bounds__(PK_SC_CUT_THREADS) void pk_scan_cut(PkWalkArgs a) {
     const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
     const uint32_t tid = threadIdx.x;
     if (!pk_sc_budget_query(a, qi)) return;
-    const uint32_t nit = a.sc_meta[2ull * qi];
+    const uint32_t nit = a.sc_meta[2ull * qi];  // points, or the overflow mark
     if (nit == PK_SC_OVERFLOW) return;  // the per-query kernel's query
     const uint32_t ftk = a.ftk;
     if (tid == 0) {
@@ -340,16 +351,33 @@ extern "C" int pk_scan_split_ok(uint32_t dim, uint32_t nsq, uint32_t F) {
     return pk_scan_score_smem(dim, nsq) <= 64 * 1024;
 }
 
-// Enqueues the three kernels and the overflow list over nq queries.
-extern "C" hipError_t pk_launch_scan_split(const PkWalkArgs* args, uint32_t nq, hipStream_t s) {
-    if (nq == 0) return hipSuccess;
-    const PkWalkArgs& a = *args;
+// Enqueues the split scan over nq queries in two parts: the cell selection
+// and the list of queries it leaves to the per-query kernel (pre), then the
+// scoring and the filter cut (post).  The per-query kernel needs only the
+// list, so a caller can run it on another stream between the two parts.
+static hipError_t pk_scan_split_check(const PkWalkArgs& a) {
     if (!a.q_nf || !a.fdist || !a.sc_items || !a.sc_keys || !a.sc_meta || !a.sc_olist || !a.sc_ocount)
         return hipErrorInvalidValue;
     if (a.sc_capp > PK_SC_CUT_KPT * PK_SC_CUT_THREADS || a.sc_capp < a.ftk) return hipErrorInvalidValue;
+    if (pk_scan_score_smem(a.dim, a.nsq) > 64 * 1024) return hipErrorInvalidValue;
+    return hipSuccess;
+}
+
+extern "C" hipError_t pk_launch_scan_split_pre(const PkWalkArgs* args, uint32_t nq, hipStream_t s) {
+    if (nq == 0) return hipSuccess;
+    const hipError_t e = pk_scan_split_check(*args);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(pk_scan_cells, dim3(nq), dim3(PK_SC_CELL_THREADS), 0, s, *args);
+    hipLaunchKernelGGL(pk_scan_overflow_list, dim3(1), dim3(1024), 0, s, *args, nq);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t pk_launch_scan_split_post(const PkWalkArgs* args, uint32_t nq, hipStream_t s) {
+    if (nq == 0) return hipSuccess;
+    const PkWalkArgs& a = *args;
+    const hipError_t e = pk_scan_split_check(a);
+    if (e != hipSuccess) return e;
     const size_t smem = pk_scan_score_smem(a.dim, a.nsq);
-    if (smem > 64 * 1024) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(pk_scan_cells, dim3(nq), dim3(PK_SC_CELL_THREADS), 0, s, a);
     const uint32_t nv = (a.nsq & 15u) == 0 && a.nsq <= 64 ? a.nsq / 16u : 0u;
     constexpr int NT = 512;
     switch (nv) {
@@ -360,6 +388,5 @@ extern "C" hipError_t pk_launch_scan_split(const PkWalkArgs* args, uint32_t nq, 
     default: hipLaunchKernelGGL((pk_scan_score<NT, 0>), dim3(nq), dim3(NT), smem, s, a); break;
     }
     hipLaunchKernelGGL(pk_scan_cut, dim3(nq), dim3(PK_SC_CUT_THREADS), 0, s, a);
-    hipLaunchKernelGGL(pk_scan_overflow_list, dim3(1), dim3(1024), 0, s, a, nq);
     return hipGetLastError();
 }

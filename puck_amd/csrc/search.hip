@@ -1523,7 +1523,8 @@ extern "C" uint32_t pk_walk_ftk_max(void) { return WALK_FTK_MAX; }
 // fork / join, capturable in a graph); ev_mid is recorded on s right after
 // the scan.
 extern "C" int pk_scan_split_ok(uint32_t dim, uint32_t nsq, uint32_t F);
-extern "C" hipError_t pk_launch_scan_split(const PkWalkArgs* args, uint32_t nq, hipStream_t s);
+extern "C" hipError_t pk_launch_scan_split_pre(const PkWalkArgs* args, uint32_t nq, hipStream_t s);
+extern "C" hipError_t pk_launch_scan_split_post(const PkWalkArgs* args, uint32_t nq, hipStream_t s);
 
 extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStream_t s, hipEvent_t ev_mid,
                                      hipStream_t s2, hipEvent_t ev_fork, hipEvent_t ev_join) {
@@ -1590,17 +1591,23 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
         if ((e = hipStreamWaitEvent(s2, ev_fork, 0)) != hipSuccess) return e;
         launch_walk(sw);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = hipEventRecord(ev_join, s2)) != hipSuccess) return e;
     }
     if (args->q_nf) {
         if (args->sc_items && pk_scan_split_ok(args->dim, args->nsq, args->F)) {
-            // the split scan, then the per-query kernel over the queries it leaves
-            if ((e = pk_launch_scan_split(&sa, nq, s)) != hipSuccess) return e;
+            // the split scan; the per-query kernel takes the queries its cell
+            // selection leaves, on the second stream beside the scoring and cut
+            if ((e = pk_launch_scan_split_pre(&sa, nq, s)) != hipSuccess) return e;
+            if (fork) {
+                if ((e = hipEventRecord(ev_fork, s)) != hipSuccess) return e;
+                if ((e = hipStreamWaitEvent(s2, ev_fork, 0)) != hipSuccess) return e;
+            }
             const uint32_t lg = nq < 1024 ? nq : 1024;
             if (args->ftk <= 1024)
-                hipLaunchKernelGGL(pk_scan_budget_list<1024>, dim3(lg), dim3(SCAN_THREADS), smem_scan, s, sa);
+                hipLaunchKernelGGL(pk_scan_budget_list<1024>, dim3(lg), dim3(SCAN_THREADS), smem_scan, sw, sa);
             else
-                hipLaunchKernelGGL(pk_scan_budget_list<2048>, dim3(lg), dim3(SCAN_THREADS), smem_scan, s, sa);
+                hipLaunchKernelGGL(pk_scan_budget_list<2048>, dim3(lg), dim3(SCAN_THREADS), smem_scan, sw, sa);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+            if ((e = pk_launch_scan_split_post(&sa, nq, s)) != hipSuccess) return e;
         } else if (args->ftk <= 1024) {
             hipLaunchKernelGGL(pk_scan_budget_kernel<1024>, dim3(nq), dim3(SCAN_THREADS), smem_scan, s, sa);
         } else {
@@ -1609,7 +1616,10 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (ev_mid) (void)hipEventRecord(ev_mid, s);
-    if (fork) return hipStreamWaitEvent(s, ev_join, 0);
+    if (fork) {
+        if ((e = hipEventRecord(ev_join, s2)) != hipSuccess) return e;
+        return hipStreamWaitEvent(s, ev_join, 0);
+    }
     launch_walk(s);
     return hipGetLastError();
 }
