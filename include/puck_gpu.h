@@ -128,6 +128,24 @@ int puckgpu_index_export(puckgpu_ctx* ctx, const puckgpu_index* index, float* co
 int puckgpu_search(puckgpu_ctx* ctx, const puckgpu_index* index, const puckgpu_search_params* params,
                    uint32_t n, const float* queries, uint32_t topk, float* distance, uint32_t* labels,
                    uint32_t* result_num, int device_ptrs);
+/* puckgpu_search with device pointers that allocates nothing: no hipMalloc,
+ * hipFree or other allocating call.  No counterpart in the reference; for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid it).  Every pass buffer lives in one array of
+ * 64 MiB that each device holds for the life of the process (so do the host
+ * staging arrays of the flags and replay lists), so calls that overlap on one
+ * device race; the array keeps the last call's contents, and nothing in it
+ * needs to start at zero.  queries [n x dim], distance / labels [n x topk] and
+ * result_num [n] (may be NULL) are device pointers on the context's device;
+ * results are those of puckgpu_search.  Limits: n <= 4096, precision 0, and
+ * the call's buffers (per query 4 (dim + 2 C + 3 F + 2 search_coarse_count +
+ * 2 filter_topk) bytes and about 100 more) within 64 MiB: a call above them
+ * returns PUCKGPU_EINVAL and touches nothing.  The filter rows of such a call
+ * are not kept for puckgpu_last_filter. */
+int puckgpu_search_noalloc(puckgpu_ctx* ctx, const puckgpu_index* index, const puckgpu_search_params* params,
+                           uint32_t n, const float* queries, uint32_t topk, float* distance, uint32_t* labels,
+                           uint32_t* result_num);
+
 /* Per-query parameters of puckgpu_search_batch.  The reference's Request
  * carries only topk (puck/index.h:60-69); search_coarse_count is a process
  * flag there (index_conf.cpp:64) and the fine budget has no counterpart, so

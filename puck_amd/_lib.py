@@ -19,7 +19,7 @@ EXPORTS = [
     "puckgpu_probe_get_params", "puckgpu_probe_features", "puckgpu_probe_train_step", "puckgpu_probe_predict",
     "puckgpu_search_learned", "puckgpu_budget_ranks", "puckgpu_server_create", "puckgpu_server_free",
     "puckgpu_server_buckets", "puckgpu_server_search", "puckgpu_server_tune", "puckgpu_last_scan_ms",
-    "puckgpu_center_distances", "puckgpu_index_attach_pq", "puckgpu_encode_pq",
+    "puckgpu_center_distances", "puckgpu_index_attach_pq", "puckgpu_encode_pq", "puckgpu_search_noalloc",
 ]
 
 
@@ -104,6 +104,8 @@ def load():
     L.puckgpu_assign_cells.restype = c_int
     L.puckgpu_encode_filter.argtypes = [vp, vp, vp, u32, u32, u32, vp, u32, u32, vp, vp, vp, c_int]
     L.puckgpu_encode_filter.restype = c_int
+    L.puckgpu_search_noalloc.argtypes = [vp, vp, ctypes.POINTER(SearchParams), u32, vp, u32, vp, vp, vp]
+    L.puckgpu_search_noalloc.restype = c_int
     L.puckgpu_index_attach_pq.argtypes = [vp, vp, u32, vp, vp, c_int]
     L.puckgpu_index_attach_pq.restype = c_int
     L.puckgpu_encode_pq.argtypes = [vp, vp, vp, u32, u32, u32, vp, u32, vp, vp, u32, u32, vp, vp, vp, c_int]

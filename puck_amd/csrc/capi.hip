@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 // ---- kernel launchers (defined in gemm/sort/search/build.hip)
@@ -108,6 +109,7 @@ struct puckgpu_ctx {
     float scan_ms = 0;  // budget-scan launch alone (inside stage 2)
     uint64_t counters[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t last_n = 0, last_ftk = 0;
+    unsigned char* noalloc = nullptr;  // this device's copy of `scratch` (puckgpu_search_noalloc)
 };
 
 struct puckgpu_index {
@@ -152,6 +154,19 @@ struct puckgpu_probe {
 
 struct PkBucket;
 
+// Pass buffers of puckgpu_search_noalloc, carved from a file-scope device
+// array: each device holds its own copy for the life of the process.  Host
+// code reaches it only through the address hipGetSymbolAddress gives on that
+// device (puckgpu_create keeps it in the context).  Calls on one device share
+// it; calls that overlap on one device race.
+#define PK_NOALLOC_BYTES (64ull << 20)
+#define PK_NOALLOC_MAXN 4096u
+static __device__ __align__(256) unsigned char scratch[PK_NOALLOC_BYTES];
+// Host staging of the same calls (flags, counters, replay lists), per device.
+static uint32_t pk_noalloc_hstats[PK_MAX_DEVICES][4 * PK_NOALLOC_MAXN];
+static uint8_t pk_noalloc_hflags[PK_MAX_DEVICES][PK_NOALLOC_MAXN];
+static uint32_t pk_noalloc_hlist[PK_MAX_DEVICES][2 * PK_NOALLOC_MAXN];
+
 struct puckgpu_server {
     puckgpu_ctx* ctx = nullptr;
     const puckgpu_index* ix = nullptr;
@@ -183,7 +198,7 @@ int set_err(puckgpu_ctx* ctx, int code, const char* fmt, ...) {
     } while (0)
 
 // scratch slot i, at least `bytes` long
-void* scratch(puckgpu_ctx* ctx, size_t slot, size_t bytes, int* rc) {
+void* scratch_slot(puckgpu_ctx* ctx, size_t slot, size_t bytes, int* rc) {
     if (ctx->bufs.size() <= slot) ctx->bufs.resize(slot + 1);
     DevBuf& b = ctx->bufs[slot];
     if (b.cap < bytes) {
@@ -210,7 +225,7 @@ enum Slot {
 // Device view of an input array: the pointer itself, or a scratch copy of it.
 const void* dev_in(puckgpu_ctx* ctx, size_t slot, const void* p, size_t bytes, int device_ptrs, int* rc) {
     if (device_ptrs || bytes == 0) return p;
-    void* d = scratch(ctx, slot, bytes, rc);
+    void* d = scratch_slot(ctx, slot, bytes, rc);
     if (!d) return nullptr;
     if (hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
         *rc = set_err(ctx, PUCKGPU_EHIP, "host-to-device copy of %zu bytes failed", bytes);
@@ -258,6 +273,9 @@ int puckgpu_create(int device, puckgpu_ctx** out) {
          hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) == hipSuccess &&
          hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming) == hipSuccess &&
          hipEventCreateWithFlags(&ctx->order_ev, hipEventDisableTiming) == hipSuccess;
+    // this device's copy of the no-alloc scratch array (its name is a host
+    // stand-in; only this address may be handed to kernels or copies)
+    ok = ok && hipGetSymbolAddress((void**)&ctx->noalloc, HIP_SYMBOL(scratch)) == hipSuccess && ctx->noalloc;
     if (!ok) {
         puckgpu_destroy(ctx);  // releases whatever was created
         return PUCKGPU_EHIP;
@@ -351,7 +369,7 @@ int puckgpu_index_load(puckgpu_ctx* ctx, const puckgpu_index_params* params, con
     PK_TRYH(hipMemcpyAsync(ix->codebook, filter_codebook, sizeof(float) * nsq * PK_KS * lsq, kind, s));
 
     // |c|^2/2, |f|^2/2 and stationary[c][f] = |f|^2/2 + <f,c> (read_coodbooks)
-    float* fnorm = (float*)scratch(ctx, S_A0, sizeof(float) * F, &rc);
+    float* fnorm = (float*)scratch_slot(ctx, S_A0, sizeof(float) * F, &rc);
     if (!fnorm) { index_release(ix); return rc; }
     PK_TRYH(pk_launch_row_norms(ix->coarse, C, d, 1, ix->coarse_norms, s));
     PK_TRYH(pk_launch_row_norms(ix->fine, F, d, 1, fnorm, s));
@@ -370,10 +388,10 @@ int puckgpu_index_load(puckgpu_ctx* ctx, const puckgpu_index_params* params, con
         if (device_ptrs) {
             PK_TRYH(hipMemcpyAsync(h_cell.data(), cell_assign, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, s));
             // offsets: first 4 bytes of each record
-            float* offs = (float*)scratch(ctx, S_A3, sizeof(float) * N, &rc);
+            float* offs = (float*)scratch_slot(ctx, S_A3, sizeof(float) * N, &rc);
             if (!offs) { index_release(ix); return rc; }
             std::vector<uint32_t> ident;  // gather with identity map via a device iota
-            uint32_t* iota = (uint32_t*)scratch(ctx, S_A4, sizeof(uint32_t) * N, &rc);
+            uint32_t* iota = (uint32_t*)scratch_slot(ctx, S_A4, sizeof(uint32_t) * N, &rc);
             if (!iota) { index_release(ix); return rc; }
             ident.resize(N);
             for (uint32_t i = 0; i < N; ++i) ident[i] = i;
@@ -737,23 +755,22 @@ void for_each_flagged(const uint8_t* flags, uint32_t m, Fn f) {
 //            heaps (the reference's fine order first when flagged 8; orders
 //            computed in stage 1 are already in place).
 // Synchronises the stream.  Returns the number of queries replayed, or < 0.
+// hlist, stage1: host arrays of m entries each (the replay lists).
 int replay_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, const PassBufs& b, uint32_t m,
-                uint8_t* hflags, int replay_all, PkWalkArgs wa, PkRerankArgs ra, std::vector<uint32_t>& hlist,
+                uint8_t* hflags, int replay_all, PkWalkArgs wa, PkRerankArgs ra, uint32_t* hlist, uint32_t* stage1,
                 hipStream_t s) {
     const uint32_t C = ix->p.coarse_cluster_count, F = ix->p.fine_cluster_count;
-    std::vector<uint32_t> stage1;
-    hlist.clear();
+    uint32_t n1 = 0, nl = 0;
     if (replay_all) {
-        for (uint32_t i = 0; i < m; ++i) hlist.push_back(i);
+        for (uint32_t i = 0; i < m; ++i) hlist[nl++] = i;
     } else {
         for_each_flagged(hflags, m, [&](uint32_t i) {
-            if ((hflags[i] & 15u) != 0u && (hflags[i] & 6u) == 0u) stage1.push_back(i);
-            else if (hflags[i] & 15u) hlist.push_back(i);
+            if ((hflags[i] & 15u) != 0u && (hflags[i] & 6u) == 0u) stage1[n1++] = i;
+            else if (hflags[i] & 15u) hlist[nl++] = i;
         });
     }
-    const uint32_t n1 = (uint32_t)stage1.size();
     if (n1) {
-        PK_CHECK(hipMemcpyAsync(b.qlist, stage1.data(), sizeof(uint32_t) * n1, hipMemcpyHostToDevice, s));
+        PK_CHECK(hipMemcpyAsync(b.qlist, stage1, sizeof(uint32_t) * n1, hipMemcpyHostToDevice, s));
         PK_CHECK(pk_launch_exact_center_order(b.cdist, C, b.fdist, F, wa.cstride, cfg.scc, wa.q_scc, b.qlist, n1,
                                               b.flags, 0, b.csd, b.csi, b.fsd, b.fsi, s));
         PkWalkArgs w1 = wa;
@@ -765,12 +782,11 @@ int replay_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, c
         PK_CHECK(pk_launch_rerank(&r1, n1, s));
         PK_CHECK(hipMemcpyAsync(hflags, b.flags, m, hipMemcpyDeviceToHost, s));
         PK_CHECK(hipStreamSynchronize(s));
-        for (uint32_t i : stage1)
-            if (hflags[i] & 14u) hlist.push_back(i);
+        for (uint32_t j = 0; j < n1; ++j)
+            if (hflags[stage1[j]] & 14u) hlist[nl++] = stage1[j];
     }
-    const uint32_t nl = (uint32_t)hlist.size();
     if (nl) {
-        PK_CHECK(hipMemcpyAsync(b.qlist, hlist.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s));
+        PK_CHECK(hipMemcpyAsync(b.qlist, hlist, sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s));
         PK_CHECK(pk_launch_exact_center_order(b.cdist, C, b.fdist, F, wa.cstride, cfg.scc, wa.q_scc, b.qlist, nl,
                                               b.flags, replay_all, b.csd, b.csi, b.fsd, b.fsi, s));
         wa.qlist = b.qlist; wa.flags = nullptr; wa.exact = 1; wa.tie_ids = nullptr; wa.tie_n = nullptr;
@@ -787,44 +803,44 @@ int scratch_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     const puckgpu_probe* pr = cfg.pu ? cfg.pu->probe : nullptr;
     const uint32_t keep = cfg.keep ? cfg.keep : 1;
     int rc = 0;
-    b.q = (float*)scratch(ctx, S_Q, sizeof(float) * cap * d, &rc);
-    b.qn = (float*)scratch(ctx, S_QN, sizeof(float) * cap, &rc);
-    b.qst = (int32_t*)scratch(ctx, S_QST, sizeof(int32_t) * cap, &rc);
-    b.cdist = (float*)scratch(ctx, S_CD, sizeof(float) * cap * C, &rc);
-    b.fdist = (float*)scratch(ctx, S_FD, sizeof(float) * cap * F, &rc);
-    b.csd = (float*)scratch(ctx, S_CSD, sizeof(float) * cap * keep, &rc);
-    b.csi = (uint32_t*)scratch(ctx, S_CSI, sizeof(uint32_t) * cap * keep, &rc);
-    b.fsd = (float*)scratch(ctx, S_FSD, sizeof(float) * cap * F, &rc);
-    b.fsi = (uint32_t*)scratch(ctx, S_FSI, sizeof(uint32_t) * cap * F, &rc);
-    b.flags = (uint8_t*)scratch(ctx, S_FLAGS, cap, &rc);
-    b.qlist = (uint32_t*)scratch(ctx, S_QLIST, sizeof(uint32_t) * cap, &rc);
-    b.tie_ids = (uint32_t*)scratch(ctx, S_TIEID, sizeof(uint32_t) * 16 * cap, &rc);
-    b.tie_n = (uint8_t*)scratch(ctx, S_TIEN, cap, &rc);
-    b.stats = (uint32_t*)scratch(ctx, S_STATS, sizeof(uint32_t) * 4 * cap, &rc);
-    b.wlist = (uint32_t*)scratch(ctx, S_WLIST, sizeof(uint32_t) * (2 * (size_t)cap + 1), &rc);
+    b.q = (float*)scratch_slot(ctx, S_Q, sizeof(float) * cap * d, &rc);
+    b.qn = (float*)scratch_slot(ctx, S_QN, sizeof(float) * cap, &rc);
+    b.qst = (int32_t*)scratch_slot(ctx, S_QST, sizeof(int32_t) * cap, &rc);
+    b.cdist = (float*)scratch_slot(ctx, S_CD, sizeof(float) * cap * C, &rc);
+    b.fdist = (float*)scratch_slot(ctx, S_FD, sizeof(float) * cap * F, &rc);
+    b.csd = (float*)scratch_slot(ctx, S_CSD, sizeof(float) * cap * keep, &rc);
+    b.csi = (uint32_t*)scratch_slot(ctx, S_CSI, sizeof(uint32_t) * cap * keep, &rc);
+    b.fsd = (float*)scratch_slot(ctx, S_FSD, sizeof(float) * cap * F, &rc);
+    b.fsi = (uint32_t*)scratch_slot(ctx, S_FSI, sizeof(uint32_t) * cap * F, &rc);
+    b.flags = (uint8_t*)scratch_slot(ctx, S_FLAGS, cap, &rc);
+    b.qlist = (uint32_t*)scratch_slot(ctx, S_QLIST, sizeof(uint32_t) * cap, &rc);
+    b.tie_ids = (uint32_t*)scratch_slot(ctx, S_TIEID, sizeof(uint32_t) * 16 * cap, &rc);
+    b.tie_n = (uint8_t*)scratch_slot(ctx, S_TIEN, cap, &rc);
+    b.stats = (uint32_t*)scratch_slot(ctx, S_STATS, sizeof(uint32_t) * 4 * cap, &rc);
+    b.wlist = (uint32_t*)scratch_slot(ctx, S_WLIST, sizeof(uint32_t) * (2 * (size_t)cap + 1), &rc);
     b.qorder = b.wlist ? b.wlist + cap : nullptr;
     b.wcount = b.wlist ? b.wlist + 2 * (size_t)cap : nullptr;
     if (cfg.budget) {
-        b.sc_items = (uint2*)scratch(ctx, S_SCI, sizeof(uint2) * (size_t)cap * PK_SC_CAPP, &rc);
-        b.sc_keys = (uint64_t*)scratch(ctx, S_SCK, sizeof(uint64_t) * (size_t)cap * PK_SC_CAPP, &rc);
-        b.sc_meta = (uint32_t*)scratch(ctx, S_SCM, sizeof(uint32_t) * (3 * (size_t)cap + 1), &rc);
+        b.sc_items = (uint2*)scratch_slot(ctx, S_SCI, sizeof(uint2) * (size_t)cap * PK_SC_CAPP, &rc);
+        b.sc_keys = (uint64_t*)scratch_slot(ctx, S_SCK, sizeof(uint64_t) * (size_t)cap * PK_SC_CAPP, &rc);
+        b.sc_meta = (uint32_t*)scratch_slot(ctx, S_SCM, sizeof(uint32_t) * (3 * (size_t)cap + 1), &rc);
         b.sc_olist = b.sc_meta ? b.sc_meta + 2 * (size_t)cap : nullptr;
         b.sc_ocount = b.sc_meta ? b.sc_meta + 3 * (size_t)cap : nullptr;
     }
     if (pr) {
-        b.pfeat = (float*)scratch(ctx, S_PFEAT, sizeof(float) * cap * pr->D, &rc);
-        b.pwork = scratch(ctx, S_PWORK, pk_probe_work_bytes(cap, pr->D, pr->H), &rc);
-        b.pnf = (uint32_t*)scratch(ctx, S_PNF, sizeof(uint32_t) * cap, &rc);
-        b.py = (float*)scratch(ctx, S_PY, sizeof(float) * cap, &rc);
-        b.pcv = (float*)scratch(ctx, S_PCV, sizeof(float) * cap * (pr->nfine ? pr->nfine : 1), &rc);
+        b.pfeat = (float*)scratch_slot(ctx, S_PFEAT, sizeof(float) * cap * pr->D, &rc);
+        b.pwork = scratch_slot(ctx, S_PWORK, pk_probe_work_bytes(cap, pr->D, pr->H), &rc);
+        b.pnf = (uint32_t*)scratch_slot(ctx, S_PNF, sizeof(uint32_t) * cap, &rc);
+        b.py = (float*)scratch_slot(ctx, S_PY, sizeof(float) * cap, &rc);
+        b.pcv = (float*)scratch_slot(ctx, S_PCV, sizeof(float) * cap * (pr->nfine ? pr->nfine : 1), &rc);
     }
     if (cfg.precision) {
         const uint32_t ld = pkc_ld(d), kc = cfg.kc ? cfg.kc : 1;
-        b.q16 = (uint16_t*)scratch(ctx, S_LP16, sizeof(uint16_t) * cap * ld, &rc);
-        b.q8 = (uint8_t*)scratch(ctx, S_LP8, (size_t)cap * ld, &rc);
-        b.qs = (float*)scratch(ctx, S_LPS, sizeof(float) * cap, &rc);
-        b.cand_d = (float*)scratch(ctx, S_CAND_D, sizeof(float) * cap * kc, &rc);
-        b.cand_i = (uint32_t*)scratch(ctx, S_CAND_I, sizeof(uint32_t) * cap * kc, &rc);
+        b.q16 = (uint16_t*)scratch_slot(ctx, S_LP16, sizeof(uint16_t) * cap * ld, &rc);
+        b.q8 = (uint8_t*)scratch_slot(ctx, S_LP8, (size_t)cap * ld, &rc);
+        b.qs = (float*)scratch_slot(ctx, S_LPS, sizeof(float) * cap, &rc);
+        b.cand_d = (float*)scratch_slot(ctx, S_CAND_D, sizeof(float) * cap * kc, &rc);
+        b.cand_i = (uint32_t*)scratch_slot(ctx, S_CAND_I, sizeof(uint32_t) * cap * kc, &rc);
     }
     return rc;
 }
@@ -891,6 +907,57 @@ int make_cfg(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_par
     return PUCKGPU_OK;
 }
 
+// One device pass over m queries and its exact-tie replay, on the pass
+// buffers b: enqueue_pass, the flags and counters copied to the host
+// (hstats: 4 m words, hflags: m bytes), replay_pass (hl, h1: m words each),
+// stage times and counters accumulated on the context.
+int run_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, const puckgpu_search_params* sp,
+             const PassBufs& b, uint32_t m, const float* qin, const uint32_t* q_topk, const uint32_t* q_scc,
+             const uint32_t* q_nf, float* ft, uint32_t* fi, uint32_t* fn, float* od, uint32_t* oi, uint32_t* on,
+             uint32_t* budget_out, int budget_out_host, uint32_t* hstats, uint8_t* hflags, uint32_t* hl, uint32_t* h1,
+             hipStream_t s) {
+    PkWalkArgs wa;
+    PkRerankArgs ra;
+    int rc = enqueue_pass(ctx, ix, cfg, b, m, qin, nullptr, q_topk, q_scc, q_nf, ft, fi, fn, od, oi, on, budget_out,
+                          budget_out_host, ctx->ev, &wa, &ra, s);
+    if (rc) return rc;
+    PK_CHECK(hipMemcpyAsync(hstats, b.stats, sizeof(uint32_t) * 4 * m, hipMemcpyDeviceToHost, s));
+    PK_CHECK(hipMemcpyAsync(hflags, b.flags, m, hipMemcpyDeviceToHost, s));
+    PK_CHECK(hipStreamSynchronize(s));
+    if (cfg.precision) std::fill(hflags, hflags + m, (uint8_t)0);  // no replay in these forms
+    for_each_flagged(hflags, m, [&](uint32_t i) {  // first-pass replay causes
+        for (int bb = 0; bb < 8; ++bb)
+            if (hflags[i] & (1u << bb)) ctx->counters[5 + bb]++;
+    });
+    const int nrep = replay_pass(ctx, ix, cfg, b, m, hflags, cfg.precision ? 0 : sp->replay_all, wa, ra, hl, h1, s);
+    if (nrep < 0) return nrep;
+    // stages 0-3 are complete (the flags copy was waited for); the replay
+    // stage is timed only when something was replayed
+    for (int k = 0; k < 4; ++k) {
+        float ms = 0;
+        PK_CHECK(hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]));
+        ctx->stage_ms[k] += ms;
+    }
+    if (nrep > 0) {
+        PK_CHECK(hipEventRecord(ctx->ev[5], s));
+        PK_CHECK(hipEventSynchronize(ctx->ev[5]));
+        float ms = 0;
+        PK_CHECK(hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
+        ctx->stage_ms[4] += ms;
+    }
+    {
+        float ms = 0;
+        PK_CHECK(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[6]));
+        ctx->scan_ms += ms;
+    }
+    uint64_t acc[4] = {0, 0, 0, 0};  // local sums: a tight loop over the host copy
+    for (uint32_t i = 0; i < m; ++i)
+        for (int k = 0; k < 4; ++k) acc[k] += hstats[4 * (size_t)i + k];
+    for (int k = 0; k < 4; ++k) ctx->counters[k] += acc[k];
+    ctx->counters[4] += (uint64_t)nrep;
+    return PUCKGPU_OK;
+}
+
 // Batched search.  `topk` is the largest per-query top-k (the rerank
 // capacity) and `ostride` the output row stride; q_topk / q_scc / q_nf are
 // optional per-query arrays (host or device as device_ptrs says).
@@ -911,20 +978,20 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
     const uint32_t nb = n < QB ? n : QB;
     PassBufs b;
     if ((rc = scratch_pass(ctx, ix, cfg, nb, b))) return rc;
-    float* ft = (float*)scratch(ctx, S_FT, sizeof(float) * (size_t)n * ftk, &rc);
-    uint32_t* fi = (uint32_t*)scratch(ctx, S_FI, sizeof(uint32_t) * (size_t)n * ftk, &rc);
-    uint32_t* fn = (uint32_t*)scratch(ctx, S_FN, sizeof(uint32_t) * n, &rc);
+    float* ft = (float*)scratch_slot(ctx, S_FT, sizeof(float) * (size_t)n * ftk, &rc);
+    uint32_t* fi = (uint32_t*)scratch_slot(ctx, S_FI, sizeof(uint32_t) * (size_t)n * ftk, &rc);
+    uint32_t* fn = (uint32_t*)scratch_slot(ctx, S_FN, sizeof(uint32_t) * n, &rc);
     if (rc) return rc;
     float* od;
     uint32_t *oi, *on;
     if (device_ptrs) {
         od = distance;
         oi = labels;
-        on = result_num ? result_num : (uint32_t*)scratch(ctx, S_ON, sizeof(uint32_t) * n, &rc);
+        on = result_num ? result_num : (uint32_t*)scratch_slot(ctx, S_ON, sizeof(uint32_t) * n, &rc);
     } else {
-        od = (float*)scratch(ctx, S_OD, sizeof(float) * (size_t)n * ostride, &rc);
-        oi = (uint32_t*)scratch(ctx, S_OI, sizeof(uint32_t) * (size_t)n * ostride, &rc);
-        on = (uint32_t*)scratch(ctx, S_ON, sizeof(uint32_t) * n, &rc);
+        od = (float*)scratch_slot(ctx, S_OD, sizeof(float) * (size_t)n * ostride, &rc);
+        oi = (uint32_t*)scratch_slot(ctx, S_OI, sizeof(uint32_t) * (size_t)n * ostride, &rc);
+        on = (uint32_t*)scratch_slot(ctx, S_ON, sizeof(uint32_t) * n, &rc);
     }
     if (rc) return rc;
     const float* qin = (const float*)dev_in(ctx, S_IN_Q, queries, sizeof(float) * (size_t)n * d, device_ptrs, &rc);
@@ -940,52 +1007,15 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
     if (!hpin) return set_err(ctx, PUCKGPU_ENOMEM, "pinned host allocation failed");
     uint32_t* hstats = (uint32_t*)hpin;
     uint8_t* hflags = hpin + 16 * (size_t)nb;
-    std::vector<uint32_t> hlist;
+    std::vector<uint32_t> hl(2 * (size_t)nb);
     for (uint32_t q0 = 0; q0 < n; q0 += QB) {
         const uint32_t m = (n - q0) < QB ? (n - q0) : QB;
-        PkWalkArgs wa;
-        PkRerankArgs ra;
-        rc = enqueue_pass(ctx, ix, cfg, b, m, qin + (size_t)q0 * d, nullptr, q_topk ? q_topk + q0 : nullptr,
-                          q_scc ? q_scc + q0 : nullptr, q_nf ? q_nf + q0 : nullptr, ft + (size_t)q0 * ftk,
-                          fi + (size_t)q0 * ftk, fn + q0, od + (size_t)q0 * ostride, oi + (size_t)q0 * ostride,
-                          on + q0, pu && pu->budget_out ? pu->budget_out + q0 : nullptr, !device_ptrs, ctx->ev, &wa,
-                          &ra, s);
+        rc = run_pass(ctx, ix, cfg, sp, b, m, qin + (size_t)q0 * d, q_topk ? q_topk + q0 : nullptr,
+                      q_scc ? q_scc + q0 : nullptr, q_nf ? q_nf + q0 : nullptr, ft + (size_t)q0 * ftk,
+                      fi + (size_t)q0 * ftk, fn + q0, od + (size_t)q0 * ostride, oi + (size_t)q0 * ostride, on + q0,
+                      pu && pu->budget_out ? pu->budget_out + q0 : nullptr, !device_ptrs, hstats, hflags, hl.data(),
+                      hl.data() + nb, s);
         if (rc) return rc;
-        PK_CHECK(hipMemcpyAsync(hstats, b.stats, sizeof(uint32_t) * 4 * m, hipMemcpyDeviceToHost, s));
-        PK_CHECK(hipMemcpyAsync(hflags, b.flags, m, hipMemcpyDeviceToHost, s));
-        PK_CHECK(hipStreamSynchronize(s));
-        if (cfg.precision) std::fill(hflags, hflags + m, (uint8_t)0);  // no replay in these forms
-        for_each_flagged(hflags, m, [&](uint32_t i) {  // first-pass replay causes
-            for (int bb = 0; bb < 8; ++bb)
-                if (hflags[i] & (1u << bb)) ctx->counters[5 + bb]++;
-        });
-        const int nrep = replay_pass(ctx, ix, cfg, b, m, hflags, cfg.precision ? 0 : sp->replay_all, wa, ra,
-                                     hlist, s);
-        if (nrep < 0) return nrep;
-        // stages 0-3 are complete (the flags copy was waited for); the replay
-        // stage is timed only when something was replayed
-        for (int k = 0; k < 4; ++k) {
-            float ms = 0;
-            PK_CHECK(hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]));
-            ctx->stage_ms[k] += ms;
-        }
-        if (nrep > 0) {
-            PK_CHECK(hipEventRecord(ctx->ev[5], s));
-            PK_CHECK(hipEventSynchronize(ctx->ev[5]));
-            float ms = 0;
-            PK_CHECK(hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
-            ctx->stage_ms[4] += ms;
-        }
-        {
-            float ms = 0;
-            PK_CHECK(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[6]));
-            ctx->scan_ms += ms;
-        }
-        uint64_t acc[4] = {0, 0, 0, 0};  // local sums: a tight loop over the pinned copy
-        for (uint32_t i = 0; i < m; ++i)
-            for (int k = 0; k < 4; ++k) acc[k] += hstats[4 * (size_t)i + k];
-        for (int k = 0; k < 4; ++k) ctx->counters[k] += acc[k];
-        ctx->counters[4] += (uint64_t)nrep;
     }
     if (!q_nf && !(pu && pu->probe)) ctx->scan_ms = -1.f;  // no budget scan was launched
     if (!device_ptrs) {
@@ -1009,6 +1039,89 @@ int puckgpu_search(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_sear
         return set_err(ctx, PUCKGPU_EINVAL, "topk should <= topk, topk = %u, requested %u", sp->topk_max, topk);
     return search_impl(ctx, ix, sp, nullptr, nullptr, nullptr, n, queries, topk, topk, distance, labels,
                        result_num, device_ptrs);
+}
+
+namespace {
+
+// Pass buffers of m queries (and the filter rows and counts, and result
+// counts when out_n is wanted) carved from base in 256-byte aligned pieces.
+// Returns the bytes they take; nothing is carved when base is null.
+size_t carve_pass(unsigned char* base, const puckgpu_index* ix, const PassCfg& cfg, uint32_t m, PassBufs& b,
+                  float** ft, uint32_t** fi, uint32_t** fn, uint32_t** on) {
+    const size_t d = ix->p.dim, C = ix->p.coarse_cluster_count, F = ix->p.fine_cluster_count;
+    const size_t keep = cfg.keep ? cfg.keep : 1, ftk = cfg.ftk;
+    size_t off = 0;
+    auto take = [&](auto** p, size_t bytes) {
+        if (base) *p = (std::remove_reference_t<decltype(**p)>*)(base + off);
+        off += (bytes + 255) & ~(size_t)255;
+    };
+    take(&b.q, 4 * m * d);
+    take(&b.qn, 4 * (size_t)m);
+    take(&b.qst, 4 * (size_t)m);
+    take(&b.cdist, 4 * m * C);
+    take(&b.fdist, 4 * m * F);
+    take(&b.csd, 4 * m * keep);
+    take(&b.csi, 4 * m * keep);
+    take(&b.fsd, 4 * m * F);
+    take(&b.fsi, 4 * m * F);
+    take(&b.flags, (size_t)m);
+    take(&b.qlist, 4 * (size_t)m);
+    take(&b.tie_ids, 64 * (size_t)m);
+    take(&b.tie_n, (size_t)m);
+    take(&b.stats, 16 * (size_t)m);
+    take(&b.wlist, 4 * (2 * (size_t)m + 1));
+    take(ft, 4 * m * ftk);
+    take(fi, 4 * m * ftk);
+    take(fn, 4 * (size_t)m);
+    take(on, 4 * (size_t)m);
+    if (base) {
+        b.qorder = b.wlist + m;
+        b.wcount = b.wlist + 2 * (size_t)m;
+    }
+    return off;
+}
+
+}  // namespace
+
+int puckgpu_search_noalloc(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_params* sp, uint32_t n,
+                           const float* queries, uint32_t topk, float* distance, uint32_t* labels,
+                           uint32_t* result_num) {
+    if (!ctx || !ix || !sp) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    if (topk == 0 || topk > sp->topk_max)
+        return set_err(ctx, PUCKGPU_EINVAL, "topk should <= topk, topk = %u, requested %u", sp->topk_max, topk);
+    if (n == 0) return PUCKGPU_OK;
+    if (!queries || !distance || !labels) return set_err(ctx, PUCKGPU_EINVAL, "null array");
+    if (n > PK_NOALLOC_MAXN)
+        return set_err(ctx, PUCKGPU_EINVAL, "no-alloc search takes at most %u queries per call, got %u",
+                       PK_NOALLOC_MAXN, n);
+    if (sp->precision != 0) return set_err(ctx, PUCKGPU_EINVAL, "no-alloc search runs the exact FP32 form only");
+    if (ctx->device < 0 || ctx->device >= PK_MAX_DEVICES || !ctx->noalloc)
+        return set_err(ctx, PUCKGPU_EINVAL, "no scratch array on this device");
+    PassCfg cfg;
+    int rc = make_cfg(ctx, ix, sp, topk, topk, nullptr, cfg);
+    if (rc) return rc;
+    PassBufs b;
+    float* ft = nullptr;
+    uint32_t *fi = nullptr, *fn = nullptr, *on = nullptr;
+    const size_t need = carve_pass(nullptr, ix, cfg, n, b, &ft, &fi, &fn, &on);
+    if (need > PK_NOALLOC_BYTES)
+        return set_err(ctx, PUCKGPU_EINVAL,
+                       "no-alloc search of %u queries needs %zu bytes of scratch, the maximum is %llu", n, need,
+                       (unsigned long long)PK_NOALLOC_BYTES);
+    carve_pass(ctx->noalloc, ix, cfg, n, b, &ft, &fi, &fn, &on);
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    for (int k = 0; k < 5; ++k) ctx->stage_ms[k] = 0;
+    ctx->scan_ms = 0;
+    for (int k = 0; k < 13; ++k) ctx->counters[k] = 0;
+    const int dv = ctx->device;
+    rc = run_pass(ctx, ix, cfg, sp, b, n, queries, nullptr, nullptr, nullptr, ft, fi, fn, distance, labels,
+                  result_num ? result_num : on, nullptr, 0, pk_noalloc_hstats[dv], pk_noalloc_hflags[dv],
+                  pk_noalloc_hlist[dv], pk_noalloc_hlist[dv] + PK_NOALLOC_MAXN, ctx->stream);
+    if (rc) return rc;
+    PK_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->scan_ms = -1.f;
+    ctx->last_n = 0;  // the filter rows of this call live in the shared array, not the context
+    return PUCKGPU_OK;
 }
 
 int puckgpu_search_batch(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_params* sp,
@@ -1052,17 +1165,17 @@ int puckgpu_budget_ranks(puckgpu_ctx* ctx, const puckgpu_index* ix, uint32_t sea
     const uint32_t QB = 16384;
     const uint32_t nb = n < QB ? n : QB;
     const uint32_t keep = scc ? scc : 1;
-    float* q = (float*)scratch(ctx, S_Q, sizeof(float) * nb * d, &rc);
-    float* qn = (float*)scratch(ctx, S_QN, sizeof(float) * nb, &rc);
-    int32_t* qst = (int32_t*)scratch(ctx, S_QST, sizeof(int32_t) * nb, &rc);
-    float* cdist = (float*)scratch(ctx, S_CD, sizeof(float) * nb * C, &rc);
-    float* fdist = (float*)scratch(ctx, S_FD, sizeof(float) * nb * F, &rc);
-    float* csd = (float*)scratch(ctx, S_CSD, sizeof(float) * nb * keep, &rc);
-    uint32_t* csi = (uint32_t*)scratch(ctx, S_CSI, sizeof(uint32_t) * nb * keep, &rc);
+    float* q = (float*)scratch_slot(ctx, S_Q, sizeof(float) * nb * d, &rc);
+    float* qn = (float*)scratch_slot(ctx, S_QN, sizeof(float) * nb, &rc);
+    int32_t* qst = (int32_t*)scratch_slot(ctx, S_QST, sizeof(int32_t) * nb, &rc);
+    float* cdist = (float*)scratch_slot(ctx, S_CD, sizeof(float) * nb * C, &rc);
+    float* fdist = (float*)scratch_slot(ctx, S_FD, sizeof(float) * nb * F, &rc);
+    float* csd = (float*)scratch_slot(ctx, S_CSD, sizeof(float) * nb * keep, &rc);
+    uint32_t* csi = (uint32_t*)scratch_slot(ctx, S_CSI, sizeof(uint32_t) * nb * keep, &rc);
     if (rc) return rc;
     const float* qin = (const float*)dev_in(ctx, S_IN_Q, queries, sizeof(float) * (size_t)n * d, device_ptrs, &rc);
     const uint32_t* cin = (const uint32_t*)dev_in(ctx, S_A0, cells, sizeof(uint32_t) * (size_t)n * k, device_ptrs, &rc);
-    uint32_t* rk = device_ptrs ? ranks : (uint32_t*)scratch(ctx, S_A1, sizeof(uint32_t) * (size_t)n * k, &rc);
+    uint32_t* rk = device_ptrs ? ranks : (uint32_t*)scratch_slot(ctx, S_A1, sizeof(uint32_t) * (size_t)n * k, &rc);
     if (rc) return rc;
     for (uint32_t q0 = 0; q0 < n; q0 += QB) {
         const uint32_t m = (n - q0) < QB ? (n - q0) : QB;
@@ -1170,16 +1283,16 @@ int puckgpu_probe_features(puckgpu_ctx* ctx, const puckgpu_index* ix, const puck
     const uint32_t QB = 16384;
     const uint32_t nb = n < QB ? n : QB;
     const uint32_t keep = p->nc ? p->nc : 1;  // the nearest coarse list is always needed
-    float* q = (float*)scratch(ctx, S_Q, sizeof(float) * nb * d, &rc);
-    float* qn = (float*)scratch(ctx, S_QN, sizeof(float) * nb, &rc);
-    int32_t* qst = (int32_t*)scratch(ctx, S_QST, sizeof(int32_t) * nb, &rc);
-    float* cdist = (float*)scratch(ctx, S_CD, sizeof(float) * nb * C, &rc);
-    float* fdist = (float*)scratch(ctx, S_FD, sizeof(float) * nb * F, &rc);
-    float* csd = (float*)scratch(ctx, S_CSD, sizeof(float) * nb * keep, &rc);
-    uint32_t* csi = (uint32_t*)scratch(ctx, S_CSI, sizeof(uint32_t) * nb * keep, &rc);
-    float* fsd = (float*)scratch(ctx, S_FSD, sizeof(float) * nb * F, &rc);
-    uint32_t* fsi = (uint32_t*)scratch(ctx, S_FSI, sizeof(uint32_t) * nb * F, &rc);
-    float* feat = device_ptrs ? features : (float*)scratch(ctx, S_PFEAT, sizeof(float) * nb * p->D, &rc);
+    float* q = (float*)scratch_slot(ctx, S_Q, sizeof(float) * nb * d, &rc);
+    float* qn = (float*)scratch_slot(ctx, S_QN, sizeof(float) * nb, &rc);
+    int32_t* qst = (int32_t*)scratch_slot(ctx, S_QST, sizeof(int32_t) * nb, &rc);
+    float* cdist = (float*)scratch_slot(ctx, S_CD, sizeof(float) * nb * C, &rc);
+    float* fdist = (float*)scratch_slot(ctx, S_FD, sizeof(float) * nb * F, &rc);
+    float* csd = (float*)scratch_slot(ctx, S_CSD, sizeof(float) * nb * keep, &rc);
+    uint32_t* csi = (uint32_t*)scratch_slot(ctx, S_CSI, sizeof(uint32_t) * nb * keep, &rc);
+    float* fsd = (float*)scratch_slot(ctx, S_FSD, sizeof(float) * nb * F, &rc);
+    uint32_t* fsi = (uint32_t*)scratch_slot(ctx, S_FSI, sizeof(uint32_t) * nb * F, &rc);
+    float* feat = device_ptrs ? features : (float*)scratch_slot(ctx, S_PFEAT, sizeof(float) * nb * p->D, &rc);
     if (rc) return rc;
     const float* qin = (const float*)dev_in(ctx, S_IN_Q, queries, sizeof(float) * (size_t)n * d, device_ptrs, &rc);
     if (!qin) return rc;
@@ -1191,7 +1304,7 @@ int puckgpu_probe_features(puckgpu_ctx* ctx, const puckgpu_index* ix, const puck
         PK_CHECK(coarse_select(cdist, m, C, keep, csd, csi, nullptr, 0, s));
         if (fine_order) PK_CHECK(pk_launch_sort_rows(fdist, m, F, F, F, fsd, fsi, nullptr, 0, s));
         float* fo = device_ptrs ? feat + (size_t)q0 * p->D : feat;
-        float* cv = (float*)scratch(ctx, S_PCV, sizeof(float) * (size_t)m * (p->nfine ? p->nfine : 1), &rc);
+        float* cv = (float*)scratch_slot(ctx, S_PCV, sizeof(float) * (size_t)m * (p->nfine ? p->nfine : 1), &rc);
         if (rc) return rc;
         if ((rc = probe_features_chunk(ctx, ix, p, m, q, csd, csi, keep, fdist, cv, fo, s))) return rc;
         if (!device_ptrs)
@@ -1215,7 +1328,7 @@ int puckgpu_probe_train_step(puckgpu_ctx* ctx, puckgpu_probe* p, uint32_t n, con
     int rc = 0;
     const float* f = (const float*)dev_in(ctx, S_A0, features, sizeof(float) * (size_t)n * p->D, device_ptrs, &rc);
     const float* t = (const float*)dev_in(ctx, S_PT, target, sizeof(float) * n, device_ptrs, &rc);
-    void* work = scratch(ctx, S_PWORK, pk_probe_work_bytes(n, p->D, p->H), &rc);
+    void* work = scratch_slot(ctx, S_PWORK, pk_probe_work_bytes(n, p->D, p->H), &rc);
     if (rc) return rc;
     PK_CHECK(pk_probe_pass(p->params, p->wbf, p->D, p->H, n, f, t, work, p->grads, p->loss, nullptr, s));
     if (grads)
@@ -1243,8 +1356,8 @@ int puckgpu_probe_predict(puckgpu_ctx* ctx, const puckgpu_probe* p, uint32_t n, 
     const uint32_t QB = 65536;
     const uint32_t nb = n < QB ? n : QB;
     const float* f = (const float*)dev_in(ctx, S_A0, features, sizeof(float) * (size_t)n * p->D, device_ptrs, &rc);
-    void* work = scratch(ctx, S_PWORK, pk_probe_work_bytes(nb, p->D, p->H), &rc);
-    float* yb = device_ptrs ? y : (float*)scratch(ctx, S_PY, sizeof(float) * n, &rc);
+    void* work = scratch_slot(ctx, S_PWORK, pk_probe_work_bytes(nb, p->D, p->H), &rc);
+    float* yb = device_ptrs ? y : (float*)scratch_slot(ctx, S_PY, sizeof(float) * n, &rc);
     if (rc) return rc;
     for (uint32_t q0 = 0; q0 < n; q0 += QB) {
         const uint32_t m = (n - q0) < QB ? (n - q0) : QB;
@@ -1493,9 +1606,10 @@ int puckgpu_server_search(puckgpu_server* sv, uint32_t n, const float* queries, 
     if (!hflags) return set_err(ctx, PUCKGPU_ENOMEM, "pinned host allocation failed");
     PK_CHECK(hipMemcpyAsync(hflags, bk->b.flags, n, hipMemcpyDeviceToHost, s));
     PK_CHECK(hipStreamSynchronize(s));
-    std::vector<uint32_t> hlist;
+    std::vector<uint32_t> hlist(2 * (size_t)n);
     if (cfg.precision) std::fill(hflags, hflags + n, (uint8_t)0);
-    const int nrep = replay_pass(ctx, sv->ix, cfg, bk->b, n, hflags, 0, bk->wa, bk->ra, hlist, s);
+    const int nrep = replay_pass(ctx, sv->ix, cfg, bk->b, n, hflags, 0, bk->wa, bk->ra, hlist.data(),
+                                 hlist.data() + n, s);
     if (nrep < 0) return nrep;
     PK_CHECK(hipMemcpyAsync(distance, bk->od, sizeof(float) * (size_t)n * os, out, s));
     PK_CHECK(hipMemcpyAsync(labels, bk->oi, sizeof(uint32_t) * (size_t)n * os, out, s));
@@ -1592,14 +1706,14 @@ int puckgpu_center_distances(puckgpu_ctx* ctx, const puckgpu_index* ix, uint32_t
     hipStream_t s = ctx->stream;
     int rc = 0;
     if (precision && (rc = ensure_lowp(ctx, ix, (int)precision))) return rc;
-    float* q = (float*)scratch(ctx, S_Q, sizeof(float) * (size_t)n * d, &rc);
-    float* qn = (float*)scratch(ctx, S_QN, sizeof(float) * n, &rc);
-    int32_t* qst = (int32_t*)scratch(ctx, S_QST, sizeof(int32_t) * n, &rc);
-    uint16_t* q16 = (uint16_t*)scratch(ctx, S_LP16, sizeof(uint16_t) * (size_t)n * ld, &rc);
-    uint8_t* q8 = (uint8_t*)scratch(ctx, S_LP8, (size_t)n * ld, &rc);
-    float* qs = (float*)scratch(ctx, S_LPS, sizeof(float) * n, &rc);
-    float* co = device_ptrs ? coarse_out : (float*)scratch(ctx, S_CD, sizeof(float) * (size_t)n * C, &rc);
-    float* fo = device_ptrs ? fine_out : (float*)scratch(ctx, S_FD, sizeof(float) * (size_t)n * F, &rc);
+    float* q = (float*)scratch_slot(ctx, S_Q, sizeof(float) * (size_t)n * d, &rc);
+    float* qn = (float*)scratch_slot(ctx, S_QN, sizeof(float) * n, &rc);
+    int32_t* qst = (int32_t*)scratch_slot(ctx, S_QST, sizeof(int32_t) * n, &rc);
+    uint16_t* q16 = (uint16_t*)scratch_slot(ctx, S_LP16, sizeof(uint16_t) * (size_t)n * ld, &rc);
+    uint8_t* q8 = (uint8_t*)scratch_slot(ctx, S_LP8, (size_t)n * ld, &rc);
+    float* qs = (float*)scratch_slot(ctx, S_LPS, sizeof(float) * n, &rc);
+    float* co = device_ptrs ? coarse_out : (float*)scratch_slot(ctx, S_CD, sizeof(float) * (size_t)n * C, &rc);
+    float* fo = device_ptrs ? fine_out : (float*)scratch_slot(ctx, S_FD, sizeof(float) * (size_t)n * F, &rc);
     if (rc) return rc;
     const float* qin = (const float*)dev_in(ctx, S_IN_Q, queries, sizeof(float) * (size_t)n * d, device_ptrs, &rc);
     if (!qin) return rc;
@@ -1639,7 +1753,7 @@ int puckgpu_lut(puckgpu_ctx* ctx, uint32_t n, uint32_t dim, const float* queries
     int rc = 0;
     const float* qd = (const float*)dev_in(ctx, S_A0, queries, 4ull * n * dim, device_ptrs, &rc);
     const float* cb = (const float*)dev_in(ctx, S_A1, codebook, 4ull * nsq * PK_KS * (dim / nsq), device_ptrs, &rc);
-    float* o = device_ptrs ? out : (float*)scratch(ctx, S_A2, 4ull * n * nsq * PK_KS, &rc);
+    float* o = device_ptrs ? out : (float*)scratch_slot(ctx, S_A2, 4ull * n * nsq * PK_KS, &rc);
     if (rc) return rc;
     PK_CHECK(pk_launch_lut(qd, n, dim, cb, nsq, dim / nsq, o, ctx->stream));
     if (!device_ptrs) PK_CHECK(hipMemcpyAsync(out, o, 4ull * n * nsq * PK_KS, hipMemcpyDeviceToHost, ctx->stream));
@@ -1660,10 +1774,10 @@ int puckgpu_assign_cells(puckgpu_ctx* ctx, const float* coarse, const float* fin
     int rc = 0;
     const float* cd = (const float*)dev_in(ctx, S_A0, coarse, 4ull * C * dim, device_ptrs, &rc);
     const float* fd = (const float*)dev_in(ctx, S_A1, fine, 4ull * F * dim, device_ptrs, &rc);
-    float* cn = (float*)scratch(ctx, S_A2, 4ull * C, &rc);
-    float* fnorm = (float*)scratch(ctx, S_A3, 4ull * F, &rc);
-    float* fb = (float*)scratch(ctx, S_A4, 4ull * F, &rc);
-    float* st = (float*)scratch(ctx, S_A5, 4ull * C * F, &rc);
+    float* cn = (float*)scratch_slot(ctx, S_A2, 4ull * C, &rc);
+    float* fnorm = (float*)scratch_slot(ctx, S_A3, 4ull * F, &rc);
+    float* fb = (float*)scratch_slot(ctx, S_A4, 4ull * F, &rc);
+    float* st = (float*)scratch_slot(ctx, S_A5, 4ull * C * F, &rc);
     if (rc) return rc;
     PK_CHECK(pk_launch_row_norms(cd, C, dim, 1, cn, s));
     PK_CHECK(pk_launch_row_norms(fd, F, dim, 1, fnorm, s));
@@ -1671,13 +1785,13 @@ int puckgpu_assign_cells(puckgpu_ctx* ctx, const float* coarse, const float* fin
     PK_CHECK(pk_launch_gemm_exact(cd, fd, st, C, F, dim, F, PK_EPI_ADD_BIAS, fnorm, s));
     const uint32_t CHK = 32768;
     const uint32_t cb = n < CHK ? n : CHK;
-    float* ipc = (float*)scratch(ctx, S_A6, 4ull * cb * C, &rc);
-    float* ipf = (float*)scratch(ctx, S_A7, 4ull * cb * F, &rc);
-    float* xn = (float*)scratch(ctx, S_QN, 4ull * cb, &rc);
-    float* xs = (float*)scratch(ctx, S_Q, 4ull * cb * dim, &rc);
-    uint32_t* co = (uint32_t*)scratch(ctx, S_OI, 4ull * cb, &rc);
-    float* cdo = (float*)scratch(ctx, S_OD, 4ull * cb, &rc);
-    uint32_t* ord_d = (uint32_t*)scratch(ctx, S_QLIST, 4ull * cb, &rc);
+    float* ipc = (float*)scratch_slot(ctx, S_A6, 4ull * cb * C, &rc);
+    float* ipf = (float*)scratch_slot(ctx, S_A7, 4ull * cb * F, &rc);
+    float* xn = (float*)scratch_slot(ctx, S_QN, 4ull * cb, &rc);
+    float* xs = (float*)scratch_slot(ctx, S_Q, 4ull * cb * dim, &rc);
+    uint32_t* co = (uint32_t*)scratch_slot(ctx, S_OI, 4ull * cb, &rc);
+    float* cdo = (float*)scratch_slot(ctx, S_OD, 4ull * cb, &rc);
+    uint32_t* ord_d = (uint32_t*)scratch_slot(ctx, S_QLIST, 4ull * cb, &rc);
     if (rc) return rc;
     std::vector<uint32_t> ord_h(cb), perm_h(cb);
     // is fbound non-decreasing (fine centers stored by descending norm)?
@@ -1750,10 +1864,10 @@ int puckgpu_encode_pq(puckgpu_ctx* ctx, const float* coarse, const float* fine, 
     const uint64_t frec = 4 + filter_nsq, prec = 4 + pq_nsq;
     const uint32_t CHK = 1u << 20;
     const uint32_t cbn = n < CHK ? n : CHK;
-    float* xs = device_ptrs ? nullptr : (float*)scratch(ctx, S_A3, 4ull * cbn * dim, &rc);
-    uint32_t* cs = device_ptrs ? nullptr : (uint32_t*)scratch(ctx, S_A4, 4ull * cbn, &rc);
-    uint8_t* fr = device_ptrs ? nullptr : (uint8_t*)scratch(ctx, S_A7, frec * cbn, &rc);
-    uint8_t* ro = device_ptrs ? nullptr : (uint8_t*)scratch(ctx, S_A5, prec * cbn, &rc);
+    float* xs = device_ptrs ? nullptr : (float*)scratch_slot(ctx, S_A3, 4ull * cbn * dim, &rc);
+    uint32_t* cs = device_ptrs ? nullptr : (uint32_t*)scratch_slot(ctx, S_A4, 4ull * cbn, &rc);
+    uint8_t* fr = device_ptrs ? nullptr : (uint8_t*)scratch_slot(ctx, S_A7, frec * cbn, &rc);
+    uint8_t* ro = device_ptrs ? nullptr : (uint8_t*)scratch_slot(ctx, S_A5, prec * cbn, &rc);
     if (rc) return rc;
     for (uint32_t p0 = 0; p0 < n; p0 += CHK) {
         const uint32_t m = (n - p0) < CHK ? (n - p0) : CHK;
@@ -1795,9 +1909,9 @@ int puckgpu_encode_filter(puckgpu_ctx* ctx, const float* coarse, const float* fi
     const uint64_t rec = 4 + nsq;
     const uint32_t CHK = 1u << 20;
     const uint32_t cbn = n < CHK ? n : CHK;
-    float* xs = device_ptrs ? nullptr : (float*)scratch(ctx, S_A3, 4ull * cbn * dim, &rc);
-    uint32_t* cs = device_ptrs ? nullptr : (uint32_t*)scratch(ctx, S_A4, 4ull * cbn, &rc);
-    uint8_t* ro = device_ptrs ? nullptr : (uint8_t*)scratch(ctx, S_A5, rec * cbn, &rc);
+    float* xs = device_ptrs ? nullptr : (float*)scratch_slot(ctx, S_A3, 4ull * cbn * dim, &rc);
+    uint32_t* cs = device_ptrs ? nullptr : (uint32_t*)scratch_slot(ctx, S_A4, 4ull * cbn, &rc);
+    uint8_t* ro = device_ptrs ? nullptr : (uint8_t*)scratch_slot(ctx, S_A5, rec * cbn, &rc);
     if (rc) return rc;
     for (uint32_t p0 = 0; p0 < n; p0 += CHK) {
         const uint32_t m = (n - p0) < CHK ? (n - p0) : CHK;
@@ -1830,9 +1944,9 @@ int puckgpu_kmeans_assign(puckgpu_ctx* ctx, uint32_t dim, const float* centroids
         // sub-vector k-means (filter-PQ codebooks): one sdot per pair, in
         // MKL's order for such lengths, instead of a GEMM with K <= 4
         const uint32_t CHS = 1u << 20;
-        float* xs = device_ptrs ? nullptr : (float*)scratch(ctx, S_A4, 4ull * (n < CHS ? n : CHS) * dim, &rc);
-        int32_t* ao = device_ptrs ? nullptr : (int32_t*)scratch(ctx, S_A5, 4ull * (n < CHS ? n : CHS), &rc);
-        float* dd = device_ptrs ? nullptr : (float*)scratch(ctx, S_A6, 4ull * (n < CHS ? n : CHS), &rc);
+        float* xs = device_ptrs ? nullptr : (float*)scratch_slot(ctx, S_A4, 4ull * (n < CHS ? n : CHS) * dim, &rc);
+        int32_t* ao = device_ptrs ? nullptr : (int32_t*)scratch_slot(ctx, S_A5, 4ull * (n < CHS ? n : CHS), &rc);
+        float* dd = device_ptrs ? nullptr : (float*)scratch_slot(ctx, S_A6, 4ull * (n < CHS ? n : CHS), &rc);
         if (rc) return rc;
         for (uint32_t p0 = 0; p0 < n; p0 += CHS) {
             const uint32_t m = (n - p0) < CHS ? (n - p0) : CHS;
@@ -1853,16 +1967,16 @@ int puckgpu_kmeans_assign(puckgpu_ctx* ctx, uint32_t dim, const float* centroids
         PK_CHECK(hipStreamSynchronize(s));
         return PUCKGPU_OK;
     }
-    float* cn = (float*)scratch(ctx, S_A1, 4ull * k, &rc);
+    float* cn = (float*)scratch_slot(ctx, S_A1, 4ull * k, &rc);
     if (rc) return rc;
     PK_CHECK(pk_launch_row_norms(cd, k, dim, 0, cn, s));
     uint32_t CHK = (uint32_t)std::max<uint64_t>(1024, std::min<uint64_t>(1u << 20, (256ull << 20) / (4ull * k)));
     const uint32_t cb = n < CHK ? n : CHK;
-    float* ip = (float*)scratch(ctx, S_A2, 4ull * cb * k, &rc);
-    float* pn = (float*)scratch(ctx, S_A3, 4ull * cb, &rc);
-    float* xs = device_ptrs ? nullptr : (float*)scratch(ctx, S_A4, 4ull * cb * dim, &rc);
-    int32_t* ao = device_ptrs ? nullptr : (int32_t*)scratch(ctx, S_A5, 4ull * cb, &rc);
-    float* dd = device_ptrs ? nullptr : (float*)scratch(ctx, S_A6, 4ull * cb, &rc);
+    float* ip = (float*)scratch_slot(ctx, S_A2, 4ull * cb * k, &rc);
+    float* pn = (float*)scratch_slot(ctx, S_A3, 4ull * cb, &rc);
+    float* xs = device_ptrs ? nullptr : (float*)scratch_slot(ctx, S_A4, 4ull * cb * dim, &rc);
+    int32_t* ao = device_ptrs ? nullptr : (int32_t*)scratch_slot(ctx, S_A5, 4ull * cb, &rc);
+    float* dd = device_ptrs ? nullptr : (float*)scratch_slot(ctx, S_A6, 4ull * cb, &rc);
     if (rc) return rc;
     for (uint32_t p0 = 0; p0 < n; p0 += CHK) {
         const uint32_t m = (n - p0) < CHK ? (n - p0) : CHK;

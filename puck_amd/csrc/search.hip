@@ -1050,7 +1050,12 @@ __device__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
     if (tid == 0) { s.red[13] = ftk; s.red[14] = 0u; }
     __syncthreads();
     if (a.qstatus[qi] != 0) {
-        if (tid == 0) a.out_n[qi] = 0;
+        if (tid == 0) {
+            a.out_n[qi] = 0;
+            if (a.tie_n) a.tie_n[qi] = 0;
+            if (a.stats)
+                for (int k = 0; k < 4; ++k) a.stats[qi * 4 + k] = 0;
+        }
         return;
     }
     const float* qg = a.q + (uint64_t)qi * a.dim;
@@ -1462,6 +1467,8 @@ __device__ void pk_scan_budget_one(const PkWalkArgs& a, uint32_t qi, unsigned ch
             a.out_n[qi] = 0;
             if (a.flags) a.flags[qi] = 0;
             if (a.tie_n) a.tie_n[qi] = 0;
+            if (a.stats)
+                for (int k = 0; k < 4; ++k) a.stats[qi * 4 + k] = 0;
         }
         return;
     }
