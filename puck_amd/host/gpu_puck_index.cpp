@@ -79,6 +79,12 @@ std::string join(const std::string& a, const std::string& b) { return a + "/" + 
 
 }  // namespace
 
+IndexType load_index_type(const GpuIndexFlags& flags) {
+    Conf c;
+    if (!read_conf(join(flags.index_path, flags.index_file_name), &c)) return (IndexType)-1;
+    return (IndexType)c.index_type;
+}
+
 GpuPuckIndex::GpuPuckIndex(const GpuIndexFlags& flags) : _flags(flags) {}
 
 GpuPuckIndex::~GpuPuckIndex() {
@@ -92,19 +98,25 @@ int GpuPuckIndex::init() {
     Conf c;
     if (!read_conf(join(P, _flags.index_file_name), &c)) { _err = "model file read error"; return -1; }
     if (c.index_type != (int32_t)IndexType::PUCK) { _err = "index_type is not PUCK"; return -1; }
-    if (c.whether_pq) { _err = "whether_pq = true is not served by the GPU engine"; return -1; }
     if (c.ip2cos) { _err = "ip2cos indexes are not served by the GPU engine"; return -1; }
     if (c.filter_nsq == 0 || c.dim % c.filter_nsq) { _err = "filter_nsq does not divide dim"; return -1; }
     const uint32_t lsq = c.dim / c.filter_nsq;
-    std::vector<float> coarse, fine, cb, feat;
+    if (c.whether_pq && (c.nsq == 0 || c.dim % c.nsq)) { _err = "nsq does not divide dim"; return -1; }
+    std::vector<float> coarse, fine, cb, feat, pcb;
     std::vector<uint32_t> cell;
-    std::vector<uint8_t> rec;
-    if (!read_fvecs(join(P, _flags.coarse_codebook_file_name), c.dim, c.C, &coarse) ||
-        !read_fvecs(join(P, _flags.fine_codebook_file_name), c.dim, c.F, &fine) ||
-        !read_fvecs(join(P, _flags.filter_codebook_file_name), lsq, 256ull * c.filter_nsq, &cb) ||
-        !read_raw(join(P, _flags.cell_assign_file_name), c.N, &cell) ||
-        !read_raw(join(P, _flags.filter_data_file_name), (uint64_t)c.N * (4 + c.filter_nsq), &rec) ||
-        !read_fvecs(join(P, _flags.feature_file_name), c.dim, c.N, &feat)) {
+    std::vector<uint8_t> rec, prec;
+    bool ok = read_fvecs(join(P, _flags.coarse_codebook_file_name), c.dim, c.C, &coarse) &&
+              read_fvecs(join(P, _flags.fine_codebook_file_name), c.dim, c.F, &fine) &&
+              read_fvecs(join(P, _flags.filter_codebook_file_name), lsq, 256ull * c.filter_nsq, &cb) &&
+              read_raw(join(P, _flags.cell_assign_file_name), c.N, &cell) &&
+              read_raw(join(P, _flags.filter_data_file_name), (uint64_t)c.N * (4 + c.filter_nsq), &rec);
+    // whether_pq: the PQ codes replace the raw vectors, which are not read
+    if (ok && c.whether_pq)
+        ok = read_fvecs(join(P, _flags.pq_codebook_file_name), c.dim / c.nsq, 256ull * c.nsq, &pcb) &&
+             read_raw(join(P, _flags.pq_data_file_name), (uint64_t)c.N * (4 + c.nsq), &prec);
+    else if (ok)
+        ok = read_fvecs(join(P, _flags.feature_file_name), c.dim, c.N, &feat);
+    if (!ok) {
         _err = "check file length has error";
         return -1;
     }
@@ -116,9 +128,12 @@ int GpuPuckIndex::init() {
     if (!_ctx && puckgpu_create(_flags.device, &_ctx) != 0) { _err = "no GPU context"; return -1; }
     if (_index) { puckgpu_index_free(_index); _index = nullptr; }
     puckgpu_index_params ip{c.dim, c.C, c.F, c.filter_nsq, c.N, c.whether_norm ? 1 : 0};
-    if (puckgpu_index_load(_ctx, &ip, coarse.data(), fine.data(), cb.data(), cell.data(), rec.data(), feat.data(), 0,
-                           &_index) != 0) {
+    if (puckgpu_index_load(_ctx, &ip, coarse.data(), fine.data(), cb.data(), cell.data(), rec.data(),
+                           c.whether_pq ? nullptr : feat.data(), 0, &_index) != 0 ||
+        (c.whether_pq &&
+         puckgpu_index_attach_pq(_ctx, _index, c.nsq, pcb.data(), prec.data(), 0) != 0)) {
         _err = puckgpu_last_error(_ctx);
+        if (_index) { puckgpu_index_free(_index); _index = nullptr; }
         return -1;
     }
     _sp.topk_max = _flags.topk;
@@ -158,11 +173,13 @@ int GpuPuckIndex::build() {
     if (!read_conf(join(P, _flags.index_file_name), &c)) { _err = "model file read error"; return -1; }
     if (c.filter_nsq == 0 || c.dim % c.filter_nsq) { _err = "filter_nsq does not divide dim"; return -1; }
     const uint32_t lsq = c.dim / c.filter_nsq;
-    std::vector<float> coarse, fine, cb, feat;
+    if (c.whether_pq && (c.nsq == 0 || c.dim % c.nsq)) { _err = "nsq does not divide dim"; return -1; }
+    std::vector<float> coarse, fine, cb, feat, pcb;
     if (!read_fvecs(join(P, _flags.coarse_codebook_file_name), c.dim, c.C, &coarse) ||
         !read_fvecs(join(P, _flags.fine_codebook_file_name), c.dim, c.F, &fine) ||
         !read_fvecs(join(P, _flags.filter_codebook_file_name), lsq, 256ull * c.filter_nsq, &cb) ||
-        !read_fvecs(join(P, _flags.feature_file_name), c.dim, c.N, &feat)) {
+        !read_fvecs(join(P, _flags.feature_file_name), c.dim, c.N, &feat) ||
+        (c.whether_pq && !read_fvecs(join(P, _flags.pq_codebook_file_name), c.dim / c.nsq, 256ull * c.nsq, &pcb))) {
         _err = "check file length has error";
         return -1;
     }
@@ -176,14 +193,32 @@ int GpuPuckIndex::build() {
         _err = puckgpu_last_error(_ctx);
         return -1;
     }
+    // whether_pq: the PQ encodes what the filter codewords leave (puck_index.cpp:602-709)
+    std::vector<uint8_t> prec;
+    if (c.whether_pq) {
+        prec.resize((uint64_t)c.N * (4 + c.nsq));
+        if (puckgpu_encode_pq(_ctx, coarse.data(), fine.data(), c.C, c.F, c.dim, cb.data(), c.filter_nsq, rec.data(),
+                              pcb.data(), c.nsq, c.N, feat.data(), cell.data(), prec.data(), 0) != 0) {
+            _err = puckgpu_last_error(_ctx);
+            return -1;
+        }
+    }
     std::ofstream oc(join(P, _flags.cell_assign_file_name), std::ios::binary);
     oc.write((const char*)cell.data(), (std::streamsize)(4ull * c.N));
     std::ofstream orc(join(P, _flags.filter_data_file_name), std::ios::binary);
     orc.write((const char*)rec.data(), (std::streamsize)rec.size());
+    bool pq_ok = true;
+    if (c.whether_pq) {
+        std::ofstream opq(join(P, _flags.pq_data_file_name), std::ios::binary);
+        opq.write((const char*)prec.data(), (std::streamsize)prec.size());
+        pq_ok = (bool)opq;
+    }
     c.index_type = (int32_t)IndexType::PUCK;
     c.whether_filter = true;
-    c.whether_pq = false;
-    if (!oc || !orc || !write_conf(join(P, _flags.index_file_name), c)) { _err = "write index files failed"; return -1; }
+    if (!oc || !orc || !pq_ok || !write_conf(join(P, _flags.index_file_name), c)) {
+        _err = "write index files failed";
+        return -1;
+    }
     return 0;
 }
 

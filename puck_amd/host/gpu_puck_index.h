@@ -2,7 +2,7 @@
 //
 // Request / Response / Index have the shape of puck/index.h:30-81 so that a
 // caller written against the reference compiles unchanged; GpuPuckIndex
-// stands in for PuckIndex (puck/puck/puck_index.h) with whether_pq = false.
+// stands in for PuckIndex (puck/puck/puck_index.h), whether_pq false or true.
 // The host code reaches the GPU only through include/puck_gpu.h.
 #pragma once
 #include <cstdint>
@@ -47,6 +47,8 @@ struct GpuIndexFlags {
     std::string cell_assign_file_name = "cell_assign.dat";
     std::string filter_codebook_file_name = "filter_codebook.dat";
     std::string filter_data_file_name = "filter_data.dat";
+    std::string pq_codebook_file_name = "learn_codebooks.dat";
+    std::string pq_data_file_name = "pq_data.dat";
     uint32_t topk = 100;
     uint32_t search_coarse_count = 200;
     int32_t filter_topk = -1;  // unset: 11 * topk (index_conf.cpp:178-180)
@@ -54,11 +56,18 @@ struct GpuIndexFlags {
     int device = 0;
 };
 
+// The index type written in index.dat (load_index_type,
+// hierarchical_cluster_index.cpp:1544-1549); IndexType(-1) when the file
+// cannot be read.
+IndexType load_index_type(const GpuIndexFlags& flags = GpuIndexFlags());
+
 class GpuPuckIndex : public Index {
 public:
     explicit GpuPuckIndex(const GpuIndexFlags& flags = GpuIndexFlags());
     ~GpuPuckIndex() override;
-    // Reads index.dat and the index files, loads them into HBM (0 / -1).
+    // Reads index.dat and the index files, loads them into HBM (0 / -1).  A
+    // whether_pq index loads learn_codebooks.dat and pq_data.dat in place of
+    // the raw vectors (puck_index.cpp:198-227).
     int init() override;
     // One query (puck_index.cpp:559-600): -1 for topk > FLAGS_topk or a null
     // feature, leaving the response untouched.
@@ -67,7 +76,8 @@ public:
     int search_batch(uint32_t n, const float* queries, uint32_t topk, float* distance, uint32_t* labels,
                      uint32_t* result_num = nullptr);
     // Assigns all_data.feat.bin to cells and writes cell_assign.dat,
-    // filter_data.dat and index.dat with the codebooks already on disk.
+    // filter_data.dat (and pq_data.dat when index.dat says whether_pq) and
+    // index.dat with the codebooks already on disk.
     int build() override;
     // Training runs in puck_amd.train (Python); returns -1 here.
     int train() override;

@@ -11,6 +11,10 @@ Formats (all little-endian):
 * cell_assign.dat -- N x u32 cell id (coarse * F + fine), local order.
 * filter_data.dat -- N x (f32 offset + filter_nsq code bytes), local order
   (quantization.cpp:276-294).
+* whether_pq = true: learn_codebooks.dat (the second-stage codebook, fvecs
+  rows of dim / nsq floats) and pq_data.dat (N x (f32 offset + nsq code
+  bytes), local order); all_data.feat.bin is not read (the reference does not
+  load raw vectors for such an index, puck_index.cpp:198-227).
 """
 import os
 import struct
@@ -73,24 +77,34 @@ def paths(index_path, flags):
     return dict(index=j("index_file_name"), coarse=j("coarse_codebook_file_name"),
                 fine=j("fine_codebook_file_name"), cell_assign=j("cell_assign_file_name"),
                 filter_codebook=j("filter_codebook_file_name"), filter_data=j("filter_data_file_name"),
-                feature=j("feature_file_name"))
+                feature=j("feature_file_name"), pq_codebook=j("pq_codebook_file_name"),
+                pq_data=j("pq_data_file_name"))
 
 
-def write_index(index_path, flags, coarse, fine, codebook, cell_assign, records, features, whether_norm):
-    """Writes a PUCK index directory (whether_pq = false) the reference can load."""
+def write_index(index_path, flags, coarse, fine, codebook, cell_assign, records, features, whether_norm,
+                pq_codebook=None, pq_records=None):
+    """Writes a PUCK index directory the reference can load: whether_pq =
+    false with the raw features, or whether_pq = true with the second-stage
+    codebook and records (features may then be None)."""
     os.makedirs(index_path, exist_ok=True)
     p = paths(index_path, flags)
     C, dim = coarse.shape
     F = fine.shape[0]
     N = cell_assign.shape[0]
     nsq = records.shape[1] - 4
-    write_index_conf(p["index"], dim, whether_norm, N, nsq, C, F)
+    pq = pq_records is not None
+    pnsq = pq_records.shape[1] - 4 if pq else dim
+    write_index_conf(p["index"], dim, whether_norm, N, nsq, C, F, whether_pq=pq, nsq=pnsq)
     write_fvecs(p["coarse"], coarse)
     write_fvecs(p["fine"], fine)
     write_fvecs(p["filter_codebook"], np.asarray(codebook, np.float32).reshape(nsq * 256, dim // nsq))
     np.ascontiguousarray(cell_assign, np.uint32).tofile(p["cell_assign"])
     np.ascontiguousarray(records, np.uint8).tofile(p["filter_data"])
-    write_fvecs(p["feature"], features)
+    if pq:
+        write_fvecs(p["pq_codebook"], np.asarray(pq_codebook, np.float32).reshape(pnsq * 256, dim // pnsq))
+        np.ascontiguousarray(pq_records, np.uint8).tofile(p["pq_data"])
+    if features is not None:
+        write_fvecs(p["feature"], features)
 
 
 def read_index(index_path, flags):
@@ -112,6 +126,17 @@ def read_index(index_path, flags):
     rec = np.fromfile(p["filter_data"], dtype=np.uint8)
     if rec.size != N * (4 + nsq):
         raise ValueError(f"{p['filter_data']}: {rec.size} bytes, expected {N * (4 + nsq)}")
-    feat = read_fvecs(p["feature"], d, N)
-    return conf, dict(coarse=coarse, fine=fine, codebook=codebook, cell_assign=cell,
-                      records=rec.reshape(N, 4 + nsq), features=feat)
+    out = dict(coarse=coarse, fine=fine, codebook=codebook, cell_assign=cell, records=rec.reshape(N, 4 + nsq),
+               features=None, pq_codebook=None, pq_records=None)
+    if conf["whether_pq"]:
+        pnsq = conf["nsq"]
+        if pnsq == 0 or d % pnsq:
+            raise ValueError(f"nsq {pnsq} does not divide dim {d}")
+        out["pq_codebook"] = read_fvecs(p["pq_codebook"], d // pnsq, pnsq * 256).reshape(-1)
+        pr = np.fromfile(p["pq_data"], dtype=np.uint8)
+        if pr.size != N * (4 + pnsq):
+            raise ValueError(f"{p['pq_data']}: {pr.size} bytes, expected {N * (4 + pnsq)}")
+        out["pq_records"] = pr.reshape(N, 4 + pnsq)
+    else:
+        out["features"] = read_fvecs(p["feature"], d, N)
+    return conf, out

@@ -12,8 +12,9 @@ Mirrors pyapi_wrapper/py_puck_api.i + py_api_wrapper.{h,cpp}:
 float32 / uint32, or CUDA tensors); a request the reference would reject
 (topk > FLAGS_topk) leaves them untouched, as PuckIndex::search returns -1
 before touching its response (puck_index.cpp:560-563) and PySearcher ignores
-per-query return codes.  Only PUCK indexes with whether_pq = false (filter PQ
-+ exact rerank) and ip2cos = 0 are served; other indexes make init() return -1.
+per-query return codes.  PUCK indexes with ip2cos = 0 are served, ranked by
+exact L2 over the raw vectors (whether_pq = false) or by the second-stage PQ
+(whether_pq = true); other indexes make init() return -1.
 """
 import sys
 
@@ -68,9 +69,6 @@ class PySearcher:
         if conf["index_type"] != index_io.INDEX_PUCK:
             _err(f"index_type {conf['index_type']} is not served by the GPU engine (PUCK only)")
             return -1
-        if conf["whether_pq"]:
-            _err("whether_pq = true (second-stage PQ rerank) is not served by the GPU engine")
-            return -1
         if conf["ip2cos"]:
             _err("ip2cos indexes (inner product stored as an extra dimension) are not served by the GPU engine")
             return -1
@@ -85,7 +83,8 @@ class PySearcher:
                 self._ctx = Context(0)
             try:
                 self._index = GpuIndex(self._ctx, arr["coarse"], arr["fine"], arr["codebook"], arr["cell_assign"],
-                                       arr["records"], arr["features"], conf["whether_norm"])
+                                       arr["records"], arr["features"], conf["whether_norm"],
+                                       pq_codebook=arr["pq_codebook"], pq_records=arr["pq_records"])
             except PuckGpuError as e:
                 _err(str(e))
                 return -1
@@ -120,12 +119,20 @@ class PySearcher:
             xd = xd / xd.norm(dim=1, keepdim=True).clamp_min(1e-30)
         if self._ctx is None:
             self._ctx = Context(0)
+        pq_nsq = None
+        if FL.get("whether_pq"):
+            # IndexConf: nsq left at its default means one subspace per dimension (index_conf.cpp:39-49)
+            pq_nsq = dim if FL.is_default("nsq") else FL.get("nsq")
+            if pq_nsq == 0 or dim % pq_nsq or dim // pq_nsq not in (1, 2, 4) or pq_nsq > 128:
+                _err(f"nsq {pq_nsq} must divide feature_dim {dim} into sub-vectors of 1, 2 or 4 (at most 128)")
+                return -1
         ix = train.build(self._ctx, xd.contiguous(), FL.get("coarse_cluster_count"), FL.get("fine_cluster_count"),
                          nsq, whether_norm, train_n=min(FL.get("train_points_count"), total_cnt),
-                         niter=FL.get("kmeans_iterations_count"))
+                         niter=FL.get("kmeans_iterations_count"), pq_nsq=pq_nsq)
         h = {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in ix.items()}
         index_io.write_index(path, FL, h["coarse"], h["fine"], h["codebook"], h["cell_assign"].view(np.uint32),
-                             h["records"], x if not whether_norm else h["raw"], whether_norm)
+                             h["records"], None if pq_nsq else (x if not whether_norm else h["raw"]), whether_norm,
+                             pq_codebook=h.get("pq_codebook"), pq_records=h.get("pq_records"))
         return 0
 
     def search(self, n, query_fea, topk, distance, labels):

@@ -80,9 +80,34 @@ def train_codebook(ctx, coarse, fine, xs, nsq, niter=10, seed=7):
     return cb
 
 
-def build(ctx, x, C, F, nsq, whether_norm, train_n=262144, niter=10, seed=0):
+def train_pq_codebook(ctx, coarse, fine, codebook, nsq, xs, pq_nsq, niter=10, seed=13):
+    """Second-stage codebook [pq_nsq, 256, d / pq_nsq] (whether_pq = true) on
+    what the filter codewords leave of the cell residuals of xs: the
+    quantizers are trained in turn, each on the residual its predecessors
+    leave (PuckIndex::train, puck_index.cpp:836-871); the filter part is taken
+    at the codes the encoder picks."""
+    d = xs.shape[1]
+    F = fine.shape[0]
+    flsq, plsq = d // nsq, d // pq_nsq
+    cell = ctx.assign_cells(coarse, fine, xs)
+    rec = ctx.encode_filter(coarse, fine, codebook, nsq, xs, cell)
+    cl = cell.long()
+    res = (xs - coarse[cl // F]) - fine[cl % F]
+    codes = rec[:, 4:].long()
+    cbf = codebook.reshape(nsq, 256, flsq)
+    dec = torch.stack([cbf[m][codes[:, m]] for m in range(nsq)], 1).reshape(xs.shape[0], d)
+    res = (res - dec).contiguous()
+    cb = torch.empty((pq_nsq, 256, plsq), dtype=torch.float32, device=xs.device)
+    for m in range(pq_nsq):
+        cb[m] = kmeans(ctx, res[:, m * plsq:(m + 1) * plsq].contiguous(), 256, niter, seed + 31 * m)
+    return cb
+
+
+def build(ctx, x, C, F, nsq, whether_norm, train_n=262144, niter=10, seed=0, pq_nsq=None):
     """Train, assign and encode x (CUDA [N, d]); returns the index arrays
-    (device tensors in the reference file layouts) ready for GpuIndex."""
+    (device tensors in the reference file layouts) ready for GpuIndex.  With
+    pq_nsq, also the second-stage PQ of a whether_pq = true index
+    (pq_codebook, pq_records)."""
     N = x.shape[0]
     g = torch.Generator(device=x.device)
     g.manual_seed(seed + 99)
@@ -90,7 +115,13 @@ def build(ctx, x, C, F, nsq, whether_norm, train_n=262144, niter=10, seed=0):
     xs = x[sel].contiguous()
     coarse, fine = train_centers(ctx, xs, C, F, niter, seed)
     codebook = train_codebook(ctx, coarse, fine, xs, nsq, niter, seed + 7)
+    cb = codebook.reshape(-1).contiguous()
     cell = ctx.assign_cells(coarse, fine, x)
-    rec = ctx.encode_filter(coarse, fine, codebook.reshape(-1).contiguous(), nsq, x, cell)
-    return dict(coarse=coarse, fine=fine, codebook=codebook.reshape(-1).contiguous(), cell_assign=cell,
-                records=rec, raw=x, whether_norm=whether_norm, nsq=nsq)
+    rec = ctx.encode_filter(coarse, fine, cb, nsq, x, cell)
+    out = dict(coarse=coarse, fine=fine, codebook=cb, cell_assign=cell, records=rec, raw=x,
+               whether_norm=whether_norm, nsq=nsq)
+    if pq_nsq:
+        pcb = train_pq_codebook(ctx, coarse, fine, cb, nsq, xs, pq_nsq, niter, seed + 13).reshape(-1).contiguous()
+        out.update(pq_codebook=pcb, pq_records=ctx.encode_pq(coarse, fine, cb, nsq, rec, pcb, pq_nsq, x, cell),
+                   pq_nsq=pq_nsq)
+    return out

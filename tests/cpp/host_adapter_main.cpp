@@ -2,6 +2,8 @@
 // init, then Request/Response searches.  Usage:
 //   host_adapter_test <index_path> <queries.f32> <nq> <dim> <topk> <FLAGS_topk> <scc> <filter_topk> <out.bin> [build]
 // Writes nq*topk floats, nq*topk uint32 labels and nq uint32 counts to out.bin.
+//   host_adapter_test <index_path> type
+// prints load_index_type() as an integer.
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -11,6 +13,12 @@
 #include "../../puck_amd/host/gpu_puck_index.h"
 
 int main(int argc, char** argv) {
+    if (argc == 3 && std::string(argv[2]) == "type") {
+        puck::GpuIndexFlags tf;
+        tf.index_path = argv[1];
+        printf("%d\n", (int)puck::load_index_type(tf));
+        return 0;
+    }
     if (argc < 10) return 2;
     puck::GpuIndexFlags f;
     f.index_path = argv[1];

@@ -87,7 +87,7 @@ def test_searcher_init_errors(tmp_path):
     index_io.write_index_conf(str(d / "index.dat"), 8, False, 0, 4, 2, 2, index_type=index_io.INDEX_TINKER)
     api.update_gflag("index_path", str(d))
     assert s.init() == -1
-    e = tmp_path / "pq"
+    e = tmp_path / "pq"  # a whether_pq index whose codebook files are missing
     e.mkdir()
     index_io.write_index_conf(str(e / "index.dat"), 8, False, 0, 4, 2, 2, whether_pq=True)
     api.update_gflag("index_path", str(e))
@@ -100,3 +100,23 @@ def test_searcher_init_errors(tmp_path):
     # a search before a successful init() is refused; the reference's wrapper
     # returns 0 from every search it runs
     assert s.search(1, np.zeros(9, np.float32), 1, np.zeros(1, np.float32), np.zeros(1, np.uint32)) == -1
+
+
+def test_index_io_pq_round_trip(tmp_path):
+    # whether_pq = true: learn_codebooks.dat and pq_data.dat written and read
+    # back in the reference layouts; no raw feature file
+    rng = np.random.default_rng(3)
+    C, F, d, nsq, pnsq, N = 3, 4, 8, 4, 8, 50
+    coarse, fine = rng.normal(size=(C, d)).astype(np.float32), rng.normal(size=(F, d)).astype(np.float32)
+    cb = rng.normal(size=(nsq, 256, d // nsq)).astype(np.float32)
+    pcb = rng.normal(size=(pnsq, 256, d // pnsq)).astype(np.float32)
+    cell = rng.integers(0, C * F, N).astype(np.uint32)
+    rec = rng.integers(0, 256, (N, 4 + nsq)).astype(np.uint8)
+    prec = rng.integers(0, 256, (N, 4 + pnsq)).astype(np.uint8)
+    index_io.write_index(str(tmp_path), FL, coarse, fine, cb, cell, rec, None, True, pq_codebook=pcb, pq_records=prec)
+    assert not (tmp_path / FL.get("feature_file_name")).exists()
+    assert (tmp_path / FL.get("pq_codebook_file_name")).stat().st_size == pnsq * 256 * (4 + 4 * (d // pnsq))
+    conf, arr = index_io.read_index(str(tmp_path), FL)
+    assert conf["whether_pq"] and conf["nsq"] == pnsq and arr["features"] is None
+    assert np.array_equal(arr["pq_codebook"], pcb.reshape(-1)) and np.array_equal(arr["pq_records"], prec)
+    assert np.array_equal(arr["records"], rec) and np.array_equal(arr["codebook"], cb.reshape(-1))

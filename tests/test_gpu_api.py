@@ -54,7 +54,10 @@ def test_searcher_matches_oracle(tmp_path):
     assert s.init() == -1
 
 
-def test_searcher_build_then_search(tmp_path):
+@pytest.mark.parametrize("whether_pq", [False, True])
+def test_searcher_build_then_search(tmp_path, whether_pq):
+    # whether_pq defaults to true (puck_gflags.cpp): the build then trains and
+    # writes the second-stage PQ and the index ranks by it
     x = D.unit_norm_mixture(8000, 32, 64, seed=40, means_seed=41)
     q = D.unit_norm_mixture(16, 32, 64, seed=42, means_seed=41)
     index_io.write_fvecs(str(tmp_path / "all_data.feat.bin"), x)
@@ -62,6 +65,8 @@ def test_searcher_build_then_search(tmp_path):
                  ("fine_cluster_count", 16), ("filter_nsq", 16), ("train_points_count", 4000),
                  ("kmeans_iterations_count", 4), ("topk", 10), ("search_coarse_count", 4), ("filter_topk", 100)]:
         api.update_gflag(k, v)
+    if not whether_pq:
+        api.update_gflag("whether_pq", "false")
     s = api.PySearcher()
     assert s.build(8000) == 0
     assert s.init() == 0
@@ -69,8 +74,37 @@ def test_searcher_build_then_search(tmp_path):
     Im = np.zeros((16, 10), np.uint32)
     s.search(16, q, 10, Dm, Im)
     conf, arr = index_io.read_index(str(tmp_path), FL)
+    assert bool(conf["whether_pq"]) == whether_pq and (arr["features"] is None) == whether_pq
     oi = O.OracleIndex(arr["coarse"], arr["fine"], arr["codebook"], arr["cell_assign"], arr["records"],
-                       arr["features"], conf["whether_norm"])
+                       x if whether_pq else arr["features"], conf["whether_norm"])
+    if whether_pq:
+        assert conf["nsq"] == 32  # one subspace per dimension by default (index_conf.cpp:39-49)
+        oi.set_pq(arr["pq_codebook"], arr["pq_records"])
+        # the GPU's PQ records equal the oracle's encoding of the same points
+        h = O.encode_puck(arr["coarse"], arr["fine"], arr["codebook"], 16, arr["pq_codebook"], 32,
+                          x / np.linalg.norm(x, axis=1, keepdims=True).astype(np.float32), arr["cell_assign"])[1]
+        assert np.mean(np.all(h == arr["pq_records"], axis=1)) > 0.99
     Do, Io, _, _ = oi.search_batch(q, 10, 4, 100)
+    assert np.array_equal(Im, Io)
+    assert np.array_equal(_bits(Dm), _bits(Do))
+
+
+def test_searcher_serves_pq_index(tmp_path):
+    # a whether_pq = true directory written from the CPU build: no raw vectors
+    # on disk, searched by the engine and the oracle alike
+    x = D.mixture_uint8_grid(6000, 64, 100, seed=33, means_seed=34)
+    q = D.mixture_uint8_grid(32, 64, 100, seed=35, means_seed=34)
+    ix = CI.build_index_pq(x, 24, 24, 16, 64, whether_norm=False, train_n=4000)
+    index_io.write_index(str(tmp_path), FL, ix["coarse"], ix["fine"], ix["codebook"], ix["cell_assign"],
+                         ix["records"], None, False, pq_codebook=ix["pq_codebook"], pq_records=ix["pq_records"])
+    assert not (tmp_path / "all_data.feat.bin").exists()
+    for k, v in [("index_path", str(tmp_path)), ("topk", 20), ("search_coarse_count", 6), ("filter_topk", 300)]:
+        api.update_gflag(k, v)
+    s = api.PySearcher()
+    assert s.init() == 0
+    Dm = np.zeros((32, 10), np.float32)
+    Im = np.zeros((32, 10), np.uint32)
+    s.search(32, q, 10, Dm, Im)
+    Do, Io, _, _ = CI.oracle_index(ix).search_batch(q, 10, 6, 300, max_topk=20)
     assert np.array_equal(Im, Io)
     assert np.array_equal(_bits(Dm), _bits(Do))

@@ -68,3 +68,36 @@ def test_cpp_adapter_build(tmp_path):
     Do, Io, _, _ = CI.oracle_index(ix).search_batch(q, 10, 4, 100)
     assert np.array_equal(Im, Io)
     assert np.array_equal(Dm.view(np.uint32), Do.view(np.uint32))
+
+
+def test_cpp_adapter_serves_and_builds_pq(tmp_path):
+    # whether_pq = true: init reads learn_codebooks.dat + pq_data.dat in place
+    # of the raw vectors; build() writes pq_data.dat with the GPU encoder
+    x = D.mixture_uint8_grid(6000, 32, 50, seed=63, means_seed=64)
+    q = D.mixture_uint8_grid(20, 32, 50, seed=65, means_seed=64)
+    ix = CI.build_index_pq(x, 12, 12, 8, 16, whether_norm=False, train_n=5000)
+    d = tmp_path / "idx"
+    index_io.write_index(str(d), FL, ix["coarse"], ix["fine"], ix["codebook"], ix["cell_assign"], ix["records"],
+                         None, False, pq_codebook=ix["pq_codebook"], pq_records=ix["pq_records"])
+    Dm, Im, R = _run(str(d), q, 10, 10, 4, 120, tmp_path)
+    Do, Io, Ro, _ = CI.oracle_index(ix).search_batch(q, 10, 4, 120)
+    assert np.array_equal(R, Ro)
+    assert np.array_equal(Im, Io)
+    assert np.array_equal(Dm.view(np.uint32), Do.view(np.uint32))
+    # build from the codebooks and the raw vectors alone
+    b = tmp_path / "built"
+    p = index_io.paths(str(b), FL)
+    os.makedirs(str(b))
+    index_io.write_index_conf(p["index"], 32, False, x.shape[0], 8, 12, 12, whether_pq=True, nsq=16)
+    index_io.write_fvecs(p["coarse"], ix["coarse"])
+    index_io.write_fvecs(p["fine"], ix["fine"])
+    index_io.write_fvecs(p["filter_codebook"], ix["codebook"].reshape(8 * 256, 4))
+    index_io.write_fvecs(p["pq_codebook"], ix["pq_codebook"].reshape(16 * 256, 2))
+    index_io.write_fvecs(p["feature"], x)
+    Db, Ib, Rb = _run(str(b), q, 10, 10, 4, 120, tmp_path, build=True)
+    conf, arr = index_io.read_index(str(b), FL)
+    assert conf["whether_pq"]
+    assert np.array_equal(arr["cell_assign"], ix["cell_assign"])
+    assert np.array_equal(arr["records"], ix["records"])
+    assert np.array_equal(arr["pq_records"], ix["pq_records"])
+    assert np.array_equal(Ib, Io) and np.array_equal(Db.view(np.uint32), Do.view(np.uint32))

@@ -93,3 +93,35 @@ def test_adapter_host_paths_clean_under_sanitizers(index_dir, tmp_path):
                            text=True, timeout=120, env=env)
         assert "Sanitizer" not in r.stderr and "runtime error" not in r.stderr, r.stderr[-2000:]
         assert r.returncode in (0, 4), (path, r.returncode, r.stderr[-500:])
+
+
+@pytest.mark.parametrize("index_type", [0, 1, 2])
+def test_load_index_type(tmp_path, index_type):
+    # load_index_type reads the type field of index.dat
+    # (hierarchical_cluster_index.cpp:1544-1549); no device is touched
+    d = tmp_path / f"t{index_type}"
+    d.mkdir()
+    index_io.write_index_conf(str(d / "index.dat"), 16, False, 10, 8, 2, 2, index_type=index_type)
+    r = subprocess.run([BIN, str(d), "type"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert int(r.stdout.strip()) == index_type
+
+
+def test_load_index_type_missing_file(tmp_path):
+    r = subprocess.run([BIN, str(tmp_path / "absent"), "type"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and int(r.stdout.strip()) == -1
+
+
+def test_init_reports_missing_pq_files(tmp_path):
+    # a whether_pq index needs learn_codebooks.dat and pq_data.dat, not the
+    # raw vectors; without them init fails before any device call
+    x = D.mixture_uint8_grid(600, 16, 8, seed=72, means_seed=73)
+    ix = CI.build_index(x, 4, 4, 8, whether_norm=False)
+    d = tmp_path / "pq"
+    index_io.write_index(str(d), FL, ix["coarse"], ix["fine"], ix["codebook"], ix["cell_assign"], ix["records"],
+                         ix["raw"], False)
+    index_io.write_index_conf(str(d / "index.dat"), 16, False, 600, 8, 4, 4, whether_pq=True, nsq=8)
+    out = str(tmp_path / "out.bin")
+    r = subprocess.run([BIN, str(d), str(d / "q.f32"), "1", "16", "1", "10", "2", "-1", out], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 4 and "check file length" in r.stderr, (r.returncode, r.stderr)
