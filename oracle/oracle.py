@@ -132,6 +132,12 @@ def load_variant(path):
     L.po_encode_puck.restype = ctypes.c_int
     L.po_encode_puck.argtypes = [_f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32, _f32p, ctypes.c_uint32, _f32p,
                                  ctypes.c_uint32, _f32p, _u32p, ctypes.c_uint32, _u8p, _u8p, ctypes.c_int]
+    L.po_index_insert.restype = ctypes.c_int
+    L.po_index_insert.argtypes = [vp, ctypes.c_uint32, _f32p, _u32p, vp, vp, vp, vp]
+    L.po_index_position.restype = ctypes.c_uint32
+    L.po_index_position.argtypes = [vp, ctypes.c_uint32]
+    L.po_index_inserted.restype = ctypes.c_uint32
+    L.po_index_inserted.argtypes = [vp]
     L.po_walk_trace.restype = ctypes.c_int
     L.po_walk_trace.argtypes = [vp, _f32p, _f32p, _f32p, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32,
                                 ctypes.c_double, ctypes.POINTER(WalkStep), ctypes.c_uint32,
@@ -197,6 +203,12 @@ def rerank_dist(x, q):
     return float(lib().po_rerank_dist(x, q, x.size))
 
 
+def empty_cell_stationary():
+    """sqrtf(FLT_MAX), the stationary value of an empty cell
+    (puck_index.cpp:287-306)."""
+    return np.sqrt(np.float32(np.finfo(np.float32).max), dtype=np.float32)
+
+
 def normalize(q):
     q = c32(q)
     out = np.zeros_like(q)
@@ -215,6 +227,7 @@ class OracleIndex:
         self.N = int(cell_assign.shape[0])
         self.nsq = records.shape[1] - 4 if records.ndim == 2 else None
         self.whether_norm = bool(whether_norm)
+        self.pq_nsq = 0
         rec = np.ascontiguousarray(records, np.uint8).reshape(-1)
         self._h = self._L.po_index_create(self.dim, self.C, self.F, self.nsq, self.N,
                                         int(whether_norm), self.coarse, self.fine, self.codebook,
@@ -237,6 +250,45 @@ class OracleIndex:
                                      pq_records.reshape(-1))
         if rc != 0:
             raise ValueError("po_index_set_pq refused its arguments")
+        self.pq_nsq = pq_records.shape[1] - 4
+
+    def insert(self, x):
+        """Realtime insert of the rows of x in order (po_index_insert).
+        Returns ids (local id, or 0xffffffff for a rejected row) and, for the
+        accepted rows, their cells, filter records, second-stage PQ records
+        (whether_pq) and stored features."""
+        x = c32(x)
+        n = x.shape[0]
+        ids = np.zeros(max(n, 1), np.uint32)
+        cells = np.zeros(max(n, 1), np.uint32)
+        rec = np.zeros((max(n, 1), 4 + self.nsq), np.uint8)
+        pn = self.pq_nsq if self.pq_nsq else 0
+        prec = np.zeros((max(n, 1), 4 + pn), np.uint8)
+        xo = np.zeros((max(n, 1), self.dim), np.float32)
+        ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        acc = self._L.po_index_insert(self._h, n, x.reshape(-1), ids, ptr(cells), ptr(rec),
+                                      ptr(prec) if pn else None, ptr(xo))
+        if acc < 0:
+            raise ValueError("po_index_insert failed")
+        out = dict(ids=ids[:n], cells=cells[:acc], records=rec[:acc], features=xo[:acc])
+        if pn:
+            out["pq_records"] = prec[:acc]
+        return out
+
+    def inserted(self):
+        return int(self._L.po_index_inserted(self._h))
+
+    def position(self, tag):
+        """Memory position of a heap tag (memory id, or N + insert ordinal)."""
+        return int(self._L.po_index_position(self._h, int(tag)))
+
+    def local_of_tag(self, tag):
+        tag = int(tag)
+        if tag >= self.N:
+            return tag
+        if getattr(self, "_m2l", None) is None:
+            self._m2l = self.export()["mem2local"]
+        return int(self._m2l[tag])
 
     def export(self):
         C, F, N = self.C, self.F, self.N

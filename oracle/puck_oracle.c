@@ -274,7 +274,40 @@ typedef struct {
     uint32_t pnsq, plsq;
     float* pq_codebook;   /* pnsq x 256 x plsq */
     uint8_t* pq_records;  /* N x (4+pnsq), memory order: [f32 offset][pnsq codes] */
+    /* realtime-insert overlay (RealtimeInsertPuckIndex,
+     * realtime_insert_puck_index.h:99-140): points inserted after the load,
+     * kept apart from the N loaded ones.  Inserted point j has heap tag and
+     * local id N + j (insert_id + _labels.size(), :408, :517); each cell keeps
+     * its inserted points in a list, newest first (InsertFineCluster, :312-318). */
+    uint32_t ins_n, ins_cap;
+    uint32_t* ins_cell;   /* cell of inserted point j */
+    uint32_t* ins_next;   /* the next older inserted point of the same cell, or PO_NONE */
+    uint32_t* ins_ord;    /* j's ordinal among its cell's inserted points (0 = oldest) */
+    uint8_t* ins_rec;     /* filter records (4 + nsq) of the inserted points */
+    float* ins_raw;       /* stored (normalised) features, whether_pq = false */
+    uint8_t* ins_pq;      /* second-stage PQ records (4 + pnsq), whether_pq = true */
+    uint32_t* ins_head;   /* C*F: newest inserted point of the cell, or PO_NONE */
+    uint32_t* ins_cnt;    /* C*F: inserted points of the cell */
+    uint32_t* ins_before; /* C*F: inserted points in the cells before (memory positions) */
+    float* fbound;        /* F: -sqrt(|f|^2/2), the assignment's fine bound */
 } po_index;
+
+#define PO_NONE 0xffffffffu
+
+/* Filter record, raw row, PQ record and local id of heap tag `tag` (a memory
+ * id below N, an inserted point N + j above). */
+static const uint8_t* po_frec(const po_index* ix, uint32_t tag) {
+    size_t rec = 4 + ix->nsq;
+    return tag < ix->N ? ix->records + (size_t)tag * rec : ix->ins_rec + (size_t)(tag - ix->N) * rec;
+}
+static const float* po_rawp(const po_index* ix, uint32_t tag) {
+    return tag < ix->N ? ix->raw + (size_t)tag * ix->dim : ix->ins_raw + (size_t)(tag - ix->N) * ix->dim;
+}
+static const uint8_t* po_pqrec(const po_index* ix, uint32_t tag) {
+    size_t rec = 4 + ix->pnsq;
+    return tag < ix->N ? ix->pq_records + (size_t)tag * rec : ix->ins_pq + (size_t)(tag - ix->N) * rec;
+}
+static uint32_t po_local(const po_index* ix, uint32_t tag) { return tag < ix->N ? ix->mem2local[tag] : tag; }
 
 typedef struct { uint32_t cell; float off; uint32_t local; } po_order;
 
@@ -372,6 +405,8 @@ void po_index_destroy(void* p) {
     free(ix->stationary); free(ix->cell_start); free(ix->min_dist_offset);
     free(ix->mem2local); free(ix->records); free(ix->raw);
     free(ix->pq_codebook); free(ix->pq_records);
+    free(ix->ins_cell); free(ix->ins_next); free(ix->ins_ord); free(ix->ins_rec); free(ix->ins_raw);
+    free(ix->ins_pq); free(ix->ins_head); free(ix->ins_cnt); free(ix->ins_before); free(ix->fbound);
     free(ix);
 }
 
@@ -395,7 +430,7 @@ void po_index_export(void* p, float* coarse_norms, float* stationary, uint32_t* 
  * in place of the raw vectors. */
 int po_index_set_pq(void* p, uint32_t pnsq, const float* pq_codebook, const uint8_t* pq_records_local) {
     po_index* ix = (po_index*)p;
-    if (pnsq == 0 || ix->dim % pnsq != 0) return -1;
+    if (pnsq == 0 || ix->dim % pnsq != 0 || ix->ins_n) return -1;
     uint32_t plsq = ix->dim / pnsq;
     if (!(plsq == 1 || plsq == 2 || plsq == 4)) return -1;
     size_t rec = 4 + pnsq;
@@ -414,7 +449,7 @@ int po_index_set_pq(void* p, uint32_t pnsq, const float* pq_codebook, const uint
  * (fd - |q|^2) + offset, then + the LUT sum of its PQ codes in the 4-lane
  * order of lookup_dist_table (:75-113). */
 static float po_pq_dist(const po_index* ix, uint32_t mid, float fd, float qn, const float* plut) {
-    const uint8_t* r = ix->pq_records + (size_t)mid * (4 + ix->pnsq);
+    const uint8_t* r = po_pqrec(ix, mid);
     float off;
     memcpy(&off, r, 4);
     float t = fd - qn + off;
@@ -476,15 +511,26 @@ static int po_u64_cmp(const void* a, const void* b) {
     return x < y ? -1 : (x > y);
 }
 
+/* A budget-scan candidate: key (filter distance, memory position), heap tag. */
+typedef struct { uint64_t key; uint32_t tag; } po_cand;
+static int po_cand_cmp(const void* a, const void* b) {
+    uint64_t x = ((const po_cand*)a)->key, y = ((const po_cand*)b)->key;
+    return x < y ? -1 : (x > y);
+}
+
 /* Budget scan (no counterpart in the reference; the learned probe's fine
  * budget).  q is the prepared query.
  *   coarse lists: the scc smallest |c|^2/2 - <c,q> by (distance, id);
  *   cells of a list: the b smallest cell distances (cd + stationary) + fine
  *     distance, compared as order-preserving keys, ties at the b-th included;
- *   filter set: the ftk smallest (filter distance, memory id) over every
- *     point of those cells, the filter distance computed as the reference's
- *     compute_quantized_distance does (puck_index.cpp:346-390) without its
- *     early exits (which drop only points that cannot be among the smallest);
+ *   filter set: the ftk smallest (filter distance, memory position) over
+ *     every point of those cells, the filter distance computed as the
+ *     reference's compute_quantized_distance does (puck_index.cpp:346-390)
+ *     without its early exits (which drop only points that cannot be among
+ *     the smallest).  The memory position is the memory id while nothing is
+ *     inserted; with inserted points it is the position in a layout where
+ *     every cell holds its loaded points, then its inserted points newest
+ *     first;
  *   result: the topk smallest (exact distance, local id) of the filter set.
  * filt_val / filt_tag receive the filter set ascending by (distance, id). */
 static int po_search_budget(const po_index* ix, const float* q, uint32_t topk, uint32_t scc, uint32_t ftk,
@@ -506,7 +552,7 @@ static int po_search_budget(const po_index* ix, const float* q, uint32_t topk, u
     uint32_t* keys = (uint32_t*)malloc(sizeof(uint32_t) * F);
     uint32_t* sorted = (uint32_t*)malloc(sizeof(uint32_t) * F);
     size_t cap = 1024, n = 0;
-    uint64_t* cand = (uint64_t*)malloc(sizeof(uint64_t) * cap);
+    po_cand* cand = (po_cand*)malloc(sizeof(po_cand) * cap);
     po_stats s = {0, 0, 0, 0};
     for (uint32_t l = 0; l < scc; ++l) {
         uint32_t c = (uint32_t)(ck[l] & 0xffffffffu);
@@ -521,23 +567,36 @@ static int po_search_budget(const po_index* ix, const float* q, uint32_t topk, u
             if (keys[k] > kcut) continue;
             s.cells_scanned++;
             float cell_dist = cd + cst[k] + fd[k];
-            uint32_t b0 = ix->cell_start[(size_t)c * F + k], e0 = ix->cell_start[(size_t)c * F + k + 1];
-            for (uint32_t i = b0; i < e0; ++i) {
-                const uint8_t* r = ix->records + (size_t)i * rec;
+            const size_t cell = (size_t)c * F + k;
+            uint32_t b0 = ix->cell_start[cell], e0 = ix->cell_start[cell + 1];
+            /* the cell's loaded points, then its inserted points newest first;
+             * a point's memory position counts the inserted points of the
+             * cells before it and, for an inserted point, the cell's loaded
+             * points and newer inserted points */
+            const uint32_t shift = ix->ins_before ? ix->ins_before[cell] : 0u;
+            uint32_t j = ix->ins_head ? ix->ins_head[cell] : PO_NONE;
+            for (uint32_t i = b0, slot = 0;; ++slot) {
+                uint32_t tag, pos;
+                if (i < e0) { tag = i; pos = i + shift; ++i; }
+                else if (j != PO_NONE) { tag = ix->N + j; pos = e0 + shift + (slot - (e0 - b0)); j = ix->ins_next[j]; }
+                else break;
+                const uint8_t* r = po_frec(ix, tag);
                 float off;
                 memcpy(&off, r, 4);
                 float t = (float)(2.0 * (double)cell_dist + (double)off);
                 t = t + po_lookup(r + 4, lut, nsq);
                 if (n == cap) {
                     cap *= 2;
-                    cand = (uint64_t*)realloc(cand, sizeof(uint64_t) * cap);
+                    cand = (po_cand*)realloc(cand, sizeof(po_cand) * cap);
                 }
-                cand[n++] = ((uint64_t)po_f2ord(t) << 32) | i;
+                cand[n].key = ((uint64_t)po_f2ord(t) << 32) | pos;
+                cand[n].tag = tag;
+                ++n;
                 s.points_offered++;
             }
         }
     }
-    qsort(cand, n, sizeof(uint64_t), po_u64_cmp);
+    qsort(cand, n, sizeof(po_cand), po_cand_cmp);
     uint32_t nf = n < ftk ? (uint32_t)n : ftk;
     uint64_t* rk = (uint64_t*)malloc(sizeof(uint64_t) * (nf ? nf : 1));
     float* plut = NULL;
@@ -548,15 +607,15 @@ static int po_search_budget(const po_index* ix, const float* q, uint32_t topk, u
         qn = po_dot(q, q, dim);
     }
     for (uint32_t i = 0; i < nf; ++i) {
-        uint32_t mid = (uint32_t)(cand[i] & 0xffffffffu);
-        uint32_t tb = (uint32_t)(cand[i] >> 32);
+        uint32_t mid = cand[i].tag;
+        uint32_t tb = (uint32_t)(cand[i].key >> 32);
         tb = (tb & 0x80000000u) ? (tb & 0x7fffffffu) : ~tb;
         float fv;
         memcpy(&fv, &tb, 4);
         if (filt_val) filt_val[i] = fv;
         if (filt_tag) filt_tag[i] = mid;
-        float d = plut ? po_pq_dist(ix, mid, fv, qn, plut) : po_rerank_dist(ix->raw + (size_t)mid * dim, q, dim);
-        rk[i] = ((uint64_t)po_f2ord(d) << 32) | ix->mem2local[mid];
+        float d = plut ? po_pq_dist(ix, mid, fv, qn, plut) : po_rerank_dist(po_rawp(ix, mid), q, dim);
+        rk[i] = ((uint64_t)po_f2ord(d) << 32) | po_local(ix, mid);
     }
     free(plut);
     if (filt_n) *filt_n = nf;
@@ -738,6 +797,25 @@ static void po_walk_query(const po_index* ix, const float* q, uint32_t topk, uin
                     s.heap_updates++;
                 }
             }
+            /* the cell's inserted points, newest first, with no early exit;
+             * each one counts towards the returned count, updated or not
+             * (RealtimeInsertPuckIndex::compute_quantized_distance,
+             * realtime_insert_puck_index.cpp:398-453) */
+            for (uint32_t j = ix->ins_head ? ix->ins_head[(size_t)c * F + k] : PO_NONE; j != PO_NONE;
+                 j = ix->ins_next[j]) {
+                const uint8_t* r = ix->ins_rec + (size_t)j * rec;
+                float off;
+                memcpy(&off, r, 4);
+                float t = (float)(2.0 * (double)cell_dist + (double)off);
+                s.points_offered++;
+                ++offered;
+                ++updated;
+                t = t + po_lookup(r + 4, lut, nsq);
+                if (t < fval[0]) {
+                    po_heap_update(&fh, t, ix->N + j);
+                    s.heap_updates++;
+                }
+            }
             if (updated > 0) pivot = PO_PIVOT(fval[0], qn, radius_rate);
 #ifndef PO_MUT_NO_MAXST_UPDATE
             float cand = pivot - cd - fine_d[idx];
@@ -767,9 +845,10 @@ static void po_walk_query(const po_index* ix, const float* q, uint32_t topk, uin
             po_lut(q, ix->pq_codebook, ix->pnsq, ix->plsq, plut);
         }
         for (uint32_t i = 0; i < nf; ++i) {
-            float t = plut ? po_pq_dist(ix, ftag[i], fval[i], qn, plut)
-                           : po_rerank_dist(ix->raw + (size_t)ftag[i] * dim, q, dim);
-            if (t < out_dist[0]) po_heap_update(&rh, t, ix->mem2local[ftag[i]]);
+            /* (RealtimeInsertPuckIndex::rank_topN_points reads an inserted
+             * point's row from the insert memory, :455-541) */
+            float t = plut ? po_pq_dist(ix, ftag[i], fval[i], qn, plut) : po_rerank_dist(po_rawp(ix, ftag[i]), q, dim);
+            if (t < out_dist[0]) po_heap_update(&rh, t, po_local(ix, ftag[i]));
         }
         free(plut);
         po_heap_reorder(&rh);
@@ -1183,3 +1262,125 @@ int po_encode_filter(const float* coarse, const float* fine, uint32_t F, uint32_
     free(cbn);
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* RealtimeInsertPuckIndex::insert (realtime_insert_puck_index.cpp:273-345)
+ * for n points, one after another:
+ *   single_build (hierarchical_cluster_index.cpp:1455-1494): normalisation
+ *     (whether_norm; a norm below 1e-6 fails and takes no insert id), then
+ *     nearest_cell_assign (:1042-1085) over the index's live stationary
+ *     values -- an empty cell holds the sentinel and is not chosen while a
+ *     populated cell is in reach;
+ *   puck_single_assign (puck_index.cpp:938-988): filter codes (and the
+ *     second-stage PQ codes of a whether_pq index) and offsets, as the build;
+ *   the point goes to the front of its cell's list; a cell that had no
+ *     point takes the stationary value sdot(c, f) - sdot(f, f) (:326-338),
+ *     and min_dist_offset keeps its loaded value.
+ * ids_out[i]: the local id N + insert ordinal, or PO_NONE for a rejected
+ * point.  cells_out, rec_out [4 + nsq], pq_out [4 + pnsq] and x_out [dim]
+ * (any may be NULL) receive the accepted points in order.  Returns the number
+ * accepted. */
+int po_index_insert(void* p, uint32_t n, const float* x, uint32_t* ids_out, uint32_t* cells_out, uint8_t* rec_out,
+                    uint8_t* pq_out, float* x_out) {
+    po_index* ix = (po_index*)p;
+    const uint32_t dim = ix->dim, C = ix->C, F = ix->F, nsq = ix->nsq, pnsq = ix->pnsq;
+    const size_t cells = (size_t)C * F, rec = 4 + nsq, prec = 4 + pnsq;
+    if (!ix->ins_head) {
+        ix->ins_head = (uint32_t*)malloc(sizeof(uint32_t) * cells);
+        for (size_t i = 0; i < cells; ++i) ix->ins_head[i] = PO_NONE;
+        ix->ins_cnt = (uint32_t*)calloc(cells, sizeof(uint32_t));
+        ix->ins_before = (uint32_t*)calloc(cells, sizeof(uint32_t));
+        /* _fine_norms of init (:245-254): -sqrt(sdot(f, f) / 2) */
+        ix->fbound = (float*)malloc(sizeof(float) * F);
+        for (uint32_t f = 0; f < F; ++f) {
+            const float* ff = ix->fine + (size_t)f * dim;
+            ix->fbound[f] = 0 - sqrtf(po_dot(ff, ff, dim) / 2);
+        }
+    }
+    float* xq = (float*)malloc(sizeof(float) * dim);
+    float* cip = (float*)malloc(sizeof(float) * C);
+    float* fip = (float*)malloc(sizeof(float) * F);
+    po_pair* pairs = (po_pair*)malloc(sizeof(po_pair) * C);
+    uint32_t acc = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (ids_out) ids_out[i] = PO_NONE;
+        const float* xi = x + (size_t)i * dim;
+        if (ix->whether_norm) {
+            if (po_normalize(xi, dim, xq) != 0) continue;
+        } else {
+            memcpy(xq, xi, sizeof(float) * dim);
+        }
+        const float xn = po_dot(xq, xq, dim) / 2;
+        for (uint32_t c = 0; c < C; ++c) cip[c] = po_dot(ix->coarse + (size_t)c * dim, xq, dim);
+        for (uint32_t f = 0; f < F; ++f) fip[f] = po_dot(ix->fine + (size_t)f * dim, xq, dim);
+        uint32_t cell = PO_NONE;
+        po_assign_one(C, F, ix->coarse_norms, ix->stationary, ix->fbound, cip, fip, xn, pairs, &cell, NULL);
+        if (cell >= cells) continue;
+        if (ix->ins_n == ix->ins_cap) {
+            uint32_t cap = ix->ins_cap ? 2 * ix->ins_cap : 64;
+            ix->ins_cell = (uint32_t*)realloc(ix->ins_cell, sizeof(uint32_t) * cap);
+            ix->ins_next = (uint32_t*)realloc(ix->ins_next, sizeof(uint32_t) * cap);
+            ix->ins_ord = (uint32_t*)realloc(ix->ins_ord, sizeof(uint32_t) * cap);
+            ix->ins_rec = (uint8_t*)realloc(ix->ins_rec, rec * cap);
+            if (pnsq) ix->ins_pq = (uint8_t*)realloc(ix->ins_pq, prec * cap);
+            else ix->ins_raw = (float*)realloc(ix->ins_raw, sizeof(float) * dim * cap);
+            ix->ins_cap = cap;
+        }
+        const uint32_t j = ix->ins_n++;
+        if (pnsq)
+            po_encode_puck(ix->coarse, ix->fine, F, dim, ix->codebook, nsq, ix->pq_codebook, pnsq, xq, &cell, 1,
+                           ix->ins_rec + (size_t)j * rec, ix->ins_pq + (size_t)j * prec, 1);
+        else
+            po_encode_filter(ix->coarse, ix->fine, F, dim, ix->codebook, nsq, xq, &cell, 1,
+                             ix->ins_rec + (size_t)j * rec, 1);
+        if (!pnsq) memcpy(ix->ins_raw + (size_t)j * dim, xq, sizeof(float) * dim);
+        ix->ins_cell[j] = cell;
+        ix->ins_ord[j] = ix->ins_cnt[cell];
+        ix->ins_next[j] = ix->ins_head[cell];
+        ix->ins_head[cell] = j;
+        ix->ins_cnt[cell]++;
+        if (ix->cell_start[cell + 1] - ix->cell_start[cell] + ix->ins_cnt[cell] == 1) {
+            const float* cc = ix->coarse + (size_t)(cell / F) * dim;
+            const float* ff = ix->fine + (size_t)(cell % F) * dim;
+            float sv = po_dot(cc, ff, dim);
+            sv -= po_dot(ff, ff, dim);
+            ix->stationary[cell] = sv;
+        }
+        if (ids_out) ids_out[i] = ix->N + j;
+        if (cells_out) cells_out[acc] = cell;
+        if (rec_out) memcpy(rec_out + (size_t)acc * rec, ix->ins_rec + (size_t)j * rec, rec);
+        if (pq_out && pnsq) memcpy(pq_out + (size_t)acc * prec, ix->ins_pq + (size_t)j * prec, prec);
+        if (x_out) memcpy(x_out + (size_t)acc * dim, xq, sizeof(float) * dim);
+        ++acc;
+    }
+    uint32_t run = 0;
+    for (size_t c = 0; c < cells; ++c) {
+        ix->ins_before[c] = run;
+        run += ix->ins_cnt[c];
+    }
+    free(xq); free(cip); free(fip); free(pairs);
+    return (int)acc;
+}
+
+/* Memory position of heap tag `tag` in the layout where every cell holds its
+ * loaded points, then its inserted points newest first (the budget scan's
+ * tie order; the memory id itself while nothing is inserted). */
+uint32_t po_index_position(const void* p, uint32_t tag) {
+    const po_index* ix = (const po_index*)p;
+    if (!ix->ins_head) return tag;
+    if (tag >= ix->N) {
+        const uint32_t j = tag - ix->N, cell = ix->ins_cell[j];
+        return ix->cell_start[cell + 1] + ix->ins_before[cell] + (ix->ins_cnt[cell] - 1 - ix->ins_ord[j]);
+    }
+    /* the cell holding memory id tag: the last cell starting at or before it
+     * among the non-empty ones */
+    size_t lo = 0, hi = (size_t)ix->C * ix->F;
+    while (hi - lo > 1) {
+        size_t mid = (lo + hi) / 2;
+        if (ix->cell_start[mid] <= tag) lo = mid; else hi = mid;
+    }
+    while (lo > 0 && ix->cell_start[lo] == ix->cell_start[lo + 1]) --lo;
+    return tag + ix->ins_before[lo];
+}
+
+uint32_t po_index_inserted(const void* p) { return ((const po_index*)p)->ins_n; }

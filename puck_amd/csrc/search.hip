@@ -1751,8 +1751,8 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
             const float t = pk_pq_dist(a, plut, mid, fd, qn);
             uint64_t key = ~0ull;
             if (r < n) {
-                key = ((uint64_t)pk_f2ord(t) << 32) | (bud ? a.mem2local[mid] : r);
-                if (exact) dist[r] = t;
+                key = ((uint64_t)pk_f2ord(t) << 32) | r;
+                if (exact || bud) dist[r] = t;
             }
             if (!exact) keys[r] = key;
         }
@@ -1762,7 +1762,7 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
         const int src = (int)(threadIdx.x & 63u & ~7u);
         const uint32_t dim = a.dim, nfull = dim >> 4;
         for (uint32_t r0 = 0; r0 < n2; r0 += RP * ng) {
-            uint32_t rr[RP], mid[RP], lid[RP];
+            uint32_t rr[RP], mid[RP];
             const float* x[RP];
             float t[RP];
 #pragma unroll
@@ -1774,8 +1774,6 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
                 x[h] = a.raw + (uint64_t)mid[h] * dim;
                 t[h] = 0.0f;
             }
-#pragma unroll
-            for (int h = 0; h < RP; ++h) lid[h] = bud ? a.mem2local[mid[h]] : 0u;  // with the row loads
             for (uint32_t s0 = 0; s0 < nfull; s0 += SB) {
                 float v0[RP][SB], v1[RP][SB];
 #pragma unroll
@@ -1810,8 +1808,8 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
                 if (j == 0 && r < n2) {
                     uint64_t key = ~0ull;
                     if (r < n) {
-                        key = ((uint64_t)pk_f2ord(t[h]) << 32) | (bud ? lid[h] : r);
-                        if (exact) dist[r] = t[h];
+                        key = ((uint64_t)pk_f2ord(t[h]) << 32) | r;
+                        if (exact || bud) dist[r] = t[h];
                     }
                     if (!exact) keys[r] = key;
                 }
@@ -1853,23 +1851,48 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
         return;
     }
     // only the first min(n, topk + 1) entries of the sorted order are read
-    if (n > 1 && !pk_sorted_prefix(keys, n, n < topk + 1 ? n : topk + 1)) {
-        uint32_t m2 = 2;
-        while (m2 < n) m2 <<= 1;
-        for (uint32_t k = 2; k <= m2; k <<= 1)
-            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                for (uint32_t i = threadIdx.x; i < (m2 >> 1); i += blockDim.x) {
-                    uint32_t lo = 2 * i - (i & (j - 1));
-                    uint32_t hi = lo + j;
-                    bool up = (lo & k) == 0;
-                    uint64_t x = keys[lo], y = keys[hi];
-                    if ((x > y) == up) {
-                        keys[lo] = y;
-                        keys[hi] = x;
+    const uint32_t kp = n < topk + 1 ? n : topk + 1;
+    auto sort_prefix = [&]() {
+        if (n > 1 && !pk_sorted_prefix(keys, n, kp)) {
+            uint32_t m2 = 2;
+            while (m2 < n) m2 <<= 1;
+            for (uint32_t k = 2; k <= m2; k <<= 1)
+                for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                    for (uint32_t i = threadIdx.x; i < (m2 >> 1); i += blockDim.x) {
+                        uint32_t lo = 2 * i - (i & (j - 1));
+                        uint32_t hi = lo + j;
+                        bool up = (lo & k) == 0;
+                        uint64_t x = keys[lo], y = keys[hi];
+                        if ((x > y) == up) {
+                            keys[lo] = y;
+                            keys[hi] = x;
+                        }
                     }
+                    __syncthreads();
                 }
-                __syncthreads();
-            }
+        }
+    };
+    sort_prefix();
+    // budget scan: equal distances rank by local id.  The keys carry the
+    // filter rank, so only when two of the first kp sorted entries share a
+    // distance are the keys rebuilt with local ids (one id lookup per
+    // survivor, in the rare case only) and the prefix sorted again.
+    bool local_keys = false;
+    if (bud) {
+        __shared__ uint32_t btie;
+        if (threadIdx.x == 0) btie = 0u;
+        __syncthreads();
+        for (uint32_t r = threadIdx.x; r + 1 < kp; r += blockDim.x)
+            if ((uint32_t)(keys[r] >> 32) == (uint32_t)(keys[r + 1] >> 32)) btie = 1u;
+        __syncthreads();
+        if (btie) {
+            for (uint32_t r = threadIdx.x; r < n2; r += blockDim.x)
+                keys[r] = r < n ? ((uint64_t)pk_f2ord(dist[r]) << 32) | a.mem2local[a.filt_id[(uint64_t)qi * ftk + r]]
+                                : ~0ull;
+            __syncthreads();
+            sort_prefix();
+            local_keys = true;
+        }
     }
     const uint32_t kout = n < topk ? n : topk;
     bool tie = false;
@@ -1880,7 +1903,7 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
             uint64_t key = keys[r];
             d = pk_ord2f((uint32_t)(key >> 32));
             uint32_t rr = (uint32_t)(key & 0xffffffffu);
-            id = bud ? rr : a.mem2local[a.filt_id[(uint64_t)qi * ftk + rr]];
+            id = local_keys ? rr : a.mem2local[a.filt_id[(uint64_t)qi * ftk + rr]];
             if (!bud && r + 1 < n && pk_ord2f((uint32_t)(keys[r + 1] >> 32)) == d) tie = true;
         }
         a.out_d[(uint64_t)qi * ostr + r] = d;

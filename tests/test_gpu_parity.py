@@ -305,6 +305,27 @@ def test_search_tied_centers_bitexact(ctx, idx_ties, scc, ftk, topk):
         assert np.array_equal(_bits(Dg[k, :o["n"]]), _bits(o["dist"][:o["n"]])), k
 
 
+@pytest.mark.parametrize("topk", [1, 10, 40])
+def test_budget_scan_exact_distance_ties_bitexact(ctx, idx_ties, topk):
+    # integer distances: many survivors share an exact distance, so the
+    # rerank's (distance, local id) order decides which ids are returned
+    ix, q = idx_ties
+    gi = _gpu_index(ctx, ix)
+    oi = CI.oracle_index(ix)
+    n = q.shape[0]
+    nf = (np.arange(n) % 5 + 1).astype(np.uint32)
+    Dg, Ig, Rg = gi.search_batch(q, topk, 2, 200, fine_budget=nf)
+    Do, Io, Ro = oi.search_batch_pq(q, topk, 2, 200, fine_budget=nf)
+    assert np.array_equal(Rg, Ro)
+    ties = 0
+    for k in range(n):
+        m = int(Ro[k])
+        assert np.array_equal(Ig[k, :m], Io[k, :m]), k
+        assert np.array_equal(_bits(Dg[k, :m]), _bits(Do[k, :m])), k
+        ties += int(np.any(np.diff(Dg[k, :m]) == 0)) if m > 1 else 0
+    assert topk == 1 or ties > 0
+
+
 def test_large_batch_query_order(ctx, idx_cos):
     # batches of 2048 queries or more run the per-query kernels in an order
     # grouped by nearest coarse list: results equal the same queries in small
