@@ -1,6 +1,7 @@
 /*
  * puck_gpu.h -- C ABI of libpuckgpu.so, the MI355X (gfx950) engine for Puck's
- * batched search path (PuckIndex / HierarchicalClusterIndex, whether_pq=false).
+ * batched search path (PuckIndex / HierarchicalClusterIndex, whether_pq false
+ * or true, with RealtimeInsertPuckIndex's inserts).
  *
  * Plain C types only; no C++ exceptions cross this boundary.  Every call
  * returns 0 on success or a negative code (the reference's convention: 0 = OK,
@@ -31,6 +32,10 @@
  *                          puck/puck/puck_index.cpp:173-227)
  *   puckgpu_encode_pq      PuckIndex::puck_assign with quantizations = {filter,
  *                          pq} (puck/puck/puck_index.cpp:602-709)
+ *   puckgpu_index_insert   RealtimeInsertPuckIndex::insert and its search
+ *                          overrides (puck/puck/realtime_insert_puck_index.cpp:
+ *                          273-345, :398-541), the file appends left to the
+ *                          caller
  * Entry points without a counterpart in the reference (each says so below):
  * the learned probe (puckgpu_probe_*, puckgpu_search_learned,
  * puckgpu_budget_ranks), the batched server (puckgpu_server_*), the
@@ -114,6 +119,32 @@ void puckgpu_index_free(puckgpu_index* index);
  * raw vectors. */
 int puckgpu_index_attach_pq(puckgpu_ctx* ctx, puckgpu_index* index, uint32_t pq_nsq, const float* pq_codebook,
                             const uint8_t* pq_records, int device_ptrs);
+/* Realtime insert of n points [n x dim], in order (RealtimeInsertPuckIndex::
+ * insert, realtime_insert_puck_index.cpp:273-345): each is normalised
+ * (whether_norm; a norm below 1e-6 refuses the point, which takes no id),
+ * assigned to its nearest cell over the index's live stationary values
+ * (nearest_cell_assign: empty cells hold the sentinel), and encoded with the
+ * filter PQ (and the second-stage PQ of a whether_pq index) as the build
+ * does.  A cell that had no point takes the stationary value
+ * sdot(c, f) - sdot(f, f) (:326-338); min_dist_offset keeps its loaded value.
+ * The points are searchable when the call returns: every cell holds its
+ * loaded points, then its inserted points newest first, the order in which
+ * the reference offers them to the filter heap (:398-453); the rerank reads
+ * their stored features or PQ codes (:455-541).  Outputs (any may be NULL):
+ * ids_out [n] the local id given (the count of points before it) or
+ * 0xffffffff for a refused point; for the m accepted points, in order,
+ * cells_out [m], filter_records_out [m x (4 + filter_nsq)] (filter_data.dat
+ * rows), pq_records_out [m x (4 + pq_nsq)] (pq_data.dat rows) and
+ * features_out [m x dim] (the stored, normalised rows) -- what the reference
+ * appends to its index files.  The call rewrites the index's point arrays
+ * (each cell's range moves), so a batch costs one pass over them: insert in
+ * batches.  Servers created on the index capture their graphs again on their
+ * next call. */
+int puckgpu_index_insert(puckgpu_ctx* ctx, puckgpu_index* index, uint32_t n, const float* features,
+                         uint32_t* ids_out, uint32_t* cells_out, uint8_t* filter_records_out,
+                         uint8_t* pq_records_out, float* features_out, int device_ptrs);
+/* Number of layout changes (insert calls that added points) of the index. */
+uint64_t puckgpu_index_version(const puckgpu_index* index);
 /* Copy derived arrays to host buffers (any may be NULL): coarse_norms [C],
  * stationary [C*F], cell_start [C*F+1], min_dist_offset [C], mem2local [N]. */
 int puckgpu_index_export(puckgpu_ctx* ctx, const puckgpu_index* index, float* coarse_norms, float* stationary,

@@ -547,3 +547,225 @@ extern "C" hipError_t pk_launch_gather_f32(const float* src, uint32_t width, flo
                        total);
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// Realtime insert (RealtimeInsertPuckIndex, realtime_insert_puck_index.cpp):
+// a batch of inserted points is merged into the memory layout, where each
+// cell holds its loaded points (ascending offset), then its inserted points
+// newest first -- the order in which the reference offers them to the filter
+// heap (:398-453).  Per cell: nbase loaded points, add[k] points of this
+// batch; shift[k] = exclusive prefix sum of add.
+
+// add[cell[i]] += 1
+__global__ void pk_cell_histogram(const uint32_t* __restrict__ cell, uint32_t n, uint32_t* __restrict__ add) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&add[cell[i]], 1u);
+}
+
+// out[k] = sum of in[0..k) for k in [0, n]: block sums, a one-block scan of
+// them, then each block's prefix.  1024 elements per block, at most 1024 * 64
+// blocks.
+#define PK_SCAN_B 1024u
+__device__ __forceinline__ uint32_t pk_block_exscan_1024(uint32_t v, uint32_t* sh, uint32_t* total) {
+    // Hillis-Steele over 1024 threads in LDS
+    const uint32_t t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (uint32_t o = 1; o < 1024u; o <<= 1) {
+        const uint32_t a = t >= o ? sh[t - o] : 0u;
+        __syncthreads();
+        sh[t] += a;
+        __syncthreads();
+    }
+    const uint32_t incl = sh[t];
+    if (total) *total = sh[1023];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(1024) void pk_scan_blocks(const uint32_t* __restrict__ in, uint64_t n,
+                                                       uint32_t* __restrict__ out, uint32_t* __restrict__ bsum) {
+    __shared__ uint32_t sh[1024];
+    const uint64_t i = (uint64_t)blockIdx.x * PK_SCAN_B + threadIdx.x;
+    const uint32_t v = i < n ? in[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = pk_block_exscan_1024(v, sh, &tot);
+    if (i <= n) out[i] = ex;
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void pk_scan_sums(uint32_t* __restrict__ bsum, uint32_t nb) {
+    __shared__ uint32_t sh[1024];
+    // 64 sums per thread, in order
+    constexpr uint32_t PER = 64;
+    uint32_t loc[PER];
+    uint32_t s = 0;
+    for (uint32_t j = 0; j < PER; ++j) {
+        const uint32_t i = threadIdx.x * PER + j;
+        loc[j] = i < nb ? bsum[i] : 0u;
+        s += loc[j];
+    }
+    uint32_t run = pk_block_exscan_1024(s, sh, nullptr);
+    for (uint32_t j = 0; j < PER; ++j) {
+        const uint32_t i = threadIdx.x * PER + j;
+        if (i < nb) bsum[i] = run;
+        run += loc[j];
+    }
+}
+
+__global__ __launch_bounds__(1024) void pk_scan_add(uint32_t* __restrict__ out, uint64_t n,
+                                                    const uint32_t* __restrict__ bsum) {
+    const uint64_t i = (uint64_t)blockIdx.x * PK_SCAN_B + threadIdx.x;
+    if (i <= n) out[i] += bsum[blockIdx.x];
+}
+
+// work: at least pk_exscan_work(n) words
+extern "C" uint64_t pk_exscan_work(uint64_t n) { return (n + 1 + PK_SCAN_B - 1) / PK_SCAN_B; }
+
+extern "C" hipError_t pk_launch_exscan(const uint32_t* in, uint64_t n, uint32_t* out, uint32_t* work, hipStream_t s) {
+    const uint64_t nb = pk_exscan_work(n);
+    if (nb > 1024ull * 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pk_scan_blocks, dim3((unsigned)nb), dim3(1024), 0, s, in, n, out, work);
+    hipLaunchKernelGGL(pk_scan_sums, dim3(1), dim3(1024), 0, s, work, (uint32_t)nb);
+    hipLaunchKernelGGL(pk_scan_add, dim3((unsigned)nb), dim3(1024), 0, s, out, n, work);
+    return hipGetLastError();
+}
+
+// Destination of every current memory position m in the merged layout: the
+// cell k holding m (last k with cs[k] <= m), its offset o in the cell; loaded
+// points move by shift[k], inserted ones also by add[k] (this batch goes in
+// front of them).
+__global__ void pk_insert_dest_old(const uint32_t* __restrict__ cs, uint32_t cells, const uint32_t* __restrict__ nbase,
+                                   const uint32_t* __restrict__ add, const uint32_t* __restrict__ shift, uint32_t n,
+                                   uint32_t* __restrict__ dest) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    uint32_t lo = 0, hi = cells;  // cs[lo] <= m < cs[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cs[mid] <= m) lo = mid; else hi = mid;
+    }
+    const uint32_t o = m - cs[lo];
+    dest[m] = m + shift[lo] + (o >= nbase[lo] ? add[lo] : 0u);
+}
+
+// Destination of batch point i (cell[i], rank[i] among the batch's points of
+// its cell, newest first): right after the cell's loaded points.
+__global__ void pk_insert_dest_new(const uint32_t* __restrict__ cell, const uint32_t* __restrict__ rank, uint32_t n,
+                                   const uint32_t* __restrict__ cs, const uint32_t* __restrict__ shift,
+                                   const uint32_t* __restrict__ nbase, uint32_t n_old, uint32_t* __restrict__ dest,
+                                   uint32_t* __restrict__ m2l_new) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = cell[i];
+    const uint32_t d = cs[k] + shift[k] + nbase[k] + rank[i];
+    dest[i] = d;
+    m2l_new[d] = n_old + i;  // local id: the count of points before it
+}
+
+// cs_new[k] = cs[k] + shift[k], k in [0, cells]
+__global__ void pk_insert_cell_start(const uint32_t* __restrict__ cs, const uint32_t* __restrict__ shift,
+                                     uint32_t cells, uint32_t* __restrict__ out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k <= cells) out[k] = cs[k] + shift[k];
+}
+
+// nbase[k] = cs[k + 1] - cs[k]
+__global__ void pk_cell_sizes(const uint32_t* __restrict__ cs, uint32_t cells, uint32_t* __restrict__ out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < cells) out[k] = cs[k + 1] - cs[k];
+}
+
+// Row scatter: row i of src (stride src_stride, from byte src_off, `bytes`
+// long) to row dest[i] of dst (stride dst_stride).  4-byte words when every
+// offset and length is a multiple of 4, else bytes; grid-stride.
+__global__ void pk_scatter_rows_w(const uint32_t* __restrict__ src, uint64_t src_stride, uint64_t src_off,
+                                  uint32_t* __restrict__ dst, uint64_t dst_stride, uint32_t words,
+                                  const uint32_t* __restrict__ dest, uint64_t total) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total; gid += step) {
+        const uint64_t i = gid / words;
+        const uint32_t w = (uint32_t)(gid % words);
+        dst[(uint64_t)dest[i] * dst_stride + w] = src[i * src_stride + src_off + w];
+    }
+}
+
+__global__ void pk_scatter_rows_b(const uint8_t* __restrict__ src, uint64_t src_stride, uint64_t src_off,
+                                  uint8_t* __restrict__ dst, uint64_t dst_stride, uint32_t bytes,
+                                  const uint32_t* __restrict__ dest, uint64_t total) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total; gid += step) {
+        const uint64_t i = gid / bytes;
+        const uint32_t b = (uint32_t)(gid % bytes);
+        dst[(uint64_t)dest[i] * dst_stride + b] = src[i * src_stride + src_off + b];
+    }
+}
+
+extern "C" hipError_t pk_launch_scatter_rows(const void* src, uint64_t src_stride, uint64_t src_off, void* dst,
+                                             uint64_t dst_stride, uint32_t bytes, const uint32_t* dest, uint32_t n,
+                                             hipStream_t s) {
+    if (n == 0 || bytes == 0) return hipSuccess;
+    if (src_stride % 4 == 0 && src_off % 4 == 0 && dst_stride % 4 == 0 && bytes % 4 == 0 &&
+        ((uintptr_t)src & 3u) == 0 && ((uintptr_t)dst & 3u) == 0) {
+        const uint64_t total = (uint64_t)n * (bytes / 4);
+        hipLaunchKernelGGL(pk_scatter_rows_w, dim3(gather_blocks(total)), dim3(256), 0, s, (const uint32_t*)src,
+                           src_stride / 4, src_off / 4, (uint32_t*)dst, dst_stride / 4, bytes / 4, dest, total);
+    } else {
+        const uint64_t total = (uint64_t)n * bytes;
+        hipLaunchKernelGGL(pk_scatter_rows_b, dim3(gather_blocks(total)), dim3(256), 0, s, (const uint8_t*)src,
+                           src_stride, src_off, (uint8_t*)dst, dst_stride, bytes, dest, total);
+    }
+    return hipGetLastError();
+}
+
+// Stationary value of a cell that receives its first point by insert:
+// sdot(c, f) - sdot(f, f) (realtime_insert_puck_index.cpp:326-338).
+__global__ void pk_insert_stationary(const float* __restrict__ coarse, const float* __restrict__ fine, uint32_t F,
+                                     uint32_t dim, const uint32_t* __restrict__ cells, uint32_t n,
+                                     float* __restrict__ st) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = cells[i];
+    const float* c = coarse + (uint64_t)(k / F) * dim;
+    const float* f = fine + (uint64_t)(k % F) * dim;
+    float v = pk_dot(c, f, dim);
+    v -= pk_dot(f, f, dim);
+    st[k] = v;
+}
+
+extern "C" hipError_t pk_launch_insert_prepare(const uint32_t* cell, uint32_t n, uint32_t cells, uint32_t* add,
+                                               uint32_t* shift, uint32_t* work, hipStream_t s) {
+    hipError_t e = hipMemsetAsync(add, 0, sizeof(uint32_t) * ((uint64_t)cells + 1), s);
+    if (e != hipSuccess) return e;
+    if (n) hipLaunchKernelGGL(pk_cell_histogram, dim3((n + 255) / 256), dim3(256), 0, s, cell, n, add);
+    return pk_launch_exscan(add, cells, shift, work, s);
+}
+
+extern "C" hipError_t pk_launch_insert_dest(const uint32_t* cs, uint32_t cells, const uint32_t* nbase,
+                                            const uint32_t* add, const uint32_t* shift, uint32_t n_old,
+                                            uint32_t* dest_old, const uint32_t* cell, const uint32_t* rank,
+                                            uint32_t n_new, uint32_t* dest_new, uint32_t* m2l_new,
+                                            uint32_t* cs_new, hipStream_t s) {
+    if (n_old)
+        hipLaunchKernelGGL(pk_insert_dest_old, dim3((n_old + 255) / 256), dim3(256), 0, s, cs, cells, nbase, add,
+                           shift, n_old, dest_old);
+    if (n_new)
+        hipLaunchKernelGGL(pk_insert_dest_new, dim3((n_new + 255) / 256), dim3(256), 0, s, cell, rank, n_new, cs,
+                           shift, nbase, n_old, dest_new, m2l_new);
+    hipLaunchKernelGGL(pk_insert_cell_start, dim3((cells + 1 + 255) / 256), dim3(256), 0, s, cs, shift, cells,
+                       cs_new);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t pk_launch_cell_sizes(const uint32_t* cs, uint32_t cells, uint32_t* out, hipStream_t s) {
+    hipLaunchKernelGGL(pk_cell_sizes, dim3((cells + 255) / 256), dim3(256), 0, s, cs, cells, out);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t pk_launch_insert_stationary(const float* coarse, const float* fine, uint32_t F, uint32_t dim,
+                                                  const uint32_t* cells, uint32_t n, float* st, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(pk_insert_stationary, dim3((n + 255) / 256), dim3(256), 0, s, coarse, fine, F, dim, cells, n,
+                       st);
+    return hipGetLastError();
+}

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <string>
 #include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 // ---- kernel launchers (defined in gemm/sort/search/build.hip)
@@ -81,6 +82,16 @@ hipError_t pk_launch_finalize_cells(float*, const uint32_t*, uint32_t, uint32_t,
 hipError_t pk_launch_gather_rows(const uint8_t*, uint64_t, uint64_t, uint8_t*, uint64_t, uint32_t,
                                  const uint32_t*, uint32_t, hipStream_t);
 hipError_t pk_launch_gather_f32(const float*, uint32_t, float*, const uint32_t*, uint32_t, hipStream_t);
+uint64_t pk_exscan_work(uint64_t);
+hipError_t pk_launch_insert_prepare(const uint32_t*, uint32_t, uint32_t, uint32_t*, uint32_t*, uint32_t*, hipStream_t);
+hipError_t pk_launch_insert_dest(const uint32_t*, uint32_t, const uint32_t*, const uint32_t*, const uint32_t*, uint32_t,
+                                 uint32_t*, const uint32_t*, const uint32_t*, uint32_t, uint32_t*, uint32_t*, uint32_t*,
+                                 hipStream_t);
+hipError_t pk_launch_cell_sizes(const uint32_t*, uint32_t, uint32_t*, hipStream_t);
+hipError_t pk_launch_insert_stationary(const float*, const float*, uint32_t, uint32_t, const uint32_t*, uint32_t, float*,
+                                       hipStream_t);
+hipError_t pk_launch_scatter_rows(const void*, uint64_t, uint64_t, void*, uint64_t, uint32_t, const uint32_t*, uint32_t,
+                                  hipStream_t);
 }
 
 enum { PK_EPI_RAW = 0, PK_EPI_SUB_FROM_BIAS = 1, PK_EPI_NEG = 2, PK_EPI_ADD_BIAS = 3 };
@@ -138,6 +149,14 @@ struct puckgpu_index {
     mutable uint8_t* fine8 = nullptr;
     mutable float* coarse8s = nullptr;
     mutable float* fine8s = nullptr;
+    // realtime insert (puckgpu_index_insert), set up by the first insert:
+    // the loaded points per cell, the host count of points per cell, the
+    // assignment's fine bound; version counts the layout changes
+    uint32_t* nbase = nullptr;
+    std::vector<uint32_t> h_cnt;
+    float* fbound = nullptr;
+    int fb_sorted = 1;
+    uint64_t version = 0;
 };
 
 struct puckgpu_probe {
@@ -219,7 +238,8 @@ enum Slot {
     S_Q = 0, S_QN, S_QST, S_CD, S_FD, S_CSD, S_CSI, S_FSD, S_FSI, S_FT, S_FI, S_FN, S_OD, S_OI, S_ON, S_STATS,
     S_IN_Q, S_A0, S_A1, S_A2, S_A3, S_A4, S_A5, S_A6, S_A7, S_FLAGS, S_QLIST, S_TIEID, S_TIEN, S_QTOPK, S_QSCC,
     S_QNF, S_PFEAT, S_PWORK, S_PNF, S_PY, S_PT, S_PCV, S_LP16, S_LP8, S_LPS, S_CAND_D, S_CAND_I, S_WLIST, S_SCI,
-    S_SCK, S_SCM, S_COUNT
+    S_SCK, S_SCM, S_INS_X, S_INS_Y, S_INS_CELL, S_INS_REC, S_INS_PQ, S_INS_RANK, S_INS_ADD, S_INS_SHIFT, S_INS_WORK,
+    S_INS_DOLD, S_INS_DNEW, S_INS_ACC, S_COUNT
 };
 
 // Device view of an input array: the pointer itself, or a scratch copy of it.
@@ -246,7 +266,8 @@ void index_release(puckgpu_index* ix) {
     if (!ix) return;
     void* ptrs[] = {ix->coarse, ix->fine, ix->codebook, ix->coarse_norms, ix->stationary, ix->cell_start,
                     ix->mdo, ix->mem2local, ix->rec_off, ix->rec_codes, ix->raw, ix->coarse16, ix->fine16,
-                    ix->coarse8, ix->fine8, ix->coarse8s, ix->fine8s, ix->pq_codebook, ix->pq_off, ix->pq_codes};
+                    ix->coarse8, ix->fine8, ix->coarse8s, ix->fine8s, ix->pq_codebook, ix->pq_off, ix->pq_codes,
+                    ix->nbase, ix->fbound};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete ix;
@@ -1390,6 +1411,7 @@ struct PkBucket {
     std::vector<void*> allocs;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    uint64_t version = 0;  // the index layout the graph was captured on
     PkWalkArgs wa{};
     PkRerankArgs ra{};
 };
@@ -1496,6 +1518,7 @@ int bucket_capture(puckgpu_server* sv, PkBucket* bk, const PassCfg& cfg) {
     if (e != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "graph capture failed: %s", hipGetErrorString(e));
     bk->graph = g;
     PK_CHECK(hipGraphInstantiate(&bk->exec, g, nullptr, nullptr, 0));
+    bk->version = sv->ix->version;
     return PUCKGPU_OK;
 }
 
@@ -1507,6 +1530,59 @@ int server_cfg(puckgpu_server* sv, PassCfg& cfg, ProbeUse& pu) {
                             cfg);
     cfg.budget = sv->probe != nullptr;
     return rc;
+}
+
+}  // namespace
+
+namespace {
+
+#define PK_ASSIGN_CHUNK 32768u
+
+// Is the fine bound -sqrt(|f|^2/2) non-decreasing (fine centers stored by
+// descending norm)?  The assignment kernel may then stop a list early.
+int fine_bound_sorted(puckgpu_ctx* ctx, const float* fb, uint32_t F, int* sorted) {
+    std::vector<float> fb_h(F);
+    PK_CHECK(hipMemcpyAsync(fb_h.data(), fb, 4ull * F, hipMemcpyDeviceToHost, ctx->stream));
+    PK_CHECK(hipStreamSynchronize(ctx->stream));
+    *sorted = 1;
+    for (uint32_t f = 1; f < F; ++f)
+        if (!(fb_h[f - 1] <= fb_h[f])) *sorted = 0;
+    return PUCKGPU_OK;
+}
+
+// nearest_cell_assign (hierarchical_cluster_index.cpp:1042-1085) of m <=
+// PK_ASSIGN_CHUNK points on the device, over the coarse half-norms cn, the
+// stationary values st and the fine bound fb; cell_out / dist_out (may be
+// NULL) on the device.
+int assign_chunk(puckgpu_ctx* ctx, const float* cd, const float* fd, const float* cn, const float* st, const float* fb,
+                 int fb_sorted, uint32_t C, uint32_t F, uint32_t dim, uint32_t m, const float* xp, uint32_t* cell_out,
+                 float* dist_out, hipStream_t s) {
+    int rc = 0;
+    float* ipc = (float*)scratch_slot(ctx, S_A6, 4ull * m * C, &rc);
+    float* ipf = (float*)scratch_slot(ctx, S_A7, 4ull * m * F, &rc);
+    float* xn = (float*)scratch_slot(ctx, S_QN, 4ull * m, &rc);
+    uint32_t* ord_d = (uint32_t*)scratch_slot(ctx, S_QLIST, 4ull * m, &rc);
+    if (rc) return rc;
+    std::vector<uint32_t> ord_h(m), perm_h(m);
+    PK_CHECK(pk_launch_row_norms(xp, m, dim, 1, xn, s));
+    PK_CHECK(pk_launch_gemm_exact(xp, cd, ipc, m, C, dim, C, PK_EPI_RAW, nullptr, s));
+    PK_CHECK(pk_launch_gemm_exact(xp, fd, ipf, m, F, dim, F, PK_EPI_RAW, nullptr, s));
+    // visit points grouped by nearest coarse cluster (L2 reuse of the
+    // stationary rows); results are written by point index
+    PK_CHECK(pk_launch_nearest_coarse(ipc, xn, cn, m, C, ord_d, s));
+    PK_CHECK(hipMemcpyAsync(ord_h.data(), ord_d, 4ull * m, hipMemcpyDeviceToHost, s));
+    PK_CHECK(hipStreamSynchronize(s));
+    {
+        std::vector<uint32_t> cnt(C + 1, 0);
+        for (uint32_t i = 0; i < m; ++i) cnt[ord_h[i] + 1]++;
+        for (uint32_t c = 0; c < C; ++c) cnt[c + 1] += cnt[c];
+        for (uint32_t i = 0; i < m; ++i) perm_h[cnt[ord_h[i]]++] = i;
+    }
+    PK_CHECK(hipMemcpyAsync(ord_d, perm_h.data(), 4ull * m, hipMemcpyHostToDevice, s));
+    PK_CHECK(pk_launch_assign_cells(ipc, ipf, xn, cn, fb, st, m, C, F, fb_sorted, ord_d, cell_out, dist_out, s));
+    // the host vectors above must outlive the copies
+    PK_CHECK(hipStreamSynchronize(s));
+    return PUCKGPU_OK;
 }
 
 }  // namespace
@@ -1595,8 +1671,9 @@ int puckgpu_server_search(puckgpu_server* sv, uint32_t n, const float* queries, 
     }
     uint32_t nv = n;
     PK_CHECK(hipMemcpyAsync(bk->nvalid, &nv, sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    if (!bk->exec) {
-        // one eager pass first (sets kernel attributes outside any capture)
+    if (!bk->exec || bk->version != sv->ix->version) {
+        // one eager pass first (sets kernel attributes outside any capture);
+        // an insert since the capture moved the index arrays: capture again
         if ((rc = bucket_enqueue(sv, bk, cfg, s))) return rc;
         PK_CHECK(hipStreamSynchronize(s));
         if ((rc = bucket_capture(sv, bk, cfg))) return rc;
@@ -1783,24 +1860,14 @@ int puckgpu_assign_cells(puckgpu_ctx* ctx, const float* coarse, const float* fin
     PK_CHECK(pk_launch_row_norms(fd, F, dim, 1, fnorm, s));
     PK_CHECK(pk_launch_fine_bound(fd, F, dim, fb, s));
     PK_CHECK(pk_launch_gemm_exact(cd, fd, st, C, F, dim, F, PK_EPI_ADD_BIAS, fnorm, s));
-    const uint32_t CHK = 32768;
+    int fb_sorted = 1;
+    if ((rc = fine_bound_sorted(ctx, fb, F, &fb_sorted))) return rc;
+    const uint32_t CHK = PK_ASSIGN_CHUNK;
     const uint32_t cb = n < CHK ? n : CHK;
-    float* ipc = (float*)scratch_slot(ctx, S_A6, 4ull * cb * C, &rc);
-    float* ipf = (float*)scratch_slot(ctx, S_A7, 4ull * cb * F, &rc);
-    float* xn = (float*)scratch_slot(ctx, S_QN, 4ull * cb, &rc);
     float* xs = (float*)scratch_slot(ctx, S_Q, 4ull * cb * dim, &rc);
     uint32_t* co = (uint32_t*)scratch_slot(ctx, S_OI, 4ull * cb, &rc);
     float* cdo = (float*)scratch_slot(ctx, S_OD, 4ull * cb, &rc);
-    uint32_t* ord_d = (uint32_t*)scratch_slot(ctx, S_QLIST, 4ull * cb, &rc);
     if (rc) return rc;
-    std::vector<uint32_t> ord_h(cb), perm_h(cb);
-    // is fbound non-decreasing (fine centers stored by descending norm)?
-    std::vector<float> fb_h(F);
-    PK_CHECK(hipMemcpyAsync(fb_h.data(), fb, 4ull * F, hipMemcpyDeviceToHost, s));
-    PK_CHECK(hipStreamSynchronize(s));
-    int fb_sorted = 1;
-    for (uint32_t f = 1; f < F; ++f)
-        if (!(fb_h[f - 1] <= fb_h[f])) fb_sorted = 0;
     for (uint32_t p0 = 0; p0 < n; p0 += CHK) {
         const uint32_t m = (n - p0) < CHK ? (n - p0) : CHK;
         const float* xp;
@@ -1809,24 +1876,9 @@ int puckgpu_assign_cells(puckgpu_ctx* ctx, const float* coarse, const float* fin
             PK_CHECK(hipMemcpyAsync(xs, x + (size_t)p0 * dim, 4ull * m * dim, hipMemcpyHostToDevice, s));
             xp = xs;
         }
-        PK_CHECK(pk_launch_row_norms(xp, m, dim, 1, xn, s));
-        PK_CHECK(pk_launch_gemm_exact(xp, cd, ipc, m, C, dim, C, PK_EPI_RAW, nullptr, s));
-        PK_CHECK(pk_launch_gemm_exact(xp, fd, ipf, m, F, dim, F, PK_EPI_RAW, nullptr, s));
         uint32_t* cout = device_ptrs ? cell_out + p0 : co;
         float* dout = device_ptrs ? (dist_out ? dist_out + p0 : nullptr) : cdo;
-        // visit points grouped by nearest coarse cluster (L2 reuse of the
-        // stationary rows); results are written by point index
-        PK_CHECK(pk_launch_nearest_coarse(ipc, xn, cn, m, C, ord_d, s));
-        PK_CHECK(hipMemcpyAsync(ord_h.data(), ord_d, 4ull * m, hipMemcpyDeviceToHost, s));
-        PK_CHECK(hipStreamSynchronize(s));
-        {
-            std::vector<uint32_t> cnt(C + 1, 0);
-            for (uint32_t i = 0; i < m; ++i) cnt[ord_h[i] + 1]++;
-            for (uint32_t c = 0; c < C; ++c) cnt[c + 1] += cnt[c];
-            for (uint32_t i = 0; i < m; ++i) perm_h[cnt[ord_h[i]]++] = i;
-        }
-        PK_CHECK(hipMemcpyAsync(ord_d, perm_h.data(), 4ull * m, hipMemcpyHostToDevice, s));
-        PK_CHECK(pk_launch_assign_cells(ipc, ipf, xn, cn, fb, st, m, C, F, fb_sorted, ord_d, cout, dout, s));
+        if ((rc = assign_chunk(ctx, cd, fd, cn, st, fb, fb_sorted, C, F, dim, m, xp, cout, dout, s))) return rc;
         if (!device_ptrs) {
             PK_CHECK(hipMemcpyAsync(cell_out + p0, co, 4ull * m, hipMemcpyDeviceToHost, s));
             if (dist_out) PK_CHECK(hipMemcpyAsync(dist_out + p0, cdo, 4ull * m, hipMemcpyDeviceToHost, s));
@@ -1999,5 +2051,210 @@ int puckgpu_kmeans_assign(puckgpu_ctx* ctx, uint32_t dim, const float* centroids
     PK_CHECK(hipStreamSynchronize(s));
     return PUCKGPU_OK;
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Realtime insert: RealtimeInsertPuckIndex::insert
+// (realtime_insert_puck_index.cpp:273-345) for a batch, merged into the
+// memory layout (each cell: loaded points, then inserted points newest first).
+namespace {
+
+// first insert: loaded points per cell, host counts, fine bound
+int insert_setup(puckgpu_ctx* ctx, puckgpu_index* ix) {
+    if (ix->nbase) return PUCKGPU_OK;
+    hipStream_t s = ctx->stream;
+    const uint32_t F = ix->p.fine_cluster_count, cells = ix->p.coarse_cluster_count * F;
+    int rc = 0;
+    if ((rc = dalloc(ctx, &ix->nbase, cells)) || (rc = dalloc(ctx, &ix->fbound, F))) return rc;
+    PK_CHECK(pk_launch_cell_sizes(ix->cell_start, cells, ix->nbase, s));
+    PK_CHECK(pk_launch_fine_bound(ix->fine, F, ix->p.dim, ix->fbound, s));
+    ix->h_cnt.assign(cells, 0u);
+    PK_CHECK(hipMemcpyAsync(ix->h_cnt.data(), ix->nbase, 4ull * cells, hipMemcpyDeviceToHost, s));
+    PK_CHECK(hipStreamSynchronize(s));
+    return fine_bound_sorted(ctx, ix->fbound, F, &ix->fb_sorted);
+}
+
+// out (host or device) <- device src
+int copy_out(puckgpu_ctx* ctx, void* out, const void* src, size_t bytes, int device_ptrs) {
+    if (!out || bytes == 0) return PUCKGPU_OK;
+    PK_CHECK(hipMemcpyAsync(out, src, bytes, device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                            ctx->stream));
+    return PUCKGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int puckgpu_index_insert(puckgpu_ctx* ctx, puckgpu_index* ix, uint32_t n, const float* features, uint32_t* ids_out,
+                         uint32_t* cells_out, uint8_t* filter_records_out, uint8_t* pq_records_out,
+                         float* features_out, int device_ptrs) {
+    if (!ctx || !ix || (n && !features)) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    if (!ix->raw && !ix->pq_codes)
+        return set_err(ctx, PUCKGPU_EINVAL, "the index holds neither raw vectors nor a second-stage PQ");
+    if (n == 0) return PUCKGPU_OK;
+    const uint32_t d = ix->p.dim, C = ix->p.coarse_cluster_count, F = ix->p.fine_cluster_count, nsq = ix->p.filter_nsq,
+                   pnsq = ix->pnsq, N0 = ix->p.total_point_count, cells = C * F;
+    if ((uint64_t)N0 + n >= 0xffffffffull) return set_err(ctx, PUCKGPU_EINVAL, "index would exceed 2^32 - 1 points");
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    hipStream_t s = ctx->stream;
+    int rc = 0;
+    if ((rc = insert_setup(ctx, ix))) return rc;
+    const size_t rec = 4 + nsq, prec = 4 + pnsq;
+
+    // single_build (hierarchical_cluster_index.cpp:1455-1494): normalisation;
+    // a zero-norm point is refused and takes no id
+    const float* xd = (const float*)dev_in(ctx, S_INS_X, features, 4ull * n * d, device_ptrs, &rc);
+    float* y = (float*)scratch_slot(ctx, S_INS_Y, 4ull * n * d, &rc);
+    float* yn = (float*)scratch_slot(ctx, S_QN, 4ull * n, &rc);
+    int32_t* st = (int32_t*)scratch_slot(ctx, S_QST, 4ull * n, &rc);
+    if (rc) return rc;
+    PK_CHECK(pk_launch_prep_queries(xd, n, d, ix->p.whether_norm, y, yn, st, s));
+    std::vector<int32_t> st_h(n);
+    PK_CHECK(hipMemcpyAsync(st_h.data(), st, 4ull * n, hipMemcpyDeviceToHost, s));
+    PK_CHECK(hipStreamSynchronize(s));
+    std::vector<uint32_t> acc;
+    for (uint32_t i = 0; i < n; ++i)
+        if (st_h[i] == 0) acc.push_back(i);
+    const uint32_t m = (uint32_t)acc.size();
+    std::vector<uint32_t> ids(n, 0xffffffffu);
+    for (uint32_t a = 0; a < m; ++a) ids[acc[a]] = N0 + a;
+    float* ya = y;
+    if (m < n && m > 0) {
+        uint32_t* accd = (uint32_t*)scratch_slot(ctx, S_INS_ACC, 4ull * m, &rc);
+        ya = (float*)scratch_slot(ctx, S_INS_X, 4ull * m * d, &rc);  // the input copy (if any) is no longer read
+        if (rc) return rc;
+        PK_CHECK(hipMemcpyAsync(accd, acc.data(), 4ull * m, hipMemcpyHostToDevice, s));
+        PK_CHECK(pk_launch_gather_f32(y, d, ya, accd, m, s));
+        PK_CHECK(hipStreamSynchronize(s));
+    }
+    if (m > 0) {
+        // nearest_cell_assign over the live stationary values, one point after
+        // another as far as it matters: a point that lands in an empty cell
+        // gives that cell its stationary value (:326-338) before the later
+        // points are assigned again
+        uint32_t* cell = (uint32_t*)scratch_slot(ctx, S_INS_CELL, 4ull * m, &rc);
+        if (rc) return rc;
+        std::vector<uint32_t> hcell(m);
+        uint32_t done = 0;
+        while (done < m) {
+            for (uint32_t p0 = done; p0 < m; p0 += PK_ASSIGN_CHUNK) {
+                const uint32_t k = (m - p0) < PK_ASSIGN_CHUNK ? (m - p0) : PK_ASSIGN_CHUNK;
+                if ((rc = assign_chunk(ctx, ix->coarse, ix->fine, ix->coarse_norms, ix->stationary, ix->fbound,
+                                       ix->fb_sorted, C, F, d, k, ya + (size_t)p0 * d, cell + p0, nullptr, s)))
+                    return rc;
+            }
+            PK_CHECK(hipMemcpyAsync(hcell.data() + done, cell + done, 4ull * (m - done), hipMemcpyDeviceToHost, s));
+            PK_CHECK(hipStreamSynchronize(s));
+            uint32_t next = m;
+            for (uint32_t i = done; i < m; ++i) {
+                if (hcell[i] >= cells) return set_err(ctx, PUCKGPU_EHIP, "assignment returned cell %u", hcell[i]);
+                if (ix->h_cnt[hcell[i]]++ == 0) {
+                    PK_CHECK(pk_launch_insert_stationary(ix->coarse, ix->fine, F, d, cell + i, 1, ix->stationary, s));
+                    next = i + 1;
+                    break;
+                }
+            }
+            done = next;
+        }
+        // puck_single_assign (puck_index.cpp:938-988): filter codes, then the
+        // second-stage PQ codes on what the filter codewords leave
+        uint8_t* recs = (uint8_t*)scratch_slot(ctx, S_INS_REC, rec * m, &rc);
+        uint8_t* precs = pnsq ? (uint8_t*)scratch_slot(ctx, S_INS_PQ, prec * m, &rc) : nullptr;
+        if (rc) return rc;
+        PK_CHECK(pk_launch_encode_filter(ya, cell, m, ix->coarse, ix->fine, F, d, ix->codebook, nsq, nullptr, recs, rec,
+                                         4, s));
+        if (pnsq)
+            PK_CHECK(pk_launch_encode_pq(ya, cell, m, ix->coarse, ix->fine, F, d, ix->codebook, nsq, recs + 4, rec,
+                                         ix->pq_codebook, pnsq, precs, s));
+        // rank of each point among the batch's points of its cell, newest first
+        std::vector<uint32_t> rank(m);
+        {
+            std::unordered_map<uint32_t, uint32_t> seen;
+            for (uint32_t i = m; i-- > 0;) rank[i] = seen[hcell[i]]++;
+        }
+        uint32_t* rank_d = (uint32_t*)scratch_slot(ctx, S_INS_RANK, 4ull * m, &rc);
+        uint32_t* add = (uint32_t*)scratch_slot(ctx, S_INS_ADD, 4ull * (cells + 1), &rc);
+        uint32_t* shift = (uint32_t*)scratch_slot(ctx, S_INS_SHIFT, 4ull * (cells + 1), &rc);
+        uint32_t* work = (uint32_t*)scratch_slot(ctx, S_INS_WORK, 4ull * pk_exscan_work(cells), &rc);
+        uint32_t* dold = (uint32_t*)scratch_slot(ctx, S_INS_DOLD, 4ull * (N0 ? N0 : 1), &rc);
+        uint32_t* dnew = (uint32_t*)scratch_slot(ctx, S_INS_DNEW, 4ull * m, &rc);
+        if (rc) return rc;
+        PK_CHECK(hipMemcpyAsync(rank_d, rank.data(), 4ull * m, hipMemcpyHostToDevice, s));
+        PK_CHECK(pk_launch_insert_prepare(cell, m, cells, add, shift, work, s));
+        // the merged arrays
+        const uint32_t N1 = N0 + m;
+        uint32_t *cs1 = nullptr, *m2l1 = nullptr;
+        float *off1 = nullptr, *raw1 = nullptr, *poff1 = nullptr;
+        uint8_t *codes1 = nullptr, *pcodes1 = nullptr;
+        auto release = [&]() {
+            for (void* q : {(void*)cs1, (void*)m2l1, (void*)off1, (void*)raw1, (void*)poff1, (void*)codes1,
+                            (void*)pcodes1})
+                if (q) (void)hipFree(q);
+        };
+        if ((rc = dalloc(ctx, &cs1, (size_t)cells + 1)) || (rc = dalloc(ctx, &m2l1, N1)) ||
+            (rc = dalloc(ctx, &off1, N1)) || (rc = dalloc(ctx, &codes1, (size_t)N1 * nsq)) ||
+            (ix->raw && (rc = dalloc(ctx, &raw1, (size_t)N1 * d))) ||
+            (pnsq && ((rc = dalloc(ctx, &poff1, N1)) || (rc = dalloc(ctx, &pcodes1, (size_t)N1 * pnsq))))) {
+            release();
+            return rc;
+        }
+        hipError_t e = pk_launch_insert_dest(ix->cell_start, cells, ix->nbase, add, shift, N0, dold, cell, rank_d, m,
+                                             dnew, m2l1, cs1, s);
+        auto mv = [&](const void* src, uint64_t sstr, uint64_t soff, void* dst, uint64_t dstr, uint32_t bytes,
+                      const uint32_t* dest, uint32_t cnt) {
+            if (e == hipSuccess) e = pk_launch_scatter_rows(src, sstr, soff, dst, dstr, bytes, dest, cnt, s);
+        };
+        mv(ix->mem2local, 4, 0, m2l1, 4, 4, dold, N0);
+        mv(ix->rec_off, 4, 0, off1, 4, 4, dold, N0);
+        mv(ix->rec_codes, nsq, 0, codes1, nsq, nsq, dold, N0);
+        mv(recs, rec, 0, off1, 4, 4, dnew, m);
+        mv(recs, rec, 4, codes1, nsq, nsq, dnew, m);
+        if (ix->raw) {
+            mv(ix->raw, 4ull * d, 0, raw1, 4ull * d, 4 * d, dold, N0);
+            mv(ya, 4ull * d, 0, raw1, 4ull * d, 4 * d, dnew, m);
+        }
+        if (pnsq) {
+            mv(ix->pq_off, 4, 0, poff1, 4, 4, dold, N0);
+            mv(ix->pq_codes, pnsq, 0, pcodes1, pnsq, pnsq, dold, N0);
+            mv(precs, prec, 0, poff1, 4, 4, dnew, m);
+            mv(precs, prec, 4, pcodes1, pnsq, pnsq, dnew, m);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            release();
+            return set_err(ctx, PUCKGPU_EHIP, "merging inserted points failed: %s", hipGetErrorString(e));
+        }
+        for (void* q : {(void*)ix->cell_start, (void*)ix->mem2local, (void*)ix->rec_off, (void*)ix->rec_codes,
+                        (void*)ix->raw, (void*)ix->pq_off, (void*)ix->pq_codes})
+            if (q) (void)hipFree(q);
+        ix->cell_start = cs1;
+        ix->mem2local = m2l1;
+        ix->rec_off = off1;
+        ix->rec_codes = codes1;
+        if (ix->raw) ix->raw = raw1;
+        if (pnsq) {
+            ix->pq_off = poff1;
+            ix->pq_codes = pcodes1;
+        }
+        ix->p.total_point_count = N1;
+        ix->version++;
+        // outputs of the accepted points, in order
+        if ((rc = copy_out(ctx, cells_out, cell, 4ull * m, device_ptrs)) ||
+            (rc = copy_out(ctx, filter_records_out, recs, rec * m, device_ptrs)) ||
+            (pnsq && (rc = copy_out(ctx, pq_records_out, precs, prec * m, device_ptrs))) ||
+            (rc = copy_out(ctx, features_out, ya, 4ull * m * d, device_ptrs)))
+            return rc;
+    }
+    if (ids_out) {
+        if (device_ptrs) PK_CHECK(hipMemcpyAsync(ids_out, ids.data(), 4ull * n, hipMemcpyHostToDevice, s));
+        else std::memcpy(ids_out, ids.data(), 4ull * n);
+    }
+    PK_CHECK(hipStreamSynchronize(s));
+    return PUCKGPU_OK;
+}
+
+uint64_t puckgpu_index_version(const puckgpu_index* ix) { return ix ? ix->version : 0; }
 
 }  // extern "C"

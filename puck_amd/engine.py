@@ -271,6 +271,48 @@ class GpuIndex:
         except Exception:
             pass
 
+    def insert(self, x):
+        """Realtime insert of the rows of x [n, dim] float32, in order
+        (puckgpu_index_insert: RealtimeInsertPuckIndex::insert).  Returns
+        dict(ids [n]: the local id given, or 0xffffffff for a refused
+        zero-norm row; for the m accepted rows: cells [m], records
+        [m, 4 + nsq], pq_records [m, 4 + pq_nsq] (whether_pq), features
+        [m, dim] (the stored rows)).  Host arrays in, host arrays out; device
+        tensors in, device tensors out (sized for every row, the first m
+        filled)."""
+        _check(x, "x", _F32, shape=(None, self.dim), device=self.ctx.device)
+        n = int(x.shape[0])
+        dev = _is_dev(x)
+        pn = self.pq_nsq
+        if dev:
+            import torch
+            mk = lambda shape, dt: torch.empty(shape, dtype=dt, device=x.device)
+            ids, cells = mk((n,), torch.int32), mk((n,), torch.int32)
+            rec, prec = mk((n, 4 + self.nsq), torch.uint8), mk((n, 4 + pn), torch.uint8)
+            feats = mk((n, self.dim), torch.float32)
+            self.ctx.sync_torch(x)
+        else:
+            ids, cells = np.zeros(max(n, 1), np.uint32), np.zeros(max(n, 1), np.uint32)
+            rec, prec = np.zeros((max(n, 1), 4 + self.nsq), np.uint8), np.zeros((max(n, 1), 4 + pn), np.uint8)
+            feats = np.zeros((max(n, 1), self.dim), np.float32)
+        self.ctx.check(self.ctx.L.puckgpu_index_insert(self.ctx.h, self.h, n, _ptr(x), _ptr(ids), _ptr(cells),
+                                                       _ptr(rec), _ptr(prec) if pn else None, _ptr(feats),
+                                                       1 if dev else 0))
+        if dev:
+            m = int((ids != -1).sum().item())
+            out = dict(ids=ids, cells=cells[:m], records=rec[:m], features=feats[:m])
+        else:
+            ids = ids[:n]
+            m = int((ids != 0xffffffff).sum())
+            out = dict(ids=ids, cells=cells[:m], records=rec[:m], features=feats[:m])
+        if pn:
+            out["pq_records"] = prec[:m]
+        self.N += m
+        return out
+
+    def version(self):
+        return int(self.ctx.L.puckgpu_index_version(self.h))
+
     def export(self):
         C, F, N = self.C, self.F, self.N
         out = dict(coarse_norms=np.zeros(C, np.float32), stationary=np.zeros(C * F, np.float32),
