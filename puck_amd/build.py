@@ -26,10 +26,17 @@ def sources():
 
 
 def build_host_adapter(verbose=False):
-    """C++ puck::Index adapter over the C ABI and its test driver (g++)."""
+    """C++ puck::Index adapters over the C ABI and their test drivers (g++)."""
     root = os.path.dirname(HERE)
     out = os.path.join(root, "tests", "cpp", "host_adapter_test")
     cmd = ["g++", "-O2", "-std=c++17", "-o", out, os.path.join(root, "tests", "cpp", "host_adapter_main.cpp"),
+           os.path.join(HERE, "host", "gpu_puck_index.cpp"), "-L" + HERE, "-lpuckgpu",
+           "-Wl,-rpath,$ORIGIN/../../puck_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    ins = os.path.join(root, "tests", "cpp", "insert_adapter_test")
+    cmd = ["g++", "-O2", "-std=c++17", "-o", ins, os.path.join(root, "tests", "cpp", "insert_adapter_main.cpp"),
            os.path.join(HERE, "host", "gpu_puck_index.cpp"), "-L" + HERE, "-lpuckgpu",
            "-Wl,-rpath,$ORIGIN/../../puck_amd"]
     if verbose:

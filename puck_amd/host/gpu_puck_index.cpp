@@ -94,6 +94,10 @@ GpuPuckIndex::~GpuPuckIndex() {
 
 int GpuPuckIndex::init() {
     std::lock_guard<std::mutex> g(_mu);
+    return init_locked();
+}
+
+int GpuPuckIndex::init_locked() {
     const std::string& P = _flags.index_path;
     Conf c;
     if (!read_conf(join(P, _flags.index_file_name), &c)) { _err = "model file read error"; return -1; }
@@ -142,6 +146,9 @@ int GpuPuckIndex::init() {
     _sp.radius_rate = _flags.radius_rate;
     _sp.replay_all = 0;
     _dim = c.dim;
+    _filter_nsq = c.filter_nsq;
+    _whether_pq = c.whether_pq;
+    _pq_nsq = c.whether_pq ? c.nsq : 0;
     return 0;
 }
 
@@ -225,6 +232,106 @@ int GpuPuckIndex::build() {
 int GpuPuckIndex::train() {
     _err = "training runs in puck_amd.train";
     return -1;
+}
+
+// ---------------------------------------------------------------------------
+namespace {
+
+// Cut a file to `bytes` when it is longer (RealtimeInsertPuckIndex::reorganize,
+// realtime_insert_puck_index.cpp:110-129); false when it is shorter.
+bool cut_to(const std::string& path, uint64_t bytes) {
+    std::ifstream in(path, std::ios::binary | std::ios::ate);
+    if (!in) return false;
+    const uint64_t have = (uint64_t)in.tellg();
+    if (have == bytes) return true;
+    if (have < bytes) return false;
+    std::vector<char> b(bytes);
+    in.seekg(0);
+    in.read(b.data(), (std::streamsize)bytes);
+    in.close();
+    std::ofstream out(path, std::ios::binary | std::ios::trunc);
+    out.write(b.data(), (std::streamsize)bytes);
+    return (bool)out;
+}
+
+}  // namespace
+
+GpuRealtimeInsertPuckIndex::GpuRealtimeInsertPuckIndex(const GpuIndexFlags& flags) : GpuPuckIndex(flags) {}
+
+int GpuRealtimeInsertPuckIndex::init() {
+    std::lock_guard<std::mutex> g(_mu);
+    const std::string& P = _flags.index_path;
+    // read_labels (:131-152): the point count is the label file's line count
+    _labels.clear();
+    {
+        std::ifstream in(join(P, _flags.label_file_name));
+        if (!in) { _err = "open label file error"; return -1; }
+        std::string line;
+        while (std::getline(in, line)) _labels.push_back(line);
+    }
+    Conf c;
+    if (!read_conf(join(P, _flags.index_file_name), &c)) { _err = "model file read error"; return -1; }
+    const uint64_t N = _labels.size();
+    // reorganize_inserted_index (:52-108)
+    if (N > 0) {
+        bool ok = cut_to(join(P, _flags.filter_data_file_name), N * (4ull + c.filter_nsq)) &&
+                  cut_to(join(P, _flags.cell_assign_file_name), N * 4ull);
+        if (ok && c.whether_pq) ok = cut_to(join(P, _flags.pq_data_file_name), N * (4ull + c.nsq));
+        else if (ok) ok = cut_to(join(P, _flags.feature_file_name), N * (4ull + 4ull * c.dim));
+        if (!ok) { _err = "index files shorter than the label file"; return -1; }
+    }
+    c.N = (uint32_t)N;
+    if (!write_conf(join(P, _flags.index_file_name), c)) { _err = "model file write error"; return -1; }
+    if (init_locked() != 0) return -1;
+    // IndexFileHandle::init (:585-611): append handles
+    _f_cell.open(join(P, _flags.cell_assign_file_name), std::ios::binary | std::ios::app);
+    _f_filter.open(join(P, _flags.filter_data_file_name), std::ios::binary | std::ios::app);
+    _f_data.open(join(P, _whether_pq ? _flags.pq_data_file_name : _flags.feature_file_name),
+                 std::ios::binary | std::ios::app);
+    _f_label.open(join(P, _flags.label_file_name), std::ios::app);
+    if (!_f_cell || !_f_filter || !_f_data || !_f_label) { _err = "open index files for append failed"; return -1; }
+    return 0;
+}
+
+int GpuRealtimeInsertPuckIndex::insert(const InsertRequest* request) {
+    std::lock_guard<std::mutex> g(_mu);
+    if (!request || !request->feature) { _err = "null request or feature"; return -1; }
+    if (!_index) { _err = "insert before init"; return -1; }
+    uint32_t id = 0, cell = 0;
+    std::vector<uint8_t> rec(4 + _filter_nsq), prec(4 + _pq_nsq);
+    std::vector<float> feat(_dim);
+    if (puckgpu_index_insert(_ctx, _index, 1, request->feature, &id, &cell, rec.data(),
+                             _whether_pq ? prec.data() : nullptr, feat.data(), 0) != 0) {
+        _err = puckgpu_last_error(_ctx);
+        return -1;
+    }
+    if (id == 0xffffffffu) { _err = "feature norm below 1e-6, could not be normalised"; return -1; }
+    // append_index (:346-396): the quantized rows, the feature row (fvecs),
+    // the cell, then the label
+    _f_filter.write((const char*)rec.data(), (std::streamsize)rec.size());
+    if (_whether_pq) {
+        _f_data.write((const char*)prec.data(), (std::streamsize)prec.size());
+    } else {
+        const int32_t d = (int32_t)_dim;
+        _f_data.write((const char*)&d, 4);
+        _f_data.write((const char*)feat.data(), 4ll * _dim);
+    }
+    _f_cell.write((const char*)&cell, 4);
+    _f_label << request->label << "\n";
+    _f_filter.flush();
+    _f_data.flush();
+    _f_cell.flush();
+    _f_label.flush();
+    if (!_f_filter || !_f_data || !_f_cell || !_f_label) { _err = "append to index files failed"; return -1; }
+    _labels.push_back(request->label);
+    return 0;
+}
+
+int GpuRealtimeInsertPuckIndex::get_label(uint32_t label_id, std::string& label) {
+    std::lock_guard<std::mutex> g(_mu);
+    if (label_id >= _labels.size()) return -1;
+    label = _labels[label_id];
+    return 0;
 }
 
 }  // namespace puck

@@ -6,9 +6,11 @@
 // The host code reaches the GPU only through include/puck_gpu.h.
 #pragma once
 #include <cstdint>
+#include <fstream>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/puck_gpu.h"
 
@@ -49,6 +51,7 @@ struct GpuIndexFlags {
     std::string filter_data_file_name = "filter_data.dat";
     std::string pq_codebook_file_name = "learn_codebooks.dat";
     std::string pq_data_file_name = "pq_data.dat";
+    std::string label_file_name = "all_data.url";
     uint32_t topk = 100;
     uint32_t search_coarse_count = 200;
     int32_t filter_topk = -1;  // unset: 11 * topk (index_conf.cpp:178-180)
@@ -84,14 +87,46 @@ public:
     const char* last_error() const { return _err.c_str(); }
     uint32_t dim() const { return _dim; }
 
-private:
+protected:
+    int init_locked();
     GpuIndexFlags _flags;
     puckgpu_ctx* _ctx = nullptr;
     puckgpu_index* _index = nullptr;
     puckgpu_search_params _sp{};
-    uint32_t _dim = 0;
+    uint32_t _dim = 0, _filter_nsq = 0, _pq_nsq = 0;
+    bool _whether_pq = false;
     std::string _err;
-    std::mutex _mu;  // one context: searches from several threads are serialised
+    std::mutex _mu;  // one context: searches (and inserts) from several threads are serialised
+};
+
+// InsertRequest of realtime_insert_puck_index.h:142-149.
+struct InsertRequest {
+    std::string label;
+    const float* feature = nullptr;
+    uint32_t dim = 0;
+};
+
+// RealtimeInsertPuckIndex (puck/puck/realtime_insert_puck_index.h): init()
+// takes the point count from the label file and cuts the index files to it
+// (a crash between the appends of one insert leaves longer files,
+// :52-129); insert() puts the point into HBM, searchable when it returns,
+// then appends it to filter_data.dat, pq_data.dat or all_data.feat.bin,
+// cell_assign.dat and the label file (:346-396).  Deviation: the reference
+// appends to the files before the point becomes searchable; here a failed
+// append returns -1 with the point already in the index.
+class GpuRealtimeInsertPuckIndex : public GpuPuckIndex {
+public:
+    explicit GpuRealtimeInsertPuckIndex(const GpuIndexFlags& flags = GpuIndexFlags());
+    int init() override;
+    // 0, or -1 (null request or feature, a zero-norm feature under
+    // whether_norm, a device or file error).
+    int insert(const InsertRequest* request);
+    // The label of local id label_id (get_label, :543-561): 0 / -1.
+    int get_label(uint32_t label_id, std::string& label);
+
+private:
+    std::vector<std::string> _labels;
+    std::ofstream _f_cell, _f_filter, _f_data, _f_label;
 };
 
 }  // namespace puck

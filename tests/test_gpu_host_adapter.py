@@ -101,3 +101,66 @@ def test_cpp_adapter_serves_and_builds_pq(tmp_path):
     assert np.array_equal(arr["records"], ix["records"])
     assert np.array_equal(arr["pq_records"], ix["pq_records"])
     assert np.array_equal(Ib, Io) and np.array_equal(Db.view(np.uint32), Do.view(np.uint32))
+
+
+INS_BIN = os.path.join(ROOT, "tests", "cpp", "insert_adapter_test")
+
+
+def _run_insert(index_path, x_ins, q, topk, scc, ftk, tmp):
+    xf, qf, out = str(tmp / "ins.f32"), str(tmp / "q.f32"), str(tmp / "out.bin")
+    np.ascontiguousarray(x_ins, np.float32).tofile(xf)
+    np.ascontiguousarray(q, np.float32).tofile(qf)
+    args = [INS_BIN, index_path, xf, str(x_ins.shape[0]), str(q.shape[1]), qf, str(q.shape[0]), str(topk), str(topk),
+            str(scc), str(ftk), out]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(out, dtype=np.uint32)
+    n = q.shape[0]
+    Dm = raw[:n * topk].view(np.float32).reshape(n, topk)
+    Im = raw[n * topk:2 * n * topk].reshape(n, topk)
+    R = raw[2 * n * topk:]
+    labels = open(out + ".labels").read().split("\n")[:-1]
+    return Dm, Im, R, labels
+
+
+@pytest.mark.parametrize("pq", [False, True])
+def test_cpp_realtime_insert_then_reload(tmp_path, pq):
+    # inserts through GpuRealtimeInsertPuckIndex equal the oracle's overlay;
+    # after a restart the appended files load as an ordinary index
+    x = D.mixture_uint8_grid(7000, 32, 50, seed=66, means_seed=67)
+    q = D.mixture_uint8_grid(24, 32, 50, seed=68, means_seed=67)
+    base, extra = x[:5500], x[5500:5800]
+    if pq:
+        ix = CI.build_index_pq(base, 12, 12, 8, 16, whether_norm=False, train_n=5000)
+    else:
+        ix = CI.build_index(base, 12, 12, 16, whether_norm=False, train_n=5000)
+    d = tmp_path / "idx"
+    index_io.write_index(str(d), FL, ix["coarse"], ix["fine"], ix["codebook"], ix["cell_assign"], ix["records"],
+                         None if pq else ix["raw"], False, pq_codebook=ix.get("pq_codebook"),
+                         pq_records=ix.get("pq_records"))
+    N = base.shape[0]
+    with open(d / FL.get("label_file_name"), "w") as f:
+        f.write("".join(f"base_{i}\n" for i in range(N)))
+    Dm, Im, R, labels = _run_insert(str(d), extra, q, 10, 4, 150, tmp_path)
+    oi = CI.oracle_index(ix)
+    oi.insert(extra)
+    Do, Io, Ro, _ = oi.search_batch(q, 10, 4, 150)
+    assert np.array_equal(R, Ro) and np.array_equal(Im, Io)
+    assert np.array_equal(Dm.view(np.uint32), Do.view(np.uint32))
+    want = [f"base_{i}" if i < N else f"ins_{i - N}" for k in range(q.shape[0]) for i in Io[k, :Ro[k]]]
+    assert labels == want
+    assert any(l.startswith("ins_") for l in labels)
+    # restart: the files now hold every point (index.dat keeps the loaded
+    # count until init takes it from the label file); the index is rebuilt
+    # from them
+    assert index_io.read_index_conf(str(d / FL.get("index_file_name")))["total_point_count"] == N
+    Dr, Ir, Rr, labels_r = _run_insert(str(d), extra[:0], q, 10, 4, 150, tmp_path)
+    conf, arr = index_io.read_index(str(d), FL)
+    assert conf["total_point_count"] == N + extra.shape[0]
+    rb = dict(ix, cell_assign=arr["cell_assign"], records=arr["records"],
+              raw=arr["features"] if not pq else np.zeros((N + extra.shape[0], 32), np.float32))
+    if pq:
+        rb["pq_records"] = arr["pq_records"]
+    Db, Ib, Rb, _ = CI.oracle_index(rb).search_batch(q, 10, 4, 150)
+    assert np.array_equal(Rr, Rb) and np.array_equal(Ir, Ib)
+    assert np.array_equal(Dr.view(np.uint32), Db.view(np.uint32))

@@ -125,3 +125,45 @@ def test_init_reports_missing_pq_files(tmp_path):
     r = subprocess.run([BIN, str(d), str(d / "q.f32"), "1", "16", "1", "10", "2", "-1", out], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode == 4 and "check file length" in r.stderr, (r.returncode, r.stderr)
+
+
+INS_BIN = os.path.join(ROOT, "tests", "cpp", "insert_adapter_test")
+
+
+def test_realtime_init_cuts_files_to_the_label_count(tmp_path):
+    # RealtimeInsertPuckIndex::init takes the point count from the label file
+    # and cuts longer index files to it (realtime_insert_puck_index.cpp:52-129),
+    # before any device call
+    x = D.mixture_uint8_grid(700, 16, 8, seed=74, means_seed=75)
+    ix = CI.build_index(x, 4, 4, 8, whether_norm=False)
+    d = tmp_path / "rt"
+    index_io.write_index(str(d), FL, ix["coarse"], ix["fine"], ix["codebook"], ix["cell_assign"], ix["records"],
+                         ix["raw"], False)
+    keep = 650
+    with open(d / FL.get("label_file_name"), "w") as f:
+        f.write("".join(f"p{i}\n" for i in range(keep)))
+    q = tmp_path / "q.f32"
+    x[:2].astype(np.float32).tofile(str(q))
+    r = subprocess.run([INS_BIN, str(d), str(q), "0", "16", str(q), "2", "10", "100", "2", "-1",
+                        str(tmp_path / "out.bin")], capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        assert r.returncode == 4 and "no GPU context" in r.stderr, (r.returncode, r.stderr)
+    assert os.path.getsize(d / FL.get("filter_data_file_name")) == keep * 12
+    assert os.path.getsize(d / FL.get("cell_assign_file_name")) == keep * 4
+    assert os.path.getsize(d / FL.get("feature_file_name")) == keep * (4 + 16 * 4)
+    conf, arr = index_io.read_index(str(d), FL)
+    assert conf["total_point_count"] == keep
+    assert np.array_equal(arr["cell_assign"], ix["cell_assign"][:keep])
+
+
+def test_realtime_init_refuses_short_files(tmp_path):
+    x = D.mixture_uint8_grid(300, 16, 8, seed=76, means_seed=77)
+    ix = CI.build_index(x, 4, 4, 8, whether_norm=False)
+    d = tmp_path / "rt"
+    index_io.write_index(str(d), FL, ix["coarse"], ix["fine"], ix["codebook"], ix["cell_assign"], ix["records"],
+                         ix["raw"], False)
+    with open(d / FL.get("label_file_name"), "w") as f:
+        f.write("".join(f"p{i}\n" for i in range(301)))
+    r = subprocess.run([INS_BIN, str(d), str(d / "none"), "0", "16", str(d / "none"), "0", "10", "100", "2", "-1",
+                        str(tmp_path / "out.bin")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 4 and "shorter than the label file" in r.stderr, (r.returncode, r.stderr)
