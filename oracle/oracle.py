@@ -18,6 +18,7 @@ _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 
 
 class Stats(ctypes.Structure):
@@ -71,6 +72,83 @@ def lib_fast():
             subprocess.check_call(["make", "-s", "-C", HERE, "all"])
         _fast = load_variant(path)
     return _fast
+
+
+_train = None
+
+
+def lib_train():
+    """liboracle_train.so: the k-means and coarse / fine training restatement."""
+    global _train
+    if _train is None:
+        path = os.path.join(HERE, "liboracle_train.so")
+        if not os.path.exists(path):
+            subprocess.check_call(["make", "-s", "-C", HERE, "all"])
+        L = ctypes.CDLL(path)
+        ip = ctypes.POINTER(ctypes.c_int)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.pot_kmeans.restype = ctypes.c_float
+        L.pot_kmeans.argtypes = [ctypes.c_uint32] * 3 + [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                                         ctypes.c_int, _f32p, _i32p, ip, ip]
+        L.pot_kmeans_core.restype = ctypes.c_int
+        L.pot_kmeans_core.argtypes = [ctypes.c_uint32] * 3 + [_f32p, ctypes.c_int, ctypes.c_uint32, ctypes.c_int,
+                                                              _f32p, _i32p, _f32p, dp, ip, ip, _f64p]
+        L.pot_train.restype = ctypes.c_int
+        L.pot_train.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _f32p, ctypes.c_uint32, ctypes.c_uint32,
+                                ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_int, _f32p, _f32p, _f32p, ip, ip]
+        _train = L
+    return _train
+
+
+def kmeans(x, k, niter=30, init_pp=True, seed=0, mean_mode=0):
+    """Kmeans::kmeans (kmeans.cpp:149-219) with a generator seeded `seed`.
+    Returns (centroids [k, d], assign [n], mean error, iterations, splits)."""
+    x = c32(x)
+    n, d = x.shape
+    cent = np.zeros((k, d), np.float32)
+    a = np.zeros(n, np.int32)
+    it, sp = ctypes.c_int(0), ctypes.c_int(0)
+    err = lib_train().pot_kmeans(d, n, k, x.reshape(-1), niter, int(bool(init_pp)), seed, mean_mode,
+                                 cent.reshape(-1), a, ctypes.byref(it), ctypes.byref(sp))
+    if err < 0:
+        raise ValueError("pot_kmeans failed")
+    return cent, a, float(err), it.value, sp.value
+
+
+def kmeans_core(x, centroids, niter, seed=0, mean_mode=0):
+    """kmeans_core (kmeans.cpp:260-329) from given centroids.  Returns
+    (centroids, assign, dis, qerr, iterations, splits, qerr per iteration)."""
+    x = c32(x)
+    n, d = x.shape
+    cent = np.array(centroids, np.float32, copy=True).reshape(-1, d)
+    k = cent.shape[0]
+    a = np.zeros(n, np.int32)
+    dis = np.zeros(n, np.float32)
+    q = ctypes.c_double(0)
+    it, sp = ctypes.c_int(0), ctypes.c_int(0)
+    tr = np.zeros(max(niter, 1), np.float64)
+    rc = lib_train().pot_kmeans_core(d, n, k, x.reshape(-1), niter, seed, mean_mode, cent.reshape(-1), a, dis,
+                                     ctypes.byref(q), ctypes.byref(it), ctypes.byref(sp), tr)
+    if rc != 0:
+        raise ValueError("pot_kmeans_core failed")
+    return cent, a, dis, q.value, it.value, sp.value, tr[:it.value]
+
+
+def train_centers(x, C, F, iterations=10, init_pp=True, seed=0, mean_mode=0):
+    """HierarchicalClusterIndex::train (hierarchical_cluster_index.cpp:882-985).
+    Returns (coarse [C, d], fine [F, d] by descending norm, errors [rounds, 2],
+    rounds run, final niter)."""
+    x = c32(x)
+    n, d = x.shape
+    co = np.zeros((C, d), np.float32)
+    fi = np.zeros((F, d), np.float32)
+    tr = np.zeros(2 * max(iterations, 1), np.float32)
+    r, ni = ctypes.c_int(0), ctypes.c_int(0)
+    rc = lib_train().pot_train(d, n, x.reshape(-1), C, F, iterations, int(bool(init_pp)), seed, mean_mode,
+                               co.reshape(-1), fi.reshape(-1), tr, ctypes.byref(r), ctypes.byref(ni))
+    if rc != 0:
+        raise ValueError("pot_train failed")
+    return co, fi, tr[:2 * r.value].reshape(-1, 2), r.value, ni.value
 
 
 def load_variant(path):
