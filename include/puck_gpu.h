@@ -27,6 +27,10 @@
  *                          (puck/puck/puck_index.cpp:602-709)
  *   puckgpu_kmeans_assign  nearest_center, the k-means assignment step
  *                          (puck/hierarchical_cluster/kmeans.cpp:221-258)
+ *   puckgpu_kmeans         Kmeans::kmeans / kmeans_core / kmeans_reassign_empty
+ *                          (puck/hierarchical_cluster/kmeans.cpp:30-329)
+ *   puckgpu_train_centers  HierarchicalClusterIndex::train
+ *                          (puck/hierarchical_cluster/hierarchical_cluster_index.cpp:882-985)
  *   puckgpu_index_attach_pq  PuckIndex::read_coodbooks / read_feature_index for
  *                          whether_pq = true (the PQ codebook and pq_data.dat,
  *                          puck/puck/puck_index.cpp:173-227)
@@ -360,6 +364,39 @@ int puckgpu_encode_pq(puckgpu_ctx* ctx, const float* coarse, const float* fine, 
  * (nearest_center's formula and strict '<'), dist_out [n] may be NULL. */
 int puckgpu_kmeans_assign(puckgpu_ctx* ctx, uint32_t dim, const float* centroids, uint32_t k, uint32_t n,
                           const float* x, int32_t* assign_out, float* dist_out, int device_ptrs);
+/* Kmeans::kmeans (kmeans.cpp:149-219, redo 1): seeding, then niter Lloyd
+ * iterations of kmeans_core (:260-329) on n points x [n x dim].  The
+ * reference's generator is a std::mt19937 seeded with time(0); here it is
+ * seeded with `seed` and drawn in the reference's sequence (k-means++ over
+ * the first min(n, 8k) points when n > 8k and n > 8192, init_pp = 1, the
+ * reference's default; otherwise k distinct uniform picks; each iteration
+ * then draws the empty-cluster split's noise scale, and one roulette pick
+ * and dim Gaussian steps per empty cluster).  Assignments are
+ * nearest_center's.  Centroid update: mean_mode = 0 is the reference's
+ * arithmetic, which clears only the first 4 bytes of the centroids before
+ * adding the points in point order (kmeans.cpp:275), so a centroid becomes
+ * (previous + sum) / count except element [0][0]; mean_mode = 1 is the plain
+ * mean, the same float additions from zero.  Stops early when the error sum
+ * changes by less than 1e-6 with no split.  centroids_out [k x dim],
+ * assign_out [n] (may be NULL: the last assignment, made before the last
+ * update), err_out: error sum / n, iters_out / split_out (may be NULL). */
+int puckgpu_kmeans(puckgpu_ctx* ctx, uint32_t dim, uint32_t n, uint32_t k, const float* x, int niter, int init_pp,
+                   uint32_t seed, int mean_mode, float* centroids_out, int32_t* assign_out, float* err_out,
+                   int* iters_out, int* split_out, int device_ptrs);
+/* HierarchicalClusterIndex::train (hierarchical_cluster_index.cpp:882-985):
+ * `iterations` rounds (kmeans_iterations_count, 10 by default) of coarse
+ * k-means on the points less their best fine centers so far, then fine
+ * k-means on the points less their new coarse centers; a round whose fine
+ * error is at least 1e-4 below the best keeps both, otherwise the Lloyd
+ * limit (30 at first) grows by 10 and the rounds end once it exceeds 80.
+ * One generator (seed) serves every k-means run.  Fine centers come out in
+ * ascending -sqrt(|f|^2/2) order (a MaxHeap sort, :958-980), i.e. by
+ * descending norm.  coarse_out [C x dim], fine_out [F x dim], err_trace
+ * [2 x iterations] (coarse and fine error of each round run, may be NULL),
+ * rounds_out / niter_out (may be NULL). */
+int puckgpu_train_centers(puckgpu_ctx* ctx, uint32_t dim, uint32_t n, const float* x, uint32_t C, uint32_t F,
+                          int iterations, int init_pp, uint32_t seed, int mean_mode, float* coarse_out,
+                          float* fine_out, float* err_trace, int* rounds_out, int* niter_out, int device_ptrs);
 
 #ifdef __cplusplus
 }

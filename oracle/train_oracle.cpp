@@ -129,7 +129,7 @@ struct Kmeans {
         int tot = 0, it = 0;
         for (int iter = 1; iter <= nit; ++iter) {
             ++it;
-            po_nearest_center(d, cent, (uint32_t)k, v, (uint32_t)n, assign, dis, 1);
+            po_nearest_center(d, cent, (uint32_t)k, v, (uint32_t)n, assign, dis, 0);
             std::memset(nassign, 0, k * sizeof(int));
             if (mean_mode) std::memset(cent, 0, sizeof(float) * k * d);
             else cent[0] = 0.f;

@@ -20,7 +20,7 @@ EXPORTS = [
     "puckgpu_search_learned", "puckgpu_budget_ranks", "puckgpu_server_create", "puckgpu_server_free",
     "puckgpu_server_buckets", "puckgpu_server_search", "puckgpu_server_tune", "puckgpu_last_scan_ms",
     "puckgpu_center_distances", "puckgpu_index_attach_pq", "puckgpu_encode_pq", "puckgpu_search_noalloc",
-    "puckgpu_index_insert", "puckgpu_index_version",
+    "puckgpu_index_insert", "puckgpu_index_version", "puckgpu_kmeans", "puckgpu_train_centers",
 ]
 
 
@@ -113,6 +113,11 @@ def load():
     L.puckgpu_encode_pq.restype = c_int
     L.puckgpu_index_insert.argtypes = [vp, vp, u32, vp, vp, vp, vp, vp, vp, c_int]
     L.puckgpu_index_insert.restype = c_int
+    L.puckgpu_kmeans.argtypes = [vp, u32, u32, u32, vp, c_int, c_int, u32, c_int, vp, vp, vp, vp, vp, c_int]
+    L.puckgpu_kmeans.restype = c_int
+    L.puckgpu_train_centers.argtypes = [vp, u32, u32, vp, u32, u32, c_int, c_int, u32, c_int, vp, vp, vp, vp, vp,
+                                        c_int]
+    L.puckgpu_train_centers.restype = c_int
     L.puckgpu_index_version.argtypes = [vp]
     L.puckgpu_index_version.restype = ctypes.c_uint64
     L.puckgpu_kmeans_assign.argtypes = [vp, u32, vp, u32, u32, vp, vp, vp, c_int]

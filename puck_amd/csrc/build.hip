@@ -769,3 +769,196 @@ extern "C" hipError_t pk_launch_insert_stationary(const float* coarse, const flo
                        st);
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// k-means training (Kmeans::kmeans_core, kmeans.cpp:260-329, and
+// HierarchicalClusterIndex::train, hierarchical_cluster_index.cpp:882-985).
+//
+// Centroid update in the reference's arithmetic: each centroid's points are
+// added in point order with float additions (cblas_saxpy with alpha 1), then
+// scaled by (float)(1.0 / count) (cblas_sscal).  The points are first put in
+// cluster order by a stable counting sort, so one workgroup per centroid can
+// add them in that order.
+
+#define KS_THREADS 256u
+#define KS_ITEMS 8u
+#define KS_BLOCK (KS_THREADS * KS_ITEMS)
+
+// hist[c * nb + b] = points of block b (KS_BLOCK consecutive points) in cluster c
+__global__ __launch_bounds__(KS_THREADS) void pk_ks_hist(const int32_t* __restrict__ key, uint32_t n, uint32_t k,
+                                                         uint32_t nb, uint32_t* __restrict__ hist) {
+    extern __shared__ uint32_t cnt[];
+    for (uint32_t c = threadIdx.x; c < k; c += blockDim.x) cnt[c] = 0;
+    __syncthreads();
+    const uint32_t b0 = blockIdx.x * KS_BLOCK;
+    for (uint32_t r = 0; r < KS_ITEMS; ++r) {
+        const uint32_t i = b0 + r * KS_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(uint32_t)key[i]], 1u);
+    }
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < k; c += blockDim.x) hist[(uint64_t)c * nb + blockIdx.x] = cnt[c];
+}
+
+// Stable scatter: point i goes to off[key * nb + block] + (points of its
+// block with the same key before it).  Rounds of 256 points in point order;
+// inside a round the four waves take their turn, and inside a wave the lanes
+// with equal keys are found with one ballot per key bit.
+__global__ __launch_bounds__(KS_THREADS) void pk_ks_scatter(const int32_t* __restrict__ key, uint32_t n, uint32_t k,
+                                                            uint32_t nb, uint32_t kbits,
+                                                            const uint32_t* __restrict__ off,
+                                                            uint32_t* __restrict__ perm) {
+    extern __shared__ uint32_t run[];
+    for (uint32_t c = threadIdx.x; c < k; c += blockDim.x) run[c] = off[(uint64_t)c * nb + blockIdx.x];
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const uint32_t b0 = blockIdx.x * KS_BLOCK;
+    for (uint32_t r = 0; r < KS_ITEMS; ++r) {
+        const uint32_t i = b0 + r * KS_THREADS + threadIdx.x;
+        const bool valid = i < n;
+        const uint32_t kv = valid ? (uint32_t)key[i] : 0u;
+        uint64_t eq = __ballot(valid);
+        for (uint32_t bit = 0; bit < kbits; ++bit) {
+            const uint64_t ones = __ballot(valid && ((kv >> bit) & 1u));
+            eq &= ((kv >> bit) & 1u) ? ones : ~ones;
+        }
+        for (uint32_t w = 0; w < KS_THREADS / 64u; ++w) {
+            if (wave == w && valid) {
+                const uint32_t base = run[kv];
+                perm[base + (uint32_t)__popcll(eq & lt)] = i;
+                // the group's lowest lane advances the counter (written from
+                // the value read, so the read comes first)
+                if ((eq & lt) == 0ull) run[kv] = base + (uint32_t)__popcll(eq);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+extern "C" uint64_t pk_ks_hist_words(uint32_t n, uint32_t k) {
+    return (uint64_t)k * ((n + KS_BLOCK - 1) / KS_BLOCK);
+}
+
+// perm [n]: point indices by cluster, point order inside a cluster; starts
+// [k + 1]: each cluster's first position.  hist / off: pk_ks_hist_words
+// words each; work: pk_exscan_work of that many.
+extern "C" hipError_t pk_launch_sort_by_cluster(const int32_t* key, uint32_t n, uint32_t k, uint32_t* hist,
+                                                uint32_t* off, uint32_t* work, uint32_t* perm, uint32_t* starts,
+                                                hipStream_t s);
+
+__global__ void pk_ks_starts(const uint32_t* __restrict__ off, uint32_t k, uint32_t nb, uint32_t n,
+                             uint32_t* __restrict__ starts) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < k) starts[c] = off[(uint64_t)c * nb];
+    if (c == k) starts[k] = n;
+}
+
+extern "C" hipError_t pk_launch_sort_by_cluster(const int32_t* key, uint32_t n, uint32_t k, uint32_t* hist,
+                                                uint32_t* off, uint32_t* work, uint32_t* perm, uint32_t* starts,
+                                                hipStream_t s) {
+    if (k == 0 || k > 16384) return hipErrorInvalidValue;
+    const uint32_t nb = (n + KS_BLOCK - 1) / KS_BLOCK;
+    uint32_t kbits = 0;
+    while ((1u << kbits) < k) ++kbits;
+    const size_t lds = sizeof(uint32_t) * k;
+    if (nb) {
+        hipLaunchKernelGGL(pk_ks_hist, dim3(nb), dim3(KS_THREADS), lds, s, key, n, k, nb, hist);
+        hipError_t e = pk_launch_exscan(hist, (uint64_t)k * nb, off, work, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(pk_ks_scatter, dim3(nb), dim3(KS_THREADS), lds, s, key, n, k, nb, kbits, off, perm);
+        hipLaunchKernelGGL(pk_ks_starts, dim3((k + 1 + 255) / 256), dim3(256), 0, s, off, k, nb, n, starts);
+    } else {
+        hipError_t e = hipMemsetAsync(starts, 0, sizeof(uint32_t) * (k + 1ull), s);
+        if (e != hipSuccess) return e;
+    }
+    return hipGetLastError();
+}
+
+// One workgroup per centroid, one thread per dimension: acc starts at 0
+// (mean_mode) or at the previous centroid with element [0][0] cleared (the
+// reference clears only the first 4 bytes, kmeans.cpp:275); the centroid's
+// points are added in point order, then the sum is scaled by
+// (float)(1.0 / count) (inf for an empty cluster, which the split replaces).
+__global__ __launch_bounds__(256) void pk_kmeans_update(const float* __restrict__ x, uint32_t dim,
+                                                        const uint32_t* __restrict__ perm,
+                                                        const uint32_t* __restrict__ starts,
+                                                        float* __restrict__ cent, int mean_mode) {
+    const uint32_t c = blockIdx.x;
+    const uint32_t b = starts[c], e = starts[c + 1];
+    const float alpha = (float)(1.0 / (double)(e - b));
+    for (uint32_t t = threadIdx.x; t < dim; t += blockDim.x) {
+        float acc = (mean_mode || (c == 0 && t == 0)) ? 0.0f : cent[(uint64_t)c * dim + t];
+        uint32_t i = b;
+        for (; i + 8 <= e; i += 8) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = x[(uint64_t)perm[i + u] * dim + t];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc = acc + v[u];
+        }
+        for (; i < e; ++i) acc = acc + x[(uint64_t)perm[i] * dim + t];
+        cent[(uint64_t)c * dim + t] = alpha * acc;
+    }
+}
+
+extern "C" hipError_t pk_launch_kmeans_update(const float* x, uint32_t dim, const uint32_t* perm,
+                                              const uint32_t* starts, uint32_t k, float* cent, int mean_mode,
+                                              hipStream_t s) {
+    if (k == 0) return hipSuccess;
+    const uint32_t nt = dim >= 256 ? 256u : (dim >= 64 ? ((dim + 63) / 64) * 64 : 64u);
+    hipLaunchKernelGGL(pk_kmeans_update, dim3(k), dim3(nt), 0, s, x, dim, perm, starts, cent, mean_mode);
+    return hipGetLastError();
+}
+
+// out[i] = x[i] - c[assign[i]] (cblas_saxpy with alpha -1)
+__global__ void pk_sub_assigned(const float* __restrict__ x, uint64_t total, uint32_t dim,
+                                const float* __restrict__ cent, const int32_t* __restrict__ assign,
+                                float* __restrict__ out) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += step) {
+        const uint64_t i = g / dim;
+        const uint32_t t = (uint32_t)(g % dim);
+        out[g] = x[g] - cent[(uint64_t)assign[i] * dim + t];
+    }
+}
+
+extern "C" hipError_t pk_launch_sub_assigned(const float* x, uint32_t n, uint32_t dim, const float* cent,
+                                             const int32_t* assign, float* out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const uint64_t total = (uint64_t)n * dim;
+    hipLaunchKernelGGL(pk_sub_assigned, dim3(gather_blocks(total)), dim3(256), 0, s, x, total, dim, cent, assign,
+                       out);
+    return hipGetLastError();
+}
+
+// Squared norms in cblas_sdot's order (MKL's for 1, 2 or 4 floats, the
+// sequential chain otherwise).
+__global__ void pk_norms_sdot(const float* __restrict__ x, uint32_t n, uint32_t dim, float* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = pk_sdot_short(x + (uint64_t)i * dim, x + (uint64_t)i * dim, dim);
+}
+
+extern "C" hipError_t pk_launch_norms_sdot(const float* x, uint32_t n, uint32_t dim, float* out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(pk_norms_sdot, dim3((n + 255) / 256), dim3(256), 0, s, x, n, dim, out);
+    return hipGetLastError();
+}
+
+// One k-means++ step (kmeanspp_init_center, kmeans.cpp:86-102): the squared
+// distance to the last chosen point, float(double(pn_j + pn_sel) - 2 sdot),
+// lowers each point's best.
+__global__ void pk_kmeanspp_step(const float* __restrict__ x, const float* __restrict__ pn, uint32_t n, uint32_t dim,
+                                 uint32_t sel, float* __restrict__ best) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const float t = (float)((double)(pn[j] + pn[sel]) -
+                            2.0 * (double)pk_sdot_short(x + (uint64_t)j * dim, x + (uint64_t)sel * dim, dim));
+    if (t < best[j]) best[j] = t;
+}
+
+extern "C" hipError_t pk_launch_kmeanspp_step(const float* x, const float* pn, uint32_t n, uint32_t dim,
+                                              uint32_t sel, float* best, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(pk_kmeanspp_step, dim3((n + 255) / 256), dim3(256), 0, s, x, pn, n, dim, sel, best);
+    return hipGetLastError();
+}

@@ -6,7 +6,10 @@
 #include "../../include/puck_gpu.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
+#include <limits>
+#include <random>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -92,6 +95,14 @@ hipError_t pk_launch_insert_stationary(const float*, const float*, uint32_t, uin
                                        hipStream_t);
 hipError_t pk_launch_scatter_rows(const void*, uint64_t, uint64_t, void*, uint64_t, uint32_t, const uint32_t*, uint32_t,
                                   hipStream_t);
+uint64_t pk_ks_hist_words(uint32_t, uint32_t);
+hipError_t pk_launch_sort_by_cluster(const int32_t*, uint32_t, uint32_t, uint32_t*, uint32_t*, uint32_t*, uint32_t*,
+                                     uint32_t*, hipStream_t);
+hipError_t pk_launch_kmeans_update(const float*, uint32_t, const uint32_t*, const uint32_t*, uint32_t, float*, int,
+                                   hipStream_t);
+hipError_t pk_launch_sub_assigned(const float*, uint32_t, uint32_t, const float*, const int32_t*, float*, hipStream_t);
+hipError_t pk_launch_norms_sdot(const float*, uint32_t, uint32_t, float*, hipStream_t);
+hipError_t pk_launch_kmeanspp_step(const float*, const float*, uint32_t, uint32_t, uint32_t, float*, hipStream_t);
 }
 
 enum { PK_EPI_RAW = 0, PK_EPI_SUB_FROM_BIAS = 1, PK_EPI_NEG = 2, PK_EPI_ADD_BIAS = 3 };
@@ -239,7 +250,9 @@ enum Slot {
     S_IN_Q, S_A0, S_A1, S_A2, S_A3, S_A4, S_A5, S_A6, S_A7, S_FLAGS, S_QLIST, S_TIEID, S_TIEN, S_QTOPK, S_QSCC,
     S_QNF, S_PFEAT, S_PWORK, S_PNF, S_PY, S_PT, S_PCV, S_LP16, S_LP8, S_LPS, S_CAND_D, S_CAND_I, S_WLIST, S_SCI,
     S_SCK, S_SCM, S_INS_X, S_INS_Y, S_INS_CELL, S_INS_REC, S_INS_PQ, S_INS_RANK, S_INS_ADD, S_INS_SHIFT, S_INS_WORK,
-    S_INS_DOLD, S_INS_DNEW, S_INS_ACC, S_COUNT
+    S_INS_DOLD, S_INS_DNEW, S_INS_ACC, S_KM_IP, S_KM_PN, S_KM_CN, S_KM_ASSIGN, S_KM_DIS, S_KM_PERM, S_KM_STARTS,
+    S_KM_HIST, S_KM_OFF, S_KM_WORK, S_KM_BEST, S_KM_IDS, S_KM_X, S_KM_TV, S_KM_C0, S_KM_F0, S_KM_CB, S_KM_FB,
+    S_KM_FA, S_KM_CENT, S_COUNT
 };
 
 // Device view of an input array: the pointer itself, or a scratch copy of it.
@@ -2256,5 +2269,365 @@ int puckgpu_index_insert(puckgpu_ctx* ctx, puckgpu_index* ix, uint32_t n, const 
 }
 
 uint64_t puckgpu_index_version(const puckgpu_index* ix) { return ix ? ix->version : 0; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// k-means on the device: Kmeans::kmeans / kmeans_core (kmeans.cpp:149-329) and
+// HierarchicalClusterIndex::train (hierarchical_cluster_index.cpp:882-985).
+// The assignments, centroid updates and residuals run on the GPU; the
+// generator draws (seeding, the empty-cluster split) and the error sum run
+// on the host, in the reference's order.
+namespace {
+
+// The reference's Kmeans object: one std::mt19937 (seeded by the caller
+// instead of time(0), kmeans.h:47-48), the Lloyd iteration limit, the
+// seeding and the centroid-update mode.
+struct KmState {
+    std::mt19937 rnd;
+    int niter = 30;
+    bool pp = true;
+    int mean_mode = 0;
+    explicit KmState(uint32_t seed) : rnd(seed) {}
+};
+
+// roulette_selection (kmeans.cpp:48-67): wheel scaled by (float)(1.0 / sum)
+// in place, then the first entry whose running subtraction from a uniform
+// double goes below zero.
+int km_roulette(KmState& st, std::vector<float>& wheel) {
+    float total = 0;
+    for (float v : wheel) total += v;
+    const float a = (float)(1.0 / total);
+    for (float& v : wheel) v = a * v;
+    std::uniform_real_distribution<double> dis(0, 1.0);
+    double rd = dis(st.rnd);
+    for (int id = 0; id < (int)wheel.size(); ++id) {
+        rd -= wheel[id];
+        if (rd < 0) return id;
+    }
+    return (int)wheel.size() - 1;
+}
+
+// Sequential FMA dot product on the host (the engine's order).
+float km_dot(const float* a, const float* b, uint32_t n) {
+    float acc = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) acc = std::fmaf(a[i], b[i], acc);
+    return acc;
+}
+
+// cblas_sdot's order on the host: MKL's for 1, 2 or 4 floats (rounded
+// products, pairwise sums), km_dot otherwise (pk_sdot_short's order).
+float km_sdot(const float* a, const float* b, uint32_t n) {
+    if (n == 1) return a[0] * b[0];
+    if (n == 2) {
+        const float p0 = a[0] * b[0], p1 = a[1] * b[1];
+        return p0 + p1;
+    }
+    if (n == 4) {
+        const float p0 = a[0] * b[0], p1 = a[1] * b[1], p2 = a[2] * b[2], p3 = a[3] * b[3];
+        const float s01 = p0 + p1, s23 = p2 + p3;
+        return s01 + s23;
+    }
+    return km_dot(a, b, n);
+}
+
+// nearest_center (kmeans.cpp:221-258) of n device points against k device
+// centroids: assign / dis on the device.
+int km_assign(puckgpu_ctx* ctx, uint32_t dim, const float* cent, uint32_t k, uint32_t n, const float* x,
+              int32_t* assign, float* dis, hipStream_t s) {
+    if (n == 0) return PUCKGPU_OK;
+    if (dim == 1 || dim == 2 || dim == 4) {
+        PK_CHECK(pk_launch_kmeans_short(x, n, dim, cent, k, assign, dis, s));
+        return PUCKGPU_OK;
+    }
+    int rc = 0;
+    const uint32_t CHK = (uint32_t)std::max<uint64_t>(1024, std::min<uint64_t>(1u << 20, (256ull << 20) / (4ull * k)));
+    const uint32_t cb = n < CHK ? n : CHK;
+    float* ip = (float*)scratch_slot(ctx, S_KM_IP, 4ull * cb * k, &rc);
+    float* pn = (float*)scratch_slot(ctx, S_KM_PN, 4ull * cb, &rc);
+    float* cn = (float*)scratch_slot(ctx, S_KM_CN, 4ull * k, &rc);
+    if (rc) return rc;
+    PK_CHECK(pk_launch_row_norms(cent, k, dim, 0, cn, s));
+    for (uint32_t p0 = 0; p0 < n; p0 += CHK) {
+        const uint32_t m = (n - p0) < CHK ? (n - p0) : CHK;
+        const float* xp = x + (size_t)p0 * dim;
+        PK_CHECK(pk_launch_row_norms(xp, m, dim, 0, pn, s));
+        PK_CHECK(pk_launch_gemm_exact(xp, cent, ip, m, k, dim, k, PK_EPI_RAW, nullptr, s));
+        PK_CHECK(pk_launch_kmeans_argmin(ip, pn, cn, m, k, assign + p0, dis + p0, s));
+    }
+    return PUCKGPU_OK;
+}
+
+// kmeans_reassign_empty (kmeans.cpp:112-147) on host centroids: every call
+// draws the noise scale; each empty cluster copies a donor picked by
+// roulette over count^2 - 1 and the pair moves apart by a small Gaussian step.
+int km_split(KmState& st, uint32_t dim, uint32_t k, float* cent, const std::vector<uint32_t>& cnt) {
+    std::vector<float> proba(k), eps(dim);
+    std::normal_distribution<> dn(0, st.rnd() / ((double)RAND_MAX + 1.0));
+    for (uint32_t c = 0; c < k; ++c)  // int arithmetic, wrapping as the reference's does
+        proba[c] = cnt[c] < 2 ? 0.f : (float)(int32_t)(uint32_t)((int64_t)cnt[c] * cnt[c] - 1);
+    int nre = 0;
+    for (uint32_t c = 0; c < k; ++c) {
+        if (cnt[c] != 0) continue;
+        nre++;
+        const int j = km_roulette(st, proba);
+        std::memcpy(cent + (size_t)c * dim, cent + (size_t)j * dim, sizeof(float) * dim);
+        float* cj = cent + (size_t)j * dim;
+        const double sc = std::sqrt(km_dot(cj, cj, dim)) * 0.0000001;
+        for (float& v : eps) v = (float)dn(st.rnd);
+        const float a = (float)sc;
+        for (float& v : eps) v = a * v;
+        for (uint32_t t = 0; t < dim; ++t) cj[t] = cj[t] + eps[t];
+        for (uint32_t t = 0; t < dim; ++t) cent[(size_t)c * dim + t] = cent[(size_t)c * dim + t] - eps[t];
+        proba[j] = 0;
+    }
+    return nre;
+}
+
+// kmeans_core (kmeans.cpp:260-329) on device centroids `cent` [k x dim]
+// (updated in place) and points x [n x dim]; assign / dis [n] on the device.
+int km_core(puckgpu_ctx* ctx, KmState& st, uint32_t d, uint32_t n, uint32_t k, int nit, float* cent,
+            const float* x, int32_t* assign, float* dis, double* qerr_out, int* iters, int* split, hipStream_t s) {
+    int rc = 0;
+    const uint64_t hw = pk_ks_hist_words(n, k);
+    uint32_t* perm = (uint32_t*)scratch_slot(ctx, S_KM_PERM, 4ull * (n ? n : 1), &rc);
+    uint32_t* starts = (uint32_t*)scratch_slot(ctx, S_KM_STARTS, 4ull * (k + 1), &rc);
+    uint32_t* hist = (uint32_t*)scratch_slot(ctx, S_KM_HIST, 4ull * (hw ? hw : 1), &rc);
+    uint32_t* off = (uint32_t*)scratch_slot(ctx, S_KM_OFF, 4ull * (hw + 1), &rc);
+    uint32_t* work = (uint32_t*)scratch_slot(ctx, S_KM_WORK, 4ull * pk_exscan_work(hw), &rc);
+    if (rc) return rc;
+    std::vector<uint32_t> st_h(k + 1), cnt(k);
+    std::vector<float> dis_h(n), cent_h;
+    double qerr = std::numeric_limits<double>::max(), qerr_old;
+    int tot = 0, it = 0;
+    for (int iter = 1; iter <= nit; ++iter) {
+        ++it;
+        if ((rc = km_assign(ctx, d, cent, k, n, x, assign, dis, s))) return rc;
+        PK_CHECK(pk_launch_sort_by_cluster(assign, n, k, hist, off, work, perm, starts, s));
+        PK_CHECK(pk_launch_kmeans_update(x, d, perm, starts, k, cent, st.mean_mode, s));
+        PK_CHECK(hipMemcpyAsync(st_h.data(), starts, 4ull * (k + 1), hipMemcpyDeviceToHost, s));
+        PK_CHECK(hipMemcpyAsync(dis_h.data(), dis, 4ull * n, hipMemcpyDeviceToHost, s));
+        PK_CHECK(hipStreamSynchronize(s));
+        bool empty = false;
+        for (uint32_t c = 0; c < k; ++c) {
+            cnt[c] = st_h[c + 1] - st_h[c];
+            empty = empty || cnt[c] == 0;
+        }
+        int nre;
+        if (empty) {
+            cent_h.resize((size_t)k * d);
+            PK_CHECK(hipMemcpy(cent_h.data(), cent, 4ull * k * d, hipMemcpyDeviceToHost));
+            nre = km_split(st, d, k, cent_h.data(), cnt);
+            PK_CHECK(hipMemcpy(cent, cent_h.data(), 4ull * k * d, hipMemcpyHostToDevice));
+        } else {
+            nre = km_split(st, d, k, nullptr, cnt);  // draws the noise scale only
+        }
+        tot += nre;
+        if (tot > (int)(n / 100) && tot > 1000)
+            return set_err(ctx, PUCKGPU_EINVAL, "kmeans: reassigned %d times, abandoning", tot);
+        qerr_old = qerr;
+        qerr = 0;
+        for (uint32_t i = 0; i < n; ++i) qerr += dis_h[i];
+        if (std::fabs(qerr_old - qerr) < 1e-6 && nre == 0) break;
+    }
+    *qerr_out = qerr;
+    if (iters) *iters = it;
+    if (split) *split = tot;
+    return PUCKGPU_OK;
+}
+
+// Kmeans::kmeans (kmeans.cpp:149-219), redo = 1: seeding (k-means++ over the
+// first min(n, 8k) points when n > 8k and n > 8192, kmeans.cpp:173-181; else
+// k distinct uniform draws), then kmeans_core.  cent_out [k x dim] and
+// assign [n] on the device; err = qerr / n.
+int km_run(puckgpu_ctx* ctx, KmState& st, uint32_t d, uint32_t n, uint32_t k, const float* x, float* cent_out,
+           int32_t* assign, float* err, int* iters, int* split, hipStream_t s) {
+    if (k == 0 || k >= n) return set_err(ctx, PUCKGPU_EINVAL, "k-means needs fewer clusters (%u) than points (%u)", k, n);
+    int rc = 0;
+    std::vector<uint32_t> sel(k);
+    if (st.pp) {
+        const uint32_t nsub = ((uint64_t)n > 8ull * k && n > 8u * 1024u) ? 8u * k : n;
+        float* pn = (float*)scratch_slot(ctx, S_KM_PN, 4ull * nsub, &rc);
+        float* best = (float*)scratch_slot(ctx, S_KM_BEST, 4ull * nsub, &rc);
+        if (rc) return rc;
+        std::vector<float> best_h(nsub, std::numeric_limits<float>::max()), tmp;
+        PK_CHECK(hipMemcpyAsync(best, best_h.data(), 4ull * nsub, hipMemcpyHostToDevice, s));
+        PK_CHECK(pk_launch_norms_sdot(x, nsub, d, pn, s));
+        sel[0] = st.rnd() % nsub;
+        for (uint32_t i = 1; i < k; ++i) {
+            PK_CHECK(pk_launch_kmeanspp_step(x, pn, nsub, d, sel[i - 1], best, s));
+            PK_CHECK(hipMemcpyAsync(best_h.data(), best, 4ull * nsub, hipMemcpyDeviceToHost, s));
+            PK_CHECK(hipStreamSynchronize(s));
+            tmp = best_h;
+            sel[i] = (uint32_t)km_roulette(st, tmp);
+        }
+    } else {
+        // random_init_center (kmeans.cpp:30-46)
+        std::uniform_int_distribution<> dis(0, (int)n - 1);
+        std::vector<bool> taken(n, false);
+        for (uint32_t got = 0; got < k;) {
+            const uint32_t v = (uint32_t)dis(st.rnd);
+            if (taken[v]) continue;
+            taken[v] = true;
+            sel[got++] = v;
+        }
+    }
+    uint32_t* ids = (uint32_t*)scratch_slot(ctx, S_KM_IDS, 4ull * k, &rc);
+    float* dis = (float*)scratch_slot(ctx, S_KM_DIS, 4ull * n, &rc);
+    if (rc) return rc;
+    PK_CHECK(hipMemcpyAsync(ids, sel.data(), 4ull * k, hipMemcpyHostToDevice, s));
+    PK_CHECK(pk_launch_gather_f32(x, d, cent_out, ids, k, s));
+    double qerr = 0;
+    if ((rc = km_core(ctx, st, d, n, k, st.niter, cent_out, x, assign, dis, &qerr, iters, split, s))) return rc;
+    *err = (float)(qerr / n);
+    return PUCKGPU_OK;
+}
+
+// MaxHeap (max_heap.cpp:28-102) over host arrays: sentinel-filled, inserts
+// into a non-full heap start at its last free slot, reorder() sorts ascending.
+struct KmHeap {
+    std::vector<float> v;
+    std::vector<uint32_t> t;
+    uint32_t size, free_slots;
+    explicit KmHeap(uint32_t n) : v(n + 1), t(n + 1, 0), size(n), free_slots(n) {
+        float sent;
+        const uint32_t b = 0x7f7f7f7fu;
+        std::memcpy(&sent, &b, 4);
+        std::fill(v.begin(), v.end(), sent);
+    }
+    void sift(uint32_t hs, uint32_t f, float val, uint32_t tag) {
+        for (;;) {
+            const uint32_t l = f << 1, r = l + 1;
+            if (l > hs) break;
+            const uint32_t c = (r > hs || v[l] > v[r]) ? l : r;
+            if (v[c] < val) break;
+            v[f] = v[c];
+            t[f] = t[c];
+            f = c;
+        }
+        v[f] = val;
+        t[f] = tag;
+    }
+    void update(float val, uint32_t tag) {
+        if (val >= v[1]) return;
+        const uint32_t f = free_slots > 0 ? free_slots-- : 1u;
+        sift(size, f, val, tag);
+    }
+    void reorder() {
+        for (uint32_t i = 0; i < size; ++i) {
+            const float vb = v[1];
+            const uint32_t tb = t[1];
+            const uint32_t hs = size - i;
+            sift(hs - 1, 1, v[hs], t[hs]);
+            v[hs] = vb;
+            t[hs] = tb;
+        }
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int puckgpu_kmeans(puckgpu_ctx* ctx, uint32_t dim, uint32_t n, uint32_t k, const float* x, int niter, int init_pp,
+                   uint32_t seed, int mean_mode, float* centroids_out, int32_t* assign_out, float* err_out,
+                   int* iters_out, int* split_out, int device_ptrs) {
+    if (!ctx || !x || !centroids_out) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    if (dim == 0 || niter < 0) return set_err(ctx, PUCKGPU_EINVAL, "bad dim or niter");
+    if (k > 16384) return set_err(ctx, PUCKGPU_EINVAL, "k must be <= 16384");
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    hipStream_t s = ctx->stream;
+    int rc = 0;
+    const float* xd = (const float*)dev_in(ctx, S_KM_X, x, 4ull * n * dim, device_ptrs, &rc);
+    float* cd = device_ptrs ? centroids_out : (float*)scratch_slot(ctx, S_KM_CENT, 4ull * (k ? k : 1) * dim, &rc);
+    int32_t* ad = (device_ptrs && assign_out) ? assign_out
+                                              : (int32_t*)scratch_slot(ctx, S_KM_ASSIGN, 4ull * (n ? n : 1), &rc);
+    if (rc) return rc;
+    KmState st(seed);
+    st.pp = init_pp != 0;
+    st.mean_mode = mean_mode;
+    st.niter = niter;
+    float err = 0;
+    if ((rc = km_run(ctx, st, dim, n, k, xd, cd, ad, &err, iters_out, split_out, s))) return rc;
+    if (!device_ptrs) {
+        PK_CHECK(hipMemcpyAsync(centroids_out, cd, 4ull * k * dim, hipMemcpyDeviceToHost, s));
+        if (assign_out) PK_CHECK(hipMemcpyAsync(assign_out, ad, 4ull * n, hipMemcpyDeviceToHost, s));
+    }
+    PK_CHECK(hipStreamSynchronize(s));
+    if (err_out) *err_out = err;
+    return PUCKGPU_OK;
+}
+
+int puckgpu_train_centers(puckgpu_ctx* ctx, uint32_t dim, uint32_t n, const float* x, uint32_t C, uint32_t F,
+                          int iterations, int init_pp, uint32_t seed, int mean_mode, float* coarse_out,
+                          float* fine_out, float* err_trace, int* rounds_out, int* niter_out, int device_ptrs) {
+    if (!ctx || !x || !coarse_out || !fine_out) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    if (dim == 0 || iterations < 0) return set_err(ctx, PUCKGPU_EINVAL, "bad dim or iterations");
+    if (C > 16384 || F > 16384) return set_err(ctx, PUCKGPU_EINVAL, "C and F must be <= 16384");
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    hipStream_t s = ctx->stream;
+    int rc = 0;
+    const size_t nd = (size_t)n * dim;
+    const float* xd = (const float*)dev_in(ctx, S_KM_X, x, 4ull * nd, device_ptrs, &rc);
+    float* tv = (float*)scratch_slot(ctx, S_KM_TV, 4ull * (nd ? nd : 1), &rc);
+    float* c0 = (float*)scratch_slot(ctx, S_KM_C0, 4ull * (C ? C : 1) * dim, &rc);
+    float* f0 = (float*)scratch_slot(ctx, S_KM_F0, 4ull * (F ? F : 1) * dim, &rc);
+    float* cb = (float*)scratch_slot(ctx, S_KM_CB, 4ull * (C ? C : 1) * dim, &rc);
+    float* fb = (float*)scratch_slot(ctx, S_KM_FB, 4ull * (F ? F : 1) * dim, &rc);
+    int32_t* a = (int32_t*)scratch_slot(ctx, S_KM_ASSIGN, 4ull * (n ? n : 1), &rc);
+    int32_t* fa = (int32_t*)scratch_slot(ctx, S_KM_FA, 4ull * (n ? n : 1), &rc);
+    if (rc) return rc;
+    PK_CHECK(hipMemsetAsync(cb, 0, 4ull * C * dim, s));
+    PK_CHECK(hipMemsetAsync(fb, 0, 4ull * F * dim, s));
+    PK_CHECK(hipMemsetAsync(fa, 0, 4ull * n, s));
+    PK_CHECK(hipMemcpyAsync(tv, xd, 4ull * nd, hipMemcpyDeviceToDevice, s));
+    KmState st(seed);
+    st.pp = init_pp != 0;
+    st.mean_mode = mean_mode;
+    float min_err = std::numeric_limits<float>::max();
+    int rounds = 0;
+    for (int ite = 0; ite < iterations; ++ite) {
+        ++rounds;
+        float err = 0;
+        // coarse centers S on the data less the best fine centers so far
+        if ((rc = km_run(ctx, st, dim, n, C, tv, c0, a, &err, nullptr, nullptr, s))) return rc;
+        if (err_trace) err_trace[2 * ite] = err;
+        PK_CHECK(pk_launch_sub_assigned(xd, n, dim, c0, a, tv, s));
+        // fine centers T on the data less the new coarse centers
+        if ((rc = km_run(ctx, st, dim, n, F, tv, f0, a, &err, nullptr, nullptr, s))) return rc;
+        if (err_trace) err_trace[2 * ite + 1] = err;
+        if ((min_err - err) >= 1e-4) {
+            PK_CHECK(hipMemcpyAsync(cb, c0, 4ull * C * dim, hipMemcpyDeviceToDevice, s));
+            PK_CHECK(hipMemcpyAsync(fb, f0, 4ull * F * dim, hipMemcpyDeviceToDevice, s));
+            PK_CHECK(hipMemcpyAsync(fa, a, 4ull * n, hipMemcpyDeviceToDevice, s));
+            min_err = err;
+        } else {
+            if (st.niter > 80) break;
+            st.niter += 10;
+        }
+        PK_CHECK(pk_launch_sub_assigned(xd, n, dim, fb, fa, tv, s));
+    }
+    // fine centers by ascending -sqrt(|f|^2 / 2) (a MaxHeap sort, :958-980)
+    std::vector<float> fh((size_t)F * dim), ch((size_t)C * dim);
+    PK_CHECK(hipMemcpyAsync(fh.data(), fb, 4ull * F * dim, hipMemcpyDeviceToHost, s));
+    PK_CHECK(hipMemcpyAsync(ch.data(), cb, 4ull * C * dim, hipMemcpyDeviceToHost, s));
+    PK_CHECK(hipStreamSynchronize(s));
+    KmHeap heap(F);
+    for (uint32_t j = 0; j < F; ++j) {
+        const float* f = fh.data() + (size_t)j * dim;
+        const float h = km_sdot(f, f, dim) / 2;
+        heap.update(0 - std::sqrt(h), j);
+    }
+    heap.reorder();
+    std::vector<float> fo((size_t)F * dim);
+    for (uint32_t j = 0; j < F; ++j)
+        std::memcpy(fo.data() + (size_t)j * dim, fh.data() + (size_t)heap.t[j + 1] * dim, 4ull * dim);
+    const hipMemcpyKind out = device_ptrs ? hipMemcpyHostToDevice : hipMemcpyHostToHost;
+    PK_CHECK(hipMemcpy(coarse_out, ch.data(), 4ull * C * dim, out));
+    PK_CHECK(hipMemcpy(fine_out, fo.data(), 4ull * F * dim, out));
+    if (rounds_out) *rounds_out = rounds;
+    if (niter_out) *niter_out = st.niter;
+    return PUCKGPU_OK;
+}
 
 }  // extern "C"

@@ -215,6 +215,48 @@ class Context:
         self.check(self.L.puckgpu_kmeans_assign(self.h, dim, _ptr(centroids), k, n, _ptr(x), _ptr(a), _ptr(dd), m))
         return (a, dd) if with_dist else a
 
+    def kmeans(self, x, k, niter=30, init_pp=True, seed=0, mean_mode=0):
+        """Kmeans::kmeans on the device (puckgpu_kmeans).  Returns
+        (centroids [k, d], assign [n], mean error, iterations, splits); host
+        arrays for host x, device tensors for device x."""
+        n, dim = x.shape
+        _check(x, "x", _F32, device=self.device)
+        if _is_dev(x):
+            import torch
+            cent = torch.empty((k, dim), dtype=torch.float32, device=x.device)
+            a = torch.empty(n, dtype=torch.int32, device=x.device)
+        else:
+            cent = np.zeros((k, dim), np.float32)
+            a = np.zeros(n, np.int32)
+        err = ctypes.c_float(0)
+        it, sp = ctypes.c_int(0), ctypes.c_int(0)
+        self.sync_torch(x)
+        self.check(self.L.puckgpu_kmeans(self.h, dim, n, k, _ptr(x), int(niter), int(bool(init_pp)), int(seed),
+                                         int(mean_mode), _ptr(cent), _ptr(a), ctypes.byref(err), ctypes.byref(it),
+                                         ctypes.byref(sp), _mode(x, cent, a)))
+        return cent, a, float(err.value), it.value, sp.value
+
+    def train_centers(self, x, C, F, iterations=10, init_pp=True, seed=0, mean_mode=0):
+        """HierarchicalClusterIndex::train on the device
+        (puckgpu_train_centers).  Returns (coarse [C, d], fine [F, d] by
+        descending norm, errors [rounds, 2], rounds, final Lloyd limit)."""
+        n, dim = x.shape
+        _check(x, "x", _F32, device=self.device)
+        if _is_dev(x):
+            import torch
+            co = torch.empty((C, dim), dtype=torch.float32, device=x.device)
+            fi = torch.empty((F, dim), dtype=torch.float32, device=x.device)
+        else:
+            co = np.zeros((C, dim), np.float32)
+            fi = np.zeros((F, dim), np.float32)
+        tr = np.zeros(2 * max(int(iterations), 1), np.float32)
+        r, ni = ctypes.c_int(0), ctypes.c_int(0)
+        self.sync_torch(x)
+        self.check(self.L.puckgpu_train_centers(self.h, dim, n, _ptr(x), C, F, int(iterations), int(bool(init_pp)),
+                                                int(seed), int(mean_mode), _ptr(co), _ptr(fi), _ptr(tr),
+                                                ctypes.byref(r), ctypes.byref(ni), _mode(x, co, fi)))
+        return co, fi, tr[:2 * r.value].reshape(-1, 2), r.value, ni.value
+
 
 class GpuIndex:
     """A Puck index resident in HBM (memory order, derived arrays on device)."""
