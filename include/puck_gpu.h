@@ -388,14 +388,15 @@ int puckgpu_kmeans(puckgpu_ctx* ctx, uint32_t dim, uint32_t n, uint32_t k, const
  * k-means on the points less their best fine centers so far, then fine
  * k-means on the points less their new coarse centers; a round whose fine
  * error is at least 1e-4 below the best keeps both, otherwise the Lloyd
- * limit (30 at first) grows by 10 and the rounds end once it exceeds 80.
- * One generator (seed) serves every k-means run.  Fine centers come out in
+ * limit (niter, or the reference's 30 when niter is 0) grows by 10 and the
+ * rounds end once it exceeds 80.  One generator (seed) serves every k-means
+ * run.  Fine centers come out in
  * ascending -sqrt(|f|^2/2) order (a MaxHeap sort, :958-980), i.e. by
  * descending norm.  coarse_out [C x dim], fine_out [F x dim], err_trace
  * [2 x iterations] (coarse and fine error of each round run, may be NULL),
  * rounds_out / niter_out (may be NULL). */
 int puckgpu_train_centers(puckgpu_ctx* ctx, uint32_t dim, uint32_t n, const float* x, uint32_t C, uint32_t F,
-                          int iterations, int init_pp, uint32_t seed, int mean_mode, float* coarse_out,
+                          int iterations, int niter, int init_pp, uint32_t seed, int mean_mode, float* coarse_out,
                           float* fine_out, float* err_trace, int* rounds_out, int* niter_out, int device_ptrs);
 
 #ifdef __cplusplus

@@ -95,7 +95,8 @@ def lib_train():
                                                               _f32p, _i32p, _f32p, dp, ip, ip, _f64p]
         L.pot_train.restype = ctypes.c_int
         L.pot_train.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _f32p, ctypes.c_uint32, ctypes.c_uint32,
-                                ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_int, _f32p, _f32p, _f32p, ip, ip]
+                                ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_int, _f32p, _f32p,
+                                _f32p, ip, ip]
         _train = L
     return _train
 
@@ -134,7 +135,7 @@ def kmeans_core(x, centroids, niter, seed=0, mean_mode=0):
     return cent, a, dis, q.value, it.value, sp.value, tr[:it.value]
 
 
-def train_centers(x, C, F, iterations=10, init_pp=True, seed=0, mean_mode=0):
+def train_centers(x, C, F, iterations=10, init_pp=True, seed=0, mean_mode=0, niter=0):
     """HierarchicalClusterIndex::train (hierarchical_cluster_index.cpp:882-985).
     Returns (coarse [C, d], fine [F, d] by descending norm, errors [rounds, 2],
     rounds run, final niter)."""
@@ -144,7 +145,7 @@ def train_centers(x, C, F, iterations=10, init_pp=True, seed=0, mean_mode=0):
     fi = np.zeros((F, d), np.float32)
     tr = np.zeros(2 * max(iterations, 1), np.float32)
     r, ni = ctypes.c_int(0), ctypes.c_int(0)
-    rc = lib_train().pot_train(d, n, x.reshape(-1), C, F, iterations, int(bool(init_pp)), seed, mean_mode,
+    rc = lib_train().pot_train(d, n, x.reshape(-1), C, F, iterations, niter, int(bool(init_pp)), seed, mean_mode,
                                co.reshape(-1), fi.reshape(-1), tr, ctypes.byref(r), ctypes.byref(ni))
     if rc != 0:
         raise ValueError("pot_train failed")

@@ -213,13 +213,15 @@ int pot_kmeans_core(uint32_t dim, uint32_t n, uint32_t k, const float* x, int ni
  * coarse centers; a round whose fine error is at least 1e-4 below the best
  * keeps both, otherwise niter grows by 10 (the loop ends once it is above
  * 80).  Fine centers are then stored in ascending -sqrt(|f|^2/2) order (a
- * MaxHeap sort).  err_trace [2 * iterations]: coarse and fine errors per
+ * MaxHeap sort).  niter: the first Lloyd limit (0: the reference's 30).
+ * err_trace [2 * iterations]: coarse and fine errors per
  * round run; rounds_out: rounds run; niter_out: the Lloyd iteration limit
  * reached.  Returns 0 / -1. */
-int pot_train(uint32_t dim, uint32_t n, const float* x, uint32_t C, uint32_t F, int iterations, int init_pp,
-              uint32_t seed, int mean_mode, float* coarse_out, float* fine_out, float* err_trace, int* rounds_out,
-              int* niter_out) {
+int pot_train(uint32_t dim, uint32_t n, const float* x, uint32_t C, uint32_t F, int iterations, int niter,
+              int init_pp, uint32_t seed, int mean_mode, float* coarse_out, float* fine_out, float* err_trace,
+              int* rounds_out, int* niter_out) {
     Kmeans km(seed, init_pp != 0, mean_mode);
+    if (niter > 0) km.niter = niter;
     const size_t nd = (size_t)n * dim;
     std::vector<float> tv(x, x + nd), cinit((size_t)C * dim), finit((size_t)F * dim);
     std::vector<float> cbest((size_t)C * dim, 0.f), fbest((size_t)F * dim, 0.f);

@@ -115,8 +115,8 @@ def load():
     L.puckgpu_index_insert.restype = c_int
     L.puckgpu_kmeans.argtypes = [vp, u32, u32, u32, vp, c_int, c_int, u32, c_int, vp, vp, vp, vp, vp, c_int]
     L.puckgpu_kmeans.restype = c_int
-    L.puckgpu_train_centers.argtypes = [vp, u32, u32, vp, u32, u32, c_int, c_int, u32, c_int, vp, vp, vp, vp, vp,
-                                        c_int]
+    L.puckgpu_train_centers.argtypes = [vp, u32, u32, vp, u32, u32, c_int, c_int, c_int, u32, c_int, vp, vp, vp, vp,
+                                        vp, c_int]
     L.puckgpu_train_centers.restype = c_int
     L.puckgpu_index_version.argtypes = [vp]
     L.puckgpu_index_version.restype = ctypes.c_uint64

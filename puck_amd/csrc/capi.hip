@@ -2559,10 +2559,10 @@ int puckgpu_kmeans(puckgpu_ctx* ctx, uint32_t dim, uint32_t n, uint32_t k, const
 }
 
 int puckgpu_train_centers(puckgpu_ctx* ctx, uint32_t dim, uint32_t n, const float* x, uint32_t C, uint32_t F,
-                          int iterations, int init_pp, uint32_t seed, int mean_mode, float* coarse_out,
+                          int iterations, int niter, int init_pp, uint32_t seed, int mean_mode, float* coarse_out,
                           float* fine_out, float* err_trace, int* rounds_out, int* niter_out, int device_ptrs) {
     if (!ctx || !x || !coarse_out || !fine_out) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
-    if (dim == 0 || iterations < 0) return set_err(ctx, PUCKGPU_EINVAL, "bad dim or iterations");
+    if (dim == 0 || iterations < 0 || niter < 0) return set_err(ctx, PUCKGPU_EINVAL, "bad dim, iterations or niter");
     if (C > 16384 || F > 16384) return set_err(ctx, PUCKGPU_EINVAL, "C and F must be <= 16384");
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
     hipStream_t s = ctx->stream;
@@ -2584,6 +2584,7 @@ int puckgpu_train_centers(puckgpu_ctx* ctx, uint32_t dim, uint32_t n, const floa
     KmState st(seed);
     st.pp = init_pp != 0;
     st.mean_mode = mean_mode;
+    if (niter > 0) st.niter = niter;
     float min_err = std::numeric_limits<float>::max();
     int rounds = 0;
     for (int ite = 0; ite < iterations; ++ite) {

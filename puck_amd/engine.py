@@ -236,7 +236,7 @@ class Context:
                                          ctypes.byref(sp), _mode(x, cent, a)))
         return cent, a, float(err.value), it.value, sp.value
 
-    def train_centers(self, x, C, F, iterations=10, init_pp=True, seed=0, mean_mode=0):
+    def train_centers(self, x, C, F, iterations=10, init_pp=True, seed=0, mean_mode=0, niter=0):
         """HierarchicalClusterIndex::train on the device
         (puckgpu_train_centers).  Returns (coarse [C, d], fine [F, d] by
         descending norm, errors [rounds, 2], rounds, final Lloyd limit)."""
@@ -252,7 +252,8 @@ class Context:
         tr = np.zeros(2 * max(int(iterations), 1), np.float32)
         r, ni = ctypes.c_int(0), ctypes.c_int(0)
         self.sync_torch(x)
-        self.check(self.L.puckgpu_train_centers(self.h, dim, n, _ptr(x), C, F, int(iterations), int(bool(init_pp)),
+        self.check(self.L.puckgpu_train_centers(self.h, dim, n, _ptr(x), C, F, int(iterations), int(niter),
+                                                int(bool(init_pp)),
                                                 int(seed), int(mean_mode), _ptr(co), _ptr(fi), _ptr(tr),
                                                 ctypes.byref(r), ctypes.byref(ni), _mode(x, co, fi)))
         return co, fi, tr[:2 * r.value].reshape(-1, 2), r.value, ni.value

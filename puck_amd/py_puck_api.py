@@ -116,7 +116,12 @@ class PySearcher:
         whether_norm = FL.get("whether_norm")
         xd = torch.from_numpy(x).cuda()
         if whether_norm:
-            xd = xd / xd.norm(dim=1, keepdim=True).clamp_min(1e-30)
+            # the reference's callers write normalised features (the Python
+            # wrapper divides by the norm before writing all_data.feat.bin);
+            # here every row is scaled by float32(1 / norm), the norm taken in
+            # float64
+            alpha = (1.0 / xd.double().square().sum(1).sqrt().clamp_min(1e-30)).float()
+            xd = xd * alpha[:, None]
         if self._ctx is None:
             self._ctx = Context(0)
         pq_nsq = None
@@ -128,7 +133,7 @@ class PySearcher:
                 return -1
         ix = train.build(self._ctx, xd.contiguous(), FL.get("coarse_cluster_count"), FL.get("fine_cluster_count"),
                          nsq, whether_norm, train_n=min(FL.get("train_points_count"), total_cnt),
-                         niter=FL.get("kmeans_iterations_count"), pq_nsq=pq_nsq)
+                         rounds=FL.get("kmeans_iterations_count"), pq_nsq=pq_nsq)
         h = {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in ix.items()}
         index_io.write_index(path, FL, h["coarse"], h["fine"], h["codebook"], h["cell_assign"].view(np.uint32),
                              h["records"], None if pq_nsq else (x if not whether_norm else h["raw"]), whether_norm,

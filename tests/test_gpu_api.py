@@ -80,10 +80,13 @@ def test_searcher_build_then_search(tmp_path, whether_pq):
     if whether_pq:
         assert conf["nsq"] == 32  # one subspace per dimension by default (index_conf.cpp:39-49)
         oi.set_pq(arr["pq_codebook"], arr["pq_records"])
-        # the GPU's PQ records equal the oracle's encoding of the same points
-        h = O.encode_puck(arr["coarse"], arr["fine"], arr["codebook"], 16, arr["pq_codebook"], 32,
-                          x / np.linalg.norm(x, axis=1, keepdims=True).astype(np.float32), arr["cell_assign"])[1]
-        assert np.mean(np.all(h == arr["pq_records"], axis=1)) > 0.99
+        # the GPU's PQ records equal the oracle's encoding of the same
+        # (normalised, as the build normalises them) points
+        alpha = (1.0 / np.sqrt((x.astype(np.float64) ** 2).sum(1))).astype(np.float32)
+        xn = (x * alpha[:, None]).astype(np.float32)
+        h = O.encode_puck(arr["coarse"], arr["fine"], arr["codebook"], 16, arr["pq_codebook"], 32, xn,
+                          arr["cell_assign"])[1]
+        assert np.array_equal(h, arr["pq_records"])
     Do, Io, _, _ = oi.search_batch(q, 10, 4, 100)
     assert np.array_equal(Im, Io)
     assert np.array_equal(_bits(Dm), _bits(Do))

@@ -527,8 +527,8 @@ def test_index_build_reproducible(ctx):
     import torch
     from puck_amd import train
     x = torch.from_numpy(D.unit_norm_mixture(20000, 32, 50, seed=80, means_seed=81)).cuda()
-    b1 = train.build(ctx, x, 32, 32, 16, whether_norm=True, train_n=8000, niter=4)
-    b2 = train.build(ctx, x, 32, 32, 16, whether_norm=True, train_n=8000, niter=4)
+    b1 = train.build(ctx, x, 32, 32, 16, whether_norm=True, train_n=8000, niter=4, rounds=2)
+    b2 = train.build(ctx, x, 32, 32, 16, whether_norm=True, train_n=8000, niter=4, rounds=2)
     for k in ("coarse", "fine", "codebook", "cell_assign", "records"):
         assert torch.equal(b1[k], b2[k]), k
     assert torch.isfinite(b1["coarse"]).all() and torch.isfinite(b1["fine"]).all()

@@ -45,7 +45,7 @@ def w1m():
     x = datasets.mixture_uint8_grid(N, d, 1000, seed=0, means_seed=1000, backend="torch", device=dev)
     q = datasets.mixture_uint8_grid(nq, d, 1000, seed=1, means_seed=1000, backend="torch", device=dev)
     ctx = Context(0)
-    ix = train.build(ctx, x, C, F, nsq, whether_norm=False, train_n=200_000, niter=10)
+    ix = train.build(ctx, x, C, F, nsq, whether_norm=False, train_n=200_000, niter=10, rounds=3)
     torch.cuda.synchronize()
     _log(f"data + train + assign + encode {time.time() - t0:.1f}s")
     gi = GpuIndex(ctx, ix["coarse"], ix["fine"], ix["codebook"], ix["cell_assign"], ix["records"], ix["raw"], False)

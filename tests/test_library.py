@@ -41,22 +41,6 @@ def test_library_rejects_null_context():
     assert L.puckgpu_last_error(None) == b"null context"
 
 
-def test_cluster_sums_fixed_order():
-    # the k-means centroid sums: equal to a float64 reference, identical on every call
-    import torch
-    from puck_amd.train import cluster_sums
-    g = torch.Generator().manual_seed(3)
-    x = torch.randn(5000, 7, generator=g)
-    a = torch.randint(0, 40, (5000,), generator=g)
-    a[a == 13] = 12  # an empty cluster
-    s1, c1 = cluster_sums(x, a, 40)
-    s2, c2 = cluster_sums(x, a, 40)
-    ref = torch.zeros(40, 7, dtype=torch.float64).index_add_(0, a, x.double())
-    assert torch.equal(c1, torch.bincount(a, minlength=40)) and c1[13] == 0
-    assert torch.allclose(s1, ref, rtol=0, atol=1e-9)
-    assert torch.equal(s1, s2) and torch.equal(c1, c2)
-
-
 def test_bench_refuses_more_than_one_gpu():
     # the bench measures one GPU; --gpus N > 1 exits with a message before touching a device
     import json
