@@ -150,7 +150,12 @@ static hipError_t pk_mm(bool at, bool bt, const PkMmArgs& a, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // Raw features of n queries from the search pipeline's sorted rows: the
 // query, the nc smallest coarse distances (csd, stride cstride) and the nfine
-// smallest cell distances of the nearest list (cell, stride nfine).
+// smallest cell distances of the nearest list (cell, stride nfine,
+// ascending).  An empty cell's distance carries the stationary sentinel
+// sqrt(FLT_MAX); such an entry (at or above half the sentinel) takes the
+// row's largest populated-cell distance instead (0 for a list with no
+// point), so that features stay in the range of the data.
+#define PK_EMPTY_CELL_FEATURE 9.2233715e18f
 __global__ void pk_probe_raw_features(const float* __restrict__ q, uint32_t dim, const float* __restrict__ csd,
                                       uint32_t cstride, uint32_t nc, const float* __restrict__ cell,
                                       uint32_t nfine, uint32_t n, float* __restrict__ out) {
@@ -161,7 +166,19 @@ __global__ void pk_probe_raw_features(const float* __restrict__ q, uint32_t dim,
     float v;
     if (j < dim) v = q[(uint64_t)i * dim + j];
     else if (j < dim + nc) v = csd[(uint64_t)i * cstride + (j - dim)];
-    else v = cell[(uint64_t)i * nfine + (j - dim - nc)];
+    else {
+        const float* row = cell + (uint64_t)i * nfine;
+        const uint32_t jj = j - dim - nc;
+        v = row[jj];
+        if (v >= PK_EMPTY_CELL_FEATURE) {
+            v = 0.f;
+            for (uint32_t t = jj; t-- > 0;)
+                if (row[t] < PK_EMPTY_CELL_FEATURE) {
+                    v = row[t];
+                    break;
+                }
+        }
+    }
     out[e] = v;
 }
 

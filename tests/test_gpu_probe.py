@@ -120,6 +120,17 @@ def test_probe_training_reduces_loss(ctx):
     assert last < 0.1 * first, (first, last)
 
 
+def _clamp_empty(row):
+    # an empty cell's distance (the stationary sentinel sqrt(FLT_MAX)) becomes
+    # the row's largest populated-cell distance (rows ascending), 0 if none
+    row = np.array(row, np.float32)
+    lim = np.float32(np.sqrt(np.finfo(np.float32).max, dtype=np.float32) / np.float32(2))
+    for t in range(row.shape[0]):
+        if row[t] >= lim:
+            row[t] = row[t - 1] if t > 0 else np.float32(0)
+    return row
+
+
 def test_probe_features_match_oracle(ctx, idx_cos):
     ix, q = idx_cos
     gi = _gpu_index(ctx, ix)
@@ -135,7 +146,7 @@ def test_probe_features_match_oracle(ctx, idx_cos):
         assert np.array_equal(f[k, 96:96 + nc].view(np.uint32), np.sort(cd)[:nc].view(np.uint32))
         c0 = int(np.argmin(cd))
         cell = (cd[c0] + o["stationary"].reshape(40, 40)[c0]) + fd      # float32, reference order
-        assert np.array_equal(f[k, 96 + nc:].view(np.uint32), np.sort(cell)[:nf].view(np.uint32))
+        assert np.array_equal(f[k, 96 + nc:].view(np.uint32), _clamp_empty(np.sort(cell)[:nf]).view(np.uint32))
         assert np.array_equal(fd[fo[k]], np.sort(fd))
         assert np.array_equal(np.sort(fo[k]), np.arange(40))
 
@@ -164,7 +175,9 @@ def test_probe_features_rows_over_64(ctx, idx_mid):
             assert np.array_equal(f[k, d:d + nc].view(np.uint32), np.sort(cd)[:nc].view(np.uint32)), (nc, k)
             c0 = int(np.argmin(cd))
             cell = (cd[c0] + o["stationary"].reshape(C, F)[c0]) + fd
-            assert np.array_equal(f[k, d + nc:].view(np.uint32), np.sort(cell)[:nf].view(np.uint32)), (nf, k)
+            want = _clamp_empty(np.sort(cell)[:nf])
+            assert np.array_equal(f[k, d + nc:].view(np.uint32), want.view(np.uint32)), (nf, k)
+            assert np.all(np.isfinite(f[k])) and np.abs(f[k]).max() < 1e18, k
 
 
 @pytest.mark.parametrize("scc,ftk,topk", [(1, 100, 10), (16, 300, 10), (64, 500, 20)])
