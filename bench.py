@@ -455,7 +455,78 @@ def precision_forms(gi, ix, q, gt_i, gt_d, topk, scc, ftk, out, args, learned_ru
                                                                    topk)[0], 5)
         res[name] = entry
         log(f"precision {name}: {entry}")
+    res["fp16_lut"] = fp16_lut_form(gi, q, gt_i, gt_d, topk, scc, ftk, out, exact_ids, learned_run)
     return res
+
+
+def fp16_lut_form(gi, q, gt_i, gt_d, topk, scc, ftk, out, exact_ids, learned_run, nsample=1000):
+    """The FP16 filter table (PUCKGPU_LUT_FP16) beside the FP32 one: QPS and
+    recall of the plain and learned forms, the scan stage time, and what the
+    rounding changes -- on a query sample with the learned budgets held fixed,
+    the filter survivors shared with the FP32 table and the filter-distance
+    difference of the shared ones (the FP32 table's distance is the
+    reference's arithmetic; the returned distances are the exact FP32 rerank
+    in both)."""
+    import torch
+    from puck_amd.metrics import recall_with_ties
+    lut16 = 16
+    nq = q.shape[0]
+    entry = {}
+
+    def timed(run, reps=5):
+        run()
+        torch.cuda.synchronize()
+        t1 = time.time()
+        for _ in range(reps):
+            run()
+        torch.cuda.synchronize()
+        return reps * nq / (time.time() - t1)
+
+    entry["qps"] = round(timed(lambda: gi.search(q, topk, scc, ftk, out=out, precision=lut16)), 1)
+    ids = out[1].cpu().numpy()
+    entry["recall_at_10"] = round(recall_with_ties(gt_i, gt_d, ids.astype(np.int64), topk)[0], 5)
+    entry["slot_agreement_with_fp32_exact"] = round(float((ids == exact_ids).mean()), 5)
+    if learned_run is not None:
+        bud = learned_run(0)[3].clone()
+        ids32 = out[1].cpu().numpy().copy()
+        ms32, _ = gi.last_stage_ms()
+        entry["learned_qps"] = round(timed(lambda: learned_run(lut16)), 1)
+        ms16, _ = gi.last_stage_ms()
+        ids16 = out[1].cpu().numpy()
+        entry["learned_recall_at_10"] = round(recall_with_ties(gt_i, gt_d, ids16.astype(np.int64), topk)[0], 5)
+        entry["learned_slot_agreement_with_fp32_table"] = round(float((ids16 == ids32).mean()), 5)
+        entry["learned_scan_ms_fp32_table"] = round(float(ms32[2]), 4)
+        entry["learned_scan_ms_fp16_table"] = round(float(ms16[2]), 4)
+    # filter sets at fixed budgets (the learned ones, or a uniform 128)
+    qs = q[:nsample]
+    nb = torch.full((qs.shape[0],), 128, dtype=torch.int32, device=q.device)
+    if learned_run is not None:
+        nb = bud[:nsample].contiguous()
+    gi.search_batch(qs, topk, scc, ftk, fine_budget=nb)
+    f32v, f32i, f32n = gi.last_filter(qs.shape[0], ftk)
+    gi.search_batch(qs, topk, scc, ftk, fine_budget=nb, precision=lut16)
+    f16v, f16i, f16n = gi.last_filter(qs.shape[0], ftk)
+    shared, tot, diffs, rels = 0, 0, [], []
+    for k in range(qs.shape[0]):
+        a = dict(zip(f32i[k, :f32n[k]].tolist(), f32v[k, :f32n[k]].astype(np.float64)))
+        b = dict(zip(f16i[k, :f16n[k]].tolist(), f16v[k, :f16n[k]].astype(np.float64)))
+        com = a.keys() & b.keys()
+        shared += len(com)
+        tot += len(a)
+        if com:
+            dv = np.array([abs(b[m] - a[m]) for m in com])
+            mag = np.array([abs(a[m]) for m in com])
+            diffs.append(dv)
+            rels.append(dv / np.maximum(mag, 1e-12))
+    dv = np.concatenate(diffs) if diffs else np.zeros(1)
+    rv = np.concatenate(rels) if rels else np.zeros(1)
+    entry["filter_survivors_shared_with_fp32_table"] = round(shared / max(tot, 1), 5)
+    entry["filter_distance_abs_diff_max"] = float(dv.max())
+    entry["filter_distance_abs_diff_mean"] = float(dv.mean())
+    entry["filter_distance_rel_diff_max"] = float(rv.max())
+    entry["returned_distances"] = "exact FP32 rerank (the table's rounding moves only the filter distances)"
+    log(f"precision fp16_lut: {entry}")
+    return entry
 
 
 def serving(gi, dev, d, nq_total, max_batch=4096):

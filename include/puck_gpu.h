@@ -87,8 +87,22 @@ typedef struct {
                                      the rerank stays exact FP32, so returned distances are exact; the
                                      fine distances, cell distances and filter distances carry the reduced
                                      precision and results are not bit-identical to the reference.  No
-                                     tie replay is made in these forms. */
+                                     tie replay is made in these forms.
+                                     Or-ed with PUCKGPU_LUT_FP16: the filter lookup table in FP16 (see
+                                     below), with either center precision. */
 } puckgpu_search_params;
+
+/* Precision flag (no counterpart in the reference, whose filter table is FP32,
+ * quantization.cpp:260-270): the filter lookup table of each query is built in
+ * FP16 -- every entry ||q_m - c||^2 is computed in FP32, scaled by 2^e and
+ * rounded to FP16 (nearest even), where e = floor(log2(65504 / (|q| + cbn)^2))
+ * clamped to [-30, 30] and cbn is the largest codeword norm, so that no entry
+ * overflows; lookups are summed in FP32 in the reference's 4-lane order and the
+ * sum is scaled by 2^-e.  The budget scan keeps the table in FP16 in LDS (half
+ * the space); the walk and the per-query scan hold the same rounded values in
+ * FP32, which gives the same sums bit for bit.  Filter distances carry the
+ * rounding (relative error below 2^-11 per entry); the rerank stays exact. */
+#define PUCKGPU_LUT_FP16 16u
 
 /* Context: one device, one stream, reusable scratch.  device = HIP ordinal. */
 int puckgpu_create(int device, puckgpu_ctx** out);

@@ -217,6 +217,14 @@ def load_variant(path):
     L.po_index_position.argtypes = [vp, ctypes.c_uint32]
     L.po_index_inserted.restype = ctypes.c_uint32
     L.po_index_inserted.argtypes = [vp]
+    L.po_half_round.restype = ctypes.c_float
+    L.po_half_round.argtypes = [ctypes.c_float]
+    L.po_lut16_exp.restype = ctypes.c_int
+    L.po_lut16_exp.argtypes = [ctypes.c_float, ctypes.c_float]
+    L.po_lut16.restype = None
+    L.po_lut16.argtypes = [_f32p, ctypes.c_uint32, ctypes.c_float, ctypes.c_float]
+    L.po_index_set_lut16.restype = None
+    L.po_index_set_lut16.argtypes = [vp, ctypes.c_int]
     L.po_walk_trace.restype = ctypes.c_int
     L.po_walk_trace.argtypes = [vp, _f32p, _f32p, _f32p, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32,
                                 ctypes.c_double, ctypes.POINTER(WalkStep), ctypes.c_uint32,
@@ -277,6 +285,23 @@ def lookup(codes, lut_, nsq):
     return float(lib().po_lookup(np.ascontiguousarray(codes, np.uint8), c32(lut_), nsq))
 
 
+def half_round(v):
+    """FP16 round-to-nearest-even of one float, returned as float32."""
+    return np.float32(lib().po_half_round(ctypes.c_float(float(v))))
+
+
+def lut16_exp(qn, cbn):
+    """Scale exponent of the FP16 filter table for |q|^2 = qn, codeword norm bound cbn."""
+    return int(lib().po_lut16_exp(ctypes.c_float(float(qn)), ctypes.c_float(float(cbn))))
+
+
+def lut16(lut_, qn, cbn):
+    """The FP16 filter table's values in float32: half(v * 2^e) * 2^-e."""
+    out = c32(lut_).copy()
+    lib().po_lut16(out, out.size, ctypes.c_float(float(qn)), ctypes.c_float(float(cbn)))
+    return out
+
+
 def rerank_dist(x, q):
     x, q = c32(x), c32(q)
     return float(lib().po_rerank_dist(x, q, x.size))
@@ -319,6 +344,10 @@ class OracleIndex:
         if getattr(self, "_h", None):
             self._L.po_index_destroy(self._h)
             self._h = None
+
+    def set_lut16(self, on=True):
+        """Searches use the FP16 filter table (the engine's PUCKGPU_LUT_FP16 form)."""
+        self._L.po_index_set_lut16(self._h, 1 if on else 0)
 
     def set_pq(self, pq_codebook, pq_records):
         """Second-stage PQ (whether_pq = true): codebook [pnsq, 256, dim/pnsq]

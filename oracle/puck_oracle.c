@@ -205,6 +205,60 @@ float po_lookup(const uint8_t* codes, const float* lut, uint32_t nsq) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* FP16 filter table (a precision form of the engine; no counterpart in the
+ * reference, whose table is FP32).  po_half_round rounds a float to the
+ * nearest IEEE binary16 value (ties to even; subnormals below 2^-14 in steps
+ * of 2^-24; magnitudes from 65520 up become infinite) and returns it as a
+ * float.  po_lut16_exp(qn, cbn) is the table's scale exponent:
+ * floor(log2(65504 / (sqrt(qn) + cbn)^2)) clamped to [-30, 30], qn = |q|^2 and
+ * cbn the largest codeword norm, so that no entry ||q_m - c||^2 <=
+ * (|q| + cbn)^2 overflows.  po_lut16 replaces each entry v by
+ * half(v * 2^e) * 2^-e (both scalings exact). */
+float po_half_round(float v) {
+    if (!(v == v)) return v;
+    float a = fabsf(v);
+    if (a >= 65520.0f) return copysignf(INFINITY, v);
+    int e = 0;
+    (void)frexpf(a, &e);          /* a = m * 2^e, m in [0.5, 1) */
+    int lead = e - 1;             /* exponent of the leading bit */
+    if (lead < -14) lead = -14;   /* binary16 subnormals: quantum 2^-24 */
+    float quantum = ldexpf(1.0f, lead - 10);
+    float r = rintf(a / quantum) * quantum;  /* a / quantum is exact; rintf: ties to even */
+    return copysignf(r, v);
+}
+
+int po_lut16_exp(float qn, float cbn) {
+    float r = sqrtf(qn) + cbn;
+    float B = r * r;
+    float x = 65504.0f / B;
+    int e = 30;
+    if (x < INFINITY) {
+        int ex = 0;
+        (void)frexpf(x, &ex);
+        e = ex - 1;
+    }
+    return e < -30 ? -30 : (e > 30 ? 30 : e);
+}
+
+void po_lut16(float* lut, uint32_t n, float qn, float cbn) {
+    int e = po_lut16_exp(qn, cbn);
+    float sc = ldexpf(1.0f, e), inv = ldexpf(1.0f, -e);
+    for (uint32_t i = 0; i < n; ++i) lut[i] = po_half_round(lut[i] * sc) * inv;
+}
+
+/* Largest codeword norm: sqrt of the largest sum of rounded squares, summed in
+ * order over each codeword's lsq floats. */
+static float po_codebook_max_norm(const float* cb, uint32_t n, uint32_t lsq) {
+    float mx = 0.f;
+    for (uint32_t k = 0; k < n; ++k) {
+        float sq = 0.f;
+        for (uint32_t j = 0; j < lsq; ++j) sq = sq + cb[(size_t)k * lsq + j] * cb[(size_t)k * lsq + j];
+        if (sq > mx) mx = sq;
+    }
+    return sqrtf(mx);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Exact squared L2 used by the rerank: similarity::L2Sqr16Ext and L2SqrExt
  * (puck/tinker/method/hnsw_distfunc_opt_impl_inline.h:41-122), called in
  * blocks of 16 by rank_topN_points (puck_index.cpp:428-453).
@@ -290,6 +344,10 @@ typedef struct {
     uint32_t* ins_cnt;    /* C*F: inserted points of the cell */
     uint32_t* ins_before; /* C*F: inserted points in the cells before (memory positions) */
     float* fbound;        /* F: -sqrt(|f|^2/2), the assignment's fine bound */
+    /* FP16 filter table (the engine's precision form; the reference's table
+     * is FP32): non-zero lut16 quantizes every table entry as po_lut16 does */
+    int lut16;
+    float cb_nmax;        /* largest codeword norm of the filter codebook */
 } po_index;
 
 #define PO_NONE 0xffffffffu
@@ -396,6 +454,13 @@ void* po_index_create(uint32_t dim, uint32_t C, uint32_t F, uint32_t nsq, uint32
     }
     free(local2mem);
     return ix;
+}
+
+/* Switches the index's searches to the FP16 filter table (on != 0) or back. */
+void po_index_set_lut16(void* p, int on) {
+    po_index* ix = (po_index*)p;
+    ix->lut16 = on;
+    ix->cb_nmax = po_codebook_max_norm(ix->codebook, ix->nsq * PO_KS, ix->lsq);
 }
 
 void po_index_destroy(void* p) {
@@ -549,6 +614,7 @@ static int po_search_budget(const po_index* ix, const float* q, uint32_t topk, u
     for (uint32_t k = 0; k < F; ++k) fd[k] = -po_dot(ix->fine + (size_t)k * dim, q, dim);
     float* lut = (float*)malloc(sizeof(float) * nsq * PO_KS);
     po_lut(q, ix->codebook, nsq, ix->lsq, lut);
+    if (ix->lut16) po_lut16(lut, nsq * PO_KS, po_dot(q, q, dim), ix->cb_nmax);
     uint32_t* keys = (uint32_t*)malloc(sizeof(uint32_t) * F);
     uint32_t* sorted = (uint32_t*)malloc(sizeof(uint32_t) * F);
     size_t cap = 1024, n = 0;
@@ -726,6 +792,7 @@ static void po_walk_query(const po_index* ix, const float* q, uint32_t topk, uin
     /* pre_filter_search -> get_dist_table (puck_index.cpp:459-483) */
     float* lut = (float*)malloc(sizeof(float) * nsq * PO_KS);
     po_lut(q, ix->codebook, nsq, ix->lsq, lut);
+    if (ix->lut16) po_lut16(lut, nsq * PO_KS, qn_in ? *qn_in : po_dot(q, q, dim), ix->cb_nmax);
     /* fine GEMV + full heap sort (puck_index.cpp:494-504) */
     float* fine_d = (float*)malloc(sizeof(float) * F);
     uint32_t* fine_t = (uint32_t*)malloc(sizeof(uint32_t) * F);

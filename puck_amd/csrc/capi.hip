@@ -140,6 +140,7 @@ struct puckgpu_index {
     float* coarse = nullptr;
     float* fine = nullptr;
     float* codebook = nullptr;
+    float cb_nmax = 0.f;  // largest filter codeword norm (scale of the FP16 table)
     float* coarse_norms = nullptr;
     float* stationary = nullptr;
     uint32_t* cell_start = nullptr;
@@ -401,6 +402,20 @@ int puckgpu_index_load(puckgpu_ctx* ctx, const puckgpu_index_params* params, con
     PK_TRYH(hipMemcpyAsync(ix->coarse, coarse, sizeof(float) * C * d, kind, s));
     PK_TRYH(hipMemcpyAsync(ix->fine, fine, sizeof(float) * F * d, kind, s));
     PK_TRYH(hipMemcpyAsync(ix->codebook, filter_codebook, sizeof(float) * nsq * PK_KS * lsq, kind, s));
+    {
+        // largest codeword norm: sqrt of the largest sum of rounded squares,
+        // summed in order (the FP16 table's scale bound)
+        std::vector<float> hcb((size_t)nsq * PK_KS * lsq);
+        PK_TRYH(hipMemcpyAsync(hcb.data(), ix->codebook, sizeof(float) * hcb.size(), hipMemcpyDeviceToHost, s));
+        PK_TRYH(hipStreamSynchronize(s));
+        float mx = 0.f;
+        for (size_t k = 0; k < (size_t)nsq * PK_KS; ++k) {
+            float sq = 0.f;
+            for (uint32_t j = 0; j < lsq; ++j) sq = sq + hcb[k * lsq + j] * hcb[k * lsq + j];
+            if (sq > mx) mx = sq;
+        }
+        ix->cb_nmax = sqrtf(mx);
+    }
 
     // |c|^2/2, |f|^2/2 and stationary[c][f] = |f|^2/2 + <f,c> (read_coodbooks)
     float* fnorm = (float*)scratch_slot(ctx, S_A0, sizeof(float) * F, &rc);
@@ -639,6 +654,7 @@ constexpr uint32_t PK_SC_CAPP = 8192;
 struct PassCfg {
     uint32_t scc = 0, keep = 0, ftk = 0, topk = 0, ostride = 0;
     int precision = 0;
+    int lut16 = 0;    // FP16 filter table
     uint32_t kc = 0;  // coarse candidates rescored in FP32 (precision 1 / 2)
     double rr = 1.0;
     int gemm_shape = 0;
@@ -734,6 +750,7 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     wa.wlist = nullptr; wa.wcount = nullptr;
     wa.sc_items = b.sc_items; wa.sc_keys = b.sc_keys; wa.sc_meta = b.sc_meta; wa.sc_olist = b.sc_olist;
     wa.sc_ocount = b.sc_ocount; wa.sc_capp = PK_SC_CAPP;
+    wa.lut16 = cfg.lut16; wa.cb_nmax = ix->cb_nmax;
     if (nf) {
         // the walk runs only over the queries without a budget below F (the list built above)
         wa.wlist = b.wlist;
@@ -925,8 +942,10 @@ int make_cfg(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_par
     cfg.pu = pu;
     if (!ix->raw && !ix->pq_codes && ix->p.total_point_count)
         return set_err(ctx, PUCKGPU_EINVAL, "the index holds neither raw vectors nor a second-stage PQ to rank by");
-    if (sp->precision > 2) return set_err(ctx, PUCKGPU_EINVAL, "precision must be 0, 1 or 2");
-    cfg.precision = (int)sp->precision;
+    if ((sp->precision & ~(15u | PUCKGPU_LUT_FP16)) || (sp->precision & 15u) > 2)
+        return set_err(ctx, PUCKGPU_EINVAL, "precision must be 0, 1 or 2, optionally with PUCKGPU_LUT_FP16");
+    cfg.precision = (int)(sp->precision & 15u);
+    cfg.lut16 = (sp->precision & PUCKGPU_LUT_FP16) ? 1 : 0;
     if (cfg.precision) {
         if (pkc_ld(d) > 256) return set_err(ctx, PUCKGPU_EINVAL, "reduced precision needs dim <= 256");
         // margin of 32 coarse candidates (at most 512 and C) rescored in FP32

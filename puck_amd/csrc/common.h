@@ -5,6 +5,7 @@
 // sequential fmaf chains from +0.0f (the engine's exact-arithmetic order); the
 // lookup-table and rerank orders follow the reference's SIMD kernels.
 #pragma once
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -99,9 +100,9 @@ __device__ __forceinline__ float pk_lut_entry(const float* x, const float* y, ui
 #ifndef PK_LUT_UNROLL
 #define PK_LUT_UNROLL 8
 #endif
-template <int LSQ>
-__device__ void pk_build_lut_t(float* __restrict__ lut, const float* __restrict__ qv,
-                               const float* __restrict__ codebook, uint32_t n) {
+template <int LSQ, typename Put>
+__device__ void pk_build_lut_t(const float* __restrict__ qv, const float* __restrict__ codebook, uint32_t n,
+                               Put put) {
     constexpr int U = PK_LUT_UNROLL;
     for (uint32_t e0 = threadIdx.x; e0 < n; e0 += U * blockDim.x) {
         float c[U][LSQ];
@@ -127,16 +128,63 @@ __device__ void pk_build_lut_t(float* __restrict__ lut, const float* __restrict_
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint32_t e = e0 + u * blockDim.x;
-            if (e < n) lut[e] = pk_lut_entry(qv + (e / 256u) * LSQ, c[u], LSQ);
+            if (e < n) put(e, pk_lut_entry(qv + (e / 256u) * LSQ, c[u], LSQ));
         }
     }
 }
 
+template <typename Put>
+__device__ __forceinline__ void pk_build_lut_put(const float* qv, const float* codebook, uint32_t nsq, uint32_t lsq,
+                                                 Put put) {
+    if (lsq == 1) pk_build_lut_t<1>(qv, codebook, nsq * 256u, put);
+    else if (lsq == 2) pk_build_lut_t<2>(qv, codebook, nsq * 256u, put);
+    else pk_build_lut_t<4>(qv, codebook, nsq * 256u, put);
+}
+
 __device__ __forceinline__ void pk_build_lut(float* lut, const float* qv, const float* codebook, uint32_t nsq,
                                              uint32_t lsq) {
-    if (lsq == 1) pk_build_lut_t<1>(lut, qv, codebook, nsq * 256u);
-    else if (lsq == 2) pk_build_lut_t<2>(lut, qv, codebook, nsq * 256u);
-    else pk_build_lut_t<4>(lut, qv, codebook, nsq * 256u);
+    pk_build_lut_put(qv, codebook, nsq, lsq, [&](uint32_t e, float v) { lut[e] = v; });
+}
+
+// FP16 filter lookup table (a precision form; the reference's table is FP32).
+// Entries are scaled by 2^e, rounded to FP16 (nearest even) and summed in
+// FP32 in the reference's 4-lane order; the sum is scaled back by 2^-e.  e is
+// chosen per query so that the largest possible entry, (|q| + cbn)^2 with cbn
+// the largest codeword norm of the codebook, stays at or below 65504 (FP16's
+// largest finite value): e = floor(log2(65504 / (sqrt(|q|^2) + cbn)^2)),
+// clamped to [-30, 30].  Scaling by a power of two is exact, so a table kept
+// in FP32 holding the rounded values times 2^-e (the walk and the per-query
+// scan) gives the same sums, bit for bit, as the FP16 table summed and then
+// scaled (the split scan).
+__device__ __forceinline__ int pk_lut16_exp(float qn, float cbn) {
+    const float r = __fadd_rn(__fsqrt_rn(qn), cbn);
+    const float B = __fmul_rn(r, r);
+    const float x = __fdiv_rn(65504.0f, B);
+    int e = 30;
+    if (x < INFINITY) {
+        int ex = 0;
+        (void)frexpf(x, &ex);  // x = m * 2^ex, m in [0.5, 1)
+        e = ex - 1;
+    }
+    return e < -30 ? -30 : (e > 30 ? 30 : e);
+}
+
+// One entry of the FP16 table in FP32: round(v * 2^e) * 2^-e.
+__device__ __forceinline__ float pk_lut16_q(float v, float s, float inv) {
+    return __fmul_rn(__half2float(__float2half_rn(__fmul_rn(v, s))), inv);
+}
+
+// The filter table of a search in its precision: FP32, or the FP16 table's
+// values in FP32 (lut16 non-zero; qn = |q|^2, cbn as above).
+__device__ __forceinline__ void pk_build_filter_lut(float* lut, const float* qv, const float* codebook, uint32_t nsq,
+                                                    uint32_t lsq, int lut16, float qn, float cbn) {
+    if (!lut16) {
+        pk_build_lut(lut, qv, codebook, nsq, lsq);
+        return;
+    }
+    const int e = pk_lut16_exp(qn, cbn);
+    const float s = ldexpf(1.0f, e), inv = ldexpf(1.0f, -e);
+    pk_build_lut_put(qv, codebook, nsq, lsq, [&](uint32_t i, float v) { lut[i] = pk_lut16_q(v, s, inv); });
 }
 
 // Pivot (puck_index.cpp:517, :544): (top - |q|^2) / radius_rate / 2.0 with the

@@ -46,6 +46,10 @@ struct PkWalkArgs {
     uint32_t* sc_olist;      // nq: the queries left to the per-query kernel
     uint32_t* sc_ocount;     // 1
     uint32_t sc_capp;
+    // filter table precision: 0 FP32 (the reference's), 1 FP16 (scaled per
+    // query by a power of two; cb_nmax = the largest codeword norm)
+    int lut16;
+    float cb_nmax;
 };
 
 struct PkRerankArgs {

@@ -154,10 +154,15 @@ __global__ __launch_bounds__(PK_SC_CELL_THREADS) void pk_scan_cells(PkWalkArgs a
 // ---------------------------------------------------------------------------
 // Filter distances of PP points per thread: item i = j * NT + tid of the
 // round (consecutive threads, consecutive records); NV: nsq / 16 code vectors
-// per record (0: byte loads for any nsq).
+// per record (0: byte loads for any nsq).  T: the table's element type, float
+// (the reference's) or __half (the FP16 form: entries scaled by 2^e, the sum
+// scaled back by inv = 2^-e).
 #define PK_SC_PP 2
-template <int NT, int NV>
-__device__ __forceinline__ void pk_sc_score_round(const PkWalkArgs& a, const float* __restrict__ lut,
+__device__ __forceinline__ float pk_sc_lv(float v) { return v; }
+__device__ __forceinline__ float pk_sc_lv(__half v) { return __half2float(v); }
+
+template <int NT, int NV, typename T>
+__device__ __forceinline__ void pk_sc_score_round(const PkWalkArgs& a, const T* __restrict__ lut, float inv,
                                                   const uint2* __restrict__ items, uint32_t i0, uint32_t n,
                                                   uint64_t* __restrict__ keys) {
     const uint32_t nsq = a.nsq;
@@ -195,36 +200,54 @@ __device__ __forceinline__ void pk_sc_score_round(const PkWalkArgs& a, const flo
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const uint32_t code = (ws[hh] >> (8 * j)) & 0xffu;
-                        acc[j] = acc[j] + lut[(16 * g + 4 * hh + j) * PK_KS + code];
+                        acc[j] = acc[j] + pk_sc_lv(lut[(16 * g + 4 * hh + j) * PK_KS + code]);
                     }
                 }
             }
         } else {
             const uint8_t* codes = a.rec_codes + (uint64_t)P * nsq;
-            for (uint32_t q = 0; q < nsq; ++q) acc[q & 3] = acc[q & 3] + lut[q * PK_KS + codes[q]];
+            for (uint32_t q = 0; q < nsq; ++q) acc[q & 3] = acc[q & 3] + pk_sc_lv(lut[q * PK_KS + codes[q]]);
         }
-        tv = tv + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
+        const float sum = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+        if constexpr (sizeof(T) == sizeof(float)) tv = tv + sum;
+        else tv = tv + __fmul_rn(sum, inv);
         if (i < n) keys[i] = ((uint64_t)pk_f2ord(tv) << 32) | P;
     }
 }
 
-template <int NT, int NV>
-__global__ __launch_bounds__(NT, 6) void pk_scan_score(PkWalkArgs a) {
+// LDS of pk_scan_score: the lookup table (FP32 or FP16), then the query.
+__host__ __device__ inline size_t pk_scan_lut_bytes(uint32_t nsq, int h16) {
+    return (((size_t)nsq * PK_KS * (h16 ? 2u : 4u)) + 15u) & ~(size_t)15u;
+}
+
+template <int NT, int NV, bool H16>
+__global__ __launch_bounds__(NT, H16 ? 8 : 6) void pk_scan_score(PkWalkArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
     if (!pk_sc_budget_query(a, qi)) return;
     const uint32_t n = a.sc_meta[2ull * qi];
     if (n == PK_SC_OVERFLOW || n == 0) return;  // block-uniform
-    float* lut = (float*)smem;
-    float* qv = lut + a.nsq * PK_KS;
+    float* qv = (float*)(smem + pk_scan_lut_bytes(a.nsq, H16));
     const float* qg = a.q + (uint64_t)qi * a.dim;
     for (uint32_t i = threadIdx.x; i < a.dim; i += NT) qv[i] = qg[i];
     __syncthreads();
-    pk_build_lut(lut, qv, a.codebook, a.nsq, a.lsq);
-    __syncthreads();
     const uint2* items = a.sc_items + (uint64_t)qi * a.sc_capp;
     uint64_t* keys = a.sc_keys + (uint64_t)qi * a.sc_capp;
-    for (uint32_t i0 = 0; i0 < n; i0 += PK_SC_PP * NT) pk_sc_score_round<NT, NV>(a, lut, items, i0, n, keys);
+    if constexpr (H16) {
+        __half* lut = (__half*)smem;
+        const int e = pk_lut16_exp(a.qn[qi], a.cb_nmax);
+        const float sc = ldexpf(1.0f, e), inv = ldexpf(1.0f, -e);
+        pk_build_lut_put(qv, a.codebook, a.nsq, a.lsq,
+                         [&](uint32_t i, float v) { lut[i] = __float2half_rn(__fmul_rn(v, sc)); });
+        __syncthreads();
+        for (uint32_t i0 = 0; i0 < n; i0 += PK_SC_PP * NT) pk_sc_score_round<NT, NV>(a, lut, inv, items, i0, n, keys);
+    } else {
+        float* lut = (float*)smem;
+        pk_build_lut(lut, qv, a.codebook, a.nsq, a.lsq);
+        __syncthreads();
+        for (uint32_t i0 = 0; i0 < n; i0 += PK_SC_PP * NT)
+            pk_sc_score_round<NT, NV>(a, lut, 1.0f, items, i0, n, keys);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -340,15 +363,16 @@ __global__ __launch_bounds__(1024) void pk_scan_overflow_list(PkWalkArgs a, uint
 }
 
 // LDS of pk_scan_score: the lookup table and the query.
-extern "C" size_t pk_scan_score_smem(uint32_t dim, uint32_t nsq) {
-    return sizeof(float) * ((size_t)nsq * PK_KS + ((dim + 3) & ~3u));
+extern "C" size_t pk_scan_score_smem(uint32_t dim, uint32_t nsq, int h16) {
+    return pk_scan_lut_bytes(nsq, h16) + sizeof(float) * ((dim + 3) & ~3u);
 }
 
 // Whether the three kernels serve an index of these shapes (else every budget
-// query takes the per-query kernel): a lookup table within 64 KB of LDS.
+// query takes the per-query kernel): a lookup table within 64 KB of LDS (its
+// FP32 size, so that both precisions take the same kernels).
 extern "C" int pk_scan_split_ok(uint32_t dim, uint32_t nsq, uint32_t F) {
     (void)F;
-    return pk_scan_score_smem(dim, nsq) <= 64 * 1024;
+    return pk_scan_score_smem(dim, nsq, 0) <= 64 * 1024;
 }
 
 // Enqueues the split scan over nq queries in two parts: the cell selection
@@ -359,7 +383,7 @@ static hipError_t pk_scan_split_check(const PkWalkArgs& a) {
     if (!a.q_nf || !a.fdist || !a.sc_items || !a.sc_keys || !a.sc_meta || !a.sc_olist || !a.sc_ocount)
         return hipErrorInvalidValue;
     if (a.sc_capp > PK_SC_CUT_KPT * PK_SC_CUT_THREADS || a.sc_capp < a.ftk) return hipErrorInvalidValue;
-    if (pk_scan_score_smem(a.dim, a.nsq) > 64 * 1024) return hipErrorInvalidValue;
+    if (pk_scan_score_smem(a.dim, a.nsq, 0) > 64 * 1024) return hipErrorInvalidValue;
     return hipSuccess;
 }
 
@@ -377,15 +401,25 @@ extern "C" hipError_t pk_launch_scan_split_post(const PkWalkArgs* args, uint32_t
     const PkWalkArgs& a = *args;
     const hipError_t e = pk_scan_split_check(a);
     if (e != hipSuccess) return e;
-    const size_t smem = pk_scan_score_smem(a.dim, a.nsq);
+    const size_t smem = pk_scan_score_smem(a.dim, a.nsq, a.lut16);
     const uint32_t nv = (a.nsq & 15u) == 0 && a.nsq <= 64 ? a.nsq / 16u : 0u;
     constexpr int NT = 512;
-    switch (nv) {
-    case 1: hipLaunchKernelGGL((pk_scan_score<NT, 1>), dim3(nq), dim3(NT), smem, s, a); break;
-    case 2: hipLaunchKernelGGL((pk_scan_score<NT, 2>), dim3(nq), dim3(NT), smem, s, a); break;
-    case 3: hipLaunchKernelGGL((pk_scan_score<NT, 3>), dim3(nq), dim3(NT), smem, s, a); break;
-    case 4: hipLaunchKernelGGL((pk_scan_score<NT, 4>), dim3(nq), dim3(NT), smem, s, a); break;
-    default: hipLaunchKernelGGL((pk_scan_score<NT, 0>), dim3(nq), dim3(NT), smem, s, a); break;
+    if (a.lut16) {
+        switch (nv) {
+        case 1: hipLaunchKernelGGL((pk_scan_score<NT, 1, true>), dim3(nq), dim3(NT), smem, s, a); break;
+        case 2: hipLaunchKernelGGL((pk_scan_score<NT, 2, true>), dim3(nq), dim3(NT), smem, s, a); break;
+        case 3: hipLaunchKernelGGL((pk_scan_score<NT, 3, true>), dim3(nq), dim3(NT), smem, s, a); break;
+        case 4: hipLaunchKernelGGL((pk_scan_score<NT, 4, true>), dim3(nq), dim3(NT), smem, s, a); break;
+        default: hipLaunchKernelGGL((pk_scan_score<NT, 0, true>), dim3(nq), dim3(NT), smem, s, a); break;
+        }
+    } else {
+        switch (nv) {
+        case 1: hipLaunchKernelGGL((pk_scan_score<NT, 1, false>), dim3(nq), dim3(NT), smem, s, a); break;
+        case 2: hipLaunchKernelGGL((pk_scan_score<NT, 2, false>), dim3(nq), dim3(NT), smem, s, a); break;
+        case 3: hipLaunchKernelGGL((pk_scan_score<NT, 3, false>), dim3(nq), dim3(NT), smem, s, a); break;
+        case 4: hipLaunchKernelGGL((pk_scan_score<NT, 4, false>), dim3(nq), dim3(NT), smem, s, a); break;
+        default: hipLaunchKernelGGL((pk_scan_score<NT, 0, false>), dim3(nq), dim3(NT), smem, s, a); break;
+        }
     }
     hipLaunchKernelGGL(pk_scan_cut, dim3(nq), dim3(PK_SC_CUT_THREADS), 0, s, a);
     return hipGetLastError();

@@ -805,7 +805,7 @@ __device__ void pk_scan_budget(const PkWalkArgs& a, PkWalkSmem& s, uint32_t qi, 
         }
         if (l == 0) {
             // the lookup table, built while the first list's cell keys load
-            pk_build_lut(s.lut, s.qv, a.codebook, nsq, a.lsq);
+            pk_build_filter_lut(s.lut, s.qv, a.codebook, nsq, a.lsq, a.lut16, a.qn[qi], a.cb_nmax);
             __syncthreads();
         }
         // the list's cut: b-th smallest cell key
@@ -1061,7 +1061,7 @@ __device__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
     const float* qg = a.q + (uint64_t)qi * a.dim;
     for (uint32_t i = tid; i < a.dim; i += blockDim.x) s.qv[i] = qg[i];
     __syncthreads();
-    pk_build_lut(s.lut, s.qv, a.codebook, nsq, a.lsq);
+    pk_build_filter_lut(s.lut, s.qv, a.codebook, nsq, a.lsq, a.lut16, a.qn[qi], a.cb_nmax);
     __syncthreads();
 
     const float qn = a.qn[qi];
