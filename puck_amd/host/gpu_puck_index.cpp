@@ -158,10 +158,47 @@ int GpuPuckIndex::search(const Request* request, Response* response) {
         _err = "topk should <= topk, or feature is nullptr";
         return -1;
     }
-    uint32_t n = 0;
-    int rc = search_batch(1, request->feature, request->topk, response->distance, response->local_idx, &n);
-    if (rc == 0) response->result_num = n;
-    return rc;
+    PendingSearch p{request->feature, request->topk, response->distance, response->local_idx, 0, -1};
+    _gc.submit(&p, [this](PendingSearch* const* items, size_t n) { run_coalesced(items, n); });
+    if (p.rc == 0) response->result_num = p.result_num;
+    return p.rc;
+}
+
+// One device batch for the queued single-query calls: per-query topk
+// (puckgpu_search_batch), rows of stride FLAGS_topk, each caller's first topk
+// entries copied to its response buffers.
+void GpuPuckIndex::run_coalesced(PendingSearch* const* items, size_t n) {
+    std::lock_guard<std::mutex> g(_mu);
+    if (!_index) {
+        _err = "search before init";
+        for (size_t i = 0; i < n; ++i) items[i]->rc = -1;
+        return;
+    }
+    const uint32_t K = _sp.topk_max;
+    _cq.resize(n * _dim);
+    _ctopk.resize(n);
+    _cd.resize(n * K);
+    _ci.resize(n * K);
+    _cn.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        std::memcpy(_cq.data() + i * _dim, items[i]->feature, sizeof(float) * _dim);
+        _ctopk[i] = items[i]->topk;
+    }
+    puckgpu_query_params qp{};
+    qp.topk = _ctopk.data();
+    const int rc = puckgpu_search_batch(_ctx, _index, &_sp, &qp, (uint32_t)n, _cq.data(), _cd.data(), _ci.data(),
+                                        _cn.data(), 0);
+    if (rc != 0) _err = puckgpu_last_error(_ctx);
+    for (size_t i = 0; i < n; ++i) {
+        PendingSearch* p = items[i];
+        p->rc = rc == 0 ? 0 : -1;
+        if (rc != 0) continue;
+        if (p->topk > 0) {
+            std::memcpy(p->distance, _cd.data() + i * K, sizeof(float) * p->topk);
+            std::memcpy(p->local_idx, _ci.data() + i * K, sizeof(uint32_t) * p->topk);
+        }
+        p->result_num = _cn[i];
+    }
 }
 
 int GpuPuckIndex::search_batch(uint32_t n, const float* queries, uint32_t topk, float* distance, uint32_t* labels,

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/puck_gpu.h"
+#include "group_commit.h"
 
 namespace puck {
 
@@ -21,12 +22,18 @@ enum class IndexType { HIERARCHICAL_CLUSTER = 0, PUCK = 1, TINKER = 2 };
 struct Request {
     uint32_t topk = 100;
     const float* feature = nullptr;
+    virtual ~Request() { feature = nullptr; }
 };
 
 struct Response {
     float* distance = nullptr;
     uint32_t* local_idx = nullptr;
     uint32_t result_num = 0;
+    virtual ~Response() {
+        distance = nullptr;
+        local_idx = nullptr;
+        result_num = 0;
+    }
 };
 
 class Index {
@@ -73,7 +80,11 @@ public:
     // the raw vectors (puck_index.cpp:198-227).
     int init() override;
     // One query (puck_index.cpp:559-600): -1 for topk > FLAGS_topk or a null
-    // feature, leaving the response untouched.
+    // feature, leaving the response untouched.  Re-entrant: calls from several
+    // threads (the reference's ParallelFor callers, py_api_wrapper.cpp:111-152)
+    // are coalesced into one batched device search per group of callers that
+    // arrive while the previous group runs (group_commit.h); every query's
+    // result is the one it gets alone.
     int search(const Request* request, Response* response) override;
     // n queries, row stride topk (PySearcher::search semantics).
     int search_batch(uint32_t n, const float* queries, uint32_t topk, float* distance, uint32_t* labels,
@@ -86,9 +97,22 @@ public:
     int train() override;
     const char* last_error() const { return _err.c_str(); }
     uint32_t dim() const { return _dim; }
+    // Coalescing counters of search(): device batches run and queries served.
+    uint64_t coalesced_batches() const { return _gc.batches(); }
+    uint64_t coalesced_queries() const { return _gc.items(); }
+    uint64_t largest_coalesced_batch() const { return _gc.largest_batch(); }
 
 protected:
+    struct PendingSearch {
+        const float* feature;
+        uint32_t topk;
+        float* distance;
+        uint32_t* local_idx;
+        uint32_t result_num;
+        int rc;
+    };
     int init_locked();
+    void run_coalesced(PendingSearch* const* items, size_t n);
     GpuIndexFlags _flags;
     puckgpu_ctx* _ctx = nullptr;
     puckgpu_index* _index = nullptr;
@@ -96,7 +120,11 @@ protected:
     uint32_t _dim = 0, _filter_nsq = 0, _pq_nsq = 0;
     bool _whether_pq = false;
     std::string _err;
-    std::mutex _mu;  // one context: searches (and inserts) from several threads are serialised
+    std::mutex _mu;  // one context: device calls (batches, inserts) from several threads are serialised
+    GroupCommit<PendingSearch> _gc{4096};
+    // staging of a coalesced batch (used under _mu)
+    std::vector<float> _cq, _cd;
+    std::vector<uint32_t> _ctopk, _ci, _cn;
 };
 
 // InsertRequest of realtime_insert_puck_index.h:142-149.

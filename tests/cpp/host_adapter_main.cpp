@@ -1,13 +1,17 @@
 // Drives puck::GpuPuckIndex the way demo/search_client.cpp drives PuckIndex:
 // init, then Request/Response searches.  Usage:
-//   host_adapter_test <index_path> <queries.f32> <nq> <dim> <topk> <FLAGS_topk> <scc> <filter_topk> <out.bin> [build]
+//   host_adapter_test <index_path> <queries.f32> <nq> <dim> <topk> <FLAGS_topk> <scc> <filter_topk> <out.bin> [build|threads=T]
 // Writes nq*topk floats, nq*topk uint32 labels and nq uint32 counts to out.bin.
+// threads=T: T threads search concurrently (thread t takes queries t, t + T,
+// ...), as the reference's ParallelFor does; the coalescing counters are
+// printed as "batches B queries Q largest L".
 //   host_adapter_test <index_path> type
 // prints load_index_type() as an integer.
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../puck_amd/host/gpu_puck_index.h"
@@ -27,6 +31,8 @@ int main(int argc, char** argv) {
     f.search_coarse_count = atoi(argv[7]);
     f.filter_topk = atoi(argv[8]);
     puck::GpuPuckIndex index(f);
+    int threads = 0;
+    if (argc > 10 && std::string(argv[10]).rfind("threads=", 0) == 0) threads = atoi(argv[10] + 8);
     if (argc > 10 && std::string(argv[10]) == "build" && index.build() != 0) {
         fprintf(stderr, "build failed: %s\n", index.last_error());
         return 3;
@@ -40,18 +46,39 @@ int main(int argc, char** argv) {
     in.read((char*)q.data(), (std::streamsize)(q.size() * 4));
     std::vector<float> D((size_t)nq * topk, 0.f);
     std::vector<uint32_t> I((size_t)nq * topk, 0u), R(nq, 0u);
-    for (uint32_t i = 0; i < nq; ++i) {
+    auto one = [&](uint32_t i) {
         puck::Request req;
         req.topk = topk;
         req.feature = q.data() + (size_t)i * dim;
         puck::Response res;
         res.distance = D.data() + (size_t)i * topk;
         res.local_idx = I.data() + (size_t)i * topk;
-        if (index.search(&req, &res) != 0) {
-            fprintf(stderr, "search failed: %s\n", index.last_error());
-            return 5;
-        }
+        if (index.search(&req, &res) != 0) return false;
         R[i] = res.result_num;
+        return true;
+    };
+    if (threads > 0) {
+        std::vector<std::thread> th;
+        std::vector<int> ok(threads, 1);
+        for (int t = 0; t < threads; ++t)
+            th.emplace_back([&, t] {
+                for (uint32_t i = t; i < nq; i += threads)
+                    if (!one(i)) ok[t] = 0;
+            });
+        for (auto& x : th) x.join();
+        for (int t = 0; t < threads; ++t)
+            if (!ok[t]) {
+                fprintf(stderr, "search failed: %s\n", index.last_error());
+                return 5;
+            }
+        printf("batches %llu queries %llu largest %llu\n", (unsigned long long)index.coalesced_batches(),
+               (unsigned long long)index.coalesced_queries(), (unsigned long long)index.largest_coalesced_batch());
+    } else {
+        for (uint32_t i = 0; i < nq; ++i)
+            if (!one(i)) {
+                fprintf(stderr, "search failed: %s\n", index.last_error());
+                return 5;
+            }
     }
     // a request above FLAGS_topk must fail with -1
     puck::Request bad;

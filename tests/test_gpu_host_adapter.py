@@ -18,15 +18,19 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "tests", "cpp", "host_adapter_test")
 
 
-def _run(index_path, q, topk, flags_topk, scc, ftk, tmp, build=False):
+def _run(index_path, q, topk, flags_topk, scc, ftk, tmp, build=False, threads=0, log=None):
     qf = str(tmp / "q.f32")
     np.ascontiguousarray(q, np.float32).tofile(qf)
     out = str(tmp / "out.bin")
     args = [BIN, index_path, qf, str(q.shape[0]), str(q.shape[1]), str(topk), str(flags_topk), str(scc), str(ftk), out]
     if build:
         args.append("build")
+    if threads:
+        args.append(f"threads={threads}")
     r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+    if log is not None:
+        log.append(r.stdout)
     raw = np.fromfile(out, dtype=np.uint32)
     n = q.shape[0]
     Dm = raw[:n * topk].view(np.float32).reshape(n, topk)
@@ -47,6 +51,30 @@ def test_cpp_adapter_search(tmp_path):
     assert np.array_equal(R, Ro)
     assert np.array_equal(Im, Io)
     assert np.array_equal(Dm.view(np.uint32), Do.view(np.uint32))
+
+
+def test_cpp_adapter_concurrent_search(tmp_path):
+    """16 threads call Index::search at once (ParallelFor's pattern,
+    py_api_wrapper.cpp:111-152): the calls are coalesced into device batches,
+    and every query's result equals the oracle's and the one-at-a-time run."""
+    x = D.mixture_uint8_grid(8000, 32, 60, seed=53, means_seed=54)
+    q = D.mixture_uint8_grid(800, 32, 60, seed=55, means_seed=54)
+    ix = CI.build_index(x, 16, 16, 8, whether_norm=False, train_n=4000)
+    d = tmp_path / "idx"
+    index_io.write_index(str(d), FL, ix["coarse"], ix["fine"], ix["codebook"], ix["cell_assign"], ix["records"],
+                         ix["raw"], False)
+    log = []
+    Dc, Ic, Rc = _run(str(d), q, 10, 50, 6, 200, tmp_path, threads=16, log=log)
+    Ds, Is, Rs = _run(str(d), q, 10, 50, 6, 200, tmp_path)
+    Do, Io, Ro, _ = CI.oracle_index(ix).search_batch(q, 10, 6, 200, max_topk=50)
+    print(log[0].strip(), flush=True)
+    f = log[0].split()
+    batches, served = int(f[1]), int(f[3])
+    assert served == q.shape[0] and 1 <= batches <= q.shape[0]
+    for Dm, Im, R in ((Dc, Ic, Rc), (Ds, Is, Rs)):
+        assert np.array_equal(R, Ro)
+        assert np.array_equal(Im, Io)
+        assert np.array_equal(Dm.view(np.uint32), Do.view(np.uint32))
 
 
 def test_cpp_adapter_build(tmp_path):

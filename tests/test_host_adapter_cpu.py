@@ -167,3 +167,19 @@ def test_realtime_init_refuses_short_files(tmp_path):
     r = subprocess.run([INS_BIN, str(d), str(d / "none"), "0", "16", str(d / "none"), "0", "10", "100", "2", "-1",
                         str(tmp_path / "out.bin")], capture_output=True, text=True, timeout=60)
     assert r.returncode == 4 and "shorter than the label file" in r.stderr, (r.returncode, r.stderr)
+
+
+def test_group_commit_under_thread_sanitizer(tmp_path):
+    """The coalescer of concurrent Index::search calls (group_commit.h) under
+    ThreadSanitizer: every item served exactly once, by one batch at a time of
+    at most max_batch items, before its submit() returns."""
+    exe = str(tmp_path / "gct")
+    src = os.path.join(ROOT, "tests", "cpp", "group_commit_main.cpp")
+    subprocess.check_call(["g++", "-O1", "-g", "-std=c++17", "-fsanitize=thread", "-o", exe, src, "-lpthread"])
+    for args in (("16", "1500", "7"), ("4", "3000", "1"), ("24", "400", "4096")):
+        r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, (args, r.stdout, r.stderr)
+        assert "ThreadSanitizer" not in r.stderr, r.stderr
+        f = r.stdout.split()
+        assert int(f[3]) == int(args[0]) * int(args[1])
+        assert int(f[5]) <= int(args[2])
