@@ -356,10 +356,23 @@ int puckgpu_lut(puckgpu_ctx* ctx, uint32_t n, uint32_t dim, const float* queries
                 uint32_t nsq, float* out, int device_ptrs);
 
 /* Build-time nearest cell of n points [n x dim]: cell_out [n] = coarse*F + fine,
- * dist_out [n] (may be NULL) = the winning score. */
+ * dist_out [n] (may be NULL) = the winning score.  The answer is the
+ * reference's (the first strict minimum over the cells it visits in (coarse
+ * term, coarse id, fine) order); the default mode finds it without the
+ * sequential walk: the nearest coarse list is scanned as the reference scans
+ * it, cells are skipped by block lower bounds, and the minimum of the rest is
+ * accepted when a triangle-check certificate shows the reference visits it;
+ * points without the certificate are walked sequentially. */
 int puckgpu_assign_cells(puckgpu_ctx* ctx, const float* coarse, const float* fine, uint32_t C, uint32_t F,
                          uint32_t dim, uint32_t n, const float* x, uint32_t* cell_out, float* dist_out,
                          int device_ptrs);
+/* Assignment strategy of puckgpu_assign_cells (no counterpart in the
+ * reference, which always walks): 0 = bounded with certificate (default),
+ * 1 = the sequential walk for every point, 2 = bounded, then every point
+ * walked again (test of the fallback path).  Counters: points assigned and
+ * points walked sequentially since the context was created. */
+int puckgpu_set_assign_mode(puckgpu_ctx* ctx, int mode);
+int puckgpu_assign_stats(puckgpu_ctx* ctx, uint64_t* points, uint64_t* sequential);
 /* Filter-PQ records of n points: records_out [n x (4 + nsq)] in the
  * filter_data.dat layout. */
 int puckgpu_encode_filter(puckgpu_ctx* ctx, const float* coarse, const float* fine, uint32_t C, uint32_t F,

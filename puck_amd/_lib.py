@@ -21,6 +21,7 @@ EXPORTS = [
     "puckgpu_server_buckets", "puckgpu_server_search", "puckgpu_server_tune", "puckgpu_last_scan_ms",
     "puckgpu_center_distances", "puckgpu_index_attach_pq", "puckgpu_encode_pq", "puckgpu_search_noalloc",
     "puckgpu_index_insert", "puckgpu_index_version", "puckgpu_kmeans", "puckgpu_train_centers",
+    "puckgpu_set_assign_mode", "puckgpu_assign_stats",
 ]
 
 
@@ -103,6 +104,10 @@ def load():
     L.puckgpu_lut.restype = c_int
     L.puckgpu_assign_cells.argtypes = [vp, vp, vp, u32, u32, u32, u32, vp, vp, vp, c_int]
     L.puckgpu_assign_cells.restype = c_int
+    L.puckgpu_set_assign_mode.argtypes = [vp, c_int]
+    L.puckgpu_set_assign_mode.restype = c_int
+    L.puckgpu_assign_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
+    L.puckgpu_assign_stats.restype = c_int
     L.puckgpu_encode_filter.argtypes = [vp, vp, vp, u32, u32, u32, vp, u32, u32, vp, vp, vp, c_int]
     L.puckgpu_encode_filter.restype = c_int
     L.puckgpu_search_noalloc.argtypes = [vp, vp, ctypes.POINTER(SearchParams), u32, vp, u32, vp, vp, vp]

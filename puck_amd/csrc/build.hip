@@ -164,6 +164,229 @@ __global__ __launch_bounds__(256) void pk_assign_cells(const float* __restrict__
     }
 }
 
+// ---------------------------------------------------------------------------
+// nearest_cell_assign without the sequential walk over all coarse clusters.
+//
+// The reference visits coarse clusters in (ct, c) order and keeps the first
+// strict minimum of score = (ct - <x,f>) + stationary over the cells it visits;
+// a cell is skipped when the triangle check fails (break) under the running
+// best.  Its answer is reproduced exactly in three steps per point:
+//   1. the nearest coarse cluster c0 (lexicographic (ct, c) minimum) is
+//      scanned exactly as the reference scans its first list (from best =
+//      FLT_MAX) -> best B and cell;
+//   2. every other cell whose float score could be <= B is evaluated: cells
+//      are skipped in blocks of PK_AB fine centers (stored order) when the
+//      block's lower bound ct + min st[c][block] - max <x,f in block> exceeds
+//      B by more than the rounding of either expression (2^-18 * (|ct| +
+//      max|st| + max|<x,f>|), far above the 4u bound of two float additions),
+//      and whole lists on the row minimum first;
+//   3. the lexicographic (score, ct, c, f) minimum over the evaluated cells is
+//      the reference's answer when the reference visits that cell, which the
+//      certificate sqrt(score) - max(fbound[0..f]) >= sqrt(ct) guarantees: the
+//      running best never drops below that score (no visited cell scores lower),
+//      so no check up to f can break.  A point failing the certificate (or
+//      with a negative score) is listed and assigned by pk_assign_cells
+//      (list_all lists every point: the sequential path's test).
+// Every cell with a smaller score, or an equal score earlier in the visiting
+// order, is evaluated, so the minimum is the first one the reference meets.
+#define PK_AB 32u
+#define PK_AB_NBMAX 256u  // blocks per row: F <= 8192
+
+// Per coarse row of the stationary table: the minimum of each block of PK_AB
+// fine centers, the row minimum, and the row's largest magnitude (folded into
+// *stabs_bits with an unsigned max on the float bits of |st|).
+__global__ void pk_st_blocks(const float* __restrict__ st, uint32_t C, uint32_t F, uint32_t nb,
+                             float* __restrict__ stmin, float* __restrict__ stminrow, uint32_t* __restrict__ stabs_bits) {
+    const uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (c >= C) return;
+    const float* row = st + (uint64_t)c * F;
+    float rmin = FLT_MAX, amax = 0.0f;
+    for (uint32_t j = lane; j < nb; j += 64) {
+        float m = FLT_MAX;
+        const uint32_t f1 = (j + 1) * PK_AB < F ? (j + 1) * PK_AB : F;
+        for (uint32_t f = j * PK_AB; f < f1; ++f) {
+            const float v = row[f];
+            m = v < m ? v : m;
+            const float a = fabsf(v);
+            amax = a > amax ? a : amax;
+        }
+        stmin[(uint64_t)c * nb + j] = m;
+        rmin = m < rmin ? m : rmin;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float r2 = __shfl_xor(rmin, o, 64), a2 = __shfl_xor(amax, o, 64);
+        rmin = r2 < rmin ? r2 : rmin;
+        amax = a2 > amax ? a2 : amax;
+    }
+    if (lane == 0) {
+        stminrow[c] = rmin;
+        atomicMax(stabs_bits, __float_as_uint(amax));
+    }
+}
+
+extern "C" hipError_t pk_launch_st_blocks(const float* st, uint32_t C, uint32_t F, float* stmin, float* stminrow,
+                                          uint32_t* stabs_bits, hipStream_t s) {
+    const uint32_t nb = (F + PK_AB - 1) / PK_AB;
+    if (nb > PK_AB_NBMAX) return hipErrorInvalidValue;
+    hipError_t e = hipMemsetAsync(stabs_bits, 0, sizeof(uint32_t), s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(pk_st_blocks, dim3((C + 3) / 4), dim3(256), 0, s, st, C, F, nb, stmin, stminrow, stabs_bits);
+    return hipGetLastError();
+}
+
+struct PkCellKey {
+    float v, ct;
+    uint32_t c, f;  // c = 0xffffffff: none
+};
+
+__device__ __forceinline__ bool pk_key_less(const PkCellKey& a, const PkCellKey& b) {
+    if (b.c == 0xffffffffu) return a.c != 0xffffffffu;
+    if (a.c == 0xffffffffu) return false;
+    if (a.v != b.v) return a.v < b.v;
+    if (a.ct != b.ct) return a.ct < b.ct;
+    if (a.c != b.c) return a.c < b.c;
+    return a.f < b.f;
+}
+
+__global__ __launch_bounds__(256) void pk_assign_bounded(
+    const float* __restrict__ ipc, const float* __restrict__ ipf, const float* __restrict__ xn,
+    const float* __restrict__ cn, const float* __restrict__ fbound, const float* __restrict__ fbpm,
+    const float* __restrict__ st, const float* __restrict__ stmin, const float* __restrict__ stminrow,
+    const uint32_t* __restrict__ stabs_bits, uint32_t n, uint32_t C, uint32_t F, uint32_t nb, int fb_sorted,
+    int list_all, uint32_t* __restrict__ cell_out, float* __restrict__ dist_out, uint32_t* __restrict__ fb_list,
+    uint32_t* __restrict__ fb_count) {
+    __shared__ float bmax_s[4][PK_AB_NBMAX];
+    const int lane = threadIdx.x & 63;
+    const uint32_t pt = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (pt >= n) return;  // wave-uniform; the kernel has no block barrier
+    float* bmax = bmax_s[threadIdx.x >> 6];
+    const float* pc = ipc + (uint64_t)pt * C;
+    const float* pf = ipf + (uint64_t)pt * F;
+    const float x2 = xn[pt];
+    // 1. c0 and the reference's scan of it
+    float ct0 = FLT_MAX;
+    uint32_t c0 = 0xffffffffu;
+    for (uint32_t c = lane; c < C; c += 64) {
+        const float ct = (cn[c] - pc[c]) + x2;
+        if (c0 == 0xffffffffu || ct < ct0) {  // c ascends per lane: '<' keeps the smaller id on ties
+            ct0 = ct;
+            c0 = c;
+        }
+    }
+    pk_wave_argmin(ct0, c0);
+    float B = FLT_MAX;
+    int32_t cell0 = -1;
+    pk_assign_one_coarse(c0, ct0, pf, st, fbound, F, fb_sorted, B, cell0);
+    // block maxima of <x,f>, their maximum, and max |<x,f>|
+    float gmax = -FLT_MAX, amax = 0.0f;
+    for (uint32_t j = lane; j < nb; j += 64) {
+        float m = -FLT_MAX;
+        const uint32_t f1 = (j + 1) * PK_AB < F ? (j + 1) * PK_AB : F;
+        for (uint32_t f = j * PK_AB; f < f1; ++f) {
+            const float v = pf[f];
+            m = v > m ? v : m;
+            const float a = fabsf(v);
+            amax = a > amax ? a : amax;
+        }
+        bmax[j] = m;
+        gmax = m > gmax ? m : gmax;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float g2 = __shfl_xor(gmax, o, 64), a2 = __shfl_xor(amax, o, 64);
+        gmax = g2 > gmax ? g2 : gmax;
+        amax = a2 > amax ? a2 : amax;
+    }
+    pk_wave_lds_sync();
+    const float stabs = __uint_as_float(*stabs_bits);
+    // 2. cells that could score <= B
+    PkCellKey best{FLT_MAX, FLT_MAX, 0xffffffffu, 0u};
+    for (uint32_t cb0 = 0; cb0 < C; cb0 += 64) {
+        const uint32_t c = cb0 + lane;
+        const uint32_t cl = c < C ? c : C - 1u;
+        const float ct = (cn[cl] - pc[cl]) + x2;
+        const float srow = stminrow[cl];
+        const float marg = 0x1p-18f * ((fabsf(ct) + stabs) + amax);
+        const bool cand = c < C && c != c0 && !(((ct + srow) - gmax) > B + marg);
+        uint64_t cm = __ballot(cand);
+        while (cm) {
+            const int src = __ffsll((unsigned long long)cm) - 1;
+            cm &= cm - 1;
+            const uint32_t cc = cb0 + (uint32_t)src;
+            const float cct = __shfl(ct, src, 64), cmg = __shfl(marg, src, 64);
+            const float* srowp = stmin + (uint64_t)cc * nb;
+            const float* strow = st + (uint64_t)cc * F;
+            for (uint32_t j0 = 0; j0 < nb; j0 += 64) {
+                const uint32_t j = j0 + lane;
+                const uint32_t jl = j < nb ? j : nb - 1u;
+                const float lb = (cct + srowp[jl]) - bmax[jl];
+                uint64_t bm = __ballot(j < nb && !(lb > B + cmg));
+                while (bm) {
+                    // two blocks per pass: lanes 0..31 score the first, 32..63 the second
+                    const int b0 = __ffsll((unsigned long long)bm) - 1;
+                    bm &= bm - 1;
+                    int b1 = -1;
+                    if (bm) {
+                        b1 = __ffsll((unsigned long long)bm) - 1;
+                        bm &= bm - 1;
+                    }
+                    const int mine = lane < 32 ? b0 : b1;
+                    const uint32_t f = (j0 + (uint32_t)(mine < 0 ? b0 : mine)) * PK_AB + (lane & 31);
+                    const uint32_t fl = f < F ? f : F - 1u;
+                    const float s = (cct - pf[fl]) + strow[fl];
+                    if (mine >= 0 && f < F) {
+                        const PkCellKey k{s, cct, cc, f};
+                        if (pk_key_less(k, best)) best = k;
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        PkCellKey k{__shfl_xor(best.v, o, 64), __shfl_xor(best.ct, o, 64), (uint32_t)__shfl_xor((int)best.c, o, 64),
+                    (uint32_t)__shfl_xor((int)best.f, o, 64)};
+        if (pk_key_less(k, best)) best = k;
+    }
+    // 3. the first coarse list's answer, or the certified minimum
+    const PkCellKey k0{B, ct0, c0, (uint32_t)cell0 - c0 * F};
+    if (lane == 0) {
+        bool ok = true;
+        uint32_t cell = (uint32_t)cell0;
+        float dist = B;
+        if (pk_key_less(best, k0)) {
+            ok = best.v >= 0.0f && !(__fsqrt_rn(best.v) - fbpm[best.f] < __fsqrt_rn(best.ct));
+            cell = best.c * F + best.f;
+            dist = best.v;
+        }
+        if (ok && !list_all) {
+            cell_out[pt] = cell;
+            if (dist_out) dist_out[pt] = dist;
+        } else {
+            fb_list[atomicAdd(fb_count, 1u)] = pt;
+        }
+    }
+}
+
+extern "C" hipError_t pk_launch_assign_bounded(const float* ipc, const float* ipf, const float* xn, const float* cn,
+                                               const float* fbound, const float* fbpm, const float* st,
+                                               const float* stmin, const float* stminrow, const uint32_t* stabs_bits,
+                                               uint32_t n, uint32_t C, uint32_t F, int fb_sorted, int list_all,
+                                               uint32_t* cell_out, float* dist_out, uint32_t* fb_list,
+                                               uint32_t* fb_count, hipStream_t s) {
+    const uint32_t nb = (F + PK_AB - 1) / PK_AB;
+    if (nb > PK_AB_NBMAX) return hipErrorInvalidValue;
+    hipError_t e = hipMemsetAsync(fb_count, 0, sizeof(uint32_t), s);
+    if (e != hipSuccess) return e;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(pk_assign_bounded, dim3((n + 3) / 4), dim3(256), 0, s, ipc, ipf, xn, cn, fbound, fbpm, st,
+                       stmin, stminrow, stabs_bits, n, C, F, nb, fb_sorted, list_all, cell_out, dist_out, fb_list,
+                       fb_count);
+    return hipGetLastError();
+}
+
 // Nearest coarse term per point (used to order points so that waves running
 // together share stationary-table rows in L2).
 __global__ void pk_nearest_coarse(const float* __restrict__ ipc, const float* __restrict__ xn,

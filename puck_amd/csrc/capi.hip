@@ -68,6 +68,10 @@ hipError_t pk_launch_budget_rank(const float*, const uint32_t*, uint32_t, uint32
 hipError_t pk_launch_assign_cells(const float*, const float*, const float*, const float*, const float*,
                                   const float*, uint32_t, uint32_t, uint32_t, int, const uint32_t*, uint32_t*,
                                   float*, hipStream_t);
+hipError_t pk_launch_st_blocks(const float*, uint32_t, uint32_t, float*, float*, uint32_t*, hipStream_t);
+hipError_t pk_launch_assign_bounded(const float*, const float*, const float*, const float*, const float*, const float*,
+                                    const float*, const float*, const float*, const uint32_t*, uint32_t, uint32_t,
+                                    uint32_t, int, int, uint32_t*, float*, uint32_t*, uint32_t*, hipStream_t);
 hipError_t pk_launch_nearest_coarse(const float*, const float*, const float*, uint32_t, uint32_t, uint32_t*,
                                     hipStream_t);
 hipError_t pk_launch_fine_bound(const float*, uint32_t, uint32_t, float*, hipStream_t);
@@ -132,6 +136,8 @@ struct puckgpu_ctx {
     uint64_t counters[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t last_n = 0, last_ftk = 0;
     unsigned char* noalloc = nullptr;  // this device's copy of `scratch` (puckgpu_search_noalloc)
+    int assign_mode = 0;                // puckgpu_set_assign_mode
+    uint64_t assign_points = 0, assign_sequential = 0;
 };
 
 struct puckgpu_index {
@@ -253,7 +259,7 @@ enum Slot {
     S_SCK, S_SCM, S_INS_X, S_INS_Y, S_INS_CELL, S_INS_REC, S_INS_PQ, S_INS_RANK, S_INS_ADD, S_INS_SHIFT, S_INS_WORK,
     S_INS_DOLD, S_INS_DNEW, S_INS_ACC, S_KM_IP, S_KM_PN, S_KM_CN, S_KM_ASSIGN, S_KM_DIS, S_KM_PERM, S_KM_STARTS,
     S_KM_HIST, S_KM_OFF, S_KM_WORK, S_KM_BEST, S_KM_IDS, S_KM_X, S_KM_TV, S_KM_C0, S_KM_F0, S_KM_CB, S_KM_FB,
-    S_KM_FA, S_KM_CENT, S_COUNT
+    S_KM_FA, S_KM_CENT, S_AB_STMIN, S_AB_ROW, S_AB_ABS, S_AB_FBPM, S_AB_LIST, S_AB_CNT, S_COUNT
 };
 
 // Device view of an input array: the pointer itself, or a scratch copy of it.
@@ -1572,33 +1578,67 @@ namespace {
 
 // Is the fine bound -sqrt(|f|^2/2) non-decreasing (fine centers stored by
 // descending norm)?  The assignment kernel may then stop a list early.
-int fine_bound_sorted(puckgpu_ctx* ctx, const float* fb, uint32_t F, int* sorted) {
+// fbpm (may be NULL) receives the prefix maxima max(fb[0..f]) on the host.
+int fine_bound_sorted(puckgpu_ctx* ctx, const float* fb, uint32_t F, int* sorted, std::vector<float>* fbpm = nullptr) {
     std::vector<float> fb_h(F);
     PK_CHECK(hipMemcpyAsync(fb_h.data(), fb, 4ull * F, hipMemcpyDeviceToHost, ctx->stream));
     PK_CHECK(hipStreamSynchronize(ctx->stream));
     *sorted = 1;
     for (uint32_t f = 1; f < F; ++f)
         if (!(fb_h[f - 1] <= fb_h[f])) *sorted = 0;
+    if (fbpm) {
+        fbpm->resize(F);
+        float m = fb_h[0];
+        for (uint32_t f = 0; f < F; ++f) {
+            m = fb_h[f] > m ? fb_h[f] : m;
+            (*fbpm)[f] = m;
+        }
+    }
     return PUCKGPU_OK;
 }
+
+// Block lower bounds of a stationary table for pk_assign_bounded.
+struct AssignBounds {
+    const float* stmin = nullptr;
+    const float* stminrow = nullptr;
+    const uint32_t* stabs = nullptr;
+    const float* fbpm = nullptr;
+};
 
 // nearest_cell_assign (hierarchical_cluster_index.cpp:1042-1085) of m <=
 // PK_ASSIGN_CHUNK points on the device, over the coarse half-norms cn, the
 // stationary values st and the fine bound fb; cell_out / dist_out (may be
-// NULL) on the device.
+// NULL) on the device.  With bounds (and assign mode 0 or 2) the bounded
+// kernel assigns every point it can certify and the sequential kernel the
+// rest; without, the sequential kernel assigns every point.
 int assign_chunk(puckgpu_ctx* ctx, const float* cd, const float* fd, const float* cn, const float* st, const float* fb,
                  int fb_sorted, uint32_t C, uint32_t F, uint32_t dim, uint32_t m, const float* xp, uint32_t* cell_out,
-                 float* dist_out, hipStream_t s) {
+                 float* dist_out, hipStream_t s, const AssignBounds* ab = nullptr) {
     int rc = 0;
     float* ipc = (float*)scratch_slot(ctx, S_A6, 4ull * m * C, &rc);
     float* ipf = (float*)scratch_slot(ctx, S_A7, 4ull * m * F, &rc);
     float* xn = (float*)scratch_slot(ctx, S_QN, 4ull * m, &rc);
     uint32_t* ord_d = (uint32_t*)scratch_slot(ctx, S_QLIST, 4ull * m, &rc);
     if (rc) return rc;
-    std::vector<uint32_t> ord_h(m), perm_h(m);
     PK_CHECK(pk_launch_row_norms(xp, m, dim, 1, xn, s));
     PK_CHECK(pk_launch_gemm_exact(xp, cd, ipc, m, C, dim, C, PK_EPI_RAW, nullptr, s));
     PK_CHECK(pk_launch_gemm_exact(xp, fd, ipf, m, F, dim, F, PK_EPI_RAW, nullptr, s));
+    ctx->assign_points += m;
+    if (ab && ctx->assign_mode != 1) {
+        uint32_t* cnt = (uint32_t*)scratch_slot(ctx, S_AB_CNT, 4, &rc);
+        if (rc) return rc;
+        PK_CHECK(pk_launch_assign_bounded(ipc, ipf, xn, cn, fb, ab->fbpm, st, ab->stmin, ab->stminrow, ab->stabs, m, C,
+                                          F, fb_sorted, ctx->assign_mode == 2, cell_out, dist_out, ord_d, cnt, s));
+        uint32_t k = 0;
+        PK_CHECK(hipMemcpyAsync(&k, cnt, 4, hipMemcpyDeviceToHost, s));
+        PK_CHECK(hipStreamSynchronize(s));
+        if (k > m) return set_err(ctx, PUCKGPU_EHIP, "assignment listed %u of %u points", k, m);
+        ctx->assign_sequential += k;
+        if (k) PK_CHECK(pk_launch_assign_cells(ipc, ipf, xn, cn, fb, st, k, C, F, fb_sorted, ord_d, cell_out, dist_out, s));
+        PK_CHECK(hipStreamSynchronize(s));
+        return PUCKGPU_OK;
+    }
+    std::vector<uint32_t> ord_h(m), perm_h(m);
     // visit points grouped by nearest coarse cluster (L2 reuse of the
     // stationary rows); results are written by point index
     PK_CHECK(pk_launch_nearest_coarse(ipc, xn, cn, m, C, ord_d, s));
@@ -1611,6 +1651,7 @@ int assign_chunk(puckgpu_ctx* ctx, const float* cd, const float* fd, const float
         for (uint32_t i = 0; i < m; ++i) perm_h[cnt[ord_h[i]]++] = i;
     }
     PK_CHECK(hipMemcpyAsync(ord_d, perm_h.data(), 4ull * m, hipMemcpyHostToDevice, s));
+    ctx->assign_sequential += m;
     PK_CHECK(pk_launch_assign_cells(ipc, ipf, xn, cn, fb, st, m, C, F, fb_sorted, ord_d, cell_out, dist_out, s));
     // the host vectors above must outlive the copies
     PK_CHECK(hipStreamSynchronize(s));
@@ -1871,6 +1912,20 @@ int puckgpu_lut(puckgpu_ctx* ctx, uint32_t n, uint32_t dim, const float* queries
 }
 
 // ---------------------------------------------------------------------------
+int puckgpu_set_assign_mode(puckgpu_ctx* ctx, int mode) {
+    if (!ctx) return PUCKGPU_EINVAL;
+    if (mode < 0 || mode > 2) return set_err(ctx, PUCKGPU_EINVAL, "assign mode must be 0, 1 or 2");
+    ctx->assign_mode = mode;
+    return PUCKGPU_OK;
+}
+
+int puckgpu_assign_stats(puckgpu_ctx* ctx, uint64_t* points, uint64_t* sequential) {
+    if (!ctx || !points || !sequential) return PUCKGPU_EINVAL;
+    *points = ctx->assign_points;
+    *sequential = ctx->assign_sequential;
+    return PUCKGPU_OK;
+}
+
 int puckgpu_assign_cells(puckgpu_ctx* ctx, const float* coarse, const float* fine, uint32_t C, uint32_t F,
                          uint32_t dim, uint32_t n, const float* x, uint32_t* cell_out, float* dist_out,
                          int device_ptrs) {
@@ -1893,7 +1948,26 @@ int puckgpu_assign_cells(puckgpu_ctx* ctx, const float* coarse, const float* fin
     PK_CHECK(pk_launch_fine_bound(fd, F, dim, fb, s));
     PK_CHECK(pk_launch_gemm_exact(cd, fd, st, C, F, dim, F, PK_EPI_ADD_BIAS, fnorm, s));
     int fb_sorted = 1;
-    if ((rc = fine_bound_sorted(ctx, fb, F, &fb_sorted))) return rc;
+    std::vector<float> fbpm_h;
+    if ((rc = fine_bound_sorted(ctx, fb, F, &fb_sorted, &fbpm_h))) return rc;
+    // block lower bounds of the stationary table (pk_assign_bounded)
+    AssignBounds ab;
+    const AssignBounds* abp = nullptr;
+    const uint32_t nb = (F + 31) / 32;
+    if (nb <= 256) {
+        float* stmin = (float*)scratch_slot(ctx, S_AB_STMIN, 4ull * C * nb, &rc);
+        float* srow = (float*)scratch_slot(ctx, S_AB_ROW, 4ull * C, &rc);
+        uint32_t* sabs = (uint32_t*)scratch_slot(ctx, S_AB_ABS, 4, &rc);
+        float* fbpm = (float*)scratch_slot(ctx, S_AB_FBPM, 4ull * F, &rc);
+        if (rc) return rc;
+        PK_CHECK(pk_launch_st_blocks(st, C, F, stmin, srow, sabs, s));
+        PK_CHECK(hipMemcpyAsync(fbpm, fbpm_h.data(), 4ull * F, hipMemcpyHostToDevice, s));
+        ab.stmin = stmin;
+        ab.stminrow = srow;
+        ab.stabs = sabs;
+        ab.fbpm = fbpm;
+        abp = &ab;
+    }
     const uint32_t CHK = PK_ASSIGN_CHUNK;
     const uint32_t cb = n < CHK ? n : CHK;
     float* xs = (float*)scratch_slot(ctx, S_Q, 4ull * cb * dim, &rc);
@@ -1910,7 +1984,7 @@ int puckgpu_assign_cells(puckgpu_ctx* ctx, const float* coarse, const float* fin
         }
         uint32_t* cout = device_ptrs ? cell_out + p0 : co;
         float* dout = device_ptrs ? (dist_out ? dist_out + p0 : nullptr) : cdo;
-        if ((rc = assign_chunk(ctx, cd, fd, cn, st, fb, fb_sorted, C, F, dim, m, xp, cout, dout, s))) return rc;
+        if ((rc = assign_chunk(ctx, cd, fd, cn, st, fb, fb_sorted, C, F, dim, m, xp, cout, dout, s, abp))) return rc;
         if (!device_ptrs) {
             PK_CHECK(hipMemcpyAsync(cell_out + p0, co, 4ull * m, hipMemcpyDeviceToHost, s));
             if (dist_out) PK_CHECK(hipMemcpyAsync(dist_out + p0, cdo, 4ull * m, hipMemcpyDeviceToHost, s));

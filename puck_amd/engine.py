@@ -145,6 +145,17 @@ class Context:
                                                _ptr(cell), _ptr(dist), m))
         return (cell, dist) if with_dist else cell
 
+    def set_assign_mode(self, mode):
+        """0: bounded assignment with certificate (default), 1: the sequential
+        walk for every point, 2: bounded, then every point walked again."""
+        self.check(self.L.puckgpu_set_assign_mode(self.h, int(mode)))
+
+    def assign_stats(self):
+        """(points assigned, points walked sequentially) since creation."""
+        a, b = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        self.check(self.L.puckgpu_assign_stats(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def encode_filter(self, coarse, fine, codebook, nsq, x, cell):
         C, dim = coarse.shape
         F = fine.shape[0]

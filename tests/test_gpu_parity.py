@@ -455,6 +455,40 @@ def test_assign_cells_bitexact(ctx, idx_l2, idx_cos):
             assert np.array_equal(_bits(dg), _bits(do))
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_assign_modes_bitexact(ctx, idx_l2, idx_cos, mode):
+    """The bounded assignment (mode 0), the sequential walk (1) and the bounded
+    kernel with every point walked again (2) all give the oracle's cells and
+    scores, also with exact ties between coarse clusters (duplicated coarse
+    centers), between fine centers (duplicated fine centers), and with fine
+    centers not stored by descending norm."""
+    rng = np.random.default_rng(19)
+    try:
+        ctx.set_assign_mode(mode)
+        for ix, _ in (idx_l2, idx_cos):
+            x = ix["raw"][:4000]
+            coarse = ix["coarse"].copy()
+            coarse[5] = coarse[3]
+            coarse[9] = coarse[1]
+            fine = ix["fine"].copy()
+            fine[7] = fine[2]
+            for fn in (fine, np.ascontiguousarray(fine[rng.permutation(fine.shape[0])])):
+                p0, s0 = ctx.assign_stats()
+                cg, dg = ctx.assign_cells(coarse, fn, x, with_dist=True)
+                p1, s1 = ctx.assign_stats()
+                co, do = O.assign_cells(coarse, fn, x)
+                assert np.array_equal(cg, co)
+                assert np.array_equal(_bits(dg), _bits(do))
+                assert p1 - p0 == x.shape[0]
+                if mode == 0:
+                    print(f"mode 0: {s1 - s0} of {x.shape[0]} points walked sequentially", flush=True)
+                    assert s1 - s0 <= x.shape[0]
+                else:
+                    assert s1 - s0 == x.shape[0]
+    finally:
+        ctx.set_assign_mode(0)
+
+
 def test_encode_filter_bitexact(ctx, idx_l2, idx_cos):
     for ix, _ in (idx_l2, idx_cos):
         x = ix["raw"][:3000]
