@@ -111,6 +111,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--index-cache", type=str, default=None,
                     help="directory: load the trained index arrays from it when present, else save them there")
+    ap.add_argument("--workload", choices=["c1", "c4"], default="c1",
+                    help="c1: the headline 10M x 96 workload; c4: 100M x 128 uint8 precision at scale")
+    ap.add_argument("--c4-train", type=int, default=10_000_000, help="c4: k-means training sample")
+    ap.add_argument("--c4-rounds", type=int, default=10, help="c4: kmeans_iterations_count")
+    ap.add_argument("--c4-sweep", type=str, default="1,2,4,8")
+    ap.add_argument("--c4-ftk", type=str, default="500,1000,2000")
+    ap.add_argument("--c4-probe-train", type=int, default=50_000)
     ap.add_argument("--calib-copy", action="store_true",
                     help="one 2 GiB device-to-device copy before the search (HBM counter calibration)")
     args = ap.parse_args()
@@ -119,6 +126,10 @@ def main():
                                    "(query replication / coarse-list sharding), not a measured path",
                           "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
+    if args.workload == "c4":
+        if args.n == 10_000_000:
+            args.n = 100_000_000
+        return c4_main(args)
 
     import torch
     from puck_amd import datasets, train
@@ -401,7 +412,124 @@ def main():
     print(json.dumps(result), flush=True)
 
 
-def precision_forms(gi, ix, q, gt_i, gt_d, topk, scc, ftk, out, args, learned_run=None, nsample=1000):
+def c4_main(args):
+    """Precision at scale: 100,000,000 x 128 integer vectors (the 1M x 128
+    generator with 10,000 means, seed 6) stored as uint8 in HBM, L2.  Index:
+    2,048 x 2,048 centers trained on the GPU over a 10M sample, filter PQ
+    nsq 32 (codebook on the first 1M of the sample), every point assigned and
+    encoded on the GPU.  Batch of 10,000 queries (seed 7), top-k 10.
+    Reported: the plain search over search_coarse_count x filter_topk, the
+    BF16 / FP8 (per-center scales) center distances and the FP16 lookup table
+    (QPS, recall, error of the center distances against FP64, slot agreement
+    with the exact form), the learned probe retrained on this index, and the
+    rerank's roofline over uint8 rows."""
+    import torch
+    from puck_amd import datasets, train
+    from puck_amd import probe as PR
+    from puck_amd.engine import Context, GpuIndex, search_learned
+    from puck_amd.metrics import exact_topk, recall_with_ties
+
+    N, d, C, F, nsq, topk, nq = args.n, 128, 2048, 2048, 32, 10, args.nq
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    x8 = datasets.mixture_uint8_grid(N, d, 10000, seed=6, means_seed=6000, backend="torch", device=dev,
+                                     as_uint8=True)
+    q = datasets.mixture_uint8_grid(nq, d, 10000, seed=7, means_seed=6000, backend="torch", device=dev)
+    torch.cuda.synchronize()
+    log(f"c4 data {N}x{d} uint8 + {nq} queries in {time.time() - t0:.1f}s")
+    ctx = Context(0)
+    t0 = time.time()
+    ix = train.build_chunked(ctx, x8, C, F, nsq, whether_norm=False, train_n=args.c4_train, rounds=args.c4_rounds,
+                             log=lambda m: log(f"c4 build: {m}"))
+    torch.cuda.synchronize()
+    build_s = time.time() - t0
+    pts, seq = ctx.assign_stats()
+    log(f"c4 train + assign + encode {build_s:.1f}s (sequential walks {seq} of {pts} points)")
+    t0 = time.time()
+    gi = GpuIndex(ctx, ix["coarse"], ix["fine"], ix["codebook"], ix["cell_assign"], ix["records"], x8, False)
+    load_s = time.time() - t0
+    log(f"c4 index load {load_s:.1f}s")
+    t0 = time.time()
+    gt_i, gt_d = exact_topk(x8, q, topk)
+    gt_i, gt_d = gt_i.cpu().numpy(), gt_d.cpu().numpy()
+    log(f"c4 ground truth {time.time() - t0:.1f}s")
+
+    def rec(ids):
+        return recall_with_ties(gt_i, gt_d, ids.cpu().numpy().astype(np.int64), topk)[0]
+
+    out = (torch.empty((nq, topk), dtype=torch.float32, device=dev),
+           torch.empty((nq, topk), dtype=torch.int32, device=dev),
+           torch.empty(nq, dtype=torch.int32, device=dev))
+    # plain curve: search_coarse_count x filter_topk; the operating point is
+    # the fastest pair reaching the target recall (else the best recall)
+    curve = []
+    for fk in [int(v) for v in args.c4_ftk.split(",")]:
+        for scc in [int(v) for v in args.c4_sweep.split(",")]:
+            gi.search(q, topk, scc, fk, out=out)
+            r = rec(out[1])
+            tm = timed(lambda: gi.search(q, topk, scc, fk, out=out), args.steps, args.warmup, gi)
+            curve.append({"search_coarse_count": scc, "filter_topk": fk, "recall": round(r, 5),
+                          "qps": round(nq * args.steps / tm["el"], 1),
+                          "rerank_ms": round(float(np.mean(tm["rerank_ms"])), 4),
+                          "walk_ms": round(float(np.mean(tm["scan_ms"])), 4)})
+            log(f"c4 plain: {curve[-1]}")
+    ok = [p for p in curve if p["recall"] >= args.target_recall]
+    op = max(ok, key=lambda p: p["qps"]) if ok else max(curve, key=lambda p: (p["recall"], p["qps"]))
+    chosen, ftk = op["search_coarse_count"], op["filter_topk"]
+    plain = timed(lambda: gi.search(q, topk, chosen, ftk, out=out), args.steps, args.warmup, gi)
+    plain["recall"] = rec(out[1])
+    surv = float(np.asarray(gi.last_filter(nq, ftk)[2], np.float64).sum())
+    rr_ms = float(np.mean(plain["rerank_ms"]))
+    rr_bytes = surv * (d + 4)  # per filter survivor its memory id and its uint8 row
+    res = {"workload": f"{N} x {d} integer vectors stored as uint8, L2, C=F={C}, filter_nsq={nsq}, "
+                       f"top-k={topk}, batch of {nq} queries",
+           "data": "synthetic (seeded 10000-mean integer mixture, generated on the GPU as uint8)",
+           "build_s": round(build_s, 1), "index_load_s": round(load_s, 1),
+           "assignment": {"points": pts, "sequential_walks": seq},
+           "train_sample": min(args.c4_train, N), "kmeans_rounds": args.c4_rounds,
+           "hbm_bytes": {"raw_uint8": N * d, "records": N * (4 + nsq), "raw_float32_would_be": N * d * 4},
+           "plain_curve": curve,
+           "plain": {"search_coarse_count": chosen, "filter_topk": ftk,
+                     "qps": round(nq * args.steps / plain["el"], 1),
+                     "recall_at_10": round(plain["recall"], 5),
+                     "stage_ms_per_step": {k: round(float(v) / args.steps, 3) for k, v in
+                                           zip(["prep+gemm", "center_sort+probe", "walk_scan", "rerank",
+                                                "tie_replay"], plain["stage"])}},
+           "rerank_roofline": {"kernel": "pk_rerank<uint8>", "bound": "hbm", "algorithmic_bytes_per_launch": int(rr_bytes),
+                               "avg_launch_ms": round(rr_ms, 4),
+                               "achieved": round(rr_bytes / (rr_ms / 1000.0) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": round(rr_bytes / (rr_ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)}}
+    res["precision_forms"] = precision_forms(gi, ix, q, gt_i, gt_d, topk, chosen, ftk, out, args, normalize=False)
+    if not args.no_probe:
+        t0 = time.time()
+        qt = datasets.mixture_uint8_grid(args.c4_probe_train, d, 10000, seed=8, means_seed=6000, backend="torch",
+                                         device=dev)
+        gtt, _ = exact_topk(x8, qt, topk)
+        log(f"c4 probe training queries {args.c4_probe_train} + ground truth {time.time() - t0:.1f}s")
+        t0 = time.time()
+        probe, info = PR.fit(ctx, gi, qt, gtt, ix["cell_assign"].view(torch.int32), chosen, ftk, topk,
+                             epochs=args.probe_epochs, log=log)
+        info["train_s"] = round(time.time() - t0, 1)
+        del qt, gtt
+        lout = out + (torch.empty(nq, dtype=torch.int32, device=dev),)
+        lc = []
+        for mg in [float(v) for v in args.margins.split(",")]:
+            run = lambda: search_learned(gi, probe, q, topk, chosen, ftk, margin=mg,
+                                         fallback_frac=args.fallback_frac, out=lout)
+            run()
+            r = rec(lout[1])
+            tm = timed(run, args.steps, args.warmup, gi)
+            lc.append({"margin": mg, "recall": round(r, 5), "qps": round(nq * args.steps / tm["el"], 1),
+                       "mean_budget": round(float(lout[3].double().mean()), 1)})
+            log(f"c4 learned margin {mg}: {lc[-1]}")
+        best = max((p for p in lc if p["recall"] >= args.target_recall), key=lambda p: p["qps"], default=None)
+        res["learned_probe"] = {"training": info, "margin_curve": lc, "best_at_target": best}
+    res["cpu_baseline"] = None
+    print(json.dumps({"c4": res}), flush=True)
+
+
+def precision_forms(gi, ix, q, gt_i, gt_d, topk, scc, ftk, out, args, learned_run=None, nsample=1000,
+                    normalize=True):
     """BF16 / FP8 center distances on the matrix cores (coarse boundary
     rescored in FP32, exact rerank) against the exact FP32 form: QPS, recall,
     slot agreement with the exact form, and the error of the center distances
@@ -413,7 +541,8 @@ def precision_forms(gi, ix, q, gt_i, gt_d, topk, scc, ftk, out, args, learned_ru
     gi.search(q, topk, scc, ftk, out=out)
     exact_ids = out[1].cpu().numpy().copy()
     qs = q[:nsample].double()
-    qs = qs / qs.norm(dim=1, keepdim=True)
+    if normalize:  # whether_norm: the centers are compared with the normalised query
+        qs = qs / qs.norm(dim=1, keepdim=True)
     C64 = torch.as_tensor(ix["coarse"], device=q.device).double()
     F64 = torch.as_tensor(ix["fine"], device=q.device).double()
     want_c = 0.5 * (C64 ** 2).sum(1)[None, :] - qs @ C64.T

@@ -127,6 +127,14 @@ int puckgpu_index_load(puckgpu_ctx* ctx, const puckgpu_index_params* params, con
                        const uint8_t* filter_records, const float* features, int device_ptrs,
                        puckgpu_index** out);
 void puckgpu_index_free(puckgpu_index* index);
+/* Raw vectors stored as uint8 (the precision form of integer-valued data, no
+ * counterpart in the reference, which keeps float rows in all_data.feat.bin,
+ * puck_index.cpp:198-227): features [N x dim] uint8 in local-id order,
+ * attached to an index loaded with features = NULL and whether_norm = 0.  The
+ * exact rerank converts each value to float, so for data whose values are the
+ * integers 0..255 its results are those of the float rows, at a quarter of
+ * the bytes.  Inserts into such an index are refused. */
+int puckgpu_index_attach_raw_u8(puckgpu_ctx* ctx, puckgpu_index* index, const uint8_t* features, int device_ptrs);
 /* Second-stage PQ of a whether_pq = true index: pq_nsq subspaces of
  * dim / pq_nsq floats (1, 2 or 4; pq_nsq <= 128), pq_codebook [pq_nsq x 256 x
  * dim / pq_nsq] (pq_codebook.dat rows), pq_records [N x (4 + pq_nsq)] in local
@@ -372,6 +380,11 @@ int puckgpu_assign_cells(puckgpu_ctx* ctx, const float* coarse, const float* fin
  * walked again (test of the fallback path).  Counters: points assigned and
  * points walked sequentially since the context was created. */
 int puckgpu_set_assign_mode(puckgpu_ctx* ctx, int mode);
+/* Progress lines of long calls (puckgpu_train_centers: one per k-means run),
+ * passed to fn(message, user) on the calling thread; fn = NULL stops them.
+ * No counterpart in the reference, which logs through glog. */
+typedef void (*puckgpu_progress_fn)(const char* message, void* user);
+int puckgpu_set_progress(puckgpu_ctx* ctx, puckgpu_progress_fn fn, void* user);
 int puckgpu_assign_stats(puckgpu_ctx* ctx, uint64_t* points, uint64_t* sequential);
 /* Filter-PQ records of n points: records_out [n x (4 + nsq)] in the
  * filter_data.dat layout. */

@@ -21,8 +21,11 @@ EXPORTS = [
     "puckgpu_server_buckets", "puckgpu_server_search", "puckgpu_server_tune", "puckgpu_last_scan_ms",
     "puckgpu_center_distances", "puckgpu_index_attach_pq", "puckgpu_encode_pq", "puckgpu_search_noalloc",
     "puckgpu_index_insert", "puckgpu_index_version", "puckgpu_kmeans", "puckgpu_train_centers",
-    "puckgpu_set_assign_mode", "puckgpu_assign_stats",
+    "puckgpu_set_assign_mode", "puckgpu_assign_stats", "puckgpu_index_attach_raw_u8", "puckgpu_set_progress",
 ]
+
+# void (*)(const char* message, void* user)
+PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_void_p)
 
 
 class IndexParams(ctypes.Structure):
@@ -104,6 +107,10 @@ def load():
     L.puckgpu_lut.restype = c_int
     L.puckgpu_assign_cells.argtypes = [vp, vp, vp, u32, u32, u32, u32, vp, vp, vp, c_int]
     L.puckgpu_assign_cells.restype = c_int
+    L.puckgpu_index_attach_raw_u8.argtypes = [vp, vp, vp, c_int]
+    L.puckgpu_index_attach_raw_u8.restype = c_int
+    L.puckgpu_set_progress.argtypes = [vp, PROGRESS_FN, vp]
+    L.puckgpu_set_progress.restype = c_int
     L.puckgpu_set_assign_mode.argtypes = [vp, c_int]
     L.puckgpu_set_assign_mode.restype = c_int
     L.puckgpu_assign_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]

@@ -138,7 +138,20 @@ struct puckgpu_ctx {
     unsigned char* noalloc = nullptr;  // this device's copy of `scratch` (puckgpu_search_noalloc)
     int assign_mode = 0;                // puckgpu_set_assign_mode
     uint64_t assign_points = 0, assign_sequential = 0;
+    puckgpu_progress_fn progress = nullptr;  // puckgpu_set_progress
+    void* progress_user = nullptr;
 };
+
+// A progress line to the caller's callback (long calls: training).
+void report(puckgpu_ctx* ctx, const char* fmt, ...) {
+    if (!ctx || !ctx->progress) return;
+    char buf[256];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    ctx->progress(buf, ctx->progress_user);
+}
 
 struct puckgpu_index {
     puckgpu_index_params p{};
@@ -155,6 +168,7 @@ struct puckgpu_index {
     float* rec_off = nullptr;
     uint8_t* rec_codes = nullptr;
     float* raw = nullptr;  // null when the index holds no raw vectors (whether_pq = true)
+    uint8_t* raw8 = nullptr;  // uint8 raw rows (puckgpu_index_attach_raw_u8) in place of raw
     // second-stage PQ (whether_pq = true), attached by puckgpu_index_attach_pq
     uint32_t pnsq = 0, plsq = 0;
     float* pq_codebook = nullptr;
@@ -287,7 +301,7 @@ void index_release(puckgpu_index* ix) {
     void* ptrs[] = {ix->coarse, ix->fine, ix->codebook, ix->coarse_norms, ix->stationary, ix->cell_start,
                     ix->mdo, ix->mem2local, ix->rec_off, ix->rec_codes, ix->raw, ix->coarse16, ix->fine16,
                     ix->coarse8, ix->fine8, ix->coarse8s, ix->fine8s, ix->pq_codebook, ix->pq_off, ix->pq_codes,
-                    ix->nbase, ix->fbound};
+                    ix->nbase, ix->fbound, ix->raw8};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete ix;
@@ -503,6 +517,36 @@ int puckgpu_index_load(puckgpu_ctx* ctx, const puckgpu_index_params* params, con
 
 void puckgpu_index_free(puckgpu_index* index) { index_release(index); }
 
+int puckgpu_index_attach_raw_u8(puckgpu_ctx* ctx, puckgpu_index* ix, const uint8_t* features, int device_ptrs) {
+    if (!ctx || !ix || (ix->p.total_point_count && !features)) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    if (ix->raw || ix->raw8 || ix->pq_codes)
+        return set_err(ctx, PUCKGPU_EINVAL, "the index already holds raw vectors or a second-stage PQ");
+    if (ix->p.whether_norm)
+        return set_err(ctx, PUCKGPU_EINVAL, "uint8 raw vectors need whether_norm = 0 (normalised rows are not integers)");
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    hipStream_t s = ctx->stream;
+    const uint32_t d = ix->p.dim, N = ix->p.total_point_count;
+    int rc = 0;
+    uint8_t* r8 = nullptr;
+    if ((rc = dalloc(ctx, &r8, (size_t)N * d))) return rc;
+    const uint8_t* src = (const uint8_t*)dev_in(ctx, S_A2, features, (size_t)N * d, device_ptrs, &rc);
+    hipError_t e = hipSuccess;
+    if (N && !src) e = hipErrorOutOfMemory;
+    // rows into memory order with the records' permutation
+    if (e == hipSuccess && N) e = pk_launch_gather_rows(src, d, 0, r8, d, d, ix->mem2local, N, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (!device_ptrs && S_A2 < ctx->bufs.size() && ctx->bufs[S_A2].p) {
+        (void)hipFree(ctx->bufs[S_A2].p);
+        ctx->bufs[S_A2] = DevBuf();
+    }
+    if (e != hipSuccess) {
+        (void)hipFree(r8);
+        return rc ? rc : set_err(ctx, PUCKGPU_EHIP, "attaching uint8 raw vectors failed: %s", hipGetErrorString(e));
+    }
+    ix->raw8 = r8;
+    return PUCKGPU_OK;
+}
+
 int puckgpu_index_attach_pq(puckgpu_ctx* ctx, puckgpu_index* ix, uint32_t pq_nsq, const float* pq_codebook,
                             const uint8_t* pq_records, int device_ptrs) {
     if (!ctx || !ix || !pq_codebook || (ix->p.total_point_count && !pq_records))
@@ -512,6 +556,7 @@ int puckgpu_index_attach_pq(puckgpu_ctx* ctx, puckgpu_index* ix, uint32_t pq_nsq
         return set_err(ctx, PUCKGPU_EINVAL, "pq_nsq %u must divide dim %u into sub-vectors of 1, 2 or 4", pq_nsq, d);
     if (pq_nsq > 128) return set_err(ctx, PUCKGPU_EINVAL, "pq_nsq must be <= 128 (its lookup table lives in LDS)");
     if (ix->pq_codes) return set_err(ctx, PUCKGPU_EINVAL, "the index already has a second-stage PQ");
+    if (ix->raw8) return set_err(ctx, PUCKGPU_EINVAL, "the index ranks by its uint8 raw vectors");
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
     hipStream_t s = ctx->stream;
     const uint32_t plsq = d / pq_nsq;
@@ -768,7 +813,7 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     if (ev) PK_CHECK(hipEventRecord(ev[3], s));
     PkRerankArgs ra;
     ra.q = b.q; ra.qstatus = b.qst; ra.filt_t = ft; ra.filt_id = fi; ra.filt_n = fn;
-    ra.raw = ix->raw; ra.mem2local = ix->mem2local; ra.dim = d; ra.ftk = cfg.ftk; ra.topk = cfg.topk;
+    ra.raw = ix->raw; ra.raw8 = ix->raw8; ra.mem2local = ix->mem2local; ra.dim = d; ra.ftk = cfg.ftk; ra.topk = cfg.topk;
     ra.out_d = od; ra.out_id = oi; ra.out_n = on;
     ra.qlist = order; ra.flags = b.flags; ra.exact = 0; ra.tie_ids = b.tie_ids; ra.tie_n = b.tie_n;
     ra.q_topk = q_topk; ra.out_stride = cfg.ostride; ra.q_nf = nf; ra.F = F;
@@ -946,7 +991,7 @@ int make_cfg(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_par
     cfg.ostride = ostride;
     cfg.rr = sp->radius_rate;
     cfg.pu = pu;
-    if (!ix->raw && !ix->pq_codes && ix->p.total_point_count)
+    if (!ix->raw && !ix->raw8 && !ix->pq_codes && ix->p.total_point_count)
         return set_err(ctx, PUCKGPU_EINVAL, "the index holds neither raw vectors nor a second-stage PQ to rank by");
     if ((sp->precision & ~(15u | PUCKGPU_LUT_FP16)) || (sp->precision & 15u) > 2)
         return set_err(ctx, PUCKGPU_EINVAL, "precision must be 0, 1 or 2, optionally with PUCKGPU_LUT_FP16");
@@ -1912,6 +1957,13 @@ int puckgpu_lut(puckgpu_ctx* ctx, uint32_t n, uint32_t dim, const float* queries
 }
 
 // ---------------------------------------------------------------------------
+int puckgpu_set_progress(puckgpu_ctx* ctx, puckgpu_progress_fn fn, void* user) {
+    if (!ctx) return PUCKGPU_EINVAL;
+    ctx->progress = fn;
+    ctx->progress_user = user;
+    return PUCKGPU_OK;
+}
+
 int puckgpu_set_assign_mode(puckgpu_ctx* ctx, int mode) {
     if (!ctx) return PUCKGPU_EINVAL;
     if (mode < 0 || mode > 2) return set_err(ctx, PUCKGPU_EINVAL, "assign mode must be 0, 1 or 2");
@@ -2197,6 +2249,7 @@ int puckgpu_index_insert(puckgpu_ctx* ctx, puckgpu_index* ix, uint32_t n, const 
                          uint32_t* cells_out, uint8_t* filter_records_out, uint8_t* pq_records_out,
                          float* features_out, int device_ptrs) {
     if (!ctx || !ix || (n && !features)) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    if (ix->raw8) return set_err(ctx, PUCKGPU_EINVAL, "inserts into an index with uint8 raw vectors are not supported");
     if (!ix->raw && !ix->pq_codes)
         return set_err(ctx, PUCKGPU_EINVAL, "the index holds neither raw vectors nor a second-stage PQ");
     if (n == 0) return PUCKGPU_OK;
@@ -2686,10 +2739,13 @@ int puckgpu_train_centers(puckgpu_ctx* ctx, uint32_t dim, uint32_t n, const floa
         // coarse centers S on the data less the best fine centers so far
         if ((rc = km_run(ctx, st, dim, n, C, tv, c0, a, &err, nullptr, nullptr, s))) return rc;
         if (err_trace) err_trace[2 * ite] = err;
+        report(ctx, "train_centers round %d: coarse k-means (%u x %u, k=%u, Lloyd limit %d) error %g", ite + 1, n,
+               dim, C, st.niter, (double)err);
         PK_CHECK(pk_launch_sub_assigned(xd, n, dim, c0, a, tv, s));
         // fine centers T on the data less the new coarse centers
         if ((rc = km_run(ctx, st, dim, n, F, tv, f0, a, &err, nullptr, nullptr, s))) return rc;
         if (err_trace) err_trace[2 * ite + 1] = err;
+        report(ctx, "train_centers round %d: fine k-means (k=%u) error %g", ite + 1, F, (double)err);
         if ((min_err - err) >= 1e-4) {
             PK_CHECK(hipMemcpyAsync(cb, c0, 4ull * C * dim, hipMemcpyDeviceToDevice, s));
             PK_CHECK(hipMemcpyAsync(fb, f0, 4ull * F * dim, hipMemcpyDeviceToDevice, s));

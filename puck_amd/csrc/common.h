@@ -199,31 +199,35 @@ __device__ __forceinline__ float pk_pivot(float top, float qn, double rr) {
 //                 t = (((l0+l1)+(l2+l3)) + ((l4+l5)+l6)) + (l7 + t)
 //   tail (<16)    4-lane fma blocks, (b0+b1)+(b2+b3), then a scalar fma chain.
 // Sum over the last dim % 16 elements (left > 0) in the compiled tail order.
-__device__ __forceinline__ float pk_rerank_tail(const float* __restrict__ x, const float* q, uint32_t left) {
+// T: float rows, or uint8_t rows of integer features in [0, 255] (each value
+// converted exactly to float, so the arithmetic is that of the float rows).
+template <typename T>
+__device__ __forceinline__ float pk_rerank_tail(const T* __restrict__ x, const float* q, uint32_t left) {
     float b[4] = {0.f, 0.f, 0.f, 0.f};
     uint32_t i = 0;
     for (; i + 4 <= left; i += 4)
         for (int j = 0; j < 4; ++j) {
-            float d = x[i + j] - q[i + j];
+            float d = (float)x[i + j] - q[i + j];
             b[j] = fmaf(d, d, b[j]);
         }
     float s = (b[0] + b[1]) + (b[2] + b[3]);
     for (; i < left; ++i) {
-        float d = x[i] - q[i];
+        float d = (float)x[i] - q[i];
         s = fmaf(d, d, s);
     }
     return s;
 }
 
-__device__ __forceinline__ float pk_rerank_dist(const float* __restrict__ x, const float* q, uint32_t dim) {
+template <typename T>
+__device__ __forceinline__ float pk_rerank_dist(const T* __restrict__ x, const float* q, uint32_t dim) {
     float t = 0.0f;
     uint32_t m = 0;
     for (; m + 16 <= dim; m += 16) {
         float l[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            float d0 = x[m + j] - q[m + j];
-            float d1 = x[m + 8 + j] - q[m + 8 + j];
+            float d0 = (float)x[m + j] - q[m + j];
+            float d1 = (float)x[m + 8 + j] - q[m + 8 + j];
             l[j] = fmaf(d0, d0, d1 * d1);
         }
         t = (((l[0] + l[1]) + (l[2] + l[3])) + ((l[4] + l[5]) + l[6])) + (l[7] + t);

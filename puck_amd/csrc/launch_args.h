@@ -59,6 +59,7 @@ struct PkRerankArgs {
     const uint32_t* filt_id; // memory ids
     const uint32_t* filt_n;
     const float* raw;        // N x dim, memory order
+    const uint8_t* raw8;     // N x dim uint8 rows (the uint8 storage form), or null
     const uint32_t* mem2local;
     uint32_t dim, ftk, topk;
     float* out_d;            // nq x topk

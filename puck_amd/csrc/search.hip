@@ -1705,8 +1705,12 @@ __device__ __forceinline__ float pk_pq_dist(const PkRerankArgs& a, const float* 
     return t + ((acc[0] + acc[1]) + (acc[2] + acc[3]));
 }
 
+// T: the element type of the raw rows (float, or uint8_t for the uint8
+// storage form: integer features, converted exactly).
+template <typename T>
 __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
+    const T* raw = (const T*)(sizeof(T) == 1 ? (const void*)a.raw8 : (const void*)a.raw);
     const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
     uint32_t topk = a.topk;
     if (a.q_topk) topk = (a.q_topk[qi] >= 1u && a.q_topk[qi] <= a.topk) ? a.q_topk[qi] : 0u;
@@ -1763,7 +1767,7 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
         const uint32_t dim = a.dim, nfull = dim >> 4;
         for (uint32_t r0 = 0; r0 < n2; r0 += RP * ng) {
             uint32_t rr[RP], mid[RP];
-            const float* x[RP];
+            const T* x[RP];
             float t[RP];
 #pragma unroll
             for (int h = 0; h < RP; ++h) {
@@ -1771,7 +1775,7 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
                 // in-row load for every lane (a lane-conditional load is waited for at once)
                 const uint32_t fid = a.filt_id[(uint64_t)qi * ftk + (rr[h] < ftk ? rr[h] : ftk - 1u)];
                 mid[h] = rr[h] < n ? fid : 0u;
-                x[h] = a.raw + (uint64_t)mid[h] * dim;
+                x[h] = raw + (uint64_t)mid[h] * dim;
                 t[h] = 0.0f;
             }
             for (uint32_t s0 = 0; s0 < nfull; s0 += SB) {
@@ -1781,8 +1785,8 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
 #pragma unroll
                     for (int st = 0; st < SB; ++st)
                         if (s0 + st < nfull) {
-                            v0[h][st] = x[h][16 * (s0 + st) + j];
-                            v1[h][st] = x[h][16 * (s0 + st) + 8 + j];
+                            v0[h][st] = (float)x[h][16 * (s0 + st) + j];
+                            v1[h][st] = (float)x[h][16 * (s0 + st) + 8 + j];
                         }
 #pragma unroll
                 for (int st = 0; st < SB; ++st) {
@@ -1825,7 +1829,7 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
                 d = pk_ord2f((uint32_t)(keys[r] >> 32));
             } else {
                 uint32_t mid = a.tie_ids[(uint64_t)qi * PK_TIE_MAX + (r - n)];
-                d = pq ? pk_pq_dist(a, plut, mid, tfin, qn) : pk_rerank_dist(a.raw + (uint64_t)mid * a.dim, qv, a.dim);
+                d = pq ? pk_pq_dist(a, plut, mid, tfin, qn) : pk_rerank_dist(raw + (uint64_t)mid * a.dim, qv, a.dim);
             }
             atomicMin(&tie_min, pk_f2ord(d));
         }
@@ -1934,13 +1938,18 @@ extern "C" hipError_t pk_launch_rerank(const PkRerankArgs* args, uint32_t nq, hi
     size_t* slot = pk_lds_slot(lds_cache);
     if (!slot) return hipErrorInvalidDevice;
     if (!*slot) {
-        size_t al = 0;
-        const hipError_t e = pk_allow_lds((const void*)pk_rerank, 160 * 1024, &al);
+        size_t al = 0, al8 = 0;
+        hipError_t e = pk_allow_lds((const void*)pk_rerank<float>, 160 * 1024, &al);
         if (e != hipSuccess) return e;
-        *slot = al;
+        e = pk_allow_lds((const void*)pk_rerank<uint8_t>, 160 * 1024, &al8);
+        if (e != hipSuccess) return e;
+        *slot = al < al8 ? al : al8;
     }
     if (smem > *slot) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(pk_rerank, dim3(nq), dim3(RERANK_THREADS), smem, s, *args);
+    if (args->raw8 && !args->pq_codes)
+        hipLaunchKernelGGL(pk_rerank<uint8_t>, dim3(nq), dim3(RERANK_THREADS), smem, s, *args);
+    else
+        hipLaunchKernelGGL(pk_rerank<float>, dim3(nq), dim3(RERANK_THREADS), smem, s, *args);
     return hipGetLastError();
 }
 

@@ -16,25 +16,28 @@ import numpy as np
 
 
 def mixture_uint8_grid(n, d=128, n_means=1000, seed=0, means_seed=1000, mean_high=160.0,
-                       sigma=25.0, backend="numpy", device="cuda", chunk=1 << 20):
+                       sigma=25.0, backend="numpy", device="cuda", chunk=1 << 20, as_uint8=False):
+    """Rows are integers 0..255: float32, or uint8 with as_uint8 (the same
+    values, a quarter of the bytes; the torch backend then never holds the
+    float rows of more than one chunk)."""
     if backend == "numpy":
         rm = np.random.Generator(np.random.PCG64(means_seed))
         means = rm.uniform(0.0, mean_high, size=(n_means, d)).astype(np.float32)
         r = np.random.Generator(np.random.PCG64(seed))
         comp = r.integers(0, n_means, size=n)
         x = means[comp] + r.standard_normal((n, d), dtype=np.float32) * np.float32(sigma)
-        return np.rint(np.clip(x, 0.0, 255.0)).astype(np.float32)
+        return np.rint(np.clip(x, 0.0, 255.0)).astype(np.uint8 if as_uint8 else np.float32)
     import torch
     g = torch.Generator(device=device)
     g.manual_seed(means_seed)
     means = torch.rand((n_means, d), generator=g, device=device) * mean_high
     g.manual_seed(seed)
-    out = torch.empty((n, d), dtype=torch.float32, device=device)
+    out = torch.empty((n, d), dtype=torch.uint8 if as_uint8 else torch.float32, device=device)
     for s in range(0, n, chunk):
         e = min(n, s + chunk)
         comp = torch.randint(0, n_means, (e - s,), generator=g, device=device)
         x = means[comp] + torch.randn((e - s, d), generator=g, device=device) * sigma
-        out[s:e] = torch.round(torch.clamp(x, 0.0, 255.0))
+        out[s:e] = torch.round(torch.clamp(x, 0.0, 255.0)).to(out.dtype)
     return out
 
 

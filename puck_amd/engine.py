@@ -145,6 +145,15 @@ class Context:
                                                _ptr(cell), _ptr(dist), m))
         return (cell, dist) if with_dist else cell
 
+    def set_progress(self, fn):
+        """fn(message: str) receives the library's progress lines (None: off)."""
+        if fn is None:
+            self._progress = None
+            self.check(self.L.puckgpu_set_progress(self.h, _lib.PROGRESS_FN(), None))
+            return
+        self._progress = _lib.PROGRESS_FN(lambda m, u: fn(m.decode()))
+        self.check(self.L.puckgpu_set_progress(self.h, self._progress, None))
+
     def set_assign_mode(self, mode):
         """0: bounded assignment with certificate (default), 1: the sequential
         walk for every point, 2: bounded, then every point walked again."""
@@ -293,7 +302,14 @@ class GpuIndex:
             raise ValueError(f"codebook: expected {nsq} x 256 x {dim // max(nsq, 1)} floats")
         _check(cell_assign, "cell_assign", _U32, shape=(N,), device=dv)
         _check(records, "records", _U8, shape=(N, 4 + nsq), device=dv)
-        _check(features, "features", _F32, shape=(N, dim), device=dv)
+        raw_u8 = features is not None and _dtype_name(features) == "uint8"
+        if raw_u8:
+            # integer features stored as uint8 (puckgpu_index_attach_raw_u8)
+            if whether_norm:
+                raise ValueError("uint8 features need whether_norm = False")
+            _check(features, "features", _U8, shape=(N, dim), device=dv)
+        else:
+            _check(features, "features", _F32, shape=(N, dim), device=dv)
         if features is None and pq_records is None:
             raise ValueError("an index needs its raw features or a second-stage PQ to rank by")
         self.pq_nsq = 0
@@ -308,8 +324,12 @@ class GpuIndex:
         m = _mode(coarse, fine, codebook, cell_assign, records, features, pq_codebook, pq_records)
         ctx.sync_torch(coarse, features, pq_records)
         ctx.check(ctx.L.puckgpu_index_load(ctx.h, ctypes.byref(p), _ptr(coarse), _ptr(fine), _ptr(codebook),
-                                           _ptr(cell_assign), _ptr(records), _ptr(features), m, ctypes.byref(h)))
+                                           _ptr(cell_assign), _ptr(records), None if raw_u8 else _ptr(features), m,
+                                           ctypes.byref(h)))
         self.h = h
+        self.raw_u8 = raw_u8
+        if raw_u8:
+            ctx.check(ctx.L.puckgpu_index_attach_raw_u8(ctx.h, self.h, _ptr(features), m))
         if pq_records is not None:
             ctx.check(ctx.L.puckgpu_index_attach_pq(ctx.h, self.h, pq_nsq, _ptr(pq_codebook), _ptr(pq_records), m))
             self.pq_nsq = pq_nsq

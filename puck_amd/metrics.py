@@ -11,6 +11,7 @@ def exact_topk(x, q, k, extra=8, qchunk=4096, bchunk=1 << 21):
     order.  Returns (ids int64 [nq, k+extra], dists float64 [nq, k+extra])."""
     kk = k + extra
     nq, N = q.shape[0], x.shape[0]
+    # x may be uint8 (integer features): blocks are converted to float32
     xn = (x.double() ** 2).sum(1).float() if N <= (1 << 20) else torch.cat(
         [(x[s:s + bchunk].double() ** 2).sum(1).float() for s in range(0, N, bchunk)])
     out_i = torch.empty((nq, kk), dtype=torch.int64, device=q.device)
@@ -19,7 +20,7 @@ def exact_topk(x, q, k, extra=8, qchunk=4096, bchunk=1 << 21):
         qq = q[qs:qs + qchunk]
         cand_i, cand_d = [], []
         for bs in range(0, N, bchunk):
-            xb = x[bs:bs + bchunk]
+            xb = x[bs:bs + bchunk].float()
             d = torch.addmm(xn[bs:bs + bchunk][None, :], qq, xb.T, beta=1.0, alpha=-2.0)
             v, i = torch.topk(d, min(kk, xb.shape[0]), dim=1, largest=False)
             cand_i.append(i + bs)

@@ -79,3 +79,42 @@ def build(ctx, x, C, F, nsq, whether_norm, train_n=262144, niter=30, seed=0, pq_
         out.update(pq_codebook=pcb, pq_records=ctx.encode_pq(coarse, fine, cb, nsq, rec, pcb, pq_nsq, x, cell),
                    pq_nsq=pq_nsq)
     return out
+
+
+def build_chunked(ctx, x, C, F, nsq, whether_norm, train_n=10_000_000, pq_train_n=1_000_000, niter=30, seed=0,
+                  rounds=10, mean_mode=0, chunk=1 << 22, log=None):
+    """build() for point sets whose float rows do not fit at once: x is a
+    CUDA [N, d] float32 or uint8 tensor (uint8: integer features, which the
+    index keeps as uint8 rows).  The centers are trained on train_n sampled
+    points (train_points_count), the codebook on the first pq_train_n of them
+    (pq_train_points_count, puck_index.cpp:40), then the points are assigned
+    and encoded chunk by chunk.  Returns build()'s dict; raw is x itself.
+    log(msg), when given, receives a line per step (and the library's
+    training progress)."""
+    log = log or (lambda m: None)
+    ctx.set_progress(log)
+    N = x.shape[0]
+    g = torch.Generator(device=x.device)
+    g.manual_seed(seed + 99)
+    sel = torch.randperm(N, generator=g, device=x.device)[: min(train_n, N)]
+    xs = x[sel].float().contiguous()
+    del sel
+    coarse, fine = train_centers(ctx, xs, C, F, rounds, niter, seed, mean_mode)
+    log(f"centers trained on {xs.shape[0]} points")
+    xq = xs[: min(pq_train_n, xs.shape[0])].contiguous()
+    del xs
+    torch.cuda.empty_cache()
+    cb = train_quantizers(ctx, coarse, fine, xq, [nsq], niter, seed + 7, mean_mode)[0].reshape(-1).contiguous()
+    log(f"filter codebook trained on {xq.shape[0]} points")
+    del xq
+    cell = torch.empty(N, dtype=torch.int32, device=x.device)
+    rec = torch.empty((N, 4 + nsq), dtype=torch.uint8, device=x.device)
+    for s0 in range(0, N, chunk):
+        xc = x[s0:s0 + chunk].float().contiguous()
+        cell[s0:s0 + xc.shape[0]] = ctx.assign_cells(coarse, fine, xc)
+        rec[s0:s0 + xc.shape[0]] = ctx.encode_filter(coarse, fine, cb, nsq, xc, cell[s0:s0 + xc.shape[0]])
+        del xc
+        log(f"assigned and encoded {min(N, s0 + chunk)} of {N}")
+    ctx.set_progress(None)
+    return dict(coarse=coarse, fine=fine, codebook=cb, cell_assign=cell, records=rec, raw=x,
+                whether_norm=whether_norm, nsq=nsq)
