@@ -270,7 +270,7 @@ enum Slot {
     S_Q = 0, S_QN, S_QST, S_CD, S_FD, S_CSD, S_CSI, S_FSD, S_FSI, S_FT, S_FI, S_FN, S_OD, S_OI, S_ON, S_STATS,
     S_IN_Q, S_A0, S_A1, S_A2, S_A3, S_A4, S_A5, S_A6, S_A7, S_FLAGS, S_QLIST, S_TIEID, S_TIEN, S_QTOPK, S_QSCC,
     S_QNF, S_PFEAT, S_PWORK, S_PNF, S_PY, S_PT, S_PCV, S_LP16, S_LP8, S_LPS, S_CAND_D, S_CAND_I, S_WLIST, S_SCI,
-    S_SCK, S_SCM, S_INS_X, S_INS_Y, S_INS_CELL, S_INS_REC, S_INS_PQ, S_INS_RANK, S_INS_ADD, S_INS_SHIFT, S_INS_WORK,
+    S_SCM, S_INS_X, S_INS_Y, S_INS_CELL, S_INS_REC, S_INS_PQ, S_INS_RANK, S_INS_ADD, S_INS_SHIFT, S_INS_WORK,
     S_INS_DOLD, S_INS_DNEW, S_INS_ACC, S_KM_IP, S_KM_PN, S_KM_CN, S_KM_ASSIGN, S_KM_DIS, S_KM_PERM, S_KM_STARTS,
     S_KM_HIST, S_KM_OFF, S_KM_WORK, S_KM_BEST, S_KM_IDS, S_KM_X, S_KM_TV, S_KM_C0, S_KM_F0, S_KM_CB, S_KM_FB,
     S_KM_FA, S_KM_CENT, S_AB_STMIN, S_AB_ROW, S_AB_ABS, S_AB_FBPM, S_AB_LIST, S_AB_CNT, S_COUNT
@@ -691,7 +691,6 @@ struct PassBufs {
     // split budget scan: work items, candidate keys, per-query counts, the
     // list of queries left to the per-query kernel and its length
     uint2* sc_items = nullptr;
-    uint64_t* sc_keys = nullptr;
     uint32_t* sc_meta = nullptr;
     uint32_t* sc_olist = nullptr;
     uint32_t* sc_ocount = nullptr;
@@ -799,7 +798,7 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     wa.flags = b.flags; wa.exact = 0; wa.tie_ids = b.tie_ids; wa.tie_n = b.tie_n;
     wa.q_scc = q_scc; wa.q_nf = nf; wa.cstride = cfg.keep ? cfg.keep : 1; wa.fdist = b.fdist;
     wa.wlist = nullptr; wa.wcount = nullptr;
-    wa.sc_items = b.sc_items; wa.sc_keys = b.sc_keys; wa.sc_meta = b.sc_meta; wa.sc_olist = b.sc_olist;
+    wa.sc_items = b.sc_items; wa.sc_meta = b.sc_meta; wa.sc_olist = b.sc_olist;
     wa.sc_ocount = b.sc_ocount; wa.sc_capp = PK_SC_CAPP;
     wa.lut16 = cfg.lut16; wa.cb_nmax = ix->cb_nmax;
     if (nf) {
@@ -924,7 +923,6 @@ int scratch_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     b.wcount = b.wlist ? b.wlist + 2 * (size_t)cap : nullptr;
     if (cfg.budget) {
         b.sc_items = (uint2*)scratch_slot(ctx, S_SCI, sizeof(uint2) * (size_t)cap * PK_SC_CAPP, &rc);
-        b.sc_keys = (uint64_t*)scratch_slot(ctx, S_SCK, sizeof(uint64_t) * (size_t)cap * PK_SC_CAPP, &rc);
         b.sc_meta = (uint32_t*)scratch_slot(ctx, S_SCM, sizeof(uint32_t) * (3 * (size_t)cap + 1), &rc);
         b.sc_olist = b.sc_meta ? b.sc_meta + 2 * (size_t)cap : nullptr;
         b.sc_ocount = b.sc_meta ? b.sc_meta + 3 * (size_t)cap : nullptr;
@@ -1565,7 +1563,6 @@ int bucket_create(puckgpu_server* sv, uint32_t cap, const PassCfg& cfg, PkBucket
     }
     if (cfg.budget) {
         if ((rc = balloc(ctx, bk, &b.sc_items, (size_t)cap * PK_SC_CAPP)) ||
-            (rc = balloc(ctx, bk, &b.sc_keys, (size_t)cap * PK_SC_CAPP)) ||
             (rc = balloc(ctx, bk, &b.sc_meta, 3 * (size_t)cap + 1))) {
             bucket_release(bk);
             return rc;

@@ -296,42 +296,67 @@ __device__ __forceinline__ void pk_block_andor(uint32_t& ka, uint32_t& ko, uint3
 // 8-bit digits that starts at the highest bit in which the keys differ (the
 // distances of one row usually share their sign and top exponent bits,
 // which would send every key of a first pass to one histogram bin).
-// blockDim.x >= 256, a multiple of 64; hist has 258 words, tmp
+// Two block barriers per pass: the histogram is double-buffered (the other
+// waves clear the next pass's buffer while wave 0 scans this one) and wave 0
+// finds the digit with one wave-wide prefix sum over 4 bins per lane.
+// blockDim.x >= 256, a multiple of 64; hist has PK_SEL_HIST words, tmp
 // blockDim.x / 64 + 1, red 2 * blockDim.x / 64.  Every thread returns it.
+#define PK_SEL_HIST 520
 template <typename EachF>
 __device__ uint32_t pk_select_rank_each(EachF each, uint32_t want, uint32_t* hist, uint32_t* tmp, uint32_t* red) {
+    (void)tmp;
     const uint32_t t = threadIdx.x;
     uint32_t ka = 0xffffffffu, ko = 0u;
     each([&](uint32_t k) {
         ka &= k;
         ko |= k;
     });
+    if (t < 256) hist[t] = 0;  // buffer 0, ordered before its use by the barriers of pk_block_andor
     pk_block_andor(ka, ko, red);
     const uint32_t diff = ka ^ ko;
     const int top = diff ? 32 - __clz(diff) : 0;  // bits [top-1, 0] vary
     uint32_t mask = top >= 32 ? 0u : ~((1u << top) - 1u);
     uint32_t prefix = ka & mask;
+    int buf = 0;
     for (int hi = top; hi > 0; hi -= 8) {
         const int shift = hi > 8 ? hi - 8 : 0;
         const uint32_t dm = (1u << (hi - shift)) - 1u;
-        if (t < 256) hist[t] = 0;
-        __syncthreads();
+        uint32_t* h = hist + 260 * buf;
+        uint32_t* hn = hist + 260 * (buf ^ 1);
         each([&](uint32_t k) {
-            if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & dm], 1u);
+            if ((k & mask) == prefix) atomicAdd(&h[(k >> shift) & dm], 1u);
         });
         __syncthreads();
-        const uint32_t c = t < 256 ? hist[t] : 0u;
-        uint32_t tot;
-        const uint32_t before = pk_block_exscan(c, tmp, &tot);
-        if (c > 0 && before < want && want <= before + c) {
-            hist[256] = t;
-            hist[257] = want - before;
+        if (t < 64) {
+            const uint32_t c0 = h[4 * t], c1 = h[4 * t + 1], c2 = h[4 * t + 2], c3 = h[4 * t + 3];
+            const uint32_t sum = (c0 + c1) + (c2 + c3);
+            uint32_t incl = sum;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(incl, o, 64);
+                if (t >= (uint32_t)o) incl += y;
+            }
+            const uint32_t before = incl - sum;
+            if (before < want && want <= incl) {  // exactly one lane
+                const uint32_t cs[4] = {c0, c1, c2, c3};
+                uint32_t bb = before, e = 0;
+#pragma unroll
+                for (uint32_t j = 0; j < 3; ++j)
+                    if (e == j && want > bb + cs[j]) {
+                        bb += cs[j];
+                        e = j + 1;
+                    }
+                h[256] = 4 * t + e;
+                h[257] = want - bb;
+            }
+        } else {
+            for (uint32_t i = t - 64; i < 256; i += blockDim.x - 64) hn[i] = 0;
         }
         __syncthreads();
-        prefix |= hist[256] << shift;
+        prefix |= h[256] << shift;
         mask |= dm << shift;
-        want = hist[257];
-        __syncthreads();
+        want = h[257];
+        buf ^= 1;
     }
     return prefix;
 }

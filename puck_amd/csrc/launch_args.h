@@ -41,7 +41,6 @@ struct PkWalkArgs {
                              // its flags are rewritten (not or-ed) with what this walk finds
     // split budget scan (scan.hip); sc_items null: every budget query takes the per-query kernel
     uint2* sc_items;         // nq x sc_capp: per scored point {memory id, cell distance bits}
-    uint64_t* sc_keys;       // nq x sc_capp (filter distance ord << 32 | memory id), same order
     uint32_t* sc_meta;       // nq x 2: points (0xffffffff: left to the per-query kernel), points
     uint32_t* sc_olist;      // nq: the queries left to the per-query kernel
     uint32_t* sc_ocount;     // 1

@@ -1,30 +1,29 @@
 // scan.hip -- the budget scan (a query's fine budget b < F; no counterpart in
-// the reference) as three data-parallel kernels:
+// the reference) as two data-parallel kernels:
 //
-//   pk_scan_cells   one workgroup per query: in each walked coarse list, the cut
-//                   at the b-th smallest cell distance (cd + stationary) + fine
-//                   distance (the reference's cell distance expression,
-//                   puck_index.cpp:535), and one work item per point of the
-//                   selected cells {memory id, cell distance}
-//   pk_scan_score   one workgroup per query: the query's filter lookup table
-//                   in LDS (quantization.cpp:260-270), then every point of every
-//                   item gets the reference's filter distance
-//                   (compute_quantized_distance, puck_index.cpp:346-390: t =
-//                   fl(2 * cell_dist + offset) + lookup sum in the 4-lane order of
-//                   puck_index.cpp:75-113), written as a (distance, memory id)
-//                   key in the item's slot (consecutive threads take
-//                   consecutive points: the record loads coalesce); no block
-//                   barrier after the table is built
-//   pk_scan_cut     one workgroup per query: the filter_topk smallest keys
-//                   (radix select on the distance bits, then on the ids of the
-//                   keys tied at the cut) into the filter rows
+//   pk_scan_cells      one workgroup per query: in each walked coarse list, the
+//                      cut at the b-th smallest cell distance (cd + stationary)
+//                      + fine distance (the reference's cell distance
+//                      expression, puck_index.cpp:535), and one work item per
+//                      point of the selected cells {memory id, cell distance}
+//   pk_scan_score_cut  one workgroup per query: the query's filter lookup table
+//                      in LDS (quantization.cpp:260-270), then every point of
+//                      every item gets the reference's filter distance
+//                      (compute_quantized_distance, puck_index.cpp:346-390:
+//                      t = fl(2 * cell_dist + offset) + lookup sum in the 4-lane
+//                      order of puck_index.cpp:75-113) as a (distance, memory
+//                      id) key kept in registers (consecutive threads take
+//                      consecutive points: the record loads coalesce); then the
+//                      filter_topk smallest keys (radix select on the distance
+//                      bits, then on the ids of the keys tied at the cut) into
+//                      the filter rows
 //
 // The result is the one of the per-query scan (pk_scan_budget in search.hip):
 // the filter set is the filter_topk smallest (distance, memory id) keys over
 // every point of the selected cells, which does not depend on the order in
-// which points are scored.  A query whose candidates exceed the item or key
-// buffers, or whose index has F > PK_SC_CPT * PK_SC_CELL_THREADS, is marked
-// and left to the per-query kernel (pk_scan_overflow_list builds its list).
+// which points are scored.  A query whose candidates exceed the item buffer,
+// or whose index has F > PK_SC_CPT * PK_SC_CELL_THREADS, is marked and left to
+// the per-query kernel (pk_scan_overflow_list builds its list).
 #include "common.h"
 #include "launch_args.h"
 
@@ -40,7 +39,7 @@ __device__ __forceinline__ bool pk_sc_budget_query(const PkWalkArgs& a, uint32_t
 #endif
 #define PK_SC_CPT (2048 / PK_SC_CELL_THREADS)  // columns (fine clusters) per thread: F <= 2048
 __global__ __launch_bounds__(PK_SC_CELL_THREADS) void pk_scan_cells(PkWalkArgs a) {
-    __shared__ uint32_t hist[260];
+    __shared__ uint32_t hist[PK_SEL_HIST];
     __shared__ uint32_t tmp[PK_SC_CELL_THREADS / 64 + 1];
     __shared__ uint32_t red[2 * PK_SC_CELL_THREADS / 64];
     // the list's selected non-empty cells: first memory id, first item, cell distance bits
@@ -161,16 +160,17 @@ __global__ __launch_bounds__(PK_SC_CELL_THREADS) void pk_scan_cells(PkWalkArgs a
 __device__ __forceinline__ float pk_sc_lv(float v) { return v; }
 __device__ __forceinline__ float pk_sc_lv(__half v) { return __half2float(v); }
 
+// Filter distances of PP points per thread, returned as keys in registers:
+// kout[h] is the key (distance bits, memory id) of item i0 + h * NT +
+// threadIdx.x (consecutive threads, consecutive records), or ~0 past n.
 template <int NT, int NV, typename T>
-__device__ __forceinline__ void pk_sc_score_round(const PkWalkArgs& a, const T* __restrict__ lut, float inv,
-                                                  const uint2* __restrict__ items, uint32_t i0, uint32_t n,
-                                                  uint64_t* __restrict__ keys) {
+__device__ __forceinline__ void pk_sc_score_keys(const PkWalkArgs& a, const T* __restrict__ lut, float inv,
+                                                 const uint2* __restrict__ items, uint32_t i0, uint32_t n,
+                                                 uint64_t (&kout)[PK_SC_PP]) {
     const uint32_t nsq = a.nsq;
     uint2 it[PK_SC_PP];
     float off[PK_SC_PP];
     uint4 cw[PK_SC_PP][NV > 0 ? NV : 1];
-    // every load at an in-bounds index (a load under a lane condition is
-    // waited for before the next one is issued)
 #pragma unroll
     for (int h = 0; h < PK_SC_PP; ++h) {
         const uint32_t i = i0 + h * NT + threadIdx.x;
@@ -211,62 +211,41 @@ __device__ __forceinline__ void pk_sc_score_round(const PkWalkArgs& a, const T* 
         const float sum = (acc[0] + acc[1]) + (acc[2] + acc[3]);
         if constexpr (sizeof(T) == sizeof(float)) tv = tv + sum;
         else tv = tv + __fmul_rn(sum, inv);
-        if (i < n) keys[i] = ((uint64_t)pk_f2ord(tv) << 32) | P;
+        kout[h] = i < n ? (((uint64_t)pk_f2ord(tv) << 32) | P) : ~0ull;
     }
 }
 
-// LDS of pk_scan_score: the lookup table (FP32 or FP16), then the query.
+// LDS of pk_scan_score_cut: the lookup table (FP32 or FP16), then the query.
 __host__ __device__ inline size_t pk_scan_lut_bytes(uint32_t nsq, int h16) {
     return (((size_t)nsq * PK_KS * (h16 ? 2u : 4u)) + 15u) & ~(size_t)15u;
 }
 
-template <int NT, int NV, bool H16>
-__global__ __launch_bounds__(NT, H16 ? 8 : 6) void pk_scan_score(PkWalkArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
-    if (!pk_sc_budget_query(a, qi)) return;
-    const uint32_t n = a.sc_meta[2ull * qi];
-    if (n == PK_SC_OVERFLOW || n == 0) return;  // block-uniform
-    float* qv = (float*)(smem + pk_scan_lut_bytes(a.nsq, H16));
-    const float* qg = a.q + (uint64_t)qi * a.dim;
-    for (uint32_t i = threadIdx.x; i < a.dim; i += NT) qv[i] = qg[i];
-    __syncthreads();
-    const uint2* items = a.sc_items + (uint64_t)qi * a.sc_capp;
-    uint64_t* keys = a.sc_keys + (uint64_t)qi * a.sc_capp;
-    if constexpr (H16) {
-        __half* lut = (__half*)smem;
-        const int e = pk_lut16_exp(a.qn[qi], a.cb_nmax);
-        const float sc = ldexpf(1.0f, e), inv = ldexpf(1.0f, -e);
-        pk_build_lut_put(qv, a.codebook, a.nsq, a.lsq,
-                         [&](uint32_t i, float v) { lut[i] = __float2half_rn(__fmul_rn(v, sc)); });
-        __syncthreads();
-        for (uint32_t i0 = 0; i0 < n; i0 += PK_SC_PP * NT) pk_sc_score_round<NT, NV>(a, lut, inv, items, i0, n, keys);
-    } else {
-        float* lut = (float*)smem;
-        pk_build_lut(lut, qv, a.codebook, a.nsq, a.lsq);
-        __syncthreads();
-        for (uint32_t i0 = 0; i0 < n; i0 += PK_SC_PP * NT)
-            pk_sc_score_round<NT, NV>(a, lut, 1.0f, items, i0, n, keys);
-    }
-}
-
 // ---------------------------------------------------------------------------
-#ifndef PK_SC_CUT_THREADS
-#define PK_SC_CUT_THREADS 256
-#endif
-#define PK_SC_CUT_KPT (8192 / PK_SC_CUT_THREADS)  // keys per thread: sc_capp <= 8192
+// Scoring and filter cut of one query in one workgroup: the keys of the
+// query's items stay in registers (PK_SC_KR rounds of PK_SC_PP per thread,
+// sc_capp <= PK_SC_PP * PK_SC_KR * NT), then the filter_topk smallest
+// (distance bits, memory id) are selected (radix select on the distance
+// bits, then on the ids of the keys tied at the cut) and written to the
+// filter rows.
+template <int NT>
+struct PkScCut {
+    static constexpr int KR = 8192 / (PK_SC_PP * NT);  // rounds
+    static constexpr int KPT = KR * PK_SC_PP;         // keys per thread
+};
 
-__global__ __launch_bounds__(PK_SC_CUT_THREADS) void pk_scan_cut(PkWalkArgs a) {
-    __shared__ uint32_t hist[260];
-    __shared__ uint32_t tmp[PK_SC_CUT_THREADS / 64 + 1];
-    __shared__ uint32_t red[2 * PK_SC_CUT_THREADS / 64];
+template <int NT, int NV, bool H16>
+__global__ __launch_bounds__(NT) void pk_scan_score_cut(PkWalkArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ uint32_t hist[PK_SEL_HIST];
+    __shared__ uint32_t tmp[NT / 64 + 1];
+    __shared__ uint32_t red[2 * NT / 64];
     __shared__ uint32_t cnt[2];
+    constexpr int KR = PkScCut<NT>::KR, KPT = PkScCut<NT>::KPT;
     const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
     const uint32_t tid = threadIdx.x;
     if (!pk_sc_budget_query(a, qi)) return;
     const uint32_t nit = a.sc_meta[2ull * qi];  // points, or the overflow mark
     if (nit == PK_SC_OVERFLOW) return;  // the per-query kernel's query
-    const uint32_t ftk = a.ftk;
     if (tid == 0) {
         if (a.flags) a.flags[qi] = 0;  // deterministic: nothing to replay
         if (a.tie_n) a.tie_n[qi] = 0;
@@ -275,35 +254,63 @@ __global__ __launch_bounds__(PK_SC_CUT_THREADS) void pk_scan_cut(PkWalkArgs a) {
         if (tid == 0) a.out_n[qi] = 0;
         return;
     }
-    const uint32_t n = a.sc_meta[2ull * qi + 1];
-    const uint64_t* keys = a.sc_keys + (uint64_t)qi * a.sc_capp;
-    // thread t holds keys t, t + NT, ... (coalesced loads, in bounds)
-    uint64_t x[PK_SC_CUT_KPT];
+    const uint32_t n = nit;
+    float* qv = (float*)(smem + pk_scan_lut_bytes(a.nsq, H16));
+    const float* qg = a.q + (uint64_t)qi * a.dim;
+    for (uint32_t i = tid; i < a.dim; i += NT) qv[i] = qg[i];
+    __syncthreads();
+    const uint2* items = a.sc_items + (uint64_t)qi * a.sc_capp;
+    uint64_t x[KPT];
+    if constexpr (H16) {
+        __half* lut = (__half*)smem;
+        const int e = pk_lut16_exp(a.qn[qi], a.cb_nmax);
+        const float sc = ldexpf(1.0f, e), inv = ldexpf(1.0f, -e);
+        pk_build_lut_put(qv, a.codebook, a.nsq, a.lsq,
+                         [&](uint32_t i, float v) { lut[i] = __float2half_rn(__fmul_rn(v, sc)); });
+        __syncthreads();
 #pragma unroll
-    for (int j = 0; j < PK_SC_CUT_KPT; ++j) {
-        // at an in-bounds index for every lane (conditional loads are waited
-        // for one at a time); keys past n are masked below
-        const uint32_t i = j * PK_SC_CUT_THREADS + tid;
-        x[j] = keys[i < n ? i : n - 1u];
+        for (int r = 0; r < KR; ++r) {
+            uint64_t k2[PK_SC_PP];
+            if (r * PK_SC_PP * NT < n) pk_sc_score_keys<NT, NV>(a, lut, inv, items, r * PK_SC_PP * NT, n, k2);
+            else
+#pragma unroll
+                for (int h = 0; h < PK_SC_PP; ++h) k2[h] = ~0ull;
+#pragma unroll
+            for (int h = 0; h < PK_SC_PP; ++h) x[r * PK_SC_PP + h] = k2[h];
+        }
+    } else {
+        float* lut = (float*)smem;
+        pk_build_lut(lut, qv, a.codebook, a.nsq, a.lsq);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < KR; ++r) {
+            uint64_t k2[PK_SC_PP];
+            if (r * PK_SC_PP * NT < n) pk_sc_score_keys<NT, NV>(a, lut, 1.0f, items, r * PK_SC_PP * NT, n, k2);
+            else
+#pragma unroll
+                for (int h = 0; h < PK_SC_PP; ++h) k2[h] = ~0ull;
+#pragma unroll
+            for (int h = 0; h < PK_SC_PP; ++h) x[r * PK_SC_PP + h] = k2[h];
+        }
     }
-#pragma unroll
-    for (int j = 0; j < PK_SC_CUT_KPT; ++j)
-        if (j * PK_SC_CUT_THREADS + tid >= n) x[j] = ~0ull;
+    // key j of this thread is item (j / PP) * PP * NT + (j % PP) * NT + tid;
+    // keys past n are ~0 and never selected (ftk < n below)
+    auto valid = [&](int j) { return (uint32_t)((j / PK_SC_PP) * PK_SC_PP * NT + (j % PK_SC_PP) * NT) + tid < n; };
+    const uint32_t ftk = a.ftk;
     uint64_t K = ~0ull;
     if (n > ftk) {
         const uint32_t kc = pk_select_rank_each(
             [&](auto sink) {
 #pragma unroll
-                for (int j = 0; j < PK_SC_CUT_KPT; ++j)
-                    if (j * PK_SC_CUT_THREADS + tid < n) sink((uint32_t)(x[j] >> 32));
+                for (int j = 0; j < KPT; ++j)
+                    if (valid(j)) sink((uint32_t)(x[j] >> 32));
             },
             ftk, hist, tmp, red);
-        // keys strictly below the cut distance, and at it
         uint32_t lt = 0, eq = 0;
 #pragma unroll
-        for (int j = 0; j < PK_SC_CUT_KPT; ++j) {
+        for (int j = 0; j < KPT; ++j) {
             const uint32_t hv = (uint32_t)(x[j] >> 32);
-            const bool v = j * PK_SC_CUT_THREADS + tid < n;
+            const bool v = valid(j);
             lt += (v && hv < kc) ? 1u : 0u;
             eq += (v && hv == kc) ? 1u : 0u;
         }
@@ -316,12 +323,11 @@ __global__ __launch_bounds__(PK_SC_CUT_THREADS) void pk_scan_cut(PkWalkArgs a) {
         if (nlt + neq == ftk) {
             K = ((uint64_t)kc << 32) | 0xffffffffull;
         } else {
-            // the cut falls inside the keys tied at kc: rank ftk - nlt of their ids
             const uint32_t idc = pk_select_rank_each(
                 [&](auto sink) {
 #pragma unroll
-                    for (int j = 0; j < PK_SC_CUT_KPT; ++j)
-                        if (j * PK_SC_CUT_THREADS + tid < n && (uint32_t)(x[j] >> 32) == kc) sink((uint32_t)x[j]);
+                    for (int j = 0; j < KPT; ++j)
+                        if (valid(j) && (uint32_t)(x[j] >> 32) == kc) sink((uint32_t)x[j]);
                 },
                 ftk - nlt, hist, tmp, red);
             K = ((uint64_t)kc << 32) | idc;
@@ -329,14 +335,14 @@ __global__ __launch_bounds__(PK_SC_CUT_THREADS) void pk_scan_cut(PkWalkArgs a) {
     }
     uint32_t keep = 0;
 #pragma unroll
-    for (int j = 0; j < PK_SC_CUT_KPT; ++j) keep += (j * PK_SC_CUT_THREADS + tid < n && x[j] <= K) ? 1u : 0u;
+    for (int j = 0; j < KPT; ++j) keep += (valid(j) && x[j] <= K) ? 1u : 0u;
     uint32_t tot;
     uint32_t p = pk_block_exscan(keep, tmp, &tot);
     float* ot = a.out_t + (uint64_t)qi * ftk;
     uint32_t* oid = a.out_id + (uint64_t)qi * ftk;
 #pragma unroll
-    for (int j = 0; j < PK_SC_CUT_KPT; ++j) {
-        if (j * PK_SC_CUT_THREADS + tid < n && x[j] <= K) {
+    for (int j = 0; j < KPT; ++j) {
+        if (valid(j) && x[j] <= K) {
             ot[p] = pk_ord2f((uint32_t)(x[j] >> 32));
             oid[p] = (uint32_t)x[j];
             ++p;
@@ -362,7 +368,7 @@ __global__ __launch_bounds__(1024) void pk_scan_overflow_list(PkWalkArgs a, uint
     if (threadIdx.x == 0) *a.sc_ocount = base;
 }
 
-// LDS of pk_scan_score: the lookup table and the query.
+// LDS of pk_scan_score_cut: the lookup table and the query.
 extern "C" size_t pk_scan_score_smem(uint32_t dim, uint32_t nsq, int h16) {
     return pk_scan_lut_bytes(nsq, h16) + sizeof(float) * ((dim + 3) & ~3u);
 }
@@ -380,9 +386,9 @@ extern "C" int pk_scan_split_ok(uint32_t dim, uint32_t nsq, uint32_t F) {
 // scoring and the filter cut (post).  The per-query kernel needs only the
 // list, so a caller can run it on another stream between the two parts.
 static hipError_t pk_scan_split_check(const PkWalkArgs& a) {
-    if (!a.q_nf || !a.fdist || !a.sc_items || !a.sc_keys || !a.sc_meta || !a.sc_olist || !a.sc_ocount)
+    if (!a.q_nf || !a.fdist || !a.sc_items || !a.sc_meta || !a.sc_olist || !a.sc_ocount)
         return hipErrorInvalidValue;
-    if (a.sc_capp > PK_SC_CUT_KPT * PK_SC_CUT_THREADS || a.sc_capp < a.ftk) return hipErrorInvalidValue;
+    if (a.sc_capp > 8192u || a.sc_capp < a.ftk) return hipErrorInvalidValue;
     if (pk_scan_score_smem(a.dim, a.nsq, 0) > 64 * 1024) return hipErrorInvalidValue;
     return hipSuccess;
 }
@@ -401,26 +407,26 @@ extern "C" hipError_t pk_launch_scan_split_post(const PkWalkArgs* args, uint32_t
     const PkWalkArgs& a = *args;
     const hipError_t e = pk_scan_split_check(a);
     if (e != hipSuccess) return e;
+    constexpr int NT = 512;
+    if (a.sc_capp > (uint32_t)(PkScCut<NT>::KPT * NT)) return hipErrorInvalidValue;
     const size_t smem = pk_scan_score_smem(a.dim, a.nsq, a.lut16);
     const uint32_t nv = (a.nsq & 15u) == 0 && a.nsq <= 64 ? a.nsq / 16u : 0u;
-    constexpr int NT = 512;
     if (a.lut16) {
         switch (nv) {
-        case 1: hipLaunchKernelGGL((pk_scan_score<NT, 1, true>), dim3(nq), dim3(NT), smem, s, a); break;
-        case 2: hipLaunchKernelGGL((pk_scan_score<NT, 2, true>), dim3(nq), dim3(NT), smem, s, a); break;
-        case 3: hipLaunchKernelGGL((pk_scan_score<NT, 3, true>), dim3(nq), dim3(NT), smem, s, a); break;
-        case 4: hipLaunchKernelGGL((pk_scan_score<NT, 4, true>), dim3(nq), dim3(NT), smem, s, a); break;
-        default: hipLaunchKernelGGL((pk_scan_score<NT, 0, true>), dim3(nq), dim3(NT), smem, s, a); break;
+        case 1: hipLaunchKernelGGL((pk_scan_score_cut<NT, 1, true>), dim3(nq), dim3(NT), smem, s, a); break;
+        case 2: hipLaunchKernelGGL((pk_scan_score_cut<NT, 2, true>), dim3(nq), dim3(NT), smem, s, a); break;
+        case 3: hipLaunchKernelGGL((pk_scan_score_cut<NT, 3, true>), dim3(nq), dim3(NT), smem, s, a); break;
+        case 4: hipLaunchKernelGGL((pk_scan_score_cut<NT, 4, true>), dim3(nq), dim3(NT), smem, s, a); break;
+        default: hipLaunchKernelGGL((pk_scan_score_cut<NT, 0, true>), dim3(nq), dim3(NT), smem, s, a); break;
         }
     } else {
         switch (nv) {
-        case 1: hipLaunchKernelGGL((pk_scan_score<NT, 1, false>), dim3(nq), dim3(NT), smem, s, a); break;
-        case 2: hipLaunchKernelGGL((pk_scan_score<NT, 2, false>), dim3(nq), dim3(NT), smem, s, a); break;
-        case 3: hipLaunchKernelGGL((pk_scan_score<NT, 3, false>), dim3(nq), dim3(NT), smem, s, a); break;
-        case 4: hipLaunchKernelGGL((pk_scan_score<NT, 4, false>), dim3(nq), dim3(NT), smem, s, a); break;
-        default: hipLaunchKernelGGL((pk_scan_score<NT, 0, false>), dim3(nq), dim3(NT), smem, s, a); break;
+        case 1: hipLaunchKernelGGL((pk_scan_score_cut<NT, 1, false>), dim3(nq), dim3(NT), smem, s, a); break;
+        case 2: hipLaunchKernelGGL((pk_scan_score_cut<NT, 2, false>), dim3(nq), dim3(NT), smem, s, a); break;
+        case 3: hipLaunchKernelGGL((pk_scan_score_cut<NT, 3, false>), dim3(nq), dim3(NT), smem, s, a); break;
+        case 4: hipLaunchKernelGGL((pk_scan_score_cut<NT, 4, false>), dim3(nq), dim3(NT), smem, s, a); break;
+        default: hipLaunchKernelGGL((pk_scan_score_cut<NT, 0, false>), dim3(nq), dim3(NT), smem, s, a); break;
         }
     }
-    hipLaunchKernelGGL(pk_scan_cut, dim3(nq), dim3(PK_SC_CUT_THREADS), 0, s, a);
     return hipGetLastError();
 }

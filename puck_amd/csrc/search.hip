@@ -1458,7 +1458,7 @@ __device__ void pk_scan_budget_one(const PkWalkArgs& a, uint32_t qi, unsigned ch
     s.c_b = (uint32_t*)ptr; ptr += sizeof(uint32_t) * SCAN_CB;
     s.c_pre = (uint32_t*)ptr; ptr += sizeof(uint32_t) * (SCAN_CB + 4);
     s.c_fd = (float*)ptr; ptr += sizeof(float) * SCAN_CB;
-    uint32_t* hist = (uint32_t*)ptr; ptr += sizeof(uint32_t) * 260;
+    uint32_t* hist = (uint32_t*)ptr; ptr += sizeof(uint32_t) * PK_SEL_HIST;
     ptr = (unsigned char*)(((uintptr_t)ptr + 15) & ~(uintptr_t)15);
     uint64_t* B = (uint64_t*)ptr;
     (void)ftk;
@@ -1500,7 +1500,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void pk_scan_budget_list(PkWalkArgs a
 extern "C" size_t pk_scan_smem_fixed(uint32_t dim, uint32_t nsq) {
     size_t b = sizeof(float) * nsq * PK_KS + sizeof(float) * ((dim + 3) & ~3u) + sizeof(uint32_t) * SCAN_RED +
                sizeof(uint32_t) * SCAN_CB + sizeof(uint32_t) * (SCAN_CB + 4) + sizeof(float) * SCAN_CB +
-               sizeof(uint32_t) * 260;
+               sizeof(uint32_t) * PK_SEL_HIST;
     return (b + 15) & ~(size_t)15;
 }
 
@@ -1647,7 +1647,7 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
 // when more than PK_PREFIX_CAP entries share or undercut that high word.
 #define PK_PREFIX_CAP 256
 __device__ bool pk_sorted_prefix(uint64_t* keys, uint32_t n, uint32_t K1) {
-    __shared__ uint32_t hist[258];
+    __shared__ uint32_t hist[PK_SEL_HIST];
     __shared__ uint32_t tmp[8];
     __shared__ uint32_t red[9];
     __shared__ uint64_t sb[PK_PREFIX_CAP];
