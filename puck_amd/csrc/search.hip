@@ -1760,6 +1760,68 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
             }
             if (!exact) keys[r] = key;
         }
+    } else if (sizeof(T) == 1 && (a.dim & 15u) == 0) {
+        // uint8 rows of a multiple of 16 bytes: lane j of a row's 8-lane group
+        // loads whole 16-element blocks (one 16-byte load) and forms the
+        // block's ((l0+l1)+(l2+l3))+((l4+l5)+l6) and l7 with
+        // l_i = fma(d_i, d_i, d_{8+i}^2); the blocks are then chained in order,
+        // t = S + (l7 + t) -- the arithmetic of the float rows' loop below
+        constexpr int RP = RERANK_RP;
+        const uint32_t g = threadIdx.x >> 3, j = threadIdx.x & 7, ng = blockDim.x >> 3;
+        const int src = (int)(threadIdx.x & 63u & ~7u);
+        const uint32_t dim = a.dim, nfull = dim >> 4;
+        for (uint32_t r0 = 0; r0 < n2; r0 += RP * ng) {
+            uint32_t rr[RP];
+            const uint8_t* x[RP];
+            float t[RP];
+#pragma unroll
+            for (int h = 0; h < RP; ++h) {
+                rr[h] = r0 + h * ng + g;
+                const uint32_t fid = a.filt_id[(uint64_t)qi * ftk + (rr[h] < ftk ? rr[h] : ftk - 1u)];
+                x[h] = (const uint8_t*)raw + (uint64_t)(rr[h] < n ? fid : 0u) * dim;
+                t[h] = 0.0f;
+            }
+            for (uint32_t k0 = 0; k0 < nfull; k0 += 8) {
+                const uint32_t sb = k0 + j < nfull ? k0 + j : k0;  // in-bounds block for every lane
+                uint4 w[RP];
+#pragma unroll
+                for (int h = 0; h < RP; ++h) w[h] = *(const uint4*)(x[h] + 16 * sb);
+                float S[RP], L7[RP];
+#pragma unroll
+                for (int h = 0; h < RP; ++h) {
+                    const uint32_t ws[4] = {w[h].x, w[h].y, w[h].z, w[h].w};
+                    float d[16];
+#pragma unroll
+                    for (int i = 0; i < 16; ++i)
+                        d[i] = (float)((ws[i >> 2] >> (8 * (i & 3))) & 0xffu) - qv[16 * sb + i];
+                    float l[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) l[i] = fmaf(d[i], d[i], d[8 + i] * d[8 + i]);
+                    S[h] = ((l[0] + l[1]) + (l[2] + l[3])) + ((l[4] + l[5]) + l[6]);
+                    L7[h] = l[7];
+                }
+                const uint32_t nb = nfull - k0 < 8 ? nfull - k0 : 8;
+                for (uint32_t jj = 0; jj < nb; ++jj) {
+#pragma unroll
+                    for (int h = 0; h < RP; ++h) {
+                        const float sj = __shfl(S[h], src + (int)jj, 64), lj = __shfl(L7[h], src + (int)jj, 64);
+                        t[h] = sj + (lj + t[h]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < RP; ++h) {
+                const uint32_t r = rr[h];
+                if (j == 0 && r < n2) {
+                    uint64_t key = ~0ull;
+                    if (r < n) {
+                        key = ((uint64_t)pk_f2ord(t[h]) << 32) | r;
+                        if (exact || bud) dist[r] = t[h];
+                    }
+                    if (!exact) keys[r] = key;
+                }
+            }
+        }
     } else {
         constexpr int RP = RERANK_RP, SB = 8;  // rows per group, 16-element steps per block
         const uint32_t g = threadIdx.x >> 3, j = threadIdx.x & 7, ng = blockDim.x >> 3;
