@@ -380,6 +380,17 @@ int puckgpu_assign_cells(puckgpu_ctx* ctx, const float* coarse, const float* fin
  * walked again (test of the fallback path).  Counters: points assigned and
  * points walked sequentially since the context was created. */
 int puckgpu_set_assign_mode(puckgpu_ctx* ctx, int mode);
+/* Exact brute-force top-k of nq queries [nq x dim] float32 in n points
+ * [n x dim] (float32, or uint8 integer features with x_u8 = 1): ids_out
+ * [nq x k] and dist_out [nq x k] (float64 squared L2), ordered by (distance,
+ * id).  Candidates come from the engine's exact FP32 GEMM (|x|^2/2 - <q,x>)
+ * and row top-k, k + extra per 65,536-point chunk, merged to the 2 (k + extra)
+ * smallest, then ordered in float64; 2 (k + extra) <= 1024.  The ground
+ * truth of recall@k as ann-benchmarks defines it (metrics.py:15-40) and the
+ * labels of the learned probe; no counterpart in the reference's search. */
+int puckgpu_exact_search(puckgpu_ctx* ctx, const void* x, int x_u8, uint64_t n, uint32_t dim, const float* queries,
+                         uint32_t nq, uint32_t k, uint32_t extra, uint32_t* ids_out, double* dist_out,
+                         int device_ptrs);
 /* Progress lines of long calls (puckgpu_train_centers: one per k-means run),
  * passed to fn(message, user) on the calling thread; fn = NULL stops them.
  * No counterpart in the reference, which logs through glog. */

@@ -22,6 +22,7 @@ EXPORTS = [
     "puckgpu_center_distances", "puckgpu_index_attach_pq", "puckgpu_encode_pq", "puckgpu_search_noalloc",
     "puckgpu_index_insert", "puckgpu_index_version", "puckgpu_kmeans", "puckgpu_train_centers",
     "puckgpu_set_assign_mode", "puckgpu_assign_stats", "puckgpu_index_attach_raw_u8", "puckgpu_set_progress",
+    "puckgpu_exact_search",
 ]
 
 # void (*)(const char* message, void* user)
@@ -109,6 +110,8 @@ def load():
     L.puckgpu_assign_cells.restype = c_int
     L.puckgpu_index_attach_raw_u8.argtypes = [vp, vp, vp, c_int]
     L.puckgpu_index_attach_raw_u8.restype = c_int
+    L.puckgpu_exact_search.argtypes = [vp, vp, c_int, ctypes.c_uint64, u32, vp, u32, u32, u32, vp, vp, c_int]
+    L.puckgpu_exact_search.restype = c_int
     L.puckgpu_set_progress.argtypes = [vp, PROGRESS_FN, vp]
     L.puckgpu_set_progress.restype = c_int
     L.puckgpu_set_assign_mode.argtypes = [vp, c_int]

@@ -69,6 +69,11 @@ hipError_t pk_launch_assign_cells(const float*, const float*, const float*, cons
                                   const float*, uint32_t, uint32_t, uint32_t, int, const uint32_t*, uint32_t*,
                                   float*, hipStream_t);
 hipError_t pk_launch_st_blocks(const float*, uint32_t, uint32_t, float*, float*, uint32_t*, hipStream_t);
+hipError_t pk_launch_u8_to_f32(const uint8_t*, uint64_t, float*, hipStream_t);
+hipError_t pk_launch_exact_append(const float*, const uint32_t*, uint32_t, uint32_t, uint32_t, float*, uint32_t*,
+                                  uint32_t, uint32_t, hipStream_t);
+hipError_t pk_launch_exact_recheck(const void*, int, uint32_t, const float*, uint32_t, const uint32_t*,
+                                   const uint32_t*, uint32_t, uint32_t, uint32_t, uint32_t*, double*, hipStream_t);
 hipError_t pk_launch_assign_bounded(const float*, const float*, const float*, const float*, const float*, const float*,
                                     const float*, const float*, const float*, const uint32_t*, uint32_t, uint32_t,
                                     uint32_t, int, int, uint32_t*, float*, uint32_t*, uint32_t*, hipStream_t);
@@ -273,7 +278,8 @@ enum Slot {
     S_SCM, S_INS_X, S_INS_Y, S_INS_CELL, S_INS_REC, S_INS_PQ, S_INS_RANK, S_INS_ADD, S_INS_SHIFT, S_INS_WORK,
     S_INS_DOLD, S_INS_DNEW, S_INS_ACC, S_KM_IP, S_KM_PN, S_KM_CN, S_KM_ASSIGN, S_KM_DIS, S_KM_PERM, S_KM_STARTS,
     S_KM_HIST, S_KM_OFF, S_KM_WORK, S_KM_BEST, S_KM_IDS, S_KM_X, S_KM_TV, S_KM_C0, S_KM_F0, S_KM_CB, S_KM_FB,
-    S_KM_FA, S_KM_CENT, S_AB_STMIN, S_AB_ROW, S_AB_ABS, S_AB_FBPM, S_AB_LIST, S_AB_CNT, S_COUNT
+    S_KM_FA, S_KM_CENT, S_AB_STMIN, S_AB_ROW, S_AB_ABS, S_AB_FBPM, S_AB_LIST, S_AB_CNT, S_EX_X, S_EX_Q, S_EX_D,
+    S_EX_XF, S_EX_XN, S_EX_TV, S_EX_TI, S_EX_CV, S_EX_CI, S_EX_MV, S_EX_MP, S_EX_OI, S_EX_OD, S_COUNT
 };
 
 // Device view of an input array: the pointer itself, or a scratch copy of it.
@@ -1954,6 +1960,74 @@ int puckgpu_lut(puckgpu_ctx* ctx, uint32_t n, uint32_t dim, const float* queries
 }
 
 // ---------------------------------------------------------------------------
+int puckgpu_exact_search(puckgpu_ctx* ctx, const void* x, int x_u8, uint64_t n, uint32_t dim, const float* queries,
+                         uint32_t nq, uint32_t k, uint32_t extra, uint32_t* ids_out, double* dist_out,
+                         int device_ptrs) {
+    if (!ctx || (n && !x) || (nq && (!queries || !ids_out || !dist_out))) return set_err(ctx, PUCKGPU_EINVAL, "null argument");
+    if (dim == 0 || k == 0) return set_err(ctx, PUCKGPU_EINVAL, "dim and k must be positive");
+    if (n >= 0xffffffffull) return set_err(ctx, PUCKGPU_EINVAL, "at most 2^32 - 2 points");
+    if (n < k) return set_err(ctx, PUCKGPU_EINVAL, "k = %u above the %llu points", k, (unsigned long long)n);
+    const uint32_t kk = k + extra, M = 2 * kk;
+    if (M > 1024) return set_err(ctx, PUCKGPU_EINVAL, "2 (k + extra) must be <= 1024");
+    if (nq == 0) return PUCKGPU_OK;
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
+    hipStream_t s = ctx->stream;
+    int rc = 0;
+    const size_t esz = x_u8 ? 1 : 4;
+    const void* xd = dev_in(ctx, S_EX_X, x, esz * n * dim, device_ptrs, &rc);
+    const float* qd = (const float*)dev_in(ctx, S_EX_Q, queries, 4ull * nq * dim, device_ptrs, &rc);
+    if (rc) return rc;
+    const uint32_t QC = nq < 2048 ? nq : 2048;
+    const uint32_t BC = n < 65536 ? (uint32_t)n : 65536u;
+    // candidates per query: min(kk, chunk size) of every base chunk
+    uint32_t total = 0;
+    for (uint64_t b0 = 0; b0 < n; b0 += BC) total += (uint32_t)std::min<uint64_t>(kk, std::min<uint64_t>(BC, n - b0));
+    const uint32_t Mq = M < total ? M : total;
+    float* D = (float*)scratch_slot(ctx, S_EX_D, 4ull * QC * BC, &rc);
+    float* xf = x_u8 ? (float*)scratch_slot(ctx, S_EX_XF, 4ull * BC * dim, &rc) : nullptr;
+    float* xnh = (float*)scratch_slot(ctx, S_EX_XN, 4ull * BC, &rc);
+    float* tv = (float*)scratch_slot(ctx, S_EX_TV, 4ull * QC * kk, &rc);
+    uint32_t* ti = (uint32_t*)scratch_slot(ctx, S_EX_TI, 4ull * QC * kk, &rc);
+    float* cv = (float*)scratch_slot(ctx, S_EX_CV, 4ull * QC * total, &rc);
+    uint32_t* ci = (uint32_t*)scratch_slot(ctx, S_EX_CI, 4ull * QC * total, &rc);
+    float* mv = (float*)scratch_slot(ctx, S_EX_MV, 4ull * QC * Mq, &rc);
+    uint32_t* mp = (uint32_t*)scratch_slot(ctx, S_EX_MP, 4ull * QC * Mq, &rc);
+    uint32_t* oi = device_ptrs ? ids_out : (uint32_t*)scratch_slot(ctx, S_EX_OI, 4ull * nq * k, &rc);
+    double* od = device_ptrs ? dist_out : (double*)scratch_slot(ctx, S_EX_OD, 8ull * nq * k, &rc);
+    if (rc) return rc;
+    for (uint32_t q0 = 0; q0 < nq; q0 += QC) {
+        const uint32_t m = nq - q0 < QC ? nq - q0 : QC;
+        const float* qc = qd + (uint64_t)q0 * dim;
+        uint32_t col = 0;
+        for (uint64_t b0 = 0; b0 < n; b0 += BC) {
+            const uint32_t bn = (uint32_t)std::min<uint64_t>(BC, n - b0);
+            const float* xb;
+            if (x_u8) {
+                PK_CHECK(pk_launch_u8_to_f32((const uint8_t*)xd + b0 * dim, (uint64_t)bn * dim, xf, s));
+                xb = xf;
+            } else {
+                xb = (const float*)xd + b0 * dim;
+            }
+            PK_CHECK(pk_launch_row_norms(xb, bn, dim, 1, xnh, s));
+            PK_CHECK(pk_launch_gemm_exact(qc, xb, D, m, bn, dim, bn, PK_EPI_SUB_FROM_BIAS, xnh, s));
+            const uint32_t kc = kk < bn ? kk : bn;
+            PK_CHECK(pk_launch_topk_rows(D, m, bn, bn, kc, tv, ti, nullptr, 0, s));
+            PK_CHECK(pk_launch_exact_append(tv, ti, m, kc, (uint32_t)b0, cv, ci, total, col, s));
+            col += kc;
+        }
+        PK_CHECK(pk_launch_topk_rows(cv, m, total, total, Mq, mv, mp, nullptr, 0, s));
+        PK_CHECK(pk_launch_exact_recheck(xd, x_u8, dim, qc, m, mp, ci, total, Mq, k, oi + (uint64_t)q0 * k,
+                                         od + (uint64_t)q0 * k, s));
+        report(ctx, "exact_search: %u of %u queries", q0 + m, nq);
+    }
+    if (!device_ptrs) {
+        PK_CHECK(hipMemcpyAsync(ids_out, oi, 4ull * nq * k, hipMemcpyDeviceToHost, s));
+        PK_CHECK(hipMemcpyAsync(dist_out, od, 8ull * nq * k, hipMemcpyDeviceToHost, s));
+    }
+    PK_CHECK(hipStreamSynchronize(s));
+    return PUCKGPU_OK;
+}
+
 int puckgpu_set_progress(puckgpu_ctx* ctx, puckgpu_progress_fn fn, void* user) {
     if (!ctx) return PUCKGPU_EINVAL;
     ctx->progress = fn;

@@ -145,6 +145,28 @@ class Context:
                                                _ptr(cell), _ptr(dist), m))
         return (cell, dist) if with_dist else cell
 
+    def exact_search(self, x, q, k, extra=8):
+        """Exact squared-L2 top-k of q [nq, d] float32 in x [N, d] (float32, or
+        uint8 integer features), ordered by (distance, id): (ids uint32
+        [nq, k], dists float64 [nq, k]); torch CUDA tensors or numpy arrays."""
+        N, d = int(x.shape[0]), int(x.shape[1])
+        nq = int(q.shape[0])
+        u8 = _dtype_name(x) == "uint8"
+        _check(x, "x", _U8 if u8 else _F32, device=self.device)
+        _check(q, "q", _F32, shape=(nq, d), device=self.device)
+        if _is_dev(q):
+            import torch
+            ids = torch.empty((nq, k), dtype=torch.int32, device=q.device)
+            dist = torch.empty((nq, k), dtype=torch.float64, device=q.device)
+        else:
+            ids = np.zeros((nq, k), np.uint32)
+            dist = np.zeros((nq, k), np.float64)
+        m = _mode(x, q, ids)
+        self.sync_torch(x, q)
+        self.check(self.L.puckgpu_exact_search(self.h, _ptr(x), int(u8), N, d, _ptr(q), nq, int(k), int(extra),
+                                               _ptr(ids), _ptr(dist), m))
+        return ids, dist
+
     def set_progress(self, fn):
         """fn(message: str) receives the library's progress lines (None: off)."""
         if fn is None:

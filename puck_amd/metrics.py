@@ -1,38 +1,31 @@
-"""Measurement helpers: exact brute-force top-k on the GPU and recall with
-distance ties (ann-benchmarks benchmark/plotting/metrics.py:15-40)."""
+"""Measurement helpers: exact brute-force top-k on the GPU (the engine's own
+exact search) and recall with distance ties (ann-benchmarks
+benchmark/plotting/metrics.py:15-40)."""
 import numpy as np
 import torch
 
 
+_CTX = {}
+
+
+def _context(device):
+    from .engine import Context
+    dv = torch.device(device).index or 0
+    if dv not in _CTX:
+        _CTX[dv] = Context(dv)
+    return _CTX[dv]
+
+
 @torch.no_grad()
-def exact_topk(x, q, k, extra=8, qchunk=4096, bchunk=1 << 21):
-    """Exact squared-L2 top-(k+extra) of q [nq, d] in x [N, d] (CUDA tensors):
-    FP32 GEMM candidates per base block, then an FP64 recheck that decides the
-    order.  Returns (ids int64 [nq, k+extra], dists float64 [nq, k+extra])."""
-    kk = k + extra
-    nq, N = q.shape[0], x.shape[0]
-    # x may be uint8 (integer features): blocks are converted to float32
-    xn = (x.double() ** 2).sum(1).float() if N <= (1 << 20) else torch.cat(
-        [(x[s:s + bchunk].double() ** 2).sum(1).float() for s in range(0, N, bchunk)])
-    out_i = torch.empty((nq, kk), dtype=torch.int64, device=q.device)
-    out_d = torch.empty((nq, kk), dtype=torch.float64, device=q.device)
-    for qs in range(0, nq, qchunk):
-        qq = q[qs:qs + qchunk]
-        cand_i, cand_d = [], []
-        for bs in range(0, N, bchunk):
-            xb = x[bs:bs + bchunk].float()
-            d = torch.addmm(xn[bs:bs + bchunk][None, :], qq, xb.T, beta=1.0, alpha=-2.0)
-            v, i = torch.topk(d, min(kk, xb.shape[0]), dim=1, largest=False)
-            cand_i.append(i + bs)
-            cand_d.append(v)
-        ci = torch.cat(cand_i, 1)
-        v, j = torch.topk(torch.cat(cand_d, 1), min(2 * kk, ci.shape[1]), dim=1, largest=False)
-        ci = torch.gather(ci, 1, j)
-        exact = ((x[ci].double() - qq.double()[:, None, :]) ** 2).sum(-1)
-        order = torch.argsort(exact, dim=1, stable=True)[:, :kk]
-        out_i[qs:qs + qchunk] = torch.gather(ci, 1, order)
-        out_d[qs:qs + qchunk] = torch.gather(exact, 1, order)
-    return out_i, out_d
+def exact_topk(x, q, k, extra=8, ctx=None):
+    """Exact squared-L2 top-(k+extra) of q [nq, d] in x [N, d] (CUDA tensors,
+    x float32 or uint8) by the engine's own exact search
+    (puckgpu_exact_search: exact FP32 GEMM candidates per 65,536-point chunk,
+    merged, ordered in FP64 by (distance, id)).  Returns (ids int64
+    [nq, k+extra], dists float64 [nq, k+extra])."""
+    ctx = ctx or _context(q.device)
+    ids, dist = ctx.exact_search(x, q.contiguous(), k + extra, extra=8)
+    return ids.long(), dist
 
 
 def recall_with_ties(true_ids, true_dists, run_ids, count):
