@@ -708,12 +708,14 @@ def serving(gi, dev, d, nq_total, max_batch=4096):
                                         out=(D[:b], I[:b], R[:b])))
     eager = run(lambda s0, b: gi.search_batch(qs[s0:s0 + b], 100, 400, 1100, topk=topk[s0:s0 + b],
                                               scc=scc[s0:s0 + b], out=(D[:b], I[:b], R[:b])))
-    log(f"serving: graph {graph} eager {eager} shapes {shapes.tolist()}")
+    tuned = sv.tuned()
+    log(f"serving: graph {graph} eager {eager} tuned (gemm shape, rerank rows) {tuned}")
     sv.free()
     return {"workload": f"{nq_total} queries in {len(sizes)} batches of floor(exp(U[0, ln {max_batch}])) "
                         "queries, topk in {1,10,100}, search_coarse_count in 20..400, filter_topk 1100",
             "graph_server": graph, "eager_batches": eager, "setup_s": round(setup_s, 1),
             "autotuned_gemm_shape_per_bucket": shapes.tolist(),
+            "autotuned_rerank_rows_per_bucket": [r for _, r in tuned],
             "gemm_ms_per_bucket_and_shape": np.round(times, 4).tolist()}
 
 

@@ -316,11 +316,18 @@ int puckgpu_server_search(puckgpu_server* server, uint32_t n, const float* queri
                           uint32_t* result_num, int device_ptrs);
 /* Autotuner: for every bucket, times the two center GEMMs in each tile shape
  * on the first cap rows of sample [max_batch x dim] (host) over reps
- * repetitions and keeps the fastest shape (every shape gives the same bits);
- * graphs are re-captured.  shapes_out [buckets] and times_out [buckets x
- * shapes] (ms per repetition) may be NULL. */
+ * repetitions and keeps the fastest shape, then times the bucket's whole pass
+ * with the rerank's rows in flight per lane group set to 1, 2 and 4 and keeps
+ * the fastest (every choice gives the same bits); graphs are re-captured.
+ * shapes_out [buckets] and times_out [buckets x shapes] (ms per repetition of
+ * the GEMMs) may be NULL; puckgpu_server_tuned reports a bucket's choices. */
 int puckgpu_server_tune(puckgpu_server* server, const float* sample, uint32_t reps, int32_t* shapes_out,
                         float* times_out);
+int puckgpu_server_tuned(const puckgpu_server* server, uint32_t bucket, int32_t* gemm_shape, uint32_t* rerank_rows);
+/* Rows in flight per 8-lane group of the exact rerank in this context's
+ * non-server searches: 1, 2 (default) or 4.  A tuning choice with no
+ * counterpart in the reference; results do not depend on it. */
+int puckgpu_set_rerank_rows(puckgpu_ctx* ctx, uint32_t rows);
 
 /* Fine-budget ranks (labels of the learned probe): for query i and entry e of
  * cells [n x k] (cell id coarse * F + fine, or 0xffffffff), the smallest

@@ -22,7 +22,7 @@ EXPORTS = [
     "puckgpu_center_distances", "puckgpu_index_attach_pq", "puckgpu_encode_pq", "puckgpu_search_noalloc",
     "puckgpu_index_insert", "puckgpu_index_version", "puckgpu_kmeans", "puckgpu_train_centers",
     "puckgpu_set_assign_mode", "puckgpu_assign_stats", "puckgpu_index_attach_raw_u8", "puckgpu_set_progress",
-    "puckgpu_exact_search",
+    "puckgpu_exact_search", "puckgpu_server_tuned", "puckgpu_set_rerank_rows",
 ]
 
 # void (*)(const char* message, void* user)
@@ -110,6 +110,10 @@ def load():
     L.puckgpu_assign_cells.restype = c_int
     L.puckgpu_index_attach_raw_u8.argtypes = [vp, vp, vp, c_int]
     L.puckgpu_index_attach_raw_u8.restype = c_int
+    L.puckgpu_server_tuned.argtypes = [vp, u32, vp, vp]
+    L.puckgpu_server_tuned.restype = c_int
+    L.puckgpu_set_rerank_rows.argtypes = [vp, u32]
+    L.puckgpu_set_rerank_rows.restype = c_int
     L.puckgpu_exact_search.argtypes = [vp, vp, c_int, ctypes.c_uint64, u32, vp, u32, u32, u32, vp, vp, c_int]
     L.puckgpu_exact_search.restype = c_int
     L.puckgpu_set_progress.argtypes = [vp, PROGRESS_FN, vp]

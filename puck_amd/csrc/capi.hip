@@ -142,6 +142,7 @@ struct puckgpu_ctx {
     uint32_t last_n = 0, last_ftk = 0;
     unsigned char* noalloc = nullptr;  // this device's copy of `scratch` (puckgpu_search_noalloc)
     int assign_mode = 0;                // puckgpu_set_assign_mode
+    uint32_t rerank_rows = 2;           // puckgpu_set_rerank_rows
     uint64_t assign_points = 0, assign_sequential = 0;
     puckgpu_progress_fn progress = nullptr;  // puckgpu_set_progress
     void* progress_user = nullptr;
@@ -714,6 +715,7 @@ struct PassCfg {
     uint32_t kc = 0;  // coarse candidates rescored in FP32 (precision 1 / 2)
     double rr = 1.0;
     int gemm_shape = 0;
+    uint32_t rerank_rows = 2;  // pk_rerank rows per lane group (autotuned per server bucket)
     const ProbeUse* pu = nullptr;
     bool budget = false;  // fine budgets may be given (explicit or predicted): the split scan's buffers
 };
@@ -816,7 +818,8 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     wa.wlist = nullptr;  // replays address queries through qlist
     wa.wcount = nullptr;
     if (ev) PK_CHECK(hipEventRecord(ev[3], s));
-    PkRerankArgs ra;
+    PkRerankArgs ra{};
+    ra.rows = cfg.rerank_rows;
     ra.q = b.q; ra.qstatus = b.qst; ra.filt_t = ft; ra.filt_id = fi; ra.filt_n = fn;
     ra.raw = ix->raw; ra.raw8 = ix->raw8; ra.mem2local = ix->mem2local; ra.dim = d; ra.ftk = cfg.ftk; ra.topk = cfg.topk;
     ra.out_d = od; ra.out_id = oi; ra.out_n = on;
@@ -987,6 +990,7 @@ int make_cfg(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_par
         return set_err(ctx, PUCKGPU_EINVAL, "probe shape (dim %u, %u coarse, %u fine features) does not fit the index",
                        pr->dim, pr->nc, pr->nfine);
     cfg.scc = sp->search_coarse_count < C ? sp->search_coarse_count : C;
+    cfg.rerank_rows = ctx->rerank_rows;
     // sorted coarse entries kept per query: the walk's scc, or the probe's features
     cfg.keep = pr && pr->nc > cfg.scc ? pr->nc : cfg.scc;
     if (pr && cfg.keep == 0) cfg.keep = 1;  // the probe's cell features need the nearest coarse list
@@ -1484,6 +1488,7 @@ int puckgpu_probe_predict(puckgpu_ctx* ctx, const puckgpu_probe* p, uint32_t n, 
 struct PkBucket {
     uint32_t cap = 0;
     int shape = 0;
+    uint32_t rerank_rows = 2;
     PassBufs b;
     float* qin = nullptr;
     uint32_t* qtopk = nullptr;
@@ -1771,6 +1776,7 @@ int puckgpu_server_search(puckgpu_server* sv, uint32_t n, const float* queries, 
     for (PkBucket* c : sv->buckets)
         if (c->cap >= n) { bk = c; break; }
     cfg.gemm_shape = bk->shape;
+    cfg.rerank_rows = bk->rerank_rows;
     hipStream_t s = ctx->stream;
     const uint32_t d = sv->ix->p.dim, os = cfg.ostride;
     const hipMemcpyKind in = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
@@ -1814,6 +1820,20 @@ int puckgpu_server_search(puckgpu_server* sv, uint32_t n, const float* queries, 
     if (result_num) PK_CHECK(hipMemcpyAsync(result_num, bk->on, sizeof(uint32_t) * n, out, s));
     PK_CHECK(hipStreamSynchronize(s));
     ctx->counters[4] = (uint64_t)nrep;
+    return PUCKGPU_OK;
+}
+
+int puckgpu_server_tuned(const puckgpu_server* sv, uint32_t bucket, int32_t* gemm_shape, uint32_t* rerank_rows) {
+    if (!sv || bucket >= sv->buckets.size()) return PUCKGPU_EINVAL;
+    if (gemm_shape) *gemm_shape = sv->buckets[bucket]->shape;
+    if (rerank_rows) *rerank_rows = sv->buckets[bucket]->rerank_rows;
+    return PUCKGPU_OK;
+}
+
+int puckgpu_set_rerank_rows(puckgpu_ctx* ctx, uint32_t rows) {
+    if (!ctx) return PUCKGPU_EINVAL;
+    if (rows != 1 && rows != 2 && rows != 4) return set_err(ctx, PUCKGPU_EINVAL, "rerank rows must be 1, 2 or 4");
+    ctx->rerank_rows = rows;
     return PUCKGPU_OK;
 }
 
@@ -1867,6 +1887,26 @@ int puckgpu_server_tune(puckgpu_server* sv, const float* sample, uint32_t reps, 
         bk->shape = bshape;
         if (shapes_out) shapes_out[j] = bshape;
         cfg.gemm_shape = bshape;
+        // the rerank's rows in flight per lane group: the whole pass timed
+        float bestp = 0.f;
+        uint32_t brows = 2;
+        for (uint32_t rows : {1u, 2u, 4u}) {
+            cfg.rerank_rows = rows;
+            if ((rc = bucket_enqueue(sv, bk, cfg, s))) return rc;  // warm-up
+            PK_CHECK(hipEventRecord(ctx->ev[0], s));
+            for (uint32_t r = 0; r < reps; ++r)
+                if ((rc = bucket_enqueue(sv, bk, cfg, s))) return rc;
+            PK_CHECK(hipEventRecord(ctx->ev[1], s));
+            PK_CHECK(hipEventSynchronize(ctx->ev[1]));
+            float ms = 0.f;
+            PK_CHECK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+            if (rows == 1 || ms < bestp) {
+                bestp = ms;
+                brows = rows;
+            }
+        }
+        bk->rerank_rows = brows;
+        cfg.rerank_rows = brows;
         if ((rc = bucket_capture(sv, bk, cfg))) return rc;
     }
     return PUCKGPU_OK;

@@ -80,4 +80,5 @@ struct PkRerankArgs {
     const uint8_t* pq_codes; // N x pnsq
     const float* pq_codebook;// pnsq x 256 x plsq
     uint32_t pnsq, plsq;
+    uint32_t rows;           // rows per 8-lane group in flight: 1, 2 or 4 (another value: 2)
 };

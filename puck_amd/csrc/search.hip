@@ -1634,9 +1634,6 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
 // ---------------------------------------------------------------------------
 
 #define RERANK_THREADS 256
-#ifndef RERANK_RP
-#define RERANK_RP 2  // rows per 8-lane group and iteration of the distance loop
-#endif
 
 // Fast mode: distances sorted by (distance, filter rank).  Exact mode: the
 // result MaxHeap of rank_topN_points fed in filter-heap order, then reorder().
@@ -1706,8 +1703,10 @@ __device__ __forceinline__ float pk_pq_dist(const PkRerankArgs& a, const float* 
 }
 
 // T: the element type of the raw rows (float, or uint8_t for the uint8
-// storage form: integer features, converted exactly).
-template <typename T>
+// storage form: integer features, converted exactly).  RP: rows per 8-lane
+// group and iteration of the distance loop (the gathers in flight; a tuning
+// choice, every value gives the same bits).
+template <typename T, int RP>
 __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const T* raw = (const T*)(sizeof(T) == 1 ? (const void*)a.raw8 : (const void*)a.raw);
@@ -1766,7 +1765,6 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
         // block's ((l0+l1)+(l2+l3))+((l4+l5)+l6) and l7 with
         // l_i = fma(d_i, d_i, d_{8+i}^2); the blocks are then chained in order,
         // t = S + (l7 + t) -- the arithmetic of the float rows' loop below
-        constexpr int RP = RERANK_RP;
         const uint32_t g = threadIdx.x >> 3, j = threadIdx.x & 7, ng = blockDim.x >> 3;
         const int src = (int)(threadIdx.x & 63u & ~7u);
         const uint32_t dim = a.dim, nfull = dim >> 4;
@@ -1823,7 +1821,7 @@ __global__ __launch_bounds__(RERANK_THREADS) void pk_rerank(PkRerankArgs a) {
             }
         }
     } else {
-        constexpr int RP = RERANK_RP, SB = 8;  // rows per group, 16-element steps per block
+        constexpr int SB = 8;  // 16-element steps per block
         const uint32_t g = threadIdx.x >> 3, j = threadIdx.x & 7, ng = blockDim.x >> 3;
         const int src = (int)(threadIdx.x & 63u & ~7u);
         const uint32_t dim = a.dim, nfull = dim >> 4;
@@ -2000,18 +1998,32 @@ extern "C" hipError_t pk_launch_rerank(const PkRerankArgs* args, uint32_t nq, hi
     size_t* slot = pk_lds_slot(lds_cache);
     if (!slot) return hipErrorInvalidDevice;
     if (!*slot) {
-        size_t al = 0, al8 = 0;
-        hipError_t e = pk_allow_lds((const void*)pk_rerank<float>, 160 * 1024, &al);
-        if (e != hipSuccess) return e;
-        e = pk_allow_lds((const void*)pk_rerank<uint8_t>, 160 * 1024, &al8);
-        if (e != hipSuccess) return e;
-        *slot = al < al8 ? al : al8;
+        const void* fns[] = {(const void*)pk_rerank<float, 1>, (const void*)pk_rerank<float, 2>,
+                             (const void*)pk_rerank<float, 4>, (const void*)pk_rerank<uint8_t, 1>,
+                             (const void*)pk_rerank<uint8_t, 2>, (const void*)pk_rerank<uint8_t, 4>};
+        size_t lo = 160 * 1024;
+        for (const void* f : fns) {
+            size_t al = 0;
+            const hipError_t e = pk_allow_lds(f, 160 * 1024, &al);
+            if (e != hipSuccess) return e;
+            lo = al < lo ? al : lo;
+        }
+        *slot = lo;
     }
     if (smem > *slot) return hipErrorInvalidValue;
-    if (args->raw8 && !args->pq_codes)
-        hipLaunchKernelGGL(pk_rerank<uint8_t>, dim3(nq), dim3(RERANK_THREADS), smem, s, *args);
-    else
-        hipLaunchKernelGGL(pk_rerank<float>, dim3(nq), dim3(RERANK_THREADS), smem, s, *args);
+    const uint32_t rows = args->rows == 1 || args->rows == 4 ? args->rows : 2u;
+    const bool u8 = args->raw8 && !args->pq_codes;
+#define PK_RR_LAUNCH(T, R) hipLaunchKernelGGL((pk_rerank<T, R>), dim3(nq), dim3(RERANK_THREADS), smem, s, *args)
+    if (u8) {
+        if (rows == 1) PK_RR_LAUNCH(uint8_t, 1);
+        else if (rows == 4) PK_RR_LAUNCH(uint8_t, 4);
+        else PK_RR_LAUNCH(uint8_t, 2);
+    } else {
+        if (rows == 1) PK_RR_LAUNCH(float, 1);
+        else if (rows == 4) PK_RR_LAUNCH(float, 4);
+        else PK_RR_LAUNCH(float, 2);
+    }
+#undef PK_RR_LAUNCH
     return hipGetLastError();
 }
 

@@ -176,6 +176,10 @@ class Context:
         self._progress = _lib.PROGRESS_FN(lambda m, u: fn(m.decode()))
         self.check(self.L.puckgpu_set_progress(self.h, self._progress, None))
 
+    def set_rerank_rows(self, rows):
+        """Rows in flight per 8-lane group of the exact rerank (1, 2 or 4)."""
+        self.check(self.L.puckgpu_set_rerank_rows(self.h, int(rows)))
+
     def set_assign_mode(self, mode):
         """0: bounded assignment with certificate (default), 1: the sequential
         walk for every point, 2: bounded, then every point walked again."""
@@ -765,3 +769,12 @@ class Server:
         times = np.zeros((self.buckets, nsh), np.float32)
         self.ctx.check(self.ctx.L.puckgpu_server_tune(self.h, _ptr(sample), reps, _ptr(shapes), _ptr(times)))
         return shapes, times
+
+    def tuned(self):
+        """Per bucket: (GEMM tile shape, rerank rows per lane group) chosen."""
+        out = []
+        for j in range(self.buckets):
+            sh, rows = ctypes.c_int32(0), ctypes.c_uint32(0)
+            self.ctx.check(self.ctx.L.puckgpu_server_tuned(self.h, j, ctypes.byref(sh), ctypes.byref(rows)))
+            out.append((sh.value, rows.value))
+        return out
