@@ -29,7 +29,7 @@ hipError_t pk_launch_sort_rows_list(const float*, uint32_t, uint32_t, uint64_t, 
 hipError_t pk_launch_topk_rows(const float*, uint32_t, uint32_t, uint64_t, uint32_t, float*, uint32_t*, uint8_t*,
                                uint32_t, hipStream_t);
 hipError_t pk_launch_cell_topk(const float*, const uint32_t*, uint32_t, const float*, const float*, uint32_t, uint32_t,
-                               uint32_t, float*, hipStream_t);
+                               uint32_t, float*, uint32_t*, hipStream_t);
 hipError_t pk_launch_exact_center_order(const float*, uint32_t, const float*, uint32_t, uint32_t, uint32_t,
                                         const uint32_t*, const uint32_t*, uint32_t, const uint8_t*, int, float*,
                                         uint32_t*, float*, uint32_t*, hipStream_t);
@@ -275,7 +275,7 @@ void* scratch_slot(puckgpu_ctx* ctx, size_t slot, size_t bytes, int* rc) {
 enum Slot {
     S_Q = 0, S_QN, S_QST, S_CD, S_FD, S_CSD, S_CSI, S_FSD, S_FSI, S_FT, S_FI, S_FN, S_OD, S_OI, S_ON, S_STATS,
     S_IN_Q, S_A0, S_A1, S_A2, S_A3, S_A4, S_A5, S_A6, S_A7, S_FLAGS, S_QLIST, S_TIEID, S_TIEN, S_QTOPK, S_QSCC,
-    S_QNF, S_PFEAT, S_PWORK, S_PNF, S_PY, S_PT, S_PCV, S_LP16, S_LP8, S_LPS, S_CAND_D, S_CAND_I, S_WLIST, S_SCI,
+    S_QNF, S_PFEAT, S_PWORK, S_PNF, S_PY, S_PT, S_PCV, S_PCID, S_LP16, S_LP8, S_LPS, S_CAND_D, S_CAND_I, S_WLIST, S_SCI,
     S_SCM, S_INS_X, S_INS_Y, S_INS_CELL, S_INS_REC, S_INS_PQ, S_INS_RANK, S_INS_ADD, S_INS_SHIFT, S_INS_WORK,
     S_INS_DOLD, S_INS_DNEW, S_INS_ACC, S_KM_IP, S_KM_PN, S_KM_CN, S_KM_ASSIGN, S_KM_DIS, S_KM_PERM, S_KM_STARTS,
     S_KM_HIST, S_KM_OFF, S_KM_WORK, S_KM_BEST, S_KM_IDS, S_KM_X, S_KM_TV, S_KM_C0, S_KM_F0, S_KM_CB, S_KM_FB,
@@ -650,12 +650,12 @@ hipError_t coarse_select(const float* cdist, uint32_t m, uint32_t C, uint32_t ke
 
 // Probe features of m prepared queries: cell-distance row of the nearest
 // coarse list, its nfine smallest values, then the raw feature rows.
-// cv: m x nfine scratch.
+// cv: m x nfine scratch; cid (may be NULL): their fine ids.
 int probe_features_chunk(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_probe* p, uint32_t m,
                          const float* q, const float* csd, const uint32_t* csi, uint32_t cstride, const float* fdist,
-                         float* cv, float* feat, hipStream_t s) {
+                         float* cv, uint32_t* cid, float* feat, hipStream_t s) {
     const uint32_t F = ix->p.fine_cluster_count;
-    if (p->nfine) PK_CHECK(pk_launch_cell_topk(csd, csi, cstride, fdist, ix->stationary, F, p->nfine, m, cv, s));
+    if (p->nfine) PK_CHECK(pk_launch_cell_topk(csd, csi, cstride, fdist, ix->stationary, F, p->nfine, m, cv, cid, s));
     PK_CHECK(pk_launch_probe_raw_features(q, ix->p.dim, csd, cstride, p->nc, cv, p->nfine, m, feat, s));
     return PUCKGPU_OK;
 }
@@ -687,6 +687,7 @@ struct PassBufs {
     uint32_t* pnf = nullptr;
     float* py = nullptr;
     float* pcv = nullptr;  // probe cell features, cap x nfine
+    uint32_t* pcid = nullptr;  // their fine ids (the budget scan's cut of the nearest list)
     uint16_t* q16 = nullptr;   // reduced-precision queries (precision 1 / 2)
     uint8_t* q8 = nullptr;
     float* qs = nullptr;
@@ -772,7 +773,8 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     const uint32_t* nf = q_nf;
     if (pr) {
         // learned fine budget: features -> MLP -> budget, on this stream
-        if ((rc = probe_features_chunk(ctx, ix, pr, m, b.q, b.csd, b.csi, cfg.keep, b.fdist, b.pcv, b.pfeat, s)))
+        if ((rc = probe_features_chunk(ctx, ix, pr, m, b.q, b.csd, b.csi, cfg.keep, b.fdist, b.pcv, b.pcid,
+                                       b.pfeat, s)))
             return rc;
         PK_CHECK(pk_probe_pass(pr->params, pr->wbf, pr->D, pr->H, m, b.pfeat, nullptr, b.pwork, nullptr, nullptr,
                                b.py, s));
@@ -809,6 +811,11 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     wa.sc_items = b.sc_items; wa.sc_meta = b.sc_meta; wa.sc_olist = b.sc_olist;
     wa.sc_ocount = b.sc_ocount; wa.sc_capp = PK_SC_CAPP;
     wa.lut16 = cfg.lut16; wa.cb_nmax = ix->cb_nmax;
+    if (pr && pr->nfine) {
+        wa.top_d = b.pcv;
+        wa.top_f = b.pcid;
+        wa.top_k = pr->nfine;
+    }
     if (nf) {
         // the walk runs only over the queries without a budget below F (the list built above)
         wa.wlist = b.wlist;
@@ -942,6 +949,7 @@ int scratch_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
         b.pnf = (uint32_t*)scratch_slot(ctx, S_PNF, sizeof(uint32_t) * cap, &rc);
         b.py = (float*)scratch_slot(ctx, S_PY, sizeof(float) * cap, &rc);
         b.pcv = (float*)scratch_slot(ctx, S_PCV, sizeof(float) * cap * (pr->nfine ? pr->nfine : 1), &rc);
+        b.pcid = (uint32_t*)scratch_slot(ctx, S_PCID, sizeof(uint32_t) * cap * (pr->nfine ? pr->nfine : 1), &rc);
     }
     if (cfg.precision) {
         const uint32_t ld = pkc_ld(d), kc = cfg.kc ? cfg.kc : 1;
@@ -1418,7 +1426,7 @@ int puckgpu_probe_features(puckgpu_ctx* ctx, const puckgpu_index* ix, const puck
         float* fo = device_ptrs ? feat + (size_t)q0 * p->D : feat;
         float* cv = (float*)scratch_slot(ctx, S_PCV, sizeof(float) * (size_t)m * (p->nfine ? p->nfine : 1), &rc);
         if (rc) return rc;
-        if ((rc = probe_features_chunk(ctx, ix, p, m, q, csd, csi, keep, fdist, cv, fo, s))) return rc;
+        if ((rc = probe_features_chunk(ctx, ix, p, m, q, csd, csi, keep, fdist, cv, nullptr, fo, s))) return rc;
         if (!device_ptrs)
             PK_CHECK(hipMemcpyAsync(features + (size_t)q0 * p->D, fo, sizeof(float) * m * p->D,
                                     hipMemcpyDeviceToHost, s));
@@ -1566,6 +1574,7 @@ int bucket_create(puckgpu_server* sv, uint32_t cap, const PassCfg& cfg, PkBucket
         void* w = nullptr;
         if ((rc = balloc(ctx, bk, &b.pfeat, (size_t)cap * pr->D)) || (rc = balloc(ctx, bk, &b.pnf, cap)) ||
             (rc = balloc(ctx, bk, &b.py, cap)) || (rc = balloc(ctx, bk, &b.pcv, (size_t)cap * (pr->nfine ? pr->nfine : 1))) ||
+            (rc = balloc(ctx, bk, &b.pcid, (size_t)cap * (pr->nfine ? pr->nfine : 1))) ||
             (rc = balloc(ctx, bk, (char**)&w, pk_probe_work_bytes(cap, pr->D, pr->H)))) {
             bucket_release(bk);
             return rc;

@@ -45,6 +45,12 @@ struct PkWalkArgs {
     uint32_t* sc_olist;      // nq: the queries left to the per-query kernel
     uint32_t* sc_ocount;     // 1
     uint32_t sc_capp;
+    // the probe's sorted smallest cell distances of the nearest coarse list
+    // (values and fine ids, nq x top_k; null without a probe): the cell cut
+    // of that list is read from them when the budget is below top_k
+    const float* top_d;
+    const uint32_t* top_f;
+    uint32_t top_k;
     // filter table precision: 0 FP32 (the reference's), 1 FP16 (scaled per
     // query by a power of two; cb_nmax = the largest codeword norm)
     int lut16;

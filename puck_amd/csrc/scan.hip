@@ -75,33 +75,64 @@ __global__ __launch_bounds__(PK_SC_CELL_THREADS) void pk_scan_cells(PkWalkArgs a
         const float cd = a.csel_d[(uint64_t)qi * a.cstride + l];
         const float* stc = a.stationary + (uint64_t)c * F;
         const uint32_t* csc = a.cell_start + (uint64_t)c * F;
+        // per column slot: the cell's point count when selected (else 0), its
+        // first memory id and its cell distance key
         uint32_t kv[PK_SC_CPT], c0[PK_SC_CPT], c1[PK_SC_CPT];
-        // every load of the list at once, at in-bounds columns (a load under a
-        // lane condition is waited for before the next one is issued)
-        float sv[PK_SC_CPT], fv[PK_SC_CPT];
-#pragma unroll
-        for (int j = 0; j < PK_SC_CPT; ++j) {
-            const uint32_t k = min(j * NT + tid, F - 1u);
-            sv[j] = stc[k];
-            fv[j] = fdq[k];
-            c0[j] = csc[k];
-            c1[j] = csc[k + 1];
+        // the nearest list with the probe's sorted cell distances: the b-th
+        // smallest is entry b - 1 when it is below the last entry (no tie can
+        // reach past the sorted prefix), and the selected cells are the entries
+        // up to its last tie (block-uniform test)
+        bool fast = false;
+        if (l == 0 && a.top_f && b < a.top_k) {
+            const float* td = a.top_d + (uint64_t)qi * a.top_k;
+            fast = td[b - 1] < td[a.top_k - 1];
         }
+        if (fast) {
+            const float* td = a.top_d + (uint64_t)qi * a.top_k;
+            const uint32_t* tf = a.top_f + (uint64_t)qi * a.top_k;
+            const float vb = td[b - 1];
+            const uint32_t j = tid < a.top_k ? tid : a.top_k - 1u;  // in-bounds for every lane
+            const float v = td[j];
+            const uint32_t f = tf[j];
+            const uint32_t s0 = csc[f], s1 = csc[f + 1];
+            const bool sel = tid < a.top_k && v <= vb;
 #pragma unroll
-        for (int j = 0; j < PK_SC_CPT; ++j) kv[j] = pk_f2ord((cd + sv[j]) + fv[j]);
-        // the list's cut: b-th smallest cell key
-        const uint32_t kcut = pk_select_rank_each(
-            [&](auto sink) {
+            for (int jj = 0; jj < PK_SC_CPT; ++jj) {
+                kv[jj] = pk_f2ord(v);
+                c0[jj] = s0;
+                c1[jj] = (jj == 0 && sel) ? s1 - s0 : 0u;
+            }
+        } else {
+            // every load of the list at once, at in-bounds columns (a load under
+            // a lane condition is waited for before the next one is issued)
+            float sv[PK_SC_CPT], fv[PK_SC_CPT];
 #pragma unroll
-                for (int j = 0; j < PK_SC_CPT; ++j)
-                    if (j * NT + tid < F) sink(kv[j]);
-            },
-            b, hist, tmp, red);
+            for (int j = 0; j < PK_SC_CPT; ++j) {
+                const uint32_t k = min(j * NT + tid, F - 1u);
+                sv[j] = stc[k];
+                fv[j] = fdq[k];
+                c0[j] = csc[k];
+                c1[j] = csc[k + 1];
+            }
+#pragma unroll
+            for (int j = 0; j < PK_SC_CPT; ++j) kv[j] = pk_f2ord((cd + sv[j]) + fv[j]);
+            // the list's cut: b-th smallest cell key
+            const uint32_t kcut = pk_select_rank_each(
+                [&](auto sink) {
+#pragma unroll
+                    for (int j = 0; j < PK_SC_CPT; ++j)
+                        if (j * NT + tid < F) sink(kv[j]);
+                },
+                b, hist, tmp, red);
+#pragma unroll
+            for (int j = 0; j < PK_SC_CPT; ++j) {
+                const bool sel = j * NT + tid < F && kv[j] <= kcut;
+                c1[j] = sel ? c1[j] - c0[j] : 0u;  // the cell's point count when selected
+            }
+        }
         uint32_t lp = 0, lc = 0;
 #pragma unroll
         for (int j = 0; j < PK_SC_CPT; ++j) {
-            const bool sel = j * NT + tid < F && kv[j] <= kcut;
-            c1[j] = sel ? c1[j] - c0[j] : 0u;  // the cell's point count when selected
             lp += c1[j];
             lc += c1[j] > 0 ? 1u : 0u;
         }

@@ -331,13 +331,15 @@ __global__ __launch_bounds__(SORT_THREADS) void pk_cell_topk(const float* __rest
                                                              const uint32_t* __restrict__ csi, uint32_t cstride,
                                                              const float* __restrict__ fdist,
                                                              const float* __restrict__ stationary, uint32_t F,
-                                                             uint32_t k, float* __restrict__ out_val) {
+                                                             uint32_t k, float* __restrict__ out_val,
+                                                             uint32_t* __restrict__ out_idx) {
     const uint64_t row = blockIdx.x;
     const float cd = csd[row * cstride];
     const float* st = stationary + (uint64_t)csi[row * cstride] * F;
     const float* fd = fdist + row * F;
     bool tie = false;
-    pk_topk_block([&](uint32_t f) { return (cd + st[f]) + fd[f]; }, F, k, out_val + row * k, nullptr, &tie, 0);
+    pk_topk_block([&](uint32_t f) { return (cd + st[f]) + fd[f]; }, F, k, out_val + row * k,
+                  out_idx ? out_idx + row * k : nullptr, &tie, 0);
 }
 
 __global__ __launch_bounds__(SORT_THREADS) void pk_topk_rows_wave(const float* __restrict__ vals, uint32_t rows,
@@ -371,7 +373,8 @@ __global__ __launch_bounds__(SORT_THREADS) void pk_cell_topk_wave(const float* _
                                                                   const float* __restrict__ fdist,
                                                                   const float* __restrict__ stationary, uint32_t F,
                                                                   uint32_t k, uint32_t rows,
-                                                                  float* __restrict__ out_val) {
+                                                                  float* __restrict__ out_val,
+                                                                  uint32_t* __restrict__ out_idx) {
     __shared__ __align__(16) unsigned char lds[(SORT_THREADS / 64) * TOPK_WAVE_LDS];
     const uint32_t w = threadIdx.x >> 6;
     const uint64_t row = (uint64_t)blockIdx.x * (SORT_THREADS / 64) + w;
@@ -393,7 +396,7 @@ __global__ __launch_bounds__(SORT_THREADS) void pk_cell_topk_wave(const float* _
 #pragma unroll
             for (int j = 0; j < TOPK_WAVE_PER; ++j) kv[j] = pk_f2ord((cd + x[j]) + y[j]);
         },
-        F, k, out_val + row * k, nullptr, &tie, 0, hist, buf);
+        F, k, out_val + row * k, out_idx ? out_idx + row * k : nullptr, &tie, 0, hist, buf);
 }
 
 extern "C" hipError_t pk_launch_topk_rows(const float* vals, uint32_t rows, uint32_t L, uint64_t ld_in, uint32_t k,
@@ -412,14 +415,14 @@ extern "C" hipError_t pk_launch_topk_rows(const float* vals, uint32_t rows, uint
 
 extern "C" hipError_t pk_launch_cell_topk(const float* csd, const uint32_t* csi, uint32_t cstride, const float* fdist,
                                           const float* stationary, uint32_t F, uint32_t k, uint32_t rows,
-                                          float* out_val, hipStream_t s) {
+                                          float* out_val, uint32_t* out_idx, hipStream_t s) {
     if (rows == 0 || k == 0) return hipSuccess;
     if (k > TOPK_CAP || k > F) return hipErrorInvalidValue;
     if (F <= 64 * TOPK_WAVE_PER && k <= TOPK_WAVE_K)
         hipLaunchKernelGGL(pk_cell_topk_wave, dim3((rows + 3) / 4), dim3(SORT_THREADS), 0, s, csd, csi, cstride, fdist,
-                           stationary, F, k, rows, out_val);
+                           stationary, F, k, rows, out_val, out_idx);
     else
         hipLaunchKernelGGL(pk_cell_topk, dim3(rows), dim3(SORT_THREADS), 0, s, csd, csi, cstride, fdist, stationary,
-                           F, k, out_val);
+                           F, k, out_val, out_idx);
     return hipGetLastError();
 }
