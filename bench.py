@@ -111,8 +111,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--index-cache", type=str, default=None,
                     help="directory: load the trained index arrays from it when present, else save them there")
-    ap.add_argument("--workload", choices=["c1", "c4"], default="c1",
-                    help="c1: the headline 10M x 96 workload; c4: 100M x 128 uint8 precision at scale")
+    ap.add_argument("--workload", choices=["c0", "c1", "c4"], default="c1",
+                    help="c1: the headline 10M x 96 workload; c0: 1M x 128 reference configuration; "
+                         "c4: 100M x 128 uint8 precision at scale")
     ap.add_argument("--c4-train", type=int, default=10_000_000, help="c4: k-means training sample")
     ap.add_argument("--c4-rounds", type=int, default=10, help="c4: kmeans_iterations_count")
     ap.add_argument("--c4-sweep", type=str, default="1,2,4,8")
@@ -130,6 +131,10 @@ def main():
         if args.n == 10_000_000:
             args.n = 100_000_000
         return c4_main(args)
+    if args.workload == "c0":
+        if args.n == 10_000_000:
+            args.n = 1_000_000
+        return c0_main(args)
 
     import torch
     from puck_amd import datasets, train
@@ -410,6 +415,57 @@ def main():
         result["cpu_baseline"] = cpu_baseline(ix, q.cpu().numpy(), topk, chosen, ftk, args.cpu_seconds, out,
                                               gt_i, gt_d)
     print(json.dumps(result), flush=True)
+
+
+def c0_main(args):
+    """The reference's CPU-runnable configuration: 1,000,000 x 128 integer
+    vectors (1,000 Gaussian means uniform in [0, 160]^128, sigma 25, clipped
+    and rounded; base seed 0, queries seed 1), L2; 1,024 x 1,024 centers
+    trained on a 200,000-point sample (10 rounds), filter PQ nsq 32; search
+    search_coarse_count 100, filter_topk 500, top-k 10, one batch of 10,000
+    queries.  Reported: the plain GPU search (QPS, recall, stages), the
+    precision forms with their center-distance error against FP64, and the
+    CPU restatement on the host cores with slot and distance-bit parity."""
+    import torch
+    from puck_amd import datasets, train
+    from puck_amd.engine import Context, GpuIndex
+    from puck_amd.metrics import exact_topk, recall_with_ties
+
+    N, d, C, F, nsq, scc, ftk, topk, nq = args.n, 128, 1024, 1024, 32, 100, 500, 10, args.nq
+    dev = torch.device("cuda:0")
+    x = datasets.mixture_uint8_grid(N, d, 1000, seed=0, means_seed=1000, backend="torch", device=dev)
+    q = datasets.mixture_uint8_grid(nq, d, 1000, seed=1, means_seed=1000, backend="torch", device=dev)
+    ctx = Context(0)
+    t0 = time.time()
+    ix = train.build(ctx, x, C, F, nsq, whether_norm=False, train_n=200_000, rounds=10)
+    torch.cuda.synchronize()
+    build_s = time.time() - t0
+    log(f"c0 train + assign + encode {build_s:.1f}s")
+    gi = GpuIndex(ctx, ix["coarse"], ix["fine"], ix["codebook"], ix["cell_assign"], ix["records"], ix["raw"], False)
+    gt_i, gt_d = exact_topk(x, q, topk)
+    gt_i, gt_d = gt_i.cpu().numpy(), gt_d.cpu().numpy()
+    out = (torch.empty((nq, topk), dtype=torch.float32, device=dev),
+           torch.empty((nq, topk), dtype=torch.int32, device=dev),
+           torch.empty(nq, dtype=torch.int32, device=dev))
+    plain = timed(lambda: gi.search(q, topk, scc, ftk, out=out), args.steps, args.warmup, gi)
+    rec = recall_with_ties(gt_i, gt_d, out[1].cpu().numpy().astype(np.int64), topk)[0]
+    res = {"workload": f"{N} x {d} integer mixture, L2, C=F={C}, filter_nsq={nsq}, search_coarse_count={scc}, "
+                       f"filter_topk={ftk}, top-k={topk}, batch of {nq} queries",
+           "build_s": round(build_s, 1),
+           "plain": {"qps": round(nq * args.steps / plain["el"], 1), "recall_at_10": round(rec, 5),
+                     "ms_per_step": round(1000.0 * plain["el"] / args.steps, 3),
+                     "stage_ms_per_step": {k: round(float(v) / args.steps, 3) for k, v in
+                                           zip(["prep+gemm", "center_sort+probe", "walk_scan", "rerank",
+                                                "tie_replay"], plain["stage"])},
+                     "replayed_queries": int(plain["cnt"][4])}}
+    log(f"c0 plain: {res['plain']}")
+    res["precision_forms"] = precision_forms(gi, ix, q, gt_i, gt_d, topk, scc, ftk, out, args, normalize=False)
+    if not args.no_cpu:
+        gi.search(q, topk, scc, ftk, out=out)
+        res["cpu_baseline"] = cpu_baseline(ix, q.cpu().numpy(), topk, scc, ftk, args.cpu_seconds, out, gt_i, gt_d,
+                                           whether_norm=False)
+        log(f"c0 cpu: {res['cpu_baseline']}")
+    print(json.dumps({"c0": res}), flush=True)
 
 
 def c4_main(args):
@@ -710,10 +766,45 @@ def serving(gi, dev, d, nq_total, max_batch=4096):
                                               scc=scc[s0:s0 + b], out=(D[:b], I[:b], R[:b])))
     tuned = sv.tuned()
     log(f"serving: graph {graph} eager {eager} tuned (gemm shape, rerank rows) {tuned}")
+    # launch-bound regime on the same server: small batches (log-uniform 1..256)
+    # of cheap queries (search_coarse_count 1..8), where the per-batch launch
+    # sequence, not the walk, sets the latency
+    rs = np.random.default_rng(9)
+    ns, small, tot = 20_000, [], 0
+    while tot < ns:
+        b = min(int(np.floor(np.exp(rs.uniform(0.0, np.log(256))))), ns - tot)
+        small.append(b)
+        tot += b
+    scc_s = torch.as_tensor(rs.integers(1, 9, ns).astype(np.int32), device=dev)
+
+    def run_small(fn):
+        lat, s0 = [], 0
+        torch.cuda.synchronize()
+        t1 = time.time()
+        for b in small:
+            t2 = time.time()
+            fn(s0, b)
+            lat.append(time.time() - t2)
+            s0 += b
+        torch.cuda.synchronize()
+        el = time.time() - t1
+        lat = np.array(lat) * 1000.0
+        return {"qps": round(ns / el, 1), "p50_batch_ms": round(float(np.percentile(lat, 50)), 3),
+                "p99_batch_ms": round(float(np.percentile(lat, 99)), 3), "wall_s": round(el, 3)}
+
+    for _ in range(2):  # graphs of the small buckets are already captured; warm both paths
+        g_small = run_small(lambda s0, b: sv.search(qs[s0:s0 + b], topk[s0:s0 + b], scc_s[s0:s0 + b],
+                                                    out=(D[:b], I[:b], R[:b])))
+        e_small = run_small(lambda s0, b: gi.search_batch(qs[s0:s0 + b], 100, 400, 1100, topk=topk[s0:s0 + b],
+                                                          scc=scc_s[s0:s0 + b], out=(D[:b], I[:b], R[:b])))
+    log(f"serving, small batches: graph {g_small} eager {e_small}")
     sv.free()
     return {"workload": f"{nq_total} queries in {len(sizes)} batches of floor(exp(U[0, ln {max_batch}])) "
                         "queries, topk in {1,10,100}, search_coarse_count in 20..400, filter_topk 1100",
             "graph_server": graph, "eager_batches": eager, "setup_s": round(setup_s, 1),
+            "small_batches": {"workload": f"{ns} queries in {len(small)} batches of floor(exp(U[0, ln 256])) queries, "
+                                          "topk in {1,10,100}, search_coarse_count in 1..8, filter_topk 1100",
+                              "graph_server": g_small, "eager_batches": e_small},
             "autotuned_gemm_shape_per_bucket": shapes.tolist(),
             "autotuned_rerank_rows_per_bucket": [r for _, r in tuned],
             "gemm_ms_per_bucket_and_shape": np.round(times, 4).tolist()}
@@ -737,7 +828,7 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(ix, qh, topk, scc, ftk, budget_s, gpu_out, gt_i, gt_d, runs=5):
+def cpu_baseline(ix, qh, topk, scc, ftk, budget_s, gpu_out, gt_i, gt_d, runs=5, whether_norm=True):
     """The oracle's restatement of PuckIndex::search (whether_pq=false), built
     -O3 -march=x86-64-v3 (oracle/liboracle_fast.so, bit-identical to the
     checker), on the host cores: one query per OpenMP thread as PySearcher's
@@ -749,7 +840,7 @@ def cpu_baseline(ix, qh, topk, scc, ftk, budget_s, gpu_out, gt_i, gt_d, runs=5):
     t0 = time.time()
     oi = O.OracleIndex(ix["coarse"].cpu().numpy(), ix["fine"].cpu().numpy(), ix["codebook"].cpu().numpy(),
                        ix["cell_assign"].cpu().numpy().view(np.uint32), ix["records"].cpu().numpy(),
-                       ix["raw"].cpu().numpy(), True, L=O.lib_fast())
+                       ix["raw"].cpu().numpy(), whether_norm, L=O.lib_fast())
     log(f"cpu oracle index {time.time() - t0:.1f}s")
     threads = cpu_share()
     n = 8 * threads
@@ -771,7 +862,7 @@ def cpu_baseline(ix, qh, topk, scc, ftk, budget_s, gpu_out, gt_i, gt_d, runs=5):
     log(f"cpu baseline: {n2} queries, best of {runs} {best:.2f}s on {threads} threads ({cpu_model()})")
     return {"value": round(n2 / best, 1), "unit": "queries/s", "cores": threads, "kind": "port",
             "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
-            "sample": f"{n2} of the 10000 queries at search_coarse_count={scc}, filter_topk={ftk}, best of {runs} "
+            "sample": f"{n2} of the {qh.shape[0]} queries at search_coarse_count={scc}, filter_topk={ftk}, best of {runs} "
                       "passes over the sample",
             "build": "oracle/puck_oracle.c, gcc -O3 -march=x86-64-v3 -ffp-contract=off (scalar restatement, "
                      "bit-identical to the checker)",
