@@ -158,6 +158,11 @@ int GpuPuckIndex::search(const Request* request, Response* response) {
         _err = "topk should <= topk, or feature is nullptr";
         return -1;
     }
+    if (request->topk > 0 && (!response->distance || !response->local_idx)) {
+        std::lock_guard<std::mutex> g(_mu);
+        _err = "response buffers are nullptr";
+        return -1;
+    }
     PendingSearch p{request->feature, request->topk, response->distance, response->local_idx, 0, -1};
     _gc.submit(&p, [this](PendingSearch* const* items, size_t n) { run_coalesced(items, n); });
     if (p.rc == 0) response->result_num = p.result_num;
