@@ -696,17 +696,18 @@ struct PassBufs {
     uint32_t* wlist = nullptr;   // queries taking the walk when budgets are given
     uint32_t* qorder = nullptr;  // workgroup -> query order of the per-query kernels
     uint32_t* wcount = nullptr;
-    // split budget scan: work items, candidate keys, per-query counts, the
-    // list of queries left to the per-query kernel and its length
-    uint2* sc_items = nullptr;
+    // split budget scan: per-query selected cells, per-query counts (points,
+    // cells), the list of queries left to the per-query kernel and its length
+    uint4* sc_cells = nullptr;
     uint32_t* sc_meta = nullptr;
     uint32_t* sc_olist = nullptr;
     uint32_t* sc_ocount = nullptr;
 };
 
-// Candidate points (work items and keys) per query of the split budget scan;
-// a query with more is scanned by the per-query kernel.
-constexpr uint32_t PK_SC_CAPP = 8192;
+// Candidate points and cells per query of the split budget scan; a query
+// with more is scanned by the per-query kernel.
+constexpr uint32_t PK_SC_CAPP = 8192;  // points per query in the split scan (keys in registers)
+constexpr uint32_t PK_SC_CELLS = 512;  // cells per query in the split scan (its LDS list)
 
 // Per-pass configuration.
 struct PassCfg {
@@ -808,7 +809,7 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     wa.flags = b.flags; wa.exact = 0; wa.tie_ids = b.tie_ids; wa.tie_n = b.tie_n;
     wa.q_scc = q_scc; wa.q_nf = nf; wa.cstride = cfg.keep ? cfg.keep : 1; wa.fdist = b.fdist;
     wa.wlist = nullptr; wa.wcount = nullptr;
-    wa.sc_items = b.sc_items; wa.sc_meta = b.sc_meta; wa.sc_olist = b.sc_olist;
+    wa.sc_cells = b.sc_cells; wa.sc_meta = b.sc_meta; wa.sc_olist = b.sc_olist;
     wa.sc_ocount = b.sc_ocount; wa.sc_capp = PK_SC_CAPP;
     wa.lut16 = cfg.lut16; wa.cb_nmax = ix->cb_nmax;
     if (pr && pr->nfine) {
@@ -938,7 +939,7 @@ int scratch_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     b.qorder = b.wlist ? b.wlist + cap : nullptr;
     b.wcount = b.wlist ? b.wlist + 2 * (size_t)cap : nullptr;
     if (cfg.budget) {
-        b.sc_items = (uint2*)scratch_slot(ctx, S_SCI, sizeof(uint2) * (size_t)cap * PK_SC_CAPP, &rc);
+        b.sc_cells = (uint4*)scratch_slot(ctx, S_SCI, sizeof(uint4) * (size_t)cap * PK_SC_CELLS, &rc);
         b.sc_meta = (uint32_t*)scratch_slot(ctx, S_SCM, sizeof(uint32_t) * (3 * (size_t)cap + 1), &rc);
         b.sc_olist = b.sc_meta ? b.sc_meta + 2 * (size_t)cap : nullptr;
         b.sc_ocount = b.sc_meta ? b.sc_meta + 3 * (size_t)cap : nullptr;
@@ -1582,7 +1583,7 @@ int bucket_create(puckgpu_server* sv, uint32_t cap, const PassCfg& cfg, PkBucket
         b.pwork = w;
     }
     if (cfg.budget) {
-        if ((rc = balloc(ctx, bk, &b.sc_items, (size_t)cap * PK_SC_CAPP)) ||
+        if ((rc = balloc(ctx, bk, &b.sc_cells, (size_t)cap * PK_SC_CELLS)) ||
             (rc = balloc(ctx, bk, &b.sc_meta, 3 * (size_t)cap + 1))) {
             bucket_release(bk);
             return rc;

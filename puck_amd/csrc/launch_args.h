@@ -39,9 +39,9 @@ struct PkWalkArgs {
     int fine_given;          // a query whose flags hold bit 3 on entry has its fine row in the
                              // reference's heap order: its fine-distance ties need no flag, and
                              // its flags are rewritten (not or-ed) with what this walk finds
-    // split budget scan (scan.hip); sc_items null: every budget query takes the per-query kernel
-    uint2* sc_items;         // nq x sc_capp: per scored point {memory id, cell distance bits}
-    uint32_t* sc_meta;       // nq x 2: points (0xffffffff: left to the per-query kernel), points
+    // split budget scan (scan.hip); sc_cells null: every budget query takes the per-query kernel
+    uint4* sc_cells;         // nq x 512: selected cells {first memory id, points before, cell distance bits, 0}
+    uint32_t* sc_meta;       // nq x 2: points (0xffffffff: left to the per-query kernel), cells
     uint32_t* sc_olist;      // nq: the queries left to the per-query kernel
     uint32_t* sc_ocount;     // 1
     uint32_t sc_capp;

@@ -4,15 +4,17 @@
 //   pk_scan_cells      one workgroup per query: in each walked coarse list, the
 //                      cut at the b-th smallest cell distance (cd + stationary)
 //                      + fine distance (the reference's cell distance
-//                      expression, puck_index.cpp:535), and one work item per
-//                      point of the selected cells {memory id, cell distance}
+//                      expression, puck_index.cpp:535), and the query's list of
+//                      selected non-empty cells {first memory id, points before
+//                      it, cell distance}
 //   pk_scan_score_cut  one workgroup per query: the query's filter lookup table
 //                      in LDS (quantization.cpp:260-270), then every point of
 //                      every item gets the reference's filter distance
 //                      (compute_quantized_distance, puck_index.cpp:346-390:
 //                      t = fl(2 * cell_dist + offset) + lookup sum in the 4-lane
 //                      order of puck_index.cpp:75-113) as a (distance, memory
-//                      id) key kept in registers (consecutive threads take
+//                      id) key kept in registers (the cell list in LDS, a
+//                      point's cell by binary search; consecutive threads take
 //                      consecutive points: the record loads coalesce); then the
 //                      filter_topk smallest keys (radix select on the distance
 //                      bits, then on the ids of the keys tied at the cut) into
@@ -21,13 +23,15 @@
 // The result is the one of the per-query scan (pk_scan_budget in search.hip):
 // the filter set is the filter_topk smallest (distance, memory id) keys over
 // every point of the selected cells, which does not depend on the order in
-// which points are scored.  A query whose candidates exceed the item buffer,
-// or whose index has F > PK_SC_CPT * PK_SC_CELL_THREADS, is marked and left to
-// the per-query kernel (pk_scan_overflow_list builds its list).
+// which points are scored.  A query with more than sc_capp points or
+// PK_SC_MAXC cells, or whose index has F > PK_SC_CPT * PK_SC_CELL_THREADS, is
+// marked and left to the per-query kernel (pk_scan_overflow_list builds its
+// list).
 #include "common.h"
 #include "launch_args.h"
 
 #define PK_SC_OVERFLOW 0xffffffffu
+#define PK_SC_MAXC 512u  // cells per query in the scoring kernel's LDS
 
 __device__ __forceinline__ bool pk_sc_budget_query(const PkWalkArgs& a, uint32_t qi) {
     return a.q_nf && a.q_nf[qi] != 0u && a.q_nf[qi] < a.F;
@@ -42,9 +46,6 @@ __global__ __launch_bounds__(PK_SC_CELL_THREADS) void pk_scan_cells(PkWalkArgs a
     __shared__ uint32_t hist[PK_SEL_HIST];
     __shared__ uint32_t tmp[PK_SC_CELL_THREADS / 64 + 1];
     __shared__ uint32_t red[2 * PK_SC_CELL_THREADS / 64];
-    // the list's selected non-empty cells: first memory id, first item, cell distance bits
-    __shared__ uint32_t sel_c0[PK_SC_CPT * PK_SC_CELL_THREADS + 1], sel_pre[PK_SC_CPT * PK_SC_CELL_THREADS + 1];
-    __shared__ uint32_t sel_cd[PK_SC_CPT * PK_SC_CELL_THREADS];
     const uint32_t tid = threadIdx.x, NT = PK_SC_CELL_THREADS;
     const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
     if (!pk_sc_budget_query(a, qi)) return;
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(PK_SC_CELL_THREADS) void pk_scan_cells(PkWalkArgs a
     uint32_t scc = a.scc;
     if (a.q_scc) scc = a.q_scc[qi] < scc ? a.q_scc[qi] : scc;
     const float* fdq = a.fdist + (uint64_t)qi * F;
-    uint2* items = a.sc_items + (uint64_t)qi * a.sc_capp;
+    uint4* cells = a.sc_cells + (uint64_t)qi * PK_SC_MAXC;
     uint32_t npt = 0, ncell = 0;
     bool ovf = false;
     for (uint32_t l = 0; l < scc; ++l) {
@@ -139,39 +140,25 @@ __global__ __launch_bounds__(PK_SC_CELL_THREADS) void pk_scan_cells(PkWalkArgs a
         uint32_t tp, tc;
         uint32_t pb = pk_block_exscan(lp, tmp, &tp);
         uint32_t cb = pk_block_exscan(lc, tmp, &tc);
-        if (npt + tp > a.sc_capp) {  // block-uniform
+        if (npt + tp > a.sc_capp || ncell + tc > PK_SC_MAXC) {  // block-uniform
             ovf = true;
             break;
         }
-        // the selected cells into LDS in column order, then every thread writes
-        // consecutive items (the item's cell by binary search over the prefixes)
+        // the selected non-empty cells, in column order within each thread
 #pragma unroll
         for (int j = 0; j < PK_SC_CPT; ++j) {
             if (c1[j] > 0) {
-                sel_c0[cb] = c0[j];
-                sel_pre[cb] = pb;
-                sel_cd[cb] = __float_as_uint(pk_ord2f(kv[j]));
+                cells[ncell + cb] = make_uint4(c0[j], npt + pb, __float_as_uint(pk_ord2f(kv[j])), 0u);
                 ++cb;
                 pb += c1[j];
             }
         }
-        if (tid == 0) sel_pre[tc] = tp;
-        __syncthreads();
-        for (uint32_t p = tid; p < tp; p += NT) {
-            uint32_t lo = 0, hi = tc;  // sel_pre[lo] <= p < sel_pre[hi]
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (sel_pre[mid] <= p) lo = mid; else hi = mid;
-            }
-            items[npt + p] = make_uint2(sel_c0[lo] + (p - sel_pre[lo]), sel_cd[lo]);
-        }
-        __syncthreads();  // the LDS list is rewritten by the next list
         npt += tp;
         ncell += tc;
     }
     if (tid == 0) {
         meta[0] = ovf ? PK_SC_OVERFLOW : npt;
-        meta[1] = npt;
+        meta[1] = ncell;
         if (a.stats && !ovf) {
             a.stats[qi * 4 + 0] = 0;
             a.stats[qi * 4 + 1] = 0;
@@ -191,12 +178,32 @@ __global__ __launch_bounds__(PK_SC_CELL_THREADS) void pk_scan_cells(PkWalkArgs a
 __device__ __forceinline__ float pk_sc_lv(float v) { return v; }
 __device__ __forceinline__ float pk_sc_lv(__half v) { return __half2float(v); }
 
+// The query's selected cells in LDS: first memory id, points before the
+// cell, cell distance bits; pre[nc] = n.
+struct PkScCells {
+    const uint32_t* c0;
+    const uint32_t* pre;
+    const uint32_t* cd;
+    uint32_t nc;
+};
+
+// Point i's {memory id, cell distance bits}: the cell with pre <= i < next pre.
+__device__ __forceinline__ uint2 pk_sc_point(const PkScCells& sc, uint32_t i) {
+    uint32_t lo = 0, hi = sc.nc;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (sc.pre[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return make_uint2(sc.c0[lo] + (i - sc.pre[lo]), sc.cd[lo]);
+}
+
 // Filter distances of PP points per thread, returned as keys in registers:
-// kout[h] is the key (distance bits, memory id) of item i0 + h * NT +
+// kout[h] is the key (distance bits, memory id) of point i0 + h * NT +
 // threadIdx.x (consecutive threads, consecutive records), or ~0 past n.
 template <int NT, int NV, typename T>
 __device__ __forceinline__ void pk_sc_score_keys(const PkWalkArgs& a, const T* __restrict__ lut, float inv,
-                                                 const uint2* __restrict__ items, uint32_t i0, uint32_t n,
+                                                 const PkScCells& sc, uint32_t i0, uint32_t n,
                                                  uint64_t (&kout)[PK_SC_PP]) {
     const uint32_t nsq = a.nsq;
     uint2 it[PK_SC_PP];
@@ -205,7 +212,7 @@ __device__ __forceinline__ void pk_sc_score_keys(const PkWalkArgs& a, const T* _
 #pragma unroll
     for (int h = 0; h < PK_SC_PP; ++h) {
         const uint32_t i = i0 + h * NT + threadIdx.x;
-        it[h] = items[i < n ? i : n - 1u];
+        it[h] = pk_sc_point(sc, i < n ? i : n - 1u);
     }
 #pragma unroll
     for (int h = 0; h < PK_SC_PP; ++h) {
@@ -246,14 +253,15 @@ __device__ __forceinline__ void pk_sc_score_keys(const PkWalkArgs& a, const T* _
     }
 }
 
-// LDS of pk_scan_score_cut: the lookup table (FP32 or FP16), then the query.
+// LDS of pk_scan_score_cut: the lookup table (FP32 or FP16), then the query
+// (the cell list is static).
 __host__ __device__ inline size_t pk_scan_lut_bytes(uint32_t nsq, int h16) {
     return (((size_t)nsq * PK_KS * (h16 ? 2u : 4u)) + 15u) & ~(size_t)15u;
 }
 
 // ---------------------------------------------------------------------------
 // Scoring and filter cut of one query in one workgroup: the keys of the
-// query's items stay in registers (PK_SC_KR rounds of PK_SC_PP per thread,
+// query's points stay in registers (PK_SC_KR rounds of PK_SC_PP per thread,
 // sc_capp <= PK_SC_PP * PK_SC_KR * NT), then the filter_topk smallest
 // (distance bits, memory id) are selected (radix select on the distance
 // bits, then on the ids of the keys tied at the cut) and written to the
@@ -271,6 +279,7 @@ __global__ __launch_bounds__(NT) void pk_scan_score_cut(PkWalkArgs a) {
     __shared__ uint32_t tmp[NT / 64 + 1];
     __shared__ uint32_t red[2 * NT / 64];
     __shared__ uint32_t cnt[2];
+    __shared__ uint32_t s_c0[PK_SC_MAXC], s_pre[PK_SC_MAXC + 1], s_cd[PK_SC_MAXC];
     constexpr int KR = PkScCut<NT>::KR, KPT = PkScCut<NT>::KPT;
     const uint32_t qi = a.qlist ? a.qlist[blockIdx.x] : blockIdx.x;
     const uint32_t tid = threadIdx.x;
@@ -285,12 +294,20 @@ __global__ __launch_bounds__(NT) void pk_scan_score_cut(PkWalkArgs a) {
         if (tid == 0) a.out_n[qi] = 0;
         return;
     }
-    const uint32_t n = nit;
+    const uint32_t n = nit, nc = a.sc_meta[2ull * qi + 1];
     float* qv = (float*)(smem + pk_scan_lut_bytes(a.nsq, H16));
     const float* qg = a.q + (uint64_t)qi * a.dim;
     for (uint32_t i = tid; i < a.dim; i += NT) qv[i] = qg[i];
+    const uint4* cg = a.sc_cells + (uint64_t)qi * PK_SC_MAXC;
+    for (uint32_t i = tid; i < nc; i += NT) {
+        const uint4 e = cg[i];
+        s_c0[i] = e.x;
+        s_pre[i] = e.y;
+        s_cd[i] = e.z;
+    }
+    if (tid == 0) s_pre[nc] = n;
     __syncthreads();
-    const uint2* items = a.sc_items + (uint64_t)qi * a.sc_capp;
+    const PkScCells cl{s_c0, s_pre, s_cd, nc};
     uint64_t x[KPT];
     if constexpr (H16) {
         __half* lut = (__half*)smem;
@@ -302,7 +319,7 @@ __global__ __launch_bounds__(NT) void pk_scan_score_cut(PkWalkArgs a) {
 #pragma unroll
         for (int r = 0; r < KR; ++r) {
             uint64_t k2[PK_SC_PP];
-            if (r * PK_SC_PP * NT < n) pk_sc_score_keys<NT, NV>(a, lut, inv, items, r * PK_SC_PP * NT, n, k2);
+            if (r * PK_SC_PP * NT < n) pk_sc_score_keys<NT, NV>(a, lut, inv, cl, r * PK_SC_PP * NT, n, k2);
             else
 #pragma unroll
                 for (int h = 0; h < PK_SC_PP; ++h) k2[h] = ~0ull;
@@ -316,7 +333,7 @@ __global__ __launch_bounds__(NT) void pk_scan_score_cut(PkWalkArgs a) {
 #pragma unroll
         for (int r = 0; r < KR; ++r) {
             uint64_t k2[PK_SC_PP];
-            if (r * PK_SC_PP * NT < n) pk_sc_score_keys<NT, NV>(a, lut, 1.0f, items, r * PK_SC_PP * NT, n, k2);
+            if (r * PK_SC_PP * NT < n) pk_sc_score_keys<NT, NV>(a, lut, 1.0f, cl, r * PK_SC_PP * NT, n, k2);
             else
 #pragma unroll
                 for (int h = 0; h < PK_SC_PP; ++h) k2[h] = ~0ull;
@@ -417,7 +434,7 @@ extern "C" int pk_scan_split_ok(uint32_t dim, uint32_t nsq, uint32_t F) {
 // scoring and the filter cut (post).  The per-query kernel needs only the
 // list, so a caller can run it on another stream between the two parts.
 static hipError_t pk_scan_split_check(const PkWalkArgs& a) {
-    if (!a.q_nf || !a.fdist || !a.sc_items || !a.sc_meta || !a.sc_olist || !a.sc_ocount)
+    if (!a.q_nf || !a.fdist || !a.sc_cells || !a.sc_meta || !a.sc_olist || !a.sc_ocount)
         return hipErrorInvalidValue;
     if (a.sc_capp > 8192u || a.sc_capp < a.ftk) return hipErrorInvalidValue;
     if (pk_scan_score_smem(a.dim, a.nsq, 0) > 64 * 1024) return hipErrorInvalidValue;

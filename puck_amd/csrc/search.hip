@@ -1600,7 +1600,7 @@ extern "C" hipError_t pk_launch_walk(const PkWalkArgs* args, uint32_t nq, hipStr
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (args->q_nf) {
-        if (args->sc_items && pk_scan_split_ok(args->dim, args->nsq, args->F)) {
+        if (args->sc_cells && pk_scan_split_ok(args->dim, args->nsq, args->F)) {
             // the split scan; the per-query kernel takes the queries its cell
             // selection leaves, on the second stream beside the scoring and cut
             if ((e = pk_launch_scan_split_pre(&sa, nq, s)) != hipSuccess) return e;
