@@ -298,11 +298,12 @@ def main():
     #     (d x 4 B); local ids are read for the k returned rows only
     #   walk (pk_walk, plain form): per examined cell its stationary distance
     #     and start offset (8 B); per scanned point its offset and nsq code bytes
-    #   budget scan (learned form: pk_scan_cells + pk_scan_score + pk_scan_cut,
-    #     and the per-query kernel for the queries they leave): per query and
-    #     walked list its stationary, fine-distance and cell-start rows (12 B
-    #     per fine cluster); per scored point its offset and nsq code bytes, its
-    #     8 B key written and read back; per filter survivor 8 B written
+    #   budget scan (learned form: pk_scan_cells + pk_scan_score_cut, and the
+    #     per-query kernel for the queries they leave): per query and walked
+    #     list its stationary, fine-distance and cell-start rows (12 B per fine
+    #     cluster); per scored point its offset and nsq code bytes; per
+    #     selected cell its 16-byte list entry written and read back; per
+    #     filter survivor 8 B written
     cells, points = float(cnt[2]), float(cnt[3])
     surv = float(np.asarray(head["ftk_n"], np.float64).sum())
     kern = {}
@@ -310,7 +311,7 @@ def main():
     if form == "learned":
         # a group of kernels (the per-query fallback runs beside the others on
         # a second stream): reported, but not a candidate for the dominant kernel
-        kern["budget_scan"] = dict(bytes=nq * chosen * F * 12 + points * (4 + nsq + 16) + surv * 8,
+        kern["budget_scan"] = dict(bytes=nq * chosen * F * 12 + points * (4 + nsq) + cells * 32 + surv * 8,
                                    ms=float(np.mean(head["scan_ms"])), single=False)
     else:
         kern["pk_walk"] = dict(bytes=cells * 8 + points * (4 + nsq), ms=float(np.mean(head["scan_ms"])), single=True)
