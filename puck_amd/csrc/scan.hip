@@ -9,7 +9,7 @@
 //                      it, cell distance}
 //   pk_scan_score_cut  one workgroup per query: the query's filter lookup table
 //                      in LDS (quantization.cpp:260-270), then every point of
-//                      every item gets the reference's filter distance
+//                      every selected cell gets the reference's filter distance
 //                      (compute_quantized_distance, puck_index.cpp:346-390:
 //                      t = fl(2 * cell_dist + offset) + lookup sum in the 4-lane
 //                      order of puck_index.cpp:75-113) as a (distance, memory

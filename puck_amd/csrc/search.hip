@@ -377,6 +377,92 @@ __device__ __forceinline__ uint32_t pk_bmax(uint32_t v, uint32_t* red) {
     return r;
 }
 
+// Bitonic sort of the n <= WALK_PMAX keys v[0..n) by the walk's 256 threads
+// with the keys in registers: element e sits in wave e / 128, lane e % 64, slot
+// (e / 64) % 2; padded with ~0 to n2 = 2^ceil(log2 n).  Exchanges at distance
+// 64 stay in the lane, those below 64 move between lanes (DPP for 1 to 8,
+// ds_swizzle for 16, ds_bpermute for 32); only the distances >= 128 (three
+// stages at n2 = 512) go through LDS with barriers.
+__device__ __forceinline__ uint32_t pk_lane_xor(uint32_t x, uint32_t j, uint32_t lane) {
+    if (j == 1u) return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0xB1, 0xF, 0xF, false);  // quad_perm 1,0,3,2
+    if (j == 2u) return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
+    if (j == 4u) {
+        const uint32_t up = (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x104, 0xF, 0xF, false);  // row_shl:4
+        const uint32_t dn = (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x114, 0xF, 0xF, false);  // row_shr:4
+        return (lane & 4u) ? dn : up;
+    }
+    if (j == 8u) {
+        const uint32_t up = (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x108, 0xF, 0xF, false);  // row_shl:8
+        const uint32_t dn = (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x118, 0xF, 0xF, false);  // row_shr:8
+        return (lane & 8u) ? dn : up;
+    }
+    if (j == 16u) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x401F);  // bitmask mode, xor 16
+    return (uint32_t)__shfl_xor((int)x, 32);
+}
+__device__ __forceinline__ uint64_t pk_lane_xor64(uint64_t x, uint32_t j, uint32_t lane) {
+    const uint32_t lo = pk_lane_xor((uint32_t)x, j, lane);
+    const uint32_t hi = pk_lane_xor((uint32_t)(x >> 32), j, lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+// compare-exchange seen from one side: the lower element of an ascending pair
+// (or the upper of a descending one) keeps the minimum
+__device__ __forceinline__ uint64_t pk_cx(uint64_t x, uint64_t y, bool lower, bool up) {
+    const bool keep_min = lower == up;
+    const bool lt = x < y;
+    return (keep_min == lt) ? x : y;
+}
+static_assert(WALK_THREADS == 256 && WALK_PMAX == 512, "pk_walk_sort holds 2 keys per thread in 4 waves");
+__device__ void pk_walk_sort(uint64_t* v, uint32_t n) {
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    uint32_t n2 = 2;
+    while (n2 < n) n2 <<= 1;
+    const uint32_t e0 = w * 128u + lane, e1 = e0 + 64u;
+    const bool on = w * 128u < n2;
+    uint64_t x0 = ~0ull, x1 = ~0ull;
+    if (on) {
+        if (e0 < n) x0 = v[e0];
+        if (e1 < n) x1 = v[e1];
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 2; k <= WALK_PMAX; k <<= 1) {
+        if (k > n2) continue;  // block-uniform
+#pragma unroll
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 128u) {
+                if (on) { v[e0] = x0; v[e1] = x1; }
+                __syncthreads();
+                if (on) {
+                    const uint64_t y0 = v[e0 ^ j], y1 = v[e1 ^ j];
+                    x0 = pk_cx(x0, y0, (e0 & j) == 0u, (e0 & k) == 0u);
+                    x1 = pk_cx(x1, y1, (e1 & j) == 0u, (e1 & k) == 0u);
+                }
+                __syncthreads();
+            } else if (j == 64u) {
+                // e0 and e1 = e0 + 64 are the pair; they agree on bit k (k >= 128)
+                const bool up = (e0 & k) == 0u;
+                const bool lt = x0 < x1;
+                const uint64_t lo = lt ? x0 : x1, hi = lt ? x1 : x0;
+                x0 = up ? lo : hi;
+                x1 = up ? hi : lo;
+            } else {
+                const bool lower = (lane & j) == 0u;
+                const uint64_t y0 = pk_lane_xor64(x0, j, lane);
+                x0 = pk_cx(x0, y0, lower, (e0 & k) == 0u);
+                if (n2 > 64u) {  // at n2 <= 64 slot 1 holds only padding
+                    const uint64_t y1 = pk_lane_xor64(x1, j, lane);
+                    x1 = pk_cx(x1, y1, lower, (e1 & k) == 0u);
+                }
+            }
+        }
+    }
+    if (on) {
+        if (e0 < n) v[e0] = x0;
+        if (e1 < n) v[e1] = x1;
+    }
+    __syncthreads();
+}
+
 // Values of local points [p_lo, p_hi) of the chunk's active cells; keeps those
 // below `thr`.  Returns the number kept (W_key[0..n)).
 __device__ uint32_t pk_walk_values(const PkWalkArgs& a, PkWalkSmem& s, uint32_t cend, uint32_t p_lo,
@@ -452,28 +538,9 @@ __device__ uint32_t pk_walk_values(const PkWalkArgs& a, PkWalkSmem& s, uint32_t 
     __syncthreads();
     uint32_t n = s.red[2];
     __syncthreads();
-    // sort W_key[0..n) ascending
-    if (n > 1) {
-        uint32_t n2 = 2;
-        while (n2 < n) n2 <<= 1;
-        for (uint32_t i = n + threadIdx.x; i < n2; i += blockDim.x) s.W_key[i] = ~0ull;
-        __syncthreads();
-        for (uint32_t k = 2; k <= n2; k <<= 1) {
-            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                for (uint32_t i = threadIdx.x; i < (n2 >> 1); i += blockDim.x) {
-                    uint32_t lo = 2 * i - (i & (j - 1));
-                    uint32_t hi = lo + j;
-                    bool up = (lo & k) == 0;
-                    uint64_t x = s.W_key[lo], y = s.W_key[hi];
-                    if ((x > y) == up) {
-                        s.W_key[lo] = y;
-                        s.W_key[hi] = x;
-                    }
-                }
-                __syncthreads();
-            }
-        }
-    }
+    // sort W_key[0..n) ascending (the keys are distinct: their low word is
+    // the point's sequence number)
+    if (n > 1) pk_walk_sort(s.W_key, n);
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s.W_done[i] = 0;
     __syncthreads();
     return n;
