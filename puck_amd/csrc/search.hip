@@ -231,17 +231,17 @@ __device__ __forceinline__ float pk_hv(uint2 e) { return __uint_as_float(e.x); }
 
 // The heap is run by one full wave (64 active lanes, uniform arguments).  A
 // sift-down (MaxHeap::insert) takes the five levels below the current node
-// in one step: lane i holds node i of that 62-node subtree with its two
-// children, three ballots give, per node, whether it has a left child,
-// whether the reference's rule takes its right child (right present and not
-// left > right) and whether it is below the value sifted down; the path and
-// the stop node are then found with scalar bit operations, and the entries
-// on the path move up one level in one masked store.
+// in one step: lane i holds node i of that 62-node subtree (lanes
+// [2^d - 2, 2^(d+1) - 2) are depth d).  Each lane decides whether its parent
+// takes it (the reference's rule: the right child when present and not
+// left > right; the sibling's entry comes by a lane exchange), is on the
+// path when every ancestor in the subtree is taken too (bit tests of one
+// ballot), and the path's entries up to the first one below the sifted value
+// move up one level in one masked store.
 __device__ __forceinline__ bool pk_bit(uint64_t m, uint32_t i) { return (m >> i) & 1ull; }
 
 __device__ void pk_heap_insert_wave(uint2* H, uint32_t heap_size, uint32_t father, float v, uint32_t t) {
     const uint32_t lane = threadIdx.x & 63u;
-    // lane -> level 1..5 below the subtree root: lanes [2^d - 2, 2^(d+1) - 2)
     const uint32_t lv = 31u - __clz(lane + 2u);
     const uint32_t off = lane + 2u - (1u << lv);
     const bool inl = lane < 62u;
@@ -252,52 +252,28 @@ __device__ void pk_heap_insert_wave(uint2* H, uint32_t heap_size, uint32_t fathe
     for (;;) {
         father = __builtin_amdgcn_readfirstlane(father);
         const uint32_t idx = inl ? (father << lv) + off : 0xffffffffu;
-        const bool has = inl && idx <= heap_size;
-        uint32_t nv = 0u, nt = 0u;
-        float fl = 0.f, fr = 0.f;
-        bool hl = false, hr = false;
-        if (has) {
-            const uint2 e = H[idx];
-            nv = e.x;
-            nt = e.y;
-            if (lv < 5u && 2u * idx <= heap_size) {
-                hl = true;
-                hr = 2u * idx + 1u <= heap_size;
-                if (hr) {
-                    const uint4 ch = *(const uint4*)(H + 2u * idx);
-                    fl = __uint_as_float(ch.x);
-                    fr = __uint_as_float(ch.z);
-                }
-            }
-        }
-        const uint64_t mL = __ballot(hl);
-        const uint64_t mR = __ballot(hr && !(fl > fr));
-        const uint64_t mLt = __ballot(has && __uint_as_float(nv) < v);
-        const uint64_t mHas = __ballot(has);
-        // the subtree root's own choice: children in lanes 0 and 1
-        const float c0 = __uint_as_float(__builtin_amdgcn_readlane(nv, 0));
-        const float c1 = __uint_as_float(__builtin_amdgcn_readlane(nv, 1));
-        const bool root_l = pk_bit(mHas, 0);
-        const bool root_r = pk_bit(mHas, 1) && !(c0 > c1);
-        uint32_t cur = father;
-        uint32_t cl = 0;  // lane of cur (from level 1 on)
-        uint64_t moved = 0;
-        bool stop = false;
+        const bool ex = inl && idx <= heap_size;
+        uint2 e = make_uint2(0u, 0u);
+        if (ex) e = H[idx];
+        const float ev = __uint_as_float(e.x);
+        const float sv = __uint_as_float((uint32_t)__shfl_xor((int)e.x, 1));
+        const bool sex = __shfl_xor((int)ex, 1) != 0;
+        const bool chosen = (off & 1u) ? (ex && !(sv > ev)) : (ex && !(sex && !(ev > sv)));
+        const uint64_t C = __ballot(chosen);
+        bool onp = chosen;
 #pragma unroll
-        for (uint32_t d = 1; d <= 5; ++d) {
-            const bool left = d == 1 ? root_l : pk_bit(mL, cl);
-            if (!left) { stop = true; break; }
-            const bool right = d == 1 ? root_r : pk_bit(mR, cl);
-            const uint32_t o = d == 1 ? 0u : cl + 2u - (1u << (d - 1));  // offset of cur in its level
-            const uint32_t ch = (1u << d) - 2u + 2u * o + (right ? 1u : 0u);
-            if (pk_bit(mLt, ch)) { stop = true; break; }  // chosen child < v: v stays at cur
-            moved |= 1ull << ch;
-            cur = 2u * cur + (right ? 1u : 0u);
-            cl = ch;
-        }
-        if (pk_bit(moved, lane)) H[idx >> 1] = make_uint2(nv, nt);
-        father = cur;
-        if (stop) break;
+        for (uint32_t k = 1; k < 5; ++k)
+            if (k < lv) onp = onp && pk_bit(C, (1u << k) - 2u + (off >> (lv - k)));
+        const uint64_t P = __ballot(onp);
+        const uint64_t S = __ballot(onp && ev < v);           // path entries below v
+        const uint64_t mv = S ? (P & ((S & (~S + 1ull)) - 1ull)) : P;  // the path before the first
+        if (mv == 0ull) break;
+        if (pk_bit(mv, lane)) H[idx >> 1] = e;
+        const uint32_t last = 63u - (uint32_t)__builtin_clzll(mv);
+        const uint32_t ld = 31u - __clz(last + 2u);
+        father = (father << ld) + (last + 2u - (1u << ld));
+        // stopped inside the subtree, or at a depth-5 node without children
+        if (S != 0ull || ld < 5u || 2u * father > heap_size) break;
     }
     if (lane == 0) H[father] = make_uint2(vb, t);
 }
@@ -380,37 +356,8 @@ __device__ __forceinline__ uint32_t pk_bmax(uint32_t v, uint32_t* red) {
 // Bitonic sort of the n <= WALK_PMAX keys v[0..n) by the walk's 256 threads
 // with the keys in registers: element e sits in wave e / 128, lane e % 64, slot
 // (e / 64) % 2; padded with ~0 to n2 = 2^ceil(log2 n).  Exchanges at distance
-// 64 stay in the lane, those below 64 move between lanes (DPP for 1 to 8,
-// ds_swizzle for 16, ds_bpermute for 32); only the distances >= 128 (three
-// stages at n2 = 512) go through LDS with barriers.
-__device__ __forceinline__ uint32_t pk_lane_xor(uint32_t x, uint32_t j, uint32_t lane) {
-    if (j == 1u) return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0xB1, 0xF, 0xF, false);  // quad_perm 1,0,3,2
-    if (j == 2u) return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
-    if (j == 4u) {
-        const uint32_t up = (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x104, 0xF, 0xF, false);  // row_shl:4
-        const uint32_t dn = (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x114, 0xF, 0xF, false);  // row_shr:4
-        return (lane & 4u) ? dn : up;
-    }
-    if (j == 8u) {
-        const uint32_t up = (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x108, 0xF, 0xF, false);  // row_shl:8
-        const uint32_t dn = (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x118, 0xF, 0xF, false);  // row_shr:8
-        return (lane & 8u) ? dn : up;
-    }
-    if (j == 16u) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x401F);  // bitmask mode, xor 16
-    return (uint32_t)__shfl_xor((int)x, 32);
-}
-__device__ __forceinline__ uint64_t pk_lane_xor64(uint64_t x, uint32_t j, uint32_t lane) {
-    const uint32_t lo = pk_lane_xor((uint32_t)x, j, lane);
-    const uint32_t hi = pk_lane_xor((uint32_t)(x >> 32), j, lane);
-    return ((uint64_t)hi << 32) | lo;
-}
-// compare-exchange seen from one side: the lower element of an ascending pair
-// (or the upper of a descending one) keeps the minimum
-__device__ __forceinline__ uint64_t pk_cx(uint64_t x, uint64_t y, bool lower, bool up) {
-    const bool keep_min = lower == up;
-    const bool lt = x < y;
-    return (keep_min == lt) ? x : y;
-}
+// 64 stay in the lane, those below 64 move between lanes (pk_lane_xor64); only
+// the distances >= 128 (three stages at n2 = 512) go through LDS with barriers.
 static_assert(WALK_THREADS == 256 && WALK_PMAX == 512, "pk_walk_sort holds 2 keys per thread in 4 waves");
 __device__ void pk_walk_sort(uint64_t* v, uint32_t n) {
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
@@ -1156,12 +1103,22 @@ __device__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
         const uint32_t* csc = a.cell_start + (uint64_t)c * F;
         uint32_t j0 = 0;
         bool done = false;
+        // the next chunk's cell data, loaded while this chunk is processed
+        // (sorted fine entry once the chunk's end is known, the cell's
+        // stationary distance and start after the chunk's values pass)
+        uint32_t pf_at = 0xffffffffu, pf_k = 0, pf_b = 0, pf_n = 0;
+        float pf_fd = 0.f, pf_st = 0.f;
         while (!done && j0 < F) {
             const uint32_t j = j0 + tid;
             const bool valid = j < F;
             float fdj = 0.f, stj = 0.f;
             uint32_t bj = 0, nj = 0;
-            if (valid) {
+            if (valid && pf_at == j0) {
+                fdj = pf_fd;
+                stj = pf_st;
+                bj = pf_b;
+                nj = pf_n;
+            } else if (valid) {
                 fdj = fsd[j];
                 uint32_t k = fsi[j];
                 stj = stc[k];
@@ -1187,6 +1144,13 @@ __device__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
                 while (g > 0 && fsd[j0 + g - 1] == fsd[j0 + g]) --g;
                 if (g > 0) cend = g;
             }
+            const uint32_t jn = j0 + cend + tid;
+            const bool pfv = jn < F;
+            if (pfv) {
+                pf_fd = fsd[jn];
+                pf_k = fsi[jn];
+            }
+            pf_at = j0 + cend;
             if ((uint32_t)tid < cend) {
                 s.c_fd[tid] = fdj;
                 s.c_st[tid] = stj;
@@ -1212,6 +1176,11 @@ __device__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
                 __syncthreads();
                 float Tlow = (hi == 0xffffffffu && lo == 0xffffffffu) ? pk_sentinel() : pk_ord2f(hi);
                 Plow = pk_pivot(Tlow, qn, rr);
+            }
+            if (pfv) {
+                pf_st = stc[pf_k];
+                pf_b = csc[pf_k];
+                pf_n = csc[pf_k + 1] - pf_b;
             }
             // While the top set is not full the pivot is exactly the sentinel
             // pivot: cells up to the one whose points complete the fill are

@@ -288,6 +288,25 @@ __device__ void pk_topk_wave(LoadF load, uint32_t L, uint32_t k, float* out_val,
     }
     pk_wave_lds_order();
     bool tf = false;
+    if (k <= 64u) {
+        // one entry per lane, bitonic-sorted across the wave by (key, column):
+        // lane r then holds rank r
+        uint64_t e = lane < k ? buf[lane] : ~0ull;
+#pragma unroll
+        for (uint32_t kk = 2; kk <= 64u; kk <<= 1)
+#pragma unroll
+            for (uint32_t j = kk >> 1; j > 0; j >>= 1)
+                e = pk_cx(e, pk_lane_xor64(e, j, lane), (lane & j) == 0u, (lane & kk) == 0u);
+        const uint32_t ek = (uint32_t)(e >> 32);
+        const uint32_t nk = (uint32_t)__shfl_down((int)ek, 1, 64);  // key of rank lane + 1
+        if (lane < k) {
+            if (out_val) out_val[lane] = pk_ord2f(ek);
+            if (out_idx) out_idx[lane] = (uint32_t)(e & 0xffffffffu);
+            if (lane < flag_lim && (lane + 1u < k ? nk == ek : nlt + neq > k)) tf = true;
+        }
+        *tie = __ballot(tf) != 0ull;
+        return;
+    }
     for (uint32_t i = lane; i < k; i += 64u) {
         const uint64_t e = buf[i];
         const uint32_t ek = (uint32_t)(e >> 32);
