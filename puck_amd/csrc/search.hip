@@ -359,7 +359,7 @@ __device__ __forceinline__ uint32_t pk_bmax(uint32_t v, uint32_t* red) {
 // 64 stay in the lane, those below 64 move between lanes (pk_lane_xor64); only
 // the distances >= 128 (three stages at n2 = 512) go through LDS with barriers.
 static_assert(WALK_THREADS == 256 && WALK_PMAX == 512, "pk_walk_sort holds 2 keys per thread in 4 waves");
-__device__ void pk_walk_sort(uint64_t* v, uint32_t n) {
+__device__ __forceinline__ void pk_walk_sort(uint64_t* v, uint32_t n) {
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     uint32_t n2 = 2;
     while (n2 < n) n2 <<= 1;
@@ -412,7 +412,7 @@ __device__ void pk_walk_sort(uint64_t* v, uint32_t n) {
 
 // Values of local points [p_lo, p_hi) of the chunk's active cells; keeps those
 // below `thr`.  Returns the number kept (W_key[0..n)).
-__device__ uint32_t pk_walk_values(const PkWalkArgs& a, PkWalkSmem& s, uint32_t cend, uint32_t p_lo,
+__device__ __forceinline__ uint32_t pk_walk_values(const PkWalkArgs& a, PkWalkSmem& s, uint32_t cend, uint32_t p_lo,
                                    uint32_t p_hi, float cd, float thr, uint32_t seqbase) {
     if (threadIdx.x == 0) s.red[2] = 0;
     __syncthreads();
@@ -506,7 +506,7 @@ __device__ __forceinline__ uint32_t pk_lower_bound(const uint64_t* arr, uint32_t
 // Selects W positions r < nW with !W_done[r] and cell (by P_cell) in [clo, chi)
 // and c_cls[cell] == want (want == 0xff: any cell, used for big cells and T_low).
 // Compacts them into X_pos; returns the count.
-__device__ uint32_t pk_walk_select(PkWalkSmem& s, uint32_t nW, uint32_t clo, uint32_t chi, uint32_t want,
+__device__ __forceinline__ uint32_t pk_walk_select(PkWalkSmem& s, uint32_t nW, uint32_t clo, uint32_t chi, uint32_t want,
                                    uint32_t seqbase, uint32_t p_lo) {
     uint32_t total_sel = 0;
     for (uint32_t base = 0; base < nW; base += blockDim.x) {
@@ -531,7 +531,7 @@ __device__ uint32_t pk_walk_select(PkWalkSmem& s, uint32_t nW, uint32_t clo, uin
 // When dry != 0 nothing is written; instead the key of rank ftk-1 of the
 // union is stored in red[8..9] (or ~0 when the union has fewer than ftk).
 template <int FTKM, bool EX>
-__device__ uint32_t pk_walk_merge(const PkWalkArgs& a, PkWalkSmem& s, uint32_t ns, uint32_t m,
+__device__ __forceinline__ uint32_t pk_walk_merge(const PkWalkArgs& a, PkWalkSmem& s, uint32_t ns, uint32_t m,
                                   uint32_t seqbase, uint32_t p_lo, int dry, uint32_t plen = 0) {
     const uint32_t ftk = a.ftk;
     if (dry && threadIdx.x == 0) {
@@ -1025,7 +1025,7 @@ __device__ __forceinline__ float pk_walk_top(const PkWalkSmem& s, uint32_t ns, u
 }
 
 template <int FTKM, bool EX>
-__device__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
+__device__ __forceinline__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x;
     const uint32_t ftk = a.ftk, F = a.F, nsq = a.nsq;
