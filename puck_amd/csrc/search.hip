@@ -218,11 +218,13 @@ struct PkWalkSmem {
     float* lut;        // nsq*256
     float* qv;         // dim
     uint32_t* red;     // 16 words of reduction scratch
+    uint32_t* rb;      // 2 x 4 words: per-wave partials of pk_wmin / pk_wmax (two sets, alternating)
     float* P_t;        // PMAX     value by local point index
     uint8_t* P_sel;    // PMAX     selected for the current merge (exact mode)
     uint2* H;          // ftk + 1  reference heap (exact mode), slot i = heap index i
     uint32_t* td_val;  // PK_TIE_MAX  dropped-at-tie entries: distance bits, memory id
     uint32_t* td_id;
+    uint32_t rbs = 0;  // set of rb the next reduction writes (block-uniform, in registers)
 };
 
 // The reference MaxHeap (max_heap.cpp:52-100) on an LDS array.  Entries are
@@ -333,24 +335,30 @@ __device__ void pk_heap_init(uint2* H, uint32_t size) {
     for (uint32_t i = threadIdx.x; i <= size; i += blockDim.x) H[i] = sent;
 }
 
-__device__ __forceinline__ uint32_t pk_bmin(uint32_t v, uint32_t* red) {
-    if (threadIdx.x == 0) red[0] = 0xffffffffu;
+// Block min / max of the walk's 256 threads with one barrier: a wave
+// reduction by lane exchanges, one partial per wave in rb, alternating between
+// two sets of rb (the next reduction's barrier separates this one's reads from
+// the writes of the one after it).  Every thread of the block calls them in
+// the same order.
+__device__ __forceinline__ uint32_t pk_wmin(uint32_t v, PkWalkSmem& s) {
+    const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+    for (uint32_t j = 1; j < 64u; j <<= 1) v = min(v, pk_lane_xor(v, j, lane));
+    uint32_t* slot = s.rb + 4u * s.rbs;
+    if (lane == 0) slot[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (v != 0xffffffffu) atomicMin(&red[0], v);
-    __syncthreads();
-    uint32_t r = red[0];
-    __syncthreads();
-    return r;
+    s.rbs ^= 1u;
+    return min(min(slot[0], slot[1]), min(slot[2], slot[3]));
 }
-
-__device__ __forceinline__ uint32_t pk_bmax(uint32_t v, uint32_t* red) {
-    if (threadIdx.x == 0) red[1] = 0u;
+__device__ __forceinline__ uint32_t pk_wmax(uint32_t v, PkWalkSmem& s) {
+    const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+    for (uint32_t j = 1; j < 64u; j <<= 1) v = max(v, pk_lane_xor(v, j, lane));
+    uint32_t* slot = s.rb + 4u * s.rbs;
+    if (lane == 0) slot[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (v != 0u) atomicMax(&red[1], v);
-    __syncthreads();
-    uint32_t r = red[1];
-    __syncthreads();
-    return r;
+    s.rbs ^= 1u;
+    return max(max(slot[0], slot[1]), max(slot[2], slot[3]));
 }
 
 // Bitonic sort of the n <= WALK_PMAX keys v[0..n) by the walk's 256 threads
@@ -1039,6 +1047,7 @@ __device__ __forceinline__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) 
     s.c_b = (uint32_t*)ptr; ptr += sizeof(uint32_t) * WALK_CH;
     s.c_pre = (uint32_t*)ptr; ptr += sizeof(uint32_t) * (WALK_CH + 4);
     s.red = (uint32_t*)ptr; ptr += sizeof(uint32_t) * 16;
+    s.rb = (uint32_t*)ptr; ptr += sizeof(uint32_t) * 8;
     s.c_fd = (float*)ptr; ptr += sizeof(float) * WALK_CH;
     s.c_st = (float*)ptr; ptr += sizeof(float) * WALK_CH;
     s.qv = (float*)ptr; ptr += sizeof(float) * ((a.dim + 3) & ~3u);
@@ -1127,7 +1136,7 @@ __device__ __forceinline__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) 
             }
             const float Phi = P, Mhi = M, Thi = T;
             const bool brk = !valid || (fdj + mind >= Phi);
-            const uint32_t Bhi = pk_bmin(brk ? (uint32_t)tid : 0xffffffffu, s.red);
+            const uint32_t Bhi = pk_wmin(brk ? (uint32_t)tid : 0xffffffffu, s);
             const uint32_t bhi = Bhi == 0xffffffffu ? WALK_CH : Bhi;
             const bool act = ((uint32_t)tid < bhi) && !(stj >= Mhi) && nj > 0;
             const uint32_t cnt = act ? nj : 0u;
@@ -1135,7 +1144,7 @@ __device__ __forceinline__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) 
             const uint32_t pre = pk_block_exscan(cnt, s.red + 4, &tot);
             uint32_t cut = 0xffffffffu;
             if (act && pre + cnt > WALK_PMAX) cut = (pre == 0) ? (uint32_t)tid + 1 : (uint32_t)tid;
-            const uint32_t cutm = pk_bmin(cut, s.red);
+            const uint32_t cutm = pk_wmin(cut, s);
             uint32_t cend = cutm < bhi ? cutm : bhi;
             // do not split a fine-distance tie group across chunks: end the
             // chunk at the group's first cell when the group starts inside it
@@ -1187,8 +1196,8 @@ __device__ __forceinline__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) 
             // classified with it; later cells with P_low.
             uint32_t tfill = 0;
             if (ns < ftk) {
-                uint32_t tf = pk_bmin(((uint32_t)tid < cend && ns + s.c_pre[tid + 1] >= ftk) ? (uint32_t)tid
-                                                                                             : 0xffffffffu, s.red);
+                uint32_t tf = pk_wmin(((uint32_t)tid < cend && ns + s.c_pre[tid + 1] >= ftk) ? (uint32_t)tid
+                                                                                             : 0xffffffffu, s);
                 tfill = tf == 0xffffffffu ? cend : tf + 1;
             }
             uint32_t t0 = 0;
@@ -1226,9 +1235,9 @@ __device__ __forceinline__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) 
                 }
                 __syncthreads();
                 if (inr) s.c_cls[tid] = (uint8_t)cls;
-                const uint32_t u0 = pk_bmin((inr && cls == 0) ? (uint32_t)tid : 0xffffffffu, s.red);
+                const uint32_t u0 = pk_wmin((inr && cls == 0) ? (uint32_t)tid : 0xffffffffu, s);
                 const uint32_t u = u0 == 0xffffffffu ? cend : u0;
-                const uint32_t last1 = pk_bmax((inr && cls == 1 && (uint32_t)tid < u) ? (uint32_t)tid + 1 : 0u, s.red);
+                const uint32_t last1 = pk_wmax((inr && cls == 1 && (uint32_t)tid < u) ? (uint32_t)tid + 1 : 0u, s);
                 if (nW > 0) {
                     uint32_t m = pk_walk_select(s, nW, t0, u, 1u, seqbase, 0);
                     ns = pk_walk_merge<FTKM, EX>(a, s, ns, m, seqbase, 0, 0, ptot);
@@ -1550,7 +1559,7 @@ extern "C" size_t pk_walk_smem_bytes(uint32_t dim, uint32_t nsq, uint32_t ftk, i
     b += sizeof(uint32_t) * WALK_PMAX;
     b += sizeof(uint32_t) * WALK_CH;
     b += sizeof(uint32_t) * (WALK_CH + 4);
-    b += sizeof(uint32_t) * 16;
+    b += sizeof(uint32_t) * 16 + sizeof(uint32_t) * 8;
     b += sizeof(float) * WALK_CH * 2;
     b += sizeof(float) * ((dim + 3) & ~3u);
     b += sizeof(float) * nsq * PK_KS;
