@@ -516,21 +516,35 @@ __device__ __forceinline__ uint32_t pk_lower_bound(const uint64_t* arr, uint32_t
 // Compacts them into X_pos; returns the count.
 __device__ __forceinline__ uint32_t pk_walk_select(PkWalkSmem& s, uint32_t nW, uint32_t clo, uint32_t chi, uint32_t want,
                                    uint32_t seqbase, uint32_t p_lo) {
+    // per 256 entries: the wave's prefix by ballot, the waves' counts through
+    // the alternating per-wave slots of pk_wmin (one barrier)
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
     uint32_t total_sel = 0;
     for (uint32_t base = 0; base < nW; base += blockDim.x) {
         uint32_t r = base + threadIdx.x;
-        uint32_t sel = 0;
+        bool sel = false;
         if (r < nW && !s.W_done[r]) {
             uint32_t p = (uint32_t)(s.W_key[r] & 0xffffffffu) - seqbase - p_lo;
             uint32_t cell = s.P_cell[p];
-            if (cell >= clo && cell < chi && (want == 0xffu || s.c_cls[cell] == want)) sel = 1;
+            sel = cell >= clo && cell < chi && (want == 0xffu || s.c_cls[cell] == want);
         }
-        uint32_t tot;
-        uint32_t pre = pk_block_exscan(sel, s.red + 4, &tot);
-        if (sel) s.X_pos[total_sel + pre] = (uint16_t)r;
-        total_sel += tot;
+        const uint64_t m = __ballot(sel);
+        uint32_t* slot = s.rb + 4u * s.rbs;
+        if (lane == 0) slot[w] = (uint32_t)__popcll(m);
         __syncthreads();
+        s.rbs ^= 1u;
+        uint32_t before = 0, tot = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < 4u; ++q) {
+            const uint32_t c = slot[q];
+            before += q < w ? c : 0u;
+            tot += c;
+        }
+        if (sel) s.X_pos[total_sel + before + (uint32_t)__popcll(m & below)] = (uint16_t)r;
+        total_sel += tot;
     }
+    __syncthreads();
     return total_sel;
 }
 
