@@ -219,12 +219,14 @@ struct PkWalkSmem {
     float* qv;         // dim
     uint32_t* red;     // 16 words of reduction scratch
     uint32_t* rb;      // 2 x 4 words: per-wave partials of pk_wmin / pk_wmax (two sets, alternating)
+    uint64_t* rbm;     // 2 x 8 words: per-wave ballots of the cell classification (two sets, alternating)
     float* P_t;        // PMAX     value by local point index
     uint8_t* P_sel;    // PMAX     selected for the current merge (exact mode)
     uint2* H;          // ftk + 1  reference heap (exact mode), slot i = heap index i
     uint32_t* td_val;  // PK_TIE_MAX  dropped-at-tie entries: distance bits, memory id
     uint32_t* td_id;
     uint32_t rbs = 0;  // set of rb the next reduction writes (block-uniform, in registers)
+    uint32_t rbms = 0; // set of rbm the next classification writes
 };
 
 // The reference MaxHeap (max_heap.cpp:52-100) on an LDS array.  Entries are
@@ -374,11 +376,10 @@ __device__ __forceinline__ void pk_walk_sort(uint64_t* v, uint32_t n) {
     const uint32_t e0 = w * 128u + lane, e1 = e0 + 64u;
     const bool on = w * 128u < n2;
     uint64_t x0 = ~0ull, x1 = ~0ull;
-    if (on) {
+    if (on) {  // a wave reads and (at distances >= 128) first writes only its own entries
         if (e0 < n) x0 = v[e0];
         if (e1 < n) x1 = v[e1];
     }
-    __syncthreads();
 #pragma unroll
     for (uint32_t k = 2; k <= WALK_PMAX; k <<= 1) {
         if (k > n2) continue;  // block-uniform
@@ -491,13 +492,12 @@ __device__ __forceinline__ uint32_t pk_walk_values(const PkWalkArgs& a, PkWalkSm
         }
     }
     __syncthreads();
-    uint32_t n = s.red[2];
-    __syncthreads();
-    // sort W_key[0..n) ascending (the keys are distinct: their low word is
-    // the point's sequence number)
-    if (n > 1) pk_walk_sort(s.W_key, n);
+    const uint32_t n = s.red[2];  // rewritten only after later barriers
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s.W_done[i] = 0;
-    __syncthreads();
+    // sort W_key[0..n) ascending (the keys are distinct: their low word is
+    // the point's sequence number); the sort ends with a barrier
+    if (n > 1) pk_walk_sort(s.W_key, n);
+    else __syncthreads();
     return n;
 }
 
@@ -572,6 +572,10 @@ __device__ __forceinline__ uint32_t pk_walk_merge(const PkWalkArgs& a, PkWalkSme
         }
         return ns;
     }
+    if (!dry && threadIdx.x == 0) {
+        s.red[10] = 0u;
+        s.red[11] = 0u;
+    }
     uint64_t sk[FTKM / WALK_THREADS];
     uint32_t sid[FTKM / WALK_THREADS], spos[FTKM / WALK_THREADS];
     uint64_t xk[WALK_PMAX / WALK_THREADS];
@@ -618,9 +622,8 @@ __device__ __forceinline__ uint32_t pk_walk_merge(const PkWalkArgs& a, PkWalkSme
         __syncthreads();
         return ns;
     }
-    // boundary tie: the first dropped key has the same distance as the last kept
-    if (threadIdx.x == 0) { s.red[10] = 0u; s.red[11] = 0u; }
-    __syncthreads();
+    // boundary tie: the first dropped key has the same distance as the last
+    // kept (red[10], red[11]: cleared before the searches' barrier)
 #pragma unroll
     for (int e = 0; e < FTKM / WALK_THREADS; ++e) {
         if (spos[e] < ftk) { s.S_key[spos[e]] = sk[e]; s.S_id[spos[e]] = sid[e]; }
@@ -1056,6 +1059,7 @@ __device__ __forceinline__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) 
     unsigned char* ptr = smem;
     s.S_key = (uint64_t*)ptr; ptr += sizeof(uint64_t) * ((ftk + 1) & ~1u);
     s.W_key = (uint64_t*)ptr; ptr += sizeof(uint64_t) * WALK_PMAX;
+    s.rbm = (uint64_t*)ptr; ptr += sizeof(uint64_t) * 16;
     s.S_id = (uint32_t*)ptr; ptr += sizeof(uint32_t) * ((ftk + 3) & ~3u);
     s.P_id = (uint32_t*)ptr; ptr += sizeof(uint32_t) * WALK_PMAX;
     s.c_b = (uint32_t*)ptr; ptr += sizeof(uint32_t) * WALK_CH;
@@ -1249,9 +1253,32 @@ __device__ __forceinline__ void pk_walk_query(const PkWalkArgs& a, uint32_t qi) 
                 }
                 __syncthreads();
                 if (inr) s.c_cls[tid] = (uint8_t)cls;
-                const uint32_t u0 = pk_wmin((inr && cls == 0) ? (uint32_t)tid : 0xffffffffu, s);
-                const uint32_t u = u0 == 0xffffffffu ? cend : u0;
-                const uint32_t last1 = pk_wmax((inr && cls == 1 && (uint32_t)tid < u) ? (uint32_t)tid + 1 : 0u, s);
+                // u: the first uncertain cell (cend if none); last1: one past the
+                // last sure-scan cell before u (0 if none); from the waves'
+                // ballots after one barrier (alternating sets of rbm)
+                uint32_t u = cend, last1 = 0;
+                {
+                    const uint64_t b0 = __ballot(inr && cls == 0), b1 = __ballot(inr && cls == 1);
+                    uint64_t* slot = s.rbm + 8u * s.rbms;
+                    if ((tid & 63) == 0) {
+                        slot[2 * (tid >> 6)] = b0;
+                        slot[2 * (tid >> 6) + 1] = b1;
+                    }
+                    __syncthreads();
+                    s.rbms ^= 1u;
+#pragma unroll
+                    for (uint32_t q = 4; q-- > 0;) {
+                        const uint64_t m0 = slot[2 * q];
+                        if (m0) u = q * 64u + (uint32_t)__builtin_ctzll(m0);
+                    }
+#pragma unroll
+                    for (uint32_t q = 0; q < 4u; ++q) {
+                        const uint32_t lim = u > q * 64u ? u - q * 64u : 0u;
+                        uint64_t m1 = slot[2 * q + 1];
+                        if (lim < 64u) m1 &= (1ull << lim) - 1ull;
+                        if (m1) last1 = q * 64u + 64u - (uint32_t)__builtin_clzll(m1);
+                    }
+                }
                 if (nW > 0) {
                     uint32_t m = pk_walk_select(s, nW, t0, u, 1u, seqbase, 0);
                     ns = pk_walk_merge<FTKM, EX>(a, s, ns, m, seqbase, 0, 0, ptot);
@@ -1568,7 +1595,7 @@ extern "C" size_t pk_walk_smem_bytes(uint32_t dim, uint32_t nsq, uint32_t ftk, i
     b += sizeof(float) * WALK_PMAX + WALK_PMAX + 2 * sizeof(uint32_t) * PK_TIE_MAX + 16;
     if (exact) b += sizeof(uint2) * ((ftk + 2) & ~1u);
     b += sizeof(uint64_t) * ((ftk + 1) & ~1u);
-    b += sizeof(uint64_t) * WALK_PMAX;
+    b += sizeof(uint64_t) * WALK_PMAX + sizeof(uint64_t) * 16;
     b += sizeof(uint32_t) * ((ftk + 3) & ~3u);
     b += sizeof(uint32_t) * WALK_PMAX;
     b += sizeof(uint32_t) * WALK_CH;
