@@ -260,8 +260,9 @@ __device__ void pk_heap_insert_wave(uint2* H, uint32_t heap_size, uint32_t fathe
         uint2 e = make_uint2(0u, 0u);
         if (ex) e = H[idx];
         const float ev = __uint_as_float(e.x);
-        const float sv = __uint_as_float((uint32_t)__shfl_xor((int)e.x, 1));
-        const bool sex = __shfl_xor((int)ex, 1) != 0;
+        // the sibling is lane ^ 1 (node idx ^ 1): a DPP swap within the quad
+        const float sv = __uint_as_float(pk_lane_xor(e.x, 1u, lane));
+        const bool sex = inl && (idx ^ 1u) <= heap_size;
         const bool chosen = (off & 1u) ? (ex && !(sv > ev)) : (ex && !(sex && !(ev > sv)));
         const uint64_t C = __ballot(chosen);
         bool onp = chosen;
