@@ -51,6 +51,83 @@ __device__ void pk_sort_row(const float* __restrict__ vals, uint64_t row, uint32
     __syncthreads();  // sk is reused by the next row of a list
 }
 
+// The same sort with the keys in registers for n2 = 256 * E (E = 4, 8):
+// thread t holds elements t * E .. t * E + E - 1.  Exchanges at distances
+// below E stay in the thread, those below 64 * E go to lane t ^ (j / E) of
+// the same wave (pk_lane_xor64), only the larger ones (three stages at
+// n2 = 2048) go through LDS with barriers.  The sorted row is then in sk as
+// pk_sort_row leaves it.
+template <int E>
+__device__ void pk_sort_row_reg(const float* __restrict__ vals, uint64_t row, uint32_t L, uint64_t ld_in,
+                                uint32_t keep, float* __restrict__ out_val, uint32_t* __restrict__ out_idx,
+                                uint8_t* __restrict__ flags, uint32_t flag_lim, uint64_t* sk) {
+    constexpr uint32_t N2 = SORT_THREADS * E;
+    const uint32_t t = threadIdx.x, lane = t & 63u;
+    const float* v = vals + row * ld_in;
+    uint64_t x[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const uint32_t e = t * E + r;
+        x[r] = e < L ? (((uint64_t)pk_f2ord(v[e]) << 32) | e) : ~0ull;
+    }
+#pragma unroll
+    for (uint32_t k = 2; k <= N2; k <<= 1) {
+#pragma unroll
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            if (j < (uint32_t)E) {
+#pragma unroll
+                for (int r = 0; r < E; ++r) {
+                    if ((r & j) != 0u) continue;
+                    const bool up = ((t * E + r) & k) == 0u;
+                    const bool lt = x[r] < x[r + j];
+                    const uint64_t lo = lt ? x[r] : x[r + j], hi = lt ? x[r + j] : x[r];
+                    x[r] = up ? lo : hi;
+                    x[r + j] = up ? hi : lo;
+                }
+            } else if (j < 64u * E) {
+                const uint32_t jl = j / E;
+                const bool lower = (lane & jl) == 0u;
+#pragma unroll
+                for (int r = 0; r < E; ++r)
+                    x[r] = pk_cx(x[r], pk_lane_xor64(x[r], jl, lane), lower, ((t * E + r) & k) == 0u);
+            } else {
+#pragma unroll
+                for (int r = 0; r < E; ++r) sk[t * E + r] = x[r];
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < E; ++r) {
+                    const uint32_t e = t * E + r;
+                    x[r] = pk_cx(x[r], sk[e ^ j], (e & j) == 0u, (e & k) == 0u);
+                }
+                __syncthreads();
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < E; ++r) sk[t * E + r] = x[r];
+    __syncthreads();
+    bool tie = false;
+    for (uint32_t i = threadIdx.x; i < keep; i += blockDim.x) {
+        uint64_t key = sk[i];
+        float fv = pk_ord2f((uint32_t)(key >> 32));
+        out_val[row * keep + i] = fv;
+        out_idx[row * keep + i] = (uint32_t)(key & 0xffffffffu);
+        if (i < flag_lim && i + 1 < L && pk_ord2f((uint32_t)(sk[i + 1] >> 32)) == fv) tie = true;
+    }
+    if (tie && flags) flags[row] = 1u | flags[row];
+    __syncthreads();  // sk is reused by the next row of a list
+}
+
+__device__ __forceinline__ void pk_sort_row_any(const float* __restrict__ vals, uint64_t row, uint32_t L,
+                                                uint64_t ld_in, uint32_t keep, float* __restrict__ out_val,
+                                                uint32_t* __restrict__ out_idx, uint32_t n2,
+                                                uint8_t* __restrict__ flags, uint32_t flag_lim, uint64_t* sk) {
+    if (n2 == SORT_THREADS * 8u) pk_sort_row_reg<8>(vals, row, L, ld_in, keep, out_val, out_idx, flags, flag_lim, sk);
+    else if (n2 == SORT_THREADS * 4u)
+        pk_sort_row_reg<4>(vals, row, L, ld_in, keep, out_val, out_idx, flags, flag_lim, sk);
+    else pk_sort_row(vals, row, L, ld_in, keep, out_val, out_idx, n2, flags, flag_lim, sk);
+}
+
 __global__ __launch_bounds__(SORT_THREADS) void pk_sort_rows(const float* __restrict__ vals, uint32_t L,
                                                              uint64_t ld_in, uint32_t keep,
                                                              float* __restrict__ out_val,
@@ -58,7 +135,7 @@ __global__ __launch_bounds__(SORT_THREADS) void pk_sort_rows(const float* __rest
                                                              uint8_t* __restrict__ flags, uint32_t flag_lim) {
     extern __shared__ uint64_t sk[];
     const uint64_t row = blockIdx.x;
-    pk_sort_row(vals, row, L, ld_in, keep, out_val, out_idx, n2, flags, flag_lim, sk);
+    pk_sort_row_any(vals, row, L, ld_in, keep, out_val, out_idx, n2, flags, flag_lim, sk);
 }
 
 // The rows listed in list[0 .. *count), grid-stride (the rows a walk will use).
@@ -71,7 +148,7 @@ __global__ __launch_bounds__(SORT_THREADS) void pk_sort_rows_list(const float* _
     extern __shared__ uint64_t sk[];
     const uint32_t n = *count;
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x)
-        pk_sort_row(vals, list[i], L, ld_in, keep, out_val, out_idx, n2, nullptr, 0, sk);
+        pk_sort_row_any(vals, list[i], L, ld_in, keep, out_val, out_idx, n2, nullptr, 0, sk);
 }
 
 extern "C" hipError_t pk_launch_sort_rows_list(const float* vals, uint32_t max_rows, uint32_t L, uint64_t ld_in,
