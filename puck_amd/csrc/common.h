@@ -236,6 +236,20 @@ __device__ __forceinline__ float pk_rerank_dist(const T* __restrict__ x, const f
     return t;
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave by DPP moves: row shifts
+// 1, 2, 4, 8 within each 16-lane row (lanes before the row's start add 0),
+// then row broadcasts of lane 15 into rows 1 and 3 and of lane 31 into rows 2
+// and 3 (the rows not enabled add 0).
+__device__ __forceinline__ uint32_t pk_wave_incscan(uint32_t x) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true);   // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true);   // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true);   // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true);   // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    return x;
+}
+
 // Block-wide exclusive scan of one uint32 per thread (blockDim.x <= 1024,
 // multiple of 64).  `tmp` needs blockDim.x/64 + 1 words of LDS.  Returns the
 // exclusive prefix; *total receives the block sum.  Must be called by every
@@ -244,12 +258,7 @@ __device__ __forceinline__ uint32_t pk_block_exscan(uint32_t v, uint32_t* tmp, u
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
+    const uint32_t x = pk_wave_incscan(v);
     if (lane == 63) tmp[wid] = x;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -330,12 +339,7 @@ __device__ uint32_t pk_select_rank_each(EachF each, uint32_t want, uint32_t* his
         if (t < 64) {
             const uint32_t c0 = h[4 * t], c1 = h[4 * t + 1], c2 = h[4 * t + 2], c3 = h[4 * t + 3];
             const uint32_t sum = (c0 + c1) + (c2 + c3);
-            uint32_t incl = sum;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t y = __shfl_up(incl, o, 64);
-                if (t >= (uint32_t)o) incl += y;
-            }
+            const uint32_t incl = pk_wave_incscan(sum);
             const uint32_t before = incl - sum;
             if (before < want && want <= incl) {  // exactly one lane
                 const uint32_t cs[4] = {c0, c1, c2, c3};

@@ -321,12 +321,7 @@ __device__ void pk_topk_wave(LoadF load, uint32_t L, uint32_t k, float* out_val,
             h[q] = hist[4u * lane + q];
             sum += h[q];
         }
-        uint32_t inc = sum;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(inc, o, 64);
-            if (lane >= (uint32_t)o) inc += y;
-        }
+        const uint32_t inc = pk_wave_incscan(sum);
         uint32_t before = inc - sum, bin = 0, rest = 0;
         const bool mine = sum > 0 && before < want && want <= inc;
 #pragma unroll
