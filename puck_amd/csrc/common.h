@@ -236,6 +236,38 @@ __device__ __forceinline__ float pk_rerank_dist(const T* __restrict__ x, const f
     return t;
 }
 
+// Lane exchanges of a wave (64 lanes): the value of lane (lane ^ j) for
+// j = 1 .. 32 (DPP quad_perm / row shifts for 1 to 8, ds_swizzle for 16,
+// ds_bpermute for 32), and one side of a bitonic compare-exchange.
+__device__ __forceinline__ uint32_t pk_lane_xor(uint32_t x, uint32_t j, uint32_t lane) {
+    if (j == 1u) return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0xB1, 0xF, 0xF, false);  // quad_perm 1,0,3,2
+    if (j == 2u) return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
+    if (j == 4u) {
+        const uint32_t up = (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x104, 0xF, 0xF, false);  // row_shl:4
+        const uint32_t dn = (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x114, 0xF, 0xF, false);  // row_shr:4
+        return (lane & 4u) ? dn : up;
+    }
+    if (j == 8u) {
+        const uint32_t up = (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x108, 0xF, 0xF, false);  // row_shl:8
+        const uint32_t dn = (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x118, 0xF, 0xF, false);  // row_shr:8
+        return (lane & 8u) ? dn : up;
+    }
+    if (j == 16u) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x401F);  // bitmask mode, xor 16
+    return (uint32_t)__shfl_xor((int)x, 32);
+}
+__device__ __forceinline__ uint64_t pk_lane_xor64(uint64_t x, uint32_t j, uint32_t lane) {
+    const uint32_t lo = pk_lane_xor((uint32_t)x, j, lane);
+    const uint32_t hi = pk_lane_xor((uint32_t)(x >> 32), j, lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+// compare-exchange seen from one side: the lower element of an ascending pair
+// (or the upper of a descending one) keeps the minimum
+__device__ __forceinline__ uint64_t pk_cx(uint64_t x, uint64_t y, bool lower, bool up) {
+    const bool keep_min = lower == up;
+    const bool lt = x < y;
+    return (keep_min == lt) ? x : y;
+}
+
 // Inclusive prefix sum over the 64 lanes of a wave by DPP moves: row shifts
 // 1, 2, 4, 8 within each 16-lane row (lanes before the row's start add 0),
 // then row broadcasts of lane 15 into rows 1 and 3 and of lane 31 into rows 2
@@ -282,8 +314,8 @@ __device__ __forceinline__ uint32_t pk_block_exscan(uint32_t v, uint32_t* tmp, u
 __device__ __forceinline__ void pk_block_andor(uint32_t& ka, uint32_t& ko, uint32_t* red) {
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-        ka &= __shfl_xor(ka, o, 64);
-        ko |= __shfl_xor(ko, o, 64);
+        ka &= pk_lane_xor(ka, (uint32_t)o, threadIdx.x & 63u);
+        ko |= pk_lane_xor(ko, (uint32_t)o, threadIdx.x & 63u);
     }
     if ((threadIdx.x & 63) == 0) {
         red[2 * (threadIdx.x >> 6)] = ka;
@@ -393,36 +425,4 @@ __device__ uint32_t pk_select_rank(KeyF key, uint32_t n, uint32_t want, uint32_t
         __syncthreads();
     }
     return prefix;
-}
-
-// Lane exchanges of a wave (64 lanes): the value of lane (lane ^ j) for
-// j = 1 .. 32 (DPP quad_perm / row shifts for 1 to 8, ds_swizzle for 16,
-// ds_bpermute for 32), and one side of a bitonic compare-exchange.
-__device__ __forceinline__ uint32_t pk_lane_xor(uint32_t x, uint32_t j, uint32_t lane) {
-    if (j == 1u) return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0xB1, 0xF, 0xF, false);  // quad_perm 1,0,3,2
-    if (j == 2u) return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
-    if (j == 4u) {
-        const uint32_t up = (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x104, 0xF, 0xF, false);  // row_shl:4
-        const uint32_t dn = (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x114, 0xF, 0xF, false);  // row_shr:4
-        return (lane & 4u) ? dn : up;
-    }
-    if (j == 8u) {
-        const uint32_t up = (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x108, 0xF, 0xF, false);  // row_shl:8
-        const uint32_t dn = (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x118, 0xF, 0xF, false);  // row_shr:8
-        return (lane & 8u) ? dn : up;
-    }
-    if (j == 16u) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x401F);  // bitmask mode, xor 16
-    return (uint32_t)__shfl_xor((int)x, 32);
-}
-__device__ __forceinline__ uint64_t pk_lane_xor64(uint64_t x, uint32_t j, uint32_t lane) {
-    const uint32_t lo = pk_lane_xor((uint32_t)x, j, lane);
-    const uint32_t hi = pk_lane_xor((uint32_t)(x >> 32), j, lane);
-    return ((uint64_t)hi << 32) | lo;
-}
-// compare-exchange seen from one side: the lower element of an ascending pair
-// (or the upper of a descending one) keeps the minimum
-__device__ __forceinline__ uint64_t pk_cx(uint64_t x, uint64_t y, bool lower, bool up) {
-    const bool keep_min = lower == up;
-    const bool lt = x < y;
-    return (keep_min == lt) ? x : y;
 }

@@ -284,7 +284,7 @@ __device__ void pk_topk_wave(LoadF load, uint32_t L, uint32_t k, float* out_val,
         for (int j = 0; j < TOPK_WAVE_PER; ++j)
             if (j * 64u + lane < L) mn = min(mn, kv[j]);
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) mn = max(mn, (uint32_t)__shfl_xor(mn, o, 64));
+        for (int o = 1; o < 64; o <<= 1) mn = max(mn, pk_lane_xor(mn, (uint32_t)o, lane));
         T = mn;
     }
 #pragma unroll
@@ -296,8 +296,8 @@ __device__ void pk_topk_wave(LoadF load, uint32_t L, uint32_t k, float* out_val,
     }
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-        ka &= __shfl_xor(ka, o, 64);
-        ko |= __shfl_xor(ko, o, 64);
+        ka &= pk_lane_xor(ka, (uint32_t)o, lane);
+        ko |= pk_lane_xor(ko, (uint32_t)o, lane);
     }
     const uint32_t diff = ka ^ ko;
     const int top = diff ? 32 - __clz(diff) : 0;  // bits [top-1, 0] vary
