@@ -33,6 +33,9 @@ hipError_t pk_launch_cell_topk(const float*, const uint32_t*, uint32_t, const fl
 hipError_t pk_launch_exact_center_order(const float*, uint32_t, const float*, uint32_t, uint32_t, uint32_t,
                                         const uint32_t*, const uint32_t*, uint32_t, const uint8_t*, int, float*,
                                         uint32_t*, float*, uint32_t*, hipStream_t);
+hipError_t pk_launch_exact_center_order_dev(const float*, uint32_t, const float*, uint32_t, uint32_t, uint32_t,
+                                            const uint32_t*, const uint8_t*, uint32_t, uint32_t, uint32_t*, uint32_t*,
+                                            float*, uint32_t*, float*, uint32_t*, hipStream_t);
 hipError_t pk_launch_prep_queries(const float*, uint32_t, uint32_t, int, float*, float*, int32_t*, hipStream_t);
 hipError_t pk_launch_prep_queries_ex(const float*, uint32_t, uint32_t, int, float*, float*, int32_t*, const uint32_t*,
                                      uint8_t*, hipStream_t);
@@ -131,6 +134,7 @@ struct puckgpu_ctx {
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;        // the fallback walk beside the budget scan
     hipEvent_t fork_ev = nullptr, join_ev = nullptr, order_ev = nullptr;
+    hipEvent_t eorder_ev = nullptr;  // exact center orders of the batch's tied queries done (second stream)
     uint8_t* hpin = nullptr;  // pinned host staging of per-query flags and walk counters
     size_t hpin_cap = 0;
     std::string err;
@@ -274,7 +278,7 @@ void* scratch_slot(puckgpu_ctx* ctx, size_t slot, size_t bytes, int* rc) {
 
 enum Slot {
     S_Q = 0, S_QN, S_QST, S_CD, S_FD, S_CSD, S_CSI, S_FSD, S_FSI, S_FT, S_FI, S_FN, S_OD, S_OI, S_ON, S_STATS,
-    S_IN_Q, S_A0, S_A1, S_A2, S_A3, S_A4, S_A5, S_A6, S_A7, S_FLAGS, S_QLIST, S_TIEID, S_TIEN, S_QTOPK, S_QSCC,
+    S_ELIST, S_IN_Q, S_A0, S_A1, S_A2, S_A3, S_A4, S_A5, S_A6, S_A7, S_FLAGS, S_QLIST, S_TIEID, S_TIEN, S_QTOPK, S_QSCC,
     S_QNF, S_PFEAT, S_PWORK, S_PNF, S_PY, S_PT, S_PCV, S_PCID, S_LP16, S_LP8, S_LPS, S_CAND_D, S_CAND_I, S_WLIST, S_SCI,
     S_SCM, S_INS_X, S_INS_Y, S_INS_CELL, S_INS_REC, S_INS_PQ, S_INS_RANK, S_INS_ADD, S_INS_SHIFT, S_INS_WORK,
     S_INS_DOLD, S_INS_DNEW, S_INS_ACC, S_KM_IP, S_KM_PN, S_KM_CN, S_KM_ASSIGN, S_KM_DIS, S_KM_PERM, S_KM_STARTS,
@@ -334,7 +338,8 @@ int puckgpu_create(int device, puckgpu_ctx** out) {
     ok = ok && hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) == hipSuccess &&
          hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) == hipSuccess &&
          hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming) == hipSuccess &&
-         hipEventCreateWithFlags(&ctx->order_ev, hipEventDisableTiming) == hipSuccess;
+         hipEventCreateWithFlags(&ctx->order_ev, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&ctx->eorder_ev, hipEventDisableTiming) == hipSuccess;
     // this device's copy of the no-alloc scratch array (its name is a host
     // stand-in; only this address may be handed to kernels or copies)
     ok = ok && hipGetSymbolAddress((void**)&ctx->noalloc, HIP_SYMBOL(scratch)) == hipSuccess && ctx->noalloc;
@@ -358,6 +363,7 @@ void puckgpu_destroy(puckgpu_ctx* ctx) {
     if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
     if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
     if (ctx->order_ev) (void)hipEventDestroy(ctx->order_ev);
+    if (ctx->eorder_ev) (void)hipEventDestroy(ctx->eorder_ev);
     if (ctx->stream2) {
         (void)hipStreamSynchronize(ctx->stream2);
         (void)hipStreamDestroy(ctx->stream2);
@@ -668,6 +674,8 @@ struct ProbeUse {
 
 // Device buffers of one pipeline pass over up to `cap` queries.
 struct PassBufs {
+    uint32_t* elist = nullptr;   // queries with a coarse or fine tie (device list) and its count
+    uint32_t* ecount = nullptr;
     float* q = nullptr;
     float* qn = nullptr;
     int32_t* qst = nullptr;
@@ -720,6 +728,10 @@ struct PassCfg {
     uint32_t rerank_rows = 2;  // pk_rerank rows per lane group (autotuned per server bucket)
     const ProbeUse* pu = nullptr;
     bool budget = false;  // fine budgets may be given (explicit or predicted): the split scan's buffers
+    // plain exact search on context buffers: the exact center orders of the
+    // queries with a coarse or fine tie are computed on the second stream
+    // beside the rerank (the replay's first stage then skips them)
+    bool early_order = false;
 };
 
 // Enqueues the device stages of one pass over m queries: no host
@@ -823,6 +835,16 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
         wa.wcount = b.wcount;
     }
     PK_CHECK(pk_launch_walk(&wa, m, s, ev ? ev[6] : nullptr, ctx->stream2, ctx->fork_ev, ctx->join_ev));
+    if (cfg.early_order) {
+        // flags 1 and 8 are final after the walk; the rerank below reads none
+        // of the center orders these kernels rewrite
+        PK_CHECK(hipEventRecord(ctx->fork_ev, s));
+        PK_CHECK(hipStreamWaitEvent(ctx->stream2, ctx->fork_ev, 0));
+        PK_CHECK(pk_launch_exact_center_order_dev(b.cdist, C, b.fdist, F, cfg.keep ? cfg.keep : 1, cfg.scc, q_scc,
+                                                  b.flags, 9u, m, b.elist, b.ecount, b.csd, b.csi, b.fsd, b.fsi,
+                                                  ctx->stream2));
+        PK_CHECK(hipEventRecord(ctx->eorder_ev, ctx->stream2));
+    }
     wa.wlist = nullptr;  // replays address queries through qlist
     wa.wcount = nullptr;
     if (ev) PK_CHECK(hipEventRecord(ev[3], s));
@@ -837,6 +859,7 @@ int enqueue_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     ra.pnsq = ix->pnsq; ra.plsq = ix->plsq;
     PK_CHECK(pk_launch_rerank(&ra, m, s));
     if (ev) PK_CHECK(hipEventRecord(ev[4], s));
+    if (cfg.early_order) PK_CHECK(hipStreamWaitEvent(s, ctx->eorder_ev, 0));
     if (wa_out) *wa_out = wa;
     if (ra_out) *ra_out = ra;
     return PUCKGPU_OK;
@@ -889,8 +912,11 @@ int replay_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, c
     }
     if (n1) {
         PK_CHECK(hipMemcpyAsync(b.qlist, stage1, sizeof(uint32_t) * n1, hipMemcpyHostToDevice, s));
-        PK_CHECK(pk_launch_exact_center_order(b.cdist, C, b.fdist, F, wa.cstride, cfg.scc, wa.q_scc, b.qlist, n1,
-                                              b.flags, 0, b.csd, b.csi, b.fsd, b.fsi, s));
+        // every stage-1 query has flag 1 or 8, so with early_order its exact
+        // center order was computed during the pass
+        if (!cfg.early_order)
+            PK_CHECK(pk_launch_exact_center_order(b.cdist, C, b.fdist, F, wa.cstride, cfg.scc, wa.q_scc, b.qlist, n1,
+                                                  b.flags, 0, b.csd, b.csi, b.fsd, b.fsi, s));
         PkWalkArgs w1 = wa;
         w1.qlist = b.qlist;
         w1.fine_given = 1;
@@ -932,6 +958,8 @@ int scratch_pass(puckgpu_ctx* ctx, const puckgpu_index* ix, const PassCfg& cfg, 
     b.fsi = (uint32_t*)scratch_slot(ctx, S_FSI, sizeof(uint32_t) * cap * F, &rc);
     b.flags = (uint8_t*)scratch_slot(ctx, S_FLAGS, cap, &rc);
     b.qlist = (uint32_t*)scratch_slot(ctx, S_QLIST, sizeof(uint32_t) * cap, &rc);
+    b.elist = (uint32_t*)scratch_slot(ctx, S_ELIST, sizeof(uint32_t) * (cap + 1), &rc);
+    b.ecount = b.elist ? b.elist + cap : nullptr;
     b.tie_ids = (uint32_t*)scratch_slot(ctx, S_TIEID, sizeof(uint32_t) * 16 * cap, &rc);
     b.tie_n = (uint8_t*)scratch_slot(ctx, S_TIEN, cap, &rc);
     b.stats = (uint32_t*)scratch_slot(ctx, S_STATS, sizeof(uint32_t) * 4 * cap, &rc);
@@ -1092,6 +1120,7 @@ int search_impl(puckgpu_ctx* ctx, const puckgpu_index* ix, const puckgpu_search_
     int rc = make_cfg(ctx, ix, sp, topk, ostride, pu, cfg);
     if (rc) return rc;
     cfg.budget = q_nf_in != nullptr || (pu && pu->probe);
+    cfg.early_order = !cfg.budget && cfg.precision == 0;
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, PUCKGPU_EHIP, "hipSetDevice failed");
     const uint32_t d = ix->p.dim, ftk = cfg.ftk;
     hipStream_t s = ctx->stream;

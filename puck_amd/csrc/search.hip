@@ -2118,16 +2118,14 @@ extern "C" hipError_t pk_launch_rerank(const PkRerankArgs* args, uint32_t nq, hi
 // Exact center order for flagged queries: the coarse top-scc MaxHeap
 // (hierarchical_cluster_index.cpp:709-726) and the fine full heap sort
 // (puck_index.cpp:497-504), replayed by one wave on LDS copies of the rows.
-__global__ __launch_bounds__(256) void pk_exact_center_order(const float* __restrict__ cdist, uint32_t C,
+__device__ void pk_exact_center_order_one(uint32_t qi, const float* __restrict__ cdist, uint32_t C,
                                                              const float* __restrict__ fdist, uint32_t F,
                                                              uint32_t scs, uint32_t scap,
                                                              const uint32_t* __restrict__ q_scc,
-                                                             const uint32_t* __restrict__ qlist,
                                                              const uint8_t* __restrict__ flags, int force,
                                                              float* __restrict__ csel_d, uint32_t* __restrict__ csel_id,
                                                              float* __restrict__ fsort_d, uint32_t* __restrict__ fsort_id) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const uint32_t qi = qlist[blockIdx.x];
     // coarse heap order for a coarse-distance tie (flag 1), fine heap order for
     // a decision on a fine-distance tie (flag 8); both when forced
     const bool do_coarse = force || (flags[qi] & 1u);
@@ -2213,6 +2211,80 @@ __global__ __launch_bounds__(256) void pk_exact_center_order(const float* __rest
         fsort_d[(uint64_t)qi * F + i] = pk_hv(H[i + 1]);
         fsort_id[(uint64_t)qi * F + i] = H[i + 1].y;
     }
+}
+
+__global__ __launch_bounds__(256) void pk_exact_center_order(const float* __restrict__ cdist, uint32_t C,
+                                                             const float* __restrict__ fdist, uint32_t F,
+                                                             uint32_t scs, uint32_t scap,
+                                                             const uint32_t* __restrict__ q_scc,
+                                                             const uint32_t* __restrict__ qlist,
+                                                             const uint8_t* __restrict__ flags, int force,
+                                                             float* __restrict__ csel_d, uint32_t* __restrict__ csel_id,
+                                                             float* __restrict__ fsort_d, uint32_t* __restrict__ fsort_id) {
+    pk_exact_center_order_one(qlist[blockIdx.x], cdist, C, fdist, F, scs, scap, q_scc, flags, force, csel_d, csel_id,
+                              fsort_d, fsort_id);
+}
+
+// The same over a list built on the device (list[0 .. *count)), grid-stride:
+// the batch's queries with a coarse or fine tie, ordered while the batch's
+// rerank runs on the other stream.
+__global__ __launch_bounds__(256) void pk_exact_center_order_dev(const float* __restrict__ cdist, uint32_t C,
+                                                                 const float* __restrict__ fdist, uint32_t F,
+                                                                 uint32_t scs, uint32_t scap,
+                                                                 const uint32_t* __restrict__ q_scc,
+                                                                 const uint32_t* __restrict__ list,
+                                                                 const uint32_t* __restrict__ count,
+                                                                 const uint8_t* __restrict__ flags,
+                                                                 float* __restrict__ csel_d,
+                                                                 uint32_t* __restrict__ csel_id,
+                                                                 float* __restrict__ fsort_d,
+                                                                 uint32_t* __restrict__ fsort_id) {
+    const uint32_t n = *count;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        pk_exact_center_order_one(list[i], cdist, C, fdist, F, scs, scap, q_scc, flags, 0, csel_d, csel_id, fsort_d,
+                                  fsort_id);
+        __syncthreads();  // LDS reused by the next query
+    }
+}
+
+// list[0 .. *count) = the i < n with flags[i] & mask, ascending (one workgroup).
+__global__ __launch_bounds__(1024) void pk_list_flagged(const uint8_t* __restrict__ flags, uint32_t n, uint32_t mask,
+                                                       uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
+    __shared__ uint32_t tmp[1024 / 64 + 1];
+    uint32_t base = 0;
+    for (uint32_t i0 = 0; i0 < n; i0 += blockDim.x) {
+        const uint32_t i = i0 + threadIdx.x;
+        const uint32_t sel = (i < n && (flags[i] & mask)) ? 1u : 0u;
+        uint32_t tot;
+        const uint32_t pre = pk_block_exscan(sel, tmp, &tot);
+        if (sel) list[base + pre] = i;
+        base += tot;
+    }
+    if (threadIdx.x == 0) *count = base;
+}
+
+extern "C" hipError_t pk_launch_exact_center_order_dev(const float* cdist, uint32_t C, const float* fdist, uint32_t F,
+                                                       uint32_t scs, uint32_t scap, const uint32_t* q_scc,
+                                                       const uint8_t* flags, uint32_t mask, uint32_t n,
+                                                       uint32_t* list, uint32_t* count, float* csel_d,
+                                                       uint32_t* csel_id, float* fsort_d, uint32_t* fsort_id,
+                                                       hipStream_t s) {
+    if (scap > scs) return hipErrorInvalidValue;
+    if (n == 0) return hipSuccess;
+    const uint32_t L = C > F ? C : F;
+    size_t smem = sizeof(uint2) * ((L + 2) & ~1u) + sizeof(float) * L;
+    static PkLdsCache lds_cache;
+    size_t* slot = pk_lds_slot(lds_cache);
+    if (!slot) return hipErrorInvalidDevice;
+    if (!*slot) {
+        const hipError_t e = pk_allow_lds((const void*)pk_exact_center_order_dev, 160 * 1024, slot);
+        if (e != hipSuccess) return e;
+    }
+    if (smem > *slot) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pk_list_flagged, dim3(1), dim3(1024), 0, s, flags, n, mask, list, count);
+    hipLaunchKernelGGL(pk_exact_center_order_dev, dim3(n < 64 ? n : 64), dim3(256), smem, s, cdist, C, fdist, F, scs,
+                       scap, q_scc, list, count, flags, csel_d, csel_id, fsort_d, fsort_id);
+    return hipGetLastError();
 }
 
 extern "C" hipError_t pk_launch_exact_center_order(const float* cdist, uint32_t C, const float* fdist, uint32_t F,
